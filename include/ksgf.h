@@ -1,0 +1,231 @@
+/*
+ * ksgf.h -- C ABI of the MI355X (gfx950) implementation of Kynema-SGF's
+ * per-step Godunov PPM advection + MAC projection hot path.
+ *
+ * Everything here is plain C: integers, doubles, raw device pointers and
+ * opaque handles; every entry point returns KSGF_OK (0) or a negative error
+ * code and never throws.  Device buffers are owned by the caller (as
+ * FieldRepo owns MultiFabs, src/core/FieldRepo.H:139-351); the library only
+ * allocates its own internal scratch (box tables, per-stage temporaries, the
+ * multigrid hierarchy).  Work is enqueued on the caller's hipStream_t;
+ * functions that return a host value (CFL number, solver statistics)
+ * synchronise that stream.
+ *
+ * Data layout.  A MultiFab is one flat array of doubles holding one FAB per
+ * box of a BoxArray, in box order, without padding.  FAB b covers box b
+ * converted to the MultiFab's index type (cell centred, or nodal in one
+ * direction for x/y/z faces) and grown by ngrow in every direction; it is
+ * stored in Fortran order (i fastest) with the component index slowest --
+ * the amrex::Array4 convention.  ksgf_multifab_size() gives the length.
+ *
+ * The MultiFab-level calls replace the reference's per-box loops: one call
+ * processes every box of the BoxArray in a single launch over a ragged box
+ * table (the MFIter loops at src/equation_systems/icns/icns_advection.H:349-374
+ * and :410-440 become one call each).
+ */
+#ifndef KSGF_H
+#define KSGF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* ksgf_stream_t; /* a hipStream_t; NULL = default stream */
+
+/* status codes */
+#define KSGF_OK 0
+#define KSGF_ERR_INVALID -1     /* bad argument (null pointer, bad enum, bad box) */
+#define KSGF_ERR_UNSUPPORTED -2 /* valid in the reference, not provided here (PLM, WENO) */
+#define KSGF_ERR_HIP -3         /* a HIP runtime call failed */
+#define KSGF_ERR_NOMEM -4       /* device allocation failed */
+#define KSGF_ERR_NOT_CONVERGED 1 /* MLMG stopped at maxiter (not an abort, do_fixed_iters) */
+
+/* index types of a MultiFab */
+#define KSGF_CELL 0
+#define KSGF_XFACE 1
+#define KSGF_YFACE 2
+#define KSGF_ZFACE 3
+
+/* amrex::BCType values used on this path (BCInterface.cpp:200-260) */
+#define KSGF_BC_INT_DIR 0
+#define KSGF_BC_FOEXTRAP 2
+#define KSGF_BC_EXT_DIR 3
+#define KSGF_BC_HOEXTRAP 4
+
+/* PPM limiter_type (icns_advection.H:195-204: ppm -> default_limiter) */
+#define KSGF_PPM_NOLIMITER 0
+#define KSGF_PPM_VANLEER 1 /* PPM::default_limiter */
+#define KSGF_PPM_WENOZ 2   /* not provided: KSGF_ERR_UNSUPPORTED */
+#define KSGF_PPM_WENO_JS 3 /* not provided: KSGF_ERR_UNSUPPORTED */
+#define KSGF_PPM_UPWIND 4
+#define KSGF_PPM_MINMOD 5
+
+/* linear-operator domain BCs (icns_advection.cpp:21-41) */
+#define KSGF_LO_PERIODIC 0
+#define KSGF_LO_NEUMANN 1
+#define KSGF_LO_DIRICHLET 2
+
+/* multigrid precision */
+#define KSGF_MG_FP64 0
+#define KSGF_MG_MIXED 1 /* FP32 V-cycles inside an FP64 defect-correction loop */
+
+typedef struct {
+    int domain_lo[3], domain_hi[3]; /* cell-centred, inclusive */
+    double prob_lo[3], prob_hi[3];
+    int is_periodic[3];
+} ksgf_geometry;
+
+typedef struct {
+    int lo[3];
+    int hi[3];
+} ksgf_bcrec; /* one per component, amrex::BCRec */
+
+typedef struct {
+    int maxiter;          /* mac_proj.maxiter (200) */
+    int num_pre_smooth;   /* 2 */
+    int num_post_smooth;  /* 2 */
+    int num_final_smooth; /* 8: smooths after a failed bottom solve */
+    int num_bottom_smooth;/* 0: smooths after a converged bottom solve */
+    int bottom_maxiter;   /* 200 */
+    double bottom_rtol;   /* eps*1e12 */
+    double bottom_atol;   /* eps*1e4 */
+    int do_semicoarsening;
+    int max_coarsening_level; /* 30 */
+    int precision;        /* KSGF_MG_FP64 or KSGF_MG_MIXED */
+} ksgf_mlmg_options;
+
+typedef struct {
+    int iters;        /* V-cycles (mixed: total inner FP32 V-cycles) */
+    int outer_iters;  /* FP64 defect-correction iterations (mixed only) */
+    double r0;        /* initial residual, inf norm */
+    double rfinal;    /* final residual, inf norm */
+    int converged;
+    int nlevels;
+    int bottom_iters; /* BiCGStab iterations summed over V-cycles */
+    int bottom_fail;
+} ksgf_mlmg_info; /* what io::print_mlmg_info prints (console_io.cpp:178-186) */
+
+typedef struct ksgf_boxarray ksgf_boxarray;
+typedef struct ksgf_macproj ksgf_macproj;
+
+const char* ksgf_last_error(void);
+const char* ksgf_version(void);
+
+/* BoxArray + Geometry + DistributionMapping on one GPU.
+ * boxes: nbox*6 ints (lo0 lo1 lo2 hi0 hi1 hi2), disjoint, covering the domain.
+ * Replaces amrex::BoxArray / Geometry as consumed by MultiFab construction. */
+int ksgf_boxarray_create(const ksgf_geometry* geom, const int* boxes, int nbox,
+                         ksgf_boxarray** out);
+int ksgf_boxarray_destroy(ksgf_boxarray* ba);
+long long ksgf_multifab_size(const ksgf_boxarray* ba, int itype, int ncomp, int ngrow);
+
+/* MultiFab::FillBoundary(periodicity): interior and periodic ghosts
+ * (icns_advection.H:239-243). */
+int ksgf_fill_boundary(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow,
+                       ksgf_stream_t stream);
+
+/* Physical-boundary ghost fill (the BC part of FillPatch / fillpatch_sibling_fields,
+ * FieldFillPatchOps.H:250-336).  bc: ncomp BCRecs; bcval: [6 orientations
+ * xlo,ylo,zlo,xhi,yhi,zhi][ncomp] Dirichlet values (host memory). */
+int ksgf_fill_physbc(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow,
+                     const ksgf_bcrec* bc, const double* bcval, ksgf_stream_t stream);
+
+/* MacProjOp::set_inflow_velocity (icns_advection.cpp:157-183): on ext_dir
+ * boundary faces set the normal MAC velocity to the boundary value. */
+int ksgf_set_inflow_velocity(const ksgf_boxarray* ba, double* umac, double* vmac, double* wmac,
+                             int ngrow_mac, const ksgf_bcrec* bc, const double* bcval,
+                             ksgf_stream_t stream);
+
+/* PPM::PredictStateOn{X,Y,Z}Face (test_mflux_schemes.cpp:87-151) for every
+ * valid cell: s (cell, ncomp_s, ng_s >= 2), component comp; vel_edge (face of
+ * direction dir, 1 comp, ng_v >= 0); Im/Ip (cell, 1 comp, ngrow 0). */
+int ksgf_ppm_predict_state_on_face(const ksgf_boxarray* ba, int dir, const double* s,
+                                   int ncomp_s, int ng_s, int comp, const double* vel_edge,
+                                   int ng_v, const ksgf_bcrec* bc, double dt, int limiter_type,
+                                   double* Im, double* Ip, ksgf_stream_t stream);
+
+/* HydroUtils::ExtrapVelToFaces (call icns_advection.H:211-216).
+ * vel: cell, 3 comp, ngrow 3 (ghosts filled); vel_forces: cell, 3 comp,
+ * ngrow 1; u/v/w_mac: faces, 1 comp, ngrow 1 (valid faces written).
+ * bcrec: 3 BCRecs (host). */
+int ksgf_extrap_vel_to_faces(const ksgf_boxarray* ba, const double* vel,
+                             const double* vel_forces, double* u_mac, double* v_mac,
+                             double* w_mac, const ksgf_bcrec* bcrec, double dt, int use_ppm,
+                             int use_forces_in_trans, int limiter_type,
+                             int allow_inflow_on_outflow, ksgf_stream_t stream);
+
+/* HydroUtils::ComputeFluxesOnBoxFromState (icns_advection.H:359-373,
+ * AdvOp_Godunov.H:231-250) over all boxes.  q, q_on_face_bc: cell, ncomp,
+ * ngrow 3; fq: cell, ncomp, ngrow 1; u/v/w_mac: faces, ngrow 1; divu: cell,
+ * 1 comp, ngrow 1, or NULL (zero); flux_* and face_*: faces, ncomp, ngrow 0.
+ * bcrec, iconserv: ncomp entries (host). */
+int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const double* q,
+                                   const double* q_on_face_bc, double* flux_x, double* flux_y,
+                                   double* flux_z, double* face_x, double* face_y,
+                                   double* face_z, int known_edge_state, const double* u_mac,
+                                   const double* v_mac, const double* w_mac, const double* divu,
+                                   const double* fq, double dt, const ksgf_bcrec* bcrec,
+                                   const int* iconserv, int use_ppm, int use_forces_in_trans,
+                                   int is_velocity, int fluxes_are_area_weighted,
+                                   int limiter_type, int allow_inflow_on_outflow,
+                                   ksgf_stream_t stream);
+
+/* HydroUtils::ComputeDivergence (icns_advection.H:418-422): div = mult *
+ * sum_d (F_d(+) - F_d(-)) / dx_d.  div: cell, ncomp, ngrow 0; fluxes: faces,
+ * ncomp, ngrow 0. */
+int ksgf_compute_divergence(const ksgf_boxarray* ba, double* div, const double* flux_x,
+                            const double* flux_y, const double* flux_z, int ncomp, double mult,
+                            int fluxes_are_area_weighted, ksgf_stream_t stream);
+
+/* Hydro::MacProjector (icns_advection.cpp:88-104,143-154,289-353):
+ * create = MacProjector(geom) + initProjector(ba, dm, lpinfo, beta) +
+ * setDomainBC(lobc, hibc) + MLMGOptions applied; set_beta = updateBeta(Real);
+ * project = setUMAC + project(rtol, atol), u/v/w_mac: faces, 1 comp, ngrow 1;
+ * the valid faces are corrected in place.  info may be NULL. */
+void ksgf_mlmg_default_options(ksgf_mlmg_options* opts);
+int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc,
+                        double beta, const ksgf_mlmg_options* opts, ksgf_macproj** out);
+int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta);
+int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac,
+                         double rtol, double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
+/* copy the last solution phi (domain-wide, n0*n1*n2, Fortran order) */
+int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream);
+int ksgf_macproj_destroy(ksgf_macproj* mp);
+
+/* Standalone Poisson solve div(beta grad phi) = rhs on the projector's
+ * domain (the MLMG solve inside project); rhs/phi domain-wide (n0*n1*n2). */
+int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol,
+                       double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
+
+/* incflo::compute_dt convective part (incflo_compute_dt.cpp:69-91):
+ * max over valid cells of max_d |u_d|/dx_d (floor -1).  Synchronises. */
+int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out,
+                 ksgf_stream_t stream);
+
+/* ComputeRHSOp::predictor_rhs, multiply_rho branch (CompRHSOps.H:93-117):
+ * vel_new = (rho_old*vel_old + dt*(conv + src + kappa*diff)) / rho_new.
+ * diff may be NULL. conv/src/diff: cell, ncomp, ngrow 0. */
+int ksgf_predictor_update(const ksgf_boxarray* ba, double* vel_new, int ng_new,
+                          const double* vel_old, int ng_old, const double* rho_old, int ng_ro,
+                          const double* rho_new, int ng_rn, const double* conv,
+                          const double* src, const double* diff, double kappa, double dt,
+                          int ncomp, ksgf_stream_t stream);
+
+/* SrcTermOp<ICNS> (icns_ops.H:98-111) in velocity form: src = -gp / rho,
+ * then (scale_by_rho) src *= rho (PDEOps multiply_rho).  gp: ngrow 0. */
+int ksgf_icns_src_term(const ksgf_boxarray* ba, double* src, int ng_src, const double* gp,
+                       const double* rho, int ng_rho, int scale_by_rho, ksgf_stream_t stream);
+
+/* out(comp n) = a(comp n) * rho over valid + ngrow cells (MultiFab::Copy +
+ * MultiFab::Multiply, icns_advection.H:281-314).  a/out: ncomp, ng_a/ng_out
+ * >= ngrow; rho: 1 comp, ng_rho >= ngrow. */
+int ksgf_scale_by_rho(const ksgf_boxarray* ba, double* out, int ng_out, const double* a,
+                      int ng_a, const double* rho, int ng_rho, int ncomp, int ngrow,
+                      int divide, ksgf_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,341 @@
+"""kynema-sgf_amd: MI355X (gfx950) implementation of Kynema-SGF's Godunov PPM
+advection + MAC projection step behind the AMReX-Hydro MultiFab-level calls.
+
+The product is libksgf.so (HIP kernels + the C ABI of include/ksgf.h).
+This module is its ctypes binding for Python callers (tests, bench): device
+buffers are torch CUDA tensors (HIP on ROCm) passed by data pointer, work is
+enqueued on torch's current stream.  torch is imported before the library is
+loaded so both share one HIP runtime.
+
+There is no CPU fallback: every entry point raises if libksgf.so is missing
+or a call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import layout  # noqa: F401
+from .layout import CELL, XFACE, YFACE, ZFACE, BoxArray, Geometry  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libksgf.so")
+
+_dp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+
+
+class KsgfGeometry(C.Structure):
+    _fields_ = [("domain_lo", C.c_int * 3), ("domain_hi", C.c_int * 3), ("prob_lo", C.c_double * 3),
+                ("prob_hi", C.c_double * 3), ("is_periodic", C.c_int * 3)]
+
+
+class KsgfBCRec(C.Structure):
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3)]
+
+
+class KsgfMLMGOptions(C.Structure):
+    _fields_ = [("maxiter", C.c_int), ("num_pre_smooth", C.c_int), ("num_post_smooth", C.c_int),
+                ("num_final_smooth", C.c_int), ("num_bottom_smooth", C.c_int), ("bottom_maxiter", C.c_int),
+                ("bottom_rtol", C.c_double), ("bottom_atol", C.c_double), ("do_semicoarsening", C.c_int),
+                ("max_coarsening_level", C.c_int), ("precision", C.c_int)]
+
+
+class KsgfMLMGInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer_iters", C.c_int), ("r0", C.c_double), ("rfinal", C.c_double),
+                ("converged", C.c_int), ("nlevels", C.c_int), ("bottom_iters", C.c_int), ("bottom_fail", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# every symbol include/ksgf.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "ksgf_last_error", "ksgf_version", "ksgf_boxarray_create", "ksgf_boxarray_destroy", "ksgf_multifab_size",
+    "ksgf_fill_boundary", "ksgf_fill_physbc", "ksgf_set_inflow_velocity", "ksgf_ppm_predict_state_on_face",
+    "ksgf_extrap_vel_to_faces", "ksgf_compute_fluxes_from_state", "ksgf_compute_divergence",
+    "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
+    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_predictor_update",
+    "ksgf_icns_src_term", "ksgf_scale_by_rho",
+]
+
+_lib = None
+
+
+class KsgfError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libksgf.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (one HIP runtime for torch and the library)
+
+    path = path or LIBPATH
+    if not os.path.exists(path):
+        raise KsgfError(f"{path} is not built: run python kynema-sgf_amd/build.py")
+    L = C.CDLL(path)
+    L.ksgf_last_error.restype = C.c_char_p
+    L.ksgf_version.restype = C.c_char_p
+    L.ksgf_boxarray_create.argtypes = [C.POINTER(KsgfGeometry), _ip, C.c_int, C.POINTER(C.c_void_p)]
+    L.ksgf_boxarray_destroy.argtypes = [C.c_void_p]
+    L.ksgf_multifab_size.restype = C.c_longlong
+    L.ksgf_multifab_size.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.ksgf_fill_boundary.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_fill_physbc.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.POINTER(KsgfBCRec),
+                                   C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_set_inflow_velocity.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.POINTER(KsgfBCRec),
+                                           C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_ppm_predict_state_on_face.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int,
+                                                 C.POINTER(KsgfBCRec), C.c_double, C.c_int, _dp, _dp, C.c_void_p]
+    L.ksgf_extrap_vel_to_faces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(KsgfBCRec), C.c_double,
+                                           C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_compute_fluxes_from_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                                 _dp, _dp, _dp, _dp, _dp, C.c_double, C.POINTER(KsgfBCRec), _ip,
+                                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_compute_divergence.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_void_p]
+    L.ksgf_mlmg_default_options.argtypes = [C.POINTER(KsgfMLMGOptions)]
+    L.ksgf_mlmg_default_options.restype = None
+    L.ksgf_macproj_create.argtypes = [C.c_void_p, _ip, _ip, C.c_double, C.POINTER(KsgfMLMGOptions),
+                                      C.POINTER(C.c_void_p)]
+    L.ksgf_macproj_set_beta.argtypes = [C.c_void_p, C.c_double]
+    L.ksgf_macproj_project.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo),
+                                       C.c_void_p]
+    L.ksgf_macproj_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo),
+                                     C.c_void_p]
+    L.ksgf_macproj_get_phi.argtypes = [C.c_void_p, _dp, C.c_void_p]
+    L.ksgf_macproj_destroy.argtypes = [C.c_void_p]
+    L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
+                                        _dp, C.c_double, C.c_double, C.c_int, C.c_void_p]
+    L.ksgf_icns_src_term.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_scale_by_rho.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_void_p]
+    _lib = L
+    return L
+
+
+def check(rc, what="ksgf call", allow=(0,)):
+    if rc not in allow:
+        msg = load().ksgf_last_error().decode()
+        raise KsgfError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def _stream():
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous(), "expected a contiguous float64 device tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+def bcrecs(bcrec):
+    import numpy as np
+
+    bcrec = np.asarray(bcrec, dtype=np.int32).reshape(-1, 6)
+    arr = (KsgfBCRec * len(bcrec))()
+    for n, row in enumerate(bcrec):
+        for d in range(3):
+            arr[n].lo[d] = int(row[d])
+            arr[n].hi[d] = int(row[3 + d])
+    return arr
+
+
+def default_options(**kw):
+    o = KsgfMLMGOptions()
+    load().ksgf_mlmg_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class DeviceBoxArray:
+    """ksgf_boxarray handle for a layout.BoxArray."""
+
+    def __init__(self, ba: BoxArray):
+        import numpy as np
+
+        L = load()
+        self.ba = ba
+        g = KsgfGeometry()
+        for d in range(3):
+            g.domain_lo[d] = ba.geom.dlo[d]
+            g.domain_hi[d] = ba.geom.dhi[d]
+            g.prob_lo[d] = ba.geom.prob_lo[d]
+            g.prob_hi[d] = ba.geom.prob_hi[d]
+            g.is_periodic[d] = int(ba.geom.periodic[d])
+        self._boxes = np.ascontiguousarray(ba.boxes, dtype=np.int32)
+        h = C.c_void_p()
+        check(L.ksgf_boxarray_create(C.byref(g), self._boxes.ctypes.data_as(_ip), ba.nbox, C.byref(h)),
+              "ksgf_boxarray_create")
+        self.h = h
+
+    def size(self, itype, ncomp, ngrow):
+        n = load().ksgf_multifab_size(self.h, itype, ncomp, ngrow)
+        if n < 0:
+            raise KsgfError("ksgf_multifab_size failed")
+        return int(n)
+
+    def alloc(self, itype, ncomp, ngrow):
+        import torch
+
+        return torch.zeros(self.size(itype, ncomp, ngrow), dtype=torch.float64, device="cuda")
+
+    def upload(self, flat):
+        import torch
+
+        return torch.from_numpy(flat).to("cuda")
+
+    def close(self):
+        if self.h:
+            load().ksgf_boxarray_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- Hydro-level calls --------------------------------------------------
+    def fill_boundary(self, mf, itype, ncomp, ngrow):
+        check(load().ksgf_fill_boundary(self.h, _ptr(mf), itype, ncomp, ngrow, _stream()), "fill_boundary")
+
+    def fill_physbc(self, mf, itype, ncomp, ngrow, bcrec, bcval):
+        import numpy as np
+
+        v = np.ascontiguousarray(bcval, dtype=np.float64)
+        check(load().ksgf_fill_physbc(self.h, _ptr(mf), itype, ncomp, ngrow, bcrecs(bcrec),
+                                      v.ctypes.data_as(C.POINTER(C.c_double)), _stream()), "fill_physbc")
+
+    def fill_ghosts(self, mf, itype, ncomp, ngrow, bcrec, bcval):
+        self.fill_boundary(mf, itype, ncomp, ngrow)
+        self.fill_physbc(mf, itype, ncomp, ngrow, bcrec, bcval)
+
+    def set_inflow_velocity(self, mac, ngrow, bcrec, bcval):
+        import numpy as np
+
+        v = np.ascontiguousarray(bcval, dtype=np.float64)
+        check(load().ksgf_set_inflow_velocity(self.h, _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]), ngrow, bcrecs(bcrec),
+                                              v.ctypes.data_as(C.POINTER(C.c_double)), _stream()),
+              "set_inflow_velocity")
+
+    def ppm_predict_state(self, d, s, ncomp_s, ng_s, comp, vel_edge, ng_v, bcrec1, dt, limiter):
+        Im = self.alloc(CELL, 1, 0)
+        Ip = self.alloc(CELL, 1, 0)
+        check(load().ksgf_ppm_predict_state_on_face(self.h, d, _ptr(s), ncomp_s, ng_s, comp, _ptr(vel_edge), ng_v,
+                                                    bcrecs(bcrec1), dt, limiter, _ptr(Im), _ptr(Ip), _stream()),
+              "ppm_predict_state_on_face")
+        return Im, Ip
+
+    def extrap_vel_to_faces(self, vel, f, bcrec, dt, use_ppm=1, use_forces_in_trans=0, limiter=1, allow_inflow=0,
+                            mac=None):
+        mac = mac or [self.alloc(XFACE + d, 1, 1) for d in range(3)]
+        check(load().ksgf_extrap_vel_to_faces(self.h, _ptr(vel), _ptr(f), _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]),
+                                              bcrecs(bcrec), dt, use_ppm, use_forces_in_trans, limiter, allow_inflow,
+                                              _stream()), "extrap_vel_to_faces")
+        return mac
+
+    def compute_fluxes(self, ncomp, q, q_bc, mac, divu, fq, dt, bcrec, iconserv, use_ppm=1, use_forces_in_trans=0,
+                       is_velocity=1, area_weighted=0, limiter=1, allow_inflow=0, known_edge_state=0, faces=None,
+                       flux=None):
+        import numpy as np
+
+        flux = flux or [self.alloc(XFACE + d, ncomp, 0) for d in range(3)]
+        face = faces or [self.alloc(XFACE + d, ncomp, 0) for d in range(3)]
+        ic = np.ascontiguousarray(iconserv, dtype=np.int32)
+        check(load().ksgf_compute_fluxes_from_state(
+            self.h, ncomp, _ptr(q), _ptr(q_bc), _ptr(flux[0]), _ptr(flux[1]), _ptr(flux[2]), _ptr(face[0]),
+            _ptr(face[1]), _ptr(face[2]), known_edge_state, _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]), _ptr(divu),
+            _ptr(fq), dt, bcrecs(bcrec), ic.ctypes.data_as(_ip), use_ppm, use_forces_in_trans, is_velocity,
+            area_weighted, limiter, allow_inflow, _stream()), "compute_fluxes_from_state")
+        return flux, face
+
+    def compute_divergence(self, flux, ncomp, mult=-1.0, area_weighted=0, out=None):
+        div = out if out is not None else self.alloc(CELL, ncomp, 0)
+        check(load().ksgf_compute_divergence(self.h, _ptr(div), _ptr(flux[0]), _ptr(flux[1]), _ptr(flux[2]), ncomp,
+                                             mult, area_weighted, _stream()), "compute_divergence")
+        return div
+
+    def cfl_max(self, vel, ngrow):
+        out = C.c_double()
+        check(load().ksgf_cfl_max(self.h, _ptr(vel), ngrow, C.byref(out), _stream()), "cfl_max")
+        return out.value
+
+    def predictor_update(self, vel_new, ng_new, vel_old, ng_old, rho_old, ng_ro, rho_new, ng_rn, conv, src, diff,
+                         kappa, dt, ncomp=3):
+        check(load().ksgf_predictor_update(self.h, _ptr(vel_new), ng_new, _ptr(vel_old), ng_old, _ptr(rho_old), ng_ro,
+                                           _ptr(rho_new), ng_rn, _ptr(conv), _ptr(src), _ptr(diff), kappa, dt, ncomp,
+                                           _stream()), "predictor_update")
+
+    def icns_src_term(self, src, ng_src, gp, rho, ng_rho, scale_by_rho):
+        check(load().ksgf_icns_src_term(self.h, _ptr(src), ng_src, _ptr(gp), _ptr(rho), ng_rho, scale_by_rho,
+                                        _stream()), "icns_src_term")
+
+    def scale_by_rho(self, out, ng_out, a, ng_a, rho, ng_rho, ncomp, ngrow, divide=0):
+        check(load().ksgf_scale_by_rho(self.h, _ptr(out), ng_out, _ptr(a), ng_a, _ptr(rho), ng_rho, ncomp, ngrow,
+                                       divide, _stream()), "scale_by_rho")
+
+
+class MacProjector:
+    """Hydro::MacProjector over a DeviceBoxArray."""
+
+    def __init__(self, dba: DeviceBoxArray, lobc, hibc, beta=1.0, options=None):
+        import numpy as np
+
+        L = load()
+        self.dba = dba
+        self._lo = np.ascontiguousarray(lobc, dtype=np.int32)
+        self._hi = np.ascontiguousarray(hibc, dtype=np.int32)
+        self.options = options or default_options()
+        h = C.c_void_p()
+        check(L.ksgf_macproj_create(dba.h, self._lo.ctypes.data_as(_ip), self._hi.ctypes.data_as(_ip), beta,
+                                    C.byref(self.options), C.byref(h)), "ksgf_macproj_create")
+        self.h = h
+        self.info = None
+
+    def update_beta(self, beta):
+        check(load().ksgf_macproj_set_beta(self.h, beta), "macproj_set_beta")
+
+    def project(self, mac, rtol, atol):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_macproj_project(self.h, _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]), rtol, atol, C.byref(info),
+                                          _stream()), "macproj_project", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def solve(self, rhs, phi, rtol, atol):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_macproj_solve(self.h, _ptr(rhs), _ptr(phi), rtol, atol, C.byref(info), _stream()),
+              "macproj_solve", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def phi(self):
+        import torch
+
+        n = int(torch.tensor(self.dba.ba.geom.n_cell).prod())
+        out = torch.zeros(n, dtype=torch.float64, device="cuda")
+        check(load().ksgf_macproj_get_phi(self.h, _ptr(out), _stream()), "macproj_get_phi")
+        return out
+
+    def close(self):
+        if self.h:
+            load().ksgf_macproj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

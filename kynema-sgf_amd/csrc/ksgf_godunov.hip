@@ -1,0 +1,862 @@
+// Godunov PPM advection on a ragged BoxArray:
+//   HydroUtils::ExtrapVelToFaces            (call site icns_advection.H:211-216)
+//   HydroUtils::ComputeFluxesOnBoxFromState (icns_advection.H:359-373)
+//   HydroUtils::ComputeDivergence           (icns_advection.H:418-422)
+//   PPM::PredictStateOn{X,Y,Z}Face          (test_mflux_schemes.cpp:87-151)
+//
+// Each predictor stage is one launch over every box of the BoxArray
+// (flat index space, binary search of the box from a prefix table), with the
+// stage results kept in per-BoxArray scratch MultiFabs:
+//   stage 0  PPM traces Im/Ip of every component in every direction
+//   stage 1  normal face states, advective velocity, upwinded edge states
+//   stage 2  corner-coupled (transverse-corrected) face states
+//   stage 3  final face states (and fluxes)
+#include "ksgf_internal.h"
+#include "ksgf_ppm.h"
+
+namespace ksgf {
+
+constexpr int kMaxComp = 8;
+
+struct BCs {
+    int lo[kMaxComp][3];
+    int hi[kMaxComp][3];
+};
+struct GeomDev {
+    int dlo[3], dhi[3];
+};
+struct MF3 {
+    MFDev m[3];
+};
+struct CMF3 {
+    CMFDev m[3];
+};
+struct Coef {
+    double v[3];
+};
+
+__device__ __forceinline__ CMFDev cview(const MFDev& m)
+{
+    CMFDev c;
+    c.p = m.p;
+    c.pre = m.pre;
+    c.itype = m.itype;
+    c.ngrow = m.ngrow;
+    c.ncomp = m.ncomp;
+    return c;
+}
+
+template <int D>
+__device__ __forceinline__ void unit(int& a0, int& a1, int& a2, int s)
+{
+    if (D == 0) a0 += s;
+    if (D == 1) a1 += s;
+    if (D == 2) a2 += s;
+}
+template <int D>
+__device__ __forceinline__ int pick(int a0, int a1, int a2)
+{
+    return D == 0 ? a0 : (D == 1 ? a1 : a2);
+}
+
+// ---------------------------------------------------------------------------
+// stage 0: PPM traces (cells of the boxes grown by one)
+// ---------------------------------------------------------------------------
+// mode 0: velocity prediction, advecting velocity = cell-centred normal
+//         component of vel (PredictVelOnFaces);
+// mode 1: state prediction with the face velocities umac/vmac/wmac.
+template <int D>
+__global__ void __launch_bounds__(256) k_ppm(CMFDev q, int ncomp, CMF3 mac, int mode, MFDev Im, MFDev Ip,
+                                             RegDev r, const int* __restrict__ box, BCs bc, GeomDev g,
+                                             double dtdx, int lim)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    CFab s = fab_at(q, box, b);
+    const long long st = (D == 0) ? 1 : (D == 1 ? (long long)s.n0 : (long long)s.n0 * s.n1);
+    const long long c = s.idx(i, j, k);
+    double um, up;
+    if (mode == 0) {
+        um = s.p[c + D * s.cs];
+        up = um;
+    } else {
+        CFab m = fab_at(mac.m[D], box, b);
+        int i1 = i, j1 = j, k1 = k;
+        unit<D>(i1, j1, k1, 1);
+        um = m(i, j, k);
+        up = m(i1, j1, k1);
+    }
+    Fab fm = fab_at(Im, box, b);
+    Fab fp = fab_at(Ip, box, b);
+    const int x = pick<D>(i, j, k);
+    for (int n = 0; n < ncomp; ++n) {
+        const double* pn = s.p + c + n * s.cs;
+        double a, bb;
+        ppm_trace(pn[-2 * st], pn[-st], pn[0], pn[st], pn[2 * st], lim, bc.lo[n][D], bc.hi[n][D], x, g.dlo[D],
+                  g.dhi[D], um, up, dtdx, a, bb);
+        fm(i, j, k, n) = a;
+        fp(i, j, k, n) = bb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage 1: normal face states on faces of direction D grown by one
+// transversely.  vel_mode: ExtrapVelToFaces (advective velocity from the
+// Burgers problem of the normal component, both sides Dirichlet);
+// otherwise the MAC velocity upwinds and is_velocity selects both-side
+// Dirichlet for the normal component only.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_face(CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev Im, CMFDev Ip,
+                                              CMFDev umacD, MFDev flo, MFDev fhi, MFDev fed, MFDev uad,
+                                              RegDev r, const int* __restrict__ box, BCs bc, GeomDev g,
+                                              double hdt, int uft, int vel_mode, int is_velocity)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    int qi = i, qj = j, qk = k;
+    unit<D>(qi, qj, qk, -1);
+    CFab fim = fab_at(Im, box, b);
+    CFab fip = fab_at(Ip, box, b);
+    CFab ff = fab_at(f, box, b);
+    CFab fqb = fab_at(qbc, box, b);
+    Fab flo_ = fab_at(flo, box, b);
+    Fab fhi_ = fab_at(fhi, box, b);
+    Fab fed_ = fab_at(fed, box, b);
+    const int x = pick<D>(i, j, k);
+    int gli = i, glj = j, glk = k, ghi_ = i, ghj = j, ghk = k;
+    if (D == 0) { gli = g.dlo[0] - 1; ghi_ = g.dhi[0] + 1; }
+    if (D == 1) { glj = g.dlo[1] - 1; ghj = g.dhi[1] + 1; }
+    if (D == 2) { glk = g.dlo[2] - 1; ghk = g.dhi[2] + 1; }
+    double u;
+    if (vel_mode) {
+        // normal component first: advective velocity
+        double lov = fip(qi, qj, qk, D);
+        double hiv = fim(i, j, k, D);
+        if (uft) {
+            lov += hdt * ff(qi, qj, qk, D);
+            hiv += hdt * ff(i, j, k, D);
+        }
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = fqb(gli, glj, glk, D);
+        if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, D);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[D][D], bc.hi[D][D], glo, ghi, true, lov, hiv);
+        u = burgers(lov, hiv);
+        Fab fu = fab_at(uad, box, b);
+        fu(i, j, k) = u;
+    } else {
+        CFab m = fab_at(umacD, box, b);
+        u = m(i, j, k);
+    }
+    for (int n = 0; n < ncomp; ++n) {
+        double lov = fip(qi, qj, qk, n);
+        double hiv = fim(i, j, k, n);
+        if (uft) {
+            lov += hdt * ff(qi, qj, qk, n);
+            hiv += hdt * ff(i, j, k, n);
+        }
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = fqb(gli, glj, glk, n);
+        if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, n);
+        bool both = vel_mode ? true : (is_velocity && n == D);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[n][D], bc.hi[n][D], glo, ghi, both, lov, hiv);
+        flo_(i, j, k, n) = lov;
+        fhi_(i, j, k, n) = hiv;
+        fed_(i, j, k, n) = upwind(lov, hiv, u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage 2: corner-coupled states on faces normal to D corrected along T.
+// vel_mode: only component E = 3-D-T, advective form with the predicted
+// advective velocities (dt/6 factor);  otherwise all components with the
+// MAC velocities, conservative (dt/3, flux difference + q * transverse
+// divergence) or advective (dt/6) per component.
+// ---------------------------------------------------------------------------
+template <int D, int T>
+__global__ void __launch_bounds__(256) k_corner(CMFDev q, CMFDev qbc, int ncomp, CMFDev flo, CMFDev fhi,
+                                                CMFDev fedT, CMFDev velT, CMFDev velD, MFDev crn, RegDev r,
+                                                const int* __restrict__ box, BCs bc, GeomDev g, double c3,
+                                                double c6, int vel_mode, int is_velocity,
+                                                const int* __restrict__ iconserv_unused, int cons_mask)
+{
+    constexpr int E = 3 - D - T;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    int qi = i, qj = j, qk = k;
+    unit<D>(qi, qj, qk, -1);
+    int qti = qi, qtj = qj, qtk = qk;
+    unit<T>(qti, qtj, qtk, 1);
+    int pti = i, ptj = j, ptk = k;
+    unit<T>(pti, ptj, ptk, 1);
+    CFab vt = fab_at(velT, box, b);
+    CFab et = fab_at(fedT, box, b);
+    CFab lo_ = fab_at(flo, box, b);
+    CFab hi_ = fab_at(fhi, box, b);
+    CFab vd = fab_at(velD, box, b);
+    CFab fqb = fab_at(qbc, box, b);
+    Fab out = fab_at(crn, box, b);
+    const int x = pick<D>(i, j, k);
+    int gli = i, glj = j, glk = k, ghi_ = i, ghj = j, ghk = k;
+    if (D == 0) { gli = g.dlo[0] - 1; ghi_ = g.dhi[0] + 1; }
+    if (D == 1) { glj = g.dlo[1] - 1; ghj = g.dhi[1] + 1; }
+    if (D == 2) { glk = g.dlo[2] - 1; ghk = g.dhi[2] + 1; }
+    const double wql = vt(qi, qj, qk), wqh = vt(qti, qtj, qtk);
+    const double wpl = vt(i, j, k), wph = vt(pti, ptj, ptk);
+    const double u = vd(i, j, k);
+    if (vel_mode) {
+        double lov = lo_(i, j, k, E) - c6 * (wqh + wql) * (et(qti, qtj, qtk, E) - et(qi, qj, qk, E));
+        double hiv = hi_(i, j, k, E) - c6 * (wph + wpl) * (et(pti, ptj, ptk, E) - et(i, j, k, E));
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = fqb(gli, glj, glk, E);
+        if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, E);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[E][D], bc.hi[E][D], glo, ghi, true, lov, hiv);
+        out(i, j, k) = upwind(lov, hiv, u);
+        return;
+    }
+    CFab fq = fab_at(q, box, b);
+    for (int n = 0; n < ncomp; ++n) {
+        double eql = et(qi, qj, qk, n), eqh = et(qti, qtj, qtk, n);
+        double epl = et(i, j, k, n), eph = et(pti, ptj, ptk, n);
+        double lov, hiv;
+        if ((cons_mask >> n) & 1) {
+            double ql = fq(qi, qj, qk, n), qh = fq(i, j, k, n);
+            lov = lo_(i, j, k, n) - c3 * (wqh * eqh - wql * eql) + c3 * ql * (wqh - wql);
+            hiv = hi_(i, j, k, n) - c3 * (wph * eph - wpl * epl) + c3 * qh * (wph - wpl);
+        } else {
+            lov = lo_(i, j, k, n) - c6 * (wqh + wql) * (eqh - eql);
+            hiv = hi_(i, j, k, n) - c6 * (wph + wpl) * (eph - epl);
+        }
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = fqb(gli, glj, glk, n);
+        if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, n);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[n][D], bc.hi[n][D], glo, ghi, is_velocity && n == D, lov, hiv);
+        out(i, j, k, n) = upwind(lov, hiv, u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage 3: final states on the valid faces of direction D.
+// crnA: corner states normal to T1 = (D+1)%3 corrected along T2;
+// crnB: normal to T2 corrected along T1.
+// vel_mode: normal velocity only -> u_mac (Burgers);  otherwise all
+// components -> face states and fluxes upwinded by the MAC velocity.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_final(CMFDev q, CMFDev qbc, CMFDev f, CMFDev divu, int has_divu,
+                                               int ncomp, CMFDev flo, CMFDev fhi, CMFDev crnA, CMFDev crnB,
+                                               CMFDev vel1, CMFDev vel2, CMFDev velD, MFDev out_state,
+                                               MFDev out_flux, RegDev r, const int* __restrict__ box, BCs bc,
+                                               GeomDev g, double c2a, double c2b, double c4a, double c4b,
+                                               double hdt, double area, int uft, int vel_mode, int is_velocity,
+                                               int area_weighted, int allow_inflow, int cons_mask)
+{
+    constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    int qi = i, qj = j, qk = k;
+    unit<D>(qi, qj, qk, -1);
+    int q1i = qi, q1j = qj, q1k = qk, p1i = i, p1j = j, p1k = k;
+    unit<T1>(q1i, q1j, q1k, 1);
+    unit<T1>(p1i, p1j, p1k, 1);
+    int q2i = qi, q2j = qj, q2k = qk, p2i = i, p2j = j, p2k = k;
+    unit<T2>(q2i, q2j, q2k, 1);
+    unit<T2>(p2i, p2j, p2k, 1);
+    CFab v1 = fab_at(vel1, box, b);
+    CFab v2 = fab_at(vel2, box, b);
+    CFab A = fab_at(crnA, box, b);
+    CFab B = fab_at(crnB, box, b);
+    CFab lo_ = fab_at(flo, box, b);
+    CFab hi_ = fab_at(fhi, box, b);
+    CFab ff = fab_at(f, box, b);
+    CFab fqb = fab_at(qbc, box, b);
+    const int x = pick<D>(i, j, k);
+    int gli = i, glj = j, glk = k, ghi_ = i, ghj = j, ghk = k;
+    if (D == 0) { gli = g.dlo[0] - 1; ghi_ = g.dhi[0] + 1; }
+    if (D == 1) { glj = g.dlo[1] - 1; ghj = g.dhi[1] + 1; }
+    if (D == 2) { glk = g.dlo[2] - 1; ghk = g.dhi[2] + 1; }
+    const double v1ql = v1(qi, qj, qk), v1qh = v1(q1i, q1j, q1k);
+    const double v1pl = v1(i, j, k), v1ph = v1(p1i, p1j, p1k);
+    const double v2ql = v2(qi, qj, qk), v2qh = v2(q2i, q2j, q2k);
+    const double v2pl = v2(i, j, k), v2ph = v2(p2i, p2j, p2k);
+    if (vel_mode) {
+        double stl = lo_(i, j, k, D) - c4a * (v1qh + v1ql) * (A(q1i, q1j, q1k) - A(qi, qj, qk)) -
+                     c4b * (v2qh + v2ql) * (B(q2i, q2j, q2k) - B(qi, qj, qk));
+        double sth = hi_(i, j, k, D) - c4a * (v1ph + v1pl) * (A(p1i, p1j, p1k) - A(i, j, k)) -
+                     c4b * (v2ph + v2pl) * (B(p2i, p2j, p2k) - B(i, j, k));
+        if (!uft) {
+            stl += hdt * ff(qi, qj, qk, D);
+            sth += hdt * ff(i, j, k, D);
+        }
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = fqb(gli, glj, glk, D);
+        if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, D);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[D][D], bc.hi[D][D], glo, ghi, true, stl, sth);
+        if (!allow_inflow) {
+            int tl = bc.lo[D][D], th = bc.hi[D][D];
+            if (x == g.dlo[D] && (tl == KSGF_BC_FOEXTRAP || tl == KSGF_BC_HOEXTRAP)) {
+                sth = fmin(sth, 0.0);
+                stl = sth;
+            }
+            if (x == g.dhi[D] + 1 && (th == KSGF_BC_FOEXTRAP || th == KSGF_BC_HOEXTRAP)) {
+                stl = fmax(stl, 0.0);
+                sth = stl;
+            }
+        }
+        Fab o = fab_at(out_state, box, b);
+        o(i, j, k) = burgers(stl, sth);
+        return;
+    }
+    CFab fq = fab_at(q, box, b);
+    CFab vd = fab_at(velD, box, b);
+    const double u = vd(i, j, k);
+    Fab os = fab_at(out_state, box, b);
+    Fab of = fab_at(out_flux, box, b);
+    double dql = 0.0, dqh = 0.0;
+    if (has_divu) {
+        CFab dv = fab_at(divu, box, b);
+        dql = dv(qi, qj, qk);
+        dqh = dv(i, j, k);
+    }
+    for (int n = 0; n < ncomp; ++n) {
+        double a_ql = A(qi, qj, qk, n), a_qh = A(q1i, q1j, q1k, n);
+        double a_pl = A(i, j, k, n), a_ph = A(p1i, p1j, p1k, n);
+        double b_ql = B(qi, qj, qk, n), b_qh = B(q2i, q2j, q2k, n);
+        double b_pl = B(i, j, k, n), b_ph = B(p2i, p2j, p2k, n);
+        double stl, sth;
+        if ((cons_mask >> n) & 1) {
+            double ql = fq(qi, qj, qk, n), qh = fq(i, j, k, n);
+            stl = lo_(i, j, k, n) - c2a * (v1qh * a_qh - v1ql * a_ql) - c2b * (v2qh * b_qh - v2ql * b_ql) +
+                  c2a * ql * (v1qh - v1ql) + c2b * ql * (v2qh - v2ql);
+            sth = hi_(i, j, k, n) - c2a * (v1ph * a_ph - v1pl * a_pl) - c2b * (v2ph * b_ph - v2pl * b_pl) +
+                  c2a * qh * (v1ph - v1pl) + c2b * qh * (v2ph - v2pl);
+            if (has_divu) {
+                stl -= hdt * ql * dql;
+                sth -= hdt * qh * dqh;
+            }
+        } else {
+            stl = lo_(i, j, k, n) - c4a * (v1qh + v1ql) * (a_qh - a_ql) - c4b * (v2qh + v2ql) * (b_qh - b_ql);
+            sth = hi_(i, j, k, n) - c4a * (v1ph + v1pl) * (a_ph - a_pl) - c4b * (v2ph + v2pl) * (b_ph - b_pl);
+        }
+        if (!uft) {
+            stl += hdt * ff(qi, qj, qk, n);
+            sth += hdt * ff(i, j, k, n);
+        }
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = fqb(gli, glj, glk, n);
+        if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, n);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[n][D], bc.hi[n][D], glo, ghi, is_velocity && n == D, stl, sth);
+        if (is_velocity && n == D && !allow_inflow) {
+            int tl = bc.lo[n][D], th = bc.hi[n][D];
+            if (x == g.dlo[D] && (tl == KSGF_BC_FOEXTRAP || tl == KSGF_BC_HOEXTRAP)) {
+                sth = fmin(sth, 0.0);
+                stl = sth;
+            }
+            if (x == g.dhi[D] + 1 && (th == KSGF_BC_FOEXTRAP || th == KSGF_BC_HOEXTRAP)) {
+                stl = fmax(stl, 0.0);
+                sth = stl;
+            }
+        }
+        double e = upwind(stl, sth, u);
+        os(i, j, k, n) = e;
+        of(i, j, k, n) = area_weighted ? (u * area) * e : u * e;
+    }
+}
+
+// known edge state: flux = u * face
+template <int D>
+__global__ void __launch_bounds__(256) k_known_flux(CMFDev face, CMFDev velD, MFDev flux, int ncomp, RegDev r,
+                                                    const int* __restrict__ box, double area, int area_weighted)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    CFab fc = fab_at(face, box, b);
+    CFab vd = fab_at(velD, box, b);
+    Fab fl = fab_at(flux, box, b);
+    double u = vd(i, j, k);
+    for (int n = 0; n < ncomp; ++n) fl(i, j, k, n) = area_weighted ? (u * area) * fc(i, j, k, n) : u * fc(i, j, k, n);
+}
+
+__global__ void __launch_bounds__(256) k_divergence(MFDev div, CMF3 fl, int ncomp, RegDev r,
+                                                    const int* __restrict__ box, double mult, double dxi,
+                                                    double dyi, double dzi, double vol, int area_weighted)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    Fab o = fab_at(div, box, b);
+    CFab fx = fab_at(fl.m[0], box, b);
+    CFab fy = fab_at(fl.m[1], box, b);
+    CFab fz = fab_at(fl.m[2], box, b);
+    for (int n = 0; n < ncomp; ++n) {
+        double r_;
+        if (area_weighted)
+            r_ = mult * ((fx(i + 1, j, k, n) - fx(i, j, k, n)) + (fy(i, j + 1, k, n) - fy(i, j, k, n)) +
+                         (fz(i, j, k + 1, n) - fz(i, j, k, n))) /
+                 vol;
+        else
+            r_ = mult * ((fx(i + 1, j, k, n) - fx(i, j, k, n)) * dxi + (fy(i, j + 1, k, n) - fy(i, j, k, n)) * dyi +
+                         (fz(i, j, k + 1, n) - fz(i, j, k, n)) * dzi);
+        o(i, j, k, n) = r_;
+    }
+}
+
+// PredictStateOn{X,Y,Z}Face for one component on valid cells
+template <int D>
+__global__ void __launch_bounds__(256) k_ppm_predict(CMFDev s, int comp, CMFDev vel, MFDev Im, MFDev Ip,
+                                                     RegDev r, const int* __restrict__ box, int bclo, int bchi,
+                                                     int dlo, int dhi, double dtdx, int lim)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    CFab sf = fab_at(s, box, b);
+    CFab vf = fab_at(vel, box, b);
+    const long long st = (D == 0) ? 1 : (D == 1 ? (long long)sf.n0 : (long long)sf.n0 * sf.n1);
+    const double* pn = sf.p + sf.idx(i, j, k) + comp * sf.cs;
+    int i1 = i, j1 = j, k1 = k;
+    unit<D>(i1, j1, k1, 1);
+    double a, bb;
+    ppm_trace(pn[-2 * st], pn[-st], pn[0], pn[st], pn[2 * st], lim, bclo, bchi, pick<D>(i, j, k), dlo, dhi,
+              vf(i, j, k), vf(i1, j1, k1), dtdx, a, bb);
+    Fab fm = fab_at(Im, box, b);
+    Fab fp = fab_at(Ip, box, b);
+    fm(i, j, k) = a;
+    fp(i, j, k) = bb;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int check_lim(int lim)
+{
+    if (lim == KSGF_PPM_WENOZ || lim == KSGF_PPM_WENO_JS) {
+        set_error("WENO limiters are not provided by this build");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    if (lim != KSGF_PPM_NOLIMITER && lim != KSGF_PPM_VANLEER && lim != KSGF_PPM_UPWIND && lim != KSGF_PPM_MINMOD) {
+        set_error("unknown limiter_type");
+        return KSGF_ERR_INVALID;
+    }
+    return KSGF_OK;
+}
+
+static int check_bc(const ksgf_bcrec* bc, int ncomp, const ksgf_boxarray* ba)
+{
+    for (int n = 0; n < ncomp; ++n)
+        for (int d = 0; d < 3; ++d) {
+            int t[2] = {bc[n].lo[d], bc[n].hi[d]};
+            for (int s = 0; s < 2; ++s) {
+                if (t[s] != KSGF_BC_INT_DIR && t[s] != KSGF_BC_FOEXTRAP && t[s] != KSGF_BC_EXT_DIR &&
+                    t[s] != KSGF_BC_HOEXTRAP) {
+                    set_error("unsupported BCType (only int_dir, foextrap, ext_dir, hoextrap)");
+                    return KSGF_ERR_UNSUPPORTED;
+                }
+                if (ba->periodic[d] && t[s] != KSGF_BC_INT_DIR) {
+                    set_error("periodic direction requires int_dir BCs");
+                    return KSGF_ERR_INVALID;
+                }
+            }
+        }
+    return KSGF_OK;
+}
+
+static BCs make_bcs(const ksgf_bcrec* bc, int ncomp)
+{
+    BCs b;
+    for (int n = 0; n < kMaxComp; ++n)
+        for (int d = 0; d < 3; ++d) {
+            b.lo[n][d] = (n < ncomp) ? bc[n].lo[d] : 0;
+            b.hi[n][d] = (n < ncomp) ? bc[n].hi[d] : 0;
+        }
+    return b;
+}
+
+static GeomDev make_geom(const ksgf_boxarray* ba)
+{
+    GeomDev g;
+    for (int d = 0; d < 3; ++d) {
+        g.dlo[d] = ba->dlo[d];
+        g.dhi[d] = ba->dhi[d];
+    }
+    return g;
+}
+
+static CMFDev ccast(const MFDev& m)
+{
+    CMFDev c;
+    c.p = m.p;
+    c.pre = m.pre;
+    c.itype = m.itype;
+    c.ngrow = m.ngrow;
+    c.ncomp = m.ncomp;
+    return c;
+}
+
+// scratch slot map
+enum {
+    S_IM = 0,    // 3 slots
+    S_IP = 3,    // 3
+    S_LO = 6,    // 3
+    S_HI = 9,    // 3
+    S_ED = 12,   // 3
+    S_UAD = 15,  // 3
+    S_CRN = 18,  // 9 (d*3+t)
+};
+
+struct Stages {
+    MFDev Im[3], Ip[3], lo[3], hi[3], ed[3], uad[3], crn[3][3];
+};
+
+static int alloc_stages(const ksgf_boxarray* ba, int ncomp, int ncrn, Stages& S)
+{
+    int err = KSGF_OK;
+    const Region* rc = fab_region(ba, KSGF_CELL, 1, &err);
+    if (!rc) return err;
+    for (int d = 0; d < 3; ++d) {
+        const Region* rf = fab_region(ba, KSGF_XFACE + d, 1, &err);
+        if (!rf) return err;
+        double* p;
+        if (!(p = get_scratch(ba, S_IM + d, (size_t)rc->total * ncomp, &err))) return err;
+        S.Im[d] = {p, rc->pre, KSGF_CELL, 1, ncomp};
+        if (!(p = get_scratch(ba, S_IP + d, (size_t)rc->total * ncomp, &err))) return err;
+        S.Ip[d] = {p, rc->pre, KSGF_CELL, 1, ncomp};
+        if (!(p = get_scratch(ba, S_LO + d, (size_t)rf->total * ncomp, &err))) return err;
+        S.lo[d] = {p, rf->pre, KSGF_XFACE + d, 1, ncomp};
+        if (!(p = get_scratch(ba, S_HI + d, (size_t)rf->total * ncomp, &err))) return err;
+        S.hi[d] = {p, rf->pre, KSGF_XFACE + d, 1, ncomp};
+        if (!(p = get_scratch(ba, S_ED + d, (size_t)rf->total * ncomp, &err))) return err;
+        S.ed[d] = {p, rf->pre, KSGF_XFACE + d, 1, ncomp};
+        if (!(p = get_scratch(ba, S_UAD + d, (size_t)rf->total, &err))) return err;
+        S.uad[d] = {p, rf->pre, KSGF_XFACE + d, 1, 1};
+        for (int t = 0; t < 3; ++t) {
+            if (t == d) continue;
+            if (!(p = get_scratch(ba, S_CRN + 3 * d + t, (size_t)rf->total * ncrn, &err))) return err;
+            S.crn[d][t] = {p, rf->pre, KSGF_XFACE + d, 1, ncrn};
+        }
+    }
+    return KSGF_OK;
+}
+
+template <int D>
+static int launch_ppm(const ksgf_boxarray* ba, CMFDev q, int ncomp, CMF3 mac, int mode, const Stages& S,
+                      const BCs& bc, const GeomDev& g, double dtdx, int lim, hipStream_t st)
+{
+    int err = KSGF_OK;
+    int one[3] = {1, 1, 1};
+    const Region* r = get_region(ba, one, one, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_ppm<D>, dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, ncomp, mac, mode, S.Im[D],
+                       S.Ip[D], regdev(r, ba->nbox), ba->dbox, bc, g, dtdx, lim);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+template <int D>
+static int launch_face(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev umacD,
+                       const Stages& S, const BCs& bc, const GeomDev& g, double hdt, int uft, int vel_mode,
+                       int is_velocity, hipStream_t st)
+{
+    int err = KSGF_OK;
+    int glo[3], ghi[3];
+    for (int e = 0; e < 3; ++e) {
+        glo[e] = (e == D) ? 0 : 1;
+        ghi[e] = 1;
+    }
+    const Region* r = get_region(ba, glo, ghi, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_face<D>, dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, qbc, f, ncomp,
+                       ccast(S.Im[D]), ccast(S.Ip[D]), umacD, S.lo[D], S.hi[D], S.ed[D], S.uad[D],
+                       regdev(r, ba->nbox), ba->dbox, bc, g, hdt, uft, vel_mode, is_velocity);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+template <int D, int T>
+static int launch_corner(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, int ncomp, CMFDev velT, CMFDev velD,
+                         const Stages& S, const BCs& bc, const GeomDev& g, double c3, double c6, int vel_mode,
+                         int is_velocity, int cons_mask, hipStream_t st)
+{
+    constexpr int E = 3 - D - T;
+    int err = KSGF_OK;
+    int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+    ghi[D] = 1;
+    glo[E] = 1;
+    ghi[E] += 1;
+    const Region* r = get_region(ba, glo, ghi, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL((k_corner<D, T>), dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, qbc, ncomp,
+                       ccast(S.lo[D]), ccast(S.hi[D]), ccast(S.ed[T]), velT, velD, S.crn[D][T],
+                       regdev(r, ba->nbox), ba->dbox, bc, g, c3, c6, vel_mode, is_velocity, (const int*)nullptr,
+                       cons_mask);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+template <int D>
+static int launch_final(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, CMFDev divu, int has_divu,
+                        int ncomp, CMFDev vel1, CMFDev vel2, CMFDev velD, MFDev out_state, MFDev out_flux,
+                        const Stages& S, const BCs& bc, const GeomDev& g, const double* c2, const double* c4,
+                        double hdt, double area, int uft, int vel_mode, int is_velocity, int area_weighted,
+                        int allow_inflow, int cons_mask, hipStream_t st)
+{
+    constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+    int err = KSGF_OK;
+    int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+    ghi[D] = 1;
+    const Region* r = get_region(ba, glo, ghi, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_final<D>, dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, qbc, f, divu, has_divu,
+                       ncomp, ccast(S.lo[D]), ccast(S.hi[D]), ccast(S.crn[T1][T2]), ccast(S.crn[T2][T1]), vel1,
+                       vel2, velD, out_state, out_flux, regdev(r, ba->nbox), ba->dbox, bc, g, c2[T1], c2[T2],
+                       c4[T1], c4[T2], hdt, area, uft, vel_mode, is_velocity, area_weighted, allow_inflow,
+                       cons_mask);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+#define KSGF_TRY(x)                 \
+    do {                            \
+        int e__ = (x);              \
+        if (e__ != KSGF_OK) return e__; \
+    } while (0)
+
+}  // namespace ksgf
+
+using namespace ksgf;
+
+extern "C" {
+
+int ksgf_ppm_predict_state_on_face(const ksgf_boxarray* ba, int dir, const double* s, int ncomp_s, int ng_s,
+                                   int comp, const double* vel_edge, int ng_v, const ksgf_bcrec* bc, double dt,
+                                   int limiter_type, double* Im, double* Ip, ksgf_stream_t stream)
+{
+    if (!ba || !s || !vel_edge || !bc || !Im || !Ip || dir < 0 || dir > 2 || comp < 0 || comp >= ncomp_s ||
+        ng_s < 2 || ng_v < 0) {
+        set_error("ksgf_ppm_predict_state_on_face: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    KSGF_TRY(check_lim(limiter_type));
+    KSGF_TRY(check_bc(bc, 1, ba));
+    CMFDev sv, vv;
+    MFDev im, ip;
+    KSGF_TRY(cmfdev(ba, s, KSGF_CELL, ncomp_s, ng_s, &sv));
+    KSGF_TRY(cmfdev(ba, vel_edge, KSGF_XFACE + dir, 1, ng_v, &vv));
+    KSGF_TRY(mfdev(ba, Im, KSGF_CELL, 1, 0, &im));
+    KSGF_TRY(mfdev(ba, Ip, KSGF_CELL, 1, 0, &ip));
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);
+    if (!r) return err;
+    double dtdx = dt / ba->dx[dir];
+    hipStream_t st = (hipStream_t)stream;
+    dim3 gr(grid_for(r->total, 256));
+    if (dir == 0)
+        hipLaunchKernelGGL(k_ppm_predict<0>, gr, dim3(256), 0, st, sv, comp, vv, im, ip, regdev(r, ba->nbox),
+                           ba->dbox, bc->lo[0], bc->hi[0], ba->dlo[0], ba->dhi[0], dtdx, limiter_type);
+    else if (dir == 1)
+        hipLaunchKernelGGL(k_ppm_predict<1>, gr, dim3(256), 0, st, sv, comp, vv, im, ip, regdev(r, ba->nbox),
+                           ba->dbox, bc->lo[1], bc->hi[1], ba->dlo[1], ba->dhi[1], dtdx, limiter_type);
+    else
+        hipLaunchKernelGGL(k_ppm_predict<2>, gr, dim3(256), 0, st, sv, comp, vv, im, ip, regdev(r, ba->nbox),
+                           ba->dbox, bc->lo[2], bc->hi[2], ba->dlo[2], ba->dhi[2], dtdx, limiter_type);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+int ksgf_extrap_vel_to_faces(const ksgf_boxarray* ba, const double* vel, const double* vel_forces, double* u_mac,
+                             double* v_mac, double* w_mac, const ksgf_bcrec* bcrec, double dt, int use_ppm,
+                             int use_forces_in_trans, int limiter_type, int allow_inflow_on_outflow,
+                             ksgf_stream_t stream)
+{
+    if (!ba || !vel || !vel_forces || !u_mac || !v_mac || !w_mac || !bcrec) {
+        set_error("ksgf_extrap_vel_to_faces: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (!use_ppm) {
+        set_error("ksgf_extrap_vel_to_faces: PLM (use_ppm = false) is not provided by this build");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    KSGF_TRY(check_lim(limiter_type));
+    KSGF_TRY(check_bc(bcrec, 3, ba));
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev q, f;
+    KSGF_TRY(cmfdev(ba, vel, KSGF_CELL, 3, 3, &q));
+    KSGF_TRY(cmfdev(ba, vel_forces, KSGF_CELL, 3, 1, &f));
+    MFDev mac[3];
+    double* macp[3] = {u_mac, v_mac, w_mac};
+    for (int d = 0; d < 3; ++d) KSGF_TRY(mfdev(ba, macp[d], KSGF_XFACE + d, 1, 1, &mac[d]));
+    Stages S;
+    KSGF_TRY(alloc_stages(ba, 3, 1, S));
+    BCs bc = make_bcs(bcrec, 3);
+    GeomDev g = make_geom(ba);
+    double dtdx[3], c6[3], c4[3], c2[3];
+    for (int d = 0; d < 3; ++d) {
+        dtdx[d] = dt / ba->dx[d];
+        c6[d] = dt / (6.0 * ba->dx[d]);
+        c4[d] = dt / (4.0 * ba->dx[d]);
+        c2[d] = dt / (2.0 * ba->dx[d]);
+    }
+    const double hdt = 0.5 * dt;
+    CMF3 none;
+    for (int d = 0; d < 3; ++d) none.m[d] = q;
+    KSGF_TRY(launch_ppm<0>(ba, q, 3, none, 0, S, bc, g, dtdx[0], limiter_type, st));
+    KSGF_TRY(launch_ppm<1>(ba, q, 3, none, 0, S, bc, g, dtdx[1], limiter_type, st));
+    KSGF_TRY(launch_ppm<2>(ba, q, 3, none, 0, S, bc, g, dtdx[2], limiter_type, st));
+    KSGF_TRY(launch_face<0>(ba, q, q, f, 3, q, S, bc, g, hdt, use_forces_in_trans, 1, 1, st));
+    KSGF_TRY(launch_face<1>(ba, q, q, f, 3, q, S, bc, g, hdt, use_forces_in_trans, 1, 1, st));
+    KSGF_TRY(launch_face<2>(ba, q, q, f, 3, q, S, bc, g, hdt, use_forces_in_trans, 1, 1, st));
+    CMFDev uad[3];
+    for (int d = 0; d < 3; ++d) uad[d] = ccast(S.uad[d]);
+    KSGF_TRY((launch_corner<0, 1>(ba, q, q, 3, uad[1], uad[0], S, bc, g, 0.0, c6[1], 1, 1, 0, st)));
+    KSGF_TRY((launch_corner<0, 2>(ba, q, q, 3, uad[2], uad[0], S, bc, g, 0.0, c6[2], 1, 1, 0, st)));
+    KSGF_TRY((launch_corner<1, 0>(ba, q, q, 3, uad[0], uad[1], S, bc, g, 0.0, c6[0], 1, 1, 0, st)));
+    KSGF_TRY((launch_corner<1, 2>(ba, q, q, 3, uad[2], uad[1], S, bc, g, 0.0, c6[2], 1, 1, 0, st)));
+    KSGF_TRY((launch_corner<2, 0>(ba, q, q, 3, uad[0], uad[2], S, bc, g, 0.0, c6[0], 1, 1, 0, st)));
+    KSGF_TRY((launch_corner<2, 1>(ba, q, q, 3, uad[1], uad[2], S, bc, g, 0.0, c6[1], 1, 1, 0, st)));
+    KSGF_TRY(launch_final<0>(ba, q, q, f, q, 0, 3, uad[1], uad[2], uad[0], mac[0], mac[0], S, bc, g, c2, c4, hdt,
+                             0.0, use_forces_in_trans, 1, 1, 0, allow_inflow_on_outflow, 0, st));
+    KSGF_TRY(launch_final<1>(ba, q, q, f, q, 0, 3, uad[2], uad[0], uad[1], mac[1], mac[1], S, bc, g, c2, c4, hdt,
+                             0.0, use_forces_in_trans, 1, 1, 0, allow_inflow_on_outflow, 0, st));
+    KSGF_TRY(launch_final<2>(ba, q, q, f, q, 0, 3, uad[0], uad[1], uad[2], mac[2], mac[2], S, bc, g, c2, c4, hdt,
+                             0.0, use_forces_in_trans, 1, 1, 0, allow_inflow_on_outflow, 0, st));
+    return KSGF_OK;
+}
+
+int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const double* q, const double* q_on_face_bc,
+                                   double* flux_x, double* flux_y, double* flux_z, double* face_x, double* face_y,
+                                   double* face_z, int known_edge_state, const double* u_mac, const double* v_mac,
+                                   const double* w_mac, const double* divu, const double* fq, double dt,
+                                   const ksgf_bcrec* bcrec, const int* iconserv, int use_ppm,
+                                   int use_forces_in_trans, int is_velocity, int fluxes_are_area_weighted,
+                                   int limiter_type, int allow_inflow_on_outflow, ksgf_stream_t stream)
+{
+    if (!ba || ncomp <= 0 || ncomp > kMaxComp || !flux_x || !flux_y || !flux_z || !face_x || !face_y || !face_z ||
+        !u_mac || !v_mac || !w_mac) {
+        set_error("ksgf_compute_fluxes_from_state: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const double area[3] = {ba->dx[1] * ba->dx[2], ba->dx[0] * ba->dx[2], ba->dx[0] * ba->dx[1]};
+    CMFDev mac[3];
+    const double* macp[3] = {u_mac, v_mac, w_mac};
+    for (int d = 0; d < 3; ++d) KSGF_TRY(cmfdev(ba, macp[d], KSGF_XFACE + d, 1, 1, &mac[d]));
+    MFDev flux[3], face[3];
+    double* fluxp[3] = {flux_x, flux_y, flux_z};
+    double* facep[3] = {face_x, face_y, face_z};
+    for (int d = 0; d < 3; ++d) {
+        KSGF_TRY(mfdev(ba, fluxp[d], KSGF_XFACE + d, ncomp, 0, &flux[d]));
+        KSGF_TRY(mfdev(ba, facep[d], KSGF_XFACE + d, ncomp, 0, &face[d]));
+    }
+    int err = KSGF_OK;
+    if (known_edge_state) {
+        for (int d = 0; d < 3; ++d) {
+            int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+            ghi[d] = 1;
+            const Region* r = get_region(ba, glo, ghi, &err);
+            if (!r) return err;
+            dim3 gr(grid_for(r->total, 256));
+            if (d == 0)
+                hipLaunchKernelGGL(k_known_flux<0>, gr, dim3(256), 0, st, ccast(face[0]), mac[0], flux[0], ncomp,
+                                   regdev(r, ba->nbox), ba->dbox, area[0], fluxes_are_area_weighted);
+            else if (d == 1)
+                hipLaunchKernelGGL(k_known_flux<1>, gr, dim3(256), 0, st, ccast(face[1]), mac[1], flux[1], ncomp,
+                                   regdev(r, ba->nbox), ba->dbox, area[1], fluxes_are_area_weighted);
+            else
+                hipLaunchKernelGGL(k_known_flux<2>, gr, dim3(256), 0, st, ccast(face[2]), mac[2], flux[2], ncomp,
+                                   regdev(r, ba->nbox), ba->dbox, area[2], fluxes_are_area_weighted);
+            KSGF_LAUNCH_CHECK();
+        }
+        return KSGF_OK;
+    }
+    if (!q || !q_on_face_bc || !fq || !bcrec || !iconserv) {
+        set_error("ksgf_compute_fluxes_from_state: null state/forcing/bc argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (!use_ppm) {
+        set_error("ksgf_compute_fluxes_from_state: PLM (use_ppm = false) is not provided by this build");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    KSGF_TRY(check_lim(limiter_type));
+    KSGF_TRY(check_bc(bcrec, ncomp, ba));
+    CMFDev qv, qb, fv, dv;
+    KSGF_TRY(cmfdev(ba, q, KSGF_CELL, ncomp, 3, &qv));
+    KSGF_TRY(cmfdev(ba, q_on_face_bc, KSGF_CELL, ncomp, 3, &qb));
+    KSGF_TRY(cmfdev(ba, fq, KSGF_CELL, ncomp, 1, &fv));
+    int has_divu = divu ? 1 : 0;
+    if (divu)
+        KSGF_TRY(cmfdev(ba, divu, KSGF_CELL, 1, 1, &dv));
+    else
+        dv = qv;
+    int cons_mask = 0;
+    for (int n = 0; n < ncomp; ++n)
+        if (iconserv[n]) cons_mask |= (1 << n);
+    Stages S;
+    KSGF_TRY(alloc_stages(ba, ncomp, ncomp, S));
+    BCs bc = make_bcs(bcrec, ncomp);
+    GeomDev g = make_geom(ba);
+    double dtdx[3], c3[3], c6[3], c2[3], c4[3];
+    for (int d = 0; d < 3; ++d) {
+        dtdx[d] = dt / ba->dx[d];
+        c3[d] = dt / (3.0 * ba->dx[d]);
+        c6[d] = dt / (6.0 * ba->dx[d]);
+        c2[d] = dt / (2.0 * ba->dx[d]);
+        c4[d] = dt / (4.0 * ba->dx[d]);
+    }
+    const double hdt = 0.5 * dt;
+    CMF3 m3;
+    for (int d = 0; d < 3; ++d) m3.m[d] = mac[d];
+    KSGF_TRY(launch_ppm<0>(ba, qv, ncomp, m3, 1, S, bc, g, dtdx[0], limiter_type, st));
+    KSGF_TRY(launch_ppm<1>(ba, qv, ncomp, m3, 1, S, bc, g, dtdx[1], limiter_type, st));
+    KSGF_TRY(launch_ppm<2>(ba, qv, ncomp, m3, 1, S, bc, g, dtdx[2], limiter_type, st));
+    KSGF_TRY(launch_face<0>(ba, qv, qb, fv, ncomp, mac[0], S, bc, g, hdt, use_forces_in_trans, 0, is_velocity, st));
+    KSGF_TRY(launch_face<1>(ba, qv, qb, fv, ncomp, mac[1], S, bc, g, hdt, use_forces_in_trans, 0, is_velocity, st));
+    KSGF_TRY(launch_face<2>(ba, qv, qb, fv, ncomp, mac[2], S, bc, g, hdt, use_forces_in_trans, 0, is_velocity, st));
+    KSGF_TRY((launch_corner<0, 1>(ba, qv, qb, ncomp, mac[1], mac[0], S, bc, g, c3[1], c6[1], 0, is_velocity, cons_mask, st)));
+    KSGF_TRY((launch_corner<0, 2>(ba, qv, qb, ncomp, mac[2], mac[0], S, bc, g, c3[2], c6[2], 0, is_velocity, cons_mask, st)));
+    KSGF_TRY((launch_corner<1, 0>(ba, qv, qb, ncomp, mac[0], mac[1], S, bc, g, c3[0], c6[0], 0, is_velocity, cons_mask, st)));
+    KSGF_TRY((launch_corner<1, 2>(ba, qv, qb, ncomp, mac[2], mac[1], S, bc, g, c3[2], c6[2], 0, is_velocity, cons_mask, st)));
+    KSGF_TRY((launch_corner<2, 0>(ba, qv, qb, ncomp, mac[0], mac[2], S, bc, g, c3[0], c6[0], 0, is_velocity, cons_mask, st)));
+    KSGF_TRY((launch_corner<2, 1>(ba, qv, qb, ncomp, mac[1], mac[2], S, bc, g, c3[1], c6[1], 0, is_velocity, cons_mask, st)));
+    KSGF_TRY(launch_final<0>(ba, qv, qb, fv, dv, has_divu, ncomp, mac[1], mac[2], mac[0], face[0], flux[0], S, bc, g,
+                             c2, c4, hdt, area[0], use_forces_in_trans, 0, is_velocity, fluxes_are_area_weighted,
+                             allow_inflow_on_outflow, cons_mask, st));
+    KSGF_TRY(launch_final<1>(ba, qv, qb, fv, dv, has_divu, ncomp, mac[2], mac[0], mac[1], face[1], flux[1], S, bc, g,
+                             c2, c4, hdt, area[1], use_forces_in_trans, 0, is_velocity, fluxes_are_area_weighted,
+                             allow_inflow_on_outflow, cons_mask, st));
+    KSGF_TRY(launch_final<2>(ba, qv, qb, fv, dv, has_divu, ncomp, mac[0], mac[1], mac[2], face[2], flux[2], S, bc, g,
+                             c2, c4, hdt, area[2], use_forces_in_trans, 0, is_velocity, fluxes_are_area_weighted,
+                             allow_inflow_on_outflow, cons_mask, st));
+    return KSGF_OK;
+}
+
+int ksgf_compute_divergence(const ksgf_boxarray* ba, double* div, const double* flux_x, const double* flux_y,
+                            const double* flux_z, int ncomp, double mult, int fluxes_are_area_weighted,
+                            ksgf_stream_t stream)
+{
+    if (!ba || !div || !flux_x || !flux_y || !flux_z || ncomp <= 0) {
+        set_error("ksgf_compute_divergence: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    MFDev dv;
+    CMF3 fl;
+    KSGF_TRY(mfdev(ba, div, KSGF_CELL, ncomp, 0, &dv));
+    const double* fp[3] = {flux_x, flux_y, flux_z};
+    for (int d = 0; d < 3; ++d) KSGF_TRY(cmfdev(ba, fp[d], KSGF_XFACE + d, ncomp, 0, &fl.m[d]));
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_divergence, dim3(grid_for(r->total, 256)), dim3(256), 0, (hipStream_t)stream, dv, fl, ncomp,
+                       regdev(r, ba->nbox), ba->dbox, mult, 1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2],
+                       ba->dx[0] * ba->dx[1] * ba->dx[2], fluxes_are_area_weighted);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+}  // extern "C"

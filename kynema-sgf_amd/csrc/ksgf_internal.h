@@ -1,0 +1,237 @@
+// Internal declarations shared by the HIP translation units of libksgf.
+//
+// Device-side conventions
+// -----------------------
+// * A MultiFab is addressed through its base pointer plus a "region prefix"
+//   table pre[0..nbox] (int64, device): pre[b] is the number of points per
+//   component in FABs 0..b-1, so FAB b starts at ncomp*pre[b] and its
+//   component stride is pre[b+1]-pre[b].  The same prefix tables drive the
+//   flat launches over ragged box sets: thread t belongs to the box b with
+//   pre[b] <= t < pre[b+1] (binary search) and decodes (i,j,k) from the
+//   remainder, x fastest, so consecutive lanes read consecutive x.
+// * A region is box b grown by glo[d] below and ghi[d] above in direction d.
+//   FAB storage of (itype, ngrow) is the region glo = ngrow,
+//   ghi = ngrow + (itype == d+1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ksgf.h"
+
+namespace ksgf {
+
+void set_error(const std::string& msg);
+
+#define KSGF_HIP_CHECK(call)                                                        \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::ksgf::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+            return KSGF_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+
+#define KSGF_LAUNCH_CHECK()                                                         \
+    do {                                                                            \
+        hipError_t e_ = hipGetLastError();                                          \
+        if (e_ != hipSuccess) {                                                     \
+            ::ksgf::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return KSGF_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+
+struct RegionKey {
+    int glo[3], ghi[3];
+    bool operator<(const RegionKey& o) const
+    {
+        for (int d = 0; d < 3; ++d) {
+            if (glo[d] != o.glo[d]) return glo[d] < o.glo[d];
+            if (ghi[d] != o.ghi[d]) return ghi[d] < o.ghi[d];
+        }
+        return false;
+    }
+};
+
+struct Region {
+    long long* pre = nullptr;  // device, nbox+1
+    std::vector<long long> hpre;
+    long long total = 0;
+    RegionKey key;
+};
+
+// ghost-exchange tables for one (itype, ngrow) storage
+struct HaloTable {
+    int* fb = nullptr;  // device: 4 ints per entry (dst box, dst local, src box, src local)
+    long long nfb = 0;
+    int* pb[3] = {nullptr, nullptr, nullptr};  // device: 2 ints per entry (box, local)
+    long long npb[3] = {0, 0, 0};
+};
+
+}  // namespace ksgf
+
+struct ksgf_boxarray {
+    ksgf_geometry geom;
+    int dlo[3], dhi[3];
+    double dx[3];
+    int periodic[3];
+    int nbox = 0;
+    std::vector<int> hbox;  // 6*nbox
+    int* dbox = nullptr;    // device copy
+    mutable std::mutex mu;
+    mutable std::map<ksgf::RegionKey, ksgf::Region*> regions;
+    mutable std::map<std::pair<int, int>, ksgf::HaloTable*> halos;
+    // per-stage scratch for the advection kernels, grown on demand
+    mutable std::vector<double*> scratch;
+    mutable std::vector<size_t> scratch_len;
+};
+
+namespace ksgf {
+
+// region helpers (host)
+const Region* get_region(const ksgf_boxarray* ba, const int glo[3], const int ghi[3], int* err);
+const Region* fab_region(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
+const HaloTable* get_halo(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
+double* get_scratch(const ksgf_boxarray* ba, int slot, size_t n, int* err);
+
+inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
+
+// ----------------------------------------------------------------------------
+// device helpers
+// ----------------------------------------------------------------------------
+struct BoxDev {
+    const int* box;  // 6*nbox
+    int nbox;
+};
+
+// FAB view: base of the component-0 data of one FAB plus its shape
+struct Fab {
+    double* p;
+    int lo0, lo1, lo2;
+    int n0, n1;
+    long long cs;  // component stride
+    __device__ __forceinline__ long long idx(int i, int j, int k) const
+    {
+        return (long long)(i - lo0) + (long long)n0 * ((j - lo1) + (long long)n1 * (k - lo2));
+    }
+    __device__ __forceinline__ double& operator()(int i, int j, int k, int n = 0) const
+    {
+        return p[idx(i, j, k) + n * cs];
+    }
+};
+
+struct CFab {
+    const double* p;
+    int lo0, lo1, lo2;
+    int n0, n1;
+    long long cs;
+    __device__ __forceinline__ long long idx(int i, int j, int k) const
+    {
+        return (long long)(i - lo0) + (long long)n0 * ((j - lo1) + (long long)n1 * (k - lo2));
+    }
+    __device__ __forceinline__ double operator()(int i, int j, int k, int n = 0) const
+    {
+        return p[idx(i, j, k) + n * cs];
+    }
+};
+
+// MultiFab handle passed to kernels: base + prefix table + storage shape
+struct MFDev {
+    double* p;
+    const long long* pre;
+    int itype, ngrow, ncomp;
+};
+struct CMFDev {
+    const double* p;
+    const long long* pre;
+    int itype, ngrow, ncomp;
+};
+
+__device__ __forceinline__ Fab fab_at(const MFDev& m, const int* box, int b)
+{
+    Fab f;
+    const int* bx = box + 6 * b;
+    f.lo0 = bx[0] - m.ngrow;
+    f.lo1 = bx[1] - m.ngrow;
+    f.lo2 = bx[2] - m.ngrow;
+    f.n0 = bx[3] - bx[0] + 1 + 2 * m.ngrow + (m.itype == 1);
+    f.n1 = bx[4] - bx[1] + 1 + 2 * m.ngrow + (m.itype == 2);
+    long long a = m.pre[b], e = m.pre[b + 1];
+    f.cs = e - a;
+    f.p = m.p + (long long)m.ncomp * a;
+    return f;
+}
+__device__ __forceinline__ CFab fab_at(const CMFDev& m, const int* box, int b)
+{
+    CFab f;
+    const int* bx = box + 6 * b;
+    f.lo0 = bx[0] - m.ngrow;
+    f.lo1 = bx[1] - m.ngrow;
+    f.lo2 = bx[2] - m.ngrow;
+    f.n0 = bx[3] - bx[0] + 1 + 2 * m.ngrow + (m.itype == 1);
+    f.n1 = bx[4] - bx[1] + 1 + 2 * m.ngrow + (m.itype == 2);
+    long long a = m.pre[b], e = m.pre[b + 1];
+    f.cs = e - a;
+    f.p = m.p + (long long)m.ncomp * a;
+    return f;
+}
+
+// region iteration: map a flat index to (box, i, j, k)
+struct RegDev {
+    const long long* pre;
+    int nbox;
+    int glo0, glo1, glo2, ghi0, ghi1, ghi2;
+    long long total;
+};
+
+__device__ __forceinline__ bool region_locate(const RegDev& r, const int* box, long long t,
+                                              int& b, int& i, int& j, int& k)
+{
+    if (t >= r.total) return false;
+    int lo = 0, hi = r.nbox - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (r.pre[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    b = lo;
+    long long loc = t - r.pre[b];
+    const int* bx = box + 6 * b;
+    int l0 = bx[0] - r.glo0, l1 = bx[1] - r.glo1, l2 = bx[2] - r.glo2;
+    int n0 = bx[3] + r.ghi0 - l0 + 1;
+    int n1 = bx[4] + r.ghi1 - l1 + 1;
+    i = l0 + (int)(loc % n0);
+    long long q = loc / n0;
+    j = l1 + (int)(q % n1);
+    k = l2 + (int)(q / n1);
+    return true;
+}
+
+inline RegDev regdev(const Region* r, int nbox)
+{
+    RegDev d;
+    d.pre = r->pre;
+    d.nbox = nbox;
+    d.glo0 = r->key.glo[0];
+    d.glo1 = r->key.glo[1];
+    d.glo2 = r->key.glo[2];
+    d.ghi0 = r->key.ghi[0];
+    d.ghi1 = r->key.ghi[1];
+    d.ghi2 = r->key.ghi[2];
+    d.total = r->total;
+    return d;
+}
+
+// host helper to build kernel MultiFab handles
+int mfdev(const ksgf_boxarray* ba, double* p, int itype, int ncomp, int ngrow, MFDev* out);
+int cmfdev(const ksgf_boxarray* ba, const double* p, int itype, int ncomp, int ngrow, CMFDev* out);
+
+}  // namespace ksgf

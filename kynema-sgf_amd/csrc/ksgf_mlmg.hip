@@ -1,0 +1,992 @@
+// Hydro::MacProjector + amrex::MLMG (MLPoisson) equivalent on one GPU.
+//
+// The projector owns its multigrid hierarchy laid out domain-wide: one
+// contiguous array per level with a single ghost layer, Fortran order.  On a
+// single GPU the BoxArray is only a storage decomposition, and red-black
+// Gauss-Seidel with a 7-point stencil, 8-cell averaging and piecewise
+// constant prolongation are all independent of it, so the domain-wide layout
+// gives the same iterates as a per-box hierarchy while every sweep is one
+// coalesced launch.  The MAC divergence is gathered from the box FABs into
+// level 0 and grad(phi) is scattered back onto every valid face.
+//
+// Options and semantics follow MLMGOptions.{H,cpp} (maxiter 200, nu1 = nu2
+// = 2, final smooth 8, bottom smooth 0, BiCGStab bottom with rtol eps*1e12,
+// atol eps*1e4, 200 iterations, no abort on non-convergence) and the solve
+// converges when |r|_inf <= max(atol, rtol*|rhs|_inf).
+//
+// Mixed precision (KSGF_MG_MIXED): each outer iteration forms the FP64
+// residual of the FP64 solution, runs one V-cycle on it with FP32 smoothing
+// and grid transfers (the bottom BiCGStab stays FP64), and adds the FP32
+// correction to the FP64 solution.
+#include <cmath>
+#include <cstring>
+
+#include "ksgf_internal.h"
+
+namespace ksgf {
+
+struct LevelGeom {
+    int n0, n1, n2;
+    long long sy, sz, tot;
+    double cx, cy, cz, dg, idg;
+    int lo[3], hi[3];  // lobc / hibc
+};
+
+template <typename T>
+__device__ __forceinline__ T bc_ghost(int t, T vl, T vr, bool low)
+{
+    // t: LinOp BC on this side; vl: value at the near interior cell of this
+    // side, vr: value at the far (periodic partner) interior cell
+    return (t == KSGF_LO_PERIODIC) ? vr : (t == KSGF_LO_NEUMANN ? vl : -vl);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom L)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nx = (long long)L.n1 * L.n2, ny = (long long)L.n0 * L.n2, nz = (long long)L.n0 * L.n1;
+    auto LI = [&](int i, int j, int k) { return (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1); };
+    if (t < 2 * nx) {
+        int side = (int)(t / nx);
+        long long r = t % nx;
+        int j = (int)(r % L.n1), k = (int)(r / L.n1);
+        if (side == 0) {
+            T vl = a[LI(0, j, k)], vr = a[LI(L.n0 - 1, j, k)];
+            a[LI(-1, j, k)] = bc_ghost(L.lo[0], vl, vr, true);
+        } else {
+            T vl = a[LI(0, j, k)], vr = a[LI(L.n0 - 1, j, k)];
+            a[LI(L.n0, j, k)] = bc_ghost(L.hi[0], vr, vl, false);
+        }
+        return;
+    }
+    t -= 2 * nx;
+    if (t < 2 * ny) {
+        int side = (int)(t / ny);
+        long long r = t % ny;
+        int i = (int)(r % L.n0), k = (int)(r / L.n0);
+        T vl = a[LI(i, 0, k)], vr = a[LI(i, L.n1 - 1, k)];
+        if (side == 0)
+            a[LI(i, -1, k)] = bc_ghost(L.lo[1], vl, vr, true);
+        else
+            a[LI(i, L.n1, k)] = bc_ghost(L.hi[1], vr, vl, false);
+        return;
+    }
+    t -= 2 * ny;
+    if (t < 2 * nz) {
+        int side = (int)(t / nz);
+        long long r = t % nz;
+        int i = (int)(r % L.n0), j = (int)(r / L.n0);
+        T vl = a[LI(i, j, 0)], vr = a[LI(i, j, L.n2 - 1)];
+        if (side == 0)
+            a[LI(i, j, -1)] = bc_ghost(L.lo[2], vl, vr, true);
+        else
+            a[LI(i, j, L.n2)] = bc_ghost(L.hi[2], vr, vl, false);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T lap_off(const T* __restrict__ a, long long c, long long sy, long long sz, T cx, T cy,
+                                     T cz)
+{
+    return cx * (a[c - 1] + a[c + 1]) + cy * (a[c - sy] + a[c + sy]) + cz * (a[c - sz] + a[c + sz]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ x, const T* __restrict__ b, LevelGeom L,
+                                                   int color)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h0 = (L.n0 + 1) / 2;
+    const long long tot = (long long)h0 * L.n1 * L.n2;
+    if (t >= tot) return;
+    int ih = (int)(t % h0);
+    long long r = t / h0;
+    int j = (int)(r % L.n1), k = (int)(r / L.n1);
+    int i = 2 * ih + ((j + k + color) & 1);
+    if (i >= L.n0) return;
+    long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
+    const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, idg = (T)L.idg;
+    x[c] = (lap_off<T>(x, c, L.sy, L.sz, cx, cy, cz) - b[c]) * idg;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v)
+{
+    // non-negative doubles order like their bit patterns
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// r = b - A x (r in R precision); |r|_inf into norm (bits of a double)
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_mg_residual(const T* __restrict__ x, const T* __restrict__ b,
+                                                     R* __restrict__ r, LevelGeom L,
+                                                     unsigned long long* __restrict__ norm)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)L.n0 * L.n1 * L.n2;
+    double m = 0.0;
+    if (t < tot) {
+        int i = (int)(t % L.n0);
+        long long q = t / L.n0;
+        int j = (int)(q % L.n1), k = (int)(q / L.n1);
+        long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
+        const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, dg = (T)L.dg;
+        T v = b[c] - (lap_off<T>(x, c, L.sy, L.sz, cx, cy, cz) - dg * x[c]);
+        r[c] = (R)v;
+        m = fabs((double)v);
+    }
+    if (norm) {
+        // wavefront max then one atomic per wavefront
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+        if ((threadIdx.x & 63) == 0) atomic_max_nonneg(norm, m);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_restrict(const T* __restrict__ f, T* __restrict__ c, LevelGeom F,
+                                                     LevelGeom C, int r0, int r1, int r2, T w)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)C.n0 * C.n1 * C.n2;
+    if (t >= tot) return;
+    int I = (int)(t % C.n0);
+    long long q = t / C.n0;
+    int J = (int)(q % C.n1), K = (int)(q / C.n1);
+    T s = (T)0;
+    for (int dk = 0; dk < r2; ++dk)
+        for (int dj = 0; dj < r1; ++dj)
+            for (int di = 0; di < r0; ++di)
+                s += f[(long long)(I * r0 + di + 1) + F.sy * (J * r1 + dj + 1) + F.sz * (K * r2 + dk + 1)];
+    c[(long long)(I + 1) + C.sy * (J + 1) + C.sz * (K + 1)] = s * w;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const T* __restrict__ c, LevelGeom F,
+                                                        LevelGeom C, int r0, int r1, int r2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)F.n0 * F.n1 * F.n2;
+    if (t >= tot) return;
+    int i = (int)(t % F.n0);
+    long long q = t / F.n0;
+    int j = (int)(q % F.n1), k = (int)(q / F.n1);
+    f[(long long)(i + 1) + F.sy * (j + 1) + F.sz * (k + 1)] +=
+        c[(long long)(i / r0 + 1) + C.sy * (j / r1 + 1) + C.sz * (k / r2 + 1)];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T* __restrict__ a, long long n, T v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(256) k_add_to(A* __restrict__ a, const B* __restrict__ b, long long n)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] += (A)b[t];
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(256) k_convert(A* __restrict__ a, const B* __restrict__ b, long long n)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = (A)b[t];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    int t = threadIdx.x;
+    if (t < n) p[t] = 0ull;
+}
+
+// deterministic sum of interior cells: per-block partial sums
+constexpr int kSumBlocks = 512;
+__global__ void __launch_bounds__(256) k_partial_sum(const double* __restrict__ a, LevelGeom L,
+                                                     double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    const long long tot = (long long)L.n0 * L.n1 * L.n2;
+    double s = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % L.n0);
+        long long q = t / L.n0;
+        int j = (int)(q % L.n1), k = (int)(q / L.n1);
+        s += a[(long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1)];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ a, LevelGeom L,
+                                                  const double* __restrict__ part, int npart)
+{
+    __shared__ double mean;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int p = 0; p < npart; ++p) s += part[p];
+        mean = s / ((double)L.n0 * L.n1 * L.n2);
+    }
+    __syncthreads();
+    const long long tot = (long long)L.n0 * L.n1 * L.n2;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % L.n0);
+        long long q = t / L.n0;
+        int j = (int)(q % L.n1), k = (int)(q / L.n1);
+        a[(long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1)] -= mean;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bottom solve: one workgroup runs the whole unpreconditioned BiCGStab on
+// the coarsest level (a few to a few thousand cells), then the smoothing
+// sweeps MLMG applies after it (nub after success, nuf after failure).
+// ---------------------------------------------------------------------------
+constexpr int kBotThreads = 256;
+
+struct BottomWork {
+    double *r, *rh, *p, *v, *s, *t;
+    int* stats;  // [0] iterations (accumulated), [1] failures (accumulated)
+};
+
+__device__ double blk_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = kBotThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__device__ double blk_max(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = kBotThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ void blk_ghosts(double* a, const LevelGeom& L)
+{
+    const long long nx = (long long)L.n1 * L.n2, ny = (long long)L.n0 * L.n2, nz = (long long)L.n0 * L.n1;
+    const long long n = 2 * (nx + ny + nz);
+    for (long long t0 = threadIdx.x; t0 < n; t0 += kBotThreads) {
+        long long t = t0;
+        auto LI = [&](int i, int j, int k) { return (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1); };
+        if (t < 2 * nx) {
+            int side = (int)(t / nx);
+            long long r = t % nx;
+            int j = (int)(r % L.n1), k = (int)(r / L.n1);
+            double vl = a[LI(0, j, k)], vr = a[LI(L.n0 - 1, j, k)];
+            if (side == 0)
+                a[LI(-1, j, k)] = bc_ghost(L.lo[0], vl, vr, true);
+            else
+                a[LI(L.n0, j, k)] = bc_ghost(L.hi[0], vr, vl, false);
+            continue;
+        }
+        t -= 2 * nx;
+        if (t < 2 * ny) {
+            int side = (int)(t / ny);
+            long long r = t % ny;
+            int i = (int)(r % L.n0), k = (int)(r / L.n0);
+            double vl = a[LI(i, 0, k)], vr = a[LI(i, L.n1 - 1, k)];
+            if (side == 0)
+                a[LI(i, -1, k)] = bc_ghost(L.lo[1], vl, vr, true);
+            else
+                a[LI(i, L.n1, k)] = bc_ghost(L.hi[1], vr, vl, false);
+            continue;
+        }
+        t -= 2 * ny;
+        int side = (int)(t / nz);
+        long long r = t % nz;
+        int i = (int)(r % L.n0), j = (int)(r / L.n0);
+        double vl = a[LI(i, j, 0)], vr = a[LI(i, j, L.n2 - 1)];
+        if (side == 0)
+            a[LI(i, j, -1)] = bc_ghost(L.lo[2], vl, vr, true);
+        else
+            a[LI(i, j, L.n2)] = bc_ghost(L.hi[2], vr, vl, false);
+    }
+    __syncthreads();
+}
+
+#define BOT_FOR(c)                                                                            \
+    for (long long t_ = threadIdx.x; t_ < ncell; t_ += kBotThreads)                          \
+        for (long long c = (long long)((t_ % L.n0) + 1) + L.sy * ((t_ / L.n0) % L.n1 + 1) +   \
+                           L.sz * ((t_ / L.n0) / L.n1 + 1),                                  \
+                       once_ = 0;                                                            \
+             once_ < 1; ++once_)
+
+__device__ void blk_apply(double* x, double* y, const LevelGeom& L, long long ncell)
+{
+    blk_ghosts(x, L);
+    BOT_FOR(c) { y[c] = lap_off<double>(x, c, L.sy, L.sz, L.cx, L.cy, L.cz) - L.dg * x[c]; }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBotThreads) k_bottom(double* __restrict__ x, double* __restrict__ b, LevelGeom L,
+                                                        BottomWork w, int singular, double rtol, double atol,
+                                                        int maxiter, int nub, int nuf)
+{
+    __shared__ double sh[kBotThreads];
+    const long long ncell = (long long)L.n0 * L.n1 * L.n2;
+    if (singular) {
+        double s = 0.0;
+        BOT_FOR(c) { s += b[c]; }
+        double mean = blk_sum(s, sh) / (double)ncell;
+        BOT_FOR(c) { b[c] -= mean; }
+        __syncthreads();
+    }
+    for (long long c = threadIdx.x; c < L.tot; c += kBotThreads) {
+        x[c] = 0.0;
+        w.r[c] = b[c];
+        w.rh[c] = b[c];
+        w.p[c] = 0.0;
+        w.v[c] = 0.0;
+        w.s[c] = 0.0;
+        w.t[c] = 0.0;
+    }
+    __syncthreads();
+    double m = 0.0;
+    BOT_FOR(c) { m = fmax(m, fabs(w.r[c])); }
+    const double rnorm0 = blk_max(m, sh);
+    const double tol = fmax(rtol * rnorm0, atol);
+    int ret = 1, it = 0;
+    if (rnorm0 == 0.0 || rnorm0 <= tol) {
+        ret = 0;
+    } else {
+        double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+        for (it = 1; it <= maxiter; ++it) {
+            double acc = 0.0;
+            BOT_FOR(c) { acc += w.rh[c] * w.r[c]; }
+            double rho = blk_sum(acc, sh);
+            if (rho == 0.0) { ret = 2; break; }
+            if (it == 1) {
+                BOT_FOR(c) { w.p[c] = w.r[c]; }
+            } else {
+                double beta = (rho / rho_prev) * (alpha / omega);
+                BOT_FOR(c) { w.p[c] = w.r[c] + beta * (w.p[c] - omega * w.v[c]); }
+            }
+            __syncthreads();
+            blk_apply(w.p, w.v, L, ncell);
+            acc = 0.0;
+            BOT_FOR(c) { acc += w.rh[c] * w.v[c]; }
+            double rv = blk_sum(acc, sh);
+            if (rv == 0.0) { ret = 2; break; }
+            alpha = rho / rv;
+            m = 0.0;
+            BOT_FOR(c) {
+                w.s[c] = w.r[c] - alpha * w.v[c];
+                m = fmax(m, fabs(w.s[c]));
+            }
+            double sn = blk_max(m, sh);
+            if (sn <= tol) {
+                BOT_FOR(c) { x[c] += alpha * w.p[c]; }
+                __syncthreads();
+                ret = 0;
+                break;
+            }
+            blk_apply(w.s, w.t, L, ncell);
+            acc = 0.0;
+            BOT_FOR(c) { acc += w.t[c] * w.t[c]; }
+            double tt = blk_sum(acc, sh);
+            if (tt == 0.0) { ret = 2; break; }
+            acc = 0.0;
+            BOT_FOR(c) { acc += w.t[c] * w.s[c]; }
+            omega = blk_sum(acc, sh) / tt;
+            m = 0.0;
+            BOT_FOR(c) {
+                x[c] += alpha * w.p[c] + omega * w.s[c];
+                w.r[c] = w.s[c] - omega * w.t[c];
+                m = fmax(m, fabs(w.r[c]));
+            }
+            double rn = blk_max(m, sh);
+            if (rn <= tol) { ret = 0; break; }
+            if (omega == 0.0) { ret = 2; break; }
+            rho_prev = rho;
+        }
+        if (it > maxiter) it = maxiter;
+    }
+    __syncthreads();
+    int nsm = (ret == 0) ? nub : nuf;
+    for (int sweep = 0; sweep < nsm; ++sweep)
+        for (int color = 0; color < 2; ++color) {
+            blk_ghosts(x, L);
+            BOT_FOR(c) {
+                long long q = t_ / L.n0;
+                int i = (int)(t_ % L.n0), j = (int)(q % L.n1), k = (int)(q / L.n1);
+                if (((i + j + k + color) & 1) == 0) x[c] = (lap_off<double>(x, c, L.sy, L.sz, L.cx, L.cy, L.cz) - b[c]) * L.idg;
+            }
+            __syncthreads();
+        }
+    if (threadIdx.x == 0) {
+        w.stats[0] += it;
+        if (ret != 0) w.stats[1] += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MAC-specific gathers / scatters between box FABs and the level-0 array
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mac_rhs(CMFDev u, CMFDev v, CMFDev w, double* __restrict__ rhs,
+                                                 LevelGeom L, RegDev r, const int* __restrict__ box, int dlo0,
+                                                 int dlo1, int dlo2, double dxi, double dyi, double dzi)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    CFab fu = fab_at(u, box, b), fv = fab_at(v, box, b), fw = fab_at(w, box, b);
+    rhs[(long long)(i - dlo0 + 1) + L.sy * (j - dlo1 + 1) + L.sz * (k - dlo2 + 1)] =
+        (fu(i + 1, j, k) - fu(i, j, k)) * dxi + (fv(i, j + 1, k) - fv(i, j, k)) * dyi +
+        (fw(i, j, k + 1) - fw(i, j, k)) * dzi;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_mac_correct(MFDev u, const double* __restrict__ phi, LevelGeom L,
+                                                     RegDev r, const int* __restrict__ box, int dlo0, int dlo1,
+                                                     int dlo2, double bd)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int b, i, j, k;
+    if (!region_locate(r, box, t, b, i, j, k)) return;
+    Fab fu = fab_at(u, box, b);
+    long long c = (long long)(i - dlo0 + 1) + L.sy * (j - dlo1 + 1) + L.sz * (k - dlo2 + 1);
+    const long long sd = (D == 0) ? 1 : (D == 1 ? L.sy : L.sz);
+    double pr = phi[c], pl = phi[c - sd];
+    fu(i, j, k) = fu(i, j, k) - bd * (pr - pl);
+}
+
+__global__ void __launch_bounds__(256) k_pack(const double* __restrict__ a, double* __restrict__ out, LevelGeom L,
+                                              int unpack)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)L.n0 * L.n1 * L.n2;
+    if (t >= tot) return;
+    int i = (int)(t % L.n0);
+    long long q = t / L.n0;
+    int j = (int)(q % L.n1), k = (int)(q / L.n1);
+    long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
+    if (unpack)
+        out[c] = a[t];
+    else
+        out[t] = a[c];
+}
+
+}  // namespace ksgf
+
+using namespace ksgf;
+
+struct ksgf_macproj {
+    const ksgf_boxarray* ba = nullptr;
+    int lobc[3], hibc[3];
+    double beta = 1.0;
+    ksgf_mlmg_options opt;
+    int singular = 1;
+    int nlev = 0;
+    std::vector<LevelGeom> G;
+    std::vector<int> ratio;  // 3 per level (to next)
+    std::vector<double*> cor, res, tmp;  // FP64 per level
+    std::vector<float*> cor32, res32, tmp32;  // FP32 per level (mixed)
+    double* phi = nullptr;
+    double* rhs = nullptr;
+    double* part = nullptr;
+    unsigned long long* dnorm = nullptr;
+    unsigned long long* hnorm = nullptr;  // pinned
+    BottomWork bw{};
+    int* hstats = nullptr;  // pinned
+};
+
+namespace ksgf {
+
+static void set_level_coefs(ksgf_macproj* mp)
+{
+    for (int l = 0; l < mp->nlev; ++l) {
+        LevelGeom& L = mp->G[l];
+        // h is stored implicitly: recompute from the domain and ratios
+        (void)L;
+    }
+}
+
+static int mp_alloc(double** p, long long n)
+{
+    if (hipMalloc(p, sizeof(double) * (size_t)n) != hipSuccess) {
+        set_error("hipMalloc multigrid level");
+        return KSGF_ERR_NOMEM;
+    }
+    return KSGF_OK;
+}
+
+static void fill_coefs(LevelGeom& L, const double h[3], double beta)
+{
+    L.cx = beta / (h[0] * h[0]);
+    L.cy = beta / (h[1] * h[1]);
+    L.cz = beta / (h[2] * h[2]);
+    L.dg = 2.0 * (L.cx + L.cy + L.cz);
+    L.idg = 1.0 / L.dg;
+}
+
+static std::vector<std::array<double, 3>> level_h(const ksgf_macproj* mp)
+{
+    std::vector<std::array<double, 3>> h(mp->nlev);
+    h[0] = {mp->ba->dx[0], mp->ba->dx[1], mp->ba->dx[2]};
+    for (int l = 1; l < mp->nlev; ++l)
+        for (int d = 0; d < 3; ++d) h[l][d] = h[l - 1][d] * mp->ratio[3 * (l - 1) + d];
+    return h;
+}
+
+template <typename T>
+static void launch_ghosts(T* a, const LevelGeom& L, hipStream_t st)
+{
+    long long n = 2 * ((long long)L.n1 * L.n2 + (long long)L.n0 * L.n2 + (long long)L.n0 * L.n1);
+    hipLaunchKernelGGL(k_mg_ghosts<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, a, L);
+}
+
+template <typename T>
+static void launch_smooth(T* x, const T* b, const LevelGeom& L, int nu, hipStream_t st)
+{
+    long long n = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
+    for (int s = 0; s < nu; ++s)
+        for (int color = 0; color < 2; ++color) {
+            launch_ghosts<T>(x, L, st);
+            hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, x, b, L, color);
+        }
+}
+
+template <typename T, typename R>
+static void launch_residual(T* x, const T* b, R* r, const LevelGeom& L, unsigned long long* norm, hipStream_t st)
+{
+    launch_ghosts<T>(x, L, st);
+    long long n = (long long)L.n0 * L.n1 * L.n2;
+    hipLaunchKernelGGL((k_mg_residual<T, R>), dim3(grid_for(n, 256)), dim3(256), 0, st, (const T*)x, b, r, L, norm);
+}
+
+template <typename T>
+static void launch_fill(T* a, long long n, T v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, a, n, v);
+}
+
+static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_partial_sum, dim3(kSumBlocks), dim3(256), 0, st, (const double*)a, L, part);
+    hipLaunchKernelGGL(k_sub_mean, dim3(kSumBlocks), dim3(256), 0, st, a, L, (const double*)part, kSumBlocks);
+}
+
+// V-cycle in precision T on levels l..nlev-1; the bottom solve is FP64.
+template <typename T>
+static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& res, std::vector<T*>& tmp,
+                   hipStream_t st)
+{
+    const LevelGeom& L = mp->G[l];
+    launch_fill<T>(cor[l], L.tot, (T)0, st);
+    if (l == mp->nlev - 1) {
+        double* x64;
+        double* b64;
+        if (sizeof(T) == sizeof(double)) {
+            x64 = (double*)cor[l];
+            b64 = (double*)res[l];
+        } else {
+            x64 = mp->cor[l];
+            b64 = mp->res[l];
+            hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_for(L.tot, 256)), dim3(256), 0, st, b64,
+                               (const T*)res[l], L.tot);
+        }
+        hipLaunchKernelGGL(k_bottom, dim3(1), dim3(kBotThreads), 0, st, x64, b64, L, mp->bw, mp->singular,
+                           mp->opt.bottom_rtol, mp->opt.bottom_atol, mp->opt.bottom_maxiter,
+                           mp->opt.num_bottom_smooth, mp->opt.num_final_smooth);
+        if (sizeof(T) != sizeof(double))
+            hipLaunchKernelGGL((k_convert<T, double>), dim3(grid_for(L.tot, 256)), dim3(256), 0, st, cor[l],
+                               (const double*)x64, L.tot);
+        return;
+    }
+    const LevelGeom& C = mp->G[l + 1];
+    const int* r = &mp->ratio[3 * l];
+    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_pre_smooth, st);
+    launch_residual<T, T>(cor[l], res[l], tmp[l], L, nullptr, st);
+    long long nc = (long long)C.n0 * C.n1 * C.n2;
+    T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+    hipLaunchKernelGGL(k_mg_restrict<T>, dim3(grid_for(nc, 256)), dim3(256), 0, st, (const T*)tmp[l], res[l + 1], L,
+                       C, r[0], r[1], r[2], w);
+    vcycle<T>(mp, l + 1, cor, res, tmp, st);
+    long long nf = (long long)L.n0 * L.n1 * L.n2;
+    hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
+                       L, C, r[0], r[1], r[2]);
+    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_post_smooth, st);
+}
+
+static int read_norm(ksgf_macproj* mp, hipStream_t st, double* out)
+{
+    KSGF_HIP_CHECK(hipMemcpyAsync(mp->hnorm, mp->dnorm, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    unsigned long long bits = *mp->hnorm;
+    double v;
+    std::memcpy(&v, &bits, sizeof(double));
+    *out = v;
+    return KSGF_OK;
+}
+
+// solve with level-0 rhs already in mp->rhs (ghost layout); phi in mp->phi
+static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* info, hipStream_t st)
+{
+    const LevelGeom& L0 = mp->G[0];
+    launch_fill<double>(mp->phi, L0.tot, 0.0, st);
+    if (mp->singular) launch_sub_mean(mp->rhs, L0, mp->part, st);
+    // |rhs|_inf
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, mp->dnorm, 1);
+    launch_residual<double, double>(mp->phi, mp->rhs, mp->res[0], L0, mp->dnorm, st);
+    KSGF_LAUNCH_CHECK();
+    double rhsnorm;
+    KSGF_HIP_CHECK(hipMemsetAsync(mp->bw.stats, 0, 2 * sizeof(int), st));
+    int e = read_norm(mp, st, &rhsnorm);
+    if (e) return e;
+    const double target = std::fmax(atol, rtol * rhsnorm);
+    double r = rhsnorm;
+    ksgf_mlmg_info inf{};
+    inf.r0 = r;
+    inf.nlevels = mp->nlev;
+    inf.converged = (r <= target);
+    const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
+    if (!inf.converged) {
+        for (int it = 1; it <= mp->opt.maxiter; ++it) {
+            if (mixed) {
+                // res32 = (float) res64 (the FP64 residual of the FP64 solution)
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(L0.tot, 256)), dim3(256), 0, st,
+                                   mp->res32[0], (const double*)mp->res[0], L0.tot);
+                vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->tmp32, st);
+                hipLaunchKernelGGL((k_add_to<double, float>), dim3(grid_for(L0.tot, 256)), dim3(256), 0, st, mp->phi,
+                                   (const float*)mp->cor32[0], L0.tot);
+            } else {
+                vcycle<double>(mp, 0, mp->cor, mp->res, mp->tmp, st);
+                hipLaunchKernelGGL((k_add_to<double, double>), dim3(grid_for(L0.tot, 256)), dim3(256), 0, st, mp->phi,
+                                   (const double*)mp->cor[0], L0.tot);
+            }
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, mp->dnorm, 1);
+            launch_residual<double, double>(mp->phi, mp->rhs, mp->res[0], L0, mp->dnorm, st);
+            KSGF_LAUNCH_CHECK();
+            e = read_norm(mp, st, &r);
+            if (e) return e;
+            inf.iters = it;
+            if (r <= target) {
+                inf.converged = 1;
+                break;
+            }
+        }
+    }
+    launch_ghosts<double>(mp->phi, L0, st);
+    KSGF_HIP_CHECK(hipMemcpyAsync(mp->hstats, mp->bw.stats, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    inf.rfinal = r;
+    inf.outer_iters = mixed ? inf.iters : 0;
+    inf.bottom_iters = mp->hstats[0];
+    inf.bottom_fail = mp->hstats[1];
+    if (info) *info = inf;
+    return inf.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+}
+
+}  // namespace ksgf
+
+extern "C" {
+
+void ksgf_mlmg_default_options(ksgf_mlmg_options* o)
+{
+    if (!o) return;
+    o->maxiter = 200;
+    o->num_pre_smooth = 2;
+    o->num_post_smooth = 2;
+    o->num_final_smooth = 8;
+    o->num_bottom_smooth = 0;
+    o->bottom_maxiter = 200;
+    o->bottom_rtol = 2.220446049250313e-16 * 1.0e12;
+    o->bottom_atol = 2.220446049250313e-16 * 1.0e4;
+    o->do_semicoarsening = 0;
+    o->max_coarsening_level = 30;
+    o->precision = KSGF_MG_FP64;
+}
+
+int ksgf_macproj_destroy(ksgf_macproj* mp)
+{
+    if (!mp) return KSGF_OK;
+    for (auto* p : mp->cor) if (p) (void)hipFree(p);
+    for (auto* p : mp->res) if (p) (void)hipFree(p);
+    for (auto* p : mp->tmp) if (p) (void)hipFree(p);
+    for (auto* p : mp->cor32) if (p) (void)hipFree(p);
+    for (auto* p : mp->res32) if (p) (void)hipFree(p);
+    for (auto* p : mp->tmp32) if (p) (void)hipFree(p);
+    double* bws[6] = {mp->bw.r, mp->bw.rh, mp->bw.p, mp->bw.v, mp->bw.s, mp->bw.t};
+    for (auto* p : bws) if (p) (void)hipFree(p);
+    if (mp->bw.stats) (void)hipFree(mp->bw.stats);
+    if (mp->phi) (void)hipFree(mp->phi);
+    if (mp->rhs) (void)hipFree(mp->rhs);
+    if (mp->part) (void)hipFree(mp->part);
+    if (mp->dnorm) (void)hipFree(mp->dnorm);
+    if (mp->hnorm) (void)hipHostFree(mp->hnorm);
+    if (mp->hstats) (void)hipHostFree(mp->hstats);
+    delete mp;
+    return KSGF_OK;
+}
+
+int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, double beta,
+                        const ksgf_mlmg_options* opts, ksgf_macproj** out)
+{
+    if (!ba || !lobc || !hibc || !out || !(beta > 0.0)) {
+        set_error("ksgf_macproj_create: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    ksgf_macproj* mp = new ksgf_macproj();
+    mp->ba = ba;
+    mp->beta = beta;
+    if (opts)
+        mp->opt = *opts;
+    else
+        ksgf_mlmg_default_options(&mp->opt);
+    if (mp->opt.precision != KSGF_MG_FP64 && mp->opt.precision != KSGF_MG_MIXED) {
+        delete mp;
+        set_error("ksgf_macproj_create: bad precision");
+        return KSGF_ERR_INVALID;
+    }
+    mp->singular = 1;
+    for (int d = 0; d < 3; ++d) {
+        mp->lobc[d] = lobc[d];
+        mp->hibc[d] = hibc[d];
+        for (int t : {lobc[d], hibc[d]})
+            if (t != KSGF_LO_PERIODIC && t != KSGF_LO_NEUMANN && t != KSGF_LO_DIRICHLET) {
+                delete mp;
+                set_error("ksgf_macproj_create: bad LinOp BC");
+                return KSGF_ERR_INVALID;
+            }
+        if ((lobc[d] == KSGF_LO_PERIODIC) != (ba->periodic[d] != 0) ||
+            (hibc[d] == KSGF_LO_PERIODIC) != (ba->periodic[d] != 0)) {
+            delete mp;
+            set_error("ksgf_macproj_create: periodic LinOp BC must match the geometry");
+            return KSGF_ERR_INVALID;
+        }
+        if (lobc[d] == KSGF_LO_DIRICHLET || hibc[d] == KSGF_LO_DIRICHLET) mp->singular = 0;
+    }
+    // hierarchy
+    int cn[3];
+    double ch[3];
+    for (int d = 0; d < 3; ++d) {
+        cn[d] = ba->dhi[d] - ba->dlo[d] + 1;
+        ch[d] = ba->dx[d];
+    }
+    for (;;) {
+        LevelGeom L{};
+        L.n0 = cn[0];
+        L.n1 = cn[1];
+        L.n2 = cn[2];
+        L.sy = cn[0] + 2;
+        L.sz = (long long)(cn[0] + 2) * (cn[1] + 2);
+        L.tot = L.sz * (cn[2] + 2);
+        for (int d = 0; d < 3; ++d) {
+            L.lo[d] = lobc[d];
+            L.hi[d] = hibc[d];
+        }
+        fill_coefs(L, ch, beta);
+        mp->G.push_back(L);
+        mp->nlev++;
+        int r[3] = {1, 1, 1};
+        bool stop = (mp->nlev >= 40 || mp->nlev > mp->opt.max_coarsening_level);
+        if (!stop) {
+            int can[3], any = 0;
+            for (int d = 0; d < 3; ++d) can[d] = (cn[d] % 2 == 0 && cn[d] / 2 >= 2);
+            if (!mp->opt.do_semicoarsening) {
+                if (can[0] && can[1] && can[2]) {
+                    r[0] = r[1] = r[2] = 2;
+                    any = 1;
+                }
+            } else {
+                double hmin = 1e300;
+                for (int d = 0; d < 3; ++d)
+                    if (can[d] && ch[d] < hmin) hmin = ch[d];
+                for (int d = 0; d < 3; ++d)
+                    if (can[d] && ch[d] <= 1.5 * hmin) {
+                        r[d] = 2;
+                        any = 1;
+                    }
+            }
+            if (!any) stop = true;
+        }
+        for (int d = 0; d < 3; ++d) mp->ratio.push_back(r[d]);
+        if (stop) break;
+        for (int d = 0; d < 3; ++d) {
+            cn[d] /= r[d];
+            ch[d] *= r[d];
+        }
+    }
+    const LevelGeom& Lb = mp->G[mp->nlev - 1];
+    if ((long long)Lb.n0 * Lb.n1 * Lb.n2 > (1LL << 22)) {
+        ksgf_macproj_destroy(mp);
+        set_error("ksgf_macproj_create: coarsest level too large for the single-workgroup bottom solve");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    int e;
+    for (int l = 0; l < mp->nlev; ++l) {
+        double *a, *b, *c;
+        if ((e = mp_alloc(&a, mp->G[l].tot)) || (e = mp_alloc(&b, mp->G[l].tot)) || (e = mp_alloc(&c, mp->G[l].tot))) {
+            ksgf_macproj_destroy(mp);
+            return e;
+        }
+        mp->cor.push_back(a);
+        mp->res.push_back(b);
+        mp->tmp.push_back(c);
+        (void)hipMemset(a, 0, sizeof(double) * mp->G[l].tot);
+        (void)hipMemset(b, 0, sizeof(double) * mp->G[l].tot);
+        (void)hipMemset(c, 0, sizeof(double) * mp->G[l].tot);
+        if (mp->opt.precision == KSGF_MG_MIXED) {
+            float *fa, *fb, *fc;
+            if (hipMalloc(&fa, sizeof(float) * mp->G[l].tot) != hipSuccess ||
+                hipMalloc(&fb, sizeof(float) * mp->G[l].tot) != hipSuccess ||
+                hipMalloc(&fc, sizeof(float) * mp->G[l].tot) != hipSuccess) {
+                ksgf_macproj_destroy(mp);
+                set_error("hipMalloc FP32 level");
+                return KSGF_ERR_NOMEM;
+            }
+            (void)hipMemset(fa, 0, sizeof(float) * mp->G[l].tot);
+            (void)hipMemset(fb, 0, sizeof(float) * mp->G[l].tot);
+            (void)hipMemset(fc, 0, sizeof(float) * mp->G[l].tot);
+            mp->cor32.push_back(fa);
+            mp->res32.push_back(fb);
+            mp->tmp32.push_back(fc);
+        }
+    }
+    if ((e = mp_alloc(&mp->phi, mp->G[0].tot)) || (e = mp_alloc(&mp->rhs, mp->G[0].tot)) ||
+        (e = mp_alloc(&mp->part, kSumBlocks))) {
+        ksgf_macproj_destroy(mp);
+        return e;
+    }
+    (void)hipMemset(mp->phi, 0, sizeof(double) * mp->G[0].tot);
+    (void)hipMemset(mp->rhs, 0, sizeof(double) * mp->G[0].tot);
+    double** bws[6] = {&mp->bw.r, &mp->bw.rh, &mp->bw.p, &mp->bw.v, &mp->bw.s, &mp->bw.t};
+    for (auto pp : bws) {
+        if ((e = mp_alloc(pp, Lb.tot))) {
+            ksgf_macproj_destroy(mp);
+            return e;
+        }
+        (void)hipMemset(*pp, 0, sizeof(double) * Lb.tot);
+    }
+    if (hipMalloc(&mp->bw.stats, 2 * sizeof(int)) != hipSuccess || hipMalloc(&mp->dnorm, 8) != hipSuccess ||
+        hipHostMalloc(&mp->hnorm, 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&mp->hstats, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        ksgf_macproj_destroy(mp);
+        set_error("ksgf_macproj_create: scalar buffers");
+        return KSGF_ERR_NOMEM;
+    }
+    (void)hipMemset(mp->bw.stats, 0, 2 * sizeof(int));
+    if (hipDeviceSynchronize() != hipSuccess) {
+        ksgf_macproj_destroy(mp);
+        set_error("ksgf_macproj_create: device synchronize");
+        return KSGF_ERR_HIP;
+    }
+    set_level_coefs(mp);
+    *out = mp;
+    return KSGF_OK;
+}
+
+int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta)
+{
+    if (!mp || !(beta > 0.0)) {
+        set_error("ksgf_macproj_set_beta: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    mp->beta = beta;
+    auto h = level_h(mp);
+    for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta);
+    return KSGF_OK;
+}
+
+int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac, double rtol, double atol,
+                         ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!mp || !u_mac || !v_mac || !w_mac) {
+        set_error("ksgf_macproj_project: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    const ksgf_boxarray* ba = mp->ba;
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev cu, cv, cw;
+    MFDev mu[3];
+    double* mac[3] = {u_mac, v_mac, w_mac};
+    int e;
+    if ((e = cmfdev(ba, u_mac, KSGF_XFACE, 1, 1, &cu)) || (e = cmfdev(ba, v_mac, KSGF_YFACE, 1, 1, &cv)) ||
+        (e = cmfdev(ba, w_mac, KSGF_ZFACE, 1, 1, &cw)))
+        return e;
+    for (int d = 0; d < 3; ++d)
+        if ((e = mfdev(ba, mac[d], KSGF_XFACE + d, 1, 1, &mu[d]))) return e;
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* rv = get_region(ba, z, z, &err);
+    if (!rv) return err;
+    const LevelGeom& L0 = mp->G[0];
+    hipLaunchKernelGGL(k_mac_rhs, dim3(grid_for(rv->total, 256)), dim3(256), 0, st, cu, cv, cw, mp->rhs, L0,
+                       regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], 1.0 / ba->dx[0],
+                       1.0 / ba->dx[1], 1.0 / ba->dx[2]);
+    KSGF_LAUNCH_CHECK();
+    int ret = mg_solve(mp, rtol, atol, info, st);
+    if (ret < 0) return ret;
+    for (int d = 0; d < 3; ++d) {
+        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        ghi[d] = 1;
+        const Region* rf = get_region(ba, glo, ghi, &err);
+        if (!rf) return err;
+        double bd = mp->beta * (1.0 / ba->dx[d]);
+        dim3 gr(grid_for(rf->total, 256));
+        if (d == 0)
+            hipLaunchKernelGGL(k_mac_correct<0>, gr, dim3(256), 0, st, mu[0], (const double*)mp->phi, L0,
+                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        else if (d == 1)
+            hipLaunchKernelGGL(k_mac_correct<1>, gr, dim3(256), 0, st, mu[1], (const double*)mp->phi, L0,
+                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        else
+            hipLaunchKernelGGL(k_mac_correct<2>, gr, dim3(256), 0, st, mu[2], (const double*)mp->phi, L0,
+                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        KSGF_LAUNCH_CHECK();
+    }
+    return ret;
+}
+
+int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol, double atol,
+                       ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!mp || !rhs || !phi) {
+        set_error("ksgf_macproj_solve: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const LevelGeom& L0 = mp->G[0];
+    long long n = (long long)L0.n0 * L0.n1 * L0.n2;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, rhs, mp->rhs, L0, 1);
+    KSGF_LAUNCH_CHECK();
+    int ret = mg_solve(mp, rtol, atol, info, st);
+    if (ret < 0) return ret;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, (const double*)mp->phi, phi, L0, 0);
+    KSGF_LAUNCH_CHECK();
+    return ret;
+}
+
+int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream)
+{
+    if (!mp || !phi) {
+        set_error("ksgf_macproj_get_phi: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    const LevelGeom& L0 = mp->G[0];
+    long long n = (long long)L0.n0 * L0.n1 * L0.n2;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, (const double*)mp->phi, phi,
+                       L0, 0);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""Synthetic workloads of the advect+project step (seeded, decomposition
+independent: fields are generated domain-wide from a counter-based Philox
+stream and then scattered into the boxes).
+
+Velocity boundary conditions follow BCInterface.cpp:200-260:
+  periodic     -> int_dir (all components)
+  no_slip_wall -> ext_dir (all components, value 0)
+  slip_wall    -> tangential hoextrap, normal ext_dir (value 0)
+MAC projection domain BCs follow icns_advection.cpp:21-41: periodic stays
+periodic, walls are Neumann.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .layout import (BC_EXT_DIR, BC_HOEXTRAP, BC_INT_DIR, LO_NEUMANN, LO_PERIODIC, BoxArray, Geometry,
+                     chop_random, chop_uniform)
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    geom: Geometry
+    ba: BoxArray
+    vel: np.ndarray          # (3, nz, ny, nx) cell-centred velocity
+    rho: np.ndarray          # (1, nz, ny, nx)
+    gp: np.ndarray           # (3, nz, ny, nx) pressure gradient
+    bcrec: np.ndarray        # (3, 6) int32 velocity BCRecs (lo xyz, hi xyz)
+    bcval: np.ndarray        # (6, 3) Dirichlet values [orientation][comp]
+    lobc: tuple
+    hibc: tuple
+    cfl: float
+    nsteps: int
+    mg_rtol: float = 1e-11
+    mg_atol: float = 2.220446049250313e-16 * 1e2
+    rho0: float = 1.0
+    mlmg: dict = dataclasses.field(default_factory=dict)  # mac_proj.* overrides
+    extra: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def ncell(self):
+        return int(np.prod([int(n) for n in self.geom.n_cell]))
+
+
+def _philox(seed):
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def _periodic_bcs():
+    bc = np.zeros((3, 6), dtype=np.int32)  # int_dir everywhere
+    return bc, np.zeros((6, 3)), (LO_PERIODIC,) * 3, (LO_PERIODIC,) * 3
+
+
+def _abl_bcs():
+    """periodic x/y, zlo no_slip_wall, zhi slip_wall."""
+    bc = np.zeros((3, 6), dtype=np.int32)
+    for n in range(3):
+        bc[n, 2] = BC_EXT_DIR                         # zlo no-slip: all ext_dir
+        bc[n, 5] = BC_EXT_DIR if n == 2 else BC_HOEXTRAP  # zhi slip: normal ext_dir
+    return bc, np.zeros((6, 3)), (LO_PERIODIC, LO_PERIODIC, LO_NEUMANN), (LO_PERIODIC, LO_PERIODIC, LO_NEUMANN)
+
+
+def taylor_green(n=64, max_grid_size=32, noise=1e-3, seed=1, cfl=0.5, nsteps=1):
+    """Single-level n^3 unit cube, triply periodic; u = sin(2 pi x) cos(2 pi y)
+    cos(2 pi z), v = -cos sin cos, w = 0 (2 pi / L as TaylorGreenVortex.cpp:45-51)
+    plus U(-noise, noise) on every component."""
+    geom = Geometry((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (1, 1, 1))
+    ba = BoxArray(geom, chop_uniform(geom.n_cell, max_grid_size))
+    x = geom.cell_centers(0)[None, None, :]
+    y = geom.cell_centers(1)[None, :, None]
+    z = geom.cell_centers(2)[:, None, None]
+    tp = 2.0 * np.pi
+    u = np.sin(tp * x) * np.cos(tp * y) * np.cos(tp * z)
+    v = -np.cos(tp * x) * np.sin(tp * y) * np.cos(tp * z)
+    w = np.zeros_like(u)
+    vel = np.stack(np.broadcast_arrays(u, v, w)).astype(np.float64)
+    if noise:
+        vel += _philox(seed).uniform(-noise, noise, size=vel.shape)
+    bc, bcval, lobc, hibc = _periodic_bcs()
+    shape = (1, n, n, n)
+    return Case(f"tgv{n}", geom, ba, np.ascontiguousarray(vel), np.ones(shape), np.zeros((3, n, n, n)), bc, bcval,
+                lobc, hibc, cfl, nsteps)
+
+
+def abl(n_cell=(256, 256, 128), extent=(5120.0, 5120.0, 1024.0), sizes=(32, 48, 64), cfl=0.9, nsteps=10,
+        seed=2, box_seed=3, sigma=0.5, z_perturb=300.0):
+    """Neutral ABL box: periodic x/y, no-slip bottom, slip top; mean log law
+    u(z) = (0.4/0.4) ln(z/0.1) along x, seeded N(0, sigma^2) perturbations of
+    all three components below z_perturb; uneven boxes with edges drawn from
+    ``sizes``.  Potential temperature 300 K with 0.003 K/m above 700 m is
+    carried as a passive field (extra['theta'])."""
+    geom = Geometry(tuple(n_cell), (0.0, 0.0, 0.0), tuple(extent), (1, 1, 0))
+    ba = BoxArray(geom, chop_random(n_cell, list(sizes), _philox(box_seed)))
+    nx, ny, nz = n_cell
+    z = geom.cell_centers(2)
+    u = (0.4 / 0.4) * np.log(z / 0.1)
+    vel = np.zeros((3, nz, ny, nx))
+    vel[0] = u[:, None, None]
+    pert = _philox(seed).normal(0.0, sigma, size=vel.shape)
+    mask = (z < z_perturb)[None, :, None, None]
+    vel += np.where(mask, pert, 0.0)
+    theta = 300.0 + 0.003 * np.maximum(z - 700.0, 0.0)
+    bc, bcval, lobc, hibc = _abl_bcs()
+    c = Case(f"abl{nx}x{ny}x{nz}", geom, ba, np.ascontiguousarray(vel), np.ones((1, nz, ny, nx)),
+             np.zeros((3, nz, ny, nx)), bc, bcval, lobc, hibc, cfl, nsteps)
+    c.extra["theta"] = np.broadcast_to(theta[:, None, None], (nz, ny, nx)).copy()
+    # cell aspect ratio dx/dz = 2.5 (256x256x128) or 5 (256^3): standard
+    # coarsening with piecewise-constant prolongation stalls or diverges, so
+    # the deck turns on mac_proj.do_semicoarsening (LPInfo option,
+    # MLMGOptions.cpp:24)
+    c.mlmg["do_semicoarsening"] = 1
+    return c
+
+
+def random_case(n_cell, boxes=None, seed=7, periodic=(1, 1, 1), extent=(1.0, 1.0, 1.0), amp=1.0, gp_amp=0.0,
+                rho_var=0.0, cfl=0.5):
+    """Seeded random velocity (and optionally pressure gradient / density)
+    for parity tests on small grids."""
+    geom = Geometry(tuple(n_cell), (0.0, 0.0, 0.0), tuple(extent), tuple(periodic))
+    if boxes is None:
+        boxes = chop_uniform(n_cell, max(n_cell))
+    ba = BoxArray(geom, boxes)
+    nx, ny, nz = n_cell
+    g = _philox(seed)
+    vel = amp * g.uniform(-1.0, 1.0, size=(3, nz, ny, nx))
+    gp = gp_amp * g.uniform(-1.0, 1.0, size=(3, nz, ny, nx))
+    rho = 1.0 + rho_var * g.uniform(0.0, 1.0, size=(1, nz, ny, nx))
+    if all(periodic):
+        bc, bcval, lobc, hibc = _periodic_bcs()
+    else:
+        bc = np.zeros((3, 6), dtype=np.int32)
+        lobc, hibc = [LO_PERIODIC] * 3, [LO_PERIODIC] * 3
+        for d in range(3):
+            if not periodic[d]:
+                lobc[d] = hibc[d] = LO_NEUMANN
+                for n in range(3):
+                    bc[n, d] = BC_EXT_DIR
+                    bc[n, 3 + d] = BC_EXT_DIR if n == d else BC_HOEXTRAP
+        bcval = np.zeros((6, 3))
+        lobc, hibc = tuple(lobc), tuple(hibc)
+    return Case(f"rand{nx}x{ny}x{nz}", geom, ba, np.ascontiguousarray(vel), rho, gp, bc, bcval, lobc, hibc, cfl, 1)

@@ -1,0 +1,1533 @@
+/*
+ * ksgf_oracle.c -- CPU FP64 oracle for the Godunov PPM / MAC-projection step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ksgf_oracle.h for scope and sources).
+ *
+ * Every floating-point expression that the HIP kernels also evaluate is
+ * written in the same operand order, and both sides are compiled without
+ * floating-point contraction, so the advection results agree bit for bit;
+ * the multigrid differs only through the summation order of reductions.
+ *
+ * Points where the published method leaves a choice that only the absent
+ * AMReX-Hydro source could fix ("parity unpinned" against the reference):
+ *   - small_vel = 1e-8 in every upwind test;
+ *   - PPM boundary edge: cubic through the ghost (face) value and three
+ *     interior cell values, clipped, monotonised on the second cell only;
+ *   - Dirichlet edge states: both sides take the ghost value for the
+ *     normal velocity component, only the outer side for other components;
+ *   - conservative transverse terms: flux difference plus q times the
+ *     transverse divergence (dt/3 corner stage, dt/2 final stage);
+ *   - MLMG: inf-norm convergence test against max(atol, rtol*|rhs|_inf),
+ *     standard (all-direction) coarsening unless semicoarsening is asked for.
+ */
+#include "ksgf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SMALL_VEL 1.0e-8
+
+/* ------------------------------------------------------------------ */
+/* layout                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    long off;
+    int lo[3];
+    int n[3];
+} fab_t;
+
+static long make_fabs(const ko_geom* g, int itype, int ncomp, int ngrow, fab_t* f)
+{
+    long off = 0;
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        for (int d = 0; d < 3; ++d) {
+            int lo = bx[d] - ngrow;
+            int hi = bx[3 + d] + ngrow + ((itype == d + 1) ? 1 : 0);
+            f[b].lo[d] = lo;
+            f[b].n[d] = hi - lo + 1;
+        }
+        f[b].off = off;
+        off += (long)f[b].n[0] * f[b].n[1] * f[b].n[2] * ncomp;
+    }
+    return off;
+}
+
+static inline long FI(const fab_t* f, int i, int j, int k)
+{
+    return f->off + ((long)i - f->lo[0]) +
+           (long)f->n[0] * (((long)j - f->lo[1]) + (long)f->n[1] * ((long)k - f->lo[2]));
+}
+static inline long FS(const fab_t* f) { return (long)f->n[0] * f->n[1] * f->n[2]; }
+
+long ko_mf_size(const ko_geom* g, int itype, int ncomp, int ngrow)
+{
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    long s = make_fabs(g, itype, ncomp, ngrow, f);
+    free(f);
+    return s;
+}
+
+/* local scratch array over an index box */
+typedef struct {
+    int lo[3], n[3], nc;
+    double* p;
+} la_t;
+
+static void la_alloc(la_t* a, int l0, int l1, int l2, int h0, int h1, int h2, int nc)
+{
+    a->lo[0] = l0; a->lo[1] = l1; a->lo[2] = l2;
+    a->n[0] = h0 - l0 + 1; a->n[1] = h1 - l1 + 1; a->n[2] = h2 - l2 + 1;
+    a->nc = nc;
+    a->p = (double*)calloc((size_t)a->n[0] * a->n[1] * a->n[2] * nc, sizeof(double));
+}
+static inline double* LA(const la_t* a, int i, int j, int k, int c)
+{
+    return a->p + (((long)i - a->lo[0]) +
+                   (long)a->n[0] * (((long)j - a->lo[1]) +
+                                    (long)a->n[1] * (((long)k - a->lo[2]) + (long)a->n[2] * c)));
+}
+
+/* ------------------------------------------------------------------ */
+/* halo exchange: interior and periodic ghosts                        */
+/* (FillBoundary, icns_advection.H:239-243)                           */
+/* ------------------------------------------------------------------ */
+static int* make_owner(const ko_geom* g)
+{
+    int n0 = g->dhi[0] - g->dlo[0] + 1, n1 = g->dhi[1] - g->dlo[1] + 1,
+        n2 = g->dhi[2] - g->dlo[2] + 1;
+    int* own = (int*)malloc(sizeof(int) * (size_t)n0 * n1 * n2);
+    for (long q = 0; q < (long)n0 * n1 * n2; ++q) own[q] = -1;
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        for (int k = bx[2]; k <= bx[5]; ++k)
+            for (int j = bx[1]; j <= bx[4]; ++j)
+                for (int i = bx[0]; i <= bx[3]; ++i)
+                    own[(i - g->dlo[0]) +
+                        (long)n0 * ((j - g->dlo[1]) + (long)n1 * (k - g->dlo[2]))] = b;
+    }
+    return own;
+}
+
+void ko_fill_boundary(const ko_geom* g, double* mf, int itype, int ncomp, int ngrow)
+{
+    if (ngrow <= 0) return;
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    make_fabs(g, itype, ncomp, ngrow, f);
+    int* own = make_owner(g);
+    int L[3], n0 = g->dhi[0] - g->dlo[0] + 1, n1 = g->dhi[1] - g->dlo[1] + 1;
+    for (int d = 0; d < 3; ++d) L[d] = g->dhi[d] - g->dlo[d] + 1;
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const fab_t* fb = &f[b];
+        int vlo[3], vhi[3];
+        for (int d = 0; d < 3; ++d) {
+            vlo[d] = bx[d];
+            vhi[d] = bx[3 + d] + ((itype == d + 1) ? 1 : 0);
+        }
+        for (int k = fb->lo[2]; k < fb->lo[2] + fb->n[2]; ++k)
+            for (int j = fb->lo[1]; j < fb->lo[1] + fb->n[1]; ++j)
+                for (int i = fb->lo[0]; i < fb->lo[0] + fb->n[0]; ++i) {
+                    int p[3] = {i, j, k};
+                    if (p[0] >= vlo[0] && p[0] <= vhi[0] && p[1] >= vlo[1] && p[1] <= vhi[1] &&
+                        p[2] >= vlo[2] && p[2] <= vhi[2])
+                        continue;
+                    int w[3], ok = 1;
+                    for (int d = 0; d < 3; ++d) {
+                        int nodal = (itype == d + 1);
+                        int x = p[d];
+                        if (g->periodic[d]) {
+                            x = g->dlo[d] + (((x - g->dlo[d]) % L[d]) + L[d]) % L[d];
+                        } else {
+                            int top = g->dhi[d] + (nodal ? 1 : 0);
+                            if (x < g->dlo[d] || x > top) ok = 0;
+                        }
+                        w[d] = x;
+                    }
+                    if (!ok) continue;
+                    int c[3] = {w[0], w[1], w[2]};
+                    if (itype >= 1) {
+                        int d = itype - 1;
+                        if (c[d] > g->dhi[d]) c[d] -= 1;
+                    }
+                    int src = own[(c[0] - g->dlo[0]) +
+                                  (long)n0 * ((c[1] - g->dlo[1]) + (long)n1 * (c[2] - g->dlo[2]))];
+                    if (src < 0) continue;
+                    const fab_t* fs = &f[src];
+                    for (int n = 0; n < ncomp; ++n)
+                        mf[FI(fb, i, j, k) + n * FS(fb)] = mf[FI(fs, w[0], w[1], w[2]) + n * FS(fs)];
+                }
+    }
+    free(own);
+    free(f);
+}
+
+/* Physical-boundary ghosts, one direction after the other (later directions
+ * overwrite corners).  Cell data: ext_dir -> boundary value; foextrap -> copy
+ * of the boundary cell; hoextrap -> first ghost from the quadratic through
+ * three interior cells evaluated at the face, deeper ghosts copy the boundary
+ * cell.  Face data: ext_dir -> value; foextrap/hoextrap -> copy of the
+ * nearest valid face (the boundary face itself in the normal direction).
+ * bcval is [orientation(6: xlo ylo zlo xhi yhi zhi)][ncomp]. */
+void ko_fill_physbc(const ko_geom* g, double* mf, int itype, int ncomp, int ngrow,
+                    const ko_bcrec* bc, const double* bcval)
+{
+    if (ngrow <= 0) return;
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    make_fabs(g, itype, ncomp, ngrow, f);
+    for (int d = 0; d < 3; ++d) {
+        if (g->periodic[d]) continue;
+        int nodal = (itype == d + 1);
+        int dlo = g->dlo[d], dhi = g->dhi[d] + (nodal ? 1 : 0);
+#pragma omp parallel for schedule(dynamic)
+        for (int b = 0; b < g->nbox; ++b) {
+            const fab_t* fb = &f[b];
+            long st[3] = {1, fb->n[0], (long)fb->n[0] * fb->n[1]};
+            for (int k = fb->lo[2]; k < fb->lo[2] + fb->n[2]; ++k)
+                for (int j = fb->lo[1]; j < fb->lo[1] + fb->n[1]; ++j)
+                    for (int i = fb->lo[0]; i < fb->lo[0] + fb->n[0]; ++i) {
+                        int p[3] = {i, j, k};
+                        int x = p[d];
+                        int side;
+                        if (x < dlo) side = 0;
+                        else if (x > dhi) side = 1;
+                        else continue;
+                        long at = FI(fb, i, j, k);
+                        int m = side ? (x - dhi) : (dlo - x); /* ghost depth >= 1 */
+                        long sgn = side ? -1 : 1;             /* step towards interior */
+                        long bnd = at + sgn * (long)m * st[d]; /* boundary cell / face */
+                        for (int n = 0; n < ncomp; ++n) {
+                            int t = side ? bc[n].hi[d] : bc[n].lo[d];
+                            long cs = n * FS(fb);
+                            double* v = mf + cs;
+                            if (t == KO_BC_EXT_DIR) {
+                                v[at] = bcval[(side ? 3 + d : d) * ncomp + n];
+                            } else if (t == KO_BC_FOEXTRAP || (t == KO_BC_HOEXTRAP && itype >= 1)) {
+                                v[at] = v[bnd];
+                            } else if (t == KO_BC_HOEXTRAP) {
+                                if (m == 1) {
+                                    int nin = g->dhi[d] - g->dlo[d] + 1;
+                                    if (nin >= 3)
+                                        v[at] = 0.125 * (15.0 * v[bnd] - 10.0 * v[bnd + sgn * st[d]] +
+                                                         3.0 * v[bnd + 2 * sgn * st[d]]);
+                                    else
+                                        v[at] = 0.5 * (3.0 * v[bnd] - v[bnd + sgn * st[d]]);
+                                } else {
+                                    v[at] = v[bnd];
+                                }
+                            }
+                        }
+                    }
+        }
+    }
+    free(f);
+}
+
+/* ------------------------------------------------------------------ */
+/* PPM (Colella & Woodward 1984)                                      */
+/* ------------------------------------------------------------------ */
+static inline double vanleer(double a, double b, double c)
+{
+    double dsc = 0.5 * (c - a);
+    double dsl = 2.0 * (b - a);
+    double dsr = 2.0 * (c - b);
+    if (dsl * dsr > 0.0) {
+        double m = fmin(fabs(dsc), fmin(fabs(dsl), fabs(dsr)));
+        return copysign(m, dsc);
+    }
+    return 0.0;
+}
+static inline double clip2(double e, double a, double b)
+{
+    e = fmax(e, fmin(a, b));
+    return fmin(e, fmax(a, b));
+}
+static inline void mono(double s0, double* sm, double* sp)
+{
+    if ((*sp - s0) * (s0 - *sm) <= 0.0) {
+        *sp = s0;
+        *sm = s0;
+    } else if (fabs(*sp - s0) >= 2.0 * fabs(*sm - s0)) {
+        *sp = 3.0 * s0 - 2.0 * (*sm);
+    } else if (fabs(*sm - s0) >= 2.0 * fabs(*sp - s0)) {
+        *sm = 3.0 * s0 - 2.0 * (*sp);
+    }
+}
+/* cubic through a face value half a cell outside cell 0 and the centres of
+ * cells 0,1,2, evaluated at the face between cells 0 and 1 */
+static inline double bnd_edge(double sb, double s0, double s1, double s2)
+{
+    return -0.2 * sb + 0.75 * s0 + 0.5 * s1 - 0.05 * s2;
+}
+
+/* left (sm) and right (sp) edge values of the cell whose 5-point stencil is s[] */
+static void ppm_edges(const double* s, int lim, int bclo, int bchi, int x, int dlo, int dhi,
+                      double* psm, double* psp)
+{
+    const double sm2 = s[0], sm1 = s[1], s0 = s[2], sp1 = s[3], sp2 = s[4];
+    double e1, e2;
+    const int limited = (lim != KO_PPM_NOLIMITER);
+    if (!limited) {
+        e1 = (7.0 / 12.0) * (sm1 + s0) - (1.0 / 12.0) * (sm2 + sp1);
+        e2 = (7.0 / 12.0) * (s0 + sp1) - (1.0 / 12.0) * (sm1 + sp2);
+    } else {
+        double d0 = vanleer(sm2, sm1, s0);
+        double d1 = vanleer(sm1, s0, sp1);
+        double d2 = vanleer(s0, sp1, sp2);
+        e1 = 0.5 * (sm1 + s0) - (d1 - d0) / 6.0;
+        e1 = clip2(e1, sm1, s0);
+        e2 = 0.5 * (s0 + sp1) - (d2 - d1) / 6.0;
+        e2 = clip2(e2, s0, sp1);
+    }
+    const int lw = (bclo == KO_BC_EXT_DIR || bclo == KO_BC_HOEXTRAP);
+    const int hw = (bchi == KO_BC_EXT_DIR || bchi == KO_BC_HOEXTRAP);
+    double sm, sp;
+    if (lw && x == dlo) {
+        sm = sm1;
+        sp = bnd_edge(sm1, s0, sp1, sp2);
+        if (limited) sp = clip2(sp, s0, sp1);
+    } else if (lw && x == dlo + 1) {
+        sm = bnd_edge(sm2, sm1, s0, sp1);
+        if (limited) sm = clip2(sm, sm1, s0);
+        sp = e2;
+        if (limited) mono(s0, &sm, &sp);
+    } else if (hw && x == dhi) {
+        sp = sp1;
+        sm = bnd_edge(sp1, s0, sm1, sm2);
+        if (limited) sm = clip2(sm, sm1, s0);
+    } else if (hw && x == dhi - 1) {
+        sp = bnd_edge(sp2, sp1, s0, sm1);
+        if (limited) sp = clip2(sp, s0, sp1);
+        sm = e1;
+        if (limited) mono(s0, &sm, &sp);
+    } else {
+        sm = e1;
+        sp = e2;
+        if (limited) mono(s0, &sm, &sp);
+    }
+    *psm = sm;
+    *psp = sp;
+}
+
+static inline double minmod_slope(const double* s, int bclo, int bchi, int x, int dlo, int dhi)
+{
+    const double sm1 = s[1], s0 = s[2], sp1 = s[3];
+    const int lw = (bclo == KO_BC_EXT_DIR || bclo == KO_BC_HOEXTRAP);
+    const int hw = (bchi == KO_BC_EXT_DIR || bchi == KO_BC_HOEXTRAP);
+    if (lw && x == dlo) return s0 - sm1;
+    if (hw && x == dhi) return sp1 - s0;
+    double a = s0 - sm1, b = sp1 - s0;
+    if (a * b <= 0.0) return 0.0;
+    return (fabs(a) < fabs(b)) ? a : b;
+}
+
+/* Time-centred states at the low (Im) and high (Ip) faces of one cell.
+ * um / up: advecting velocity at the low / high face (for velocity
+ * prediction both are the cell-centred normal velocity). */
+static void ppm_trace(const double* s, int lim, int bclo, int bchi, int x, int dlo, int dhi,
+                      double um, double up, double dtdx, double* Im, double* Ip)
+{
+    const double s0 = s[2];
+    if (lim == KO_PPM_UPWIND) {
+        *Im = s0;
+        *Ip = s0;
+        return;
+    }
+    if (lim == KO_PPM_MINMOD) {
+        double sl = minmod_slope(s, bclo, bchi, x, dlo, dhi);
+        *Ip = (up > SMALL_VEL) ? s0 + 0.5 * (1.0 - up * dtdx) * sl : s0;
+        *Im = (um < -SMALL_VEL) ? s0 - 0.5 * (1.0 + um * dtdx) * sl : s0;
+        return;
+    }
+    double sm, sp;
+    ppm_edges(s, lim, bclo, bchi, x, dlo, dhi, &sm, &sp);
+    double s6 = 6.0 * s0 - 3.0 * (sm + sp);
+    if (up > SMALL_VEL) {
+        double sig = fabs(up) * dtdx;
+        *Ip = sp - (0.5 * sig) * ((sp - sm) - (1.0 - (2.0 / 3.0) * sig) * s6);
+    } else {
+        *Ip = s0;
+    }
+    if (um < -SMALL_VEL) {
+        double sig = fabs(um) * dtdx;
+        *Im = sm + (0.5 * sig) * ((sp - sm) + (1.0 - (2.0 / 3.0) * sig) * s6);
+    } else {
+        *Im = s0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Riemann / upwind helpers and edge boundary conditions              */
+/* ------------------------------------------------------------------ */
+static inline double burgers(double lo, double hi)
+{
+    double st = ((lo + hi) >= 0.0) ? lo : hi;
+    int ltm = ((lo <= 0.0 && hi >= 0.0) || (fabs(lo + hi) < SMALL_VEL));
+    return ltm ? 0.0 : st;
+}
+static inline double upwind(double lo, double hi, double u)
+{
+    if (fabs(u) < SMALL_VEL) return 0.5 * (lo + hi);
+    return (u >= 0.0) ? lo : hi;
+}
+/* Edge BCs on the face states lo/hi at face index x of direction d:
+ * glo / ghi are the ghost (boundary) values just outside the domain.
+ * both_dir: Dirichlet sets both sides (normal velocity component). */
+static inline void edge_bcs(int x, int dlo, int dhi, int bclo, int bchi, double glo,
+                            double ghi, int both_dir, double* lo, double* hi)
+{
+    if (x == dlo) {
+        if (bclo == KO_BC_EXT_DIR) {
+            *lo = glo;
+            if (both_dir) *hi = *lo;
+        } else if (bclo == KO_BC_FOEXTRAP || bclo == KO_BC_HOEXTRAP) {
+            *lo = *hi;
+        }
+    }
+    if (x == dhi + 1) {
+        if (bchi == KO_BC_EXT_DIR) {
+            *hi = ghi;
+            if (both_dir) *lo = *hi;
+        } else if (bchi == KO_BC_FOEXTRAP || bchi == KO_BC_HOEXTRAP) {
+            *hi = *lo;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* PredictStateOnX/Y/ZFace (a15; test_mflux_schemes.cpp:87-151)       */
+/* ------------------------------------------------------------------ */
+void ko_ppm_predict_state(const ko_geom* g, int dir, const double* s, int ncomp_s, int ng_s,
+                          int comp, const double* vel_edge, int ng_v, const ko_bcrec* bc,
+                          double dt, int limiter, double* Im, double* Ip)
+{
+    fab_t* fs = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    fab_t* fv = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    fab_t* fo = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    make_fabs(g, KO_CELL, ncomp_s, ng_s, fs);
+    make_fabs(g, KO_XFACE + dir, 1, ng_v, fv);
+    make_fabs(g, KO_CELL, 1, 0, fo);
+    const double dtdx = dt / g->dx[dir];
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        long st[3] = {1, fs[b].n[0], (long)fs[b].n[0] * fs[b].n[1]};
+        long sv[3] = {1, fv[b].n[0], (long)fv[b].n[0] * fv[b].n[1]};
+        for (int k = bx[2]; k <= bx[5]; ++k)
+            for (int j = bx[1]; j <= bx[4]; ++j)
+                for (int i = bx[0]; i <= bx[3]; ++i) {
+                    int p[3] = {i, j, k};
+                    long c = FI(&fs[b], i, j, k) + comp * FS(&fs[b]);
+                    double w[5];
+                    for (int m = 0; m < 5; ++m) w[m] = s[c + (m - 2) * st[dir]];
+                    long v = FI(&fv[b], i, j, k);
+                    double um = vel_edge[v], up = vel_edge[v + sv[dir]];
+                    double a, bb;
+                    ppm_trace(w, limiter, bc->lo[dir], bc->hi[dir], p[dir], g->dlo[dir],
+                              g->dhi[dir], um, up, dtdx, &a, &bb);
+                    Im[FI(&fo[b], i, j, k)] = a;
+                    Ip[FI(&fo[b], i, j, k)] = bb;
+                }
+    }
+    free(fs);
+    free(fv);
+    free(fo);
+}
+
+/* ------------------------------------------------------------------ */
+/* ExtrapVelToFaces (a6; call site icns_advection.H:211-216)           */
+/* ------------------------------------------------------------------ */
+int ko_extrap_vel_to_faces(const ko_geom* g, const double* vel, const double* f,
+                           double* umac, double* vmac, double* wmac, const ko_bcrec* bc,
+                           double dt, int use_ppm, int use_forces_in_trans, int limiter,
+                           int allow_inflow_on_outflow)
+{
+    if (!use_ppm) return -2;
+    const int NB = g->nbox;
+    fab_t* fvel = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t* ff = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t* fm[3];
+    make_fabs(g, KO_CELL, 3, 3, fvel);
+    make_fabs(g, KO_CELL, 3, 1, ff);
+    double* mac[3] = {umac, vmac, wmac};
+    for (int d = 0; d < 3; ++d) {
+        fm[d] = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+        make_fabs(g, KO_XFACE + d, 1, 1, fm[d]);
+    }
+    const double hdt = 0.5 * dt;
+    double dtdx[3], c6[3], c4[3];
+    for (int d = 0; d < 3; ++d) {
+        dtdx[d] = dt / g->dx[d];
+        c6[d] = dt / (6.0 * g->dx[d]);
+        c4[d] = dt / (4.0 * g->dx[d]);
+    }
+    const int* dlo = g->dlo;
+    const int* dhi = g->dhi;
+
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const int lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
+        const fab_t* fv = &fvel[b];
+        const fab_t* fb = &ff[b];
+        const long vs = FS(fv), fsz = FS(fb);
+        long st[3] = {1, fv->n[0], (long)fv->n[0] * fv->n[1]};
+
+        /* stage 0: PPM traces on the box grown by one, all components */
+        la_t Im[3], Ip[3];
+        for (int d = 0; d < 3; ++d) {
+            la_alloc(&Im[d], lo[0] - 1, lo[1] - 1, lo[2] - 1, hi[0] + 1, hi[1] + 1, hi[2] + 1, 3);
+            la_alloc(&Ip[d], lo[0] - 1, lo[1] - 1, lo[2] - 1, hi[0] + 1, hi[1] + 1, hi[2] + 1, 3);
+        }
+        for (int k = lo[2] - 1; k <= hi[2] + 1; ++k)
+            for (int j = lo[1] - 1; j <= hi[1] + 1; ++j)
+                for (int i = lo[0] - 1; i <= hi[0] + 1; ++i) {
+                    int p[3] = {i, j, k};
+                    long c = FI(fv, i, j, k);
+                    for (int d = 0; d < 3; ++d) {
+                        double ud = vel[c + d * vs];
+                        for (int n = 0; n < 3; ++n) {
+                            double w[5];
+                            for (int m = 0; m < 5; ++m) w[m] = vel[c + n * vs + (m - 2) * st[d]];
+                            ppm_trace(w, limiter, bc[n].lo[d], bc[n].hi[d], p[d], dlo[d], dhi[d],
+                                      ud, ud, dtdx[d], LA(&Im[d], i, j, k, n),
+                                      LA(&Ip[d], i, j, k, n));
+                        }
+                    }
+                }
+
+        /* stages 1-2: normal face states, advective velocity, upwinded edges
+         * on face boxes grown by one in the two transverse directions */
+        la_t flo[3], fhi[3], fed[3], uad[3];
+        for (int d = 0; d < 3; ++d) {
+            int l[3], h[3];
+            for (int e = 0; e < 3; ++e) {
+                l[e] = lo[e] - (e == d ? 0 : 1);
+                h[e] = hi[e] + 1;
+            }
+            la_alloc(&flo[d], l[0], l[1], l[2], h[0], h[1], h[2], 3);
+            la_alloc(&fhi[d], l[0], l[1], l[2], h[0], h[1], h[2], 3);
+            la_alloc(&fed[d], l[0], l[1], l[2], h[0], h[1], h[2], 3);
+            la_alloc(&uad[d], l[0], l[1], l[2], h[0], h[1], h[2], 1);
+            for (int k = l[2]; k <= h[2]; ++k)
+                for (int j = l[1]; j <= h[1]; ++j)
+                    for (int i = l[0]; i <= h[0]; ++i) {
+                        int p[3] = {i, j, k};
+                        int q[3] = {i, j, k};
+                        q[d] -= 1; /* cell on the low side */
+                        int x = p[d];
+                        /* ghost values just outside the domain along d */
+                        int gl[3] = {i, j, k}, gh[3] = {i, j, k};
+                        gl[d] = dlo[d] - 1;
+                        gh[d] = dhi[d] + 1;
+                        for (int n = 0; n < 3; ++n) {
+                            double lov = *LA(&Ip[d], q[0], q[1], q[2], n);
+                            double hiv = *LA(&Im[d], i, j, k, n);
+                            if (use_forces_in_trans) {
+                                lov += hdt * f[FI(fb, q[0], q[1], q[2]) + n * fsz];
+                                hiv += hdt * f[FI(fb, i, j, k) + n * fsz];
+                            }
+                            double glo = 0.0, ghi = 0.0;
+                            if (x == dlo[d]) glo = vel[FI(fv, gl[0], gl[1], gl[2]) + n * vs];
+                            if (x == dhi[d] + 1) ghi = vel[FI(fv, gh[0], gh[1], gh[2]) + n * vs];
+                            edge_bcs(x, dlo[d], dhi[d], bc[n].lo[d], bc[n].hi[d], glo, ghi, 1,
+                                     &lov, &hiv);
+                            *LA(&flo[d], i, j, k, n) = lov;
+                            *LA(&fhi[d], i, j, k, n) = hiv;
+                        }
+                        double ua = burgers(*LA(&flo[d], i, j, k, d), *LA(&fhi[d], i, j, k, d));
+                        *LA(&uad[d], i, j, k, 0) = ua;
+                        for (int n = 0; n < 3; ++n)
+                            *LA(&fed[d], i, j, k, n) =
+                                upwind(*LA(&flo[d], i, j, k, n), *LA(&fhi[d], i, j, k, n), ua);
+                    }
+        }
+
+        /* stage 3: corner-coupled states.  crn[d][t]: faces normal to d,
+         * component = the third direction e (!= d, != t), corrected by the
+         * transverse derivative along t. Only the component needed by the
+         * final normal-velocity state is formed: for faces normal to d we
+         * need component e (the final state of component e on faces
+         * normal to e uses corner states normal to d and t). */
+        la_t crn[3][3];
+        for (int d = 0; d < 3; ++d)
+            for (int t = 0; t < 3; ++t) {
+                crn[d][t].p = NULL;
+                if (t == d) continue;
+                int e = 3 - d - t; /* component index, also the final face direction */
+                int l[3], h[3];
+                for (int a = 0; a < 3; ++a) {
+                    l[a] = lo[a];
+                    h[a] = hi[a] + (a == d ? 1 : 0);
+                }
+                l[e] -= 1;
+                h[e] += 1;
+                la_alloc(&crn[d][t], l[0], l[1], l[2], h[0], h[1], h[2], 1);
+                for (int k = l[2]; k <= h[2]; ++k)
+                    for (int j = l[1]; j <= h[1]; ++j)
+                        for (int i = l[0]; i <= h[0]; ++i) {
+                            int p[3] = {i, j, k};
+                            int q[3] = {i, j, k};
+                            q[d] -= 1;
+                            int qt[3] = {q[0], q[1], q[2]}, pt[3] = {i, j, k};
+                            qt[t] += 1;
+                            pt[t] += 1;
+                            /* transverse derivative along t at the low cell q and high cell p */
+                            double lov = *LA(&flo[d], i, j, k, e) -
+                                         c6[t] * (*LA(&uad[t], qt[0], qt[1], qt[2], 0) +
+                                                  *LA(&uad[t], q[0], q[1], q[2], 0)) *
+                                             (*LA(&fed[t], qt[0], qt[1], qt[2], e) -
+                                              *LA(&fed[t], q[0], q[1], q[2], e));
+                            double hiv = *LA(&fhi[d], i, j, k, e) -
+                                         c6[t] * (*LA(&uad[t], pt[0], pt[1], pt[2], 0) +
+                                                  *LA(&uad[t], i, j, k, 0)) *
+                                             (*LA(&fed[t], pt[0], pt[1], pt[2], e) -
+                                              *LA(&fed[t], i, j, k, e));
+                            int x = p[d];
+                            int gl[3] = {i, j, k}, gh[3] = {i, j, k};
+                            gl[d] = dlo[d] - 1;
+                            gh[d] = dhi[d] + 1;
+                            double glo = 0.0, ghi = 0.0;
+                            if (x == dlo[d]) glo = vel[FI(fv, gl[0], gl[1], gl[2]) + e * vs];
+                            if (x == dhi[d] + 1) ghi = vel[FI(fv, gh[0], gh[1], gh[2]) + e * vs];
+                            edge_bcs(x, dlo[d], dhi[d], bc[e].lo[d], bc[e].hi[d], glo, ghi, 1, &lov,
+                                     &hiv);
+                            *LA(&crn[d][t], i, j, k, 0) =
+                                upwind(lov, hiv, *LA(&uad[d], i, j, k, 0));
+                        }
+            }
+
+        /* stage 4: final normal velocity on the faces of the valid box */
+        for (int d = 0; d < 3; ++d) {
+            int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+            const fab_t* fo = &fm[d][b];
+            double* out = mac[d];
+            for (int k = lo[2]; k <= hi[2] + (d == 2); ++k)
+                for (int j = lo[1]; j <= hi[1] + (d == 1); ++j)
+                    for (int i = lo[0]; i <= hi[0] + (d == 0); ++i) {
+                        int p[3] = {i, j, k};
+                        int q[3] = {i, j, k};
+                        q[d] -= 1;
+                        /* corner states normal to t1 carry component d corrected along t2 */
+                        int q1[3] = {q[0], q[1], q[2]}, p1[3] = {i, j, k};
+                        q1[t1] += 1;
+                        p1[t1] += 1;
+                        int q2[3] = {q[0], q[1], q[2]}, p2[3] = {i, j, k};
+                        q2[t2] += 1;
+                        p2[t2] += 1;
+                        double stl = *LA(&flo[d], i, j, k, d) -
+                                     c4[t1] * (*LA(&uad[t1], q1[0], q1[1], q1[2], 0) +
+                                               *LA(&uad[t1], q[0], q[1], q[2], 0)) *
+                                         (*LA(&crn[t1][t2], q1[0], q1[1], q1[2], 0) -
+                                          *LA(&crn[t1][t2], q[0], q[1], q[2], 0)) -
+                                     c4[t2] * (*LA(&uad[t2], q2[0], q2[1], q2[2], 0) +
+                                               *LA(&uad[t2], q[0], q[1], q[2], 0)) *
+                                         (*LA(&crn[t2][t1], q2[0], q2[1], q2[2], 0) -
+                                          *LA(&crn[t2][t1], q[0], q[1], q[2], 0));
+                        double sth = *LA(&fhi[d], i, j, k, d) -
+                                     c4[t1] * (*LA(&uad[t1], p1[0], p1[1], p1[2], 0) +
+                                               *LA(&uad[t1], i, j, k, 0)) *
+                                         (*LA(&crn[t1][t2], p1[0], p1[1], p1[2], 0) -
+                                          *LA(&crn[t1][t2], i, j, k, 0)) -
+                                     c4[t2] * (*LA(&uad[t2], p2[0], p2[1], p2[2], 0) +
+                                               *LA(&uad[t2], i, j, k, 0)) *
+                                         (*LA(&crn[t2][t1], p2[0], p2[1], p2[2], 0) -
+                                          *LA(&crn[t2][t1], i, j, k, 0));
+                        if (!use_forces_in_trans) {
+                            stl += hdt * f[FI(fb, q[0], q[1], q[2]) + d * fsz];
+                            sth += hdt * f[FI(fb, i, j, k) + d * fsz];
+                        }
+                        int x = p[d];
+                        int gl[3] = {i, j, k}, gh[3] = {i, j, k};
+                        gl[d] = dlo[d] - 1;
+                        gh[d] = dhi[d] + 1;
+                        double glo = 0.0, ghi = 0.0;
+                        if (x == dlo[d]) glo = vel[FI(fv, gl[0], gl[1], gl[2]) + d * vs];
+                        if (x == dhi[d] + 1) ghi = vel[FI(fv, gh[0], gh[1], gh[2]) + d * vs];
+                        edge_bcs(x, dlo[d], dhi[d], bc[d].lo[d], bc[d].hi[d], glo, ghi, 1, &stl,
+                                 &sth);
+                        if (!allow_inflow_on_outflow) {
+                            int tl = bc[d].lo[d], th = bc[d].hi[d];
+                            if (x == dlo[d] && (tl == KO_BC_FOEXTRAP || tl == KO_BC_HOEXTRAP)) {
+                                sth = fmin(sth, 0.0);
+                                stl = sth;
+                            }
+                            if (x == dhi[d] + 1 && (th == KO_BC_FOEXTRAP || th == KO_BC_HOEXTRAP)) {
+                                stl = fmax(stl, 0.0);
+                                sth = stl;
+                            }
+                        }
+                        out[FI(fo, i, j, k)] = burgers(stl, sth);
+                    }
+        }
+
+        for (int d = 0; d < 3; ++d) {
+            free(Im[d].p); free(Ip[d].p);
+            free(flo[d].p); free(fhi[d].p); free(fed[d].p); free(uad[d].p);
+            for (int t = 0; t < 3; ++t) free(crn[d][t].p);
+        }
+    }
+    free(fvel);
+    free(ff);
+    for (int d = 0; d < 3; ++d) free(fm[d]);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* ComputeFluxesOnBoxFromState (a13; icns_advection.H:359-373,         */
+/* AdvOp_Godunov.H:231-250)                                            */
+/* ------------------------------------------------------------------ */
+int ko_compute_fluxes_from_state(const ko_geom* g, int ncomp, const double* q,
+                                 const double* q_bc, double* flux_x, double* flux_y,
+                                 double* flux_z, double* face_x, double* face_y,
+                                 double* face_z, int known_edge_state, const double* umac,
+                                 const double* vmac, const double* wmac, const double* divu,
+                                 const double* fq, double dt, const ko_bcrec* bc,
+                                 const int* iconserv, int use_ppm, int use_forces_in_trans,
+                                 int is_velocity, int area_weighted, int limiter,
+                                 int allow_inflow_on_outflow)
+{
+    if (!use_ppm && !known_edge_state) return -2;
+    const int NB = g->nbox;
+    fab_t* fq3 = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t* ff1 = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t* fdv = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t *fm[3], *fo[3];
+    make_fabs(g, KO_CELL, ncomp, 3, fq3);
+    make_fabs(g, KO_CELL, ncomp, 1, ff1);
+    make_fabs(g, KO_CELL, 1, 1, fdv);
+    const double* mac[3] = {umac, vmac, wmac};
+    double* flux[3] = {flux_x, flux_y, flux_z};
+    double* face[3] = {face_x, face_y, face_z};
+    for (int d = 0; d < 3; ++d) {
+        fm[d] = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+        fo[d] = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+        make_fabs(g, KO_XFACE + d, 1, 1, fm[d]);
+        make_fabs(g, KO_XFACE + d, ncomp, 0, fo[d]);
+    }
+    const double hdt = 0.5 * dt;
+    double dtdx[3], c3[3], c6[3], c2[3], c4[3], area[3];
+    for (int d = 0; d < 3; ++d) {
+        dtdx[d] = dt / g->dx[d];
+        c3[d] = dt / (3.0 * g->dx[d]);
+        c6[d] = dt / (6.0 * g->dx[d]);
+        c2[d] = dt / (2.0 * g->dx[d]);
+        c4[d] = dt / (4.0 * g->dx[d]);
+    }
+    area[0] = g->dx[1] * g->dx[2];
+    area[1] = g->dx[0] * g->dx[2];
+    area[2] = g->dx[0] * g->dx[1];
+    const int* dlo = g->dlo;
+    const int* dhi = g->dhi;
+
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const int lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
+        if (known_edge_state) {
+            for (int d = 0; d < 3; ++d) {
+                const fab_t* fmm = &fm[d][b];
+                const fab_t* foo = &fo[d][b];
+                for (int n = 0; n < ncomp; ++n)
+                    for (int k = lo[2]; k <= hi[2] + (d == 2); ++k)
+                        for (int j = lo[1]; j <= hi[1] + (d == 1); ++j)
+                            for (int i = lo[0]; i <= hi[0] + (d == 0); ++i) {
+                                long o = FI(foo, i, j, k) + n * FS(foo);
+                                double u = mac[d][FI(fmm, i, j, k)];
+                                flux[d][o] = area_weighted ? (u * area[d]) * face[d][o] : u * face[d][o];
+                            }
+            }
+            continue;
+        }
+        const fab_t* fqb = &fq3[b];
+        const fab_t* ffb = &ff1[b];
+        const fab_t* fdb = &fdv[b];
+        const long qs = FS(fqb), fs1 = FS(ffb);
+        long st[3] = {1, fqb->n[0], (long)fqb->n[0] * fqb->n[1]};
+        long sm[3][3];
+        for (int d = 0; d < 3; ++d) {
+            sm[d][0] = 1;
+            sm[d][1] = fm[d][b].n[0];
+            sm[d][2] = (long)fm[d][b].n[0] * fm[d][b].n[1];
+        }
+
+        /* stage 0: PPM traces with the MAC velocity */
+        la_t Im[3], Ip[3];
+        for (int d = 0; d < 3; ++d) {
+            la_alloc(&Im[d], lo[0] - 1, lo[1] - 1, lo[2] - 1, hi[0] + 1, hi[1] + 1, hi[2] + 1, ncomp);
+            la_alloc(&Ip[d], lo[0] - 1, lo[1] - 1, lo[2] - 1, hi[0] + 1, hi[1] + 1, hi[2] + 1, ncomp);
+        }
+        for (int k = lo[2] - 1; k <= hi[2] + 1; ++k)
+            for (int j = lo[1] - 1; j <= hi[1] + 1; ++j)
+                for (int i = lo[0] - 1; i <= hi[0] + 1; ++i) {
+                    int p[3] = {i, j, k};
+                    long c = FI(fqb, i, j, k);
+                    for (int d = 0; d < 3; ++d) {
+                        long v = FI(&fm[d][b], i, j, k);
+                        double um = mac[d][v], up = mac[d][v + sm[d][d]];
+                        for (int n = 0; n < ncomp; ++n) {
+                            double w[5];
+                            for (int m = 0; m < 5; ++m) w[m] = q[c + n * qs + (m - 2) * st[d]];
+                            ppm_trace(w, limiter, bc[n].lo[d], bc[n].hi[d], p[d], dlo[d], dhi[d],
+                                      um, up, dtdx[d], LA(&Im[d], i, j, k, n),
+                                      LA(&Ip[d], i, j, k, n));
+                        }
+                    }
+                }
+
+        /* stage 1: normal states and upwinded edges on transversely grown faces */
+        la_t flo[3], fhi[3], fed[3];
+        for (int d = 0; d < 3; ++d) {
+            int l[3], h[3];
+            for (int e = 0; e < 3; ++e) {
+                l[e] = lo[e] - (e == d ? 0 : 1);
+                h[e] = hi[e] + 1;
+            }
+            la_alloc(&flo[d], l[0], l[1], l[2], h[0], h[1], h[2], ncomp);
+            la_alloc(&fhi[d], l[0], l[1], l[2], h[0], h[1], h[2], ncomp);
+            la_alloc(&fed[d], l[0], l[1], l[2], h[0], h[1], h[2], ncomp);
+            for (int k = l[2]; k <= h[2]; ++k)
+                for (int j = l[1]; j <= h[1]; ++j)
+                    for (int i = l[0]; i <= h[0]; ++i) {
+                        int p[3] = {i, j, k};
+                        int qq[3] = {i, j, k};
+                        qq[d] -= 1;
+                        int x = p[d];
+                        int gl[3] = {i, j, k}, gh[3] = {i, j, k};
+                        gl[d] = dlo[d] - 1;
+                        gh[d] = dhi[d] + 1;
+                        double u = mac[d][FI(&fm[d][b], i, j, k)];
+                        for (int n = 0; n < ncomp; ++n) {
+                            double lov = *LA(&Ip[d], qq[0], qq[1], qq[2], n);
+                            double hiv = *LA(&Im[d], i, j, k, n);
+                            if (use_forces_in_trans) {
+                                lov += hdt * fq[FI(ffb, qq[0], qq[1], qq[2]) + n * fs1];
+                                hiv += hdt * fq[FI(ffb, i, j, k) + n * fs1];
+                            }
+                            double glo = 0.0, ghi = 0.0;
+                            if (x == dlo[d]) glo = q_bc[FI(fqb, gl[0], gl[1], gl[2]) + n * qs];
+                            if (x == dhi[d] + 1) ghi = q_bc[FI(fqb, gh[0], gh[1], gh[2]) + n * qs];
+                            edge_bcs(x, dlo[d], dhi[d], bc[n].lo[d], bc[n].hi[d], glo, ghi,
+                                     is_velocity && n == d, &lov, &hiv);
+                            *LA(&flo[d], i, j, k, n) = lov;
+                            *LA(&fhi[d], i, j, k, n) = hiv;
+                            *LA(&fed[d], i, j, k, n) = upwind(lov, hiv, u);
+                        }
+                    }
+        }
+
+        /* stage 2: corner-coupled states crn[d][t]: faces normal to d,
+         * all components, corrected by the transverse derivative along t;
+         * grown by one in the third direction e */
+        la_t crn[3][3];
+        for (int d = 0; d < 3; ++d)
+            for (int t = 0; t < 3; ++t) {
+                crn[d][t].p = NULL;
+                if (t == d) continue;
+                int e = 3 - d - t;
+                int l[3], h[3];
+                for (int a = 0; a < 3; ++a) {
+                    l[a] = lo[a];
+                    h[a] = hi[a] + (a == d ? 1 : 0);
+                }
+                l[e] -= 1;
+                h[e] += 1;
+                la_alloc(&crn[d][t], l[0], l[1], l[2], h[0], h[1], h[2], ncomp);
+                for (int k = l[2]; k <= h[2]; ++k)
+                    for (int j = l[1]; j <= h[1]; ++j)
+                        for (int i = l[0]; i <= h[0]; ++i) {
+                            int p[3] = {i, j, k};
+                            int qq[3] = {i, j, k};
+                            qq[d] -= 1;
+                            int qt[3] = {qq[0], qq[1], qq[2]}, pt[3] = {i, j, k};
+                            qt[t] += 1;
+                            pt[t] += 1;
+                            const double* wt = mac[t];
+                            const fab_t* fmt = &fm[t][b];
+                            double wql = wt[FI(fmt, qq[0], qq[1], qq[2])];
+                            double wqh = wt[FI(fmt, qt[0], qt[1], qt[2])];
+                            double wpl = wt[FI(fmt, i, j, k)];
+                            double wph = wt[FI(fmt, pt[0], pt[1], pt[2])];
+                            int x = p[d];
+                            int gl[3] = {i, j, k}, gh[3] = {i, j, k};
+                            gl[d] = dlo[d] - 1;
+                            gh[d] = dhi[d] + 1;
+                            double u = mac[d][FI(&fm[d][b], i, j, k)];
+                            for (int n = 0; n < ncomp; ++n) {
+                                double lov, hiv;
+                                double eql = *LA(&fed[t], qq[0], qq[1], qq[2], n);
+                                double eqh = *LA(&fed[t], qt[0], qt[1], qt[2], n);
+                                double epl = *LA(&fed[t], i, j, k, n);
+                                double eph = *LA(&fed[t], pt[0], pt[1], pt[2], n);
+                                if (iconserv[n]) {
+                                    double ql = q[FI(fqb, qq[0], qq[1], qq[2]) + n * qs];
+                                    double qh = q[FI(fqb, i, j, k) + n * qs];
+                                    lov = *LA(&flo[d], i, j, k, n) - c3[t] * (wqh * eqh - wql * eql) +
+                                          c3[t] * ql * (wqh - wql);
+                                    hiv = *LA(&fhi[d], i, j, k, n) - c3[t] * (wph * eph - wpl * epl) +
+                                          c3[t] * qh * (wph - wpl);
+                                } else {
+                                    lov = *LA(&flo[d], i, j, k, n) - c6[t] * (wqh + wql) * (eqh - eql);
+                                    hiv = *LA(&fhi[d], i, j, k, n) - c6[t] * (wph + wpl) * (eph - epl);
+                                }
+                                double glo = 0.0, ghi = 0.0;
+                                if (x == dlo[d]) glo = q_bc[FI(fqb, gl[0], gl[1], gl[2]) + n * qs];
+                                if (x == dhi[d] + 1) ghi = q_bc[FI(fqb, gh[0], gh[1], gh[2]) + n * qs];
+                                edge_bcs(x, dlo[d], dhi[d], bc[n].lo[d], bc[n].hi[d], glo, ghi,
+                                         is_velocity && n == d, &lov, &hiv);
+                                *LA(&crn[d][t], i, j, k, n) = upwind(lov, hiv, u);
+                            }
+                        }
+            }
+
+        /* stage 3: final edge states and fluxes on the valid faces */
+        for (int d = 0; d < 3; ++d) {
+            int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+            const fab_t* fmd = &fm[d][b];
+            const fab_t* fm1 = &fm[t1][b];
+            const fab_t* fm2 = &fm[t2][b];
+            const fab_t* foo = &fo[d][b];
+            for (int k = lo[2]; k <= hi[2] + (d == 2); ++k)
+                for (int j = lo[1]; j <= hi[1] + (d == 1); ++j)
+                    for (int i = lo[0]; i <= hi[0] + (d == 0); ++i) {
+                        int p[3] = {i, j, k};
+                        int qq[3] = {i, j, k};
+                        qq[d] -= 1;
+                        int q1[3] = {qq[0], qq[1], qq[2]}, p1[3] = {i, j, k};
+                        q1[t1] += 1;
+                        p1[t1] += 1;
+                        int q2[3] = {qq[0], qq[1], qq[2]}, p2[3] = {i, j, k};
+                        q2[t2] += 1;
+                        p2[t2] += 1;
+                        double v1ql = mac[t1][FI(fm1, qq[0], qq[1], qq[2])];
+                        double v1qh = mac[t1][FI(fm1, q1[0], q1[1], q1[2])];
+                        double v1pl = mac[t1][FI(fm1, i, j, k)];
+                        double v1ph = mac[t1][FI(fm1, p1[0], p1[1], p1[2])];
+                        double v2ql = mac[t2][FI(fm2, qq[0], qq[1], qq[2])];
+                        double v2qh = mac[t2][FI(fm2, q2[0], q2[1], q2[2])];
+                        double v2pl = mac[t2][FI(fm2, i, j, k)];
+                        double v2ph = mac[t2][FI(fm2, p2[0], p2[1], p2[2])];
+                        double u = mac[d][FI(fmd, i, j, k)];
+                        int x = p[d];
+                        int gl[3] = {i, j, k}, gh[3] = {i, j, k};
+                        gl[d] = dlo[d] - 1;
+                        gh[d] = dhi[d] + 1;
+                        for (int n = 0; n < ncomp; ++n) {
+                            /* crn[t1][t2]: faces normal to t1 corrected along t2 */
+                            double a_ql = *LA(&crn[t1][t2], qq[0], qq[1], qq[2], n);
+                            double a_qh = *LA(&crn[t1][t2], q1[0], q1[1], q1[2], n);
+                            double a_pl = *LA(&crn[t1][t2], i, j, k, n);
+                            double a_ph = *LA(&crn[t1][t2], p1[0], p1[1], p1[2], n);
+                            double b_ql = *LA(&crn[t2][t1], qq[0], qq[1], qq[2], n);
+                            double b_qh = *LA(&crn[t2][t1], q2[0], q2[1], q2[2], n);
+                            double b_pl = *LA(&crn[t2][t1], i, j, k, n);
+                            double b_ph = *LA(&crn[t2][t1], p2[0], p2[1], p2[2], n);
+                            double stl, sth;
+                            if (iconserv[n]) {
+                                double ql = q[FI(fqb, qq[0], qq[1], qq[2]) + n * qs];
+                                double qh = q[FI(fqb, i, j, k) + n * qs];
+                                stl = *LA(&flo[d], i, j, k, n) - c2[t1] * (v1qh * a_qh - v1ql * a_ql) -
+                                      c2[t2] * (v2qh * b_qh - v2ql * b_ql) +
+                                      c2[t1] * ql * (v1qh - v1ql) + c2[t2] * ql * (v2qh - v2ql);
+                                sth = *LA(&fhi[d], i, j, k, n) - c2[t1] * (v1ph * a_ph - v1pl * a_pl) -
+                                      c2[t2] * (v2ph * b_ph - v2pl * b_pl) +
+                                      c2[t1] * qh * (v1ph - v1pl) + c2[t2] * qh * (v2ph - v2pl);
+                                if (divu) {
+                                    stl -= hdt * ql * divu[FI(fdb, qq[0], qq[1], qq[2])];
+                                    sth -= hdt * qh * divu[FI(fdb, i, j, k)];
+                                }
+                            } else {
+                                stl = *LA(&flo[d], i, j, k, n) - c4[t1] * (v1qh + v1ql) * (a_qh - a_ql) -
+                                      c4[t2] * (v2qh + v2ql) * (b_qh - b_ql);
+                                sth = *LA(&fhi[d], i, j, k, n) - c4[t1] * (v1ph + v1pl) * (a_ph - a_pl) -
+                                      c4[t2] * (v2ph + v2pl) * (b_ph - b_pl);
+                            }
+                            if (!use_forces_in_trans) {
+                                stl += hdt * fq[FI(ffb, qq[0], qq[1], qq[2]) + n * fs1];
+                                sth += hdt * fq[FI(ffb, i, j, k) + n * fs1];
+                            }
+                            double glo = 0.0, ghi = 0.0;
+                            if (x == dlo[d]) glo = q_bc[FI(fqb, gl[0], gl[1], gl[2]) + n * qs];
+                            if (x == dhi[d] + 1) ghi = q_bc[FI(fqb, gh[0], gh[1], gh[2]) + n * qs];
+                            edge_bcs(x, dlo[d], dhi[d], bc[n].lo[d], bc[n].hi[d], glo, ghi,
+                                     is_velocity && n == d, &stl, &sth);
+                            if (is_velocity && n == d && !allow_inflow_on_outflow) {
+                                int tl = bc[n].lo[d], th = bc[n].hi[d];
+                                if (x == dlo[d] && (tl == KO_BC_FOEXTRAP || tl == KO_BC_HOEXTRAP)) {
+                                    sth = fmin(sth, 0.0);
+                                    stl = sth;
+                                }
+                                if (x == dhi[d] + 1 && (th == KO_BC_FOEXTRAP || th == KO_BC_HOEXTRAP)) {
+                                    stl = fmax(stl, 0.0);
+                                    sth = stl;
+                                }
+                            }
+                            double e = upwind(stl, sth, u);
+                            long o = FI(foo, i, j, k) + n * FS(foo);
+                            face[d][o] = e;
+                            flux[d][o] = area_weighted ? (u * area[d]) * e : u * e;
+                        }
+                    }
+        }
+        for (int d = 0; d < 3; ++d) {
+            free(Im[d].p); free(Ip[d].p);
+            free(flo[d].p); free(fhi[d].p); free(fed[d].p);
+            for (int t = 0; t < 3; ++t) free(crn[d][t].p);
+        }
+    }
+    free(fq3); free(ff1); free(fdv);
+    for (int d = 0; d < 3; ++d) { free(fm[d]); free(fo[d]); }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* ComputeDivergence (a14; icns_advection.H:418-422)                   */
+/* ------------------------------------------------------------------ */
+void ko_compute_divergence(const ko_geom* g, double* div, const double* fx, const double* fy,
+                           const double* fz, int ncomp, double mult, int area_weighted)
+{
+    const int NB = g->nbox;
+    fab_t* fc = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t* ff[3];
+    make_fabs(g, KO_CELL, ncomp, 0, fc);
+    for (int d = 0; d < 3; ++d) {
+        ff[d] = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+        make_fabs(g, KO_XFACE + d, ncomp, 0, ff[d]);
+    }
+    const double dxi = 1.0 / g->dx[0], dyi = 1.0 / g->dx[1], dzi = 1.0 / g->dx[2];
+    const double vol = g->dx[0] * g->dx[1] * g->dx[2];
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const fab_t *c = &fc[b], *a = &ff[0][b], *bb = &ff[1][b], *e = &ff[2][b];
+        long sa = 1, sb = bb->n[0], se = (long)e->n[0] * e->n[1];
+        for (int n = 0; n < ncomp; ++n)
+            for (int k = bx[2]; k <= bx[5]; ++k)
+                for (int j = bx[1]; j <= bx[4]; ++j)
+                    for (int i = bx[0]; i <= bx[3]; ++i) {
+                        long ia = FI(a, i, j, k) + n * FS(a);
+                        long ib = FI(bb, i, j, k) + n * FS(bb);
+                        long ie = FI(e, i, j, k) + n * FS(e);
+                        double r;
+                        if (area_weighted)
+                            r = mult * ((fx[ia + sa] - fx[ia]) + (fy[ib + sb] - fy[ib]) +
+                                        (fz[ie + se] - fz[ie])) / vol;
+                        else
+                            r = mult * ((fx[ia + sa] - fx[ia]) * dxi + (fy[ib + sb] - fy[ib]) * dyi +
+                                        (fz[ie + se] - fz[ie]) * dzi);
+                        div[FI(c, i, j, k) + n * FS(c)] = r;
+                    }
+    }
+    free(fc);
+    for (int d = 0; d < 3; ++d) free(ff[d]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Geometric multigrid for div(beta grad phi) = rhs (MLMG + MLPoisson  */
+/* semantics as configured by MLMGOptions.cpp:59-110)                  */
+/* ------------------------------------------------------------------ */
+void ko_mlmg_default_opts(ko_mlmg_opts* o)
+{
+    o->maxiter = 200;
+    o->nu1 = 2;
+    o->nu2 = 2;
+    o->nuf = 8;
+    o->nub = 0;
+    o->bottom_maxiter = 200;
+    o->bottom_rtol = 2.220446049250313e-16 * 1.0e12;
+    o->bottom_atol = 2.220446049250313e-16 * 1.0e4;
+    o->semicoarsening = 0;
+    o->max_coarsening_level = 30;
+}
+
+#define MAXLEV 40
+typedef struct {
+    int n[3];
+    double h[3];
+    int r[3];   /* coarsening ratio from this level to the next */
+    long sy, sz, tot;
+    double cx, cy, cz, dg, idg;
+    double *cor, *res, *tmp;
+} lev_t;
+
+static inline long LI(const lev_t* L, int i, int j, int k)
+{
+    return (long)(i + 1) + L->sy * (j + 1) + L->sz * (k + 1);
+}
+
+static void lev_ghosts(const lev_t* L, double* a, const int* lobc, const int* hibc)
+{
+    const int n0 = L->n[0], n1 = L->n[1], n2 = L->n[2];
+    /* x faces */
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j) {
+            long l = LI(L, -1, j, k), r = LI(L, n0, j, k);
+            double vl = a[LI(L, 0, j, k)], vr = a[LI(L, n0 - 1, j, k)];
+            a[l] = (lobc[0] == KO_LO_PERIODIC) ? vr : (lobc[0] == KO_LO_NEUMANN ? vl : -vl);
+            a[r] = (hibc[0] == KO_LO_PERIODIC) ? vl : (hibc[0] == KO_LO_NEUMANN ? vr : -vr);
+        }
+    for (int k = 0; k < n2; ++k)
+        for (int i = 0; i < n0; ++i) {
+            long l = LI(L, i, -1, k), r = LI(L, i, n1, k);
+            double vl = a[LI(L, i, 0, k)], vr = a[LI(L, i, n1 - 1, k)];
+            a[l] = (lobc[1] == KO_LO_PERIODIC) ? vr : (lobc[1] == KO_LO_NEUMANN ? vl : -vl);
+            a[r] = (hibc[1] == KO_LO_PERIODIC) ? vl : (hibc[1] == KO_LO_NEUMANN ? vr : -vr);
+        }
+    for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n0; ++i) {
+            long l = LI(L, i, j, -1), r = LI(L, i, j, n2);
+            double vl = a[LI(L, i, j, 0)], vr = a[LI(L, i, j, n2 - 1)];
+            a[l] = (lobc[2] == KO_LO_PERIODIC) ? vr : (lobc[2] == KO_LO_NEUMANN ? vl : -vl);
+            a[r] = (hibc[2] == KO_LO_PERIODIC) ? vl : (hibc[2] == KO_LO_NEUMANN ? vr : -vr);
+        }
+}
+
+static inline double lap_off(const lev_t* L, const double* a, long c)
+{
+    return L->cx * (a[c - 1] + a[c + 1]) + L->cy * (a[c - L->sy] + a[c + L->sy]) +
+           L->cz * (a[c - L->sz] + a[c + L->sz]);
+}
+
+static void smooth(const lev_t* L, double* x, const double* b, int nu, const int* lobc,
+                   const int* hibc)
+{
+    for (int s = 0; s < nu; ++s)
+        for (int color = 0; color < 2; ++color) {
+            lev_ghosts(L, x, lobc, hibc);
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < L->n[2]; ++k)
+                for (int j = 0; j < L->n[1]; ++j)
+                    for (int i = (j + k + color) & 1; i < L->n[0]; i += 2) {
+                        long c = LI(L, i, j, k);
+                        x[c] = (lap_off(L, x, c) - b[c]) * L->idg;
+                    }
+        }
+}
+
+/* r = b - A x (x ghosts filled here); returns |r|_inf */
+static double residual(const lev_t* L, double* x, const double* b, double* r, const int* lobc,
+                       const int* hibc)
+{
+    lev_ghosts(L, x, lobc, hibc);
+    double mx = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long c = LI(L, i, j, k);
+                double v = b[c] - (lap_off(L, x, c) - L->dg * x[c]);
+                r[c] = v;
+                mx = fmax(mx, fabs(v));
+            }
+    return mx;
+}
+
+static void apply_op(const lev_t* L, double* x, double* y, const int* lobc, const int* hibc)
+{
+    lev_ghosts(L, x, lobc, hibc);
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long c = LI(L, i, j, k);
+                y[c] = lap_off(L, x, c) - L->dg * x[c];
+            }
+}
+
+static double dotp(const lev_t* L, const double* a, const double* b)
+{
+    double s = 0.0;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long c = LI(L, i, j, k);
+                s += a[c] * b[c];
+            }
+    return s;
+}
+static double norminf(const lev_t* L, const double* a)
+{
+    double m = 0.0;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) m = fmax(m, fabs(a[LI(L, i, j, k)]));
+    return m;
+}
+static void sub_mean(const lev_t* L, double* a)
+{
+    double s = 0.0;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) s += a[LI(L, i, j, k)];
+    double m = s / ((double)L->n[0] * L->n[1] * L->n[2]);
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) a[LI(L, i, j, k)] -= m;
+}
+
+/* unpreconditioned BiCGStab on the bottom level; returns 0 on convergence */
+static int bicgstab(const lev_t* L, double* x, const double* b, double rtol, double atol,
+                    int maxiter, const int* lobc, const int* hibc, int* iters)
+{
+    long T = L->tot;
+    double* r = (double*)calloc((size_t)T, sizeof(double));
+    double* rh = (double*)calloc((size_t)T, sizeof(double));
+    double* p = (double*)calloc((size_t)T, sizeof(double));
+    double* v = (double*)calloc((size_t)T, sizeof(double));
+    double* s = (double*)calloc((size_t)T, sizeof(double));
+    double* t = (double*)calloc((size_t)T, sizeof(double));
+    for (long c = 0; c < T; ++c) { x[c] = 0.0; r[c] = b[c]; rh[c] = b[c]; }
+    double rnorm0 = norminf(L, r);
+    double tol = fmax(rtol * rnorm0, atol);
+    int ret = 1, it = 0;
+    *iters = 0;
+    if (rnorm0 == 0.0 || rnorm0 <= tol) { ret = 0; goto done; }
+    double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+    for (it = 1; it <= maxiter; ++it) {
+        double rho = dotp(L, rh, r);
+        if (rho == 0.0) { ret = 2; break; }
+        if (it == 1) {
+            for (long c = 0; c < T; ++c) p[c] = r[c];
+        } else {
+            double beta = (rho / rho_prev) * (alpha / omega);
+            for (long c = 0; c < T; ++c) p[c] = r[c] + beta * (p[c] - omega * v[c]);
+        }
+        apply_op(L, p, v, lobc, hibc);
+        double rv = dotp(L, rh, v);
+        if (rv == 0.0) { ret = 2; break; }
+        alpha = rho / rv;
+        for (long c = 0; c < T; ++c) s[c] = r[c] - alpha * v[c];
+        if (norminf(L, s) <= tol) {
+            for (long c = 0; c < T; ++c) x[c] += alpha * p[c];
+            ret = 0;
+            break;
+        }
+        apply_op(L, s, t, lobc, hibc);
+        double tt = dotp(L, t, t);
+        if (tt == 0.0) { ret = 2; break; }
+        omega = dotp(L, t, s) / tt;
+        for (long c = 0; c < T; ++c) {
+            x[c] += alpha * p[c] + omega * s[c];
+            r[c] = s[c] - omega * t[c];
+        }
+        if (norminf(L, r) <= tol) { ret = 0; break; }
+        if (omega == 0.0) { ret = 2; break; }
+        rho_prev = rho;
+    }
+    if (it > maxiter) it = maxiter;
+done:
+    *iters = it;
+    free(r); free(rh); free(p); free(v); free(s); free(t);
+    return ret;
+}
+
+static void restrict_avg(const lev_t* F, const lev_t* C, const double* f, double* c)
+{
+    const int r0 = F->r[0], r1 = F->r[1], r2 = F->r[2];
+    const double w = 1.0 / (double)(r0 * r1 * r2);
+#pragma omp parallel for schedule(static)
+    for (int K = 0; K < C->n[2]; ++K)
+        for (int J = 0; J < C->n[1]; ++J)
+            for (int I = 0; I < C->n[0]; ++I) {
+                double s = 0.0;
+                for (int dk = 0; dk < r2; ++dk)
+                    for (int dj = 0; dj < r1; ++dj)
+                        for (int di = 0; di < r0; ++di)
+                            s += f[LI(F, I * r0 + di, J * r1 + dj, K * r2 + dk)];
+                c[LI(C, I, J, K)] = s * w;
+            }
+}
+
+static void prolong_add(const lev_t* F, const lev_t* C, double* f, const double* c)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < F->n[2]; ++k)
+        for (int j = 0; j < F->n[1]; ++j)
+            for (int i = 0; i < F->n[0]; ++i)
+                f[LI(F, i, j, k)] += c[LI(C, i / F->r[0], j / F->r[1], k / F->r[2])];
+}
+
+typedef struct {
+    int nlev;
+    lev_t L[MAXLEV];
+    const int *lobc, *hibc;
+    const ko_mlmg_opts* o;
+    int singular;
+    int bottom_iters, bottom_fail;
+} mg_t;
+
+static void vcycle(mg_t* M, int l)
+{
+    lev_t* L = &M->L[l];
+    for (long c = 0; c < L->tot; ++c) L->cor[c] = 0.0;
+    if (l == M->nlev - 1) {
+        if (M->singular) sub_mean(L, L->res);
+        int it = 0;
+        int ret = bicgstab(L, L->cor, L->res, M->o->bottom_rtol, M->o->bottom_atol,
+                           M->o->bottom_maxiter, M->lobc, M->hibc, &it);
+        M->bottom_iters += it;
+        if (ret != 0) M->bottom_fail += 1;
+        smooth(L, L->cor, L->res, ret == 0 ? M->o->nub : M->o->nuf, M->lobc, M->hibc);
+        return;
+    }
+    smooth(L, L->cor, L->res, M->o->nu1, M->lobc, M->hibc);
+    residual(L, L->cor, L->res, L->tmp, M->lobc, M->hibc);
+    restrict_avg(L, &M->L[l + 1], L->tmp, M->L[l + 1].res);
+    vcycle(M, l + 1);
+    prolong_add(L, &M->L[l + 1], L->cor, M->L[l + 1].cor);
+    smooth(L, L->cor, L->res, M->o->nu2, M->lobc, M->hibc);
+}
+
+static void build_levels(mg_t* M, const int* n, const double* dx, double beta)
+{
+    const ko_mlmg_opts* o = M->o;
+    int nl = 0;
+    int cn[3] = {n[0], n[1], n[2]};
+    double ch[3] = {dx[0], dx[1], dx[2]};
+    for (;;) {
+        lev_t* L = &M->L[nl];
+        for (int d = 0; d < 3; ++d) { L->n[d] = cn[d]; L->h[d] = ch[d]; L->r[d] = 1; }
+        L->sy = cn[0] + 2;
+        L->sz = (long)(cn[0] + 2) * (cn[1] + 2);
+        L->tot = L->sz * (cn[2] + 2);
+        L->cx = beta / (ch[0] * ch[0]);
+        L->cy = beta / (ch[1] * ch[1]);
+        L->cz = beta / (ch[2] * ch[2]);
+        L->dg = 2.0 * (L->cx + L->cy + L->cz);
+        L->idg = 1.0 / L->dg;
+        ++nl;
+        if (nl >= MAXLEV || nl > o->max_coarsening_level) break;
+        int can[3], any = 0;
+        for (int d = 0; d < 3; ++d) can[d] = (cn[d] % 2 == 0 && cn[d] / 2 >= 2);
+        if (!o->semicoarsening) {
+            if (!(can[0] && can[1] && can[2])) break;
+            for (int d = 0; d < 3; ++d) L->r[d] = 2;
+            any = 1;
+        } else {
+            double hmin = 1e300;
+            for (int d = 0; d < 3; ++d) if (can[d] && ch[d] < hmin) hmin = ch[d];
+            for (int d = 0; d < 3; ++d)
+                if (can[d] && ch[d] <= 1.5 * hmin) { L->r[d] = 2; any = 1; }
+            if (!any) break;
+        }
+        for (int d = 0; d < 3; ++d) { cn[d] /= L->r[d]; ch[d] *= L->r[d]; }
+    }
+    M->nlev = nl;
+    for (int l = 0; l < nl; ++l) {
+        lev_t* L = &M->L[l];
+        L->cor = (double*)calloc((size_t)L->tot, sizeof(double));
+        L->res = (double*)calloc((size_t)L->tot, sizeof(double));
+        L->tmp = (double*)calloc((size_t)L->tot, sizeof(double));
+    }
+}
+
+static void free_levels(mg_t* M)
+{
+    for (int l = 0; l < M->nlev; ++l) {
+        free(M->L[l].cor);
+        free(M->L[l].res);
+        free(M->L[l].tmp);
+    }
+}
+
+/* phi and rhs: level-0 arrays with one ghost layer (LI indexing) */
+static int mg_solve(mg_t* M, double* phi, double* rhs, double rtol, double atol, ko_mlmg_info* info)
+{
+    lev_t* L0 = &M->L[0];
+    for (long c = 0; c < L0->tot; ++c) phi[c] = 0.0;
+    if (M->singular) sub_mean(L0, rhs);
+    double rhsnorm = norminf(L0, rhs);
+    for (long c = 0; c < L0->tot; ++c) L0->res[c] = rhs[c];
+    double r = rhsnorm;
+    double target = fmax(atol, rtol * rhsnorm);
+    info->r0 = r;
+    info->iters = 0;
+    info->converged = (r <= target);
+    info->nlevels = M->nlev;
+    M->bottom_iters = 0;
+    M->bottom_fail = 0;
+    if (!info->converged) {
+        for (int it = 1; it <= M->o->maxiter; ++it) {
+            vcycle(M, 0);
+            for (long c = 0; c < L0->tot; ++c) phi[c] += L0->cor[c];
+            r = residual(L0, phi, rhs, L0->res, M->lobc, M->hibc);
+            info->iters = it;
+            if (r <= target) { info->converged = 1; break; }
+        }
+    }
+    lev_ghosts(L0, phi, M->lobc, M->hibc);
+    info->rfinal = r;
+    info->bottom_iters = M->bottom_iters;
+    info->bottom_fail = M->bottom_fail;
+    return info->converged ? 0 : 1;
+}
+
+int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int* hibc,
+                     double beta, const double* rhs_in, double* phi_out, double rtol, double atol,
+                     const ko_mlmg_opts* opts, ko_mlmg_info* info)
+{
+    mg_t* M = (mg_t*)calloc(1, sizeof(mg_t));
+    M->o = opts;
+    M->lobc = lobc;
+    M->hibc = hibc;
+    M->singular = 1;
+    for (int d = 0; d < 3; ++d)
+        if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) M->singular = 0;
+    build_levels(M, n, dx, beta);
+    lev_t* L0 = &M->L[0];
+    double* rhs = (double*)calloc((size_t)L0->tot, sizeof(double));
+    double* phi = (double*)calloc((size_t)L0->tot, sizeof(double));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                rhs[LI(L0, i, j, k)] = rhs_in[i + (long)n[0] * (j + (long)n[1] * k)];
+    int ret = mg_solve(M, phi, rhs, rtol, atol, info);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                phi_out[i + (long)n[0] * (j + (long)n[1] * k)] = phi[LI(L0, i, j, k)];
+    free(rhs);
+    free(phi);
+    free_levels(M);
+    free(M);
+    return ret;
+}
+
+/* ------------------------------------------------------------------ */
+/* MacProjector::project (a8/a9; icns_advection.cpp:206-375)           */
+/* ------------------------------------------------------------------ */
+int ko_mac_project(const ko_geom* g, double* umac, double* vmac, double* wmac,
+                   const int* lobc, const int* hibc, double beta, double rtol, double atol,
+                   const ko_mlmg_opts* opts, ko_mlmg_info* info, double* phi_out)
+{
+    const int NB = g->nbox;
+    int n[3];
+    for (int d = 0; d < 3; ++d) n[d] = g->dhi[d] - g->dlo[d] + 1;
+    mg_t* M = (mg_t*)calloc(1, sizeof(mg_t));
+    M->o = opts;
+    M->lobc = lobc;
+    M->hibc = hibc;
+    M->singular = 1;
+    for (int d = 0; d < 3; ++d)
+        if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) M->singular = 0;
+    build_levels(M, n, g->dx, beta);
+    lev_t* L0 = &M->L[0];
+    double* rhs = (double*)calloc((size_t)L0->tot, sizeof(double));
+    double* phi = (double*)calloc((size_t)L0->tot, sizeof(double));
+    double* mac[3] = {umac, vmac, wmac};
+    fab_t* fm[3];
+    for (int d = 0; d < 3; ++d) {
+        fm[d] = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+        make_fabs(g, KO_XFACE + d, 1, 1, fm[d]);
+    }
+    const double dxi = 1.0 / g->dx[0], dyi = 1.0 / g->dx[1], dzi = 1.0 / g->dx[2];
+    /* rhs = div u_mac over valid cells */
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const fab_t *a = &fm[0][b], *bb = &fm[1][b], *e = &fm[2][b];
+        long sb = bb->n[0], se = (long)e->n[0] * e->n[1];
+        for (int k = bx[2]; k <= bx[5]; ++k)
+            for (int j = bx[1]; j <= bx[4]; ++j)
+                for (int i = bx[0]; i <= bx[3]; ++i) {
+                    long ia = FI(a, i, j, k), ib = FI(bb, i, j, k), ie = FI(e, i, j, k);
+                    rhs[LI(L0, i - g->dlo[0], j - g->dlo[1], k - g->dlo[2])] =
+                        (umac[ia + 1] - umac[ia]) * dxi + (vmac[ib + sb] - vmac[ib]) * dyi +
+                        (wmac[ie + se] - wmac[ie]) * dzi;
+                }
+    }
+    int ret = mg_solve(M, phi, rhs, rtol, atol, info);
+    /* u_mac -= beta grad phi on every valid face (ghosts of phi hold the BC) */
+    const double bd[3] = {beta * dxi, beta * dyi, beta * dzi};
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        for (int d = 0; d < 3; ++d) {
+            const fab_t* fo = &fm[d][b];
+            long sd = (d == 0) ? 1 : (d == 1 ? L0->sy : L0->sz);
+            for (int k = bx[2]; k <= bx[5] + (d == 2); ++k)
+                for (int j = bx[1]; j <= bx[4] + (d == 1); ++j)
+                    for (int i = bx[0]; i <= bx[3] + (d == 0); ++i) {
+                        long c = LI(L0, i - g->dlo[0], j - g->dlo[1], k - g->dlo[2]);
+                        long o = FI(fo, i, j, k);
+                        double pr = phi[c], pl = phi[c - sd];
+                        mac[d][o] = mac[d][o] - bd[d] * (pr - pl);
+                    }
+        }
+    }
+    if (phi_out) {
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                    phi_out[i + (long)n[0] * (j + (long)n[1] * k)] = phi[LI(L0, i, j, k)];
+    }
+    for (int d = 0; d < 3; ++d) free(fm[d]);
+    free(rhs);
+    free(phi);
+    free_levels(M);
+    free(M);
+    return ret;
+}
+
+/* ------------------------------------------------------------------ */
+/* CFL (a2; incflo_compute_dt.cpp:69-91) and predictor (a16;           */
+/* CompRHSOps.H:93-117)                                                */
+/* ------------------------------------------------------------------ */
+double ko_cfl_max(const ko_geom* g, const double* vel, int ng)
+{
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    make_fabs(g, KO_CELL, 3, ng, f);
+    const double dxi[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
+    double mx = -1.0;
+#pragma omp parallel for schedule(dynamic) reduction(max : mx)
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const fab_t* fb = &f[b];
+        for (int k = bx[2]; k <= bx[5]; ++k)
+            for (int j = bx[1]; j <= bx[4]; ++j)
+                for (int i = bx[0]; i <= bx[3]; ++i) {
+                    long c = FI(fb, i, j, k);
+                    double a = fabs(vel[c]) * dxi[0];
+                    double bb = fabs(vel[c + FS(fb)]) * dxi[1];
+                    double e = fabs(vel[c + 2 * FS(fb)]) * dxi[2];
+                    mx = fmax(mx, fmax(a, fmax(bb, e)));
+                }
+    }
+    free(f);
+    return mx;
+}
+
+void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,
+                         int ng_old, const double* rho_old, int ng_ro, const double* rho_new,
+                         int ng_rn, const double* conv, const double* src, const double* diff,
+                         double kappa, double dt, int ncomp)
+{
+    const int NB = g->nbox;
+    fab_t *fn = malloc(sizeof(fab_t) * NB), *fo = malloc(sizeof(fab_t) * NB),
+          *fro = malloc(sizeof(fab_t) * NB), *frn = malloc(sizeof(fab_t) * NB),
+          *fc = malloc(sizeof(fab_t) * NB);
+    make_fabs(g, KO_CELL, ncomp, ng_new, fn);
+    make_fabs(g, KO_CELL, ncomp, ng_old, fo);
+    make_fabs(g, KO_CELL, 1, ng_ro, fro);
+    make_fabs(g, KO_CELL, 1, ng_rn, frn);
+    make_fabs(g, KO_CELL, ncomp, 0, fc);
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        for (int n = 0; n < ncomp; ++n)
+            for (int k = bx[2]; k <= bx[5]; ++k)
+                for (int j = bx[1]; j <= bx[4]; ++j)
+                    for (int i = bx[0]; i <= bx[3]; ++i) {
+                        long cn = FI(&fn[b], i, j, k) + n * FS(&fn[b]);
+                        long co = FI(&fo[b], i, j, k) + n * FS(&fo[b]);
+                        long cc = FI(&fc[b], i, j, k) + n * FS(&fc[b]);
+                        double ro = rho_old[FI(&fro[b], i, j, k)];
+                        double rn = rho_new[FI(&frn[b], i, j, k)];
+                        double df = diff ? diff[cc] : 0.0;
+                        double t = ro * vel_old[co] + dt * (conv[cc] + src[cc] + kappa * df);
+                        vel_new[cn] = t / rn;
+                    }
+    }
+    free(fn); free(fo); free(fro); free(frn); free(fc);
+}

@@ -1,0 +1,326 @@
+"""ctypes binding of the CPU FP64 oracle (liboracle.so, built from
+ksgf_oracle.c by the Makefile beside this file).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the bench's
+cpu_baseline leg as the checker and the CPU baseline.  The product library
+(libksgf.so) never loads it.
+
+Also provides ``advect_project_step``: the Godunov branch of
+incflo::ApplyPredictor (incflo_advance.cpp:186-438) restricted to the
+advect + MAC-project part (sources, ExtrapVelToFaces, MAC projection, face
+ghost fill, momentum fluxes, convective term, predictor update), evaluated
+with the oracle functions on numpy MultiFabs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+CELL, XFACE, YFACE, ZFACE = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class KoGeom(C.Structure):
+    _fields_ = [("nbox", C.c_int), ("boxes", _ip), ("dlo", C.c_int * 3), ("dhi", C.c_int * 3),
+                ("dx", C.c_double * 3), ("periodic", C.c_int * 3)]
+
+
+class KoBC(C.Structure):
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3)]
+
+
+class KoOpts(C.Structure):
+    _fields_ = [("maxiter", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int), ("nuf", C.c_int), ("nub", C.c_int),
+                ("bottom_maxiter", C.c_int), ("bottom_rtol", C.c_double), ("bottom_atol", C.c_double),
+                ("semicoarsening", C.c_int), ("max_coarsening_level", C.c_int)]
+
+
+class KoInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("r0", C.c_double), ("rfinal", C.c_double), ("converged", C.c_int),
+                ("nlevels", C.c_int), ("bottom_iters", C.c_int), ("bottom_fail", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(
+            os.path.getmtime(os.path.join(HERE, f)) for f in ("ksgf_oracle.c", "ksgf_oracle.h")):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.ko_mf_size.restype = C.c_long
+        L.ko_mf_size.argtypes = [C.POINTER(KoGeom), C.c_int, C.c_int, C.c_int]
+        L.ko_fill_boundary.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.c_int, C.c_int]
+        L.ko_fill_physbc.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.c_int, C.c_int, C.POINTER(KoBC), _dp]
+        L.ko_ppm_predict_state.argtypes = [C.POINTER(KoGeom), C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int,
+                                           C.POINTER(KoBC), C.c_double, C.c_int, _dp, _dp]
+        L.ko_extrap_vel_to_faces.argtypes = [C.POINTER(KoGeom), _dp, _dp, _dp, _dp, _dp, C.POINTER(KoBC), C.c_double,
+                                             C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ko_compute_fluxes_from_state.argtypes = [C.POINTER(KoGeom), C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                                   C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.POINTER(KoBC), _ip,
+                                                   C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ko_compute_divergence.argtypes = [C.POINTER(KoGeom), _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int]
+        L.ko_mlmg_default_opts.argtypes = [C.POINTER(KoOpts)]
+        L.ko_mac_project.argtypes = [C.POINTER(KoGeom), _dp, _dp, _dp, _ip, _ip, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(KoOpts), C.POINTER(KoInfo), _dp]
+        L.ko_poisson_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, _dp, C.c_double, C.c_double,
+                                       C.POINTER(KoOpts), C.POINTER(KoInfo)]
+        L.ko_cfl_max.restype = C.c_double
+        L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
+        L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
+                                          _dp, _dp, _dp, C.c_double, C.c_double, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Geom:
+    """Keeps the ctypes geometry and the box array alive together."""
+
+    def __init__(self, ba):
+        self.ba = ba
+        self.boxes = np.ascontiguousarray(ba.boxes, dtype=np.int32)
+        g = KoGeom()
+        g.nbox = ba.nbox
+        g.boxes = self.boxes.ctypes.data_as(_ip)
+        for d in range(3):
+            g.dlo[d] = ba.geom.dlo[d]
+            g.dhi[d] = ba.geom.dhi[d]
+            g.dx[d] = ba.geom.dx[d]
+            g.periodic[d] = int(ba.geom.periodic[d])
+        self.g = g
+
+    @property
+    def ref(self):
+        return C.byref(self.g)
+
+
+def bc_structs(bcrec):
+    bcrec = np.asarray(bcrec, dtype=np.int32).reshape(-1, 6)
+    arr = (KoBC * len(bcrec))()
+    for n, row in enumerate(bcrec):
+        for d in range(3):
+            arr[n].lo[d] = int(row[d])
+            arr[n].hi[d] = int(row[3 + d])
+    return arr
+
+
+def default_opts(**kw):
+    o = KoOpts()
+    lib().ko_mlmg_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# --------------------------------------------------------------------------
+# thin wrappers
+# --------------------------------------------------------------------------
+def fill_boundary(G, mf, itype, ncomp, ngrow):
+    lib().ko_fill_boundary(G.ref, _p(mf), itype, ncomp, ngrow)
+
+
+def fill_physbc(G, mf, itype, ncomp, ngrow, bcrec, bcval):
+    bcval = np.ascontiguousarray(bcval, dtype=np.float64)
+    lib().ko_fill_physbc(G.ref, _p(mf), itype, ncomp, ngrow, bc_structs(bcrec), _p(bcval))
+
+
+def fill_ghosts(G, mf, itype, ncomp, ngrow, bcrec, bcval):
+    fill_boundary(G, mf, itype, ncomp, ngrow)
+    fill_physbc(G, mf, itype, ncomp, ngrow, bcrec, bcval)
+
+
+def ppm_predict_state(G, d, s, ncomp_s, ng_s, comp, vel_edge, ng_v, bcrec1, dt, limiter):
+    n = G.ba.mf_size(CELL, 1, 0)
+    Im = np.zeros(n)
+    Ip = np.zeros(n)
+    lib().ko_ppm_predict_state(G.ref, d, _p(s), ncomp_s, ng_s, comp, _p(vel_edge), ng_v, bc_structs(bcrec1), dt,
+                               limiter, _p(Im), _p(Ip))
+    return Im, Ip
+
+
+def extrap_vel_to_faces(G, vel, f, bcrec, dt, use_ppm=1, use_forces_in_trans=0, limiter=1, allow_inflow=0):
+    ba = G.ba
+    mac = [np.zeros(ba.mf_size(XFACE + d, 1, 1)) for d in range(3)]
+    rc = lib().ko_extrap_vel_to_faces(G.ref, _p(vel), _p(f), _p(mac[0]), _p(mac[1]), _p(mac[2]), bc_structs(bcrec),
+                                      dt, use_ppm, use_forces_in_trans, limiter, allow_inflow)
+    if rc != 0:
+        raise RuntimeError(f"ko_extrap_vel_to_faces: {rc}")
+    return mac
+
+
+def compute_fluxes(G, ncomp, q, q_bc, mac, divu, fq, dt, bcrec, iconserv, use_ppm=1, use_forces_in_trans=0,
+                   is_velocity=1, area_weighted=0, limiter=1, allow_inflow=0, known_edge_state=0, faces=None):
+    ba = G.ba
+    flux = [np.zeros(ba.mf_size(XFACE + d, ncomp, 0)) for d in range(3)]
+    face = faces if faces is not None else [np.zeros(ba.mf_size(XFACE + d, ncomp, 0)) for d in range(3)]
+    ic = np.ascontiguousarray(iconserv, dtype=np.int32)
+    rc = lib().ko_compute_fluxes_from_state(G.ref, ncomp, _p(q), _p(q_bc), _p(flux[0]), _p(flux[1]), _p(flux[2]),
+                                            _p(face[0]), _p(face[1]), _p(face[2]), known_edge_state, _p(mac[0]),
+                                            _p(mac[1]), _p(mac[2]), _p(divu), _p(fq), dt, bc_structs(bcrec),
+                                            ic.ctypes.data_as(_ip), use_ppm, use_forces_in_trans, is_velocity,
+                                            area_weighted, limiter, allow_inflow)
+    if rc != 0:
+        raise RuntimeError(f"ko_compute_fluxes_from_state: {rc}")
+    return flux, face
+
+
+def compute_divergence(G, flux, ncomp, mult=-1.0, area_weighted=0):
+    div = np.zeros(G.ba.mf_size(CELL, ncomp, 0))
+    lib().ko_compute_divergence(G.ref, _p(div), _p(flux[0]), _p(flux[1]), _p(flux[2]), ncomp, mult, area_weighted)
+    return div
+
+
+def mac_project(G, mac, lobc, hibc, beta, rtol, atol, opts=None, want_phi=False):
+    opts = opts or default_opts()
+    info = KoInfo()
+    lo = np.ascontiguousarray(lobc, dtype=np.int32)
+    hi = np.ascontiguousarray(hibc, dtype=np.int32)
+    phi = np.zeros(int(np.prod(G.ba.geom.n_cell))) if want_phi else None
+    lib().ko_mac_project(G.ref, _p(mac[0]), _p(mac[1]), _p(mac[2]), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip),
+                         beta, rtol, atol, C.byref(opts), C.byref(info), _p(phi))
+    return info.as_dict(), phi
+
+
+def poisson_solve(n, dx, lobc, hibc, beta, rhs, rtol, atol, opts=None):
+    opts = opts or default_opts()
+    info = KoInfo()
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    dd = np.ascontiguousarray(dx, dtype=np.float64)
+    lo = np.ascontiguousarray(lobc, dtype=np.int32)
+    hi = np.ascontiguousarray(hibc, dtype=np.int32)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64).ravel()
+    phi = np.zeros_like(rhs)
+    lib().ko_poisson_solve(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip), beta,
+                           _p(rhs), _p(phi), rtol, atol, C.byref(opts), C.byref(info))
+    return phi, info.as_dict()
+
+
+def cfl_max(G, vel, ng):
+    return lib().ko_cfl_max(G.ref, _p(vel), ng)
+
+
+def predictor_update(G, vel_new, ng_new, vel_old, ng_old, rho_old, ng_ro, rho_new, ng_rn, conv, src, diff, kappa, dt,
+                     ncomp=3):
+    lib().ko_predictor_update(G.ref, _p(vel_new), ng_new, _p(vel_old), ng_old, _p(rho_old), ng_ro, _p(rho_new), ng_rn,
+                              _p(conv), _p(src), _p(diff), kappa, dt, ncomp)
+
+
+# --------------------------------------------------------------------------
+# one advect + MAC-project step (Godunov branch of ApplyPredictor)
+# --------------------------------------------------------------------------
+def _cellwise(ba, flat, ncomp, ngrow, fn):
+    """apply fn(view) to every FAB's (ncomp, nz, ny, nx) view in place"""
+    for b in range(ba.nbox):
+        fn(ba.fab_view(flat, b, CELL, ncomp, ngrow))
+
+
+def make_state(case):
+    """velocity (ng 3), density (ng 3), gp (ng 0) MultiFabs with ghosts filled"""
+    ba = case.ba
+    G = Geom(ba)
+    vel = ba.scatter(case.vel, CELL, 3)
+    fill_ghosts(G, vel, CELL, 3, 3, case.bcrec, case.bcval)
+    rho = ba.scatter(case.rho, CELL, 3)
+    rbc = np.zeros((1, 6), dtype=np.int32)
+    for d in range(3):
+        if not case.geom.periodic[d]:
+            rbc[0, d] = rbc[0, 3 + d] = 2  # foextrap for density
+    fill_ghosts(G, rho, CELL, 1, 3, rbc, np.zeros((6, 1)))
+    gp = ba.scatter(case.gp, CELL, 0)
+    return G, vel, rho, gp
+
+
+def case_opts(case):
+    kw = {}
+    for k, v in case.mlmg.items():
+        kw["semicoarsening" if k == "do_semicoarsening" else k] = v
+    return default_opts(**kw)
+
+
+def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1):
+    """One step; returns (vel_new (ng 3, ghosts filled), diagnostics dict)."""
+    ba = case.ba
+    opts = opts or case_opts(case)
+    if dt is None:
+        c = cfl_max(G, vel, 3)
+        dt = case.cfl / c
+    # sources: src = -gp/rho (velocity form), ghost 1 (foextrap-like fill)
+    src = np.zeros(ba.mf_size(CELL, 3, 1))
+    for b in range(ba.nbox):
+        s = ba.fab_view(src, b, CELL, 3, 1)
+        g_ = ba.fab_view(gp, b, CELL, 3, 0)
+        r_ = ba.fab_view(rho, b, CELL, 1, 3)[:, 2:-2, 2:-2, 2:-2]
+        rv = r_[0, 1:-1, 1:-1, 1:-1]
+        s[:, 1:-1, 1:-1, 1:-1] = -(1.0 * g_) * (1.0 / rv)
+    sbc = np.array(case.bcrec)
+    sbc[sbc != 0] = 2
+    fill_ghosts(G, src, CELL, 3, 1, sbc, np.zeros((6, 3)))
+    # face velocities
+    mac = extrap_vel_to_faces(G, vel, src, case.bcrec, dt, 1, 0, limiter, 0)
+    # set_inflow_velocity on ext_dir normal faces
+    for d in range(3):
+        if case.geom.periodic[d]:
+            continue
+        for side in (0, 1):
+            if case.bcrec[d, 3 * side + d] != 3:
+                continue
+            val = case.bcval[3 * side + d, d]
+            for b in range(ba.nbox):
+                v = ba.fab_view(mac[d], b, XFACE + d, 1, 1)
+                lo, hi = ba.boxes[b, d], ba.boxes[b, 3 + d] + 1
+                x = 0 if side == 0 else case.geom.dhi[d] + 1
+                if lo <= x <= hi:
+                    idx = [slice(1, -1)] * 3
+                    idx[2 - d] = x - lo + 1
+                    v[(0, *idx)] = val
+    info, _ = mac_project(G, mac, case.lobc, case.hibc, 1.0 / case.rho0, case.mg_rtol, case.mg_atol, opts)
+    for d in range(3):
+        bc1 = case.bcrec[d:d + 1]
+        fill_boundary(G, mac[d], XFACE + d, 1, 1)
+        fill_physbc(G, mac[d], XFACE + d, 1, 1, bc1, np.ascontiguousarray(case.bcval[:, d:d + 1]))
+    # momentum and forces
+    q = vel.copy()
+    fq = src.copy()
+    for b in range(ba.nbox):
+        r3 = ba.fab_view(rho, b, CELL, 1, 3)
+        ba.fab_view(q, b, CELL, 3, 3)[...] *= r3
+        ba.fab_view(fq, b, CELL, 3, 1)[...] *= r3[:, 2:-2, 2:-2, 2:-2]
+    q_nph = q.copy()
+    flux, face = compute_fluxes(G, 3, q, q_nph, mac, None, fq, dt, case.bcrec, [1, 1, 1], 1, 0, 1, 0, limiter, 0)
+    conv = compute_divergence(G, flux, 3, -1.0, 0)
+    # src at n+1/2 in momentum form: -gp (constant density)
+    src_nph = np.zeros(ba.mf_size(CELL, 3, 0))
+    for b in range(ba.nbox):
+        g_ = ba.fab_view(gp, b, CELL, 3, 0)
+        r_ = ba.fab_view(rho, b, CELL, 1, 3)[:, 3:-3, 3:-3, 3:-3]
+        ba.fab_view(src_nph, b, CELL, 3, 0)[...] = (-(1.0 * g_) * (1.0 / r_)) * r_
+    vel_new = np.zeros_like(vel)
+    predictor_update(G, vel_new, 3, vel, 3, rho, 3, rho, 3, conv, src_nph, None, 0.0, dt)
+    fill_ghosts(G, vel_new, CELL, 3, 3, case.bcrec, case.bcval)
+    return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face}
