@@ -225,6 +225,14 @@ int ksgf_scale_by_rho(const ksgf_boxarray* ba, double* out, int ng_out, const do
                       int ng_a, const double* rho, int ng_rho, int ncomp, int ngrow,
                       int divide, ksgf_stream_t stream);
 
+/* Live timing of the finest-level smoother launches (bench instrumentation;
+ * no counterpart in the reference): reset starts recording HIP events around
+ * each launch on its stream; read synchronises on them and returns the
+ * summed duration, launch count, algorithmic bytes and kernel name, then
+ * stops recording. */
+int ksgf_prof_reset(void);
+int ksgf_prof_read(double* ms, long long* launches, double* bytes, char* name, int name_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,7 +56,7 @@ EXPORTS = [
     "ksgf_extrap_vel_to_faces", "ksgf_compute_fluxes_from_state", "ksgf_compute_divergence",
     "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
     "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_predictor_update",
-    "ksgf_icns_src_term", "ksgf_scale_by_rho",
+    "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read",
 ]
 
 _lib = None
@@ -113,8 +113,21 @@ def load(path=None):
     L.ksgf_icns_src_term.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_scale_by_rho.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_void_p]
+    L.ksgf_prof_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.c_char_p,
+                                 C.c_int]
     _lib = L
     return L
+
+
+def prof_reset():
+    check(load().ksgf_prof_reset(), "prof_reset")
+
+
+def prof_read():
+    ms, n, b = C.c_double(), C.c_longlong(), C.c_double()
+    name = C.create_string_buffer(128)
+    check(load().ksgf_prof_read(C.byref(ms), C.byref(n), C.byref(b), name, 128), "prof_read")
+    return {"ms": ms.value, "launches": n.value, "bytes": b.value, "name": name.value.decode()}
 
 
 def check(rc, what="ksgf call", allow=(0,)):

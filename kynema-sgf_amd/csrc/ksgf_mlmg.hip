@@ -115,30 +115,42 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, doub
     atomicMax(addr, (unsigned long long)__double_as_longlong(v));
 }
 
-// r = b - A x (r in R precision); |r|_inf into norm (bits of a double)
+// wavefront then workgroup max of a non-negative value; one atomic per workgroup
+__device__ __forceinline__ void block_max_atomic(double m, unsigned long long* norm)
+{
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, sh[w]);
+        atomic_max_nonneg(norm, r);
+    }
+}
+
+constexpr int kReduceBlocks = 2048;
+
+// r = b - A x (r in R precision); |r|_inf into norm (bits of a double).
+// Grid-stride so that the norm costs one atomic per workgroup.
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_mg_residual(const T* __restrict__ x, const T* __restrict__ b,
                                                      R* __restrict__ r, LevelGeom L,
                                                      unsigned long long* __restrict__ norm)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
+    const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, dg = (T)L.dg;
     double m = 0.0;
-    if (t < tot) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
         int i = (int)(t % L.n0);
         long long q = t / L.n0;
         int j = (int)(q % L.n1), k = (int)(q / L.n1);
         long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
-        const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, dg = (T)L.dg;
         T v = b[c] - (lap_off<T>(x, c, L.sy, L.sz, cx, cy, cz) - dg * x[c]);
         r[c] = (R)v;
-        m = fabs((double)v);
+        m = fmax(m, fabs((double)v));
     }
-    if (norm) {
-        // wavefront max then one atomic per wavefront
-        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-        if ((threadIdx.x & 63) == 0) atomic_max_nonneg(norm, m);
-    }
+    if (norm) block_max_atomic(m, norm);
 }
 
 template <typename T>
@@ -553,14 +565,52 @@ static void launch_ghosts(T* a, const LevelGeom& L, hipStream_t st)
     hipLaunchKernelGGL(k_mg_ghosts<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, a, L);
 }
 
+// ---------------------------------------------------------------------------
+// live kernel timing for the bench: HIP events around every launch of the
+// finest-level smoother on the launch stream, with its algorithmic bytes
+// (a red or black half sweep counts half of the 24 B/cell FP64 (12 B/cell
+// FP32) of a full read-phi/read-rhs/write-phi sweep)
+// ---------------------------------------------------------------------------
+struct KernelProf {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    double bytes = 0.0;
+    long long launches = 0;
+    std::string name = "k_mg_smooth(level 0)";
+};
+static KernelProf g_prof;
+
+static hipEvent_t prof_event()
+{
+    if (g_prof.used == g_prof.ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        g_prof.ev.push_back(e);
+    }
+    return g_prof.ev[g_prof.used++];
+}
+
 template <typename T>
-static void launch_smooth(T* x, const T* b, const LevelGeom& L, int nu, hipStream_t st)
+static void launch_smooth(T* x, const T* b, const LevelGeom& L, int nu, hipStream_t st, bool prof = false)
 {
     long long n = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
     for (int s = 0; s < nu; ++s)
         for (int color = 0; color < 2; ++color) {
             launch_ghosts<T>(x, L, st);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (prof && g_prof.on) {
+                e0 = prof_event();
+                e1 = prof_event();
+                if (e0 && e1) (void)hipEventRecord(e0, st);
+            }
             hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, x, b, L, color);
+            if (e0 && e1) {
+                (void)hipEventRecord(e1, st);
+                g_prof.launches += 1;
+                g_prof.bytes += 0.5 * 3.0 * sizeof(T) * (double)L.n0 * L.n1 * L.n2;
+                g_prof.name = sizeof(T) == 8 ? "k_mg_smooth<double>(level 0)" : "k_mg_smooth<float>(level 0)";
+            }
         }
 }
 
@@ -569,7 +619,9 @@ static void launch_residual(T* x, const T* b, R* r, const LevelGeom& L, unsigned
 {
     launch_ghosts<T>(x, L, st);
     long long n = (long long)L.n0 * L.n1 * L.n2;
-    hipLaunchKernelGGL((k_mg_residual<T, R>), dim3(grid_for(n, 256)), dim3(256), 0, st, (const T*)x, b, r, L, norm);
+    int nb = grid_for(n, 256);
+    if (nb > kReduceBlocks) nb = kReduceBlocks;
+    hipLaunchKernelGGL((k_mg_residual<T, R>), dim3(nb), dim3(256), 0, st, (const T*)x, b, r, L, norm);
 }
 
 template <typename T>
@@ -613,7 +665,7 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     }
     const LevelGeom& C = mp->G[l + 1];
     const int* r = &mp->ratio[3 * l];
-    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_pre_smooth, st);
+    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_pre_smooth, st, l == 0);
     launch_residual<T, T>(cor[l], res[l], tmp[l], L, nullptr, st);
     long long nc = (long long)C.n0 * C.n1 * C.n2;
     T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
@@ -623,7 +675,7 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     long long nf = (long long)L.n0 * L.n1 * L.n2;
     hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
                        L, C, r[0], r[1], r[2]);
-    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_post_smooth, st);
+    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_post_smooth, st, l == 0);
 }
 
 static int read_norm(ksgf_macproj* mp, hipStream_t st, double* out)
@@ -698,6 +750,36 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
 }  // namespace ksgf
 
 extern "C" {
+
+int ksgf_prof_reset(void)
+{
+    g_prof.on = true;
+    g_prof.used = 0;
+    g_prof.bytes = 0.0;
+    g_prof.launches = 0;
+    return KSGF_OK;
+}
+
+int ksgf_prof_read(double* ms, long long* launches, double* bytes, char* name, int name_len)
+{
+    if (!ms || !launches || !bytes) return KSGF_ERR_INVALID;
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+        KSGF_HIP_CHECK(hipEventSynchronize(g_prof.ev[i + 1]));
+        float t = 0.f;
+        KSGF_HIP_CHECK(hipEventElapsedTime(&t, g_prof.ev[i], g_prof.ev[i + 1]));
+        tot += t;
+    }
+    *ms = tot;
+    *launches = g_prof.launches;
+    *bytes = g_prof.bytes;
+    if (name && name_len > 0) {
+        std::strncpy(name, g_prof.name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    g_prof.on = false;
+    return KSGF_OK;
+}
 
 void ksgf_mlmg_default_options(ksgf_mlmg_options* o)
 {

@@ -12,18 +12,25 @@ namespace ksgf {
 __global__ void __launch_bounds__(256) k_cfl(CMFDev v, RegDev r, const int* __restrict__ box, double dxi,
                                              double dyi, double dzi, unsigned long long* __restrict__ out)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int b, i, j, k;
+    __shared__ double sh[4];
     double m = 0.0;
-    if (region_locate(r, box, t, b, i, j, k)) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < r.total; t += (long long)gridDim.x * blockDim.x) {
+        int b, i, j, k;
+        region_locate(r, box, t, b, i, j, k);
         CFab f = fab_at(v, box, b);
         double a = fabs(f(i, j, k, 0)) * dxi;
         double bb = fabs(f(i, j, k, 1)) * dyi;
         double e = fabs(f(i, j, k, 2)) * dzi;
-        m = fmax(a, fmax(bb, e));
+        m = fmax(m, fmax(a, fmax(bb, e)));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) q = fmax(q, sh[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(q));
+    }
 }
 
 __global__ void __launch_bounds__(256) k_predictor(MFDev vn, CMFDev vo, CMFDev ro, CMFDev rn, CMFDev conv,
@@ -103,7 +110,9 @@ int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* 
     d = (unsigned long long*)get_scratch(ba, 40, 1, scratch_err);
     if (!d) return err;
     KSGF_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_cfl, dim3(grid_for(r->total, 256)), dim3(256), 0, st, v, regdev(r, ba->nbox), ba->dbox,
+    int nb = grid_for(r->total, 256);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, v, regdev(r, ba->nbox), ba->dbox,
                        1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], d);
     KSGF_LAUNCH_CHECK();
     unsigned long long bits = 0;
