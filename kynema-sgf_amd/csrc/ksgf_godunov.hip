@@ -4,13 +4,14 @@
 //   HydroUtils::ComputeDivergence           (icns_advection.H:418-422)
 //   PPM::PredictStateOn{X,Y,Z}Face          (test_mflux_schemes.cpp:87-151)
 //
-// Each predictor stage is one launch over every box of the BoxArray
-// (flat index space, binary search of the box from a prefix table), with the
-// stage results kept in per-BoxArray scratch MultiFabs:
-//   stage 0  PPM traces Im/Ip of every component in every direction
-//   stage 1  normal face states, advective velocity, upwinded edge states
-//   stage 2  corner-coupled (transverse-corrected) face states
-//   stage 3  final face states (and fluxes)
+// Two launches per face direction over every box of the BoxArray (flat
+// index space, binary search of the box from a prefix table):
+//   stage A  PPM traces of the two cells of each face, normal face states,
+//            advective velocity and upwinded normal edge states, on faces
+//            grown by one cell transversely (kept in scratch MultiFabs);
+//   stage B  corner-coupled (transverse-corrected) face states, one
+//            launch per (normal, transverse) direction pair;
+//   stage C  final face states (and fluxes).
 #include "ksgf_internal.h"
 #include "ksgf_ppm.h"
 
@@ -60,66 +61,31 @@ __device__ __forceinline__ int pick(int a0, int a1, int a2)
 }
 
 // ---------------------------------------------------------------------------
-// stage 0: PPM traces (cells of the boxes grown by one)
-// ---------------------------------------------------------------------------
-// mode 0: velocity prediction, advecting velocity = cell-centred normal
-//         component of vel (PredictVelOnFaces);
-// mode 1: state prediction with the face velocities umac/vmac/wmac.
-template <int D>
-__global__ void __launch_bounds__(256) k_ppm(CMFDev q, int ncomp, CMF3 mac, int mode, MFDev Im, MFDev Ip,
-                                             RegDev r, const int* __restrict__ box, BCs bc, GeomDev g,
-                                             double dtdx, int lim)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
-    CFab s = fab_at(q, box, b);
-    const long long st = (D == 0) ? 1 : (D == 1 ? (long long)s.n0 : (long long)s.n0 * s.n1);
-    const long long c = s.idx(i, j, k);
-    double um, up;
-    if (mode == 0) {
-        um = s.p[c + D * s.cs];
-        up = um;
-    } else {
-        CFab m = fab_at(mac.m[D], box, b);
-        int i1 = i, j1 = j, k1 = k;
-        unit<D>(i1, j1, k1, 1);
-        um = m(i, j, k);
-        up = m(i1, j1, k1);
-    }
-    Fab fm = fab_at(Im, box, b);
-    Fab fp = fab_at(Ip, box, b);
-    const int x = pick<D>(i, j, k);
-    for (int n = 0; n < ncomp; ++n) {
-        const double* pn = s.p + c + n * s.cs;
-        double a, bb;
-        ppm_trace(pn[-2 * st], pn[-st], pn[0], pn[st], pn[2 * st], lim, bc.lo[n][D], bc.hi[n][D], x, g.dlo[D],
-                  g.dhi[D], um, up, dtdx, a, bb);
-        fm(i, j, k, n) = a;
-        fp(i, j, k, n) = bb;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// stage 1: normal face states on faces of direction D grown by one
-// transversely.  vel_mode: ExtrapVelToFaces (advective velocity from the
-// Burgers problem of the normal component, both sides Dirichlet);
-// otherwise the MAC velocity upwinds and is_velocity selects both-side
-// Dirichlet for the normal component only.
+// stage A: normal face states on faces of direction D grown by one
+// transversely.  The PPM traces of the two cells sharing the face (Ip of
+// the low cell, Im of the high cell) are evaluated here from a 6-point
+// stencil along D.
+//   vel_mode: ExtrapVelToFaces -- the advecting velocity of a cell is its
+//             own normal component; the advective velocity uad solves the
+//             Burgers problem of the normal component; Dirichlet sets both
+//             sides for every component.
+//   otherwise: ComputeEdgeState -- the MAC velocity advects and upwinds;
+//             is_velocity selects both-side Dirichlet for the normal
+//             component only.
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(256) k_face(CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev Im, CMFDev Ip,
-                                              CMFDev umacD, MFDev flo, MFDev fhi, MFDev fed, MFDev uad,
-                                              RegDev r, const int* __restrict__ box, BCs bc, GeomDev g,
-                                              double hdt, int uft, int vel_mode, int is_velocity)
+__global__ void __launch_bounds__(256) k_faceA(CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev macD, MFDev flo,
+                                               MFDev fhi, MFDev fed, MFDev uad, RegDev r,
+                                               const int* __restrict__ box, BCs bc, GeomDev g, double dtdx,
+                                               double hdt, int uft, int vel_mode, int is_velocity, int lim)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     int qi = i, qj = j, qk = k;
     unit<D>(qi, qj, qk, -1);
-    CFab fim = fab_at(Im, box, b);
-    CFab fip = fab_at(Ip, box, b);
+    CFab s = fab_at(q, box, b);
+    const long long st = (D == 0) ? 1 : (D == 1 ? (long long)s.n0 : (long long)s.n0 * s.n1);
+    const long long cp = s.idx(i, j, k);
     CFab ff = fab_at(f, box, b);
     CFab fqb = fab_at(qbc, box, b);
     Fab flo_ = fab_at(flo, box, b);
@@ -130,15 +96,40 @@ __global__ void __launch_bounds__(256) k_face(CMFDev q, CMFDev qbc, CMFDev f, in
     if (D == 0) { gli = g.dlo[0] - 1; ghi_ = g.dhi[0] + 1; }
     if (D == 1) { glj = g.dlo[1] - 1; ghj = g.dhi[1] + 1; }
     if (D == 2) { glk = g.dlo[2] - 1; ghk = g.dhi[2] + 1; }
+    // advecting velocities of the low cell q and the high cell p
+    double uqm, uqp, upm, upp;
+    if (vel_mode) {
+        uqm = uqp = s.p[cp - st + D * s.cs];
+        upm = upp = s.p[cp + D * s.cs];
+    } else {
+        CFab m = fab_at(macD, box, b);
+        int i1 = i, j1 = j, k1 = k;
+        unit<D>(i1, j1, k1, 1);
+        uqm = m(qi, qj, qk);
+        uqp = m(i, j, k);
+        upm = uqp;
+        upp = m(i1, j1, k1);
+    }
+    auto face_states = [&](int n, double& lov, double& hiv) {
+        const double* pn = s.p + cp + n * s.cs;
+        double a, bb, dmy;
+        // low cell q: stencil p-3 .. p+1
+        ppm_trace(pn[-3 * st], pn[-2 * st], pn[-st], pn[0], pn[st], lim, bc.lo[n][D], bc.hi[n][D], x - 1, g.dlo[D],
+                  g.dhi[D], uqm, uqp, dtdx, dmy, a);
+        // high cell p: stencil p-2 .. p+2
+        ppm_trace(pn[-2 * st], pn[-st], pn[0], pn[st], pn[2 * st], lim, bc.lo[n][D], bc.hi[n][D], x, g.dlo[D],
+                  g.dhi[D], upm, upp, dtdx, bb, dmy);
+        lov = a;
+        hiv = bb;
+        if (uft) {
+            lov += hdt * ff(qi, qj, qk, n);
+            hiv += hdt * ff(i, j, k, n);
+        }
+    };
     double u;
     if (vel_mode) {
-        // normal component first: advective velocity
-        double lov = fip(qi, qj, qk, D);
-        double hiv = fim(i, j, k, D);
-        if (uft) {
-            lov += hdt * ff(qi, qj, qk, D);
-            hiv += hdt * ff(i, j, k, D);
-        }
+        double lov, hiv;
+        face_states(D, lov, hiv);
         double glo = 0.0, ghi = 0.0;
         if (x == g.dlo[D]) glo = fqb(gli, glj, glk, D);
         if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, D);
@@ -147,16 +138,11 @@ __global__ void __launch_bounds__(256) k_face(CMFDev q, CMFDev qbc, CMFDev f, in
         Fab fu = fab_at(uad, box, b);
         fu(i, j, k) = u;
     } else {
-        CFab m = fab_at(umacD, box, b);
-        u = m(i, j, k);
+        u = uqp;
     }
     for (int n = 0; n < ncomp; ++n) {
-        double lov = fip(qi, qj, qk, n);
-        double hiv = fim(i, j, k, n);
-        if (uft) {
-            lov += hdt * ff(qi, qj, qk, n);
-            hiv += hdt * ff(i, j, k, n);
-        }
+        double lov, hiv;
+        face_states(n, lov, hiv);
         double glo = 0.0, ghi = 0.0;
         if (x == g.dlo[D]) glo = fqb(gli, glj, glk, n);
         if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, n);
@@ -169,7 +155,7 @@ __global__ void __launch_bounds__(256) k_face(CMFDev q, CMFDev qbc, CMFDev f, in
 }
 
 // ---------------------------------------------------------------------------
-// stage 2: corner-coupled states on faces normal to D corrected along T.
+// stage B: corner-coupled states on faces normal to D corrected along T.
 // vel_mode: only component E = 3-D-T, advective form with the predicted
 // advective velocities (dt/6 factor);  otherwise all components with the
 // MAC velocities, conservative (dt/3, flux difference + q * transverse
@@ -183,9 +169,8 @@ __global__ void __launch_bounds__(256) k_corner(CMFDev q, CMFDev qbc, int ncomp,
                                                 const int* __restrict__ iconserv_unused, int cons_mask)
 {
     constexpr int E = 3 - D - T;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     int qi = i, qj = j, qk = k;
     unit<D>(qi, qj, qk, -1);
     int qti = qi, qtj = qj, qtk = qk;
@@ -239,7 +224,7 @@ __global__ void __launch_bounds__(256) k_corner(CMFDev q, CMFDev qbc, int ncomp,
 }
 
 // ---------------------------------------------------------------------------
-// stage 3: final states on the valid faces of direction D.
+// stage C: final states on the valid faces of direction D.
 // crnA: corner states normal to T1 = (D+1)%3 corrected along T2;
 // crnB: normal to T2 corrected along T1.
 // vel_mode: normal velocity only -> u_mac (Burgers);  otherwise all
@@ -255,9 +240,8 @@ __global__ void __launch_bounds__(256) k_final(CMFDev q, CMFDev qbc, CMFDev f, C
                                                int area_weighted, int allow_inflow, int cons_mask)
 {
     constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     int qi = i, qj = j, qk = k;
     unit<D>(qi, qj, qk, -1);
     int q1i = qi, q1j = qj, q1k = qk, p1i = i, p1j = j, p1k = k;
@@ -372,9 +356,8 @@ template <int D>
 __global__ void __launch_bounds__(256) k_known_flux(CMFDev face, CMFDev velD, MFDev flux, int ncomp, RegDev r,
                                                     const int* __restrict__ box, double area, int area_weighted)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     CFab fc = fab_at(face, box, b);
     CFab vd = fab_at(velD, box, b);
     Fab fl = fab_at(flux, box, b);
@@ -386,9 +369,8 @@ __global__ void __launch_bounds__(256) k_divergence(MFDev div, CMF3 fl, int ncom
                                                     const int* __restrict__ box, double mult, double dxi,
                                                     double dyi, double dzi, double vol, int area_weighted)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     Fab o = fab_at(div, box, b);
     CFab fx = fab_at(fl.m[0], box, b);
     CFab fy = fab_at(fl.m[1], box, b);
@@ -412,9 +394,8 @@ __global__ void __launch_bounds__(256) k_ppm_predict(CMFDev s, int comp, CMFDev 
                                                      RegDev r, const int* __restrict__ box, int bclo, int bchi,
                                                      int dlo, int dhi, double dtdx, int lim)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     CFab sf = fab_at(s, box, b);
     CFab vf = fab_at(vel, box, b);
     const long long st = (D == 0) ? 1 : (D == 1 ? (long long)sf.n0 : (long long)sf.n0 * sf.n1);
@@ -500,9 +481,7 @@ static CMFDev ccast(const MFDev& m)
 
 // scratch slot map
 enum {
-    S_IM = 0,    // 3 slots
-    S_IP = 3,    // 3
-    S_LO = 6,    // 3
+    S_LO = 6,    // 3 slots
     S_HI = 9,    // 3
     S_ED = 12,   // 3
     S_UAD = 15,  // 3
@@ -510,22 +489,16 @@ enum {
 };
 
 struct Stages {
-    MFDev Im[3], Ip[3], lo[3], hi[3], ed[3], uad[3], crn[3][3];
+    MFDev lo[3], hi[3], ed[3], uad[3], crn[3][3];
 };
 
 static int alloc_stages(const ksgf_boxarray* ba, int ncomp, int ncrn, Stages& S)
 {
     int err = KSGF_OK;
-    const Region* rc = fab_region(ba, KSGF_CELL, 1, &err);
-    if (!rc) return err;
     for (int d = 0; d < 3; ++d) {
         const Region* rf = fab_region(ba, KSGF_XFACE + d, 1, &err);
         if (!rf) return err;
         double* p;
-        if (!(p = get_scratch(ba, S_IM + d, (size_t)rc->total * ncomp, &err))) return err;
-        S.Im[d] = {p, rc->pre, KSGF_CELL, 1, ncomp};
-        if (!(p = get_scratch(ba, S_IP + d, (size_t)rc->total * ncomp, &err))) return err;
-        S.Ip[d] = {p, rc->pre, KSGF_CELL, 1, ncomp};
         if (!(p = get_scratch(ba, S_LO + d, (size_t)rf->total * ncomp, &err))) return err;
         S.lo[d] = {p, rf->pre, KSGF_XFACE + d, 1, ncomp};
         if (!(p = get_scratch(ba, S_HI + d, (size_t)rf->total * ncomp, &err))) return err;
@@ -544,23 +517,9 @@ static int alloc_stages(const ksgf_boxarray* ba, int ncomp, int ncrn, Stages& S)
 }
 
 template <int D>
-static int launch_ppm(const ksgf_boxarray* ba, CMFDev q, int ncomp, CMF3 mac, int mode, const Stages& S,
-                      const BCs& bc, const GeomDev& g, double dtdx, int lim, hipStream_t st)
-{
-    int err = KSGF_OK;
-    int one[3] = {1, 1, 1};
-    const Region* r = get_region(ba, one, one, &err);
-    if (!r) return err;
-    hipLaunchKernelGGL(k_ppm<D>, dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, ncomp, mac, mode, S.Im[D],
-                       S.Ip[D], regdev(r, ba->nbox), ba->dbox, bc, g, dtdx, lim);
-    KSGF_LAUNCH_CHECK();
-    return KSGF_OK;
-}
-
-template <int D>
-static int launch_face(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev umacD,
-                       const Stages& S, const BCs& bc, const GeomDev& g, double hdt, int uft, int vel_mode,
-                       int is_velocity, hipStream_t st)
+static int launch_faceA(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev macD,
+                        const Stages& S, const BCs& bc, const GeomDev& g, double dtdx, double hdt, int uft,
+                        int vel_mode, int is_velocity, int lim, hipStream_t st)
 {
     int err = KSGF_OK;
     int glo[3], ghi[3];
@@ -570,9 +529,9 @@ static int launch_face(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, 
     }
     const Region* r = get_region(ba, glo, ghi, &err);
     if (!r) return err;
-    hipLaunchKernelGGL(k_face<D>, dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, qbc, f, ncomp,
-                       ccast(S.Im[D]), ccast(S.Ip[D]), umacD, S.lo[D], S.hi[D], S.ed[D], S.uad[D],
-                       regdev(r, ba->nbox), ba->dbox, bc, g, hdt, uft, vel_mode, is_velocity);
+    hipLaunchKernelGGL(k_faceA<D>, dim3(r->nchunks), dim3(256), 0, st, q, qbc, f, ncomp, macD, S.lo[D],
+                       S.hi[D], S.ed[D], S.uad[D], regdev(r, ba->nbox), ba->dbox, bc, g, dtdx, hdt, uft, vel_mode,
+                       is_velocity, lim);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
 }
@@ -590,7 +549,7 @@ static int launch_corner(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, int ncom
     ghi[E] += 1;
     const Region* r = get_region(ba, glo, ghi, &err);
     if (!r) return err;
-    hipLaunchKernelGGL((k_corner<D, T>), dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, qbc, ncomp,
+    hipLaunchKernelGGL((k_corner<D, T>), dim3(r->nchunks), dim3(256), 0, st, q, qbc, ncomp,
                        ccast(S.lo[D]), ccast(S.hi[D]), ccast(S.ed[T]), velT, velD, S.crn[D][T],
                        regdev(r, ba->nbox), ba->dbox, bc, g, c3, c6, vel_mode, is_velocity, (const int*)nullptr,
                        cons_mask);
@@ -611,7 +570,7 @@ static int launch_final(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f,
     ghi[D] = 1;
     const Region* r = get_region(ba, glo, ghi, &err);
     if (!r) return err;
-    hipLaunchKernelGGL(k_final<D>, dim3(grid_for(r->total, 256)), dim3(256), 0, st, q, qbc, f, divu, has_divu,
+    hipLaunchKernelGGL(k_final<D>, dim3(r->nchunks), dim3(256), 0, st, q, qbc, f, divu, has_divu,
                        ncomp, ccast(S.lo[D]), ccast(S.hi[D]), ccast(S.crn[T1][T2]), ccast(S.crn[T2][T1]), vel1,
                        vel2, velD, out_state, out_flux, regdev(r, ba->nbox), ba->dbox, bc, g, c2[T1], c2[T2],
                        c4[T1], c4[T2], hdt, area, uft, vel_mode, is_velocity, area_weighted, allow_inflow,
@@ -655,7 +614,7 @@ int ksgf_ppm_predict_state_on_face(const ksgf_boxarray* ba, int dir, const doubl
     if (!r) return err;
     double dtdx = dt / ba->dx[dir];
     hipStream_t st = (hipStream_t)stream;
-    dim3 gr(grid_for(r->total, 256));
+    dim3 gr(r->nchunks);
     if (dir == 0)
         hipLaunchKernelGGL(k_ppm_predict<0>, gr, dim3(256), 0, st, sv, comp, vv, im, ip, regdev(r, ba->nbox),
                            ba->dbox, bc->lo[0], bc->hi[0], ba->dlo[0], ba->dhi[0], dtdx, limiter_type);
@@ -703,14 +662,9 @@ int ksgf_extrap_vel_to_faces(const ksgf_boxarray* ba, const double* vel, const d
         c2[d] = dt / (2.0 * ba->dx[d]);
     }
     const double hdt = 0.5 * dt;
-    CMF3 none;
-    for (int d = 0; d < 3; ++d) none.m[d] = q;
-    KSGF_TRY(launch_ppm<0>(ba, q, 3, none, 0, S, bc, g, dtdx[0], limiter_type, st));
-    KSGF_TRY(launch_ppm<1>(ba, q, 3, none, 0, S, bc, g, dtdx[1], limiter_type, st));
-    KSGF_TRY(launch_ppm<2>(ba, q, 3, none, 0, S, bc, g, dtdx[2], limiter_type, st));
-    KSGF_TRY(launch_face<0>(ba, q, q, f, 3, q, S, bc, g, hdt, use_forces_in_trans, 1, 1, st));
-    KSGF_TRY(launch_face<1>(ba, q, q, f, 3, q, S, bc, g, hdt, use_forces_in_trans, 1, 1, st));
-    KSGF_TRY(launch_face<2>(ba, q, q, f, 3, q, S, bc, g, hdt, use_forces_in_trans, 1, 1, st));
+    KSGF_TRY(launch_faceA<0>(ba, q, q, f, 3, q, S, bc, g, dtdx[0], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
+    KSGF_TRY(launch_faceA<1>(ba, q, q, f, 3, q, S, bc, g, dtdx[1], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
+    KSGF_TRY(launch_faceA<2>(ba, q, q, f, 3, q, S, bc, g, dtdx[2], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
     CMFDev uad[3];
     for (int d = 0; d < 3; ++d) uad[d] = ccast(S.uad[d]);
     KSGF_TRY((launch_corner<0, 1>(ba, q, q, 3, uad[1], uad[0], S, bc, g, 0.0, c6[1], 1, 1, 0, st)));
@@ -760,7 +714,7 @@ int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const dou
             ghi[d] = 1;
             const Region* r = get_region(ba, glo, ghi, &err);
             if (!r) return err;
-            dim3 gr(grid_for(r->total, 256));
+            dim3 gr(r->nchunks);
             if (d == 0)
                 hipLaunchKernelGGL(k_known_flux<0>, gr, dim3(256), 0, st, ccast(face[0]), mac[0], flux[0], ncomp,
                                    regdev(r, ba->nbox), ba->dbox, area[0], fluxes_are_area_weighted);
@@ -809,14 +763,9 @@ int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const dou
         c4[d] = dt / (4.0 * ba->dx[d]);
     }
     const double hdt = 0.5 * dt;
-    CMF3 m3;
-    for (int d = 0; d < 3; ++d) m3.m[d] = mac[d];
-    KSGF_TRY(launch_ppm<0>(ba, qv, ncomp, m3, 1, S, bc, g, dtdx[0], limiter_type, st));
-    KSGF_TRY(launch_ppm<1>(ba, qv, ncomp, m3, 1, S, bc, g, dtdx[1], limiter_type, st));
-    KSGF_TRY(launch_ppm<2>(ba, qv, ncomp, m3, 1, S, bc, g, dtdx[2], limiter_type, st));
-    KSGF_TRY(launch_face<0>(ba, qv, qb, fv, ncomp, mac[0], S, bc, g, hdt, use_forces_in_trans, 0, is_velocity, st));
-    KSGF_TRY(launch_face<1>(ba, qv, qb, fv, ncomp, mac[1], S, bc, g, hdt, use_forces_in_trans, 0, is_velocity, st));
-    KSGF_TRY(launch_face<2>(ba, qv, qb, fv, ncomp, mac[2], S, bc, g, hdt, use_forces_in_trans, 0, is_velocity, st));
+    KSGF_TRY(launch_faceA<0>(ba, qv, qb, fv, ncomp, mac[0], S, bc, g, dtdx[0], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
+    KSGF_TRY(launch_faceA<1>(ba, qv, qb, fv, ncomp, mac[1], S, bc, g, dtdx[1], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
+    KSGF_TRY(launch_faceA<2>(ba, qv, qb, fv, ncomp, mac[2], S, bc, g, dtdx[2], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
     KSGF_TRY((launch_corner<0, 1>(ba, qv, qb, ncomp, mac[1], mac[0], S, bc, g, c3[1], c6[1], 0, is_velocity, cons_mask, st)));
     KSGF_TRY((launch_corner<0, 2>(ba, qv, qb, ncomp, mac[2], mac[0], S, bc, g, c3[2], c6[2], 0, is_velocity, cons_mask, st)));
     KSGF_TRY((launch_corner<1, 0>(ba, qv, qb, ncomp, mac[0], mac[1], S, bc, g, c3[0], c6[0], 0, is_velocity, cons_mask, st)));
@@ -852,7 +801,7 @@ int ksgf_compute_divergence(const ksgf_boxarray* ba, double* div, const double* 
     int z[3] = {0, 0, 0};
     const Region* r = get_region(ba, z, z, &err);
     if (!r) return err;
-    hipLaunchKernelGGL(k_divergence, dim3(grid_for(r->total, 256)), dim3(256), 0, (hipStream_t)stream, dv, fl, ncomp,
+    hipLaunchKernelGGL(k_divergence, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, dv, fl, ncomp,
                        regdev(r, ba->nbox), ba->dbox, mult, 1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2],
                        ba->dx[0] * ba->dx[1] * ba->dx[2], fluxes_are_area_weighted);
     KSGF_LAUNCH_CHECK();

@@ -5,10 +5,9 @@
 // * A MultiFab is addressed through its base pointer plus a "region prefix"
 //   table pre[0..nbox] (int64, device): pre[b] is the number of points per
 //   component in FABs 0..b-1, so FAB b starts at ncomp*pre[b] and its
-//   component stride is pre[b+1]-pre[b].  The same prefix tables drive the
-//   flat launches over ragged box sets: thread t belongs to the box b with
-//   pre[b] <= t < pre[b+1] (binary search) and decodes (i,j,k) from the
-//   remainder, x fastest, so consecutive lanes read consecutive x.
+//   component stride is pre[b+1]-pre[b].
+// * Launches over ragged box sets use a chunk table per region: one
+//   256-thread workgroup per 256 consecutive points of one box.
 // * A region is box b grown by glo[d] below and ghi[d] above in direction d.
 //   FAB storage of (itype, ngrow) is the region glo = ngrow,
 //   ghi = ngrow + (itype == d+1).
@@ -61,10 +60,14 @@ struct RegionKey {
 
 struct Region {
     long long* pre = nullptr;  // device, nbox+1
+    int* chunks = nullptr;     // device, 2 ints per 256-point chunk: box, first point
     std::vector<long long> hpre;
     long long total = 0;
+    int nchunks = 0;
     RegionKey key;
 };
+
+constexpr int kChunk = 256;  // points per chunk = threads per workgroup of region launches
 
 // ghost-exchange tables for one (itype, ngrow) storage
 struct HaloTable {
@@ -115,10 +118,10 @@ struct Fab {
     double* p;
     int lo0, lo1, lo2;
     int n0, n1;
-    long long cs;  // component stride
-    __device__ __forceinline__ long long idx(int i, int j, int k) const
+    int cs;  // component stride (a FAB holds fewer than 2^31 points)
+    __device__ __forceinline__ int idx(int i, int j, int k) const
     {
-        return (long long)(i - lo0) + (long long)n0 * ((j - lo1) + (long long)n1 * (k - lo2));
+        return (i - lo0) + n0 * ((j - lo1) + n1 * (k - lo2));
     }
     __device__ __forceinline__ double& operator()(int i, int j, int k, int n = 0) const
     {
@@ -130,10 +133,10 @@ struct CFab {
     const double* p;
     int lo0, lo1, lo2;
     int n0, n1;
-    long long cs;
-    __device__ __forceinline__ long long idx(int i, int j, int k) const
+    int cs;
+    __device__ __forceinline__ int idx(int i, int j, int k) const
     {
-        return (long long)(i - lo0) + (long long)n0 * ((j - lo1) + (long long)n1 * (k - lo2));
+        return (i - lo0) + n0 * ((j - lo1) + n1 * (k - lo2));
     }
     __device__ __forceinline__ double operator()(int i, int j, int k, int n = 0) const
     {
@@ -163,7 +166,7 @@ __device__ __forceinline__ Fab fab_at(const MFDev& m, const int* box, int b)
     f.n0 = bx[3] - bx[0] + 1 + 2 * m.ngrow + (m.itype == 1);
     f.n1 = bx[4] - bx[1] + 1 + 2 * m.ngrow + (m.itype == 2);
     long long a = m.pre[b], e = m.pre[b + 1];
-    f.cs = e - a;
+    f.cs = (int)(e - a);
     f.p = m.p + (long long)m.ncomp * a;
     return f;
 }
@@ -177,21 +180,51 @@ __device__ __forceinline__ CFab fab_at(const CMFDev& m, const int* box, int b)
     f.n0 = bx[3] - bx[0] + 1 + 2 * m.ngrow + (m.itype == 1);
     f.n1 = bx[4] - bx[1] + 1 + 2 * m.ngrow + (m.itype == 2);
     long long a = m.pre[b], e = m.pre[b + 1];
-    f.cs = e - a;
+    f.cs = (int)(e - a);
     f.p = m.p + (long long)m.ncomp * a;
     return f;
 }
 
-// region iteration: map a flat index to (box, i, j, k)
+// region iteration.  Launches over a region use one 256-thread workgroup
+// per chunk of 256 consecutive points of one box (chunk table: box, first
+// point), so a thread finds its box without a search and decodes (i,j,k)
+// with 32-bit arithmetic; x is fastest, consecutive lanes read consecutive x.
 struct RegDev {
     const long long* pre;
+    const int* chunks;
     int nbox;
     int glo0, glo1, glo2, ghi0, ghi1, ghi2;
     long long total;
 };
 
-__device__ __forceinline__ bool region_locate(const RegDev& r, const int* box, long long t,
-                                              int& b, int& i, int& j, int& k)
+__device__ __forceinline__ bool decode_in_box(const RegDev& r, const int* box, int b, unsigned loc, int& i, int& j,
+                                              int& k)
+{
+    const int* bx = box + 6 * b;
+    int l0 = bx[0] - r.glo0, l1 = bx[1] - r.glo1, l2 = bx[2] - r.glo2;
+    unsigned n0 = (unsigned)(bx[3] + r.ghi0 - l0 + 1);
+    unsigned n1 = (unsigned)(bx[4] + r.ghi1 - l1 + 1);
+    unsigned n2 = (unsigned)(bx[5] + r.ghi2 - l2 + 1);
+    if (loc >= n0 * n1 * n2) return false;
+    unsigned q = loc / n0;
+    i = l0 + (int)(loc - q * n0);
+    unsigned kk = q / n1;
+    j = l1 + (int)(q - kk * n1);
+    k = l2 + (int)kk;
+    return true;
+}
+
+// chunked launch: grid = nchunks, block = kChunk
+__device__ __forceinline__ bool region_locate(const RegDev& r, const int* box, int& b, int& i, int& j, int& k)
+{
+    b = r.chunks[2 * blockIdx.x];
+    unsigned loc = (unsigned)r.chunks[2 * blockIdx.x + 1] + threadIdx.x;
+    return decode_in_box(r, box, b, loc, i, j, k);
+}
+
+// flat index (grid-stride loops): binary search of the box
+__device__ __forceinline__ bool region_locate_flat(const RegDev& r, const int* box, long long t, int& b, int& i,
+                                                   int& j, int& k)
 {
     if (t >= r.total) return false;
     int lo = 0, hi = r.nbox - 1;
@@ -203,22 +236,14 @@ __device__ __forceinline__ bool region_locate(const RegDev& r, const int* box, l
             hi = mid - 1;
     }
     b = lo;
-    long long loc = t - r.pre[b];
-    const int* bx = box + 6 * b;
-    int l0 = bx[0] - r.glo0, l1 = bx[1] - r.glo1, l2 = bx[2] - r.glo2;
-    int n0 = bx[3] + r.ghi0 - l0 + 1;
-    int n1 = bx[4] + r.ghi1 - l1 + 1;
-    i = l0 + (int)(loc % n0);
-    long long q = loc / n0;
-    j = l1 + (int)(q % n1);
-    k = l2 + (int)(q / n1);
-    return true;
+    return decode_in_box(r, box, b, (unsigned)(t - r.pre[b]), i, j, k);
 }
 
 inline RegDev regdev(const Region* r, int nbox)
 {
     RegDev d;
     d.pre = r->pre;
+    d.chunks = r->chunks;
     d.nbox = nbox;
     d.glo0 = r->key.glo[0];
     d.glo1 = r->key.glo[1];

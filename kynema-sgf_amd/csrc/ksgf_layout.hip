@@ -50,6 +50,22 @@ const Region* get_region(const ksgf_boxarray* ba, const int glo[3], const int gh
     }
     r->hpre[ba->nbox] = acc;
     r->total = acc;
+    std::vector<int> ch;
+    for (int b = 0; b < ba->nbox; ++b) {
+        long long n = r->hpre[b + 1] - r->hpre[b];
+        for (long long s0 = 0; s0 < n; s0 += kChunk) {
+            ch.push_back(b);
+            ch.push_back((int)s0);
+        }
+    }
+    r->nchunks = (int)(ch.size() / 2);
+    if (hipMalloc(&r->chunks, sizeof(int) * ch.size()) != hipSuccess ||
+        hipMemcpy(r->chunks, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        delete r;
+        set_error("chunk table upload");
+        *err = KSGF_ERR_NOMEM;
+        return nullptr;
+    }
     if (hipMalloc(&r->pre, sizeof(long long) * (ba->nbox + 1)) != hipSuccess) {
         delete r;
         set_error("hipMalloc region table");
@@ -324,9 +340,8 @@ __global__ void __launch_bounds__(256) k_set_inflow(MFDev m, RegDev r, const int
                                                     int dlo, int dhi1, int tlo, int thi, double vlo,
                                                     double vhi)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     int p[3] = {i, j, k};
     Fab f = fab_at(m, box, b);
     if (p[d] == dlo && tlo == KSGF_BC_EXT_DIR) f(i, j, k) = vlo;
@@ -399,6 +414,7 @@ int ksgf_boxarray_destroy(ksgf_boxarray* ba)
     if (!ba) return KSGF_OK;
     for (auto& kv : ba->regions) {
         if (kv.second->pre) (void)hipFree(kv.second->pre);
+        if (kv.second->chunks) (void)hipFree(kv.second->chunks);
         delete kv.second;
     }
     for (auto& kv : ba->halos) {
@@ -495,7 +511,7 @@ int ksgf_set_inflow_velocity(const ksgf_boxarray* ba, double* umac, double* vmac
         int err = KSGF_OK;
         const Region* r = get_region(ba, glo, ghi, &err);
         if (!r) return err;
-        hipLaunchKernelGGL(k_set_inflow, dim3(grid_for(r->total, 256)), dim3(256), 0, (hipStream_t)stream, m,
+        hipLaunchKernelGGL(k_set_inflow, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, m,
                            regdev(r, ba->nbox), ba->dbox, d, ba->dlo[d], ba->dhi[d] + 1, tlo, thi,
                            bcval[d * 3 + d], bcval[(3 + d) * 3 + d]);
         KSGF_LAUNCH_CHECK();

@@ -14,10 +14,23 @@
 // atol eps*1e4, 200 iterations, no abort on non-convergence) and the solve
 // converges when |r|_inf <= max(atol, rtol*|rhs|_inf).
 //
+// Kernel organisation of one V-cycle on the large levels:
+//   down:  zero + first red half sweep fused; remaining half sweeps;
+//          residual fused with the 8-cell (or semicoarsened 2/4-cell)
+//          average restriction (no residual array is written);
+//   up:    piecewise-constant prolongation fused with the first red half
+//          sweep; remaining half sweeps.
+// Domain boundaries are handled in the stencil itself (periodic wrap,
+// Neumann mirror, Dirichlet odd reflection), so no ghost-fill launches are
+// needed; a level with an odd periodic extent (where wrap-around partners
+// share a colour) falls back to explicit ghost fills.  All levels with at
+// most kCoarseCells cells are handled by a single workgroup that runs the
+// rest of the V-cycle, including the BiCGStab bottom solve, in one launch.
+//
 // Mixed precision (KSGF_MG_MIXED): each outer iteration forms the FP64
-// residual of the FP64 solution, runs one V-cycle on it with FP32 smoothing
-// and grid transfers (the bottom BiCGStab stays FP64), and adds the FP32
-// correction to the FP64 solution.
+// residual of the FP64 solution (fused with adding the previous correction),
+// runs one V-cycle on it with FP32 smoothing and grid transfers (the bottom
+// BiCGStab stays FP64), and adds the FP32 correction to the FP64 solution.
 #include <cmath>
 #include <cstring>
 
@@ -29,34 +42,69 @@ struct LevelGeom {
     int n0, n1, n2;
     long long sy, sz, tot;
     double cx, cy, cz, dg, idg;
-    int lo[3], hi[3];  // lobc / hibc
+    int lo[3], hi[3];  // LinOp BCs
+    int ghost_mode;    // 1: explicit ghost fills before every colour
 };
 
-template <typename T>
-__device__ __forceinline__ T bc_ghost(int t, T vl, T vr, bool low)
+constexpr long long kCoarseCells = 16384;
+constexpr int kCoarseThreads = 1024;
+
+__host__ __device__ __forceinline__ long long LIdx(const LevelGeom& L, int i, int j, int k)
 {
-    // t: LinOp BC on this side; vl: value at the near interior cell of this
-    // side, vr: value at the far (periodic partner) interior cell
-    return (t == KSGF_LO_PERIODIC) ? vr : (t == KSGF_LO_NEUMANN ? vl : -vl);
+    return (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom L)
+__device__ __forceinline__ T bc_ghost(int t, T near_v, T far_v)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    return (t == KSGF_LO_PERIODIC) ? far_v : (t == KSGF_LO_NEUMANN ? near_v : -near_v);
+}
+
+// value of the low / high neighbour of cell c along one direction; `self`
+// is the cell's own value as a Neumann / Dirichlet boundary mirrors it
+template <typename T>
+__device__ __forceinline__ T nb_lo(const T* __restrict__ a, long long c, int pos, int n, long long s, int bc, int gm,
+                                   T self)
+{
+    if (gm || pos > 0) return a[c - s];
+    return bc_ghost<T>(bc, self, a[c + (long long)(n - 1) * s]);
+}
+template <typename T>
+__device__ __forceinline__ T nb_hi(const T* __restrict__ a, long long c, int pos, int n, long long s, int bc, int gm,
+                                   T self)
+{
+    if (gm || pos < n - 1) return a[c + s];
+    return bc_ghost<T>(bc, self, a[c - (long long)(n - 1) * s]);
+}
+
+// cx*(W+E) + cy*(S+N) + cz*(B+T), the same operand order as the oracle
+template <typename T>
+__device__ __forceinline__ T stencil(const T* __restrict__ a, long long c, int i, int j, int k, const LevelGeom& L, T cx,
+                                     T cy, T cz, T self)
+{
+    const int gm = L.ghost_mode;
+    T w = nb_lo<T>(a, c, i, L.n0, 1, L.lo[0], gm, self), e = nb_hi<T>(a, c, i, L.n0, 1, L.hi[0], gm, self);
+    T s = nb_lo<T>(a, c, j, L.n1, L.sy, L.lo[1], gm, self), n = nb_hi<T>(a, c, j, L.n1, L.sy, L.hi[1], gm, self);
+    T b = nb_lo<T>(a, c, k, L.n2, L.sz, L.lo[2], gm, self), t = nb_hi<T>(a, c, k, L.n2, L.sz, L.hi[2], gm, self);
+    return cx * (w + e) + cy * (s + n) + cz * (b + t);
+}
+
+// ---------------------------------------------------------------------------
+// ghost layer (only for ghost_mode levels, the coarse part and phi)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void ghost_point(T* __restrict__ a, const LevelGeom& L, long long t)
+{
     const long long nx = (long long)L.n1 * L.n2, ny = (long long)L.n0 * L.n2, nz = (long long)L.n0 * L.n1;
-    auto LI = [&](int i, int j, int k) { return (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1); };
     if (t < 2 * nx) {
         int side = (int)(t / nx);
         long long r = t % nx;
         int j = (int)(r % L.n1), k = (int)(r / L.n1);
-        if (side == 0) {
-            T vl = a[LI(0, j, k)], vr = a[LI(L.n0 - 1, j, k)];
-            a[LI(-1, j, k)] = bc_ghost(L.lo[0], vl, vr, true);
-        } else {
-            T vl = a[LI(0, j, k)], vr = a[LI(L.n0 - 1, j, k)];
-            a[LI(L.n0, j, k)] = bc_ghost(L.hi[0], vr, vl, false);
-        }
+        T vl = a[LIdx(L, 0, j, k)], vr = a[LIdx(L, L.n0 - 1, j, k)];
+        if (side == 0)
+            a[LIdx(L, -1, j, k)] = bc_ghost<T>(L.lo[0], vl, vr);
+        else
+            a[LIdx(L, L.n0, j, k)] = bc_ghost<T>(L.hi[0], vr, vl);
         return;
     }
     t -= 2 * nx;
@@ -64,11 +112,11 @@ __global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom 
         int side = (int)(t / ny);
         long long r = t % ny;
         int i = (int)(r % L.n0), k = (int)(r / L.n0);
-        T vl = a[LI(i, 0, k)], vr = a[LI(i, L.n1 - 1, k)];
+        T vl = a[LIdx(L, i, 0, k)], vr = a[LIdx(L, i, L.n1 - 1, k)];
         if (side == 0)
-            a[LI(i, -1, k)] = bc_ghost(L.lo[1], vl, vr, true);
+            a[LIdx(L, i, -1, k)] = bc_ghost<T>(L.lo[1], vl, vr);
         else
-            a[LI(i, L.n1, k)] = bc_ghost(L.hi[1], vr, vl, false);
+            a[LIdx(L, i, L.n1, k)] = bc_ghost<T>(L.hi[1], vr, vl);
         return;
     }
     t -= 2 * ny;
@@ -76,37 +124,156 @@ __global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom 
         int side = (int)(t / nz);
         long long r = t % nz;
         int i = (int)(r % L.n0), j = (int)(r / L.n0);
-        T vl = a[LI(i, j, 0)], vr = a[LI(i, j, L.n2 - 1)];
+        T vl = a[LIdx(L, i, j, 0)], vr = a[LIdx(L, i, j, L.n2 - 1)];
         if (side == 0)
-            a[LI(i, j, -1)] = bc_ghost(L.lo[2], vl, vr, true);
+            a[LIdx(L, i, j, -1)] = bc_ghost<T>(L.lo[2], vl, vr);
         else
-            a[LI(i, j, L.n2)] = bc_ghost(L.hi[2], vr, vl, false);
+            a[LIdx(L, i, j, L.n2)] = bc_ghost<T>(L.hi[2], vr, vl);
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T lap_off(const T* __restrict__ a, long long c, long long sy, long long sz, T cx, T cy,
-                                     T cz)
+__host__ __device__ __forceinline__ long long ghost_count(const LevelGeom& L)
 {
-    return cx * (a[c - 1] + a[c + 1]) + cy * (a[c - sy] + a[c + sy]) + cz * (a[c - sz] + a[c + sz]);
+    return 2 * ((long long)L.n1 * L.n2 + (long long)L.n0 * L.n2 + (long long)L.n0 * L.n1);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ x, const T* __restrict__ b, LevelGeom L,
-                                                   int color)
+__global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom L)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ghost_count(L)) ghost_point<T>(a, L, t);
+}
+
+// ---------------------------------------------------------------------------
+// grid-level smoothing / transfer kernels
+// ---------------------------------------------------------------------------
+// one red (color 0) or black (color 1) Gauss-Seidel half sweep.  With xc,
+// this is the black half sweep right after the fused prolongation + red
+// sweep: black cells still hold their pre-prolongation value, so the
+// value a wall mirrors is x + xc(parent).
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ x, const T* __restrict__ b, LevelGeom L, int color,
+                                                   const T* __restrict__ xc, LevelGeom C, int r0, int r1, int r2)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int h0 = (L.n0 + 1) / 2;
     const long long tot = (long long)h0 * L.n1 * L.n2;
     if (t >= tot) return;
-    int ih = (int)(t % h0);
-    long long r = t / h0;
-    int j = (int)(r % L.n1), k = (int)(r / L.n1);
+    const unsigned tt = (unsigned)t, r = tt / (unsigned)h0;
+    int ih = (int)(tt - r * (unsigned)h0);
+    int j = (int)(r % (unsigned)L.n1), k = (int)(r / (unsigned)L.n1);
     int i = 2 * ih + ((j + k + color) & 1);
     if (i >= L.n0) return;
-    long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
+    long long c = LIdx(L, i, j, k);
     const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, idg = (T)L.idg;
-    x[c] = (lap_off<T>(x, c, L.sy, L.sz, cx, cy, cz) - b[c]) * idg;
+    T self = x[c];
+    if (xc) self = self + xc[LIdx(C, i / r0, j / r1, k / r2)];
+    x[c] = (stencil<T>(x, c, i, j, k, L, cx, cy, cz, self) - b[c]) * idg;
+}
+
+// zero initial guess + the first red half sweep: red = (0 - b)/diag, black = 0
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth_first(T* __restrict__ x, const T* __restrict__ b, LevelGeom L)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)L.n0 * L.n1 * L.n2;
+    if (t >= tot) return;
+    const unsigned tt = (unsigned)t, q = tt / (unsigned)L.n0;
+    int i = (int)(tt - q * (unsigned)L.n0);
+    int j = (int)(q % (unsigned)L.n1), k = (int)(q / (unsigned)L.n1);
+    long long c = LIdx(L, i, j, k);
+    x[c] = (((i + j + k) & 1) == 0) ? ((T)0 - b[c]) * (T)L.idg : (T)0;
+}
+
+// residual of the fine children summed into the coarse right-hand side
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_resid_restrict(const T* __restrict__ x, const T* __restrict__ b,
+                                                           T* __restrict__ cres, LevelGeom F, LevelGeom C, int r0,
+                                                           int r1, int r2, T w)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)C.n0 * C.n1 * C.n2;
+    if (t >= tot) return;
+    const unsigned tt = (unsigned)t, q = tt / (unsigned)C.n0;
+    int I = (int)(tt - q * (unsigned)C.n0);
+    int J = (int)(q % (unsigned)C.n1), K = (int)(q / (unsigned)C.n1);
+    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
+    T s = (T)0;
+    for (int dk = 0; dk < r2; ++dk)
+        for (int dj = 0; dj < r1; ++dj)
+            for (int di = 0; di < r0; ++di) {
+                int i = I * r0 + di, j = J * r1 + dj, k = K * r2 + dk;
+                long long c = LIdx(F, i, j, k);
+                T v = b[c] - (stencil<T>(x, c, i, j, k, F, cx, cy, cz, x[c]) - dg * x[c]);
+                s += v;
+            }
+    cres[LIdx(C, I, J, K)] = s * w;
+}
+
+// piecewise-constant prolongation fused with the first red half sweep: red
+// cells are relaxed against their prolonged black neighbours (x + xc(parent),
+// formed on the fly).  Black cells are not written: the black half sweep
+// that follows overwrites them without reading their own value, so their
+// prolonged value is never needed (and writing it here would race with the
+// red cells reading the same neighbours).
+template <typename T>
+__device__ __forceinline__ T prolonged(const T* __restrict__ x, const T* __restrict__ xc, const LevelGeom& F,
+                                       const LevelGeom& C, int i, int j, int k, int r0, int r1, int r2)
+{
+    return x[LIdx(F, i, j, k)] + xc[LIdx(C, i / r0, j / r1, k / r2)];
+}
+
+template <typename T>
+__device__ __forceinline__ T pnb(const T* __restrict__ x, const T* __restrict__ xc, const LevelGeom& F,
+                                 const LevelGeom& C, int i, int j, int k, int d, int s, int r0, int r1, int r2)
+{
+    int p[3] = {i, j, k};
+    const int n[3] = {F.n0, F.n1, F.n2};
+    int q = p[d] + s;
+    if (q >= 0 && q < n[d]) {
+        p[d] = q;
+        return prolonged<T>(x, xc, F, C, p[0], p[1], p[2], r0, r1, r2);
+    }
+    const int bc = (s < 0) ? F.lo[d] : F.hi[d];
+    if (bc == KSGF_LO_PERIODIC) {
+        p[d] = (s < 0) ? n[d] - 1 : 0;
+        return prolonged<T>(x, xc, F, C, p[0], p[1], p[2], r0, r1, r2);
+    }
+    T self = prolonged<T>(x, xc, F, C, i, j, k, r0, r1, r2);
+    return (bc == KSGF_LO_NEUMANN) ? self : -self;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong_red(T* __restrict__ x, const T* __restrict__ xc,
+                                                        const T* __restrict__ b, LevelGeom F, LevelGeom C, int r0,
+                                                        int r1, int r2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)F.n0 * F.n1 * F.n2;
+    if (t >= tot) return;
+    const unsigned tt = (unsigned)t, q = tt / (unsigned)F.n0;
+    int i = (int)(tt - q * (unsigned)F.n0);
+    int j = (int)(q % (unsigned)F.n1), k = (int)(q / (unsigned)F.n1);
+    long long c = LIdx(F, i, j, k);
+    if (((i + j + k) & 1) != 0) return;
+    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, idg = (T)F.idg;
+    T w = pnb<T>(x, xc, F, C, i, j, k, 0, -1, r0, r1, r2), e = pnb<T>(x, xc, F, C, i, j, k, 0, 1, r0, r1, r2);
+    T s = pnb<T>(x, xc, F, C, i, j, k, 1, -1, r0, r1, r2), n = pnb<T>(x, xc, F, C, i, j, k, 1, 1, r0, r1, r2);
+    T bo = pnb<T>(x, xc, F, C, i, j, k, 2, -1, r0, r1, r2), to = pnb<T>(x, xc, F, C, i, j, k, 2, 1, r0, r1, r2);
+    x[c] = ((cx * (w + e) + cy * (s + n) + cz * (bo + to)) - b[c]) * idg;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const T* __restrict__ c, LevelGeom F,
+                                                        LevelGeom C, int r0, int r1, int r2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)F.n0 * F.n1 * F.n2;
+    if (t >= tot) return;
+    const unsigned tt = (unsigned)t, q = tt / (unsigned)F.n0;
+    int i = (int)(tt - q * (unsigned)F.n0);
+    int j = (int)(q % (unsigned)F.n1), k = (int)(q / (unsigned)F.n1);
+    f[LIdx(F, i, j, k)] += c[LIdx(C, i / r0, j / r1, k / r2)];
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v)
@@ -115,10 +282,9 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, doub
     atomicMax(addr, (unsigned long long)__double_as_longlong(v));
 }
 
-// wavefront then workgroup max of a non-negative value; one atomic per workgroup
 __device__ __forceinline__ void block_max_atomic(double m, unsigned long long* norm)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[16];
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -131,58 +297,49 @@ __device__ __forceinline__ void block_max_atomic(double m, unsigned long long* n
 
 constexpr int kReduceBlocks = 2048;
 
-// r = b - A x (r in R precision); |r|_inf into norm (bits of a double).
-// Grid-stride so that the norm costs one atomic per workgroup.
-template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_mg_residual(const T* __restrict__ x, const T* __restrict__ b,
-                                                     R* __restrict__ r, LevelGeom L,
-                                                     unsigned long long* __restrict__ norm)
+// FP64 outer residual.  With a correction: phi_out = phi + cor and
+// r = rhs - A(phi + cor) in one pass (ping-pong phi buffers); without:
+// r = rhs - A phi.  r is written in precision R (FP32 feeds the mixed
+// V-cycle directly) and |r|_inf goes to norm.
+template <typename C, typename R>
+__global__ void __launch_bounds__(256) k_update_residual(const double* __restrict__ phi, const C* __restrict__ cor,
+                                                         double* __restrict__ phi_out, const double* __restrict__ rhs,
+                                                         R* __restrict__ r, LevelGeom L,
+                                                         unsigned long long* __restrict__ norm)
 {
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
-    const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, dg = (T)L.dg;
+    const int gm = L.ghost_mode;
     double m = 0.0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
-        int i = (int)(t % L.n0);
-        long long q = t / L.n0;
-        int j = (int)(q % L.n1), k = (int)(q / L.n1);
-        long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
-        T v = b[c] - (lap_off<T>(x, c, L.sy, L.sz, cx, cy, cz) - dg * x[c]);
+        const unsigned tt = (unsigned)t, q = tt / (unsigned)L.n0;
+        int i = (int)(tt - q * (unsigned)L.n0);
+        int j = (int)(q % (unsigned)L.n1), k = (int)(q / (unsigned)L.n1);
+        long long c = LIdx(L, i, j, k);
+        double v;
+        if (cor) {
+            // neighbour values of phi + cor, boundary handled on the fly
+            auto val = [&](long long cc) { return phi[cc] + (double)cor[cc]; };
+            double self = val(c);
+            auto lo = [&](int pos, int n, long long s, int bc) {
+                if (gm || pos > 0) return val(c - s);
+                return bc == KSGF_LO_PERIODIC ? val(c + (long long)(n - 1) * s) : (bc == KSGF_LO_NEUMANN ? self : -self);
+            };
+            auto hi = [&](int pos, int n, long long s, int bc) {
+                if (gm || pos < n - 1) return val(c + s);
+                return bc == KSGF_LO_PERIODIC ? val(c - (long long)(n - 1) * s) : (bc == KSGF_LO_NEUMANN ? self : -self);
+            };
+            double st = L.cx * (lo(i, L.n0, 1, L.lo[0]) + hi(i, L.n0, 1, L.hi[0])) +
+                        L.cy * (lo(j, L.n1, L.sy, L.lo[1]) + hi(j, L.n1, L.sy, L.hi[1])) +
+                        L.cz * (lo(k, L.n2, L.sz, L.lo[2]) + hi(k, L.n2, L.sz, L.hi[2]));
+            v = rhs[c] - (st - L.dg * self);
+            phi_out[c] = self;
+        } else {
+            v = rhs[c] - (stencil<double>(phi, c, i, j, k, L, L.cx, L.cy, L.cz, phi[c]) - L.dg * phi[c]);
+        }
         r[c] = (R)v;
-        m = fmax(m, fabs((double)v));
+        m = fmax(m, fabs(v));
     }
     if (norm) block_max_atomic(m, norm);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_mg_restrict(const T* __restrict__ f, T* __restrict__ c, LevelGeom F,
-                                                     LevelGeom C, int r0, int r1, int r2, T w)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long tot = (long long)C.n0 * C.n1 * C.n2;
-    if (t >= tot) return;
-    int I = (int)(t % C.n0);
-    long long q = t / C.n0;
-    int J = (int)(q % C.n1), K = (int)(q / C.n1);
-    T s = (T)0;
-    for (int dk = 0; dk < r2; ++dk)
-        for (int dj = 0; dj < r1; ++dj)
-            for (int di = 0; di < r0; ++di)
-                s += f[(long long)(I * r0 + di + 1) + F.sy * (J * r1 + dj + 1) + F.sz * (K * r2 + dk + 1)];
-    c[(long long)(I + 1) + C.sy * (J + 1) + C.sz * (K + 1)] = s * w;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const T* __restrict__ c, LevelGeom F,
-                                                        LevelGeom C, int r0, int r1, int r2)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long tot = (long long)F.n0 * F.n1 * F.n2;
-    if (t >= tot) return;
-    int i = (int)(t % F.n0);
-    long long q = t / F.n0;
-    int j = (int)(q % F.n1), k = (int)(q / F.n1);
-    f[(long long)(i + 1) + F.sy * (j + 1) + F.sz * (k + 1)] +=
-        c[(long long)(i / r0 + 1) + C.sy * (j / r1 + 1) + C.sz * (k / r2 + 1)];
 }
 
 template <typename T>
@@ -190,13 +347,6 @@ __global__ void __launch_bounds__(256) k_fill(T* __restrict__ a, long long n, T 
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) a[t] = v;
-}
-
-template <typename A, typename B>
-__global__ void __launch_bounds__(256) k_add_to(A* __restrict__ a, const B* __restrict__ b, long long n)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) a[t] += (A)b[t];
 }
 
 template <typename A, typename B>
@@ -214,17 +364,16 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 
 // deterministic sum of interior cells: per-block partial sums
 constexpr int kSumBlocks = 512;
-__global__ void __launch_bounds__(256) k_partial_sum(const double* __restrict__ a, LevelGeom L,
-                                                     double* __restrict__ part)
+__global__ void __launch_bounds__(256) k_partial_sum(const double* __restrict__ a, LevelGeom L, double* __restrict__ part)
 {
     __shared__ double sh[256];
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
     double s = 0.0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
-        int i = (int)(t % L.n0);
-        long long q = t / L.n0;
-        int j = (int)(q % L.n1), k = (int)(q / L.n1);
-        s += a[(long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1)];
+        const unsigned tt = (unsigned)t, q = tt / (unsigned)L.n0;
+        int i = (int)(tt - q * (unsigned)L.n0);
+        int j = (int)(q % (unsigned)L.n1), k = (int)(q / (unsigned)L.n1);
+        s += a[LIdx(L, i, j, k)];
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -235,8 +384,8 @@ __global__ void __launch_bounds__(256) k_partial_sum(const double* __restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ a, LevelGeom L,
-                                                  const double* __restrict__ part, int npart)
+__global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ a, LevelGeom L, const double* __restrict__ part,
+                                                  int npart)
 {
     __shared__ double mean;
     if (threadIdx.x == 0) {
@@ -247,30 +396,35 @@ __global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ a, LevelG
     __syncthreads();
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
-        int i = (int)(t % L.n0);
-        long long q = t / L.n0;
-        int j = (int)(q % L.n1), k = (int)(q / L.n1);
-        a[(long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1)] -= mean;
+        const unsigned tt = (unsigned)t, q = tt / (unsigned)L.n0;
+        int i = (int)(tt - q * (unsigned)L.n0);
+        int j = (int)(q % (unsigned)L.n1), k = (int)(q / (unsigned)L.n1);
+        a[LIdx(L, i, j, k)] -= mean;
     }
 }
 
 // ---------------------------------------------------------------------------
-// bottom solve: one workgroup runs the whole unpreconditioned BiCGStab on
-// the coarsest level (a few to a few thousand cells), then the smoothing
-// sweeps MLMG applies after it (nub after success, nuf after failure).
+// single-workgroup part: the coarse end of the V-cycle and the bottom solve
 // ---------------------------------------------------------------------------
-constexpr int kBotThreads = 256;
-
 struct BottomWork {
     double *r, *rh, *p, *v, *s, *t;
-    int* stats;  // [0] iterations (accumulated), [1] failures (accumulated)
+    double *x, *b;  // FP64 bottom unknown / rhs (FP32 V-cycles convert in and out)
+    int* stats;     // [0] BiCGStab iterations (accumulated), [1] failures
+};
+
+template <typename T>
+struct LevelPtrs {
+    LevelGeom G;
+    T* cor;
+    T* res;
+    int r0, r1, r2;  // ratio to the next coarser level
 };
 
 __device__ double blk_sum(double v, double* sh)
 {
     sh[threadIdx.x] = v;
     __syncthreads();
-    for (int o = kBotThreads / 2; o > 0; o >>= 1) {
+    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
@@ -282,7 +436,7 @@ __device__ double blk_max(double v, double* sh)
 {
     sh[threadIdx.x] = v;
     __syncthreads();
-    for (int o = kBotThreads / 2; o > 0; o >>= 1) {
+    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
         __syncthreads();
     }
@@ -291,77 +445,67 @@ __device__ double blk_max(double v, double* sh)
     return r;
 }
 
-__device__ void blk_ghosts(double* a, const LevelGeom& L)
+template <typename T>
+__device__ void blk_ghosts(T* a, const LevelGeom& L)
 {
-    const long long nx = (long long)L.n1 * L.n2, ny = (long long)L.n0 * L.n2, nz = (long long)L.n0 * L.n1;
-    const long long n = 2 * (nx + ny + nz);
-    for (long long t0 = threadIdx.x; t0 < n; t0 += kBotThreads) {
-        long long t = t0;
-        auto LI = [&](int i, int j, int k) { return (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1); };
-        if (t < 2 * nx) {
-            int side = (int)(t / nx);
-            long long r = t % nx;
-            int j = (int)(r % L.n1), k = (int)(r / L.n1);
-            double vl = a[LI(0, j, k)], vr = a[LI(L.n0 - 1, j, k)];
-            if (side == 0)
-                a[LI(-1, j, k)] = bc_ghost(L.lo[0], vl, vr, true);
-            else
-                a[LI(L.n0, j, k)] = bc_ghost(L.hi[0], vr, vl, false);
-            continue;
+    const long long n = ghost_count(L);
+    for (long long t = threadIdx.x; t < n; t += blockDim.x) ghost_point<T>(a, L, t);
+    __syncthreads();
+}
+
+#define BLK_FOR(L, t_, c)                                                                              \
+    for (long long t_ = threadIdx.x, ncell_ = (long long)(L).n0 * (L).n1 * (L).n2; t_ < ncell_;          \
+         t_ += blockDim.x)                                                                             \
+        for (long long c = LIdx((L), (int)(t_ % (L).n0), (int)((t_ / (L).n0) % (L).n1),                 \
+                                (int)((t_ / (L).n0) / (L).n1)),                                        \
+                       once_ = 0;                                                                      \
+             once_ < 1; ++once_)
+
+template <typename T>
+__device__ void blk_smooth(T* x, const T* b, const LevelGeom& L, int nu)
+{
+    const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, idg = (T)L.idg;
+    for (int sweep = 0; sweep < nu; ++sweep)
+        for (int color = 0; color < 2; ++color) {
+            blk_ghosts<T>(x, L);
+            BLK_FOR(L, t_, c)
+            {
+                long long q = t_ / L.n0;
+                int i = (int)(t_ % L.n0), j = (int)(q % L.n1), k = (int)(q / L.n1);
+                if (((i + j + k + color) & 1) == 0)
+                    x[c] = (cx * (x[c - 1] + x[c + 1]) + cy * (x[c - L.sy] + x[c + L.sy]) +
+                            cz * (x[c - L.sz] + x[c + L.sz]) - b[c]) *
+                           idg;
+            }
+            __syncthreads();
         }
-        t -= 2 * nx;
-        if (t < 2 * ny) {
-            int side = (int)(t / ny);
-            long long r = t % ny;
-            int i = (int)(r % L.n0), k = (int)(r / L.n0);
-            double vl = a[LI(i, 0, k)], vr = a[LI(i, L.n1 - 1, k)];
-            if (side == 0)
-                a[LI(i, -1, k)] = bc_ghost(L.lo[1], vl, vr, true);
-            else
-                a[LI(i, L.n1, k)] = bc_ghost(L.hi[1], vr, vl, false);
-            continue;
-        }
-        t -= 2 * ny;
-        int side = (int)(t / nz);
-        long long r = t % nz;
-        int i = (int)(r % L.n0), j = (int)(r / L.n0);
-        double vl = a[LI(i, j, 0)], vr = a[LI(i, j, L.n2 - 1)];
-        if (side == 0)
-            a[LI(i, j, -1)] = bc_ghost(L.lo[2], vl, vr, true);
-        else
-            a[LI(i, j, L.n2)] = bc_ghost(L.hi[2], vr, vl, false);
+}
+
+__device__ void blk_apply(double* x, double* y, const LevelGeom& L)
+{
+    blk_ghosts<double>(x, L);
+    BLK_FOR(L, t_, c)
+    {
+        y[c] = (L.cx * (x[c - 1] + x[c + 1]) + L.cy * (x[c - L.sy] + x[c + L.sy]) + L.cz * (x[c - L.sz] + x[c + L.sz])) -
+               L.dg * x[c];
     }
     __syncthreads();
 }
 
-#define BOT_FOR(c)                                                                            \
-    for (long long t_ = threadIdx.x; t_ < ncell; t_ += kBotThreads)                          \
-        for (long long c = (long long)((t_ % L.n0) + 1) + L.sy * ((t_ / L.n0) % L.n1 + 1) +   \
-                           L.sz * ((t_ / L.n0) / L.n1 + 1),                                  \
-                       once_ = 0;                                                            \
-             once_ < 1; ++once_)
-
-__device__ void blk_apply(double* x, double* y, const LevelGeom& L, long long ncell)
+// unpreconditioned BiCGStab (van der Vorst 1992) on the bottom level, then
+// nub (converged) or nuf (not converged) smoothing sweeps
+__device__ void bottom_solve(double* x, double* b, const LevelGeom& L, const BottomWork& w, int singular, double rtol,
+                             double atol, int maxiter, int nub, int nuf, double* sh)
 {
-    blk_ghosts(x, L);
-    BOT_FOR(c) { y[c] = lap_off<double>(x, c, L.sy, L.sz, L.cx, L.cy, L.cz) - L.dg * x[c]; }
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(kBotThreads) k_bottom(double* __restrict__ x, double* __restrict__ b, LevelGeom L,
-                                                        BottomWork w, int singular, double rtol, double atol,
-                                                        int maxiter, int nub, int nuf)
-{
-    __shared__ double sh[kBotThreads];
     const long long ncell = (long long)L.n0 * L.n1 * L.n2;
     if (singular) {
         double s = 0.0;
-        BOT_FOR(c) { s += b[c]; }
+        BLK_FOR(L, t_, c) { s += b[c]; }
         double mean = blk_sum(s, sh) / (double)ncell;
-        BOT_FOR(c) { b[c] -= mean; }
+        BLK_FOR(L, t_, c) { b[c] -= mean; }
         __syncthreads();
     }
-    for (long long c = threadIdx.x; c < L.tot; c += kBotThreads) {
+    for (long long c = threadIdx.x; c < L.tot; c += blockDim.x) {
         x[c] = 0.0;
         w.r[c] = b[c];
         w.rh[c] = b[c];
@@ -372,7 +516,7 @@ __global__ void __launch_bounds__(kBotThreads) k_bottom(double* __restrict__ x, 
     }
     __syncthreads();
     double m = 0.0;
-    BOT_FOR(c) { m = fmax(m, fabs(w.r[c])); }
+    BLK_FOR(L, t_, c) { m = fmax(m, fabs(w.r[c])); }
     const double rnorm0 = blk_max(m, sh);
     const double tol = fmax(rtol * rnorm0, atol);
     int ret = 1, it = 0;
@@ -382,44 +526,46 @@ __global__ void __launch_bounds__(kBotThreads) k_bottom(double* __restrict__ x, 
         double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
         for (it = 1; it <= maxiter; ++it) {
             double acc = 0.0;
-            BOT_FOR(c) { acc += w.rh[c] * w.r[c]; }
+            BLK_FOR(L, t_, c) { acc += w.rh[c] * w.r[c]; }
             double rho = blk_sum(acc, sh);
             if (rho == 0.0) { ret = 2; break; }
             if (it == 1) {
-                BOT_FOR(c) { w.p[c] = w.r[c]; }
+                BLK_FOR(L, t_, c) { w.p[c] = w.r[c]; }
             } else {
                 double beta = (rho / rho_prev) * (alpha / omega);
-                BOT_FOR(c) { w.p[c] = w.r[c] + beta * (w.p[c] - omega * w.v[c]); }
+                BLK_FOR(L, t_, c) { w.p[c] = w.r[c] + beta * (w.p[c] - omega * w.v[c]); }
             }
             __syncthreads();
-            blk_apply(w.p, w.v, L, ncell);
+            blk_apply(w.p, w.v, L);
             acc = 0.0;
-            BOT_FOR(c) { acc += w.rh[c] * w.v[c]; }
+            BLK_FOR(L, t_, c) { acc += w.rh[c] * w.v[c]; }
             double rv = blk_sum(acc, sh);
             if (rv == 0.0) { ret = 2; break; }
             alpha = rho / rv;
             m = 0.0;
-            BOT_FOR(c) {
+            BLK_FOR(L, t_, c)
+            {
                 w.s[c] = w.r[c] - alpha * w.v[c];
                 m = fmax(m, fabs(w.s[c]));
             }
             double sn = blk_max(m, sh);
             if (sn <= tol) {
-                BOT_FOR(c) { x[c] += alpha * w.p[c]; }
+                BLK_FOR(L, t_, c) { x[c] += alpha * w.p[c]; }
                 __syncthreads();
                 ret = 0;
                 break;
             }
-            blk_apply(w.s, w.t, L, ncell);
+            blk_apply(w.s, w.t, L);
             acc = 0.0;
-            BOT_FOR(c) { acc += w.t[c] * w.t[c]; }
+            BLK_FOR(L, t_, c) { acc += w.t[c] * w.t[c]; }
             double tt = blk_sum(acc, sh);
             if (tt == 0.0) { ret = 2; break; }
             acc = 0.0;
-            BOT_FOR(c) { acc += w.t[c] * w.s[c]; }
+            BLK_FOR(L, t_, c) { acc += w.t[c] * w.s[c]; }
             omega = blk_sum(acc, sh) / tt;
             m = 0.0;
-            BOT_FOR(c) {
+            BLK_FOR(L, t_, c)
+            {
                 x[c] += alpha * w.p[c] + omega * w.s[c];
                 w.r[c] = w.s[c] - omega * w.t[c];
                 m = fmax(m, fabs(w.r[c]));
@@ -432,49 +578,111 @@ __global__ void __launch_bounds__(kBotThreads) k_bottom(double* __restrict__ x, 
         if (it > maxiter) it = maxiter;
     }
     __syncthreads();
-    int nsm = (ret == 0) ? nub : nuf;
-    for (int sweep = 0; sweep < nsm; ++sweep)
-        for (int color = 0; color < 2; ++color) {
-            blk_ghosts(x, L);
-            BOT_FOR(c) {
-                long long q = t_ / L.n0;
-                int i = (int)(t_ % L.n0), j = (int)(q % L.n1), k = (int)(q / L.n1);
-                if (((i + j + k + color) & 1) == 0) x[c] = (lap_off<double>(x, c, L.sy, L.sz, L.cx, L.cy, L.cz) - b[c]) * L.idg;
-            }
-            __syncthreads();
-        }
+    blk_smooth<double>(x, b, L, (ret == 0) ? nub : nuf);
     if (threadIdx.x == 0) {
         w.stats[0] += it;
         if (ret != 0) w.stats[1] += 1;
+    }
+    __syncthreads();
+}
+
+// levels lc .. nlev-1 of a V-cycle: zero, pre-smooth, residual, restrict
+// down to the bottom, bottom solve (FP64), prolong, post-smooth back up
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev,
+                                                                  BottomWork w, int singular, double rtol, double atol,
+                                                                  int maxiter, int nub, int nuf, int nu1, int nu2)
+{
+    __shared__ double sh[kCoarseThreads];
+    for (int l = lc; l < nlev - 1; ++l) {
+        const LevelGeom F = lp[l].G;
+        const LevelGeom C = lp[l + 1].G;
+        T* x = lp[l].cor;
+        const T* b = lp[l].res;
+        for (long long c = threadIdx.x; c < F.tot; c += blockDim.x) x[c] = (T)0;
+        __syncthreads();
+        blk_smooth<T>(x, b, F, nu1);
+        blk_ghosts<T>(x, F);
+        const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
+        const T wgt = (T)(1.0 / (double)(r0 * r1 * r2));
+        const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
+        T* cres = lp[l + 1].res;
+        BLK_FOR(C, t_, cc)
+        {
+            long long q = t_ / C.n0;
+            int I = (int)(t_ % C.n0), J = (int)(q % C.n1), K = (int)(q / C.n1);
+            T s = (T)0;
+            for (int dk = 0; dk < r2; ++dk)
+                for (int dj = 0; dj < r1; ++dj)
+                    for (int di = 0; di < r0; ++di) {
+                        long long c = LIdx(F, I * r0 + di, J * r1 + dj, K * r2 + dk);
+                        T v = b[c] - ((cx * (x[c - 1] + x[c + 1]) + cy * (x[c - F.sy] + x[c + F.sy]) +
+                                       cz * (x[c - F.sz] + x[c + F.sz])) -
+                                      dg * x[c]);
+                        s += v;
+                    }
+            cres[cc] = s * wgt;
+        }
+        __syncthreads();
+    }
+    // bottom
+    {
+        const LevelGeom Lb = lp[nlev - 1].G;
+        T* xb = lp[nlev - 1].cor;
+        T* bb = lp[nlev - 1].res;
+        double* x64 = (sizeof(T) == sizeof(double)) ? (double*)xb : w.x;
+        double* b64 = (sizeof(T) == sizeof(double)) ? (double*)bb : w.b;
+        if (sizeof(T) != sizeof(double)) {
+            for (long long c = threadIdx.x; c < Lb.tot; c += blockDim.x) b64[c] = (double)bb[c];
+            __syncthreads();
+        }
+        bottom_solve(x64, b64, Lb, w, singular, rtol, atol, maxiter, nub, nuf, sh);
+        if (sizeof(T) != sizeof(double)) {
+            for (long long c = threadIdx.x; c < Lb.tot; c += blockDim.x) xb[c] = (T)x64[c];
+            __syncthreads();
+        }
+    }
+    for (int l = nlev - 2; l >= lc; --l) {
+        const LevelGeom F = lp[l].G;
+        const LevelGeom C = lp[l + 1].G;
+        T* x = lp[l].cor;
+        const T* xc = lp[l + 1].cor;
+        const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
+        BLK_FOR(F, t_, c)
+        {
+            long long q = t_ / F.n0;
+            int i = (int)(t_ % F.n0), j = (int)(q % F.n1), k = (int)(q / F.n1);
+            x[c] += xc[LIdx(C, i / r0, j / r1, k / r2)];
+        }
+        __syncthreads();
+        blk_smooth<T>(x, lp[l].res, F, nu2);
     }
 }
 
 // ---------------------------------------------------------------------------
 // MAC-specific gathers / scatters between box FABs and the level-0 array
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_mac_rhs(CMFDev u, CMFDev v, CMFDev w, double* __restrict__ rhs,
-                                                 LevelGeom L, RegDev r, const int* __restrict__ box, int dlo0,
-                                                 int dlo1, int dlo2, double dxi, double dyi, double dzi)
+__global__ void __launch_bounds__(256) k_mac_rhs(CMFDev u, CMFDev v, CMFDev w, double* __restrict__ rhs, LevelGeom L,
+                                                 RegDev r, const int* __restrict__ box, int dlo0, int dlo1, int dlo2,
+                                                 double dxi, double dyi, double dzi)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     CFab fu = fab_at(u, box, b), fv = fab_at(v, box, b), fw = fab_at(w, box, b);
-    rhs[(long long)(i - dlo0 + 1) + L.sy * (j - dlo1 + 1) + L.sz * (k - dlo2 + 1)] =
-        (fu(i + 1, j, k) - fu(i, j, k)) * dxi + (fv(i, j + 1, k) - fv(i, j, k)) * dyi +
-        (fw(i, j, k + 1) - fw(i, j, k)) * dzi;
+    rhs[LIdx(L, i - dlo0, j - dlo1, k - dlo2)] = (fu(i + 1, j, k) - fu(i, j, k)) * dxi +
+                                                 (fv(i, j + 1, k) - fv(i, j, k)) * dyi +
+                                                 (fw(i, j, k + 1) - fw(i, j, k)) * dzi;
 }
 
 template <int D>
-__global__ void __launch_bounds__(256) k_mac_correct(MFDev u, const double* __restrict__ phi, LevelGeom L,
-                                                     RegDev r, const int* __restrict__ box, int dlo0, int dlo1,
-                                                     int dlo2, double bd)
+__global__ void __launch_bounds__(256) k_mac_correct(MFDev u, const double* __restrict__ phi, LevelGeom L, RegDev r,
+                                                     const int* __restrict__ box, int dlo0, int dlo1, int dlo2,
+                                                     double bd)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     Fab fu = fab_at(u, box, b);
-    long long c = (long long)(i - dlo0 + 1) + L.sy * (j - dlo1 + 1) + L.sz * (k - dlo2 + 1);
+    long long c = LIdx(L, i - dlo0, j - dlo1, k - dlo2);
     const long long sd = (D == 0) ? 1 : (D == 1 ? L.sy : L.sz);
     double pr = phi[c], pl = phi[c - sd];
     fu(i, j, k) = fu(i, j, k) - bd * (pr - pl);
@@ -486,10 +694,10 @@ __global__ void __launch_bounds__(256) k_pack(const double* __restrict__ a, doub
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
     if (t >= tot) return;
-    int i = (int)(t % L.n0);
-    long long q = t / L.n0;
-    int j = (int)(q % L.n1), k = (int)(q / L.n1);
-    long long c = (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
+    const unsigned tt = (unsigned)t, q = tt / (unsigned)L.n0;
+    int i = (int)(tt - q * (unsigned)L.n0);
+    int j = (int)(q % (unsigned)L.n1), k = (int)(q / (unsigned)L.n1);
+    long long c = LIdx(L, i, j, k);
     if (unpack)
         out[c] = a[t];
     else
@@ -507,11 +715,15 @@ struct ksgf_macproj {
     ksgf_mlmg_options opt;
     int singular = 1;
     int nlev = 0;
+    int lc = 0;  // first level handled by the single-workgroup coarse kernel
     std::vector<LevelGeom> G;
-    std::vector<int> ratio;  // 3 per level (to next)
-    std::vector<double*> cor, res, tmp;  // FP64 per level
-    std::vector<float*> cor32, res32, tmp32;  // FP32 per level (mixed)
+    std::vector<int> ratio;               // 3 per level (to the next)
+    std::vector<double*> cor, res;        // FP64 per level (FP64 mode)
+    std::vector<float*> cor32, res32;     // FP32 per level (mixed mode)
+    LevelPtrs<double>* dlp64 = nullptr;   // device level tables for the coarse kernel
+    LevelPtrs<float>* dlp32 = nullptr;
     double* phi = nullptr;
+    double* phi2 = nullptr;  // ping-pong partner of phi
     double* rhs = nullptr;
     double* part = nullptr;
     unsigned long long* dnorm = nullptr;
@@ -522,20 +734,15 @@ struct ksgf_macproj {
 
 namespace ksgf {
 
-static void set_level_coefs(ksgf_macproj* mp)
+static int mp_alloc(void** p, size_t bytes)
 {
-    for (int l = 0; l < mp->nlev; ++l) {
-        LevelGeom& L = mp->G[l];
-        // h is stored implicitly: recompute from the domain and ratios
-        (void)L;
-    }
-}
-
-static int mp_alloc(double** p, long long n)
-{
-    if (hipMalloc(p, sizeof(double) * (size_t)n) != hipSuccess) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
         set_error("hipMalloc multigrid level");
         return KSGF_ERR_NOMEM;
+    }
+    if (hipMemset(*p, 0, bytes) != hipSuccess) {
+        set_error("hipMemset multigrid level");
+        return KSGF_ERR_HIP;
     }
     return KSGF_OK;
 }
@@ -558,18 +765,33 @@ static std::vector<std::array<double, 3>> level_h(const ksgf_macproj* mp)
     return h;
 }
 
+static int upload_level_tables(ksgf_macproj* mp)
+{
+    if (mp->opt.precision == KSGF_MG_MIXED) {
+        std::vector<LevelPtrs<float>> t(mp->nlev);
+        for (int l = 0; l < mp->nlev; ++l)
+            t[l] = {mp->G[l], mp->cor32[l], mp->res32[l], mp->ratio[3 * l], mp->ratio[3 * l + 1], mp->ratio[3 * l + 2]};
+        KSGF_HIP_CHECK(hipMemcpy(mp->dlp32, t.data(), sizeof(LevelPtrs<float>) * t.size(), hipMemcpyHostToDevice));
+    } else {
+        std::vector<LevelPtrs<double>> t(mp->nlev);
+        for (int l = 0; l < mp->nlev; ++l)
+            t[l] = {mp->G[l], mp->cor[l], mp->res[l], mp->ratio[3 * l], mp->ratio[3 * l + 1], mp->ratio[3 * l + 2]};
+        KSGF_HIP_CHECK(hipMemcpy(mp->dlp64, t.data(), sizeof(LevelPtrs<double>) * t.size(), hipMemcpyHostToDevice));
+    }
+    return KSGF_OK;
+}
+
 template <typename T>
 static void launch_ghosts(T* a, const LevelGeom& L, hipStream_t st)
 {
-    long long n = 2 * ((long long)L.n1 * L.n2 + (long long)L.n0 * L.n2 + (long long)L.n0 * L.n1);
-    hipLaunchKernelGGL(k_mg_ghosts<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, a, L);
+    hipLaunchKernelGGL(k_mg_ghosts<T>, dim3(grid_for(ghost_count(L), 256)), dim3(256), 0, st, a, L);
 }
 
 // ---------------------------------------------------------------------------
 // live kernel timing for the bench: HIP events around every launch of the
-// finest-level smoother on the launch stream, with its algorithmic bytes
-// (a red or black half sweep counts half of the 24 B/cell FP64 (12 B/cell
-// FP32) of a full read-phi/read-rhs/write-phi sweep)
+// finest-level half-sweep smoother on the launch stream, with its
+// algorithmic bytes (a red or black half sweep counts half of the 24 B/cell
+// FP64 (12 B/cell FP32) of a full read-phi/read-rhs/write-phi sweep)
 // ---------------------------------------------------------------------------
 struct KernelProf {
     bool on = false;
@@ -591,37 +813,34 @@ static hipEvent_t prof_event()
     return g_prof.ev[g_prof.used++];
 }
 
+// half sweeps number `first` .. 2*nu-1 of nu red-black sweeps (half sweep
+// 2s is red, 2s+1 black)
 template <typename T>
-static void launch_smooth(T* x, const T* b, const LevelGeom& L, int nu, hipStream_t st, bool prof = false)
+static void launch_half_sweeps(T* x, const T* b, const LevelGeom& L, int first, int nu, hipStream_t st, bool prof,
+                               const T* xc = nullptr, const LevelGeom* C = nullptr, const int* r = nullptr)
 {
     long long n = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
-    for (int s = 0; s < nu; ++s)
-        for (int color = 0; color < 2; ++color) {
-            launch_ghosts<T>(x, L, st);
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (prof && g_prof.on) {
-                e0 = prof_event();
-                e1 = prof_event();
-                if (e0 && e1) (void)hipEventRecord(e0, st);
-            }
-            hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, x, b, L, color);
-            if (e0 && e1) {
-                (void)hipEventRecord(e1, st);
-                g_prof.launches += 1;
-                g_prof.bytes += 0.5 * 3.0 * sizeof(T) * (double)L.n0 * L.n1 * L.n2;
-                g_prof.name = sizeof(T) == 8 ? "k_mg_smooth<double>(level 0)" : "k_mg_smooth<float>(level 0)";
-            }
+    for (int h = first; h < 2 * nu; ++h) {
+        const T* xch = (xc && h == first) ? xc : nullptr;
+        const LevelGeom Cg = C ? *C : L;
+        const int r0 = r ? r[0] : 1, r1 = r ? r[1] : 1, r2 = r ? r[2] : 1;
+        int color = h & 1;
+        if (L.ghost_mode) launch_ghosts<T>(x, L, st);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof && g_prof.on) {
+            e0 = prof_event();
+            e1 = prof_event();
+            if (e0 && e1) (void)hipEventRecord(e0, st);
         }
-}
-
-template <typename T, typename R>
-static void launch_residual(T* x, const T* b, R* r, const LevelGeom& L, unsigned long long* norm, hipStream_t st)
-{
-    launch_ghosts<T>(x, L, st);
-    long long n = (long long)L.n0 * L.n1 * L.n2;
-    int nb = grid_for(n, 256);
-    if (nb > kReduceBlocks) nb = kReduceBlocks;
-    hipLaunchKernelGGL((k_mg_residual<T, R>), dim3(nb), dim3(256), 0, st, (const T*)x, b, r, L, norm);
+        hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, x, b, L, color, xch, Cg, r0, r1,
+                           r2);
+        if (e0 && e1) {
+            (void)hipEventRecord(e1, st);
+            g_prof.launches += 1;
+            g_prof.bytes += 0.5 * 3.0 * sizeof(T) * (double)L.n0 * L.n1 * L.n2;
+            g_prof.name = sizeof(T) == 8 ? "k_mg_smooth<double>(level 0)" : "k_mg_smooth<float>(level 0)";
+        }
+    }
 }
 
 template <typename T>
@@ -636,46 +855,47 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
     hipLaunchKernelGGL(k_sub_mean, dim3(kSumBlocks), dim3(256), 0, st, a, L, (const double*)part, kSumBlocks);
 }
 
-// V-cycle in precision T on levels l..nlev-1; the bottom solve is FP64.
+// V-cycle in precision T from level l down (levels >= lc in one workgroup)
 template <typename T>
-static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& res, std::vector<T*>& tmp,
-                   hipStream_t st)
+static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& res,
+                   const LevelPtrs<T>* dlp, hipStream_t st)
 {
-    const LevelGeom& L = mp->G[l];
-    launch_fill<T>(cor[l], L.tot, (T)0, st);
-    if (l == mp->nlev - 1) {
-        double* x64;
-        double* b64;
-        if (sizeof(T) == sizeof(double)) {
-            x64 = (double*)cor[l];
-            b64 = (double*)res[l];
-        } else {
-            x64 = mp->cor[l];
-            b64 = mp->res[l];
-            hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_for(L.tot, 256)), dim3(256), 0, st, b64,
-                               (const T*)res[l], L.tot);
-        }
-        hipLaunchKernelGGL(k_bottom, dim3(1), dim3(kBotThreads), 0, st, x64, b64, L, mp->bw, mp->singular,
-                           mp->opt.bottom_rtol, mp->opt.bottom_atol, mp->opt.bottom_maxiter,
-                           mp->opt.num_bottom_smooth, mp->opt.num_final_smooth);
-        if (sizeof(T) != sizeof(double))
-            hipLaunchKernelGGL((k_convert<T, double>), dim3(grid_for(L.tot, 256)), dim3(256), 0, st, cor[l],
-                               (const double*)x64, L.tot);
+    if (l >= mp->lc) {
+        hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(kCoarseThreads), 0, st, dlp, l, mp->nlev, mp->bw,
+                           mp->singular, mp->opt.bottom_rtol, mp->opt.bottom_atol, mp->opt.bottom_maxiter,
+                           mp->opt.num_bottom_smooth, mp->opt.num_final_smooth, mp->opt.num_pre_smooth,
+                           mp->opt.num_post_smooth);
         return;
     }
+    const LevelGeom& L = mp->G[l];
     const LevelGeom& C = mp->G[l + 1];
     const int* r = &mp->ratio[3 * l];
-    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_pre_smooth, st, l == 0);
-    launch_residual<T, T>(cor[l], res[l], tmp[l], L, nullptr, st);
-    long long nc = (long long)C.n0 * C.n1 * C.n2;
-    T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
-    hipLaunchKernelGGL(k_mg_restrict<T>, dim3(grid_for(nc, 256)), dim3(256), 0, st, (const T*)tmp[l], res[l + 1], L,
-                       C, r[0], r[1], r[2], w);
-    vcycle<T>(mp, l + 1, cor, res, tmp, st);
-    long long nf = (long long)L.n0 * L.n1 * L.n2;
-    hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
-                       L, C, r[0], r[1], r[2]);
-    launch_smooth<T>(cor[l], res[l], L, mp->opt.num_post_smooth, st, l == 0);
+    const bool prof = (l == 0);
+    const long long nf = (long long)L.n0 * L.n1 * L.n2;
+    const long long nc = (long long)C.n0 * C.n1 * C.n2;
+    const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+    // down
+    if (mp->opt.num_pre_smooth > 0 && !L.ghost_mode) {
+        hipLaunchKernelGGL(k_mg_smooth_first<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)res[l], L);
+        launch_half_sweeps<T>(cor[l], res[l], L, 1, mp->opt.num_pre_smooth, st, prof);
+    } else {
+        launch_fill<T>(cor[l], L.tot, (T)0, st);
+        launch_half_sweeps<T>(cor[l], res[l], L, 0, mp->opt.num_pre_smooth, st, prof);
+    }
+    if (L.ghost_mode) launch_ghosts<T>(cor[l], L, st);
+    hipLaunchKernelGGL(k_mg_resid_restrict<T>, dim3(grid_for(nc, 256)), dim3(256), 0, st, (const T*)cor[l],
+                       (const T*)res[l], res[l + 1], L, C, r[0], r[1], r[2], w);
+    vcycle<T>(mp, l + 1, cor, res, dlp, st);
+    // up
+    if (mp->opt.num_post_smooth > 0 && !L.ghost_mode) {
+        hipLaunchKernelGGL(k_mg_prolong_red<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
+                           (const T*)res[l], L, C, r[0], r[1], r[2]);
+        launch_half_sweeps<T>(cor[l], res[l], L, 1, mp->opt.num_post_smooth, st, prof, (const T*)cor[l + 1], &C, r);
+    } else {
+        hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
+                           L, C, r[0], r[1], r[2]);
+        launch_half_sweeps<T>(cor[l], res[l], L, 0, mp->opt.num_post_smooth, st, prof);
+    }
 }
 
 static int read_norm(ksgf_macproj* mp, hipStream_t st, double* out)
@@ -689,18 +909,46 @@ static int read_norm(ksgf_macproj* mp, hipStream_t st, double* out)
     return KSGF_OK;
 }
 
-// solve with level-0 rhs already in mp->rhs (ghost layout); phi in mp->phi
+template <typename C, typename R>
+static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStream_t st)
+{
+    const LevelGeom& L0 = mp->G[0];
+    long long n = (long long)L0.n0 * L0.n1 * L0.n2;
+    int nb = grid_for(n, 256);
+    if (nb > kReduceBlocks) nb = kReduceBlocks;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, mp->dnorm, 1);
+    if (L0.ghost_mode) {
+        // explicit ghosts: phi += cor first, then the residual from ghosts
+        if (cor) {
+            hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
+                               mp->phi2, (const double*)mp->rhs, r, L0, (unsigned long long*)nullptr);
+            std::swap(mp->phi, mp->phi2);
+        }
+        launch_ghosts<double>(mp->phi, L0, st);
+        hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
+                           (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, L0, mp->dnorm);
+        return;
+    }
+    hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor, mp->phi2,
+                       (const double*)mp->rhs, r, L0, mp->dnorm);
+    if (cor) std::swap(mp->phi, mp->phi2);
+}
+
+// solve with the level-0 rhs in mp->rhs (ghost layout); phi in mp->phi
 static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* info, hipStream_t st)
 {
     const LevelGeom& L0 = mp->G[0];
+    const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
     launch_fill<double>(mp->phi, L0.tot, 0.0, st);
     if (mp->singular) launch_sub_mean(mp->rhs, L0, mp->part, st);
-    // |rhs|_inf
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, mp->dnorm, 1);
-    launch_residual<double, double>(mp->phi, mp->rhs, mp->res[0], L0, mp->dnorm, st);
+    KSGF_HIP_CHECK(hipMemsetAsync(mp->bw.stats, 0, 2 * sizeof(int), st));
+    // |rhs|_inf and the first residual (= rhs)
+    if (mixed)
+        launch_update_residual<float, float>(mp, nullptr, mp->res32[0], st);
+    else
+        launch_update_residual<double, double>(mp, nullptr, mp->res[0], st);
     KSGF_LAUNCH_CHECK();
     double rhsnorm;
-    KSGF_HIP_CHECK(hipMemsetAsync(mp->bw.stats, 0, 2 * sizeof(int), st));
     int e = read_norm(mp, st, &rhsnorm);
     if (e) return e;
     const double target = std::fmax(atol, rtol * rhsnorm);
@@ -709,23 +957,15 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
     inf.r0 = r;
     inf.nlevels = mp->nlev;
     inf.converged = (r <= target);
-    const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
     if (!inf.converged) {
         for (int it = 1; it <= mp->opt.maxiter; ++it) {
             if (mixed) {
-                // res32 = (float) res64 (the FP64 residual of the FP64 solution)
-                hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(L0.tot, 256)), dim3(256), 0, st,
-                                   mp->res32[0], (const double*)mp->res[0], L0.tot);
-                vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->tmp32, st);
-                hipLaunchKernelGGL((k_add_to<double, float>), dim3(grid_for(L0.tot, 256)), dim3(256), 0, st, mp->phi,
-                                   (const float*)mp->cor32[0], L0.tot);
+                vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->dlp32, st);
+                launch_update_residual<float, float>(mp, mp->cor32[0], mp->res32[0], st);
             } else {
-                vcycle<double>(mp, 0, mp->cor, mp->res, mp->tmp, st);
-                hipLaunchKernelGGL((k_add_to<double, double>), dim3(grid_for(L0.tot, 256)), dim3(256), 0, st, mp->phi,
-                                   (const double*)mp->cor[0], L0.tot);
+                vcycle<double>(mp, 0, mp->cor, mp->res, mp->dlp64, st);
+                launch_update_residual<double, double>(mp, mp->cor[0], mp->res[0], st);
             }
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, mp->dnorm, 1);
-            launch_residual<double, double>(mp->phi, mp->rhs, mp->res[0], L0, mp->dnorm, st);
             KSGF_LAUNCH_CHECK();
             e = read_norm(mp, st, &r);
             if (e) return e;
@@ -802,14 +1042,15 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     if (!mp) return KSGF_OK;
     for (auto* p : mp->cor) if (p) (void)hipFree(p);
     for (auto* p : mp->res) if (p) (void)hipFree(p);
-    for (auto* p : mp->tmp) if (p) (void)hipFree(p);
     for (auto* p : mp->cor32) if (p) (void)hipFree(p);
     for (auto* p : mp->res32) if (p) (void)hipFree(p);
-    for (auto* p : mp->tmp32) if (p) (void)hipFree(p);
-    double* bws[6] = {mp->bw.r, mp->bw.rh, mp->bw.p, mp->bw.v, mp->bw.s, mp->bw.t};
+    double* bws[8] = {mp->bw.r, mp->bw.rh, mp->bw.p, mp->bw.v, mp->bw.s, mp->bw.t, mp->bw.x, mp->bw.b};
     for (auto* p : bws) if (p) (void)hipFree(p);
     if (mp->bw.stats) (void)hipFree(mp->bw.stats);
+    if (mp->dlp64) (void)hipFree(mp->dlp64);
+    if (mp->dlp32) (void)hipFree(mp->dlp32);
     if (mp->phi) (void)hipFree(mp->phi);
+    if (mp->phi2) (void)hipFree(mp->phi2);
     if (mp->rhs) (void)hipFree(mp->rhs);
     if (mp->part) (void)hipFree(mp->part);
     if (mp->dnorm) (void)hipFree(mp->dnorm);
@@ -836,6 +1077,11 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     if (mp->opt.precision != KSGF_MG_FP64 && mp->opt.precision != KSGF_MG_MIXED) {
         delete mp;
         set_error("ksgf_macproj_create: bad precision");
+        return KSGF_ERR_INVALID;
+    }
+    if (mp->opt.num_pre_smooth < 0 || mp->opt.num_post_smooth < 0 || mp->opt.maxiter < 0) {
+        delete mp;
+        set_error("ksgf_macproj_create: negative smoothing/iteration count");
         return KSGF_ERR_INVALID;
     }
     mp->singular = 1;
@@ -871,9 +1117,11 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         L.sy = cn[0] + 2;
         L.sz = (long long)(cn[0] + 2) * (cn[1] + 2);
         L.tot = L.sz * (cn[2] + 2);
+        L.ghost_mode = 0;
         for (int d = 0; d < 3; ++d) {
             L.lo[d] = lobc[d];
             L.hi[d] = hibc[d];
+            if (lobc[d] == KSGF_LO_PERIODIC && (cn[d] % 2) == 1) L.ghost_mode = 1;
         }
         fill_coefs(L, ch, beta);
         mp->G.push_back(L);
@@ -907,6 +1155,10 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
             ch[d] *= r[d];
         }
     }
+    // first level of the single-workgroup coarse part (the bottom at least)
+    mp->lc = mp->nlev - 1;
+    while (mp->lc > 0 && (long long)mp->G[mp->lc - 1].n0 * mp->G[mp->lc - 1].n1 * mp->G[mp->lc - 1].n2 <= kCoarseCells)
+        mp->lc--;
     const LevelGeom& Lb = mp->G[mp->nlev - 1];
     if ((long long)Lb.n0 * Lb.n1 * Lb.n2 > (1LL << 22)) {
         ksgf_macproj_destroy(mp);
@@ -914,64 +1166,70 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         return KSGF_ERR_UNSUPPORTED;
     }
     int e;
+    const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
     for (int l = 0; l < mp->nlev; ++l) {
-        double *a, *b, *c;
-        if ((e = mp_alloc(&a, mp->G[l].tot)) || (e = mp_alloc(&b, mp->G[l].tot)) || (e = mp_alloc(&c, mp->G[l].tot))) {
+        size_t n = (size_t)mp->G[l].tot;
+        void *a, *b;
+        if (mixed) {
+            if ((e = mp_alloc(&a, n * sizeof(float))) || (e = mp_alloc(&b, n * sizeof(float)))) {
+                ksgf_macproj_destroy(mp);
+                return e;
+            }
+            mp->cor32.push_back((float*)a);
+            mp->res32.push_back((float*)b);
+        } else {
+            if ((e = mp_alloc(&a, n * sizeof(double))) || (e = mp_alloc(&b, n * sizeof(double)))) {
+                ksgf_macproj_destroy(mp);
+                return e;
+            }
+            mp->cor.push_back((double*)a);
+            mp->res.push_back((double*)b);
+        }
+    }
+    void* p;
+    const size_t n0 = (size_t)mp->G[0].tot;
+    if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->phi = (double*)p;
+    if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->phi2 = (double*)p;
+    if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->rhs = (double*)p;
+    if ((e = mp_alloc(&p, kSumBlocks * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->part = (double*)p;
+    double** bws[8] = {&mp->bw.r, &mp->bw.rh, &mp->bw.p, &mp->bw.v, &mp->bw.s, &mp->bw.t, &mp->bw.x, &mp->bw.b};
+    for (auto pp : bws) {
+        if ((e = mp_alloc(&p, (size_t)Lb.tot * sizeof(double)))) {
             ksgf_macproj_destroy(mp);
             return e;
         }
-        mp->cor.push_back(a);
-        mp->res.push_back(b);
-        mp->tmp.push_back(c);
-        (void)hipMemset(a, 0, sizeof(double) * mp->G[l].tot);
-        (void)hipMemset(b, 0, sizeof(double) * mp->G[l].tot);
-        (void)hipMemset(c, 0, sizeof(double) * mp->G[l].tot);
-        if (mp->opt.precision == KSGF_MG_MIXED) {
-            float *fa, *fb, *fc;
-            if (hipMalloc(&fa, sizeof(float) * mp->G[l].tot) != hipSuccess ||
-                hipMalloc(&fb, sizeof(float) * mp->G[l].tot) != hipSuccess ||
-                hipMalloc(&fc, sizeof(float) * mp->G[l].tot) != hipSuccess) {
-                ksgf_macproj_destroy(mp);
-                set_error("hipMalloc FP32 level");
-                return KSGF_ERR_NOMEM;
-            }
-            (void)hipMemset(fa, 0, sizeof(float) * mp->G[l].tot);
-            (void)hipMemset(fb, 0, sizeof(float) * mp->G[l].tot);
-            (void)hipMemset(fc, 0, sizeof(float) * mp->G[l].tot);
-            mp->cor32.push_back(fa);
-            mp->res32.push_back(fb);
-            mp->tmp32.push_back(fc);
-        }
+        *pp = (double*)p;
     }
-    if ((e = mp_alloc(&mp->phi, mp->G[0].tot)) || (e = mp_alloc(&mp->rhs, mp->G[0].tot)) ||
-        (e = mp_alloc(&mp->part, kSumBlocks))) {
+    if ((e = mp_alloc(&p, 2 * sizeof(int)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->bw.stats = (int*)p;
+    if ((e = mp_alloc(&p, sizeof(unsigned long long)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->dnorm = (unsigned long long*)p;
+    if (mixed) {
+        if ((e = mp_alloc(&p, sizeof(LevelPtrs<float>) * mp->nlev))) { ksgf_macproj_destroy(mp); return e; }
+        mp->dlp32 = (LevelPtrs<float>*)p;
+    } else {
+        if ((e = mp_alloc(&p, sizeof(LevelPtrs<double>) * mp->nlev))) { ksgf_macproj_destroy(mp); return e; }
+        mp->dlp64 = (LevelPtrs<double>*)p;
+    }
+    if (hipHostMalloc(&mp->hnorm, 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&mp->hstats, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        ksgf_macproj_destroy(mp);
+        set_error("ksgf_macproj_create: pinned scalars");
+        return KSGF_ERR_NOMEM;
+    }
+    if ((e = upload_level_tables(mp))) {
         ksgf_macproj_destroy(mp);
         return e;
     }
-    (void)hipMemset(mp->phi, 0, sizeof(double) * mp->G[0].tot);
-    (void)hipMemset(mp->rhs, 0, sizeof(double) * mp->G[0].tot);
-    double** bws[6] = {&mp->bw.r, &mp->bw.rh, &mp->bw.p, &mp->bw.v, &mp->bw.s, &mp->bw.t};
-    for (auto pp : bws) {
-        if ((e = mp_alloc(pp, Lb.tot))) {
-            ksgf_macproj_destroy(mp);
-            return e;
-        }
-        (void)hipMemset(*pp, 0, sizeof(double) * Lb.tot);
-    }
-    if (hipMalloc(&mp->bw.stats, 2 * sizeof(int)) != hipSuccess || hipMalloc(&mp->dnorm, 8) != hipSuccess ||
-        hipHostMalloc(&mp->hnorm, 8, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&mp->hstats, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-        ksgf_macproj_destroy(mp);
-        set_error("ksgf_macproj_create: scalar buffers");
-        return KSGF_ERR_NOMEM;
-    }
-    (void)hipMemset(mp->bw.stats, 0, 2 * sizeof(int));
     if (hipDeviceSynchronize() != hipSuccess) {
         ksgf_macproj_destroy(mp);
         set_error("ksgf_macproj_create: device synchronize");
         return KSGF_ERR_HIP;
     }
-    set_level_coefs(mp);
     *out = mp;
     return KSGF_OK;
 }
@@ -985,7 +1243,7 @@ int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta)
     mp->beta = beta;
     auto h = level_h(mp);
     for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta);
-    return KSGF_OK;
+    return upload_level_tables(mp);
 }
 
 int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac, double rtol, double atol,
@@ -1011,7 +1269,7 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
     const Region* rv = get_region(ba, z, z, &err);
     if (!rv) return err;
     const LevelGeom& L0 = mp->G[0];
-    hipLaunchKernelGGL(k_mac_rhs, dim3(grid_for(rv->total, 256)), dim3(256), 0, st, cu, cv, cw, mp->rhs, L0,
+    hipLaunchKernelGGL(k_mac_rhs, dim3(rv->nchunks), dim3(256), 0, st, cu, cv, cw, mp->rhs, L0,
                        regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], 1.0 / ba->dx[0],
                        1.0 / ba->dx[1], 1.0 / ba->dx[2]);
     KSGF_LAUNCH_CHECK();
@@ -1023,7 +1281,7 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
         const Region* rf = get_region(ba, glo, ghi, &err);
         if (!rf) return err;
         double bd = mp->beta * (1.0 / ba->dx[d]);
-        dim3 gr(grid_for(rf->total, 256));
+        dim3 gr(rf->nchunks);
         if (d == 0)
             hipLaunchKernelGGL(k_mac_correct<0>, gr, dim3(256), 0, st, mu[0], (const double*)mp->phi, L0,
                                regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
@@ -1038,8 +1296,8 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
     return ret;
 }
 
-int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol, double atol,
-                       ksgf_mlmg_info* info, ksgf_stream_t stream)
+int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol, double atol, ksgf_mlmg_info* info,
+                       ksgf_stream_t stream)
 {
     if (!mp || !rhs || !phi) {
         set_error("ksgf_macproj_solve: null argument");

@@ -16,7 +16,7 @@ __global__ void __launch_bounds__(256) k_cfl(CMFDev v, RegDev r, const int* __re
     double m = 0.0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < r.total; t += (long long)gridDim.x * blockDim.x) {
         int b, i, j, k;
-        region_locate(r, box, t, b, i, j, k);
+        region_locate_flat(r, box, t, b, i, j, k);
         CFab f = fab_at(v, box, b);
         double a = fabs(f(i, j, k, 0)) * dxi;
         double bb = fabs(f(i, j, k, 1)) * dyi;
@@ -37,9 +37,8 @@ __global__ void __launch_bounds__(256) k_predictor(MFDev vn, CMFDev vo, CMFDev r
                                                    CMFDev src, CMFDev diff, int has_diff, double kappa, double dt,
                                                    int ncomp, RegDev r, const int* __restrict__ box)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     Fab fn = fab_at(vn, box, b);
     CFab fo = fab_at(vo, box, b), fro = fab_at(ro, box, b), frn = fab_at(rn, box, b);
     CFab fc = fab_at(conv, box, b), fs = fab_at(src, box, b);
@@ -58,9 +57,8 @@ __global__ void __launch_bounds__(256) k_predictor(MFDev vn, CMFDev vo, CMFDev r
 __global__ void __launch_bounds__(256) k_src_term(MFDev src, CMFDev gp, CMFDev rho, int scale, RegDev r,
                                                   const int* __restrict__ box)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     Fab fs = fab_at(src, box, b);
     CFab fg = fab_at(gp, box, b), fr = fab_at(rho, box, b);
     double rv = fr(i, j, k);
@@ -75,9 +73,8 @@ __global__ void __launch_bounds__(256) k_src_term(MFDev src, CMFDev gp, CMFDev r
 __global__ void __launch_bounds__(256) k_scale_rho(MFDev out, CMFDev a, CMFDev rho, int ncomp, int divide, RegDev r,
                                                    const int* __restrict__ box)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int b, i, j, k;
-    if (!region_locate(r, box, t, b, i, j, k)) return;
+    if (!region_locate(r, box, b, i, j, k)) return;
     Fab fo = fab_at(out, box, b);
     CFab fa = fab_at(a, box, b), fr = fab_at(rho, box, b);
     double rv = fr(i, j, k);
@@ -152,7 +149,7 @@ int ksgf_predictor_update(const ksgf_boxarray* ba, double* vel_new, int ng_new, 
     int z[3] = {0, 0, 0};
     const Region* r = get_region(ba, z, z, &err);
     if (!r) return err;
-    hipLaunchKernelGGL(k_predictor, dim3(grid_for(r->total, 256)), dim3(256), 0, (hipStream_t)stream, vn, vo, ro, rn,
+    hipLaunchKernelGGL(k_predictor, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, vn, vo, ro, rn,
                        cv, sv, dv, diff ? 1 : 0, kappa, dt, ncomp, regdev(r, ba->nbox), ba->dbox);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
@@ -175,7 +172,7 @@ int ksgf_icns_src_term(const ksgf_boxarray* ba, double* src, int ng_src, const d
     int z[3] = {0, 0, 0};
     const Region* r = get_region(ba, z, z, &err);
     if (!r) return err;
-    hipLaunchKernelGGL(k_src_term, dim3(grid_for(r->total, 256)), dim3(256), 0, (hipStream_t)stream, s, g, rr,
+    hipLaunchKernelGGL(k_src_term, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, s, g, rr,
                        scale_by_rho, regdev(r, ba->nbox), ba->dbox);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
@@ -198,7 +195,7 @@ int ksgf_scale_by_rho(const ksgf_boxarray* ba, double* out, int ng_out, const do
     int g[3] = {ngrow, ngrow, ngrow};
     const Region* r = get_region(ba, g, g, &err);
     if (!r) return err;
-    hipLaunchKernelGGL(k_scale_rho, dim3(grid_for(r->total, 256)), dim3(256), 0, (hipStream_t)stream, o, av, rv, ncomp,
+    hipLaunchKernelGGL(k_scale_rho, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, o, av, rv, ncomp,
                        divide, regdev(r, ba->nbox), ba->dbox);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;

@@ -118,6 +118,32 @@ def test_macproj_parity(ksgf, oracle, workloads, gpu):
             assert rel_l2(a, b) <= 1e-12, f"{case.name} dir {d}: {rel_l2(a, b)}"
 
 
+@pytest.mark.parametrize("n,prec", [((64, 64, 64), 0), ((96, 64, 48), 0), ((64, 64, 64), 1)])
+def test_macproj_abl_semicoarsening_parity(ksgf, oracle, workloads, gpu, n, prec):
+    """Large enough that the finest levels run on the grid-wide kernels
+    (semicoarsened ratios, Neumann walls); FP64 must track the oracle's
+    iterates, the mixed path must meet the same tolerance."""
+    case = workloads.abl(n, extent=(20.0 * n[0], 20.0 * n[1], 4.0 * n[2]), sizes=(16, 32), nsteps=1)
+    G, vel, rho, gp = oracle.make_state(case)
+    dba = ksgf.DeviceBoxArray(case.ba)
+    dt = case.cfl / oracle.cfl_max(G, vel, 3)
+    mac = oracle.extrap_vel_to_faces(G, vel, np.zeros(case.ba.mf_size(CELL, 3, 1)), case.bcrec, dt)
+    gmac = [dba.upload(m.copy()) for m in mac]
+    info_r, _ = oracle.mac_project(G, mac, case.lobc, case.hibc, 1.0, 1e-11, 1e-14, oracle.case_opts(case))
+    opts = ksgf.default_options(**case.mlmg)
+    opts.precision = prec
+    info_g = ksgf.MacProjector(dba, case.lobc, case.hibc, 1.0, opts).project(gmac, 1e-11, 1e-14)
+    assert info_g["converged"] and info_r["converged"]
+    if prec == 0:
+        assert info_g["iters"] == info_r["iters"], (info_g, info_r)
+    else:
+        assert info_g["iters"] <= info_r["iters"] + 2, (info_g, info_r)
+    for d in range(3):
+        a = case.ba.gather(to_np(gmac[d]), XFACE + d, 1, 1)
+        b = case.ba.gather(mac[d], XFACE + d, 1, 1)
+        assert rel_l2(a, b) <= (1e-12 if prec == 0 else 1e-9), f"dir {d}: {rel_l2(a, b)}"
+
+
 @pytest.mark.parametrize("which", ["tgv32", "wall"])
 def test_step_parity(ksgf, oracle, workloads, gpu, which):
     st_mod = __import__("importlib").import_module("kynema-sgf_amd.stepper")
