@@ -204,6 +204,12 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
 int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out,
                  ksgf_stream_t stream);
 
+/* max over valid cells of |div u_mac| (the post-projection check the MAC
+ * projector's convergence implies; diagnostic, synchronises).  u/v/w_mac:
+ * faces, 1 comp, ngrow ngrow_mac. */
+int ksgf_mac_divergence_max(const ksgf_boxarray* ba, const double* u_mac, const double* v_mac,
+                            const double* w_mac, int ngrow_mac, double* out, ksgf_stream_t stream);
+
 /* ComputeRHSOp::predictor_rhs, multiply_rho branch (CompRHSOps.H:93-117):
  * vel_new = (rho_old*vel_old + dt*(conv + src + kappa*diff)) / rho_new.
  * diff may be NULL. conv/src/diff: cell, ncomp, ngrow 0. */

@@ -56,7 +56,7 @@ EXPORTS = [
     "ksgf_extrap_vel_to_faces", "ksgf_compute_fluxes_from_state", "ksgf_compute_divergence",
     "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
     "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_predictor_update",
-    "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read",
+    "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
 ]
 
 _lib = None
@@ -113,6 +113,7 @@ def load(path=None):
     L.ksgf_icns_src_term.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_scale_by_rho.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_void_p]
+    L.ksgf_mac_divergence_max.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_prof_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.c_char_p,
                                  C.c_int]
     _lib = L
@@ -283,6 +284,12 @@ class DeviceBoxArray:
     def cfl_max(self, vel, ngrow):
         out = C.c_double()
         check(load().ksgf_cfl_max(self.h, _ptr(vel), ngrow, C.byref(out), _stream()), "cfl_max")
+        return out.value
+
+    def mac_divergence_max(self, mac, ngrow=1):
+        out = C.c_double()
+        check(load().ksgf_mac_divergence_max(self.h, _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]), ngrow, C.byref(out),
+                                             _stream()), "mac_divergence_max")
         return out.value
 
     def predictor_update(self, vel_new, ng_new, vel_old, ng_old, rho_old, ng_ro, rho_new, ng_rn, conv, src, diff,

@@ -16,6 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libksgf.so")
+HOSTLIB = os.path.join(HERE, "libksgf_host.so")
+STEPEXE = os.path.join(HERE, "ksgf_step")
+HOSTDIR = os.path.join(HERE, "host")
 
 SOURCES = ["ksgf_layout.hip", "ksgf_godunov.hip", "ksgf_mlmg.hip", "ksgf_ops.hip"]
 
@@ -55,8 +58,27 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ host layer (Hydro API + ApplyPredictor driver) over the C ABI
+    hobj = os.path.join(BUILD, "ksgf_host.o")
+    cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(HOSTDIR, "ksgf_host.cpp"), "-o", hobj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host layer compile failed:\n{r.stderr}")
+    tmp = HOSTLIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", hobj, "-L" + HERE, "-l:libksgf.so", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host layer link failed:\n{r.stderr}")
+    os.replace(tmp, HOSTLIB)
+    tmp = STEPEXE + ".tmp"
+    cmd = [HIPCC, "-O3", "-std=c++17", "-Wall", os.path.join(HOSTDIR, "ksgf_step_main.cpp"), "-L" + HERE,
+           "-l:libksgf.so", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ksgf_step build failed:\n{r.stderr}")
+    os.replace(tmp, STEPEXE)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, HOSTLIB, STEPEXE)
     return LIB
 
 

@@ -81,6 +81,29 @@ __global__ void __launch_bounds__(256) k_scale_rho(MFDev out, CMFDev a, CMFDev r
     for (int n = 0; n < ncomp; ++n) fo(i, j, k, n) = divide ? fa(i, j, k, n) / rv : fa(i, j, k, n) * rv;
 }
 
+__global__ void __launch_bounds__(256) k_mac_divmax(CMFDev u, CMFDev v, CMFDev w, RegDev r, const int* __restrict__ box,
+                                                    double dxi, double dyi, double dzi, unsigned long long* __restrict__ out)
+{
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < r.total; t += (long long)gridDim.x * blockDim.x) {
+        int b, i, j, k;
+        region_locate_flat(r, box, t, b, i, j, k);
+        CFab fu = fab_at(u, box, b), fv = fab_at(v, box, b), fw = fab_at(w, box, b);
+        double d = (fu(i + 1, j, k) - fu(i, j, k)) * dxi + (fv(i, j + 1, k) - fv(i, j, k)) * dyi +
+                   (fw(i, j, k + 1) - fw(i, j, k)) * dzi;
+        m = fmax(m, fabs(d));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = sh[0];
+        for (int w2 = 1; w2 < (int)(blockDim.x >> 6); ++w2) q = fmax(q, sh[w2]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(q));
+    }
+}
+
 }  // namespace ksgf
 
 using namespace ksgf;
@@ -121,6 +144,38 @@ int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* 
     // ParReduce floor of -1 (incflo_compute_dt.cpp:86-90): an empty or zero
     // field reports 0 here, which the caller treats as "no constraint"
     *out = m;
+    return KSGF_OK;
+}
+
+int ksgf_mac_divergence_max(const ksgf_boxarray* ba, const double* u_mac, const double* v_mac, const double* w_mac,
+                            int ngrow_mac, double* out, ksgf_stream_t stream)
+{
+    if (!ba || !u_mac || !v_mac || !w_mac || !out || ngrow_mac < 0) {
+        set_error("ksgf_mac_divergence_max: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev u, v, w;
+    int e;
+    if ((e = cmfdev(ba, u_mac, KSGF_XFACE, 1, ngrow_mac, &u)) || (e = cmfdev(ba, v_mac, KSGF_YFACE, 1, ngrow_mac, &v)) ||
+        (e = cmfdev(ba, w_mac, KSGF_ZFACE, 1, ngrow_mac, &w)))
+        return e;
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);
+    if (!r) return err;
+    unsigned long long* d = (unsigned long long*)get_scratch(ba, 41, 1, &err);
+    if (!d) return err;
+    KSGF_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
+    int nb = grid_for(r->total, 256);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_mac_divmax, dim3(nb), dim3(256), 0, st, u, v, w, regdev(r, ba->nbox), ba->dbox, 1.0 / ba->dx[0],
+                       1.0 / ba->dx[1], 1.0 / ba->dx[2], d);
+    KSGF_LAUNCH_CHECK();
+    unsigned long long bits = 0;
+    KSGF_HIP_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    std::memcpy(out, &bits, sizeof(double));
     return KSGF_OK;
 }
 

@@ -1,0 +1,190 @@
+// Godunov branch of incflo::ApplyPredictor (incflo_advance.cpp:186-438) up to
+// the predictor update, on the C++ host layer: the advect + MAC-project step
+// without the diffusion solve and the nodal projection.
+//
+//   compute_dt (convective CFL)        incflo_compute_dt.cpp:69-91, SimTime.cpp:162-193
+//   src = -gp/rho (velocity form), 1-ghost fill
+//                                      icns_ops.H:98-111, incflo_advance.cpp:283-291
+//   AdvectionOp<ICNS,Godunov>::preadvect
+//       ExtrapVelToFaces               icns_advection.H:211-216
+//       MacProjOp::set_inflow_velocity icns_advection.cpp:157-183
+//       MacProjOp::operator()          icns_advection.cpp:206-375
+//       face ghost fill                icns_advection.H:233-243
+//   AdvectionOp<ICNS,Godunov>::operator()
+//       q = rho u, fq = rho f          icns_advection.H:281-314
+//       ComputeFluxesOnBoxFromState    icns_advection.H:352-374
+//       ComputeDivergence (mult -1)    icns_advection.H:418-422
+//   compute_source_term(NPH) + ComputeRHSOp::predictor_rhs
+//                                      icns_ops.H:84-121, CompRHSOps.H:93-117
+#pragma once
+
+#include <cmath>
+#include <memory>
+#include <utility>
+
+#include "ksgf_hydro.H"
+
+namespace ksgf::amr {
+
+struct StepStats {
+    double dt = 0.0;
+    ksgf_mlmg_info mlmg{};
+    double divmax = -1.0;  // max |div u_mac| after the projection (when asked for)
+};
+
+class GodunovPredictor {
+public:
+    GodunovPredictor(const Geometry& geom, std::vector<Box> boxes, std::vector<BCRec> vel_bc,
+                     std::vector<double> vel_bcval, std::array<Hydro::LinOpBCType, 3> lobc,
+                     std::array<Hydro::LinOpBCType, 3> hibc, double rho0, double cfl, double mg_rtol, double mg_atol,
+                     const ksgf_mlmg_options& mlmg_opts)
+        : ba_(geom, std::move(boxes)),
+          vel_bc_(std::move(vel_bc)),
+          vel_bcval_(std::move(vel_bcval)),
+          rho0_(rho0),
+          cfl_(cfl),
+          rtol_(mg_rtol),
+          atol_(mg_atol),
+          vel_(std::make_unique<MultiFab>(ba_, 3, 3)),
+          vel_new_(std::make_unique<MultiFab>(ba_, 3, 3)),
+          rho_(ba_, 1, 3),
+          gp_(ba_, 3, 0),
+          src_(ba_, 3, 1),
+          src_nph_(ba_, 3, 0),
+          q_(ba_, 3, 3),
+          fq_(ba_, 3, 1),
+          umac_(ba_, 1, 1, IndexType::XFace),
+          vmac_(ba_, 1, 1, IndexType::YFace),
+          wmac_(ba_, 1, 1, IndexType::ZFace),
+          fx_(ba_, 3, 0, IndexType::XFace),
+          fy_(ba_, 3, 0, IndexType::YFace),
+          fz_(ba_, 3, 0, IndexType::ZFace),
+          ex_(ba_, 3, 0, IndexType::XFace),
+          ey_(ba_, 3, 0, IndexType::YFace),
+          ez_(ba_, 3, 0, IndexType::ZFace),
+          conv_(ba_, 3, 0),
+          mac_proj_({geom})
+    {
+        if (vel_bc_.size() != 3 || vel_bcval_.size() != 18) throw std::runtime_error("GodunovPredictor: BC sizes");
+        // forcing ghosts: first-order extrapolation where the velocity is not periodic
+        src_bc_ = vel_bc_;
+        for (auto& b : src_bc_)
+            for (int d = 0; d < 3; ++d) {
+                if (b.lo[d] != KSGF_BC_INT_DIR) b.lo[d] = KSGF_BC_FOEXTRAP;
+                if (b.hi[d] != KSGF_BC_INT_DIR) b.hi[d] = KSGF_BC_FOEXTRAP;
+            }
+        rho_bc_.resize(1);
+        for (int d = 0; d < 3; ++d) {
+            rho_bc_[0].lo[d] = geom.isPeriodic(d) ? KSGF_BC_INT_DIR : KSGF_BC_FOEXTRAP;
+            rho_bc_[0].hi[d] = geom.isPeriodic(d) ? KSGF_BC_INT_DIR : KSGF_BC_FOEXTRAP;
+        }
+        // MacProjOp::init_projector(Real beta) (icns_advection.cpp:109-155)
+        Hydro::LPInfo info;
+        info.do_semicoarsening = mlmg_opts.do_semicoarsening != 0;
+        info.max_coarsening_level = mlmg_opts.max_coarsening_level;
+        mac_proj_.getMLMG().opts = mlmg_opts;
+        mac_proj_.initProjector({&ba_}, info, 1.0 / rho0_);
+        mac_proj_.setDomainBC(lobc, hibc);
+        mac_proj_.setUMAC({{&umac_, &vmac_, &wmac_}});
+    }
+
+    void setStream(hipStream_t st) { st_ = st; }
+    const BoxArray& boxArray() const { return ba_; }
+
+    // domain-wide host arrays: vel (3 x N), rho (N), gp (3 x N)
+    void setState(const double* vel, const double* rho, const double* gp)
+    {
+        vel_->copyFromDomain(vel);
+        rho_.copyFromDomain(rho);
+        gp_.copyFromDomain(gp);
+        vel_->FillBoundary(st_);
+        vel_->FillPhysBC(vel_bc_, vel_bcval_, st_);
+        rho_.FillBoundary(st_);
+        rho_.FillPhysBC(rho_bc_, std::vector<double>(6, 0.0), st_);
+    }
+    void getVelocity(double* vel) const
+    {
+        (void)hipStreamSynchronize(st_);
+        vel_->copyToDomain(vel);
+    }
+    MultiFab& velocity() { return *vel_; }
+    MultiFab& umac() { return umac_; }
+    MultiFab& vmac() { return vmac_; }
+    MultiFab& wmac() { return wmac_; }
+
+    void setCheckDivergence(bool on) { check_div_ = on; }
+
+    StepStats advance(double fixed_dt = -1.0)
+    {
+        StepStats s;
+        const ksgf_boxarray* h = ba_.handle();
+        ksgf_stream_t st = (ksgf_stream_t)st_;
+        // compute_dt
+        double dt = fixed_dt;
+        if (!(dt > 0.0)) {
+            double c = 0.0;
+            check(ksgf_cfl_max(h, vel_->data(), 3, &c, st), "compute_dt");
+            if (!(c > 0.0)) throw std::runtime_error("compute_dt: CFL below machine epsilon, set a fixed dt");
+            const double cd = c;
+            dt = 2.0 * cfl_ / (cd + std::sqrt(cd * cd));
+        }
+        s.dt = dt;
+        // forcing in velocity form with one ghost
+        check(ksgf_icns_src_term(h, src_.data(), 1, gp_.data(), rho_.data(), 3, 0, st), "compute_source_term");
+        src_.FillBoundary(st_);
+        src_.FillPhysBC(src_bc_, std::vector<double>(18, 0.0), st_);
+        // preadvect
+        HydroUtils::ExtrapVelToFaces(*vel_, src_, umac_, vmac_, wmac_, vel_bc_, nullptr, ba_.geom(), dt, true, false,
+                                     "Godunov", KSGF_PPM_VANLEER, false, st_);
+        check(ksgf_set_inflow_velocity(h, umac_.data(), vmac_.data(), wmac_.data(), 1, vel_bc_.data(),
+                                       vel_bcval_.data(), st),
+              "set_inflow_velocity");
+        mac_proj_.updateBeta(1.0 / rho0_);
+        mac_proj_.project(rtol_, atol_, st_);
+        s.mlmg = mac_proj_.getMLMG().last;
+        if (check_div_)
+            check(ksgf_mac_divergence_max(h, umac_.data(), vmac_.data(), wmac_.data(), 1, &s.divmax, st),
+                  "divergence check");
+        MultiFab* mac[3] = {&umac_, &vmac_, &wmac_};
+        for (int d = 0; d < 3; ++d) {
+            std::vector<BCRec> bc1{vel_bc_[d]};
+            std::vector<double> v1(6);
+            for (int o = 0; o < 6; ++o) v1[o] = vel_bcval_[o * 3 + d];
+            mac[d]->FillBoundary(st_);
+            mac[d]->FillPhysBC(bc1, v1, st_);
+        }
+        // advection of momentum
+        check(ksgf_scale_by_rho(h, q_.data(), 3, vel_->data(), 3, rho_.data(), 3, 3, 3, 0, st), "q = rho u");
+        check(ksgf_scale_by_rho(h, fq_.data(), 1, src_.data(), 1, rho_.data(), 3, 3, 1, 0, st), "fq = rho f");
+        const int iconserv[3] = {1, 1, 1};
+        HydroUtils::ComputeFluxesOnBoxFromState(3, q_, q_, fx_, fy_, fz_, ex_, ey_, ez_, false, umac_, vmac_, wmac_,
+                                                nullptr, fq_, ba_.geom(), dt, vel_bc_, iconserv, true, false, true,
+                                                false, "Godunov", KSGF_PPM_VANLEER, false, st_);
+        HydroUtils::ComputeDivergence(conv_, fx_, fy_, fz_, 3, ba_.geom(), -1.0, false, st_);
+        // source at n+1/2 (momentum form) and the predictor update
+        check(ksgf_icns_src_term(h, src_nph_.data(), 0, gp_.data(), rho_.data(), 3, 1, st), "src nph");
+        check(ksgf_predictor_update(h, vel_new_->data(), 3, vel_->data(), 3, rho_.data(), 3, rho_.data(), 3,
+                                    conv_.data(), src_nph_.data(), nullptr, 0.0, dt, 3, st),
+              "predictor_rhs");
+        vel_new_->FillBoundary(st_);
+        vel_new_->FillPhysBC(vel_bc_, vel_bcval_, st_);
+        std::swap(vel_, vel_new_);
+        return s;
+    }
+
+private:
+    BoxArray ba_;
+    std::vector<BCRec> vel_bc_, src_bc_, rho_bc_;
+    std::vector<double> vel_bcval_;
+    double rho0_, cfl_, rtol_, atol_;
+    hipStream_t st_ = nullptr;
+    bool check_div_ = false;
+    std::unique_ptr<MultiFab> vel_, vel_new_;
+    MultiFab rho_, gp_, src_, src_nph_, q_, fq_;
+    MultiFab umac_, vmac_, wmac_;
+    MultiFab fx_, fy_, fz_, ex_, ey_, ez_;
+    MultiFab conv_;
+    Hydro::MacProjector mac_proj_;
+};
+
+}  // namespace ksgf::amr

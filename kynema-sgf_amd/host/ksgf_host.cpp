@@ -1,0 +1,119 @@
+// C entry points of the C++ host layer (libksgf_host.so): the Godunov
+// advect + MAC-project step driver for callers that are not C++ (the
+// Python bench and tests).  Exceptions never cross this boundary.
+#include <cstring>
+#include <string>
+
+#include "incflo_godunov.H"
+#include "ksgf_host.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+using ksgf::amr::Box;
+using ksgf::amr::Geometry;
+using ksgf::amr::GodunovPredictor;
+
+extern "C" {
+
+const char* ksgf_host_last_error(void) { return g_err.c_str(); }
+
+int ksgf_host_stepper_create(const ksgf_geometry* geom, const int* boxes, int nbox, const ksgf_bcrec* vel_bc,
+                             const double* vel_bcval, const int* lobc, const int* hibc, double rho0, double cfl,
+                             double mg_rtol, double mg_atol, const ksgf_mlmg_options* opts, ksgf_host_stepper** out)
+{
+    try {
+        if (!geom || !boxes || nbox <= 0 || !vel_bc || !vel_bcval || !lobc || !hibc || !out)
+            throw std::runtime_error("null argument");
+        Geometry g;
+        for (int d = 0; d < 3; ++d) {
+            g.domain.lo[d] = geom->domain_lo[d];
+            g.domain.hi[d] = geom->domain_hi[d];
+            g.prob_lo[d] = geom->prob_lo[d];
+            g.prob_hi[d] = geom->prob_hi[d];
+            g.is_periodic[d] = geom->is_periodic[d];
+        }
+        std::vector<Box> bx(nbox);
+        for (int b = 0; b < nbox; ++b)
+            for (int d = 0; d < 3; ++d) {
+                bx[b].lo[d] = boxes[6 * b + d];
+                bx[b].hi[d] = boxes[6 * b + 3 + d];
+            }
+        std::vector<ksgf::amr::BCRec> bc(vel_bc, vel_bc + 3);
+        std::vector<double> bv(vel_bcval, vel_bcval + 18);
+        std::array<Hydro::LinOpBCType, 3> lo, hi;
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = (Hydro::LinOpBCType)lobc[d];
+            hi[d] = (Hydro::LinOpBCType)hibc[d];
+        }
+        ksgf_mlmg_options o;
+        if (opts)
+            o = *opts;
+        else
+            ksgf_mlmg_default_options(&o);
+        auto* s = new GodunovPredictor(g, bx, bc, bv, lo, hi, rho0, cfl, mg_rtol, mg_atol, o);
+        *out = reinterpret_cast<ksgf_host_stepper*>(s);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_create: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_set_stream(ksgf_host_stepper* s, ksgf_stream_t stream)
+{
+    if (!s) return KSGF_ERR_INVALID;
+    reinterpret_cast<GodunovPredictor*>(s)->setStream((hipStream_t)stream);
+    return KSGF_OK;
+}
+
+int ksgf_host_stepper_set_state(ksgf_host_stepper* s, const double* vel, const double* rho, const double* gp)
+{
+    try {
+        if (!s || !vel || !rho || !gp) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->setState(vel, rho, gp);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_set_state: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_step(ksgf_host_stepper* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info,
+                           double* divmax)
+{
+    try {
+        if (!s) throw std::runtime_error("null stepper");
+        auto* g = reinterpret_cast<GodunovPredictor*>(s);
+        g->setCheckDivergence(divmax != nullptr);
+        auto st = g->advance(fixed_dt);
+        if (dt_used) *dt_used = st.dt;
+        if (info) *info = st.mlmg;
+        if (divmax) *divmax = st.divmax;
+        return st.mlmg.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_step: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel)
+{
+    try {
+        if (!s || !vel) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->getVelocity(vel);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_get_velocity: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
+{
+    delete reinterpret_cast<GodunovPredictor*>(s);
+    return KSGF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,34 @@
+/* C entry points of the C++ host layer (libksgf_host.so).
+ * The advect + MAC-project step of incflo::ApplyPredictor's Godunov branch
+ * (incflo_advance.cpp:186-438, without diffusion and nodal projection),
+ * driven by the C++ GodunovPredictor over the ksgf C ABI. */
+#ifndef KSGF_HOST_H
+#define KSGF_HOST_H
+
+#include "../../include/ksgf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ksgf_host_stepper ksgf_host_stepper;
+
+const char* ksgf_host_last_error(void);
+/* vel_bc: 3 BCRecs; vel_bcval: [6 orientations][3 comps]; lobc/hibc: LinOp BCs */
+int ksgf_host_stepper_create(const ksgf_geometry* geom, const int* boxes, int nbox, const ksgf_bcrec* vel_bc,
+                             const double* vel_bcval, const int* lobc, const int* hibc, double rho0, double cfl,
+                             double mg_rtol, double mg_atol, const ksgf_mlmg_options* opts, ksgf_host_stepper** out);
+int ksgf_host_stepper_set_stream(ksgf_host_stepper* s, ksgf_stream_t stream);
+/* domain-wide host arrays, x fastest: vel 3*N, rho N, gp 3*N */
+int ksgf_host_stepper_set_state(ksgf_host_stepper* s, const double* vel, const double* rho, const double* gp);
+/* fixed_dt <= 0: dt from the convective CFL; divmax (may be NULL): max
+ * |div u_mac| after the MAC projection */
+int ksgf_host_stepper_step(ksgf_host_stepper* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info,
+                           double* divmax);
+int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel);
+int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

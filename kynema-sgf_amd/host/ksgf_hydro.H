@@ -1,0 +1,344 @@
+// C++ host layer: the AMReX / AMReX-Hydro names and signatures the
+// reference calls on this path, over the C ABI of include/ksgf.h.
+//
+//   amrex-like data model   Geometry, Box, BoxArray, BCRec, MultiFab
+//                           (device storage in the ksgf MultiFab layout)
+//   HydroUtils              ExtrapVelToFaces   (icns_advection.H:211-216)
+//                           ComputeFluxesOnBoxFromState (:359-373), one call
+//                           for all boxes of the MultiFab (the MFIter loop)
+//                           ComputeDivergence  (:418-422)
+//   Hydro::MacProjector     MacProjector(geom), initProjector(ba, lpinfo,
+//                           beta), setDomainBC, updateBeta, setUMAC, project,
+//                           getMLMG (icns_advection.cpp:88-104,143-154,
+//                           289-353,372-374)
+//
+// Errors from the C ABI become std::runtime_error (amrex.throw_exception=1
+// semantics, main.cpp:60-71).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ksgf.h"
+
+namespace ksgf::amr {
+
+inline void check(int rc, const char* what)
+{
+    if (rc < 0) throw std::runtime_error(std::string(what) + ": " + ksgf_last_error());
+}
+
+enum class IndexType { Cell = KSGF_CELL, XFace = KSGF_XFACE, YFace = KSGF_YFACE, ZFace = KSGF_ZFACE };
+
+struct Box {
+    std::array<int, 3> lo{}, hi{};
+    long long numPts() const
+    {
+        return (long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+    }
+};
+
+struct Geometry {
+    Box domain;
+    std::array<double, 3> prob_lo{}, prob_hi{};
+    std::array<int, 3> is_periodic{};
+    double CellSize(int d) const { return (prob_hi[d] - prob_lo[d]) / (domain.hi[d] - domain.lo[d] + 1); }
+    bool isPeriodic(int d) const { return is_periodic[d] != 0; }
+    ksgf_geometry c() const
+    {
+        ksgf_geometry g{};
+        for (int d = 0; d < 3; ++d) {
+            g.domain_lo[d] = domain.lo[d];
+            g.domain_hi[d] = domain.hi[d];
+            g.prob_lo[d] = prob_lo[d];
+            g.prob_hi[d] = prob_hi[d];
+            g.is_periodic[d] = is_periodic[d];
+        }
+        return g;
+    }
+};
+
+using BCRec = ksgf_bcrec;
+
+// BoxArray bound to its Geometry (the library's device box tables live with it)
+class BoxArray {
+public:
+    BoxArray(const Geometry& geom, std::vector<Box> boxes) : geom_(geom), boxes_(std::move(boxes))
+    {
+        std::vector<int> flat;
+        for (const auto& b : boxes_)
+            for (int v : {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]}) flat.push_back(v);
+        ksgf_geometry g = geom_.c();
+        ksgf_boxarray* h = nullptr;
+        check(ksgf_boxarray_create(&g, flat.data(), (int)boxes_.size(), &h), "BoxArray");
+        h_ = std::shared_ptr<ksgf_boxarray>(h, [](ksgf_boxarray* p) { ksgf_boxarray_destroy(p); });
+    }
+    int size() const { return (int)boxes_.size(); }
+    const Box& operator[](int b) const { return boxes_[b]; }
+    const Geometry& geom() const { return geom_; }
+    ksgf_boxarray* handle() const { return h_.get(); }
+
+private:
+    Geometry geom_;
+    std::vector<Box> boxes_;
+    std::shared_ptr<ksgf_boxarray> h_;
+};
+
+// Device-resident MultiFab in the ksgf layout (one FAB per box, ghosts
+// included, Fortran order, component slowest).
+class MultiFab {
+public:
+    MultiFab(const BoxArray& ba, int ncomp, int ngrow, IndexType it = IndexType::Cell)
+        : ba_(ba), ncomp_(ncomp), ngrow_(ngrow), it_(it)
+    {
+        n_ = ksgf_multifab_size(ba.handle(), (int)it, ncomp, ngrow);
+        if (n_ < 0) throw std::runtime_error("MultiFab: bad layout");
+        if (hipMalloc(&p_, sizeof(double) * (size_t)n_) != hipSuccess) throw std::runtime_error("MultiFab: hipMalloc");
+        if (hipMemset(p_, 0, sizeof(double) * (size_t)n_) != hipSuccess) throw std::runtime_error("MultiFab: hipMemset");
+    }
+    ~MultiFab()
+    {
+        if (p_) (void)hipFree(p_);
+    }
+    MultiFab(const MultiFab&) = delete;
+    MultiFab& operator=(const MultiFab&) = delete;
+
+    double* data() { return p_; }
+    const double* data() const { return p_; }
+    long long size() const { return n_; }
+    int nComp() const { return ncomp_; }
+    int nGrow() const { return ngrow_; }
+    IndexType ixType() const { return it_; }
+    const BoxArray& boxArray() const { return ba_; }
+
+    void FillBoundary(hipStream_t st = nullptr)
+    {
+        check(ksgf_fill_boundary(ba_.handle(), p_, (int)it_, ncomp_, ngrow_, (ksgf_stream_t)st), "FillBoundary");
+    }
+    void FillPhysBC(const std::vector<BCRec>& bc, const std::vector<double>& bcval, hipStream_t st = nullptr)
+    {
+        check(ksgf_fill_physbc(ba_.handle(), p_, (int)it_, ncomp_, ngrow_, bc.data(), bcval.data(),
+                               (ksgf_stream_t)st),
+              "FillPhysBC");
+    }
+    // copy a domain-wide host array (ncomp x nz x ny x nx, x fastest) into
+    // the valid region of every FAB
+    void copyFromDomain(const double* host)
+    {
+        std::vector<double> h((size_t)n_, 0.0);
+        const auto& dom = ba_.geom().domain;
+        int N[3];
+        for (int d = 0; d < 3; ++d) N[d] = dom.hi[d] - dom.lo[d] + 1 + ((int)it_ == d + 1 ? 1 : 0);
+        long long off = 0;
+        for (int b = 0; b < ba_.size(); ++b) {
+            const Box& bx = ba_[b];
+            int lo[3], n[3], vhi[3];
+            for (int d = 0; d < 3; ++d) {
+                int nodal = ((int)it_ == d + 1);
+                lo[d] = bx.lo[d] - ngrow_;
+                n[d] = bx.hi[d] - bx.lo[d] + 1 + 2 * ngrow_ + nodal;
+                vhi[d] = bx.hi[d] + nodal;
+            }
+            long long cs = (long long)n[0] * n[1] * n[2];
+            for (int c = 0; c < ncomp_; ++c)
+                for (int k = bx.lo[2]; k <= vhi[2]; ++k)
+                    for (int j = bx.lo[1]; j <= vhi[1]; ++j)
+                        for (int i = bx.lo[0]; i <= vhi[0]; ++i)
+                            h[off + c * cs + (i - lo[0]) + (long long)n[0] * ((j - lo[1]) + (long long)n[1] * (k - lo[2]))] =
+                                host[(long long)c * N[0] * N[1] * N[2] + (i - dom.lo[0]) +
+                                     (long long)N[0] * ((j - dom.lo[1]) + (long long)N[1] * (k - dom.lo[2]))];
+            off += cs * ncomp_;
+        }
+        if (hipMemcpy(p_, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("MultiFab::copyFromDomain");
+    }
+    void copyToDomain(double* host) const
+    {
+        std::vector<double> h((size_t)n_);
+        if (hipMemcpy(h.data(), p_, sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("MultiFab::copyToDomain");
+        const auto& dom = ba_.geom().domain;
+        int N[3];
+        for (int d = 0; d < 3; ++d) N[d] = dom.hi[d] - dom.lo[d] + 1 + ((int)it_ == d + 1 ? 1 : 0);
+        long long off = 0;
+        for (int b = 0; b < ba_.size(); ++b) {
+            const Box& bx = ba_[b];
+            int lo[3], n[3], vhi[3];
+            for (int d = 0; d < 3; ++d) {
+                int nodal = ((int)it_ == d + 1);
+                lo[d] = bx.lo[d] - ngrow_;
+                n[d] = bx.hi[d] - bx.lo[d] + 1 + 2 * ngrow_ + nodal;
+                vhi[d] = bx.hi[d] + nodal;
+            }
+            long long cs = (long long)n[0] * n[1] * n[2];
+            for (int c = 0; c < ncomp_; ++c)
+                for (int k = bx.lo[2]; k <= vhi[2]; ++k)
+                    for (int j = bx.lo[1]; j <= vhi[1]; ++j)
+                        for (int i = bx.lo[0]; i <= vhi[0]; ++i)
+                            host[(long long)c * N[0] * N[1] * N[2] + (i - dom.lo[0]) +
+                                 (long long)N[0] * ((j - dom.lo[1]) + (long long)N[1] * (k - dom.lo[2]))] =
+                                h[off + c * cs + (i - lo[0]) + (long long)n[0] * ((j - lo[1]) + (long long)n[1] * (k - lo[2]))];
+            off += cs * ncomp_;
+        }
+    }
+
+private:
+    const BoxArray& ba_;
+    int ncomp_, ngrow_;
+    IndexType it_;
+    long long n_ = 0;
+    double* p_ = nullptr;
+};
+
+}  // namespace ksgf::amr
+
+// ---------------------------------------------------------------------------
+// HydroUtils (AMReX-Hydro) MultiFab-level calls
+// ---------------------------------------------------------------------------
+namespace HydroUtils {
+
+using ksgf::amr::BCRec;
+using ksgf::amr::Geometry;
+using ksgf::amr::MultiFab;
+
+// the advection_type string is "Godunov" on this path
+inline void ExtrapVelToFaces(const MultiFab& vel, const MultiFab& vel_forces, MultiFab& u_mac, MultiFab& v_mac,
+                             MultiFab& w_mac, const std::vector<BCRec>& h_bcrec, const BCRec* /*d_bcrec*/,
+                             const Geometry& /*geom*/, double dt, bool use_ppm, bool use_forces_in_trans,
+                             const std::string& /*advection_type*/, int limiter_type, bool allow_inflow_on_outflow,
+                             hipStream_t st = nullptr)
+{
+    ksgf::amr::check(ksgf_extrap_vel_to_faces(vel.boxArray().handle(), vel.data(), vel_forces.data(), u_mac.data(),
+                                              v_mac.data(), w_mac.data(), h_bcrec.data(), dt, use_ppm,
+                                              use_forces_in_trans, limiter_type, allow_inflow_on_outflow,
+                                              (ksgf_stream_t)st),
+                     "ExtrapVelToFaces");
+}
+
+inline void ComputeFluxesOnBoxFromState(int ncomp, const MultiFab& q, const MultiFab& q_on_face_bc,
+                                        MultiFab& flux_x, MultiFab& flux_y, MultiFab& flux_z, MultiFab& face_x,
+                                        MultiFab& face_y, MultiFab& face_z, bool known_edge_state,
+                                        const MultiFab& umac, const MultiFab& vmac, const MultiFab& wmac,
+                                        const MultiFab* divu, const MultiFab& fq, const Geometry& /*geom*/,
+                                        double dt, const std::vector<BCRec>& h_bcrec, const int* iconserv,
+                                        bool use_ppm, bool use_forces_in_trans, bool is_velocity,
+                                        bool fluxes_are_area_weighted, const std::string& /*advection_type*/,
+                                        int limiter_type, bool allow_inflow_on_outflow, hipStream_t st = nullptr)
+{
+    ksgf::amr::check(ksgf_compute_fluxes_from_state(
+                         q.boxArray().handle(), ncomp, q.data(), q_on_face_bc.data(), flux_x.data(), flux_y.data(),
+                         flux_z.data(), face_x.data(), face_y.data(), face_z.data(), known_edge_state, umac.data(),
+                         vmac.data(), wmac.data(), divu ? divu->data() : nullptr, fq.data(), dt, h_bcrec.data(),
+                         iconserv, use_ppm, use_forces_in_trans, is_velocity, fluxes_are_area_weighted,
+                         limiter_type, allow_inflow_on_outflow, (ksgf_stream_t)st),
+                     "ComputeFluxesOnBoxFromState");
+}
+
+inline void ComputeDivergence(MultiFab& div, const MultiFab& fx, const MultiFab& fy, const MultiFab& fz, int ncomp,
+                              const Geometry& /*geom*/, double mult, bool fluxes_are_area_weighted,
+                              hipStream_t st = nullptr)
+{
+    ksgf::amr::check(ksgf_compute_divergence(div.boxArray().handle(), div.data(), fx.data(), fy.data(), fz.data(),
+                                             ncomp, mult, fluxes_are_area_weighted, (ksgf_stream_t)st),
+                     "ComputeDivergence");
+}
+
+}  // namespace HydroUtils
+
+// ---------------------------------------------------------------------------
+// Hydro::MacProjector
+// ---------------------------------------------------------------------------
+namespace Hydro {
+
+enum class LinOpBCType { Periodic = KSGF_LO_PERIODIC, Neumann = KSGF_LO_NEUMANN, Dirichlet = KSGF_LO_DIRICHLET };
+
+// amrex::LPInfo knobs used on this path
+struct LPInfo {
+    bool do_semicoarsening = false;
+    int max_coarsening_level = 30;
+};
+
+// what io::print_mlmg_info reads (console_io.cpp:178-186)
+struct MLMG {
+    ksgf_mlmg_options opts{};
+    ksgf_mlmg_info last{};
+    MLMG() { ksgf_mlmg_default_options(&opts); }
+    void setMaxIter(int n) { opts.maxiter = n; }
+    void setPreSmooth(int n) { opts.num_pre_smooth = n; }
+    void setPostSmooth(int n) { opts.num_post_smooth = n; }
+    void setFinalSmooth(int n) { opts.num_final_smooth = n; }
+    void setBottomSmooth(int n) { opts.num_bottom_smooth = n; }
+    void setBottomTolerance(double t) { opts.bottom_rtol = t; }
+    void setBottomToleranceAbs(double t) { opts.bottom_atol = t; }
+    void setBottomMaxIter(int n) { opts.bottom_maxiter = n; }
+    void setMixedPrecision(bool on) { opts.precision = on ? KSGF_MG_MIXED : KSGF_MG_FP64; }
+    int getNumIters() const { return last.iters; }
+    double getInitResidual() const { return last.r0; }
+    double getFinalResidual() const { return last.rfinal; }
+};
+
+class MacProjector {
+public:
+    explicit MacProjector(const std::vector<ksgf::amr::Geometry>& geom) : geom_(geom)
+    {
+        if (geom_.size() != 1) throw std::runtime_error("MacProjector: one level on this path");
+    }
+    ~MacProjector()
+    {
+        if (mp_) ksgf_macproj_destroy(mp_);
+    }
+    void initProjector(const std::vector<const ksgf::amr::BoxArray*>& ba, const LPInfo& info, double beta)
+    {
+        ba_ = ba.at(0);
+        beta_ = beta;
+        mlmg_.opts.do_semicoarsening = info.do_semicoarsening ? 1 : 0;
+        mlmg_.opts.max_coarsening_level = info.max_coarsening_level;
+    }
+    void setDomainBC(const std::array<LinOpBCType, 3>& lo, const std::array<LinOpBCType, 3>& hi)
+    {
+        for (int d = 0; d < 3; ++d) {
+            lo_[d] = (int)lo[d];
+            hi_[d] = (int)hi[d];
+        }
+        build();
+    }
+    void updateBeta(double beta)
+    {
+        beta_ = beta;
+        if (mp_) ksgf::amr::check(ksgf_macproj_set_beta(mp_, beta), "updateBeta");
+    }
+    void setUMAC(const std::vector<std::array<ksgf::amr::MultiFab*, 3>>& umac) { umac_ = umac.at(0); }
+    void project(double rtol, double atol, hipStream_t st = nullptr)
+    {
+        if (!mp_) build();
+        int rc = ksgf_macproj_project(mp_, umac_[0]->data(), umac_[1]->data(), umac_[2]->data(), rtol, atol,
+                                      &mlmg_.last, (ksgf_stream_t)st);
+        ksgf::amr::check(rc, "MacProjector::project");  // KSGF_ERR_NOT_CONVERGED (> 0) is not an abort
+    }
+    MLMG& getMLMG() { return mlmg_; }
+    void rebuild() { build(); }
+
+private:
+    void build()
+    {
+        if (!ba_) throw std::runtime_error("MacProjector: initProjector first");
+        if (mp_) ksgf_macproj_destroy(mp_);
+        mp_ = nullptr;
+        ksgf::amr::check(ksgf_macproj_create(ba_->handle(), lo_, hi_, beta_, &mlmg_.opts, &mp_), "MacProjector");
+    }
+    std::vector<ksgf::amr::Geometry> geom_;
+    const ksgf::amr::BoxArray* ba_ = nullptr;
+    double beta_ = 1.0;
+    int lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
+    std::array<ksgf::amr::MultiFab*, 3> umac_{};
+    MLMG mlmg_;
+    ksgf_macproj* mp_ = nullptr;
+};
+
+}  // namespace Hydro

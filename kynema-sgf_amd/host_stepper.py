@@ -1,0 +1,110 @@
+"""ctypes binding of libksgf_host.so: the C++ GodunovPredictor (the Godunov
+branch of incflo::ApplyPredictor without diffusion and nodal projection)
+driven from Python.  Work goes on torch's current stream."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import HERE, KsgfBCRec, KsgfError, KsgfGeometry, KsgfMLMGInfo, KsgfMLMGOptions, bcrecs, default_options, load
+
+HOSTLIB = os.path.join(HERE, "libksgf_host.so")
+_h = None
+
+
+def host_lib():
+    global _h
+    if _h is None:
+        load()  # libksgf.so first (one HIP runtime, shared with torch)
+        if not os.path.exists(HOSTLIB):
+            raise KsgfError(f"{HOSTLIB} is not built: run python kynema-sgf_amd/build.py")
+        L = C.CDLL(HOSTLIB)
+        L.ksgf_host_last_error.restype = C.c_char_p
+        L.ksgf_host_stepper_create.argtypes = [C.POINTER(KsgfGeometry), C.POINTER(C.c_int), C.c_int,
+                                               C.POINTER(KsgfBCRec), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), C.c_double, C.c_double, C.c_double, C.c_double,
+                                               C.POINTER(KsgfMLMGOptions), C.POINTER(C.c_void_p)]
+        L.ksgf_host_stepper_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+        L.ksgf_host_stepper_set_state.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(KsgfMLMGInfo),
+                                             C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_get_velocity.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
+        _h = L
+    return _h
+
+
+def _chk(rc, what, allow=(0,)):
+    if rc not in allow:
+        raise KsgfError(f"{what} failed ({rc}): {host_lib().ksgf_host_last_error().decode()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class HostStepper:
+    def __init__(self, case, options=None):
+        import torch
+
+        L = host_lib()
+        self.case = case
+        g = KsgfGeometry()
+        for d in range(3):
+            g.domain_lo[d] = case.geom.dlo[d]
+            g.domain_hi[d] = case.geom.dhi[d]
+            g.prob_lo[d] = case.geom.prob_lo[d]
+            g.prob_hi[d] = case.geom.prob_hi[d]
+            g.is_periodic[d] = int(case.geom.periodic[d])
+        self._boxes = np.ascontiguousarray(case.ba.boxes, dtype=np.int32)
+        self._bcval = np.ascontiguousarray(case.bcval, dtype=np.float64)
+        self._lo = np.ascontiguousarray(case.lobc, dtype=np.int32)
+        self._hi = np.ascontiguousarray(case.hibc, dtype=np.int32)
+        self.options = options or default_options(**case.mlmg)
+        h = C.c_void_p()
+        _chk(L.ksgf_host_stepper_create(C.byref(g), self._boxes.ctypes.data_as(C.POINTER(C.c_int)), case.ba.nbox,
+                                        bcrecs(case.bcrec), _dp(self._bcval), self._lo.ctypes.data_as(C.POINTER(C.c_int)),
+                                        self._hi.ctypes.data_as(C.POINTER(C.c_int)), case.rho0, case.cfl, case.mg_rtol,
+                                        case.mg_atol, C.byref(self.options), C.byref(h)), "stepper_create")
+        self.h = h
+        _chk(L.ksgf_host_stepper_set_stream(h, C.c_void_p(torch.cuda.current_stream().cuda_stream)), "set_stream")
+        self.last_info = None
+        self.last_dt = None
+
+    def load(self, vel=None, rho=None, gp=None):
+        c = self.case
+        v = np.ascontiguousarray(c.vel if vel is None else vel, dtype=np.float64)
+        r = np.ascontiguousarray(c.rho if rho is None else rho, dtype=np.float64)
+        g = np.ascontiguousarray(c.gp if gp is None else gp, dtype=np.float64)
+        _chk(host_lib().ksgf_host_stepper_set_state(self.h, _dp(v), _dp(r), _dp(g)), "set_state")
+
+    def step(self, dt=-1.0, check_divergence=False):
+        info = KsgfMLMGInfo()
+        dtu = C.c_double()
+        dv = C.c_double(-1.0)
+        _chk(host_lib().ksgf_host_stepper_step(self.h, dt if dt is not None else -1.0, C.byref(dtu), C.byref(info),
+                                               C.byref(dv) if check_divergence else None), "step", allow=(0, 1))
+        self.last_info = info.as_dict()
+        self.last_dt = dtu.value
+        self.last_divmax = dv.value if check_divergence else None
+        return dtu.value
+
+    def velocity(self):
+        n = self.case.geom.n_cell
+        out = np.zeros((3, n[2], n[1], n[0]))
+        _chk(host_lib().ksgf_host_stepper_get_velocity(self.h, _dp(out)), "get_velocity")
+        return out
+
+    def close(self):
+        if self.h:
+            host_lib().ksgf_host_stepper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

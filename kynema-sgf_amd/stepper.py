@@ -16,6 +16,8 @@ diffusion solve and the nodal projection:
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from . import CELL, XFACE, DeviceBoxArray, MacProjector, default_options
@@ -70,7 +72,8 @@ class AdvectProjectStep:
     def step(self, dt=None, swap=True):
         c, dba = self.case, self.dba
         if dt is None:
-            dt = c.cfl / dba.cfl_max(self.vel, 3)
+            cc = dba.cfl_max(self.vel, 3)
+            dt = 2.0 * c.cfl / (cc + math.sqrt(cc * cc))
         self.last_dt = dt
         dba.icns_src_term(self.src, 1, self.gp, self.rho, 3, 0)
         dba.fill_ghosts(self.src, CELL, 3, 1, self.src_bc, np.zeros((6, 3)))

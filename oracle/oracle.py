@@ -14,6 +14,7 @@ with the oracle functions on numpy MultiFabs.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -268,8 +269,10 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1):
     ba = case.ba
     opts = opts or case_opts(case)
     if dt is None:
+        # SimTime::set_current_cfl (SimTime.cpp:168-193) with no diffusive or
+        # forcing CFL: dt = 2 cfl / (C + sqrt(C^2))
         c = cfl_max(G, vel, 3)
-        dt = case.cfl / c
+        dt = 2.0 * case.cfl / (c + math.sqrt(c * c))
     # sources: src = -gp/rho (velocity form), ghost 1 (foextrap-like fill)
     src = np.zeros(ba.mf_size(CELL, 3, 1))
     for b in range(ba.nbox):

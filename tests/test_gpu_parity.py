@@ -144,20 +144,33 @@ def test_macproj_abl_semicoarsening_parity(ksgf, oracle, workloads, gpu, n, prec
         assert rel_l2(a, b) <= (1e-12 if prec == 0 else 1e-9), f"dir {d}: {rel_l2(a, b)}"
 
 
-@pytest.mark.parametrize("which", ["tgv32", "wall"])
+@pytest.mark.parametrize("which", ["tgv32", "wall", "abl_host"])
 def test_step_parity(ksgf, oracle, workloads, gpu, which):
-    st_mod = __import__("importlib").import_module("kynema-sgf_amd.stepper")
+    imp = __import__("importlib").import_module
+    st_mod = imp("kynema-sgf_amd.stepper")
     if which == "tgv32":
         case = workloads.taylor_green(32, 16)
-    else:
+    elif which == "wall":
         case = _cases(workloads)[1]
+    else:
+        case = workloads.abl((64, 48, 32), extent=(1280.0, 960.0, 128.0), sizes=(16, 32), nsteps=2)
     G, vel, rho, gp = oracle.make_state(case)
-    S = st_mod.AdvectProjectStep(case)
+    if which == "abl_host":
+        # the C++ host layer (GodunovPredictor over the C ABI)
+        S = imp("kynema-sgf_amd.host_stepper").HostStepper(case)
+    else:
+        S = st_mod.AdvectProjectStep(case)
     S.load()
     for n in range(2):
         vel, diag = oracle.advect_project_step(case, G, vel, rho, gp)
         S.step()
-        assert S.last_info["iters"] == diag["mlmg"]["iters"]
-        a = case.ba.gather(to_np(S.vel), CELL, 3, 3)
+        assert S.last_info["iters"] == diag["mlmg"]["iters"], f"step {n}"
+        # step 0 starts from identical bits; later steps inherit the MG's
+        # reduction-order differences (a few ulp of velocity) through the CFL
+        if n == 0:
+            assert S.last_dt == diag["dt"]
+        else:
+            assert abs(S.last_dt - diag["dt"]) <= 1e-13 * diag["dt"], f"step {n}: {S.last_dt} vs {diag['dt']}"
+        a = case.ba.gather(to_np(S.vel), CELL, 3, 3) if which != "abl_host" else S.velocity()
         b = case.ba.gather(vel, CELL, 3, 3)
         assert rel_l2(a, b) <= 1e-12, f"step {n}: {rel_l2(a, b)}"
