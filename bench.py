@@ -40,7 +40,23 @@ def parse():
     p.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only run a plain 1 GiB -> 1 GiB elementwise copy (x*1.0) 4 times: the known-size "
+                        "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
     return p.parse_args()
+
+
+def copy_probe():
+    import torch
+
+    n = 1 << 27
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(4):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 4}))
 
 
 def run_gpu(case, precision, steps, warmup, profile):
@@ -109,6 +125,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; runs on more GPUs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
+        return
     wl = importlib.import_module("kynema-sgf_amd.workloads")
     n = tuple(a.n)
     ext = (20.0 * n[0], 20.0 * n[1], 4.0 * n[2])

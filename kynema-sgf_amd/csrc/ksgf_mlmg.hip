@@ -35,24 +35,12 @@
 #include <cstring>
 
 #include "ksgf_internal.h"
+#include "ksgf_mg_grid.h"
 
 namespace ksgf {
 
-struct LevelGeom {
-    int n0, n1, n2;
-    long long sy, sz, tot;
-    double cx, cy, cz, dg, idg;
-    int lo[3], hi[3];  // LinOp BCs
-    int ghost_mode;    // 1: explicit ghost fills before every colour
-};
-
 constexpr long long kCoarseCells = 16384;
 constexpr int kCoarseThreads = 1024;
-
-__host__ __device__ __forceinline__ long long LIdx(const LevelGeom& L, int i, int j, int k)
-{
-    return (long long)(i + 1) + L.sy * (j + 1) + L.sz * (k + 1);
-}
 
 template <typename T>
 __device__ __forceinline__ T bc_ghost(int t, T near_v, T far_v)
@@ -185,31 +173,6 @@ __global__ void __launch_bounds__(256) k_mg_smooth_first(T* __restrict__ x, cons
     x[c] = (((i + j + k) & 1) == 0) ? ((T)0 - b[c]) * (T)L.idg : (T)0;
 }
 
-// residual of the fine children summed into the coarse right-hand side
-template <typename T>
-__global__ void __launch_bounds__(256) k_mg_resid_restrict(const T* __restrict__ x, const T* __restrict__ b,
-                                                           T* __restrict__ cres, LevelGeom F, LevelGeom C, int r0,
-                                                           int r1, int r2, T w)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long tot = (long long)C.n0 * C.n1 * C.n2;
-    if (t >= tot) return;
-    const unsigned tt = (unsigned)t, q = tt / (unsigned)C.n0;
-    int I = (int)(tt - q * (unsigned)C.n0);
-    int J = (int)(q % (unsigned)C.n1), K = (int)(q / (unsigned)C.n1);
-    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
-    T s = (T)0;
-    for (int dk = 0; dk < r2; ++dk)
-        for (int dj = 0; dj < r1; ++dj)
-            for (int di = 0; di < r0; ++di) {
-                int i = I * r0 + di, j = J * r1 + dj, k = K * r2 + dk;
-                long long c = LIdx(F, i, j, k);
-                T v = b[c] - (stencil<T>(x, c, i, j, k, F, cx, cy, cz, x[c]) - dg * x[c]);
-                s += v;
-            }
-    cres[LIdx(C, I, J, K)] = s * w;
-}
-
 // piecewise-constant prolongation fused with the first red half sweep: red
 // cells are relaxed against their prolonged black neighbours (x + xc(parent),
 // formed on the fly).  Black cells are not written: the black half sweep
@@ -276,71 +239,168 @@ __global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const
     f[LIdx(F, i, j, k)] += c[LIdx(C, i / r0, j / r1, k / r2)];
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v)
+template <typename T>
+struct Pair;
+template <>
+struct Pair<float> {
+    using type = float2;
+};
+template <>
+struct Pair<double> {
+    using type = double2;
+};
+
+// one full red-black sweep (red, then black) over a grid level, x_in -> x_out.
+// Plane p of the old iterate lives in X[(p - k0 + 2) & 3], of the rhs in
+// B[(p - k0 + 1) % 3] and of the relaxed red cells in R[(p - k0 + 1) % 3].
+// Global loads run two planes ahead in two register sets (steps alternate
+// between them), so each load has two plane steps to arrive.
+template <typename T, int MODE>
+__device__ __forceinline__ void rb_step(const SweepArgs<T>& a, const SwThread& th, T (*X)[kSwXPlane],
+                                        T (*B)[kSwPlane], T (*R)[kSwPlane], T* xv, T* bv, int tid, int k0, int k1,
+                                        int k)
 {
-    // non-negative doubles order like their bit patterns
-    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+    auto xs = [&](int p) { return (p - k0 + 2) & 3; };
+    auto bs = [&](int p) { return (p - k0 + 1) % 3; };
+    sw_store_x<T>(X[xs(k + 1)], xv, tid);
+    sw_store_b<T>(B[bs(k)], bv, tid);
+    if (k + 3 <= k1 + 1) sw_load_x<T, MODE>(a, th, xv, k + 3);
+    if (k + 2 <= k1) sw_load_b<T>(a, th, bv, k + 2);
+    __syncthreads();
+    sw_phase_red<T>(a.L, th, X[xs(k - 1)], X[xs(k)], X[xs(k + 1)], B[bs(k)], R[bs(k)], k);
+    __syncthreads();
+    if (k > k0) {
+        const int kb = k - 1;
+        long long at;
+        T v0, v1;
+        if (sw_phase_black<T>(a.L, th, R[bs(kb - 1)], R[bs(kb)], R[bs(k)], X[xs(kb)], B[bs(kb)], kb, at, v0, v1)) {
+            typename Pair<T>::type v;
+            v.x = v0;
+            v.y = v1;
+            *reinterpret_cast<typename Pair<T>::type*>(a.xout + at) = v;
+        }
+    }
 }
 
-__device__ __forceinline__ void block_max_atomic(double m, unsigned long long* norm)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a)
 {
-    __shared__ double sh[16];
+    __shared__ T X[4][kSwXPlane];
+    __shared__ T B[3][kSwPlane];
+    __shared__ T R[3][kSwPlane];
+    int tx, ty, kc;
+    sweep_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), a.ntx, a.nty, tx, ty, kc);
+    const int tid = (int)threadIdx.x;
+    SwThread th;
+    sw_thread_init<T>(a, th, tid, tx, ty);
+    const int k0 = kc * a.kz;
+    const int k1 = min(k0 + a.kz, a.L.n2);
+    T xa[kSwXPer], ba[kSwBPer], xb[kSwXPer], bb[kSwBPer];
+    sw_load_x<T, MODE>(a, th, xa, k0 - 2);
+    sw_store_x<T>(X[0], xa, tid);
+    sw_load_x<T, MODE>(a, th, xa, k0 - 1);
+    sw_store_x<T>(X[1], xa, tid);
+    sw_load_x<T, MODE>(a, th, xa, k0);
+    sw_load_b<T>(a, th, ba, k0 - 1);
+    sw_load_x<T, MODE>(a, th, xb, k0 + 1);
+    sw_load_b<T>(a, th, bb, k0);
+    for (int k = k0 - 1; k <= k1; k += 2) {
+        rb_step<T, MODE>(a, th, X, B, R, xa, ba, tid, k0, k1, k);
+        if (k + 1 <= k1) rb_step<T, MODE>(a, th, X, B, R, xb, bb, tid, k0, k1, k + 1);
+    }
+}
+
+// residual + restriction over row tiles of the coarse level
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__ x, const T* __restrict__ b,
+                                                           T* __restrict__ cres, LevelGeom F, LevelGeom C, RowTiles t,
+                                                           int r0, int r1, int r2, T w)
+{
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = lid % t.ntx, q = lid / t.ntx, ty = q % t.nty, kc = q / t.nty;
+    const int I = tx * kRowTX + ((int)threadIdx.x & (kRowTX - 1)), J = ty * kRowTY + ((int)threadIdx.x / kRowTX);
+    if (I >= C.n0 || J >= C.n1) return;
+    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz;
+    const int K1 = min((kc + 1) * t.kz, C.n2);
+    for (int K = kc * t.kz; K < K1; ++K) {
+        auto st = [&](long long c, int i, int j, int k, T self) { return stencil<T>(x, c, i, j, k, F, cx, cy, cz, self); };
+        cres[LIdx(C, I, J, K)] = restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w, st);
+    }
+}
+
+// block maximum of non-negative values into part[blockIdx.x]
+__device__ __forceinline__ void block_max_part(double m, double* __restrict__ part)
+{
+    __shared__ double sh[4];
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = sh[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, sh[w]);
-        atomic_max_nonneg(norm, r);
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-constexpr int kReduceBlocks = 2048;
-
-// FP64 outer residual.  With a correction: phi_out = phi + cor and
-// r = rhs - A(phi + cor) in one pass (ping-pong phi buffers); without:
-// r = rhs - A phi.  r is written in precision R (FP32 feeds the mixed
-// V-cycle directly) and |r|_inf goes to norm.
+// FP64 outer residual over row tiles.  With a correction: phi_out = phi +
+// cor and r = rhs - A(phi + cor) in one pass (ping-pong phi buffers);
+// without: r = rhs - A phi.  r is written in precision R (FP32 feeds the
+// mixed V-cycle directly); per-block maxima of |r| go to part.
 template <typename C, typename R>
 __global__ void __launch_bounds__(256) k_update_residual(const double* __restrict__ phi, const C* __restrict__ cor,
                                                          double* __restrict__ phi_out, const double* __restrict__ rhs,
-                                                         R* __restrict__ r, LevelGeom L,
-                                                         unsigned long long* __restrict__ norm)
+                                                         R* __restrict__ r, LevelGeom L, RowTiles tl,
+                                                         double* __restrict__ part)
 {
-    const long long tot = (long long)L.n0 * L.n1 * L.n2;
     const int gm = L.ghost_mode;
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = lid % tl.ntx, q = lid / tl.ntx, ty = q % tl.nty, kc = q / tl.nty;
+    const int i = tx * kRowTX + ((int)threadIdx.x & (kRowTX - 1)), j = ty * kRowTY + ((int)threadIdx.x / kRowTX);
     double m = 0.0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
-        const unsigned tt = (unsigned)t, q = tt / (unsigned)L.n0;
-        int i = (int)(tt - q * (unsigned)L.n0);
-        int j = (int)(q % (unsigned)L.n1), k = (int)(q / (unsigned)L.n1);
-        long long c = LIdx(L, i, j, k);
-        double v;
-        if (cor) {
-            // neighbour values of phi + cor, boundary handled on the fly
-            auto val = [&](long long cc) { return phi[cc] + (double)cor[cc]; };
-            double self = val(c);
-            auto lo = [&](int pos, int n, long long s, int bc) {
-                if (gm || pos > 0) return val(c - s);
-                return bc == KSGF_LO_PERIODIC ? val(c + (long long)(n - 1) * s) : (bc == KSGF_LO_NEUMANN ? self : -self);
-            };
-            auto hi = [&](int pos, int n, long long s, int bc) {
-                if (gm || pos < n - 1) return val(c + s);
-                return bc == KSGF_LO_PERIODIC ? val(c - (long long)(n - 1) * s) : (bc == KSGF_LO_NEUMANN ? self : -self);
-            };
-            double st = L.cx * (lo(i, L.n0, 1, L.lo[0]) + hi(i, L.n0, 1, L.hi[0])) +
-                        L.cy * (lo(j, L.n1, L.sy, L.lo[1]) + hi(j, L.n1, L.sy, L.hi[1])) +
-                        L.cz * (lo(k, L.n2, L.sz, L.lo[2]) + hi(k, L.n2, L.sz, L.hi[2]));
-            v = rhs[c] - (st - L.dg * self);
-            phi_out[c] = self;
-        } else {
-            v = rhs[c] - (stencil<double>(phi, c, i, j, k, L, L.cx, L.cy, L.cz, phi[c]) - L.dg * phi[c]);
+    if (i < L.n0 && j < L.n1) {
+        const int k1 = min((kc + 1) * tl.kz, L.n2);
+        for (int k = kc * tl.kz; k < k1; ++k) {
+            const long long c = LIdx(L, i, j, k);
+            double v;
+            if (cor) {
+                // neighbour values of phi + cor, boundary handled on the fly
+                auto val = [&](long long cc) { return phi[cc] + (double)cor[cc]; };
+                const double self = val(c);
+                auto lo = [&](int pos, int n, long long s, int bc) {
+                    if (gm || pos > 0) return val(c - s);
+                    return bc == KSGF_LO_PERIODIC ? val(c + (long long)(n - 1) * s)
+                                                  : (bc == KSGF_LO_NEUMANN ? self : -self);
+                };
+                auto hi = [&](int pos, int n, long long s, int bc) {
+                    if (gm || pos < n - 1) return val(c + s);
+                    return bc == KSGF_LO_PERIODIC ? val(c - (long long)(n - 1) * s)
+                                                  : (bc == KSGF_LO_NEUMANN ? self : -self);
+                };
+                const double st = L.cx * (lo(i, L.n0, 1, L.lo[0]) + hi(i, L.n0, 1, L.hi[0])) +
+                                  L.cy * (lo(j, L.n1, L.sy, L.lo[1]) + hi(j, L.n1, L.sy, L.hi[1])) +
+                                  L.cz * (lo(k, L.n2, L.sz, L.lo[2]) + hi(k, L.n2, L.sz, L.hi[2]));
+                v = rhs[c] - (st - L.dg * self);
+                phi_out[c] = self;
+            } else {
+                v = rhs[c] - (stencil<double>(phi, c, i, j, k, L, L.cx, L.cy, L.cz, phi[c]) - L.dg * phi[c]);
+            }
+            r[c] = (R)v;
+            m = fmax(m, fabs(v));
         }
-        r[c] = (R)v;
-        m = fmax(m, fabs(v));
     }
-    if (norm) block_max_atomic(m, norm);
+    block_max_part(m, part);
 }
+
+// maximum of the n partials into *norm (as the bits of a non-negative double)
+__global__ void __launch_bounds__(256) k_max_parts(const double* __restrict__ part, int n,
+                                                   unsigned long long* __restrict__ norm)
+{
+    double m = 0.0;
+    for (int t = (int)threadIdx.x; t < n; t += (int)blockDim.x) m = fmax(m, part[t]);
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *norm = (unsigned long long)__double_as_longlong(fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3])));
+}
+
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_fill(T* __restrict__ a, long long n, T v)
@@ -356,11 +416,6 @@ __global__ void __launch_bounds__(256) k_convert(A* __restrict__ a, const B* __r
     if (t < n) a[t] = (A)b[t];
 }
 
-__global__ void k_zero_u64(unsigned long long* p, int n)
-{
-    int t = threadIdx.x;
-    if (t < n) p[t] = 0ull;
-}
 
 // deterministic sum of interior cells: per-block partial sums
 constexpr int kSumBlocks = 512;
@@ -718,8 +773,10 @@ struct ksgf_macproj {
     int lc = 0;  // first level handled by the single-workgroup coarse kernel
     std::vector<LevelGeom> G;
     std::vector<int> ratio;               // 3 per level (to the next)
-    std::vector<double*> cor, res;        // FP64 per level (FP64 mode)
-    std::vector<float*> cor32, res32;     // FP32 per level (mixed mode)
+    std::vector<double*> cor, tmp, res;   // FP64 per level (FP64 mode); tmp: sweep ping-pong partner
+    std::vector<float*> cor32, tmp32, res32;  // FP32 per level (mixed mode)
+    std::vector<int> kz_sweep;            // autotuned planes per workgroup of the fused sweep
+    double* rpart = nullptr;              // per-block maxima of the outer residual
     LevelPtrs<double>* dlp64 = nullptr;   // device level tables for the coarse kernel
     LevelPtrs<float>* dlp32 = nullptr;
     double* phi = nullptr;
@@ -788,10 +845,10 @@ static void launch_ghosts(T* a, const LevelGeom& L, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// live kernel timing for the bench: HIP events around every launch of the
-// finest-level half-sweep smoother on the launch stream, with its
-// algorithmic bytes (a red or black half sweep counts half of the 24 B/cell
-// FP64 (12 B/cell FP32) of a full read-phi/read-rhs/write-phi sweep)
+// live kernel timing for the bench: HIP events on the launch stream around
+// every finest-level fused sweep, with its algorithmic bytes (read x_in and
+// rhs, write x_out: 3 words per cell; a zero-initial-guess sweep reads no
+// x_in; a prolongation sweep also reads the coarse correction once)
 // ---------------------------------------------------------------------------
 struct KernelProf {
     bool on = false;
@@ -799,7 +856,7 @@ struct KernelProf {
     size_t used = 0;
     double bytes = 0.0;
     long long launches = 0;
-    std::string name = "k_mg_smooth(level 0)";
+    std::string name = "k_rb_sweep(level 0)";
 };
 static KernelProf g_prof;
 
@@ -814,9 +871,10 @@ static hipEvent_t prof_event()
 }
 
 // half sweeps number `first` .. 2*nu-1 of nu red-black sweeps (half sweep
-// 2s is red, 2s+1 black)
+// 2s is red, 2s+1 black), in place; used on levels the fused sweep does not
+// take (odd extents, explicit ghosts, odd post-smoothing counts)
 template <typename T>
-static void launch_half_sweeps(T* x, const T* b, const LevelGeom& L, int first, int nu, hipStream_t st, bool prof,
+static void launch_half_sweeps(T* x, const T* b, const LevelGeom& L, int first, int nu, hipStream_t st,
                                const T* xc = nullptr, const LevelGeom* C = nullptr, const int* r = nullptr)
 {
     long long n = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
@@ -826,20 +884,8 @@ static void launch_half_sweeps(T* x, const T* b, const LevelGeom& L, int first, 
         const int r0 = r ? r[0] : 1, r1 = r ? r[1] : 1, r2 = r ? r[2] : 1;
         int color = h & 1;
         if (L.ghost_mode) launch_ghosts<T>(x, L, st);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (prof && g_prof.on) {
-            e0 = prof_event();
-            e1 = prof_event();
-            if (e0 && e1) (void)hipEventRecord(e0, st);
-        }
         hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, x, b, L, color, xch, Cg, r0, r1,
                            r2);
-        if (e0 && e1) {
-            (void)hipEventRecord(e1, st);
-            g_prof.launches += 1;
-            g_prof.bytes += 0.5 * 3.0 * sizeof(T) * (double)L.n0 * L.n1 * L.n2;
-            g_prof.name = sizeof(T) == 8 ? "k_mg_smooth<double>(level 0)" : "k_mg_smooth<float>(level 0)";
-        }
     }
 }
 
@@ -855,9 +901,123 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
     hipLaunchKernelGGL(k_sub_mean, dim3(kSumBlocks), dim3(256), 0, st, a, L, (const double*)part, kSumBlocks);
 }
 
+// planes per workgroup of the row-tile kernels
+constexpr int kRestrictKz = 4;
+constexpr int kResidualKz = 8;
+
+// levels the fused out-of-place sweep handles: even x extent (aligned
+// pairs), no explicit ghosts (odd periodic extents), and an even number of
+// post-smoothing sweeps (the first one reads the pre-smoothed iterate and
+// must not write it)
+static bool fused_level(const ksgf_macproj* mp, int l)
+{
+    const LevelGeom& L = mp->G[l];
+    return !L.ghost_mode && (L.n0 % 2) == 0 && mp->opt.num_pre_smooth >= 1 && mp->opt.num_post_smooth >= 2 &&
+           (mp->opt.num_post_smooth % 2) == 0;
+}
+
+template <typename T>
+static SweepArgs<T> sweep_args(const ksgf_macproj* mp, int l, const T* xin, T* xout, const T* b, const T* xc, int kz,
+                               int* nblocks)
+{
+    SweepArgs<T> a{};
+    a.xin = xin;
+    a.xout = xout;
+    a.b = b;
+    a.xc = xc;
+    a.L = mp->G[l];
+    a.C = (l + 1 < mp->nlev) ? mp->G[l + 1] : mp->G[l];
+    const int* r = &mp->ratio[3 * l];
+    a.s0 = (r[0] == 2);
+    a.s1 = (r[1] == 2);
+    a.s2 = (r[2] == 2);
+    *nblocks = sweep_blocks(a, kz);
+    return a;
+}
+
+template <typename T>
+static void launch_sweep_kernel(const SweepArgs<T>& a, int nb, int mode, hipStream_t st)
+{
+    if (mode == kSwZero)
+        hipLaunchKernelGGL((k_rb_sweep<T, kSwZero>), dim3(nb), dim3(kSwThreads), 0, st, a);
+    else if (mode == kSwProlong)
+        hipLaunchKernelGGL((k_rb_sweep<T, kSwProlong>), dim3(nb), dim3(kSwThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_rb_sweep<T, kSwRead>), dim3(nb), dim3(kSwThreads), 0, st, a);
+}
+
+template <typename T>
+static void launch_sweep(ksgf_macproj* mp, int l, int mode, const T* xin, T* xout, const T* b, const T* xc,
+                         hipStream_t st)
+{
+    int nb;
+    SweepArgs<T> a = sweep_args<T>(mp, l, xin, xout, b, xc, mp->kz_sweep[l], &nb);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (l == 0 && g_prof.on) {
+        e0 = prof_event();
+        e1 = prof_event();
+        if (e0 && e1) (void)hipEventRecord(e0, st);
+    }
+    launch_sweep_kernel<T>(a, nb, mode, st);
+    if (e0 && e1) {
+        (void)hipEventRecord(e1, st);
+        const double n = (double)a.L.n0 * a.L.n1 * a.L.n2, nc = (double)a.C.n0 * a.C.n1 * a.C.n2;
+        g_prof.launches += 1;
+        g_prof.bytes += sizeof(T) * ((mode == kSwZero ? 2.0 : 3.0) * n + (mode == kSwProlong ? nc : 0.0));
+        g_prof.name = sizeof(T) == 8 ? "k_rb_sweep<double>(level 0)" : "k_rb_sweep<float>(level 0)";
+    }
+}
+
+// Autotuner for the planes per workgroup of the fused sweep: every
+// candidate is timed on the level's own arrays (a sweep from cor into the
+// scratch array, which the solve overwrites before reading), the fastest
+// is kept per (level shape, precision) for the life of the process.
+static std::map<std::array<long long, 4>, int>& kz_cache()
+{
+    static std::map<std::array<long long, 4>, int> m;
+    return m;
+}
+
+template <typename T>
+static int tune_sweep(ksgf_macproj* mp, int l, T* x, T* tmp, const T* b)
+{
+    const LevelGeom& L = mp->G[l];
+    const std::array<long long, 4> key{L.n0, L.n1, L.n2, (long long)sizeof(T)};
+    auto it = kz_cache().find(key);
+    if (it != kz_cache().end()) return it->second;
+    const int cand[4] = {4, 8, 16, 32};
+    int best = 8;
+    float best_ms = 1e30f;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return best;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return best;
+    }
+    for (int c : cand) {
+        if (c > 2 * L.n2 && c != cand[0]) continue;
+        int nb;
+        SweepArgs<T> a = sweep_args<T>(mp, l, x, tmp, b, nullptr, c, &nb);
+        launch_sweep_kernel<T>(a, nb, kSwRead, nullptr);
+        (void)hipEventRecord(e0, nullptr);
+        for (int rep = 0; rep < 3; ++rep) launch_sweep_kernel<T>(a, nb, kSwRead, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    kz_cache()[key] = best;
+    return best;
+}
+
 // V-cycle in precision T from level l down (levels >= lc in one workgroup)
 template <typename T>
-static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& res,
+static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& tmp, std::vector<T*>& res,
                    const LevelPtrs<T>* dlp, hipStream_t st)
 {
     if (l >= mp->lc) {
@@ -870,31 +1030,44 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     const LevelGeom& L = mp->G[l];
     const LevelGeom& C = mp->G[l + 1];
     const int* r = &mp->ratio[3 * l];
-    const bool prof = (l == 0);
     const long long nf = (long long)L.n0 * L.n1 * L.n2;
-    const long long nc = (long long)C.n0 * C.n1 * C.n2;
     const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+    const bool fused = fused_level(mp, l);
+    // sweep s of nu writes the array that makes sweep nu-1 land in cor[l]
+    auto out = [&](int s, int nu) { return ((nu - 1 - s) % 2 == 0) ? cor[l] : tmp[l]; };
     // down
-    if (mp->opt.num_pre_smooth > 0 && !L.ghost_mode) {
+    const int nu1 = mp->opt.num_pre_smooth, nu2 = mp->opt.num_post_smooth;
+    if (fused) {
+        for (int s = 0; s < nu1; ++s)
+            launch_sweep<T>(mp, l, s == 0 ? kSwZero : kSwRead, s == 0 ? nullptr : out(s - 1, nu1), out(s, nu1),
+                            res[l], nullptr, st);
+    } else if (nu1 > 0 && !L.ghost_mode) {
         hipLaunchKernelGGL(k_mg_smooth_first<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)res[l], L);
-        launch_half_sweeps<T>(cor[l], res[l], L, 1, mp->opt.num_pre_smooth, st, prof);
+        launch_half_sweeps<T>(cor[l], res[l], L, 1, nu1, st);
     } else {
         launch_fill<T>(cor[l], L.tot, (T)0, st);
-        launch_half_sweeps<T>(cor[l], res[l], L, 0, mp->opt.num_pre_smooth, st, prof);
+        launch_half_sweeps<T>(cor[l], res[l], L, 0, nu1, st);
     }
     if (L.ghost_mode) launch_ghosts<T>(cor[l], L, st);
-    hipLaunchKernelGGL(k_mg_resid_restrict<T>, dim3(grid_for(nc, 256)), dim3(256), 0, st, (const T*)cor[l],
-                       (const T*)res[l], res[l + 1], L, C, r[0], r[1], r[2], w);
-    vcycle<T>(mp, l + 1, cor, res, dlp, st);
+    {
+        RowTiles t = row_tiles(C.n0, C.n1, C.n2, kRestrictKz);
+        hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, (const T*)cor[l],
+                           (const T*)res[l], res[l + 1], L, C, t, r[0], r[1], r[2], w);
+    }
+    vcycle<T>(mp, l + 1, cor, tmp, res, dlp, st);
     // up
-    if (mp->opt.num_post_smooth > 0 && !L.ghost_mode) {
+    if (fused) {
+        for (int s = 0; s < nu2; ++s)
+            launch_sweep<T>(mp, l, s == 0 ? kSwProlong : kSwRead, s == 0 ? cor[l] : out(s - 1, nu2), out(s, nu2),
+                            res[l], s == 0 ? cor[l + 1] : nullptr, st);
+    } else if (nu2 > 0 && !L.ghost_mode) {
         hipLaunchKernelGGL(k_mg_prolong_red<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
                            (const T*)res[l], L, C, r[0], r[1], r[2]);
-        launch_half_sweeps<T>(cor[l], res[l], L, 1, mp->opt.num_post_smooth, st, prof, (const T*)cor[l + 1], &C, r);
+        launch_half_sweeps<T>(cor[l], res[l], L, 1, nu2, st, (const T*)cor[l + 1], &C, r);
     } else {
         hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
                            L, C, r[0], r[1], r[2]);
-        launch_half_sweeps<T>(cor[l], res[l], L, 0, mp->opt.num_post_smooth, st, prof);
+        launch_half_sweeps<T>(cor[l], res[l], L, 0, nu2, st);
     }
 }
 
@@ -913,25 +1086,24 @@ template <typename C, typename R>
 static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStream_t st)
 {
     const LevelGeom& L0 = mp->G[0];
-    long long n = (long long)L0.n0 * L0.n1 * L0.n2;
-    int nb = grid_for(n, 256);
-    if (nb > kReduceBlocks) nb = kReduceBlocks;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, mp->dnorm, 1);
+    const RowTiles t = row_tiles(L0.n0, L0.n1, L0.n2, kResidualKz);
+    const int nb = t.ntx * t.nty * t.nkc;
     if (L0.ghost_mode) {
         // explicit ghosts: phi += cor first, then the residual from ghosts
         if (cor) {
             hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
-                               mp->phi2, (const double*)mp->rhs, r, L0, (unsigned long long*)nullptr);
+                               mp->phi2, (const double*)mp->rhs, r, L0, t, mp->rpart);
             std::swap(mp->phi, mp->phi2);
         }
         launch_ghosts<double>(mp->phi, L0, st);
         hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
-                           (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, L0, mp->dnorm);
-        return;
+                           (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, L0, t, mp->rpart);
+    } else {
+        hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor, mp->phi2,
+                           (const double*)mp->rhs, r, L0, t, mp->rpart);
+        if (cor) std::swap(mp->phi, mp->phi2);
     }
-    hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor, mp->phi2,
-                       (const double*)mp->rhs, r, L0, mp->dnorm);
-    if (cor) std::swap(mp->phi, mp->phi2);
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(256), 0, st, (const double*)mp->rpart, nb, mp->dnorm);
 }
 
 // solve with the level-0 rhs in mp->rhs (ghost layout); phi in mp->phi
@@ -960,10 +1132,10 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
     if (!inf.converged) {
         for (int it = 1; it <= mp->opt.maxiter; ++it) {
             if (mixed) {
-                vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->dlp32, st);
+                vcycle<float>(mp, 0, mp->cor32, mp->tmp32, mp->res32, mp->dlp32, st);
                 launch_update_residual<float, float>(mp, mp->cor32[0], mp->res32[0], st);
             } else {
-                vcycle<double>(mp, 0, mp->cor, mp->res, mp->dlp64, st);
+                vcycle<double>(mp, 0, mp->cor, mp->tmp, mp->res, mp->dlp64, st);
                 launch_update_residual<double, double>(mp, mp->cor[0], mp->res[0], st);
             }
             KSGF_LAUNCH_CHECK();
@@ -1044,6 +1216,9 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     for (auto* p : mp->res) if (p) (void)hipFree(p);
     for (auto* p : mp->cor32) if (p) (void)hipFree(p);
     for (auto* p : mp->res32) if (p) (void)hipFree(p);
+    for (auto* p : mp->tmp) if (p) (void)hipFree(p);
+    for (auto* p : mp->tmp32) if (p) (void)hipFree(p);
+    if (mp->rpart) (void)hipFree(mp->rpart);
     double* bws[8] = {mp->bw.r, mp->bw.rh, mp->bw.p, mp->bw.v, mp->bw.s, mp->bw.t, mp->bw.x, mp->bw.b};
     for (auto* p : bws) if (p) (void)hipFree(p);
     if (mp->bw.stats) (void)hipFree(mp->bw.stats);
@@ -1114,9 +1289,7 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         L.n0 = cn[0];
         L.n1 = cn[1];
         L.n2 = cn[2];
-        L.sy = cn[0] + 2;
-        L.sz = (long long)(cn[0] + 2) * (cn[1] + 2);
-        L.tot = L.sz * (cn[2] + 2);
+        level_layout(L);
         L.ghost_mode = 0;
         for (int d = 0; d < 3; ++d) {
             L.lo[d] = lobc[d];
@@ -1170,20 +1343,21 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     for (int l = 0; l < mp->nlev; ++l) {
         size_t n = (size_t)mp->G[l].tot;
         void *a, *b;
+        void* c = nullptr;
+        const bool grid = (l < mp->lc);  // the ping-pong partner is only needed on grid levels
+        const size_t w = mixed ? sizeof(float) : sizeof(double);
+        if ((e = mp_alloc(&a, n * w)) || (e = mp_alloc(&b, n * w)) || (grid && (e = mp_alloc(&c, n * w)))) {
+            ksgf_macproj_destroy(mp);
+            return e;
+        }
         if (mixed) {
-            if ((e = mp_alloc(&a, n * sizeof(float))) || (e = mp_alloc(&b, n * sizeof(float)))) {
-                ksgf_macproj_destroy(mp);
-                return e;
-            }
             mp->cor32.push_back((float*)a);
             mp->res32.push_back((float*)b);
+            mp->tmp32.push_back((float*)c);
         } else {
-            if ((e = mp_alloc(&a, n * sizeof(double))) || (e = mp_alloc(&b, n * sizeof(double)))) {
-                ksgf_macproj_destroy(mp);
-                return e;
-            }
             mp->cor.push_back((double*)a);
             mp->res.push_back((double*)b);
+            mp->tmp.push_back((double*)c);
         }
     }
     void* p;
@@ -1196,6 +1370,11 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     mp->rhs = (double*)p;
     if ((e = mp_alloc(&p, kSumBlocks * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
     mp->part = (double*)p;
+    {
+        const RowTiles t = row_tiles(mp->G[0].n0, mp->G[0].n1, mp->G[0].n2, kResidualKz);
+        if ((e = mp_alloc(&p, (size_t)t.ntx * t.nty * t.nkc * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
+        mp->rpart = (double*)p;
+    }
     double** bws[8] = {&mp->bw.r, &mp->bw.rh, &mp->bw.p, &mp->bw.v, &mp->bw.s, &mp->bw.t, &mp->bw.x, &mp->bw.b};
     for (auto pp : bws) {
         if ((e = mp_alloc(&p, (size_t)Lb.tot * sizeof(double)))) {
@@ -1224,6 +1403,12 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     if ((e = upload_level_tables(mp))) {
         ksgf_macproj_destroy(mp);
         return e;
+    }
+    mp->kz_sweep.assign(mp->nlev, 8);
+    for (int l = 0; l < mp->lc; ++l) {
+        if (!fused_level(mp, l)) continue;
+        mp->kz_sweep[l] = mixed ? tune_sweep<float>(mp, l, mp->cor32[l], mp->tmp32[l], mp->res32[l])
+                                : tune_sweep<double>(mp, l, mp->cor[l], mp->tmp[l], mp->res[l]);
     }
     if (hipDeviceSynchronize() != hipSuccess) {
         ksgf_macproj_destroy(mp);
