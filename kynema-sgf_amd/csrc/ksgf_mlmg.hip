@@ -239,92 +239,50 @@ __global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const
     f[LIdx(F, i, j, k)] += c[LIdx(C, i / r0, j / r1, k / r2)];
 }
 
+// one red (color 0) or black (color 1) half sweep of a packed level over
+// row tiles of packed positions (i/2, j), kz planes per workgroup
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTiles t)
+{
+    int h, j, k0, k1;
+    row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, h, j, k0, k1, a.L.n2);
+    if (h >= a.L.prow || j >= a.L.n1) return;
+    for (int k = k0; k < k1; ++k) pk_relax<T, MODE>(a, color, h, j, k);
+}
+
+// x += prolongation of the coarse correction on a packed level (only when
+// there is no post-smoothing to fuse it into)
 template <typename T>
-struct Pair;
-template <>
-struct Pair<float> {
-    using type = float2;
-};
-template <>
-struct Pair<double> {
-    using type = double2;
-};
-
-// one full red-black sweep (red, then black) over a grid level, x_in -> x_out.
-// Plane p of the old iterate lives in X[(p - k0 + 2) & 3], of the rhs in
-// B[(p - k0 + 1) % 3] and of the relaxed red cells in R[(p - k0 + 1) % 3].
-// Global loads run two planes ahead in two register sets (steps alternate
-// between them), so each load has two plane steps to arrive.
-template <typename T, int MODE>
-__device__ __forceinline__ void rb_step(const SweepArgs<T>& a, const SwThread& th, T (*X)[kSwXPlane],
-                                        T (*B)[kSwPlane], T (*R)[kSwPlane], T* xv, T* bv, int tid, int k0, int k1,
-                                        int k)
+__global__ void __launch_bounds__(256) k_pk_prolong_add(T* __restrict__ x, const T* __restrict__ xc, LevelGeom F,
+                                                        LevelGeom C, int s0, int s1, int s2, RowTiles t)
 {
-    auto xs = [&](int p) { return (p - k0 + 2) & 3; };
-    auto bs = [&](int p) { return (p - k0 + 1) % 3; };
-    sw_store_x<T>(X[xs(k + 1)], xv, tid);
-    sw_store_b<T>(B[bs(k)], bv, tid);
-    if (k + 3 <= k1 + 1) sw_load_x<T, MODE>(a, th, xv, k + 3);
-    if (k + 2 <= k1) sw_load_b<T>(a, th, bv, k + 2);
-    __syncthreads();
-    sw_phase_red<T>(a.L, th, X[xs(k - 1)], X[xs(k)], X[xs(k + 1)], B[bs(k)], R[bs(k)], k);
-    __syncthreads();
-    if (k > k0) {
-        const int kb = k - 1;
-        long long at;
-        T v0, v1;
-        if (sw_phase_black<T>(a.L, th, R[bs(kb - 1)], R[bs(kb)], R[bs(k)], X[xs(kb)], B[bs(kb)], kb, at, v0, v1)) {
-            typename Pair<T>::type v;
-            v.x = v0;
-            v.y = v1;
-            *reinterpret_cast<typename Pair<T>::type*>(a.xout + at) = v;
-        }
-    }
+    int i, j, k0, k1;
+    row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, i, j, k0, k1, F.n2);
+    if (i >= F.n0 || j >= F.n1) return;
+    for (int k = k0; k < k1; ++k) x[PIdx(F, i, j, k)] += xc[AnyIdx(C, i >> s0, j >> s1, k >> s2)];
 }
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a)
-{
-    __shared__ T X[4][kSwXPlane];
-    __shared__ T B[3][kSwPlane];
-    __shared__ T R[3][kSwPlane];
-    int tx, ty, kc;
-    sweep_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), a.ntx, a.nty, tx, ty, kc);
-    const int tid = (int)threadIdx.x;
-    SwThread th;
-    sw_thread_init<T>(a, th, tid, tx, ty);
-    const int k0 = kc * a.kz;
-    const int k1 = min(k0 + a.kz, a.L.n2);
-    T xa[kSwXPer], ba[kSwBPer], xb[kSwXPer], bb[kSwBPer];
-    sw_load_x<T, MODE>(a, th, xa, k0 - 2);
-    sw_store_x<T>(X[0], xa, tid);
-    sw_load_x<T, MODE>(a, th, xa, k0 - 1);
-    sw_store_x<T>(X[1], xa, tid);
-    sw_load_x<T, MODE>(a, th, xa, k0);
-    sw_load_b<T>(a, th, ba, k0 - 1);
-    sw_load_x<T, MODE>(a, th, xb, k0 + 1);
-    sw_load_b<T>(a, th, bb, k0);
-    for (int k = k0 - 1; k <= k1; k += 2) {
-        rb_step<T, MODE>(a, th, X, B, R, xa, ba, tid, k0, k1, k);
-        if (k + 1 <= k1) rb_step<T, MODE>(a, th, X, B, R, xb, bb, tid, k0, k1, k + 1);
-    }
-}
-
-// residual + restriction over row tiles of the coarse level
+// residual + restriction over row tiles of the coarse level; a packed fine
+// level may restrict into a packed or a natural coarse level, a natural one
+// only into a natural one
 template <typename T>
 __global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__ x, const T* __restrict__ b,
                                                            T* __restrict__ cres, LevelGeom F, LevelGeom C, RowTiles t,
                                                            int r0, int r1, int r2, T w)
 {
-    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int tx = lid % t.ntx, q = lid / t.ntx, ty = q % t.nty, kc = q / t.nty;
-    const int I = tx * kRowTX + ((int)threadIdx.x & (kRowTX - 1)), J = ty * kRowTY + ((int)threadIdx.x / kRowTX);
+    int I, J, K0, K1;
+    row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, I, J, K0, K1, C.n2);
     if (I >= C.n0 || J >= C.n1) return;
     const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz;
-    const int K1 = min((kc + 1) * t.kz, C.n2);
-    for (int K = kc * t.kz; K < K1; ++K) {
-        auto st = [&](long long c, int i, int j, int k, T self) { return stencil<T>(x, c, i, j, k, F, cx, cy, cz, self); };
-        cres[LIdx(C, I, J, K)] = restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w, st);
+    for (int K = K0; K < K1; ++K) {
+        if (F.packed) {
+            cres[AnyIdx(C, I, J, K)] = pk_restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w);
+        } else {
+            auto st = [&](long long c, int i, int j, int k, T self) {
+                return stencil<T>(x, c, i, j, k, F, cx, cy, cz, self);
+            };
+            cres[LIdx(C, I, J, K)] = restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w, st);
+        }
     }
 }
 
@@ -338,49 +296,56 @@ __device__ __forceinline__ void block_max_part(double m, double* __restrict__ pa
     if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-// FP64 outer residual over row tiles.  With a correction: phi_out = phi +
-// cor and r = rhs - A(phi + cor) in one pass (ping-pong phi buffers);
-// without: r = rhs - A phi.  r is written in precision R (FP32 feeds the
-// mixed V-cycle directly); per-block maxima of |r| go to part.
+// FP64 outer residual over row tiles.  phi and rhs are FP64 in the natural
+// layout P; the correction cor and the residual r are in the level-0 layout
+// L (packed or natural).  With a correction: phi_out = phi + cor and r = rhs
+// - A(phi + cor) in one pass (ping-pong phi buffers); without: r = rhs - A
+// phi.  r is written in precision R (FP32 feeds the mixed V-cycle
+// directly); per-block maxima of |r| go to part.
 template <typename C, typename R>
 __global__ void __launch_bounds__(256) k_update_residual(const double* __restrict__ phi, const C* __restrict__ cor,
                                                          double* __restrict__ phi_out, const double* __restrict__ rhs,
-                                                         R* __restrict__ r, LevelGeom L, RowTiles tl,
+                                                         R* __restrict__ r, LevelGeom P, LevelGeom L, RowTiles tl,
                                                          double* __restrict__ part)
 {
-    const int gm = L.ghost_mode;
-    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int tx = lid % tl.ntx, q = lid / tl.ntx, ty = q % tl.nty, kc = q / tl.nty;
-    const int i = tx * kRowTX + ((int)threadIdx.x & (kRowTX - 1)), j = ty * kRowTY + ((int)threadIdx.x / kRowTX);
+    int i, j, k0, k1;
+    row_tile(tl, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, i, j, k0, k1, P.n2);
     double m = 0.0;
-    if (i < L.n0 && j < L.n1) {
-        const int k1 = min((kc + 1) * tl.kz, L.n2);
-        for (int k = kc * tl.kz; k < k1; ++k) {
-            const long long c = LIdx(L, i, j, k);
+    if (i < P.n0 && j < P.n1) {
+        const bool px = P.lo[0] == KSGF_LO_PERIODIC, py = P.lo[1] == KSGF_LO_PERIODIC, pz = P.lo[2] == KSGF_LO_PERIODIC;
+        for (int k = k0; k < k1; ++k) {
+            const long long c = LIdx(P, i, j, k);
+            const long long q = AnyIdx(L, i, j, k);
             double v;
-            if (cor) {
-                // neighbour values of phi + cor, boundary handled on the fly
-                auto val = [&](long long cc) { return phi[cc] + (double)cor[cc]; };
-                const double self = val(c);
-                auto lo = [&](int pos, int n, long long s, int bc) {
-                    if (gm || pos > 0) return val(c - s);
-                    return bc == KSGF_LO_PERIODIC ? val(c + (long long)(n - 1) * s)
-                                                  : (bc == KSGF_LO_NEUMANN ? self : -self);
-                };
-                auto hi = [&](int pos, int n, long long s, int bc) {
-                    if (gm || pos < n - 1) return val(c + s);
-                    return bc == KSGF_LO_PERIODIC ? val(c - (long long)(n - 1) * s)
-                                                  : (bc == KSGF_LO_NEUMANN ? self : -self);
-                };
-                const double st = L.cx * (lo(i, L.n0, 1, L.lo[0]) + hi(i, L.n0, 1, L.hi[0])) +
-                                  L.cy * (lo(j, L.n1, L.sy, L.lo[1]) + hi(j, L.n1, L.sy, L.hi[1])) +
-                                  L.cz * (lo(k, L.n2, L.sz, L.lo[2]) + hi(k, L.n2, L.sz, L.hi[2]));
-                v = rhs[c] - (st - L.dg * self);
-                phi_out[c] = self;
+            if (P.ghost_mode) {
+                // explicit ghosts (odd periodic extents): with a correction only
+                // phi += cor (the caller fills the ghosts and calls again without
+                // one); without, the plain residual from the filled ghosts
+                if (cor) {
+                    phi_out[c] = phi[c] + (double)cor[q];
+                    v = 0.0;
+                } else {
+                    v = rhs[c] - (stencil<double>(phi, c, i, j, k, P, P.cx, P.cy, P.cz, phi[c]) - P.dg * phi[c]);
+                }
             } else {
-                v = rhs[c] - (stencil<double>(phi, c, i, j, k, L, L.cx, L.cy, L.cz, phi[c]) - L.dg * phi[c]);
+                auto val = [&](int ii, int jj, int kk) {
+                    double x = phi[LIdx(P, ii, jj, kk)];
+                    if (cor) x = x + (double)cor[AnyIdx(L, ii, jj, kk)];
+                    return x;
+                };
+                const double self = cor ? phi[c] + (double)cor[q] : phi[c];
+                auto wall = [&](int bc) { return bc == KSGF_LO_NEUMANN ? self : -self; };
+                const double w = (i > 0 || px) ? val(i > 0 ? i - 1 : P.n0 - 1, j, k) : wall(P.lo[0]);
+                const double e = (i < P.n0 - 1 || px) ? val(i < P.n0 - 1 ? i + 1 : 0, j, k) : wall(P.hi[0]);
+                const double s = (j > 0 || py) ? val(i, j > 0 ? j - 1 : P.n1 - 1, k) : wall(P.lo[1]);
+                const double n = (j < P.n1 - 1 || py) ? val(i, j < P.n1 - 1 ? j + 1 : 0, k) : wall(P.hi[1]);
+                const double bo = (k > 0 || pz) ? val(i, j, k > 0 ? k - 1 : P.n2 - 1) : wall(P.lo[2]);
+                const double t = (k < P.n2 - 1 || pz) ? val(i, j, k < P.n2 - 1 ? k + 1 : 0) : wall(P.hi[2]);
+                const double st = P.cx * (w + e) + P.cy * (s + n) + P.cz * (bo + t);
+                v = rhs[c] - (st - P.dg * self);
+                if (cor) phi_out[c] = self;
             }
-            r[c] = (R)v;
+            r[q] = (R)v;
             m = fmax(m, fabs(v));
         }
     }
@@ -773,9 +738,10 @@ struct ksgf_macproj {
     int lc = 0;  // first level handled by the single-workgroup coarse kernel
     std::vector<LevelGeom> G;
     std::vector<int> ratio;               // 3 per level (to the next)
-    std::vector<double*> cor, tmp, res;   // FP64 per level (FP64 mode); tmp: sweep ping-pong partner
-    std::vector<float*> cor32, tmp32, res32;  // FP32 per level (mixed mode)
-    std::vector<int> kz_sweep;            // autotuned planes per workgroup of the fused sweep
+    LevelGeom P{};                        // natural level-0 geometry of phi and rhs (FP64)
+    std::vector<double*> cor, res;        // FP64 per level (FP64 mode)
+    std::vector<float*> cor32, res32;     // FP32 per level (mixed mode)
+    std::vector<int> kz_sweep;            // autotuned planes per workgroup of the packed half sweep
     double* rpart = nullptr;              // per-block maxima of the outer residual
     LevelPtrs<double>* dlp64 = nullptr;   // device level tables for the coarse kernel
     LevelPtrs<float>* dlp32 = nullptr;
@@ -846,9 +812,11 @@ static void launch_ghosts(T* a, const LevelGeom& L, hipStream_t st)
 
 // ---------------------------------------------------------------------------
 // live kernel timing for the bench: HIP events on the launch stream around
-// every finest-level fused sweep, with its algorithmic bytes (read x_in and
-// rhs, write x_out: 3 words per cell; a zero-initial-guess sweep reads no
-// x_in; a prolongation sweep also reads the coarse correction once)
+// every finest-level packed half sweep, with its algorithmic bytes (the
+// half of the cells it relaxes: read their rhs, read the other colour,
+// write them -- 3 words per relaxed cell, 1.5 per level cell; a red half on
+// a zero initial guess reads no other colour; the red half of a
+// prolongation sweep also reads the coarse correction once)
 // ---------------------------------------------------------------------------
 struct KernelProf {
     bool on = false;
@@ -856,7 +824,7 @@ struct KernelProf {
     size_t used = 0;
     double bytes = 0.0;
     long long launches = 0;
-    std::string name = "k_rb_sweep(level 0)";
+    std::string name = "k_pk_half(level 0)";
 };
 static KernelProf g_prof;
 
@@ -871,8 +839,8 @@ static hipEvent_t prof_event()
 }
 
 // half sweeps number `first` .. 2*nu-1 of nu red-black sweeps (half sweep
-// 2s is red, 2s+1 black), in place; used on levels the fused sweep does not
-// take (odd extents, explicit ghosts, odd post-smoothing counts)
+// 2s is red, 2s+1 black) on a natural-layout level (odd extents, explicit
+// ghosts)
 template <typename T>
 static void launch_half_sweeps(T* x, const T* b, const LevelGeom& L, int first, int nu, hipStream_t st,
                                const T* xc = nullptr, const LevelGeom* C = nullptr, const int* r = nullptr)
@@ -905,24 +873,11 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
 constexpr int kRestrictKz = 4;
 constexpr int kResidualKz = 8;
 
-// levels the fused out-of-place sweep handles: even x extent (aligned
-// pairs), no explicit ghosts (odd periodic extents), and an even number of
-// post-smoothing sweeps (the first one reads the pre-smoothed iterate and
-// must not write it)
-static bool fused_level(const ksgf_macproj* mp, int l)
-{
-    const LevelGeom& L = mp->G[l];
-    return !L.ghost_mode && (L.n0 % 2) == 0 && mp->opt.num_pre_smooth >= 1 && mp->opt.num_post_smooth >= 2 &&
-           (mp->opt.num_post_smooth % 2) == 0;
-}
-
 template <typename T>
-static SweepArgs<T> sweep_args(const ksgf_macproj* mp, int l, const T* xin, T* xout, const T* b, const T* xc, int kz,
-                               int* nblocks)
+static PkArgs<T> pk_args(const ksgf_macproj* mp, int l, T* x, const T* b, const T* xc)
 {
-    SweepArgs<T> a{};
-    a.xin = xin;
-    a.xout = xout;
+    PkArgs<T> a{};
+    a.x = x;
     a.b = b;
     a.xc = xc;
     a.L = mp->G[l];
@@ -931,47 +886,53 @@ static SweepArgs<T> sweep_args(const ksgf_macproj* mp, int l, const T* xin, T* x
     a.s0 = (r[0] == 2);
     a.s1 = (r[1] == 2);
     a.s2 = (r[2] == 2);
-    *nblocks = sweep_blocks(a, kz);
     return a;
 }
 
 template <typename T>
-static void launch_sweep_kernel(const SweepArgs<T>& a, int nb, int mode, hipStream_t st)
+static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hipStream_t st)
 {
-    if (mode == kSwZero)
-        hipLaunchKernelGGL((k_rb_sweep<T, kSwZero>), dim3(nb), dim3(kSwThreads), 0, st, a);
-    else if (mode == kSwProlong)
-        hipLaunchKernelGGL((k_rb_sweep<T, kSwProlong>), dim3(nb), dim3(kSwThreads), 0, st, a);
+    const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, kz);
+    const dim3 g(t.ntx * t.nty * t.nkc), blk(256);
+    if (mode == kPkZero)
+        hipLaunchKernelGGL((k_pk_half<T, kPkZero>), g, blk, 0, st, a, color, t);
+    else if (mode == kPkProlong)
+        hipLaunchKernelGGL((k_pk_half<T, kPkProlong>), g, blk, 0, st, a, color, t);
     else
-        hipLaunchKernelGGL((k_rb_sweep<T, kSwRead>), dim3(nb), dim3(kSwThreads), 0, st, a);
+        hipLaunchKernelGGL((k_pk_half<T, kPkPlain>), g, blk, 0, st, a, color, t);
 }
 
+// one red-black sweep (red half, black half) of packed level l
 template <typename T>
-static void launch_sweep(ksgf_macproj* mp, int l, int mode, const T* xin, T* xout, const T* b, const T* xc,
-                         hipStream_t st)
+static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b, const T* xc, hipStream_t st)
 {
-    int nb;
-    SweepArgs<T> a = sweep_args<T>(mp, l, xin, xout, b, xc, mp->kz_sweep[l], &nb);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (l == 0 && g_prof.on) {
-        e0 = prof_event();
-        e1 = prof_event();
-        if (e0 && e1) (void)hipEventRecord(e0, st);
-    }
-    launch_sweep_kernel<T>(a, nb, mode, st);
-    if (e0 && e1) {
-        (void)hipEventRecord(e1, st);
-        const double n = (double)a.L.n0 * a.L.n1 * a.L.n2, nc = (double)a.C.n0 * a.C.n1 * a.C.n2;
-        g_prof.launches += 1;
-        g_prof.bytes += sizeof(T) * ((mode == kSwZero ? 2.0 : 3.0) * n + (mode == kSwProlong ? nc : 0.0));
-        g_prof.name = sizeof(T) == 8 ? "k_rb_sweep<double>(level 0)" : "k_rb_sweep<float>(level 0)";
+    const PkArgs<T> a = pk_args<T>(mp, l, x, b, xc);
+    for (int color = 0; color < 2; ++color) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (l == 0 && g_prof.on) {
+            e0 = prof_event();
+            e1 = prof_event();
+            if (e0 && e1) (void)hipEventRecord(e0, st);
+        }
+        launch_pk_kernel<T>(a, color, mode, mp->kz_sweep[l], st);
+        if (e0 && e1) {
+            (void)hipEventRecord(e1, st);
+            const double half = 0.5 * (double)a.L.n0 * a.L.n1 * a.L.n2;
+            const double nc = (double)a.C.n0 * a.C.n1 * a.C.n2;
+            const double words = (mode == kPkZero && color == 0) ? 2.0 * half
+                                 : (mode == kPkProlong && color == 0) ? 3.0 * half + nc
+                                                                      : 3.0 * half;
+            g_prof.launches += 1;
+            g_prof.bytes += sizeof(T) * words;
+            g_prof.name = sizeof(T) == 8 ? "k_pk_half<double>(level 0)" : "k_pk_half<float>(level 0)";
+        }
     }
 }
 
-// Autotuner for the planes per workgroup of the fused sweep: every
-// candidate is timed on the level's own arrays (a sweep from cor into the
-// scratch array, which the solve overwrites before reading), the fastest
-// is kept per (level shape, precision) for the life of the process.
+// Autotuner for the planes per workgroup of the packed half sweep: every
+// candidate is timed on the level's own arrays (red halves on the
+// correction array, which every solve re-initialises before reading), the
+// fastest is kept per (level shape, precision) for the life of the process.
 static std::map<std::array<long long, 4>, int>& kz_cache()
 {
     static std::map<std::array<long long, 4>, int> m;
@@ -979,14 +940,14 @@ static std::map<std::array<long long, 4>, int>& kz_cache()
 }
 
 template <typename T>
-static int tune_sweep(ksgf_macproj* mp, int l, T* x, T* tmp, const T* b)
+static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
 {
     const LevelGeom& L = mp->G[l];
     const std::array<long long, 4> key{L.n0, L.n1, L.n2, (long long)sizeof(T)};
     auto it = kz_cache().find(key);
     if (it != kz_cache().end()) return it->second;
-    const int cand[4] = {4, 8, 16, 32};
-    int best = 8;
+    const int cand[4] = {1, 2, 4, 8};
+    int best = 1;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return best;
@@ -994,13 +955,12 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, T* tmp, const T* b)
         (void)hipEventDestroy(e0);
         return best;
     }
+    const PkArgs<T> a = pk_args<T>(mp, l, x, b, nullptr);
     for (int c : cand) {
-        if (c > 2 * L.n2 && c != cand[0]) continue;
-        int nb;
-        SweepArgs<T> a = sweep_args<T>(mp, l, x, tmp, b, nullptr, c, &nb);
-        launch_sweep_kernel<T>(a, nb, kSwRead, nullptr);
+        if (c > L.n2) continue;
+        launch_pk_kernel<T>(a, 0, kPkPlain, c, nullptr);
         (void)hipEventRecord(e0, nullptr);
-        for (int rep = 0; rep < 3; ++rep) launch_sweep_kernel<T>(a, nb, kSwRead, nullptr);
+        for (int rep = 0; rep < 4; ++rep) launch_pk_kernel<T>(a, rep & 1, kPkPlain, c, nullptr);
         (void)hipEventRecord(e1, nullptr);
         float ms = 0.f;
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
@@ -1017,8 +977,8 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, T* tmp, const T* b)
 
 // V-cycle in precision T from level l down (levels >= lc in one workgroup)
 template <typename T>
-static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& tmp, std::vector<T*>& res,
-                   const LevelPtrs<T>* dlp, hipStream_t st)
+static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*>& res, const LevelPtrs<T>* dlp,
+                   hipStream_t st)
 {
     if (l >= mp->lc) {
         hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(kCoarseThreads), 0, st, dlp, l, mp->nlev, mp->bw,
@@ -1032,15 +992,12 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     const int* r = &mp->ratio[3 * l];
     const long long nf = (long long)L.n0 * L.n1 * L.n2;
     const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
-    const bool fused = fused_level(mp, l);
-    // sweep s of nu writes the array that makes sweep nu-1 land in cor[l]
-    auto out = [&](int s, int nu) { return ((nu - 1 - s) % 2 == 0) ? cor[l] : tmp[l]; };
-    // down
     const int nu1 = mp->opt.num_pre_smooth, nu2 = mp->opt.num_post_smooth;
-    if (fused) {
+    // down
+    if (L.packed) {
+        if (nu1 == 0) launch_fill<T>(cor[l], L.tot, (T)0, st);
         for (int s = 0; s < nu1; ++s)
-            launch_sweep<T>(mp, l, s == 0 ? kSwZero : kSwRead, s == 0 ? nullptr : out(s - 1, nu1), out(s, nu1),
-                            res[l], nullptr, st);
+            launch_pk_sweep<T>(mp, l, s == 0 ? kPkZero : kPkPlain, cor[l], res[l], nullptr, st);
     } else if (nu1 > 0 && !L.ghost_mode) {
         hipLaunchKernelGGL(k_mg_smooth_first<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)res[l], L);
         launch_half_sweeps<T>(cor[l], res[l], L, 1, nu1, st);
@@ -1054,12 +1011,17 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
         hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, (const T*)cor[l],
                            (const T*)res[l], res[l + 1], L, C, t, r[0], r[1], r[2], w);
     }
-    vcycle<T>(mp, l + 1, cor, tmp, res, dlp, st);
+    vcycle<T>(mp, l + 1, cor, res, dlp, st);
     // up
-    if (fused) {
+    if (L.packed) {
+        if (nu2 == 0) {
+            RowTiles t = row_tiles(L.n0, L.n1, L.n2, kRestrictKz);
+            hipLaunchKernelGGL(k_pk_prolong_add<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, cor[l],
+                               (const T*)cor[l + 1], L, C, (int)(r[0] == 2), (int)(r[1] == 2), (int)(r[2] == 2), t);
+        }
         for (int s = 0; s < nu2; ++s)
-            launch_sweep<T>(mp, l, s == 0 ? kSwProlong : kSwRead, s == 0 ? cor[l] : out(s - 1, nu2), out(s, nu2),
-                            res[l], s == 0 ? cor[l + 1] : nullptr, st);
+            launch_pk_sweep<T>(mp, l, s == 0 ? kPkProlong : kPkPlain, cor[l], res[l], s == 0 ? cor[l + 1] : nullptr,
+                               st);
     } else if (nu2 > 0 && !L.ghost_mode) {
         hipLaunchKernelGGL(k_mg_prolong_red<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)cor[l + 1],
                            (const T*)res[l], L, C, r[0], r[1], r[2]);
@@ -1085,22 +1047,23 @@ static int read_norm(ksgf_macproj* mp, hipStream_t st, double* out)
 template <typename C, typename R>
 static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStream_t st)
 {
+    const LevelGeom& P = mp->P;
     const LevelGeom& L0 = mp->G[0];
-    const RowTiles t = row_tiles(L0.n0, L0.n1, L0.n2, kResidualKz);
+    const RowTiles t = row_tiles(P.n0, P.n1, P.n2, kResidualKz);
     const int nb = t.ntx * t.nty * t.nkc;
-    if (L0.ghost_mode) {
+    if (P.ghost_mode) {
         // explicit ghosts: phi += cor first, then the residual from ghosts
         if (cor) {
             hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
-                               mp->phi2, (const double*)mp->rhs, r, L0, t, mp->rpart);
+                               mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
             std::swap(mp->phi, mp->phi2);
         }
-        launch_ghosts<double>(mp->phi, L0, st);
+        launch_ghosts<double>(mp->phi, P, st);
         hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
-                           (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, L0, t, mp->rpart);
+                           (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
     } else {
         hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor, mp->phi2,
-                           (const double*)mp->rhs, r, L0, t, mp->rpart);
+                           (const double*)mp->rhs, r, P, L0, t, mp->rpart);
         if (cor) std::swap(mp->phi, mp->phi2);
     }
     hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(256), 0, st, (const double*)mp->rpart, nb, mp->dnorm);
@@ -1109,7 +1072,7 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
 // solve with the level-0 rhs in mp->rhs (ghost layout); phi in mp->phi
 static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* info, hipStream_t st)
 {
-    const LevelGeom& L0 = mp->G[0];
+    const LevelGeom& L0 = mp->P;  // phi / rhs
     const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
     launch_fill<double>(mp->phi, L0.tot, 0.0, st);
     if (mp->singular) launch_sub_mean(mp->rhs, L0, mp->part, st);
@@ -1132,10 +1095,10 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
     if (!inf.converged) {
         for (int it = 1; it <= mp->opt.maxiter; ++it) {
             if (mixed) {
-                vcycle<float>(mp, 0, mp->cor32, mp->tmp32, mp->res32, mp->dlp32, st);
+                vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->dlp32, st);
                 launch_update_residual<float, float>(mp, mp->cor32[0], mp->res32[0], st);
             } else {
-                vcycle<double>(mp, 0, mp->cor, mp->tmp, mp->res, mp->dlp64, st);
+                vcycle<double>(mp, 0, mp->cor, mp->res, mp->dlp64, st);
                 launch_update_residual<double, double>(mp, mp->cor[0], mp->res[0], st);
             }
             KSGF_LAUNCH_CHECK();
@@ -1216,8 +1179,6 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     for (auto* p : mp->res) if (p) (void)hipFree(p);
     for (auto* p : mp->cor32) if (p) (void)hipFree(p);
     for (auto* p : mp->res32) if (p) (void)hipFree(p);
-    for (auto* p : mp->tmp) if (p) (void)hipFree(p);
-    for (auto* p : mp->tmp32) if (p) (void)hipFree(p);
     if (mp->rpart) (void)hipFree(mp->rpart);
     double* bws[8] = {mp->bw.r, mp->bw.rh, mp->bw.p, mp->bw.v, mp->bw.s, mp->bw.t, mp->bw.x, mp->bw.b};
     for (auto* p : bws) if (p) (void)hipFree(p);
@@ -1289,7 +1250,7 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         L.n0 = cn[0];
         L.n1 = cn[1];
         L.n2 = cn[2];
-        level_layout(L);
+        level_layout(L, false);
         L.ghost_mode = 0;
         for (int d = 0; d < 3; ++d) {
             L.lo[d] = lobc[d];
@@ -1332,6 +1293,11 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     mp->lc = mp->nlev - 1;
     while (mp->lc > 0 && (long long)mp->G[mp->lc - 1].n0 * mp->G[mp->lc - 1].n1 * mp->G[mp->lc - 1].n2 <= kCoarseCells)
         mp->lc--;
+    // phi and rhs stay in the natural layout; grid levels without explicit
+    // ghosts and with an even x extent take the red-black packed layout
+    mp->P = mp->G[0];
+    for (int l = 0; l < mp->lc; ++l)
+        if (!mp->G[l].ghost_mode && mp->G[l].n0 % 2 == 0) level_layout(mp->G[l], true);
     const LevelGeom& Lb = mp->G[mp->nlev - 1];
     if ((long long)Lb.n0 * Lb.n1 * Lb.n2 > (1LL << 22)) {
         ksgf_macproj_destroy(mp);
@@ -1343,25 +1309,21 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     for (int l = 0; l < mp->nlev; ++l) {
         size_t n = (size_t)mp->G[l].tot;
         void *a, *b;
-        void* c = nullptr;
-        const bool grid = (l < mp->lc);  // the ping-pong partner is only needed on grid levels
         const size_t w = mixed ? sizeof(float) : sizeof(double);
-        if ((e = mp_alloc(&a, n * w)) || (e = mp_alloc(&b, n * w)) || (grid && (e = mp_alloc(&c, n * w)))) {
+        if ((e = mp_alloc(&a, n * w)) || (e = mp_alloc(&b, n * w))) {
             ksgf_macproj_destroy(mp);
             return e;
         }
         if (mixed) {
             mp->cor32.push_back((float*)a);
             mp->res32.push_back((float*)b);
-            mp->tmp32.push_back((float*)c);
         } else {
             mp->cor.push_back((double*)a);
             mp->res.push_back((double*)b);
-            mp->tmp.push_back((double*)c);
         }
     }
     void* p;
-    const size_t n0 = (size_t)mp->G[0].tot;
+    const size_t n0 = (size_t)mp->P.tot;
     if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
     mp->phi = (double*)p;
     if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
@@ -1371,7 +1333,7 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     if ((e = mp_alloc(&p, kSumBlocks * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
     mp->part = (double*)p;
     {
-        const RowTiles t = row_tiles(mp->G[0].n0, mp->G[0].n1, mp->G[0].n2, kResidualKz);
+        const RowTiles t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, kResidualKz);
         if ((e = mp_alloc(&p, (size_t)t.ntx * t.nty * t.nkc * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
         mp->rpart = (double*)p;
     }
@@ -1406,9 +1368,9 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     }
     mp->kz_sweep.assign(mp->nlev, 8);
     for (int l = 0; l < mp->lc; ++l) {
-        if (!fused_level(mp, l)) continue;
-        mp->kz_sweep[l] = mixed ? tune_sweep<float>(mp, l, mp->cor32[l], mp->tmp32[l], mp->res32[l])
-                                : tune_sweep<double>(mp, l, mp->cor[l], mp->tmp[l], mp->res[l]);
+        if (!mp->G[l].packed) continue;
+        mp->kz_sweep[l] = mixed ? tune_sweep<float>(mp, l, mp->cor32[l], mp->res32[l])
+                                : tune_sweep<double>(mp, l, mp->cor[l], mp->res[l]);
     }
     if (hipDeviceSynchronize() != hipSuccess) {
         ksgf_macproj_destroy(mp);
@@ -1428,6 +1390,7 @@ int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta)
     mp->beta = beta;
     auto h = level_h(mp);
     for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta);
+    fill_coefs(mp->P, h[0].data(), beta);
     return upload_level_tables(mp);
 }
 
@@ -1453,7 +1416,7 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
     int z[3] = {0, 0, 0};
     const Region* rv = get_region(ba, z, z, &err);
     if (!rv) return err;
-    const LevelGeom& L0 = mp->G[0];
+    const LevelGeom& L0 = mp->P;
     hipLaunchKernelGGL(k_mac_rhs, dim3(rv->nchunks), dim3(256), 0, st, cu, cv, cw, mp->rhs, L0,
                        regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], 1.0 / ba->dx[0],
                        1.0 / ba->dx[1], 1.0 / ba->dx[2]);
@@ -1489,7 +1452,7 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
         return KSGF_ERR_INVALID;
     }
     hipStream_t st = (hipStream_t)stream;
-    const LevelGeom& L0 = mp->G[0];
+    const LevelGeom& L0 = mp->P;
     long long n = (long long)L0.n0 * L0.n1 * L0.n2;
     hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, rhs, mp->rhs, L0, 1);
     KSGF_LAUNCH_CHECK();
@@ -1506,7 +1469,7 @@ int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream)
         set_error("ksgf_macproj_get_phi: null argument");
         return KSGF_ERR_INVALID;
     }
-    const LevelGeom& L0 = mp->G[0];
+    const LevelGeom& L0 = mp->P;
     long long n = (long long)L0.n0 * L0.n1 * L0.n2;
     hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, (const double*)mp->phi, phi,
                        L0, 0);

@@ -1,14 +1,13 @@
-// CPU emulation of the fused red-black sweep kernel (k_rb_sweep).
+// CPU check of the packed red-black multigrid kernels.
 //
-// The workgroup-phase bodies in ksgf_mg_grid.h are driven here thread by
-// thread -- every thread of the red phase, then every thread of the black
-// phase, for each plane and each workgroup, exactly the order the barriers
-// impose on the GPU -- and the result is compared bit for bit with a plain
-// in-place half-sweep red-black Gauss-Seidel (explicit ghost fills before
-// each colour).  Ghost layers and row padding of the input hold NaN, so a
-// read of anything but an interior cell shows up in the output.  Built with
-// AddressSanitizer, every array access (including the LDS planes,
-// which are separate heap arrays here) is bounds checked.
+// The per-cell bodies the HIP kernels run (ksgf_mg_grid.h: pk_relax,
+// pk_restrict_cell) are driven here cell by cell on packed arrays and
+// compared bit for bit with a plain natural-layout implementation: in-place
+// half sweeps after explicit ghost fills, and residual + average restriction
+// from a ghost-filled array.  Every sweep mode (plain, zero initial guess,
+// fused prolongation), both colours, periodic / Neumann / Dirichlet sides,
+// semicoarsened ratios, FP32 and FP64.  Built with AddressSanitizer, so
+// every array access is bounds checked.
 //
 // Prints "ok <n>" and exits 0 when every case matches.
 #include <cmath>
@@ -23,13 +22,14 @@
 
 using namespace ksgf;
 
-static LevelGeom make_level(int n0, int n1, int n2, const int lo[3], const int hi[3], double h0, double h1, double h2)
+static LevelGeom make_level(int n0, int n1, int n2, const int lo[3], const int hi[3], double h0, double h1, double h2,
+                            bool packed)
 {
     LevelGeom L{};
     L.n0 = n0;
     L.n1 = n1;
     L.n2 = n2;
-    level_layout(L);
+    level_layout(L, packed);
     L.cx = 1.0 / (h0 * h0);
     L.cy = 1.0 / (h1 * h1);
     L.cz = 1.0 / (h2 * h2);
@@ -69,201 +69,167 @@ static void fill_ghosts(std::vector<T>& a, const LevelGeom& L)
         }
 }
 
-// in-place half sweeps (red then black) from the initial iterate y
+// in-place half sweep of one colour on a natural array
 template <typename T>
-static void reference_sweep(std::vector<T>& y, const std::vector<T>& b, const LevelGeom& L)
+static void reference_half(std::vector<T>& y, const std::vector<T>& b, const LevelGeom& L, int color)
 {
     const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, idg = (T)L.idg;
-    for (int color = 0; color < 2; ++color) {
-        fill_ghosts(y, L);
-        for (int k = 0; k < L.n2; ++k)
-            for (int j = 0; j < L.n1; ++j)
-                for (int i = 0; i < L.n0; ++i) {
-                    if (((i + j + k + color) & 1) != 0) continue;
-                    const long long c = LIdx(L, i, j, k);
-                    y[c] = ((cx * (y[c - 1] + y[c + 1]) + cy * (y[c - L.sy] + y[c + L.sy]) +
-                             cz * (y[c - L.sz] + y[c + L.sz])) -
-                            b[c]) *
-                           idg;
-                }
-    }
-}
-
-template <typename T, int MODE>
-static void emulate(const SweepArgs<T>& a0, int kz)
-{
-    SweepArgs<T> a = a0;
-    const int nb = sweep_blocks(a, kz);
-    const T nan = std::numeric_limits<T>::quiet_NaN();
-    std::vector<std::vector<T>> X(4, std::vector<T>(kSwXPlane)), B(3, std::vector<T>(kSwPlane)),
-        R(3, std::vector<T>(kSwPlane));
-    std::vector<std::vector<T>> xa(kSwThreads, std::vector<T>(kSwXPer)), ba(kSwThreads, std::vector<T>(kSwBPer));
-    std::vector<std::vector<T>> xb(kSwThreads, std::vector<T>(kSwXPer)), bb(kSwThreads, std::vector<T>(kSwBPer));
-    std::vector<bool> seen(nb, false);
-    for (int bid = 0; bid < nb; ++bid) {
-        const int lid = xcd_remap(bid, nb);
-        if (lid < 0 || lid >= nb || seen[lid]) {
-            std::printf("xcd_remap not a permutation (nb %d)\n", nb);
-            std::exit(1);
-        }
-        seen[lid] = true;
-        int tx, ty, kc;
-        sweep_tile(lid, a.ntx, a.nty, tx, ty, kc);
-        for (auto& p : X) std::fill(p.begin(), p.end(), nan);
-        for (auto& p : B) std::fill(p.begin(), p.end(), nan);
-        for (auto& p : R) std::fill(p.begin(), p.end(), nan);
-        const int k0 = kc * a.kz;
-        const int k1 = std::min(k0 + a.kz, a.L.n2);
-        auto xs = [&](int p) { return (p - k0 + 2) & 3; };
-        auto bs = [&](int p) { return (p - k0 + 1) % 3; };
-        // the kernel's pipeline: two register sets, loads two planes ahead
-        std::vector<SwThread> th(kSwThreads);
-        for (int tid = 0; tid < kSwThreads; ++tid) {
-            sw_thread_init<T>(a, th[tid], tid, tx, ty);
-            sw_load_x<T, MODE>(a, th[tid], xa[tid].data(), k0 - 2);
-            sw_store_x<T>(X[0].data(), xa[tid].data(), tid);
-            sw_load_x<T, MODE>(a, th[tid], xa[tid].data(), k0 - 1);
-            sw_store_x<T>(X[1].data(), xa[tid].data(), tid);
-            sw_load_x<T, MODE>(a, th[tid], xa[tid].data(), k0);
-            sw_load_b<T>(a, th[tid], ba[tid].data(), k0 - 1);
-            sw_load_x<T, MODE>(a, th[tid], xb[tid].data(), k0 + 1);
-            sw_load_b<T>(a, th[tid], bb[tid].data(), k0);
-        }
-        auto step = [&](std::vector<std::vector<T>>& xv, std::vector<std::vector<T>>& bv, int k) {
-            for (int tid = 0; tid < kSwThreads; ++tid) {
-                sw_store_x<T>(X[xs(k + 1)].data(), xv[tid].data(), tid);
-                sw_store_b<T>(B[bs(k)].data(), bv[tid].data(), tid);
-                if (k + 3 <= k1 + 1) sw_load_x<T, MODE>(a, th[tid], xv[tid].data(), k + 3);
-                if (k + 2 <= k1) sw_load_b<T>(a, th[tid], bv[tid].data(), k + 2);
+    fill_ghosts(y, L);
+    for (int k = 0; k < L.n2; ++k)
+        for (int j = 0; j < L.n1; ++j)
+            for (int i = 0; i < L.n0; ++i) {
+                if (((i + j + k + color) & 1) != 0) continue;
+                const long long c = LIdx(L, i, j, k);
+                y[c] = ((cx * (y[c - 1] + y[c + 1]) + cy * (y[c - L.sy] + y[c + L.sy]) +
+                         cz * (y[c - L.sz] + y[c + L.sz])) -
+                        b[c]) *
+                       idg;
             }
-            for (int tid = 0; tid < kSwThreads; ++tid)
-                sw_phase_red<T>(a.L, th[tid], X[xs(k - 1)].data(), X[xs(k)].data(), X[xs(k + 1)].data(),
-                                B[bs(k)].data(), R[bs(k)].data(), k);
-            if (k > k0) {
-                const int kb = k - 1;
-                for (int tid = 0; tid < kSwThreads; ++tid) {
-                    long long at;
-                    T v0, v1;
-                    if (sw_phase_black<T>(a.L, th[tid], R[bs(kb - 1)].data(), R[bs(kb)].data(), R[bs(k)].data(),
-                                          X[xs(kb)].data(), B[bs(kb)].data(), kb, at, v0, v1)) {
-                        if ((at & 1) != 0) {
-                            std::printf("unaligned pair store at %lld\n", at);
-                            std::exit(1);
-                        }
-                        a.xout[at] = v0;
-                        a.xout[at + 1] = v1;
-                    }
-                }
-            }
-        };
-        for (int k = k0 - 1; k <= k1; k += 2) {
-            step(xa, ba, k);
-            if (k + 1 <= k1) step(xb, bb, k + 1);
-        }
-    }
 }
 
 static int g_cases = 0;
 
-template <typename T, int MODE>
-static void run_case(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3], int kz,
-                     unsigned seed)
+template <typename T>
+static void fail(const char* what, int n0, int n1, int n2, int mode, int color, long long bad)
 {
-    const LevelGeom L = make_level(n0, n1, n2, lo, hi, 1.0, 1.7, 0.45);
-    const LevelGeom C = make_level(n0 / ratio[0], n1 / ratio[1], n2 / ratio[2], lo, hi, 2.0, 3.4, 0.9);
+    std::printf("FAIL %s %dx%dx%d mode %d colour %d T%zu: %lld bad\n", what, n0, n1, n2, mode, color, sizeof(T), bad);
+    std::exit(1);
+}
+
+template <typename T, int MODE>
+static void run_half(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3], int color,
+                     bool coarse_packed, unsigned seed)
+{
+    const LevelGeom N = make_level(n0, n1, n2, lo, hi, 1.0, 1.7, 0.45, false);
+    const LevelGeom P = make_level(n0, n1, n2, lo, hi, 1.0, 1.7, 0.45, true);
+    const LevelGeom C = make_level(n0 / ratio[0], n1 / ratio[1], n2 / ratio[2], lo, hi, 2.0, 3.4, 0.9,
+                                   coarse_packed && (n0 / ratio[0]) % 2 == 0);
     const T nan = std::numeric_limits<T>::quiet_NaN();
-    std::vector<T> x(L.tot, nan), b(L.tot, nan), xc(C.tot, nan), out(L.tot, nan);
+    std::vector<T> xn(N.tot, nan), bn(N.tot, nan), xp(P.tot, nan), bp(P.tot, nan), xc(C.tot, nan);
     std::mt19937 rng(seed);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n0; ++i) {
-                x[LIdx(L, i, j, k)] = (T)U(rng);
-                b[LIdx(L, i, j, k)] = (T)U(rng);
+                const T x = (T)U(rng), b = (T)U(rng);
+                xn[LIdx(N, i, j, k)] = x;
+                bn[LIdx(N, i, j, k)] = b;
+                xp[PIdx(P, i, j, k)] = x;
+                bp[PIdx(P, i, j, k)] = b;
             }
     for (int k = 0; k < C.n2; ++k)
         for (int j = 0; j < C.n1; ++j)
-            for (int i = 0; i < C.n0; ++i) xc[LIdx(C, i, j, k)] = (T)U(rng);
-    // reference
-    std::vector<T> y(L.tot, nan);
+            for (int i = 0; i < C.n0; ++i) xc[AnyIdx(C, i, j, k)] = (T)U(rng);
+    // reference: the natural array the mode stands for, then the half sweep
+    std::vector<T> y(N.tot, nan);
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n0; ++i) {
-                const long long c = LIdx(L, i, j, k);
-                if (MODE == kSwZero)
-                    y[c] = (T)0;
-                else if (MODE == kSwProlong)
-                    y[c] = x[c] + xc[LIdx(C, i / ratio[0], j / ratio[1], k / ratio[2])];
+                const long long c = LIdx(N, i, j, k);
+                const bool mine = ((i + j + k) & 1) == color;
+                const T pro = xn[c] + xc[AnyIdx(C, i / ratio[0], j / ratio[1], k / ratio[2])];
+                if (MODE == kPkZero)
+                    y[c] = (color == 0 || mine) ? (T)0 : xn[c];  // red: all zero; black: blacks zero
+                else if (MODE == kPkProlong)
+                    y[c] = (color == 0 || mine) ? pro : xn[c];   // red: all prolonged; black: blacks
                 else
-                    y[c] = x[c];
+                    y[c] = xn[c];
             }
-    reference_sweep(y, b, L);
-    SweepArgs<T> a{};
-    a.xin = x.data();
-    a.xout = out.data();
-    a.b = b.data();
+    reference_half(y, bn, N, color);
+    PkArgs<T> a{};
+    a.x = xp.data();
+    a.b = bp.data();
     a.xc = xc.data();
-    a.L = L;
+    a.L = P;
     a.C = C;
     a.s0 = ratio[0] == 2;
     a.s1 = ratio[1] == 2;
     a.s2 = ratio[2] == 2;
-    emulate<T, MODE>(a, kz);
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int h = 0; h < n0 / 2; ++h) pk_relax<T, MODE>(a, color, h, j, k);
     long long bad = 0;
-    for (long long c = 0; c < L.tot; ++c) {
-        // interior cells must match bit for bit, everything else stays untouched
-        const long long q = c / L.sz - 1, r = c % L.sz, jj = r / L.sy - 1, ii = r % L.sy - kXOff;
-        const bool inside = q >= 0 && q < n2 && jj >= 0 && jj < n1 && ii >= 0 && ii < n0;
-        if (inside) {
-            if (std::memcmp(&out[c], &y[c], sizeof(T)) != 0) {
-                if (bad < 5)
-                    std::printf("mismatch at (%lld,%lld,%lld): %.17g vs %.17g\n", ii, jj, q, (double)out[c],
-                                (double)y[c]);
-                ++bad;
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                const bool mine = ((i + j + k) & 1) == color;
+                const T g = xp[PIdx(P, i, j, k)];
+                const T r = mine ? y[LIdx(N, i, j, k)] : xn[LIdx(N, i, j, k)];  // the other colour is not written
+                if (std::memcmp(&g, &r, sizeof(T)) != 0) {
+                    if (bad < 3) std::printf("  (%d,%d,%d): %.17g vs %.17g\n", i, j, k, (double)g, (double)r);
+                    ++bad;
+                }
             }
-        } else if (!std::isnan((double)out[c])) {
-            if (bad < 5) std::printf("write outside the interior at %lld\n", c);
-            ++bad;
-        }
-    }
-    if (bad) {
-        std::printf("FAIL case %dx%dx%d mode %d kz %d T%zu bc lo %d%d%d hi %d%d%d: %lld bad\n", n0, n1, n2, MODE, kz,
-                    sizeof(T), lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], bad);
-        std::exit(1);
-    }
+    if (bad) fail<T>("half sweep", n0, n1, n2, MODE, color, bad);
     ++g_cases;
 }
 
 template <typename T>
-static void run_all_modes(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3], int kz,
-                          unsigned seed)
+static void run_restrict(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3],
+                         unsigned seed)
 {
-    run_case<T, kSwRead>(n0, n1, n2, lo, hi, ratio, kz, seed);
-    run_case<T, kSwZero>(n0, n1, n2, lo, hi, ratio, kz, seed + 1);
-    run_case<T, kSwProlong>(n0, n1, n2, lo, hi, ratio, kz, seed + 2);
+    const LevelGeom N = make_level(n0, n1, n2, lo, hi, 1.0, 1.7, 0.45, false);
+    const LevelGeom P = make_level(n0, n1, n2, lo, hi, 1.0, 1.7, 0.45, true);
+    const T nan = std::numeric_limits<T>::quiet_NaN();
+    std::vector<T> xn(N.tot, nan), bn(N.tot, nan), xp(P.tot, nan), bp(P.tot, nan);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                const T x = (T)U(rng), b = (T)U(rng);
+                xn[LIdx(N, i, j, k)] = x;
+                bn[LIdx(N, i, j, k)] = b;
+                xp[PIdx(P, i, j, k)] = x;
+                bp[PIdx(P, i, j, k)] = b;
+            }
+    fill_ghosts(xn, N);
+    const T cx = (T)N.cx, cy = (T)N.cy, cz = (T)N.cz;
+    auto st = [&](long long c, int, int, int, T) {
+        return cx * (xn[c - 1] + xn[c + 1]) + cy * (xn[c - N.sy] + xn[c + N.sy]) + cz * (xn[c - N.sz] + xn[c + N.sz]);
+    };
+    const T w = (T)(1.0 / (double)(ratio[0] * ratio[1] * ratio[2]));
+    long long bad = 0;
+    for (int K = 0; K < n2 / ratio[2]; ++K)
+        for (int J = 0; J < n1 / ratio[1]; ++J)
+            for (int I = 0; I < n0 / ratio[0]; ++I) {
+                const T g = pk_restrict_cell<T>(xp.data(), bp.data(), P, I, J, K, ratio[0], ratio[1], ratio[2], w);
+                const T r = restrict_cell<T>(xn.data(), bn.data(), N, I, J, K, ratio[0], ratio[1], ratio[2], w, st);
+                if (std::memcmp(&g, &r, sizeof(T)) != 0) ++bad;
+            }
+    if (bad) fail<T>("restriction", n0, n1, n2, -1, -1, bad);
+    ++g_cases;
+}
+
+template <typename T>
+static void run_all(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3], unsigned seed)
+{
+    for (int color = 0; color < 2; ++color)
+        for (int cp = 0; cp < 2; ++cp) {
+            run_half<T, kPkPlain>(n0, n1, n2, lo, hi, ratio, color, cp, seed + 10 * color + cp);
+            run_half<T, kPkZero>(n0, n1, n2, lo, hi, ratio, color, cp, seed + 10 * color + cp + 2);
+            run_half<T, kPkProlong>(n0, n1, n2, lo, hi, ratio, color, cp, seed + 10 * color + cp + 4);
+        }
+    run_restrict<T>(n0, n1, n2, lo, hi, ratio, seed + 100);
 }
 
 int main()
 {
     const int P = KSGF_LO_PERIODIC, N = KSGF_LO_NEUMANN, D = KSGF_LO_DIRICHLET;
     const int allp[3] = {P, P, P};
-    const int abl_lo[3] = {P, P, N}, abl_hi[3] = {P, P, N};
+    const int abl[3] = {P, P, N};
     const int r222[3] = {2, 2, 2}, r112[3] = {1, 1, 2}, r221[3] = {2, 2, 1};
     const int mixlo[3] = {D, N, P}, mixhi[3] = {N, D, P};
     const int wall_lo[3] = {D, N, N}, wall_hi[3] = {D, N, D};
-    for (int kz : {1, 3, 4, 16}) {
-        run_all_modes<double>(128, 64, 32, allp, allp, r222, kz, 11u + kz);
-        run_all_modes<float>(128, 64, 32, allp, allp, r222, kz, 21u + kz);
-        run_all_modes<double>(96, 40, 20, abl_lo, abl_hi, r112, kz, 31u + kz);
-        run_all_modes<float>(96, 40, 20, abl_lo, abl_hi, r112, kz, 41u + kz);
-    }
-    for (int kz : {2, 5}) {
-        run_all_modes<double>(34, 6, 5, wall_lo, wall_hi, r221, kz, 51u + kz);
-        run_all_modes<double>(66, 10, 6, mixlo, mixhi, r222, kz, 61u + kz);
-        run_all_modes<float>(66, 10, 6, mixlo, mixhi, r222, kz, 71u + kz);
-        run_all_modes<double>(2, 2, 2, allp, allp, r222, kz, 81u + kz);
-        run_all_modes<double>(4, 18, 8, abl_lo, abl_hi, r222, kz, 91u + kz);
-    }
+    run_all<double>(16, 8, 12, allp, allp, r222, 11u);
+    run_all<float>(16, 8, 12, allp, allp, r222, 21u);
+    run_all<double>(12, 10, 8, abl, abl, r112, 31u);
+    run_all<float>(12, 10, 8, abl, abl, r112, 41u);
+    run_all<double>(10, 6, 5, wall_lo, wall_hi, r221, 51u);
+    run_all<double>(6, 10, 6, mixlo, mixhi, r222, 61u);
+    run_all<float>(6, 10, 6, mixlo, mixhi, r222, 71u);
+    run_all<double>(2, 2, 2, allp, allp, r222, 81u);
+    run_all<double>(4, 18, 8, abl, abl, r221, 91u);
     std::printf("ok %d\n", g_cases);
     return 0;
 }

@@ -1,8 +1,8 @@
-"""CPU emulation of the fused red-black sweep: the kernel's workgroup-phase
-bodies (kynema-sgf_amd/csrc/ksgf_mg_grid.h) are compiled as host C++ with
-AddressSanitizer and driven thread by thread against an in-place half-sweep
-Gauss-Seidel, bit for bit, over periodic / Neumann / Dirichlet sides,
-partial tiles and every sweep mode (see tests/mg_sweep_emulation.cpp)."""
+"""CPU check of the packed red-black multigrid kernels: the per-cell bodies
+the HIP kernels run (kynema-sgf_amd/csrc/ksgf_mg_grid.h) are compiled as host
+C++ with AddressSanitizer and compared bit for bit with natural-layout
+half sweeps and restriction, over periodic / Neumann / Dirichlet sides,
+semicoarsened ratios and every sweep mode (tests/mg_sweep_emulation.cpp)."""
 import os
 import shutil
 import subprocess
@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-def test_fused_sweep_emulation(tmp_path):
+def test_packed_mg_kernels_match_natural_layout(tmp_path):
     exe = tmp_path / "mg_emu"
     src = os.path.join(HERE, "mg_sweep_emulation.cpp")
     cmd = ["g++", "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-fsanitize=address,undefined",
