@@ -45,9 +45,11 @@ struct LevelGeom {
 
 constexpr int kXOff = 16;
 
+// level arrays hold fewer than 2^31 elements (checked when a hierarchy is
+// built), so offsets are formed in 32-bit arithmetic
 KSGF_HD long long LIdx(const LevelGeom& L, int i, int j, int k)
 {
-    return (long long)(i + kXOff) + L.sy * (j + 1) + L.sz * (k + 1);
+    return (long long)((i + kXOff) + (int)L.sy * (j + 1) + (int)L.sz * (k + 1));
 }
 
 KSGF_HD long long PIdx(const LevelGeom& L, int i, int j, int k)
@@ -58,6 +60,14 @@ KSGF_HD long long PIdx(const LevelGeom& L, int i, int j, int k)
 KSGF_HD long long AnyIdx(const LevelGeom& L, int i, int j, int k)
 {
     return L.packed ? PIdx(L, i, j, k) : LIdx(L, i, j, k);
+}
+
+// offset of cell (i, j, k) in a level whose layout is known at compile time
+template <bool PACKED>
+KSGF_HD int LvIdx(const LevelGeom& L, int i, int j, int k)
+{
+    if (PACKED) return ((i + j + k) & 1) * L.pcol + k * L.pplane + j * L.prow + (i >> 1);
+    return (i + kXOff) + (int)L.sy * (j + 1) + (int)L.sz * (k + 1);
 }
 
 inline void level_layout(LevelGeom& L, bool packed)
@@ -146,7 +156,7 @@ KSGF_HD T pk_wall(int bc)
 
 // relax the colour-`color` cell at packed position (h, j, k):
 // x = (cx*(W+E) + cy*(S+N) + cz*(B+T) - b) / diag, the oracle's operand order
-template <typename T, int MODE>
+template <typename T, int MODE, bool CP>
 KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
 {
     const LevelGeom& L = a.L;
@@ -167,7 +177,7 @@ KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
         if (MODE == kPkZero)
             self = (T)0;
         else if (MODE == kPkProlong)
-            self = a.x[c] + a.xc[AnyIdx(a.C, i >> a.s0, j >> a.s1, k >> a.s2)];
+            self = a.x[c] + a.xc[LvIdx<CP>(a.C, i >> a.s0, j >> a.s1, k >> a.s2)];
         else
             self = a.x[c];
     }
@@ -177,7 +187,7 @@ KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
     const int oth = (1 - color) * L.pcol;
     auto nb = [&](int ii, int jj, int kk) {
         T v = a.x[oth + kk * L.pplane + jj * L.prow + (ii >> 1)];
-        if (pro) v = v + a.xc[AnyIdx(a.C, ii >> a.s0, jj >> a.s1, kk >> a.s2)];
+        if (pro) v = v + a.xc[LvIdx<CP>(a.C, ii >> a.s0, jj >> a.s1, kk >> a.s2)];
         return v;
     };
     const T w = in_w ? nb(i > 0 ? i - 1 : L.n0 - 1, j, k) : pk_wall<T>(L.lo[0]) * self;
@@ -189,41 +199,35 @@ KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
     a.x[c] = ((cx * (w + e) + cy * (s + n) + cz * (bo + t)) - a.b[c]) * idg;
 }
 
-// value of the D-neighbour (offset s) of in-domain cell (i, j, k) of a packed
-// level (walls mirror the cell's value `self`, periodic sides wrap)
-template <typename T, int D>
-KSGF_HD T pk_nb(const T* x, const LevelGeom& L, int i, int j, int k, int s, T self)
-{
-    int p = (D == 0 ? i : (D == 1 ? j : k)) + s;
-    const int n = (D == 0 ? L.n0 : (D == 1 ? L.n1 : L.n2));
-    if (p < 0 || p >= n) {
-        const int bc = (s < 0) ? L.lo[D] : L.hi[D];
-        if (bc != KSGF_LO_PERIODIC) return pk_wall<T>(bc) * self;
-        p = (s < 0) ? n - 1 : 0;
-    }
-    if (D == 0) return x[PIdx(L, p, j, k)];
-    if (D == 1) return x[PIdx(L, i, p, k)];
-    return x[PIdx(L, i, j, p)];
-}
-
 // residual of the fine children of coarse cell (I, J, K) of a packed fine
 // level, averaged: sum of (b - (stencil(x) - diag*x)) times w, the oracle's
-// summation order (i fastest)
+// summation order (i fastest); walls mirror the cell, periodic sides wrap
 template <typename T>
 KSGF_HD T pk_restrict_cell(const T* x, const T* b, const LevelGeom& F, int I, int J, int K, int r0, int r1, int r2,
                            T w)
 {
     const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
+    const bool px = F.lo[0] == KSGF_LO_PERIODIC, py = F.lo[1] == KSGF_LO_PERIODIC, pz = F.lo[2] == KSGF_LO_PERIODIC;
     T s = (T)0;
     for (int dk = 0; dk < r2; ++dk)
         for (int dj = 0; dj < r1; ++dj)
             for (int di = 0; di < r0; ++di) {
                 const int i = I * r0 + di, j = J * r1 + dj, k = K * r2 + dk;
-                const long long c = PIdx(F, i, j, k);
+                const int col = (i + j + k) & 1, hh = i >> 1;
+                const int row = k * F.pplane + j * F.prow;
+                const int c = col * F.pcol + row + hh;
+                const int o = (1 - col) * F.pcol + row;  // other colour, same row
                 const T self = x[c];
-                const T st = cx * (pk_nb<T, 0>(x, F, i, j, k, -1, self) + pk_nb<T, 0>(x, F, i, j, k, 1, self)) +
-                             cy * (pk_nb<T, 1>(x, F, i, j, k, -1, self) + pk_nb<T, 1>(x, F, i, j, k, 1, self)) +
-                             cz * (pk_nb<T, 2>(x, F, i, j, k, -1, self) + pk_nb<T, 2>(x, F, i, j, k, 1, self));
+                const T vw = (i > 0 || px) ? x[o + ((i > 0 ? i - 1 : F.n0 - 1) >> 1)] : pk_wall<T>(F.lo[0]) * self;
+                const T ve = (i < F.n0 - 1 || px) ? x[o + ((i < F.n0 - 1 ? i + 1 : 0) >> 1)] : pk_wall<T>(F.hi[0]) * self;
+                const T vs = (j > 0 || py) ? x[o + hh + (j > 0 ? -F.prow : (F.n1 - 1) * F.prow)] : pk_wall<T>(F.lo[1]) * self;
+                const T vn = (j < F.n1 - 1 || py) ? x[o + hh + (j < F.n1 - 1 ? F.prow : -(F.n1 - 1) * F.prow)]
+                                                  : pk_wall<T>(F.hi[1]) * self;
+                const T vb = (k > 0 || pz) ? x[o + hh + (k > 0 ? -F.pplane : (F.n2 - 1) * F.pplane)]
+                                           : pk_wall<T>(F.lo[2]) * self;
+                const T vt = (k < F.n2 - 1 || pz) ? x[o + hh + (k < F.n2 - 1 ? F.pplane : -(F.n2 - 1) * F.pplane)]
+                                                  : pk_wall<T>(F.hi[2]) * self;
+                const T st = cx * (vw + ve) + cy * (vs + vn) + cz * (vb + vt);
                 const T v = b[c] - (st - dg * self);
                 s += v;
             }

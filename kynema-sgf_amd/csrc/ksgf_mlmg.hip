@@ -81,13 +81,12 @@ __device__ __forceinline__ T stencil(const T* __restrict__ a, long long c, int i
 // ghost layer (only for ghost_mode levels, the coarse part and phi)
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void ghost_point(T* __restrict__ a, const LevelGeom& L, long long t)
+__device__ __forceinline__ void ghost_point(T* __restrict__ a, const LevelGeom& L, int t)
 {
-    const long long nx = (long long)L.n1 * L.n2, ny = (long long)L.n0 * L.n2, nz = (long long)L.n0 * L.n1;
+    const int nx = L.n1 * L.n2, ny = L.n0 * L.n2, nz = L.n0 * L.n1;
     if (t < 2 * nx) {
-        int side = (int)(t / nx);
-        long long r = t % nx;
-        int j = (int)(r % L.n1), k = (int)(r / L.n1);
+        const int side = t / nx, r = t - side * nx;
+        const int j = r % L.n1, k = r / L.n1;
         T vl = a[LIdx(L, 0, j, k)], vr = a[LIdx(L, L.n0 - 1, j, k)];
         if (side == 0)
             a[LIdx(L, -1, j, k)] = bc_ghost<T>(L.lo[0], vl, vr);
@@ -97,9 +96,8 @@ __device__ __forceinline__ void ghost_point(T* __restrict__ a, const LevelGeom& 
     }
     t -= 2 * nx;
     if (t < 2 * ny) {
-        int side = (int)(t / ny);
-        long long r = t % ny;
-        int i = (int)(r % L.n0), k = (int)(r / L.n0);
+        const int side = t / ny, r = t - side * ny;
+        const int i = r % L.n0, k = r / L.n0;
         T vl = a[LIdx(L, i, 0, k)], vr = a[LIdx(L, i, L.n1 - 1, k)];
         if (side == 0)
             a[LIdx(L, i, -1, k)] = bc_ghost<T>(L.lo[1], vl, vr);
@@ -109,9 +107,8 @@ __device__ __forceinline__ void ghost_point(T* __restrict__ a, const LevelGeom& 
     }
     t -= 2 * ny;
     if (t < 2 * nz) {
-        int side = (int)(t / nz);
-        long long r = t % nz;
-        int i = (int)(r % L.n0), j = (int)(r / L.n0);
+        const int side = t / nz, r = t - side * nz;
+        const int i = r % L.n0, j = r / L.n0;
         T vl = a[LIdx(L, i, j, 0)], vr = a[LIdx(L, i, j, L.n2 - 1)];
         if (side == 0)
             a[LIdx(L, i, j, -1)] = bc_ghost<T>(L.lo[2], vl, vr);
@@ -128,8 +125,8 @@ __host__ __device__ __forceinline__ long long ghost_count(const LevelGeom& L)
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom L)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ghost_count(L)) ghost_point<T>(a, L, t);
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t < (int)ghost_count(L)) ghost_point<T>(a, L, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,13 +238,13 @@ __global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const
 
 // one red (color 0) or black (color 1) half sweep of a packed level over
 // row tiles of packed positions (i/2, j), kz planes per workgroup
-template <typename T, int MODE>
+template <typename T, int MODE, bool CP>
 __global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTiles t)
 {
     int h, j, k0, k1;
     row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, h, j, k0, k1, a.L.n2);
     if (h >= a.L.prow || j >= a.L.n1) return;
-    for (int k = k0; k < k1; ++k) pk_relax<T, MODE>(a, color, h, j, k);
+    for (int k = k0; k < k1; ++k) pk_relax<T, MODE, CP>(a, color, h, j, k);
 }
 
 // x += prolongation of the coarse correction on a packed level (only when
@@ -298,11 +295,42 @@ __device__ __forceinline__ void block_max_part(double m, double* __restrict__ pa
 
 // FP64 outer residual over row tiles.  phi and rhs are FP64 in the natural
 // layout P; the correction cor and the residual r are in the level-0 layout
-// L (packed or natural).  With a correction: phi_out = phi + cor and r = rhs
-// - A(phi + cor) in one pass (ping-pong phi buffers); without: r = rhs - A
-// phi.  r is written in precision R (FP32 feeds the mixed V-cycle
-// directly); per-block maxima of |r| go to part.
-template <typename C, typename R>
+// (packed when CP, else the natural layout P).  With a correction: phi_out =
+// phi + cor and r = rhs - A(phi + cor) in one pass (ping-pong phi buffers);
+// without: r = rhs - A phi.  r is written in precision R (FP32 feeds the
+// mixed V-cycle directly); per-block maxima of |r| go to part.  Each thread
+// marches up its column keeping phi + cor of planes k-1, k, k+1 in
+// registers, so the z neighbours are not fetched again.
+template <bool CP>
+struct ResOffsets {
+    int q, w, e, s, n;  // correction offsets: the cell and its x/y neighbours
+};
+
+template <bool CP>
+__device__ __forceinline__ ResOffsets<CP> res_offsets(const LevelGeom& L, int c, int i, int j, int k, int dw, int de,
+                                                      int ds, int dn, int iw, int ie)
+{
+    ResOffsets<CP> o;
+    if (CP) {
+        const int col = (i + j + k) & 1, h = i >> 1;
+        const int row = k * L.pplane + j * L.prow;
+        const int ob = (1 - col) * L.pcol + row;
+        o.q = col * L.pcol + row + h;
+        o.w = ob + (iw >> 1);
+        o.e = ob + (ie >> 1);
+        o.s = ob + h + ((j > 0) ? -L.prow : (L.n1 - 1) * L.prow);
+        o.n = ob + h + ((j < L.n1 - 1) ? L.prow : -(L.n1 - 1) * L.prow);
+    } else {
+        o.q = c;
+        o.w = c + dw;
+        o.e = c + de;
+        o.s = c + ds;
+        o.n = c + dn;
+    }
+    return o;
+}
+
+template <typename C, typename R, bool CP>
 __global__ void __launch_bounds__(256) k_update_residual(const double* __restrict__ phi, const C* __restrict__ cor,
                                                          double* __restrict__ phi_out, const double* __restrict__ rhs,
                                                          R* __restrict__ r, LevelGeom P, LevelGeom L, RowTiles tl,
@@ -312,41 +340,64 @@ __global__ void __launch_bounds__(256) k_update_residual(const double* __restric
     row_tile(tl, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, i, j, k0, k1, P.n2);
     double m = 0.0;
     if (i < P.n0 && j < P.n1) {
-        const bool px = P.lo[0] == KSGF_LO_PERIODIC, py = P.lo[1] == KSGF_LO_PERIODIC, pz = P.lo[2] == KSGF_LO_PERIODIC;
-        for (int k = k0; k < k1; ++k) {
-            const long long c = LIdx(P, i, j, k);
-            const long long q = AnyIdx(L, i, j, k);
-            double v;
-            if (P.ghost_mode) {
-                // explicit ghosts (odd periodic extents): with a correction only
-                // phi += cor (the caller fills the ghosts and calls again without
-                // one); without, the plain residual from the filled ghosts
+        if (P.ghost_mode) {
+            // explicit ghosts (odd periodic extents, natural layout): with a
+            // correction only phi += cor (the caller fills the ghosts and calls
+            // again without one); without, the plain residual from the ghosts
+            for (int k = k0; k < k1; ++k) {
+                const int c = (int)LIdx(P, i, j, k);
+                double v;
                 if (cor) {
-                    phi_out[c] = phi[c] + (double)cor[q];
+                    phi_out[c] = phi[c] + (double)cor[c];
                     v = 0.0;
                 } else {
                     v = rhs[c] - (stencil<double>(phi, c, i, j, k, P, P.cx, P.cy, P.cz, phi[c]) - P.dg * phi[c]);
                 }
-            } else {
-                auto val = [&](int ii, int jj, int kk) {
-                    double x = phi[LIdx(P, ii, jj, kk)];
-                    if (cor) x = x + (double)cor[AnyIdx(L, ii, jj, kk)];
+                r[c] = (R)v;
+                m = fmax(m, fabs(v));
+            }
+        } else {
+            const bool px = P.lo[0] == KSGF_LO_PERIODIC, py = P.lo[1] == KSGF_LO_PERIODIC,
+                       pz = P.lo[2] == KSGF_LO_PERIODIC;
+            const int sy = (int)P.sy;
+            const int dw = (i > 0) ? -1 : P.n0 - 1, de = (i < P.n0 - 1) ? 1 : -(P.n0 - 1);
+            const int ds = (j > 0) ? -sy : (P.n1 - 1) * sy, dn = (j < P.n1 - 1) ? sy : -(P.n1 - 1) * sy;
+            const int iw = (i > 0) ? i - 1 : P.n0 - 1, ie = (i < P.n0 - 1) ? i + 1 : 0;
+            const bool in_w = i > 0 || px, in_e = i < P.n0 - 1 || px, in_s = j > 0 || py, in_n = j < P.n1 - 1 || py;
+            // phi + cor of this column at plane kk (wrapped); only called in range
+            auto colv = [&](int kk) {
+                const int c = (int)LIdx(P, i, j, kk);
+                double x = phi[c];
+                if (cor) x = x + (double)cor[CP ? (int)PIdx(L, i, j, kk) : c];
+                return x;
+            };
+            double vm = 0.0, v0 = colv(k0), vp = 0.0;
+            if (k0 > 0 || pz) vm = colv(k0 > 0 ? k0 - 1 : P.n2 - 1);
+            for (int k = k0; k < k1; ++k) {
+                if (k < P.n2 - 1 || pz) vp = colv(k < P.n2 - 1 ? k + 1 : 0);
+                const int c = (int)LIdx(P, i, j, k);
+                const ResOffsets<CP> o = res_offsets<CP>(L, c, i, j, k, dw, de, ds, dn, iw, ie);
+                auto val = [&](int pc, int qc) {
+                    double x = phi[pc];
+                    if (cor) x = x + (double)cor[qc];
                     return x;
                 };
-                const double self = cor ? phi[c] + (double)cor[q] : phi[c];
+                const double self = v0;
                 auto wall = [&](int bc) { return bc == KSGF_LO_NEUMANN ? self : -self; };
-                const double w = (i > 0 || px) ? val(i > 0 ? i - 1 : P.n0 - 1, j, k) : wall(P.lo[0]);
-                const double e = (i < P.n0 - 1 || px) ? val(i < P.n0 - 1 ? i + 1 : 0, j, k) : wall(P.hi[0]);
-                const double s = (j > 0 || py) ? val(i, j > 0 ? j - 1 : P.n1 - 1, k) : wall(P.lo[1]);
-                const double n = (j < P.n1 - 1 || py) ? val(i, j < P.n1 - 1 ? j + 1 : 0, k) : wall(P.hi[1]);
-                const double bo = (k > 0 || pz) ? val(i, j, k > 0 ? k - 1 : P.n2 - 1) : wall(P.lo[2]);
-                const double t = (k < P.n2 - 1 || pz) ? val(i, j, k < P.n2 - 1 ? k + 1 : 0) : wall(P.hi[2]);
+                const double w = in_w ? val(c + dw, o.w) : wall(P.lo[0]);
+                const double e = in_e ? val(c + de, o.e) : wall(P.hi[0]);
+                const double s = in_s ? val(c + ds, o.s) : wall(P.lo[1]);
+                const double n = in_n ? val(c + dn, o.n) : wall(P.hi[1]);
+                const double bo = (k > 0 || pz) ? vm : wall(P.lo[2]);
+                const double t = (k < P.n2 - 1 || pz) ? vp : wall(P.hi[2]);
                 const double st = P.cx * (w + e) + P.cy * (s + n) + P.cz * (bo + t);
-                v = rhs[c] - (st - P.dg * self);
+                const double v = rhs[c] - (st - P.dg * self);
                 if (cor) phi_out[c] = self;
+                r[o.q] = (R)v;
+                m = fmax(m, fabs(v));
+                vm = v0;
+                v0 = vp;
             }
-            r[q] = (R)v;
-            m = fmax(m, fabs(v));
         }
     }
     block_max_part(m, part);
@@ -440,27 +491,25 @@ struct LevelPtrs {
     int r0, r1, r2;  // ratio to the next coarser level
 };
 
+// block-wide sum / maximum: a wavefront butterfly, then the per-wave
+// results in fixed order (deterministic, two barriers)
 __device__ double blk_sum(double v, double* sh)
 {
-    sh[threadIdx.x] = v;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    double r = sh[0];
+    double r = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r += sh[q];
     __syncthreads();
     return r;
 }
 __device__ double blk_max(double v, double* sh)
 {
-    sh[threadIdx.x] = v;
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
-        __syncthreads();
-    }
-    double r = sh[0];
+    double r = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r = fmax(r, sh[q]);
     __syncthreads();
     return r;
 }
@@ -468,17 +517,14 @@ __device__ double blk_max(double v, double* sh)
 template <typename T>
 __device__ void blk_ghosts(T* a, const LevelGeom& L)
 {
-    const long long n = ghost_count(L);
-    for (long long t = threadIdx.x; t < n; t += blockDim.x) ghost_point<T>(a, L, t);
+    const int n = (int)ghost_count(L);
+    for (int t = (int)threadIdx.x; t < n; t += (int)blockDim.x) ghost_point<T>(a, L, t);
     __syncthreads();
 }
 
 #define BLK_FOR(L, t_, c)                                                                              \
-    for (long long t_ = threadIdx.x, ncell_ = (long long)(L).n0 * (L).n1 * (L).n2; t_ < ncell_;          \
-         t_ += blockDim.x)                                                                             \
-        for (long long c = LIdx((L), (int)(t_ % (L).n0), (int)((t_ / (L).n0) % (L).n1),                 \
-                                (int)((t_ / (L).n0) / (L).n1)),                                        \
-                       once_ = 0;                                                                      \
+    for (int t_ = (int)threadIdx.x, ncell_ = (L).n0 * (L).n1 * (L).n2; t_ < ncell_; t_ += (int)blockDim.x)  \
+        for (long long c = LIdx((L), t_ % (L).n0, (t_ / (L).n0) % (L).n1, (t_ / (L).n0) / (L).n1), once_ = 0; \
              once_ < 1; ++once_)
 
 template <typename T>
@@ -490,8 +536,8 @@ __device__ void blk_smooth(T* x, const T* b, const LevelGeom& L, int nu)
             blk_ghosts<T>(x, L);
             BLK_FOR(L, t_, c)
             {
-                long long q = t_ / L.n0;
-                int i = (int)(t_ % L.n0), j = (int)(q % L.n1), k = (int)(q / L.n1);
+                const int q = t_ / L.n0;
+                const int i = t_ % L.n0, j = q % L.n1, k = q / L.n1;
                 if (((i + j + k + color) & 1) == 0)
                     x[c] = (cx * (x[c - 1] + x[c + 1]) + cy * (x[c - L.sy] + x[c + L.sy]) +
                             cz * (x[c - L.sz] + x[c + L.sz]) - b[c]) *
@@ -629,8 +675,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtr
         T* cres = lp[l + 1].res;
         BLK_FOR(C, t_, cc)
         {
-            long long q = t_ / C.n0;
-            int I = (int)(t_ % C.n0), J = (int)(q % C.n1), K = (int)(q / C.n1);
+            const int q = t_ / C.n0;
+            const int I = t_ % C.n0, J = q % C.n1, K = q / C.n1;
             T s = (T)0;
             for (int dk = 0; dk < r2; ++dk)
                 for (int dj = 0; dj < r1; ++dj)
@@ -670,8 +716,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtr
         const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
         BLK_FOR(F, t_, c)
         {
-            long long q = t_ / F.n0;
-            int i = (int)(t_ % F.n0), j = (int)(q % F.n1), k = (int)(q / F.n1);
+            const int q = t_ / F.n0;
+            const int i = t_ % F.n0, j = q % F.n1, k = q / F.n1;
             x[c] += xc[LIdx(C, i / r0, j / r1, k / r2)];
         }
         __syncthreads();
@@ -871,7 +917,7 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
 
 // planes per workgroup of the row-tile kernels
 constexpr int kRestrictKz = 4;
-constexpr int kResidualKz = 8;
+constexpr int kResidualKz = 16;
 
 template <typename T>
 static PkArgs<T> pk_args(const ksgf_macproj* mp, int l, T* x, const T* b, const T* xc)
@@ -895,11 +941,13 @@ static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hi
     const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, kz);
     const dim3 g(t.ntx * t.nty * t.nkc), blk(256);
     if (mode == kPkZero)
-        hipLaunchKernelGGL((k_pk_half<T, kPkZero>), g, blk, 0, st, a, color, t);
+        hipLaunchKernelGGL((k_pk_half<T, kPkZero, false>), g, blk, 0, st, a, color, t);
+    else if (mode == kPkProlong && a.C.packed)
+        hipLaunchKernelGGL((k_pk_half<T, kPkProlong, true>), g, blk, 0, st, a, color, t);
     else if (mode == kPkProlong)
-        hipLaunchKernelGGL((k_pk_half<T, kPkProlong>), g, blk, 0, st, a, color, t);
+        hipLaunchKernelGGL((k_pk_half<T, kPkProlong, false>), g, blk, 0, st, a, color, t);
     else
-        hipLaunchKernelGGL((k_pk_half<T, kPkPlain>), g, blk, 0, st, a, color, t);
+        hipLaunchKernelGGL((k_pk_half<T, kPkPlain, false>), g, blk, 0, st, a, color, t);
 }
 
 // one red-black sweep (red half, black half) of packed level l
@@ -1008,6 +1056,7 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     if (L.ghost_mode) launch_ghosts<T>(cor[l], L, st);
     {
         RowTiles t = row_tiles(C.n0, C.n1, C.n2, kRestrictKz);
+        if (t.ntx * t.nty * t.nkc < 2048) t = row_tiles(C.n0, C.n1, C.n2, 1);
         hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, (const T*)cor[l],
                            (const T*)res[l], res[l + 1], L, C, t, r[0], r[1], r[2], w);
     }
@@ -1054,16 +1103,20 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
     if (P.ghost_mode) {
         // explicit ghosts: phi += cor first, then the residual from ghosts
         if (cor) {
-            hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
+            hipLaunchKernelGGL((k_update_residual<C, R, false>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
                                mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
             std::swap(mp->phi, mp->phi2);
         }
         launch_ghosts<double>(mp->phi, P, st);
-        hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
+        hipLaunchKernelGGL((k_update_residual<C, R, false>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
                            (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
     } else {
-        hipLaunchKernelGGL((k_update_residual<C, R>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor, mp->phi2,
-                           (const double*)mp->rhs, r, P, L0, t, mp->rpart);
+        if (L0.packed)
+            hipLaunchKernelGGL((k_update_residual<C, R, true>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
+                               mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
+        else
+            hipLaunchKernelGGL((k_update_residual<C, R, false>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
+                               cor, mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
         if (cor) std::swap(mp->phi, mp->phi2);
     }
     hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(256), 0, st, (const double*)mp->rpart, nb, mp->dnorm);
@@ -1293,6 +1346,11 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     mp->lc = mp->nlev - 1;
     while (mp->lc > 0 && (long long)mp->G[mp->lc - 1].n0 * mp->G[mp->lc - 1].n1 * mp->G[mp->lc - 1].n2 <= kCoarseCells)
         mp->lc--;
+    if (mp->G[0].tot >= (1LL << 31)) {
+        ksgf_macproj_destroy(mp);
+        set_error("ksgf_macproj_create: level 0 exceeds 2^31 elements");
+        return KSGF_ERR_UNSUPPORTED;
+    }
     // phi and rhs stay in the natural layout; grid levels without explicit
     // ghosts and with an even x extent take the red-black packed layout
     mp->P = mp->G[0];

@@ -147,7 +147,12 @@ static void run_half(int n0, int n1, int n2, const int lo[3], const int hi[3], c
     a.s2 = ratio[2] == 2;
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
-            for (int h = 0; h < n0 / 2; ++h) pk_relax<T, MODE>(a, color, h, j, k);
+            for (int h = 0; h < n0 / 2; ++h) {
+                if (C.packed)
+                    pk_relax<T, MODE, true>(a, color, h, j, k);
+                else
+                    pk_relax<T, MODE, false>(a, color, h, j, k);
+            }
     long long bad = 0;
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
