@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/ksgf.h"
+#include "ksgf_mg_grid.h"
 
 namespace ksgf {
 
