@@ -8,13 +8,7 @@
 #pragma once
 
 #include "../../include/ksgf.h"
-
-#if defined(__HIPCC__)
-#include <hip/hip_runtime.h>
-#define KSGF_HD __host__ __device__ __forceinline__
-#else
-#define KSGF_HD inline
-#endif
+#include "ksgf_hd.h"
 
 namespace ksgf {
 

@@ -13,15 +13,14 @@
 // bit for bit against a CPU evaluation of the same formulas.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
 #include "../../include/ksgf.h"
+#include "ksgf_hd.h"
 
 namespace ksgf {
 
 constexpr double kSmallVel = 1.0e-8;
 
-__device__ __forceinline__ double vanleer(double a, double b, double c)
+KSGF_HD double vanleer(double a, double b, double c)
 {
     double dsc = 0.5 * (c - a);
     double dsl = 2.0 * (b - a);
@@ -33,13 +32,13 @@ __device__ __forceinline__ double vanleer(double a, double b, double c)
     return 0.0;
 }
 
-__device__ __forceinline__ double clip2(double e, double a, double b)
+KSGF_HD double clip2(double e, double a, double b)
 {
     e = fmax(e, fmin(a, b));
     return fmin(e, fmax(a, b));
 }
 
-__device__ __forceinline__ void mono(double s0, double& sm, double& sp)
+KSGF_HD void mono(double s0, double& sm, double& sp)
 {
     if ((sp - s0) * (s0 - sm) <= 0.0) {
         sp = s0;
@@ -51,13 +50,13 @@ __device__ __forceinline__ void mono(double s0, double& sm, double& sp)
     }
 }
 
-__device__ __forceinline__ double bnd_edge(double sb, double s0, double s1, double s2)
+KSGF_HD double bnd_edge(double sb, double s0, double s1, double s2)
 {
     return -0.2 * sb + 0.75 * s0 + 0.5 * s1 - 0.05 * s2;
 }
 
 // edge values of the cell whose 5-point stencil is (sm2 .. sp2)
-__device__ __forceinline__ void ppm_edges(double sm2, double sm1, double s0, double sp1, double sp2,
+KSGF_HD void ppm_edges(double sm2, double sm1, double s0, double sp1, double sp2,
                                           int lim, int bclo, int bchi, int x, int dlo, int dhi,
                                           double& sm, double& sp)
 {
@@ -102,7 +101,7 @@ __device__ __forceinline__ void ppm_edges(double sm2, double sm1, double s0, dou
     }
 }
 
-__device__ __forceinline__ double minmod_slope(double sm1, double s0, double sp1, int bclo, int bchi,
+KSGF_HD double minmod_slope(double sm1, double s0, double sp1, int bclo, int bchi,
                                                int x, int dlo, int dhi)
 {
     const bool lw = (bclo == KSGF_BC_EXT_DIR || bclo == KSGF_BC_HOEXTRAP);
@@ -116,7 +115,7 @@ __device__ __forceinline__ double minmod_slope(double sm1, double s0, double sp1
 
 // time-centred states at the low (Im) and high (Ip) faces of one cell;
 // um/up: advecting velocity at the low/high face
-__device__ __forceinline__ void ppm_trace(double sm2, double sm1, double s0, double sp1, double sp2,
+KSGF_HD void ppm_trace(double sm2, double sm1, double s0, double sp1, double sp2,
                                           int lim, int bclo, int bchi, int x, int dlo, int dhi,
                                           double um, double up, double dtdx, double& Im, double& Ip)
 {
@@ -148,20 +147,20 @@ __device__ __forceinline__ void ppm_trace(double sm2, double sm1, double s0, dou
     }
 }
 
-__device__ __forceinline__ double burgers(double lo, double hi)
+KSGF_HD double burgers(double lo, double hi)
 {
     double st = ((lo + hi) >= 0.0) ? lo : hi;
     bool ltm = ((lo <= 0.0 && hi >= 0.0) || (fabs(lo + hi) < kSmallVel));
     return ltm ? 0.0 : st;
 }
 
-__device__ __forceinline__ double upwind(double lo, double hi, double u)
+KSGF_HD double upwind(double lo, double hi, double u)
 {
     if (fabs(u) < kSmallVel) return 0.5 * (lo + hi);
     return (u >= 0.0) ? lo : hi;
 }
 
-__device__ __forceinline__ void edge_bcs(int x, int dlo, int dhi, int bclo, int bchi, double glo, double ghi,
+KSGF_HD void edge_bcs(int x, int dlo, int dhi, int bclo, int bchi, double glo, double ghi,
                                          bool both_dir, double& lo, double& hi)
 {
     if (x == dlo) {
