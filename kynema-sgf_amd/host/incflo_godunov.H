@@ -16,6 +16,14 @@
 //       ComputeDivergence (mult -1)    icns_advection.H:418-422
 //   compute_source_term(NPH) + ComputeRHSOp::predictor_rhs
 //                                      icns_ops.H:84-121, CompRHSOps.H:93-117
+// With a passive scalar (enableScalar), the scalar equation runs between the
+// MAC projection and the momentum advection, as ApplyPredictor orders the
+// scalar equations (incflo_advance.cpp:316-324):
+//   AdvectionOp<PDE,Godunov>::operator() with multiply_rho
+//       q = rho s, ComputeFluxesOnBoxFromState (ncomp 1, is_velocity false,
+//       zero forcing and divu), ComputeDivergence (mult -1)
+//                                      AdvOp_Godunov.H:116-300
+//   compute_predictor_rhs              CompRHSOps.H:93-117
 #pragma once
 
 #include <cmath>
@@ -114,6 +122,29 @@ public:
 
     void setCheckDivergence(bool on) { check_div_ = on; }
 
+    // passive scalar: one component, ghost 3, its own BCRec and values
+    // (6 orientations)
+    void enableScalar(const BCRec& bc, std::vector<double> bcval)
+    {
+        if (bcval.size() != 6) throw std::runtime_error("enableScalar: 6 BC values");
+        sc_ = std::make_unique<Scalar>(ba_);
+        sc_->bc = {bc};
+        sc_->bcval = std::move(bcval);
+    }
+    void setScalar(const double* s)
+    {
+        if (!sc_) throw std::runtime_error("setScalar: no scalar enabled");
+        sc_->s->copyFromDomain(s);
+        sc_->s->FillBoundary(st_);
+        sc_->s->FillPhysBC(sc_->bc, sc_->bcval, st_);
+    }
+    void getScalar(double* s) const
+    {
+        if (!sc_) throw std::runtime_error("getScalar: no scalar enabled");
+        (void)hipStreamSynchronize(st_);
+        sc_->s->copyToDomain(s);
+    }
+
     StepStats advance(double fixed_dt = -1.0)
     {
         StepStats s;
@@ -153,6 +184,7 @@ public:
             mac[d]->FillBoundary(st_);
             mac[d]->FillPhysBC(bc1, v1, st_);
         }
+        if (sc_) advanceScalar(dt);
         // advection of momentum
         check(ksgf_scale_by_rho(h, q_.data(), 3, vel_->data(), 3, rho_.data(), 3, 3, 3, 0, st), "q = rho u");
         check(ksgf_scale_by_rho(h, fq_.data(), 1, src_.data(), 1, rho_.data(), 3, 3, 1, 0, st), "fq = rho f");
@@ -173,6 +205,48 @@ public:
     }
 
 private:
+    struct Scalar {
+        explicit Scalar(const BoxArray& ba)
+            : s(std::make_unique<MultiFab>(ba, 1, 3)),
+              s_new(std::make_unique<MultiFab>(ba, 1, 3)),
+              q(ba, 1, 3),
+              fq(ba, 1, 1),
+              src(ba, 1, 0),
+              conv(ba, 1, 0),
+              fx(ba, 1, 0, IndexType::XFace),
+              fy(ba, 1, 0, IndexType::YFace),
+              fz(ba, 1, 0, IndexType::ZFace),
+              ex(ba, 1, 0, IndexType::XFace),
+              ey(ba, 1, 0, IndexType::YFace),
+              ez(ba, 1, 0, IndexType::ZFace)
+        {
+        }
+        std::unique_ptr<MultiFab> s, s_new;
+        MultiFab q, fq, src, conv, fx, fy, fz, ex, ey, ez;  // fq, src: zero (no scalar forcing)
+        std::vector<BCRec> bc;
+        std::vector<double> bcval;
+    };
+
+    void advanceScalar(double dt)
+    {
+        const ksgf_boxarray* h = ba_.handle();
+        ksgf_stream_t st = (ksgf_stream_t)st_;
+        Scalar& S = *sc_;
+        check(ksgf_scale_by_rho(h, S.q.data(), 3, S.s->data(), 3, rho_.data(), 3, 1, 3, 0, st), "q = rho s");
+        const int iconserv[1] = {1};
+        HydroUtils::ComputeFluxesOnBoxFromState(1, S.q, S.q, S.fx, S.fy, S.fz, S.ex, S.ey, S.ez, false, umac_, vmac_,
+                                                wmac_, nullptr, S.fq, ba_.geom(), dt, S.bc, iconserv, true, false,
+                                                false, false, "Godunov", KSGF_PPM_VANLEER, false, st_);
+        HydroUtils::ComputeDivergence(S.conv, S.fx, S.fy, S.fz, 1, ba_.geom(), -1.0, false, st_);
+        check(ksgf_predictor_update(h, S.s_new->data(), 3, S.s->data(), 3, rho_.data(), 3, rho_.data(), 3,
+                                    S.conv.data(), S.src.data(), nullptr, 0.0, dt, 1, st),
+              "scalar predictor_rhs");
+        S.s_new->FillBoundary(st_);
+        S.s_new->FillPhysBC(S.bc, S.bcval, st_);
+        std::swap(S.s, S.s_new);
+    }
+
+    std::unique_ptr<Scalar> sc_;
     BoxArray ba_;
     std::vector<BCRec> vel_bc_, src_bc_, rho_bc_;
     std::vector<double> vel_bcval_;

@@ -110,6 +110,42 @@ int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel)
     }
 }
 
+int ksgf_host_stepper_enable_scalar(ksgf_host_stepper* s, const ksgf_bcrec* bc, const double* bcval)
+{
+    try {
+        if (!s || !bc || !bcval) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->enableScalar(*bc, std::vector<double>(bcval, bcval + 6));
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_enable_scalar: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_set_scalar(ksgf_host_stepper* s, const double* scal)
+{
+    try {
+        if (!s || !scal) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->setScalar(scal);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_set_scalar: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_get_scalar(ksgf_host_stepper* s, double* scal)
+{
+    try {
+        if (!s || !scal) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->getScalar(scal);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_get_scalar: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
 {
     delete reinterpret_cast<GodunovPredictor*>(s);

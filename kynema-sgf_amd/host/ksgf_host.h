@@ -26,6 +26,11 @@ int ksgf_host_stepper_set_state(ksgf_host_stepper* s, const double* vel, const d
 int ksgf_host_stepper_step(ksgf_host_stepper* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info,
                            double* divmax);
 int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel);
+/* passive scalar advected with the step (AdvOp_Godunov.H:116-300):
+ * bc: its BCRec, bcval: 6 orientation values; then set / get domain-wide */
+int ksgf_host_stepper_enable_scalar(ksgf_host_stepper* s, const ksgf_bcrec* bc, const double* bcval);
+int ksgf_host_stepper_set_scalar(ksgf_host_stepper* s, const double* scal);
+int ksgf_host_stepper_get_scalar(ksgf_host_stepper* s, double* scal);
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
 
 #ifdef __cplusplus

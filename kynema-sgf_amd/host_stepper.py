@@ -32,6 +32,9 @@ def host_lib():
         L.ksgf_host_stepper_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(KsgfMLMGInfo),
                                              C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_velocity.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_enable_scalar.argtypes = [C.c_void_p, C.POINTER(KsgfBCRec), C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_set_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_get_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
         _h = L
     return _h
@@ -91,6 +94,22 @@ class HostStepper:
         self.last_dt = dtu.value
         self.last_divmax = dv.value if check_divergence else None
         return dtu.value
+
+    def enable_scalar(self, bc=None, bcval=None):
+        """a passive scalar advected with the step (1 BCRec, 6 values)"""
+        bc = np.zeros((1, 6), dtype=np.int32) if bc is None else np.asarray(bc, dtype=np.int32).reshape(1, 6)
+        self._sbcval = np.zeros(6) if bcval is None else np.ascontiguousarray(bcval, dtype=np.float64).reshape(6)
+        _chk(host_lib().ksgf_host_stepper_enable_scalar(self.h, bcrecs(bc), _dp(self._sbcval)), "enable_scalar")
+
+    def load_scalar(self, s):
+        a = np.ascontiguousarray(s, dtype=np.float64)
+        _chk(host_lib().ksgf_host_stepper_set_scalar(self.h, _dp(a)), "set_scalar")
+
+    def scalar(self):
+        n = self.case.geom.n_cell
+        out = np.zeros((1, n[2], n[1], n[0]))
+        _chk(host_lib().ksgf_host_stepper_get_scalar(self.h, _dp(out)), "get_scalar")
+        return out
 
     def velocity(self):
         n = self.case.geom.n_cell

@@ -13,6 +13,12 @@ diffusion solve and the nodal projection:
   q = rho u, fq = rho f, fluxes              icns_advection.H:281-374
   conv = -div F                              icns_advection.H:418-422
   u_new = (rho_o u_o + dt (conv + src))/rho  CompRHSOps.H:93-117
+
+With a passive scalar (scalar=True), the scalar is advected by the projected
+u_mac after the face fill and before the momentum, as ApplyPredictor orders
+the scalar equations (incflo_advance.cpp:316-324; AdvOp_Godunov.H:116-300
+with multiply_rho): q = rho s, conservative fluxes, conv = -div F,
+s_new = (rho_o s_o + dt conv) / rho_n.
 """
 from __future__ import annotations
 
@@ -24,7 +30,7 @@ from . import CELL, XFACE, DeviceBoxArray, MacProjector, default_options
 
 
 class AdvectProjectStep:
-    def __init__(self, case, options=None, limiter=1):
+    def __init__(self, case, options=None, limiter=1, scalar=False, scal_bc=None, scal_bcval=None):
         import torch
 
         self.case = case
@@ -56,6 +62,20 @@ class AdvectProjectStep:
         self.last_info = None
         self.last_dt = None
         self._torch = torch
+        self.scalar = scalar
+        if scalar:
+            self.s = dba.alloc(CELL, 1, 3)
+            self.s_new = dba.alloc(CELL, 1, 3)
+            self.qs = dba.alloc(CELL, 1, 3)
+            self.fqs = dba.alloc(CELL, 1, 1)
+            self.fqs.zero_()
+            self.flux_s = [dba.alloc(XFACE + d, 1, 0) for d in range(3)]
+            self.face_s = [dba.alloc(XFACE + d, 1, 0) for d in range(3)]
+            self.conv_s = dba.alloc(CELL, 1, 0)
+            self.src_s = dba.alloc(CELL, 1, 0)
+            self.src_s.zero_()
+            self.scal_bc = np.zeros((1, 6), dtype=np.int32) if scal_bc is None else np.asarray(scal_bc, np.int32)
+            self.scal_bcval = np.zeros((6, 1)) if scal_bcval is None else np.asarray(scal_bcval, np.float64)
 
     def load(self, vel=None, rho=None, gp=None):
         """Upload domain-wide fields (defaults: the case's) and fill ghosts."""
@@ -68,6 +88,11 @@ class AdvectProjectStep:
         self.rho.copy_(dba.upload(ba.scatter(rho, CELL, 3)))
         dba.fill_ghosts(self.rho, CELL, 1, 3, self.rho_bc, np.zeros((6, 1)))
         self.gp.copy_(dba.upload(ba.scatter(gp, CELL, 0)))
+
+    def load_scalar(self, s):
+        """Upload a domain-wide passive scalar (1, nz, ny, nx) and fill ghosts."""
+        self.s.copy_(self.dba.upload(self.case.ba.scatter(s, CELL, 3)))
+        self.dba.fill_ghosts(self.s, CELL, 1, 3, self.scal_bc, self.scal_bcval)
 
     def step(self, dt=None, swap=True):
         c, dba = self.case, self.dba
@@ -84,6 +109,16 @@ class AdvectProjectStep:
             dba.fill_boundary(self.mac[d], XFACE + d, 1, 1)
             dba.fill_physbc(self.mac[d], XFACE + d, 1, 1, self.bcrec[d:d + 1],
                             np.ascontiguousarray(self.bcval[:, d:d + 1]))
+        if self.scalar:
+            dba.scale_by_rho(self.qs, 3, self.s, 3, self.rho, 3, 1, 3)
+            dba.compute_fluxes(1, self.qs, self.qs, self.mac, None, self.fqs, dt, self.scal_bc, [1], 1, 0, 0, 0,
+                               self.limiter, 0, faces=self.face_s, flux=self.flux_s)
+            dba.compute_divergence(self.flux_s, 1, -1.0, 0, out=self.conv_s)
+            dba.predictor_update(self.s_new, 3, self.s, 3, self.rho, 3, self.rho, 3, self.conv_s, self.src_s, None,
+                                 0.0, dt, ncomp=1)
+            dba.fill_ghosts(self.s_new, CELL, 1, 3, self.scal_bc, self.scal_bcval)
+            if swap:
+                self.s, self.s_new = self.s_new, self.s
         dba.scale_by_rho(self.q, 3, self.vel, 3, self.rho, 3, 3, 3)
         dba.scale_by_rho(self.fq, 1, self.src, 1, self.rho, 3, 3, 1)
         dba.compute_fluxes(3, self.q, self.q, self.mac, None, self.fq, dt, self.bcrec, [1, 1, 1], 1, 0, 1, 0,

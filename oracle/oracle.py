@@ -264,8 +264,32 @@ def case_opts(case):
     return default_opts(**kw)
 
 
-def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1):
-    """One step; returns (vel_new (ng 3, ghosts filled), diagnostics dict)."""
+def advect_scalar(case, G, scal, rho, mac, dt, scal_bc, scal_bcval, limiter=1):
+    """Godunov advection of a passive scalar by the projected u_mac
+    (AdvectionOp<PDE, Godunov>::operator(), AdvOp_Godunov.H:116-300, with
+    multiply_rho; ApplyPredictor order incflo_advance.cpp:316-324): q = rho s,
+    conservative fluxes (iconserv 1, is_velocity 0, zero forcing and divu),
+    conv = -div F, s_new = (rho_o s_o + dt conv) / rho_n (CompRHSOps.H:93-117).
+    Returns s_new (ng 3, ghosts filled)."""
+    ba = case.ba
+    qs = scal.copy()
+    for b in range(ba.nbox):
+        ba.fab_view(qs, b, CELL, 1, 3)[...] *= ba.fab_view(rho, b, CELL, 1, 3)
+    fqs = np.zeros(ba.mf_size(CELL, 1, 1))
+    flux, _ = compute_fluxes(G, 1, qs, qs, mac, None, fqs, dt, scal_bc, [1], 1, 0, 0, 0, limiter, 0)
+    conv = compute_divergence(G, flux, 1, -1.0, 0)
+    s_new = np.zeros_like(scal)
+    predictor_update(G, s_new, 3, scal, 3, rho, 3, rho, 3, conv, np.zeros(ba.mf_size(CELL, 1, 0)), None, 0.0, dt,
+                     ncomp=1)
+    fill_ghosts(G, s_new, CELL, 1, 3, scal_bc, scal_bcval)
+    return s_new
+
+
+def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, scal=None, scal_bc=None,
+                        scal_bcval=None):
+    """One step; returns (vel_new (ng 3, ghosts filled), diagnostics dict).
+    With scal (cell, 1 comp, ng 3) the passive scalar is advected too; the
+    new scalar is diag["scal"]."""
     ba = case.ba
     opts = opts or case_opts(case)
     if dt is None:
@@ -307,6 +331,9 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1):
         bc1 = case.bcrec[d:d + 1]
         fill_boundary(G, mac[d], XFACE + d, 1, 1)
         fill_physbc(G, mac[d], XFACE + d, 1, 1, bc1, np.ascontiguousarray(case.bcval[:, d:d + 1]))
+    scal_new = None
+    if scal is not None:
+        scal_new = advect_scalar(case, G, scal, rho, mac, dt, scal_bc, scal_bcval, limiter)
     # momentum and forces
     q = vel.copy()
     fq = src.copy()
@@ -326,4 +353,4 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1):
     vel_new = np.zeros_like(vel)
     predictor_update(G, vel_new, 3, vel, 3, rho, 3, rho, 3, conv, src_nph, None, 0.0, dt)
     fill_ghosts(G, vel_new, CELL, 3, 3, case.bcrec, case.bcval)
-    return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face}
+    return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face, "scal": scal_new}
