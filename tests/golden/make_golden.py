@@ -1,0 +1,85 @@
+"""Write the golden fixtures of the advect + MAC-project step.
+
+The reference repository holds no golden vectors for this path (its plotfile
+goldens live outside the repository), so the fixtures are written by this
+project's FP64 oracle on small seeded inputs: the inputs and every stage
+output are stored domain-wide, x fastest, so a test can rebuild the case
+without the workload generators.
+
+  python tests/golden/make_golden.py      # rewrites tests/golden/*.npz
+
+Fixtures:
+  tgv8_step.npz   8^3 periodic Taylor-Green + noise, 8 boxes of 4^3
+  wall16_step.npz 16x12x16, periodic x/y, no-slip bottom / slip top, ragged
+                  boxes, varying density and a pressure gradient
+"""
+import importlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+
+lay = importlib.import_module("kynema-sgf_amd.layout")
+wl = importlib.import_module("kynema-sgf_amd.workloads")
+CELL, XFACE = 0, 1
+
+
+def cases():
+    tgv = wl.taylor_green(8, 4, noise=1e-3, seed=1, cfl=0.5)
+    rng = np.random.Generator(np.random.Philox(12))
+    wall = wl.random_case((16, 12, 16), boxes=lay.chop_random((16, 12, 16), [4, 8], rng), seed=5, periodic=(1, 1, 0),
+                          gp_amp=0.3, rho_var=0.5, cfl=0.7)
+    return {"tgv8_step": tgv, "wall16_step": wall}
+
+
+def record(case):
+    ba = case.ba
+    G, vel, rho, gp = O.make_state(case)
+    out = {
+        "n_cell": np.asarray(case.geom.n_cell, dtype=np.int32),
+        "prob_hi": np.asarray(case.geom.prob_hi, dtype=np.float64),
+        "periodic": np.asarray(case.geom.periodic, dtype=np.int32),
+        "boxes": np.asarray(ba.boxes, dtype=np.int32),
+        "bcrec": np.asarray(case.bcrec, dtype=np.int32),
+        "bcval": np.asarray(case.bcval, dtype=np.float64),
+        "lobc": np.asarray(case.lobc, dtype=np.int32),
+        "hibc": np.asarray(case.hibc, dtype=np.int32),
+        "cfl": np.float64(case.cfl),
+        "mg_rtol": np.float64(case.mg_rtol),
+        "mg_atol": np.float64(case.mg_atol),
+        "vel": case.vel,
+        "rho": case.rho,
+        "gp": case.gp,
+    }
+    vel_new, diag = O.advect_project_step(case, G, vel, rho, gp)
+    out["dt"] = np.float64(diag["dt"])
+    out["mlmg_iters"] = np.int32(diag["mlmg"]["iters"])
+    for d in range(3):
+        out[f"umac{d}"] = ba.gather(diag["mac"][d], XFACE + d, 1, 1)
+        out[f"flux{d}"] = ba.gather(diag["flux"][d], XFACE + d, 3, 0)
+    out["conv"] = ba.gather(diag["conv"], CELL, 3, 0)
+    out["vel_new"] = ba.gather(vel_new, CELL, 3, 3)
+    # the face states before the projection (ExtrapVelToFaces alone)
+    src = np.zeros(ba.mf_size(CELL, 3, 1))
+    mac0 = O.extrap_vel_to_faces(G, vel, src, case.bcrec, diag["dt"], 1, 0, 1, 0)
+    for d in range(3):
+        out[f"umac_pred{d}"] = ba.gather(mac0[d], XFACE + d, 1, 1)
+    return out
+
+
+def main():
+    O.build()
+    for name, case in cases().items():
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **record(case))
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()
