@@ -41,6 +41,7 @@ typedef struct ihipStream_t* ksgf_stream_t; /* a hipStream_t; NULL = default str
 #define KSGF_ERR_HIP -3         /* a HIP runtime call failed */
 #define KSGF_ERR_NOMEM -4       /* device allocation failed */
 #define KSGF_ERR_NOT_CONVERGED 1 /* MLMG stopped at maxiter (not an abort, do_fixed_iters) */
+#define KSGF_ERR_DIVERGED -5     /* MLMG residual above 1e20 |rhs| or not finite (AMReX aborts) */
 
 /* index types of a MultiFab */
 #define KSGF_CELL 0
@@ -198,6 +199,29 @@ int ksgf_macproj_destroy(ksgf_macproj* mp);
  * domain (the MLMG solve inside project); rhs/phi domain-wide (n0*n1*n2). */
 int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol,
                        double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
+
+/* Many independent Poisson problems solved together.  The reference solves
+ * each with its own MLMG (MLMG::solve behind MacProjector::project,
+ * icns_advection.cpp:331, one after another); here every multigrid stage of
+ * every problem is one launch over a table of (problem, tile) blocks, and
+ * one iteration is captured once into a hipGraph and replayed.  Each result
+ * equals ksgf_macproj_solve on that problem alone (same iterations).
+ * Problem p: n[3p..3p+2] cells over one box, dx[3p..], lobc/hibc[3p..]
+ * (KSGF_LO_*), beta[p].  Needs an even x extent on every grid level and at
+ * least one pre- and one post-smoothing sweep (else KSGF_ERR_UNSUPPORTED).
+ * solve: rhs[p], phi[p] domain-wide device arrays (n0*n1*n2, Fortran
+ * order); infos: np entries or NULL; use_graph 0 launches eagerly.
+ * Synchronises the stream once per iteration.  Returns KSGF_ERR_DIVERGED
+ * if any problem diverged, else KSGF_ERR_NOT_CONVERGED if any did not
+ * converge. */
+typedef struct ksgf_poisson_batch ksgf_poisson_batch;
+int ksgf_poisson_batch_create(int nprob, const int* n, const double* dx, const int* lobc,
+                              const int* hibc, const double* beta,
+                              const ksgf_mlmg_options* opts, ksgf_poisson_batch** out);
+int ksgf_poisson_batch_solve(ksgf_poisson_batch* pb, const double* const* rhs,
+                             double* const* phi, double rtol, double atol, int use_graph,
+                             ksgf_mlmg_info* infos, ksgf_stream_t stream);
+int ksgf_poisson_batch_destroy(ksgf_poisson_batch* pb);
 
 /* incflo::compute_dt convective part (incflo_compute_dt.cpp:69-91):
  * max over valid cells of max_d |u_d|/dx_d (floor -1).  Synchronises. */

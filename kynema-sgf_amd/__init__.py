@@ -57,6 +57,7 @@ EXPORTS = [
     "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
     "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_predictor_update",
     "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
+    "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
 ]
 
 _lib = None
@@ -106,6 +107,11 @@ def load(path=None):
     L.ksgf_macproj_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo),
                                      C.c_void_p]
     L.ksgf_macproj_get_phi.argtypes = [C.c_void_p, _dp, C.c_void_p]
+    L.ksgf_poisson_batch_create.argtypes = [C.c_int, _ip, C.POINTER(C.c_double), _ip, _ip, C.POINTER(C.c_double),
+                                            C.POINTER(KsgfMLMGOptions), C.POINTER(C.c_void_p)]
+    L.ksgf_poisson_batch_solve.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_double,
+                                           C.c_double, C.c_int, C.POINTER(KsgfMLMGInfo), C.c_void_p]
+    L.ksgf_poisson_batch_destroy.argtypes = [C.c_void_p]
     L.ksgf_macproj_destroy.argtypes = [C.c_void_p]
     L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
@@ -352,6 +358,61 @@ class MacProjector:
     def close(self):
         if self.h:
             load().ksgf_macproj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PoissonBatch:
+    """Many independent single-box Poisson problems solved together.
+
+    problems: list of dicts with n (3 ints), dx (3 floats), lobc/hibc (3 ints
+    each, KSGF_LO_*) and beta.  solve(rhs_list, phi_list) takes flat float64
+    device tensors of n0*n1*n2 each (Fortran order)."""
+
+    def __init__(self, problems, options=None):
+        import numpy as np
+
+        L = load()
+        self.np = len(problems)
+        self.n = [tuple(int(v) for v in p["n"]) for p in problems]
+        n = np.ascontiguousarray([p["n"] for p in problems], dtype=np.int32).ravel()
+        dx = np.ascontiguousarray([p["dx"] for p in problems], dtype=np.float64).ravel()
+        lo = np.ascontiguousarray([p["lobc"] for p in problems], dtype=np.int32).ravel()
+        hi = np.ascontiguousarray([p["hibc"] for p in problems], dtype=np.int32).ravel()
+        beta = np.ascontiguousarray([p.get("beta", 1.0) for p in problems], dtype=np.float64)
+        self.options = options or default_options()
+        h = C.c_void_p()
+        dp = C.POINTER(C.c_double)
+        check(L.ksgf_poisson_batch_create(self.np, n.ctypes.data_as(_ip), dx.ctypes.data_as(dp),
+                                          lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip), beta.ctypes.data_as(dp),
+                                          C.byref(self.options), C.byref(h)), "ksgf_poisson_batch_create")
+        self.h = h
+        self.infos = None
+
+    def solve(self, rhs, phi, rtol, atol, use_graph=True):
+        if len(rhs) != self.np or len(phi) != self.np:
+            raise ValueError("one rhs and one phi per problem")
+        for p in range(self.np):
+            m = self.n[p][0] * self.n[p][1] * self.n[p][2]
+            for t in (rhs[p], phi[p]):
+                if t.numel() != m or not t.is_contiguous() or str(t.dtype) != "torch.float64":
+                    raise ValueError(f"problem {p}: contiguous float64 arrays of {m} cells expected")
+        r = (C.c_void_p * self.np)(*[_ptr(t) for t in rhs])
+        f = (C.c_void_p * self.np)(*[_ptr(t) for t in phi])
+        infos = (KsgfMLMGInfo * self.np)()
+        check(load().ksgf_poisson_batch_solve(self.h, r, f, rtol, atol, int(use_graph), infos, _stream()),
+              "poisson_batch_solve", allow=(0, 1))
+        self.infos = [i.as_dict() for i in infos]
+        return self.infos
+
+    def close(self):
+        if self.h:
+            load().ksgf_poisson_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -39,6 +39,10 @@
 
 namespace ksgf {
 
+// a multigrid iteration whose residual exceeds this multiple of |rhs|_inf
+// (or is not finite) is diverging and stops
+constexpr double kDivergeFactor = 1.0e20;
+
 constexpr long long kCoarseCells = 16384;
 constexpr int kCoarseThreads = 1024;
 
@@ -330,14 +334,21 @@ __device__ __forceinline__ ResOffsets<CP> res_offsets(const LevelGeom& L, int c,
     return o;
 }
 
+// |v| for a residual norm; NaN counts as +inf (fmax alone would drop it)
+__device__ __forceinline__ double abs_or_inf(double v)
+{
+    const double a = fabs(v);
+    return a == a ? a : __builtin_huge_val();
+}
+
 template <typename C, typename R, bool CP>
-__global__ void __launch_bounds__(256) k_update_residual(const double* __restrict__ phi, const C* __restrict__ cor,
-                                                         double* __restrict__ phi_out, const double* __restrict__ rhs,
-                                                         R* __restrict__ r, LevelGeom P, LevelGeom L, RowTiles tl,
-                                                         double* __restrict__ part)
+__device__ __forceinline__ double update_residual_tile(const double* __restrict__ phi, const C* __restrict__ cor,
+                                                       double* __restrict__ phi_out, const double* __restrict__ rhs,
+                                                       R* __restrict__ r, const LevelGeom& P, const LevelGeom& L,
+                                                       const RowTiles& tl, int lid)
 {
     int i, j, k0, k1;
-    row_tile(tl, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, i, j, k0, k1, P.n2);
+    row_tile(tl, lid, (int)threadIdx.x, i, j, k0, k1, P.n2);
     double m = 0.0;
     if (i < P.n0 && j < P.n1) {
         if (P.ghost_mode) {
@@ -354,7 +365,7 @@ __global__ void __launch_bounds__(256) k_update_residual(const double* __restric
                     v = rhs[c] - (stencil<double>(phi, c, i, j, k, P, P.cx, P.cy, P.cz, phi[c]) - P.dg * phi[c]);
                 }
                 r[c] = (R)v;
-                m = fmax(m, fabs(v));
+                m = fmax(m, abs_or_inf(v));
             }
         } else {
             const bool px = P.lo[0] == KSGF_LO_PERIODIC, py = P.lo[1] == KSGF_LO_PERIODIC,
@@ -394,12 +405,23 @@ __global__ void __launch_bounds__(256) k_update_residual(const double* __restric
                 const double v = rhs[c] - (st - P.dg * self);
                 if (cor) phi_out[c] = self;
                 r[o.q] = (R)v;
-                m = fmax(m, fabs(v));
+                m = fmax(m, abs_or_inf(v));
                 vm = v0;
                 v0 = vp;
             }
         }
     }
+    return m;
+}
+
+template <typename C, typename R, bool CP>
+__global__ void __launch_bounds__(256) k_update_residual(const double* __restrict__ phi, const C* __restrict__ cor,
+                                                         double* __restrict__ phi_out, const double* __restrict__ rhs,
+                                                         R* __restrict__ r, LevelGeom P, LevelGeom L, RowTiles tl,
+                                                         double* __restrict__ part)
+{
+    const double m = update_residual_tile<C, R, CP>(phi, cor, phi_out, rhs, r, P, L, tl,
+                                                    xcd_remap((int)blockIdx.x, (int)gridDim.x));
     block_max_part(m, part);
 }
 
@@ -655,11 +677,10 @@ __device__ void bottom_solve(double* x, double* b, const LevelGeom& L, const Bot
 // levels lc .. nlev-1 of a V-cycle: zero, pre-smooth, residual, restrict
 // down to the bottom, bottom solve (FP64), prolong, post-smooth back up
 template <typename T>
-__global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev,
-                                                                  BottomWork w, int singular, double rtol, double atol,
-                                                                  int maxiter, int nub, int nuf, int nu1, int nu2)
+__device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev, const BottomWork& w,
+                                   int singular, double rtol, double atol, int maxiter, int nub, int nuf, int nu1,
+                                   int nu2, double* sh)
 {
-    __shared__ double sh[kCoarseThreads];
     for (int l = lc; l < nlev - 1; ++l) {
         const LevelGeom F = lp[l].G;
         const LevelGeom C = lp[l + 1].G;
@@ -723,6 +744,15 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtr
         __syncthreads();
         blk_smooth<T>(x, lp[l].res, F, nu2);
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev,
+                                                                  BottomWork w, int singular, double rtol, double atol,
+                                                                  int maxiter, int nub, int nuf, int nu1, int nu2)
+{
+    __shared__ double sh[kCoarseThreads];
+    coarse_vcycle_body<T>(lp, lc, nlev, w, singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1145,6 +1175,7 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
     inf.r0 = r;
     inf.nlevels = mp->nlev;
     inf.converged = (r <= target);
+    bool diverged = false;
     if (!inf.converged) {
         for (int it = 1; it <= mp->opt.maxiter; ++it) {
             if (mixed) {
@@ -1162,6 +1193,10 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
                 inf.converged = 1;
                 break;
             }
+            if (!(r <= kDivergeFactor * rhsnorm)) {
+                diverged = true;
+                break;
+            }
         }
     }
     launch_ghosts<double>(mp->phi, L0, st);
@@ -1172,6 +1207,11 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
     inf.bottom_iters = mp->hstats[0];
     inf.bottom_fail = mp->hstats[1];
     if (info) *info = inf;
+    if (diverged) {
+        set_error("MLMG: the residual " + std::to_string(r) + " exceeds 1e20 |rhs| = " + std::to_string(rhsnorm) +
+                  " after " + std::to_string(inf.iters) + " iterations (diverging)");
+        return KSGF_ERR_DIVERGED;
+    }
     return inf.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
 }
 
@@ -1533,6 +1573,603 @@ int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream)
                        L0, 0);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Many independent Poisson problems solved together.  Each problem has its
+// own hierarchy (a projector over a one-box domain, autotuned as usual); the
+// batch runs every multigrid stage of every problem as one launch over a
+// ragged table of (problem, tile) blocks, the coarse part as one workgroup
+// per problem, and the FP64 outer residual of all problems in one launch.
+// Per-problem state (active flag, current phi buffer, iteration count,
+// norm, target) lives on the device, so the launch sequence of one
+// iteration never changes: it is captured once into a hipGraph and replayed
+// until no problem is active.
+// ===========================================================================
+namespace ksgf {
+
+template <typename T>
+struct BLevel {  // one problem at one grid level
+    PkArgs<T> pk;  // x = cor[l], b = res[l], xc = cor[l+1]
+    RowTiles th;   // half-sweep tiles over packed positions
+    RowTiles tr;   // restriction tiles over the coarse level
+    T* cres;       // res[l+1]
+    T w;
+    int r0, r1, r2;
+    int prob;
+};
+
+template <typename T>
+struct BTop {  // one problem's FP64 outer residual (level 0)
+    LevelGeom P, L0;
+    double* phi[2];
+    const double* rhs;
+    const T* cor;
+    T* r;
+    RowTiles t;
+    int prob;
+};
+
+template <typename T>
+struct BCoarse {
+    const LevelPtrs<T>* lp;
+    BottomWork w;
+    int lc, nlev, singular, prob;
+};
+
+struct BState {
+    int* active;     // 1 while the problem iterates
+    int* cur;        // phi buffer holding the current iterate
+    int* iters;
+    double* norm;    // last |r|_inf
+    double* target;  // max(atol, rtol |rhs|_inf)
+    double* bound;   // kDivergeFactor |rhs|_inf
+    double* part;    // per-block maxima
+    const int* part0;  // first partial slot of each problem
+    int maxiter;
+};
+
+// entry e of a batched launch with pre[e] <= b < pre[e+1]
+__device__ __forceinline__ int batch_entry(const int* __restrict__ pre, int ne, int b)
+{
+    int lo = 0, hi = ne - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_pk_half_batch(const BLevel<T>* __restrict__ lv, const int* __restrict__ pre,
+                                                       int ne, int color, BState S)
+{
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int e = batch_entry(pre, ne, lid);
+    const BLevel<T>& L = lv[e];
+    if (!S.active[L.prob]) return;
+    int h, j, k0, k1;
+    row_tile(L.th, lid - pre[e], (int)threadIdx.x, h, j, k0, k1, L.pk.L.n2);
+    if (h >= L.pk.L.prow || j >= L.pk.L.n1) return;
+    if (MODE == kPkProlong && L.pk.C.packed) {
+        for (int k = k0; k < k1; ++k) pk_relax<T, MODE, true>(L.pk, color, h, j, k);
+    } else {
+        for (int k = k0; k < k1; ++k) pk_relax<T, MODE, false>(L.pk, color, h, j, k);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict_batch(const BLevel<T>* __restrict__ lv, const int* __restrict__ pre,
+                                                        int ne, BState S)
+{
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int e = batch_entry(pre, ne, lid);
+    const BLevel<T>& L = lv[e];
+    if (!S.active[L.prob]) return;
+    const LevelGeom& C = L.pk.C;
+    int I, J, K0, K1;
+    row_tile(L.tr, lid - pre[e], (int)threadIdx.x, I, J, K0, K1, C.n2);
+    if (I >= C.n0 || J >= C.n1) return;
+    for (int K = K0; K < K1; ++K)
+        L.cres[AnyIdx(C, I, J, K)] =
+            pk_restrict_cell<T>(L.pk.x, L.pk.b, L.pk.L, I, J, K, L.r0, L.r1, L.r2, L.w);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse_batch(const BCoarse<T>* __restrict__ cb, BState S,
+                                                                 double rtol, double atol, int maxiter, int nub,
+                                                                 int nuf, int nu1, int nu2)
+{
+    __shared__ double sh[kCoarseThreads];
+    const BCoarse<T>& c = cb[blockIdx.x];
+    if (!S.active[c.prob]) return;
+    coarse_vcycle_body<T>(c.lp, c.lc, c.nlev, c.w, c.singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh);
+}
+
+// FP64 outer residual of every problem; with_cor: phi(next) = phi + cor
+template <typename T>
+__global__ void __launch_bounds__(256) k_update_residual_batch(const BTop<T>* __restrict__ tp,
+                                                               const int* __restrict__ pre, int ne, int with_cor,
+                                                               BState S)
+{
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int e = batch_entry(pre, ne, lid);
+    const BTop<T>& t = tp[e];
+    if (!S.active[t.prob]) return;
+    const int c = S.cur[t.prob];
+    const double m = update_residual_tile<T, T, true>(t.phi[c], with_cor ? t.cor : (const T*)nullptr, t.phi[1 - c],
+                                                      t.rhs, t.r, t.P, t.L0, t.t, lid - pre[e]);
+    __shared__ double sh[4];
+    double v = m;
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) S.part[S.part0[t.prob] + (lid - pre[e])] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// one workgroup per problem: the new norm, the phi buffer flip, the
+// iteration count and whether the problem keeps iterating.  first: the
+// norm of the right-hand side (sets the target, no flip, no count).
+__global__ void __launch_bounds__(256) k_batch_finalize(const int* __restrict__ nparts, int first, double rtol,
+                                                        double atol, BState S)
+{
+    const int p = blockIdx.x;
+    if (!S.active[p]) return;
+    double m = 0.0;
+    for (int q = (int)threadIdx.x; q < nparts[p]; q += (int)blockDim.x) m = fmax(m, S.part[S.part0[p] + q]);
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        S.norm[p] = m;
+        if (first) {
+            S.target[p] = fmax(atol, rtol * m);
+            S.bound[p] = kDivergeFactor * m;
+            S.iters[p] = 0;
+            S.active[p] = (m > S.target[p] && S.maxiter > 0) ? 1 : 0;
+        } else {
+            S.cur[p] ^= 1;
+            S.iters[p] += 1;
+            S.active[p] = (m > S.target[p] && m <= S.bound[p] && S.iters[p] < S.maxiter) ? 1 : 0;
+        }
+    }
+}
+
+}  // namespace ksgf
+
+struct ksgf_poisson_batch {
+    int np = 0;
+    bool mixed = false;
+    std::vector<ksgf_boxarray*> bas;
+    std::vector<ksgf_macproj*> mps;
+    int lmax = 0;  // grid levels of the deepest problem
+    struct Tab {
+        void* lv = nullptr;  // BLevel<T>[ne]
+        int* pre_h = nullptr;  // half-sweep block prefix [ne+1]
+        int* pre_r = nullptr;  // restriction block prefix
+        int ne = 0, nb_h = 0, nb_r = 0;
+    };
+    std::vector<Tab> lv;
+    void* top = nullptr;
+    int* top_pre = nullptr;
+    int top_nb = 0;
+    void* coarse = nullptr;
+    int ncoarse = 0;
+    int* nparts = nullptr;
+    int* part0 = nullptr;
+    double* part = nullptr;
+    int *active = nullptr, *cur = nullptr, *iters = nullptr;
+    double *norm = nullptr, *target = nullptr, *bound = nullptr;
+    int* h_state = nullptr;     // pinned: active, cur, iters (3*np)
+    double* h_norm = nullptr;   // pinned: norm, target, bound (3*np)
+    hipStream_t cap = nullptr;  // capture stream
+    hipGraphExec_t exec = nullptr;
+    ksgf_mlmg_options opt{};
+};
+
+namespace ksgf {
+
+static BState batch_state(const ksgf_poisson_batch* pb)
+{
+    BState S;
+    S.active = pb->active;
+    S.cur = pb->cur;
+    S.iters = pb->iters;
+    S.norm = pb->norm;
+    S.target = pb->target;
+    S.bound = pb->bound;
+    S.part = pb->part;
+    S.part0 = pb->part0;
+    S.maxiter = pb->opt.maxiter;
+    return S;
+}
+
+template <typename T>
+static std::vector<T*>& cor_of(ksgf_macproj* mp);
+template <>
+std::vector<float*>& cor_of<float>(ksgf_macproj* mp)
+{
+    return mp->cor32;
+}
+template <>
+std::vector<double*>& cor_of<double>(ksgf_macproj* mp)
+{
+    return mp->cor;
+}
+template <typename T>
+static std::vector<T*>& res_of(ksgf_macproj* mp);
+template <>
+std::vector<float*>& res_of<float>(ksgf_macproj* mp)
+{
+    return mp->res32;
+}
+template <>
+std::vector<double*>& res_of<double>(ksgf_macproj* mp)
+{
+    return mp->res;
+}
+template <typename T>
+static const LevelPtrs<T>* dlp_of(ksgf_macproj* mp);
+template <>
+const LevelPtrs<float>* dlp_of<float>(ksgf_macproj* mp)
+{
+    return mp->dlp32;
+}
+template <>
+const LevelPtrs<double>* dlp_of<double>(ksgf_macproj* mp)
+{
+    return mp->dlp64;
+}
+
+template <typename X>
+static int upload(void** d, const std::vector<X>& v)
+{
+    if (v.empty()) return KSGF_OK;
+    if (hipMalloc(d, sizeof(X) * v.size()) != hipSuccess) {
+        set_error("poisson batch: hipMalloc");
+        return KSGF_ERR_NOMEM;
+    }
+    KSGF_HIP_CHECK(hipMemcpy(*d, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+    return KSGF_OK;
+}
+
+// device tables of every stage of the batch
+template <typename T>
+static int batch_tables(ksgf_poisson_batch* pb)
+{
+    int e;
+    pb->lmax = 0;
+    for (auto* mp : pb->mps) pb->lmax = std::max(pb->lmax, mp->lc);
+    pb->lv.assign(pb->lmax, {});
+    for (int l = 0; l < pb->lmax; ++l) {
+        std::vector<BLevel<T>> t;
+        std::vector<int> ph{0}, pr{0};
+        for (int p = 0; p < pb->np; ++p) {
+            ksgf_macproj* mp = pb->mps[p];
+            if (l >= mp->lc) continue;
+            BLevel<T> b{};
+            b.pk = pk_args<T>(mp, l, cor_of<T>(mp)[l], res_of<T>(mp)[l], cor_of<T>(mp)[l + 1]);
+            b.th = row_tiles(b.pk.L.prow, b.pk.L.n1, b.pk.L.n2, mp->kz_sweep[l]);
+            const LevelGeom& C = mp->G[l + 1];
+            b.tr = row_tiles(C.n0, C.n1, C.n2, kRestrictKz);
+            if (b.tr.ntx * b.tr.nty * b.tr.nkc < 2048) b.tr = row_tiles(C.n0, C.n1, C.n2, 1);
+            b.cres = res_of<T>(mp)[l + 1];
+            const int* r = &mp->ratio[3 * l];
+            b.r0 = r[0];
+            b.r1 = r[1];
+            b.r2 = r[2];
+            b.w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+            b.prob = p;
+            t.push_back(b);
+            ph.push_back(ph.back() + b.th.ntx * b.th.nty * b.th.nkc);
+            pr.push_back(pr.back() + b.tr.ntx * b.tr.nty * b.tr.nkc);
+        }
+        auto& tab = pb->lv[l];
+        tab.ne = (int)t.size();
+        tab.nb_h = ph.back();
+        tab.nb_r = pr.back();
+        if ((e = upload(&tab.lv, t)) || (e = upload((void**)&tab.pre_h, ph)) || (e = upload((void**)&tab.pre_r, pr)))
+            return e;
+    }
+    std::vector<BTop<T>> top;
+    std::vector<int> tpre{0}, nparts, part0;
+    for (int p = 0; p < pb->np; ++p) {
+        ksgf_macproj* mp = pb->mps[p];
+        BTop<T> b{};
+        b.P = mp->P;
+        b.L0 = mp->G[0];
+        b.phi[0] = mp->phi;
+        b.phi[1] = mp->phi2;
+        b.rhs = mp->rhs;
+        b.cor = cor_of<T>(mp)[0];
+        b.r = res_of<T>(mp)[0];
+        b.t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, kResidualKz);
+        b.prob = p;
+        const int nb = b.t.ntx * b.t.nty * b.t.nkc;
+        part0.push_back(tpre.back());
+        nparts.push_back(nb);
+        tpre.push_back(tpre.back() + nb);
+        top.push_back(b);
+    }
+    pb->top_nb = tpre.back();
+    if ((e = upload(&pb->top, top)) || (e = upload((void**)&pb->top_pre, tpre)) ||
+        (e = upload((void**)&pb->nparts, nparts)) || (e = upload((void**)&pb->part0, part0)))
+        return e;
+    if (hipMalloc(&pb->part, sizeof(double) * pb->top_nb) != hipSuccess) {
+        set_error("poisson batch: hipMalloc");
+        return KSGF_ERR_NOMEM;
+    }
+    std::vector<BCoarse<T>> cb;
+    for (int p = 0; p < pb->np; ++p) {
+        ksgf_macproj* mp = pb->mps[p];
+        BCoarse<T> c{};
+        c.lp = dlp_of<T>(mp);
+        c.w = mp->bw;
+        c.lc = mp->lc;
+        c.nlev = mp->nlev;
+        c.singular = mp->singular;
+        c.prob = p;
+        cb.push_back(c);
+    }
+    pb->ncoarse = (int)cb.size();
+    return upload(&pb->coarse, cb);
+}
+
+// one iteration of every active problem: V-cycle, FP64 residual, finalize
+template <typename T>
+static void batch_iteration(ksgf_poisson_batch* pb, hipStream_t st)
+{
+    const BState S = batch_state(pb);
+    const ksgf_mlmg_options& o = pb->opt;
+    auto half = [&](int l, int mode) {
+        const auto& t = pb->lv[l];
+        for (int color = 0; color < 2; ++color) {
+            const BLevel<T>* lv = (const BLevel<T>*)t.lv;
+            if (mode == kPkZero)
+                hipLaunchKernelGGL((k_pk_half_batch<T, kPkZero>), dim3(t.nb_h), dim3(256), 0, st, lv, t.pre_h, t.ne,
+                                   color, S);
+            else if (mode == kPkProlong)
+                hipLaunchKernelGGL((k_pk_half_batch<T, kPkProlong>), dim3(t.nb_h), dim3(256), 0, st, lv, t.pre_h,
+                                   t.ne, color, S);
+            else
+                hipLaunchKernelGGL((k_pk_half_batch<T, kPkPlain>), dim3(t.nb_h), dim3(256), 0, st, lv, t.pre_h, t.ne,
+                                   color, S);
+        }
+    };
+    for (int l = 0; l < pb->lmax; ++l) {
+        for (int s = 0; s < o.num_pre_smooth; ++s) half(l, s == 0 ? kPkZero : kPkPlain);
+        const auto& t = pb->lv[l];
+        hipLaunchKernelGGL(k_restrict_batch<T>, dim3(t.nb_r), dim3(256), 0, st, (const BLevel<T>*)t.lv, t.pre_r, t.ne,
+                           S);
+    }
+    hipLaunchKernelGGL(k_coarse_batch<T>, dim3(pb->ncoarse), dim3(kCoarseThreads), 0, st,
+                       (const BCoarse<T>*)pb->coarse, S, o.bottom_rtol, o.bottom_atol, o.bottom_maxiter,
+                       o.num_bottom_smooth, o.num_final_smooth, o.num_pre_smooth, o.num_post_smooth);
+    for (int l = pb->lmax - 1; l >= 0; --l)
+        for (int s = 0; s < o.num_post_smooth; ++s) half(l, s == 0 ? kPkProlong : kPkPlain);
+    hipLaunchKernelGGL(k_update_residual_batch<T>, dim3(pb->top_nb), dim3(256), 0, st, (const BTop<T>*)pb->top,
+                       pb->top_pre, pb->np, 1, S);
+    hipLaunchKernelGGL(k_batch_finalize, dim3(pb->np), dim3(256), 0, st, (const int*)pb->nparts, 0, 0.0, 0.0, S);
+}
+
+template <typename T>
+static int batch_solve(ksgf_poisson_batch* pb, const double* const* rhs, double* const* phi, double rtol, double atol,
+                       int use_graph, ksgf_mlmg_info* infos, hipStream_t st)
+{
+    const BState S = batch_state(pb);
+    // set up: pack, zero phi, subtract means, rhs norms and first residuals
+    std::vector<int> ones(pb->np, 1), zeros(pb->np, 0);
+    KSGF_HIP_CHECK(hipMemcpyAsync(pb->active, ones.data(), sizeof(int) * pb->np, hipMemcpyHostToDevice, st));
+    KSGF_HIP_CHECK(hipMemcpyAsync(pb->cur, zeros.data(), sizeof(int) * pb->np, hipMemcpyHostToDevice, st));
+    for (int p = 0; p < pb->np; ++p) {
+        ksgf_macproj* mp = pb->mps[p];
+        const LevelGeom& P = mp->P;
+        const long long n = (long long)P.n0 * P.n1 * P.n2;
+        hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, rhs[p], mp->rhs, P, 1);
+        launch_fill<double>(mp->phi, P.tot, 0.0, st);
+        if (mp->singular) launch_sub_mean(mp->rhs, P, mp->part, st);
+        KSGF_HIP_CHECK(hipMemsetAsync(mp->bw.stats, 0, 2 * sizeof(int), st));
+    }
+    hipLaunchKernelGGL(k_update_residual_batch<T>, dim3(pb->top_nb), dim3(256), 0, st, (const BTop<T>*)pb->top,
+                       pb->top_pre, pb->np, 0, S);
+    hipLaunchKernelGGL(k_batch_finalize, dim3(pb->np), dim3(256), 0, st, (const int*)pb->nparts, 1, rtol, atol, S);
+    KSGF_LAUNCH_CHECK();
+    auto read_state = [&]() -> int {
+        KSGF_HIP_CHECK(hipMemcpyAsync(pb->h_state, pb->active, sizeof(int) * pb->np, hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(
+            hipMemcpyAsync(pb->h_state + pb->np, pb->cur, sizeof(int) * pb->np, hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(
+            hipMemcpyAsync(pb->h_state + 2 * pb->np, pb->iters, sizeof(int) * pb->np, hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(hipMemcpyAsync(pb->h_norm, pb->norm, sizeof(double) * pb->np, hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(
+            hipMemcpyAsync(pb->h_norm + pb->np, pb->target, sizeof(double) * pb->np, hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(hipMemcpyAsync(pb->h_norm + 2 * pb->np, pb->bound, sizeof(double) * pb->np,
+                                      hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(hipStreamSynchronize(st));
+        return KSGF_OK;
+    };
+    int e = read_state();
+    if (e) return e;
+    std::vector<double> r0(pb->h_norm, pb->h_norm + pb->np);
+    if (use_graph && !pb->exec) {
+        // capture one iteration on the batch's own stream
+        KSGF_HIP_CHECK(hipStreamBeginCapture(pb->cap, hipStreamCaptureModeThreadLocal));
+        batch_iteration<T>(pb, pb->cap);
+        hipGraph_t g;
+        KSGF_HIP_CHECK(hipStreamEndCapture(pb->cap, &g));
+        const hipError_t ie = hipGraphInstantiate(&pb->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            pb->exec = nullptr;
+            set_error(std::string("poisson batch: hipGraphInstantiate: ") + hipGetErrorString(ie));
+            return KSGF_ERR_HIP;
+        }
+    }
+    for (int it = 0; it < pb->opt.maxiter; ++it) {
+        bool any = false;
+        for (int p = 0; p < pb->np; ++p) any = any || pb->h_state[p];
+        if (!any) break;
+        if (use_graph)
+            KSGF_HIP_CHECK(hipGraphLaunch(pb->exec, st));
+        else
+            batch_iteration<T>(pb, st);
+        KSGF_LAUNCH_CHECK();
+        if ((e = read_state())) return e;
+    }
+    int ret = KSGF_OK;
+    std::string diverged;
+    for (int p = 0; p < pb->np; ++p) {
+        ksgf_macproj* mp = pb->mps[p];
+        const LevelGeom& P = mp->P;
+        const long long n = (long long)P.n0 * P.n1 * P.n2;
+        const double* cur = pb->h_state[pb->np + p] ? mp->phi2 : mp->phi;
+        hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, cur, phi[p], P, 0);
+        KSGF_HIP_CHECK(hipMemcpyAsync(mp->hstats, mp->bw.stats, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(hipStreamSynchronize(st));
+        ksgf_mlmg_info inf{};
+        inf.iters = pb->h_state[2 * pb->np + p];
+        inf.r0 = r0[p];
+        inf.rfinal = pb->h_norm[p];
+        inf.converged = pb->h_norm[p] <= pb->h_norm[pb->np + p];
+        inf.nlevels = mp->nlev;
+        inf.outer_iters = pb->mixed ? inf.iters : 0;
+        inf.bottom_iters = mp->hstats[0];
+        inf.bottom_fail = mp->hstats[1];
+        if (infos) infos[p] = inf;
+        if (!inf.converged) {
+            if (!(inf.rfinal <= pb->h_norm[2 * pb->np + p]))
+                diverged += (diverged.empty() ? "" : ", ") + std::to_string(p);
+            else if (ret == KSGF_OK)
+                ret = KSGF_ERR_NOT_CONVERGED;
+        }
+    }
+    if (!diverged.empty()) {
+        set_error("poisson batch: the residual of problem(s) " + diverged + " exceeds 1e20 |rhs| (diverging)");
+        return KSGF_ERR_DIVERGED;
+    }
+    return ret;
+}
+
+}  // namespace ksgf
+
+extern "C" {
+
+int ksgf_poisson_batch_destroy(ksgf_poisson_batch* pb)
+{
+    if (!pb) return KSGF_OK;
+    if (pb->exec) (void)hipGraphExecDestroy(pb->exec);
+    if (pb->cap) (void)hipStreamDestroy(pb->cap);
+    for (auto& t : pb->lv) {
+        if (t.lv) (void)hipFree(t.lv);
+        if (t.pre_h) (void)hipFree(t.pre_h);
+        if (t.pre_r) (void)hipFree(t.pre_r);
+    }
+    void* ps[] = {pb->top, pb->top_pre, pb->coarse, pb->nparts, pb->part0, pb->part,
+                  pb->active, pb->cur, pb->iters, pb->norm, pb->target, pb->bound};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (pb->h_state) (void)hipHostFree(pb->h_state);
+    if (pb->h_norm) (void)hipHostFree(pb->h_norm);
+    for (auto* mp : pb->mps) ksgf_macproj_destroy(mp);
+    for (auto* ba : pb->bas) ksgf_boxarray_destroy(ba);
+    delete pb;
+    return KSGF_OK;
+}
+
+int ksgf_poisson_batch_create(int nprob, const int* n, const double* dx, const int* lobc, const int* hibc,
+                              const double* beta, const ksgf_mlmg_options* opts, ksgf_poisson_batch** out)
+{
+    if (nprob <= 0 || !n || !dx || !lobc || !hibc || !beta || !out) {
+        set_error("ksgf_poisson_batch_create: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    auto* pb = new ksgf_poisson_batch();
+    pb->np = nprob;
+    if (opts)
+        pb->opt = *opts;
+    else
+        ksgf_mlmg_default_options(&pb->opt);
+    pb->mixed = (pb->opt.precision == KSGF_MG_MIXED);
+    if (pb->opt.num_pre_smooth < 1 || pb->opt.num_post_smooth < 1) {
+        delete pb;
+        set_error("ksgf_poisson_batch_create: the batch needs at least one pre- and one post-smoothing sweep");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    int e;
+    for (int p = 0; p < nprob; ++p) {
+        ksgf_geometry g{};
+        int box[6];
+        for (int d = 0; d < 3; ++d) {
+            g.domain_lo[d] = 0;
+            g.domain_hi[d] = n[3 * p + d] - 1;
+            g.prob_lo[d] = 0.0;
+            g.prob_hi[d] = dx[3 * p + d] * n[3 * p + d];
+            g.is_periodic[d] = (lobc[3 * p + d] == KSGF_LO_PERIODIC);
+            box[d] = 0;
+            box[3 + d] = n[3 * p + d] - 1;
+        }
+        ksgf_boxarray* ba = nullptr;
+        if ((e = ksgf_boxarray_create(&g, box, 1, &ba))) {
+            ksgf_poisson_batch_destroy(pb);
+            return e;
+        }
+        pb->bas.push_back(ba);
+        ksgf_macproj* mp = nullptr;
+        if ((e = ksgf_macproj_create(ba, lobc + 3 * p, hibc + 3 * p, beta[p], &pb->opt, &mp))) {
+            ksgf_poisson_batch_destroy(pb);
+            return e;
+        }
+        pb->mps.push_back(mp);
+        for (int l = 0; l < mp->lc; ++l)
+            if (!mp->G[l].packed || mp->P.ghost_mode) {
+                ksgf_poisson_batch_destroy(pb);
+                set_error("ksgf_poisson_batch_create: problem " + std::to_string(p) +
+                          " has a grid level with an odd x extent or odd periodic extent; solve it on its own");
+                return KSGF_ERR_UNSUPPORTED;
+            }
+    }
+    if ((e = pb->mixed ? batch_tables<float>(pb) : batch_tables<double>(pb))) {
+        ksgf_poisson_batch_destroy(pb);
+        return e;
+    }
+    const size_t np = (size_t)nprob;
+    if (hipMalloc(&pb->active, sizeof(int) * np) != hipSuccess || hipMalloc(&pb->cur, sizeof(int) * np) != hipSuccess ||
+        hipMalloc(&pb->iters, sizeof(int) * np) != hipSuccess || hipMalloc(&pb->norm, sizeof(double) * np) != hipSuccess ||
+        hipMalloc(&pb->target, sizeof(double) * np) != hipSuccess ||
+        hipMalloc(&pb->bound, sizeof(double) * np) != hipSuccess ||
+        hipHostMalloc(&pb->h_state, sizeof(int) * 3 * np, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&pb->h_norm, sizeof(double) * 3 * np, hipHostMallocDefault) != hipSuccess ||
+        hipStreamCreateWithFlags(&pb->cap, hipStreamNonBlocking) != hipSuccess) {
+        ksgf_poisson_batch_destroy(pb);
+        set_error("ksgf_poisson_batch_create: allocation");
+        return KSGF_ERR_NOMEM;
+    }
+    *out = pb;
+    return KSGF_OK;
+}
+
+int ksgf_poisson_batch_solve(ksgf_poisson_batch* pb, const double* const* rhs, double* const* phi, double rtol,
+                             double atol, int use_graph, ksgf_mlmg_info* infos, ksgf_stream_t stream)
+{
+    if (!pb || !rhs || !phi) {
+        set_error("ksgf_poisson_batch_solve: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    for (int p = 0; p < pb->np; ++p)
+        if (!rhs[p] || !phi[p]) {
+            set_error("ksgf_poisson_batch_solve: null rhs/phi pointer");
+            return KSGF_ERR_INVALID;
+        }
+    hipStream_t st = (hipStream_t)stream;
+    return pb->mixed ? batch_solve<float>(pb, rhs, phi, rtol, atol, use_graph, infos, st)
+                     : batch_solve<double>(pb, rhs, phi, rtol, atol, use_graph, infos, st);
 }
 
 }  // extern "C"

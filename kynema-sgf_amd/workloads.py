@@ -141,3 +141,24 @@ def random_case(n_cell, boxes=None, seed=7, periodic=(1, 1, 1), extent=(1.0, 1.0
         bcval = np.zeros((6, 3))
         lobc, hibc = tuple(lobc), tuple(hibc)
     return Case(f"rand{nx}x{ny}x{nz}", geom, ba, np.ascontiguousarray(vel), rho, gp, bc, bcval, lobc, hibc, cfl, 1)
+
+
+def poisson_problems(nprob=64, edges=(32, 48, 64, 80, 96, 112, 128), seed=5, bcs=None):
+    """Independent single-box Poisson problems: cube edges drawn from
+    `edges`, unit extent, beta 1, right-hand sides N(0,1) minus their mean
+    (the mean is removed for every singular problem: all periodic/Neumann).
+    bcs: list of (lobc, hibc) cycled over the problems (default periodic).
+    Returns (problems, rhs list of flat float64 arrays, Fortran order)."""
+    g = _philox(seed)
+    bcs = bcs or [((LO_PERIODIC,) * 3, (LO_PERIODIC,) * 3)]
+    probs, rhs = [], []
+    for q in range(nprob):
+        e = int(edges[int(g.integers(0, len(edges)))])
+        lo, hi = bcs[q % len(bcs)]
+        n = (e, e, e)
+        f = g.standard_normal(e * e * e)
+        if 2 not in tuple(lo) + tuple(hi):  # no Dirichlet side: singular
+            f -= f.mean()
+        probs.append(dict(n=n, dx=(1.0 / e,) * 3, lobc=tuple(lo), hibc=tuple(hi), beta=1.0))
+        rhs.append(f)
+    return probs, rhs

@@ -1107,6 +1107,13 @@ static void smooth(const lev_t* L, double* x, const double* b, int nu, const int
 }
 
 /* r = b - A x (x ghosts filled here); returns |r|_inf */
+/* |v| for a residual norm; NaN counts as +inf (fmax alone would drop it) */
+static double abs_or_inf(double v)
+{
+    const double a = fabs(v);
+    return a == a ? a : INFINITY;
+}
+
 static double residual(const lev_t* L, double* x, const double* b, double* r, const int* lobc,
                        const int* hibc)
 {
@@ -1119,7 +1126,7 @@ static double residual(const lev_t* L, double* x, const double* b, double* r, co
                 long c = LI(L, i, j, k);
                 double v = b[c] - (lap_off(L, x, c) - L->dg * x[c]);
                 r[c] = v;
-                mx = fmax(mx, fabs(v));
+                mx = fmax(mx, abs_or_inf(v));
             }
     return mx;
 }
@@ -1151,7 +1158,7 @@ static double norminf(const lev_t* L, const double* a)
     double m = 0.0;
     for (int k = 0; k < L->n[2]; ++k)
         for (int j = 0; j < L->n[1]; ++j)
-            for (int i = 0; i < L->n[0]; ++i) m = fmax(m, fabs(a[LI(L, i, j, k)]));
+            for (int i = 0; i < L->n[0]; ++i) m = fmax(m, abs_or_inf(a[LI(L, i, j, k)]));
     return m;
 }
 static void sub_mean(const lev_t* L, double* a)
@@ -1345,6 +1352,7 @@ static int mg_solve(mg_t* M, double* phi, double* rhs, double rtol, double atol,
     info->iters = 0;
     info->converged = (r <= target);
     info->nlevels = M->nlev;
+    int diverged = 0;
     M->bottom_iters = 0;
     M->bottom_fail = 0;
     if (!info->converged) {
@@ -1354,13 +1362,17 @@ static int mg_solve(mg_t* M, double* phi, double* rhs, double rtol, double atol,
             r = residual(L0, phi, rhs, L0->res, M->lobc, M->hibc);
             info->iters = it;
             if (r <= target) { info->converged = 1; break; }
+            /* residual 1e20 times the right-hand side, or not finite: the
+             * iteration diverges (AMReX's MLMG::solve aborts "MLMG failing
+             * so lets stop here" at this bound) */
+            if (!(r <= KO_MG_DIVERGE * rhsnorm)) { diverged = 1; break; }
         }
     }
     lev_ghosts(L0, phi, M->lobc, M->hibc);
     info->rfinal = r;
     info->bottom_iters = M->bottom_iters;
     info->bottom_fail = M->bottom_fail;
-    return info->converged ? 0 : 1;
+    return diverged ? KO_MG_DIVERGED : info->converged ? 0 : 1;
 }
 
 int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int* hibc,

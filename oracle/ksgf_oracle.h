@@ -108,6 +108,11 @@ int ko_mac_project(const ko_geom* g, double* umac, double* vmac, double* wmac,
                    const int* lobc, const int* hibc, double beta, double rtol, double atol,
                    const ko_mlmg_opts* opts, ko_mlmg_info* info, double* phi_out);
 
+/* mg solves return 0 (converged), 1 (maxiter reached) or KO_MG_DIVERGED
+ * (residual above KO_MG_DIVERGE times |rhs|_inf, or not finite) */
+#define KO_MG_DIVERGE 1.0e20
+#define KO_MG_DIVERGED (-5)
+
 int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int* hibc,
                      double beta, const double* rhs, double* phi, double rtol, double atol,
                      const ko_mlmg_opts* opts, ko_mlmg_info* info);

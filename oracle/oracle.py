@@ -203,9 +203,9 @@ def mac_project(G, mac, lobc, hibc, beta, rtol, atol, opts=None, want_phi=False)
     lo = np.ascontiguousarray(lobc, dtype=np.int32)
     hi = np.ascontiguousarray(hibc, dtype=np.int32)
     phi = np.zeros(int(np.prod(G.ba.geom.n_cell))) if want_phi else None
-    lib().ko_mac_project(G.ref, _p(mac[0]), _p(mac[1]), _p(mac[2]), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip),
-                         beta, rtol, atol, C.byref(opts), C.byref(info), _p(phi))
-    return info.as_dict(), phi
+    st = lib().ko_mac_project(G.ref, _p(mac[0]), _p(mac[1]), _p(mac[2]), lo.ctypes.data_as(_ip),
+                              hi.ctypes.data_as(_ip), beta, rtol, atol, C.byref(opts), C.byref(info), _p(phi))
+    return dict(info.as_dict(), status=st), phi
 
 
 def poisson_solve(n, dx, lobc, hibc, beta, rhs, rtol, atol, opts=None):
@@ -217,9 +217,9 @@ def poisson_solve(n, dx, lobc, hibc, beta, rhs, rtol, atol, opts=None):
     hi = np.ascontiguousarray(hibc, dtype=np.int32)
     rhs = np.ascontiguousarray(rhs, dtype=np.float64).ravel()
     phi = np.zeros_like(rhs)
-    lib().ko_poisson_solve(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip), beta,
-                           _p(rhs), _p(phi), rtol, atol, C.byref(opts), C.byref(info))
-    return phi, info.as_dict()
+    st = lib().ko_poisson_solve(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip),
+                                beta, _p(rhs), _p(phi), rtol, atol, C.byref(opts), C.byref(info))
+    return phi, dict(info.as_dict(), status=st)
 
 
 def cfl_max(G, vel, ng):

@@ -246,6 +246,25 @@ def test_mlmg_fourier_mode(oracle):
     assert rel_l2(phi - phi.mean(), (f / lam).ravel()) < 1e-10
 
 
+def test_mlmg_divergence_guard(oracle):
+    """A strongly anisotropic brick (dz = dx/5) under full coarsening: the
+    red-black point smoother does not damp the coarse levels' x/y error and
+    the V-cycle diverges.  The solve must stop once the residual passes
+    1e20 |rhs| (status -5, not converged) instead of running into NaN and
+    reporting a NaN-free maximum; with semicoarsening the same problem
+    converges."""
+    n, dx = (32, 32, 16), (20.0, 20.0, 4.0)
+    f = np.random.default_rng(3).standard_normal(int(np.prod(n)))
+    f -= f.mean()
+    bc = (0, 0, 1)
+    phi, info = oracle.poisson_solve(n, dx, bc, bc, 1.0, f, 1e-11, 1e-14)
+    assert info["status"] == -5 and not info["converged"], info
+    assert not info["rfinal"] <= 1e20 * info["r0"]
+    assert info["iters"] < 200
+    phi, info = oracle.poisson_solve(n, dx, bc, bc, 1.0, f, 1e-11, 1e-14, oracle.default_opts(semicoarsening=1))
+    assert info["status"] == 0 and info["converged"] and np.isfinite(phi).all()
+
+
 def test_macproj_divergence_idempotence_decomposition(oracle, workloads):
     c1 = workloads.random_case((16, 16, 16), seed=5, periodic=(1, 1, 0))
     c8 = workloads.random_case((16, 16, 16), boxes=lay.chop_uniform((16, 16, 16), 8), seed=5, periodic=(1, 1, 0))
