@@ -165,6 +165,65 @@ def chop_random(n_cell, choices, rng, granularity=8):
     return _tensor_boxes(segs)
 
 
+def chop_bricks(n_cell, choices, rng, nbox=None):
+    """Ragged brick-wall layout: z slabs with thicknesses drawn from
+    ``choices``; every slab has its own y rows and every row its own x
+    pieces, all drawn from ``choices`` (so boxes of neighbouring rows and
+    slabs do not line up).  Extents must be multiples of min(choices), and
+    every choice a multiple of it.  With ``nbox``, x pieces are then split in
+    two or merged pairwise (lengths staying in ``choices``) until there are
+    exactly ``nbox`` boxes."""
+    ch = sorted(int(c) for c in choices)
+    g = ch[0]
+    if any(c % g for c in ch) or any(int(n) % g for n in n_cell):
+        raise ValueError("extents and choices must be multiples of the smallest choice")
+
+    def compose(L):
+        parts, rem = [], L
+        while rem > 0:
+            ok = [c for c in ch if c <= rem]
+            c = ok[int(rng.integers(len(ok)))]
+            parts.append(c)
+            rem -= c
+        return parts
+
+    rows = []  # [z0, dz, y0, dy, [x lengths]]
+    z0 = 0
+    for dz in compose(int(n_cell[2])):
+        y0 = 0
+        for dy in compose(int(n_cell[1])):
+            rows.append([z0, dz, y0, dy, compose(int(n_cell[0]))])
+            y0 += dy
+        z0 += dz
+    count = sum(len(r[4]) for r in rows)
+    if nbox is not None and nbox > len(rows) * (int(n_cell[0]) // g):
+        raise ValueError(f"{nbox} boxes need more than {len(rows)} rows of {n_cell[0]} cells")
+    chs = set(ch)
+    guard = 0
+    while nbox is not None and count != nbox:
+        guard += 1
+        if guard > 100000:
+            raise ValueError(f"cannot reach {nbox} boxes")
+        r = rows[int(rng.integers(len(rows)))][4]
+        q = int(rng.integers(len(r)))
+        if count < nbox:
+            cuts = [a for a in ch if a < r[q] and r[q] - a in chs]
+            if cuts:
+                a = cuts[int(rng.integers(len(cuts)))]
+                r[q:q + 1] = [a, r[q] - a]
+                count += 1
+        elif q + 1 < len(r) and r[q] + r[q + 1] in chs:
+            r[q:q + 2] = [r[q] + r[q + 1]]
+            count -= 1
+    boxes = []
+    for z0, dz, y0, dy, xs in rows:
+        x0 = 0
+        for dx in xs:
+            boxes.append([x0, y0, z0, x0 + dx - 1, y0 + dy - 1, z0 + dz - 1])
+            x0 += dx
+    return boxes
+
+
 def _tensor_boxes(segs):
     starts = []
     for lens in segs:

@@ -16,7 +16,7 @@ import dataclasses
 import numpy as np
 
 from .layout import (BC_EXT_DIR, BC_HOEXTRAP, BC_INT_DIR, LO_NEUMANN, LO_PERIODIC, BoxArray, Geometry,
-                     chop_random, chop_uniform)
+                     chop_bricks, chop_random, chop_uniform)
 
 
 @dataclasses.dataclass
@@ -82,6 +82,30 @@ def taylor_green(n=64, max_grid_size=32, noise=1e-3, seed=1, cfl=0.5, nsteps=1):
     shape = (1, n, n, n)
     return Case(f"tgv{n}", geom, ba, np.ascontiguousarray(vel), np.ones(shape), np.zeros((3, n, n, n)), bc, bcval,
                 lobc, hibc, cfl, nsteps)
+
+
+def ragged(n_cell=(320, 320, 480), nbox=1500, edges=(8, 16, 24, 32, 48, 64), box_seed=4, noise=1e-3, seed=1,
+           cfl=0.5, nsteps=1):
+    """Ragged single-level layout: ``nbox`` boxes with edges drawn from
+    ``edges`` per axis (brick wall: rows and slabs do not line up), over a
+    triply periodic domain of unit cell size 1/n_cell[0]; Taylor-Green plus
+    U(-noise, noise) velocity as in taylor_green()."""
+    h = 1.0 / n_cell[0]
+    geom = Geometry(tuple(n_cell), (0.0, 0.0, 0.0), tuple(h * n for n in n_cell), (1, 1, 1))
+    ba = BoxArray(geom, chop_bricks(n_cell, edges, _philox(box_seed), nbox))
+    x = geom.cell_centers(0)[None, None, :]
+    y = geom.cell_centers(1)[None, :, None]
+    z = geom.cell_centers(2)[:, None, None]
+    tp = 2.0 * np.pi
+    u = np.sin(tp * x) * np.cos(tp * y) * np.cos(tp * z)
+    v = -np.cos(tp * x) * np.sin(tp * y) * np.cos(tp * z)
+    vel = np.stack(np.broadcast_arrays(u, v, np.zeros_like(u))).astype(np.float64)
+    if noise:
+        vel += _philox(seed).uniform(-noise, noise, size=vel.shape)
+    bc, bcval, lobc, hibc = _periodic_bcs()
+    nx, ny, nz = n_cell
+    return Case(f"ragged{ba.nbox}_{nx}x{ny}x{nz}", geom, ba, np.ascontiguousarray(vel), np.ones((1, nz, ny, nx)),
+                np.zeros((3, nz, ny, nx)), bc, bcval, lobc, hibc, cfl, nsteps)
 
 
 def abl(n_cell=(256, 256, 128), extent=(5120.0, 5120.0, 1024.0), sizes=(32, 48, 64), cfl=0.9, nsteps=10,

@@ -150,10 +150,11 @@ def test_batch_sixty_four_problems(ksgf, workloads, gpu):
 
 
 def test_batch_rejects_odd_extent(ksgf, gpu):
-    """An odd x extent on a grid level has no packed red-black layout."""
+    """An odd x extent on a grid level has no packed red-black layout (with
+    semicoarsening, y and z coarsen while x stays 65 wide)."""
     prob = dict(n=(65, 64, 64), dx=(1.0 / 64,) * 3, lobc=(DIR,) * 3, hibc=(DIR,) * 3)
-    with pytest.raises(ksgf.KsgfError):
-        ksgf.PoissonBatch([prob])
+    with pytest.raises(ksgf.KsgfError, match="odd x extent"):
+        ksgf.PoissonBatch([prob], ksgf.default_options(do_semicoarsening=1))
 
 
 def test_diverging_solve_stops(ksgf, oracle, gpu):
