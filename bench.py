@@ -6,7 +6,7 @@ correction); the headline value is the mixed path, the FP64 path is
 reported beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
-                  [--precision both|mixed|fp64] [--no-cpu]
+                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch]
 
 A step is one pass of the hot path over the whole grid, driven by the C++
 host layer (GodunovPredictor): convective CFL, source terms,
@@ -40,6 +40,10 @@ def parse():
     p.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
+    p.add_argument("--workload", choices=["abl", "poisson-batch"], default="abl",
+                   help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
+                        "N(0,1) right-hand sides, seed 5) solved as one batch (hipGraph replay and eager) and one "
+                        "after another; prints its own JSON line")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB -> 1 GiB elementwise copy (x*1.0) 4 times: the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -119,6 +123,63 @@ def cpu_baseline(n_cell):
                       f"physics, {d['mlmg']['iters']} V-cycles in the last), {dt:.2f} s"}
 
 
+def poisson_batch(steps, warmup):
+    """Time the batched solve of the 64-problem set against the same
+    problems solved one at a time through ksgf_macproj_solve."""
+    import torch
+
+    ksgf = importlib.import_module("kynema-sgf_amd")
+    lay = importlib.import_module("kynema-sgf_amd.layout")
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    probs, rhs = wl.poisson_problems(64, seed=5)
+    cells = sum(int(np.prod(p["n"])) for p in probs)
+    rt = [torch.from_numpy(r).to("cuda") for r in rhs]
+    phis = [torch.zeros_like(r) for r in rt]
+    pb = ksgf.PoissonBatch(probs)
+    singles = []
+    for p in probs:
+        n = p["n"]
+        geom = lay.Geometry(tuple(n), (0.0, 0.0, 0.0), tuple(p["dx"][d] * n[d] for d in range(3)), (1, 1, 1))
+        ba = lay.BoxArray(geom, [0, 0, 0, n[0] - 1, n[1] - 1, n[2] - 1])
+        singles.append(ksgf.MacProjector(ksgf.DeviceBoxArray(ba), p["lobc"], p["hibc"], p["beta"]))
+    infos = {}
+
+    def timed(name, fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            infos[name] = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    t_graph = timed("graph", lambda: pb.solve(rt, phis, 1e-11, 1e-14, use_graph=True))
+    t_eager = timed("eager", lambda: pb.solve(rt, phis, 1e-11, 1e-14, use_graph=False))
+    t_seq = timed("single", lambda: [mp.solve(rt[q], phis[q], 1e-11, 1e-14) for q, mp in enumerate(singles)])
+    its = [i["iters"] for i in infos["graph"]]
+    assert its == [i["iters"] for i in infos["single"]]
+    return {
+        "metric": "Poisson cells solved/s: 64 independent periodic problems 32^3..128^3, MLMG FP64 to rtol 1e-11",
+        "value": cells / t_graph,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": t_graph * 1e3,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"64 periodic Poisson problems, edges {sorted(set(p['n'][0] for p in probs))}, "
+                               f"{cells} cells, rhs N(0,1) minus mean (seed 5)",
+                   "mlmg_iters_max": max(its), "mlmg_iters_mean": sum(its) / len(its)},
+        "ms_batch_graph": t_graph * 1e3,
+        "ms_batch_eager": t_eager * 1e3,
+        "ms_one_at_a_time": t_seq * 1e3,
+        "speedup_graph_vs_one_at_a_time": t_seq / t_graph,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -127,6 +188,9 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe()
+        return
+    if a.workload == "poisson-batch":
+        print(json.dumps(poisson_batch(a.steps, a.warmup)))
         return
     wl = importlib.import_module("kynema-sgf_amd.workloads")
     n = tuple(a.n)

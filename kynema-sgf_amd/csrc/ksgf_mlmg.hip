@@ -480,12 +480,15 @@ __global__ void __launch_bounds__(256) k_partial_sum(const double* __restrict__ 
 __global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ a, LevelGeom L, const double* __restrict__ part,
                                                   int npart)
 {
+    // every block sums the partials in the same fixed order (256 threads)
+    __shared__ double sh[4];
     __shared__ double mean;
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int p = 0; p < npart; ++p) s += part[p];
-        mean = s / ((double)L.n0 * L.n1 * L.n2);
-    }
+    double s = 0.0;
+    for (int p = (int)threadIdx.x; p < npart; p += (int)blockDim.x) s += part[p];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) mean = ((sh[0] + sh[1]) + (sh[2] + sh[3])) / ((double)L.n0 * L.n1 * L.n2);
     __syncthreads();
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
@@ -1631,30 +1634,20 @@ struct BState {
     int maxiter;
 };
 
-// entry e of a batched launch with pre[e] <= b < pre[e+1]
-__device__ __forceinline__ int batch_entry(const int* __restrict__ pre, int ne, int b)
-{
-    int lo = 0, hi = ne - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
+// logical block b of a batched launch: table entry and tile within it
+struct BlkMap {
+    int e, t;
+};
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_pk_half_batch(const BLevel<T>* __restrict__ lv, const int* __restrict__ pre,
-                                                       int ne, int color, BState S)
+__global__ void __launch_bounds__(256) k_pk_half_batch(const BLevel<T>* __restrict__ lv, const BlkMap* __restrict__ bm,
+                                                       int color, BState S)
 {
-    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int e = batch_entry(pre, ne, lid);
-    const BLevel<T>& L = lv[e];
+    const BlkMap m = bm[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BLevel<T> L = lv[m.e];
     if (!S.active[L.prob]) return;
     int h, j, k0, k1;
-    row_tile(L.th, lid - pre[e], (int)threadIdx.x, h, j, k0, k1, L.pk.L.n2);
+    row_tile(L.th, m.t, (int)threadIdx.x, h, j, k0, k1, L.pk.L.n2);
     if (h >= L.pk.L.prow || j >= L.pk.L.n1) return;
     if (MODE == kPkProlong && L.pk.C.packed) {
         for (int k = k0; k < k1; ++k) pk_relax<T, MODE, true>(L.pk, color, h, j, k);
@@ -1664,16 +1657,15 @@ __global__ void __launch_bounds__(256) k_pk_half_batch(const BLevel<T>* __restri
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict_batch(const BLevel<T>* __restrict__ lv, const int* __restrict__ pre,
-                                                        int ne, BState S)
+__global__ void __launch_bounds__(256) k_restrict_batch(const BLevel<T>* __restrict__ lv, const BlkMap* __restrict__ bm,
+                                                        BState S)
 {
-    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int e = batch_entry(pre, ne, lid);
-    const BLevel<T>& L = lv[e];
+    const BlkMap m = bm[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BLevel<T> L = lv[m.e];
     if (!S.active[L.prob]) return;
     const LevelGeom& C = L.pk.C;
     int I, J, K0, K1;
-    row_tile(L.tr, lid - pre[e], (int)threadIdx.x, I, J, K0, K1, C.n2);
+    row_tile(L.tr, m.t, (int)threadIdx.x, I, J, K0, K1, C.n2);
     if (I >= C.n0 || J >= C.n1) return;
     for (int K = K0; K < K1; ++K)
         L.cres[AnyIdx(C, I, J, K)] =
@@ -1694,22 +1686,22 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_batch(const BCoarse<T
 // FP64 outer residual of every problem; with_cor: phi(next) = phi + cor
 template <typename T>
 __global__ void __launch_bounds__(256) k_update_residual_batch(const BTop<T>* __restrict__ tp,
-                                                               const int* __restrict__ pre, int ne, int with_cor,
-                                                               BState S)
+                                                               const BlkMap* __restrict__ bm, int with_cor, BState S)
 {
-    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int e = batch_entry(pre, ne, lid);
-    const BTop<T>& t = tp[e];
+    const BlkMap mb = bm[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BTop<T>& t = tp[mb.e];
     if (!S.active[t.prob]) return;
-    const int c = S.cur[t.prob];
-    const double m = update_residual_tile<T, T, true>(t.phi[c], with_cor ? t.cor : (const T*)nullptr, t.phi[1 - c],
-                                                      t.rhs, t.r, t.P, t.L0, t.t, lid - pre[e]);
+    const bool c = S.cur[t.prob] != 0;
+    double* const p0 = t.phi[0];
+    double* const p1 = t.phi[1];
+    const double m = update_residual_tile<T, T, true>(c ? p1 : p0, with_cor ? t.cor : (const T*)nullptr, c ? p0 : p1,
+                                                      t.rhs, t.r, t.P, t.L0, t.t, mb.t);
     __shared__ double sh[4];
     double v = m;
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) S.part[S.part0[t.prob] + (lid - pre[e])] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    if (threadIdx.x == 0) S.part[S.part0[t.prob] + mb.t] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
 // one workgroup per problem: the new norm, the phi buffer flip, the
@@ -1752,13 +1744,13 @@ struct ksgf_poisson_batch {
     int lmax = 0;  // grid levels of the deepest problem
     struct Tab {
         void* lv = nullptr;  // BLevel<T>[ne]
-        int* pre_h = nullptr;  // half-sweep block prefix [ne+1]
-        int* pre_r = nullptr;  // restriction block prefix
+        void* bm_h = nullptr;  // BlkMap per half-sweep block
+        void* bm_r = nullptr;  // BlkMap per restriction block
         int ne = 0, nb_h = 0, nb_r = 0;
     };
     std::vector<Tab> lv;
     void* top = nullptr;
-    int* top_pre = nullptr;
+    void* top_bm = nullptr;
     int top_nb = 0;
     void* coarse = nullptr;
     int ncoarse = 0;
@@ -1850,7 +1842,7 @@ static int batch_tables(ksgf_poisson_batch* pb)
     pb->lv.assign(pb->lmax, {});
     for (int l = 0; l < pb->lmax; ++l) {
         std::vector<BLevel<T>> t;
-        std::vector<int> ph{0}, pr{0};
+        std::vector<BlkMap> mh, mr;
         for (int p = 0; p < pb->np; ++p) {
             ksgf_macproj* mp = pb->mps[p];
             if (l >= mp->lc) continue;
@@ -1868,18 +1860,19 @@ static int batch_tables(ksgf_poisson_batch* pb)
             b.w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
             b.prob = p;
             t.push_back(b);
-            ph.push_back(ph.back() + b.th.ntx * b.th.nty * b.th.nkc);
-            pr.push_back(pr.back() + b.tr.ntx * b.tr.nty * b.tr.nkc);
+            for (int q = 0; q < b.th.ntx * b.th.nty * b.th.nkc; ++q) mh.push_back({(int)t.size() - 1, q});
+            for (int q = 0; q < b.tr.ntx * b.tr.nty * b.tr.nkc; ++q) mr.push_back({(int)t.size() - 1, q});
         }
         auto& tab = pb->lv[l];
         tab.ne = (int)t.size();
-        tab.nb_h = ph.back();
-        tab.nb_r = pr.back();
-        if ((e = upload(&tab.lv, t)) || (e = upload((void**)&tab.pre_h, ph)) || (e = upload((void**)&tab.pre_r, pr)))
+        tab.nb_h = (int)mh.size();
+        tab.nb_r = (int)mr.size();
+        if ((e = upload(&tab.lv, t)) || (e = upload(&tab.bm_h, mh)) || (e = upload(&tab.bm_r, mr)))
             return e;
     }
     std::vector<BTop<T>> top;
-    std::vector<int> tpre{0}, nparts, part0;
+    std::vector<BlkMap> tbm;
+    std::vector<int> nparts, part0;
     for (int p = 0; p < pb->np; ++p) {
         ksgf_macproj* mp = pb->mps[p];
         BTop<T> b{};
@@ -1893,13 +1886,13 @@ static int batch_tables(ksgf_poisson_batch* pb)
         b.t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, kResidualKz);
         b.prob = p;
         const int nb = b.t.ntx * b.t.nty * b.t.nkc;
-        part0.push_back(tpre.back());
+        part0.push_back((int)tbm.size());
         nparts.push_back(nb);
-        tpre.push_back(tpre.back() + nb);
+        for (int q = 0; q < nb; ++q) tbm.push_back({p, q});
         top.push_back(b);
     }
-    pb->top_nb = tpre.back();
-    if ((e = upload(&pb->top, top)) || (e = upload((void**)&pb->top_pre, tpre)) ||
+    pb->top_nb = (int)tbm.size();
+    if ((e = upload(&pb->top, top)) || (e = upload(&pb->top_bm, tbm)) ||
         (e = upload((void**)&pb->nparts, nparts)) || (e = upload((void**)&pb->part0, part0)))
         return e;
     if (hipMalloc(&pb->part, sizeof(double) * pb->top_nb) != hipSuccess) {
@@ -1932,22 +1925,20 @@ static void batch_iteration(ksgf_poisson_batch* pb, hipStream_t st)
         const auto& t = pb->lv[l];
         for (int color = 0; color < 2; ++color) {
             const BLevel<T>* lv = (const BLevel<T>*)t.lv;
+            const BlkMap* bm = (const BlkMap*)t.bm_h;
             if (mode == kPkZero)
-                hipLaunchKernelGGL((k_pk_half_batch<T, kPkZero>), dim3(t.nb_h), dim3(256), 0, st, lv, t.pre_h, t.ne,
-                                   color, S);
+                hipLaunchKernelGGL((k_pk_half_batch<T, kPkZero>), dim3(t.nb_h), dim3(256), 0, st, lv, bm, color, S);
             else if (mode == kPkProlong)
-                hipLaunchKernelGGL((k_pk_half_batch<T, kPkProlong>), dim3(t.nb_h), dim3(256), 0, st, lv, t.pre_h,
-                                   t.ne, color, S);
+                hipLaunchKernelGGL((k_pk_half_batch<T, kPkProlong>), dim3(t.nb_h), dim3(256), 0, st, lv, bm, color, S);
             else
-                hipLaunchKernelGGL((k_pk_half_batch<T, kPkPlain>), dim3(t.nb_h), dim3(256), 0, st, lv, t.pre_h, t.ne,
-                                   color, S);
+                hipLaunchKernelGGL((k_pk_half_batch<T, kPkPlain>), dim3(t.nb_h), dim3(256), 0, st, lv, bm, color, S);
         }
     };
     for (int l = 0; l < pb->lmax; ++l) {
         for (int s = 0; s < o.num_pre_smooth; ++s) half(l, s == 0 ? kPkZero : kPkPlain);
         const auto& t = pb->lv[l];
-        hipLaunchKernelGGL(k_restrict_batch<T>, dim3(t.nb_r), dim3(256), 0, st, (const BLevel<T>*)t.lv, t.pre_r, t.ne,
-                           S);
+        hipLaunchKernelGGL(k_restrict_batch<T>, dim3(t.nb_r), dim3(256), 0, st, (const BLevel<T>*)t.lv,
+                           (const BlkMap*)t.bm_r, S);
     }
     hipLaunchKernelGGL(k_coarse_batch<T>, dim3(pb->ncoarse), dim3(kCoarseThreads), 0, st,
                        (const BCoarse<T>*)pb->coarse, S, o.bottom_rtol, o.bottom_atol, o.bottom_maxiter,
@@ -1955,7 +1946,7 @@ static void batch_iteration(ksgf_poisson_batch* pb, hipStream_t st)
     for (int l = pb->lmax - 1; l >= 0; --l)
         for (int s = 0; s < o.num_post_smooth; ++s) half(l, s == 0 ? kPkProlong : kPkPlain);
     hipLaunchKernelGGL(k_update_residual_batch<T>, dim3(pb->top_nb), dim3(256), 0, st, (const BTop<T>*)pb->top,
-                       pb->top_pre, pb->np, 1, S);
+                       (const BlkMap*)pb->top_bm, 1, S);
     hipLaunchKernelGGL(k_batch_finalize, dim3(pb->np), dim3(256), 0, st, (const int*)pb->nparts, 0, 0.0, 0.0, S);
 }
 
@@ -1978,7 +1969,7 @@ static int batch_solve(ksgf_poisson_batch* pb, const double* const* rhs, double*
         KSGF_HIP_CHECK(hipMemsetAsync(mp->bw.stats, 0, 2 * sizeof(int), st));
     }
     hipLaunchKernelGGL(k_update_residual_batch<T>, dim3(pb->top_nb), dim3(256), 0, st, (const BTop<T>*)pb->top,
-                       pb->top_pre, pb->np, 0, S);
+                       (const BlkMap*)pb->top_bm, 0, S);
     hipLaunchKernelGGL(k_batch_finalize, dim3(pb->np), dim3(256), 0, st, (const int*)pb->nparts, 1, rtol, atol, S);
     KSGF_LAUNCH_CHECK();
     auto read_state = [&]() -> int {
@@ -2068,10 +2059,10 @@ int ksgf_poisson_batch_destroy(ksgf_poisson_batch* pb)
     if (pb->cap) (void)hipStreamDestroy(pb->cap);
     for (auto& t : pb->lv) {
         if (t.lv) (void)hipFree(t.lv);
-        if (t.pre_h) (void)hipFree(t.pre_h);
-        if (t.pre_r) (void)hipFree(t.pre_r);
+        if (t.bm_h) (void)hipFree(t.bm_h);
+        if (t.bm_r) (void)hipFree(t.bm_r);
     }
-    void* ps[] = {pb->top, pb->top_pre, pb->coarse, pb->nparts, pb->part0, pb->part,
+    void* ps[] = {pb->top, pb->top_bm, pb->coarse, pb->nparts, pb->part0, pb->part,
                   pb->active, pb->cur, pb->iters, pb->norm, pb->target, pb->bound};
     for (void* p : ps)
         if (p) (void)hipFree(p);
