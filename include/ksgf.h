@@ -214,6 +214,34 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
  * Synchronises the stream once per iteration.  Returns KSGF_ERR_DIVERGED
  * if any problem diverged, else KSGF_ERR_NOT_CONVERGED if any did not
  * converge. */
+/* Reverse mode of the advect + project step (differentiation workload; no
+ * counterpart in the reference, which has no adjoint).  A triply periodic
+ * one-box domain of n[0..2] >= 8 cells over [0, prob_hi], density 1, zero
+ * pressure gradient, a passive scalar s and the velocity u advanced at a
+ * fixed dt by the step of ksgf_host_stepper / GodunovPredictor:
+ *   u_mac = P ExtrapVelToFaces(u);  s' = s - dt div F(s);  u' = u - dt div F(u)
+ * (conservative PPM fluxes with the given limiter, MAC projection to
+ * mg_rtol / mg_atol with opts).  Fields are domain-wide device arrays,
+ * Fortran order, velocity component-major (3*N), scalar N.
+ * forward: nsteps steps from (vel0, s0), keeping a checkpoint of the state
+ *   every checkpoint_every steps; writes the final fields (vel_out, s_out may
+ *   be NULL) and, if mlmg_iters is not NULL, the V-cycles of each step (host).
+ * backward: given the adjoint (gradient of a scalar function) of the final
+ *   fields, returns the adjoint of the initial fields of the last forward
+ *   call, recomputing each checkpoint segment and reversing its steps
+ *   (limiter and upwind branches frozen at the forward state; the MAC
+ *   projection's adjoint is the projection).  Atomic accumulation: the
+ *   result is reproducible to rounding. */
+typedef struct ksgf_adjoint ksgf_adjoint;
+int ksgf_adjoint_create(const int* n, const double* prob_hi, int limiter_type, double mg_rtol,
+                        double mg_atol, const ksgf_mlmg_options* opts, ksgf_adjoint** out);
+int ksgf_adjoint_forward(ksgf_adjoint* a, const double* vel0, const double* s0, int nsteps,
+                         double dt, int checkpoint_every, double* vel_out, double* s_out,
+                         int* mlmg_iters, ksgf_stream_t stream);
+int ksgf_adjoint_backward(ksgf_adjoint* a, const double* vel_bar, const double* s_bar,
+                          double* vel0_bar, double* s0_bar, ksgf_stream_t stream);
+int ksgf_adjoint_destroy(ksgf_adjoint* a);
+
 typedef struct ksgf_poisson_batch ksgf_poisson_batch;
 int ksgf_poisson_batch_create(int nprob, const int* n, const double* dx, const int* lobc,
                               const int* hibc, const double* beta,

@@ -58,6 +58,7 @@ EXPORTS = [
     "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_predictor_update",
     "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
     "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
+    "ksgf_adjoint_create", "ksgf_adjoint_forward", "ksgf_adjoint_backward", "ksgf_adjoint_destroy",
 ]
 
 _lib = None
@@ -112,6 +113,12 @@ def load(path=None):
     L.ksgf_poisson_batch_solve.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_double,
                                            C.c_double, C.c_int, C.POINTER(KsgfMLMGInfo), C.c_void_p]
     L.ksgf_poisson_batch_destroy.argtypes = [C.c_void_p]
+    L.ksgf_adjoint_create.argtypes = [_ip, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_double,
+                                      C.POINTER(KsgfMLMGOptions), C.POINTER(C.c_void_p)]
+    L.ksgf_adjoint_forward.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp, _dp, _ip,
+                                       C.c_void_p]
+    L.ksgf_adjoint_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_void_p]
+    L.ksgf_adjoint_destroy.argtypes = [C.c_void_p]
     L.ksgf_macproj_destroy.argtypes = [C.c_void_p]
     L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,

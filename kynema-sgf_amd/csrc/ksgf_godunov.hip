@@ -488,9 +488,7 @@ enum {
     S_CRN = 18,  // 9 (d*3+t)
 };
 
-struct Stages {
-    MFDev lo[3], hi[3], ed[3], uad[3], crn[3][3];
-};
+using Stages = GodunovStages;
 
 static int alloc_stages(const ksgf_boxarray* ba, int ncomp, int ncrn, Stages& S)
 {
@@ -633,6 +631,33 @@ int ksgf_extrap_vel_to_faces(const ksgf_boxarray* ba, const double* vel, const d
                              int use_forces_in_trans, int limiter_type, int allow_inflow_on_outflow,
                              ksgf_stream_t stream)
 {
+    return extrap_vel_to_faces_keep(ba, vel, vel_forces, u_mac, v_mac, w_mac, bcrec, dt, use_ppm, use_forces_in_trans,
+                                    limiter_type, allow_inflow_on_outflow, stream, nullptr);
+}
+
+int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const double* q, const double* q_on_face_bc,
+                                   double* flux_x, double* flux_y, double* flux_z, double* face_x, double* face_y,
+                                   double* face_z, int known_edge_state, const double* u_mac, const double* v_mac,
+                                   const double* w_mac, const double* divu, const double* fq, double dt,
+                                   const ksgf_bcrec* bcrec, const int* iconserv, int use_ppm,
+                                   int use_forces_in_trans, int is_velocity, int fluxes_are_area_weighted,
+                                   int limiter_type, int allow_inflow_on_outflow, ksgf_stream_t stream)
+{
+    return compute_fluxes_keep(ba, ncomp, q, q_on_face_bc, flux_x, flux_y, flux_z, face_x, face_y, face_z,
+                               known_edge_state, u_mac, v_mac, w_mac, divu, fq, dt, bcrec, iconserv, use_ppm,
+                               use_forces_in_trans, is_velocity, fluxes_are_area_weighted, limiter_type,
+                               allow_inflow_on_outflow, stream, nullptr);
+}
+
+}  // extern "C"
+
+namespace ksgf {
+
+int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const double* vel_forces, double* u_mac,
+                             double* v_mac, double* w_mac, const ksgf_bcrec* bcrec, double dt, int use_ppm,
+                             int use_forces_in_trans, int limiter_type, int allow_inflow_on_outflow,
+                             ksgf_stream_t stream, GodunovStages* keep)
+{
     if (!ba || !vel || !vel_forces || !u_mac || !v_mac || !w_mac || !bcrec) {
         set_error("ksgf_extrap_vel_to_faces: null argument");
         return KSGF_ERR_INVALID;
@@ -679,16 +704,17 @@ int ksgf_extrap_vel_to_faces(const ksgf_boxarray* ba, const double* vel, const d
                              0.0, use_forces_in_trans, 1, 1, 0, allow_inflow_on_outflow, 0, st));
     KSGF_TRY(launch_final<2>(ba, q, q, f, q, 0, 3, uad[0], uad[1], uad[2], mac[2], mac[2], S, bc, g, c2, c4, hdt,
                              0.0, use_forces_in_trans, 1, 1, 0, allow_inflow_on_outflow, 0, st));
+    if (keep) *keep = S;
     return KSGF_OK;
 }
 
-int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const double* q, const double* q_on_face_bc,
-                                   double* flux_x, double* flux_y, double* flux_z, double* face_x, double* face_y,
-                                   double* face_z, int known_edge_state, const double* u_mac, const double* v_mac,
-                                   const double* w_mac, const double* divu, const double* fq, double dt,
-                                   const ksgf_bcrec* bcrec, const int* iconserv, int use_ppm,
-                                   int use_forces_in_trans, int is_velocity, int fluxes_are_area_weighted,
-                                   int limiter_type, int allow_inflow_on_outflow, ksgf_stream_t stream)
+int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, const double* q_on_face_bc,
+                        double* flux_x, double* flux_y, double* flux_z, double* face_x, double* face_y,
+                        double* face_z, int known_edge_state, const double* u_mac, const double* v_mac,
+                        const double* w_mac, const double* divu, const double* fq, double dt,
+                        const ksgf_bcrec* bcrec, const int* iconserv, int use_ppm, int use_forces_in_trans,
+                        int is_velocity, int fluxes_are_area_weighted, int limiter_type,
+                        int allow_inflow_on_outflow, ksgf_stream_t stream, GodunovStages* keep)
 {
     if (!ba || ncomp <= 0 || ncomp > kMaxComp || !flux_x || !flux_y || !flux_z || !face_x || !face_y || !face_z ||
         !u_mac || !v_mac || !w_mac) {
@@ -781,8 +807,13 @@ int ksgf_compute_fluxes_from_state(const ksgf_boxarray* ba, int ncomp, const dou
     KSGF_TRY(launch_final<2>(ba, qv, qb, fv, dv, has_divu, ncomp, mac[0], mac[1], mac[2], face[2], flux[2], S, bc, g,
                              c2, c4, hdt, area[2], use_forces_in_trans, 0, is_velocity, fluxes_are_area_weighted,
                              allow_inflow_on_outflow, cons_mask, st));
+    if (keep) *keep = S;
     return KSGF_OK;
 }
+
+}  // namespace ksgf
+
+extern "C" {
 
 int ksgf_compute_divergence(const ksgf_boxarray* ba, double* div, const double* flux_x, const double* flux_y,
                             const double* flux_z, int ncomp, double mult, int fluxes_are_area_weighted,

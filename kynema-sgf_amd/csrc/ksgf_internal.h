@@ -260,4 +260,27 @@ inline RegDev regdev(const Region* r, int nbox)
 int mfdev(const ksgf_boxarray* ba, double* p, int itype, int ncomp, int ngrow, MFDev* out);
 int cmfdev(const ksgf_boxarray* ba, const double* p, int itype, int ncomp, int ngrow, CMFDev* out);
 
+// Intermediate face arrays of the three Godunov stages (faces of each
+// direction, ngrow 1, in per-BoxArray scratch that the next Godunov call on
+// the same BoxArray overwrites): PPM face states lo/hi, upwinded normal
+// states ed, advective (Burgers) velocities uad, corner-coupled states
+// crn[normal][transverse].
+struct GodunovStages {
+    MFDev lo[3], hi[3], ed[3], uad[3], crn[3][3];
+};
+
+// ksgf_extrap_vel_to_faces / ksgf_compute_fluxes_from_state that also hand
+// back their stage arrays (keep may be null)
+int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const double* vel_forces, double* u_mac,
+                             double* v_mac, double* w_mac, const ksgf_bcrec* bcrec, double dt, int use_ppm,
+                             int use_forces_in_trans, int limiter_type, int allow_inflow_on_outflow,
+                             ksgf_stream_t stream, GodunovStages* keep);
+int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, const double* q_on_face_bc,
+                        double* flux_x, double* flux_y, double* flux_z, double* face_x, double* face_y,
+                        double* face_z, int known_edge_state, const double* u_mac, const double* v_mac,
+                        const double* w_mac, const double* divu, const double* fq, double dt,
+                        const ksgf_bcrec* bcrec, const int* iconserv, int use_ppm, int use_forces_in_trans,
+                        int is_velocity, int fluxes_are_area_weighted, int limiter_type,
+                        int allow_inflow_on_outflow, ksgf_stream_t stream, GodunovStages* keep);
+
 }  // namespace ksgf

@@ -11,64 +11,74 @@
 // The expressions are written in a fixed operand order and the library is
 // compiled without floating-point contraction, so results are reproducible
 // bit for bit against a CPU evaluation of the same formulas.
+//
+// The reconstruction is written once for a number type T: double in the
+// forward kernels, Dual<N> (ksgf_dual.h) where the adjoint needs the local
+// Jacobian of a trace.  Branches compare values only, so a Dual evaluation
+// takes the branches of the double evaluation at the same point.
 #pragma once
 
 #include "../../include/ksgf.h"
+#include "ksgf_dual.h"
 #include "ksgf_hd.h"
 
 namespace ksgf {
 
 constexpr double kSmallVel = 1.0e-8;
 
-KSGF_HD double vanleer(double a, double b, double c)
+template <typename T>
+KSGF_HD T vanleer(const T& a, const T& b, const T& c)
 {
-    double dsc = 0.5 * (c - a);
-    double dsl = 2.0 * (b - a);
-    double dsr = 2.0 * (c - b);
+    T dsc = 0.5 * (c - a);
+    T dsl = 2.0 * (b - a);
+    T dsr = 2.0 * (c - b);
     if (dsl * dsr > 0.0) {
-        double m = fmin(fabs(dsc), fmin(fabs(dsl), fabs(dsr)));
-        return copysign(m, dsc);
+        T m = vmin(vabs(dsc), vmin(vabs(dsl), vabs(dsr)));
+        return vcopysign(m, dsc);
     }
-    return 0.0;
+    return zero_like(dsc);
 }
 
-KSGF_HD double clip2(double e, double a, double b)
+template <typename T>
+KSGF_HD T clip2(T e, const T& a, const T& b)
 {
-    e = fmax(e, fmin(a, b));
-    return fmin(e, fmax(a, b));
+    e = vmax(e, vmin(a, b));
+    return vmin(e, vmax(a, b));
 }
 
-KSGF_HD void mono(double s0, double& sm, double& sp)
+template <typename T>
+KSGF_HD void mono(const T& s0, T& sm, T& sp)
 {
     if ((sp - s0) * (s0 - sm) <= 0.0) {
         sp = s0;
         sm = s0;
-    } else if (fabs(sp - s0) >= 2.0 * fabs(sm - s0)) {
+    } else if (vabs(sp - s0) >= 2.0 * vabs(sm - s0)) {
         sp = 3.0 * s0 - 2.0 * sm;
-    } else if (fabs(sm - s0) >= 2.0 * fabs(sp - s0)) {
+    } else if (vabs(sm - s0) >= 2.0 * vabs(sp - s0)) {
         sm = 3.0 * s0 - 2.0 * sp;
     }
 }
 
-KSGF_HD double bnd_edge(double sb, double s0, double s1, double s2)
+template <typename T>
+KSGF_HD T bnd_edge(const T& sb, const T& s0, const T& s1, const T& s2)
 {
     return -0.2 * sb + 0.75 * s0 + 0.5 * s1 - 0.05 * s2;
 }
 
 // edge values of the cell whose 5-point stencil is (sm2 .. sp2)
-KSGF_HD void ppm_edges(double sm2, double sm1, double s0, double sp1, double sp2,
-                                          int lim, int bclo, int bchi, int x, int dlo, int dhi,
-                                          double& sm, double& sp)
+template <typename T>
+KSGF_HD void ppm_edges(const T& sm2, const T& sm1, const T& s0, const T& sp1, const T& sp2, int lim, int bclo,
+                       int bchi, int x, int dlo, int dhi, T& sm, T& sp)
 {
-    double e1, e2;
+    T e1, e2;
     const bool limited = (lim != KSGF_PPM_NOLIMITER);
     if (!limited) {
         e1 = (7.0 / 12.0) * (sm1 + s0) - (1.0 / 12.0) * (sm2 + sp1);
         e2 = (7.0 / 12.0) * (s0 + sp1) - (1.0 / 12.0) * (sm1 + sp2);
     } else {
-        double d0 = vanleer(sm2, sm1, s0);
-        double d1 = vanleer(sm1, s0, sp1);
-        double d2 = vanleer(s0, sp1, sp2);
+        T d0 = vanleer(sm2, sm1, s0);
+        T d1 = vanleer(sm1, s0, sp1);
+        T d2 = vanleer(s0, sp1, sp2);
         e1 = 0.5 * (sm1 + s0) - (d1 - d0) / 6.0;
         e1 = clip2(e1, sm1, s0);
         e2 = 0.5 * (s0 + sp1) - (d2 - d1) / 6.0;
@@ -101,23 +111,23 @@ KSGF_HD void ppm_edges(double sm2, double sm1, double s0, double sp1, double sp2
     }
 }
 
-KSGF_HD double minmod_slope(double sm1, double s0, double sp1, int bclo, int bchi,
-                                               int x, int dlo, int dhi)
+template <typename T>
+KSGF_HD T minmod_slope(const T& sm1, const T& s0, const T& sp1, int bclo, int bchi, int x, int dlo, int dhi)
 {
     const bool lw = (bclo == KSGF_BC_EXT_DIR || bclo == KSGF_BC_HOEXTRAP);
     const bool hw = (bchi == KSGF_BC_EXT_DIR || bchi == KSGF_BC_HOEXTRAP);
     if (lw && x == dlo) return s0 - sm1;
     if (hw && x == dhi) return sp1 - s0;
-    double a = s0 - sm1, b = sp1 - s0;
-    if (a * b <= 0.0) return 0.0;
-    return (fabs(a) < fabs(b)) ? a : b;
+    T a = s0 - sm1, b = sp1 - s0;
+    if (a * b <= 0.0) return zero_like(a);
+    return (vabs(a) < vabs(b)) ? a : b;
 }
 
 // time-centred states at the low (Im) and high (Ip) faces of one cell;
 // um/up: advecting velocity at the low/high face
-KSGF_HD void ppm_trace(double sm2, double sm1, double s0, double sp1, double sp2,
-                                          int lim, int bclo, int bchi, int x, int dlo, int dhi,
-                                          double um, double up, double dtdx, double& Im, double& Ip)
+template <typename T>
+KSGF_HD void ppm_trace(const T& sm2, const T& sm1, const T& s0, const T& sp1, const T& sp2, int lim, int bclo,
+                       int bchi, int x, int dlo, int dhi, const T& um, const T& up, double dtdx, T& Im, T& Ip)
 {
     if (lim == KSGF_PPM_UPWIND) {
         Im = s0;
@@ -125,22 +135,22 @@ KSGF_HD void ppm_trace(double sm2, double sm1, double s0, double sp1, double sp2
         return;
     }
     if (lim == KSGF_PPM_MINMOD) {
-        double sl = minmod_slope(sm1, s0, sp1, bclo, bchi, x, dlo, dhi);
+        T sl = minmod_slope(sm1, s0, sp1, bclo, bchi, x, dlo, dhi);
         Ip = (up > kSmallVel) ? s0 + 0.5 * (1.0 - up * dtdx) * sl : s0;
         Im = (um < -kSmallVel) ? s0 - 0.5 * (1.0 + um * dtdx) * sl : s0;
         return;
     }
-    double sm, sp;
+    T sm, sp;
     ppm_edges(sm2, sm1, s0, sp1, sp2, lim, bclo, bchi, x, dlo, dhi, sm, sp);
-    double s6 = 6.0 * s0 - 3.0 * (sm + sp);
+    T s6 = 6.0 * s0 - 3.0 * (sm + sp);
     if (up > kSmallVel) {
-        double sig = fabs(up) * dtdx;
+        T sig = vabs(up) * dtdx;
         Ip = sp - (0.5 * sig) * ((sp - sm) - (1.0 - (2.0 / 3.0) * sig) * s6);
     } else {
         Ip = s0;
     }
     if (um < -kSmallVel) {
-        double sig = fabs(um) * dtdx;
+        T sig = vabs(um) * dtdx;
         Im = sm + (0.5 * sig) * ((sp - sm) + (1.0 - (2.0 / 3.0) * sig) * s6);
     } else {
         Im = s0;
@@ -154,14 +164,36 @@ KSGF_HD double burgers(double lo, double hi)
     return ltm ? 0.0 : st;
 }
 
+// derivatives of burgers(lo, hi) in the branch it takes
+KSGF_HD void burgers_partials(double lo, double hi, double& dlo, double& dhi)
+{
+    const bool ltm = ((lo <= 0.0 && hi >= 0.0) || (fabs(lo + hi) < kSmallVel));
+    const bool left = (lo + hi) >= 0.0;
+    dlo = (!ltm && left) ? 1.0 : 0.0;
+    dhi = (!ltm && !left) ? 1.0 : 0.0;
+}
+
 KSGF_HD double upwind(double lo, double hi, double u)
 {
     if (fabs(u) < kSmallVel) return 0.5 * (lo + hi);
     return (u >= 0.0) ? lo : hi;
 }
 
-KSGF_HD void edge_bcs(int x, int dlo, int dhi, int bclo, int bchi, double glo, double ghi,
-                                         bool both_dir, double& lo, double& hi)
+// derivatives of upwind(lo, hi, u) with respect to lo and hi (the selector
+// is piecewise constant in u)
+KSGF_HD void upwind_partials(double u, double& dlo, double& dhi)
+{
+    if (fabs(u) < kSmallVel) {
+        dlo = 0.5;
+        dhi = 0.5;
+    } else {
+        dlo = (u >= 0.0) ? 1.0 : 0.0;
+        dhi = (u >= 0.0) ? 0.0 : 1.0;
+    }
+}
+
+KSGF_HD void edge_bcs(int x, int dlo, int dhi, int bclo, int bchi, double glo, double ghi, bool both_dir,
+                      double& lo, double& hi)
 {
     if (x == dlo) {
         if (bclo == KSGF_BC_EXT_DIR) {

@@ -186,3 +186,27 @@ def poisson_problems(nprob=64, edges=(32, 48, 64, 80, 96, 112, 128), seed=5, bcs
         probs.append(dict(n=n, dx=(1.0 / e,) * 3, lobc=tuple(lo), hibc=tuple(hi), beta=1.0))
         rhs.append(f)
     return probs, rhs
+
+
+def differentiation(n=128, noise=1e-3, seed=1, sigma=0.1, cfl=0.5, nsteps=20, center=None, blob_seed=5):
+    """Periodic unit cube n^3: Taylor-Green plus U(-noise, noise) velocity
+    (as taylor_green()), a Gaussian blob s = exp(-|x - x0|^2 / (2 sigma^2))
+    as the passive scalar (extra['scalar'], (1, n, n, n)), and the fixed
+    time step of CFL `cfl` for the initial velocity (extra['dt'],
+    dt = cfl / max_d max |u_d| / dx).  x0 is the box centre moved by a
+    seeded U(-1/2, 1/2) cell in each direction (blob_seed), unless `center`
+    is given: with the blob exactly at the centre (a face of an even grid)
+    mirror cells hold equal values and the PPM limiters sit exactly on their
+    switch points, where the step has no derivative."""
+    c = taylor_green(n, n, noise=noise, seed=seed, cfl=cfl, nsteps=nsteps)
+    if center is None:
+        center = tuple(0.5 + (_philox(blob_seed).uniform(-0.5, 0.5, size=3) / n))
+    x = [c.geom.cell_centers(d) for d in range(3)]
+    r2 = (((x[0] - center[0]) ** 2)[None, None, :] + ((x[1] - center[1]) ** 2)[None, :, None]
+          + ((x[2] - center[2]) ** 2)[:, None, None])
+    c.extra["scalar"] = np.exp(-r2 / (2.0 * sigma * sigma))[None].copy()
+    h = 1.0 / n
+    cmax = max(float(np.max(np.abs(c.vel[d]))) / h for d in range(3))
+    c.extra["dt"] = cfl / cmax
+    c.name = f"diff{n}"
+    return c
