@@ -44,6 +44,10 @@ def parse():
                    help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
                         "N(0,1) right-hand sides, seed 5) solved as one batch (hipGraph replay and eager) and one "
                         "after another; prints its own JSON line")
+    p.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                   help="summarise four rocprofv3 --pmc output directories (FETCH_SIZE and WRITE_SIZE of a "
+                        "1-step mixed run with KSGF_SWEEP_PLANES fixed, and of --copy-probe) into HBM bytes per "
+                        "launch of the level-0 half sweep; prints text for profiles/")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB -> 1 GiB elementwise copy (x*1.0) 4 times: the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -123,6 +127,70 @@ def cpu_baseline(n_cell):
                       f"physics, {d['mlmg']['iters']} V-cycles in the last), {dt:.2f} s"}
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_pk_half_abl256_mixed.txt")
+
+
+def _pmc_rows(d):
+    import csv
+    import glob
+    import re
+
+    out = []
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            out.append((int(r["Dispatch_Id"]), re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", ""),
+                        int(r["Grid_Size"]), float(r["Counter_Value"])))
+    out.sort()
+    return out
+
+
+def pmc_summary(fdir, wdir, cfdir, cwdir, n=(256, 256, 256), kz=4):
+    """HBM bytes per launch of the level-0 k_pk_half<float> launches of a
+    one-step run: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) matched launch
+    by launch, each scaled by what the counter reads for the 1 GiB copy."""
+    import collections
+
+    true_kib = (1 << 27) * 8 / 1024.0
+    scale = []
+    for d in (cfdir, cwdir):
+        v = [x[3] for x in _pmc_rows(d) if "MulFunctor" in x[1]]
+        scale.append(true_kib / (sum(v) / len(v)))
+    # level-0 launch grid with kz planes per workgroup: (n0/2/64) x (n1/4) x (n2/kz) tiles of 256
+    ntx = -(-(n[0] // 2) // 64)
+    grid0 = ntx * (-(-n[1] // 4)) * (-(-n[2] // kz)) * 256
+    F = [x for x in _pmc_rows(fdir) if "k_pk_half<float" in x[1] and x[2] == grid0]
+    W = [x for x in _pmc_rows(wdir) if "k_pk_half<float" in x[1] and x[2] == grid0]
+    if len(F) != len(W) or not F:
+        raise SystemExit(f"unmatched level-0 launches: {len(F)} fetch vs {len(W)} write")
+    per = collections.defaultdict(list)
+    for f, w in zip(F, W):
+        if f[1] != w[1]:
+            raise SystemExit("launch order differs between the two counter runs")
+        per[f[1]].append((f[3] * scale[0] * 1024.0, w[3] * scale[1] * 1024.0))
+    lines = [f"copy probe scale: FETCH_SIZE x{scale[0]:.4f}, WRITE_SIZE x{scale[1]:.4f} "
+             f"(1 GiB read + 1 GiB written by torch.mul(x, 1.0))",
+             f"level-0 k_pk_half<float> launches (grid {grid0}, {len(F)} launches):"]
+    allb = []
+    for name, v in sorted(per.items()):
+        fb = sum(x[0] for x in v) / len(v)
+        wb = sum(x[1] for x in v) / len(v)
+        allb += [x[0] + x[1] for x in v]
+        lines.append(f"  {name:40s} n={len(v):4d} fetch {fb / 1e6:8.2f} MB  write {wb / 1e6:8.2f} MB")
+    lines.append(f"traffic_bytes_per_launch: {sum(allb) / len(allb):.0f}")
+    return "\n".join(lines)
+
+
+def pmc_traffic():
+    """HBM bytes per level-0 half-sweep launch from the committed summary, or None."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def poisson_batch(steps, warmup):
     """Time the batched solve of the 64-problem set against the same
     problems solved one at a time through ksgf_macproj_solve."""
@@ -189,6 +257,9 @@ def main():
     if a.copy_probe:
         copy_probe()
         return
+    if a.pmc_summary:
+        print(pmc_summary(*a.pmc_summary))
+        return
     if a.workload == "poisson-batch":
         print(json.dumps(poisson_batch(a.steps, a.warmup)))
         return
@@ -232,8 +303,9 @@ def main():
         avg_s = prof["ms"] * 1e-3 / prof["launches"]
         bpl = prof["bytes"] / prof["launches"]
         ach = bpl / avg_s / 1e9
+        traffic = pmc_traffic() if (n == (256, 256, 256) and head == "mixed") else None
         out["roofline"] = {"bound": "hbm", "kernel": prof["name"], "achieved": ach, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                            "launches": prof["launches"], "avg_us": avg_s * 1e6,
                            "bytes_per_launch": bpl}
     if not a.no_cpu:

@@ -129,7 +129,12 @@ KSGF_HD void row_tile(const RowTiles& t, int lid, int tid, int& x, int& y, int& 
 // array plus the piecewise-constant prolongation of the coarse correction,
 // formed on the fly (the red half reads prolonged black neighbours, the
 // black half mirrors its prolonged own value at walls)
-enum { kPkPlain = 0, kPkZero = 1, kPkProlong = 2 };
+// kPkProlong: first sweep after the coarse correction, prolonging it on the
+// fly (the red half adds it to every neighbour, the black half to itself at
+// walls); kPkProlongWalls: red half of that sweep after pk_prolong_color has
+// already added the correction to the black cells (so only a wall-mirrored
+// self needs it)
+enum { kPkPlain = 0, kPkZero = 1, kPkProlong = 2, kPkProlongWalls = 3 };
 
 template <typename T>
 struct PkArgs {
@@ -170,7 +175,7 @@ KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
     if (!(in_w && in_e && in_s && in_n && in_b && in_t)) {
         if (MODE == kPkZero)
             self = (T)0;
-        else if (MODE == kPkProlong)
+        else if (MODE == kPkProlong || MODE == kPkProlongWalls)
             self = a.x[c] + a.xc[LvIdx<CP>(a.C, i >> a.s0, j >> a.s1, k >> a.s2)];
         else
             self = a.x[c];
@@ -191,6 +196,17 @@ KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
     const T bo = in_b ? nb(i, j, k > 0 ? k - 1 : L.n2 - 1) : pk_wall<T>(L.lo[2]) * self;
     const T t = in_t ? nb(i, j, k < L.n2 - 1 ? k + 1 : 0) : pk_wall<T>(L.hi[2]) * self;
     a.x[c] = ((cx * (w + e) + cy * (s + n) + cz * (bo + t)) - a.b[c]) * idg;
+}
+
+// x += coarse correction of the parent, for the colour-`color` cell at
+// packed position (h, j, k) (the same sum the on-the-fly prolongation forms)
+template <typename T, bool CP>
+KSGF_HD void pk_prolong_color(const PkArgs<T>& a, int color, int h, int j, int k)
+{
+    const LevelGeom& L = a.L;
+    const int i = 2 * h + ((j + k + color) & 1);
+    const int c = color * L.pcol + k * L.pplane + j * L.prow + h;
+    a.x[c] = a.x[c] + a.xc[LvIdx<CP>(a.C, i >> a.s0, j >> a.s1, k >> a.s2)];
 }
 
 // residual of the fine children of coarse cell (I, J, K) of a packed fine

@@ -34,6 +34,10 @@
 #include <cmath>
 #include <cstring>
 
+#include <cstdlib>
+
+#include <hip/hip_ext.h>
+
 #include "ksgf_internal.h"
 #include "ksgf_mg_grid.h"
 
@@ -249,6 +253,16 @@ __global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTile
     row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, h, j, k0, k1, a.L.n2);
     if (h >= a.L.prow || j >= a.L.n1) return;
     for (int k = k0; k < k1; ++k) pk_relax<T, MODE, CP>(a, color, h, j, k);
+}
+
+// x += coarse correction on the colour-`color` cells of a packed level
+template <typename T, bool CP>
+__global__ void __launch_bounds__(256) k_pk_prolong_color(PkArgs<T> a, int color, RowTiles t)
+{
+    int h, j, k0, k1;
+    row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, h, j, k0, k1, a.L.n2);
+    if (h >= a.L.prow || j >= a.L.n1) return;
+    for (int k = k0; k < k1; ++k) pk_prolong_color<T, CP>(a, color, h, j, k);
 }
 
 // x += prolongation of the coarse correction on a packed level (only when
@@ -506,6 +520,7 @@ struct BottomWork {
     double *r, *rh, *p, *v, *s, *t;
     double *x, *b;  // FP64 bottom unknown / rhs (FP32 V-cycles convert in and out)
     int* stats;     // [0] BiCGStab iterations (accumulated), [1] failures
+    int lds;        // byte offset of the eight vectors in the LDS pool, or -1
 };
 
 template <typename T>
@@ -514,7 +529,20 @@ struct LevelPtrs {
     T* cor;
     T* res;
     int r0, r1, r2;  // ratio to the next coarser level
+    int lds;         // byte offset of cor (then res) in the LDS pool, or -1
 };
+
+// LDS pool of the single-workgroup coarse cycle.  The levels below its
+// first level live only inside the kernel, so from the bottom up as many of
+// them as fit (and the bottom solver's vectors) are kept in LDS instead of
+// HBM: every pass of the coarse cycle ends in a workgroup barrier, and LDS
+// latency instead of L2 latency is what makes those passes cheap.
+constexpr int kLdsPoolBytes = 96 * 1024;
+
+__host__ __device__ inline int lds_align(long long bytes)
+{
+    return (int)((bytes + 15) / 16 * 16);
+}
 
 // block-wide sum / maximum: a wavefront butterfly, then the per-wave
 // results in fixed order (deterministic, two barriers)
@@ -680,15 +708,32 @@ __device__ void bottom_solve(double* x, double* b, const LevelGeom& L, const Bot
 // levels lc .. nlev-1 of a V-cycle: zero, pre-smooth, residual, restrict
 // down to the bottom, bottom solve (FP64), prolong, post-smooth back up
 template <typename T>
-__device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev, const BottomWork& w,
-                                   int singular, double rtol, double atol, int maxiter, int nub, int nuf, int nu1,
-                                   int nu2, double* sh)
+__device__ __forceinline__ T* lv_cor(const LevelPtrs<T>* lp, int l, char* pool)
 {
+    return lp[l].lds >= 0 ? (T*)(pool + lp[l].lds) : lp[l].cor;
+}
+template <typename T>
+__device__ __forceinline__ T* lv_res(const LevelPtrs<T>* lp, int l, char* pool)
+{
+    return lp[l].lds >= 0 ? (T*)(pool + lp[l].lds + lds_align((long long)lp[l].G.tot * sizeof(T))) : lp[l].res;
+}
+
+template <typename T>
+__device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev, const BottomWork& wg,
+                                   int singular, double rtol, double atol, int maxiter, int nub, int nuf, int nu1,
+                                   int nu2, double* sh, char* pool)
+{
+    BottomWork w = wg;
+    if (w.lds >= 0) {
+        const int vb = lds_align((long long)lp[nlev - 1].G.tot * sizeof(double));
+        double** v[8] = {&w.r, &w.rh, &w.p, &w.v, &w.s, &w.t, &w.x, &w.b};
+        for (int q = 0; q < 8; ++q) *v[q] = (double*)(pool + w.lds + q * vb);
+    }
     for (int l = lc; l < nlev - 1; ++l) {
         const LevelGeom F = lp[l].G;
         const LevelGeom C = lp[l + 1].G;
-        T* x = lp[l].cor;
-        const T* b = lp[l].res;
+        T* x = lv_cor(lp, l, pool);
+        const T* b = lv_res(lp, l, pool);
         for (long long c = threadIdx.x; c < F.tot; c += blockDim.x) x[c] = (T)0;
         __syncthreads();
         blk_smooth<T>(x, b, F, nu1);
@@ -696,7 +741,7 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
         const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
         const T wgt = (T)(1.0 / (double)(r0 * r1 * r2));
         const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
-        T* cres = lp[l + 1].res;
+        T* cres = lv_res(lp, l + 1, pool);
         BLK_FOR(C, t_, cc)
         {
             const int q = t_ / C.n0;
@@ -718,8 +763,8 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
     // bottom
     {
         const LevelGeom Lb = lp[nlev - 1].G;
-        T* xb = lp[nlev - 1].cor;
-        T* bb = lp[nlev - 1].res;
+        T* xb = lv_cor(lp, nlev - 1, pool);
+        T* bb = lv_res(lp, nlev - 1, pool);
         double* x64 = (sizeof(T) == sizeof(double)) ? (double*)xb : w.x;
         double* b64 = (sizeof(T) == sizeof(double)) ? (double*)bb : w.b;
         if (sizeof(T) != sizeof(double)) {
@@ -735,8 +780,8 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
     for (int l = nlev - 2; l >= lc; --l) {
         const LevelGeom F = lp[l].G;
         const LevelGeom C = lp[l + 1].G;
-        T* x = lp[l].cor;
-        const T* xc = lp[l + 1].cor;
+        T* x = lv_cor(lp, l, pool);
+        const T* xc = lv_cor(lp, l + 1, pool);
         const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
         BLK_FOR(F, t_, c)
         {
@@ -745,7 +790,7 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
             x[c] += xc[LIdx(C, i / r0, j / r1, k / r2)];
         }
         __syncthreads();
-        blk_smooth<T>(x, lp[l].res, F, nu2);
+        blk_smooth<T>(x, lv_res(lp, l, pool), F, nu2);
     }
 }
 
@@ -755,7 +800,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtr
                                                                   int maxiter, int nub, int nuf, int nu1, int nu2)
 {
     __shared__ double sh[kCoarseThreads];
-    coarse_vcycle_body<T>(lp, lc, nlev, w, singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh);
+    __shared__ __align__(16) char pool[kLdsPoolBytes];
+    coarse_vcycle_body<T>(lp, lc, nlev, w, singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh, pool);
 }
 
 // ---------------------------------------------------------------------------
@@ -822,6 +868,7 @@ struct ksgf_macproj {
     std::vector<float*> cor32, res32;     // FP32 per level (mixed mode)
     std::vector<int> kz_sweep;            // autotuned planes per workgroup of the packed half sweep
     double* rpart = nullptr;              // per-block maxima of the outer residual
+    int kz_res = 16;                      // autotuned planes per thread of the outer residual
     LevelPtrs<double>* dlp64 = nullptr;   // device level tables for the coarse kernel
     LevelPtrs<float>* dlp32 = nullptr;
     double* phi = nullptr;
@@ -867,17 +914,43 @@ static std::vector<std::array<double, 3>> level_h(const ksgf_macproj* mp)
     return h;
 }
 
+// LDS placement of the coarse cycle: the bottom solver's vectors first, then
+// levels from the bottom up while they fit; never level lc, whose arrays the
+// grid kernels read and write
+static std::vector<int> lds_layout(ksgf_macproj* mp, size_t word, int* bw_lds)
+{
+    std::vector<int> off(mp->nlev, -1);
+    long long used = 0;
+    *bw_lds = -1;
+    const long long bw = 8LL * lds_align(mp->G[mp->nlev - 1].tot * (long long)sizeof(double));
+    if (bw <= kLdsPoolBytes) {
+        *bw_lds = 0;
+        used = bw;
+    }
+    for (int l = mp->nlev - 1; l > mp->lc; --l) {
+        const long long need = 2LL * lds_align(mp->G[l].tot * (long long)word);
+        if (used + need > kLdsPoolBytes) break;
+        off[l] = (int)used;
+        used += need;
+    }
+    return off;
+}
+
 static int upload_level_tables(ksgf_macproj* mp)
 {
     if (mp->opt.precision == KSGF_MG_MIXED) {
+        const std::vector<int> off = lds_layout(mp, sizeof(float), &mp->bw.lds);
         std::vector<LevelPtrs<float>> t(mp->nlev);
         for (int l = 0; l < mp->nlev; ++l)
-            t[l] = {mp->G[l], mp->cor32[l], mp->res32[l], mp->ratio[3 * l], mp->ratio[3 * l + 1], mp->ratio[3 * l + 2]};
+            t[l] = {mp->G[l], mp->cor32[l], mp->res32[l], mp->ratio[3 * l], mp->ratio[3 * l + 1], mp->ratio[3 * l + 2],
+                    off[l]};
         KSGF_HIP_CHECK(hipMemcpy(mp->dlp32, t.data(), sizeof(LevelPtrs<float>) * t.size(), hipMemcpyHostToDevice));
     } else {
+        const std::vector<int> off = lds_layout(mp, sizeof(double), &mp->bw.lds);
         std::vector<LevelPtrs<double>> t(mp->nlev);
         for (int l = 0; l < mp->nlev; ++l)
-            t[l] = {mp->G[l], mp->cor[l], mp->res[l], mp->ratio[3 * l], mp->ratio[3 * l + 1], mp->ratio[3 * l + 2]};
+            t[l] = {mp->G[l], mp->cor[l], mp->res[l], mp->ratio[3 * l], mp->ratio[3 * l + 1], mp->ratio[3 * l + 2],
+                    off[l]};
         KSGF_HIP_CHECK(hipMemcpy(mp->dlp64, t.data(), sizeof(LevelPtrs<double>) * t.size(), hipMemcpyHostToDevice));
     }
     return KSGF_OK;
@@ -950,7 +1023,7 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
 
 // planes per workgroup of the row-tile kernels
 constexpr int kRestrictKz = 4;
-constexpr int kResidualKz = 16;
+constexpr int kResidualKzMin = 4;  // candidates 4, 8, 16, 32 planes per thread
 
 template <typename T>
 static PkArgs<T> pk_args(const ksgf_macproj* mp, int l, T* x, const T* b, const T* xc)
@@ -969,18 +1042,26 @@ static PkArgs<T> pk_args(const ksgf_macproj* mp, int l, T* x, const T* b, const 
 }
 
 template <typename T>
-static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hipStream_t st)
+// e0/e1 (optional): events the dispatch itself records at the kernel's
+// start and end (hipExtLaunchKernel), so the interval is the kernel's own
+// execution time without launch gaps
+static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hipStream_t st,
+                             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, kz);
     const dim3 g(t.ntx * t.nty * t.nkc), blk(256);
     if (mode == kPkZero)
-        hipLaunchKernelGGL((k_pk_half<T, kPkZero, false>), g, blk, 0, st, a, color, t);
+        hipExtLaunchKernelGGL((k_pk_half<T, kPkZero, false>), g, blk, 0, st, e0, e1, 0, a, color, t);
+    else if (mode == kPkProlongWalls && a.C.packed)
+        hipExtLaunchKernelGGL((k_pk_half<T, kPkProlongWalls, true>), g, blk, 0, st, e0, e1, 0, a, color, t);
+    else if (mode == kPkProlongWalls)
+        hipExtLaunchKernelGGL((k_pk_half<T, kPkProlongWalls, false>), g, blk, 0, st, e0, e1, 0, a, color, t);
     else if (mode == kPkProlong && a.C.packed)
-        hipLaunchKernelGGL((k_pk_half<T, kPkProlong, true>), g, blk, 0, st, a, color, t);
+        hipExtLaunchKernelGGL((k_pk_half<T, kPkProlong, true>), g, blk, 0, st, e0, e1, 0, a, color, t);
     else if (mode == kPkProlong)
-        hipLaunchKernelGGL((k_pk_half<T, kPkProlong, false>), g, blk, 0, st, a, color, t);
+        hipExtLaunchKernelGGL((k_pk_half<T, kPkProlong, false>), g, blk, 0, st, e0, e1, 0, a, color, t);
     else
-        hipLaunchKernelGGL((k_pk_half<T, kPkPlain, false>), g, blk, 0, st, a, color, t);
+        hipExtLaunchKernelGGL((k_pk_half<T, kPkPlain, false>), g, blk, 0, st, e0, e1, 0, a, color, t);
 }
 
 // one red-black sweep (red half, black half) of packed level l
@@ -988,21 +1069,29 @@ template <typename T>
 static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b, const T* xc, hipStream_t st)
 {
     const PkArgs<T> a = pk_args<T>(mp, l, x, b, xc);
+    if (mode == kPkProlong) {
+        // the coarse correction goes into the black cells in a short pass of
+        // its own; the red half then relaxes against plain neighbours (a
+        // prolongation inside the red half's six neighbour reads was latency
+        // bound at twice a plain half's time), the black half is plain
+        const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, mp->kz_sweep[l]);
+        if (a.C.packed)
+            hipLaunchKernelGGL((k_pk_prolong_color<T, true>), dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, a, 1, t);
+        else
+            hipLaunchKernelGGL((k_pk_prolong_color<T, false>), dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, a, 1, t);
+    }
     for (int color = 0; color < 2; ++color) {
+        const int m = (mode == kPkProlong) ? (color == 0 ? kPkProlongWalls : kPkPlain) : mode;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (l == 0 && g_prof.on) {
             e0 = prof_event();
             e1 = prof_event();
-            if (e0 && e1) (void)hipEventRecord(e0, st);
+            if (!e0 || !e1) e0 = e1 = nullptr;
         }
-        launch_pk_kernel<T>(a, color, mode, mp->kz_sweep[l], st);
+        launch_pk_kernel<T>(a, color, m, mp->kz_sweep[l], st, e0, e1);
         if (e0 && e1) {
-            (void)hipEventRecord(e1, st);
             const double half = 0.5 * (double)a.L.n0 * a.L.n1 * a.L.n2;
-            const double nc = (double)a.C.n0 * a.C.n1 * a.C.n2;
-            const double words = (mode == kPkZero && color == 0) ? 2.0 * half
-                                 : (mode == kPkProlong && color == 0) ? 3.0 * half + nc
-                                                                      : 3.0 * half;
+            const double words = (m == kPkZero && color == 0) ? 2.0 * half : 3.0 * half;
             g_prof.launches += 1;
             g_prof.bytes += sizeof(T) * words;
             g_prof.name = sizeof(T) == 8 ? "k_pk_half<double>(level 0)" : "k_pk_half<float>(level 0)";
@@ -1024,6 +1113,12 @@ template <typename T>
 static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
 {
     const LevelGeom& L = mp->G[l];
+    // KSGF_SWEEP_PLANES=1|2|4|8 fixes the choice (reproducible launch shapes,
+    // e.g. for hardware-counter runs that must match launches by grid size)
+    if (const char* env = std::getenv("KSGF_SWEEP_PLANES")) {
+        const int v = std::atoi(env);
+        if (v == 1 || v == 2 || v == 4 || v == 8) return std::min(v, std::max(1, L.n2));
+    }
     const std::array<long long, 4> key{L.n0, L.n1, L.n2, (long long)sizeof(T)};
     auto it = kz_cache().find(key);
     if (it != kz_cache().end()) return it->second;
@@ -1053,6 +1148,57 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     kz_cache()[key] = best;
+    return best;
+}
+
+// Autotuner of the planes per thread of the FP64 outer residual (the
+// z-march keeps a column's planes in registers: more planes fetch less halo,
+// fewer planes give more threads in flight), timed on the projector's own
+// arrays without a correction (which writes only the residual and the
+// partial maxima), cached per (shape, precision)
+template <typename T>
+static int tune_residual(ksgf_macproj* mp)
+{
+    static std::map<std::array<long long, 4>, int> cache;
+    const LevelGeom& P = mp->P;
+    const std::array<long long, 4> key{P.n0, P.n1, P.n2, (long long)sizeof(T)};
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    T* r = (sizeof(T) == sizeof(float)) ? (T*)mp->res32[0] : (T*)mp->res[0];
+    hipEvent_t e0, e1;
+    int best = 16;
+    if (hipEventCreate(&e0) != hipSuccess) return best;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return best;
+    }
+    float best_ms = 1e30f;
+    for (int kz : {4, 8, 16, 32}) {
+        if (kz < kResidualKzMin || kz > std::max(P.n2, kResidualKzMin)) continue;
+        const RowTiles t = row_tiles(P.n0, P.n1, P.n2, kz);
+        const dim3 g(t.ntx * t.nty * t.nkc);
+        auto launch = [&]() {
+            if (mp->G[0].packed)
+                hipLaunchKernelGGL((k_update_residual<T, T, true>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
+                                   (const T*)nullptr, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
+            else
+                hipLaunchKernelGGL((k_update_residual<T, T, false>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
+                                   (const T*)nullptr, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
+        };
+        launch();
+        (void)hipEventRecord(e0, nullptr);
+        for (int rep = 0; rep < 3; ++rep) launch();
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = kz;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cache[key] = best;
     return best;
 }
 
@@ -1131,7 +1277,7 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
 {
     const LevelGeom& P = mp->P;
     const LevelGeom& L0 = mp->G[0];
-    const RowTiles t = row_tiles(P.n0, P.n1, P.n2, kResidualKz);
+    const RowTiles t = row_tiles(P.n0, P.n1, P.n2, mp->kz_res);
     const int nb = t.ntx * t.nty * t.nkc;
     if (P.ghost_mode) {
         // explicit ghosts: phi += cor first, then the residual from ghosts
@@ -1434,7 +1580,8 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     if ((e = mp_alloc(&p, kSumBlocks * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
     mp->part = (double*)p;
     {
-        const RowTiles t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, kResidualKz);
+        // sized for the smallest candidate of the residual tuner
+        const RowTiles t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, kResidualKzMin);
         if ((e = mp_alloc(&p, (size_t)t.ntx * t.nty * t.nkc * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
         mp->rpart = (double*)p;
     }
@@ -1467,6 +1614,7 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         ksgf_macproj_destroy(mp);
         return e;
     }
+    if (!mp->P.ghost_mode) mp->kz_res = mixed ? tune_residual<float>(mp) : tune_residual<double>(mp);
     mp->kz_sweep.assign(mp->nlev, 8);
     for (int l = 0; l < mp->lc; ++l) {
         if (!mp->G[l].packed) continue;
@@ -1678,9 +1826,10 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_batch(const BCoarse<T
                                                                  int nuf, int nu1, int nu2)
 {
     __shared__ double sh[kCoarseThreads];
+    __shared__ __align__(16) char pool[kLdsPoolBytes];
     const BCoarse<T>& c = cb[blockIdx.x];
     if (!S.active[c.prob]) return;
-    coarse_vcycle_body<T>(c.lp, c.lc, c.nlev, c.w, c.singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh);
+    coarse_vcycle_body<T>(c.lp, c.lc, c.nlev, c.w, c.singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh, pool);
 }
 
 // FP64 outer residual of every problem; with_cor: phi(next) = phi + cor
@@ -1883,7 +2032,7 @@ static int batch_tables(ksgf_poisson_batch* pb)
         b.rhs = mp->rhs;
         b.cor = cor_of<T>(mp)[0];
         b.r = res_of<T>(mp)[0];
-        b.t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, kResidualKz);
+        b.t = row_tiles(mp->P.n0, mp->P.n1, mp->P.n2, mp->kz_res);
         b.prob = p;
         const int nb = b.t.ntx * b.t.nty * b.t.nkc;
         part0.push_back((int)tbm.size());

@@ -169,6 +169,66 @@ static void run_half(int n0, int n1, int n2, const int lo[3], const int hi[3], c
     ++g_cases;
 }
 
+// the first post-smoothing sweep in two forms: on-the-fly prolongation
+// (red: kPkProlong, black: kPkProlong) against the correction added to the
+// black cells first (pk_prolong_color), then red kPkProlongWalls and black
+// kPkPlain.  Both colours must come out bit for bit the same.
+template <typename T>
+static void run_split_prolong(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3],
+                              bool coarse_packed, unsigned seed)
+{
+    const LevelGeom P = make_level(n0, n1, n2, lo, hi, 1.0, 1.7, 0.45, true);
+    const LevelGeom C = make_level(n0 / ratio[0], n1 / ratio[1], n2 / ratio[2], lo, hi, 2.0, 3.4, 0.9,
+                                   coarse_packed && (n0 / ratio[0]) % 2 == 0);
+    const T nan = std::numeric_limits<T>::quiet_NaN();
+    std::vector<T> x1(P.tot, nan), bp(P.tot, nan), xc(C.tot, nan);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                x1[PIdx(P, i, j, k)] = (T)U(rng);
+                bp[PIdx(P, i, j, k)] = (T)U(rng);
+            }
+    for (int k = 0; k < C.n2; ++k)
+        for (int j = 0; j < C.n1; ++j)
+            for (int i = 0; i < C.n0; ++i) xc[AnyIdx(C, i, j, k)] = (T)U(rng);
+    std::vector<T> x2 = x1;
+    auto args = [&](std::vector<T>& x) {
+        PkArgs<T> a{};
+        a.x = x.data();
+        a.b = bp.data();
+        a.xc = xc.data();
+        a.L = P;
+        a.C = C;
+        a.s0 = ratio[0] == 2;
+        a.s1 = ratio[1] == 2;
+        a.s2 = ratio[2] == 2;
+        return a;
+    };
+    auto each = [&](auto&& f) {
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int h = 0; h < n0 / 2; ++h) f(h, j, k);
+    };
+    const PkArgs<T> a1 = args(x1), a2 = args(x2);
+    const bool cp = C.packed;
+    each([&](int h, int j, int k) { cp ? pk_relax<T, kPkProlong, true>(a1, 0, h, j, k) : pk_relax<T, kPkProlong, false>(a1, 0, h, j, k); });
+    each([&](int h, int j, int k) { cp ? pk_relax<T, kPkProlong, true>(a1, 1, h, j, k) : pk_relax<T, kPkProlong, false>(a1, 1, h, j, k); });
+    each([&](int h, int j, int k) { cp ? pk_prolong_color<T, true>(a2, 1, h, j, k) : pk_prolong_color<T, false>(a2, 1, h, j, k); });
+    each([&](int h, int j, int k) { cp ? pk_relax<T, kPkProlongWalls, true>(a2, 0, h, j, k) : pk_relax<T, kPkProlongWalls, false>(a2, 0, h, j, k); });
+    each([&](int h, int j, int k) { pk_relax<T, kPkPlain, false>(a2, 1, h, j, k); });
+    long long bad = 0;
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                const T g = x2[PIdx(P, i, j, k)], r = x1[PIdx(P, i, j, k)];
+                if (std::memcmp(&g, &r, sizeof(T)) != 0) ++bad;
+            }
+    if (bad) fail<T>("split prolongation sweep", n0, n1, n2, kPkProlongWalls, -1, bad);
+    ++g_cases;
+}
+
 template <typename T>
 static void run_restrict(int n0, int n1, int n2, const int lo[3], const int hi[3], const int ratio[3],
                          unsigned seed)
@@ -216,6 +276,7 @@ static void run_all(int n0, int n1, int n2, const int lo[3], const int hi[3], co
             run_half<T, kPkProlong>(n0, n1, n2, lo, hi, ratio, color, cp, seed + 10 * color + cp + 4);
         }
     run_restrict<T>(n0, n1, n2, lo, hi, ratio, seed + 100);
+    for (int cp = 0; cp < 2; ++cp) run_split_prolong<T>(n0, n1, n2, lo, hi, ratio, cp, seed + 200 + cp);
 }
 
 int main()
