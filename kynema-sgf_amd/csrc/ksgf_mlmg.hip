@@ -1513,9 +1513,12 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
                     any = 1;
                 }
             } else {
+                // h_min over every direction, also one that cannot coarsen
+                // any more: the others then stop as soon as they would exceed
+                // 1.5x its cell size, the anisotropy a point smoother damps
                 double hmin = 1e300;
                 for (int d = 0; d < 3; ++d)
-                    if (can[d] && ch[d] < hmin) hmin = ch[d];
+                    if (ch[d] < hmin) hmin = ch[d];
                 for (int d = 0; d < 3; ++d)
                     if (can[d] && ch[d] <= 1.5 * hmin) {
                         r[d] = 2;

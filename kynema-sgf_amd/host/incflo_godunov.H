@@ -115,6 +115,25 @@ public:
         (void)hipStreamSynchronize(st_);
         vel_->copyToDomain(vel);
     }
+    // the MAC projection's phi of the last step, domain-wide (host array)
+    void getPhi(double* phi)
+    {
+        const Box& dom = ba_.geom().domain;
+        size_t n = 1;
+        for (int d = 0; d < 3; ++d) n *= (size_t)(dom.hi[d] - dom.lo[d] + 1);
+        double* d = nullptr;
+        if (hipMalloc((void**)&d, n * sizeof(double)) != hipSuccess) throw std::runtime_error("getPhi: hipMalloc");
+        try {
+            mac_proj_.getPhi(d, st_);
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        const hipError_t e = hipMemcpyAsync(phi, d, n * sizeof(double), hipMemcpyDeviceToHost, st_);
+        (void)hipStreamSynchronize(st_);
+        (void)hipFree(d);
+        if (e != hipSuccess) throw std::runtime_error("getPhi: copy");
+    }
     MultiFab& velocity() { return *vel_; }
     MultiFab& umac() { return umac_; }
     MultiFab& vmac() { return vmac_; }

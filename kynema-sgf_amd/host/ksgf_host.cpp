@@ -110,6 +110,18 @@ int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel)
     }
 }
 
+int ksgf_host_stepper_get_phi(ksgf_host_stepper* s, double* phi)
+{
+    try {
+        if (!s || !phi) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->getPhi(phi);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_get_phi: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
 int ksgf_host_stepper_enable_scalar(ksgf_host_stepper* s, const ksgf_bcrec* bc, const double* bcval)
 {
     try {

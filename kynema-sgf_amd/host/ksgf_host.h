@@ -26,6 +26,8 @@ int ksgf_host_stepper_set_state(ksgf_host_stepper* s, const double* vel, const d
 int ksgf_host_stepper_step(ksgf_host_stepper* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info,
                            double* divmax);
 int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel);
+/* the MAC projection's phi of the last step, domain-wide host array (n0*n1*n2) */
+int ksgf_host_stepper_get_phi(ksgf_host_stepper* s, double* phi);
 /* passive scalar advected with the step (AdvOp_Godunov.H:116-300):
  * bc: its BCRec, bcval: 6 orientation values; then set / get domain-wide */
 int ksgf_host_stepper_enable_scalar(ksgf_host_stepper* s, const ksgf_bcrec* bc, const double* bcval);

@@ -323,6 +323,13 @@ public:
     }
     MLMG& getMLMG() { return mlmg_; }
     void rebuild() { build(); }
+    // the last solution phi, domain-wide (n0*n1*n2 doubles, device), as
+    // project(Vector<MultiFab*> phi, ...) hands it back
+    void getPhi(double* phi, hipStream_t st = nullptr)
+    {
+        if (!mp_) throw std::runtime_error("MacProjector::getPhi: no projection yet");
+        ksgf::amr::check(ksgf_macproj_get_phi(mp_, phi, (ksgf_stream_t)st), "MacProjector::getPhi");
+    }
 
 private:
     void build()

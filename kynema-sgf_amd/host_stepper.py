@@ -32,6 +32,7 @@ def host_lib():
         L.ksgf_host_stepper_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(KsgfMLMGInfo),
                                              C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_velocity.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_get_phi.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_enable_scalar.argtypes = [C.c_void_p, C.POINTER(KsgfBCRec), C.POINTER(C.c_double)]
         L.ksgf_host_stepper_set_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
@@ -115,6 +116,13 @@ class HostStepper:
         n = self.case.geom.n_cell
         out = np.zeros((3, n[2], n[1], n[0]))
         _chk(host_lib().ksgf_host_stepper_get_velocity(self.h, _dp(out)), "get_velocity")
+        return out
+
+    def phi(self):
+        """the MAC projection's phi of the last step, (n2, n1, n0)"""
+        n = self.case.geom.n_cell
+        out = np.zeros((n[2], n[1], n[0]))
+        _chk(host_lib().ksgf_host_stepper_get_phi(self.h, _dp(out)), "get_phi")
         return out
 
     def close(self):

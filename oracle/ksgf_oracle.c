@@ -18,7 +18,10 @@
  *   - conservative transverse terms: flux difference plus q times the
  *     transverse divergence (dt/3 corner stage, dt/2 final stage);
  *   - MLMG: inf-norm convergence test against max(atol, rtol*|rhs|_inf),
- *     standard (all-direction) coarsening unless semicoarsening is asked for.
+ *     standard (all-direction) coarsening unless semicoarsening is asked for;
+ *     with semicoarsening a direction is coarsened when h_d <= 1.5 h_min,
+ *     h_min over all three directions (including one that can no longer
+ *     coarsen), and the 1e20 |rhs| divergence bound.
  */
 #include "ksgf_oracle.h"
 
@@ -1312,8 +1315,12 @@ static void build_levels(mg_t* M, const int* n, const double* dx, double beta)
             for (int d = 0; d < 3; ++d) L->r[d] = 2;
             any = 1;
         } else {
+            /* h_min over every direction, also one that cannot coarsen any
+             * more: once the thin direction stops, the others stop as soon as
+             * they would exceed 1.5x its cell size (a point smoother cannot
+             * damp more anisotropy than that) */
             double hmin = 1e300;
-            for (int d = 0; d < 3; ++d) if (can[d] && ch[d] < hmin) hmin = ch[d];
+            for (int d = 0; d < 3; ++d) if (ch[d] < hmin) hmin = ch[d];
             for (int d = 0; d < 3; ++d)
                 if (can[d] && ch[d] <= 1.5 * hmin) { L->r[d] = 2; any = 1; }
             if (!any) break;
