@@ -110,31 +110,26 @@ __global__ void __launch_bounds__(256) k_faceA(CMFDev q, CMFDev qbc, CMFDev f, i
         upm = uqp;
         upp = m(i1, j1, k1);
     }
+    // Ip of the low cell q (stencil x-3 .. x+1) and Im of the high cell p
+    // (stencil x-2 .. x+2), sharing their common slopes and edge
     auto face_states = [&](int n, double& lov, double& hiv) {
         const double* pn = s.p + cp + n * s.cs;
-        double a, bb, dmy;
-        // low cell q: stencil p-3 .. p+1
-        ppm_trace(pn[-3 * st], pn[-2 * st], pn[-st], pn[0], pn[st], lim, bc.lo[n][D], bc.hi[n][D], x - 1, g.dlo[D],
-                  g.dhi[D], uqm, uqp, dtdx, dmy, a);
-        // high cell p: stencil p-2 .. p+2
-        ppm_trace(pn[-2 * st], pn[-st], pn[0], pn[st], pn[2 * st], lim, bc.lo[n][D], bc.hi[n][D], x, g.dlo[D],
-                  g.dhi[D], upm, upp, dtdx, bb, dmy);
-        lov = a;
-        hiv = bb;
+        const double w[6] = {pn[-3 * st], pn[-2 * st], pn[-st], pn[0], pn[st], pn[2 * st]};
+        ppm_trace_pair(w, lim, bc.lo[n][D], bc.hi[n][D], x, g.dlo[D], g.dhi[D], uqm, uqp, upm, upp, dtdx, lov, hiv);
         if (uft) {
             lov += hdt * ff(qi, qj, qk, n);
             hiv += hdt * ff(i, j, k, n);
         }
     };
     double u;
+    double lovD = 0.0, hivD = 0.0;  // velocity mode: the normal component's states, reused below
     if (vel_mode) {
-        double lov, hiv;
-        face_states(D, lov, hiv);
+        face_states(D, lovD, hivD);
         double glo = 0.0, ghi = 0.0;
         if (x == g.dlo[D]) glo = fqb(gli, glj, glk, D);
         if (x == g.dhi[D] + 1) ghi = fqb(ghi_, ghj, ghk, D);
-        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[D][D], bc.hi[D][D], glo, ghi, true, lov, hiv);
-        u = burgers(lov, hiv);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[D][D], bc.hi[D][D], glo, ghi, true, lovD, hivD);
+        u = burgers(lovD, hivD);
         Fab fu = fab_at(uad, box, b);
         fu(i, j, k) = u;
     } else {
@@ -142,6 +137,14 @@ __global__ void __launch_bounds__(256) k_faceA(CMFDev q, CMFDev qbc, CMFDev f, i
     }
     for (int n = 0; n < ncomp; ++n) {
         double lov, hiv;
+        if (vel_mode && n == D) {
+            lov = lovD;
+            hiv = hivD;
+            flo_(i, j, k, n) = lov;
+            fhi_(i, j, k, n) = hiv;
+            fed_(i, j, k, n) = upwind(lov, hiv, u);
+            continue;
+        }
         face_states(n, lov, hiv);
         double glo = 0.0, ghi = 0.0;
         if (x == g.dlo[D]) glo = fqb(gli, glj, glk, n);

@@ -65,25 +65,27 @@ KSGF_HD T bnd_edge(const T& sb, const T& s0, const T& s1, const T& s2)
     return -0.2 * sb + 0.75 * s0 + 0.5 * s1 - 0.05 * s2;
 }
 
-// edge values of the cell whose 5-point stencil is (sm2 .. sp2)
+// interior value of the edge between cells a and b (stencil aa, a, b, bb):
+// fourth-order interpolant, or with the van Leer slopes da, db of a and b
+// (limited) the slope-corrected average clipped to [a, b]
 template <typename T>
-KSGF_HD void ppm_edges(const T& sm2, const T& sm1, const T& s0, const T& sp1, const T& sp2, int lim, int bclo,
-                       int bchi, int x, int dlo, int dhi, T& sm, T& sp)
+KSGF_HD T ppm_edge4(const T& aa, const T& a, const T& b, const T& bb)
 {
-    T e1, e2;
+    return (7.0 / 12.0) * (a + b) - (1.0 / 12.0) * (aa + bb);
+}
+template <typename T>
+KSGF_HD T ppm_edge_vl(const T& a, const T& b, const T& da, const T& db)
+{
+    return clip2(0.5 * (a + b) - (db - da) / 6.0, a, b);
+}
+
+// boundary treatment and monotonisation of the cell's interior edge values
+// (e1 low, e2 high) into (sm, sp)
+template <typename T>
+KSGF_HD void ppm_edges_finish(const T& e1, const T& e2, const T& sm2, const T& sm1, const T& s0, const T& sp1,
+                              const T& sp2, int lim, int bclo, int bchi, int x, int dlo, int dhi, T& sm, T& sp)
+{
     const bool limited = (lim != KSGF_PPM_NOLIMITER);
-    if (!limited) {
-        e1 = (7.0 / 12.0) * (sm1 + s0) - (1.0 / 12.0) * (sm2 + sp1);
-        e2 = (7.0 / 12.0) * (s0 + sp1) - (1.0 / 12.0) * (sm1 + sp2);
-    } else {
-        T d0 = vanleer(sm2, sm1, s0);
-        T d1 = vanleer(sm1, s0, sp1);
-        T d2 = vanleer(s0, sp1, sp2);
-        e1 = 0.5 * (sm1 + s0) - (d1 - d0) / 6.0;
-        e1 = clip2(e1, sm1, s0);
-        e2 = 0.5 * (s0 + sp1) - (d2 - d1) / 6.0;
-        e2 = clip2(e2, s0, sp1);
-    }
     const bool lw = (bclo == KSGF_BC_EXT_DIR || bclo == KSGF_BC_HOEXTRAP);
     const bool hw = (bchi == KSGF_BC_EXT_DIR || bchi == KSGF_BC_HOEXTRAP);
     if (lw && x == dlo) {
@@ -111,6 +113,25 @@ KSGF_HD void ppm_edges(const T& sm2, const T& sm1, const T& s0, const T& sp1, co
     }
 }
 
+// edge values of the cell whose 5-point stencil is (sm2 .. sp2)
+template <typename T>
+KSGF_HD void ppm_edges(const T& sm2, const T& sm1, const T& s0, const T& sp1, const T& sp2, int lim, int bclo,
+                       int bchi, int x, int dlo, int dhi, T& sm, T& sp)
+{
+    T e1, e2;
+    if (lim == KSGF_PPM_NOLIMITER) {
+        e1 = ppm_edge4(sm2, sm1, s0, sp1);
+        e2 = ppm_edge4(sm1, s0, sp1, sp2);
+    } else {
+        T d0 = vanleer(sm2, sm1, s0);
+        T d1 = vanleer(sm1, s0, sp1);
+        T d2 = vanleer(s0, sp1, sp2);
+        e1 = ppm_edge_vl(sm1, s0, d0, d1);
+        e2 = ppm_edge_vl(s0, sp1, d1, d2);
+    }
+    ppm_edges_finish(e1, e2, sm2, sm1, s0, sp1, sp2, lim, bclo, bchi, x, dlo, dhi, sm, sp);
+}
+
 template <typename T>
 KSGF_HD T minmod_slope(const T& sm1, const T& s0, const T& sp1, int bclo, int bchi, int x, int dlo, int dhi)
 {
@@ -121,6 +142,29 @@ KSGF_HD T minmod_slope(const T& sm1, const T& s0, const T& sp1, int bclo, int bc
     T a = s0 - sm1, b = sp1 - s0;
     if (a * b <= 0.0) return zero_like(a);
     return (vabs(a) < vabs(b)) ? a : b;
+}
+
+// time-centred state at the high face (advected by up > 0) and at the low
+// face (um < 0) of a parabola (sm, s0, sp)
+template <typename T>
+KSGF_HD T ppm_ip(const T& sm, const T& s0, const T& sp, const T& up, double dtdx)
+{
+    if (up > kSmallVel) {
+        const T s6 = 6.0 * s0 - 3.0 * (sm + sp);
+        const T sig = vabs(up) * dtdx;
+        return sp - (0.5 * sig) * ((sp - sm) - (1.0 - (2.0 / 3.0) * sig) * s6);
+    }
+    return s0;
+}
+template <typename T>
+KSGF_HD T ppm_im(const T& sm, const T& s0, const T& sp, const T& um, double dtdx)
+{
+    if (um < -kSmallVel) {
+        const T s6 = 6.0 * s0 - 3.0 * (sm + sp);
+        const T sig = vabs(um) * dtdx;
+        return sm + (0.5 * sig) * ((sp - sm) + (1.0 - (2.0 / 3.0) * sig) * s6);
+    }
+    return s0;
 }
 
 // time-centred states at the low (Im) and high (Ip) faces of one cell;
@@ -142,19 +186,44 @@ KSGF_HD void ppm_trace(const T& sm2, const T& sm1, const T& s0, const T& sp1, co
     }
     T sm, sp;
     ppm_edges(sm2, sm1, s0, sp1, sp2, lim, bclo, bchi, x, dlo, dhi, sm, sp);
-    T s6 = 6.0 * s0 - 3.0 * (sm + sp);
-    if (up > kSmallVel) {
-        T sig = vabs(up) * dtdx;
-        Ip = sp - (0.5 * sig) * ((sp - sm) - (1.0 - (2.0 / 3.0) * sig) * s6);
-    } else {
-        Ip = s0;
+    Ip = ppm_ip(sm, s0, sp, up, dtdx);
+    Im = ppm_im(sm, s0, sp, um, dtdx);
+}
+
+// the two traces one face needs from the six cells x-3 .. x+2 around it
+// (cell q = x-1 below, cell p = x above): Ip of q and Im of p.  The van
+// Leer slopes of the four middle cells and the interior edge between q and
+// p are shared by the two parabolas; every value equals the one ppm_trace
+// forms for the same cell.
+template <typename T>
+KSGF_HD void ppm_trace_pair(const T* s, int lim, int bclo, int bchi, int x, int dlo, int dhi, const T& uqm,
+                            const T& uqp, const T& upm, const T& upp, double dtdx, T& Ip_q, T& Im_p)
+{
+    if (lim != KSGF_PPM_VANLEER && lim != KSGF_PPM_NOLIMITER) {
+        T dmy;
+        ppm_trace(s[0], s[1], s[2], s[3], s[4], lim, bclo, bchi, x - 1, dlo, dhi, uqm, uqp, dtdx, dmy, Ip_q);
+        ppm_trace(s[1], s[2], s[3], s[4], s[5], lim, bclo, bchi, x, dlo, dhi, upm, upp, dtdx, Im_p, dmy);
+        return;
     }
-    if (um < -kSmallVel) {
-        T sig = vabs(um) * dtdx;
-        Im = sm + (0.5 * sig) * ((sp - sm) + (1.0 - (2.0 / 3.0) * sig) * s6);
+    T ea, eb, ec;  // interior edges q-1|q, q|p, p|p+1
+    if (lim == KSGF_PPM_NOLIMITER) {
+        ea = ppm_edge4(s[0], s[1], s[2], s[3]);
+        eb = ppm_edge4(s[1], s[2], s[3], s[4]);
+        ec = ppm_edge4(s[2], s[3], s[4], s[5]);
     } else {
-        Im = s0;
+        const T d1 = vanleer(s[0], s[1], s[2]);
+        const T d2 = vanleer(s[1], s[2], s[3]);
+        const T d3 = vanleer(s[2], s[3], s[4]);
+        const T d4 = vanleer(s[3], s[4], s[5]);
+        ea = ppm_edge_vl(s[1], s[2], d1, d2);
+        eb = ppm_edge_vl(s[2], s[3], d2, d3);
+        ec = ppm_edge_vl(s[3], s[4], d3, d4);
     }
+    T sm, sp;
+    ppm_edges_finish(ea, eb, s[0], s[1], s[2], s[3], s[4], lim, bclo, bchi, x - 1, dlo, dhi, sm, sp);
+    Ip_q = ppm_ip(sm, s[2], sp, uqp, dtdx);
+    ppm_edges_finish(eb, ec, s[1], s[2], s[3], s[4], s[5], lim, bclo, bchi, x, dlo, dhi, sm, sp);
+    Im_p = ppm_im(sm, s[3], sp, upm, dtdx);
 }
 
 KSGF_HD double burgers(double lo, double hi)
