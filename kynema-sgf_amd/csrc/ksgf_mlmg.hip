@@ -439,18 +439,33 @@ __global__ void __launch_bounds__(256) k_update_residual(const double* __restric
     block_max_part(m, part);
 }
 
-// maximum of the n partials into *norm (as the bits of a non-negative double)
-__global__ void __launch_bounds__(256) k_max_parts(const double* __restrict__ part, int n,
-                                                   unsigned long long* __restrict__ norm)
+// maximum of the n partials into *norm (as the bits of a non-negative double);
+// kMaxPartsThreads threads with four independent loads in flight each (the
+// partials are finite or +inf, so the maximum does not depend on the order)
+constexpr int kMaxPartsThreads = 1024;
+__global__ void __launch_bounds__(kMaxPartsThreads) k_max_parts(const double* __restrict__ part, int n,
+                                                                unsigned long long* __restrict__ norm)
 {
-    double m = 0.0;
-    for (int t = (int)threadIdx.x; t < n; t += (int)blockDim.x) m = fmax(m, part[t]);
-    __shared__ double sh[4];
+    constexpr int B = kMaxPartsThreads;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    int t = (int)threadIdx.x;
+    for (; t + 3 * B < n; t += 4 * B) {
+        m0 = fmax(m0, part[t]);
+        m1 = fmax(m1, part[t + B]);
+        m2 = fmax(m2, part[t + 2 * B]);
+        m3 = fmax(m3, part[t + 3 * B]);
+    }
+    for (; t < n; t += B) m0 = fmax(m0, part[t]);
+    double m = fmax(fmax(m0, m1), fmax(m2, m3));
+    __shared__ double sh[B / 64];
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0)
-        *norm = (unsigned long long)__double_as_longlong(fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3])));
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int w = 1; w < B / 64; ++w) r = fmax(r, sh[w]);
+        *norm = (unsigned long long)__double_as_longlong(r);
+    }
 }
 
 
@@ -869,6 +884,7 @@ struct ksgf_macproj {
     std::vector<int> kz_sweep;            // autotuned planes per workgroup of the packed half sweep
     double* rpart = nullptr;              // per-block maxima of the outer residual
     int kz_res = 16;                      // autotuned planes per thread of the outer residual
+    std::vector<int> kz_rst;              // autotuned coarse planes per thread of the restriction from each level
     LevelPtrs<double>* dlp64 = nullptr;   // device level tables for the coarse kernel
     LevelPtrs<float>* dlp32 = nullptr;
     double* phi = nullptr;
@@ -1023,7 +1039,7 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
 
 // planes per workgroup of the row-tile kernels
 constexpr int kRestrictKz = 4;
-constexpr int kResidualKzMin = 4;  // candidates 4, 8, 16, 32 planes per thread
+constexpr int kResidualKzMin = 1;  // candidates 1, 2, 4, 8, 16, 32 planes per thread
 
 template <typename T>
 static PkArgs<T> pk_args(const ksgf_macproj* mp, int l, T* x, const T* b, const T* xc)
@@ -1153,9 +1169,11 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
 
 // Autotuner of the planes per thread of the FP64 outer residual (the
 // z-march keeps a column's planes in registers: more planes fetch less halo,
-// fewer planes give more threads in flight), timed on the projector's own
-// arrays without a correction (which writes only the residual and the
-// partial maxima), cached per (shape, precision)
+// fewer planes give more threads in flight).  Each candidate is timed as the
+// solve calls it, with a correction (phi2 = phi + cor, r = rhs - A phi2) and
+// the reduction of its per-block maxima, on the projector's own arrays
+// (phi2 and the level-0 residual and correction are scratch until a solve
+// writes them); the fastest is cached per (shape, precision).
 template <typename T>
 static int tune_residual(ksgf_macproj* mp)
 {
@@ -1165,6 +1183,7 @@ static int tune_residual(ksgf_macproj* mp)
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     T* r = (sizeof(T) == sizeof(float)) ? (T*)mp->res32[0] : (T*)mp->res[0];
+    const T* cor = (sizeof(T) == sizeof(float)) ? (const T*)mp->cor32[0] : (const T*)mp->cor[0];
     hipEvent_t e0, e1;
     int best = 16;
     if (hipEventCreate(&e0) != hipSuccess) return best;
@@ -1173,21 +1192,76 @@ static int tune_residual(ksgf_macproj* mp)
         return best;
     }
     float best_ms = 1e30f;
-    for (int kz : {4, 8, 16, 32}) {
+    for (int kz : {1, 2, 4, 8, 16, 32}) {
         if (kz < kResidualKzMin || kz > std::max(P.n2, kResidualKzMin)) continue;
         const RowTiles t = row_tiles(P.n0, P.n1, P.n2, kz);
-        const dim3 g(t.ntx * t.nty * t.nkc);
+        const int nb = t.ntx * t.nty * t.nkc;
+        const dim3 g(nb);
         auto launch = [&]() {
             if (mp->G[0].packed)
                 hipLaunchKernelGGL((k_update_residual<T, T, true>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
-                                   (const T*)nullptr, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
+                                   cor, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
             else
                 hipLaunchKernelGGL((k_update_residual<T, T, false>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
-                                   (const T*)nullptr, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
+                                   cor, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
+            hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, nullptr, (const double*)mp->rpart, nb,
+                               mp->dnorm);
         };
         launch();
         (void)hipEventRecord(e0, nullptr);
         for (int rep = 0; rep < 3; ++rep) launch();
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = kz;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cache[key] = best;
+    return best;
+}
+
+// Autotuner of the coarse planes per thread of the residual + restriction
+// from level l (candidates 1, 2, 4, 8), timed on the level's own correction
+// and residual arrays into the next level's residual (all scratch at set-up),
+// cached per (fine shape, coarse shape, precision)
+template <typename T>
+static void launch_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* out, int kz, hipStream_t st)
+{
+    const LevelGeom& L = mp->G[l];
+    const LevelGeom& C = mp->G[l + 1];
+    const int* r = &mp->ratio[3 * l];
+    const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+    const RowTiles t = row_tiles(C.n0, C.n1, C.n2, kz);
+    hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, L, C, t, r[0],
+                       r[1], r[2], w);
+}
+
+template <typename T>
+static int tune_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* out)
+{
+    const LevelGeom& L = mp->G[l];
+    const LevelGeom& C = mp->G[l + 1];
+    static std::map<std::array<long long, 7>, int> cache;
+    const std::array<long long, 7> key{L.n0, L.n1, L.n2, C.n0, C.n1, C.n2, (long long)sizeof(T)};
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int best = 1;
+    float best_ms = 1e30f;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return best;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return best;
+    }
+    for (int kz : {1, 2, 4, 8}) {
+        if (kz > C.n2) continue;
+        launch_restrict<T>(mp, l, x, b, out, kz, nullptr);
+        (void)hipEventRecord(e0, nullptr);
+        for (int rep = 0; rep < 3; ++rep) launch_restrict<T>(mp, l, x, b, out, kz, nullptr);
         (void)hipEventRecord(e1, nullptr);
         float ms = 0.f;
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
@@ -1218,7 +1292,6 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     const LevelGeom& C = mp->G[l + 1];
     const int* r = &mp->ratio[3 * l];
     const long long nf = (long long)L.n0 * L.n1 * L.n2;
-    const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
     const int nu1 = mp->opt.num_pre_smooth, nu2 = mp->opt.num_post_smooth;
     // down
     if (L.packed) {
@@ -1233,12 +1306,7 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
         launch_half_sweeps<T>(cor[l], res[l], L, 0, nu1, st);
     }
     if (L.ghost_mode) launch_ghosts<T>(cor[l], L, st);
-    {
-        RowTiles t = row_tiles(C.n0, C.n1, C.n2, kRestrictKz);
-        if (t.ntx * t.nty * t.nkc < 2048) t = row_tiles(C.n0, C.n1, C.n2, 1);
-        hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, (const T*)cor[l],
-                           (const T*)res[l], res[l + 1], L, C, t, r[0], r[1], r[2], w);
-    }
+    launch_restrict<T>(mp, l, (const T*)cor[l], (const T*)res[l], res[l + 1], mp->kz_rst[l], st);
     vcycle<T>(mp, l + 1, cor, res, dlp, st);
     // up
     if (L.packed) {
@@ -1298,7 +1366,7 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
                                cor, mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
         if (cor) std::swap(mp->phi, mp->phi2);
     }
-    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(256), 0, st, (const double*)mp->rpart, nb, mp->dnorm);
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, st, (const double*)mp->rpart, nb, mp->dnorm);
 }
 
 // solve with the level-0 rhs in mp->rhs (ghost layout); phi in mp->phi
@@ -1619,7 +1687,12 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     }
     if (!mp->P.ghost_mode) mp->kz_res = mixed ? tune_residual<float>(mp) : tune_residual<double>(mp);
     mp->kz_sweep.assign(mp->nlev, 8);
+    mp->kz_rst.assign(mp->nlev, 1);
     for (int l = 0; l < mp->lc; ++l) {
+        if (mixed)
+            mp->kz_rst[l] = tune_restrict<float>(mp, l, mp->cor32[l], mp->res32[l], mp->res32[l + 1]);
+        else
+            mp->kz_rst[l] = tune_restrict<double>(mp, l, mp->cor[l], mp->res[l], mp->res[l + 1]);
         if (!mp->G[l].packed) continue;
         mp->kz_sweep[l] = mixed ? tune_sweep<float>(mp, l, mp->cor32[l], mp->res32[l])
                                 : tune_sweep<double>(mp, l, mp->cor[l], mp->res[l]);
