@@ -283,6 +283,43 @@ int ksgf_scale_by_rho(const ksgf_boxarray* ba, double* out, int ng_out, const do
                       int ng_a, const double* rho, int ng_rho, int ncomp, int ngrow,
                       int divide, ksgf_stream_t stream);
 
+/* ---- AMReX plotfile / VisMF files of MultiFabs in HBM (host I/O) ----
+ * Text layouts as the reference's readers parse them (tools/amrex_plotfile.py,
+ * tools/utilities/CheckpointToCSV/CheckpointFileUtil.cpp); FAB payloads in IEEE
+ * double, this host's byte order.  The MultiFab is copied to or from the host
+ * once (the stream is synchronised). */
+
+/* amrex::WriteMultiLevelPlotfile with one level (IOManager::write_plot_file,
+ * IOManager.cpp:181-187): <dir>/Header, <dir>/Level_0/Cell_H and
+ * <dir>/Level_0/Cell_D_00000, valid cells of the cell-centred MultiFab mf
+ * (ncomp components, ngrow ghosts in memory); names: ncomp variable names. */
+int ksgf_plotfile_write(const char* dir, const ksgf_boxarray* ba, const double* mf, int ncomp,
+                        int ngrow, const char* const* names, double time, int step,
+                        ksgf_stream_t stream);
+
+/* Level 0 of a plotfile into the valid cells of mf (ghosts untouched): every file
+ * box's data is copied where it overlaps the BoxArray's boxes (ParallelCopy).
+ * IEEE double or float data in either byte order; other formats give
+ * KSGF_ERR_UNSUPPORTED, a component count other than ncomp KSGF_ERR_INVALID. */
+int ksgf_plotfile_read(const char* dir, const ksgf_boxarray* ba, double* mf, int ncomp, int ngrow,
+                       ksgf_stream_t stream);
+
+/* amrex::VisMF::Write of one MultiFab, ghost cells included (checkpoint fields,
+ * IOManager.cpp:214-219): <prefix>_H and <prefix>_D_00000. */
+int ksgf_vismf_write(const char* prefix, const ksgf_boxarray* ba, const double* mf, int itype,
+                     int ncomp, int ngrow, ksgf_stream_t stream);
+
+/* amrex::VisMF::Read (IOManager.cpp:249-266): with the file's boxes equal to the
+ * BoxArray's, each FAB's points held on both sides, ghosts included; otherwise the
+ * file's valid data into the valid boxes it overlaps. */
+int ksgf_vismf_read(const char* prefix, const ksgf_boxarray* ba, double* mf, int itype, int ncomp,
+                    int ngrow, ksgf_stream_t stream);
+
+/* Checkpoint <dir>/Header of one level (IOManager::write_header,
+ * IOManager.cpp:308-356); creates <dir>/Level_0 for the VisMF fields. */
+int ksgf_checkpoint_write_header(const char* dir, const ksgf_boxarray* ba, int step, double time,
+                                 double dt, double dt_nm1, double dt_nm2);
+
 /* Live timing of the finest-level smoother launches (bench instrumentation;
  * no counterpart in the reference): reset starts recording HIP events around
  * each launch on its stream; read synchronises on them and returns the

@@ -59,6 +59,8 @@ EXPORTS = [
     "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
     "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
     "ksgf_adjoint_create", "ksgf_adjoint_forward", "ksgf_adjoint_backward", "ksgf_adjoint_destroy",
+    "ksgf_plotfile_write", "ksgf_plotfile_read", "ksgf_vismf_write", "ksgf_vismf_read",
+    "ksgf_checkpoint_write_header",
 ]
 
 _lib = None
@@ -129,6 +131,13 @@ def load(path=None):
     L.ksgf_mac_divergence_max.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_prof_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.c_char_p,
                                  C.c_int]
+    L.ksgf_plotfile_write.argtypes = [C.c_char_p, C.c_void_p, _dp, C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_double,
+                                      C.c_int, C.c_void_p]
+    L.ksgf_plotfile_read.argtypes = [C.c_char_p, C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_vismf_write.argtypes = [C.c_char_p, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_vismf_read.argtypes = [C.c_char_p, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_checkpoint_write_header.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                               C.c_double]
     _lib = L
     return L
 
@@ -314,6 +323,28 @@ class DeviceBoxArray:
     def icns_src_term(self, src, ng_src, gp, rho, ng_rho, scale_by_rho):
         check(load().ksgf_icns_src_term(self.h, _ptr(src), ng_src, _ptr(gp), _ptr(rho), ng_rho, scale_by_rho,
                                         _stream()), "icns_src_term")
+
+    # --- plotfile / VisMF files ---------------------------------------------
+    def write_plotfile(self, path, mf, ncomp, ngrow, names, time=0.0, step=0):
+        arr = (C.c_char_p * ncomp)(*[n.encode() for n in names])
+        check(load().ksgf_plotfile_write(str(path).encode(), self.h, _ptr(mf), ncomp, ngrow, arr, time, step,
+                                         _stream()), "plotfile_write")
+
+    def read_plotfile(self, path, mf, ncomp, ngrow):
+        check(load().ksgf_plotfile_read(str(path).encode(), self.h, _ptr(mf), ncomp, ngrow, _stream()),
+              "plotfile_read")
+
+    def write_vismf(self, prefix, mf, itype, ncomp, ngrow):
+        check(load().ksgf_vismf_write(str(prefix).encode(), self.h, _ptr(mf), itype, ncomp, ngrow, _stream()),
+              "vismf_write")
+
+    def read_vismf(self, prefix, mf, itype, ncomp, ngrow):
+        check(load().ksgf_vismf_read(str(prefix).encode(), self.h, _ptr(mf), itype, ncomp, ngrow, _stream()),
+              "vismf_read")
+
+    def write_checkpoint_header(self, path, step, time, dt, dt_nm1=0.0, dt_nm2=0.0):
+        check(load().ksgf_checkpoint_write_header(str(path).encode(), self.h, step, time, dt, dt_nm1, dt_nm2),
+              "checkpoint_write_header")
 
     def scale_by_rho(self, out, ng_out, a, ng_a, rho, ng_rho, ncomp, ngrow, divide=0):
         check(load().ksgf_scale_by_rho(self.h, _ptr(out), ng_out, _ptr(a), ng_a, _ptr(rho), ng_rho, ncomp, ngrow,

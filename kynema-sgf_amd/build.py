@@ -20,7 +20,8 @@ HOSTLIB = os.path.join(HERE, "libksgf_host.so")
 STEPEXE = os.path.join(HERE, "ksgf_step")
 HOSTDIR = os.path.join(HERE, "host")
 
-SOURCES = ["ksgf_layout.hip", "ksgf_godunov.hip", "ksgf_mlmg.hip", "ksgf_ops.hip", "ksgf_adjoint.hip"]
+SOURCES = ["ksgf_layout.hip", "ksgf_godunov.hip", "ksgf_mlmg.hip", "ksgf_ops.hip", "ksgf_adjoint.hip",
+           "ksgf_io.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

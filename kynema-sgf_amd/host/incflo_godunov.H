@@ -164,6 +164,28 @@ public:
         sc_->s->copyToDomain(s);
     }
 
+    // IOManager::write_plot_file (IOManager.cpp:136-190) for the fields this
+    // step owns: velocity, gp, density and the passive scalar when enabled,
+    // in one cell-centred MultiFab (staged through the host, as the
+    // reference copies them into one scratch field first)
+    void writePlotfile(const std::string& name, double time, int step)
+    {
+        const auto& dom = ba_.geom().domain;
+        const size_t N = (size_t)dom.numPts();
+        const int nc = sc_ ? 8 : 7;
+        std::vector<double> all((size_t)nc * N);
+        (void)hipStreamSynchronize(st_);
+        vel_->copyToDomain(all.data());
+        gp_.copyToDomain(all.data() + 3 * N);
+        rho_.copyToDomain(all.data() + 6 * N);
+        if (sc_) sc_->s->copyToDomain(all.data() + 7 * N);
+        MultiFab out(ba_, nc, 0);
+        out.copyFromDomain(all.data());
+        std::vector<std::string> names = {"velocityx", "velocityy", "velocityz", "gpx", "gpy", "gpz", "density"};
+        if (sc_) names.push_back("temperature");
+        ksgf::amr::WriteSingleLevelPlotfile(name, out, names, time, step, st_);
+    }
+
     StepStats advance(double fixed_dt = -1.0)
     {
         StepStats s;

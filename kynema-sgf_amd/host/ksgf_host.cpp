@@ -158,6 +158,18 @@ int ksgf_host_stepper_get_scalar(ksgf_host_stepper* s, double* scal)
     }
 }
 
+int ksgf_host_stepper_write_plotfile(ksgf_host_stepper* s, const char* dir, double time, int step)
+{
+    try {
+        if (!s || !dir) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->writePlotfile(dir, time, step);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_write_plotfile: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
 {
     delete reinterpret_cast<GodunovPredictor*>(s);

@@ -33,6 +33,9 @@ int ksgf_host_stepper_get_phi(ksgf_host_stepper* s, double* phi);
 int ksgf_host_stepper_enable_scalar(ksgf_host_stepper* s, const ksgf_bcrec* bc, const double* bcval);
 int ksgf_host_stepper_set_scalar(ksgf_host_stepper* s, const double* scal);
 int ksgf_host_stepper_get_scalar(ksgf_host_stepper* s, double* scal);
+/* plotfile of the step's fields (IOManager::write_plot_file): velocity, gp,
+ * density (and the scalar when enabled) */
+int ksgf_host_stepper_write_plotfile(ksgf_host_stepper* s, const char* dir, double time, int step);
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
 
 #ifdef __cplusplus

@@ -11,6 +11,9 @@
 //                           beta), setDomainBC, updateBeta, setUMAC, project,
 //                           getMLMG (icns_advection.cpp:88-104,143-154,
 //                           289-353,372-374)
+//   plotfile / VisMF        WriteSingleLevelPlotfile, ReadSingleLevelPlotfile,
+//                           VisMF::Write / Read (IOManager.cpp:181-187,
+//                           214-219, 249-266)
 //
 // Errors from the C ABI become std::runtime_error (amrex.throw_exception=1
 // semantics, main.cpp:60-71).
@@ -194,6 +197,44 @@ private:
     IndexType it_;
     long long n_ = 0;
     double* p_ = nullptr;
+};
+
+// amrex::WriteMultiLevelPlotfile with one level (IOManager::write_plot_file,
+// IOManager.cpp:181-187): the valid cells of a cell-centred MultiFab
+inline void WriteSingleLevelPlotfile(const std::string& name, const MultiFab& mf,
+                                     const std::vector<std::string>& varnames, double time, int step,
+                                     hipStream_t st = nullptr)
+{
+    if ((int)varnames.size() != mf.nComp()) throw std::runtime_error("WriteSingleLevelPlotfile: one name per component");
+    std::vector<const char*> names;
+    for (const auto& v : varnames) names.push_back(v.c_str());
+    check(ksgf_plotfile_write(name.c_str(), mf.boxArray().handle(), mf.data(), mf.nComp(), mf.nGrow(), names.data(),
+                              time, step, (ksgf_stream_t)st),
+          "WriteSingleLevelPlotfile");
+}
+
+// level 0 of a plotfile into the valid cells of mf
+inline void ReadSingleLevelPlotfile(const std::string& name, MultiFab& mf, hipStream_t st = nullptr)
+{
+    check(ksgf_plotfile_read(name.c_str(), mf.boxArray().handle(), mf.data(), mf.nComp(), mf.nGrow(),
+                             (ksgf_stream_t)st),
+          "ReadSingleLevelPlotfile");
+}
+
+// amrex::VisMF::Write / Read of checkpoint fields (IOManager.cpp:214-219, 249-266)
+struct VisMF {
+    static void Write(const MultiFab& mf, const std::string& prefix, hipStream_t st = nullptr)
+    {
+        check(ksgf_vismf_write(prefix.c_str(), mf.boxArray().handle(), mf.data(), (int)mf.ixType(), mf.nComp(),
+                               mf.nGrow(), (ksgf_stream_t)st),
+              "VisMF::Write");
+    }
+    static void Read(MultiFab& mf, const std::string& prefix, hipStream_t st = nullptr)
+    {
+        check(ksgf_vismf_read(prefix.c_str(), mf.boxArray().handle(), mf.data(), (int)mf.ixType(), mf.nComp(),
+                              mf.nGrow(), (ksgf_stream_t)st),
+              "VisMF::Read");
+    }
 };
 
 }  // namespace ksgf::amr

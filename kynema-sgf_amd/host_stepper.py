@@ -36,6 +36,7 @@ def host_lib():
         L.ksgf_host_stepper_enable_scalar.argtypes = [C.c_void_p, C.POINTER(KsgfBCRec), C.POINTER(C.c_double)]
         L.ksgf_host_stepper_set_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_write_plotfile.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_int]
         L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
         _h = L
     return _h
@@ -124,6 +125,10 @@ class HostStepper:
         out = np.zeros((n[2], n[1], n[0]))
         _chk(host_lib().ksgf_host_stepper_get_phi(self.h, _dp(out)), "get_phi")
         return out
+
+    def write_plotfile(self, path, time=0.0, step=0):
+        """plotfile of velocity, gp, density (and the scalar when enabled)"""
+        _chk(host_lib().ksgf_host_stepper_write_plotfile(self.h, str(path).encode(), time, step), "write_plotfile")
 
     def close(self):
         if self.h:
