@@ -1550,3 +1550,592 @@ void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const do
     }
     free(fn); free(fo); free(fro); free(frn); free(fc);
 }
+
+/* ------------------------------------------------------------------ */
+/* Nodal projection (incflo::ApplyProjection ->                        */
+/* Hydro::NodalProjector / MLNodeLaplacian with a constant sigma,      */
+/* src/projection/incflo_apply_nodal_projection.cpp:149-465)           */
+/*                                                                     */
+/* Unknowns live on the nodes of the cell grid: N_d = n_d nodes in a   */
+/* periodic direction (node n_d is node 0), n_d + 1 otherwise.  The    */
+/* operator is the trilinear (Q1) finite-element stiffness matrix      */
+/* divided by the cell volume, with the sign of div(sigma grad):       */
+/* a 27-point stencil (nodal_weights below).  A Neumann boundary node  */
+/* uses the full stencil with mirrored neighbours (node -1 = node 1),  */
+/* which is twice its finite-element row; its right-hand side is       */
+/* doubled to match, once per Neumann plane the node lies on.          */
+/* Dirichlet (pressure_outflow) boundary nodes hold phi = 0.           */
+/* Smoother: weighted Jacobi, four sweeps per smoothing step, with     */
+/* omega = 1 / lambda_max(D^-1 A) (2/3 on a cubic grid; smaller on     */
+/* stretched cells, where 2/3 would amplify the stiff direction's      */
+/* checkerboard); restriction: full weighting (the transpose of trilinear       */
+/* interpolation over the coarse/fine volume ratio, mirrored at        */
+/* Neumann sides), prolongation: trilinear interpolation -- the        */
+/* Galerkin transfers of nested Q1 spaces, so the rediscretised coarse */
+/* stencil equals the Galerkin coarse operator.  Bottom: BiCGStab.     */
+/* Parity unpinned against AMReX (absent): the smoother (its weight    */
+/* and sweep count), the offset that makes a singular right-hand side  */
+/* solvable (weighted mean, weights 2^-#Neumann planes), and the       */
+/* coarsening rule (the MAC rule above).                               */
+/* ------------------------------------------------------------------ */
+#define ND_SWEEPS 4
+
+typedef struct {
+    int n[3];  /* cells */
+    int N[3];  /* nodes */
+    int bclo[3], bchi[3];
+    int r[3];  /* coarsening ratio to the next level */
+    double h[3];
+    double w[8]; /* stencil weights: centre, face x/y/z, edge xy/xz/yz, corner */
+    double wd;   /* omega / centre */
+    long tot;
+    double *cor, *res, *tmp;
+} nlev_t;
+
+static void nodal_weights(double sigma, const double* h, double* w)
+{
+    const double ax = sigma / (h[0] * h[0]), ay = sigma / (h[1] * h[1]), az = sigma / (h[2] * h[2]);
+    const double s = (ax + ay) + az;
+    w[0] = -(8.0 / 9.0) * s;
+    w[1] = (2.0 / 9.0) * ((2.0 * ax - ay) - az);
+    w[2] = (2.0 / 9.0) * ((2.0 * ay - ax) - az);
+    w[3] = (2.0 / 9.0) * ((2.0 * az - ax) - ay);
+    w[4] = (ax + ay) / 9.0 - az / 18.0;
+    w[5] = (ax + az) / 9.0 - ay / 18.0;
+    w[6] = (ay + az) / 9.0 - ax / 18.0;
+    w[7] = s / 36.0;
+}
+
+/* Jacobi weight 1 / lambda_max of D^-1 A.  The symbol of the stencil is
+ * multilinear in (cos tx, cos ty, cos tz), so its extremes sit at the
+ * corners of {0, pi}^3. */
+static double nodal_omega(const double* w)
+{
+    double lmax = 0.0;
+    for (int m = 1; m < 8; ++m) {
+        const double cx = (m & 1) ? -1.0 : 1.0, cy = (m & 2) ? -1.0 : 1.0, cz = (m & 4) ? -1.0 : 1.0;
+        const double sym = ((w[0] + 2.0 * (w[1] * cx + w[2] * cy + w[3] * cz)) +
+                            4.0 * (w[4] * cx * cy + w[5] * cx * cz + w[6] * cy * cz)) +
+                           8.0 * w[7] * cx * cy * cz;
+        const double lam = sym / w[0];
+        if (lam > lmax) lmax = lam;
+    }
+    return 1.0 / lmax;
+}
+
+/* neighbour node index along one direction: periodic wrap, mirror at a
+ * non-periodic side (only a boundary node has an outside neighbour) */
+static inline int nb_idx(int i, int s, int N, int per)
+{
+    int m = i + s;
+    if (per) return m < 0 ? m + N : (m >= N ? m - N : m);
+    return m < 0 ? 1 : (m >= N ? N - 2 : m);
+}
+
+static inline int nd_dir_node(const nlev_t* L, int d, int i)
+{
+    return (L->bclo[d] == KO_LO_DIRICHLET && i == 0) || (L->bchi[d] == KO_LO_DIRICHLET && i == L->N[d] - 1);
+}
+static inline int nd_dirichlet(const nlev_t* L, int i, int j, int k)
+{
+    return nd_dir_node(L, 0, i) || nd_dir_node(L, 1, j) || nd_dir_node(L, 2, k);
+}
+static inline long NI(const nlev_t* L, int i, int j, int k)
+{
+    return (long)i + (long)L->N[0] * ((long)j + (long)L->N[1] * k);
+}
+
+/* (L x) at node (i,j,k); grouping fixed (shared with the HIP kernels) */
+static double nd_apply_at(const nlev_t* L, const double* x, int i, int j, int k)
+{
+    const int p0 = L->bclo[0] == KO_LO_PERIODIC, p1 = L->bclo[1] == KO_LO_PERIODIC,
+              p2 = L->bclo[2] == KO_LO_PERIODIC;
+    const int im = nb_idx(i, -1, L->N[0], p0), ip = nb_idx(i, 1, L->N[0], p0);
+    const int jm = nb_idx(j, -1, L->N[1], p1), jp = nb_idx(j, 1, L->N[1], p1);
+    const int km = nb_idx(k, -1, L->N[2], p2), kp = nb_idx(k, 1, L->N[2], p2);
+#define X(a, b, c) x[NI(L, a, b, c)]
+    const double fx = X(im, j, k) + X(ip, j, k);
+    const double fy = X(i, jm, k) + X(i, jp, k);
+    const double fz = X(i, j, km) + X(i, j, kp);
+    const double exy = (X(im, jm, k) + X(ip, jm, k)) + (X(im, jp, k) + X(ip, jp, k));
+    const double exz = (X(im, j, km) + X(ip, j, km)) + (X(im, j, kp) + X(ip, j, kp));
+    const double eyz = (X(i, jm, km) + X(i, jp, km)) + (X(i, jm, kp) + X(i, jp, kp));
+    const double cr = ((X(im, jm, km) + X(ip, jm, km)) + (X(im, jp, km) + X(ip, jp, km))) +
+                      ((X(im, jm, kp) + X(ip, jm, kp)) + (X(im, jp, kp) + X(ip, jp, kp)));
+#undef X
+    const double* w = L->w;
+    return (((w[0] * x[NI(L, i, j, k)] + w[1] * fx) + (w[2] * fy + w[3] * fz)) +
+            ((w[4] * exy + w[5] * exz) + (w[6] * eyz + w[7] * cr)));
+}
+
+static void nd_jacobi(const nlev_t* L, double* x, const double* b, double* t, int nsweep)
+{
+    for (int s = 0; s < nsweep; ++s) {
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < L->N[2]; ++k)
+            for (int j = 0; j < L->N[1]; ++j)
+                for (int i = 0; i < L->N[0]; ++i) {
+                    long c = NI(L, i, j, k);
+                    if (nd_dirichlet(L, i, j, k)) { t[c] = 0.0; continue; }
+                    t[c] = x[c] + (b[c] - nd_apply_at(L, x, i, j, k)) * L->wd;
+                }
+        memcpy(x, t, sizeof(double) * (size_t)L->tot);
+    }
+}
+
+static double nd_residual(const nlev_t* L, const double* x, const double* b, double* r)
+{
+    double mx = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+    for (int k = 0; k < L->N[2]; ++k)
+        for (int j = 0; j < L->N[1]; ++j)
+            for (int i = 0; i < L->N[0]; ++i) {
+                long c = NI(L, i, j, k);
+                double v = nd_dirichlet(L, i, j, k) ? 0.0 : b[c] - nd_apply_at(L, x, i, j, k);
+                r[c] = v;
+                mx = fmax(mx, abs_or_inf(v));
+            }
+    return mx;
+}
+
+static void nd_apply(const nlev_t* L, const double* x, double* y)
+{
+    for (int k = 0; k < L->N[2]; ++k)
+        for (int j = 0; j < L->N[1]; ++j)
+            for (int i = 0; i < L->N[0]; ++i)
+                y[NI(L, i, j, k)] = nd_dirichlet(L, i, j, k) ? 0.0 : nd_apply_at(L, x, i, j, k);
+}
+
+/* weight of a node in the solvability condition: 2^-(Neumann planes) */
+static double nd_weight(const nlev_t* L, int i, int j, int k)
+{
+    double w = 1.0;
+    const int id[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) {
+        if (L->bclo[d] == KO_LO_NEUMANN && id[d] == 0) w *= 0.5;
+        if (L->bchi[d] == KO_LO_NEUMANN && id[d] == L->N[d] - 1) w *= 0.5;
+    }
+    return w;
+}
+
+static void nd_sub_mean(const nlev_t* L, double* a)
+{
+    double s = 0.0, sw = 0.0;
+    for (int k = 0; k < L->N[2]; ++k)
+        for (int j = 0; j < L->N[1]; ++j)
+            for (int i = 0; i < L->N[0]; ++i) {
+                double w = nd_weight(L, i, j, k);
+                s += w * a[NI(L, i, j, k)];
+                sw += w;
+            }
+    const double m = s / sw;
+    for (long c = 0; c < L->tot; ++c) a[c] -= m;
+}
+
+static double nd_dot(const nlev_t* L, const double* a, const double* b)
+{
+    double s = 0.0;
+    for (long c = 0; c < L->tot; ++c) s += a[c] * b[c];
+    return s;
+}
+static double nd_norminf(const nlev_t* L, const double* a)
+{
+    double m = 0.0;
+    for (long c = 0; c < L->tot; ++c) m = fmax(m, abs_or_inf(a[c]));
+    return m;
+}
+
+/* unpreconditioned BiCGStab on the bottom level (the steps of bicgstab above) */
+static int nd_bicgstab(const nlev_t* L, double* x, const double* b, double rtol, double atol, int maxiter,
+                       int* iters)
+{
+    long T = L->tot;
+    double* r = (double*)calloc((size_t)T, sizeof(double));
+    double* rh = (double*)calloc((size_t)T, sizeof(double));
+    double* p = (double*)calloc((size_t)T, sizeof(double));
+    double* v = (double*)calloc((size_t)T, sizeof(double));
+    double* s = (double*)calloc((size_t)T, sizeof(double));
+    double* t = (double*)calloc((size_t)T, sizeof(double));
+    for (long c = 0; c < T; ++c) { x[c] = 0.0; r[c] = b[c]; rh[c] = b[c]; }
+    double rnorm0 = nd_norminf(L, r);
+    double tol = fmax(rtol * rnorm0, atol);
+    int ret = 1, it = 0;
+    *iters = 0;
+    if (rnorm0 == 0.0 || rnorm0 <= tol) { ret = 0; goto done; }
+    double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+    for (it = 1; it <= maxiter; ++it) {
+        double rho = nd_dot(L, rh, r);
+        if (rho == 0.0) { ret = 2; break; }
+        if (it == 1) {
+            for (long c = 0; c < T; ++c) p[c] = r[c];
+        } else {
+            double beta = (rho / rho_prev) * (alpha / omega);
+            for (long c = 0; c < T; ++c) p[c] = r[c] + beta * (p[c] - omega * v[c]);
+        }
+        nd_apply(L, p, v);
+        double rv = nd_dot(L, rh, v);
+        if (rv == 0.0) { ret = 2; break; }
+        alpha = rho / rv;
+        for (long c = 0; c < T; ++c) s[c] = r[c] - alpha * v[c];
+        if (nd_norminf(L, s) <= tol) {
+            for (long c = 0; c < T; ++c) x[c] += alpha * p[c];
+            ret = 0;
+            break;
+        }
+        nd_apply(L, s, t);
+        double tt = nd_dot(L, t, t);
+        if (tt == 0.0) { ret = 2; break; }
+        omega = nd_dot(L, t, s) / tt;
+        for (long c = 0; c < T; ++c) {
+            x[c] += alpha * p[c] + omega * s[c];
+            r[c] = s[c] - omega * t[c];
+        }
+        if (nd_norminf(L, r) <= tol) { ret = 0; break; }
+        if (omega == 0.0) { ret = 2; break; }
+        rho_prev = rho;
+    }
+    if (it > maxiter) it = maxiter;
+done:
+    *iters = it;
+    free(r); free(rh); free(p); free(v); free(s); free(t);
+    return ret;
+}
+
+/* full weighting: weights (1/4, 1/2, 1/4) along each coarsened direction */
+static void nd_restrict(const nlev_t* F, const nlev_t* C, const double* f, double* c)
+{
+    const int per[3] = {F->bclo[0] == KO_LO_PERIODIC, F->bclo[1] == KO_LO_PERIODIC, F->bclo[2] == KO_LO_PERIODIC};
+#pragma omp parallel for schedule(static)
+    for (int K = 0; K < C->N[2]; ++K)
+        for (int J = 0; J < C->N[1]; ++J)
+            for (int I = 0; I < C->N[0]; ++I) {
+                long o = NI(C, I, J, K);
+                if (nd_dirichlet(C, I, J, K)) { c[o] = 0.0; continue; }
+                const int ci[3] = {I, J, K};
+                int idx[3][3];
+                double wt[3][3];
+                int cnt[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (F->r[d] == 2) {
+                        const int fi = 2 * ci[d];
+                        idx[d][0] = nb_idx(fi, -1, F->N[d], per[d]);
+                        idx[d][1] = fi;
+                        idx[d][2] = nb_idx(fi, 1, F->N[d], per[d]);
+                        wt[d][0] = 0.25; wt[d][1] = 0.5; wt[d][2] = 0.25;
+                        cnt[d] = 3;
+                    } else {
+                        idx[d][0] = ci[d];
+                        wt[d][0] = 1.0;
+                        cnt[d] = 1;
+                    }
+                }
+                double s = 0.0;
+                for (int a = 0; a < cnt[2]; ++a) {
+                    double sy = 0.0;
+                    for (int b = 0; b < cnt[1]; ++b) {
+                        double sx = 0.0;
+                        for (int e = 0; e < cnt[0]; ++e) sx += wt[0][e] * f[NI(F, idx[0][e], idx[1][b], idx[2][a])];
+                        sy += wt[1][b] * sx;
+                    }
+                    s += wt[2][a] * sy;
+                }
+                c[o] = s;
+            }
+}
+
+/* trilinear interpolation of the coarse correction, added to the fine one */
+static void nd_prolong_add(const nlev_t* F, const nlev_t* C, double* f, const double* c)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < F->N[2]; ++k)
+        for (int j = 0; j < F->N[1]; ++j)
+            for (int i = 0; i < F->N[0]; ++i) {
+                if (nd_dirichlet(F, i, j, k)) continue;
+                const int fi[3] = {i, j, k};
+                int lo[3], hi[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (F->r[d] == 2) {
+                        lo[d] = fi[d] >> 1;
+                        hi[d] = (fi[d] & 1) ? ((lo[d] + 1 == C->N[d]) ? 0 : lo[d] + 1) : lo[d];
+                    } else {
+                        lo[d] = hi[d] = fi[d];
+                    }
+                }
+                double v;
+                const int ox = lo[0] != hi[0], oy = lo[1] != hi[1], oz = lo[2] != hi[2];
+                double z0, z1;
+                {
+                    double a = c[NI(C, lo[0], lo[1], lo[2])], b = c[NI(C, hi[0], lo[1], lo[2])];
+                    double e = c[NI(C, lo[0], hi[1], lo[2])], g = c[NI(C, hi[0], hi[1], lo[2])];
+                    double y0 = ox ? 0.5 * (a + b) : a, y1 = ox ? 0.5 * (e + g) : e;
+                    z0 = oy ? 0.5 * (y0 + y1) : y0;
+                }
+                {
+                    double a = c[NI(C, lo[0], lo[1], hi[2])], b = c[NI(C, hi[0], lo[1], hi[2])];
+                    double e = c[NI(C, lo[0], hi[1], hi[2])], g = c[NI(C, hi[0], hi[1], hi[2])];
+                    double y0 = ox ? 0.5 * (a + b) : a, y1 = ox ? 0.5 * (e + g) : e;
+                    z1 = oy ? 0.5 * (y0 + y1) : y0;
+                }
+                v = oz ? 0.5 * (z0 + z1) : z0;
+                f[NI(F, i, j, k)] += v;
+            }
+}
+
+typedef struct {
+    int nlev;
+    nlev_t L[MAXLEV];
+    const ko_mlmg_opts* o;
+    int singular;
+    int bottom_iters, bottom_fail;
+} nmg_t;
+
+static void nd_vcycle(nmg_t* M, int l)
+{
+    nlev_t* L = &M->L[l];
+    for (long c = 0; c < L->tot; ++c) L->cor[c] = 0.0;
+    if (l == M->nlev - 1) {
+        if (M->singular) nd_sub_mean(L, L->res);
+        int it = 0;
+        int ret = nd_bicgstab(L, L->cor, L->res, M->o->bottom_rtol, M->o->bottom_atol, M->o->bottom_maxiter, &it);
+        M->bottom_iters += it;
+        if (ret != 0) M->bottom_fail += 1;
+        nd_jacobi(L, L->cor, L->res, L->tmp, ND_SWEEPS * (ret == 0 ? M->o->nub : M->o->nuf));
+        return;
+    }
+    nd_jacobi(L, L->cor, L->res, L->tmp, ND_SWEEPS * M->o->nu1);
+    nd_residual(L, L->cor, L->res, L->tmp);
+    nd_restrict(L, &M->L[l + 1], L->tmp, M->L[l + 1].res);
+    nd_vcycle(M, l + 1);
+    nd_prolong_add(L, &M->L[l + 1], L->cor, M->L[l + 1].cor);
+    nd_jacobi(L, L->cor, L->res, L->tmp, ND_SWEEPS * M->o->nu2);
+}
+
+static void nd_build(nmg_t* M, const int* n, const double* dx, const int* lobc, const int* hibc, double sigma)
+{
+    const ko_mlmg_opts* o = M->o;
+    int nl = 0;
+    int cn[3] = {n[0], n[1], n[2]};
+    double ch[3] = {dx[0], dx[1], dx[2]};
+    for (;;) {
+        nlev_t* L = &M->L[nl];
+        for (int d = 0; d < 3; ++d) {
+            L->n[d] = cn[d];
+            L->N[d] = (lobc[d] == KO_LO_PERIODIC) ? cn[d] : cn[d] + 1;
+            L->bclo[d] = lobc[d];
+            L->bchi[d] = hibc[d];
+            L->h[d] = ch[d];
+            L->r[d] = 1;
+        }
+        nodal_weights(sigma, ch, L->w);
+        L->wd = nodal_omega(L->w) / L->w[0];
+        L->tot = (long)L->N[0] * L->N[1] * L->N[2];
+        ++nl;
+        if (nl >= MAXLEV || nl > o->max_coarsening_level) break;
+        int can[3], any = 0;
+        for (int d = 0; d < 3; ++d) can[d] = (cn[d] % 2 == 0 && cn[d] / 2 >= 2);
+        if (!o->semicoarsening) {
+            if (!(can[0] && can[1] && can[2])) break;
+            for (int d = 0; d < 3; ++d) L->r[d] = 2;
+            any = 1;
+        } else {
+            double hmin = 1e300;
+            for (int d = 0; d < 3; ++d) if (ch[d] < hmin) hmin = ch[d];
+            for (int d = 0; d < 3; ++d)
+                if (can[d] && ch[d] <= 1.5 * hmin) { L->r[d] = 2; any = 1; }
+            if (!any) break;
+        }
+        for (int d = 0; d < 3; ++d) { cn[d] /= L->r[d]; ch[d] *= L->r[d]; }
+    }
+    M->nlev = nl;
+    for (int l = 0; l < nl; ++l) {
+        nlev_t* L = &M->L[l];
+        L->cor = (double*)calloc((size_t)L->tot, sizeof(double));
+        L->res = (double*)calloc((size_t)L->tot, sizeof(double));
+        L->tmp = (double*)calloc((size_t)L->tot, sizeof(double));
+    }
+}
+
+static void nd_free(nmg_t* M)
+{
+    for (int l = 0; l < M->nlev; ++l) {
+        free(M->L[l].cor);
+        free(M->L[l].res);
+        free(M->L[l].tmp);
+    }
+}
+
+int ko_nodal_dims(const int* n, const int* lobc, int* N)
+{
+    for (int d = 0; d < 3; ++d) N[d] = (lobc[d] == KO_LO_PERIODIC) ? n[d] : n[d] + 1;
+    return 0;
+}
+
+int ko_nodal_solve(const int* n, const double* dx, const int* lobc, const int* hibc, double sigma,
+                   const double* rhs_in, double* phi, double rtol, double atol, const ko_mlmg_opts* opts,
+                   ko_mlmg_info* info)
+{
+    nmg_t* M = (nmg_t*)calloc(1, sizeof(nmg_t));
+    M->o = opts;
+    M->singular = 1;
+    for (int d = 0; d < 3; ++d)
+        if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) M->singular = 0;
+    nd_build(M, n, dx, lobc, hibc, sigma);
+    nlev_t* L0 = &M->L[0];
+    double* rhs = (double*)calloc((size_t)L0->tot, sizeof(double));
+    for (int k = 0; k < L0->N[2]; ++k)
+        for (int j = 0; j < L0->N[1]; ++j)
+            for (int i = 0; i < L0->N[0]; ++i) {
+                long c = NI(L0, i, j, k);
+                rhs[c] = nd_dirichlet(L0, i, j, k) ? 0.0 : rhs_in[c];
+            }
+    if (M->singular) nd_sub_mean(L0, rhs);
+    for (long c = 0; c < L0->tot; ++c) phi[c] = 0.0;
+    double rhsnorm = nd_norminf(L0, rhs);
+    for (long c = 0; c < L0->tot; ++c) L0->res[c] = rhs[c];
+    double r = rhsnorm;
+    double target = fmax(atol, rtol * rhsnorm);
+    info->r0 = r;
+    info->iters = 0;
+    info->converged = (r <= target);
+    info->nlevels = M->nlev;
+    int diverged = 0;
+    M->bottom_iters = 0;
+    M->bottom_fail = 0;
+    if (!info->converged) {
+        for (int it = 1; it <= opts->maxiter; ++it) {
+            nd_vcycle(M, 0);
+            for (long c = 0; c < L0->tot; ++c) phi[c] += L0->cor[c];
+            r = nd_residual(L0, phi, rhs, L0->res);
+            info->iters = it;
+            if (r <= target) { info->converged = 1; break; }
+            if (!(r <= KO_MG_DIVERGE * rhsnorm)) { diverged = 1; break; }
+        }
+    }
+    info->rfinal = r;
+    info->bottom_iters = M->bottom_iters;
+    info->bottom_fail = M->bottom_fail;
+    free(rhs);
+    nd_free(M);
+    free(M);
+    return diverged ? KO_MG_DIVERGED : info->converged ? 0 : 1;
+}
+
+/* nodal divergence of the cell velocity (vel: cell MultiFab, 3 comp, ng >= 1,
+ * ghosts as ApplyProjection leaves them: zero on physical boundaries,
+ * interior and periodic ghosts filled); doubled once per Neumann plane */
+void ko_nodal_rhs(const ko_geom* g, const double* vel, int ng, const int* lobc, const int* hibc, double* rhs)
+{
+    const int NB = g->nbox;
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    make_fabs(g, KO_CELL, 3, ng, f);
+    int n[3], N[3];
+    for (int d = 0; d < 3; ++d) n[d] = g->dhi[d] - g->dlo[d] + 1;
+    ko_nodal_dims(n, lobc, N);
+    const double fx = 0.25 / g->dx[0], fy = 0.25 / g->dx[1], fz = 0.25 / g->dx[2];
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const fab_t* fb = &f[b];
+        const long cs = FS(fb);
+        int hi[3];
+        for (int d = 0; d < 3; ++d) hi[d] = bx[3 + d] + ((lobc[d] != KO_LO_PERIODIC && bx[3 + d] == g->dhi[d]) ? 1 : 0);
+        for (int k = bx[2]; k <= hi[2]; ++k)
+            for (int j = bx[1]; j <= hi[1]; ++j)
+                for (int i = bx[0]; i <= hi[0]; ++i) {
+                    /* the 8 cells around node (i,j,k): (i-1|i, j-1|j, k-1|k) */
+                    long c000 = FI(fb, i - 1, j - 1, k - 1);
+                    const long sy = fb->n[0], sz = (long)fb->n[0] * fb->n[1];
+                    const double* u = vel + c000;
+                    const double* v = vel + c000 + cs;
+                    const double* w = vel + c000 + 2 * cs;
+#define A(p, a, bb, c) p[(a) + sy * (bb) + sz * (c)]
+                    double dux = ((A(u, 1, 0, 0) - A(u, 0, 0, 0)) + (A(u, 1, 1, 0) - A(u, 0, 1, 0))) +
+                                 ((A(u, 1, 0, 1) - A(u, 0, 0, 1)) + (A(u, 1, 1, 1) - A(u, 0, 1, 1)));
+                    double dvy = ((A(v, 0, 1, 0) - A(v, 0, 0, 0)) + (A(v, 1, 1, 0) - A(v, 1, 0, 0))) +
+                                 ((A(v, 0, 1, 1) - A(v, 0, 0, 1)) + (A(v, 1, 1, 1) - A(v, 1, 0, 1)));
+                    double dwz = ((A(w, 0, 0, 1) - A(w, 0, 0, 0)) + (A(w, 1, 0, 1) - A(w, 1, 0, 0))) +
+                                 ((A(w, 0, 1, 1) - A(w, 0, 1, 0)) + (A(w, 1, 1, 1) - A(w, 1, 1, 0)));
+#undef A
+                    double dv = (dux * fx + dvy * fy) + dwz * fz;
+                    const int id[3] = {i - g->dlo[0], j - g->dlo[1], k - g->dlo[2]};
+                    for (int d = 0; d < 3; ++d) {
+                        if (lobc[d] == KO_LO_NEUMANN && id[d] == 0) dv *= 2.0;
+                        if (hibc[d] == KO_LO_NEUMANN && id[d] == N[d] - 1) dv *= 2.0;
+                    }
+                    rhs[id[0] + (long)N[0] * (id[1] + (long)N[1] * id[2])] = dv;
+                }
+    }
+    free(f);
+}
+
+/* vel -= sigma grad phi on valid cells; gp (cell, 3 comp, ngrow 0, may be
+ * NULL) = grad phi: the average of the four node differences per direction */
+void ko_nodal_correct(const ko_geom* g, double* vel, int ng, double* gp, const double* phi, const int* lobc,
+                      double sigma)
+{
+    const int NB = g->nbox;
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    fab_t* fg = (fab_t*)malloc(sizeof(fab_t) * (size_t)NB);
+    make_fabs(g, KO_CELL, 3, ng, f);
+    make_fabs(g, KO_CELL, 3, 0, fg);
+    int n[3], N[3];
+    for (int d = 0; d < 3; ++d) n[d] = g->dhi[d] - g->dlo[d] + 1;
+    ko_nodal_dims(n, lobc, N);
+    const double fx = 0.25 / g->dx[0], fy = 0.25 / g->dx[1], fz = 0.25 / g->dx[2];
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < NB; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        for (int k = bx[2]; k <= bx[5]; ++k)
+            for (int j = bx[1]; j <= bx[4]; ++j)
+                for (int i = bx[0]; i <= bx[3]; ++i) {
+                    const int a0 = i - g->dlo[0], b0 = j - g->dlo[1], c0 = k - g->dlo[2];
+                    const int a1 = (a0 + 1 == N[0]) ? 0 : a0 + 1;
+                    const int b1 = (b0 + 1 == N[1]) ? 0 : b0 + 1;
+                    const int c1 = (c0 + 1 == N[2]) ? 0 : c0 + 1;
+#define P(a, bb, c) phi[(a) + (long)N[0] * ((bb) + (long)N[1] * (c))]
+                    double gx = ((P(a1, b0, c0) - P(a0, b0, c0)) + (P(a1, b1, c0) - P(a0, b1, c0))) +
+                                ((P(a1, b0, c1) - P(a0, b0, c1)) + (P(a1, b1, c1) - P(a0, b1, c1)));
+                    double gy = ((P(a0, b1, c0) - P(a0, b0, c0)) + (P(a1, b1, c0) - P(a1, b0, c0))) +
+                                ((P(a0, b1, c1) - P(a0, b0, c1)) + (P(a1, b1, c1) - P(a1, b0, c1)));
+                    double gz = ((P(a0, b0, c1) - P(a0, b0, c0)) + (P(a1, b0, c1) - P(a1, b0, c0))) +
+                                ((P(a0, b1, c1) - P(a0, b1, c0)) + (P(a1, b1, c1) - P(a1, b1, c0)));
+#undef P
+                    gx *= fx;
+                    gy *= fy;
+                    gz *= fz;
+                    long c = FI(&f[b], i, j, k), cs = FS(&f[b]);
+                    vel[c] = vel[c] - sigma * gx;
+                    vel[c + cs] = vel[c + cs] - sigma * gy;
+                    vel[c + 2 * cs] = vel[c + 2 * cs] - sigma * gz;
+                    if (gp) {
+                        long o = FI(&fg[b], i, j, k), os = FS(&fg[b]);
+                        gp[o] = gx;
+                        gp[o + os] = gy;
+                        gp[o + 2 * os] = gz;
+                    }
+                }
+    }
+    free(f);
+    free(fg);
+}
+
+/* NodalProjector::project with a constant sigma: rhs, solve, correction.
+ * phi_out: domain-wide nodes (may be NULL); gp_out as in ko_nodal_correct */
+int ko_nodal_project(const ko_geom* g, double* vel, int ng, const int* lobc, const int* hibc, double sigma,
+                     double rtol, double atol, const ko_mlmg_opts* opts, ko_mlmg_info* info, double* phi_out,
+                     double* gp_out)
+{
+    int n[3], N[3];
+    for (int d = 0; d < 3; ++d) n[d] = g->dhi[d] - g->dlo[d] + 1;
+    ko_nodal_dims(n, lobc, N);
+    long T = (long)N[0] * N[1] * N[2];
+    double* rhs = (double*)calloc((size_t)T, sizeof(double));
+    double* phi = (double*)calloc((size_t)T, sizeof(double));
+    ko_nodal_rhs(g, vel, ng, lobc, hibc, rhs);
+    int ret = ko_nodal_solve(n, g->dx, lobc, hibc, sigma, rhs, phi, rtol, atol, opts, info);
+    ko_nodal_correct(g, vel, ng, gp_out, phi, lobc, sigma);
+    if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)T);
+    free(rhs);
+    free(phi);
+    return ret;
+}

@@ -117,6 +117,22 @@ int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int*
                      double beta, const double* rhs, double* phi, double rtol, double atol,
                      const ko_mlmg_opts* opts, ko_mlmg_info* info);
 
+/* Nodal projection with a constant sigma (NodalProjector / MLNodeLaplacian;
+ * src/projection/incflo_apply_nodal_projection.cpp:149-465).  Node arrays
+ * are domain-wide, Fortran order, N_d = n_d nodes in a periodic direction and
+ * n_d + 1 otherwise (ko_nodal_dims). */
+int ko_nodal_dims(const int* n, const int* lobc, int* N);
+void ko_nodal_rhs(const ko_geom* g, const double* vel, int ng, const int* lobc, const int* hibc,
+                  double* rhs);
+int ko_nodal_solve(const int* n, const double* dx, const int* lobc, const int* hibc, double sigma,
+                   const double* rhs, double* phi, double rtol, double atol,
+                   const ko_mlmg_opts* opts, ko_mlmg_info* info);
+void ko_nodal_correct(const ko_geom* g, double* vel, int ng, double* gp, const double* phi,
+                      const int* lobc, double sigma);
+int ko_nodal_project(const ko_geom* g, double* vel, int ng, const int* lobc, const int* hibc,
+                     double sigma, double rtol, double atol, const ko_mlmg_opts* opts,
+                     ko_mlmg_info* info, double* phi_out, double* gp_out);
+
 double ko_cfl_max(const ko_geom* g, const double* vel, int ng);
 
 void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,

@@ -85,6 +85,13 @@ def lib():
                                      C.POINTER(KoOpts), C.POINTER(KoInfo), _dp]
         L.ko_poisson_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, _dp, C.c_double, C.c_double,
                                        C.POINTER(KoOpts), C.POINTER(KoInfo)]
+        L.ko_nodal_dims.argtypes = [_ip, _ip, _ip]
+        L.ko_nodal_rhs.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _ip, _ip, _dp]
+        L.ko_nodal_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, _dp, C.c_double, C.c_double,
+                                     C.POINTER(KoOpts), C.POINTER(KoInfo)]
+        L.ko_nodal_correct.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, _dp, _ip, C.c_double]
+        L.ko_nodal_project.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _ip, _ip, C.c_double, C.c_double,
+                                       C.c_double, C.POINTER(KoOpts), C.POINTER(KoInfo), _dp, _dp]
         L.ko_cfl_max.restype = C.c_double
         L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
         L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
@@ -220,6 +227,55 @@ def poisson_solve(n, dx, lobc, hibc, beta, rhs, rtol, atol, opts=None):
     st = lib().ko_poisson_solve(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip),
                                 beta, _p(rhs), _p(phi), rtol, atol, C.byref(opts), C.byref(info))
     return phi, dict(info.as_dict(), status=st)
+
+
+def _iarr(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def nodal_dims(n, lobc):
+    return tuple(n[d] if lobc[d] == 0 else n[d] + 1 for d in range(3))
+
+
+def nodal_rhs(G, vel, ng, lobc, hibc):
+    """nodal divergence of a cell velocity MultiFab (ghosts as ApplyProjection
+    leaves them), domain-wide nodes, Fortran order"""
+    N = nodal_dims(G.ba.geom.n_cell, lobc)
+    rhs = np.zeros(int(np.prod(N)))
+    lo, plo = _iarr(lobc)
+    hi, phi_ = _iarr(hibc)
+    lib().ko_nodal_rhs(G.ref, _p(vel), ng, plo, phi_, _p(rhs))
+    return rhs
+
+
+def nodal_solve(n, dx, lobc, hibc, sigma, rhs, rtol, atol, opts=None):
+    opts = opts or default_opts()
+    info = KoInfo()
+    nn, pn = _iarr(n)
+    dd = np.ascontiguousarray(dx, dtype=np.float64)
+    lo, plo = _iarr(lobc)
+    hi, phi_ = _iarr(hibc)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64).ravel()
+    phi = np.zeros_like(rhs)
+    st = lib().ko_nodal_solve(pn, _p(dd), plo, phi_, sigma, _p(rhs), _p(phi), rtol, atol, C.byref(opts),
+                              C.byref(info))
+    return phi, dict(info.as_dict(), status=st)
+
+
+def nodal_project(G, vel, ng, lobc, hibc, sigma, rtol, atol, opts=None):
+    """NodalProjector::project with a constant sigma; vel corrected in place.
+    Returns (info, phi (domain-wide nodes), gradphi (cell MultiFab, 3 comp, ngrow 0))"""
+    opts = opts or default_opts()
+    info = KoInfo()
+    N = nodal_dims(G.ba.geom.n_cell, lobc)
+    phi = np.zeros(int(np.prod(N)))
+    gp = np.zeros(G.ba.mf_size(CELL, 3, 0))
+    lo, plo = _iarr(lobc)
+    hi, phi_ = _iarr(hibc)
+    st = lib().ko_nodal_project(G.ref, _p(vel), ng, plo, phi_, sigma, rtol, atol, C.byref(opts), C.byref(info),
+                                _p(phi), _p(gp))
+    return dict(info.as_dict(), status=st), phi, gp
 
 
 def cfl_max(G, vel, ng):
