@@ -251,6 +251,41 @@ int ksgf_poisson_batch_solve(ksgf_poisson_batch* pb, const double* const* rhs,
                              ksgf_mlmg_info* infos, ksgf_stream_t stream);
 int ksgf_poisson_batch_destroy(ksgf_poisson_batch* pb);
 
+/* Hydro::NodalProjector with a constant sigma (incflo::ApplyProjection,
+ * src/projection/incflo_apply_nodal_projection.cpp:340-353 for the constant-
+ * density branch, options from MLMGOptions::operator()(NodalProjector&),
+ * src/core/MLMGOptions.cpp:118-130): create = NodalProjector(vel,
+ * scaling_factor/rho_0, geom, lpinfo) + setDomainBC(lobc, hibc) (KSGF_LO_*;
+ * inflow gives KSGF_ERR_UNSUPPORTED) + the options; set_sigma changes sigma.
+ * project = project(rtol, atol): vel is a cell MultiFab, 3 comp, ngrow
+ * ng_vel >= 1, whose ghosts are as ApplyProjection leaves them (setBndry(0)
+ * then FillBoundary: zero on physical boundaries, filled elsewhere); the
+ * nodal divergence of vel is solved for and vel -= sigma grad phi on the valid
+ * cells.  gradphi (getGradPhi; cell, 3 comp, ngrow ng_gp, may be NULL)
+ * receives grad phi on the valid cells.  Node arrays (getPhi, solve,
+ * compute_rhs) are domain-wide, Fortran order, N_d = n_d nodes along a
+ * periodic direction and n_d + 1 otherwise (node_dims). */
+typedef struct ksgf_nodalproj ksgf_nodalproj;
+int ksgf_nodalproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc,
+                          double sigma, const ksgf_mlmg_options* opts, ksgf_nodalproj** out);
+int ksgf_nodalproj_set_sigma(ksgf_nodalproj* np, double sigma);
+int ksgf_nodalproj_node_dims(const ksgf_nodalproj* np, int* N);
+int ksgf_nodalproj_project(ksgf_nodalproj* np, double* vel, int ng_vel, double* gradphi, int ng_gp,
+                           double rtol, double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
+int ksgf_nodalproj_get_phi(ksgf_nodalproj* np, double* phi, ksgf_stream_t stream);
+/* NodalProjector::computeRHS: nodal divergence of vel (ghosts as for project) */
+int ksgf_nodalproj_compute_rhs(ksgf_nodalproj* np, const double* vel, int ng_vel, double* rhs,
+                               ksgf_stream_t stream);
+/* the MLMG solve inside project, for a nodal right-hand side */
+int ksgf_nodalproj_solve(ksgf_nodalproj* np, const double* rhs, double* phi, double rtol,
+                         double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
+int ksgf_nodalproj_destroy(ksgf_nodalproj* np);
+
+/* MultiFab::setBndry(val): every stored point outside the valid box of each
+ * FAB (incflo_apply_nodal_projection.cpp:310). */
+int ksgf_set_bndry(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow, double val,
+                   ksgf_stream_t stream);
+
 /* incflo::compute_dt convective part (incflo_compute_dt.cpp:69-91):
  * max over valid cells of max_d |u_d|/dx_d (floor -1).  Synchronises. */
 int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out,

@@ -60,7 +60,9 @@ EXPORTS = [
     "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
     "ksgf_adjoint_create", "ksgf_adjoint_forward", "ksgf_adjoint_backward", "ksgf_adjoint_destroy",
     "ksgf_plotfile_write", "ksgf_plotfile_read", "ksgf_vismf_write", "ksgf_vismf_read",
-    "ksgf_checkpoint_write_header",
+    "ksgf_checkpoint_write_header", "ksgf_nodalproj_create", "ksgf_nodalproj_set_sigma", "ksgf_nodalproj_node_dims",
+    "ksgf_nodalproj_project", "ksgf_nodalproj_get_phi", "ksgf_nodalproj_compute_rhs", "ksgf_nodalproj_solve",
+    "ksgf_nodalproj_destroy", "ksgf_set_bndry",
 ]
 
 _lib = None
@@ -138,6 +140,18 @@ def load(path=None):
     L.ksgf_vismf_read.argtypes = [C.c_char_p, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_checkpoint_write_header.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                                C.c_double]
+    L.ksgf_nodalproj_create.argtypes = [C.c_void_p, _ip, _ip, C.c_double, C.POINTER(KsgfMLMGOptions),
+                                        C.POINTER(C.c_void_p)]
+    L.ksgf_nodalproj_set_sigma.argtypes = [C.c_void_p, C.c_double]
+    L.ksgf_nodalproj_node_dims.argtypes = [C.c_void_p, _ip]
+    L.ksgf_nodalproj_project.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double,
+                                         C.POINTER(KsgfMLMGInfo), C.c_void_p]
+    L.ksgf_nodalproj_get_phi.argtypes = [C.c_void_p, _dp, C.c_void_p]
+    L.ksgf_nodalproj_compute_rhs.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_void_p]
+    L.ksgf_nodalproj_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo),
+                                       C.c_void_p]
+    L.ksgf_nodalproj_destroy.argtypes = [C.c_void_p]
+    L.ksgf_set_bndry.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
     _lib = L
     return L
 
@@ -346,6 +360,9 @@ class DeviceBoxArray:
         check(load().ksgf_checkpoint_write_header(str(path).encode(), self.h, step, time, dt, dt_nm1, dt_nm2),
               "checkpoint_write_header")
 
+    def set_bndry(self, mf, itype, ncomp, ngrow, val=0.0):
+        check(load().ksgf_set_bndry(self.h, _ptr(mf), itype, ncomp, ngrow, val, _stream()), "set_bndry")
+
     def scale_by_rho(self, out, ng_out, a, ng_a, rho, ng_rho, ncomp, ngrow, divide=0):
         check(load().ksgf_scale_by_rho(self.h, _ptr(out), ng_out, _ptr(a), ng_a, _ptr(rho), ng_rho, ncomp, ngrow,
                                        divide, _stream()), "scale_by_rho")
@@ -396,6 +413,70 @@ class MacProjector:
     def close(self):
         if self.h:
             load().ksgf_macproj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NodalProjector:
+    """Hydro::NodalProjector with a constant sigma over a DeviceBoxArray."""
+
+    def __init__(self, dba: DeviceBoxArray, lobc, hibc, sigma=1.0, options=None):
+        import numpy as np
+
+        L = load()
+        self.dba = dba
+        self._lo = np.ascontiguousarray(lobc, dtype=np.int32)
+        self._hi = np.ascontiguousarray(hibc, dtype=np.int32)
+        self.options = options or default_options()
+        h = C.c_void_p()
+        check(L.ksgf_nodalproj_create(dba.h, self._lo.ctypes.data_as(_ip), self._hi.ctypes.data_as(_ip), sigma,
+                                      C.byref(self.options), C.byref(h)), "ksgf_nodalproj_create")
+        self.h = h
+        N = (C.c_int * 3)()
+        check(L.ksgf_nodalproj_node_dims(h, N), "nodalproj_node_dims")
+        self.N = tuple(N)
+        self.info = None
+
+    def set_sigma(self, sigma):
+        check(load().ksgf_nodalproj_set_sigma(self.h, sigma), "nodalproj_set_sigma")
+
+    def project(self, vel, ng_vel, rtol, atol, gradphi=None, ng_gp=0):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_nodalproj_project(self.h, _ptr(vel), ng_vel, _ptr(gradphi), ng_gp, rtol, atol,
+                                            C.byref(info), _stream()), "nodalproj_project", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def compute_rhs(self, vel, ng_vel):
+        import torch
+
+        out = torch.zeros(self.N[0] * self.N[1] * self.N[2], dtype=torch.float64, device="cuda")
+        check(load().ksgf_nodalproj_compute_rhs(self.h, _ptr(vel), ng_vel, _ptr(out), _stream()),
+              "nodalproj_compute_rhs")
+        return out
+
+    def solve(self, rhs, phi, rtol, atol):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_nodalproj_solve(self.h, _ptr(rhs), _ptr(phi), rtol, atol, C.byref(info), _stream()),
+              "nodalproj_solve", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def phi(self):
+        import torch
+
+        out = torch.zeros(self.N[0] * self.N[1] * self.N[2], dtype=torch.float64, device="cuda")
+        check(load().ksgf_nodalproj_get_phi(self.h, _ptr(out), _stream()), "nodalproj_get_phi")
+        return out
+
+    def close(self):
+        if self.h:
+            load().ksgf_nodalproj_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,1114 @@
+// Nodal projection with a constant sigma:
+//   incflo::ApplyProjection -> Hydro::NodalProjector(vel, sigma, geom, lpinfo)
+//   + setDomainBC + MLMGOptions + project(rtol, atol) + getPhi / getGradPhi
+//   (src/projection/incflo_apply_nodal_projection.cpp:149-465,
+//    src/core/MLMGOptions.cpp:118-130).
+//
+// Unknowns are the nodes of the cell grid, one domain-wide array per
+// multigrid level (Fortran order, no ghosts): N_d = n_d nodes in a periodic
+// direction (node n_d is node 0), n_d + 1 otherwise.  Periodic wrap, the
+// Neumann mirror (node -1 reads node 1) and Dirichlet nodes (held at zero)
+// are resolved in the stencil index arithmetic, so no ghost fills run
+// between sweeps.
+//
+// Operator: the trilinear (Q1) finite-element stiffness matrix over the cell
+// volume with the sign of div(sigma grad) -- a 27-point stencil whose eight
+// weights depend only on sigma and the cell sizes.  A Neumann boundary row is
+// the full stencil with mirrored neighbours (twice the finite-element row)
+// and its right-hand side is doubled to match.
+//
+// V-cycle (MLMG defaults nu1 = nu2 = 2): each smoothing step is four weighted
+// Jacobi sweeps with omega = 1/lambda_max(D^-1 A) (2/3 on a cubic grid); the
+// residual is restricted by full weighting (the transpose of trilinear
+// interpolation over the coarse/fine volume ratio) and the coarse correction
+// comes back by trilinear interpolation.  For nested Q1 spaces these are the
+// Galerkin transfers, so the rediscretised coarse stencil is the Galerkin
+// coarse operator.  The coarsest level is solved by unpreconditioned
+// BiCGStab in one workgroup.
+//
+// The stencil, transfer and correction expressions are evaluated in the
+// operand order of the CPU oracle (oracle/ksgf_oracle.c, nodal section) and
+// built without contraction; reductions (norm, weighted mean, BiCGStab dot
+// products) are blocked, so iterates agree with the oracle to rounding.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ksgf_internal.h"
+
+namespace ksgf {
+
+constexpr int kNdSweeps = 4;       // Jacobi sweeps per smoothing step
+constexpr int kNdBottomThreads = 1024;
+constexpr int kNdMaxLev = 40;
+
+struct NLev {
+    int N[3];
+    int per[3];
+    int dlo[3], dhi[3];  // Dirichlet sides
+    int nlo[3], nhi[3];  // Neumann sides
+    int r[3];            // coarsening ratio to the next level
+    double w[8];         // centre, faces x/y/z, edges xy/xz/yz, corners
+    double wd;           // omega / centre
+    int tot;
+};
+
+// neighbour node along one direction: periodic wrap, or the mirror at a
+// non-periodic side (only a boundary node has a neighbour outside)
+__device__ __forceinline__ int nd_nb(int i, int s, int N, int per)
+{
+    int m = i + s;
+    if (per) return m < 0 ? m + N : (m >= N ? m - N : m);
+    return m < 0 ? 1 : (m >= N ? N - 2 : m);
+}
+
+__device__ __forceinline__ bool nd_dirichlet(const NLev& L, int i, int j, int k)
+{
+    return (L.dlo[0] && i == 0) || (L.dhi[0] && i == L.N[0] - 1) || (L.dlo[1] && j == 0) ||
+           (L.dhi[1] && j == L.N[1] - 1) || (L.dlo[2] && k == 0) || (L.dhi[2] && k == L.N[2] - 1);
+}
+
+__device__ __forceinline__ int nd_idx(const NLev& L, int i, int j, int k) { return i + L.N[0] * (j + L.N[1] * k); }
+
+// (L x) at node (i,j,k), same grouping as the oracle
+__device__ __forceinline__ double nd_apply_at(const NLev& L, const double* __restrict__ x, int i, int j, int k)
+{
+    const int im = nd_nb(i, -1, L.N[0], L.per[0]), ip = nd_nb(i, 1, L.N[0], L.per[0]);
+    const int jm = nd_nb(j, -1, L.N[1], L.per[1]), jp = nd_nb(j, 1, L.N[1], L.per[1]);
+    const int km = nd_nb(k, -1, L.N[2], L.per[2]), kp = nd_nb(k, 1, L.N[2], L.per[2]);
+    const int sy = L.N[0], sz = L.N[0] * L.N[1];
+    const int y0 = j * sy, ym = jm * sy, yp = jp * sy;
+    const int z0 = k * sz, zm = km * sz, zp = kp * sz;
+#define X(a, b, c) x[(a) + (b) + (c)]
+    const double fx = X(im, y0, z0) + X(ip, y0, z0);
+    const double fy = X(i, ym, z0) + X(i, yp, z0);
+    const double fz = X(i, y0, zm) + X(i, y0, zp);
+    const double exy = (X(im, ym, z0) + X(ip, ym, z0)) + (X(im, yp, z0) + X(ip, yp, z0));
+    const double exz = (X(im, y0, zm) + X(ip, y0, zm)) + (X(im, y0, zp) + X(ip, y0, zp));
+    const double eyz = (X(i, ym, zm) + X(i, yp, zm)) + (X(i, ym, zp) + X(i, yp, zp));
+    const double cr = ((X(im, ym, zm) + X(ip, ym, zm)) + (X(im, yp, zm) + X(ip, yp, zm))) +
+                      ((X(im, ym, zp) + X(ip, ym, zp)) + (X(im, yp, zp) + X(ip, yp, zp)));
+    const double c0 = X(i, y0, z0);
+#undef X
+    return (((L.w[0] * c0 + L.w[1] * fx) + (L.w[2] * fy + L.w[3] * fz)) +
+            ((L.w[4] * exy + L.w[5] * exz) + (L.w[6] * eyz + L.w[7] * cr)));
+}
+
+__device__ __forceinline__ double nd_abs_or_inf(double v)
+{
+    const double a = fabs(v);
+    return a == a ? a : __longlong_as_double(0x7ff0000000000000LL);
+}
+
+// block max of non-negative values into one word (bit order = value order)
+__device__ __forceinline__ void nd_block_max(double m, unsigned long long* out)
+{
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    if ((t & 63) == 0) sh[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        double q = sh[0];
+        const int nw = (int)(blockDim.x * blockDim.y) >> 6;
+        for (int w = 1; w < nw; ++w) q = fmax(q, sh[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(q));
+    }
+}
+
+// launch shape of the level kernels: 64 x 4 nodes per workgroup, one plane
+__device__ __forceinline__ bool nd_node(const NLev& L, int& i, int& j, int& k)
+{
+    i = blockIdx.x * 64 + threadIdx.x;
+    j = blockIdx.y * 4 + threadIdx.y;
+    k = blockIdx.z;
+    return i < L.N[0] && j < L.N[1];
+}
+
+// one Jacobi sweep: xo = xi + (b - L xi) * wd; zero guess: xo = b * wd
+template <bool ZERO>
+__global__ void __launch_bounds__(256) k_nd_jacobi(double* __restrict__ xo, const double* __restrict__ xi,
+                                                   const double* __restrict__ b, NLev L)
+{
+    int i, j, k;
+    if (!nd_node(L, i, j, k)) return;
+    const int c = nd_idx(L, i, j, k);
+    if (nd_dirichlet(L, i, j, k)) {
+        xo[c] = 0.0;
+        return;
+    }
+    if (ZERO)
+        xo[c] = b[c] * L.wd;
+    else
+        xo[c] = xi[c] + (b[c] - nd_apply_at(L, xi, i, j, k)) * L.wd;
+}
+
+// r = b - L x (zero on Dirichlet nodes)
+__global__ void __launch_bounds__(256) k_nd_resid(double* __restrict__ r, const double* __restrict__ x,
+                                                  const double* __restrict__ b, NLev L)
+{
+    int i, j, k;
+    if (!nd_node(L, i, j, k)) return;
+    const int c = nd_idx(L, i, j, k);
+    r[c] = nd_dirichlet(L, i, j, k) ? 0.0 : b[c] - nd_apply_at(L, x, i, j, k);
+}
+
+// outer iteration: phi_o = phi_i + cor, r = rhs - L phi_o, |r|_inf.  phi_o is
+// read back through phi_i + cor at every stencil point, so the update and the
+// residual are one pass.
+__global__ void __launch_bounds__(256) k_nd_update_resid(double* __restrict__ phio, const double* __restrict__ phii,
+                                                         const double* __restrict__ cor, const double* __restrict__ rhs,
+                                                         double* __restrict__ r, NLev L,
+                                                         unsigned long long* __restrict__ norm)
+{
+    int i, j, k;
+    double m = 0.0;
+    if (nd_node(L, i, j, k)) {
+        const int c = nd_idx(L, i, j, k);
+        phio[c] = phii[c] + cor[c];
+        double v = 0.0;
+        if (!nd_dirichlet(L, i, j, k)) {
+            // L (phi + cor) evaluated on the summed values, as the oracle
+            // evaluates L phi after its update
+            const int im = nd_nb(i, -1, L.N[0], L.per[0]), ip = nd_nb(i, 1, L.N[0], L.per[0]);
+            const int jm = nd_nb(j, -1, L.N[1], L.per[1]), jp = nd_nb(j, 1, L.N[1], L.per[1]);
+            const int km = nd_nb(k, -1, L.N[2], L.per[2]), kp = nd_nb(k, 1, L.N[2], L.per[2]);
+            const int sy = L.N[0], sz = L.N[0] * L.N[1];
+            const int y0 = j * sy, ym = jm * sy, yp = jp * sy;
+            const int z0 = k * sz, zm = km * sz, zp = kp * sz;
+#define X(a, b, c) (phii[(a) + (b) + (c)] + cor[(a) + (b) + (c)])
+            const double fx = X(im, y0, z0) + X(ip, y0, z0);
+            const double fy = X(i, ym, z0) + X(i, yp, z0);
+            const double fz = X(i, y0, zm) + X(i, y0, zp);
+            const double exy = (X(im, ym, z0) + X(ip, ym, z0)) + (X(im, yp, z0) + X(ip, yp, z0));
+            const double exz = (X(im, y0, zm) + X(ip, y0, zm)) + (X(im, y0, zp) + X(ip, y0, zp));
+            const double eyz = (X(i, ym, zm) + X(i, yp, zm)) + (X(i, ym, zp) + X(i, yp, zp));
+            const double cr = ((X(im, ym, zm) + X(ip, ym, zm)) + (X(im, yp, zm) + X(ip, yp, zm))) +
+                              ((X(im, ym, zp) + X(ip, ym, zp)) + (X(im, yp, zp) + X(ip, yp, zp)));
+            const double c0 = X(i, y0, z0);
+#undef X
+            const double ax = (((L.w[0] * c0 + L.w[1] * fx) + (L.w[2] * fy + L.w[3] * fz)) +
+                               ((L.w[4] * exy + L.w[5] * exz) + (L.w[6] * eyz + L.w[7] * cr)));
+            v = rhs[c] - ax;
+        }
+        r[c] = v;
+        m = nd_abs_or_inf(v);
+    }
+    nd_block_max(m, norm);
+}
+
+// |a|_inf over a level (Dirichlet nodes hold zero)
+__global__ void __launch_bounds__(256) k_nd_norm(const double* __restrict__ a, NLev L,
+                                                 unsigned long long* __restrict__ norm)
+{
+    int i, j, k;
+    double m = 0.0;
+    if (nd_node(L, i, j, k)) m = nd_abs_or_inf(a[nd_idx(L, i, j, k)]);
+    nd_block_max(m, norm);
+}
+
+// full weighting: weights (1/4, 1/2, 1/4) along each coarsened direction
+__global__ void __launch_bounds__(256) k_nd_restrict(double* __restrict__ c, const double* __restrict__ f, NLev F,
+                                                     NLev C)
+{
+    int I, J, K;
+    if (!nd_node(C, I, J, K)) return;
+    const int o = nd_idx(C, I, J, K);
+    if (nd_dirichlet(C, I, J, K)) {
+        c[o] = 0.0;
+        return;
+    }
+    const int ci[3] = {I, J, K};
+    int idx[3][3];
+    double wt[3][3];
+    int cnt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (F.r[d] == 2) {
+            const int fi = 2 * ci[d];
+            idx[d][0] = nd_nb(fi, -1, F.N[d], F.per[d]);
+            idx[d][1] = fi;
+            idx[d][2] = nd_nb(fi, 1, F.N[d], F.per[d]);
+            wt[d][0] = 0.25;
+            wt[d][1] = 0.5;
+            wt[d][2] = 0.25;
+            cnt[d] = 3;
+        } else {
+            idx[d][0] = idx[d][1] = idx[d][2] = ci[d];
+            wt[d][0] = 1.0;
+            wt[d][1] = wt[d][2] = 0.0;
+            cnt[d] = 1;
+        }
+    }
+    double s = 0.0;
+    for (int a = 0; a < cnt[2]; ++a) {
+        double sy = 0.0;
+        for (int b = 0; b < cnt[1]; ++b) {
+            double sx = 0.0;
+            for (int e = 0; e < cnt[0]; ++e) sx += wt[0][e] * f[nd_idx(F, idx[0][e], idx[1][b], idx[2][a])];
+            sy += wt[1][b] * sx;
+        }
+        s += wt[2][a] * sy;
+    }
+    c[o] = s;
+}
+
+// trilinear interpolation of the coarse correction, added to the fine one
+__global__ void __launch_bounds__(256) k_nd_prolong_add(double* __restrict__ f, const double* __restrict__ c, NLev F,
+                                                        NLev C)
+{
+    int i, j, k;
+    if (!nd_node(F, i, j, k)) return;
+    if (nd_dirichlet(F, i, j, k)) return;
+    const int fi[3] = {i, j, k};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (F.r[d] == 2) {
+            lo[d] = fi[d] >> 1;
+            hi[d] = (fi[d] & 1) ? ((lo[d] + 1 == C.N[d]) ? 0 : lo[d] + 1) : lo[d];
+        } else {
+            lo[d] = hi[d] = fi[d];
+        }
+    }
+    const bool ox = lo[0] != hi[0], oy = lo[1] != hi[1], oz = lo[2] != hi[2];
+    double z0, z1;
+    {
+        double a = c[nd_idx(C, lo[0], lo[1], lo[2])], b = c[nd_idx(C, hi[0], lo[1], lo[2])];
+        double e = c[nd_idx(C, lo[0], hi[1], lo[2])], g = c[nd_idx(C, hi[0], hi[1], lo[2])];
+        double y0 = ox ? 0.5 * (a + b) : a, y1 = ox ? 0.5 * (e + g) : e;
+        z0 = oy ? 0.5 * (y0 + y1) : y0;
+    }
+    {
+        double a = c[nd_idx(C, lo[0], lo[1], hi[2])], b = c[nd_idx(C, hi[0], lo[1], hi[2])];
+        double e = c[nd_idx(C, lo[0], hi[1], hi[2])], g = c[nd_idx(C, hi[0], hi[1], hi[2])];
+        double y0 = ox ? 0.5 * (a + b) : a, y1 = ox ? 0.5 * (e + g) : e;
+        z1 = oy ? 0.5 * (y0 + y1) : y0;
+    }
+    const double v = oz ? 0.5 * (z0 + z1) : z0;
+    const int o = nd_idx(F, i, j, k);
+    f[o] += v;
+}
+
+// weight of a node in the solvability condition: 2^-(Neumann planes)
+__device__ __forceinline__ double nd_weight(const NLev& L, int i, int j, int k)
+{
+    double w = 1.0;
+    if (L.nlo[0] && i == 0) w *= 0.5;
+    if (L.nhi[0] && i == L.N[0] - 1) w *= 0.5;
+    if (L.nlo[1] && j == 0) w *= 0.5;
+    if (L.nhi[1] && j == L.N[1] - 1) w *= 0.5;
+    if (L.nlo[2] && k == 0) w *= 0.5;
+    if (L.nhi[2] && k == L.N[2] - 1) w *= 0.5;
+    return w;
+}
+
+// per-workgroup partial sums of w * a (fixed order within the workgroup)
+__global__ void __launch_bounds__(256) k_nd_wsum(const double* __restrict__ a, NLev L, double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    int i, j, k;
+    double s = 0.0;
+    if (nd_node(L, i, j, k)) s = nd_weight(L, i, j, k) * a[nd_idx(L, i, j, k)];
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    sh[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) sh[t] += sh[t + h];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = sh[0];
+}
+
+// sum of the partials in a fixed order, then mean = sum / wsum
+__global__ void __launch_bounds__(1024) k_nd_mean(const double* __restrict__ part, int n, double wsum,
+                                                  double* __restrict__ mean)
+{
+    __shared__ double sh[1024];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += 1024) s += part[q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *mean = sh[0] / wsum;
+}
+
+__global__ void __launch_bounds__(256) k_nd_sub(double* __restrict__ a, NLev L, const double* __restrict__ mean,
+                                                int zero_dirichlet)
+{
+    int i, j, k;
+    if (!nd_node(L, i, j, k)) return;
+    const int c = nd_idx(L, i, j, k);
+    if (zero_dirichlet && nd_dirichlet(L, i, j, k))
+        a[c] = 0.0;
+    else if (mean)
+        a[c] -= *mean;
+}
+
+__global__ void __launch_bounds__(256) k_nd_copy(double* __restrict__ a, const double* __restrict__ b, int n)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) a[t] = b[t];
+}
+
+__global__ void __launch_bounds__(256) k_nd_zero(double* __restrict__ a, int n)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) a[t] = 0.0;
+}
+
+__global__ void k_nd_reset(unsigned long long* w) { *w = 0ull; }
+
+// ---------------------------------------------------------------------------
+// bottom level in one workgroup: (singular) weighted mean removal of the
+// right-hand side, BiCGStab from a zero guess, then nsmooth Jacobi sweeps --
+// the steps of the oracle's bottom solve
+// ---------------------------------------------------------------------------
+struct NdBottom {
+    double *r, *rh, *p, *v, *s, *t, *tmp;
+};
+
+__device__ __forceinline__ void nd_decode(const NLev& L, int c, int& i, int& j, int& k)
+{
+    i = c % L.N[0];
+    const int q = c / L.N[0];
+    j = q % L.N[1];
+    k = q / L.N[1];
+}
+
+__device__ double nd_bsum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kNdBottomThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__device__ double nd_bmax(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kNdBottomThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ void nd_bapply(const NLev& L, const double* x, double* y)
+{
+    for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
+        int i, j, k;
+        nd_decode(L, c, i, j, k);
+        y[c] = nd_dirichlet(L, i, j, k) ? 0.0 : nd_apply_at(L, x, i, j, k);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kNdBottomThreads) k_nd_bottom(double* __restrict__ x, double* __restrict__ b, NLev L,
+                                                                NdBottom W, int singular, double wsum, double rtol,
+                                                                double atol, int maxiter, int nsmooth_ok,
+                                                                int nsmooth_fail, int* __restrict__ stats)
+{
+    __shared__ double sh[kNdBottomThreads];
+    const int T = L.tot;
+    if (singular) {
+        double s = 0.0;
+        for (int c = threadIdx.x; c < T; c += kNdBottomThreads) {
+            int i, j, k;
+            nd_decode(L, c, i, j, k);
+            s += nd_weight(L, i, j, k) * b[c];
+        }
+        const double m = nd_bsum(s, sh) / wsum;
+        for (int c = threadIdx.x; c < T; c += kNdBottomThreads) b[c] -= m;
+        __syncthreads();
+    }
+    double mx = 0.0;
+    for (int c = threadIdx.x; c < T; c += kNdBottomThreads) {
+        x[c] = 0.0;
+        W.r[c] = b[c];
+        W.rh[c] = b[c];
+        mx = fmax(mx, nd_abs_or_inf(b[c]));
+    }
+    const double rnorm0 = nd_bmax(mx, sh);
+    const double tol = fmax(rtol * rnorm0, atol);
+    int ret = 1, it = 0;
+    if (rnorm0 == 0.0 || rnorm0 <= tol) {
+        ret = 0;
+    } else {
+        double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+        for (it = 1; it <= maxiter; ++it) {
+            double a = 0.0;
+            for (int c = threadIdx.x; c < T; c += kNdBottomThreads) a += W.rh[c] * W.r[c];
+            const double rho = nd_bsum(a, sh);
+            if (rho == 0.0) { ret = 2; break; }
+            if (it == 1) {
+                for (int c = threadIdx.x; c < T; c += kNdBottomThreads) W.p[c] = W.r[c];
+            } else {
+                const double beta = (rho / rho_prev) * (alpha / omega);
+                for (int c = threadIdx.x; c < T; c += kNdBottomThreads)
+                    W.p[c] = W.r[c] + beta * (W.p[c] - omega * W.v[c]);
+            }
+            __syncthreads();
+            nd_bapply(L, W.p, W.v);
+            a = 0.0;
+            for (int c = threadIdx.x; c < T; c += kNdBottomThreads) a += W.rh[c] * W.v[c];
+            const double rv = nd_bsum(a, sh);
+            if (rv == 0.0) { ret = 2; break; }
+            alpha = rho / rv;
+            mx = 0.0;
+            for (int c = threadIdx.x; c < T; c += kNdBottomThreads) {
+                W.s[c] = W.r[c] - alpha * W.v[c];
+                mx = fmax(mx, nd_abs_or_inf(W.s[c]));
+            }
+            if (nd_bmax(mx, sh) <= tol) {
+                for (int c = threadIdx.x; c < T; c += kNdBottomThreads) x[c] += alpha * W.p[c];
+                __syncthreads();
+                ret = 0;
+                break;
+            }
+            nd_bapply(L, W.s, W.t);
+            double tt = 0.0, ts = 0.0;
+            for (int c = threadIdx.x; c < T; c += kNdBottomThreads) {
+                tt += W.t[c] * W.t[c];
+                ts += W.t[c] * W.s[c];
+            }
+            const double ttt = nd_bsum(tt, sh);
+            if (ttt == 0.0) { ret = 2; break; }
+            omega = nd_bsum(ts, sh) / ttt;
+            mx = 0.0;
+            for (int c = threadIdx.x; c < T; c += kNdBottomThreads) {
+                x[c] += alpha * W.p[c] + omega * W.s[c];
+                W.r[c] = W.s[c] - omega * W.t[c];
+                mx = fmax(mx, nd_abs_or_inf(W.r[c]));
+            }
+            if (nd_bmax(mx, sh) <= tol) { ret = 0; break; }
+            if (omega == 0.0) { ret = 2; break; }
+            rho_prev = rho;
+        }
+        if (it > maxiter) it = maxiter;
+    }
+    __syncthreads();
+    const int ns = kNdSweeps * (ret == 0 ? nsmooth_ok : nsmooth_fail);
+    for (int sw = 0; sw < ns; ++sw) {
+        for (int c = threadIdx.x; c < T; c += kNdBottomThreads) {
+            int i, j, k;
+            nd_decode(L, c, i, j, k);
+            W.tmp[c] = nd_dirichlet(L, i, j, k) ? 0.0 : x[c] + (b[c] - nd_apply_at(L, x, i, j, k)) * L.wd;
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < T; c += kNdBottomThreads) x[c] = W.tmp[c];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats[0] += it;
+        stats[1] += (ret != 0) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side and correction on the BoxArray
+// ---------------------------------------------------------------------------
+struct NdGeom {
+    int N[3];
+    int per[3];
+    int nlo[3], nhi[3];
+    int dlo[3], dhi[3];
+    double f[3];  // 0.25 / dx
+};
+
+// nodal divergence of the cell velocity.  Launched over each box's cells grown
+// by one at the high side; a thread owns node (i,j,k) if it lies in the box,
+// or on the domain's high side of a non-periodic direction.  The eight cells
+// (i-1|i, j-1|j, k-1|k) come from the box's FAB (ghosts as ApplyProjection
+// leaves them: zero on physical boundaries, filled elsewhere).
+__global__ void __launch_bounds__(256) k_nd_rhs(CMFDev vel, double* __restrict__ rhs, NdGeom g, RegDev r,
+                                                const int* __restrict__ box)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    const int* bx = box + 6 * b;
+    const int gi[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (gi[d] > bx[3 + d] && (g.per[d] || gi[d] != g.dhi[d] + 1)) return;
+    }
+    const int id[3] = {i - g.dlo[0], j - g.dlo[1], k - g.dlo[2]};
+    CFab f = fab_at(vel, box, b);
+    const int c000 = f.idx(i - 1, j - 1, k - 1);
+    const int sy = f.n0, sz = f.n0 * f.n1;
+    const double* u = f.p + c000;
+    const double* v = f.p + c000 + f.cs;
+    const double* w = f.p + c000 + 2 * f.cs;
+#define A(p, a, bb, c) p[(a) + sy * (bb) + sz * (c)]
+    const double dux = ((A(u, 1, 0, 0) - A(u, 0, 0, 0)) + (A(u, 1, 1, 0) - A(u, 0, 1, 0))) +
+                       ((A(u, 1, 0, 1) - A(u, 0, 0, 1)) + (A(u, 1, 1, 1) - A(u, 0, 1, 1)));
+    const double dvy = ((A(v, 0, 1, 0) - A(v, 0, 0, 0)) + (A(v, 1, 1, 0) - A(v, 1, 0, 0))) +
+                       ((A(v, 0, 1, 1) - A(v, 0, 0, 1)) + (A(v, 1, 1, 1) - A(v, 1, 0, 1)));
+    const double dwz = ((A(w, 0, 0, 1) - A(w, 0, 0, 0)) + (A(w, 1, 0, 1) - A(w, 1, 0, 0))) +
+                       ((A(w, 0, 1, 1) - A(w, 0, 1, 0)) + (A(w, 1, 1, 1) - A(w, 1, 1, 0)));
+#undef A
+    double dv = (dux * g.f[0] + dvy * g.f[1]) + dwz * g.f[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (g.nlo[d] && id[d] == 0) dv *= 2.0;
+        if (g.nhi[d] && id[d] == g.N[d] - 1) dv *= 2.0;
+    }
+    rhs[id[0] + g.N[0] * (id[1] + g.N[1] * id[2])] = dv;
+}
+
+// vel -= sigma grad phi on valid cells; gp = grad phi (average of the four
+// node differences per direction) if has_gp
+__global__ void __launch_bounds__(256) k_nd_correct(MFDev vel, MFDev gp, int has_gp, const double* __restrict__ phi,
+                                                    NdGeom g, double sigma, RegDev r, const int* __restrict__ box)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    const int a0 = i - g.dlo[0], b0 = j - g.dlo[1], c0 = k - g.dlo[2];
+    const int a1 = (a0 + 1 == g.N[0]) ? 0 : a0 + 1;
+    const int b1 = (b0 + 1 == g.N[1]) ? 0 : b0 + 1;
+    const int c1 = (c0 + 1 == g.N[2]) ? 0 : c0 + 1;
+#define P(a, bb, c) phi[(a) + g.N[0] * ((bb) + g.N[1] * (c))]
+    double gx = ((P(a1, b0, c0) - P(a0, b0, c0)) + (P(a1, b1, c0) - P(a0, b1, c0))) +
+                ((P(a1, b0, c1) - P(a0, b0, c1)) + (P(a1, b1, c1) - P(a0, b1, c1)));
+    double gy = ((P(a0, b1, c0) - P(a0, b0, c0)) + (P(a1, b1, c0) - P(a1, b0, c0))) +
+                ((P(a0, b1, c1) - P(a0, b0, c1)) + (P(a1, b1, c1) - P(a1, b0, c1)));
+    double gz = ((P(a0, b0, c1) - P(a0, b0, c0)) + (P(a1, b0, c1) - P(a1, b0, c0))) +
+                ((P(a0, b1, c1) - P(a0, b1, c0)) + (P(a1, b1, c1) - P(a1, b1, c0)));
+#undef P
+    gx *= g.f[0];
+    gy *= g.f[1];
+    gz *= g.f[2];
+    Fab fv = fab_at(vel, box, b);
+    fv(i, j, k, 0) = fv(i, j, k, 0) - sigma * gx;
+    fv(i, j, k, 1) = fv(i, j, k, 1) - sigma * gy;
+    fv(i, j, k, 2) = fv(i, j, k, 2) - sigma * gz;
+    if (has_gp) {
+        Fab fg = fab_at(gp, box, b);
+        fg(i, j, k, 0) = gx;
+        fg(i, j, k, 1) = gy;
+        fg(i, j, k, 2) = gz;
+    }
+}
+
+// MultiFab::setBndry(val): every point of the FAB storage outside the box
+__global__ void __launch_bounds__(256) k_set_bndry(MFDev m, int ncomp, double val, RegDev r,
+                                                   const int* __restrict__ box)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    const int* bx = box + 6 * b;
+    const int hi0 = bx[3] + (m.itype == 1), hi1 = bx[4] + (m.itype == 2), hi2 = bx[5] + (m.itype == 3);
+    if (i >= bx[0] && i <= hi0 && j >= bx[1] && j <= hi1 && k >= bx[2] && k <= hi2) return;
+    Fab f = fab_at(m, box, b);
+    for (int n = 0; n < ncomp; ++n) f(i, j, k, n) = val;
+}
+
+}  // namespace ksgf
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct ksgf_nodalproj {
+    const ksgf_boxarray* ba = nullptr;
+    ksgf_mlmg_options opts;
+    int lobc[3], hibc[3];
+    double sigma = 1.0;
+    int nlev = 0;
+    ksgf::NLev L[ksgf::kNdMaxLev];
+    double wsum[ksgf::kNdMaxLev];
+    int singular = 0;
+    // level arrays: correction, residual (the level's right-hand side), scratch
+    std::vector<double*> cor, res, tmp;
+    double *phi[2] = {nullptr, nullptr}, *rhs = nullptr;
+    int cur = 0;  // phi[cur] holds the solution
+    double* part = nullptr;
+    int npart = 0;
+    double* mean = nullptr;
+    unsigned long long* norm = nullptr;
+    int* stats = nullptr;
+    ksgf::NdBottom bw{};
+    std::vector<double*> allocs;
+};
+
+namespace ksgf {
+
+static void nd_weights(double sigma, const double* h, double* w)
+{
+    const double ax = sigma / (h[0] * h[0]), ay = sigma / (h[1] * h[1]), az = sigma / (h[2] * h[2]);
+    const double s = (ax + ay) + az;
+    w[0] = -(8.0 / 9.0) * s;
+    w[1] = (2.0 / 9.0) * ((2.0 * ax - ay) - az);
+    w[2] = (2.0 / 9.0) * ((2.0 * ay - ax) - az);
+    w[3] = (2.0 / 9.0) * ((2.0 * az - ax) - ay);
+    w[4] = (ax + ay) / 9.0 - az / 18.0;
+    w[5] = (ax + az) / 9.0 - ay / 18.0;
+    w[6] = (ay + az) / 9.0 - ax / 18.0;
+    w[7] = s / 36.0;
+}
+
+// Jacobi weight 1 / lambda_max of D^-1 A; the symbol is multilinear in the
+// cosines, so its extremes are at the corners of {0, pi}^3
+static double nd_omega(const double* w)
+{
+    double lmax = 0.0;
+    for (int m = 1; m < 8; ++m) {
+        const double cx = (m & 1) ? -1.0 : 1.0, cy = (m & 2) ? -1.0 : 1.0, cz = (m & 4) ? -1.0 : 1.0;
+        const double sym = ((w[0] + 2.0 * (w[1] * cx + w[2] * cy + w[3] * cz)) +
+                            4.0 * (w[4] * cx * cy + w[5] * cx * cz + w[6] * cy * cz)) +
+                           8.0 * w[7] * cx * cy * cz;
+        const double lam = sym / w[0];
+        if (lam > lmax) lmax = lam;
+    }
+    return 1.0 / lmax;
+}
+
+static void nd_set_coefs(ksgf_nodalproj* np)
+{
+    for (int l = 0; l < np->nlev; ++l) {
+        NLev& L = np->L[l];
+        double h[3];
+        for (int d = 0; d < 3; ++d) {
+            int f = 1;
+            for (int q = 0; q < l; ++q) f *= np->L[q].r[d];
+            h[d] = np->ba->dx[d] * f;
+        }
+        nd_weights(np->sigma, h, L.w);
+        L.wd = nd_omega(L.w) / L.w[0];
+    }
+}
+
+static dim3 nd_grid(const NLev& L) { return dim3((L.N[0] + 63) / 64, (L.N[1] + 3) / 4, L.N[2]); }
+static const dim3 kNdBlock(64, 4, 1);
+
+static int nd_alloc(ksgf_nodalproj* np, double** p, size_t n)
+{
+    if (hipMalloc((void**)p, sizeof(double) * std::max<size_t>(n, 1)) != hipSuccess) {
+        set_error("hipMalloc (nodal projector)");
+        return KSGF_ERR_NOMEM;
+    }
+    np->allocs.push_back(*p);
+    return KSGF_OK;
+}
+
+static void nd_weighted_mean_sub(ksgf_nodalproj* np, int l, double* a, hipStream_t st)
+{
+    const NLev& L = np->L[l];
+    dim3 gr = nd_grid(L);
+    hipLaunchKernelGGL(k_nd_wsum, gr, kNdBlock, 0, st, a, L, np->part);
+    const int n = (int)(gr.x * gr.y * gr.z);
+    hipLaunchKernelGGL(k_nd_mean, dim3(1), dim3(1024), 0, st, np->part, n, np->wsum[l], np->mean);
+    hipLaunchKernelGGL(k_nd_sub, gr, kNdBlock, 0, st, a, L, np->mean, 0);
+}
+
+// nsweep Jacobi sweeps on x (zero guess if zero); the result ends in x
+static void nd_smooth(ksgf_nodalproj* np, int l, double* x, const double* b, int nsweep, bool zero, hipStream_t st)
+{
+    const NLev& L = np->L[l];
+    dim3 gr = nd_grid(L);
+    if (nsweep == 0) {
+        if (zero) hipLaunchKernelGGL(k_nd_zero, dim3((L.tot + 255) / 256), dim3(256), 0, st, x, L.tot);
+        return;
+    }
+    double* t = np->tmp[l];
+    // out-of-place sweeps alternating between x and t, starting where the
+    // last one lands in x; a zero-guess first sweep reads nothing, so it may
+    // write x itself (an odd count); otherwise an odd count ends with a copy
+    double* src = x;
+    double* dst = (nsweep % 2 == 1 && zero) ? x : t;
+    for (int s = 0; s < nsweep; ++s) {
+        if (s == 0 && zero)
+            hipLaunchKernelGGL(k_nd_jacobi<true>, gr, kNdBlock, 0, st, dst, src, b, L);
+        else
+            hipLaunchKernelGGL(k_nd_jacobi<false>, gr, kNdBlock, 0, st, dst, src, b, L);
+        src = dst;
+        dst = (src == x) ? t : x;
+    }
+    if (src != x) hipLaunchKernelGGL(k_nd_copy, dim3((L.tot + 255) / 256), dim3(256), 0, st, x, src, L.tot);
+}
+
+static void nd_vcycle(ksgf_nodalproj* np, int l, hipStream_t st)
+{
+    const ksgf_mlmg_options& o = np->opts;
+    if (l == np->nlev - 1) {
+        hipLaunchKernelGGL(k_nd_bottom, dim3(1), dim3(kNdBottomThreads), 0, st, np->cor[l], np->res[l], np->L[l],
+                           np->bw, np->singular, np->wsum[l], o.bottom_rtol, o.bottom_atol, o.bottom_maxiter,
+                           o.num_bottom_smooth, o.num_final_smooth, np->stats);
+        return;
+    }
+    const NLev& L = np->L[l];
+    dim3 gr = nd_grid(L);
+    nd_smooth(np, l, np->cor[l], np->res[l], kNdSweeps * o.num_pre_smooth, true, st);
+    hipLaunchKernelGGL(k_nd_resid, gr, kNdBlock, 0, st, np->tmp[l], np->cor[l], np->res[l], L);
+    hipLaunchKernelGGL(k_nd_restrict, nd_grid(np->L[l + 1]), kNdBlock, 0, st, np->res[l + 1], np->tmp[l], L,
+                       np->L[l + 1]);
+    nd_vcycle(np, l + 1, st);
+    hipLaunchKernelGGL(k_nd_prolong_add, gr, kNdBlock, 0, st, np->cor[l], np->cor[l + 1], L, np->L[l + 1]);
+    nd_smooth(np, l, np->cor[l], np->res[l], kNdSweeps * o.num_post_smooth, false, st);
+}
+
+static int nd_read_norm(ksgf_nodalproj* np, hipStream_t st, double* out)
+{
+    unsigned long long bits = 0;
+    KSGF_HIP_CHECK(hipMemcpyAsync(&bits, np->norm, sizeof(bits), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    std::memcpy(out, &bits, sizeof(double));
+    return KSGF_OK;
+}
+
+// solve with np->rhs (level-0 right-hand side, device) already set; result in
+// np->phi[np->cur]
+static int nd_solve(ksgf_nodalproj* np, double rtol, double atol, ksgf_mlmg_info* info, hipStream_t st)
+{
+    const NLev& L0 = np->L[0];
+    dim3 gr = nd_grid(L0);
+    const int nb = (L0.tot + 255) / 256;
+    hipLaunchKernelGGL(k_nd_sub, gr, kNdBlock, 0, st, np->rhs, L0, (const double*)nullptr, 1);
+    if (np->singular) nd_weighted_mean_sub(np, 0, np->rhs, st);
+    hipLaunchKernelGGL(k_nd_reset, dim3(1), dim3(1), 0, st, np->norm);
+    hipLaunchKernelGGL(k_nd_norm, gr, kNdBlock, 0, st, np->rhs, L0, np->norm);
+    KSGF_LAUNCH_CHECK();
+    KSGF_HIP_CHECK(hipMemsetAsync(np->stats, 0, 2 * sizeof(int), st));
+    np->cur = 0;
+    hipLaunchKernelGGL(k_nd_zero, dim3(nb), dim3(256), 0, st, np->phi[0], L0.tot);
+    hipLaunchKernelGGL(k_nd_copy, dim3(nb), dim3(256), 0, st, np->res[0], np->rhs, L0.tot);
+    double rhsnorm = 0.0;
+    int e = nd_read_norm(np, st, &rhsnorm);
+    if (e) return e;
+    double r = rhsnorm;
+    const double target = std::max(atol, rtol * rhsnorm);
+    ksgf_mlmg_info inf{};
+    inf.r0 = r;
+    inf.converged = (r <= target);
+    inf.nlevels = np->nlev;
+    int diverged = 0;
+    if (!inf.converged) {
+        for (int it = 1; it <= np->opts.maxiter; ++it) {
+            nd_vcycle(np, 0, st);
+            hipLaunchKernelGGL(k_nd_reset, dim3(1), dim3(1), 0, st, np->norm);
+            hipLaunchKernelGGL(k_nd_update_resid, gr, kNdBlock, 0, st, np->phi[1 - np->cur], np->phi[np->cur],
+                               np->cor[0], np->rhs, np->res[0], L0, np->norm);
+            KSGF_LAUNCH_CHECK();
+            np->cur = 1 - np->cur;
+            e = nd_read_norm(np, st, &r);
+            if (e) return e;
+            inf.iters = it;
+            if (r <= target) {
+                inf.converged = 1;
+                break;
+            }
+            // 1e20 |rhs| or not finite: MLMG::solve's "failing so lets stop here"
+            if (!(r <= 1.0e20 * rhsnorm)) {
+                diverged = 1;
+                break;
+            }
+        }
+    }
+    int hs[2] = {0, 0};
+    KSGF_HIP_CHECK(hipMemcpyAsync(hs, np->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    inf.rfinal = r;
+    inf.bottom_iters = hs[0];
+    inf.bottom_fail = hs[1];
+    if (info) *info = inf;
+    if (diverged) {
+        set_error("nodal MLMG diverged (residual above 1e20 |rhs| or not finite)");
+        return KSGF_ERR_DIVERGED;
+    }
+    return inf.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+}
+
+static NdGeom nd_geom(const ksgf_nodalproj* np)
+{
+    NdGeom g;
+    for (int d = 0; d < 3; ++d) {
+        g.N[d] = np->L[0].N[d];
+        g.per[d] = np->L[0].per[d];
+        g.nlo[d] = np->L[0].nlo[d];
+        g.nhi[d] = np->L[0].nhi[d];
+        g.dlo[d] = np->ba->dlo[d];
+        g.dhi[d] = np->ba->dhi[d];
+        g.f[d] = 0.25 / np->ba->dx[d];
+    }
+    return g;
+}
+
+static int nd_compute_rhs(ksgf_nodalproj* np, const double* vel, int ng_vel, double* rhs, hipStream_t st)
+{
+    CMFDev v;
+    int e = cmfdev(np->ba, vel, KSGF_CELL, 3, ng_vel, &v);
+    if (e) return e;
+    int err = KSGF_OK;
+    const int glo[3] = {0, 0, 0}, ghi[3] = {1, 1, 1};
+    const Region* r = get_region(np->ba, glo, ghi, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_nd_rhs, dim3(r->nchunks), dim3(kChunk), 0, st, v, rhs, nd_geom(np), regdev(r, np->ba->nbox),
+                       np->ba->dbox);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+}  // namespace ksgf
+
+using namespace ksgf;
+
+extern "C" {
+
+int ksgf_nodalproj_destroy(ksgf_nodalproj* np)
+{
+    if (!np) return KSGF_OK;
+    for (double* p : np->allocs) (void)hipFree(p);
+    if (np->norm) (void)hipFree(np->norm);
+    if (np->stats) (void)hipFree(np->stats);
+    delete np;
+    return KSGF_OK;
+}
+
+int ksgf_nodalproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, double sigma,
+                          const ksgf_mlmg_options* opts, ksgf_nodalproj** out)
+{
+    if (!ba || !lobc || !hibc || !out) {
+        set_error("ksgf_nodalproj_create: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d) {
+        for (int bc : {lobc[d], hibc[d]})
+            if (bc != KSGF_LO_PERIODIC && bc != KSGF_LO_NEUMANN && bc != KSGF_LO_DIRICHLET) {
+                set_error("ksgf_nodalproj_create: domain BC must be periodic, Neumann or Dirichlet (inflow is not provided)");
+                return bc >= 0 ? KSGF_ERR_UNSUPPORTED : KSGF_ERR_INVALID;
+            }
+        if ((lobc[d] == KSGF_LO_PERIODIC) != (hibc[d] == KSGF_LO_PERIODIC) ||
+            (lobc[d] == KSGF_LO_PERIODIC) != (ba->periodic[d] != 0)) {
+            set_error("ksgf_nodalproj_create: periodic BCs must match the geometry on both sides");
+            return KSGF_ERR_INVALID;
+        }
+    }
+    if (!(sigma > 0.0)) {
+        set_error("ksgf_nodalproj_create: sigma must be positive");
+        return KSGF_ERR_INVALID;
+    }
+    ksgf_nodalproj* np = new ksgf_nodalproj();
+    np->ba = ba;
+    if (opts)
+        np->opts = *opts;
+    else
+        ksgf_mlmg_default_options(&np->opts);
+    if (np->opts.precision != KSGF_MG_FP64) {
+        delete np;
+        set_error("ksgf_nodalproj_create: the nodal solver runs in FP64 only");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    np->sigma = sigma;
+    np->singular = 1;
+    for (int d = 0; d < 3; ++d) {
+        np->lobc[d] = lobc[d];
+        np->hibc[d] = hibc[d];
+        if (lobc[d] == KSGF_LO_DIRICHLET || hibc[d] == KSGF_LO_DIRICHLET) np->singular = 0;
+    }
+    // hierarchy: the MAC projector's coarsening rule on the cell counts
+    int cn[3];
+    double ch[3];
+    for (int d = 0; d < 3; ++d) {
+        cn[d] = ba->dhi[d] - ba->dlo[d] + 1;
+        ch[d] = ba->dx[d];
+    }
+    int nl = 0;
+    for (;;) {
+        NLev& L = np->L[nl];
+        long long tot = 1;
+        for (int d = 0; d < 3; ++d) {
+            L.per[d] = lobc[d] == KSGF_LO_PERIODIC;
+            L.N[d] = L.per[d] ? cn[d] : cn[d] + 1;
+            L.dlo[d] = lobc[d] == KSGF_LO_DIRICHLET;
+            L.dhi[d] = hibc[d] == KSGF_LO_DIRICHLET;
+            L.nlo[d] = lobc[d] == KSGF_LO_NEUMANN;
+            L.nhi[d] = hibc[d] == KSGF_LO_NEUMANN;
+            L.r[d] = 1;
+            tot *= L.N[d];
+        }
+        if (tot >= (1LL << 31) || L.N[2] > 65535 || (L.N[1] + 3) / 4 > 65535) {
+            delete np;
+            set_error("ksgf_nodalproj_create: level too large");
+            return KSGF_ERR_INVALID;
+        }
+        L.tot = (int)tot;
+        // sum of the solvability weights 2^-(Neumann planes), exact
+        double ws[3];
+        for (int d = 0; d < 3; ++d) ws[d] = L.N[d] - 0.5 * (L.nlo[d] + L.nhi[d]);
+        np->wsum[nl] = ws[0] * ws[1] * ws[2];
+        ++nl;
+        if (nl >= kNdMaxLev || nl > np->opts.max_coarsening_level) break;
+        int can[3], any = 0;
+        for (int d = 0; d < 3; ++d) can[d] = (cn[d] % 2 == 0 && cn[d] / 2 >= 2);
+        if (!np->opts.do_semicoarsening) {
+            if (!(can[0] && can[1] && can[2])) break;
+            for (int d = 0; d < 3; ++d) L.r[d] = 2;
+            any = 1;
+        } else {
+            double hmin = 1e300;
+            for (int d = 0; d < 3; ++d) hmin = std::min(hmin, ch[d]);
+            for (int d = 0; d < 3; ++d)
+                if (can[d] && ch[d] <= 1.5 * hmin) {
+                    L.r[d] = 2;
+                    any = 1;
+                }
+            if (!any) break;
+        }
+        for (int d = 0; d < 3; ++d) {
+            cn[d] /= L.r[d];
+            ch[d] *= L.r[d];
+        }
+    }
+    np->nlev = nl;
+    nd_set_coefs(np);
+    int e = KSGF_OK;
+    np->cor.resize(nl);
+    np->res.resize(nl);
+    np->tmp.resize(nl);
+    for (int l = 0; l < nl && !e; ++l) {
+        const size_t n = (size_t)np->L[l].tot;
+        if ((e = nd_alloc(np, &np->cor[l], n)) || (e = nd_alloc(np, &np->res[l], n)) ||
+            (e = nd_alloc(np, &np->tmp[l], n)))
+            break;
+    }
+    const size_t n0 = (size_t)np->L[0].tot;
+    const NLev& Lb = np->L[nl - 1];
+    dim3 gr = nd_grid(np->L[0]);
+    np->npart = (int)(gr.x * gr.y * gr.z);
+    if (!e && ((e = nd_alloc(np, &np->phi[0], n0)) || (e = nd_alloc(np, &np->phi[1], n0)) ||
+               (e = nd_alloc(np, &np->rhs, n0)) || (e = nd_alloc(np, &np->part, (size_t)np->npart)) ||
+               (e = nd_alloc(np, &np->mean, 1)) || (e = nd_alloc(np, &np->bw.r, Lb.tot)) ||
+               (e = nd_alloc(np, &np->bw.rh, Lb.tot)) || (e = nd_alloc(np, &np->bw.p, Lb.tot)) ||
+               (e = nd_alloc(np, &np->bw.v, Lb.tot)) || (e = nd_alloc(np, &np->bw.s, Lb.tot)) ||
+               (e = nd_alloc(np, &np->bw.t, Lb.tot)) || (e = nd_alloc(np, &np->bw.tmp, Lb.tot)))) {
+    }
+    if (!e && (hipMalloc((void**)&np->norm, sizeof(unsigned long long)) != hipSuccess ||
+               hipMalloc((void**)&np->stats, 2 * sizeof(int)) != hipSuccess)) {
+        set_error("hipMalloc (nodal projector)");
+        e = KSGF_ERR_NOMEM;
+    }
+    if (!e && hipMemset(np->phi[0], 0, sizeof(double) * n0) != hipSuccess) {
+        set_error("hipMemset (nodal projector)");
+        e = KSGF_ERR_HIP;
+    }
+    if (e) {
+        ksgf_nodalproj_destroy(np);
+        return e;
+    }
+    *out = np;
+    return KSGF_OK;
+}
+
+int ksgf_nodalproj_set_sigma(ksgf_nodalproj* np, double sigma)
+{
+    if (!np || !(sigma > 0.0)) {
+        set_error("ksgf_nodalproj_set_sigma: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    np->sigma = sigma;
+    nd_set_coefs(np);
+    return KSGF_OK;
+}
+
+int ksgf_nodalproj_node_dims(const ksgf_nodalproj* np, int* N)
+{
+    if (!np || !N) {
+        set_error("ksgf_nodalproj_node_dims: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d) N[d] = np->L[0].N[d];
+    return KSGF_OK;
+}
+
+int ksgf_nodalproj_compute_rhs(ksgf_nodalproj* np, const double* vel, int ng_vel, double* rhs, ksgf_stream_t stream)
+{
+    if (!np || !vel || !rhs || ng_vel < 1) {
+        set_error("ksgf_nodalproj_compute_rhs: bad argument (vel needs one ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    return nd_compute_rhs(np, vel, ng_vel, rhs, (hipStream_t)stream);
+}
+
+int ksgf_nodalproj_solve(ksgf_nodalproj* np, const double* rhs, double* phi, double rtol, double atol,
+                         ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!np || !rhs) {
+        set_error("ksgf_nodalproj_solve: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int n = np->L[0].tot;
+    KSGF_HIP_CHECK(hipMemcpyAsync(np->rhs, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    int ret = nd_solve(np, rtol, atol, info, st);
+    if (ret < 0) return ret;
+    if (phi) KSGF_HIP_CHECK(hipMemcpyAsync(phi, np->phi[np->cur], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    return ret;
+}
+
+int ksgf_nodalproj_project(ksgf_nodalproj* np, double* vel, int ng_vel, double* gradphi, int ng_gp, double rtol,
+                           double atol, ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!np || !vel || ng_vel < 1 || ng_gp < 0) {
+        set_error("ksgf_nodalproj_project: bad argument (vel needs one ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int e = nd_compute_rhs(np, vel, ng_vel, np->rhs, st);
+    if (e) return e;
+    int ret = nd_solve(np, rtol, atol, info, st);
+    if (ret < 0) return ret;
+    MFDev v, g{};
+    if ((e = mfdev(np->ba, vel, KSGF_CELL, 3, ng_vel, &v))) return e;
+    if (gradphi && (e = mfdev(np->ba, gradphi, KSGF_CELL, 3, ng_gp, &g))) return e;
+    int err = KSGF_OK;
+    const int z[3] = {0, 0, 0};
+    const Region* r = get_region(np->ba, z, z, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_nd_correct, dim3(r->nchunks), dim3(kChunk), 0, st, v, g, gradphi ? 1 : 0,
+                       np->phi[np->cur], nd_geom(np), np->sigma, regdev(r, np->ba->nbox), np->ba->dbox);
+    KSGF_LAUNCH_CHECK();
+    return ret;
+}
+
+int ksgf_nodalproj_get_phi(ksgf_nodalproj* np, double* phi, ksgf_stream_t stream)
+{
+    if (!np || !phi) {
+        set_error("ksgf_nodalproj_get_phi: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    KSGF_HIP_CHECK(hipMemcpyAsync(phi, np->phi[np->cur], sizeof(double) * np->L[0].tot, hipMemcpyDeviceToDevice,
+                                  (hipStream_t)stream));
+    return KSGF_OK;
+}
+
+int ksgf_set_bndry(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow, double val,
+                   ksgf_stream_t stream)
+{
+    if (!ba || !mf || itype < 0 || itype > 3 || ncomp <= 0 || ngrow < 0) {
+        set_error("ksgf_set_bndry: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (ngrow == 0) return KSGF_OK;
+    MFDev m;
+    int e = mfdev(ba, mf, itype, ncomp, ngrow, &m);
+    if (e) return e;
+    int err = KSGF_OK;
+    const Region* r = fab_region(ba, itype, ngrow, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_set_bndry, dim3(r->nchunks), dim3(kChunk), 0, (hipStream_t)stream, m, ncomp, val,
+                       regdev(r, ba->nbox), ba->dbox);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+}  // extern "C"

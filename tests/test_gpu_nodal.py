@@ -1,0 +1,196 @@
+"""Nodal projection on the GPU (ksgf_nodalproj_*, ksgf_set_bndry) against the
+CPU FP64 oracle on identical seeded inputs.
+
+The right-hand side, the correction and setBndry evaluate the oracle's
+expressions and must agree bit for bit; the multigrid solve differs only
+through reduction order (weighted mean, BiCGStab dot products), so the V-cycle
+counts must be equal and phi, grad phi and the projected velocity agree to
+1e-12 relative.  The reference's own known answer
+(unit_tests/projection/test_pressure_offset.cpp) is checked through the GPU
+path too."""
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2, to_np
+
+pytestmark = pytest.mark.gpu
+
+lay = importlib.import_module("kynema-sgf_amd.layout")
+CELL = 0
+P_, N_, D_ = 0, 1, 2
+EPS = np.finfo(float).eps
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _apply_projection_ghosts(dba, d_vel, ng):
+    """vel.setBndry(0); vel.FillBoundary(periodicity) on the device"""
+    dba.set_bndry(d_vel, CELL, 3, ng, 0.0)
+    dba.fill_boundary(d_vel, CELL, 3, ng)
+
+
+def _oracle_ghosts(oracle, G, vel, ng):
+    ba = G.ba
+    for b in range(ba.nbox):
+        v = ba.fab_view(vel, b, CELL, 3, ng)
+        inner = v[:, ng:-ng, ng:-ng, ng:-ng].copy()
+        v[...] = 0.0
+        v[:, ng:-ng, ng:-ng, ng:-ng] = inner
+    oracle.fill_boundary(G, vel, CELL, 3, ng)
+
+
+def _case(name):
+    rng = np.random.default_rng(17)
+    if name == "periodic":
+        n = (32, 24, 16)
+        geom = lay.Geometry(n, prob_hi=(1.0, 0.75, 0.5), periodic=(1, 1, 1))
+        boxes = lay.chop_random(n, [8, 16], rng)
+        return geom, boxes, (P_, P_, P_), (P_, P_, P_), 0
+    if name == "abl_walls":
+        n = (32, 32, 16)
+        geom = lay.Geometry(n, prob_hi=(640.0, 640.0, 64.0), periodic=(1, 1, 0))
+        boxes = lay.chop_random(n, [8, 16], rng)
+        return geom, boxes, (P_, P_, N_), (P_, P_, N_), 1
+    if name == "outflow":
+        n = (16, 12, 20)
+        geom = lay.Geometry(n, prob_hi=(1.0, 0.75, 1.25), periodic=(0, 1, 0))
+        boxes = lay.chop_random(n, [4, 8], rng, granularity=4)
+        return geom, boxes, (N_, P_, N_), (N_, P_, D_), 0
+    raise ValueError(name)
+
+
+def _velocity(geom, seed):
+    rng = np.random.default_rng(seed)
+    n = geom.n_cell
+    x = [geom.cell_centers(d) / (geom.prob_hi[d] - geom.prob_lo[d]) for d in range(3)]
+    Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+    tp = 2 * np.pi
+    u = np.stack([np.sin(tp * X) * np.cos(tp * Y), np.cos(tp * Y) * np.sin(tp * Z), np.sin(tp * Z) * np.cos(tp * X)])
+    return u + 0.1 * rng.standard_normal((3, n[2], n[1], n[0]))
+
+
+@pytest.mark.parametrize("name", ["periodic", "abl_walls", "outflow"])
+def test_nodal_rhs_and_set_bndry_bitexact(ksgf, oracle, gpu, name):
+    _say(f"nodal rhs case={name}")
+    geom, boxes, lobc, hibc, semi = _case(name)
+    ba = lay.BoxArray(geom, boxes)
+    G = oracle.Geom(ba)
+    ng = 2
+    vel = ba.scatter(_velocity(geom, 3), CELL, ng)
+    # garbage in the ghosts: setBndry must clear them
+    for b in range(ba.nbox):
+        v = ba.fab_view(vel, b, CELL, 3, ng)
+        v[:, :1] = 7.0
+    dba = ksgf.DeviceBoxArray(ba)
+    d = dba.upload(vel.copy())
+    _apply_projection_ghosts(dba, d, ng)
+    _oracle_ghosts(oracle, G, vel, ng)
+    assert np.array_equal(to_np(d), vel)
+    npj = ksgf.NodalProjector(dba, lobc, hibc, 1.0, ksgf.default_options(do_semicoarsening=semi))
+    got = to_np(npj.compute_rhs(d, ng))
+    ref = oracle.nodal_rhs(G, vel, ng, lobc, hibc)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("name", ["periodic", "abl_walls", "outflow"])
+def test_nodal_project_tracks_oracle(ksgf, oracle, gpu, name):
+    _say(f"nodal project case={name}")
+    geom, boxes, lobc, hibc, semi = _case(name)
+    ba = lay.BoxArray(geom, boxes)
+    G = oracle.Geom(ba)
+    ng = 1
+    vel = ba.scatter(_velocity(geom, 5), CELL, ng)
+    _oracle_ghosts(oracle, G, vel, ng)
+    sigma = 0.37
+    rtol, atol = 1e-11, 1e-14
+    dba = ksgf.DeviceBoxArray(ba)
+    d = dba.upload(vel.copy())
+    gp = dba.alloc(CELL, 3, 0)
+    npj = ksgf.NodalProjector(dba, lobc, hibc, sigma, ksgf.default_options(do_semicoarsening=semi))
+    info = npj.project(d, ng, rtol, atol, gradphi=gp, ng_gp=0)
+    phi = to_np(npj.phi())
+    oinfo, ophi, ogp = oracle.nodal_project(G, vel, ng, lobc, hibc, sigma, rtol, atol,
+                                            oracle.default_opts(semicoarsening=semi))
+    _say(f"  gpu iters={info['iters']} oracle iters={oinfo['iters']}")
+    assert info["converged"] and oinfo["converged"]
+    assert info["iters"] == oinfo["iters"]
+    assert info["nlevels"] == oinfo["nlevels"]
+    singular = all(b != D_ for b in lobc + hibc)
+    if singular:
+        phi = phi - phi.mean()
+        ophi = ophi - ophi.mean()
+    assert rel_l2(phi, ophi) <= 1e-12
+    assert rel_l2(ba.gather(to_np(gp), CELL, 3, 0), ba.gather(ogp, CELL, 3, 0)) <= 1e-11
+    assert rel_l2(ba.gather(to_np(d), CELL, 3, ng), ba.gather(vel, CELL, 3, ng)) <= 1e-12
+
+
+def test_nodal_solve_tracks_oracle_semicoarsened(ksgf, oracle, gpu):
+    _say("nodal solve 64x64x32 dz=dx/5")
+    n = (64, 64, 32)
+    geom = lay.Geometry(n, prob_hi=(1280.0, 1280.0, 128.0), periodic=(1, 1, 0))
+    ba = lay.BoxArray(geom, lay.chop_uniform(n, 32))
+    dba = ksgf.DeviceBoxArray(ba)
+    lobc, hibc = (P_, P_, N_), (P_, P_, N_)
+    npj = ksgf.NodalProjector(dba, lobc, hibc, 1.0, ksgf.default_options(do_semicoarsening=1))
+    N = npj.N
+    assert N == (64, 64, 33)
+    b = np.random.default_rng(2).standard_normal(int(np.prod(N)))
+    import torch
+
+    phi = torch.zeros(int(np.prod(N)), dtype=torch.float64, device="cuda")
+    info = npj.solve(dba.upload(b), phi, 1e-11, 0.0)
+    ophi, oinfo = oracle.nodal_solve(n, geom.dx, lobc, hibc, 1.0, b, 1e-11, 0.0, oracle.default_opts(semicoarsening=1))
+    _say(f"  gpu iters={info['iters']} oracle iters={oinfo['iters']}")
+    assert info["converged"] and info["iters"] == oinfo["iters"]
+    g = to_np(phi)
+    assert rel_l2(g - g.mean(), ophi - ophi.mean()) <= 1e-12
+
+
+@pytest.mark.parametrize("w0", [0.0, -9.81])
+def test_pressure_offset_known_answer_gpu(ksgf, gpu, w0):
+    """unit_tests/projection/test_pressure_offset.cpp through the GPU path"""
+    _say(f"nodal pressure offset w0={w0}")
+    n = (2, 2, 16)
+    geom = lay.Geometry(n, prob_hi=(1.0, 1.0, 1.0), periodic=(0, 0, 0))
+    ba = lay.BoxArray(geom, lay.chop_uniform(n, 16))
+    dba = ksgf.DeviceBoxArray(ba)
+    ng = 3
+    u = np.zeros((3, n[2], n[1], n[0]))
+    u[2] = w0
+    flat = ba.scatter(u, CELL, ng)
+    for b in range(ba.nbox):  # ghosts hold w0 too, as the test's init_vel_z leaves them
+        ba.fab_view(flat, b, CELL, 3, ng)[2] = w0
+    d = dba.upload(flat)
+    _apply_projection_ghosts(dba, d, ng)
+    npj = ksgf.NodalProjector(dba, (N_, N_, N_), (N_, N_, D_), 1.0)
+    gp = dba.alloc(CELL, 3, 0)
+    info = npj.project(d, ng, EPS * 1e5, EPS * 1e2, gradphi=gp)
+    assert info["converged"]
+    p = to_np(npj.phi()).reshape(17, 3, 3)
+    assert abs(p[0].sum() / 9.0 - (-w0)) <= EPS * 1e8
+    g = ba.gather(to_np(gp), CELL, 3, 0)
+    assert np.abs(g[2] - w0).max() <= 1e-9
+    assert np.abs(ba.gather(to_np(d), CELL, 3, ng)).max() <= 1e-9
+
+
+def test_nodal_decomposition_independent_gpu(ksgf, gpu):
+    _say("nodal decomposition independence 32^3")
+    n = (32, 32, 32)
+    geom = lay.Geometry(n, periodic=(1, 1, 1))
+    u = _velocity(geom, 8)
+    outs = []
+    for mgs in (32, 8):
+        ba = lay.BoxArray(geom, lay.chop_uniform(n, mgs))
+        dba = ksgf.DeviceBoxArray(ba)
+        d = dba.upload(ba.scatter(u, CELL, 1))
+        _apply_projection_ghosts(dba, d, 1)
+        npj = ksgf.NodalProjector(dba, (P_,) * 3, (P_,) * 3, 0.5)
+        info = npj.project(d, 1, 1e-11, 0.0)
+        outs.append((to_np(npj.phi()), ba.gather(to_np(d), CELL, 3, 1), info["iters"]))
+    assert outs[0][2] == outs[1][2]
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
