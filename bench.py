@@ -6,7 +6,7 @@ correction); the headline value is the mixed path, the FP64 path is
 reported beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
-                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch]
+                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch|nodal]
 
 A step is one pass of the hot path over the whole grid, driven by the C++
 host layer (GodunovPredictor): convective CFL, source terms,
@@ -40,10 +40,12 @@ def parse():
     p.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
-    p.add_argument("--workload", choices=["abl", "poisson-batch"], default="abl",
+    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal"], default="abl",
                    help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
                         "N(0,1) right-hand sides, seed 5) solved as one batch (hipGraph replay and eager) and one "
-                        "after another; prints its own JSON line")
+                        "after another; nodal: the full incflo step (advect + MAC projection + ApplyProjection's "
+                        "nodal projection) on the same ABL box next to the step without it; each prints its own "
+                        "JSON line")
     p.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                    help="summarise four rocprofv3 --pmc output directories (FETCH_SIZE and WRITE_SIZE of a "
                         "1-step mixed run with KSGF_SWEEP_PLANES fixed, and of --copy-probe) into HBM bytes per "
@@ -278,6 +280,82 @@ def poisson_batch(steps, warmup):
     }
 
 
+def nodal_step(n, steps, warmup, no_cpu, cpu_n):
+    """The ABL step followed by incflo::ApplyProjection (nodal projection,
+    nodal_proj rtol/atol defaults, semicoarsening as the MAC solve), timed
+    beside the same step without it; the oracle's nodal projection on a
+    bounded sample is the CPU baseline."""
+    import torch
+
+    ksgf = importlib.import_module("kynema-sgf_amd")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    ext = (20.0 * n[0], 20.0 * n[1], 4.0 * n[2])
+    case = wl.abl(tuple(n), extent=ext, nsteps=steps)
+    eps = np.finfo(float).eps
+    res = {}
+    for nodal in (False, True):
+        opts = ksgf.default_options(**case.mlmg)
+        opts.precision = 1
+        S = hs.HostStepper(case, options=opts)
+        S.load()
+        if nodal:
+            S.enable_nodal_projection(eps * 1e5, eps * 1e2, ksgf.default_options(**case.mlmg))
+        for _ in range(warmup):
+            S.step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            S.step()
+            its.append(S.nodal_info()["iters"] if nodal else S.last_info["iters"])
+        torch.cuda.synchronize()
+        res[nodal] = ((time.perf_counter() - t) / steps, its)
+        S.close()
+    ms_full, nits = res[True][0] * 1e3, res[True][1]
+    ms_pred = res[False][0] * 1e3
+    out = {
+        "metric": "cell-updates/s per full incflo step (advect + MAC projection + nodal projection)",
+        "value": case.ncell / (ms_full * 1e-3),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms_full,
+        "higher_is_better": True,
+        "dtype": "f64+f32mg",
+        "data": "synthetic",
+        "config": {"workload": f"neutral ABL {n[0]}x{n[1]}x{n[2]}, dx=dy=20 m dz=4 m, uneven boxes {{32,48,64}}, "
+                               f"CFL 0.9, MAC mixed precision rtol 1e-11, nodal FP64 rtol eps*1e5",
+                   "nodal_vcycles": nits},
+        "ms_step_without_nodal": ms_pred,
+        "ms_nodal_projection": ms_full - ms_pred,
+    }
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.build()
+        ext = (20.0 * cpu_n[0], 20.0 * cpu_n[1], 4.0 * cpu_n[2])
+        c2 = wl.abl(tuple(cpu_n), extent=ext, nsteps=1)
+        G, vel, rho, gp = O.make_state(c2)
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            v = vel.copy()
+            g = gp.copy()
+            info, _ = O.apply_projection(c2, G, v, rho, g, 1.0, eps * 1e5, eps * 1e2)
+            reps += 1
+            if time.perf_counter() - t0 > 10.0 or reps >= 5:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": c2.ncell * reps / dt, "unit": "cell-updates/s (nodal projection only)",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": f"{reps} oracle nodal projections of the {cpu_n[0]}x{cpu_n[1]}x{cpu_n[2]} "
+                                         f"ABL box ({info['iters']} V-cycles), {dt:.2f} s"}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -292,6 +370,9 @@ def main():
         return
     if a.workload == "poisson-batch":
         print(json.dumps(poisson_batch(a.steps, a.warmup)))
+        return
+    if a.workload == "nodal":
+        print(json.dumps(nodal_step(tuple(a.n), a.steps, a.warmup, a.no_cpu, a.cpu_n)))
         return
     wl = importlib.import_module("kynema-sgf_amd.workloads")
     n = tuple(a.n)

@@ -281,6 +281,13 @@ int ksgf_nodalproj_solve(ksgf_nodalproj* np, const double* rhs, double* phi, dou
                          double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
 int ksgf_nodalproj_destroy(ksgf_nodalproj* np);
 
+/* The non-incremental "add (grad p / rho) back" loop of incflo::ApplyProjection
+ * (incflo_apply_nodal_projection.cpp:209-233, no mesh mapping): on valid cells
+ * vel += gp * (scaling_factor / rho).  vel, gp: cell, 3 comp; rho: 1 comp. */
+int ksgf_incflo_add_gradp(const ksgf_boxarray* ba, double* vel, int ng_vel, const double* gp,
+                          int ng_gp, const double* rho, int ng_rho, double scaling_factor,
+                          ksgf_stream_t stream);
+
 /* MultiFab::setBndry(val): every stored point outside the valid box of each
  * FAB (incflo_apply_nodal_projection.cpp:310). */
 int ksgf_set_bndry(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow, double val,

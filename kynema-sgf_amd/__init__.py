@@ -62,7 +62,7 @@ EXPORTS = [
     "ksgf_plotfile_write", "ksgf_plotfile_read", "ksgf_vismf_write", "ksgf_vismf_read",
     "ksgf_checkpoint_write_header", "ksgf_nodalproj_create", "ksgf_nodalproj_set_sigma", "ksgf_nodalproj_node_dims",
     "ksgf_nodalproj_project", "ksgf_nodalproj_get_phi", "ksgf_nodalproj_compute_rhs", "ksgf_nodalproj_solve",
-    "ksgf_nodalproj_destroy", "ksgf_set_bndry",
+    "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp",
 ]
 
 _lib = None
@@ -152,6 +152,8 @@ def load(path=None):
                                        C.c_void_p]
     L.ksgf_nodalproj_destroy.argtypes = [C.c_void_p]
     L.ksgf_set_bndry.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
+    L.ksgf_incflo_add_gradp.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
+                                        C.c_void_p]
     _lib = L
     return L
 

@@ -31,6 +31,7 @@
 // built without contraction; reductions (norm, weighted mean, BiCGStab dot
 // products) are blocked, so iterates agree with the oracle to rounding.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -143,58 +144,108 @@ __global__ void __launch_bounds__(256) k_nd_jacobi(double* __restrict__ xo, cons
         xo[c] = xi[c] + (b[c] - nd_apply_at(L, xi, i, j, k)) * L.wd;
 }
 
-// r = b - L x (zero on Dirichlet nodes)
-__global__ void __launch_bounds__(256) k_nd_resid(double* __restrict__ r, const double* __restrict__ x,
-                                                  const double* __restrict__ b, NLev L)
+// ---------------------------------------------------------------------------
+// z-marching stencil kernels.  The stencil's grouping is a sum over planes:
+// per plane the centre c, the x pair, the y pair and the four xy diagonals;
+// plane k contributes (c, x, y, xy) with weights (w0, w1, w2, w4) and planes
+// k-1 and k+1 contribute the sums of theirs with weights (w3, w5, w6, w7).
+// A thread owns one (i, j) column over kz planes and keeps the sums of three
+// planes in registers, so each node costs the 3 x 3 loads of one new plane
+// instead of 27.  The sums are formed in the oracle's operand order, so the
+// results are those of nd_apply_at bit for bit.
+// ---------------------------------------------------------------------------
+struct NdPlane {
+    double c, x, y, xy;
+};
+
+template <bool SUM2>
+__device__ __forceinline__ double nd_val(const double* __restrict__ a, const double* __restrict__ a2, int q)
 {
-    int i, j, k;
-    if (!nd_node(L, i, j, k)) return;
-    const int c = nd_idx(L, i, j, k);
-    r[c] = nd_dirichlet(L, i, j, k) ? 0.0 : b[c] - nd_apply_at(L, x, i, j, k);
+    return SUM2 ? a[q] + a2[q] : a[q];
 }
 
-// outer iteration: phi_o = phi_i + cor, r = rhs - L phi_o, |r|_inf.  phi_o is
-// read back through phi_i + cor at every stencil point, so the update and the
-// residual are one pass.
-__global__ void __launch_bounds__(256) k_nd_update_resid(double* __restrict__ phio, const double* __restrict__ phii,
-                                                         const double* __restrict__ cor, const double* __restrict__ rhs,
-                                                         double* __restrict__ r, NLev L,
-                                                         unsigned long long* __restrict__ norm)
+template <bool SUM2>
+__device__ __forceinline__ NdPlane nd_plane(const double* __restrict__ a, const double* __restrict__ a2, int z, int i,
+                                            int im, int ip, int y0, int ym, int yp)
 {
-    int i, j, k;
+    NdPlane s;
+    s.c = nd_val<SUM2>(a, a2, z + y0 + i);
+    s.x = nd_val<SUM2>(a, a2, z + y0 + im) + nd_val<SUM2>(a, a2, z + y0 + ip);
+    s.y = nd_val<SUM2>(a, a2, z + ym + i) + nd_val<SUM2>(a, a2, z + yp + i);
+    s.xy = (nd_val<SUM2>(a, a2, z + ym + im) + nd_val<SUM2>(a, a2, z + ym + ip)) +
+           (nd_val<SUM2>(a, a2, z + yp + im) + nd_val<SUM2>(a, a2, z + yp + ip));
+    return s;
+}
+
+__device__ __forceinline__ double nd_apply_planes(const NLev& L, const NdPlane& m, const NdPlane& o, const NdPlane& p)
+{
+    const double fz = m.c + p.c, exz = m.x + p.x, eyz = m.y + p.y, cr = m.xy + p.xy;
+    return (((L.w[0] * o.c + L.w[1] * o.x) + (L.w[2] * o.y + L.w[3] * fz)) +
+            ((L.w[4] * o.xy + L.w[5] * exz) + (L.w[6] * eyz + L.w[7] * cr)));
+}
+
+// MODE 0  Jacobi sweep:  out = x + (b - L x) * wd          (x2, r unused)
+// MODE 1  residual:      out = b - L x                      (x2, r unused)
+// MODE 2  outer update:  out = x + x2,  r = b - L (x + x2),  |r|_inf -> norm
+template <int MODE>
+__global__ void __launch_bounds__(256) k_nd_march(double* __restrict__ out, const double* __restrict__ x,
+                                                  const double* __restrict__ x2, const double* __restrict__ b,
+                                                  double* __restrict__ r, NLev L, int kz,
+                                                  unsigned long long* __restrict__ norm)
+{
+    constexpr bool SUM2 = (MODE == 2);
+    // 1-D grid of (x tile, y tile, z chunk), x fastest, remapped so that each
+    // XCD owns a contiguous run of rows and shares their halo rows in its L2
+    const int ntx = (L.N[0] + 63) >> 6, nty = (L.N[1] + 3) >> 2;
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = t % ntx, q = t / ntx;
+    const int i = tx * 64 + threadIdx.x;
+    const int j = (q % nty) * 4 + threadIdx.y;
+    const int k0 = (q / nty) * kz;
+    const int k1 = min(k0 + kz, L.N[2]);
     double m = 0.0;
-    if (nd_node(L, i, j, k)) {
-        const int c = nd_idx(L, i, j, k);
-        phio[c] = phii[c] + cor[c];
-        double v = 0.0;
-        if (!nd_dirichlet(L, i, j, k)) {
-            // L (phi + cor) evaluated on the summed values, as the oracle
-            // evaluates L phi after its update
-            const int im = nd_nb(i, -1, L.N[0], L.per[0]), ip = nd_nb(i, 1, L.N[0], L.per[0]);
-            const int jm = nd_nb(j, -1, L.N[1], L.per[1]), jp = nd_nb(j, 1, L.N[1], L.per[1]);
-            const int km = nd_nb(k, -1, L.N[2], L.per[2]), kp = nd_nb(k, 1, L.N[2], L.per[2]);
-            const int sy = L.N[0], sz = L.N[0] * L.N[1];
-            const int y0 = j * sy, ym = jm * sy, yp = jp * sy;
-            const int z0 = k * sz, zm = km * sz, zp = kp * sz;
-#define X(a, b, c) (phii[(a) + (b) + (c)] + cor[(a) + (b) + (c)])
-            const double fx = X(im, y0, z0) + X(ip, y0, z0);
-            const double fy = X(i, ym, z0) + X(i, yp, z0);
-            const double fz = X(i, y0, zm) + X(i, y0, zp);
-            const double exy = (X(im, ym, z0) + X(ip, ym, z0)) + (X(im, yp, z0) + X(ip, yp, z0));
-            const double exz = (X(im, y0, zm) + X(ip, y0, zm)) + (X(im, y0, zp) + X(ip, y0, zp));
-            const double eyz = (X(i, ym, zm) + X(i, yp, zm)) + (X(i, ym, zp) + X(i, yp, zp));
-            const double cr = ((X(im, ym, zm) + X(ip, ym, zm)) + (X(im, yp, zm) + X(ip, yp, zm))) +
-                              ((X(im, ym, zp) + X(ip, ym, zp)) + (X(im, yp, zp) + X(ip, yp, zp)));
-            const double c0 = X(i, y0, z0);
-#undef X
-            const double ax = (((L.w[0] * c0 + L.w[1] * fx) + (L.w[2] * fy + L.w[3] * fz)) +
-                               ((L.w[4] * exy + L.w[5] * exz) + (L.w[6] * eyz + L.w[7] * cr)));
-            v = rhs[c] - ax;
+    if (i < L.N[0] && j < L.N[1]) {
+        const int im = nd_nb(i, -1, L.N[0], L.per[0]), ip = nd_nb(i, 1, L.N[0], L.per[0]);
+        const int jm = nd_nb(j, -1, L.N[1], L.per[1]), jp = nd_nb(j, 1, L.N[1], L.per[1]);
+        const int sy = L.N[0], sz = L.N[0] * L.N[1];
+        const int y0 = j * sy, ym = jm * sy, yp = jp * sy;
+        const bool dxy = (L.dlo[0] && i == 0) || (L.dhi[0] && i == L.N[0] - 1) || (L.dlo[1] && j == 0) ||
+                         (L.dhi[1] && j == L.N[1] - 1);
+        NdPlane pm = nd_plane<SUM2>(x, x2, nd_nb(k0, -1, L.N[2], L.per[2]) * sz, i, im, ip, y0, ym, yp);
+        NdPlane po = nd_plane<SUM2>(x, x2, k0 * sz, i, im, ip, y0, ym, yp);
+        // two planes per step: both planes' loads are issued before either
+        // node is formed (a store between them would otherwise make the next
+        // plane's loads wait for it)
+        auto node = [&](int k, const NdPlane& a, const NdPlane& o, const NdPlane& p, double bc) {
+            const int c = k * sz + y0 + i;
+            const bool dir = dxy || (L.dlo[2] && k == 0) || (L.dhi[2] && k == L.N[2] - 1);
+            if (MODE == 0) {
+                out[c] = dir ? 0.0 : o.c + (bc - nd_apply_planes(L, a, o, p)) * L.wd;
+            } else if (MODE == 1) {
+                out[c] = dir ? 0.0 : bc - nd_apply_planes(L, a, o, p);
+            } else {
+                out[c] = o.c;
+                const double v = dir ? 0.0 : bc - nd_apply_planes(L, a, o, p);
+                r[c] = v;
+                m = fmax(m, nd_abs_or_inf(v));
+            }
+        };
+        int k = k0;
+        for (; k + 1 < k1; k += 2) {
+            const NdPlane pp = nd_plane<SUM2>(x, x2, nd_nb(k, 1, L.N[2], L.per[2]) * sz, i, im, ip, y0, ym, yp);
+            const NdPlane pq = nd_plane<SUM2>(x, x2, nd_nb(k + 1, 1, L.N[2], L.per[2]) * sz, i, im, ip, y0, ym, yp);
+            const double b0 = b[k * sz + y0 + i], b1 = b[(k + 1) * sz + y0 + i];
+            node(k, pm, po, pp, b0);
+            node(k + 1, po, pp, pq, b1);
+            pm = pp;
+            po = pq;
         }
-        r[c] = v;
-        m = nd_abs_or_inf(v);
+        if (k < k1) {
+            const NdPlane pp = nd_plane<SUM2>(x, x2, nd_nb(k, 1, L.N[2], L.per[2]) * sz, i, im, ip, y0, ym, yp);
+            node(k, pm, po, pp, b[k * sz + y0 + i]);
+        }
     }
-    nd_block_max(m, norm);
+    if (MODE == 2) nd_block_max(m, norm);
 }
 
 // |a|_inf over a level (Dirichlet nodes hold zero)
@@ -613,6 +664,20 @@ __global__ void __launch_bounds__(256) k_set_bndry(MFDev m, int ncomp, double va
     for (int n = 0; n < ncomp; ++n) f(i, j, k, n) = val;
 }
 
+// vel += gp * (scale / rho) on valid cells (the non-incremental branch of
+// ApplyProjection, incflo_apply_nodal_projection.cpp:209-233, without mesh
+// mapping: fac = 1)
+__global__ void __launch_bounds__(256) k_add_gradp(MFDev vel, CMFDev gp, CMFDev rho, double scale, RegDev r,
+                                                   const int* __restrict__ box)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    Fab fv = fab_at(vel, box, b);
+    CFab fg = fab_at(gp, box, b), fr = fab_at(rho, box, b);
+    const double soverrho = scale / fr(i, j, k);
+    for (int n = 0; n < 3; ++n) fv(i, j, k, n) += fg(i, j, k, n) * soverrho;
+}
+
 }  // namespace ksgf
 
 // ---------------------------------------------------------------------------
@@ -690,6 +755,29 @@ static void nd_set_coefs(ksgf_nodalproj* np)
 static dim3 nd_grid(const NLev& L) { return dim3((L.N[0] + 63) / 64, (L.N[1] + 3) / 4, L.N[2]); }
 static const dim3 kNdBlock(64, 4, 1);
 
+// planes per thread of the z-marching kernels: up to kz_max, keeping about
+// 2048 workgroups or more on the large levels (KSGF_ND_KZ overrides kz_max)
+static int nd_kz(const NLev& L)
+{
+    static const int kz_max = [] {
+        const char* e = std::getenv("KSGF_ND_KZ");
+        const int v = e ? std::atoi(e) : 0;
+        return (v >= 1 && v <= 64) ? v : 16;
+    }();
+    const long long bxy = (long long)((L.N[0] + 63) / 64) * ((L.N[1] + 3) / 4);
+    long long kz = (long long)L.N[2] * bxy / 2048;
+    return (int)std::max(1LL, std::min((long long)kz_max, kz));
+}
+
+template <int MODE>
+static void nd_launch_march(double* out, const double* x, const double* x2, const double* b, double* r, const NLev& L,
+                            unsigned long long* norm, hipStream_t st)
+{
+    const int kz = nd_kz(L);
+    const long long nb = (long long)((L.N[0] + 63) / 64) * ((L.N[1] + 3) / 4) * ((L.N[2] + kz - 1) / kz);
+    hipLaunchKernelGGL(k_nd_march<MODE>, dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz, norm);
+}
+
 static int nd_alloc(ksgf_nodalproj* np, double** p, size_t n)
 {
     if (hipMalloc((void**)p, sizeof(double) * std::max<size_t>(n, 1)) != hipSuccess) {
@@ -729,7 +817,7 @@ static void nd_smooth(ksgf_nodalproj* np, int l, double* x, const double* b, int
         if (s == 0 && zero)
             hipLaunchKernelGGL(k_nd_jacobi<true>, gr, kNdBlock, 0, st, dst, src, b, L);
         else
-            hipLaunchKernelGGL(k_nd_jacobi<false>, gr, kNdBlock, 0, st, dst, src, b, L);
+            nd_launch_march<0>(dst, src, nullptr, b, nullptr, L, nullptr, st);
         src = dst;
         dst = (src == x) ? t : x;
     }
@@ -748,7 +836,7 @@ static void nd_vcycle(ksgf_nodalproj* np, int l, hipStream_t st)
     const NLev& L = np->L[l];
     dim3 gr = nd_grid(L);
     nd_smooth(np, l, np->cor[l], np->res[l], kNdSweeps * o.num_pre_smooth, true, st);
-    hipLaunchKernelGGL(k_nd_resid, gr, kNdBlock, 0, st, np->tmp[l], np->cor[l], np->res[l], L);
+    nd_launch_march<1>(np->tmp[l], np->cor[l], nullptr, np->res[l], nullptr, L, nullptr, st);
     hipLaunchKernelGGL(k_nd_restrict, nd_grid(np->L[l + 1]), kNdBlock, 0, st, np->res[l + 1], np->tmp[l], L,
                        np->L[l + 1]);
     nd_vcycle(np, l + 1, st);
@@ -795,8 +883,8 @@ static int nd_solve(ksgf_nodalproj* np, double rtol, double atol, ksgf_mlmg_info
         for (int it = 1; it <= np->opts.maxiter; ++it) {
             nd_vcycle(np, 0, st);
             hipLaunchKernelGGL(k_nd_reset, dim3(1), dim3(1), 0, st, np->norm);
-            hipLaunchKernelGGL(k_nd_update_resid, gr, kNdBlock, 0, st, np->phi[1 - np->cur], np->phi[np->cur],
-                               np->cor[0], np->rhs, np->res[0], L0, np->norm);
+            nd_launch_march<2>(np->phi[1 - np->cur], np->phi[np->cur], np->cor[0], np->rhs, np->res[0], L0, np->norm,
+                               st);
             KSGF_LAUNCH_CHECK();
             np->cur = 1 - np->cur;
             e = nd_read_norm(np, st, &r);
@@ -1088,6 +1176,29 @@ int ksgf_nodalproj_get_phi(ksgf_nodalproj* np, double* phi, ksgf_stream_t stream
     }
     KSGF_HIP_CHECK(hipMemcpyAsync(phi, np->phi[np->cur], sizeof(double) * np->L[0].tot, hipMemcpyDeviceToDevice,
                                   (hipStream_t)stream));
+    return KSGF_OK;
+}
+
+int ksgf_incflo_add_gradp(const ksgf_boxarray* ba, double* vel, int ng_vel, const double* gp, int ng_gp,
+                          const double* rho, int ng_rho, double scaling_factor, ksgf_stream_t stream)
+{
+    if (!ba || !vel || !gp || !rho || ng_vel < 0 || ng_gp < 0 || ng_rho < 0) {
+        set_error("ksgf_incflo_add_gradp: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    MFDev v;
+    CMFDev g, rr;
+    int e;
+    if ((e = mfdev(ba, vel, KSGF_CELL, 3, ng_vel, &v)) || (e = cmfdev(ba, gp, KSGF_CELL, 3, ng_gp, &g)) ||
+        (e = cmfdev(ba, rho, KSGF_CELL, 1, ng_rho, &rr)))
+        return e;
+    int err = KSGF_OK;
+    const int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_add_gradp, dim3(r->nchunks), dim3(kChunk), 0, (hipStream_t)stream, v, g, rr, scaling_factor,
+                       regdev(r, ba->nbox), ba->dbox);
+    KSGF_LAUNCH_CHECK();
     return KSGF_OK;
 }
 

@@ -16,6 +16,11 @@
 //       ComputeDivergence (mult -1)    icns_advection.H:418-422
 //   compute_source_term(NPH) + ComputeRHSOp::predictor_rhs
 //                                      icns_ops.H:84-121, CompRHSOps.H:93-117
+// With enableNodalProjection the step ends with incflo::ApplyProjection
+// (incflo_advance.cpp:412-414; incflo_apply_nodal_projection.cpp:149-465,
+// constant density, not incremental, no small-dt variant):
+//   vel += dt gp / rho, vel.setBndry(0), FillBoundary, NodalProjector with
+//   sigma = dt / rho_0, project, gp = grad phi, p = phi, velocity fillpatch
 // With a passive scalar (enableScalar), the scalar equation runs between the
 // MAC projection and the momentum advection, as ApplyPredictor orders the
 // scalar equations (incflo_advance.cpp:316-324):
@@ -37,6 +42,7 @@ namespace ksgf::amr {
 struct StepStats {
     double dt = 0.0;
     ksgf_mlmg_info mlmg{};
+    ksgf_mlmg_info nodal{};  // the nodal projection's solve (when enabled)
     double divmax = -1.0;  // max |div u_mac| after the projection (when asked for)
 };
 
@@ -94,6 +100,8 @@ public:
         mac_proj_.initProjector({&ba_}, info, 1.0 / rho0_);
         mac_proj_.setDomainBC(lobc, hibc);
         mac_proj_.setUMAC({{&umac_, &vmac_, &wmac_}});
+        lobc_ = lobc;
+        hibc_ = hibc;
     }
 
     void setStream(hipStream_t st) { st_ = st; }
@@ -140,6 +148,42 @@ public:
     MultiFab& wmac() { return wmac_; }
 
     void setCheckDivergence(bool on) { check_div_ = on; }
+
+    // end every step with the nodal projection (nodal_proj.mg_rtol / mg_atol);
+    // its domain BCs are the MAC projection's (get_projection_bc maps the
+    // pressure BCs the same way, incflo_apply_nodal_projection.cpp:28-58)
+    void enableNodalProjection(double rtol, double atol, const ksgf_mlmg_options& opts)
+    {
+        Hydro::LPInfo info;
+        info.do_semicoarsening = opts.do_semicoarsening != 0;
+        info.max_coarsening_level = opts.max_coarsening_level;
+        nodal_ = std::make_unique<Hydro::NodalProjector>(std::vector<MultiFab*>{vel_.get()}, 1.0,
+                                                         std::vector<Geometry>{ba_.geom()}, info);
+        nodal_->getMLMG().opts = opts;
+        nodal_->setDomainBC(lobc_, hibc_);
+        nodal_rtol_ = rtol;
+        nodal_atol_ = atol;
+    }
+    // the last nodal projection's phi (domain-wide nodes, host) and gp (cell, host)
+    void getNodalPhi(double* phi)
+    {
+        if (!nodal_) throw std::runtime_error("getNodalPhi: nodal projection not enabled");
+        const auto N = nodal_->nodeDims();
+        const size_t n = (size_t)N[0] * N[1] * N[2];
+        double* d = nullptr;
+        if (hipMalloc((void**)&d, n * sizeof(double)) != hipSuccess) throw std::runtime_error("getNodalPhi: hipMalloc");
+        nodal_->getPhi(d, st_);
+        const hipError_t e = hipMemcpyAsync(phi, d, n * sizeof(double), hipMemcpyDeviceToHost, st_);
+        (void)hipStreamSynchronize(st_);
+        (void)hipFree(d);
+        if (e != hipSuccess) throw std::runtime_error("getNodalPhi: copy");
+    }
+    const ksgf_mlmg_info& lastNodalInfo() const { return last_nodal_; }
+    void getGradP(double* gp) const
+    {
+        (void)hipStreamSynchronize(st_);
+        gp_.copyToDomain(gp);
+    }
 
     // passive scalar: one component, ghost 3, its own BCRec and values
     // (6 orientations)
@@ -242,10 +286,28 @@ public:
         vel_new_->FillBoundary(st_);
         vel_new_->FillPhysBC(vel_bc_, vel_bcval_, st_);
         std::swap(vel_, vel_new_);
+        if (nodal_) applyProjection(dt, s);
         return s;
     }
 
 private:
+    void applyProjection(double dt, StepStats& s)
+    {
+        const ksgf_boxarray* h = ba_.handle();
+        ksgf_stream_t st = (ksgf_stream_t)st_;
+        check(ksgf_incflo_add_gradp(h, vel_->data(), 3, gp_.data(), 0, rho_.data(), 3, dt, st),
+              "ApplyProjection: vel += dt gp / rho");
+        check(ksgf_set_bndry(h, vel_->data(), KSGF_CELL, 3, 3, 0.0, st), "ApplyProjection: setBndry");
+        vel_->FillBoundary(st_);
+        nodal_->setVelocity(vel_.get());
+        nodal_->setSigma(dt / rho0_);
+        nodal_->project(nodal_rtol_, nodal_atol_, &gp_, st_);  // gp = grad phi
+        s.nodal = nodal_->getMLMG().last;
+        last_nodal_ = s.nodal;
+        vel_->FillBoundary(st_);
+        vel_->FillPhysBC(vel_bc_, vel_bcval_, st_);
+    }
+
     struct Scalar {
         explicit Scalar(const BoxArray& ba)
             : s(std::make_unique<MultiFab>(ba, 1, 3)),
@@ -288,6 +350,10 @@ private:
     }
 
     std::unique_ptr<Scalar> sc_;
+    std::unique_ptr<Hydro::NodalProjector> nodal_;
+    double nodal_rtol_ = 0.0, nodal_atol_ = 0.0;
+    ksgf_mlmg_info last_nodal_{};
+    std::array<Hydro::LinOpBCType, 3> lobc_{}, hibc_{};
     BoxArray ba_;
     std::vector<BCRec> vel_bc_, src_bc_, rho_bc_;
     std::vector<double> vel_bcval_;

@@ -170,6 +170,58 @@ int ksgf_host_stepper_write_plotfile(ksgf_host_stepper* s, const char* dir, doub
     }
 }
 
+int ksgf_host_stepper_enable_nodal_projection(ksgf_host_stepper* s, double rtol, double atol,
+                                              const ksgf_mlmg_options* opts)
+{
+    try {
+        if (!s) throw std::runtime_error("null stepper");
+        ksgf_mlmg_options o;
+        if (opts)
+            o = *opts;
+        else
+            ksgf_mlmg_default_options(&o);
+        reinterpret_cast<GodunovPredictor*>(s)->enableNodalProjection(rtol, atol, o);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_enable_nodal_projection: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_nodal_info(ksgf_host_stepper* s, ksgf_mlmg_info* info)
+{
+    if (!s || !info) {
+        g_err = "ksgf_host_stepper_nodal_info: null argument";
+        return KSGF_ERR_INVALID;
+    }
+    *info = reinterpret_cast<GodunovPredictor*>(s)->lastNodalInfo();
+    return KSGF_OK;
+}
+
+int ksgf_host_stepper_get_gradp(ksgf_host_stepper* s, double* gp)
+{
+    try {
+        if (!s || !gp) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->getGradP(gp);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_get_gradp: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_get_nodal_phi(ksgf_host_stepper* s, double* phi)
+{
+    try {
+        if (!s || !phi) throw std::runtime_error("null argument");
+        reinterpret_cast<GodunovPredictor*>(s)->getNodalPhi(phi);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_get_nodal_phi: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
 {
     delete reinterpret_cast<GodunovPredictor*>(s);

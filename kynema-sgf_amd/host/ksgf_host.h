@@ -36,6 +36,15 @@ int ksgf_host_stepper_get_scalar(ksgf_host_stepper* s, double* scal);
 /* plotfile of the step's fields (IOManager::write_plot_file): velocity, gp,
  * density (and the scalar when enabled) */
 int ksgf_host_stepper_write_plotfile(ksgf_host_stepper* s, const char* dir, double time, int step);
+/* end every step with incflo::ApplyProjection (nodal projection, constant
+ * density; incflo_advance.cpp:412-414) to nodal_proj rtol/atol with opts
+ * (NULL: defaults); nodal_info: the last step's nodal solve */
+int ksgf_host_stepper_enable_nodal_projection(ksgf_host_stepper* s, double rtol, double atol,
+                                              const ksgf_mlmg_options* opts);
+int ksgf_host_stepper_nodal_info(ksgf_host_stepper* s, ksgf_mlmg_info* info);
+/* gp (3*N, domain-wide host) and the nodal phi (domain-wide nodes, host) */
+int ksgf_host_stepper_get_gradp(ksgf_host_stepper* s, double* gp);
+int ksgf_host_stepper_get_nodal_phi(ksgf_host_stepper* s, double* phi);
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
 
 #ifdef __cplusplus

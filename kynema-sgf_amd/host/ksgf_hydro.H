@@ -389,4 +389,86 @@ private:
     ksgf_macproj* mp_ = nullptr;
 };
 
+// Hydro::NodalProjector with a constant sigma, as incflo::ApplyProjection
+// builds it for constant density (incflo_apply_nodal_projection.cpp:340-353):
+// NodalProjector(vel, scaling_factor / rho_0, geom, lpinfo), setDomainBC,
+// the MLMG options, project(rtol, atol), getPhi, getGradPhi.  vel is held by
+// pointer (the reference keeps non-owning pointers too); phi is a domain-wide
+// node array (N_d = n_d along a periodic direction, n_d + 1 otherwise).
+class NodalProjector {
+public:
+    NodalProjector(const std::vector<ksgf::amr::MultiFab*>& vel, double sigma,
+                   const std::vector<ksgf::amr::Geometry>& geom, const LPInfo& info)
+        : vel_(vel.at(0)), sigma_(sigma)
+    {
+        if (geom.size() != 1 || vel.size() != 1) throw std::runtime_error("NodalProjector: one level on this path");
+        ksgf_mlmg_default_options(&mlmg_.opts);
+        mlmg_.opts.do_semicoarsening = info.do_semicoarsening ? 1 : 0;
+        mlmg_.opts.max_coarsening_level = info.max_coarsening_level;
+    }
+    ~NodalProjector()
+    {
+        if (np_) ksgf_nodalproj_destroy(np_);
+    }
+    NodalProjector(const NodalProjector&) = delete;
+    NodalProjector& operator=(const NodalProjector&) = delete;
+
+    void setDomainBC(const std::array<LinOpBCType, 3>& lo, const std::array<LinOpBCType, 3>& hi)
+    {
+        for (int d = 0; d < 3; ++d) {
+            lo_[d] = (int)lo[d];
+            hi_[d] = (int)hi[d];
+        }
+        build();
+    }
+    void setSigma(double sigma)
+    {
+        sigma_ = sigma;
+        if (np_) ksgf::amr::check(ksgf_nodalproj_set_sigma(np_, sigma), "NodalProjector::setSigma");
+    }
+    // gradphi: cell MultiFab, 3 comp (may be null: not kept)
+    void project(double rtol, double atol, ksgf::amr::MultiFab* gradphi = nullptr, hipStream_t st = nullptr)
+    {
+        if (!np_) build();
+        int rc = ksgf_nodalproj_project(np_, vel_->data(), vel_->nGrow(), gradphi ? gradphi->data() : nullptr,
+                                        gradphi ? gradphi->nGrow() : 0, rtol, atol, &mlmg_.last, (ksgf_stream_t)st);
+        ksgf::amr::check(rc, "NodalProjector::project");
+    }
+    MLMG& getMLMG() { return mlmg_; }
+    // point the projector at another velocity MultiFab on the same BoxArray
+    // (the reference builds a new NodalProjector per projection; this keeps
+    // the hierarchy and re-targets it)
+    void setVelocity(ksgf::amr::MultiFab* vel)
+    {
+        if (vel->boxArray().handle() != vel_->boxArray().handle())
+            throw std::runtime_error("NodalProjector: other BoxArray");
+        vel_ = vel;
+    }
+    void getPhi(double* phi, hipStream_t st = nullptr)
+    {
+        if (!np_) throw std::runtime_error("NodalProjector::getPhi: no projection yet");
+        ksgf::amr::check(ksgf_nodalproj_get_phi(np_, phi, (ksgf_stream_t)st), "NodalProjector::getPhi");
+    }
+    std::array<int, 3> nodeDims() const
+    {
+        std::array<int, 3> n{};
+        if (np_) ksgf::amr::check(ksgf_nodalproj_node_dims(np_, n.data()), "NodalProjector::nodeDims");
+        return n;
+    }
+
+private:
+    void build()
+    {
+        if (np_) ksgf_nodalproj_destroy(np_);
+        np_ = nullptr;
+        ksgf::amr::check(ksgf_nodalproj_create(vel_->boxArray().handle(), lo_, hi_, sigma_, &mlmg_.opts, &np_),
+                         "NodalProjector");
+    }
+    ksgf::amr::MultiFab* vel_;
+    double sigma_;
+    int lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
+    MLMG mlmg_;
+    ksgf_nodalproj* np_ = nullptr;
+};
+
 }  // namespace Hydro

@@ -1,6 +1,7 @@
 """ctypes binding of libksgf_host.so: the C++ GodunovPredictor (the Godunov
-branch of incflo::ApplyPredictor without diffusion and nodal projection)
-driven from Python.  Work goes on torch's current stream."""
+branch of incflo::ApplyPredictor without diffusion, optionally followed by
+incflo::ApplyProjection) driven from Python.  Work goes on torch's current
+stream."""
 from __future__ import annotations
 
 import ctypes as C
@@ -37,6 +38,11 @@ def host_lib():
         L.ksgf_host_stepper_set_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_write_plotfile.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_int]
+        L.ksgf_host_stepper_enable_nodal_projection.argtypes = [C.c_void_p, C.c_double, C.c_double,
+                                                                C.POINTER(KsgfMLMGOptions)]
+        L.ksgf_host_stepper_nodal_info.argtypes = [C.c_void_p, C.POINTER(KsgfMLMGInfo)]
+        L.ksgf_host_stepper_get_gradp.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_get_nodal_phi.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
         _h = L
     return _h
@@ -124,6 +130,32 @@ class HostStepper:
         n = self.case.geom.n_cell
         out = np.zeros((n[2], n[1], n[0]))
         _chk(host_lib().ksgf_host_stepper_get_phi(self.h, _dp(out)), "get_phi")
+        return out
+
+    def enable_nodal_projection(self, rtol, atol, options=None):
+        """end every step with incflo::ApplyProjection (constant density)"""
+        self._nopts = options or default_options(**self.case.mlmg)
+        _chk(host_lib().ksgf_host_stepper_enable_nodal_projection(self.h, rtol, atol, C.byref(self._nopts)),
+             "enable_nodal_projection")
+        self.nodal = True
+
+    def nodal_info(self):
+        info = KsgfMLMGInfo()
+        _chk(host_lib().ksgf_host_stepper_nodal_info(self.h, C.byref(info)), "nodal_info")
+        return info.as_dict()
+
+    def gradp(self):
+        n = self.case.geom.n_cell
+        out = np.zeros((3, n[2], n[1], n[0]))
+        _chk(host_lib().ksgf_host_stepper_get_gradp(self.h, _dp(out)), "get_gradp")
+        return out
+
+    def nodal_phi(self):
+        """the last nodal projection's phi, domain-wide nodes (flat, x fastest)"""
+        n = self.case.geom.n_cell
+        N = [n[d] if self.case.lobc[d] == 0 else n[d] + 1 for d in range(3)]
+        out = np.zeros(N[0] * N[1] * N[2])
+        _chk(host_lib().ksgf_host_stepper_get_nodal_phi(self.h, _dp(out)), "get_nodal_phi")
         return out
 
     def write_plotfile(self, path, time=0.0, step=0):

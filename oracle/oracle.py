@@ -410,3 +410,27 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
     predictor_update(G, vel_new, 3, vel, 3, rho, 3, rho, 3, conv, src_nph, None, 0.0, dt)
     fill_ghosts(G, vel_new, CELL, 3, 3, case.bcrec, case.bcval)
     return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face, "scal": scal_new}
+
+
+def apply_projection(case, G, vel, rho, gp, dt, rtol, atol, opts=None):
+    """incflo::ApplyProjection, constant density, not incremental
+    (incflo_apply_nodal_projection.cpp:149-465): vel += gp * (dt / rho) on
+    valid cells, vel.setBndry(0) + FillBoundary, nodal projection with
+    sigma = dt / rho_0, gp = grad phi, velocity ghosts refilled.  vel (ng 3)
+    and gp (ng 0) are updated in place; returns (info, phi)."""
+    ba = case.ba
+    opts = opts or case_opts(case)
+    for b in range(ba.nbox):
+        v = ba.fab_view(vel, b, CELL, 3, 3)
+        g = ba.fab_view(gp, b, CELL, 3, 0)
+        r = ba.fab_view(rho, b, CELL, 1, 3)
+        vv = v[:, 3:-3, 3:-3, 3:-3]
+        vv += g * (dt / r[:, 3:-3, 3:-3, 3:-3])
+        inner = vv.copy()
+        v[...] = 0.0
+        v[:, 3:-3, 3:-3, 3:-3] = inner
+    fill_boundary(G, vel, CELL, 3, 3)
+    info, phi, gpn = nodal_project(G, vel, 3, case.lobc, case.hibc, dt / case.rho0, rtol, atol, opts)
+    gp[:] = gpn
+    fill_ghosts(G, vel, CELL, 3, 3, case.bcrec, case.bcval)
+    return info, phi

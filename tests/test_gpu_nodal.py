@@ -194,3 +194,32 @@ def test_nodal_decomposition_independent_gpu(ksgf, gpu):
     assert outs[0][2] == outs[1][2]
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("which", ["tgv32", "abl"])
+def test_host_step_with_nodal_projection(ksgf, oracle, workloads, gpu, which):
+    """GodunovPredictor with ApplyProjection at the end of each step (the C++
+    host layer) against the oracle's step followed by its ApplyProjection"""
+    _say(f"host step + nodal projection case={which}")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    if which == "tgv32":
+        case = workloads.taylor_green(32, 16)
+    else:
+        case = workloads.abl((64, 48, 32), extent=(1280.0, 960.0, 128.0), sizes=(16, 32), nsteps=2)
+    G, vel, rho, gp = oracle.make_state(case)
+    S = hs.HostStepper(case)
+    S.load()
+    rtol, atol = EPS * 1e5, EPS * 1e2  # nodal_proj defaults (MLMGOptions.H:61-65)
+    S.enable_nodal_projection(rtol, atol)
+    for n in range(2):
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp)
+        oinfo, ophi = oracle.apply_projection(case, G, vel, rho, gp, diag["dt"], rtol, atol)
+        S.step()
+        ninfo = S.nodal_info()
+        _say(f"  step {n}: nodal iters gpu={ninfo['iters']} oracle={oinfo['iters']}")
+        assert ninfo["converged"] and ninfo["iters"] == oinfo["iters"]
+        assert rel_l2(S.velocity(), case.ba.gather(vel, CELL, 3, 3)) <= 1e-11, f"step {n}"
+        assert rel_l2(S.gradp(), case.ba.gather(gp, CELL, 3, 0)) <= 1e-10, f"step {n}"
+        p = S.nodal_phi()
+        assert rel_l2(p - p.mean(), ophi - ophi.mean()) <= 1e-10, f"step {n}"
+    S.close()
