@@ -251,6 +251,31 @@ int ksgf_poisson_batch_solve(ksgf_poisson_batch* pb, const double* const* rhs,
                              ksgf_mlmg_info* infos, ksgf_stream_t stream);
 int ksgf_poisson_batch_destroy(ksgf_poisson_batch* pb);
 
+/* amrex::MLABecLaplacian with constant coefficients solved by MLMG: the
+ * implicit (or Crank-Nicolson) diffusion solve of DiffSolverIface::linsys_solve
+ * (src/equation_systems/DiffusionOps.cpp:154-160: setScalars(1, dt_diff),
+ * setACoeffs(rho), setBCoeffs(mu on faces), MLMG::solve(field, rho*field)),
+ * one component at a time as ICNSDiffScalarOp solves the velocity
+ * (icns_diffusion.H:60-140):  alpha a phi - beta b lap(phi) = rhs.
+ * Domain BCs are KSGF_LO_* per direction (get_diffuse_tensor_bc); Dirichlet
+ * sides are homogeneous with the value on the face (walls at rest).  The MAC
+ * projector's multigrid with the diagonal grown by alpha a; phi starts from
+ * zero.  solve: sol and rhs are cell MultiFabs of ncomp components (valid
+ * cells of component comp are read / written).  apply = MLMG::apply
+ * (compute_diff_term uses alpha = 0, beta = -1: out = b lap(in)); apply
+ * reuses the solve's level-0 scratch. */
+typedef struct ksgf_abeclap ksgf_abeclap;
+int ksgf_abeclap_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, double alpha,
+                        double a, double beta, double b, const ksgf_mlmg_options* opts,
+                        ksgf_abeclap** out);
+int ksgf_abeclap_set_scalars(ksgf_abeclap* ab, double alpha, double beta);
+int ksgf_abeclap_solve(ksgf_abeclap* ab, double* sol, int ng_sol, const double* rhs, int ng_rhs,
+                       int ncomp, int comp, double rtol, double atol, ksgf_mlmg_info* info,
+                       ksgf_stream_t stream);
+int ksgf_abeclap_apply(ksgf_abeclap* ab, double* out, int ng_out, const double* in, int ng_in,
+                       int ncomp, int comp, ksgf_stream_t stream);
+int ksgf_abeclap_destroy(ksgf_abeclap* ab);
+
 /* Hydro::NodalProjector with a constant sigma (incflo::ApplyProjection,
  * src/projection/incflo_apply_nodal_projection.cpp:340-353 for the constant-
  * density branch, options from MLMGOptions::operator()(NodalProjector&),

@@ -62,7 +62,8 @@ EXPORTS = [
     "ksgf_plotfile_write", "ksgf_plotfile_read", "ksgf_vismf_write", "ksgf_vismf_read",
     "ksgf_checkpoint_write_header", "ksgf_nodalproj_create", "ksgf_nodalproj_set_sigma", "ksgf_nodalproj_node_dims",
     "ksgf_nodalproj_project", "ksgf_nodalproj_get_phi", "ksgf_nodalproj_compute_rhs", "ksgf_nodalproj_solve",
-    "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp",
+    "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp", "ksgf_abeclap_create",
+    "ksgf_abeclap_set_scalars", "ksgf_abeclap_solve", "ksgf_abeclap_apply", "ksgf_abeclap_destroy",
 ]
 
 _lib = None
@@ -152,6 +153,13 @@ def load(path=None):
                                        C.c_void_p]
     L.ksgf_nodalproj_destroy.argtypes = [C.c_void_p]
     L.ksgf_set_bndry.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
+    L.ksgf_abeclap_create.argtypes = [C.c_void_p, _ip, _ip, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.POINTER(KsgfMLMGOptions), C.POINTER(C.c_void_p)]
+    L.ksgf_abeclap_set_scalars.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    L.ksgf_abeclap_solve.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, C.POINTER(KsgfMLMGInfo), C.c_void_p]
+    L.ksgf_abeclap_apply.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_abeclap_destroy.argtypes = [C.c_void_p]
     L.ksgf_incflo_add_gradp.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
                                         C.c_void_p]
     _lib = L
@@ -415,6 +423,50 @@ class MacProjector:
     def close(self):
         if self.h:
             load().ksgf_macproj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ABecLaplacian:
+    """amrex::MLABecLaplacian (constant a, b) + MLMG over a DeviceBoxArray:
+    alpha a phi - beta b lap(phi) = rhs, one component at a time."""
+
+    def __init__(self, dba: DeviceBoxArray, lobc, hibc, alpha, a, beta, b, options=None):
+        import numpy as np
+
+        L = load()
+        self.dba = dba
+        self._lo = np.ascontiguousarray(lobc, dtype=np.int32)
+        self._hi = np.ascontiguousarray(hibc, dtype=np.int32)
+        self.options = options or default_options()
+        h = C.c_void_p()
+        check(L.ksgf_abeclap_create(dba.h, self._lo.ctypes.data_as(_ip), self._hi.ctypes.data_as(_ip), alpha, a, beta,
+                                    b, C.byref(self.options), C.byref(h)), "ksgf_abeclap_create")
+        self.h = h
+        self.info = None
+
+    def set_scalars(self, alpha, beta):
+        check(load().ksgf_abeclap_set_scalars(self.h, alpha, beta), "abeclap_set_scalars")
+
+    def solve(self, sol, ng_sol, rhs, ng_rhs, ncomp, comp, rtol, atol):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_abeclap_solve(self.h, _ptr(sol), ng_sol, _ptr(rhs), ng_rhs, ncomp, comp, rtol, atol,
+                                        C.byref(info), _stream()), "abeclap_solve", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def apply(self, out, ng_out, x, ng_x, ncomp, comp):
+        check(load().ksgf_abeclap_apply(self.h, _ptr(out), ng_out, _ptr(x), ng_x, ncomp, comp, _stream()),
+              "abeclap_apply")
+
+    def close(self):
+        if self.h:
+            load().ksgf_abeclap_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -848,6 +848,37 @@ __global__ void __launch_bounds__(256) k_mac_correct(MFDev u, const double* __re
     fu(i, j, k) = fu(i, j, k) - bd * (pr - pl);
 }
 
+// one component of a cell MultiFab into the level-0 array (times scale), and back
+__global__ void __launch_bounds__(256) k_mf_gather(CMFDev m, int comp, double scale, double* __restrict__ out,
+                                                   LevelGeom L, RegDev r, const int* __restrict__ box, int dlo0,
+                                                   int dlo1, int dlo2)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    CFab f = fab_at(m, box, b);
+    out[LIdx(L, i - dlo0, j - dlo1, k - dlo2)] = scale * f(i, j, k, comp);
+}
+__global__ void __launch_bounds__(256) k_mf_scatter(MFDev m, int comp, const double* __restrict__ a, LevelGeom L,
+                                                    RegDev r, const int* __restrict__ box, int dlo0, int dlo1, int dlo2)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    Fab f = fab_at(m, box, b);
+    f(i, j, k, comp) = a[LIdx(L, i - dlo0, j - dlo1, k - dlo2)];
+}
+// y = alpha x - div(beta grad x) from a level-0 array whose ghosts hold the BCs
+__global__ void __launch_bounds__(256) k_abec_apply(const double* __restrict__ x, MFDev y, int comp, LevelGeom L,
+                                                    RegDev r, const int* __restrict__ box, int dlo0, int dlo1, int dlo2)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    const long long c = LIdx(L, i - dlo0, j - dlo1, k - dlo2);
+    const double off = L.cx * (x[c - 1] + x[c + 1]) + L.cy * (x[c - L.sy] + x[c + L.sy]) +
+                       L.cz * (x[c - L.sz] + x[c + L.sz]);
+    Fab f = fab_at(y, box, b);
+    f(i, j, k, comp) = -(off - L.dg * x[c]);
+}
+
 __global__ void __launch_bounds__(256) k_pack(const double* __restrict__ a, double* __restrict__ out, LevelGeom L,
                                               int unpack)
 {
@@ -872,6 +903,7 @@ struct ksgf_macproj {
     const ksgf_boxarray* ba = nullptr;
     int lobc[3], hibc[3];
     double beta = 1.0;
+    double alpha = 0.0;  // diagonal shift: div(beta grad phi) - alpha phi (ksgf_abeclap_*)
     ksgf_mlmg_options opt;
     int singular = 1;
     int nlev = 0;
@@ -912,12 +944,14 @@ static int mp_alloc(void** p, size_t bytes)
     return KSGF_OK;
 }
 
-static void fill_coefs(LevelGeom& L, const double h[3], double beta)
+// stencil coefficients of div(beta grad) - alpha on a level of cell size h
+// (alpha, a zeroth-order term, is the same on every level)
+static void fill_coefs(LevelGeom& L, const double h[3], double beta, double alpha = 0.0)
 {
     L.cx = beta / (h[0] * h[0]);
     L.cy = beta / (h[1] * h[1]);
     L.cz = beta / (h[2] * h[2]);
-    L.dg = 2.0 * (L.cx + L.cy + L.cz);
+    L.dg = 2.0 * (L.cx + L.cy + L.cz) + alpha;
     L.idg = 1.0 / L.dg;
 }
 
@@ -1714,8 +1748,8 @@ int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta)
     }
     mp->beta = beta;
     auto h = level_h(mp);
-    for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta);
-    fill_coefs(mp->P, h[0].data(), beta);
+    for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta, mp->alpha);
+    fill_coefs(mp->P, h[0].data(), beta, mp->alpha);
     return upload_level_tables(mp);
 }
 
@@ -1786,6 +1820,130 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
     hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, (const double*)mp->phi, phi, L0, 0);
     KSGF_LAUNCH_CHECK();
     return ret;
+}
+
+// ---------------------------------------------------------------------------
+// MLABecLaplacian with constant coefficients over the same multigrid
+// ---------------------------------------------------------------------------
+struct ksgf_abeclap {
+    ksgf_macproj* mp = nullptr;
+    double alpha = 1.0, a = 1.0, beta = 1.0, b = 1.0;
+    int base_singular = 1;  // singular without the alpha term (no Dirichlet side)
+};
+
+static int abec_refill(ksgf_abeclap* ab)
+{
+    ksgf_macproj* mp = ab->mp;
+    mp->alpha = ab->alpha * ab->a;
+    mp->singular = (mp->alpha > 0.0) ? 0 : ab->base_singular;
+    return ksgf_macproj_set_beta(mp, ab->beta * ab->b);
+}
+
+int ksgf_abeclap_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, double alpha, double a,
+                        double beta, double b, const ksgf_mlmg_options* opts, ksgf_abeclap** out)
+{
+    if (!ba || !lobc || !hibc || !out || !(beta * b > 0.0) || !(alpha * a >= 0.0)) {
+        set_error("ksgf_abeclap_create: bad argument (beta b > 0 and alpha a >= 0 required)");
+        return KSGF_ERR_INVALID;
+    }
+    *out = nullptr;
+    ksgf_abeclap* ab = new ksgf_abeclap();
+    ab->alpha = alpha;
+    ab->a = a;
+    ab->beta = beta;
+    ab->b = b;
+    int e = ksgf_macproj_create(ba, lobc, hibc, beta * b, opts, &ab->mp);
+    if (e) {
+        delete ab;
+        return e;
+    }
+    ab->base_singular = ab->mp->singular;
+    if ((e = abec_refill(ab))) {
+        ksgf_macproj_destroy(ab->mp);
+        delete ab;
+        return e;
+    }
+    *out = ab;
+    return KSGF_OK;
+}
+
+int ksgf_abeclap_set_scalars(ksgf_abeclap* ab, double alpha, double beta)
+{
+    if (!ab || !(beta * ab->b > 0.0) || !(alpha * ab->a >= 0.0)) {
+        set_error("ksgf_abeclap_set_scalars: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    ab->alpha = alpha;
+    ab->beta = beta;
+    return abec_refill(ab);
+}
+
+int ksgf_abeclap_solve(ksgf_abeclap* ab, double* sol, int ng_sol, const double* rhs, int ng_rhs, int ncomp,
+                       int comp, double rtol, double atol, ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!ab || !sol || !rhs || ncomp <= 0 || comp < 0 || comp >= ncomp || ng_sol < 0 || ng_rhs < 0) {
+        set_error("ksgf_abeclap_solve: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    ksgf_macproj* mp = ab->mp;
+    const ksgf_boxarray* ba = mp->ba;
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev cr;
+    MFDev ms;
+    int e;
+    if ((e = cmfdev(ba, rhs, KSGF_CELL, ncomp, ng_rhs, &cr)) || (e = mfdev(ba, sol, KSGF_CELL, ncomp, ng_sol, &ms)))
+        return e;
+    int err = KSGF_OK;
+    const int z[3] = {0, 0, 0};
+    const Region* rv = get_region(ba, z, z, &err);
+    if (!rv) return err;
+    // the multigrid solves div(beta b grad phi) - alpha a phi = -rhs
+    hipLaunchKernelGGL(k_mf_gather, dim3(rv->nchunks), dim3(256), 0, st, cr, comp, -1.0, mp->rhs, mp->P,
+                       regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2]);
+    KSGF_LAUNCH_CHECK();
+    int ret = mg_solve(mp, rtol, atol, info, st);
+    if (ret < 0) return ret;
+    hipLaunchKernelGGL(k_mf_scatter, dim3(rv->nchunks), dim3(256), 0, st, ms, comp, (const double*)mp->phi, mp->P,
+                       regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2]);
+    KSGF_LAUNCH_CHECK();
+    return ret;
+}
+
+int ksgf_abeclap_apply(ksgf_abeclap* ab, double* out, int ng_out, const double* in, int ng_in, int ncomp, int comp,
+                       ksgf_stream_t stream)
+{
+    if (!ab || !out || !in || ncomp <= 0 || comp < 0 || comp >= ncomp || ng_out < 0 || ng_in < 0) {
+        set_error("ksgf_abeclap_apply: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    ksgf_macproj* mp = ab->mp;
+    const ksgf_boxarray* ba = mp->ba;
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev ci;
+    MFDev mo;
+    int e;
+    if ((e = cmfdev(ba, in, KSGF_CELL, ncomp, ng_in, &ci)) || (e = mfdev(ba, out, KSGF_CELL, ncomp, ng_out, &mo)))
+        return e;
+    int err = KSGF_OK;
+    const int z[3] = {0, 0, 0};
+    const Region* rv = get_region(ba, z, z, &err);
+    if (!rv) return err;
+    // x into the level-0 array (phi is scratch between solves), BC ghosts, stencil
+    hipLaunchKernelGGL(k_mf_gather, dim3(rv->nchunks), dim3(256), 0, st, ci, comp, 1.0, mp->phi, mp->P,
+                       regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2]);
+    launch_ghosts<double>(mp->phi, mp->P, st);
+    hipLaunchKernelGGL(k_abec_apply, dim3(rv->nchunks), dim3(256), 0, st, (const double*)mp->phi, mo, comp, mp->P,
+                       regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2]);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+int ksgf_abeclap_destroy(ksgf_abeclap* ab)
+{
+    if (!ab) return KSGF_OK;
+    ksgf_macproj_destroy(ab->mp);
+    delete ab;
+    return KSGF_OK;
 }
 
 int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream)

@@ -1289,7 +1289,7 @@ static void vcycle(mg_t* M, int l)
     smooth(L, L->cor, L->res, M->o->nu2, M->lobc, M->hibc);
 }
 
-static void build_levels(mg_t* M, const int* n, const double* dx, double beta)
+static void build_levels(mg_t* M, const int* n, const double* dx, double beta, double alpha)
 {
     const ko_mlmg_opts* o = M->o;
     int nl = 0;
@@ -1304,7 +1304,7 @@ static void build_levels(mg_t* M, const int* n, const double* dx, double beta)
         L->cx = beta / (ch[0] * ch[0]);
         L->cy = beta / (ch[1] * ch[1]);
         L->cz = beta / (ch[2] * ch[2]);
-        L->dg = 2.0 * (L->cx + L->cy + L->cz);
+        L->dg = 2.0 * (L->cx + L->cy + L->cz) + alpha;
         L->idg = 1.0 / L->dg;
         ++nl;
         if (nl >= MAXLEV || nl > o->max_coarsening_level) break;
@@ -1393,7 +1393,7 @@ int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int*
     M->singular = 1;
     for (int d = 0; d < 3; ++d)
         if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) M->singular = 0;
-    build_levels(M, n, dx, beta);
+    build_levels(M, n, dx, beta, 0.0);
     lev_t* L0 = &M->L[0];
     double* rhs = (double*)calloc((size_t)L0->tot, sizeof(double));
     double* phi = (double*)calloc((size_t)L0->tot, sizeof(double));
@@ -1414,6 +1414,76 @@ int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int*
 }
 
 /* ------------------------------------------------------------------ */
+/* MLABecLaplacian with constant coefficients (the implicit diffusion  */
+/* solve, DiffusionOps.cpp:154-160: setScalars(1, dt), a = rho,        */
+/* b = mu): alpha a phi - beta b lap(phi) = rhs.  The multigrid above  */
+/* solves div(beta b grad phi) - alpha a phi = -rhs: the same V-cycle  */
+/* with the diagonal grown by alpha a on every level (a zeroth-order   */
+/* term is not rescaled when coarsening).  Dirichlet sides are         */
+/* homogeneous with the boundary value on the face (the ghost is minus */
+/* the first cell, AMReX's max_order 2 treatment of a zero value).     */
+/* ------------------------------------------------------------------ */
+int ko_abeclap_solve(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha_a,
+                     double beta_b, const double* rhs_in, double* phi_out, double rtol, double atol,
+                     const ko_mlmg_opts* opts, ko_mlmg_info* info)
+{
+    mg_t* M = (mg_t*)calloc(1, sizeof(mg_t));
+    M->o = opts;
+    M->lobc = lobc;
+    M->hibc = hibc;
+    M->singular = !(alpha_a > 0.0);
+    for (int d = 0; d < 3; ++d)
+        if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) M->singular = 0;
+    build_levels(M, n, dx, beta_b, alpha_a);
+    lev_t* L0 = &M->L[0];
+    double* rhs = (double*)calloc((size_t)L0->tot, sizeof(double));
+    double* phi = (double*)calloc((size_t)L0->tot, sizeof(double));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                rhs[LI(L0, i, j, k)] = -rhs_in[i + (long)n[0] * (j + (long)n[1] * k)];
+    int ret = mg_solve(M, phi, rhs, rtol, atol, info);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                phi_out[i + (long)n[0] * (j + (long)n[1] * k)] = phi[LI(L0, i, j, k)];
+    free(rhs);
+    free(phi);
+    free_levels(M);
+    free(M);
+    return ret;
+}
+
+/* MLMG::apply of the same operator: y = alpha a x - beta b lap(x), the
+ * boundary as in the solve (compute_diff_term uses (0, -1): y = b lap x) */
+void ko_abeclap_apply(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha_a,
+                      double beta_b, const double* x_in, double* y_out)
+{
+    lev_t L;
+    memset(&L, 0, sizeof(L));
+    for (int d = 0; d < 3; ++d) L.n[d] = n[d];
+    L.sy = n[0] + 2;
+    L.sz = (long)(n[0] + 2) * (n[1] + 2);
+    L.tot = L.sz * (n[2] + 2);
+    L.cx = beta_b / (dx[0] * dx[0]);
+    L.cy = beta_b / (dx[1] * dx[1]);
+    L.cz = beta_b / (dx[2] * dx[2]);
+    L.dg = 2.0 * (L.cx + L.cy + L.cz) + alpha_a;
+    double* x = (double*)calloc((size_t)L.tot, sizeof(double));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) x[LI(&L, i, j, k)] = x_in[i + (long)n[0] * (j + (long)n[1] * k)];
+    lev_ghosts(&L, x, lobc, hibc);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                long c = LI(&L, i, j, k);
+                y_out[i + (long)n[0] * (j + (long)n[1] * k)] = -(lap_off(&L, x, c) - L.dg * x[c]);
+            }
+    free(x);
+}
+
+/* ------------------------------------------------------------------ */
 /* MacProjector::project (a8/a9; icns_advection.cpp:206-375)           */
 /* ------------------------------------------------------------------ */
 int ko_mac_project(const ko_geom* g, double* umac, double* vmac, double* wmac,
@@ -1430,7 +1500,7 @@ int ko_mac_project(const ko_geom* g, double* umac, double* vmac, double* wmac,
     M->singular = 1;
     for (int d = 0; d < 3; ++d)
         if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) M->singular = 0;
-    build_levels(M, n, g->dx, beta);
+    build_levels(M, n, g->dx, beta, 0.0);
     lev_t* L0 = &M->L[0];
     double* rhs = (double*)calloc((size_t)L0->tot, sizeof(double));
     double* phi = (double*)calloc((size_t)L0->tot, sizeof(double));

@@ -133,6 +133,15 @@ int ko_nodal_project(const ko_geom* g, double* vel, int ng, const int* lobc, con
                      double sigma, double rtol, double atol, const ko_mlmg_opts* opts,
                      ko_mlmg_info* info, double* phi_out, double* gp_out);
 
+/* MLABecLaplacian, constant coefficients (implicit diffusion,
+ * DiffusionOps.cpp:154-160): alpha_a phi - beta_b lap(phi) = rhs, and its
+ * apply; domain-wide cell arrays, Fortran order */
+int ko_abeclap_solve(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha_a,
+                     double beta_b, const double* rhs, double* phi, double rtol, double atol,
+                     const ko_mlmg_opts* opts, ko_mlmg_info* info);
+void ko_abeclap_apply(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha_a,
+                      double beta_b, const double* x, double* y);
+
 double ko_cfl_max(const ko_geom* g, const double* vel, int ng);
 
 void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,

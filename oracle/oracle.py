@@ -92,6 +92,9 @@ def lib():
         L.ko_nodal_correct.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, _dp, _ip, C.c_double]
         L.ko_nodal_project.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _ip, _ip, C.c_double, C.c_double,
                                        C.c_double, C.POINTER(KoOpts), C.POINTER(KoInfo), _dp, _dp]
+        L.ko_abeclap_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_double,
+                                       C.POINTER(KoOpts), C.POINTER(KoInfo)]
+        L.ko_abeclap_apply.argtypes = [_ip, _dp, _ip, _ip, C.c_double, C.c_double, _dp, _dp]
         L.ko_cfl_max.restype = C.c_double
         L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
         L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
@@ -276,6 +279,32 @@ def nodal_project(G, vel, ng, lobc, hibc, sigma, rtol, atol, opts=None):
     st = lib().ko_nodal_project(G.ref, _p(vel), ng, plo, phi_, sigma, rtol, atol, C.byref(opts), C.byref(info),
                                 _p(phi), _p(gp))
     return dict(info.as_dict(), status=st), phi, gp
+
+
+def abeclap_solve(n, dx, lobc, hibc, alpha_a, beta_b, rhs, rtol, atol, opts=None):
+    """alpha_a phi - beta_b lap(phi) = rhs (domain-wide, Fortran order)"""
+    opts = opts or default_opts()
+    info = KoInfo()
+    nn, pn = _iarr(n)
+    dd = np.ascontiguousarray(dx, dtype=np.float64)
+    lo, plo = _iarr(lobc)
+    hi, phi_ = _iarr(hibc)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64).ravel()
+    phi = np.zeros_like(rhs)
+    st = lib().ko_abeclap_solve(pn, _p(dd), plo, phi_, alpha_a, beta_b, _p(rhs), _p(phi), rtol, atol,
+                                C.byref(opts), C.byref(info))
+    return phi, dict(info.as_dict(), status=st)
+
+
+def abeclap_apply(n, dx, lobc, hibc, alpha_a, beta_b, x):
+    nn, pn = _iarr(n)
+    dd = np.ascontiguousarray(dx, dtype=np.float64)
+    lo, plo = _iarr(lobc)
+    hi, phi_ = _iarr(hibc)
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    y = np.zeros_like(x)
+    lib().ko_abeclap_apply(pn, _p(dd), plo, phi_, alpha_a, beta_b, _p(x), _p(y))
+    return y
 
 
 def cfl_max(G, vel, ng):
