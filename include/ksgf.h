@@ -263,7 +263,8 @@ int ksgf_poisson_batch_destroy(ksgf_poisson_batch* pb);
  * zero.  solve: sol and rhs are cell MultiFabs of ncomp components (valid
  * cells of component comp are read / written).  apply = MLMG::apply
  * (compute_diff_term uses alpha = 0, beta = -1: out = b lap(in)); apply
- * reuses the solve's level-0 scratch. */
+ * reuses the solve's level-0 scratch.  set_scalars takes any finite values;
+ * a solve needs beta b > 0 and alpha a >= 0 (else KSGF_ERR_INVALID). */
 typedef struct ksgf_abeclap ksgf_abeclap;
 int ksgf_abeclap_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, double alpha,
                         double a, double beta, double b, const ksgf_mlmg_options* opts,
@@ -342,6 +343,14 @@ int ksgf_predictor_update(const ksgf_boxarray* ba, double* vel_new, int ng_new,
  * then (scale_by_rho) src *= rho (PDEOps multiply_rho).  gp: ngrow 0. */
 int ksgf_icns_src_term(const ksgf_boxarray* ba, double* src, int ng_src, const double* gp,
                        const double* rho, int ng_rho, int scale_by_rho, ksgf_stream_t stream);
+
+/* The same with the explicit viscous term divtau (cell, 3 comp, ngrow 0)
+ * added to the momentum-form forcing before the division by density, as
+ * ApplyPredictor adds it for Godunov (incflo_advance.cpp:246-291):
+ * src = (-gp + divtau) / rho. */
+int ksgf_icns_src_term_divtau(const ksgf_boxarray* ba, double* src, int ng_src, const double* gp,
+                              const double* divtau, const double* rho, int ng_rho,
+                              ksgf_stream_t stream);
 
 /* out(comp n) = a(comp n) * rho over valid + ngrow cells (MultiFab::Copy +
  * MultiFab::Multiply, icns_advection.H:281-314).  a/out: ncomp, ng_a/ng_out

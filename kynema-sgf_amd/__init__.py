@@ -64,6 +64,7 @@ EXPORTS = [
     "ksgf_nodalproj_project", "ksgf_nodalproj_get_phi", "ksgf_nodalproj_compute_rhs", "ksgf_nodalproj_solve",
     "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp", "ksgf_abeclap_create",
     "ksgf_abeclap_set_scalars", "ksgf_abeclap_solve", "ksgf_abeclap_apply", "ksgf_abeclap_destroy",
+    "ksgf_icns_src_term_divtau",
 ]
 
 _lib = None
@@ -160,6 +161,7 @@ def load(path=None):
                                      C.c_double, C.POINTER(KsgfMLMGInfo), C.c_void_p]
     L.ksgf_abeclap_apply.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_abeclap_destroy.argtypes = [C.c_void_p]
+    L.ksgf_icns_src_term_divtau.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_incflo_add_gradp.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
                                         C.c_void_p]
     _lib = L

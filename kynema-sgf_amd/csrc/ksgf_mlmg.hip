@@ -1829,21 +1829,29 @@ struct ksgf_abeclap {
     ksgf_macproj* mp = nullptr;
     double alpha = 1.0, a = 1.0, beta = 1.0, b = 1.0;
     int base_singular = 1;  // singular without the alpha term (no Dirichlet side)
+    bool dirty = false;     // scalars changed since the hierarchy's coefficients were set
 };
 
+// the multigrid's coefficients from the current scalars (a solve needs
+// beta b > 0 and alpha a >= 0; an apply takes any scalars)
 static int abec_refill(ksgf_abeclap* ab)
 {
     ksgf_macproj* mp = ab->mp;
+    if (!(ab->beta * ab->b > 0.0) || !(ab->alpha * ab->a >= 0.0)) {
+        set_error("ksgf_abeclap_solve: the solve needs beta b > 0 and alpha a >= 0");
+        return KSGF_ERR_INVALID;
+    }
     mp->alpha = ab->alpha * ab->a;
     mp->singular = (mp->alpha > 0.0) ? 0 : ab->base_singular;
+    ab->dirty = false;
     return ksgf_macproj_set_beta(mp, ab->beta * ab->b);
 }
 
 int ksgf_abeclap_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, double alpha, double a,
                         double beta, double b, const ksgf_mlmg_options* opts, ksgf_abeclap** out)
 {
-    if (!ba || !lobc || !hibc || !out || !(beta * b > 0.0) || !(alpha * a >= 0.0)) {
-        set_error("ksgf_abeclap_create: bad argument (beta b > 0 and alpha a >= 0 required)");
+    if (!ba || !lobc || !hibc || !out || !(b > 0.0) || !(a >= 0.0) || !std::isfinite(alpha) || !std::isfinite(beta)) {
+        set_error("ksgf_abeclap_create: bad argument (b > 0 and a >= 0 required)");
         return KSGF_ERR_INVALID;
     }
     *out = nullptr;
@@ -1852,30 +1860,29 @@ int ksgf_abeclap_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     ab->a = a;
     ab->beta = beta;
     ab->b = b;
-    int e = ksgf_macproj_create(ba, lobc, hibc, beta * b, opts, &ab->mp);
+    // the hierarchy is built with a positive beta; the scalars are applied
+    // before the first solve
+    int e = ksgf_macproj_create(ba, lobc, hibc, (beta * b > 0.0) ? beta * b : b, opts, &ab->mp);
     if (e) {
         delete ab;
         return e;
     }
     ab->base_singular = ab->mp->singular;
-    if ((e = abec_refill(ab))) {
-        ksgf_macproj_destroy(ab->mp);
-        delete ab;
-        return e;
-    }
+    ab->dirty = true;
     *out = ab;
     return KSGF_OK;
 }
 
 int ksgf_abeclap_set_scalars(ksgf_abeclap* ab, double alpha, double beta)
 {
-    if (!ab || !(beta * ab->b > 0.0) || !(alpha * ab->a >= 0.0)) {
+    if (!ab || !std::isfinite(alpha) || !std::isfinite(beta)) {
         set_error("ksgf_abeclap_set_scalars: bad argument");
         return KSGF_ERR_INVALID;
     }
     ab->alpha = alpha;
     ab->beta = beta;
-    return abec_refill(ab);
+    ab->dirty = true;
+    return KSGF_OK;
 }
 
 int ksgf_abeclap_solve(ksgf_abeclap* ab, double* sol, int ng_sol, const double* rhs, int ng_rhs, int ncomp,
@@ -1891,6 +1898,7 @@ int ksgf_abeclap_solve(ksgf_abeclap* ab, double* sol, int ng_sol, const double* 
     CMFDev cr;
     MFDev ms;
     int e;
+    if (ab->dirty && (e = abec_refill(ab))) return e;
     if ((e = cmfdev(ba, rhs, KSGF_CELL, ncomp, ng_rhs, &cr)) || (e = mfdev(ba, sol, KSGF_CELL, ncomp, ng_sol, &ms)))
         return e;
     int err = KSGF_OK;
@@ -1928,11 +1936,15 @@ int ksgf_abeclap_apply(ksgf_abeclap* ab, double* out, int ng_out, const double* 
     const int z[3] = {0, 0, 0};
     const Region* rv = get_region(ba, z, z, &err);
     if (!rv) return err;
-    // x into the level-0 array (phi is scratch between solves), BC ghosts, stencil
+    // x into the level-0 array (phi is scratch between solves), BC ghosts, and
+    // the stencil with the current scalars (of either sign)
+    LevelGeom A = mp->P;
+    const double h0[3] = {ba->dx[0], ba->dx[1], ba->dx[2]};
+    fill_coefs(A, h0, ab->beta * ab->b, ab->alpha * ab->a);
     hipLaunchKernelGGL(k_mf_gather, dim3(rv->nchunks), dim3(256), 0, st, ci, comp, 1.0, mp->phi, mp->P,
                        regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2]);
     launch_ghosts<double>(mp->phi, mp->P, st);
-    hipLaunchKernelGGL(k_abec_apply, dim3(rv->nchunks), dim3(256), 0, st, (const double*)mp->phi, mo, comp, mp->P,
+    hipLaunchKernelGGL(k_abec_apply, dim3(rv->nchunks), dim3(256), 0, st, (const double*)mp->phi, mo, comp, A,
                        regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2]);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;

@@ -70,6 +70,17 @@ __global__ void __launch_bounds__(256) k_src_term(MFDev src, CMFDev gp, CMFDev r
     }
 }
 
+__global__ void __launch_bounds__(256) k_src_term_divtau(MFDev src, CMFDev gp, CMFDev tau, CMFDev rho, RegDev r,
+                                                         const int* __restrict__ box)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    Fab fs = fab_at(src, box, b);
+    CFab fg = fab_at(gp, box, b), ft = fab_at(tau, box, b), fr = fab_at(rho, box, b);
+    double rhoinv = 1.0 / fr(i, j, k);
+    for (int n = 0; n < 3; ++n) fs(i, j, k, n) = (-(1.0 * fg(i, j, k, n)) + ft(i, j, k, n)) * rhoinv;
+}
+
 __global__ void __launch_bounds__(256) k_scale_rho(MFDev out, CMFDev a, CMFDev rho, int ncomp, int divide, RegDev r,
                                                    const int* __restrict__ box)
 {
@@ -252,6 +263,29 @@ int ksgf_scale_by_rho(const ksgf_boxarray* ba, double* out, int ng_out, const do
     if (!r) return err;
     hipLaunchKernelGGL(k_scale_rho, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, o, av, rv, ncomp,
                        divide, regdev(r, ba->nbox), ba->dbox);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+int ksgf_icns_src_term_divtau(const ksgf_boxarray* ba, double* src, int ng_src, const double* gp, const double* divtau,
+                              const double* rho, int ng_rho, ksgf_stream_t stream)
+{
+    if (!ba || !src || !gp || !divtau || !rho) {
+        set_error("ksgf_icns_src_term_divtau: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    MFDev s;
+    CMFDev g, t, rr;
+    int e;
+    if ((e = mfdev(ba, src, KSGF_CELL, 3, ng_src, &s)) || (e = cmfdev(ba, gp, KSGF_CELL, 3, 0, &g)) ||
+        (e = cmfdev(ba, divtau, KSGF_CELL, 3, 0, &t)) || (e = cmfdev(ba, rho, KSGF_CELL, 1, ng_rho, &rr)))
+        return e;
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_src_term_divtau, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, s, g, t, rr,
+                       regdev(r, ba->nbox), ba->dbox);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
 }

@@ -16,6 +16,16 @@
 //       ComputeDivergence (mult -1)    icns_advection.H:418-422
 //   compute_source_term(NPH) + ComputeRHSOp::predictor_rhs
 //                                      icns_ops.H:84-121, CompRHSOps.H:93-117
+// With enableDiffusion (constant viscosity mu and density rho_0) the step
+// carries the viscous term as ApplyPredictor does for Godunov
+// (incflo_advance.cpp:246-291, 380-389; DiffusionOps.cpp:154-160):
+//   Crank-Nicolson: divtau = mu lap(u^n) (MLMG::apply with scalars (0, -1)),
+//     added to the forcing before the division by density, kappa = 1/2 in
+//     the predictor, then (rho - dt/2 mu lap) u = rho u*;
+//   Implicit: kappa = 0, then (rho - dt mu lap) u = rho u*;
+// one MLABecLaplacian solve per velocity component with that component's
+// BCs (get_diffuse_tensor_bc, incflo_diffusion.cpp:11-69: periodic; ext_dir
+// -> Dirichlet; foextrap / hoextrap -> Neumann).
 // With enableNodalProjection the step ends with incflo::ApplyProjection
 // (incflo_advance.cpp:412-414; incflo_apply_nodal_projection.cpp:149-465,
 // constant density, not incremental, no small-dt variant):
@@ -43,6 +53,7 @@ struct StepStats {
     double dt = 0.0;
     ksgf_mlmg_info mlmg{};
     ksgf_mlmg_info nodal{};  // the nodal projection's solve (when enabled)
+    ksgf_mlmg_info diffusion[3]{};  // the velocity components' diffusion solves (when enabled)
     double divmax = -1.0;  // max |div u_mac| after the projection (when asked for)
 };
 
@@ -149,6 +160,42 @@ public:
 
     void setCheckDivergence(bool on) { check_div_ = on; }
 
+    // diffusion: type 1 Crank-Nicolson, 2 Implicit (incflo.diffusion_type);
+    // Dirichlet velocity BCs must hold zero (walls at rest)
+    void enableDiffusion(double mu, int type, double rtol, double atol, const ksgf_mlmg_options& opts)
+    {
+        if (type != 1 && type != 2) throw std::runtime_error("enableDiffusion: type 1 (Crank-Nicolson) or 2 (Implicit)");
+        if (!(mu > 0.0)) throw std::runtime_error("enableDiffusion: mu must be positive");
+        Hydro::LPInfo info;
+        info.do_semicoarsening = opts.do_semicoarsening != 0;
+        info.max_coarsening_level = opts.max_coarsening_level;
+        for (int c = 0; c < 3; ++c) {
+            std::array<Hydro::LinOpBCType, 3> lo{}, hi{};
+            for (int d = 0; d < 3; ++d) {
+                for (int side = 0; side < 2; ++side) {
+                    const int t = side ? vel_bc_[c].hi[d] : vel_bc_[c].lo[d];
+                    Hydro::LinOpBCType lt = Hydro::LinOpBCType::Neumann;
+                    if (t == KSGF_BC_INT_DIR) lt = Hydro::LinOpBCType::Periodic;
+                    if (t == KSGF_BC_EXT_DIR) {
+                        lt = Hydro::LinOpBCType::Dirichlet;
+                        if (vel_bcval_[(side * 3 + d) * 3 + c] != 0.0)
+                            throw std::runtime_error("enableDiffusion: inhomogeneous Dirichlet velocity is not provided");
+                    }
+                    (side ? hi : lo)[d] = lt;
+                }
+            }
+            lap_[c] = std::make_unique<Hydro::MLABecLaplacian>(ba_, info, opts);
+            lap_[c]->setDomainBC(lo, hi);
+            lap_[c]->setACoeffs(rho0_);
+            lap_[c]->setBCoeffs(mu);
+        }
+        divtau_ = std::make_unique<MultiFab>(ba_, 3, 0);
+        drhs_ = std::make_unique<MultiFab>(ba_, 3, 0);
+        diff_type_ = type;
+        diff_rtol_ = rtol;
+        diff_atol_ = atol;
+    }
+
     // end every step with the nodal projection (nodal_proj.mg_rtol / mg_atol);
     // its domain BCs are the MAC projection's (get_projection_bc maps the
     // pressure BCs the same way, incflo_apply_nodal_projection.cpp:28-58)
@@ -179,6 +226,7 @@ public:
         if (e != hipSuccess) throw std::runtime_error("getNodalPhi: copy");
     }
     const ksgf_mlmg_info& lastNodalInfo() const { return last_nodal_; }
+    const StepStats& lastStats() const { return last_stats_; }
     void getGradP(double* gp) const
     {
         (void)hipStreamSynchronize(st_);
@@ -245,8 +293,19 @@ public:
             dt = 2.0 * cfl_ / (cd + std::sqrt(cd * cd));
         }
         s.dt = dt;
+        // explicit viscous term of the old state (Crank-Nicolson)
+        const bool cn = (diff_type_ == 1);
+        if (cn)
+            for (int c = 0; c < 3; ++c) {
+                lap_[c]->setScalars(0.0, -1.0);
+                lap_[c]->apply(*divtau_, *vel_, c, st_);
+            }
         // forcing in velocity form with one ghost
-        check(ksgf_icns_src_term(h, src_.data(), 1, gp_.data(), rho_.data(), 3, 0, st), "compute_source_term");
+        if (cn)
+            check(ksgf_icns_src_term_divtau(h, src_.data(), 1, gp_.data(), divtau_->data(), rho_.data(), 3, st),
+                  "compute_source_term + divtau");
+        else
+            check(ksgf_icns_src_term(h, src_.data(), 1, gp_.data(), rho_.data(), 3, 0, st), "compute_source_term");
         src_.FillBoundary(st_);
         src_.FillPhysBC(src_bc_, std::vector<double>(18, 0.0), st_);
         // preadvect
@@ -281,12 +340,24 @@ public:
         // source at n+1/2 (momentum form) and the predictor update
         check(ksgf_icns_src_term(h, src_nph_.data(), 0, gp_.data(), rho_.data(), 3, 1, st), "src nph");
         check(ksgf_predictor_update(h, vel_new_->data(), 3, vel_->data(), 3, rho_.data(), 3, rho_.data(), 3,
-                                    conv_.data(), src_nph_.data(), nullptr, 0.0, dt, 3, st),
+                                    conv_.data(), src_nph_.data(), cn ? divtau_->data() : nullptr, cn ? 0.5 : 0.0, dt,
+                                    3, st),
               "predictor_rhs");
+        if (diff_type_ > 0) {
+            // (rho - dt_diff mu lap) u = rho u*, component by component
+            const double dt_diff = cn ? 0.5 * dt : dt;
+            check(ksgf_scale_by_rho(h, drhs_->data(), 0, vel_new_->data(), 3, rho_.data(), 3, 3, 0, 0, st),
+                  "diffusion rhs = rho u*");
+            for (int c = 0; c < 3; ++c) {
+                lap_[c]->setScalars(1.0, dt_diff);
+                s.diffusion[c] = lap_[c]->solve(*vel_new_, *drhs_, c, diff_rtol_, diff_atol_, st_);
+            }
+        }
         vel_new_->FillBoundary(st_);
         vel_new_->FillPhysBC(vel_bc_, vel_bcval_, st_);
         std::swap(vel_, vel_new_);
         if (nodal_) applyProjection(dt, s);
+        last_stats_ = s;
         return s;
     }
 
@@ -351,8 +422,13 @@ private:
 
     std::unique_ptr<Scalar> sc_;
     std::unique_ptr<Hydro::NodalProjector> nodal_;
+    std::unique_ptr<Hydro::MLABecLaplacian> lap_[3];
+    std::unique_ptr<MultiFab> divtau_, drhs_;
+    int diff_type_ = 0;
+    double diff_rtol_ = 0.0, diff_atol_ = 0.0;
     double nodal_rtol_ = 0.0, nodal_atol_ = 0.0;
     ksgf_mlmg_info last_nodal_{};
+    StepStats last_stats_{};
     std::array<Hydro::LinOpBCType, 3> lobc_{}, hibc_{};
     BoxArray ba_;
     std::vector<BCRec> vel_bc_, src_bc_, rho_bc_;

@@ -222,6 +222,35 @@ int ksgf_host_stepper_get_nodal_phi(ksgf_host_stepper* s, double* phi)
     }
 }
 
+int ksgf_host_stepper_enable_diffusion(ksgf_host_stepper* s, double mu, int type, double rtol, double atol,
+                                       const ksgf_mlmg_options* opts)
+{
+    try {
+        if (!s) throw std::runtime_error("null stepper");
+        ksgf_mlmg_options o;
+        if (opts)
+            o = *opts;
+        else
+            ksgf_mlmg_default_options(&o);
+        reinterpret_cast<GodunovPredictor*>(s)->enableDiffusion(mu, type, rtol, atol, o);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_enable_diffusion: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_stepper_diffusion_info(ksgf_host_stepper* s, ksgf_mlmg_info* info3)
+{
+    if (!s || !info3) {
+        g_err = "ksgf_host_stepper_diffusion_info: null argument";
+        return KSGF_ERR_INVALID;
+    }
+    const auto& st = reinterpret_cast<GodunovPredictor*>(s)->lastStats();
+    for (int c = 0; c < 3; ++c) info3[c] = st.diffusion[c];
+    return KSGF_OK;
+}
+
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
 {
     delete reinterpret_cast<GodunovPredictor*>(s);

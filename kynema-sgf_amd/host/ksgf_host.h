@@ -45,6 +45,12 @@ int ksgf_host_stepper_nodal_info(ksgf_host_stepper* s, ksgf_mlmg_info* info);
 /* gp (3*N, domain-wide host) and the nodal phi (domain-wide nodes, host) */
 int ksgf_host_stepper_get_gradp(ksgf_host_stepper* s, double* gp);
 int ksgf_host_stepper_get_nodal_phi(ksgf_host_stepper* s, double* phi);
+/* viscous term with constant viscosity mu: type 1 Crank-Nicolson, 2 Implicit
+ * (incflo.diffusion_type); diffusion_info: the last step's three
+ * component solves */
+int ksgf_host_stepper_enable_diffusion(ksgf_host_stepper* s, double mu, int type, double rtol, double atol,
+                                       const ksgf_mlmg_options* opts);
+int ksgf_host_stepper_diffusion_info(ksgf_host_stepper* s, ksgf_mlmg_info* info3);
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
 
 #ifdef __cplusplus

@@ -389,6 +389,80 @@ private:
     ksgf_macproj* mp_ = nullptr;
 };
 
+// amrex::MLABecLaplacian with constant coefficients and its MLMG, one
+// component at a time (the implicit diffusion solve of
+// DiffSolverIface::linsys_solve, DiffusionOps.cpp:154-160, and the apply of
+// compute_diff_term): setScalars(alpha, beta), setACoeffs(a),
+// setBCoeffs(b), setDomainBC, solve(sol, rhs, comp), apply(out, in, comp).
+class MLABecLaplacian {
+public:
+    MLABecLaplacian(const ksgf::amr::BoxArray& ba, const LPInfo& info, const ksgf_mlmg_options& opts)
+        : ba_(&ba), opts_(opts)
+    {
+        opts_.do_semicoarsening = info.do_semicoarsening ? 1 : 0;
+        opts_.max_coarsening_level = info.max_coarsening_level;
+    }
+    ~MLABecLaplacian()
+    {
+        if (ab_) ksgf_abeclap_destroy(ab_);
+    }
+    MLABecLaplacian(const MLABecLaplacian&) = delete;
+    MLABecLaplacian& operator=(const MLABecLaplacian&) = delete;
+
+    void setScalars(double alpha, double beta)
+    {
+        alpha_ = alpha;
+        beta_ = beta;
+        if (ab_) ksgf::amr::check(ksgf_abeclap_set_scalars(ab_, alpha, beta), "MLABecLaplacian::setScalars");
+    }
+    void setACoeffs(double a) { a_ = a, rebuild_ = true; }
+    void setBCoeffs(double b) { b_ = b, rebuild_ = true; }
+    void setDomainBC(const std::array<LinOpBCType, 3>& lo, const std::array<LinOpBCType, 3>& hi)
+    {
+        for (int d = 0; d < 3; ++d) {
+            lo_[d] = (int)lo[d];
+            hi_[d] = (int)hi[d];
+        }
+        rebuild_ = true;
+    }
+    // MLMG::solve for component comp of sol (rhs: same component of rhs)
+    ksgf_mlmg_info solve(ksgf::amr::MultiFab& sol, const ksgf::amr::MultiFab& rhs, int comp, double rtol,
+                         double atol, hipStream_t st = nullptr)
+    {
+        build();
+        ksgf_mlmg_info info{};
+        ksgf::amr::check(ksgf_abeclap_solve(ab_, sol.data(), sol.nGrow(), rhs.data(), rhs.nGrow(), sol.nComp(), comp,
+                                            rtol, atol, &info, (ksgf_stream_t)st),
+                         "MLABecLaplacian::solve");
+        return info;
+    }
+    // MLMG::apply for component comp
+    void apply(ksgf::amr::MultiFab& out, const ksgf::amr::MultiFab& in, int comp, hipStream_t st = nullptr)
+    {
+        build();
+        ksgf::amr::check(ksgf_abeclap_apply(ab_, out.data(), out.nGrow(), in.data(), in.nGrow(), in.nComp(), comp,
+                                            (ksgf_stream_t)st),
+                         "MLABecLaplacian::apply");
+    }
+
+private:
+    void build()
+    {
+        if (ab_ && !rebuild_) return;
+        if (ab_) ksgf_abeclap_destroy(ab_);
+        ab_ = nullptr;
+        ksgf::amr::check(ksgf_abeclap_create(ba_->handle(), lo_, hi_, alpha_, a_, beta_, b_, &opts_, &ab_),
+                         "MLABecLaplacian");
+        rebuild_ = false;
+    }
+    const ksgf::amr::BoxArray* ba_;
+    ksgf_mlmg_options opts_;
+    double alpha_ = 1.0, beta_ = 1.0, a_ = 1.0, b_ = 1.0;
+    int lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
+    bool rebuild_ = true;
+    ksgf_abeclap* ab_ = nullptr;
+};
+
 // Hydro::NodalProjector with a constant sigma, as incflo::ApplyProjection
 // builds it for constant density (incflo_apply_nodal_projection.cpp:340-353):
 // NodalProjector(vel, scaling_factor / rho_0, geom, lpinfo), setDomainBC,

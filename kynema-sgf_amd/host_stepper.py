@@ -43,6 +43,9 @@ def host_lib():
         L.ksgf_host_stepper_nodal_info.argtypes = [C.c_void_p, C.POINTER(KsgfMLMGInfo)]
         L.ksgf_host_stepper_get_gradp.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_nodal_phi.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_enable_diffusion.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double,
+                                                         C.POINTER(KsgfMLMGOptions)]
+        L.ksgf_host_stepper_diffusion_info.argtypes = [C.c_void_p, C.POINTER(KsgfMLMGInfo)]
         L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
         _h = L
     return _h
@@ -138,6 +141,17 @@ class HostStepper:
         _chk(host_lib().ksgf_host_stepper_enable_nodal_projection(self.h, rtol, atol, C.byref(self._nopts)),
              "enable_nodal_projection")
         self.nodal = True
+
+    def enable_diffusion(self, mu, diff_type, rtol, atol, options=None):
+        """viscous term, constant mu: diff_type 1 Crank-Nicolson, 2 Implicit"""
+        self._dopts = options or default_options(**self.case.mlmg)
+        _chk(host_lib().ksgf_host_stepper_enable_diffusion(self.h, mu, diff_type, rtol, atol, C.byref(self._dopts)),
+             "enable_diffusion")
+
+    def diffusion_info(self):
+        infos = (KsgfMLMGInfo * 3)()
+        _chk(host_lib().ksgf_host_stepper_diffusion_info(self.h, infos), "diffusion_info")
+        return [i.as_dict() for i in infos]
 
     def nodal_info(self):
         info = KsgfMLMGInfo()

@@ -370,11 +370,47 @@ def advect_scalar(case, G, scal, rho, mac, dt, scal_bc, scal_bcval, limiter=1):
     return s_new
 
 
+def diffusion_bcs(case, c):
+    """LinOp BCs of velocity component c for the diffusion solve
+    (get_diffuse_tensor_bc, incflo_diffusion.cpp:11-69): int_dir -> periodic,
+    ext_dir -> Dirichlet, extrapolation -> Neumann"""
+    lo, hi = [], []
+    for d in range(3):
+        for side, arr in ((0, lo), (1, hi)):
+            t = int(case.bcrec[c, 3 * side + d])
+            arr.append(0 if t == 0 else (2 if t == 3 else 1))
+    return tuple(lo), tuple(hi)
+
+
+def _diffusion(case, ba, opts, vel, mu, dt_diff, rtol, atol, comp_apply=None):
+    """component-wise (rho_0 - dt_diff mu lap) u = rho_0 u (solve) on the
+    valid cells of vel (3 comp, ng 3), or with comp_apply the apply
+    mu lap(u) (scalars (0, -1)) returned as a (3, ncell) array"""
+    n = tuple(int(v) for v in case.geom.n_cell)
+    dx = case.geom.dx
+    u = ba.gather(vel, CELL, 3, 3)
+    out = []
+    infos = []
+    for c in range(3):
+        lo, hi = diffusion_bcs(case, c)
+        if comp_apply:
+            out.append(abeclap_apply(n, dx, lo, hi, 0.0 * case.rho0, -1.0 * mu, u[c].ravel()))
+        else:
+            rhs = (u[c] * case.rho0).ravel()
+            x, info = abeclap_solve(n, dx, lo, hi, 1.0 * case.rho0, dt_diff * mu, rhs, rtol, atol, opts)
+            out.append(x)
+            infos.append(info)
+    return np.stack(out), infos
+
+
 def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, scal=None, scal_bc=None,
-                        scal_bcval=None):
+                        scal_bcval=None, diffusion=None):
     """One step; returns (vel_new (ng 3, ghosts filled), diagnostics dict).
     With scal (cell, 1 comp, ng 3) the passive scalar is advected too; the
-    new scalar is diag["scal"]."""
+    new scalar is diag["scal"].  diffusion = dict(mu, type (1 Crank-Nicolson,
+    2 Implicit), rtol, atol): the viscous term with constant mu and density
+    rho_0 as ApplyPredictor carries it for Godunov (incflo_advance.cpp:246-291,
+    380-389)."""
     ba = case.ba
     opts = opts or case_opts(case)
     if dt is None:
@@ -382,6 +418,12 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
         # forcing CFL: dt = 2 cfl / (C + sqrt(C^2))
         c = cfl_max(G, vel, 3)
         dt = 2.0 * case.cfl / (c + math.sqrt(c * c))
+    cn = diffusion is not None and diffusion["type"] == 1
+    divtau = None
+    if cn:
+        tau, _ = _diffusion(case, ba, opts, vel, diffusion["mu"], 0.0, 0.0, 0.0, comp_apply=True)
+        n = case.geom.n_cell
+        divtau = ba.scatter(tau.reshape(3, n[2], n[1], n[0]), CELL, 0)
     # sources: src = -gp/rho (velocity form), ghost 1 (foextrap-like fill)
     src = np.zeros(ba.mf_size(CELL, 3, 1))
     for b in range(ba.nbox):
@@ -389,7 +431,10 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
         g_ = ba.fab_view(gp, b, CELL, 3, 0)
         r_ = ba.fab_view(rho, b, CELL, 1, 3)[:, 2:-2, 2:-2, 2:-2]
         rv = r_[0, 1:-1, 1:-1, 1:-1]
-        s[:, 1:-1, 1:-1, 1:-1] = -(1.0 * g_) * (1.0 / rv)
+        if cn:
+            s[:, 1:-1, 1:-1, 1:-1] = (-(1.0 * g_) + ba.fab_view(divtau, b, CELL, 3, 0)) * (1.0 / rv)
+        else:
+            s[:, 1:-1, 1:-1, 1:-1] = -(1.0 * g_) * (1.0 / rv)
     sbc = np.array(case.bcrec)
     sbc[sbc != 0] = 2
     fill_ghosts(G, src, CELL, 3, 1, sbc, np.zeros((6, 3)))
@@ -436,9 +481,16 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
         r_ = ba.fab_view(rho, b, CELL, 1, 3)[:, 3:-3, 3:-3, 3:-3]
         ba.fab_view(src_nph, b, CELL, 3, 0)[...] = (-(1.0 * g_) * (1.0 / r_)) * r_
     vel_new = np.zeros_like(vel)
-    predictor_update(G, vel_new, 3, vel, 3, rho, 3, rho, 3, conv, src_nph, None, 0.0, dt)
+    predictor_update(G, vel_new, 3, vel, 3, rho, 3, rho, 3, conv, src_nph, divtau, 0.5 if cn else 0.0, dt)
+    dinfo = None
+    if diffusion is not None:
+        dt_diff = 0.5 * dt if cn else dt
+        x, dinfo = _diffusion(case, ba, opts, vel_new, diffusion["mu"], dt_diff, diffusion["rtol"], diffusion["atol"])
+        n = case.geom.n_cell
+        ba.scatter(x.reshape(3, n[2], n[1], n[0]), CELL, 3, out=vel_new)
     fill_ghosts(G, vel_new, CELL, 3, 3, case.bcrec, case.bcval)
-    return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face, "scal": scal_new}
+    return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face, "scal": scal_new,
+                     "diffusion": dinfo}
 
 
 def apply_projection(case, G, vel, rho, gp, dt, rtol, atol, opts=None):

@@ -66,3 +66,27 @@ def test_abeclap_tracks_oracle(ksgf, oracle, gpu, name):
     ab.apply(out, 0, rhs, 0, ncomp, 2)
     yref = oracle.abeclap_apply(n, geom.dx, lobc, hibc, alpha * rho, beta * mu, f[2].ravel())
     assert np.array_equal(ba.gather(to_np(out), CELL, ncomp, 0)[2].ravel(), yref)
+
+
+@pytest.mark.parametrize("diff_type", [1, 2])
+def test_host_step_with_diffusion(ksgf, oracle, workloads, gpu, diff_type):
+    """GodunovPredictor with the viscous term (Crank-Nicolson / Implicit,
+    constant mu) against the oracle's step with the same diffusion"""
+    _say(f"host step + diffusion type={diff_type}")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    case = workloads.abl((64, 48, 32), extent=(1280.0, 960.0, 128.0), sizes=(16, 32), nsteps=2)
+    G, vel, rho, gp = oracle.make_state(case)
+    S = hs.HostStepper(case)
+    S.load()
+    mu, rtol, atol = 30.0, 1e-11, 1e-14
+    S.enable_diffusion(mu, diff_type, rtol, atol)
+    diff = {"mu": mu, "type": diff_type, "rtol": rtol, "atol": atol}
+    for n in range(2):
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, diffusion=diff)
+        S.step()
+        di = S.diffusion_info()
+        _say(f"  step {n}: iters gpu={[d['iters'] for d in di]} oracle={[d['iters'] for d in diag['diffusion']]}")
+        assert [d["iters"] for d in di] == [d["iters"] for d in diag["diffusion"]]
+        assert S.last_info["iters"] == diag["mlmg"]["iters"]
+        assert rel_l2(S.velocity(), case.ba.gather(vel, CELL, 3, 3)) <= 1e-11, f"step {n}"
+    S.close()
