@@ -158,6 +158,185 @@ __global__ void __launch_bounds__(256) k_faceA(CMFDev q, CMFDev qbc, CMFDev f, i
 }
 
 // ---------------------------------------------------------------------------
+// stage A, marching form (van Leer-limited and unlimited PPM; faces of y and
+// z).  One thread owns a segment of kMarch consecutive faces along D of one
+// (transverse) face column and walks it: every cell's van Leer slope, interior edge and
+// parabola is formed once and serves the two faces of the cell (Ip for the
+// face above, Im for the face below), where k_faceA forms them once per
+// face side.  The slopes, edges, parabolas and traces are the same
+// ppm_* expressions on the same values, so the states are k_faceA's bit for
+// bit.
+// ---------------------------------------------------------------------------
+constexpr int kMarch = 8;
+
+// launch table of the marching kernels: the faces of direction D of every
+// box grown by glo / ghi, with the D extent cut into segments of kMarch
+struct SegDev {
+    const int* chunks;
+    int glo0, glo1, glo2, ghi0, ghi1, ghi2;
+};
+
+template <int D>
+__device__ __forceinline__ bool seg_locate(const SegDev& r, const int* box, int& b, int& i, int& j, int& k, int& len)
+{
+    b = r.chunks[2 * blockIdx.x];
+    const unsigned loc = (unsigned)r.chunks[2 * blockIdx.x + 1] + threadIdx.x;
+    const int* bx = box + 6 * b;
+    const int l0 = bx[0] - r.glo0, l1 = bx[1] - r.glo1, l2 = bx[2] - r.glo2;
+    const int e0 = bx[3] + r.ghi0 - l0 + 1, e1 = bx[4] + r.ghi1 - l1 + 1, e2 = bx[5] + r.ghi2 - l2 + 1;
+    const int eD = pick<D>(e0, e1, e2);
+    const int ns = (eD + kMarch - 1) / kMarch;
+    const unsigned n0 = (D == 0) ? ns : e0, n1 = (D == 1) ? ns : e1, n2 = (D == 2) ? ns : e2;
+    if (loc >= n0 * n1 * n2) return false;
+    const unsigned q = loc / n0;
+    const int a0 = (int)(loc - q * n0), a1 = (int)(q % n1), a2 = (int)(q / n1);
+    const int seg = pick<D>(a0, a1, a2);
+    i = l0 + ((D == 0) ? seg * kMarch : a0);
+    j = l1 + ((D == 1) ? seg * kMarch : a1);
+    k = l2 + ((D == 2) ? seg * kMarch : a2);
+    len = min(kMarch, eD - seg * kMarch);
+    return true;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_faceA_march(CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev macD,
+                                                     MFDev flo, MFDev fhi, MFDev fed, MFDev uad, SegDev r,
+                                                     const int* __restrict__ box, BCs bc, GeomDev g, double dtdx,
+                                                     double hdt, int uft, int vel_mode, int is_velocity, int lim)
+{
+    int b, i, j, k, len;
+    if (!seg_locate<D>(r, box, b, i, j, k, len)) return;
+    CFab s = fab_at(q, box, b);
+    const int st = (D == 0) ? 1 : (D == 1 ? s.n0 : s.n0 * s.n1);
+    const int cp = s.idx(i, j, k);  // cell p of the segment's first face
+    CFab ff = fab_at(f, box, b);
+    const int fst = (D == 0) ? 1 : (D == 1 ? ff.n0 : ff.n0 * ff.n1);
+    const int fp = ff.idx(i, j, k);
+    CFab fqb = fab_at(qbc, box, b);
+    Fab flo_ = fab_at(flo, box, b);
+    Fab fhi_ = fab_at(fhi, box, b);
+    Fab fed_ = fab_at(fed, box, b);
+    const int ost = (D == 0) ? 1 : (D == 1 ? flo_.n0 : flo_.n0 * flo_.n1);
+    const int op = flo_.idx(i, j, k);
+    const int x0 = pick<D>(i, j, k);
+    const bool vl = (lim == KSGF_PPM_VANLEER);
+    int gli = i, glj = j, glk = k, ghi_ = i, ghj = j, ghk = k;
+    if (D == 0) { gli = g.dlo[0] - 1; ghi_ = g.dhi[0] + 1; }
+    if (D == 1) { glj = g.dlo[1] - 1; ghj = g.dhi[1] + 1; }
+    if (D == 2) { glk = g.dlo[2] - 1; ghk = g.dhi[2] + 1; }
+    // advecting velocity at each face: the MAC velocity, or (velocity mode)
+    // the Burgers solution of the normal component, found first
+    double uf[kMarch];
+    const double* pD = s.p + cp + D * s.cs;  // normal component (velocity mode)
+    if (!vel_mode) {
+        CFab m = fab_at(macD, box, b);
+        const int mst = (D == 0) ? 1 : (D == 1 ? m.n0 : m.n0 * m.n1);
+        const int mp = m.idx(i, j, k);
+#pragma unroll
+        for (int t = 0; t < kMarch; ++t)
+            if (t < len) uf[t] = m.p[mp + t * mst];
+    }
+    for (int tc = 0; tc < ncomp; ++tc) {
+        const int n = vel_mode ? ((tc == 0) ? D : ((tc - 1 < D) ? tc - 1 : tc)) : tc;
+        const bool normal_first = vel_mode && tc == 0;
+        const bool both = vel_mode ? true : (is_velocity && n == D);
+        const double* pn = s.p + cp + n * s.cs;
+        // cells x0-3 .. x0+1: slopes of x0-2 .. x0, the parabola of q = x0-1
+        double s_m3 = pn[-3 * st], s_m2 = pn[-2 * st], s_m1 = pn[-st], s_0 = pn[0], s_p1 = pn[st];
+        // edges e(x0-2|x0-1) and e(x0-1|x0); d_0 is the slope of cell x0
+        double d_0 = 0.0, e_lo, e_hi;
+        if (vl) {
+            const double d_m2 = vanleer(s_m3, s_m2, s_m1);
+            const double d_m1 = vanleer(s_m2, s_m1, s_0);
+            d_0 = vanleer(s_m1, s_0, s_p1);
+            e_lo = ppm_edge_vl(s_m2, s_m1, d_m2, d_m1);
+            e_hi = ppm_edge_vl(s_m1, s_0, d_m1, d_0);
+        } else {
+            e_lo = ppm_edge4(s_m3, s_m2, s_m1, s_0);
+            e_hi = ppm_edge4(s_m2, s_m1, s_0, s_p1);
+        }
+        double smq, spq;
+        ppm_edges_finish(e_lo, e_hi, s_m3, s_m2, s_m1, s_0, s_p1, lim, bc.lo[n][D], bc.hi[n][D], x0 - 1, g.dlo[D],
+                         g.dhi[D], smq, spq);
+        double s_q = s_m1;
+        double uq = vel_mode ? pD[-st] : 0.0;  // velocity mode: the low cell's normal velocity
+        double fq_q = uft ? ff.p[fp - fst + n * ff.cs] : 0.0;
+        // window for face x: cells x-2 .. x+1 in (w_m2, w_m1, w_0, w_p1)
+        double w_m2 = s_m2, w_m1 = s_m1, w_0 = s_0, w_p1 = s_p1;
+        double e_q_hi = e_hi;  // edge x-1|x
+#pragma unroll
+        for (int t = 0; t < kMarch; ++t) {
+            if (t >= len) break;
+            const int x = x0 + t;
+            const double w_p2 = pn[(t + 2) * st];
+            double e_p_hi;
+            if (vl) {
+                const double d_p1 = vanleer(w_0, w_p1, w_p2);
+                e_p_hi = ppm_edge_vl(w_0, w_p1, d_0, d_p1);
+                d_0 = d_p1;
+            } else {
+                e_p_hi = ppm_edge4(w_m1, w_0, w_p1, w_p2);
+            }
+            double smp, spp;
+            ppm_edges_finish(e_q_hi, e_p_hi, w_m2, w_m1, w_0, w_p1, w_p2, lim, bc.lo[n][D], bc.hi[n][D], x, g.dlo[D],
+                             g.dhi[D], smp, spp);
+            const double up = vel_mode ? pD[t * st] : 0.0;
+            const double u_q = vel_mode ? uq : uf[t];
+            const double u_p = vel_mode ? up : uf[t];
+            double lov = ppm_ip(smq, s_q, spq, u_q, dtdx);
+            double hiv = ppm_im(smp, w_0, spp, u_p, dtdx);
+            const double fq_p = uft ? ff.p[fp + t * fst + n * ff.cs] : 0.0;
+            if (uft) {
+                lov += hdt * fq_q;
+                hiv += hdt * fq_p;
+            }
+            double glo = 0.0, ghi = 0.0;
+            int gi = i, gj = j, gk = k;
+            unit<D>(gi, gj, gk, t);
+            if (x == g.dlo[D]) {
+                int a = gi, c = gj, e = gk;
+                if (D == 0) a = gli;
+                if (D == 1) c = glj;
+                if (D == 2) e = glk;
+                glo = fqb(a, c, e, n);
+            }
+            if (x == g.dhi[D] + 1) {
+                int a = gi, c = gj, e = gk;
+                if (D == 0) a = ghi_;
+                if (D == 1) c = ghj;
+                if (D == 2) e = ghk;
+                ghi = fqb(a, c, e, n);
+            }
+            edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[n][D], bc.hi[n][D], glo, ghi, both, lov, hiv);
+            double u;
+            if (normal_first) {
+                u = burgers(lov, hiv);
+                uf[t] = u;
+                Fab fu = fab_at(uad, box, b);
+                fu.p[fu.idx(i, j, k) + t * ost] = u;
+            } else {
+                u = uf[t];
+            }
+            const int o = op + t * ost;
+            flo_.p[o + n * flo_.cs] = lov;
+            fhi_.p[o + n * fhi_.cs] = hiv;
+            fed_.p[o + n * fed_.cs] = upwind(lov, hiv, u);
+            // shift: p becomes the next face's q
+            smq = smp;
+            spq = spp;
+            s_q = w_0;
+            uq = up;
+            fq_q = fq_p;
+            e_q_hi = e_p_hi;
+            w_m2 = w_m1;
+            w_m1 = w_0;
+            w_0 = w_p1;
+            w_p1 = w_p2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // stage B: corner-coupled states on faces normal to D corrected along T.
 // vel_mode: only component E = 3-D-T, advective form with the predicted
 // advective velocities (dt/6 factor);  otherwise all components with the
@@ -517,12 +696,75 @@ static int alloc_stages(const ksgf_boxarray* ba, int ncomp, int ncrn, Stages& S)
     return KSGF_OK;
 }
 
+// chunk table of the marching stage-A launch for direction D (faces of D,
+// grown by one transversely, the D extent in segments of kMarch), kept with
+// the BoxArray's other launch tables under a key outside their range
+static const Region* get_seg_region(const ksgf_boxarray* ba, int D, int* err)
+{
+    RegionKey key;
+    for (int d = 0; d < 3; ++d) key.glo[d] = key.ghi[d] = 0;
+    key.glo[0] = 100 + D;
+    key.glo[1] = kMarch;
+    std::lock_guard<std::mutex> lk(ba->mu);
+    auto it = ba->regions.find(key);
+    if (it != ba->regions.end()) return it->second;
+    Region* r = new Region();
+    r->key = key;
+    r->hpre.resize(ba->nbox + 1);
+    std::vector<int> ch;
+    long long acc = 0;
+    for (int b = 0; b < ba->nbox; ++b) {
+        r->hpre[b] = acc;
+        long long n = 1;
+        for (int d = 0; d < 3; ++d) {
+            const int e = ba->hbox[6 * b + 3 + d] - ba->hbox[6 * b + d] + 1 + ((d == D) ? 1 : 2);
+            n *= (d == D) ? (e + kMarch - 1) / kMarch : e;
+        }
+        for (long long s0 = 0; s0 < n; s0 += kChunk) {
+            ch.push_back(b);
+            ch.push_back((int)s0);
+        }
+        acc += n;
+    }
+    r->hpre[ba->nbox] = acc;
+    r->total = acc;
+    r->nchunks = (int)(ch.size() / 2);
+    if (hipMalloc(&r->chunks, sizeof(int) * ch.size()) != hipSuccess ||
+        hipMemcpy(r->chunks, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (r->chunks) (void)hipFree(r->chunks);
+        delete r;
+        set_error("segment table upload");
+        *err = KSGF_ERR_NOMEM;
+        return nullptr;
+    }
+    ba->regions[key] = r;
+    return r;
+}
+
 template <int D>
 static int launch_faceA(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev macD,
                         const Stages& S, const BCs& bc, const GeomDev& g, double dtdx, double hdt, int uft,
                         int vel_mode, int is_velocity, int lim, hipStream_t st)
 {
     int err = KSGF_OK;
+    // marching along y and z: consecutive lanes are consecutive x columns, so
+    // loads and stores stay coalesced.  Along x the lanes of a march would sit
+    // kMarch faces apart (measured 3.5x slower), so x keeps one face per thread.
+    if (D > 0 && (lim == KSGF_PPM_VANLEER || lim == KSGF_PPM_NOLIMITER)) {
+        const Region* r = get_seg_region(ba, D, &err);
+        if (!r) return err;
+        SegDev sd;
+        sd.chunks = r->chunks;
+        sd.glo0 = (D == 0) ? 0 : 1;
+        sd.glo1 = (D == 1) ? 0 : 1;
+        sd.glo2 = (D == 2) ? 0 : 1;
+        sd.ghi0 = sd.ghi1 = sd.ghi2 = 1;
+        hipLaunchKernelGGL(k_faceA_march<D>, dim3(r->nchunks), dim3(256), 0, st, q, qbc, f, ncomp, macD, S.lo[D],
+                           S.hi[D], S.ed[D], S.uad[D], sd, ba->dbox, bc, g, dtdx, hdt, uft, vel_mode, is_velocity,
+                           lim);
+        KSGF_LAUNCH_CHECK();
+        return KSGF_OK;
+    }
     int glo[3], ghi[3];
     for (int e = 0; e < 3; ++e) {
         glo[e] = (e == D) ? 0 : 1;
