@@ -119,6 +119,10 @@ const char* ksgf_version(void);
  * Replaces amrex::BoxArray / Geometry as consumed by MultiFab construction. */
 int ksgf_boxarray_create(const ksgf_geometry* geom, const int* boxes, int nbox,
                          ksgf_boxarray** out);
+/* A BoxArray of one AMR level: covers_domain = 0 for a refined level whose
+ * boxes cover only part of its domain (disjoint boxes inside the domain). */
+int ksgf_boxarray_create_level(const ksgf_geometry* geom, const int* boxes, int nbox,
+                               int covers_domain, ksgf_boxarray** out);
 int ksgf_boxarray_destroy(ksgf_boxarray* ba);
 long long ksgf_multifab_size(const ksgf_boxarray* ba, int itype, int ncomp, int ngrow);
 
@@ -276,6 +280,23 @@ int ksgf_abeclap_solve(ksgf_abeclap* ab, double* sol, int ng_sol, const double* 
 int ksgf_abeclap_apply(ksgf_abeclap* ab, double* out, int ng_out, const double* in, int ng_in,
                        int ncomp, int comp, ksgf_stream_t stream);
 int ksgf_abeclap_destroy(ksgf_abeclap* ab);
+
+/* Two-level averaging, fine level at refinement ratio `ratio` (2 or 4; the fine
+ * domain is the coarse domain refined, the fine boxes coarsen exactly):
+ * amrex::average_down of cell data (gp after the nodal projection,
+ * incflo_apply_nodal_projection.cpp:457-460): every coarse cell under fine
+ * data gets the mean of its ratio^3 children; amrex::average_down_faces
+ * (advective fluxes before the convective term, icns_advection.H:401-407):
+ * every coarse face under fine faces gets the mean of its ratio^2 children,
+ * in each coarse FAB that holds the face (and its periodic image).  Other
+ * coarse data is left as it is.  fine: ng_f ghosts, crse: ng_c ghosts. */
+int ksgf_average_down(const ksgf_boxarray* fine_ba, const double* fine, int ng_f,
+                      const ksgf_boxarray* crse_ba, double* crse, int ng_c, int ncomp, int ratio,
+                      ksgf_stream_t stream);
+int ksgf_average_down_faces(const ksgf_boxarray* fine_ba, const double* fine_x,
+                            const double* fine_y, const double* fine_z, int ng_f,
+                            const ksgf_boxarray* crse_ba, double* crse_x, double* crse_y,
+                            double* crse_z, int ng_c, int ncomp, int ratio, ksgf_stream_t stream);
 
 /* Hydro::NodalProjector with a constant sigma (incflo::ApplyProjection,
  * src/projection/incflo_apply_nodal_projection.cpp:340-353 for the constant-

@@ -64,7 +64,7 @@ EXPORTS = [
     "ksgf_nodalproj_project", "ksgf_nodalproj_get_phi", "ksgf_nodalproj_compute_rhs", "ksgf_nodalproj_solve",
     "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp", "ksgf_abeclap_create",
     "ksgf_abeclap_set_scalars", "ksgf_abeclap_solve", "ksgf_abeclap_apply", "ksgf_abeclap_destroy",
-    "ksgf_icns_src_term_divtau",
+    "ksgf_icns_src_term_divtau", "ksgf_boxarray_create_level", "ksgf_average_down", "ksgf_average_down_faces",
 ]
 
 _lib = None
@@ -161,6 +161,10 @@ def load(path=None):
                                      C.c_double, C.POINTER(KsgfMLMGInfo), C.c_void_p]
     L.ksgf_abeclap_apply.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_abeclap_destroy.argtypes = [C.c_void_p]
+    L.ksgf_boxarray_create_level.argtypes = [C.POINTER(KsgfGeometry), _ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.ksgf_average_down.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_average_down_faces.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_void_p, _dp, _dp, _dp, C.c_int,
+                                          C.c_int, C.c_int, C.c_void_p]
     L.ksgf_icns_src_term_divtau.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_incflo_add_gradp.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
                                         C.c_void_p]
@@ -224,7 +228,7 @@ def default_options(**kw):
 class DeviceBoxArray:
     """ksgf_boxarray handle for a layout.BoxArray."""
 
-    def __init__(self, ba: BoxArray):
+    def __init__(self, ba: BoxArray, covers_domain=True):
         import numpy as np
 
         L = load()
@@ -238,8 +242,8 @@ class DeviceBoxArray:
             g.is_periodic[d] = int(ba.geom.periodic[d])
         self._boxes = np.ascontiguousarray(ba.boxes, dtype=np.int32)
         h = C.c_void_p()
-        check(L.ksgf_boxarray_create(C.byref(g), self._boxes.ctypes.data_as(_ip), ba.nbox, C.byref(h)),
-              "ksgf_boxarray_create")
+        check(L.ksgf_boxarray_create_level(C.byref(g), self._boxes.ctypes.data_as(_ip), ba.nbox, int(covers_domain),
+                                           C.byref(h)), "ksgf_boxarray_create")
         self.h = h
 
     def size(self, itype, ncomp, ngrow):
@@ -371,6 +375,17 @@ class DeviceBoxArray:
     def write_checkpoint_header(self, path, step, time, dt, dt_nm1=0.0, dt_nm2=0.0):
         check(load().ksgf_checkpoint_write_header(str(path).encode(), self.h, step, time, dt, dt_nm1, dt_nm2),
               "checkpoint_write_header")
+
+    def average_down_to(self, fine, ng_f, crse_dba, crse, ng_c, ncomp, ratio=2):
+        """amrex::average_down of this (fine) level's cell MultiFab into crse"""
+        check(load().ksgf_average_down(self.h, _ptr(fine), ng_f, crse_dba.h, _ptr(crse), ng_c, ncomp, ratio,
+                                       _stream()), "average_down")
+
+    def average_down_faces_to(self, fine3, ng_f, crse_dba, crse3, ng_c, ncomp, ratio=2):
+        """amrex::average_down_faces of this level's three face MultiFabs into crse3"""
+        check(load().ksgf_average_down_faces(self.h, _ptr(fine3[0]), _ptr(fine3[1]), _ptr(fine3[2]), ng_f, crse_dba.h,
+                                             _ptr(crse3[0]), _ptr(crse3[1]), _ptr(crse3[2]), ng_c, ncomp, ratio,
+                                             _stream()), "average_down_faces")
 
     def set_bndry(self, mf, itype, ncomp, ngrow, val=0.0):
         check(load().ksgf_set_bndry(self.h, _ptr(mf), itype, ncomp, ngrow, val, _stream()), "set_bndry")

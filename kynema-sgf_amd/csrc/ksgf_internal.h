@@ -94,6 +94,9 @@ struct ksgf_boxarray {
     // per-stage scratch for the advection kernels, grown on demand
     mutable std::vector<double*> scratch;
     mutable std::vector<size_t> scratch_len;
+    // box index of every domain cell (-1 where no box), built on first use
+    mutable int* owner = nullptr;
+    bool covers_domain = true;
 };
 
 namespace ksgf {
@@ -103,6 +106,7 @@ const Region* get_region(const ksgf_boxarray* ba, const int glo[3], const int gh
 const Region* fab_region(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
 const HaloTable* get_halo(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
 double* get_scratch(const ksgf_boxarray* ba, int slot, size_t n, int* err);
+int owner_map(const ksgf_boxarray* ba, hipStream_t st, const int** out);
 
 inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
 

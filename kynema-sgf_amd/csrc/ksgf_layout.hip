@@ -350,6 +350,56 @@ __global__ void __launch_bounds__(256) k_set_inflow(MFDev m, RegDev r, const int
 
 }  // namespace ksgf
 
+namespace ksgf {
+
+__global__ void __launch_bounds__(256) k_owner(int* __restrict__ owner, RegDev r, const int* __restrict__ box, int dlo0,
+                                               int dlo1, int dlo2, int n0, int n1)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    owner[(i - dlo0) + n0 * ((j - dlo1) + n1 * (k - dlo2))] = b;
+}
+
+__global__ void __launch_bounds__(256) k_fill_int(int* __restrict__ a, long long n, int v)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// box index of every domain cell (-1 where the BoxArray has no box), kept
+// with the BoxArray after the first call
+int owner_map(const ksgf_boxarray* ba, hipStream_t st, const int** out)
+{
+    int err = KSGF_OK;
+    const int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);  // takes the BoxArray's lock itself
+    if (!r) return err;
+    std::lock_guard<std::mutex> lk(ba->mu);
+    if (!ba->owner) {
+        const long long n = (long long)(ba->dhi[0] - ba->dlo[0] + 1) * (ba->dhi[1] - ba->dlo[1] + 1) *
+                            (ba->dhi[2] - ba->dlo[2] + 1);
+        int* o = nullptr;
+        if (hipMalloc(&o, sizeof(int) * n) != hipSuccess) {
+            set_error("hipMalloc owner map");
+            return KSGF_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n, 256)), dim3(256), 0, st, o, n, -1);
+        hipLaunchKernelGGL(k_owner, dim3(r->nchunks), dim3(kChunk), 0, st, o, regdev(r, ba->nbox), ba->dbox, ba->dlo[0],
+                           ba->dlo[1], ba->dlo[2], ba->dhi[0] - ba->dlo[0] + 1, ba->dhi[1] - ba->dlo[1] + 1);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            (void)hipFree(o);
+            set_error(std::string("owner map: ") + hipGetErrorString(e));
+            return KSGF_ERR_HIP;
+        }
+        ba->owner = o;
+    }
+    *out = ba->owner;
+    return KSGF_OK;
+}
+
+}  // namespace ksgf
+
 using namespace ksgf;
 
 extern "C" {
@@ -358,6 +408,12 @@ const char* ksgf_last_error(void) { return g_last_error.c_str(); }
 const char* ksgf_version(void) { return "ksgf-gfx950 0.1"; }
 
 int ksgf_boxarray_create(const ksgf_geometry* geom, const int* boxes, int nbox, ksgf_boxarray** out)
+{
+    return ksgf_boxarray_create_level(geom, boxes, nbox, 1, out);
+}
+
+int ksgf_boxarray_create_level(const ksgf_geometry* geom, const int* boxes, int nbox, int covers_domain,
+                               ksgf_boxarray** out)
 {
     if (!geom || !boxes || nbox <= 0 || !out) {
         set_error("ksgf_boxarray_create: null argument or empty box list");
@@ -394,7 +450,8 @@ int ksgf_boxarray_create(const ksgf_geometry* geom, const int* boxes, int nbox, 
         }
         covered += c;
     }
-    if (covered != ncell) {
+    ba->covers_domain = covers_domain != 0;
+    if (covers_domain ? (covered != ncell) : (covered > ncell)) {
         delete ba;
         set_error("ksgf_boxarray_create: boxes do not tile the domain");
         return KSGF_ERR_INVALID;
@@ -426,6 +483,7 @@ int ksgf_boxarray_destroy(ksgf_boxarray* ba)
     for (double* s : ba->scratch)
         if (s) (void)hipFree(s);
     if (ba->dbox) (void)hipFree(ba->dbox);
+    if (ba->owner) (void)hipFree(ba->owner);
     delete ba;
     return KSGF_OK;
 }

@@ -115,9 +115,132 @@ __global__ void __launch_bounds__(256) k_mac_divmax(CMFDev u, CMFDev v, CMFDev w
     }
 }
 
+// amrex::average_down: one thread per coarse cell of a coarsened fine box
+// (launched over the coarsened copy of the fine BoxArray, box b = fine box b)
+__global__ void __launch_bounds__(256) k_avgdown(CMFDev fine, const int* __restrict__ fbox, MFDev crse,
+                                                 const int* __restrict__ cbox, const int* __restrict__ owner,
+                                                 int ncomp, int ratio, double vf, int dlo0, int dlo1, int dlo2, int n0,
+                                                 int n1, RegDev r, const int* __restrict__ rbox)
+{
+    int b, I, J, K;
+    if (!region_locate(r, rbox, b, I, J, K)) return;
+    const int cb = owner[(I - dlo0) + n0 * ((J - dlo1) + n1 * (K - dlo2))];
+    if (cb < 0) return;
+    CFab f = fab_at(fine, fbox, b);
+    Fab c = fab_at(crse, cbox, cb);
+    for (int n = 0; n < ncomp; ++n) {
+        double sum = 0.0;
+        for (int kk = 0; kk < ratio; ++kk)
+            for (int jj = 0; jj < ratio; ++jj)
+                for (int ii = 0; ii < ratio; ++ii) sum += f(I * ratio + ii, J * ratio + jj, K * ratio + kk, n);
+        c(I, J, K, n) = sum * vf;
+    }
+}
+
+// amrex::average_down_faces for faces of direction D: the coarse face goes to
+// the FAB of the cell above it, the FAB of the cell below it (when another
+// box), and on a periodic direction the image face's FAB
+template <int D>
+__global__ void __launch_bounds__(256) k_avgdown_faces(CMFDev fine, const int* __restrict__ fbox, MFDev crse,
+                                                       const int* __restrict__ cbox, const int* __restrict__ owner,
+                                                       int ncomp, int ratio, double af, int dlo0, int dlo1, int dlo2,
+                                                       int dhiD, int n0, int n1, int periodic, RegDev r,
+                                                       const int* __restrict__ rbox)
+{
+    constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+    int b, I, J, K;
+    if (!region_locate(r, rbox, b, I, J, K)) return;
+    CFab f = fab_at(fine, fbox, b);
+    const int C[3] = {I, J, K};
+    const int dlo[3] = {dlo0, dlo1, dlo2};
+    const int nd = dhiD - dlo[D] + 1;
+    auto own = [&](int a0, int a1, int a2) {
+        return owner[(a0 - dlo0) + n0 * ((a1 - dlo1) + n1 * (a2 - dlo2))];
+    };
+    // targets: (box, face coordinates)
+    int tb[3], tc[3][3], nt = 0;
+    if (C[D] <= dhiD) {
+        tb[nt] = own(I, J, K);
+        for (int e = 0; e < 3; ++e) tc[nt][e] = C[e];
+        ++nt;
+    }
+    if (C[D] > dlo[D]) {
+        int q[3] = {I, J, K};
+        q[D] -= 1;
+        const int ob = own(q[0], q[1], q[2]);
+        if (nt == 0 || ob != tb[0]) {
+            tb[nt] = ob;
+            for (int e = 0; e < 3; ++e) tc[nt][e] = C[e];
+            ++nt;
+        }
+    }
+    if (periodic && (C[D] == dlo[D] || C[D] == dhiD + 1)) {
+        int p[3] = {I, J, K};
+        int q[3] = {I, J, K};
+        if (C[D] == dlo[D]) {  // image at the high side, held by the cell below it
+            p[D] = C[D] + nd;
+            q[D] = dhiD;
+        } else {  // image at the low side, held by the cell above it
+            p[D] = C[D] - nd;
+            q[D] = dlo[D];
+        }
+        tb[nt] = own(q[0], q[1], q[2]);
+        for (int e = 0; e < 3; ++e) tc[nt][e] = p[e];
+        ++nt;
+    }
+    for (int n = 0; n < ncomp; ++n) {
+        double sum = 0.0;
+        for (int b2 = 0; b2 < ratio; ++b2)
+            for (int b1 = 0; b1 < ratio; ++b1) {
+                int g3[3] = {C[0] * ratio, C[1] * ratio, C[2] * ratio};
+                g3[T1] += b1;
+                g3[T2] += b2;
+                sum += f(g3[0], g3[1], g3[2], n);
+            }
+        const double v = sum * af;
+        for (int t = 0; t < nt; ++t) {
+            if (tb[t] < 0) continue;
+            Fab c = fab_at(crse, cbox, tb[t]);
+            c(tc[t][0], tc[t][1], tc[t][2], n) = v;
+        }
+    }
+}
+
 }  // namespace ksgf
 
 using namespace ksgf;
+
+namespace {
+
+// the fine BoxArray coarsened by ratio on the coarse geometry (a level that
+// need not cover the domain); checks that the two levels match
+int coarsened_level(const ksgf_boxarray* fba, const ksgf_boxarray* cba, int ratio, ksgf_boxarray** out)
+{
+    if (ratio != 2 && ratio != 4) {
+        set_error("average_down: refinement ratio 2 or 4");
+        return KSGF_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (fba->dlo[d] != ratio * cba->dlo[d] || fba->dhi[d] + 1 != ratio * (cba->dhi[d] + 1) ||
+            fba->periodic[d] != cba->periodic[d]) {
+            set_error("average_down: the fine domain is not the coarse domain refined by ratio");
+            return KSGF_ERR_INVALID;
+        }
+    std::vector<int> cb(6 * fba->nbox);
+    for (int b = 0; b < fba->nbox; ++b)
+        for (int d = 0; d < 3; ++d) {
+            const int lo = fba->hbox[6 * b + d], hi = fba->hbox[6 * b + 3 + d];
+            if (lo % ratio != 0 || (hi + 1) % ratio != 0) {
+                set_error("average_down: a fine box does not coarsen by ratio");
+                return KSGF_ERR_INVALID;
+            }
+            cb[6 * b + d] = lo / ratio;
+            cb[6 * b + 3 + d] = (hi + 1) / ratio - 1;
+        }
+    return ksgf_boxarray_create_level(&cba->geom, cb.data(), fba->nbox, 0, out);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -288,6 +411,101 @@ int ksgf_icns_src_term_divtau(const ksgf_boxarray* ba, double* src, int ng_src, 
                        regdev(r, ba->nbox), ba->dbox);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
+}
+
+int ksgf_average_down(const ksgf_boxarray* fine_ba, const double* fine, int ng_f, const ksgf_boxarray* crse_ba,
+                      double* crse, int ng_c, int ncomp, int ratio, ksgf_stream_t stream)
+{
+    if (!fine_ba || !fine || !crse_ba || !crse || ncomp <= 0 || ng_f < 0 || ng_c < 0) {
+        set_error("ksgf_average_down: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev f;
+    MFDev c;
+    int e;
+    if ((e = cmfdev(fine_ba, fine, KSGF_CELL, ncomp, ng_f, &f)) || (e = mfdev(crse_ba, crse, KSGF_CELL, ncomp, ng_c, &c)))
+        return e;
+    const int* owner = nullptr;
+    if ((e = owner_map(crse_ba, st, &owner))) return e;
+    ksgf_boxarray* cl = nullptr;
+    if ((e = coarsened_level(fine_ba, crse_ba, ratio, &cl))) return e;
+    int err = KSGF_OK;
+    const int z[3] = {0, 0, 0};
+    const Region* r = get_region(cl, z, z, &err);
+    if (r)
+        hipLaunchKernelGGL(k_avgdown, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c, crse_ba->dbox, owner, ncomp,
+                           ratio, 1.0 / (double)(ratio * ratio * ratio), crse_ba->dlo[0], crse_ba->dlo[1],
+                           crse_ba->dlo[2], crse_ba->dhi[0] - crse_ba->dlo[0] + 1, crse_ba->dhi[1] - crse_ba->dlo[1] + 1,
+                           regdev(r, cl->nbox), cl->dbox);
+    hipError_t le = hipGetLastError();
+    // the coarsened copy's tables may be in use by the launch: release after it
+    (void)hipStreamSynchronize(st);
+    ksgf_boxarray_destroy(cl);
+    if (!r) return err;
+    if (le != hipSuccess) {
+        set_error(std::string("ksgf_average_down: ") + hipGetErrorString(le));
+        return KSGF_ERR_HIP;
+    }
+    return KSGF_OK;
+}
+
+int ksgf_average_down_faces(const ksgf_boxarray* fine_ba, const double* fine_x, const double* fine_y,
+                            const double* fine_z, int ng_f, const ksgf_boxarray* crse_ba, double* crse_x,
+                            double* crse_y, double* crse_z, int ng_c, int ncomp, int ratio, ksgf_stream_t stream)
+{
+    const double* fp[3] = {fine_x, fine_y, fine_z};
+    double* cp[3] = {crse_x, crse_y, crse_z};
+    if (!fine_ba || !crse_ba || !fine_x || !fine_y || !fine_z || !crse_x || !crse_y || !crse_z || ncomp <= 0 ||
+        ng_f < 0 || ng_c < 0) {
+        set_error("ksgf_average_down_faces: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int e;
+    const int* owner = nullptr;
+    if ((e = owner_map(crse_ba, st, &owner))) return e;
+    ksgf_boxarray* cl = nullptr;
+    if ((e = coarsened_level(fine_ba, crse_ba, ratio, &cl))) return e;
+    const double af = 1.0 / (double)(ratio * ratio);
+    const int n0 = crse_ba->dhi[0] - crse_ba->dlo[0] + 1, n1 = crse_ba->dhi[1] - crse_ba->dlo[1] + 1;
+    int ret = KSGF_OK;
+    for (int d = 0; d < 3 && ret == KSGF_OK; ++d) {
+        CMFDev f;
+        MFDev c;
+        if ((ret = cmfdev(fine_ba, fp[d], KSGF_XFACE + d, ncomp, ng_f, &f)) ||
+            (ret = mfdev(crse_ba, cp[d], KSGF_XFACE + d, ncomp, ng_c, &c)))
+            break;
+        int err = KSGF_OK;
+        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        ghi[d] = 1;
+        const Region* r = get_region(cl, glo, ghi, &err);
+        if (!r) {
+            ret = err;
+            break;
+        }
+        const RegDev rd = regdev(r, cl->nbox);
+        if (d == 0)
+            hipLaunchKernelGGL(k_avgdown_faces<0>, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c,
+                               crse_ba->dbox, owner, ncomp, ratio, af, crse_ba->dlo[0], crse_ba->dlo[1],
+                               crse_ba->dlo[2], crse_ba->dhi[0], n0, n1, crse_ba->periodic[0], rd, cl->dbox);
+        else if (d == 1)
+            hipLaunchKernelGGL(k_avgdown_faces<1>, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c,
+                               crse_ba->dbox, owner, ncomp, ratio, af, crse_ba->dlo[0], crse_ba->dlo[1],
+                               crse_ba->dlo[2], crse_ba->dhi[1], n0, n1, crse_ba->periodic[1], rd, cl->dbox);
+        else
+            hipLaunchKernelGGL(k_avgdown_faces<2>, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c,
+                               crse_ba->dbox, owner, ncomp, ratio, af, crse_ba->dlo[0], crse_ba->dlo[1],
+                               crse_ba->dlo[2], crse_ba->dhi[2], n0, n1, crse_ba->periodic[2], rd, cl->dbox);
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            set_error(std::string("ksgf_average_down_faces: ") + hipGetErrorString(le));
+            ret = KSGF_ERR_HIP;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    ksgf_boxarray_destroy(cl);
+    return ret;
 }
 
 }  // extern "C"

@@ -1560,6 +1560,115 @@ int ko_mac_project(const ko_geom* g, double* umac, double* vmac, double* wmac,
 }
 
 /* ------------------------------------------------------------------ */
+/* Two-level averaging (amrex::average_down for cell data, used on gp  */
+/* after the projection, incflo_apply_nodal_projection.cpp:457-460;    */
+/* amrex::average_down_faces on the advective fluxes,                  */
+/* icns_advection.H:401-407).  The fine BoxArray is at refinement      */
+/* ratio r (its boxes coarsen exactly); every coarse cell (face)        */
+/* covered by fine data gets the sum of its r^3 children (r^2 faces),  */
+/* k outermost and i innermost, times 1/r^3 (1/r^2).                   */
+/* ------------------------------------------------------------------ */
+static int find_box(const ko_geom* g, int i, int j, int k, int itype)
+{
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        if (i >= bx[0] && i <= bx[3] + (itype == KO_XFACE) && j >= bx[1] && j <= bx[4] + (itype == KO_YFACE) &&
+            k >= bx[2] && k <= bx[5] + (itype == KO_ZFACE))
+            return b;
+    }
+    return -1;
+}
+
+void ko_average_down(const ko_geom* fg, const double* fine, int ng_f, const ko_geom* cg, double* crse, int ng_c,
+                     int ncomp, int ratio)
+{
+    fab_t* ff = (fab_t*)malloc(sizeof(fab_t) * (size_t)fg->nbox);
+    fab_t* fc = (fab_t*)malloc(sizeof(fab_t) * (size_t)cg->nbox);
+    make_fabs(fg, KO_CELL, ncomp, ng_f, ff);
+    make_fabs(cg, KO_CELL, ncomp, ng_c, fc);
+    const double vf = 1.0 / (double)(ratio * ratio * ratio);
+    for (int b = 0; b < fg->nbox; ++b) {
+        const int* bx = fg->boxes + 6 * b;
+        for (int K = bx[2] / ratio; K <= bx[5] / ratio; ++K)
+            for (int J = bx[1] / ratio; J <= bx[4] / ratio; ++J)
+                for (int I = bx[0] / ratio; I <= bx[3] / ratio; ++I) {
+                    const int cb = find_box(cg, I, J, K, KO_CELL);
+                    if (cb < 0) continue;
+                    for (int n = 0; n < ncomp; ++n) {
+                        double sum = 0.0;
+                        for (int kk = 0; kk < ratio; ++kk)
+                            for (int jj = 0; jj < ratio; ++jj)
+                                for (int ii = 0; ii < ratio; ++ii)
+                                    sum += fine[FI(&ff[b], I * ratio + ii, J * ratio + jj, K * ratio + kk) +
+                                                n * FS(&ff[b])];
+                        crse[FI(&fc[cb], I, J, K) + n * FS(&fc[cb])] = sum * vf;
+                    }
+                }
+    }
+    free(ff);
+    free(fc);
+}
+
+/* fine / crse: three face MultiFabs (x, y, z) each */
+void ko_average_down_faces(const ko_geom* fg, const double* const* fine, int ng_f, const ko_geom* cg,
+                           double* const* crse, int ng_c, int ncomp, int ratio)
+{
+    fab_t* ff = (fab_t*)malloc(sizeof(fab_t) * (size_t)fg->nbox);
+    fab_t* fc = (fab_t*)malloc(sizeof(fab_t) * (size_t)cg->nbox);
+    const double af = 1.0 / (double)(ratio * ratio);
+    for (int d = 0; d < 3; ++d) {
+        make_fabs(fg, KO_XFACE + d, ncomp, ng_f, ff);
+        make_fabs(cg, KO_XFACE + d, ncomp, ng_c, fc);
+        const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+        for (int b = 0; b < fg->nbox; ++b) {
+            const int* bx = fg->boxes + 6 * b;
+            int lo[3], hi[3];
+            for (int e = 0; e < 3; ++e) {
+                lo[e] = bx[e] / ratio;
+                hi[e] = bx[3 + e] / ratio + (e == d);
+            }
+            for (int K = lo[2]; K <= hi[2]; ++K)
+                for (int J = lo[1]; J <= hi[1]; ++J)
+                    for (int I = lo[0]; I <= hi[0]; ++I) {
+                        const int C[3] = {I, J, K};
+                        /* every coarse FAB holding the face gets it, and on a
+                         * periodic direction the image face at the other side */
+                        int img[3] = {I, J, K}, has_img = 0;
+                        if (cg->periodic[d]) {
+                            const int nd = cg->dhi[d] - cg->dlo[d] + 1;
+                            if (C[d] == cg->dlo[d]) { img[d] = C[d] + nd; has_img = 1; }
+                            if (C[d] == cg->dhi[d] + 1) { img[d] = C[d] - nd; has_img = 1; }
+                        }
+                        for (int n = 0; n < ncomp; ++n) {
+                            double sum = 0.0;
+                            for (int b2 = 0; b2 < ratio; ++b2)
+                                for (int b1 = 0; b1 < ratio; ++b1) {
+                                    int f3[3];
+                                    for (int e = 0; e < 3; ++e) f3[e] = C[e] * ratio;
+                                    f3[t1] += b1;
+                                    f3[t2] += b2;
+                                    sum += fine[d][FI(&ff[b], f3[0], f3[1], f3[2]) + n * FS(&ff[b])];
+                                }
+                            const double v = sum * af;
+                            for (int cb = 0; cb < cg->nbox; ++cb) {
+                                const int* cx = cg->boxes + 6 * cb;
+                                for (int w = 0; w < 1 + has_img; ++w) {
+                                    const int* P = w ? img : C;
+                                    int in = 1;
+                                    for (int e = 0; e < 3; ++e)
+                                        if (P[e] < cx[e] || P[e] > cx[3 + e] + (e == d)) in = 0;
+                                    if (in) crse[d][FI(&fc[cb], P[0], P[1], P[2]) + n * FS(&fc[cb])] = v;
+                                }
+                            }
+                        }
+                    }
+        }
+    }
+    free(ff);
+    free(fc);
+}
+
+/* ------------------------------------------------------------------ */
 /* CFL (a2; incflo_compute_dt.cpp:69-91) and predictor (a16;           */
 /* CompRHSOps.H:93-117)                                                */
 /* ------------------------------------------------------------------ */

@@ -142,6 +142,12 @@ int ko_abeclap_solve(const int* n, const double* dx, const int* lobc, const int*
 void ko_abeclap_apply(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha_a,
                       double beta_b, const double* x, double* y);
 
+/* amrex::average_down / average_down_faces at refinement ratio `ratio` */
+void ko_average_down(const ko_geom* fg, const double* fine, int ng_f, const ko_geom* cg, double* crse,
+                     int ng_c, int ncomp, int ratio);
+void ko_average_down_faces(const ko_geom* fg, const double* const* fine, int ng_f, const ko_geom* cg,
+                           double* const* crse, int ng_c, int ncomp, int ratio);
+
 double ko_cfl_max(const ko_geom* g, const double* vel, int ng);
 
 void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,

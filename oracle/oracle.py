@@ -95,6 +95,10 @@ def lib():
         L.ko_abeclap_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_double,
                                        C.POINTER(KoOpts), C.POINTER(KoInfo)]
         L.ko_abeclap_apply.argtypes = [_ip, _dp, _ip, _ip, C.c_double, C.c_double, _dp, _dp]
+        L.ko_average_down.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.POINTER(KoGeom), _dp, C.c_int, C.c_int,
+                                      C.c_int]
+        L.ko_average_down_faces.argtypes = [C.POINTER(KoGeom), C.POINTER(_dp), C.c_int, C.POINTER(KoGeom),
+                                            C.POINTER(_dp), C.c_int, C.c_int, C.c_int]
         L.ko_cfl_max.restype = C.c_double
         L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
         L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
@@ -305,6 +309,18 @@ def abeclap_apply(n, dx, lobc, hibc, alpha_a, beta_b, x):
     y = np.zeros_like(x)
     lib().ko_abeclap_apply(pn, _p(dd), plo, phi_, alpha_a, beta_b, _p(x), _p(y))
     return y
+
+
+def average_down(Gf, fine, ng_f, Gc, crse, ng_c, ncomp, ratio):
+    """amrex::average_down into crse (in place)"""
+    lib().ko_average_down(Gf.ref, _p(fine), ng_f, Gc.ref, _p(crse), ng_c, ncomp, ratio)
+
+
+def average_down_faces(Gf, fine3, ng_f, Gc, crse3, ng_c, ncomp, ratio):
+    """amrex::average_down_faces into crse3 (three face MultiFabs, in place)"""
+    f = (_dp * 3)(*[_p(a) for a in fine3])
+    c = (_dp * 3)(*[_p(a) for a in crse3])
+    lib().ko_average_down_faces(Gf.ref, f, ng_f, Gc.ref, c, ng_c, ncomp, ratio)
 
 
 def cfl_max(G, vel, ng):
