@@ -298,6 +298,19 @@ int ksgf_average_down_faces(const ksgf_boxarray* fine_ba, const double* fine_x,
                             const ksgf_boxarray* crse_ba, double* crse_x, double* crse_y,
                             double* crse_z, int ng_c, int ncomp, int ratio, ksgf_stream_t stream);
 
+/* amrex::FillPatchTwoLevels for the ghost cells of a refined level's cell
+ * MultiFab, as Field::fillpatch fills level > 0 (FieldFillPatchOps.H:272-336),
+ * without time interpolation: ghost cells inside the domain take the coarse data
+ * interpolated (conservative linear, the default CellConsLinear: MC-limited
+ * slopes of the parent, scaled so no child leaves the range of the parent and
+ * its face neighbours), then the fine level's own data where a fine box holds
+ * them (FillBoundary), then the physical BCs (bc / bcval as ksgf_fill_physbc).
+ * crse: coarse cell MultiFab with ng_c >= 1 ghosts already filled. */
+int ksgf_fillpatch_two_levels(const ksgf_boxarray* fine_ba, double* fine, int ng,
+                              const ksgf_boxarray* crse_ba, const double* crse, int ng_c,
+                              int ncomp, int ratio, const ksgf_bcrec* bc, const double* bcval,
+                              ksgf_stream_t stream);
+
 /* Hydro::NodalProjector with a constant sigma (incflo::ApplyProjection,
  * src/projection/incflo_apply_nodal_projection.cpp:340-353 for the constant-
  * density branch, options from MLMGOptions::operator()(NodalProjector&),

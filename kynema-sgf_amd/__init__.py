@@ -65,6 +65,7 @@ EXPORTS = [
     "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp", "ksgf_abeclap_create",
     "ksgf_abeclap_set_scalars", "ksgf_abeclap_solve", "ksgf_abeclap_apply", "ksgf_abeclap_destroy",
     "ksgf_icns_src_term_divtau", "ksgf_boxarray_create_level", "ksgf_average_down", "ksgf_average_down_faces",
+    "ksgf_fillpatch_two_levels",
 ]
 
 _lib = None
@@ -165,6 +166,8 @@ def load(path=None):
     L.ksgf_average_down.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_average_down_faces.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_void_p, _dp, _dp, _dp, C.c_int,
                                           C.c_int, C.c_int, C.c_void_p]
+    L.ksgf_fillpatch_two_levels.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(KsgfBCRec), C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_icns_src_term_divtau.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_incflo_add_gradp.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
                                         C.c_void_p]
@@ -386,6 +389,15 @@ class DeviceBoxArray:
         check(load().ksgf_average_down_faces(self.h, _ptr(fine3[0]), _ptr(fine3[1]), _ptr(fine3[2]), ng_f, crse_dba.h,
                                              _ptr(crse3[0]), _ptr(crse3[1]), _ptr(crse3[2]), ng_c, ncomp, ratio,
                                              _stream()), "average_down_faces")
+
+    def fillpatch_two_levels(self, fine, ng, crse_dba, crse, ng_c, ncomp, bcrec, bcval, ratio=2):
+        """FillPatchTwoLevels of this (fine) level's ghost cells from crse"""
+        import numpy as np
+
+        bv = np.ascontiguousarray(bcval, dtype=np.float64)
+        check(load().ksgf_fillpatch_two_levels(self.h, _ptr(fine), ng, crse_dba.h, _ptr(crse), ng_c, ncomp, ratio,
+                                               bcrecs(bcrec), bv.ctypes.data_as(C.POINTER(C.c_double)), _stream()),
+              "fillpatch_two_levels")
 
     def set_bndry(self, mf, itype, ncomp, ngrow, val=0.0):
         check(load().ksgf_set_bndry(self.h, _ptr(mf), itype, ncomp, ngrow, val, _stream()), "set_bndry")

@@ -115,6 +115,11 @@ __global__ void __launch_bounds__(256) k_mac_divmax(CMFDev u, CMFDev v, CMFDev w
     }
 }
 
+struct GeomFP {
+    int fdlo[3], fdhi[3], per[3], cdlo[3];
+    int cn0, cn1;
+};
+
 // amrex::average_down: one thread per coarse cell of a coarsened fine box
 // (launched over the coarsened copy of the fine BoxArray, box b = fine box b)
 __global__ void __launch_bounds__(256) k_avgdown(CMFDev fine, const int* __restrict__ fbox, MFDev crse,
@@ -203,6 +208,63 @@ __global__ void __launch_bounds__(256) k_avgdown_faces(CMFDev fine, const int* _
             Fab c = fab_at(crse, cbox, tb[t]);
             c(tc[t][0], tc[t][1], tc[t][2], n) = v;
         }
+    }
+}
+
+// conservative linear interpolation of a refined level's ghost cells from the
+// coarse level (FillPatchTwoLevels' interpolation step): MC-limited slopes of
+// the parent coarse cell, scaled by one alpha <= 1 that keeps every child in
+// [min, max] of the parent and its face neighbours
+__device__ __forceinline__ double fp_mc_slope(double um, double uc, double up)
+{
+    const double dc = 0.5 * (up - um), dl = uc - um, dr = up - uc;
+    if (dl * dr > 0.0) return copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc);
+    return 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_fp_interp(MFDev fine, const int* __restrict__ fbox, CMFDev crse,
+                                                   const int* __restrict__ cbox, const int* __restrict__ cowner,
+                                                   int ncomp, int ratio, double omax, GeomFP g, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, fbox, b, i, j, k)) return;
+    const int* bx = fbox + 6 * b;
+    if (i >= bx[0] && i <= bx[3] && j >= bx[1] && j <= bx[4] && k >= bx[2] && k <= bx[5]) return;
+    const int p[3] = {i, j, k};
+    int P[3];
+    double off[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int L = g.fdhi[d] - g.fdlo[d] + 1;
+        int x = p[d];
+        if (g.per[d])
+            x = g.fdlo[d] + (((x - g.fdlo[d]) % L) + L) % L;
+        else if (x < g.fdlo[d] || x > g.fdhi[d])
+            return;  // outside a non-periodic domain: physical BC
+        P[d] = (x >= 0) ? x / ratio : -((-x + ratio - 1) / ratio);
+        off[d] = ((double)(x - ratio * P[d]) + 0.5) / (double)ratio - 0.5;
+    }
+    const int cb = cowner[(P[0] - g.cdlo[0]) + g.cn0 * ((P[1] - g.cdlo[1]) + g.cn1 * (P[2] - g.cdlo[2]))];
+    if (cb < 0) return;
+    CFab c = fab_at(crse, cbox, cb);
+    Fab f = fab_at(fine, fbox, b);
+    const int c0 = c.idx(P[0], P[1], P[2]);
+    const int st[3] = {1, c.n0, c.n0 * c.n1};
+    for (int n = 0; n < ncomp; ++n) {
+        const double* cs = c.p + n * c.cs;
+        const double uc = cs[c0];
+        double sl[3], umax = uc, umin = uc, dev = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double um = cs[c0 - st[d]], up = cs[c0 + st[d]];
+            sl[d] = fp_mc_slope(um, uc, up);
+            umax = fmax(umax, fmax(um, up));
+            umin = fmin(umin, fmin(um, up));
+            dev += fabs(sl[d]) * omax;
+        }
+        double alpha = 1.0;
+        if (dev > 0.0) alpha = fmin(1.0, fmin((umax - uc) / dev, (uc - umin) / dev));
+        f(i, j, k, n) = uc + alpha * ((sl[0] * off[0] + sl[1] * off[1]) + sl[2] * off[2]);
     }
 }
 
@@ -506,6 +568,49 @@ int ksgf_average_down_faces(const ksgf_boxarray* fine_ba, const double* fine_x, 
     (void)hipStreamSynchronize(st);
     ksgf_boxarray_destroy(cl);
     return ret;
+}
+
+int ksgf_fillpatch_two_levels(const ksgf_boxarray* fine_ba, double* fine, int ng, const ksgf_boxarray* crse_ba,
+                              const double* crse, int ng_c, int ncomp, int ratio, const ksgf_bcrec* bc,
+                              const double* bcval, ksgf_stream_t stream)
+{
+    if (!fine_ba || !fine || !crse_ba || !crse || !bc || !bcval || ncomp <= 0 || ncomp > 8 || ng < 0 || ng_c < 1 ||
+        (ratio != 2 && ratio != 4)) {
+        set_error("ksgf_fillpatch_two_levels: bad argument (coarse data needs one ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (fine_ba->dlo[d] != ratio * crse_ba->dlo[d] || fine_ba->dhi[d] + 1 != ratio * (crse_ba->dhi[d] + 1) ||
+            fine_ba->periodic[d] != crse_ba->periodic[d]) {
+            set_error("ksgf_fillpatch_two_levels: the fine domain is not the coarse domain refined by ratio");
+            return KSGF_ERR_INVALID;
+        }
+    if (ng == 0) return KSGF_OK;
+    hipStream_t st = (hipStream_t)stream;
+    MFDev f;
+    CMFDev c;
+    int e;
+    if ((e = mfdev(fine_ba, fine, KSGF_CELL, ncomp, ng, &f)) || (e = cmfdev(crse_ba, crse, KSGF_CELL, ncomp, ng_c, &c)))
+        return e;
+    const int* owner = nullptr;
+    if ((e = owner_map(crse_ba, st, &owner))) return e;
+    int err = KSGF_OK;
+    const Region* r = fab_region(fine_ba, KSGF_CELL, ng, &err);
+    if (!r) return err;
+    GeomFP g;
+    for (int d = 0; d < 3; ++d) {
+        g.fdlo[d] = fine_ba->dlo[d];
+        g.fdhi[d] = fine_ba->dhi[d];
+        g.per[d] = fine_ba->periodic[d];
+        g.cdlo[d] = crse_ba->dlo[d];
+    }
+    g.cn0 = crse_ba->dhi[0] - crse_ba->dlo[0] + 1;
+    g.cn1 = crse_ba->dhi[1] - crse_ba->dlo[1] + 1;
+    hipLaunchKernelGGL(k_fp_interp, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c, crse_ba->dbox, owner, ncomp,
+                       ratio, 0.5 - 0.5 / (double)ratio, g, regdev(r, fine_ba->nbox));
+    KSGF_LAUNCH_CHECK();
+    if ((e = ksgf_fill_boundary(fine_ba, fine, KSGF_CELL, ncomp, ng, stream))) return e;
+    return ksgf_fill_physbc(fine_ba, fine, KSGF_CELL, ncomp, ng, bc, bcval, stream);
 }
 
 }  // extern "C"

@@ -1669,6 +1669,101 @@ void ko_average_down_faces(const ko_geom* fg, const double* const* fine, int ng_
 }
 
 /* ------------------------------------------------------------------ */
+/* Two-level fill of a refined level's ghost cells (FillPatchTwoLevels */
+/* as Field::fillpatch does it on level > 0, FieldFillPatchOps.H:      */
+/* 272-336, without time interpolation): every ghost cell inside the   */
+/* domain first gets the coarse data interpolated, then the fine       */
+/* level's own data where a fine box holds it (FillBoundary), then the */
+/* physical BCs outside a non-periodic domain.                         */
+/* Interpolation: conservative linear (the default CellConsLinear,     */
+/* amrex::cell_cons_interp): MC-limited slopes of the parent coarse    */
+/* cell, scaled by one factor alpha <= 1 so that no child of the       */
+/* parent leaves [min, max] of the parent and its six face neighbours; */
+/* the children average to the parent for any alpha.  Parity unpinned: */
+/* AMReX's limiter details are in the absent AMReX source.             */
+/* ------------------------------------------------------------------ */
+static double mc_slope(double um, double uc, double up)
+{
+    const double dc = 0.5 * (up - um), dl = uc - um, dr = up - uc;
+    if (dl * dr > 0.0) return copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc);
+    return 0.0;
+}
+
+/* value of the fine cell with offsets off[d] (coarse cell units) from the
+ * parent whose value is uc and face neighbours um[d], up[d] */
+static double cons_lin_interp(double uc, const double* um, const double* up, const double* off, double omax)
+{
+    double sl[3], umax = uc, umin = uc, dev = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        sl[d] = mc_slope(um[d], uc, up[d]);
+        umax = fmax(umax, fmax(um[d], up[d]));
+        umin = fmin(umin, fmin(um[d], up[d]));
+        dev += fabs(sl[d]) * omax;
+    }
+    double alpha = 1.0;
+    if (dev > 0.0) alpha = fmin(1.0, fmin((umax - uc) / dev, (uc - umin) / dev));
+    return uc + alpha * ((sl[0] * off[0] + sl[1] * off[1]) + sl[2] * off[2]);
+}
+
+void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_geom* cg, const double* crse,
+                             int ng_c, int ncomp, int ratio, const ko_bcrec* bc, const double* bcval)
+{
+    fab_t* ff = (fab_t*)malloc(sizeof(fab_t) * (size_t)fg->nbox);
+    fab_t* fc = (fab_t*)malloc(sizeof(fab_t) * (size_t)cg->nbox);
+    make_fabs(fg, KO_CELL, ncomp, ng, ff);
+    make_fabs(cg, KO_CELL, ncomp, ng_c, fc);
+    int* cown = make_owner(cg);
+    const int cn0 = cg->dhi[0] - cg->dlo[0] + 1, cn1 = cg->dhi[1] - cg->dlo[1] + 1;
+    const double omax = 0.5 - 0.5 / (double)ratio;
+    for (int b = 0; b < fg->nbox; ++b) {
+        const int* bx = fg->boxes + 6 * b;
+        const fab_t* fb = &ff[b];
+        for (int k = fb->lo[2]; k < fb->lo[2] + fb->n[2]; ++k)
+            for (int j = fb->lo[1]; j < fb->lo[1] + fb->n[1]; ++j)
+                for (int i = fb->lo[0]; i < fb->lo[0] + fb->n[0]; ++i) {
+                    const int p[3] = {i, j, k};
+                    if (i >= bx[0] && i <= bx[3] && j >= bx[1] && j <= bx[4] && k >= bx[2] && k <= bx[5]) continue;
+                    int w[3], ok = 1;
+                    for (int d = 0; d < 3; ++d) {
+                        const int L = fg->dhi[d] - fg->dlo[d] + 1;
+                        int x = p[d];
+                        if (fg->periodic[d])
+                            x = fg->dlo[d] + (((x - fg->dlo[d]) % L) + L) % L;
+                        else if (x < fg->dlo[d] || x > fg->dhi[d])
+                            ok = 0;
+                        w[d] = x;
+                    }
+                    if (!ok) continue;
+                    int P[3];
+                    double off[3];
+                    for (int d = 0; d < 3; ++d) {
+                        P[d] = (w[d] >= 0) ? w[d] / ratio : -((-w[d] + ratio - 1) / ratio);  /* floor */
+                        off[d] = ((double)(w[d] - ratio * P[d]) + 0.5) / (double)ratio - 0.5;
+                    }
+                    const int cb = cown[(P[0] - cg->dlo[0]) + (long)cn0 * ((P[1] - cg->dlo[1]) + (long)cn1 * (P[2] - cg->dlo[2]))];
+                    if (cb < 0) continue;
+                    const fab_t* cf = &fc[cb];
+                    for (int n = 0; n < ncomp; ++n) {
+                        const double* cs = crse + n * FS(cf);
+                        const long c0 = FI(cf, P[0], P[1], P[2]);
+                        const long st[3] = {1, cf->n[0], (long)cf->n[0] * cf->n[1]};
+                        double um[3], up[3];
+                        for (int d = 0; d < 3; ++d) {
+                            um[d] = cs[c0 - st[d]];
+                            up[d] = cs[c0 + st[d]];
+                        }
+                        fine[FI(fb, i, j, k) + n * FS(fb)] = cons_lin_interp(cs[c0], um, up, off, omax);
+                    }
+                }
+    }
+    free(cown);
+    free(ff);
+    free(fc);
+    ko_fill_boundary(fg, fine, KO_CELL, ncomp, ng);
+    ko_fill_physbc(fg, fine, KO_CELL, ncomp, ng, bc, bcval);
+}
+
+/* ------------------------------------------------------------------ */
 /* CFL (a2; incflo_compute_dt.cpp:69-91) and predictor (a16;           */
 /* CompRHSOps.H:93-117)                                                */
 /* ------------------------------------------------------------------ */

@@ -148,6 +148,11 @@ void ko_average_down(const ko_geom* fg, const double* fine, int ng_f, const ko_g
 void ko_average_down_faces(const ko_geom* fg, const double* const* fine, int ng_f, const ko_geom* cg,
                            double* const* crse, int ng_c, int ncomp, int ratio);
 
+/* FillPatchTwoLevels of a refined level's cell MultiFab (ghosts): coarse data
+ * by conservative linear interpolation, then FillBoundary, then physical BCs */
+void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_geom* cg, const double* crse,
+                             int ng_c, int ncomp, int ratio, const ko_bcrec* bc, const double* bcval);
+
 double ko_cfl_max(const ko_geom* g, const double* vel, int ng);
 
 void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,

@@ -99,6 +99,8 @@ def lib():
                                       C.c_int]
         L.ko_average_down_faces.argtypes = [C.POINTER(KoGeom), C.POINTER(_dp), C.c_int, C.POINTER(KoGeom),
                                             C.POINTER(_dp), C.c_int, C.c_int, C.c_int]
+        L.ko_fillpatch_two_levels.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.POINTER(KoGeom), _dp, C.c_int,
+                                              C.c_int, C.c_int, C.POINTER(KoBC), _dp]
         L.ko_cfl_max.restype = C.c_double
         L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
         L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
@@ -321,6 +323,13 @@ def average_down_faces(Gf, fine3, ng_f, Gc, crse3, ng_c, ncomp, ratio):
     f = (_dp * 3)(*[_p(a) for a in fine3])
     c = (_dp * 3)(*[_p(a) for a in crse3])
     lib().ko_average_down_faces(Gf.ref, f, ng_f, Gc.ref, c, ng_c, ncomp, ratio)
+
+
+def fillpatch_two_levels(Gf, fine, ng, Gc, crse, ng_c, ncomp, ratio, bcrec, bcval):
+    """FillPatchTwoLevels of the fine level's ghost cells (in place)"""
+    bcval = np.ascontiguousarray(bcval, dtype=np.float64)
+    lib().ko_fillpatch_two_levels(Gf.ref, _p(fine), ng, Gc.ref, _p(crse), ng_c, ncomp, ratio, bc_structs(bcrec),
+                                  _p(bcval))
 
 
 def cfl_max(G, vel, ng):

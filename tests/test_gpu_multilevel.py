@@ -73,3 +73,30 @@ def test_average_down_faces_bitexact(ksgf, oracle, gpu, reach_sides):
     df.average_down_faces_to([df.upload(f) for f in fine], 0, dc, dcr, 1, 2, 2)
     for d in range(3):
         assert np.array_equal(to_np(dcr[d]), ref[d]), f"dir {d}"
+
+
+@pytest.mark.parametrize("reach_sides", [False, True])
+def test_fillpatch_two_levels_bitexact(ksgf, oracle, gpu, reach_sides):
+    """ghost cells of the fine level from the coarse level (conservative
+    linear interpolation), then the fine level's own data, then the physical
+    BCs (y walls: Dirichlet in x/z velocity, extrapolation in y)"""
+    _say(f"fillpatch_two_levels reach_sides={reach_sides}")
+    cba, fba = _levels(reach_sides)
+    Gc, Gf = oracle.Geom(cba), oracle.Geom(fba)
+    dc = ksgf.DeviceBoxArray(cba)
+    df = ksgf.DeviceBoxArray(fba, covers_domain=False)
+    rng = np.random.default_rng(7)
+    bc = np.array([[0, 3, 0, 0, 3, 0], [0, 2, 0, 0, 2, 0], [0, 3, 0, 0, 3, 0]], dtype=np.int32)
+    bcval = rng.standard_normal((6, 3))
+    nc, nf = cba.geom.n_cell, fba.geom.n_cell
+    crse = cba.scatter(rng.standard_normal((3, nc[2], nc[1], nc[0])), CELL, 1)
+    oracle.fill_ghosts(Gc, crse, CELL, 3, 1, bc, bcval)
+    for ng in (1, 3):
+        fine = fba.scatter(rng.standard_normal((3, nf[2], nf[1], nf[0])), CELL, ng)
+        ref = fine.copy()
+        oracle.fillpatch_two_levels(Gf, ref, ng, Gc, crse, 1, 3, 2, bc, bcval)
+        d = df.upload(fine)
+        df.fillpatch_two_levels(d, ng, dc, dc.upload(crse), 1, 3, bc, bcval)
+        got = to_np(d)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        assert np.array_equal(np.nan_to_num(got), np.nan_to_num(ref)), f"ng {ng}"

@@ -160,3 +160,104 @@ def test_average_down_faces_linear_and_shared_faces():
             ext = np.minimum(bhi, chi) - np.maximum(blo, clo) + 1
             expect += int(np.prod(np.maximum(ext, 0)))
         assert int(np.isfinite(crse[d]).sum()) == expect
+
+
+# --------------------------------------------------------------------------
+# FillPatchTwoLevels (Field::fillpatch on level > 0, FieldFillPatchOps.H:272-336):
+# conservative linear interpolation of the ghost cells from the coarse level
+# --------------------------------------------------------------------------
+def _one_box_levels(seed=11):
+    """ragged coarse level 16x12x16 and one fine box over coarse [4,7]^3, so
+    every ghost cell of the fine box (ng 2) lies inside the domain and its
+    parent's face neighbours lie in the coarse valid region"""
+    rng = np.random.default_rng(seed)
+    n = (16, 12, 16)
+    cg = lay.Geometry(n, prob_hi=(1.0, 0.75, 1.0), periodic=(1, 0, 1))
+    cba = lay.BoxArray(cg, lay.chop_random(n, [4, 8], rng, granularity=4))
+    fg = lay.Geometry(tuple(2 * v for v in n), prob_hi=cg.prob_hi, periodic=(1, 0, 1))
+    return cba, FineBoxArray(fg, [[8, 8, 8, 15, 15, 15]])
+
+
+_BC = np.array([[0, 3, 0, 0, 3, 0]], dtype=np.int32)
+
+
+def _fillpatch(cba, fba, crse_dom, fine_dom, ng=2):
+    """run the oracle's fill-patch; crse_dom/fine_dom: domain-wide (nz, ny, nx)"""
+    Gc, Gf = oracle.Geom(cba), oracle.Geom(fba)
+    crse = cba.scatter(crse_dom[None], CELL, 1)
+    oracle.fill_ghosts(Gc, crse, CELL, 1, 1, _BC, np.zeros((6, 1)))
+    fine = np.full(fba.mf_size(CELL, 1, ng), np.nan)
+    v = fba.fab_view(fine, 0, CELL, 1, ng)[0]
+    lo, hi = fba.boxes[0, :3], fba.boxes[0, 3:]
+    v[ng:-ng, ng:-ng, ng:-ng] = fine_dom[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+    oracle.fillpatch_two_levels(Gf, fine, ng, Gc, crse, 1, 1, 2, _BC, np.zeros((6, 1)))
+    return fba.fab_view(fine, 0, CELL, 1, ng)[0], lo - ng
+
+
+def test_fillpatch_linear_field_exact():
+    """MC slopes of a linear field are exact and the bounding factor stays 1,
+    so the ghosts take the linear field at the fine cell centres"""
+    cba, fba = _one_box_levels()
+    lin = lambda x, y, z: 0.3 + 2.0 * x - 1.5 * y + 0.75 * z  # noqa: E731
+
+    def dom(g):
+        n = g.n_cell
+        z, y, x = np.meshgrid(*[(np.arange(n[d]) + 0.5) * g.dx[d] for d in (2, 1, 0)], indexing="ij")
+        return lin(x, y, z)
+
+    v, lo = _fillpatch(cba, fba, dom(cba.geom), dom(fba.geom))
+    h = fba.geom.dx
+    z, y, x = np.meshgrid(*[(lo[d] + np.arange(v.shape[2 - d]) + 0.5) * h[d] for d in (2, 1, 0)], indexing="ij")
+    assert np.abs(v - lin(x, y, z)).max() <= 1e-14
+
+
+def test_fillpatch_conservative_and_bounded():
+    """random coarse data: the 8 children of every coarse cell whose children
+    are all ghosts average to the parent, and no child leaves the range of its
+    parent and the parent's 6 face neighbours"""
+    cba, fba = _one_box_levels(seed=12)
+    rng = np.random.default_rng(8)
+    cd = rng.standard_normal(tuple(cba.geom.n_cell[::-1]))
+    v, lo = _fillpatch(cba, fba, cd, rng.standard_normal(tuple(fba.geom.n_cell[::-1])))
+    nchecked = 0
+    for K in range(3, 9):
+        for J in range(3, 9):
+            for I in range(3, 9):
+                if 4 <= I <= 7 and 4 <= J <= 7 and 4 <= K <= 7:
+                    continue
+                ch = v[2 * K - lo[2]:2 * K - lo[2] + 2, 2 * J - lo[1]:2 * J - lo[1] + 2,
+                       2 * I - lo[0]:2 * I - lo[0] + 2]
+                nb = [cd[K, J, I], cd[K, J, I - 1], cd[K, J, I + 1], cd[K, J - 1, I], cd[K, J + 1, I],
+                      cd[K - 1, J, I], cd[K + 1, J, I]]
+                assert ch.min() >= min(nb) - 1e-15 and ch.max() <= max(nb) + 1e-15
+                assert abs(ch.mean() - cd[K, J, I]) <= 1e-15 * 8
+                nchecked += 1
+    assert nchecked == 6 ** 3 - 4 ** 3
+    assert not np.isnan(v).any()
+
+
+def test_fillpatch_known_answers():
+    """hand-worked parents on the fine box's low-x ghost slab (children
+    i = 6, 7 at offsets -1/4, +1/4).
+    A = coarse (3,5,5): x neighbours 0 | 1 | 4, y flat, z 1 | 1 | 0.9: MC slope
+    in x min(2, 2, 6) = 2 (minmod would give 1), none in y or z, bound factor
+    1: children 0.5 and 1.5.
+    B = coarse (3,5,7): neighbours 0.9 | 1 | 1.3 in every direction: slopes
+    0.2 each, deviation 3 * 0.2 / 4 = 0.15 > 1 - 0.9, so alpha = 2/3 and the
+    children are 1 + (+-1 +-1 +-1) / 30 (extremes exactly 0.9 and 1.1)."""
+    cba, fba = _one_box_levels(seed=13)
+    cd = np.ones(tuple(cba.geom.n_cell[::-1]))  # cd[K, J, I]
+    cd[5, 5, 2], cd[5, 5, 4] = 0.0, 4.0
+    cd[7, 5, 2], cd[7, 5, 4] = 0.9, 1.3
+    cd[7, 4, 3], cd[7, 6, 3] = 0.9, 1.3
+    cd[6, 5, 3], cd[8, 5, 3] = 0.9, 1.3
+    v, lo = _fillpatch(cba, fba, cd, np.zeros(tuple(fba.geom.n_cell[::-1])))
+
+    def kids(I, J, K):
+        return v[2 * K - lo[2]:2 * K - lo[2] + 2, 2 * J - lo[1]:2 * J - lo[1] + 2, 2 * I - lo[0]:2 * I - lo[0] + 2]
+
+    a = kids(3, 5, 5)
+    assert np.abs(a[:, :, 0] - 0.5).max() <= 1e-15 and np.abs(a[:, :, 1] - 1.5).max() <= 1e-15
+    s = np.array([-1.0, 1.0])
+    ref = 1.0 + (s[:, None, None] + s[None, :, None] + s[None, None, :]) / 30.0
+    assert np.abs(kids(3, 5, 7) - ref).max() <= 1e-14
