@@ -1764,6 +1764,527 @@ void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_g
 }
 
 /* ------------------------------------------------------------------ */
+/* Composite two-level MAC projection (MacProjector over levels 0..1, */
+/* icns_advection.cpp:88-95,331,353; AMReX MLMG's composite solve).   */
+/* Restated from the published method (the AMReX source is absent):   */
+/*  - fine ghost cells on the coarse-fine boundary: quadratic in the  */
+/*    normal direction through the coarse value b (tangentially       */
+/*    interpolated, located at the coarse cell centre, h_f from the   */
+/*    face) and the two fine cells inside: 8/15 b + 2/3 u0 - 1/5 u1;  */
+/*    b: quadratic tangential interpolation with the cross term,      */
+/*    one-sided where a coarse neighbour is covered or outside;       */
+/*  - uncovered coarse cells next to the fine level take the average  */
+/*    of the fine fluxes on the coarse-fine faces (reflux);            */
+/*  - iteration: fine damped-Jacobi smoothing, a coarse correction of  */
+/*    the composite residual (average of the fine residual under the   */
+/*    fine level) by one single-level V-cycle, piecewise-constant      */
+/*    prolongation, post-smoothing.                                    */
+/* All arrays are domain-wide, Fortran order: coarse n, fine 2n.      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int n[3], nf[3], per[3], lobc[3], hibc[3];
+    double hc[3], hf[3], beta;
+    unsigned char* fm;  /* fine cell covered by a fine box */
+    unsigned char* cv;  /* coarse cell covered */
+} tl_t;
+
+#define TLC(T, i, j, k) ((i) + (long)(T)->n[0] * ((j) + (long)(T)->n[1] * (k)))
+#define TLF(T, i, j, k) ((i) + (long)(T)->nf[0] * ((j) + (long)(T)->nf[1] * (k)))
+
+static void tl_init(tl_t* T, const int* n, const double* dxc, const int* lobc, const int* hibc, const int* fb,
+                    int nfb, double beta)
+{
+    for (int d = 0; d < 3; ++d) {
+        T->n[d] = n[d];
+        T->nf[d] = 2 * n[d];
+        T->per[d] = (lobc[d] == KO_LO_PERIODIC);
+        T->lobc[d] = lobc[d];
+        T->hibc[d] = hibc[d];
+        T->hc[d] = dxc[d];
+        T->hf[d] = 0.5 * dxc[d];
+    }
+    T->beta = beta;
+    const long nc = (long)n[0] * n[1] * n[2];
+    T->fm = (unsigned char*)calloc((size_t)(8 * nc), 1);
+    T->cv = (unsigned char*)calloc((size_t)nc, 1);
+    for (int b = 0; b < nfb; ++b) {
+        const int* x = fb + 6 * b;
+        for (int k = x[2]; k <= x[5]; ++k)
+            for (int j = x[1]; j <= x[4]; ++j)
+                for (int i = x[0]; i <= x[3]; ++i) {
+                    T->fm[TLF(T, i, j, k)] = 1;
+                    T->cv[TLC(T, i / 2, j / 2, k / 2)] = 1;
+                }
+    }
+}
+
+static void tl_free(tl_t* T)
+{
+    free(T->fm);
+    free(T->cv);
+}
+
+/* coarse value at Q (wrapped); 0 and *ok = 0 where covered or outside */
+static double tl_cval(const tl_t* T, const double* pc, const int* Q, int* ok)
+{
+    int q[3];
+    for (int d = 0; d < 3; ++d) {
+        q[d] = Q[d];
+        if (q[d] < 0 || q[d] >= T->n[d]) {
+            if (!T->per[d]) { *ok = 0; return 0.0; }
+            q[d] = ((q[d] % T->n[d]) + T->n[d]) % T->n[d];
+        }
+    }
+    const long c = TLC(T, q[0], q[1], q[2]);
+    if (T->cv[c]) { *ok = 0; return 0.0; }
+    *ok = 1;
+    return pc[c];
+}
+
+/* tangential interpolation of the coarse data about Q at offsets x (in
+ * coarse cells) in the two directions other than dn */
+static double tl_cinterp(const tl_t* T, const double* pc, const int* Q, int dn, const double* x)
+{
+    int ok;
+    const double uc = tl_cval(T, pc, Q, &ok);
+    const int t1 = (dn == 0) ? 1 : 0, t2 = (dn == 2) ? 1 : 2;
+    const int tt[2] = {t1, t2};
+    double term[2];
+    for (int a = 0; a < 2; ++a) {
+        const int t = tt[a];
+        int P[3] = {Q[0], Q[1], Q[2]}, M[3] = {Q[0], Q[1], Q[2]}, okp, okm;
+        P[t] += 1;
+        M[t] -= 1;
+        const double up = tl_cval(T, pc, P, &okp), um = tl_cval(T, pc, M, &okm);
+        double d1 = 0.0, d2 = 0.0;
+        if (okp && okm) {
+            d1 = 0.5 * (up - um);
+            d2 = 0.5 * ((up - 2.0 * uc) + um);
+        } else if (okp) {
+            d1 = up - uc;
+        } else if (okm) {
+            d1 = uc - um;
+        }
+        term[a] = x[t] * (d1 + x[t] * d2);
+    }
+    double dxy = 0.0;
+    {
+        int C[4][3], okc[4];
+        double v[4];
+        for (int m = 0; m < 4; ++m) {
+            C[m][0] = Q[0];
+            C[m][1] = Q[1];
+            C[m][2] = Q[2];
+            C[m][t1] += (m & 1) ? 1 : -1;
+            C[m][t2] += (m & 2) ? 1 : -1;
+            v[m] = tl_cval(T, pc, C[m], &okc[m]);
+        }
+        /* v[0] = (-,-), v[1] = (+,-), v[2] = (-,+), v[3] = (+,+) */
+        if (okc[0] && okc[1] && okc[2] && okc[3]) dxy = 0.25 * ((v[3] - v[2]) - (v[1] - v[0]));
+    }
+    return ((uc + term[0]) + term[1]) + x[t1] * x[t2] * dxy;
+}
+
+/* value of fine cell i's neighbour in direction d, side s (+1/-1): a fine
+ * cell, a physical-boundary ghost, or a coarse-fine ghost */
+static double tl_fnb(const tl_t* T, const double* pc, const double* pf, const int* i, int d, int s)
+{
+    int q[3] = {i[0], i[1], i[2]};
+    q[d] += s;
+    const double u0 = pf[TLF(T, i[0], i[1], i[2])];
+    if (q[d] < 0 || q[d] >= T->nf[d]) {
+        if (!T->per[d]) {
+            const int bc = (s < 0) ? T->lobc[d] : T->hibc[d];
+            return (bc == KO_LO_DIRICHLET) ? -u0 : u0;
+        }
+        q[d] = ((q[d] % T->nf[d]) + T->nf[d]) % T->nf[d];
+    }
+    const long fq = TLF(T, q[0], q[1], q[2]);
+    if (T->fm[fq]) return pf[fq];
+    int Q[3];
+    double x[3];
+    for (int e = 0; e < 3; ++e) {
+        Q[e] = q[e] / 2;
+        x[e] = ((double)(q[e] - 2 * Q[e]) + 0.5) * 0.5 - 0.5;
+    }
+    x[d] = 0.0;
+    const double b = tl_cinterp(T, pc, Q, d, x);
+    int i1[3] = {i[0], i[1], i[2]};
+    i1[d] -= s;
+    const double u1 = pf[TLF(T, i1[0], i1[1], i1[2])];
+    return ((8.0 / 15.0) * b + (2.0 / 3.0) * u0) - 0.2 * u1;
+}
+
+/* fine flux beta dphi/dx_d on the face of fine cell i on side s */
+static double tl_fflux(const tl_t* T, const double* pc, const double* pf, const int* i, int d, int s)
+{
+    const double u0 = pf[TLF(T, i[0], i[1], i[2])], g = tl_fnb(T, pc, pf, i, d, s);
+    return (s > 0) ? T->beta * (g - u0) / T->hf[d] : T->beta * (u0 - g) / T->hf[d];
+}
+
+/* composite operator div(beta grad phi): Lf on fine cells, Lc on uncovered
+ * coarse cells (0 elsewhere) */
+static void tl_apply(const tl_t* T, const double* pc, const double* pf, double* Lc, double* Lf)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < T->nf[2]; ++k)
+        for (int j = 0; j < T->nf[1]; ++j)
+            for (int i = 0; i < T->nf[0]; ++i) {
+                const long f = TLF(T, i, j, k);
+                if (!T->fm[f]) { Lf[f] = 0.0; continue; }
+                const int c[3] = {i, j, k};
+                double acc = 0.0;
+                for (int d = 0; d < 3; ++d)
+                    acc += (tl_fflux(T, pc, pf, c, d, 1) - tl_fflux(T, pc, pf, c, d, -1)) / T->hf[d];
+                Lf[f] = acc;
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < T->n[2]; ++k)
+        for (int j = 0; j < T->n[1]; ++j)
+            for (int i = 0; i < T->n[0]; ++i) {
+                const long c = TLC(T, i, j, k);
+                if (T->cv[c]) { Lc[c] = 0.0; continue; }
+                const int C[3] = {i, j, k};
+                const double uc = pc[c];
+                double acc = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    double F[2];
+                    for (int a = 0; a < 2; ++a) {
+                        const int s = a ? 1 : -1;
+                        int N[3] = {C[0], C[1], C[2]};
+                        N[d] += s;
+                        if (N[d] < 0 || N[d] >= T->n[d]) {
+                            if (!T->per[d]) {
+                                const int bc = (s < 0) ? T->lobc[d] : T->hibc[d];
+                                F[a] = (bc == KO_LO_DIRICHLET) ? ((s > 0) ? T->beta * (-uc - uc) / T->hc[d]
+                                                                          : T->beta * (uc + uc) / T->hc[d])
+                                                               : 0.0;
+                                continue;
+                            }
+                            N[d] = ((N[d] % T->n[d]) + T->n[d]) % T->n[d];
+                        }
+                        const long nc = TLC(T, N[0], N[1], N[2]);
+                        if (T->cv[nc]) {
+                            /* the 4 fine cells of N on this face, their fluxes toward C */
+                            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                            double ff[4];
+                            for (int m = 0; m < 4; ++m) {
+                                int fi[3];
+                                fi[d] = (s > 0) ? 2 * N[d] : 2 * N[d] + 1;
+                                fi[t1] = 2 * N[t1] + (m & 1);
+                                fi[t2] = 2 * N[t2] + ((m >> 1) & 1);
+                                ff[m] = tl_fflux(T, pc, pf, fi, d, -s);
+                            }
+                            F[a] = 0.25 * ((ff[0] + ff[1]) + (ff[2] + ff[3]));
+                        } else {
+                            F[a] = (s > 0) ? T->beta * (pc[nc] - uc) / T->hc[d] : T->beta * (uc - pc[nc]) / T->hc[d];
+                        }
+                    }
+                    acc += (F[1] - F[0]) / T->hc[d];
+                }
+                Lc[c] = acc;
+            }
+}
+
+void ko_tl_apply(const int* n, const double* dxc, const int* lobc, const int* hibc, const int* fboxes, int nfbox,
+                 double beta, const double* phic, const double* phif, double* Lc, double* Lf)
+{
+    tl_t T;
+    tl_init(&T, n, dxc, lobc, hibc, fboxes, nfbox, beta);
+    tl_apply(&T, phic, phif, Lc, Lf);
+    tl_free(&T);
+}
+
+/* max |R| over fine cells and uncovered coarse cells */
+static double tl_norm(const tl_t* T, const double* Rc, const double* Rf)
+{
+    const long nc = (long)T->n[0] * T->n[1] * T->n[2];
+    double m = 0.0;
+    for (long c = 0; c < nc; ++c)
+        if (!T->cv[c]) m = fmax(m, fabs(Rc[c]));
+    for (long f = 0; f < 8 * nc; ++f)
+        if (T->fm[f]) m = fmax(m, fabs(Rf[f]));
+    return m;
+}
+
+static void tl_residual(const tl_t* T, const double* pc, const double* pf, const double* bc, const double* bf,
+                        double* Rc, double* Rf)
+{
+    const long nc = (long)T->n[0] * T->n[1] * T->n[2];
+    tl_apply(T, pc, pf, Rc, Rf);
+    for (long c = 0; c < nc; ++c) Rc[c] = T->cv[c] ? 0.0 : bc[c] - Rc[c];
+    for (long f = 0; f < 8 * nc; ++f) Rf[f] = T->fm[f] ? bf[f] - Rf[f] : 0.0;
+}
+
+/* the composite (volume-weighted) mean over fine cells and uncovered coarse cells */
+static double tl_mean(const tl_t* T, const double* xc, const double* xf)
+{
+    const long nc = (long)T->n[0] * T->n[1] * T->n[2];
+    double sc = 0.0, sf = 0.0;
+    long ncu = 0, nff = 0;
+    for (long c = 0; c < nc; ++c)
+        if (!T->cv[c]) { sc += xc[c]; ++ncu; }
+    for (long f = 0; f < 8 * nc; ++f)
+        if (T->fm[f]) { sf += xf[f]; ++nff; }
+    return (8.0 * sc + sf) / (8.0 * (double)ncu + (double)nff);
+}
+
+/* damped Jacobi on the fine cells (ghosts from the current iterate) */
+static void tl_smooth(const tl_t* T, const double* pc, double* pf, const double* bc, const double* bf, double* Rc,
+                      double* Rf, int nsweep)
+{
+    const long nf = 8L * T->n[0] * T->n[1] * T->n[2];
+    const double diag = -2.0 * T->beta * ((1.0 / (T->hf[0] * T->hf[0]) + 1.0 / (T->hf[1] * T->hf[1])) +
+                                          1.0 / (T->hf[2] * T->hf[2]));
+    const double w = KO_TL_OMEGA / diag;
+    for (int s = 0; s < nsweep; ++s) {
+        tl_residual(T, pc, pf, bc, bf, Rc, Rf);
+        for (long f = 0; f < nf; ++f)
+            if (T->fm[f]) pf[f] += w * Rf[f];
+    }
+}
+
+/* coarse faces with a covered cell on either side (periodic wrap) take the
+ * average of the 4 fine faces of the covered side's fine cells */
+static void tl_avg_faces(const tl_t* T, double* const* umc, double* const* umf)
+{
+    const int* n = T->n;
+    for (int d = 0; d < 3; ++d) {
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        int sc[3] = {n[0], n[1], n[2]}, sf[3] = {T->nf[0], T->nf[1], T->nf[2]};
+        sc[d] += 1;
+        sf[d] += 1;
+        for (int k = 0; k < sc[2]; ++k)
+            for (int j = 0; j < sc[1]; ++j)
+                for (int i = 0; i < sc[0]; ++i) {
+                    const int F[3] = {i, j, k};
+                    int fd = -1;
+                    for (int a = 1; a >= 0 && fd < 0; --a) {
+                        int C[3] = {i, j, k};
+                        if (!a) C[d] -= 1;
+                        if (C[d] < 0 || C[d] >= n[d]) {
+                            if (!T->per[d]) continue;
+                            C[d] = ((C[d] % n[d]) + n[d]) % n[d];
+                        }
+                        if (T->cv[TLC(T, C[0], C[1], C[2])]) fd = a ? 2 * C[d] : 2 * C[d] + 2;
+                    }
+                    if (fd < 0) continue;
+                    double v[4];
+                    for (int m = 0; m < 4; ++m) {
+                        int fi[3];
+                        fi[d] = fd;
+                        fi[t1] = 2 * F[t1] + (m & 1);
+                        fi[t2] = 2 * F[t2] + ((m >> 1) & 1);
+                        v[m] = umf[d][fi[0] + (long)sf[0] * (fi[1] + (long)sf[1] * fi[2])];
+                    }
+                    umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] = 0.25 * ((v[0] + v[1]) + (v[2] + v[3]));
+                }
+    }
+}
+
+int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, const int* hibc, const int* fboxes,
+                              int nfbox, double beta, double* const* umc, double* const* umf, double rtol,
+                              double atol, const ko_mlmg_opts* opts, ko_mlmg_info* info, double* phic_out,
+                              double* phif_out)
+{
+    tl_t T;
+    tl_init(&T, n, dxc, lobc, hibc, fboxes, nfbox, beta);
+    const long nc = (long)n[0] * n[1] * n[2], nf = 8 * nc;
+    int singular = 1;
+    for (int d = 0; d < 3; ++d)
+        if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) singular = 0;
+    tl_avg_faces(&T, umc, umf);
+    /* rhs = div u on both levels */
+    double* bc = (double*)calloc((size_t)nc, sizeof(double));
+    double* bf = (double*)calloc((size_t)nf, sizeof(double));
+    for (int lev = 0; lev < 2; ++lev) {
+        const int* m = lev ? T.nf : T.n;
+        const double* h = lev ? T.hf : T.hc;
+        double* const* u = lev ? umf : umc;
+        double* b = lev ? bf : bc;
+        for (int k = 0; k < m[2]; ++k)
+            for (int j = 0; j < m[1]; ++j)
+                for (int i = 0; i < m[0]; ++i) {
+                    const long c = i + (long)m[0] * (j + (long)m[1] * k);
+                    if (lev ? !T.fm[c] : T.cv[c]) continue;
+                    const double ux = (u[0][(i + 1) + (long)(m[0] + 1) * (j + (long)m[1] * k)] -
+                                       u[0][i + (long)(m[0] + 1) * (j + (long)m[1] * k)]) / h[0];
+                    const double uy = (u[1][i + (long)m[0] * ((j + 1) + (long)(m[1] + 1) * k)] -
+                                       u[1][i + (long)m[0] * (j + (long)(m[1] + 1) * k)]) / h[1];
+                    const double uz = (u[2][i + (long)m[0] * (j + (long)m[1] * (k + 1))] -
+                                       u[2][i + (long)m[0] * (j + (long)m[1] * k)]) / h[2];
+                    b[c] = (ux + uy) + uz;
+                }
+    }
+    if (singular) {
+        const double mu = tl_mean(&T, bc, bf);
+        for (long c = 0; c < nc; ++c)
+            if (!T.cv[c]) bc[c] -= mu;
+        for (long f = 0; f < nf; ++f)
+            if (T.fm[f]) bf[f] -= mu;
+    }
+    double* pc = (double*)calloc((size_t)nc, sizeof(double));
+    double* pf = (double*)calloc((size_t)nf, sizeof(double));
+    double* Rc = (double*)calloc((size_t)nc, sizeof(double));
+    double* Rf = (double*)calloc((size_t)nf, sizeof(double));
+    double* ec = (double*)calloc((size_t)nc, sizeof(double));
+    tl_residual(&T, pc, pf, bc, bf, Rc, Rf);
+    const double r0 = tl_norm(&T, Rc, Rf);
+    const double target = fmax(atol, rtol * r0);
+    double r = r0;
+    ko_mlmg_opts co = *opts;
+    co.maxiter = 1;
+    ko_mlmg_info ci;
+    memset(info, 0, sizeof(*info));
+    info->r0 = r0;
+    info->converged = (r <= target);
+    int ret = 0;
+    for (int it = 1; !info->converged && it <= opts->maxiter; ++it) {
+        tl_smooth(&T, pc, pf, bc, bf, Rc, Rf, opts->nu1);
+        tl_residual(&T, pc, pf, bc, bf, Rc, Rf);
+        /* coarse rhs: composite residual, the fine residual averaged under
+         * the fine level; one V-cycle of the coarse operator */
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i) {
+                    const long c = TLC(&T, i, j, k);
+                    if (!T.cv[c]) { Rc[c] = -Rc[c]; continue; }
+                    double s8 = 0.0;
+                    for (int kk = 0; kk < 2; ++kk)
+                        for (int jj = 0; jj < 2; ++jj)
+                            for (int ii = 0; ii < 2; ++ii) s8 += Rf[TLF(&T, 2 * i + ii, 2 * j + jj, 2 * k + kk)];
+                    Rc[c] = -(0.125 * s8);
+                }
+        /* -beta lap e = -R */
+        const int cr = ko_abeclap_solve(n, dxc, lobc, hibc, 0.0, beta, Rc, ec, 0.0, 0.0, &co, &ci);
+        info->bottom_iters += ci.bottom_iters;
+        if (cr < 0) { ret = cr; break; }
+        for (long c = 0; c < nc; ++c) pc[c] += ec[c];
+        for (int k = 0; k < T.nf[2]; ++k)
+            for (int j = 0; j < T.nf[1]; ++j)
+                for (int i = 0; i < T.nf[0]; ++i) {
+                    const long f = TLF(&T, i, j, k);
+                    if (T.fm[f]) pf[f] += ec[TLC(&T, i / 2, j / 2, k / 2)];
+                }
+        tl_smooth(&T, pc, pf, bc, bf, Rc, Rf, opts->nu2);
+        tl_residual(&T, pc, pf, bc, bf, Rc, Rf);
+        r = tl_norm(&T, Rc, Rf);
+        info->iters = it;
+        if (r <= target) info->converged = 1;
+        else if (!(r <= KO_MG_DIVERGE * r0)) { ret = KO_MG_DIVERGED; break; }
+    }
+    info->rfinal = r;
+    info->nlevels = 2;
+    if (ret == 0 && !info->converged) ret = 1;
+    /* covered coarse cells: the average of their children; singular: the
+     * composite mean removed */
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const long c = TLC(&T, i, j, k);
+                if (!T.cv[c]) continue;
+                double s8 = 0.0;
+                for (int kk = 0; kk < 2; ++kk)
+                    for (int jj = 0; jj < 2; ++jj)
+                        for (int ii = 0; ii < 2; ++ii) s8 += pf[TLF(&T, 2 * i + ii, 2 * j + jj, 2 * k + kk)];
+                pc[c] = 0.125 * s8;
+            }
+    if (singular) {
+        const double mu = tl_mean(&T, pc, pf);
+        for (long c = 0; c < nc; ++c) pc[c] -= mu;
+        for (long f = 0; f < nf; ++f)
+            if (T.fm[f]) pf[f] -= mu;
+    }
+    /* u -= beta grad phi: fine faces of fine cells (ghosts as in the
+     * operator), coarse faces; then the fine average onto the coarse faces */
+    for (int d = 0; d < 3; ++d) {
+        int sf[3] = {T.nf[0], T.nf[1], T.nf[2]};
+        sf[d] += 1;
+        for (int k = 0; k < T.nf[2]; ++k)
+            for (int j = 0; j < T.nf[1]; ++j)
+                for (int i = 0; i < T.nf[0]; ++i) {
+                    if (!T.fm[TLF(&T, i, j, k)]) continue;
+                    const int c[3] = {i, j, k};
+                    for (int a = 0; a < 2; ++a) {
+                        const int s = a ? 1 : -1;
+                        int q[3] = {i, j, k};
+                        if (s > 0) q[d] += 1;
+                        /* a face shared by two fine cells is corrected once, from its low cell */
+                        if (s < 0) {
+                            int lo[3] = {i, j, k};
+                            lo[d] -= 1;
+                            /* the low face at index 0 of a periodic direction
+                             * is its own entry (the image of index nf) */
+                            if (lo[d] >= 0 && T.fm[TLF(&T, lo[0], lo[1], lo[2])]) continue;
+                        }
+                        const double F = tl_fflux(&T, pc, pf, c, d, s);
+                        umf[d][q[0] + (long)sf[0] * (q[1] + (long)sf[1] * q[2])] -= F;
+                    }
+                }
+        int sc[3] = {n[0], n[1], n[2]};
+        sc[d] += 1;
+        for (int k = 0; k < sc[2]; ++k)
+            for (int j = 0; j < sc[1]; ++j)
+                for (int i = 0; i < sc[0]; ++i) {
+                    int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+                    lo[d] -= 1;
+                    double F;
+                    if (lo[d] < 0 || hi[d] >= n[d]) {
+                        if (!T.per[d]) {
+                            const long in = (lo[d] < 0) ? TLC(&T, hi[0], hi[1], hi[2]) : TLC(&T, lo[0], lo[1], lo[2]);
+                            const int bcd = (lo[d] < 0) ? lobc[d] : hibc[d];
+                            const double u0 = pc[in];
+                            F = (bcd == KO_LO_DIRICHLET) ? ((lo[d] < 0) ? beta * (u0 + u0) / T.hc[d]
+                                                                        : beta * (-u0 - u0) / T.hc[d])
+                                                         : 0.0;
+                            umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] -= F;
+                            continue;
+                        }
+                        if (lo[d] < 0) lo[d] += n[d];
+                        if (hi[d] >= n[d]) hi[d] -= n[d];
+                    }
+                    F = beta * (pc[TLC(&T, hi[0], hi[1], hi[2])] - pc[TLC(&T, lo[0], lo[1], lo[2])]) / T.hc[d];
+                    umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] -= F;
+                }
+    }
+    /* the image faces of a periodic direction hold one value wherever a fine
+     * cell touches either entry */
+    for (int d = 0; d < 3; ++d) {
+        if (!T.per[d]) continue;
+        int sf[3] = {T.nf[0], T.nf[1], T.nf[2]};
+        sf[d] += 1;
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        for (int b = 0; b < T.nf[t2]; ++b)
+            for (int a = 0; a < T.nf[t1]; ++a) {
+                int c0[3], c1[3], f0[3], f1[3];
+                c0[t1] = c1[t1] = f0[t1] = f1[t1] = a;
+                c0[t2] = c1[t2] = f0[t2] = f1[t2] = b;
+                c0[d] = 0;
+                c1[d] = T.nf[d] - 1;
+                f0[d] = 0;
+                f1[d] = T.nf[d];
+                const long i0 = f0[0] + (long)sf[0] * (f0[1] + (long)sf[1] * f0[2]);
+                const long i1 = f1[0] + (long)sf[0] * (f1[1] + (long)sf[1] * f1[2]);
+                if (T.fm[TLF(&T, c0[0], c0[1], c0[2])]) umf[d][i1] = umf[d][i0];
+                else if (T.fm[TLF(&T, c1[0], c1[1], c1[2])]) umf[d][i0] = umf[d][i1];
+            }
+    }
+    tl_avg_faces(&T, umc, umf);
+    if (phic_out) memcpy(phic_out, pc, sizeof(double) * (size_t)nc);
+    if (phif_out)
+        for (long f = 0; f < nf; ++f) phif_out[f] = T.fm[f] ? pf[f] : 0.0;
+    free(bc);
+    free(bf);
+    free(pc);
+    free(pf);
+    free(Rc);
+    free(Rf);
+    free(ec);
+    tl_free(&T);
+    return ret;
+}
+
+/* ------------------------------------------------------------------ */
 /* CFL (a2; incflo_compute_dt.cpp:69-91) and predictor (a16;           */
 /* CompRHSOps.H:93-117)                                                */
 /* ------------------------------------------------------------------ */

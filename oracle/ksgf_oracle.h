@@ -153,6 +153,19 @@ void ko_average_down_faces(const ko_geom* fg, const double* const* fine, int ng_
 void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_geom* cg, const double* crse,
                              int ng_c, int ncomp, int ratio, const ko_bcrec* bc, const double* bcval);
 
+/* Composite two-level MAC projection (MacProjector over levels 0 and 1 at
+ * ratio 2).  Domain-wide arrays: coarse cells n, fine cells 2n (valid where a
+ * fine box covers), faces with one more entry along their direction.  fboxes:
+ * fine boxes in fine indices, aligned to 2.  umc/umf: x, y, z face arrays of
+ * each level, corrected in place; phic/phif (nullable): the solution. */
+#define KO_TL_OMEGA (6.0 / 7.0)
+void ko_tl_apply(const int* n, const double* dxc, const int* lobc, const int* hibc, const int* fboxes, int nfbox,
+                 double beta, const double* phic, const double* phif, double* Lc, double* Lf);
+int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, const int* hibc, const int* fboxes,
+                              int nfbox, double beta, double* const* umc, double* const* umf, double rtol,
+                              double atol, const ko_mlmg_opts* opts, ko_mlmg_info* info, double* phic_out,
+                              double* phif_out);
+
 double ko_cfl_max(const ko_geom* g, const double* vel, int ng);
 
 void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,

@@ -101,6 +101,10 @@ def lib():
                                             C.POINTER(_dp), C.c_int, C.c_int, C.c_int]
         L.ko_fillpatch_two_levels.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.POINTER(KoGeom), _dp, C.c_int,
                                               C.c_int, C.c_int, C.POINTER(KoBC), _dp]
+        L.ko_tl_apply.argtypes = [_ip, _dp, _ip, _ip, _ip, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        L.ko_mac_project_two_levels.argtypes = [_ip, _dp, _ip, _ip, _ip, C.c_int, C.c_double, C.POINTER(_dp),
+                                                C.POINTER(_dp), C.c_double, C.c_double, C.POINTER(KoOpts),
+                                                C.POINTER(KoInfo), _dp, _dp]
         L.ko_cfl_max.restype = C.c_double
         L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
         L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
@@ -330,6 +334,46 @@ def fillpatch_two_levels(Gf, fine, ng, Gc, crse, ng_c, ncomp, ratio, bcrec, bcva
     bcval = np.ascontiguousarray(bcval, dtype=np.float64)
     lib().ko_fillpatch_two_levels(Gf.ref, _p(fine), ng, Gc.ref, _p(crse), ng_c, ncomp, ratio, bc_structs(bcrec),
                                   _p(bcval))
+
+
+def _i3(v):
+    return (C.c_int * 3)(*[int(x) for x in v])
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def _fboxes(fboxes):
+    fb = np.ascontiguousarray(np.asarray(fboxes, dtype=np.int32).reshape(-1, 6))
+    return fb, fb.ctypes.data_as(_ip)
+
+
+def tl_apply(n, dxc, lobc, hibc, fboxes, beta, phic, phif):
+    """composite two-level operator div(beta grad phi): (Lc, Lf) domain-wide"""
+    fb, fp = _fboxes(fboxes)
+    phic = np.ascontiguousarray(phic, dtype=np.float64)
+    phif = np.ascontiguousarray(phif, dtype=np.float64)
+    Lc, Lf = np.zeros_like(phic), np.zeros_like(phif)
+    lib().ko_tl_apply(_i3(n), _d3(dxc), _i3(lobc), _i3(hibc), fp, fb.shape[0], beta, _p(phic), _p(phif), _p(Lc),
+                      _p(Lf))
+    return Lc, Lf
+
+
+def mac_project_two_levels(n, dxc, lobc, hibc, fboxes, beta, umc, umf, rtol=1e-11, atol=0.0, opts=None):
+    """composite two-level MAC projection; umc/umf: three domain-wide face
+    arrays per level, corrected in place.  Returns (info, phic, phif)."""
+    fb, fp = _fboxes(fboxes)
+    for a in list(umc) + list(umf):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    o = opts if opts is not None else default_opts()
+    info = KoInfo()
+    nc = int(np.prod(n))
+    phic, phif = np.zeros(nc), np.zeros(8 * nc)
+    ret = lib().ko_mac_project_two_levels(_i3(n), _d3(dxc), _i3(lobc), _i3(hibc), fp, fb.shape[0], beta,
+                                          (_dp * 3)(*[_p(a) for a in umc]), (_dp * 3)(*[_p(a) for a in umf]), rtol,
+                                          atol, C.byref(o), C.byref(info), _p(phic), _p(phif))
+    return dict(info.as_dict(), status=ret), phic, phif
 
 
 def cfl_max(G, vel, ng):
