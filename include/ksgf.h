@@ -204,6 +204,34 @@ int ksgf_macproj_destroy(ksgf_macproj* mp);
 int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol,
                        double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
 
+/* Hydro::MacProjector over two AMR levels (icns_advection.cpp:88-95 builds
+ * it on Geom(0..finest_level), :331 setUMAC(mac_vec) with every level's face
+ * velocities, :353 project(rtol, atol)): the composite solve of
+ * div(beta grad phi) = div(u*) on the coarse level outside the fine level
+ * and on the fine level (ratio 2), then u -= beta grad phi on both levels and
+ * the fine average onto the coarse faces under and around the fine level.
+ * crse_ba covers the domain (origin 0); fine_ba: a level created with
+ * ksgf_boxarray_create_level, boxes aligned to 2.  Coarse-fine ghosts are
+ * quadratic through the tangentially interpolated coarse value; uncovered
+ * coarse cells next to the fine level use the averaged fine fluxes (reflux).
+ * opts: maxiter (composite iterations), num_pre/post_smooth (fine Jacobi
+ * sweeps); each iteration takes one FP64 V-cycle on the coarse grid.
+ * project: umac_c / umac_f: the x, y, z face MultiFabs of each level, ngrow
+ * ng, corrected in place; returns KSGF_ERR_NOT_CONVERGED (after correcting)
+ * if maxiter is reached.  get_phi: the coarse solution domain-wide (n cells,
+ * covered cells hold the average of their children) and the fine one as a
+ * cell MultiFab with one ghost layer (ng_f must be 1). */
+typedef struct ksgf_macproj_levels ksgf_macproj_levels;
+int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray* fine_ba,
+                               const int* lobc, const int* hibc, double beta,
+                               const ksgf_mlmg_options* opts, ksgf_macproj_levels** out);
+int ksgf_macproj_levels_project(ksgf_macproj_levels* mp, double* const* umac_c,
+                                double* const* umac_f, int ng, double rtol, double atol,
+                                ksgf_mlmg_info* info, ksgf_stream_t stream);
+int ksgf_macproj_levels_get_phi(ksgf_macproj_levels* mp, double* phi_c, double* phi_f, int ng_f,
+                                ksgf_stream_t stream);
+int ksgf_macproj_levels_destroy(ksgf_macproj_levels* mp);
+
 /* Many independent Poisson problems solved together.  The reference solves
  * each with its own MLMG (MLMG::solve behind MacProjector::project,
  * icns_advection.cpp:331, one after another); here every multigrid stage of

@@ -65,7 +65,8 @@ EXPORTS = [
     "ksgf_nodalproj_destroy", "ksgf_set_bndry", "ksgf_incflo_add_gradp", "ksgf_abeclap_create",
     "ksgf_abeclap_set_scalars", "ksgf_abeclap_solve", "ksgf_abeclap_apply", "ksgf_abeclap_destroy",
     "ksgf_icns_src_term_divtau", "ksgf_boxarray_create_level", "ksgf_average_down", "ksgf_average_down_faces",
-    "ksgf_fillpatch_two_levels",
+    "ksgf_fillpatch_two_levels", "ksgf_macproj_levels_create", "ksgf_macproj_levels_project",
+    "ksgf_macproj_levels_get_phi", "ksgf_macproj_levels_destroy",
 ]
 
 _lib = None
@@ -127,6 +128,12 @@ def load(path=None):
     L.ksgf_adjoint_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_void_p]
     L.ksgf_adjoint_destroy.argtypes = [C.c_void_p]
     L.ksgf_macproj_destroy.argtypes = [C.c_void_p]
+    L.ksgf_macproj_levels_create.argtypes = [C.c_void_p, C.c_void_p, _ip, _ip, C.c_double,
+                                             C.POINTER(KsgfMLMGOptions), C.POINTER(C.c_void_p)]
+    L.ksgf_macproj_levels_project.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int,
+                                              C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo), C.c_void_p]
+    L.ksgf_macproj_levels_get_phi.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_void_p]
+    L.ksgf_macproj_levels_destroy.argtypes = [C.c_void_p]
     L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
                                         _dp, C.c_double, C.c_double, C.c_int, C.c_void_p]
@@ -452,6 +459,56 @@ class MacProjector:
     def close(self):
         if self.h:
             load().ksgf_macproj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MacProjectorLevels:
+    """Hydro::MacProjector over two AMR levels (coarse DeviceBoxArray
+    covering the domain, fine DeviceBoxArray at ratio 2): composite solve."""
+
+    def __init__(self, crse: DeviceBoxArray, fine: DeviceBoxArray, lobc, hibc, beta=1.0, options=None):
+        import numpy as np
+
+        L = load()
+        self.crse, self.fine = crse, fine
+        self._lo = np.ascontiguousarray(lobc, dtype=np.int32)
+        self._hi = np.ascontiguousarray(hibc, dtype=np.int32)
+        self.options = options or default_options()
+        h = C.c_void_p()
+        check(L.ksgf_macproj_levels_create(crse.h, fine.h, self._lo.ctypes.data_as(_ip),
+                                           self._hi.ctypes.data_as(_ip), beta, C.byref(self.options), C.byref(h)),
+              "ksgf_macproj_levels_create")
+        self.h = h
+        self.info = None
+
+    def project(self, mac_c, mac_f, ng, rtol, atol):
+        info = KsgfMLMGInfo()
+        pc = (C.c_void_p * 3)(*[_ptr(a) for a in mac_c])
+        pf = (C.c_void_p * 3)(*[_ptr(a) for a in mac_f])
+        check(load().ksgf_macproj_levels_project(self.h, pc, pf, ng, rtol, atol, C.byref(info), _stream()),
+              "macproj_levels_project", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def phi(self):
+        """(coarse domain-wide, fine cell MultiFab with one ghost layer)"""
+        import torch
+
+        nc = int(torch.tensor(self.crse.ba.geom.n_cell).prod())
+        pc = torch.zeros(nc, dtype=torch.float64, device="cuda")
+        pf = torch.zeros(self.fine.ba.mf_size(0, 1, 1), dtype=torch.float64, device="cuda")
+        check(load().ksgf_macproj_levels_get_phi(self.h, _ptr(pc), _ptr(pf), 1, _stream()), "macproj_levels_get_phi")
+        return pc, pf
+
+    def close(self):
+        if self.h:
+            load().ksgf_macproj_levels_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -21,7 +21,7 @@ STEPEXE = os.path.join(HERE, "ksgf_step")
 HOSTDIR = os.path.join(HERE, "host")
 
 SOURCES = ["ksgf_layout.hip", "ksgf_godunov.hip", "ksgf_mlmg.hip", "ksgf_ops.hip", "ksgf_adjoint.hip",
-           "ksgf_io.hip", "ksgf_nodal.hip"]
+           "ksgf_io.hip", "ksgf_nodal.hip", "ksgf_levels.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -107,6 +107,9 @@ const Region* fab_region(const ksgf_boxarray* ba, int itype, int ngrow, int* err
 const HaloTable* get_halo(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
 double* get_scratch(const ksgf_boxarray* ba, int slot, size_t n, int* err);
 int owner_map(const ksgf_boxarray* ba, hipStream_t st, const int** out);
+// V-cycles of a single-level projector's solve, phi written even when the
+// tolerance is not met (ksgf_mlmg.hip)
+int macproj_vcycles(ksgf_macproj* mp, const double* rhs, double* phi, ksgf_mlmg_info* info, hipStream_t st);
 
 inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
 

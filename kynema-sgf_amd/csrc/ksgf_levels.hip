@@ -1,0 +1,844 @@
+// Composite two-level MAC projection: Hydro::MacProjector over levels 0 and 1
+// (icns_advection.cpp:88-95 builds it on Geom(0..finest), :331 setUMAC with
+// every level's face velocities, :353 project), i.e. AMReX MLMG's composite
+// solve of div(beta grad phi) = div(u*) on the coarse level outside the fine
+// level and the fine level inside it, at refinement ratio 2.
+//
+// Discretisation (restated from AMReX's published method; its source is not
+// part of the reference):
+//  * a fine cell's neighbour outside the fine level and inside the domain is
+//    a coarse-fine ghost: the quadratic through the coarse value b (located at
+//    the coarse cell centre, one fine cell beyond the face) and the two fine
+//    cells inside, 8/15 b + 2/3 u0 - 1/5 u1; b interpolates the coarse data
+//    tangentially (centred slopes and curvature, one-sided slope where a
+//    coarse neighbour is covered or outside, the cross term where the four
+//    diagonal neighbours are usable);
+//  * an uncovered coarse cell next to the fine level takes the average of the
+//    four fine fluxes on each coarse-fine face (reflux), so the composite
+//    operator conserves;
+//  * covered coarse cells carry no unknown; they end at the average of their
+//    children.
+// Iteration: damped Jacobi (omega 6/7) on the fine cells, a coarse
+// correction of the composite residual (the fine residual averaged under the
+// fine level) by one V-cycle of the single-level projector on the coarse
+// grid, piecewise-constant prolongation to the fine cells, damped Jacobi
+// again, until max |composite residual| <= max(atol, rtol r0).
+//
+// Data: coarse unknowns, right-hand side and residual are domain-wide arrays
+// (the coarse level covers the domain; neighbours resolve by index
+// arithmetic with periodic wrap); fine ones are MultiFabs of the fine
+// BoxArray (phi with one ghost layer, filled by k_tl_ghosts before every
+// fine stencil pass).  Every expression is evaluated in the operand order of
+// the CPU oracle (oracle/ksgf_oracle.c, composite two-level section).
+#include <algorithm>
+#include <cstring>
+
+#include "ksgf_internal.h"
+
+using namespace ksgf;
+
+namespace {
+
+constexpr double kOmega = 6.0 / 7.0;
+
+struct TLGeom {
+    int n[3], nf[3], per[3], lobc[3], hibc[3];
+    double hc[3], hf[3], beta;
+};
+
+__device__ __forceinline__ long long tl_ci(const TLGeom& T, int i, int j, int k)
+{
+    return i + (long long)T.n[0] * (j + (long long)T.n[1] * k);
+}
+__device__ __forceinline__ long long tl_fi(const TLGeom& T, int i, int j, int k)
+{
+    return i + (long long)T.nf[0] * (j + (long long)T.nf[1] * k);
+}
+__device__ __forceinline__ int wrapi(int x, int n) { return ((x % n) + n) % n; }
+
+// coarse value at Q (wrapped); ok = 0 where covered or outside the domain
+__device__ __forceinline__ double tl_cval(const TLGeom& T, const double* pc, const unsigned char* cv, const int* Q,
+                                          int& ok)
+{
+    int q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q[d] = Q[d];
+        if (q[d] < 0 || q[d] >= T.n[d]) {
+            if (!T.per[d]) {
+                ok = 0;
+                return 0.0;
+            }
+            q[d] = wrapi(q[d], T.n[d]);
+        }
+    }
+    const long long c = tl_ci(T, q[0], q[1], q[2]);
+    if (cv[c]) {
+        ok = 0;
+        return 0.0;
+    }
+    ok = 1;
+    return pc[c];
+}
+
+__device__ double tl_cinterp(const TLGeom& T, const double* pc, const unsigned char* cv, const int* Q, int dn,
+                             const double* x)
+{
+    int ok;
+    const double uc = tl_cval(T, pc, cv, Q, ok);
+    const int t1 = (dn == 0) ? 1 : 0, t2 = (dn == 2) ? 1 : 2;
+    const int tt[2] = {t1, t2};
+    double term[2];
+    for (int a = 0; a < 2; ++a) {
+        const int t = tt[a];
+        int P[3] = {Q[0], Q[1], Q[2]}, M[3] = {Q[0], Q[1], Q[2]}, okp, okm;
+        P[t] += 1;
+        M[t] -= 1;
+        const double up = tl_cval(T, pc, cv, P, okp), um = tl_cval(T, pc, cv, M, okm);
+        double d1 = 0.0, d2 = 0.0;
+        if (okp && okm) {
+            d1 = 0.5 * (up - um);
+            d2 = 0.5 * ((up - 2.0 * uc) + um);
+        } else if (okp) {
+            d1 = up - uc;
+        } else if (okm) {
+            d1 = uc - um;
+        }
+        term[a] = x[t] * (d1 + x[t] * d2);
+    }
+    double dxy = 0.0;
+    {
+        double v[4];
+        int okc[4];
+        for (int m = 0; m < 4; ++m) {
+            int C[3] = {Q[0], Q[1], Q[2]};
+            C[t1] += (m & 1) ? 1 : -1;
+            C[t2] += (m & 2) ? 1 : -1;
+            v[m] = tl_cval(T, pc, cv, C, okc[m]);
+        }
+        if (okc[0] && okc[1] && okc[2] && okc[3]) dxy = 0.25 * ((v[3] - v[2]) - (v[1] - v[0]));
+    }
+    return ((uc + term[0]) + term[1]) + x[t1] * x[t2] * dxy;
+}
+
+// coarse covered mask from the fine owner map (fine boxes are aligned to 2)
+__global__ void __launch_bounds__(256) k_tl_cover(const int* __restrict__ fown, unsigned char* __restrict__ cv,
+                                                  TLGeom T)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nc = (long long)T.n[0] * T.n[1] * T.n[2];
+    if (t >= nc) return;
+    const int i = (int)(t % T.n[0]), j = (int)((t / T.n[0]) % T.n[1]), k = (int)(t / ((long long)T.n[0] * T.n[1]));
+    cv[t] = fown[tl_fi(T, 2 * i, 2 * j, 2 * k)] >= 0 ? 1 : 0;
+}
+
+// the face-neighbour ghosts of every fine FAB (edges and corners are not
+// read by the 7-point stencil): another fine box's cell, a physical-boundary
+// ghost, or a coarse-fine ghost
+__global__ void __launch_bounds__(256) k_tl_ghosts(MFDev pf, const int* __restrict__ fbox,
+                                                   const int* __restrict__ fown, const double* __restrict__ pc,
+                                                   const unsigned char* __restrict__ cv, TLGeom T, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, fbox, b, i, j, k)) return;
+    const int* bx = fbox + 6 * b;
+    const int q[3] = {i, j, k};
+    int nout = 0, d = 0, s = 0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        if (q[e] < bx[e]) {
+            ++nout;
+            d = e;
+            s = -1;
+        } else if (q[e] > bx[3 + e]) {
+            ++nout;
+            d = e;
+            s = 1;
+        }
+    }
+    if (nout != 1) return;
+    Fab f = fab_at(pf, fbox, b);
+    int i0[3] = {i, j, k};
+    i0[d] -= s;
+    const double u0 = f(i0[0], i0[1], i0[2]);
+    double v;
+    int w[3] = {i, j, k};
+    if (w[d] < 0 || w[d] >= T.nf[d]) {
+        if (!T.per[d]) {
+            const int bc = (s < 0) ? T.lobc[d] : T.hibc[d];
+            f(i, j, k) = (bc == KSGF_LO_DIRICHLET) ? -u0 : u0;
+            return;
+        }
+        w[d] = wrapi(w[d], T.nf[d]);
+    }
+    const int o = fown[tl_fi(T, w[0], w[1], w[2])];
+    if (o >= 0) {
+        Fab g = fab_at(pf, fbox, o);
+        v = g(w[0], w[1], w[2]);
+    } else {
+        int Q[3];
+        double x[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            Q[e] = w[e] / 2;
+            x[e] = ((double)(w[e] - 2 * Q[e]) + 0.5) * 0.5 - 0.5;
+        }
+        x[d] = 0.0;
+        const double bq = tl_cinterp(T, pc, cv, Q, d, x);
+        int i1[3] = {i0[0], i0[1], i0[2]};
+        i1[d] -= s;
+        const double u1 = f(i1[0], i1[1], i1[2]);
+        v = ((8.0 / 15.0) * bq + (2.0 / 3.0) * u0) - 0.2 * u1;
+    }
+    f(i, j, k) = v;
+}
+
+// fine residual R = b - div(beta grad phi) over the valid cells (ghosts filled)
+__global__ void __launch_bounds__(256) k_tl_fres(CMFDev pf, CMFDev bf, MFDev rf, const int* __restrict__ fbox, TLGeom T,
+                                                 RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, fbox, b, i, j, k)) return;
+    CFab p = fab_at(pf, fbox, b);
+    const double u0 = p(i, j, k);
+    const int c[3] = {i, j, k};
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int hi[3] = {c[0], c[1], c[2]}, lo[3] = {c[0], c[1], c[2]};
+        hi[d] += 1;
+        lo[d] -= 1;
+        const double Fp = T.beta * (p(hi[0], hi[1], hi[2]) - u0) / T.hf[d];
+        const double Fm = T.beta * (u0 - p(lo[0], lo[1], lo[2])) / T.hf[d];
+        acc += (Fp - Fm) / T.hf[d];
+    }
+    fab_at(rf, fbox, b)(i, j, k) = fab_at(bf, fbox, b)(i, j, k) - acc;
+}
+
+__global__ void __launch_bounds__(256) k_tl_jacobi(MFDev pf, CMFDev rf, const int* __restrict__ fbox, double w,
+                                                   RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, fbox, b, i, j, k)) return;
+    fab_at(pf, fbox, b)(i, j, k) += w * fab_at(rf, fbox, b)(i, j, k);
+}
+
+// fine flux beta dphi/dx_d through the face of fine cell f on side s (the
+// neighbour is in the FAB: valid or ghost)
+__device__ __forceinline__ double tl_fflux(const TLGeom& T, const CFab& p, const int* f, int d, int s)
+{
+    int g[3] = {f[0], f[1], f[2]};
+    g[d] += s;
+    const double u0 = p(f[0], f[1], f[2]), gv = p(g[0], g[1], g[2]);
+    return (s > 0) ? T.beta * (gv - u0) / T.hf[d] : T.beta * (u0 - gv) / T.hf[d];
+}
+
+// composite residual on the coarse cells: uncovered R = b - Lc (averaged fine
+// fluxes on coarse-fine faces), covered 0
+__global__ void __launch_bounds__(256) k_tl_cres(const double* __restrict__ pc, const double* __restrict__ bc,
+                                                 double* __restrict__ rc, const unsigned char* __restrict__ cv,
+                                                 CMFDev pf, const int* __restrict__ fbox, const int* __restrict__ fown,
+                                                 TLGeom T)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nc = (long long)T.n[0] * T.n[1] * T.n[2];
+    if (t >= nc) return;
+    if (cv[t]) {
+        rc[t] = 0.0;
+        return;
+    }
+    const int C[3] = {(int)(t % T.n[0]), (int)((t / T.n[0]) % T.n[1]), (int)(t / ((long long)T.n[0] * T.n[1]))};
+    const double uc = pc[t];
+    double acc = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        double F[2];
+        for (int a = 0; a < 2; ++a) {
+            const int s = a ? 1 : -1;
+            int N[3] = {C[0], C[1], C[2]};
+            N[d] += s;
+            if (N[d] < 0 || N[d] >= T.n[d]) {
+                if (!T.per[d]) {
+                    const int bcd = (s < 0) ? T.lobc[d] : T.hibc[d];
+                    F[a] = (bcd == KSGF_LO_DIRICHLET)
+                               ? ((s > 0) ? T.beta * (-uc - uc) / T.hc[d] : T.beta * (uc + uc) / T.hc[d])
+                               : 0.0;
+                    continue;
+                }
+                N[d] = wrapi(N[d], T.n[d]);
+            }
+            const long long nci = tl_ci(T, N[0], N[1], N[2]);
+            if (cv[nci]) {
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                double ff[4];
+                for (int m = 0; m < 4; ++m) {
+                    int fi[3];
+                    fi[d] = (s > 0) ? 2 * N[d] : 2 * N[d] + 1;
+                    fi[t1] = 2 * N[t1] + (m & 1);
+                    fi[t2] = 2 * N[t2] + ((m >> 1) & 1);
+                    const int o = fown[tl_fi(T, fi[0], fi[1], fi[2])];
+                    ff[m] = tl_fflux(T, fab_at(pf, fbox, o), fi, d, -s);
+                }
+                F[a] = 0.25 * ((ff[0] + ff[1]) + (ff[2] + ff[3]));
+            } else {
+                F[a] = (s > 0) ? T.beta * (pc[nci] - uc) / T.hc[d] : T.beta * (uc - pc[nci]) / T.hc[d];
+            }
+        }
+        acc += (F[1] - F[0]) / T.hc[d];
+    }
+    rc[t] = bc[t] - acc;
+}
+
+// the average of the 8 children of every covered coarse cell: the coarse
+// correction's right-hand side (from the fine residual) or the final coarse
+// solution (from the fine solution)
+__global__ void __launch_bounds__(256) k_tl_avg8(double* __restrict__ out, const unsigned char* __restrict__ cv,
+                                                 CMFDev fm, const int* __restrict__ fbox, const int* __restrict__ fown,
+                                                 TLGeom T)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nc = (long long)T.n[0] * T.n[1] * T.n[2];
+    if (t >= nc || !cv[t]) return;
+    const int i = (int)(t % T.n[0]), j = (int)((t / T.n[0]) % T.n[1]), k = (int)(t / ((long long)T.n[0] * T.n[1]));
+    const int o = fown[tl_fi(T, 2 * i, 2 * j, 2 * k)];
+    CFab f = fab_at(fm, fbox, o);
+    double s8 = 0.0;
+    for (int kk = 0; kk < 2; ++kk)
+        for (int jj = 0; jj < 2; ++jj)
+            for (int ii = 0; ii < 2; ++ii) s8 += f(2 * i + ii, 2 * j + jj, 2 * k + kk);
+    out[t] = 0.125 * s8;
+}
+
+// coarse correction: pc += ec everywhere, fine cells += ec of the parent
+__global__ void __launch_bounds__(256) k_tl_addc(double* __restrict__ pc, const double* __restrict__ ec, long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) pc[t] += ec[t];
+}
+__global__ void __launch_bounds__(256) k_tl_prolong(MFDev pf, const double* __restrict__ ec,
+                                                    const int* __restrict__ fbox, TLGeom T, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, fbox, b, i, j, k)) return;
+    fab_at(pf, fbox, b)(i, j, k) += ec[tl_ci(T, i / 2, j / 2, k / 2)];
+}
+
+// div u over the valid cells: fine into a MultiFab, coarse into a
+// domain-wide array (covered cells 0)
+__global__ void __launch_bounds__(256) k_tl_div_f(MFDev out, CMFDev ux, CMFDev uy, CMFDev uz,
+                                                  const int* __restrict__ box, double hx, double hy, double hz,
+                                                  RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    CFab a = fab_at(ux, box, b), e = fab_at(uy, box, b), g = fab_at(uz, box, b);
+    const double dx = (a(i + 1, j, k) - a(i, j, k)) / hx;
+    const double dy = (e(i, j + 1, k) - e(i, j, k)) / hy;
+    const double dz = (g(i, j, k + 1) - g(i, j, k)) / hz;
+    fab_at(out, box, b)(i, j, k) = (dx + dy) + dz;
+}
+__global__ void __launch_bounds__(256) k_tl_div_c(double* __restrict__ out, const unsigned char* __restrict__ cv,
+                                                  CMFDev ux, CMFDev uy, CMFDev uz, const int* __restrict__ box,
+                                                  TLGeom T, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    const long long c = tl_ci(T, i, j, k);
+    if (cv[c]) {
+        out[c] = 0.0;
+        return;
+    }
+    CFab a = fab_at(ux, box, b), e = fab_at(uy, box, b), g = fab_at(uz, box, b);
+    const double dx = (a(i + 1, j, k) - a(i, j, k)) / T.hc[0];
+    const double dy = (e(i, j + 1, k) - e(i, j, k)) / T.hc[1];
+    const double dz = (g(i, j, k + 1) - g(i, j, k)) / T.hc[2];
+    out[c] = (dx + dy) + dz;
+}
+
+// u -= beta grad phi on the valid faces of direction D: fine from the FAB
+// (ghosts filled), coarse from the domain-wide solution with the domain BCs
+template <int D>
+__global__ void __launch_bounds__(256) k_tl_fix_f(MFDev u, CMFDev pf, const int* __restrict__ box, TLGeom T, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    CFab p = fab_at(pf, box, b);
+    const double hi = p(i, j, k);
+    const double lo = p(i - (D == 0), j - (D == 1), k - (D == 2));
+    fab_at(u, box, b)(i, j, k) -= T.beta * (hi - lo) / T.hf[D];
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_tl_fix_c(MFDev u, const double* __restrict__ pc, const int* __restrict__ box,
+                                                  TLGeom T, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+    lo[D] -= 1;
+    double F;
+    if (lo[D] < 0 || hi[D] >= T.n[D]) {
+        if (!T.per[D]) {
+            const bool low = lo[D] < 0;
+            const double u0 = low ? pc[tl_ci(T, hi[0], hi[1], hi[2])] : pc[tl_ci(T, lo[0], lo[1], lo[2])];
+            const int bcd = low ? T.lobc[D] : T.hibc[D];
+            F = (bcd == KSGF_LO_DIRICHLET) ? (low ? T.beta * (u0 + u0) / T.hc[D] : T.beta * (-u0 - u0) / T.hc[D])
+                                           : 0.0;
+            fab_at(u, box, b)(i, j, k) -= F;
+            return;
+        }
+        if (lo[D] < 0) lo[D] += T.n[D];
+        if (hi[D] >= T.n[D]) hi[D] -= T.n[D];
+    }
+    F = T.beta * (pc[tl_ci(T, hi[0], hi[1], hi[2])] - pc[tl_ci(T, lo[0], lo[1], lo[2])]) / T.hc[D];
+    fab_at(u, box, b)(i, j, k) -= F;
+}
+
+// max |a| (non-negative doubles order as their bit patterns)
+__global__ void __launch_bounds__(256) k_tl_absmax(const double* __restrict__ a, long long n,
+                                                   unsigned long long* __restrict__ out)
+{
+    __shared__ double sm[256];
+    double m = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const double v = fabs(a[t]);
+        m = (v > m || v != v) ? v : m;
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = sm[threadIdx.x + w];
+            if (o > sm[threadIdx.x] || o != o) sm[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// sums for the composite mean: coarse over uncovered cells, fine over the
+// valid cells of a MultiFab
+__global__ void __launch_bounds__(256) k_tl_sum_c(const double* __restrict__ a, const unsigned char* __restrict__ cv,
+                                                  long long n, double* __restrict__ out)
+{
+    __shared__ double sm[256];
+    double s = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        if (!cv[t]) s += a[t];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, sm[0]);
+}
+__global__ void __launch_bounds__(256) k_tl_sum_f(CMFDev f, const int* __restrict__ box, RegDev r,
+                                                  double* __restrict__ out)
+{
+    __shared__ double sm[256];
+    int b, i, j, k;
+    double s = 0.0;
+    if (region_locate(r, box, b, i, j, k)) s = fab_at(f, box, b)(i, j, k);
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, sm[0]);
+}
+__global__ void __launch_bounds__(256) k_tl_shift_c(double* __restrict__ a, const unsigned char* __restrict__ cv,
+                                                    long long n, const double* __restrict__ mu, int all)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && (all || !cv[t])) a[t] -= *mu;
+}
+__global__ void __launch_bounds__(256) k_tl_shift_f(MFDev f, const int* __restrict__ box, RegDev r,
+                                                    const double* __restrict__ mu)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    fab_at(f, box, b)(i, j, k) -= *mu;
+}
+__global__ void k_tl_mean(double* __restrict__ s, double wc, double wf, double den)
+{
+    // s[0]: coarse sum, s[1]: fine sum -> s[2] = (8 sc + sf) / den
+    s[2] = (wc * s[0] + wf * s[1]) / den;
+}
+
+}  // namespace
+
+struct ksgf_macproj_levels {
+    const ksgf_boxarray* cba = nullptr;
+    const ksgf_boxarray* fba = nullptr;
+    TLGeom T{};
+    ksgf_mlmg_options opt{};
+    int singular = 1;
+    ksgf_macproj* crse = nullptr;  // single-level projector on the coarse grid: one V-cycle per correction
+    const int* fown = nullptr;     // fine owner map (owned by the fine BoxArray)
+    unsigned char* cv = nullptr;
+    double *pc = nullptr, *bc = nullptr, *rc = nullptr, *ec = nullptr;
+    double *pf = nullptr, *bf = nullptr, *rf = nullptr;
+    long long nc = 0, nfg = 0, nf0 = 0;
+    long long ncov = 0, nfine = 0;
+    unsigned long long* dmax = nullptr;  // 2 slots
+    double* dsum = nullptr;              // 3 slots
+    unsigned long long* hmax = nullptr;  // pinned
+};
+
+namespace {
+
+int tl_mfs(ksgf_macproj_levels* m, MFDev* pf, CMFDev* cpf, CMFDev* bf, MFDev* rf, CMFDev* crf)
+{
+    int e;
+    if ((e = mfdev(m->fba, m->pf, KSGF_CELL, 1, 1, pf)) || (e = cmfdev(m->fba, m->pf, KSGF_CELL, 1, 1, cpf)) ||
+        (e = cmfdev(m->fba, m->bf, KSGF_CELL, 1, 0, bf)) || (e = mfdev(m->fba, m->rf, KSGF_CELL, 1, 0, rf)) ||
+        (e = cmfdev(m->fba, m->rf, KSGF_CELL, 1, 0, crf)))
+        return e;
+    return KSGF_OK;
+}
+
+struct TLRun {
+    ksgf_macproj_levels* m;
+    hipStream_t st;
+    MFDev pf;
+    CMFDev cpf, bf, crf;
+    MFDev rf;
+    RegDev rg, rv;  // fine FAB storage (ng 1), fine valid cells
+    int ng_chunks, nv_chunks;
+
+    void ghosts()
+    {
+        hipLaunchKernelGGL(k_tl_ghosts, dim3(ng_chunks), dim3(256), 0, st, pf, m->fba->dbox, m->fown,
+                           (const double*)m->pc, (const unsigned char*)m->cv, m->T, rg);
+    }
+    void fres()
+    {
+        hipLaunchKernelGGL(k_tl_fres, dim3(nv_chunks), dim3(256), 0, st, cpf, bf, rf, m->fba->dbox, m->T, rv);
+    }
+    void cres()
+    {
+        hipLaunchKernelGGL(k_tl_cres, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, (const double*)m->pc,
+                           (const double*)m->bc, m->rc, (const unsigned char*)m->cv, cpf, m->fba->dbox, m->fown, m->T);
+    }
+    // composite residual on both levels
+    void residual()
+    {
+        ghosts();
+        fres();
+        cres();
+    }
+    void smooth(int nsweep, double w)
+    {
+        for (int s = 0; s < nsweep; ++s) {
+            ghosts();
+            fres();
+            hipLaunchKernelGGL(k_tl_jacobi, dim3(nv_chunks), dim3(256), 0, st, pf, crf, m->fba->dbox, w, rv);
+        }
+    }
+    int norm(double* out)
+    {
+        KSGF_HIP_CHECK(hipMemsetAsync(m->dmax, 0, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_tl_absmax, dim3(std::min(1024, grid_for(m->nc, 256))), dim3(256), 0, st,
+                           (const double*)m->rc, m->nc, m->dmax);
+        hipLaunchKernelGGL(k_tl_absmax, dim3(std::min(1024, grid_for(m->nf0, 256))), dim3(256), 0, st,
+                           (const double*)m->rf, m->nf0, m->dmax);
+        KSGF_LAUNCH_CHECK();
+        KSGF_HIP_CHECK(hipMemcpyAsync(m->hmax, m->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        KSGF_HIP_CHECK(hipStreamSynchronize(st));
+        unsigned long long bits = *m->hmax;
+        double v;
+        memcpy(&v, &bits, sizeof(v));
+        *out = v;
+        return KSGF_OK;
+    }
+    // subtract the composite mean of (coarse a, fine f); all_coarse: also
+    // from the covered coarse cells
+    int sub_mean(double* a, MFDev f, CMFDev cf, int all_coarse)
+    {
+        KSGF_HIP_CHECK(hipMemsetAsync(m->dsum, 0, 3 * sizeof(double), st));
+        hipLaunchKernelGGL(k_tl_sum_c, dim3(std::min(1024, grid_for(m->nc, 256))), dim3(256), 0, st, (const double*)a,
+                           (const unsigned char*)m->cv, m->nc, m->dsum);
+        hipLaunchKernelGGL(k_tl_sum_f, dim3(nv_chunks), dim3(256), 0, st, cf, m->fba->dbox, rv, m->dsum + 1);
+        const double den = 8.0 * (double)(m->nc - m->ncov) + (double)m->nfine;
+        hipLaunchKernelGGL(k_tl_mean, dim3(1), dim3(1), 0, st, m->dsum, 8.0, 1.0, den);
+        hipLaunchKernelGGL(k_tl_shift_c, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, a, (const unsigned char*)m->cv,
+                           m->nc, (const double*)(m->dsum + 2), all_coarse);
+        hipLaunchKernelGGL(k_tl_shift_f, dim3(nv_chunks), dim3(256), 0, st, f, m->fba->dbox, rv,
+                           (const double*)(m->dsum + 2));
+        KSGF_LAUNCH_CHECK();
+        return KSGF_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray* fine_ba, const int* lobc,
+                               const int* hibc, double beta, const ksgf_mlmg_options* opts, ksgf_macproj_levels** out)
+{
+    if (!crse_ba || !fine_ba || !lobc || !hibc || !out || !(beta > 0.0)) {
+        set_error("ksgf_macproj_levels_create: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (!crse_ba->covers_domain) {
+        set_error("ksgf_macproj_levels_create: the coarse level must cover its domain");
+        return KSGF_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const int per = (lobc[d] == KSGF_LO_PERIODIC);
+        if (fine_ba->dlo[d] != 2 * crse_ba->dlo[d] || fine_ba->dhi[d] + 1 != 2 * (crse_ba->dhi[d] + 1) ||
+            crse_ba->dlo[d] != 0 || per != (hibc[d] == KSGF_LO_PERIODIC) || per != crse_ba->periodic[d] ||
+            per != fine_ba->periodic[d]) {
+            set_error("ksgf_macproj_levels_create: the fine domain must be the coarse domain (origin 0) refined by 2, "
+                      "with matching periodicity");
+            return KSGF_ERR_INVALID;
+        }
+    }
+    for (int b = 0; b < fine_ba->nbox; ++b) {
+        const int* x = &fine_ba->hbox[6 * b];
+        for (int d = 0; d < 3; ++d)
+            if ((x[d] & 1) || !(x[3 + d] & 1)) {
+                set_error("ksgf_macproj_levels_create: fine boxes must be aligned to the refinement ratio 2");
+                return KSGF_ERR_INVALID;
+            }
+    }
+    ksgf_macproj_levels* m = new ksgf_macproj_levels();
+    m->cba = crse_ba;
+    m->fba = fine_ba;
+    if (opts)
+        m->opt = *opts;
+    else
+        ksgf_mlmg_default_options(&m->opt);
+    TLGeom& T = m->T;
+    m->singular = 1;
+    for (int d = 0; d < 3; ++d) {
+        T.n[d] = crse_ba->dhi[d] + 1;
+        T.nf[d] = 2 * T.n[d];
+        T.per[d] = (lobc[d] == KSGF_LO_PERIODIC);
+        T.lobc[d] = lobc[d];
+        T.hibc[d] = hibc[d];
+        T.hc[d] = crse_ba->dx[d];
+        T.hf[d] = 0.5 * crse_ba->dx[d];
+        if (lobc[d] == KSGF_LO_DIRICHLET || hibc[d] == KSGF_LO_DIRICHLET) m->singular = 0;
+    }
+    T.beta = beta;
+    m->nc = (long long)T.n[0] * T.n[1] * T.n[2];
+    for (int b = 0; b < fine_ba->nbox; ++b) {
+        const int* x = &fine_ba->hbox[6 * b];
+        m->nfine += (long long)(x[3] - x[0] + 1) * (x[4] - x[1] + 1) * (x[5] - x[2] + 1);
+    }
+    m->ncov = m->nfine / 8;
+    int e = KSGF_OK, err = KSGF_OK;
+    ksgf_mlmg_options co = m->opt;
+    co.maxiter = 1;
+    co.precision = KSGF_MG_FP64;
+    const Region* rg = fab_region(fine_ba, KSGF_CELL, 1, &err);
+    const Region* r0 = fab_region(fine_ba, KSGF_CELL, 0, &err);
+    if (!rg || !r0) {
+        delete m;
+        return err;
+    }
+    m->nfg = rg->total;
+    m->nf0 = r0->total;
+    if ((e = ksgf_macproj_create(crse_ba, lobc, hibc, beta, &co, &m->crse))) {
+        delete m;
+        return e;
+    }
+    auto fail = [&](int code) {
+        ksgf_macproj_levels_destroy(m);
+        return code;
+    };
+    if (hipMalloc(&m->cv, m->nc) != hipSuccess || hipMalloc(&m->pc, m->nc * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->bc, m->nc * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->rc, m->nc * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->ec, m->nc * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->pf, m->nfg * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->bf, m->nf0 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->rf, m->nf0 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->dmax, 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&m->dsum, 3 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&m->hmax, sizeof(unsigned long long)) != hipSuccess) {
+        set_error("ksgf_macproj_levels_create: out of device memory");
+        return fail(KSGF_ERR_HIP);
+    }
+    if ((e = owner_map(fine_ba, nullptr, &m->fown))) return fail(e);
+    hipLaunchKernelGGL(k_tl_cover, dim3(grid_for(m->nc, 256)), dim3(256), 0, nullptr, m->fown, m->cv, T);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_error("ksgf_macproj_levels_create: covered mask");
+        return fail(KSGF_ERR_HIP);
+    }
+    *out = m;
+    return KSGF_OK;
+}
+
+int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, double* const* umac_f, int ng,
+                                double rtol, double atol, ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!m || !umac_c || !umac_f || ng < 0) {
+        set_error("ksgf_macproj_levels_project: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (!umac_c[d] || !umac_f[d]) {
+            set_error("ksgf_macproj_levels_project: null face MultiFab");
+            return KSGF_ERR_INVALID;
+        }
+    hipStream_t st = (hipStream_t)stream;
+    const ksgf_boxarray* cba = m->cba;
+    const ksgf_boxarray* fba = m->fba;
+    int e, err = KSGF_OK;
+    TLRun R;
+    R.m = m;
+    R.st = st;
+    if ((e = tl_mfs(m, &R.pf, &R.cpf, &R.bf, &R.rf, &R.crf))) return e;
+    const Region* rg = fab_region(fba, KSGF_CELL, 1, &err);
+    const int z[3] = {0, 0, 0};
+    const Region* rv = get_region(fba, z, z, &err);
+    const Region* rcv = get_region(cba, z, z, &err);
+    if (!rg || !rv || !rcv) return err;
+    R.rg = regdev(rg, fba->nbox);
+    R.rv = regdev(rv, fba->nbox);
+    R.ng_chunks = rg->nchunks;
+    R.nv_chunks = rv->nchunks;
+    MFDev uf[3], uc[3];
+    CMFDev cuf[3], cuc[3];
+    for (int d = 0; d < 3; ++d)
+        if ((e = mfdev(fba, umac_f[d], KSGF_XFACE + d, 1, ng, &uf[d])) ||
+            (e = cmfdev(fba, umac_f[d], KSGF_XFACE + d, 1, ng, &cuf[d])) ||
+            (e = mfdev(cba, umac_c[d], KSGF_XFACE + d, 1, ng, &uc[d])) ||
+            (e = cmfdev(cba, umac_c[d], KSGF_XFACE + d, 1, ng, &cuc[d])))
+            return e;
+    // one velocity per coarse-fine face: the fine average on the coarse faces
+    // under and around the fine level
+    if ((e = ksgf_average_down_faces(fba, umac_f[0], umac_f[1], umac_f[2], ng, cba, umac_c[0], umac_c[1], umac_c[2],
+                                     ng, 1, 2, stream)))
+        return e;
+    // right-hand sides
+    const TLGeom& T = m->T;
+    hipLaunchKernelGGL(k_tl_div_f, dim3(R.nv_chunks), dim3(256), 0, st, MFDev{m->bf, R.bf.pre, KSGF_CELL, 0, 1},
+                       cuf[0], cuf[1], cuf[2], fba->dbox, T.hf[0], T.hf[1], T.hf[2], R.rv);
+    hipLaunchKernelGGL(k_tl_div_c, dim3(rcv->nchunks), dim3(256), 0, st, m->bc, (const unsigned char*)m->cv, cuc[0],
+                       cuc[1], cuc[2], cba->dbox, T, regdev(rcv, cba->nbox));
+    KSGF_LAUNCH_CHECK();
+    if (m->singular && (e = R.sub_mean(m->bc, MFDev{m->bf, R.bf.pre, KSGF_CELL, 0, 1}, R.bf, 0))) return e;
+    KSGF_HIP_CHECK(hipMemsetAsync(m->pc, 0, m->nc * sizeof(double), st));
+    KSGF_HIP_CHECK(hipMemsetAsync(m->pf, 0, m->nfg * sizeof(double), st));
+    const double diag = -2.0 * T.beta * ((1.0 / (T.hf[0] * T.hf[0]) + 1.0 / (T.hf[1] * T.hf[1])) +
+                                         1.0 / (T.hf[2] * T.hf[2]));
+    const double w = kOmega / diag;
+    R.residual();
+    KSGF_LAUNCH_CHECK();
+    double r0 = 0.0, r = 0.0;
+    if ((e = R.norm(&r0))) return e;
+    const double target = std::max(atol, rtol * r0);
+    r = r0;
+    ksgf_mlmg_info inf{};
+    inf.r0 = r0;
+    inf.converged = (r <= target);
+    inf.nlevels = 2;
+    int ret = KSGF_OK;
+    for (int it = 1; !inf.converged && it <= m->opt.maxiter; ++it) {
+        R.smooth(m->opt.num_pre_smooth, w);
+        R.residual();
+        hipLaunchKernelGGL(k_tl_avg8, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->rc,
+                           (const unsigned char*)m->cv, R.crf, fba->dbox, m->fown, T);
+        KSGF_LAUNCH_CHECK();
+        ksgf_mlmg_info ci{};
+        if ((e = macproj_vcycles(m->crse, m->rc, m->ec, &ci, st))) return e;
+        inf.bottom_iters += ci.bottom_iters;
+        hipLaunchKernelGGL(k_tl_addc, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const double*)m->ec, m->nc);
+        hipLaunchKernelGGL(k_tl_prolong, dim3(R.nv_chunks), dim3(256), 0, st, R.pf, (const double*)m->ec, fba->dbox, T,
+                           R.rv);
+        R.smooth(m->opt.num_post_smooth, w);
+        R.residual();
+        KSGF_LAUNCH_CHECK();
+        if ((e = R.norm(&r))) return e;
+        inf.iters = it;
+        if (r <= target)
+            inf.converged = 1;
+        else if (!(r <= 1.0e20 * r0)) {
+            ret = KSGF_ERR_DIVERGED;
+            break;
+        }
+    }
+    inf.rfinal = r;
+    if (info) *info = inf;
+    if (ret) {
+        set_error("ksgf_macproj_levels_project: the composite iteration diverged");
+        return ret;
+    }
+    // covered coarse cells: the average of their children; singular: the
+    // composite mean removed
+    hipLaunchKernelGGL(k_tl_avg8, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const unsigned char*)m->cv,
+                       CMFDev{m->pf, R.cpf.pre, KSGF_CELL, 1, 1}, fba->dbox, m->fown, T);
+    KSGF_LAUNCH_CHECK();
+    if (m->singular && (e = R.sub_mean(m->pc, R.pf, R.cpf, 1))) return e;
+    // u -= beta grad phi on both levels, then the fine average on the coarse
+    // faces under and around the fine level
+    R.ghosts();
+    for (int d = 0; d < 3; ++d) {
+        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        ghi[d] = 1;
+        const Region* ff = get_region(fba, glo, ghi, &err);
+        const Region* fc = get_region(cba, glo, ghi, &err);
+        if (!ff || !fc) return err;
+        RegDev rf = regdev(ff, fba->nbox), rc = regdev(fc, cba->nbox);
+        switch (d) {
+        case 0:
+            hipLaunchKernelGGL(k_tl_fix_f<0>, dim3(ff->nchunks), dim3(256), 0, st, uf[0], R.cpf, fba->dbox, T, rf);
+            hipLaunchKernelGGL(k_tl_fix_c<0>, dim3(fc->nchunks), dim3(256), 0, st, uc[0], (const double*)m->pc,
+                               cba->dbox, T, rc);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_tl_fix_f<1>, dim3(ff->nchunks), dim3(256), 0, st, uf[1], R.cpf, fba->dbox, T, rf);
+            hipLaunchKernelGGL(k_tl_fix_c<1>, dim3(fc->nchunks), dim3(256), 0, st, uc[1], (const double*)m->pc,
+                               cba->dbox, T, rc);
+            break;
+        default:
+            hipLaunchKernelGGL(k_tl_fix_f<2>, dim3(ff->nchunks), dim3(256), 0, st, uf[2], R.cpf, fba->dbox, T, rf);
+            hipLaunchKernelGGL(k_tl_fix_c<2>, dim3(fc->nchunks), dim3(256), 0, st, uc[2], (const double*)m->pc,
+                               cba->dbox, T, rc);
+            break;
+        }
+        KSGF_LAUNCH_CHECK();
+    }
+    if ((e = ksgf_average_down_faces(fba, umac_f[0], umac_f[1], umac_f[2], ng, cba, umac_c[0], umac_c[1], umac_c[2],
+                                     ng, 1, 2, stream)))
+        return e;
+    return inf.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+}
+
+int ksgf_macproj_levels_get_phi(ksgf_macproj_levels* m, double* phi_c, double* phi_f, int ng_f, ksgf_stream_t stream)
+{
+    if (!m || !phi_c || !phi_f || ng_f != 1) {
+        set_error("ksgf_macproj_levels_get_phi: bad argument (the fine solution has one ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    KSGF_HIP_CHECK(hipMemcpyAsync(phi_c, m->pc, m->nc * sizeof(double), hipMemcpyDeviceToDevice, st));
+    KSGF_HIP_CHECK(hipMemcpyAsync(phi_f, m->pf, m->nfg * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return KSGF_OK;
+}
+
+int ksgf_macproj_levels_destroy(ksgf_macproj_levels* m)
+{
+    if (!m) return KSGF_OK;
+    if (m->crse) ksgf_macproj_destroy(m->crse);
+    hipFree(m->cv);
+    hipFree(m->pc);
+    hipFree(m->bc);
+    hipFree(m->rc);
+    hipFree(m->ec);
+    hipFree(m->pf);
+    hipFree(m->bf);
+    hipFree(m->rf);
+    hipFree(m->dmax);
+    hipFree(m->dsum);
+    if (m->hmax) hipHostFree(m->hmax);
+    delete m;
+    return KSGF_OK;
+}
+
+}  // extern "C"

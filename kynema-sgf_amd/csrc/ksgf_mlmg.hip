@@ -1468,6 +1468,24 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
 
 }  // namespace ksgf
 
+namespace ksgf {
+// the projector's V-cycles (opts.maxiter of them, or fewer if the residual
+// reaches zero) on rhs, phi out whether or not the tolerance was met: the
+// coarse correction of a composite two-level solve
+int macproj_vcycles(ksgf_macproj* mp, const double* rhs, double* phi, ksgf_mlmg_info* info, hipStream_t st)
+{
+    const LevelGeom& L0 = mp->P;
+    long long n = (long long)L0.n0 * L0.n1 * L0.n2;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, rhs, mp->rhs, L0, 1);
+    KSGF_LAUNCH_CHECK();
+    int ret = mg_solve(mp, 0.0, 0.0, info, st);
+    if (ret < 0 && ret != KSGF_ERR_NOT_CONVERGED) return ret;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, (const double*)mp->phi, phi, L0, 0);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+}  // namespace ksgf
+
 extern "C" {
 
 int ksgf_prof_reset(void)
@@ -1821,6 +1839,7 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
     KSGF_LAUNCH_CHECK();
     return ret;
 }
+
 
 // ---------------------------------------------------------------------------
 // MLABecLaplacian with constant coefficients over the same multigrid

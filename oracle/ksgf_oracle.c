@@ -2077,7 +2077,8 @@ static void tl_avg_faces(const tl_t* T, double* const* umc, double* const* umf)
                         fi[t2] = 2 * F[t2] + ((m >> 1) & 1);
                         v[m] = umf[d][fi[0] + (long)sf[0] * (fi[1] + (long)sf[1] * fi[2])];
                     }
-                    umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] = 0.25 * ((v[0] + v[1]) + (v[2] + v[3]));
+                    /* the summation order of average_down_faces */
+                    umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] = ((((0.0 + v[0]) + v[1]) + v[2]) + v[3]) * 0.25;
                 }
     }
 }
