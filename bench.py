@@ -6,7 +6,7 @@ correction); the headline value is the mixed path, the FP64 path is
 reported beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
-                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch|nodal]
+                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch|nodal|levels]
 
 A step is one pass of the hot path over the whole grid, driven by the C++
 host layer (GodunovPredictor): convective CFL, source terms,
@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
-    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal"], default="abl",
+    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal", "levels"], default="abl",
                    help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
                         "N(0,1) right-hand sides, seed 5) solved as one batch (hipGraph replay and eager) and one "
                         "after another; nodal: the full incflo step (advect + MAC projection + ApplyProjection's "
@@ -356,6 +356,100 @@ def nodal_step(n, steps, warmup, no_cpu, cpu_n):
     return out
 
 
+def _faces_on_device(ba, d, ng, fn):
+    """face MultiFab (1 comp, ngrow ng) of direction d on the device, each FAB
+    (ghost faces included) filled with fn(X, Y, Z) at its face centres"""
+    import torch
+
+    out = torch.empty(ba.mf_size(1 + d, 1, ng), dtype=torch.float64, device="cuda")
+    lo, n = ba.fab_lo_n(1 + d, ng)
+    off = ba.fab_offsets(1 + d, 1, ng)
+    h = ba.geom.dx
+    for b in range(ba.nbox):
+        ax = [(int(lo[b, e]) + torch.arange(int(n[b, e]), dtype=torch.float64, device="cuda") +
+               (0.0 if e == d else 0.5)) * h[e] for e in range(3)]
+        Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        out[int(off[b]):int(off[b + 1])] = fn(X, Y, Z).reshape(-1)
+    return out
+
+
+def levels_project(steps, warmup, no_cpu):
+    """Composite two-level MAC projection (Hydro::MacProjector over levels 0
+    and 1) on the 1,500-box ragged 320x320x480 level 0 with a level 1 at
+    ratio 2 over 40% of the domain; the oracle's composite projection on a
+    64x64x96 copy of the layout is the CPU baseline."""
+    import torch
+
+    ksgf = importlib.import_module("kynema-sgf_amd")
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    case, fba = wl.ragged_two_level()
+    cba = case.ba
+    dc = ksgf.DeviceBoxArray(cba)
+    df = ksgf.DeviceBoxArray(fba, covers_domain=False)
+    u0c = [_faces_on_device(cba, d, 1, lambda X, Y, Z, d=d: wl.two_level_faces(cba.geom, torch, X, Y, Z, d))
+           for d in range(3)]
+    u0f = [_faces_on_device(fba, d, 1, lambda X, Y, Z, d=d: wl.two_level_faces(fba.geom, torch, X, Y, Z, d))
+           for d in range(3)]
+    uc = [a.clone() for a in u0c]
+    uf = [a.clone() for a in u0f]
+    rtol, atol = 1e-11, 0.0
+    mp = ksgf.MacProjectorLevels(dc, df, case.lobc, case.hibc, 1.0)
+    st = torch.cuda.current_stream()
+    ms, its = [], []
+    for s in range(warmup + steps):
+        for a, b in zip(uc + uf, u0c + u0f):
+            a.copy_(b)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        info = mp.project(uc, uf, 1, rtol, atol)
+        e1.record(st)
+        torch.cuda.synchronize()
+        if s >= warmup:
+            ms.append(e0.elapsed_time(e1))
+            its.append(info["iters"])
+    nfine = int(np.sum(np.prod(fba.boxes[:, 3:] - fba.boxes[:, :3] + 1, axis=1)))
+    ncomp = case.ncell - nfine // 8 + nfine
+    t = float(np.mean(ms))
+    out = {
+        "metric": "composite cell-updates/s of the two-level MAC projection",
+        "value": ncomp / (t * 1e-3),
+        "unit": "cell-updates/s (uncovered level-0 cells + level-1 cells)",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": t,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ragged {cba.nbox} boxes 320x320x480 level 0 + {fba.nbox} level-1 boxes (ratio 2, "
+                               f"{nfine} cells over 40% of the domain), triply periodic, rtol 1e-11",
+                   "composite_iterations": its, "r0": info["r0"], "rfinal": info["rfinal"]},
+    }
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.build()
+        c2, f2 = wl.ragged_two_level((64, 64, 96), 32, fine_z=(20, 58), fine_box=32)
+        umc, umf = [], []
+        for g, lst in ((c2.geom, umc), (f2.geom, umf)):
+            for d in range(3):
+                ax = [(np.arange(g.n_cell[e] + (e == d)) + (0.0 if e == d else 0.5)) * g.dx[e] for e in range(3)]
+                Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+                lst.append(np.ascontiguousarray(wl.two_level_faces(g, np, X, Y, Z, d)).ravel())
+        n2 = int(np.sum(np.prod(f2.boxes[:, 3:] - f2.boxes[:, :3] + 1, axis=1)))
+        t0 = time.perf_counter()
+        rinfo, _, _ = O.mac_project_two_levels(c2.geom.n_cell, c2.geom.dx, c2.lobc, c2.hibc, f2.boxes, 1.0, umc, umf,
+                                               rtol, atol)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": (c2.ncell - n2 // 8 + n2) / dt, "unit": out["unit"],
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": f"one oracle composite projection of the 64x64x96 two-level copy "
+                                         f"({rinfo['iters']} iterations), {dt:.2f} s"}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -373,6 +467,9 @@ def main():
         return
     if a.workload == "nodal":
         print(json.dumps(nodal_step(tuple(a.n), a.steps, a.warmup, a.no_cpu, a.cpu_n)))
+        return
+    if a.workload == "levels":
+        print(json.dumps(levels_project(a.steps, a.warmup, a.no_cpu)))
         return
     wl = importlib.import_module("kynema-sgf_amd.workloads")
     n = tuple(a.n)

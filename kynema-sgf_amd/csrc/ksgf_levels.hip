@@ -27,8 +27,9 @@
 // Data: coarse unknowns, right-hand side and residual are domain-wide arrays
 // (the coarse level covers the domain; neighbours resolve by index
 // arithmetic with periodic wrap); fine ones are MultiFabs of the fine
-// BoxArray (phi with one ghost layer, filled by k_tl_ghosts before every
-// fine stencil pass).  Every expression is evaluated in the operand order of
+// BoxArray (phi with one ghost layer, filled by k_tl_ghosts from a list of
+// the FABs' face ghosts before every fine stencil pass; the Jacobi sweeps
+// write the other of two phi buffers).  Every expression is evaluated in the operand order of
 // the CPU oracle (oracle/ksgf_oracle.c, composite two-level section).
 #include <algorithm>
 #include <cstring>
@@ -43,7 +44,7 @@ constexpr double kOmega = 6.0 / 7.0;
 
 struct TLGeom {
     int n[3], nf[3], per[3], lobc[3], hibc[3];
-    double hc[3], hf[3], beta;
+    double hc[3], hf[3], ihc[3], ihf[3], beta;  // spacings and their inverses
 };
 
 __device__ __forceinline__ long long tl_ci(const TLGeom& T, int i, int j, int k)
@@ -134,14 +135,20 @@ __global__ void __launch_bounds__(256) k_tl_cover(const int* __restrict__ fown, 
 
 // the face-neighbour ghosts of every fine FAB (edges and corners are not
 // read by the 7-point stencil): another fine box's cell, a physical-boundary
-// ghost, or a coarse-fine ghost
+// ghost, or a coarse-fine ghost.  One thread per entry of the ghost list:
+// (fine box, linear index of the ghost in its FAB storage of ngrow 1).
 __global__ void __launch_bounds__(256) k_tl_ghosts(MFDev pf, const int* __restrict__ fbox,
+                                                   const int2* __restrict__ gl, long long ngl,
                                                    const int* __restrict__ fown, const double* __restrict__ pc,
-                                                   const unsigned char* __restrict__ cv, TLGeom T, RegDev r)
+                                                   const unsigned char* __restrict__ cv, TLGeom T)
 {
-    int b, i, j, k;
-    if (!region_locate(r, fbox, b, i, j, k)) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ngl) return;
+    const int2 ent = gl[t];
+    const int b = ent.x;
     const int* bx = fbox + 6 * b;
+    const int n0 = bx[3] - bx[0] + 3, n1 = bx[4] - bx[1] + 3;
+    const int i = bx[0] - 1 + ent.y % n0, j = bx[1] - 1 + (ent.y / n0) % n1, k = bx[2] - 1 + ent.y / (n0 * n1);
     const int q[3] = {i, j, k};
     int nout = 0, d = 0, s = 0;
 #pragma unroll
@@ -208,19 +215,35 @@ __global__ void __launch_bounds__(256) k_tl_fres(CMFDev pf, CMFDev bf, MFDev rf,
         int hi[3] = {c[0], c[1], c[2]}, lo[3] = {c[0], c[1], c[2]};
         hi[d] += 1;
         lo[d] -= 1;
-        const double Fp = T.beta * (p(hi[0], hi[1], hi[2]) - u0) / T.hf[d];
-        const double Fm = T.beta * (u0 - p(lo[0], lo[1], lo[2])) / T.hf[d];
-        acc += (Fp - Fm) / T.hf[d];
+        const double Fp = T.beta * (p(hi[0], hi[1], hi[2]) - u0) * T.ihf[d];
+        const double Fm = T.beta * (u0 - p(lo[0], lo[1], lo[2])) * T.ihf[d];
+        acc += (Fp - Fm) * T.ihf[d];
     }
     fab_at(rf, fbox, b)(i, j, k) = fab_at(bf, fbox, b)(i, j, k) - acc;
 }
 
-__global__ void __launch_bounds__(256) k_tl_jacobi(MFDev pf, CMFDev rf, const int* __restrict__ fbox, double w,
-                                                   RegDev r)
+// one damped-Jacobi sweep on the fine cells, pin (ghosts filled) to pout:
+// phi + w (b - div(beta grad phi)), the residual as k_tl_fres forms it
+__global__ void __launch_bounds__(256) k_tl_smooth(CMFDev pin, CMFDev bf, MFDev pout, const int* __restrict__ fbox,
+                                                   TLGeom T, double w, RegDev r)
 {
     int b, i, j, k;
     if (!region_locate(r, fbox, b, i, j, k)) return;
-    fab_at(pf, fbox, b)(i, j, k) += w * fab_at(rf, fbox, b)(i, j, k);
+    CFab p = fab_at(pin, fbox, b);
+    const double u0 = p(i, j, k);
+    const int c[3] = {i, j, k};
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int hi[3] = {c[0], c[1], c[2]}, lo[3] = {c[0], c[1], c[2]};
+        hi[d] += 1;
+        lo[d] -= 1;
+        const double Fp = T.beta * (p(hi[0], hi[1], hi[2]) - u0) * T.ihf[d];
+        const double Fm = T.beta * (u0 - p(lo[0], lo[1], lo[2])) * T.ihf[d];
+        acc += (Fp - Fm) * T.ihf[d];
+    }
+    const double R = fab_at(bf, fbox, b)(i, j, k) - acc;
+    fab_at(pout, fbox, b)(i, j, k) = u0 + w * R;
 }
 
 // fine flux beta dphi/dx_d through the face of fine cell f on side s (the
@@ -230,7 +253,7 @@ __device__ __forceinline__ double tl_fflux(const TLGeom& T, const CFab& p, const
     int g[3] = {f[0], f[1], f[2]};
     g[d] += s;
     const double u0 = p(f[0], f[1], f[2]), gv = p(g[0], g[1], g[2]);
-    return (s > 0) ? T.beta * (gv - u0) / T.hf[d] : T.beta * (u0 - gv) / T.hf[d];
+    return (s > 0) ? T.beta * (gv - u0) * T.ihf[d] : T.beta * (u0 - gv) * T.ihf[d];
 }
 
 // composite residual on the coarse cells: uncovered R = b - Lc (averaged fine
@@ -260,7 +283,7 @@ __global__ void __launch_bounds__(256) k_tl_cres(const double* __restrict__ pc, 
                 if (!T.per[d]) {
                     const int bcd = (s < 0) ? T.lobc[d] : T.hibc[d];
                     F[a] = (bcd == KSGF_LO_DIRICHLET)
-                               ? ((s > 0) ? T.beta * (-uc - uc) / T.hc[d] : T.beta * (uc + uc) / T.hc[d])
+                               ? ((s > 0) ? T.beta * (-uc - uc) * T.ihc[d] : T.beta * (uc + uc) * T.ihc[d])
                                : 0.0;
                     continue;
                 }
@@ -280,10 +303,10 @@ __global__ void __launch_bounds__(256) k_tl_cres(const double* __restrict__ pc, 
                 }
                 F[a] = 0.25 * ((ff[0] + ff[1]) + (ff[2] + ff[3]));
             } else {
-                F[a] = (s > 0) ? T.beta * (pc[nci] - uc) / T.hc[d] : T.beta * (uc - pc[nci]) / T.hc[d];
+                F[a] = (s > 0) ? T.beta * (pc[nci] - uc) * T.ihc[d] : T.beta * (uc - pc[nci]) * T.ihc[d];
             }
         }
-        acc += (F[1] - F[0]) / T.hc[d];
+        acc += (F[1] - F[0]) * T.ihc[d];
     }
     rc[t] = bc[t] - acc;
 }
@@ -364,7 +387,7 @@ __global__ void __launch_bounds__(256) k_tl_fix_f(MFDev u, CMFDev pf, const int*
     CFab p = fab_at(pf, box, b);
     const double hi = p(i, j, k);
     const double lo = p(i - (D == 0), j - (D == 1), k - (D == 2));
-    fab_at(u, box, b)(i, j, k) -= T.beta * (hi - lo) / T.hf[D];
+    fab_at(u, box, b)(i, j, k) -= T.beta * (hi - lo) * T.ihf[D];
 }
 template <int D>
 __global__ void __launch_bounds__(256) k_tl_fix_c(MFDev u, const double* __restrict__ pc, const int* __restrict__ box,
@@ -380,7 +403,7 @@ __global__ void __launch_bounds__(256) k_tl_fix_c(MFDev u, const double* __restr
             const bool low = lo[D] < 0;
             const double u0 = low ? pc[tl_ci(T, hi[0], hi[1], hi[2])] : pc[tl_ci(T, lo[0], lo[1], lo[2])];
             const int bcd = low ? T.lobc[D] : T.hibc[D];
-            F = (bcd == KSGF_LO_DIRICHLET) ? (low ? T.beta * (u0 + u0) / T.hc[D] : T.beta * (-u0 - u0) / T.hc[D])
+            F = (bcd == KSGF_LO_DIRICHLET) ? (low ? T.beta * (u0 + u0) * T.ihc[D] : T.beta * (-u0 - u0) * T.ihc[D])
                                            : 0.0;
             fab_at(u, box, b)(i, j, k) -= F;
             return;
@@ -388,7 +411,7 @@ __global__ void __launch_bounds__(256) k_tl_fix_c(MFDev u, const double* __restr
         if (lo[D] < 0) lo[D] += T.n[D];
         if (hi[D] >= T.n[D]) hi[D] -= T.n[D];
     }
-    F = T.beta * (pc[tl_ci(T, hi[0], hi[1], hi[2])] - pc[tl_ci(T, lo[0], lo[1], lo[2])]) / T.hc[D];
+    F = T.beta * (pc[tl_ci(T, hi[0], hi[1], hi[2])] - pc[tl_ci(T, lo[0], lo[1], lo[2])]) * T.ihc[D];
     fab_at(u, box, b)(i, j, k) -= F;
 }
 
@@ -431,13 +454,17 @@ __global__ void __launch_bounds__(256) k_tl_sum_c(const double* __restrict__ a, 
     }
     if (threadIdx.x == 0) atomicAdd(out, sm[0]);
 }
-__global__ void __launch_bounds__(256) k_tl_sum_f(CMFDev f, const int* __restrict__ box, RegDev r,
+__global__ void __launch_bounds__(256) k_tl_sum_f(CMFDev f, const int* __restrict__ box, RegDev r, int nchunks,
                                                   double* __restrict__ out)
 {
     __shared__ double sm[256];
-    int b, i, j, k;
     double s = 0.0;
-    if (region_locate(r, box, b, i, j, k)) s = fab_at(f, box, b)(i, j, k);
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const int b = r.chunks[2 * c];
+        int i, j, k;
+        if (decode_in_box(r, box, b, (unsigned)r.chunks[2 * c + 1] + threadIdx.x, i, j, k))
+            s += fab_at(f, box, b)(i, j, k);
+    }
     sm[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -477,7 +504,11 @@ struct ksgf_macproj_levels {
     const int* fown = nullptr;     // fine owner map (owned by the fine BoxArray)
     unsigned char* cv = nullptr;
     double *pc = nullptr, *bc = nullptr, *rc = nullptr, *ec = nullptr;
-    double *pf = nullptr, *bf = nullptr, *rf = nullptr;
+    double* pf[2] = {nullptr, nullptr};  // fine phi (ngrow 1), ping-pong of the Jacobi sweeps
+    int cur = 0;                         // the buffer holding the current iterate
+    double *bf = nullptr, *rf = nullptr;
+    int2* gl = nullptr;  // face-ghost list of the fine FABs
+    long long ngl = 0;
     long long nc = 0, nfg = 0, nf0 = 0;
     long long ncov = 0, nfine = 0;
     unsigned long long* dmax = nullptr;  // 2 slots
@@ -490,8 +521,11 @@ namespace {
 int tl_mfs(ksgf_macproj_levels* m, MFDev* pf, CMFDev* cpf, CMFDev* bf, MFDev* rf, CMFDev* crf)
 {
     int e;
-    if ((e = mfdev(m->fba, m->pf, KSGF_CELL, 1, 1, pf)) || (e = cmfdev(m->fba, m->pf, KSGF_CELL, 1, 1, cpf)) ||
-        (e = cmfdev(m->fba, m->bf, KSGF_CELL, 1, 0, bf)) || (e = mfdev(m->fba, m->rf, KSGF_CELL, 1, 0, rf)) ||
+    for (int a = 0; a < 2; ++a)
+        if ((e = mfdev(m->fba, m->pf[a], KSGF_CELL, 1, 1, &pf[a])) ||
+            (e = cmfdev(m->fba, m->pf[a], KSGF_CELL, 1, 1, &cpf[a])))
+            return e;
+    if ((e = cmfdev(m->fba, m->bf, KSGF_CELL, 1, 0, bf)) || (e = mfdev(m->fba, m->rf, KSGF_CELL, 1, 0, rf)) ||
         (e = cmfdev(m->fba, m->rf, KSGF_CELL, 1, 0, crf)))
         return e;
     return KSGF_OK;
@@ -500,25 +534,30 @@ int tl_mfs(ksgf_macproj_levels* m, MFDev* pf, CMFDev* cpf, CMFDev* bf, MFDev* rf
 struct TLRun {
     ksgf_macproj_levels* m;
     hipStream_t st;
-    MFDev pf;
-    CMFDev cpf, bf, crf;
+    MFDev pfm[2];
+    CMFDev cpfm[2], bf, crf;
     MFDev rf;
-    RegDev rg, rv;  // fine FAB storage (ng 1), fine valid cells
-    int ng_chunks, nv_chunks;
+    RegDev rv;  // fine valid cells
+    int nv_chunks;
 
+    const MFDev& pf() const { return pfm[m->cur]; }
+    const CMFDev& cpf() const { return cpfm[m->cur]; }
     void ghosts()
     {
-        hipLaunchKernelGGL(k_tl_ghosts, dim3(ng_chunks), dim3(256), 0, st, pf, m->fba->dbox, m->fown,
-                           (const double*)m->pc, (const unsigned char*)m->cv, m->T, rg);
+        if (m->ngl == 0) return;
+        hipLaunchKernelGGL(k_tl_ghosts, dim3(grid_for(m->ngl, 256)), dim3(256), 0, st, pf(), m->fba->dbox,
+                           (const int2*)m->gl, m->ngl, m->fown, (const double*)m->pc, (const unsigned char*)m->cv,
+                           m->T);
     }
     void fres()
     {
-        hipLaunchKernelGGL(k_tl_fres, dim3(nv_chunks), dim3(256), 0, st, cpf, bf, rf, m->fba->dbox, m->T, rv);
+        hipLaunchKernelGGL(k_tl_fres, dim3(nv_chunks), dim3(256), 0, st, cpf(), bf, rf, m->fba->dbox, m->T, rv);
     }
     void cres()
     {
         hipLaunchKernelGGL(k_tl_cres, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, (const double*)m->pc,
-                           (const double*)m->bc, m->rc, (const unsigned char*)m->cv, cpf, m->fba->dbox, m->fown, m->T);
+                           (const double*)m->bc, m->rc, (const unsigned char*)m->cv, cpf(), m->fba->dbox, m->fown,
+                           m->T);
     }
     // composite residual on both levels
     void residual()
@@ -531,8 +570,9 @@ struct TLRun {
     {
         for (int s = 0; s < nsweep; ++s) {
             ghosts();
-            fres();
-            hipLaunchKernelGGL(k_tl_jacobi, dim3(nv_chunks), dim3(256), 0, st, pf, crf, m->fba->dbox, w, rv);
+            hipLaunchKernelGGL(k_tl_smooth, dim3(nv_chunks), dim3(256), 0, st, cpf(), bf, pfm[1 - m->cur],
+                               m->fba->dbox, m->T, w, rv);
+            m->cur ^= 1;
         }
     }
     int norm(double* out)
@@ -558,7 +598,8 @@ struct TLRun {
         KSGF_HIP_CHECK(hipMemsetAsync(m->dsum, 0, 3 * sizeof(double), st));
         hipLaunchKernelGGL(k_tl_sum_c, dim3(std::min(1024, grid_for(m->nc, 256))), dim3(256), 0, st, (const double*)a,
                            (const unsigned char*)m->cv, m->nc, m->dsum);
-        hipLaunchKernelGGL(k_tl_sum_f, dim3(nv_chunks), dim3(256), 0, st, cf, m->fba->dbox, rv, m->dsum + 1);
+        hipLaunchKernelGGL(k_tl_sum_f, dim3(std::min(nv_chunks, 2048)), dim3(256), 0, st, cf, m->fba->dbox, rv,
+                           nv_chunks, m->dsum + 1);
         const double den = 8.0 * (double)(m->nc - m->ncov) + (double)m->nfine;
         hipLaunchKernelGGL(k_tl_mean, dim3(1), dim3(1), 0, st, m->dsum, 8.0, 1.0, den);
         hipLaunchKernelGGL(k_tl_shift_c, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, a, (const unsigned char*)m->cv,
@@ -621,6 +662,8 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
         T.hibc[d] = hibc[d];
         T.hc[d] = crse_ba->dx[d];
         T.hf[d] = 0.5 * crse_ba->dx[d];
+        T.ihc[d] = 1.0 / T.hc[d];
+        T.ihf[d] = 1.0 / T.hf[d];
         if (lobc[d] == KSGF_LO_DIRICHLET || hibc[d] == KSGF_LO_DIRICHLET) m->singular = 0;
     }
     T.beta = beta;
@@ -654,7 +697,8 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
         hipMalloc(&m->bc, m->nc * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->rc, m->nc * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->ec, m->nc * sizeof(double)) != hipSuccess ||
-        hipMalloc(&m->pf, m->nfg * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->pf[0], m->nfg * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->pf[1], m->nfg * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->bf, m->nf0 * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->rf, m->nf0 * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->dmax, 2 * sizeof(unsigned long long)) != hipSuccess ||
@@ -664,6 +708,30 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
         return fail(KSGF_ERR_HIP);
     }
     if ((e = owner_map(fine_ba, nullptr, &m->fown))) return fail(e);
+    // face-ghost list: for every fine box, the six one-cell slabs outside its
+    // faces (edges and corners excluded), x fastest within a slab
+    {
+        std::vector<int2> h;
+        for (int b = 0; b < fine_ba->nbox; ++b) {
+            const int* x = &fine_ba->hbox[6 * b];
+            const int n0 = x[3] - x[0] + 3, n1 = x[4] - x[1] + 3;
+            for (int d = 0; d < 3; ++d)
+                for (int side = 0; side < 2; ++side) {
+                    int lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[3], x[4], x[5]};
+                    lo[d] = hi[d] = side ? x[3 + d] + 1 : x[d] - 1;
+                    for (int k = lo[2]; k <= hi[2]; ++k)
+                        for (int j = lo[1]; j <= hi[1]; ++j)
+                            for (int i = lo[0]; i <= hi[0]; ++i)
+                                h.push_back(make_int2(b, (i - x[0] + 1) + n0 * ((j - x[1] + 1) + n1 * (k - x[2] + 1))));
+                }
+        }
+        m->ngl = (long long)h.size();
+        if (m->ngl && (hipMalloc(&m->gl, h.size() * sizeof(int2)) != hipSuccess ||
+                       hipMemcpy(m->gl, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)) {
+            set_error("ksgf_macproj_levels_create: ghost list");
+            return fail(KSGF_ERR_HIP);
+        }
+    }
     hipLaunchKernelGGL(k_tl_cover, dim3(grid_for(m->nc, 256)), dim3(256), 0, nullptr, m->fown, m->cv, T);
     if (hipDeviceSynchronize() != hipSuccess) {
         set_error("ksgf_macproj_levels_create: covered mask");
@@ -692,15 +760,12 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     TLRun R;
     R.m = m;
     R.st = st;
-    if ((e = tl_mfs(m, &R.pf, &R.cpf, &R.bf, &R.rf, &R.crf))) return e;
-    const Region* rg = fab_region(fba, KSGF_CELL, 1, &err);
+    if ((e = tl_mfs(m, R.pfm, R.cpfm, &R.bf, &R.rf, &R.crf))) return e;
     const int z[3] = {0, 0, 0};
     const Region* rv = get_region(fba, z, z, &err);
     const Region* rcv = get_region(cba, z, z, &err);
-    if (!rg || !rv || !rcv) return err;
-    R.rg = regdev(rg, fba->nbox);
+    if (!rv || !rcv) return err;
     R.rv = regdev(rv, fba->nbox);
-    R.ng_chunks = rg->nchunks;
     R.nv_chunks = rv->nchunks;
     MFDev uf[3], uc[3];
     CMFDev cuf[3], cuc[3];
@@ -724,7 +789,8 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     KSGF_LAUNCH_CHECK();
     if (m->singular && (e = R.sub_mean(m->bc, MFDev{m->bf, R.bf.pre, KSGF_CELL, 0, 1}, R.bf, 0))) return e;
     KSGF_HIP_CHECK(hipMemsetAsync(m->pc, 0, m->nc * sizeof(double), st));
-    KSGF_HIP_CHECK(hipMemsetAsync(m->pf, 0, m->nfg * sizeof(double), st));
+    m->cur = 0;
+    KSGF_HIP_CHECK(hipMemsetAsync(m->pf[0], 0, m->nfg * sizeof(double), st));
     const double diag = -2.0 * T.beta * ((1.0 / (T.hf[0] * T.hf[0]) + 1.0 / (T.hf[1] * T.hf[1])) +
                                          1.0 / (T.hf[2] * T.hf[2]));
     const double w = kOmega / diag;
@@ -749,8 +815,8 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
         if ((e = macproj_vcycles(m->crse, m->rc, m->ec, &ci, st))) return e;
         inf.bottom_iters += ci.bottom_iters;
         hipLaunchKernelGGL(k_tl_addc, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const double*)m->ec, m->nc);
-        hipLaunchKernelGGL(k_tl_prolong, dim3(R.nv_chunks), dim3(256), 0, st, R.pf, (const double*)m->ec, fba->dbox, T,
-                           R.rv);
+        hipLaunchKernelGGL(k_tl_prolong, dim3(R.nv_chunks), dim3(256), 0, st, R.pf(), (const double*)m->ec, fba->dbox,
+                           T, R.rv);
         R.smooth(m->opt.num_post_smooth, w);
         R.residual();
         KSGF_LAUNCH_CHECK();
@@ -772,9 +838,9 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     // covered coarse cells: the average of their children; singular: the
     // composite mean removed
     hipLaunchKernelGGL(k_tl_avg8, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const unsigned char*)m->cv,
-                       CMFDev{m->pf, R.cpf.pre, KSGF_CELL, 1, 1}, fba->dbox, m->fown, T);
+                       R.cpf(), fba->dbox, m->fown, T);
     KSGF_LAUNCH_CHECK();
-    if (m->singular && (e = R.sub_mean(m->pc, R.pf, R.cpf, 1))) return e;
+    if (m->singular && (e = R.sub_mean(m->pc, R.pf(), R.cpf(), 1))) return e;
     // u -= beta grad phi on both levels, then the fine average on the coarse
     // faces under and around the fine level
     R.ghosts();
@@ -787,17 +853,17 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
         RegDev rf = regdev(ff, fba->nbox), rc = regdev(fc, cba->nbox);
         switch (d) {
         case 0:
-            hipLaunchKernelGGL(k_tl_fix_f<0>, dim3(ff->nchunks), dim3(256), 0, st, uf[0], R.cpf, fba->dbox, T, rf);
+            hipLaunchKernelGGL(k_tl_fix_f<0>, dim3(ff->nchunks), dim3(256), 0, st, uf[0], R.cpf(), fba->dbox, T, rf);
             hipLaunchKernelGGL(k_tl_fix_c<0>, dim3(fc->nchunks), dim3(256), 0, st, uc[0], (const double*)m->pc,
                                cba->dbox, T, rc);
             break;
         case 1:
-            hipLaunchKernelGGL(k_tl_fix_f<1>, dim3(ff->nchunks), dim3(256), 0, st, uf[1], R.cpf, fba->dbox, T, rf);
+            hipLaunchKernelGGL(k_tl_fix_f<1>, dim3(ff->nchunks), dim3(256), 0, st, uf[1], R.cpf(), fba->dbox, T, rf);
             hipLaunchKernelGGL(k_tl_fix_c<1>, dim3(fc->nchunks), dim3(256), 0, st, uc[1], (const double*)m->pc,
                                cba->dbox, T, rc);
             break;
         default:
-            hipLaunchKernelGGL(k_tl_fix_f<2>, dim3(ff->nchunks), dim3(256), 0, st, uf[2], R.cpf, fba->dbox, T, rf);
+            hipLaunchKernelGGL(k_tl_fix_f<2>, dim3(ff->nchunks), dim3(256), 0, st, uf[2], R.cpf(), fba->dbox, T, rf);
             hipLaunchKernelGGL(k_tl_fix_c<2>, dim3(fc->nchunks), dim3(256), 0, st, uc[2], (const double*)m->pc,
                                cba->dbox, T, rc);
             break;
@@ -818,7 +884,7 @@ int ksgf_macproj_levels_get_phi(ksgf_macproj_levels* m, double* phi_c, double* p
     }
     hipStream_t st = (hipStream_t)stream;
     KSGF_HIP_CHECK(hipMemcpyAsync(phi_c, m->pc, m->nc * sizeof(double), hipMemcpyDeviceToDevice, st));
-    KSGF_HIP_CHECK(hipMemcpyAsync(phi_f, m->pf, m->nfg * sizeof(double), hipMemcpyDeviceToDevice, st));
+    KSGF_HIP_CHECK(hipMemcpyAsync(phi_f, m->pf[m->cur], m->nfg * sizeof(double), hipMemcpyDeviceToDevice, st));
     return KSGF_OK;
 }
 
@@ -831,7 +897,9 @@ int ksgf_macproj_levels_destroy(ksgf_macproj_levels* m)
     hipFree(m->bc);
     hipFree(m->rc);
     hipFree(m->ec);
-    hipFree(m->pf);
+    hipFree(m->pf[0]);
+    hipFree(m->pf[1]);
+    hipFree(m->gl);
     hipFree(m->bf);
     hipFree(m->rf);
     hipFree(m->dmax);

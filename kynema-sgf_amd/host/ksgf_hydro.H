@@ -72,14 +72,20 @@ using BCRec = ksgf_bcrec;
 // BoxArray bound to its Geometry (the library's device box tables live with it)
 class BoxArray {
 public:
-    BoxArray(const Geometry& geom, std::vector<Box> boxes) : geom_(geom), boxes_(std::move(boxes))
+    // covers_domain = false: a refined AMR level (its boxes cover part of
+    // the domain)
+    BoxArray(const Geometry& geom, std::vector<Box> boxes, bool covers_domain = true)
+        : geom_(geom), boxes_(std::move(boxes))
     {
         std::vector<int> flat;
         for (const auto& b : boxes_)
             for (int v : {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]}) flat.push_back(v);
         ksgf_geometry g = geom_.c();
         ksgf_boxarray* h = nullptr;
-        check(ksgf_boxarray_create(&g, flat.data(), (int)boxes_.size(), &h), "BoxArray");
+        if (covers_domain)
+            check(ksgf_boxarray_create(&g, flat.data(), (int)boxes_.size(), &h), "BoxArray");
+        else
+            check(ksgf_boxarray_create_level(&g, flat.data(), (int)boxes_.size(), 0, &h), "BoxArray");
         h_ = std::shared_ptr<ksgf_boxarray>(h, [](ksgf_boxarray* p) { ksgf_boxarray_destroy(p); });
     }
     int size() const { return (int)boxes_.size(); }
@@ -326,17 +332,22 @@ struct MLMG {
 
 class MacProjector {
 public:
+    // one level, or two (level 1 at ratio 2: the composite solve)
     explicit MacProjector(const std::vector<ksgf::amr::Geometry>& geom) : geom_(geom)
     {
-        if (geom_.size() != 1) throw std::runtime_error("MacProjector: one level on this path");
+        if (geom_.size() != 1 && geom_.size() != 2)
+            throw std::runtime_error("MacProjector: one or two levels on this path");
     }
     ~MacProjector()
     {
         if (mp_) ksgf_macproj_destroy(mp_);
+        if (mpl_) ksgf_macproj_levels_destroy(mpl_);
     }
     void initProjector(const std::vector<const ksgf::amr::BoxArray*>& ba, const LPInfo& info, double beta)
     {
+        if (ba.size() != geom_.size()) throw std::runtime_error("MacProjector: one BoxArray per level");
         ba_ = ba.at(0);
+        fba_ = (ba.size() > 1) ? ba[1] : nullptr;
         beta_ = beta;
         mlmg_.opts.do_semicoarsening = info.do_semicoarsening ? 1 : 0;
         mlmg_.opts.max_coarsening_level = info.max_coarsening_level;
@@ -353,13 +364,29 @@ public:
     {
         beta_ = beta;
         if (mp_) ksgf::amr::check(ksgf_macproj_set_beta(mp_, beta), "updateBeta");
+        if (mpl_) build();
     }
-    void setUMAC(const std::vector<std::array<ksgf::amr::MultiFab*, 3>>& umac) { umac_ = umac.at(0); }
+    void setUMAC(const std::vector<std::array<ksgf::amr::MultiFab*, 3>>& umac)
+    {
+        if (umac.size() != geom_.size()) throw std::runtime_error("MacProjector::setUMAC: one set per level");
+        umac_ = umac.at(0);
+        if (umac.size() > 1) umacf_ = umac[1];
+    }
     void project(double rtol, double atol, hipStream_t st = nullptr)
     {
-        if (!mp_) build();
-        int rc = ksgf_macproj_project(mp_, umac_[0]->data(), umac_[1]->data(), umac_[2]->data(), rtol, atol,
+        if (!mp_ && !mpl_) build();
+        int rc;
+        if (mpl_) {
+            double* c[3] = {umac_[0]->data(), umac_[1]->data(), umac_[2]->data()};
+            double* f[3] = {umacf_[0]->data(), umacf_[1]->data(), umacf_[2]->data()};
+            if (umacf_[0]->nGrow() != umac_[0]->nGrow())
+                throw std::runtime_error("MacProjector::project: both levels' face MultiFabs need one ngrow");
+            rc = ksgf_macproj_levels_project(mpl_, c, f, umac_[0]->nGrow(), rtol, atol, &mlmg_.last,
+                                             (ksgf_stream_t)st);
+        } else {
+            rc = ksgf_macproj_project(mp_, umac_[0]->data(), umac_[1]->data(), umac_[2]->data(), rtol, atol,
                                       &mlmg_.last, (ksgf_stream_t)st);
+        }
         ksgf::amr::check(rc, "MacProjector::project");  // KSGF_ERR_NOT_CONVERGED (> 0) is not an abort
     }
     MLMG& getMLMG() { return mlmg_; }
@@ -368,8 +395,18 @@ public:
     // project(Vector<MultiFab*> phi, ...) hands it back
     void getPhi(double* phi, hipStream_t st = nullptr)
     {
+        if (mpl_) throw std::runtime_error("MacProjector::getPhi: two levels hand back phi per level (getPhiLevels)");
         if (!mp_) throw std::runtime_error("MacProjector::getPhi: no projection yet");
         ksgf::amr::check(ksgf_macproj_get_phi(mp_, phi, (ksgf_stream_t)st), "MacProjector::getPhi");
+    }
+
+    // two levels: phi_c domain-wide on level 0 (n cells, device), phi_f a
+    // level-1 cell MultiFab with one ghost layer
+    void getPhiLevels(double* phi_c, ksgf::amr::MultiFab& phi_f, hipStream_t st = nullptr)
+    {
+        if (!mpl_) throw std::runtime_error("MacProjector::getPhiLevels: no two-level projection yet");
+        ksgf::amr::check(ksgf_macproj_levels_get_phi(mpl_, phi_c, phi_f.data(), phi_f.nGrow(), (ksgf_stream_t)st),
+                         "MacProjector::getPhiLevels");
     }
 
 private:
@@ -377,16 +414,25 @@ private:
     {
         if (!ba_) throw std::runtime_error("MacProjector: initProjector first");
         if (mp_) ksgf_macproj_destroy(mp_);
+        if (mpl_) ksgf_macproj_levels_destroy(mpl_);
         mp_ = nullptr;
-        ksgf::amr::check(ksgf_macproj_create(ba_->handle(), lo_, hi_, beta_, &mlmg_.opts, &mp_), "MacProjector");
+        mpl_ = nullptr;
+        if (fba_)
+            ksgf::amr::check(ksgf_macproj_levels_create(ba_->handle(), fba_->handle(), lo_, hi_, beta_, &mlmg_.opts,
+                                                        &mpl_),
+                             "MacProjector");
+        else
+            ksgf::amr::check(ksgf_macproj_create(ba_->handle(), lo_, hi_, beta_, &mlmg_.opts, &mp_), "MacProjector");
     }
     std::vector<ksgf::amr::Geometry> geom_;
     const ksgf::amr::BoxArray* ba_ = nullptr;
+    const ksgf::amr::BoxArray* fba_ = nullptr;
     double beta_ = 1.0;
     int lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
-    std::array<ksgf::amr::MultiFab*, 3> umac_{};
+    std::array<ksgf::amr::MultiFab*, 3> umac_{}, umacf_{};
     MLMG mlmg_;
     ksgf_macproj* mp_ = nullptr;
+    ksgf_macproj_levels* mpl_ = nullptr;
 };
 
 // amrex::MLABecLaplacian with constant coefficients and its MLMG, one

@@ -130,6 +130,16 @@ class BoxArray:
         return out
 
 
+class LevelBoxArray(BoxArray):
+    """The BoxArray of a refined AMR level: disjoint boxes that cover part of
+    the domain (no tiling check)."""
+
+    def __init__(self, geom: Geometry, boxes):
+        self.geom = geom
+        self.boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self.nbox = self.boxes.shape[0]
+
+
 def chop_uniform(n_cell, max_grid_size):
     """amr.max_grid_size chopping: equal-as-possible boxes no larger than
     max_grid_size along each axis."""

@@ -16,7 +16,7 @@ import dataclasses
 import numpy as np
 
 from .layout import (BC_EXT_DIR, BC_HOEXTRAP, BC_INT_DIR, LO_NEUMANN, LO_PERIODIC, BoxArray, Geometry,
-                     chop_bricks, chop_random, chop_uniform)
+                     LevelBoxArray, chop_bricks, chop_random, chop_uniform)
 
 
 @dataclasses.dataclass
@@ -106,6 +106,40 @@ def ragged(n_cell=(320, 320, 480), nbox=1500, edges=(8, 16, 24, 32, 48, 64), box
     nx, ny, nz = n_cell
     return Case(f"ragged{ba.nbox}_{nx}x{ny}x{nz}", geom, ba, np.ascontiguousarray(vel), np.ones((1, nz, ny, nx)),
                 np.zeros((3, nz, ny, nx)), bc, bcval, lobc, hibc, cfl, nsteps)
+
+
+def two_level_faces(geom, xp, X, Y, Z, d, amp=0.1):
+    """Face velocity of the two-level workload at positions (X, Y, Z): the
+    Taylor-Green field with wavenumbers 2 pi / L per axis (periodic on the
+    domain) plus amp * grad(cos cos cos), a divergent part for the
+    projection to remove.  xp: numpy or torch."""
+    k = [2.0 * np.pi / (geom.prob_hi[e] - geom.prob_lo[e]) for e in range(3)]
+    sx, cx = xp.sin(k[0] * X), xp.cos(k[0] * X)
+    sy, cy = xp.sin(k[1] * Y), xp.cos(k[1] * Y)
+    sz, cz = xp.sin(k[2] * Z), xp.cos(k[2] * Z)
+    if d == 0:
+        return sx * cy * cz - amp * k[0] * sx * cy * cz
+    if d == 1:
+        return -cx * sy * cz - amp * k[1] * cx * sy * cz
+    return -amp * k[2] * cx * cy * sz
+
+
+def ragged_two_level(n_cell=(320, 320, 480), nbox=1500, fine_z=(96, 288), fine_box=64, box_seed=4):
+    """The ragged layout (ragged()) as AMR level 0 and a level-1 BoxArray at
+    ratio 2 over the coarse slab z in [fine_z[0], fine_z[1]) (40% of the
+    domain by default), chopped into fine boxes of up to fine_box cells per
+    edge (aligned to 2).  Returns (case, fine LevelBoxArray)."""
+    case = ragged(n_cell, nbox, box_seed=box_seed, noise=0.0)
+    nf = tuple(2 * v for v in n_cell)
+    fg = Geometry(nf, (0.0, 0.0, 0.0), case.geom.prob_hi, (1, 1, 1))
+    lo = (0, 0, 2 * fine_z[0])
+    hi = (nf[0], nf[1], 2 * fine_z[1])
+    edges = [list(range(lo[d], hi[d], fine_box)) + [hi[d]] for d in range(3)]
+    boxes = [[x0, y0, z0, x1 - 1, y1 - 1, z1 - 1]
+             for z0, z1 in zip(edges[2][:-1], edges[2][1:])
+             for y0, y1 in zip(edges[1][:-1], edges[1][1:])
+             for x0, x1 in zip(edges[0][:-1], edges[0][1:])]
+    return case, LevelBoxArray(fg, boxes)
 
 
 def abl(n_cell=(256, 256, 128), extent=(5120.0, 5120.0, 1024.0), sizes=(32, 48, 64), cfl=0.9, nsteps=10,

@@ -1783,7 +1783,7 @@ void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_g
 /* ------------------------------------------------------------------ */
 typedef struct {
     int n[3], nf[3], per[3], lobc[3], hibc[3];
-    double hc[3], hf[3], beta;
+    double hc[3], hf[3], ihc[3], ihf[3], beta;  /* spacings and their inverses */
     unsigned char* fm;  /* fine cell covered by a fine box */
     unsigned char* cv;  /* coarse cell covered */
 } tl_t;
@@ -1802,6 +1802,8 @@ static void tl_init(tl_t* T, const int* n, const double* dxc, const int* lobc, c
         T->hibc[d] = hibc[d];
         T->hc[d] = dxc[d];
         T->hf[d] = 0.5 * dxc[d];
+        T->ihc[d] = 1.0 / T->hc[d];
+        T->ihf[d] = 1.0 / T->hf[d];
     }
     T->beta = beta;
     const long nc = (long)n[0] * n[1] * n[2];
@@ -1919,7 +1921,7 @@ static double tl_fnb(const tl_t* T, const double* pc, const double* pf, const in
 static double tl_fflux(const tl_t* T, const double* pc, const double* pf, const int* i, int d, int s)
 {
     const double u0 = pf[TLF(T, i[0], i[1], i[2])], g = tl_fnb(T, pc, pf, i, d, s);
-    return (s > 0) ? T->beta * (g - u0) / T->hf[d] : T->beta * (u0 - g) / T->hf[d];
+    return (s > 0) ? T->beta * (g - u0) * T->ihf[d] : T->beta * (u0 - g) * T->ihf[d];
 }
 
 /* composite operator div(beta grad phi): Lf on fine cells, Lc on uncovered
@@ -1935,7 +1937,7 @@ static void tl_apply(const tl_t* T, const double* pc, const double* pf, double* 
                 const int c[3] = {i, j, k};
                 double acc = 0.0;
                 for (int d = 0; d < 3; ++d)
-                    acc += (tl_fflux(T, pc, pf, c, d, 1) - tl_fflux(T, pc, pf, c, d, -1)) / T->hf[d];
+                    acc += (tl_fflux(T, pc, pf, c, d, 1) - tl_fflux(T, pc, pf, c, d, -1)) * T->ihf[d];
                 Lf[f] = acc;
             }
 #pragma omp parallel for schedule(static)
@@ -1956,8 +1958,8 @@ static void tl_apply(const tl_t* T, const double* pc, const double* pf, double* 
                         if (N[d] < 0 || N[d] >= T->n[d]) {
                             if (!T->per[d]) {
                                 const int bc = (s < 0) ? T->lobc[d] : T->hibc[d];
-                                F[a] = (bc == KO_LO_DIRICHLET) ? ((s > 0) ? T->beta * (-uc - uc) / T->hc[d]
-                                                                          : T->beta * (uc + uc) / T->hc[d])
+                                F[a] = (bc == KO_LO_DIRICHLET) ? ((s > 0) ? T->beta * (-uc - uc) * T->ihc[d]
+                                                                          : T->beta * (uc + uc) * T->ihc[d])
                                                                : 0.0;
                                 continue;
                             }
@@ -1977,10 +1979,10 @@ static void tl_apply(const tl_t* T, const double* pc, const double* pf, double* 
                             }
                             F[a] = 0.25 * ((ff[0] + ff[1]) + (ff[2] + ff[3]));
                         } else {
-                            F[a] = (s > 0) ? T->beta * (pc[nc] - uc) / T->hc[d] : T->beta * (uc - pc[nc]) / T->hc[d];
+                            F[a] = (s > 0) ? T->beta * (pc[nc] - uc) * T->ihc[d] : T->beta * (uc - pc[nc]) * T->ihc[d];
                         }
                     }
-                    acc += (F[1] - F[0]) / T->hc[d];
+                    acc += (F[1] - F[0]) * T->ihc[d];
                 }
                 Lc[c] = acc;
             }
@@ -2235,8 +2237,8 @@ int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, 
                             const long in = (lo[d] < 0) ? TLC(&T, hi[0], hi[1], hi[2]) : TLC(&T, lo[0], lo[1], lo[2]);
                             const int bcd = (lo[d] < 0) ? lobc[d] : hibc[d];
                             const double u0 = pc[in];
-                            F = (bcd == KO_LO_DIRICHLET) ? ((lo[d] < 0) ? beta * (u0 + u0) / T.hc[d]
-                                                                        : beta * (-u0 - u0) / T.hc[d])
+                            F = (bcd == KO_LO_DIRICHLET) ? ((lo[d] < 0) ? beta * (u0 + u0) * T.ihc[d]
+                                                                        : beta * (-u0 - u0) * T.ihc[d])
                                                          : 0.0;
                             umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] -= F;
                             continue;
@@ -2244,7 +2246,7 @@ int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, 
                         if (lo[d] < 0) lo[d] += n[d];
                         if (hi[d] >= n[d]) hi[d] -= n[d];
                     }
-                    F = beta * (pc[TLC(&T, hi[0], hi[1], hi[2])] - pc[TLC(&T, lo[0], lo[1], lo[2])]) / T.hc[d];
+                    F = beta * (pc[TLC(&T, hi[0], hi[1], hi[2])] - pc[TLC(&T, lo[0], lo[1], lo[2])]) * T.ihc[d];
                     umc[d][i + (long)sc[0] * (j + (long)sc[1] * k)] -= F;
                 }
     }
