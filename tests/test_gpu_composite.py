@@ -102,3 +102,57 @@ def test_macproj_levels_matches_oracle(ksgf, oracle, gpu, name, lobc, hibc, beta
     sp = np.abs(rpc).max()
     assert np.abs(pc - rpc).max() <= 1e-9 * sp
     assert np.abs(gpf[fm] - rpf.reshape(nf[::-1])[fm]).max() <= 1e-9 * sp
+
+
+def test_macproj_levels_rejects_bad_levels(ksgf, gpu):
+    """boxes not aligned to the ratio, and a fine domain that is not the coarse
+    one refined by 2, are errors at creation (before any solve)"""
+    import test_multilevel_oracle as T
+
+    _say("macproj_levels bad levels")
+    n = (16, 12, 16)
+    cg = lay.Geometry(n, prob_hi=(1.0, 0.75, 1.0), periodic=(1, 0, 1))
+    cba = lay.BoxArray(cg, [[0, 0, 0, 15, 11, 15]])
+    dc = ksgf.DeviceBoxArray(cba)
+    fg = lay.Geometry((32, 24, 32), prob_hi=cg.prob_hi, periodic=(1, 0, 1))
+    bad_align = ksgf.DeviceBoxArray(T.FineBoxArray(fg, [[1, 4, 8, 8, 11, 15]]), covers_domain=False)
+    with pytest.raises(ksgf.KsgfError, match="aligned"):
+        ksgf.MacProjectorLevels(dc, bad_align, (P_, N_, P_), (P_, N_, P_))
+    fg3 = lay.Geometry((48, 36, 48), prob_hi=cg.prob_hi, periodic=(1, 0, 1))
+    bad_ratio = ksgf.DeviceBoxArray(T.FineBoxArray(fg3, [[0, 0, 0, 11, 11, 11]]), covers_domain=False)
+    with pytest.raises(ksgf.KsgfError, match="refined by 2"):
+        ksgf.MacProjectorLevels(dc, bad_ratio, (P_, N_, P_), (P_, N_, P_))
+
+
+def test_macproj_levels_maxiter_reports_not_converged(ksgf, oracle, gpu):
+    """maxiter 2: the projection corrects with the iterate it has and reports
+    KSGF_ERR_NOT_CONVERGED (a status, not an abort), as MLMG does at maxiter"""
+    import test_multilevel_oracle as T
+
+    _say("macproj_levels maxiter 2")
+    rng = np.random.default_rng(5)
+    n = (16, 12, 16)
+    cg = lay.Geometry(n, prob_hi=(1.0, 0.75, 1.0), periodic=(1, 0, 1))
+    cba = lay.BoxArray(cg, lay.chop_random(n, [4, 8], rng, granularity=4))
+    nf = tuple(2 * v for v in n)
+    fg = lay.Geometry(nf, prob_hi=cg.prob_hi, periodic=(1, 0, 1))
+    fboxes = [[8, 4, 8, 23, 19, 23]]
+    fba = T.FineBoxArray(fg, fboxes)
+    ucd = _random_faces(rng, n, True)
+    ufd = _random_faces(rng, nf, True)
+    dc = ksgf.DeviceBoxArray(cba)
+    df = ksgf.DeviceBoxArray(fba, covers_domain=False)
+    mc = [dc.upload(cba.scatter(ucd[d][None], 1 + d, 1)) for d in range(3)]
+    mf = [df.upload(fba.scatter(ufd[d][None], 1 + d, 1)) for d in range(3)]
+    opts = ksgf.default_options(maxiter=2)
+    mp = ksgf.MacProjectorLevels(dc, df, (P_, N_, P_), (P_, N_, P_), 1.0, opts)
+    info = mp.project(mc, mf, 1, 1e-11, 0.0)
+    assert info["iters"] == 2 and info["converged"] == 0
+    assert 0.0 < info["rfinal"] < info["r0"]
+    o = oracle.default_opts(maxiter=2)
+    umc = [a.ravel().copy() for a in ucd]
+    umf = [a.ravel().copy() for a in ufd]
+    rinfo, _, _ = oracle.mac_project_two_levels(n, cg.dx, (P_, N_, P_), (P_, N_, P_), fboxes, 1.0, umc, umf, 1e-11,
+                                                0.0, o)
+    assert rinfo["status"] == 1 and rinfo["iters"] == 2
+    assert abs(info["rfinal"] - rinfo["rfinal"]) <= 1e-8 * rinfo["r0"]
