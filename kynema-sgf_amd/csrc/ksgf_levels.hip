@@ -200,50 +200,70 @@ __global__ void __launch_bounds__(256) k_tl_ghosts(MFDev pf, const int* __restri
     f(i, j, k) = v;
 }
 
-// fine residual R = b - div(beta grad phi) over the valid cells (ghosts filled)
-__global__ void __launch_bounds__(256) k_tl_fres(CMFDev pf, CMFDev bf, MFDev rf, const int* __restrict__ fbox, TLGeom T,
-                                                 RegDev r)
-{
-    int b, i, j, k;
-    if (!region_locate(r, fbox, b, i, j, k)) return;
-    CFab p = fab_at(pf, fbox, b);
-    const double u0 = p(i, j, k);
-    const int c[3] = {i, j, k};
-    double acc = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        int hi[3] = {c[0], c[1], c[2]}, lo[3] = {c[0], c[1], c[2]};
-        hi[d] += 1;
-        lo[d] -= 1;
-        const double Fp = T.beta * (p(hi[0], hi[1], hi[2]) - u0) * T.ihf[d];
-        const double Fm = T.beta * (u0 - p(lo[0], lo[1], lo[2])) * T.ihf[d];
-        acc += (Fp - Fm) * T.ihf[d];
-    }
-    fab_at(rf, fbox, b)(i, j, k) = fab_at(bf, fbox, b)(i, j, k) - acc;
-}
+// The fine stencil passes (ghosts filled): one thread per segment of kTlKz
+// cells of one (i, j) column of a fine box, walking up in z with the cell
+// and its z neighbours in registers, its FABs located once.  The residual
+// is R = b - div(beta grad phi) with the fluxes of tl_fflux (the oracle's
+// operand order).  SMOOTH: one damped-Jacobi sweep, phi + w R, into the
+// other phi buffer; otherwise R into rf.
+constexpr int kTlKz = 16;
 
-// one damped-Jacobi sweep on the fine cells, pin (ghosts filled) to pout:
-// phi + w (b - div(beta grad phi)), the residual as k_tl_fres forms it
-__global__ void __launch_bounds__(256) k_tl_smooth(CMFDev pin, CMFDev bf, MFDev pout, const int* __restrict__ fbox,
-                                                   TLGeom T, double w, RegDev r)
+struct ColDev {
+    const int* chunks;  // (box, first column-segment) per 256 threads
+};
+
+template <bool SMOOTH>
+__global__ void __launch_bounds__(256) k_tl_march(CMFDev pin, CMFDev bf, MFDev out, const int* __restrict__ fbox,
+                                                  TLGeom T, double w, ColDev cd)
 {
-    int b, i, j, k;
-    if (!region_locate(r, fbox, b, i, j, k)) return;
-    CFab p = fab_at(pin, fbox, b);
-    const double u0 = p(i, j, k);
-    const int c[3] = {i, j, k};
-    double acc = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        int hi[3] = {c[0], c[1], c[2]}, lo[3] = {c[0], c[1], c[2]};
-        hi[d] += 1;
-        lo[d] -= 1;
-        const double Fp = T.beta * (p(hi[0], hi[1], hi[2]) - u0) * T.ihf[d];
-        const double Fm = T.beta * (u0 - p(lo[0], lo[1], lo[2])) * T.ihf[d];
-        acc += (Fp - Fm) * T.ihf[d];
+    const int b = cd.chunks[2 * blockIdx.x];
+    const unsigned loc = (unsigned)cd.chunks[2 * blockIdx.x + 1] + threadIdx.x;
+    const int* bx = fbox + 6 * b;
+    const unsigned nx = (unsigned)(bx[3] - bx[0] + 1), ny = (unsigned)(bx[4] - bx[1] + 1);
+    const int nz = bx[5] - bx[2] + 1, nseg = (nz + kTlKz - 1) / kTlKz;
+    if (loc >= nx * ny * (unsigned)nseg) return;
+    const unsigned q = loc / nx;
+    const int i = bx[0] + (int)(loc - q * nx), j = bx[1] + (int)(q % ny), sg = (int)(q / ny);
+    const int k0 = bx[2] + sg * kTlKz, k1 = min(k0 + kTlKz, bx[5] + 1);
+    const CFab p = fab_at(pin, fbox, b);
+    const CFab rb = fab_at(bf, fbox, b);
+    const Fab o = fab_at(out, fbox, b);
+    // input, right-hand side and output never overlap: restrict-qualified
+    // bases let the compiler issue the loads of later planes early
+    const double* __restrict__ pp = p.p;
+    const double* __restrict__ bp = rb.p;
+    double* __restrict__ op = o.p;
+    const int sy = p.n0, sz = p.n0 * p.n1, sbz = rb.n0 * rb.n1, soz = o.n0 * o.n1;
+    int c = p.idx(i, j, k0), cb = rb.idx(i, j, k0), co = o.idx(i, j, k0);
+    double um = pp[c - sz], u0 = pp[c];
+    const int len = k1 - k0;
+#pragma unroll 4
+    for (int t = 0; t < len; ++t) {
+        const double up = pp[c + sz];
+        double acc = 0.0;
+        {
+            const double Fp = T.beta * (pp[c + 1] - u0) * T.ihf[0];
+            const double Fm = T.beta * (u0 - pp[c - 1]) * T.ihf[0];
+            acc += (Fp - Fm) * T.ihf[0];
+        }
+        {
+            const double Fp = T.beta * (pp[c + sy] - u0) * T.ihf[1];
+            const double Fm = T.beta * (u0 - pp[c - sy]) * T.ihf[1];
+            acc += (Fp - Fm) * T.ihf[1];
+        }
+        {
+            const double Fp = T.beta * (up - u0) * T.ihf[2];
+            const double Fm = T.beta * (u0 - um) * T.ihf[2];
+            acc += (Fp - Fm) * T.ihf[2];
+        }
+        const double R = bp[cb] - acc;
+        op[co] = SMOOTH ? u0 + w * R : R;
+        um = u0;
+        u0 = up;
+        c += sz;
+        cb += sbz;
+        co += soz;
     }
-    const double R = fab_at(bf, fbox, b)(i, j, k) - acc;
-    fab_at(pout, fbox, b)(i, j, k) = u0 + w * R;
 }
 
 // fine flux beta dphi/dx_d through the face of fine cell f on side s (the
@@ -509,6 +529,8 @@ struct ksgf_macproj_levels {
     double *bf = nullptr, *rf = nullptr;
     int2* gl = nullptr;  // face-ghost list of the fine FABs
     long long ngl = 0;
+    int* cols = nullptr;  // column-segment launch table of the fine boxes (k_tl_march)
+    int ncols_chunks = 0;
     long long nc = 0, nfg = 0, nf0 = 0;
     long long ncov = 0, nfine = 0;
     unsigned long long* dmax = nullptr;  // 2 slots
@@ -551,7 +573,8 @@ struct TLRun {
     }
     void fres()
     {
-        hipLaunchKernelGGL(k_tl_fres, dim3(nv_chunks), dim3(256), 0, st, cpf(), bf, rf, m->fba->dbox, m->T, rv);
+        hipLaunchKernelGGL(k_tl_march<false>, dim3(m->ncols_chunks), dim3(256), 0, st, cpf(), bf, rf, m->fba->dbox,
+                           m->T, 0.0, ColDev{m->cols});
     }
     void cres()
     {
@@ -570,8 +593,8 @@ struct TLRun {
     {
         for (int s = 0; s < nsweep; ++s) {
             ghosts();
-            hipLaunchKernelGGL(k_tl_smooth, dim3(nv_chunks), dim3(256), 0, st, cpf(), bf, pfm[1 - m->cur],
-                               m->fba->dbox, m->T, w, rv);
+            hipLaunchKernelGGL(k_tl_march<true>, dim3(m->ncols_chunks), dim3(256), 0, st, cpf(), bf, pfm[1 - m->cur],
+                               m->fba->dbox, m->T, w, ColDev{m->cols});
             m->cur ^= 1;
         }
     }
@@ -726,6 +749,23 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
                 }
         }
         m->ngl = (long long)h.size();
+        // column segments: per box, nx * ny columns times ceil(nz / kTlKz)
+        // segments, x fastest; chunks of 256 consecutive segments of one box
+        std::vector<int> ch;
+        for (int b = 0; b < fine_ba->nbox; ++b) {
+            const int* x = &fine_ba->hbox[6 * b];
+            const long long n = (long long)(x[3] - x[0] + 1) * (x[4] - x[1] + 1) * ((x[5] - x[2] + kTlKz) / kTlKz);
+            for (long long s0 = 0; s0 < n; s0 += 256) {
+                ch.push_back(b);
+                ch.push_back((int)s0);
+            }
+        }
+        m->ncols_chunks = (int)(ch.size() / 2);
+        if (hipMalloc(&m->cols, ch.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpy(m->cols, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("ksgf_macproj_levels_create: column table");
+            return fail(KSGF_ERR_HIP);
+        }
         if (m->ngl && (hipMalloc(&m->gl, h.size() * sizeof(int2)) != hipSuccess ||
                        hipMemcpy(m->gl, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)) {
             set_error("ksgf_macproj_levels_create: ghost list");
@@ -900,6 +940,7 @@ int ksgf_macproj_levels_destroy(ksgf_macproj_levels* m)
     hipFree(m->pf[0]);
     hipFree(m->pf[1]);
     hipFree(m->gl);
+    hipFree(m->cols);
     hipFree(m->bf);
     hipFree(m->rf);
     hipFree(m->dmax);
