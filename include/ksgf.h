@@ -339,6 +339,21 @@ int ksgf_fillpatch_two_levels(const ksgf_boxarray* fine_ba, double* fine, int ng
                               int ncomp, int ratio, const ksgf_bcrec* bc, const double* bcval,
                               ksgf_stream_t stream);
 
+/* FillPatchTwoLevels of a refined level's face MultiFab (itype x/y/z
+ * faces), as fillpatch_sibling_fields fills the MAC velocities' ghost faces
+ * after the projection (icns_advection.H:231-237): face-linear interpolation
+ * at ratio 2 (AMReX's FaceLinear: a fine face on a coarse face takes its
+ * value, one midway between two coarse faces their mean, piecewise constant
+ * across the face); a ghost face that is a valid face of another fine box
+ * (a fine cell on either side, e.g. that box's high face on the coarse-fine
+ * boundary) takes that box's value; then FillBoundary and the physical BCs
+ * (bc / bcval as ksgf_fill_physbc).  crse: the coarse face MultiFab (valid
+ * faces used). */
+int ksgf_fillpatch_faces_two_levels(const ksgf_boxarray* fine_ba, double* fine, int itype, int ng,
+                                    const ksgf_boxarray* crse_ba, const double* crse, int ng_c,
+                                    int ncomp, const ksgf_bcrec* bc, const double* bcval,
+                                    ksgf_stream_t stream);
+
 /* Hydro::NodalProjector with a constant sigma (incflo::ApplyProjection,
  * src/projection/incflo_apply_nodal_projection.cpp:340-353 for the constant-
  * density branch, options from MLMGOptions::operator()(NodalProjector&),

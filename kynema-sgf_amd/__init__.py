@@ -66,7 +66,7 @@ EXPORTS = [
     "ksgf_abeclap_set_scalars", "ksgf_abeclap_solve", "ksgf_abeclap_apply", "ksgf_abeclap_destroy",
     "ksgf_icns_src_term_divtau", "ksgf_boxarray_create_level", "ksgf_average_down", "ksgf_average_down_faces",
     "ksgf_fillpatch_two_levels", "ksgf_macproj_levels_create", "ksgf_macproj_levels_project",
-    "ksgf_macproj_levels_get_phi", "ksgf_macproj_levels_destroy",
+    "ksgf_macproj_levels_get_phi", "ksgf_macproj_levels_destroy", "ksgf_fillpatch_faces_two_levels",
 ]
 
 _lib = None
@@ -175,6 +175,8 @@ def load(path=None):
                                           C.c_int, C.c_int, C.c_void_p]
     L.ksgf_fillpatch_two_levels.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int,
                                             C.POINTER(KsgfBCRec), C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_fillpatch_faces_two_levels.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p, _dp, C.c_int,
+                                                  C.c_int, C.POINTER(KsgfBCRec), C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_icns_src_term_divtau.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_incflo_add_gradp.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
                                         C.c_void_p]
@@ -405,6 +407,15 @@ class DeviceBoxArray:
         check(load().ksgf_fillpatch_two_levels(self.h, _ptr(fine), ng, crse_dba.h, _ptr(crse), ng_c, ncomp, ratio,
                                                bcrecs(bcrec), bv.ctypes.data_as(C.POINTER(C.c_double)), _stream()),
               "fillpatch_two_levels")
+
+    def fillpatch_faces_two_levels(self, fine, itype, ng, crse_dba, crse, ng_c, ncomp, bcrec, bcval):
+        """FillPatchTwoLevels of this (fine) level's ghost faces from crse"""
+        import numpy as np
+
+        bv = np.ascontiguousarray(bcval, dtype=np.float64)
+        check(load().ksgf_fillpatch_faces_two_levels(self.h, _ptr(fine), itype, ng, crse_dba.h, _ptr(crse), ng_c,
+                                                     ncomp, bcrecs(bcrec), bv.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     _stream()), "fillpatch_faces_two_levels")
 
     def set_bndry(self, mf, itype, ncomp, ngrow, val=0.0):
         check(load().ksgf_set_bndry(self.h, _ptr(mf), itype, ncomp, ngrow, val, _stream()), "set_bndry")

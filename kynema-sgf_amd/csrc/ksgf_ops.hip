@@ -118,6 +118,7 @@ __global__ void __launch_bounds__(256) k_mac_divmax(CMFDev u, CMFDev v, CMFDev w
 struct GeomFP {
     int fdlo[3], fdhi[3], per[3], cdlo[3];
     int cn0, cn1;
+    int cdhi2;  // last coarse cell along z
 };
 
 // amrex::average_down: one thread per coarse cell of a coarsened fine box
@@ -265,6 +266,84 @@ __global__ void __launch_bounds__(256) k_fp_interp(MFDev fine, const int* __rest
         double alpha = 1.0;
         if (dev > 0.0) alpha = fmin(1.0, fmin((umax - uc) / dev, (uc - umin) / dev));
         f(i, j, k, n) = uc + alpha * ((sl[0] * off[0] + sl[1] * off[1]) + sl[2] * off[2]);
+    }
+}
+
+// face-linear interpolation of a refined level's ghost faces of direction D
+// from the coarse faces (ratio 2): on a coarse face its value, midway between
+// two coarse faces their mean, piecewise constant across the face
+template <int D>
+__global__ void __launch_bounds__(256) k_fp_faces(MFDev fine, const int* __restrict__ fbox, CMFDev crse,
+                                                  const int* __restrict__ cbox, const int* __restrict__ cowner,
+                                                  const int* __restrict__ fowner, int ncomp, GeomFP g, RegDev r)
+{
+    int b, i, j, k;
+    if (!region_locate(r, fbox, b, i, j, k)) return;
+    const int* bx = fbox + 6 * b;
+    const int p[3] = {i, j, k};
+    bool valid = true;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+        if (p[e] < bx[e] || p[e] > bx[3 + e] + (e == D)) valid = false;
+    if (valid) return;
+    int w[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int L = g.fdhi[e] - g.fdlo[e] + 1;
+        const int hi = g.fdhi[e] + (e == D);
+        int x = p[e];
+        if (g.per[e])
+            x = g.fdlo[e] + (((x - g.fdlo[e]) % L) + L) % L;
+        else if (x < g.fdlo[e] || x > hi)
+            return;  // outside a non-periodic domain: physical BC
+        w[e] = x;
+    }
+    // a valid face of another fine box (a fine cell on either side) keeps
+    // that box's value
+    {
+        const int fn0 = g.fdhi[0] - g.fdlo[0] + 1, fn1 = g.fdhi[1] - g.fdlo[1] + 1, L = g.fdhi[D] - g.fdlo[D] + 1;
+        int fb2 = -1;
+        if (w[D] <= g.fdhi[D])
+            fb2 = fowner[(w[0] - g.fdlo[0]) + fn0 * ((w[1] - g.fdlo[1]) + fn1 * (w[2] - g.fdlo[2]))];
+        if (fb2 < 0) {
+            int lo[3] = {w[0], w[1], w[2]};
+            lo[D] -= 1;
+            if (lo[D] < g.fdlo[D]) lo[D] = g.per[D] ? g.fdhi[D] : g.fdlo[D] - 1;
+            if (lo[D] >= g.fdlo[D])
+                fb2 = fowner[(lo[0] - g.fdlo[0]) + fn0 * ((lo[1] - g.fdlo[1]) + fn1 * (lo[2] - g.fdlo[2]))];
+        }
+        if (fb2 >= 0) {
+            const int* ob = fbox + 6 * fb2;
+            int q[3] = {w[0], w[1], w[2]};
+            if (q[D] > ob[3 + D] + 1) q[D] -= L;
+            if (q[D] < ob[D]) q[D] += L;
+            Fab src = fab_at(fine, fbox, fb2);
+            Fab f = fab_at(fine, fbox, b);
+            for (int n = 0; n < ncomp; ++n) f(i, j, k, n) = src(q[0], q[1], q[2], n);
+            return;
+        }
+    }
+    int C[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) C[e] = (w[e] >= 0) ? w[e] / 2 : -((-w[e] + 1) / 2);
+    const bool even = (w[D] - 2 * C[D]) == 0;
+    const int chi = (D == 0) ? g.cdlo[0] + g.cn0 - 1 : (D == 1 ? g.cdlo[1] + g.cn1 - 1 : g.cdhi2);
+    if (even && C[D] > chi) C[D] -= 1;  // the domain's high face: the cell below holds it
+    const int cb = cowner[(C[0] - g.cdlo[0]) + g.cn0 * ((C[1] - g.cdlo[1]) + g.cn1 * (C[2] - g.cdlo[2]))];
+    if (cb < 0) return;
+    CFab c = fab_at(crse, cbox, cb);
+    Fab f = fab_at(fine, fbox, b);
+    int F0[3] = {C[0], C[1], C[2]};
+    if (even) F0[D] = w[D] / 2;
+    int F1[3] = {C[0], C[1], C[2]};
+    F1[D] = C[D] + 1;
+    for (int n = 0; n < ncomp; ++n) {
+        double v;
+        if (even)
+            v = c(F0[0], F0[1], F0[2], n);
+        else
+            v = 0.5 * (c(C[0], C[1], C[2], n) + c(F1[0], F1[1], F1[2], n));
+        f(i, j, k, n) = v;
     }
 }
 
@@ -606,11 +685,69 @@ int ksgf_fillpatch_two_levels(const ksgf_boxarray* fine_ba, double* fine, int ng
     }
     g.cn0 = crse_ba->dhi[0] - crse_ba->dlo[0] + 1;
     g.cn1 = crse_ba->dhi[1] - crse_ba->dlo[1] + 1;
+    g.cdhi2 = crse_ba->dhi[2];
     hipLaunchKernelGGL(k_fp_interp, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c, crse_ba->dbox, owner, ncomp,
                        ratio, 0.5 - 0.5 / (double)ratio, g, regdev(r, fine_ba->nbox));
     KSGF_LAUNCH_CHECK();
     if ((e = ksgf_fill_boundary(fine_ba, fine, KSGF_CELL, ncomp, ng, stream))) return e;
     return ksgf_fill_physbc(fine_ba, fine, KSGF_CELL, ncomp, ng, bc, bcval, stream);
+}
+
+int ksgf_fillpatch_faces_two_levels(const ksgf_boxarray* fine_ba, double* fine, int itype, int ng,
+                                    const ksgf_boxarray* crse_ba, const double* crse, int ng_c, int ncomp,
+                                    const ksgf_bcrec* bc, const double* bcval, ksgf_stream_t stream)
+{
+    if (!fine_ba || !fine || !crse_ba || !crse || !bc || !bcval || ncomp <= 0 || ncomp > 8 || ng < 0 || ng_c < 0 ||
+        itype < KSGF_XFACE || itype > KSGF_ZFACE) {
+        set_error("ksgf_fillpatch_faces_two_levels: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (fine_ba->dlo[d] != 2 * crse_ba->dlo[d] || fine_ba->dhi[d] + 1 != 2 * (crse_ba->dhi[d] + 1) ||
+            fine_ba->periodic[d] != crse_ba->periodic[d]) {
+            set_error("ksgf_fillpatch_faces_two_levels: the fine domain is not the coarse domain refined by 2");
+            return KSGF_ERR_INVALID;
+        }
+    if (ng == 0) return KSGF_OK;
+    hipStream_t st = (hipStream_t)stream;
+    MFDev f;
+    CMFDev c;
+    int e;
+    if ((e = mfdev(fine_ba, fine, itype, ncomp, ng, &f)) || (e = cmfdev(crse_ba, crse, itype, ncomp, ng_c, &c)))
+        return e;
+    const int *owner = nullptr, *fowner = nullptr;
+    if ((e = owner_map(crse_ba, st, &owner)) || (e = owner_map(fine_ba, st, &fowner))) return e;
+    int err = KSGF_OK;
+    const Region* r = fab_region(fine_ba, itype, ng, &err);
+    if (!r) return err;
+    GeomFP g;
+    for (int d = 0; d < 3; ++d) {
+        g.fdlo[d] = fine_ba->dlo[d];
+        g.fdhi[d] = fine_ba->dhi[d];
+        g.per[d] = fine_ba->periodic[d];
+        g.cdlo[d] = crse_ba->dlo[d];
+    }
+    g.cn0 = crse_ba->dhi[0] - crse_ba->dlo[0] + 1;
+    g.cn1 = crse_ba->dhi[1] - crse_ba->dlo[1] + 1;
+    g.cdhi2 = crse_ba->dhi[2];
+    const RegDev rd = regdev(r, fine_ba->nbox);
+    switch (itype) {
+    case KSGF_XFACE:
+        hipLaunchKernelGGL(k_fp_faces<0>, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c, crse_ba->dbox,
+                           owner, fowner, ncomp, g, rd);
+        break;
+    case KSGF_YFACE:
+        hipLaunchKernelGGL(k_fp_faces<1>, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c, crse_ba->dbox,
+                           owner, fowner, ncomp, g, rd);
+        break;
+    default:
+        hipLaunchKernelGGL(k_fp_faces<2>, dim3(r->nchunks), dim3(256), 0, st, f, fine_ba->dbox, c, crse_ba->dbox,
+                           owner, fowner, ncomp, g, rd);
+        break;
+    }
+    KSGF_LAUNCH_CHECK();
+    if ((e = ksgf_fill_boundary(fine_ba, fine, itype, ncomp, ng, stream))) return e;
+    return ksgf_fill_physbc(fine_ba, fine, itype, ncomp, ng, bc, bcval, stream);
 }
 
 }  // extern "C"

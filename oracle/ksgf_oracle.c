@@ -1763,6 +1763,104 @@ void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_g
     ko_fill_physbc(fg, fine, KO_CELL, ncomp, ng, bc, bcval);
 }
 
+/* FillPatchTwoLevels of a refined level's face MultiFab (direction d =
+ * itype - 1), as fillpatch_sibling_fields fills the MAC velocities after the
+ * projection (icns_advection.H:231-237): AMReX's face-linear interpolation at
+ * ratio 2, restated -- a fine face on a coarse face takes that face's value, a
+ * fine face midway between two coarse faces their mean, piecewise constant
+ * across the face; a ghost face that is a valid face of another fine box
+ * takes that box's value (cell-based FillBoundary would miss a box's high
+ * face on the coarse-fine boundary); then FillBoundary and the physical
+ * BCs. */
+void ko_fillpatch_faces_two_levels(const ko_geom* fg, double* fine, int itype, int ng, const ko_geom* cg,
+                                   const double* crse, int ng_c, int ncomp, const ko_bcrec* bc, const double* bcval)
+{
+    const int d = itype - 1;
+    fab_t* ff = (fab_t*)malloc(sizeof(fab_t) * (size_t)fg->nbox);
+    fab_t* fc = (fab_t*)malloc(sizeof(fab_t) * (size_t)cg->nbox);
+    make_fabs(fg, itype, ncomp, ng, ff);
+    make_fabs(cg, itype, ncomp, ng_c, fc);
+    int* cown = make_owner(cg);
+    int* fown = make_owner(fg);
+    const int cn0 = cg->dhi[0] - cg->dlo[0] + 1, cn1 = cg->dhi[1] - cg->dlo[1] + 1;
+    const int fn0 = fg->dhi[0] - fg->dlo[0] + 1, fn1 = fg->dhi[1] - fg->dlo[1] + 1;
+    for (int b = 0; b < fg->nbox; ++b) {
+        const int* bx = fg->boxes + 6 * b;
+        const fab_t* fb = &ff[b];
+        for (int k = fb->lo[2]; k < fb->lo[2] + fb->n[2]; ++k)
+            for (int j = fb->lo[1]; j < fb->lo[1] + fb->n[1]; ++j)
+                for (int i = fb->lo[0]; i < fb->lo[0] + fb->n[0]; ++i) {
+                    const int p[3] = {i, j, k};
+                    int valid = 1;
+                    for (int e = 0; e < 3; ++e)
+                        if (p[e] < bx[e] || p[e] > bx[3 + e] + (e == d)) valid = 0;
+                    if (valid) continue;
+                    int w[3], ok = 1;
+                    for (int e = 0; e < 3; ++e) {
+                        const int L = fg->dhi[e] - fg->dlo[e] + 1;
+                        const int hi = fg->dhi[e] + (e == d);  /* last face / cell index in the domain */
+                        int x = p[e];
+                        if (fg->periodic[e])
+                            x = fg->dlo[e] + (((x - fg->dlo[e]) % L) + L) % L;
+                        else if (x < fg->dlo[e] || x > hi)
+                            ok = 0;
+                        w[e] = x;
+                    }
+                    if (!ok) continue;
+                    /* a valid face of another fine box (a fine cell on either
+                     * side) keeps that box's value */
+                    {
+                        int lo[3] = {w[0], w[1], w[2]}, fb2 = -1;
+                        int hiok = w[d] <= fg->dhi[d];
+                        if (hiok) fb2 = fown[(w[0] - fg->dlo[0]) + (long)fn0 * ((w[1] - fg->dlo[1]) + (long)fn1 * (w[2] - fg->dlo[2]))];
+                        if (fb2 < 0) {
+                            lo[d] -= 1;
+                            if (lo[d] < fg->dlo[d]) lo[d] = fg->periodic[d] ? fg->dhi[d] : fg->dlo[d] - 1;
+                            if (lo[d] >= fg->dlo[d])
+                                fb2 = fown[(lo[0] - fg->dlo[0]) + (long)fn0 * ((lo[1] - fg->dlo[1]) + (long)fn1 * (lo[2] - fg->dlo[2]))];
+                        }
+                        if (fb2 >= 0) {
+                            const fab_t* fo = &ff[fb2];
+                            int q[3] = {w[0], w[1], w[2]};
+                            /* the face index inside box fb2's face range */
+                            if (q[d] > fg->boxes[6 * fb2 + 3 + d] + 1) q[d] -= fg->dhi[d] - fg->dlo[d] + 1;
+                            if (q[d] < fg->boxes[6 * fb2 + d]) q[d] += fg->dhi[d] - fg->dlo[d] + 1;
+                            for (int n = 0; n < ncomp; ++n)
+                                fine[FI(fb, i, j, k) + n * FS(fb)] = fine[FI(fo, q[0], q[1], q[2]) + n * FS(fo)];
+                            continue;
+                        }
+                    }
+                    /* the coarse cell holding the coarse face(s) */
+                    int C[3];
+                    for (int e = 0; e < 3; ++e) C[e] = (w[e] >= 0) ? w[e] / 2 : -((-w[e] + 1) / 2);
+                    const int even = ((w[d] - 2 * C[d]) == 0);
+                    if (even && C[d] > cg->dhi[d]) C[d] -= 1; /* the domain's high face: the cell below holds it */
+                    const int cb = cown[(C[0] - cg->dlo[0]) + (long)cn0 * ((C[1] - cg->dlo[1]) + (long)cn1 * (C[2] - cg->dlo[2]))];
+                    if (cb < 0) continue;
+                    const fab_t* cf = &fc[cb];
+                    int F0[3] = {C[0], C[1], C[2]};
+                    if (even) F0[d] = w[d] / 2;
+                    int F1[3] = {C[0], C[1], C[2]};
+                    F1[d] = C[d] + 1;
+                    for (int n = 0; n < ncomp; ++n) {
+                        const double* cs = crse + n * FS(cf);
+                        double v;
+                        if (even)
+                            v = cs[FI(cf, F0[0], F0[1], F0[2])];
+                        else
+                            v = 0.5 * (cs[FI(cf, C[0], C[1], C[2])] + cs[FI(cf, F1[0], F1[1], F1[2])]);
+                        fine[FI(fb, i, j, k) + n * FS(fb)] = v;
+                    }
+                }
+    }
+    free(cown);
+    free(fown);
+    free(ff);
+    free(fc);
+    ko_fill_boundary(fg, fine, itype, ncomp, ng);
+    ko_fill_physbc(fg, fine, itype, ncomp, ng, bc, bcval);
+}
+
 /* ------------------------------------------------------------------ */
 /* Composite two-level MAC projection (MacProjector over levels 0..1, */
 /* icns_advection.cpp:88-95,331,353; AMReX MLMG's composite solve).   */

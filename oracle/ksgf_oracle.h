@@ -153,6 +153,12 @@ void ko_average_down_faces(const ko_geom* fg, const double* const* fine, int ng_
 void ko_fillpatch_two_levels(const ko_geom* fg, double* fine, int ng, const ko_geom* cg, const double* crse,
                              int ng_c, int ncomp, int ratio, const ko_bcrec* bc, const double* bcval);
 
+/* FillPatchTwoLevels of a refined level's face MultiFab (itype: x/y/z
+ * faces), face-linear interpolation at ratio 2, then FillBoundary and
+ * physical BCs (the MAC velocities' fillpatch_sibling_fields) */
+void ko_fillpatch_faces_two_levels(const ko_geom* fg, double* fine, int itype, int ng, const ko_geom* cg,
+                                   const double* crse, int ng_c, int ncomp, const ko_bcrec* bc, const double* bcval);
+
 /* Composite two-level MAC projection (MacProjector over levels 0 and 1 at
  * ratio 2).  Domain-wide arrays: coarse cells n, fine cells 2n (valid where a
  * fine box covers), faces with one more entry along their direction.  fboxes:

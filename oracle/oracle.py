@@ -101,6 +101,8 @@ def lib():
                                             C.POINTER(_dp), C.c_int, C.c_int, C.c_int]
         L.ko_fillpatch_two_levels.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.POINTER(KoGeom), _dp, C.c_int,
                                               C.c_int, C.c_int, C.POINTER(KoBC), _dp]
+        L.ko_fillpatch_faces_two_levels.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.c_int, C.POINTER(KoGeom), _dp,
+                                                    C.c_int, C.c_int, C.POINTER(KoBC), _dp]
         L.ko_tl_apply.argtypes = [_ip, _dp, _ip, _ip, _ip, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         L.ko_mac_project_two_levels.argtypes = [_ip, _dp, _ip, _ip, _ip, C.c_int, C.c_double, C.POINTER(_dp),
                                                 C.POINTER(_dp), C.c_double, C.c_double, C.POINTER(KoOpts),
@@ -336,6 +338,13 @@ def fillpatch_two_levels(Gf, fine, ng, Gc, crse, ng_c, ncomp, ratio, bcrec, bcva
                                   _p(bcval))
 
 
+def fillpatch_faces_two_levels(Gf, fine, itype, ng, Gc, crse, ng_c, ncomp, bcrec, bcval):
+    """FillPatchTwoLevels of the fine level's ghost faces (in place)"""
+    bcval = np.ascontiguousarray(bcval, dtype=np.float64)
+    lib().ko_fillpatch_faces_two_levels(Gf.ref, _p(fine), itype, ng, Gc.ref, _p(crse), ng_c, ncomp,
+                                        bc_structs(bcrec), _p(bcval))
+
+
 def _i3(v):
     return (C.c_int * 3)(*[int(x) for x in v])
 
@@ -470,6 +479,115 @@ def _diffusion(case, ba, opts, vel, mu, dt_diff, rtol, atol, comp_apply=None):
             out.append(x)
             infos.append(info)
     return np.stack(out), infos
+
+
+def two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f, dt=None, opts=None, limiter=1):
+    """One advect + MAC-project step of two AMR levels (level 1 = fba at
+    ratio 2, no subcycling), constant density case.rho0: the Godunov branch of
+    ApplyPredictor with every operator looping over levels
+    (icns_advection.H:210-217 ExtrapVelToFaces per level, :226 set_inflow,
+    :229 MacProjector over both levels, :231-243 face fill-patch and
+    FillBoundary, :359-373 fluxes per level, :401-407 fine fluxes averaged
+    onto the coarse faces, :418-422 divergence, CompRHSOps.H:93-117 predictor
+    per level), then average_down of the new velocity.  vel_c / vel_f: cell
+    MultiFabs with 3 ghosts (filled here); gp_*: ngrow 0.  Returns (vel_c
+    new, vel_f new, diagnostics)."""
+    ba = case.ba
+    Gc, Gf = Geom(ba), Geom(fba)
+    opts = opts or case_opts(case)
+    bcrec = np.asarray(case.bcrec, dtype=np.int32)
+    bcval = np.asarray(case.bcval, dtype=np.float64)
+    fill_ghosts(Gc, vel_c, CELL, 3, 3, bcrec, bcval)
+    fillpatch_two_levels(Gf, vel_f, 3, Gc, vel_c, 3, 3, 2, bcrec, bcval)
+    if dt is None:
+        c = max(cfl_max(Gc, vel_c, 3), cfl_max(Gf, vel_f, 3))
+        dt = 2.0 * case.cfl / (c + math.sqrt(c * c))
+    sbc = bcrec.copy()
+    sbc[sbc != 0] = 2
+    lev = []
+    for G, b, vel, gp in ((Gc, ba, vel_c, gp_c), (Gf, fba, vel_f, gp_f)):
+        rho = np.full(b.mf_size(CELL, 1, 3), case.rho0)
+        src = np.zeros(b.mf_size(CELL, 3, 1))
+        for i in range(b.nbox):
+            s_ = b.fab_view(src, i, CELL, 3, 1)
+            g_ = b.fab_view(gp, i, CELL, 3, 0)
+            rv = b.fab_view(rho, i, CELL, 1, 3)[0, 3:-3, 3:-3, 3:-3]
+            s_[:, 1:-1, 1:-1, 1:-1] = -(1.0 * g_) * (1.0 / rv)
+        lev.append(dict(G=G, ba=b, vel=vel, gp=gp, rho=rho, src=src))
+    fill_ghosts(Gc, lev[0]["src"], CELL, 3, 1, sbc, np.zeros((6, 3)))
+    fillpatch_two_levels(Gf, lev[1]["src"], 1, Gc, lev[0]["src"], 1, 3, 2, sbc, np.zeros((6, 3)))
+    for L in lev:
+        L["mac"] = extrap_vel_to_faces(L["G"], L["vel"], L["src"], bcrec, dt, 1, 0, limiter, 0)
+        _set_inflow(case, L["ba"], L["mac"])
+    # composite projection on domain-wide face arrays
+    nc = case.geom.n_cell
+    umc = [np.ascontiguousarray(ba.gather(lev[0]["mac"][d], XFACE + d, 1, 1)[0]).ravel() for d in range(3)]
+    umf = [np.ascontiguousarray(fba.gather(lev[1]["mac"][d], XFACE + d, 1, 1)[0]).ravel() for d in range(3)]
+    info, _, _ = mac_project_two_levels(nc, case.geom.dx, case.lobc, case.hibc, fba.boxes, 1.0 / case.rho0, umc, umf,
+                                        case.mg_rtol, case.mg_atol, opts)
+    nf = fba.geom.n_cell
+    for d in range(3):
+        sc = [1, nc[2], nc[1], nc[0]]
+        sc[3 - d] += 1
+        sf = [1, nf[2], nf[1], nf[0]]
+        sf[3 - d] += 1
+        ba.scatter(umc[d].reshape(sc), XFACE + d, 1, out=lev[0]["mac"][d])
+        fba.scatter(umf[d].reshape(sf), XFACE + d, 1, out=lev[1]["mac"][d])
+    for d in range(3):
+        bc1, bv1 = bcrec[d:d + 1], np.ascontiguousarray(bcval[:, d:d + 1])
+        fill_boundary(Gc, lev[0]["mac"][d], XFACE + d, 1, 1)
+        fill_physbc(Gc, lev[0]["mac"][d], XFACE + d, 1, 1, bc1, bv1)
+        fillpatch_faces_two_levels(Gf, lev[1]["mac"][d], XFACE + d, 1, Gc, lev[0]["mac"][d], 1, 1, bc1, bv1)
+    for L in lev:
+        b = L["ba"]
+        q = L["vel"].copy()
+        fq = L["src"].copy()
+        for i in range(b.nbox):
+            r3 = b.fab_view(L["rho"], i, CELL, 1, 3)
+            b.fab_view(q, i, CELL, 3, 3)[...] *= r3
+            b.fab_view(fq, i, CELL, 3, 1)[...] *= r3[:, 2:-2, 2:-2, 2:-2]
+        L["flux"], _ = compute_fluxes(L["G"], 3, q, q.copy(), L["mac"], None, fq, dt, bcrec, [1, 1, 1], 1, 0, 1, 0,
+                                      limiter, 0)
+    # conservation across the coarse-fine boundary: the fine fluxes onto the
+    # coarse faces before the divergence
+    average_down_faces(Gf, lev[1]["flux"], 0, Gc, lev[0]["flux"], 0, 3, 2)
+    out = []
+    for L in lev:
+        b = L["ba"]
+        conv = compute_divergence(L["G"], L["flux"], 3, -1.0, 0)
+        src_nph = np.zeros(b.mf_size(CELL, 3, 0))
+        for i in range(b.nbox):
+            g_ = b.fab_view(L["gp"], i, CELL, 3, 0)
+            r_ = b.fab_view(L["rho"], i, CELL, 1, 3)[:, 3:-3, 3:-3, 3:-3]
+            b.fab_view(src_nph, i, CELL, 3, 0)[...] = (-(1.0 * g_) * (1.0 / r_)) * r_
+        vn = np.zeros_like(L["vel"])
+        predictor_update(L["G"], vn, 3, L["vel"], 3, L["rho"], 3, L["rho"], 3, conv, src_nph, None, 0.0, dt)
+        out.append(vn)
+    average_down(Gf, out[1], 3, Gc, out[0], 3, 3, 2)
+    fill_ghosts(Gc, out[0], CELL, 3, 3, bcrec, bcval)
+    fillpatch_two_levels(Gf, out[1], 3, Gc, out[0], 3, 3, 2, bcrec, bcval)
+    return out[0], out[1], dict(info, dt=dt)
+
+
+def _set_inflow(case, ba, mac):
+    """set_inflow_velocity (icns_advection.cpp:157-183): the Dirichlet value on
+    ext_dir normal faces of the domain"""
+    dhi = [int(n) - 1 for n in ba.geom.n_cell]
+    for d in range(3):
+        if case.geom.periodic[d]:
+            continue
+        for side in (0, 1):
+            if case.bcrec[d, 3 * side + d] != 3:
+                continue
+            val = case.bcval[3 * side + d, d]
+            for b in range(ba.nbox):
+                v = ba.fab_view(mac[d], b, XFACE + d, 1, 1)
+                lo, hi = ba.boxes[b, d], ba.boxes[b, 3 + d] + 1
+                x = 0 if side == 0 else dhi[d] + 1
+                if lo <= x <= hi:
+                    idx = [slice(1, -1)] * 3
+                    idx[2 - d] = x - lo + 1
+                    v[(0, *idx)] = val
 
 
 def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, scal=None, scal_bc=None,

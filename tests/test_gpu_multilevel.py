@@ -100,3 +100,32 @@ def test_fillpatch_two_levels_bitexact(ksgf, oracle, gpu, reach_sides):
         got = to_np(d)
         assert np.array_equal(np.isnan(got), np.isnan(ref))
         assert np.array_equal(np.nan_to_num(got), np.nan_to_num(ref)), f"ng {ng}"
+
+
+@pytest.mark.parametrize("reach_sides", [False, True])
+def test_fillpatch_faces_two_levels_bitexact(ksgf, oracle, gpu, reach_sides):
+    """ghost faces of the fine level from the coarse faces (face-linear), then
+    the fine level's own faces, then the physical BCs, every direction"""
+    _say(f"fillpatch_faces_two_levels reach_sides={reach_sides}")
+    cba, fba = _levels(reach_sides)
+    Gc, Gf = oracle.Geom(cba), oracle.Geom(fba)
+    dc = ksgf.DeviceBoxArray(cba)
+    df = ksgf.DeviceBoxArray(fba, covers_domain=False)
+    rng = np.random.default_rng(8)
+    nc, nf = cba.geom.n_cell, fba.geom.n_cell
+    for d in range(3):
+        bc = np.array([[0, 3, 0, 0, 3, 0], [0, 2, 0, 0, 2, 0], [0, 3, 0, 0, 3, 0]], dtype=np.int32)[d:d + 1]
+        bcval = rng.standard_normal((6, 1))
+        shp_c = [2, nc[2], nc[1], nc[0]]
+        shp_c[3 - d] += 1
+        crse = cba.scatter(rng.standard_normal(shp_c), XFACE + d, 1)
+        shp_f = [2, nf[2], nf[1], nf[0]]
+        shp_f[3 - d] += 1
+        fine = fba.scatter(rng.standard_normal(shp_f), XFACE + d, 2)
+        ref = fine.copy()
+        oracle.fillpatch_faces_two_levels(Gf, ref, XFACE + d, 2, Gc, crse, 1, 2, np.repeat(bc, 2, axis=0),
+                                          np.repeat(bcval, 2, axis=1))
+        t = df.upload(fine)
+        df.fillpatch_faces_two_levels(t, XFACE + d, 2, dc, dc.upload(crse), 1, 2, np.repeat(bc, 2, axis=0),
+                                      np.repeat(bcval, 2, axis=1))
+        assert np.array_equal(to_np(t), ref), f"dir {d}"

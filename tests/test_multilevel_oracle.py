@@ -261,3 +261,153 @@ def test_fillpatch_known_answers():
     s = np.array([-1.0, 1.0])
     ref = 1.0 + (s[:, None, None] + s[None, :, None] + s[None, None, :]) / 30.0
     assert np.abs(kids(3, 5, 7) - ref).max() <= 1e-14
+
+
+# --------------------------------------------------------------------------
+# face fill-patch (fillpatch_sibling_fields on the MAC velocities,
+# icns_advection.H:231-237): face-linear interpolation at ratio 2
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_fillpatch_faces_linear_normal_exact_and_fine_data_kept(d):
+    """a face field linear in its normal coordinate (constant across) is
+    reproduced exactly on every fine ghost face inside the domain (coarse-face
+    copies and midpoint means are exact for it), valid faces are untouched, and
+    a ghost face another fine box holds carries that box's value"""
+    cba, fba = two_level(seed=31)
+    Gc, Gf = oracle.Geom(cba), oracle.Geom(fba)
+    fn = lambda x: 0.25 + 3.0 * x  # noqa: E731
+
+    def faces(ba, ng):
+        out = np.zeros(ba.mf_size(XFACE + d, 1, ng))
+        h = ba.geom.dx[d]
+        for b in range(ba.nbox):
+            v = ba.fab_view(out, b, XFACE + d, 1, ng)[0]
+            lo = ba.boxes[b, d] - ng
+            shape = [1, 1, 1]
+            shape[2 - d] = v.shape[2 - d]
+            x = ((lo + np.arange(v.shape[2 - d])) * h).reshape(shape)
+            v[...] = fn(x)
+        return out
+
+    ng = 2
+    crse = faces(cba, 0)
+    ref = faces(fba, ng)
+    fine = np.full(fba.mf_size(XFACE + d, 1, ng), np.nan)
+    for b in range(fba.nbox):  # valid faces only
+        vf = fba.fab_view(fine, b, XFACE + d, 1, ng)[0]
+        vr = fba.fab_view(ref, b, XFACE + d, 1, ng)[0]
+        sl = tuple(slice(ng, -ng) for _ in range(3))
+        vf[sl] = vr[sl]
+    bc = np.zeros((1, 6), dtype=np.int32)
+    bc[0, 1] = bc[0, 4] = 2  # y walls: first-order extrapolation
+    oracle.fillpatch_faces_two_levels(Gf, fine, XFACE + d, ng, Gc, crse, 0, 1, bc, np.zeros((6, 1)))
+    nf = fba.geom.n_cell
+    for b in range(fba.nbox):
+        vf = fba.fab_view(fine, b, XFACE + d, 1, ng)[0]
+        vr = fba.fab_view(ref, b, XFACE + d, 1, ng)[0]
+        lo = fba.boxes[b, :3] - ng
+        nz, ny, nx = vf.shape
+        idx = np.meshgrid(np.arange(nz) + lo[2], np.arange(ny) + lo[1], np.arange(nx) + lo[0], indexing="ij")
+        inside = np.ones(vf.shape, dtype=bool)
+        if d != 1:  # y is the only wall direction: faces beyond it take the physical BC
+            inside &= (idx[1] >= 0) & (idx[1] <= nf[1] - 1)
+        else:
+            inside &= (idx[1] >= 0) & (idx[1] <= nf[1])
+        assert np.abs(vf[inside] - vr[inside]).max() <= 1e-14, b
+
+
+# --------------------------------------------------------------------------
+# one two-level advect + MAC-project step (every operator over both levels)
+# --------------------------------------------------------------------------
+def tgv_two_level(n=16, seed=2):
+    """Taylor-Green on a periodic n^3 level 0 (boxes of n/2) and a level 1 at
+    ratio 2 over the coarse cells [n/4, 3n/4)^3 in three boxes; the covered
+    coarse cells hold the average of the fine velocity"""
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    case = wl.taylor_green(n, max_grid_size=n // 2, noise=0.0)
+    fg = lay.Geometry((2 * n,) * 3, prob_hi=(1.0, 1.0, 1.0), periodic=(1, 1, 1))
+    a, b, m = n // 2, 3 * n // 2, n
+    fba = FineBoxArray(fg, [[a, a, a, m - 1, b - 1, b - 1], [m, a, a, b - 1, m - 1, b - 1],
+                            [m, m, a, b - 1, b - 1, b - 1]])
+    x = fg.cell_centers(0)[None, None, :]
+    y = fg.cell_centers(1)[None, :, None]
+    z = fg.cell_centers(2)[:, None, None]
+    tp = 2.0 * np.pi
+    rng = np.random.default_rng(seed)
+    u = np.sin(tp * x) * np.cos(tp * y) * np.cos(tp * z)
+    v = -np.cos(tp * x) * np.sin(tp * y) * np.cos(tp * z)
+    vf = np.stack(np.broadcast_arrays(u, v, 0.1 * u * v)).astype(np.float64) + 1e-3 * rng.standard_normal((3,) + u.shape)
+    vel_f = fba.scatter(vf, CELL, 3)
+    vel_c = case.ba.scatter(case.vel, CELL, 3)
+    oracle.average_down(oracle.Geom(fba), vel_f, 3, oracle.Geom(case.ba), vel_c, 3, 3, 2)
+    gp_c = np.zeros(case.ba.mf_size(CELL, 3, 0))
+    gp_f = np.zeros(fba.mf_size(CELL, 3, 0))
+    return case, fba, vel_c, vel_f, gp_c, gp_f
+
+
+def test_two_level_step_conserves_momentum():
+    """periodic, no pressure gradient: the level-0 total momentum (covered
+    cells averaged down) is conserved to rounding over a two-level step, which
+    needs the fine fluxes on the coarse faces of the coarse-fine boundary
+    (average_down_faces) and the final average_down"""
+    case, fba, vel_c, vel_f, gp_c, gp_f = tgv_two_level()
+    tot0 = case.ba.gather(vel_c, CELL, 3, 3).sum(axis=(1, 2, 3))
+    scale = np.abs(case.ba.gather(vel_c, CELL, 3, 3)).sum()
+    vc1, vf1, info = oracle.two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f)
+    assert info["status"] == 0, info
+    tot1 = case.ba.gather(vc1, CELL, 3, 3).sum(axis=(1, 2, 3))
+    assert np.abs(tot1 - tot0).max() <= 1e-14 * scale, (tot0, tot1)
+    # the step moved the field
+    assert np.abs(case.ba.gather(vc1, CELL, 3, 3) - case.ba.gather(vel_c, CELL, 3, 3)).max() > 1e-3
+    # covered coarse cells are the average of the new fine velocity
+    chk = vc1.copy()
+    oracle.average_down(oracle.Geom(fba), vf1, 3, oracle.Geom(case.ba), chk, 3, 3, 2)
+    assert np.array_equal(chk, vc1)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_fillpatch_faces_shared_faces_keep_fine_values(d):
+    """a ghost face that is a valid face of another fine box -- including that
+    box's high face on the coarse-fine boundary, which no fine cell beyond it
+    owns -- takes that box's value, not the coarse interpolation"""
+    case, fba, *_ = tgv_two_level()
+    cba = case.ba
+    Gc, Gf = oracle.Geom(cba), oracle.Geom(fba)
+    rng = np.random.default_rng(12 + d)
+    nc, nf = cba.geom.n_cell, fba.geom.n_cell
+    shp_c = [1, nc[2], nc[1], nc[0]]
+    shp_c[3 - d] += 1
+    crse = cba.scatter(rng.standard_normal(shp_c), XFACE + d, 0)
+    shp_f = [1, nf[2], nf[1], nf[0]]
+    shp_f[3 - d] += 1
+    dom = rng.standard_normal(shp_f)
+    fine = fba.scatter(dom, XFACE + d, 1)
+    oracle.fillpatch_faces_two_levels(Gf, fine, XFACE + d, 1, Gc, crse, 0, 1, np.zeros((1, 6), np.int32),
+                                      np.zeros((6, 1)))
+    nshared = 0
+    for b in range(fba.nbox):
+        v = fba.fab_view(fine, b, XFACE + d, 1, 1)[0]
+        lo = fba.boxes[b, :3] - 1
+        for b2 in range(fba.nbox):
+            if b2 == b:
+                continue
+            flo = fba.boxes[b2, :3].copy()
+            fhi = fba.boxes[b2, 3:].copy()
+            fhi[d] += 1
+            # faces of b2 inside b's FAB storage but outside b's valid faces
+            glo = np.maximum(flo, lo)
+            ghi = np.minimum(fhi, lo + np.array(v.shape[::-1]) - 1)
+            if np.any(glo > ghi):
+                continue
+            for k in range(glo[2], ghi[2] + 1):
+                for j in range(glo[1], ghi[1] + 1):
+                    for i in range(glo[0], ghi[0] + 1):
+                        p = np.array([i, j, k])
+                        vb = fba.boxes[b]
+                        vhi = vb[3:].copy()
+                        vhi[d] += 1
+                        if np.all(p >= vb[:3]) and np.all(p <= vhi):
+                            continue
+                        assert v[k - lo[2], j - lo[1], i - lo[0]] == dom[0, k, j, i]
+                        nshared += 1
+    assert nshared > 0
