@@ -6,7 +6,7 @@ correction); the headline value is the mixed path, the FP64 path is
 reported beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
-                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch|nodal|levels]
+                  [--precision both|mixed|fp64] [--no-cpu] [--workload abl|poisson-batch|nodal|levels|levels-step]
 
 A step is one pass of the hot path over the whole grid, driven by the C++
 host layer (GodunovPredictor): convective CFL, source terms,
@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
-    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal", "levels"], default="abl",
+    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal", "levels", "levels-step"], default="abl",
                    help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
                         "N(0,1) right-hand sides, seed 5) solved as one batch (hipGraph replay and eager) and one "
                         "after another; nodal: the full incflo step (advect + MAC projection + ApplyProjection's "
@@ -450,6 +450,122 @@ def levels_project(steps, warmup, no_cpu):
     return out
 
 
+def _cells_on_device(ba, ncomp, ng, fn):
+    """cell MultiFab (ncomp, ngrow ng) on the device, each FAB (ghosts
+    included) filled with fn(c, X, Y, Z) at its cell centres"""
+    import torch
+
+    out = torch.empty(ba.mf_size(0, ncomp, ng), dtype=torch.float64, device="cuda")
+    lo, n = ba.fab_lo_n(0, ng)
+    off = ba.fab_offsets(0, ncomp, ng)
+    h = ba.geom.dx
+    for b in range(ba.nbox):
+        ax = [(int(lo[b, e]) + torch.arange(int(n[b, e]), dtype=torch.float64, device="cuda") + 0.5) * h[e]
+              for e in range(3)]
+        Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        out[int(off[b]):int(off[b + 1])] = torch.stack([fn(c, X, Y, Z) for c in range(ncomp)]).reshape(-1)
+    return out
+
+
+def _tgv_cells(geom):
+    """Taylor-Green velocity with wavenumbers 2 pi / L per axis and a w
+    component, periodic on the domain"""
+    import torch
+
+    k = [2.0 * np.pi / (geom.prob_hi[e] - geom.prob_lo[e]) for e in range(3)]
+
+    def fn(c, X, Y, Z):
+        u = torch.sin(k[0] * X) * torch.cos(k[1] * Y) * torch.cos(k[2] * Z)
+        v = -torch.cos(k[0] * X) * torch.sin(k[1] * Y) * torch.cos(k[2] * Z)
+        return u if c == 0 else (v if c == 1 else 0.1 * u * v)
+
+    return fn
+
+
+def levels_step(steps, warmup, no_cpu):
+    """Two-level advect + MAC-project steps (levels.TwoLevelStep: per-level
+    Godunov, composite MAC projection, fill-patches, flux averaging,
+    average_down) on the 1,500-box ragged 320x320x480 level 0 with 600
+    level-1 boxes over 40% of the domain; the oracle's two-level step on the
+    16^3 Taylor-Green copy with three level-1 boxes is the CPU baseline."""
+    import torch
+
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    lv = importlib.import_module("kynema-sgf_amd.levels")
+    case, fba = wl.ragged_two_level()
+    S = lv.TwoLevelStep(case, fba)
+    vc = _cells_on_device(case.ba, 3, 3, _tgv_cells(case.geom))
+    vf = _cells_on_device(fba, 3, 3, _tgv_cells(fba.geom))
+    S.df.average_down_to(vf, 3, S.dc, vc, 3, 3)
+    S.load(vc, vf)
+    del vc, vf
+    st = torch.cuda.current_stream()
+    ms, its = [], []
+    for s in range(warmup + steps):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        S.step()
+        e1.record(st)
+        torch.cuda.synchronize()
+        if s >= warmup:
+            ms.append(e0.elapsed_time(e1))
+            its.append(S.last_info["iters"])
+    nfine = int(np.sum(np.prod(fba.boxes[:, 3:] - fba.boxes[:, :3] + 1, axis=1)))
+    ncomp = case.ncell - nfine // 8 + nfine
+    t = float(np.mean(ms))
+    out = {
+        "metric": "composite cell-updates/s of the two-level advect + MAC-project step",
+        "value": ncomp / (t * 1e-3),
+        "unit": "cell-updates/s (uncovered level-0 cells + level-1 cells)",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": t,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ragged {case.ba.nbox} boxes 320x320x480 level 0 + {fba.nbox} level-1 boxes (ratio 2, "
+                               f"{nfine} cells over 40% of the domain), triply periodic Taylor-Green, CFL 0.5, "
+                               f"MAC rtol 1e-11", "composite_iterations": its, "last_dt": S.last_dt},
+    }
+    S.close()
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.build()
+        c2, f2 = wl.ragged_two_level((64, 64, 96), 32, fine_z=(20, 58), fine_box=32)
+        vel = []
+        for ba in (c2.ba, f2):
+            g = ba.geom
+            k = [2.0 * np.pi / (g.prob_hi[e] - g.prob_lo[e]) for e in range(3)]
+            x = g.cell_centers(0)[None, None, :]
+            y = g.cell_centers(1)[None, :, None]
+            z = g.cell_centers(2)[:, None, None]
+            u = np.sin(k[0] * x) * np.cos(k[1] * y) * np.cos(k[2] * z)
+            v = -np.cos(k[0] * x) * np.sin(k[1] * y) * np.cos(k[2] * z)
+            vel.append(ba.scatter(np.stack(np.broadcast_arrays(u, v, 0.1 * u * v)), 0, 3))
+        O.average_down(O.Geom(f2), vel[1], 3, O.Geom(c2.ba), vel[0], 3, 3, 2)
+        gp_c = np.zeros(c2.ba.mf_size(0, 3, 0))
+        gp_f = np.zeros(f2.mf_size(0, 3, 0))
+        n2 = int(np.sum(np.prod(f2.boxes[:, 3:] - f2.boxes[:, :3] + 1, axis=1)))
+        t0 = time.perf_counter()
+        reps = 0
+        vel_c, vel_f = vel
+        while True:
+            vel_c, vel_f, rinfo = O.two_level_step(c2, f2, vel_c, vel_f, gp_c, gp_f)
+            reps += 1
+            if time.perf_counter() - t0 > 10.0 or reps >= 5:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": (c2.ncell - n2 // 8 + n2) * reps / dt, "unit": out["unit"],
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": f"{reps} oracle two-level steps of the 64x64x96 copy of the layout "
+                                         f"({rinfo['iters']} composite iterations), {dt:.2f} s"}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -470,6 +586,9 @@ def main():
         return
     if a.workload == "levels":
         print(json.dumps(levels_project(a.steps, a.warmup, a.no_cpu)))
+        return
+    if a.workload == "levels-step":
+        print(json.dumps(levels_step(a.steps, a.warmup, a.no_cpu)))
         return
     wl = importlib.import_module("kynema-sgf_amd.workloads")
     n = tuple(a.n)

@@ -68,15 +68,22 @@ class TwoLevelStep:
         self.last_dt = None
 
     def load(self, vel_c, vel_f, gp_c=None, gp_f=None):
-        """Upload the levels' velocity MultiFabs (host arrays in the FAB layout,
-        3 ghosts) and pressure gradients (ngrow 0)."""
+        """Set the levels' velocity MultiFabs (3 ghosts) and pressure
+        gradients (ngrow 0) from host arrays in the FAB layout or device
+        tensors of the same size."""
+        import torch
+
         L0, L1 = self.lev
-        L0.vel.copy_(self.dc.upload(vel_c))
-        L1.vel.copy_(self.df.upload(vel_f))
+
+        def put(dst, src, dba):
+            dst.copy_(src if torch.is_tensor(src) else dba.upload(src))
+
+        put(L0.vel, vel_c, self.dc)
+        put(L1.vel, vel_f, self.df)
         if gp_c is not None:
-            L0.gp.copy_(self.dc.upload(gp_c))
+            put(L0.gp, gp_c, self.dc)
         if gp_f is not None:
-            L1.gp.copy_(self.df.upload(gp_f))
+            put(L1.gp, gp_f, self.df)
 
     def _fill_vel(self, which):
         L0, L1 = self.lev
