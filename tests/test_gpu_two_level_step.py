@@ -1,8 +1,9 @@
 """Two-level advect + MAC-project steps on the GPU (levels.TwoLevelStep:
 per-level Godunov, composite MAC projection, face and cell fill-patch, fine
 fluxes averaged onto the coarse faces, average_down) against the oracle's
-two_level_step: Taylor-Green on a periodic 16^3 level 0 with three level-1
-boxes at ratio 2, two steps.  Everything but the composite solve is
+two_level_step: Taylor-Green on a 16^3 level 0 (periodic, or with no-slip
+walls in y that level-1 boxes touch) with three level-1 boxes at ratio 2,
+two steps.  Everything but the composite solve is
 bit-exact; the solve agrees to its tolerance, so the velocities agree to
 1e-9 of their scale.  The GPU step conserves the level-0 momentum too."""
 import importlib
@@ -17,12 +18,13 @@ pytestmark = pytest.mark.gpu
 CELL = 0
 
 
-def test_two_level_steps_match_oracle(ksgf, oracle, gpu):
+@pytest.mark.parametrize("walls", [False, True])
+def test_two_level_steps_match_oracle(ksgf, oracle, gpu, walls):
     import test_multilevel_oracle as T
 
-    print("two-level steps, tgv16 + 3 fine boxes", flush=True)
+    print(f"two-level steps, tgv16 + 3 fine boxes, y walls {walls}", flush=True)
     lv = importlib.import_module("kynema-sgf_amd.levels")
-    case, fba, vel_c, vel_f, gp_c, gp_f = T.tgv_two_level()
+    case, fba, vel_c, vel_f, gp_c, gp_f = T.tgv_two_level_walls() if walls else T.tgv_two_level()
     S = lv.TwoLevelStep(case, fba)
     S.load(vel_c, vel_f, gp_c, gp_f)
     ref_c, ref_f = vel_c.copy(), vel_f.copy()

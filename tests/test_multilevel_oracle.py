@@ -345,12 +345,49 @@ def tgv_two_level(n=16, seed=2):
     return case, fba, vel_c, vel_f, gp_c, gp_f
 
 
-def test_two_level_step_conserves_momentum():
-    """periodic, no pressure gradient: the level-0 total momentum (covered
-    cells averaged down) is conserved to rounding over a two-level step, which
-    needs the fine fluxes on the coarse faces of the coarse-fine boundary
+def tgv_two_level_walls(n=16, seed=3):
+    """as tgv_two_level, with no-slip walls in y (velocity ext_dir 0, MAC
+    Neumann) and level-1 boxes that touch the low wall"""
+    import dataclasses
+
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    g = lay.Geometry((n,) * 3, prob_hi=(1.0, 1.0, 1.0), periodic=(1, 0, 1))
+    ba = lay.BoxArray(g, lay.chop_uniform(g.n_cell, n // 2))
+    base = wl.taylor_green(n, max_grid_size=n // 2, noise=0.0)
+    bcrec = np.array([[0, 3, 0, 0, 3, 0]] * 3, dtype=np.int32)
+    case = dataclasses.replace(base, name=f"tgv{n}_ywalls", geom=g, ba=ba, bcrec=bcrec, bcval=np.zeros((6, 3)),
+                               lobc=(0, 1, 0), hibc=(0, 1, 0))
+    fg = lay.Geometry((2 * n,) * 3, prob_hi=(1.0, 1.0, 1.0), periodic=(1, 0, 1))
+    a, b, m = n // 2, 3 * n // 2, n
+    fba = FineBoxArray(fg, [[a, 0, a, m - 1, m // 2 - 1, b - 1], [m, 0, a, b - 1, m - 1, b - 1],
+                            [a, m // 2, a, m - 1, m - 1, b - 1]])
+    x = fg.cell_centers(0)[None, None, :]
+    y = fg.cell_centers(1)[None, :, None]
+    z = fg.cell_centers(2)[:, None, None]
+    tp = 2.0 * np.pi
+    rng = np.random.default_rng(seed)
+    u = np.sin(tp * x) * np.sin(np.pi * y) * np.cos(tp * z)
+    w = np.cos(tp * x) * np.sin(np.pi * y) * np.sin(tp * z)
+    vf = np.stack(np.broadcast_arrays(u, 0.0 * u, w)).astype(np.float64) + 1e-3 * rng.standard_normal((3,) + u.shape)
+    xc = g.cell_centers(0)[None, None, :]
+    yc = g.cell_centers(1)[None, :, None]
+    zc = g.cell_centers(2)[:, None, None]
+    uc = np.sin(tp * xc) * np.sin(np.pi * yc) * np.cos(tp * zc)
+    wc = np.cos(tp * xc) * np.sin(np.pi * yc) * np.sin(tp * zc)
+    vel_c = ba.scatter(np.stack(np.broadcast_arrays(uc, 0.0 * uc, wc)).astype(np.float64), CELL, 3)
+    vel_f = fba.scatter(vf, CELL, 3)
+    oracle.average_down(oracle.Geom(fba), vel_f, 3, oracle.Geom(ba), vel_c, 3, 3, 2)
+    return case, fba, vel_c, vel_f, np.zeros(ba.mf_size(CELL, 3, 0)), np.zeros(fba.mf_size(CELL, 3, 0))
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_two_level_step_conserves_momentum(walls):
+    """no pressure gradient, periodic or no-slip walls (no advective flux
+    through a wall): the level-0 total momentum (covered cells averaged down)
+    is conserved to rounding over a two-level step, which needs the fine
+    fluxes on the coarse faces of the coarse-fine boundary
     (average_down_faces) and the final average_down"""
-    case, fba, vel_c, vel_f, gp_c, gp_f = tgv_two_level()
+    case, fba, vel_c, vel_f, gp_c, gp_f = tgv_two_level_walls() if walls else tgv_two_level()
     tot0 = case.ba.gather(vel_c, CELL, 3, 3).sum(axis=(1, 2, 3))
     scale = np.abs(case.ba.gather(vel_c, CELL, 3, 3)).sum()
     vc1, vf1, info = oracle.two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f)
