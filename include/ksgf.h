@@ -220,7 +220,9 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
  * ng, corrected in place; returns KSGF_ERR_NOT_CONVERGED (after correcting)
  * if maxiter is reached.  get_phi: the coarse solution domain-wide (n cells,
  * covered cells hold the average of their children) and the fine one as a
- * cell MultiFab with one ghost layer (ng_f must be 1). */
+ * cell MultiFab with one ghost layer (ng_f must be 1).  The handle holds the
+ * iterate between calls: calls on one handle must not overlap, and both
+ * BoxArrays must outlive it. */
 typedef struct ksgf_macproj_levels ksgf_macproj_levels;
 int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray* fine_ba,
                                const int* lobc, const int* hibc, double beta,

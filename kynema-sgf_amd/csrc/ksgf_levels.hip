@@ -646,6 +646,10 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
         return KSGF_ERR_INVALID;
     }
     *out = nullptr;
+    if (fine_ba->nbox <= 0) {
+        set_error("ksgf_macproj_levels_create: the fine level has no boxes (use ksgf_macproj_create)");
+        return KSGF_ERR_INVALID;
+    }
     if (!crse_ba->covers_domain) {
         set_error("ksgf_macproj_levels_create: the coarse level must cover its domain");
         return KSGF_ERR_INVALID;
