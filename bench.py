@@ -154,9 +154,30 @@ def cpu_baseline(n_cell):
             break
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": case.ncell * reps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} oracle steps of the {n_cell[0]}x{n_cell[1]}x{n_cell[2]} ABL box (same dx/dz and "
-                      f"physics, {d['mlmg']['iters']} V-cycles in the last), {dt:.2f} s"}
+    out = {"value": case.ncell * reps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+           "sample": f"{reps} oracle steps of the {n_cell[0]}x{n_cell[1]}x{n_cell[2]} ABL box (same dx/dz and "
+                     f"physics, {d['mlmg']['iters']} V-cycles in the last), {dt:.2f} s"}
+    return out, (case, reps, case.ba.gather(vel, 0, 3, 3))
+
+
+def vs_oracle(case, reps, vel_ref):
+    """The GPU step (mixed precision, as the headline) against the FP64
+    oracle after the same steps of the same case: velocity rel-L2 and the
+    post-projection max |div u_mac| of the last step"""
+    ksgf = importlib.import_module("kynema-sgf_amd")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    o = ksgf.default_options(**case.mlmg)
+    o.precision = 1
+    S = hs.HostStepper(case, options=o)
+    S.load()
+    for r in range(reps):
+        S.step(check_divergence=(r == reps - 1))
+    v = S.velocity()
+    out = {"workload": f"ABL {case.geom.n_cell[0]}x{case.geom.n_cell[1]}x{case.geom.n_cell[2]}", "steps": reps,
+           "precision": "mixed", "velocity_rel_l2": float(np.linalg.norm(v - vel_ref) / np.linalg.norm(vel_ref)),
+           "post_projection_max_div": S.last_divmax}
+    S.close()
+    return out
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_pk_half_abl256_mixed.txt")
@@ -635,7 +656,8 @@ def main():
                            "launches": prof["launches"], "avg_us": avg_s * 1e6,
                            "bytes_per_launch": bpl}
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_n)
+        out["cpu_baseline"], (c2, reps, vref) = cpu_baseline(a.cpu_n)
+        out["vs_oracle"] = vs_oracle(c2, reps, vref)
     print(json.dumps(out))
 
 
