@@ -157,13 +157,15 @@ def cpu_baseline(n_cell):
     out = {"value": case.ncell * reps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
            "sample": f"{reps} oracle steps of the {n_cell[0]}x{n_cell[1]}x{n_cell[2]} ABL box (same dx/dz and "
                      f"physics, {d['mlmg']['iters']} V-cycles in the last), {dt:.2f} s"}
-    return out, (case, reps, case.ba.gather(vel, 0, 3, 3))
+    n = case.geom.n_cell
+    return out, (case, reps, case.ba.gather(vel, 0, 3, 3), d["phi"].reshape(n[2], n[1], n[0]))
 
 
-def vs_oracle(case, reps, vel_ref):
+def vs_oracle(case, reps, vel_ref, phi_ref):
     """The GPU step (mixed precision, as the headline) against the FP64
-    oracle after the same steps of the same case: velocity rel-L2 and the
-    post-projection max |div u_mac| of the last step"""
+    oracle after the same steps of the same case: velocity rel-L2, the
+    rel-L2 of the last MAC projection's grad phi (cell differences along
+    each axis) and the post-projection max |div u_mac| of the last step"""
     ksgf = importlib.import_module("kynema-sgf_amd")
     hs = importlib.import_module("kynema-sgf_amd.host_stepper")
     o = ksgf.default_options(**case.mlmg)
@@ -173,8 +175,15 @@ def vs_oracle(case, reps, vel_ref):
     for r in range(reps):
         S.step(check_divergence=(r == reps - 1))
     v = S.velocity()
+    phi = S.phi()
+
+    def grad(p):
+        return np.concatenate([np.diff(p, axis=a).ravel() for a in (2, 1, 0)])
+
+    gr = grad(phi_ref)
     out = {"workload": f"ABL {case.geom.n_cell[0]}x{case.geom.n_cell[1]}x{case.geom.n_cell[2]}", "steps": reps,
            "precision": "mixed", "velocity_rel_l2": float(np.linalg.norm(v - vel_ref) / np.linalg.norm(vel_ref)),
+           "grad_phi_rel_l2": float(np.linalg.norm(grad(phi) - gr) / np.linalg.norm(gr)),
            "post_projection_max_div": S.last_divmax}
     S.close()
     return out
@@ -638,6 +647,7 @@ def main():
         "config": {"workload": f"neutral ABL {n[0]}x{n[1]}x{n[2]}, dx=dy=20 m dz=4 m, uneven boxes "
                                f"{{32,48,64}} ({case.ba.nbox} boxes), CFL 0.9, PPM, MAC mg_rtol 1e-11",
                    "precision": head, "mlmg_iters": [i["iters"] for i in infos],
+                   "mlmg_r0": [i["r0"] for i in infos], "mlmg_rfinal": [i["rfinal"] for i in infos],
                    "post_projection_max_div": divmax},
     }
     if "fp64" in res and head != "fp64":
@@ -656,8 +666,8 @@ def main():
                            "launches": prof["launches"], "avg_us": avg_s * 1e6,
                            "bytes_per_launch": bpl}
     if not a.no_cpu:
-        out["cpu_baseline"], (c2, reps, vref) = cpu_baseline(a.cpu_n)
-        out["vs_oracle"] = vs_oracle(c2, reps, vref)
+        out["cpu_baseline"], (c2, reps, vref, pref) = cpu_baseline(a.cpu_n)
+        out["vs_oracle"] = vs_oracle(c2, reps, vref, pref)
     print(json.dumps(out))
 
 

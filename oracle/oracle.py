@@ -643,7 +643,8 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
                     idx = [slice(1, -1)] * 3
                     idx[2 - d] = x - lo + 1
                     v[(0, *idx)] = val
-    info, _ = mac_project(G, mac, case.lobc, case.hibc, 1.0 / case.rho0, case.mg_rtol, case.mg_atol, opts)
+    info, phi = mac_project(G, mac, case.lobc, case.hibc, 1.0 / case.rho0, case.mg_rtol, case.mg_atol, opts,
+                            want_phi=True)
     for d in range(3):
         bc1 = case.bcrec[d:d + 1]
         fill_boundary(G, mac[d], XFACE + d, 1, 1)
@@ -676,8 +677,8 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
         n = case.geom.n_cell
         ba.scatter(x.reshape(3, n[2], n[1], n[0]), CELL, 3, out=vel_new)
     fill_ghosts(G, vel_new, CELL, 3, 3, case.bcrec, case.bcval)
-    return vel_new, {"dt": dt, "mlmg": info, "mac": mac, "conv": conv, "flux": flux, "face": face, "scal": scal_new,
-                     "diffusion": dinfo}
+    return vel_new, {"dt": dt, "mlmg": info, "phi": phi, "mac": mac, "conv": conv, "flux": flux, "face": face,
+                     "scal": scal_new, "diffusion": dinfo}
 
 
 def apply_projection(case, G, vel, rho, gp, dt, rtol, atol, opts=None):
