@@ -157,6 +157,26 @@ def cpu_baseline(n_cell):
     out = {"value": case.ncell * reps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
            "sample": f"{reps} oracle steps of the {n_cell[0]}x{n_cell[1]}x{n_cell[2]} ABL box (same dx/dz and "
                      f"physics, {d['mlmg']['iters']} V-cycles in the last), {dt:.2f} s"}
+    # the same oracle on one thread (the OpenMP runtime the oracle links)
+    try:
+        import ctypes
+
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        G1, v1, r1, g1 = O.make_state(case)
+        t1 = time.perf_counter()
+        n1 = 0
+        while True:
+            v1, _ = O.advect_project_step(case, G1, v1, r1, g1)
+            n1 += 1
+            if time.perf_counter() - t1 > 5.0 or n1 >= 2:
+                break
+        dt1 = time.perf_counter() - t1
+        gomp.omp_set_num_threads(cores)
+        out["one_thread"] = {"value": case.ncell * n1 / dt1, "unit": "cell-updates/s",
+                             "sample": f"{n1} oracle steps of the same box on 1 thread, {dt1:.2f} s"}
+    except OSError:
+        out["one_thread"] = None
     n = case.geom.n_cell
     return out, (case, reps, case.ba.gather(vel, 0, 3, 3), d["phi"].reshape(n[2], n[1], n[0]))
 
