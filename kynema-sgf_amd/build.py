@@ -18,6 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libksgf.so")
 HOSTLIB = os.path.join(HERE, "libksgf_host.so")
 STEPEXE = os.path.join(HERE, "ksgf_step")
+LEVCHK = os.path.join(HERE, "ksgf_levels_check")
 HOSTDIR = os.path.join(HERE, "host")
 
 SOURCES = ["ksgf_layout.hip", "ksgf_godunov.hip", "ksgf_mlmg.hip", "ksgf_ops.hip", "ksgf_adjoint.hip",
@@ -78,8 +79,15 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"ksgf_step build failed:\n{r.stderr}")
     os.replace(tmp, STEPEXE)
+    tmp = LEVCHK + ".tmp"
+    cmd = [HIPCC, "-O3", "-std=c++17", "-Wall", os.path.join(HOSTDIR, "ksgf_levels_check.cpp"), "-L" + HERE,
+           "-l:libksgf.so", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ksgf_levels_check build failed:\n{r.stderr}")
+    os.replace(tmp, LEVCHK)
     if verbose:
-        print("built", LIB, HOSTLIB, STEPEXE)
+        print("built", LIB, HOSTLIB, STEPEXE, LEVCHK)
     return LIB
 
 
