@@ -214,8 +214,9 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
  * ksgf_boxarray_create_level, boxes aligned to 2.  Coarse-fine ghosts are
  * quadratic through the tangentially interpolated coarse value; uncovered
  * coarse cells next to the fine level use the averaged fine fluxes (reflux).
- * opts: maxiter (composite iterations), num_pre/post_smooth (fine Jacobi
- * sweeps); each iteration takes one FP64 V-cycle on the coarse grid.
+ * opts: maxiter (composite iterations), num_pre/post_smooth (smoothing
+ * steps on the fine level, each a pair of damped-Jacobi sweeps); each
+ * iteration takes one FP64 V-cycle on the coarse grid.
  * project: umac_c / umac_f: the x, y, z face MultiFabs of each level, ngrow
  * ng, corrected in place; returns KSGF_ERR_NOT_CONVERGED (after correcting)
  * if maxiter is reached.  get_phi: the coarse solution domain-wide (n cells,

@@ -18,7 +18,8 @@
 //    operator conserves;
 //  * covered coarse cells carry no unknown; they end at the average of their
 //    children.
-// Iteration: damped Jacobi (omega 6/7) on the fine cells, a coarse
+// Iteration: damped Jacobi (omega 6/7) on the fine cells (each MLMG
+// smoothing step a pair of sweeps: 2 nu1 before, 2 nu2 after), a coarse
 // correction of the composite residual (the fine residual averaged under the
 // fine level) by one V-cycle of the single-level projector on the coarse
 // grid, piecewise-constant prolongation to the fine cells, damped Jacobi
@@ -850,7 +851,7 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     inf.nlevels = 2;
     int ret = KSGF_OK;
     for (int it = 1; !inf.converged && it <= m->opt.maxiter; ++it) {
-        R.smooth(m->opt.num_pre_smooth, w);
+        R.smooth(2 * m->opt.num_pre_smooth, w);
         R.residual();
         hipLaunchKernelGGL(k_tl_avg8, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->rc,
                            (const unsigned char*)m->cv, R.crf, fba->dbox, m->fown, T);
@@ -861,7 +862,7 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
         hipLaunchKernelGGL(k_tl_addc, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const double*)m->ec, m->nc);
         hipLaunchKernelGGL(k_tl_prolong, dim3(R.nv_chunks), dim3(256), 0, st, R.pf(), (const double*)m->ec, fba->dbox,
                            T, R.rv);
-        R.smooth(m->opt.num_post_smooth, w);
+        R.smooth(2 * m->opt.num_post_smooth, w);
         R.residual();
         KSGF_LAUNCH_CHECK();
         if ((e = R.norm(&r))) return e;

@@ -1873,10 +1873,11 @@ void ko_fillpatch_faces_two_levels(const ko_geom* fg, double* fine, int itype, i
 /*    one-sided where a coarse neighbour is covered or outside;       */
 /*  - uncovered coarse cells next to the fine level take the average  */
 /*    of the fine fluxes on the coarse-fine faces (reflux);            */
-/*  - iteration: fine damped-Jacobi smoothing, a coarse correction of  */
-/*    the composite residual (average of the fine residual under the   */
-/*    fine level) by one single-level V-cycle, piecewise-constant      */
-/*    prolongation, post-smoothing.                                    */
+/*  - iteration: fine damped-Jacobi smoothing (each MLMG smoothing    */
+/*    step a pair of sweeps: 2 nu1 before, 2 nu2 after), a coarse      */
+/*    correction of the composite residual (average of the fine        */
+/*    residual under the fine level) by one single-level V-cycle,      */
+/*    piecewise-constant prolongation, post-smoothing.                 */
 /* All arrays are domain-wide, Fortran order: coarse n, fine 2n.      */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -2241,7 +2242,7 @@ int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, 
     info->converged = (r <= target);
     int ret = 0;
     for (int it = 1; !info->converged && it <= opts->maxiter; ++it) {
-        tl_smooth(&T, pc, pf, bc, bf, Rc, Rf, opts->nu1);
+        tl_smooth(&T, pc, pf, bc, bf, Rc, Rf, 2 * opts->nu1);
         tl_residual(&T, pc, pf, bc, bf, Rc, Rf);
         /* coarse rhs: composite residual, the fine residual averaged under
          * the fine level; one V-cycle of the coarse operator */
@@ -2267,7 +2268,7 @@ int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, 
                     const long f = TLF(&T, i, j, k);
                     if (T.fm[f]) pf[f] += ec[TLC(&T, i / 2, j / 2, k / 2)];
                 }
-        tl_smooth(&T, pc, pf, bc, bf, Rc, Rf, opts->nu2);
+        tl_smooth(&T, pc, pf, bc, bf, Rc, Rf, 2 * opts->nu2);
         tl_residual(&T, pc, pf, bc, bf, Rc, Rf);
         r = tl_norm(&T, Rc, Rf);
         info->iters = it;
