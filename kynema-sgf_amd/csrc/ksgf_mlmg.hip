@@ -1073,7 +1073,7 @@ static void launch_sub_mean(double* a, const LevelGeom& L, double* part, hipStre
 
 // planes per workgroup of the row-tile kernels
 constexpr int kRestrictKz = 4;
-constexpr int kResidualKzMin = 1;  // candidates 1, 2, 4, 8, 16, 32 planes per thread
+constexpr int kResidualKzMin = 1;  // candidates 1, 2, 4, 8, 16, 32, 64 planes per thread
 
 template <typename T>
 static PkArgs<T> pk_args(const ksgf_macproj* mp, int l, T* x, const T* b, const T* xc)
@@ -1172,7 +1172,7 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
     const std::array<long long, 4> key{L.n0, L.n1, L.n2, (long long)sizeof(T)};
     auto it = kz_cache().find(key);
     if (it != kz_cache().end()) return it->second;
-    const int cand[4] = {1, 2, 4, 8};
+    const int cand[5] = {1, 2, 4, 8, 16};
     int best = 1;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
@@ -1226,7 +1226,7 @@ static int tune_residual(ksgf_macproj* mp)
         return best;
     }
     float best_ms = 1e30f;
-    for (int kz : {1, 2, 4, 8, 16, 32}) {
+    for (int kz : {1, 2, 4, 8, 16, 32, 64}) {
         if (kz < kResidualKzMin || kz > std::max(P.n2, kResidualKzMin)) continue;
         const RowTiles t = row_tiles(P.n0, P.n1, P.n2, kz);
         const int nb = t.ntx * t.nty * t.nkc;
@@ -1259,7 +1259,7 @@ static int tune_residual(ksgf_macproj* mp)
 }
 
 // Autotuner of the coarse planes per thread of the residual + restriction
-// from level l (candidates 1, 2, 4, 8), timed on the level's own correction
+// from level l (candidates 1, 2, 4, 8, 16), timed on the level's own correction
 // and residual arrays into the next level's residual (all scratch at set-up),
 // cached per (fine shape, coarse shape, precision)
 template <typename T>
@@ -1291,7 +1291,7 @@ static int tune_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* out
         (void)hipEventDestroy(e0);
         return best;
     }
-    for (int kz : {1, 2, 4, 8}) {
+    for (int kz : {1, 2, 4, 8, 16}) {
         if (kz > C.n2) continue;
         launch_restrict<T>(mp, l, x, b, out, kz, nullptr);
         (void)hipEventRecord(e0, nullptr);
