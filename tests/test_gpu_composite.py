@@ -44,6 +44,7 @@ def _random_faces(rng, m, walls_y):
 CASES = [
     ("neumann_y", (P_, N_, P_), (P_, N_, P_), 1.3),
     ("dirichlet_ylo", (P_, D_, P_), (P_, N_, P_), 0.7),
+    ("dirichlet_y", (P_, D_, P_), (P_, D_, P_), 1.0),  # a level-1 box on the high Dirichlet wall
 ]
 
 
