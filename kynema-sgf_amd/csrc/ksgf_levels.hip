@@ -562,11 +562,14 @@ struct TLRun {
     MFDev rf;
     RegDev rv;  // fine valid cells
     int nv_chunks;
+    bool ghosts_ok = false;  // the current phi buffer's ghosts match phi and the coarse solution
 
     const MFDev& pf() const { return pfm[m->cur]; }
     const CMFDev& cpf() const { return cpfm[m->cur]; }
     void ghosts()
     {
+        if (ghosts_ok) return;
+        ghosts_ok = true;
         if (m->ngl == 0) return;
         hipLaunchKernelGGL(k_tl_ghosts, dim3(grid_for(m->ngl, 256)), dim3(256), 0, st, pf(), m->fba->dbox,
                            (const int2*)m->gl, m->ngl, m->fown, (const double*)m->pc, (const unsigned char*)m->cv,
@@ -597,6 +600,7 @@ struct TLRun {
             hipLaunchKernelGGL(k_tl_march<true>, dim3(m->ncols_chunks), dim3(256), 0, st, cpf(), bf, pfm[1 - m->cur],
                                m->fba->dbox, m->T, w, ColDev{m->cols});
             m->cur ^= 1;
+            ghosts_ok = false;
         }
     }
     int norm(double* out)
@@ -862,6 +866,7 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
         hipLaunchKernelGGL(k_tl_addc, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const double*)m->ec, m->nc);
         hipLaunchKernelGGL(k_tl_prolong, dim3(R.nv_chunks), dim3(256), 0, st, R.pf(), (const double*)m->ec, fba->dbox,
                            T, R.rv);
+        R.ghosts_ok = false;
         R.smooth(2 * m->opt.num_post_smooth, w);
         R.residual();
         KSGF_LAUNCH_CHECK();
@@ -888,6 +893,7 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     if (m->singular && (e = R.sub_mean(m->pc, R.pf(), R.cpf(), 1))) return e;
     // u -= beta grad phi on both levels, then the fine average on the coarse
     // faces under and around the fine level
+    R.ghosts_ok = false;
     R.ghosts();
     for (int d = 0; d < 3; ++d) {
         int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
