@@ -147,8 +147,9 @@ def cpu_baseline(n_cell):
     G, vel, rho, gp = O.make_state(case)
     t0 = time.perf_counter()
     reps = 0
+    clock = {}
     while True:
-        vel, d = O.advect_project_step(case, G, vel, rho, gp)
+        vel, d = O.advect_project_step(case, G, vel, rho, gp, clock=clock)
         reps += 1
         if time.perf_counter() - t0 > 10.0 or reps >= 10:
             break
@@ -166,8 +167,9 @@ def cpu_baseline(n_cell):
         G1, v1, r1, g1 = O.make_state(case)
         t1 = time.perf_counter()
         n1 = 0
+        clock1 = {}
         while True:
-            v1, _ = O.advect_project_step(case, G1, v1, r1, g1)
+            v1, _ = O.advect_project_step(case, G1, v1, r1, g1, clock=clock1)
             n1 += 1
             if time.perf_counter() - t1 > 5.0 or n1 >= 2:
                 break
@@ -501,35 +503,11 @@ def levels_project(steps, warmup, no_cpu):
 
 
 def _cells_on_device(ba, ncomp, ng, fn):
-    """cell MultiFab (ncomp, ngrow ng) on the device, each FAB (ghosts
-    included) filled with fn(c, X, Y, Z) at its cell centres"""
-    import torch
-
-    out = torch.empty(ba.mf_size(0, ncomp, ng), dtype=torch.float64, device="cuda")
-    lo, n = ba.fab_lo_n(0, ng)
-    off = ba.fab_offsets(0, ncomp, ng)
-    h = ba.geom.dx
-    for b in range(ba.nbox):
-        ax = [(int(lo[b, e]) + torch.arange(int(n[b, e]), dtype=torch.float64, device="cuda") + 0.5) * h[e]
-              for e in range(3)]
-        Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
-        out[int(off[b]):int(off[b + 1])] = torch.stack([fn(c, X, Y, Z) for c in range(ncomp)]).reshape(-1)
-    return out
+    return importlib.import_module("kynema-sgf_amd.workloads").cells_on_device(ba, ncomp, ng, fn)
 
 
 def _tgv_cells(geom):
-    """Taylor-Green velocity with wavenumbers 2 pi / L per axis and a w
-    component, periodic on the domain"""
-    import torch
-
-    k = [2.0 * np.pi / (geom.prob_hi[e] - geom.prob_lo[e]) for e in range(3)]
-
-    def fn(c, X, Y, Z):
-        u = torch.sin(k[0] * X) * torch.cos(k[1] * Y) * torch.cos(k[2] * Z)
-        v = -torch.cos(k[0] * X) * torch.sin(k[1] * Y) * torch.cos(k[2] * Z)
-        return u if c == 0 else (v if c == 1 else 0.1 * u * v)
-
-    return fn
+    return importlib.import_module("kynema-sgf_amd.workloads").tgv_cells(geom)
 
 
 def levels_step(steps, warmup, no_cpu):
@@ -603,8 +581,9 @@ def levels_step(steps, warmup, no_cpu):
         t0 = time.perf_counter()
         reps = 0
         vel_c, vel_f = vel
+        clock = {}
         while True:
-            vel_c, vel_f, rinfo = O.two_level_step(c2, f2, vel_c, vel_f, gp_c, gp_f)
+            vel_c, vel_f, rinfo = O.two_level_step(c2, f2, vel_c, vel_f, gp_c, gp_f, clock=clock)
             reps += 1
             if time.perf_counter() - t0 > 10.0 or reps >= 5:
                 break

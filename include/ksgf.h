@@ -404,6 +404,15 @@ int ksgf_set_bndry(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, in
 int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out,
                  ksgf_stream_t stream);
 
+/* incflo::compute_dt forcing part (incflo_compute_dt.cpp:168-198, taken when
+ * time.use_force_cfl, the default): max over valid cells of
+ * max_d |src_d| / dx_d / rho, with src the ICNS source term the step last
+ * formed (momentum form).  src: cell, 3 comp; rho: 1 comp.  Synchronises.
+ * SimTime::set_current_cfl (SimTime.cpp:162-205) then takes
+ * dt = 2 cfl / (C + sqrt(C^2 + 4 F)), capped at (1 + max_dt_growth) dt_prev. */
+int ksgf_force_cfl_max(const ksgf_boxarray* ba, const double* src, int ng_src, const double* rho,
+                       int ng_rho, double* out, ksgf_stream_t stream);
+
 /* max over valid cells of |div u_mac| (the post-projection check the MAC
  * projector's convergence implies; diagnostic, synchronises).  u/v/w_mac:
  * faces, 1 comp, ngrow ngrow_mac. */

@@ -55,7 +55,8 @@ EXPORTS = [
     "ksgf_fill_boundary", "ksgf_fill_physbc", "ksgf_set_inflow_velocity", "ksgf_ppm_predict_state_on_face",
     "ksgf_extrap_vel_to_faces", "ksgf_compute_fluxes_from_state", "ksgf_compute_divergence",
     "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
-    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_predictor_update",
+    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_force_cfl_max",
+    "ksgf_predictor_update",
     "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
     "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
     "ksgf_adjoint_create", "ksgf_adjoint_forward", "ksgf_adjoint_backward", "ksgf_adjoint_destroy",
@@ -72,8 +73,22 @@ EXPORTS = [
 _lib = None
 
 
+# status codes of include/ksgf.h
+KSGF_OK = 0
+KSGF_ERR_INVALID = -1
+KSGF_ERR_UNSUPPORTED = -2
+KSGF_ERR_HIP = -3
+KSGF_ERR_NOMEM = -4
+KSGF_ERR_NOT_CONVERGED = 1
+KSGF_ERR_DIVERGED = -5
+
+
 class KsgfError(RuntimeError):
-    pass
+    """A failed call; ``code`` is the ksgf status code where there is one."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load(path=None):
@@ -135,6 +150,7 @@ def load(path=None):
     L.ksgf_macproj_levels_get_phi.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_macproj_levels_destroy.argtypes = [C.c_void_p]
     L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_force_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
                                         _dp, C.c_double, C.c_double, C.c_int, C.c_void_p]
     L.ksgf_icns_src_term.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_void_p]
@@ -198,7 +214,7 @@ def prof_read():
 def check(rc, what="ksgf call", allow=(0,)):
     if rc not in allow:
         msg = load().ksgf_last_error().decode()
-        raise KsgfError(f"{what} failed ({rc}): {msg}")
+        raise KsgfError(f"{what} failed ({rc}): {msg}", rc)
     return rc
 
 
@@ -235,6 +251,29 @@ def default_options(**kw):
     for k, v in kw.items():
         setattr(o, k, v)
     return o
+
+
+DT_GROWTH = 0.1  # time.max_dt_growth default (SimTime.H:248)
+
+
+def set_current_cfl(cfl, conv_cfl, force_cfl, dt_prev=0.0, dt_growth=DT_GROWTH):
+    """SimTime::set_current_cfl (SimTime.cpp:162-205) for an adaptive step
+    after initialisation, without a diffusive CFL (Godunov with implicit or
+    Crank-Nicolson diffusion) and without output-time clipping:
+    dt = 2 cfl / (C + sqrt(C^2 + 4 F)), then at most (1 + dt_growth) dt_prev
+    once a previous step exists.  Raises where the reference aborts (CFL
+    below machine epsilon)."""
+    import math
+
+    eps = 2.220446049250313e-16
+    cd = conv_cfl + 0.0
+    unit = cd + math.sqrt((cd * cd) + (4.0 * force_cfl))
+    if unit < eps:
+        raise KsgfError("CFL is below machine epsilon and the time step is adaptive: set a fixed dt")
+    dt = 2.0 * cfl / max(unit, eps)
+    if dt_prev > 0.0:
+        dt = min(dt, (1.0 + dt_growth) * dt_prev)
+    return dt
 
 
 class DeviceBoxArray:
@@ -348,6 +387,12 @@ class DeviceBoxArray:
     def cfl_max(self, vel, ngrow):
         out = C.c_double()
         check(load().ksgf_cfl_max(self.h, _ptr(vel), ngrow, C.byref(out), _stream()), "cfl_max")
+        return out.value
+
+    def force_cfl_max(self, src, ng_src, rho, ng_rho):
+        out = C.c_double()
+        check(load().ksgf_force_cfl_max(self.h, _ptr(src), ng_src, _ptr(rho), ng_rho, C.byref(out), _stream()),
+              "force_cfl_max")
         return out.value
 
     def mac_divergence_max(self, mac, ngrow=1):

@@ -450,6 +450,29 @@ int ksgf_boxarray_create_level(const ksgf_geometry* geom, const int* boxes, int 
         }
         covered += c;
     }
+    // disjointness: boxes sorted by their low z, each tested against the
+    // boxes that start below its top (with disjoint boxes, a cell count equal
+    // to the domain's then means an exact tiling)
+    {
+        std::vector<int> order(nbox);
+        for (int b = 0; b < nbox; ++b) order[b] = b;
+        std::sort(order.begin(), order.end(), [&](int a, int c) { return boxes[6 * a + 2] < boxes[6 * c + 2]; });
+        for (int x = 0; x < nbox; ++x) {
+            const int* a = boxes + 6 * order[x];
+            for (int y = x + 1; y < nbox; ++y) {
+                const int* c = boxes + 6 * order[y];
+                if (c[2] > a[5]) break;
+                bool meet = true;
+                for (int d = 0; d < 3; ++d) meet = meet && !(c[3 + d] < a[d] || a[3 + d] < c[d]);
+                if (meet) {
+                    delete ba;
+                    set_error("ksgf_boxarray_create: boxes " + std::to_string(order[x]) + " and " +
+                              std::to_string(order[y]) + " overlap");
+                    return KSGF_ERR_INVALID;
+                }
+            }
+        }
+    }
     ba->covers_domain = covers_domain != 0;
     if (covers_domain ? (covered != ncell) : (covered > ncell)) {
         delete ba;

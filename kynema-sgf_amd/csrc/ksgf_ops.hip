@@ -33,6 +33,32 @@ __global__ void __launch_bounds__(256) k_cfl(CMFDev v, RegDev r, const int* __re
     }
 }
 
+__global__ void __launch_bounds__(256) k_force_cfl(CMFDev s, CMFDev rho, RegDev r, const int* __restrict__ box,
+                                                   double dxi, double dyi, double dzi,
+                                                   unsigned long long* __restrict__ out)
+{
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < r.total; t += (long long)gridDim.x * blockDim.x) {
+        int b, i, j, k;
+        region_locate_flat(r, box, t, b, i, j, k);
+        CFab f = fab_at(s, box, b);
+        const double rr = fab_at(rho, box, b)(i, j, k);
+        double a = fabs(f(i, j, k, 0)) * dxi / rr;
+        double bb = fabs(f(i, j, k, 1)) * dyi / rr;
+        double e = fabs(f(i, j, k, 2)) * dzi / rr;
+        m = fmax(m, fmax(a, fmax(bb, e)));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) q = fmax(q, sh[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(q));
+    }
+}
+
 __global__ void __launch_bounds__(256) k_predictor(MFDev vn, CMFDev vo, CMFDev ro, CMFDev rn, CMFDev conv,
                                                    CMFDev src, CMFDev diff, int has_diff, double kappa, double dt,
                                                    int ncomp, RegDev r, const int* __restrict__ box)
@@ -418,6 +444,39 @@ int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* 
     (void)h;
     // ParReduce floor of -1 (incflo_compute_dt.cpp:86-90): an empty or zero
     // field reports 0 here, which the caller treats as "no constraint"
+    *out = m;
+    return KSGF_OK;
+}
+
+int ksgf_force_cfl_max(const ksgf_boxarray* ba, const double* src, int ng_src, const double* rho, int ng_rho,
+                       double* out, ksgf_stream_t stream)
+{
+    if (!ba || !src || !rho || !out || ng_src < 0 || ng_rho < 0) {
+        set_error("ksgf_force_cfl_max: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev s, rh;
+    int e = cmfdev(ba, src, KSGF_CELL, 3, ng_src, &s);
+    if (e) return e;
+    if ((e = cmfdev(ba, rho, KSGF_CELL, 1, ng_rho, &rh))) return e;
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* r = get_region(ba, z, z, &err);
+    if (!r) return err;
+    unsigned long long* d = (unsigned long long*)get_scratch(ba, 42, 1, &err);
+    if (!d) return err;
+    KSGF_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
+    int nb = grid_for(r->total, 256);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_force_cfl, dim3(nb), dim3(256), 0, st, s, rh, regdev(r, ba->nbox), ba->dbox,
+                       1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], d);
+    KSGF_LAUNCH_CHECK();
+    unsigned long long bits = 0;
+    KSGF_HIP_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    double m;
+    std::memcpy(&m, &bits, sizeof(double));
     *out = m;
     return KSGF_OK;
 }

@@ -2,7 +2,8 @@
 // the predictor update, on the C++ host layer: the advect + MAC-project step
 // without the diffusion solve and the nodal projection.
 //
-//   compute_dt (convective CFL)        incflo_compute_dt.cpp:69-91, SimTime.cpp:162-193
+//   compute_dt (convective and forcing CFL, dt growth cap)
+//                                      incflo_compute_dt.cpp:33-220, SimTime.cpp:162-205
 //   src = -gp/rho (velocity form), 1-ghost fill
 //                                      icns_ops.H:98-111, incflo_advance.cpp:283-291
 //   AdvectionOp<ICNS,Godunov>::preadvect
@@ -41,7 +42,9 @@
 //   compute_predictor_rhs              CompRHSOps.H:93-117
 #pragma once
 
+#include <algorithm>
 #include <cmath>
+#include <limits>
 #include <memory>
 #include <utility>
 
@@ -118,9 +121,18 @@ public:
     void setStream(hipStream_t st) { st_ = st; }
     const BoxArray& boxArray() const { return ba_; }
 
-    // domain-wide host arrays: vel (3 x N), rho (N), gp (3 x N)
+    // domain-wide host arrays: vel (3 x N), rho (N), gp (3 x N).  The density
+    // must equal rho_0 everywhere: the projections here use beta = 1 / rho_0
+    // and sigma = dt / rho_0, where the reference switches to face and cell
+    // coefficients 1 / rho (icns_advection.cpp:234-291,
+    // incflo_apply_nodal_projection.cpp:259-290) once density varies.
     void setState(const double* vel, const double* rho, const double* gp)
     {
+        const size_t N = (size_t)ba_.geom().domain.numPts();
+        for (size_t c = 0; c < N; ++c)
+            if (rho[c] != rho0_)
+                throw UnsupportedError("setState: density differs from rho_0 (variable-density projections are not "
+                                       "provided)");
         vel_->copyFromDomain(vel);
         rho_.copyFromDomain(rho);
         gp_.copyFromDomain(gp);
@@ -128,6 +140,20 @@ public:
         vel_->FillPhysBC(vel_bc_, vel_bcval_, st_);
         rho_.FillBoundary(st_);
         rho_.FillPhysBC(rho_bc_, std::vector<double>(6, 0.0), st_);
+        // the forcing compute_dt reads before the first step: the source
+        // term (momentum form) of the loaded gp, as the initial iterations
+        // leave src_term (incflo_advance.cpp:367, icns_ops.H:98-120)
+        check(ksgf_icns_src_term(ba_.handle(), src_nph_.data(), 0, gp_.data(), rho_.data(), 3, 1,
+                                 (ksgf_stream_t)st_),
+              "setState: src_term");
+        dt_prev_ = 0.0;
+    }
+    // time.max_dt_growth (SimTime.H:248, default 0.1) and time.use_force_cfl
+    // (SimTime.H:332, default true)
+    void setTimeOptions(double max_dt_growth, bool use_force_cfl)
+    {
+        dt_growth_ = max_dt_growth;
+        use_force_cfl_ = use_force_cfl;
     }
     void getVelocity(double* vel) const
     {
@@ -286,12 +312,21 @@ public:
         // compute_dt
         double dt = fixed_dt;
         if (!(dt > 0.0)) {
-            double c = 0.0;
+            // incflo::compute_dt: convective CFL of the velocity, forcing CFL
+            // of the source term the last step formed; then
+            // SimTime::set_current_cfl (no diffusive CFL: Godunov treats
+            // viscosity implicitly or by Crank-Nicolson)
+            double c = 0.0, f = 0.0;
             check(ksgf_cfl_max(h, vel_->data(), 3, &c, st), "compute_dt");
-            if (!(c > 0.0)) throw std::runtime_error("compute_dt: CFL below machine epsilon, set a fixed dt");
-            const double cd = c;
-            dt = 2.0 * cfl_ / (cd + std::sqrt(cd * cd));
+            if (use_force_cfl_) check(ksgf_force_cfl_max(h, src_nph_.data(), 0, rho_.data(), 3, &f, st), "compute_dt");
+            const double cd = c + 0.0;
+            const double unit = cd + std::sqrt((cd * cd) + (4.0 * f));
+            const double eps = std::numeric_limits<double>::epsilon();
+            if (unit < eps) throw std::runtime_error("compute_dt: CFL below machine epsilon, set a fixed dt");
+            dt = 2.0 * cfl_ / std::max(unit, eps);
+            if (dt_prev_ > 0.0) dt = std::min(dt, (1.0 + dt_growth_) * dt_prev_);
         }
+        dt_prev_ = dt;
         s.dt = dt;
         // explicit viscous term of the old state (Crank-Nicolson)
         const bool cn = (diff_type_ == 1);
@@ -434,6 +469,8 @@ private:
     std::vector<BCRec> vel_bc_, src_bc_, rho_bc_;
     std::vector<double> vel_bcval_;
     double rho0_, cfl_, rtol_, atol_;
+    double dt_prev_ = 0.0, dt_growth_ = 0.1;
+    bool use_force_cfl_ = true;
     hipStream_t st_ = nullptr;
     bool check_div_ = false;
     std::unique_ptr<MultiFab> vel_, vel_new_;

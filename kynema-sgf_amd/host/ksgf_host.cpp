@@ -74,6 +74,9 @@ int ksgf_host_stepper_set_state(ksgf_host_stepper* s, const double* vel, const d
         if (!s || !vel || !rho || !gp) throw std::runtime_error("null argument");
         reinterpret_cast<GodunovPredictor*>(s)->setState(vel, rho, gp);
         return KSGF_OK;
+    } catch (const ksgf::amr::UnsupportedError& e) {
+        g_err = std::string("ksgf_host_stepper_set_state: ") + e.what();
+        return KSGF_ERR_UNSUPPORTED;
     } catch (const std::exception& e) {
         g_err = std::string("ksgf_host_stepper_set_state: ") + e.what();
         return KSGF_ERR_INVALID;

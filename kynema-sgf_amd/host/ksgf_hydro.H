@@ -32,6 +32,12 @@
 
 namespace ksgf::amr {
 
+// a request that is valid in the reference but not provided on this path
+// (the C entry points of the host layer return KSGF_ERR_UNSUPPORTED for it)
+struct UnsupportedError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 inline void check(int rc, const char* what)
 {
     if (rc < 0) throw std::runtime_error(std::string(what) + ": " + ksgf_last_error());

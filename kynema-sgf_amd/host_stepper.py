@@ -53,7 +53,7 @@ def host_lib():
 
 def _chk(rc, what, allow=(0,)):
     if rc not in allow:
-        raise KsgfError(f"{what} failed ({rc}): {host_lib().ksgf_host_last_error().decode()}")
+        raise KsgfError(f"{what} failed ({rc}): {host_lib().ksgf_host_last_error().decode()}", rc)
 
 
 def _dp(a):

@@ -4,7 +4,8 @@ the C ABI.
 The Godunov branch of incflo::ApplyPredictor with every operator looping over
 the levels (level 1 at ratio 2, one dt for both levels, constant density):
 
-  compute_dt: the convective CFL over both levels  incflo_compute_dt.cpp:69-91
+  compute_dt: convective and forcing CFL over both levels, dt growth cap
+                                                   incflo_compute_dt.cpp:33-220, SimTime.cpp:162-205
   src = -gp/rho; level-1 ghosts by fill-patch      icns_ops.H:98-111
   ExtrapVelToFaces per level                       icns_advection.H:210-217
   set_inflow_velocity                              icns_advection.cpp:157-183
@@ -21,11 +22,9 @@ interpolation), its ghost faces from ksgf_fillpatch_faces_two_levels.
 """
 from __future__ import annotations
 
-import math
-
 import numpy as np
 
-from . import CELL, XFACE, DeviceBoxArray, MacProjectorLevels, default_options
+from . import CELL, XFACE, DeviceBoxArray, MacProjectorLevels, default_options, set_current_cfl
 
 
 class _Level:
@@ -39,6 +38,7 @@ class _Level:
         self.gp.zero_()
         self.src = dba.alloc(CELL, 3, 1)
         self.src_nph = dba.alloc(CELL, 3, 0)
+        self.src_nph.zero_()
         self.q = dba.alloc(CELL, 3, 3)
         self.fq = dba.alloc(CELL, 3, 1)
         self.mac = [dba.alloc(XFACE + d, 1, 1) for d in range(3)]
@@ -66,6 +66,7 @@ class TwoLevelStep:
         self.mp = MacProjectorLevels(self.dc, self.df, case.lobc, case.hibc, 1.0 / case.rho0, self.options)
         self.last_info = None
         self.last_dt = None
+        self.dt_prev = 0.0
 
     def load(self, vel_c, vel_f, gp_c=None, gp_f=None):
         """Set the levels' velocity MultiFabs (3 ghosts) and pressure
@@ -84,6 +85,11 @@ class TwoLevelStep:
             put(L0.gp, gp_c, self.dc)
         if gp_f is not None:
             put(L1.gp, gp_f, self.df)
+        # the forcing compute_dt reads before the first step: the source term
+        # (momentum form) of the loaded gp
+        for L in self.lev:
+            L.dba.icns_src_term(L.src_nph, 0, L.gp, L.rho, 3, 1)
+        self.dt_prev = 0.0
 
     def _fill_vel(self, which):
         L0, L1 = self.lev
@@ -98,8 +104,10 @@ class TwoLevelStep:
         self._fill_vel("vel")
         if dt is None:
             cc = max(dc.cfl_max(L0.vel, 3), df.cfl_max(L1.vel, 3))
-            dt = 2.0 * c.cfl / (cc + math.sqrt(cc * cc))
+            ff = max(L.dba.force_cfl_max(L.src_nph, 0, L.rho, 3) for L in self.lev)
+            dt = set_current_cfl(c.cfl, cc, ff, self.dt_prev)
         self.last_dt = dt
+        self.dt_prev = dt
         for L in self.lev:
             L.dba.icns_src_term(L.src, 1, L.gp, L.rho, 3, 0)
         dc.fill_ghosts(L0.src, CELL, 3, 1, self.src_bc, np.zeros((6, 3)))

@@ -4,7 +4,7 @@ Mirrors the Godunov branch of incflo::ApplyPredictor
 (incflo_advance.cpp:186-438) up to the predictor update, without the
 diffusion solve and the nodal projection:
 
-  compute_dt (conv CFL)                      incflo_compute_dt.cpp:69-91
+  compute_dt (conv + forcing CFL, growth cap) incflo_compute_dt.cpp:33-220, SimTime.cpp:162-205
   src = -gp/rho, ghost fill                  icns_ops.H:98-111, incflo_advance.cpp:283-291
   ExtrapVelToFaces                           icns_advection.H:211-216
   set_inflow_velocity                        icns_advection.cpp:157-183
@@ -22,11 +22,10 @@ s_new = (rho_o s_o + dt conv) / rho_n.
 """
 from __future__ import annotations
 
-import math
-
 import numpy as np
 
-from . import CELL, XFACE, DeviceBoxArray, MacProjector, default_options
+from . import (CELL, KSGF_ERR_UNSUPPORTED, XFACE, DeviceBoxArray, KsgfError, MacProjector, default_options,
+               set_current_cfl)
 
 
 class AdvectProjectStep:
@@ -61,6 +60,8 @@ class AdvectProjectStep:
         self.mp = MacProjector(dba, case.lobc, case.hibc, 1.0 / case.rho0, self.options)
         self.last_info = None
         self.last_dt = None
+        self.dt_prev = 0.0
+        self.src_nph.zero_()
         self._torch = torch
         self.scalar = scalar
         if scalar:
@@ -83,11 +84,21 @@ class AdvectProjectStep:
         vel = c.vel if vel is None else vel
         rho = c.rho if rho is None else rho
         gp = c.gp if gp is None else gp
+        if np.any(np.asarray(rho) != c.rho0):
+            # the projections use beta = 1/rho_0 (and sigma = dt/rho_0); the
+            # reference switches to face coefficients 1/rho once density
+            # varies (icns_advection.cpp:234-291)
+            raise KsgfError("AdvectProjectStep.load: density differs from rho_0 (variable-density projections are "
+                            "not provided)", KSGF_ERR_UNSUPPORTED)
         self.vel.copy_(dba.upload(ba.scatter(vel, CELL, 3)))
         dba.fill_ghosts(self.vel, CELL, 3, 3, self.bcrec, self.bcval)
         self.rho.copy_(dba.upload(ba.scatter(rho, CELL, 3)))
         dba.fill_ghosts(self.rho, CELL, 1, 3, self.rho_bc, np.zeros((6, 1)))
         self.gp.copy_(dba.upload(ba.scatter(gp, CELL, 0)))
+        # the forcing compute_dt reads before the first step: the source term
+        # of the loaded gp (as the initial iterations leave src_term)
+        dba.icns_src_term(self.src_nph, 0, self.gp, self.rho, 3, 1)
+        self.dt_prev = 0.0
 
     def load_scalar(self, s):
         """Upload a domain-wide passive scalar (1, nz, ny, nx) and fill ghosts."""
@@ -98,8 +109,10 @@ class AdvectProjectStep:
         c, dba = self.case, self.dba
         if dt is None:
             cc = dba.cfl_max(self.vel, 3)
-            dt = 2.0 * c.cfl / (cc + math.sqrt(cc * cc))
+            ff = dba.force_cfl_max(self.src_nph, 0, self.rho, 3)
+            dt = set_current_cfl(c.cfl, cc, ff, self.dt_prev)
         self.last_dt = dt
+        self.dt_prev = dt
         dba.icns_src_term(self.src, 1, self.gp, self.rho, 3, 0)
         dba.fill_ghosts(self.src, CELL, 3, 1, self.src_bc, np.zeros((6, 3)))
         dba.extrap_vel_to_faces(self.vel, self.src, self.bcrec, dt, 1, 0, self.limiter, 0, mac=self.mac)

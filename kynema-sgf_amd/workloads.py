@@ -84,6 +84,44 @@ def taylor_green(n=64, max_grid_size=32, noise=1e-3, seed=1, cfl=0.5, nsteps=1):
                 lobc, hibc, cfl, nsteps)
 
 
+def ctv_exact(geom, t, u0=1.0, v0=1.0, omega=0.0):
+    """The convecting Taylor vortex at time t on the cell centres of geom
+    (ConvectingTaylorVortex.cpp:17-83): u = u0 - cos(pi (x - u0 t)) sin(pi (y - v0 t)) e^{-2 omega t},
+    v = v0 + sin(pi (x - u0 t)) cos(pi (y - v0 t)) e^{-2 omega t}, w = 0, and the
+    pressure gradient gpx = pi/2 sin(2 pi (x - u0 t)) e^{-4 omega t}, gpy alike.
+    Returns (vel, gp), each (3, nz, ny, nx)."""
+    n = geom.n_cell
+    x = geom.cell_centers(0)[None, None, :]
+    y = geom.cell_centers(1)[None, :, None]
+    pi = np.pi
+    ex2, ex4 = np.exp(-2.0 * omega * t), np.exp(-4.0 * omega * t)
+    u = u0 - np.cos(pi * (x - u0 * t)) * np.sin(pi * (y - v0 * t)) * ex2
+    v = v0 + np.sin(pi * (x - u0 * t)) * np.cos(pi * (y - v0 * t)) * ex2
+    gx = 0.5 * pi * np.sin(2.0 * pi * (x - u0 * t)) * ex4 + 0.0 * y
+    gy = 0.5 * pi * np.sin(2.0 * pi * (y - v0 * t)) * ex4 + 0.0 * x
+    sh = (int(n[2]), int(n[1]), int(n[0]))
+    z = np.zeros(sh)
+    vel = np.stack([np.broadcast_to(u, sh), np.broadcast_to(v, sh), z])
+    gp = np.stack([np.broadcast_to(gx, sh), np.broadcast_to(gy, sh), z])
+    return np.ascontiguousarray(vel), np.ascontiguousarray(gp)
+
+
+def convecting_taylor_vortex(n=32, nz=4, u0=1.0, v0=1.0, max_grid_size=32, cfl=0.5):
+    """The reference's convecting Taylor vortex verification case
+    (test/test_files/ctv_godunov_plm: [0, 2]^3, triply periodic, CTV.u0 =
+    CTV.v0 = 1, activate_pressure so gp starts exact), inviscid, on an
+    n x n x nz slab of cubic cells (the field does not depend on z).
+    extra['exact'](t) gives the exact (vel, gp) at time t."""
+    h = 2.0 / n
+    geom = Geometry((n, n, nz), (0.0, 0.0, 0.0), (2.0, 2.0, h * nz), (1, 1, 1))
+    ba = BoxArray(geom, chop_uniform(geom.n_cell, max_grid_size))
+    vel, gp = ctv_exact(geom, 0.0, u0, v0)
+    bc, bcval, lobc, hibc = _periodic_bcs()
+    c = Case(f"ctv{n}", geom, ba, vel, np.ones((1, nz, n, n)), gp, bc, bcval, lobc, hibc, cfl, 1)
+    c.extra["exact"] = lambda t: ctv_exact(geom, t, u0, v0)
+    return c
+
+
 def ragged(n_cell=(320, 320, 480), nbox=1500, edges=(8, 16, 24, 32, 48, 64), box_seed=4, noise=1e-3, seed=1,
            cfl=0.5, nsteps=1):
     """Ragged single-level layout: ``nbox`` boxes with edges drawn from
@@ -244,3 +282,37 @@ def differentiation(n=128, noise=1e-3, seed=1, sigma=0.1, cfl=0.5, nsteps=20, ce
     c.extra["dt"] = cfl / cmax
     c.name = f"diff{n}"
     return c
+
+
+def cells_on_device(ba, ncomp, ng, fn):
+    """A cell MultiFab (ncomp, ngrow ng) built on the device, each FAB
+    (ghosts included) filled with fn(c, X, Y, Z) at its cell centres (torch
+    tensors): large layouts without a domain-wide host array."""
+    import torch
+
+    out = torch.empty(ba.mf_size(0, ncomp, ng), dtype=torch.float64, device="cuda")
+    lo, n = ba.fab_lo_n(0, ng)
+    off = ba.fab_offsets(0, ncomp, ng)
+    h = ba.geom.dx
+    for b in range(ba.nbox):
+        ax = [(int(lo[b, e]) + torch.arange(int(n[b, e]), dtype=torch.float64, device="cuda") + 0.5) * h[e]
+              for e in range(3)]
+        Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        out[int(off[b]):int(off[b + 1])] = torch.stack([fn(c, X, Y, Z) for c in range(ncomp)]).reshape(-1)
+    return out
+
+
+def tgv_cells(geom):
+    """fn(c, X, Y, Z) of the two-level workload's velocity: Taylor-Green with
+    wavenumbers 2 pi / L per axis and a w component 0.1 u v, periodic on the
+    domain"""
+    import torch
+
+    k = [2.0 * np.pi / (geom.prob_hi[e] - geom.prob_lo[e]) for e in range(3)]
+
+    def fn(c, X, Y, Z):
+        u = torch.sin(k[0] * X) * torch.cos(k[1] * Y) * torch.cos(k[2] * Z)
+        v = -torch.cos(k[0] * X) * torch.sin(k[1] * Y) * torch.cos(k[2] * Z)
+        return u if c == 0 else (v if c == 1 else 0.1 * u * v)
+
+    return fn

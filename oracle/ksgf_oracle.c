@@ -31,6 +31,13 @@
 
 #define SMALL_VEL 1.0e-8
 
+/* Planted mistakes for the tests that show a check can fail (never set
+ * outside tests): bit 0 drops the transverse terms of ExtrapVelToFaces,
+ * bit 1 doubles them, bit 2 flips their sign, bit 3 drops the transverse
+ * terms of ComputeFluxesOnBoxFromState. */
+static int ko_plant = 0;
+void ko_set_plant(int bits) { ko_plant = bits; }
+
 /* ------------------------------------------------------------------ */
 /* layout                                                             */
 /* ------------------------------------------------------------------ */
@@ -465,6 +472,15 @@ int ko_extrap_vel_to_faces(const ko_geom* g, const double* vel, const double* f,
         dtdx[d] = dt / g->dx[d];
         c6[d] = dt / (6.0 * g->dx[d]);
         c4[d] = dt / (4.0 * g->dx[d]);
+        if (ko_plant & 1) c6[d] = c4[d] = 0.0;
+        if (ko_plant & 2) {
+            c6[d] *= 2.0;
+            c4[d] *= 2.0;
+        }
+        if (ko_plant & 4) {
+            c6[d] = -c6[d];
+            c4[d] = -c4[d];
+        }
     }
     const int* dlo = g->dlo;
     const int* dhi = g->dhi;
@@ -718,6 +734,7 @@ int ko_compute_fluxes_from_state(const ko_geom* g, int ncomp, const double* q,
         c6[d] = dt / (6.0 * g->dx[d]);
         c2[d] = dt / (2.0 * g->dx[d]);
         c4[d] = dt / (4.0 * g->dx[d]);
+        if (ko_plant & 8) c3[d] = c6[d] = c2[d] = c4[d] = 0.0;
     }
     area[0] = g->dx[1] * g->dx[2];
     area[1] = g->dx[0] * g->dx[2];
@@ -2411,6 +2428,37 @@ double ko_cfl_max(const ko_geom* g, const double* vel, int ng)
                 }
     }
     free(f);
+    return mx;
+}
+
+/* incflo::compute_dt forcing CFL (incflo_compute_dt.cpp:168-198, no mesh
+ * mapping, mask 1): max over valid cells and directions of
+ * |src_d| dxinv_d / rho, floor -1 as the reference's ParReduce. */
+double ko_force_cfl_max(const ko_geom* g, const double* src, int ng_src, const double* rho, int ng_rho)
+{
+    fab_t* f = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    fab_t* fr = (fab_t*)malloc(sizeof(fab_t) * (size_t)g->nbox);
+    make_fabs(g, KO_CELL, 3, ng_src, f);
+    make_fabs(g, KO_CELL, 1, ng_rho, fr);
+    const double dxi[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
+    double mx = -1.0;
+#pragma omp parallel for schedule(dynamic) reduction(max : mx)
+    for (int b = 0; b < g->nbox; ++b) {
+        const int* bx = g->boxes + 6 * b;
+        const fab_t* fb = &f[b];
+        for (int k = bx[2]; k <= bx[5]; ++k)
+            for (int j = bx[1]; j <= bx[4]; ++j)
+                for (int i = bx[0]; i <= bx[3]; ++i) {
+                    long c = FI(fb, i, j, k);
+                    const double r = rho[FI(&fr[b], i, j, k)];
+                    double a = fabs(src[c]) * dxi[0] / r;
+                    double bb = fabs(src[c + FS(fb)]) * dxi[1] / r;
+                    double e = fabs(src[c + 2 * FS(fb)]) * dxi[2] / r;
+                    mx = fmax(mx, fmax(a, fmax(bb, e)));
+                }
+    }
+    free(f);
+    free(fr);
     return mx;
 }
 

@@ -57,6 +57,10 @@ typedef struct {
 
 typedef struct { int lo[3]; int hi[3]; } ko_bcrec;
 
+/* Planted mistakes, for tests that show their check fails on a wrong
+ * oracle (0 = none; bits documented in ksgf_oracle.c). */
+void ko_set_plant(int bits);
+
 typedef struct {
     int maxiter, nu1, nu2, nuf, nub;
     int bottom_maxiter;
@@ -173,6 +177,7 @@ int ko_mac_project_two_levels(const int* n, const double* dxc, const int* lobc, 
                               double* phif_out);
 
 double ko_cfl_max(const ko_geom* g, const double* vel, int ng);
+double ko_force_cfl_max(const ko_geom* g, const double* src, int ng_src, const double* rho, int ng_rho);
 
 void ko_predictor_update(const ko_geom* g, double* vel_new, int ng_new, const double* vel_old,
                          int ng_old, const double* rho_old, int ng_ro, const double* rho_new,

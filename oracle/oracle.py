@@ -109,6 +109,9 @@ def lib():
                                                 C.POINTER(KoInfo), _dp, _dp]
         L.ko_cfl_max.restype = C.c_double
         L.ko_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int]
+        L.ko_force_cfl_max.restype = C.c_double
+        L.ko_force_cfl_max.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int]
+        L.ko_set_plant.argtypes = [C.c_int]
         L.ko_predictor_update.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
                                           _dp, _dp, _dp, C.c_double, C.c_double, C.c_int]
         _lib = L
@@ -385,6 +388,64 @@ def mac_project_two_levels(n, dxc, lobc, hibc, fboxes, beta, umc, umf, rtol=1e-1
     return dict(info.as_dict(), status=ret), phic, phif
 
 
+def set_plant(bits):
+    """Plant a mistake in the oracle (0 = none): bit 0 drops the transverse
+    terms of ExtrapVelToFaces, bit 1 doubles them, bit 2 flips their sign,
+    bit 3 drops the transverse terms of ComputeFluxesOnBoxFromState.  Used
+    only by tests that show a check fails on a wrong oracle."""
+    lib().ko_set_plant(int(bits))
+
+
+def force_cfl_max(G, src, ng_src, rho, ng_rho):
+    return lib().ko_force_cfl_max(C.byref(G.g), _p(src), ng_src, _p(rho), ng_rho)
+
+
+DT_GROWTH = 0.1  # time.max_dt_growth default (SimTime.H:248)
+
+
+def set_current_cfl(cfl, conv_cfl, force_cfl, dt_prev=0.0, dt_growth=DT_GROWTH):
+    """SimTime::set_current_cfl (SimTime.cpp:162-205) for an adaptive step
+    after initialisation, without a diffusive CFL (Godunov with implicit or
+    Crank-Nicolson diffusion) and without output-time clipping:
+    dt = 2 cfl / (C + sqrt(C^2 + 4 F)), then at most (1 + dt_growth) dt_prev
+    when a previous step exists."""
+    cd = conv_cfl + 0.0
+    unit = cd + math.sqrt((cd * cd) + (4.0 * force_cfl))
+    if unit < np.finfo(float).eps:
+        raise ValueError("CFL is below machine epsilon and the time step is adaptive")
+    dt = 2.0 * cfl / max(unit, np.finfo(float).eps)
+    if dt_prev > 0.0:
+        dt = min(dt, (1.0 + dt_growth) * dt_prev)
+    return dt
+
+
+def src_nph_of(ba, gp, rho):
+    """The ICNS source term in momentum form, (-gp / rho) rho on valid cells
+    (SrcTermOp<ICNS> then multiply_rho, icns_ops.H:98-120, PDEOps.H:94-113):
+    what src_term holds when the next step's compute_dt reads it."""
+    out = np.zeros(ba.mf_size(CELL, 3, 0))
+    for b in range(ba.nbox):
+        g_ = ba.fab_view(gp, b, CELL, 3, 0)
+        r_ = ba.fab_view(rho, b, CELL, 1, 3)[:, 3:-3, 3:-3, 3:-3]
+        ba.fab_view(out, b, CELL, 3, 0)[...] = (-(1.0 * g_) * (1.0 / r_)) * r_
+    return out
+
+
+def compute_dt(case, G, vel, rho, gp, clock=None):
+    """incflo::compute_dt + SimTime::set_current_cfl (incflo_compute_dt.cpp:33-220,
+    SimTime.cpp:162-205) with use_force_cfl (the default): the convective CFL
+    of vel, the forcing CFL of the source term the previous step formed
+    (clock["src_nph"]; before a first step, the one of the loaded gp), and
+    the dt growth cap against clock["dt_prev"]."""
+    clock = {} if clock is None else clock
+    src = clock.get("src_nph")
+    if src is None:
+        src = src_nph_of(case.ba, gp, rho)
+    c = cfl_max(G, vel, 3)
+    f = force_cfl_max(G, src, 0, rho, 3)
+    return set_current_cfl(case.cfl, c, f, clock.get("dt_prev", 0.0))
+
+
 def cfl_max(G, vel, ng):
     return lib().ko_cfl_max(G.ref, _p(vel), ng)
 
@@ -481,7 +542,7 @@ def _diffusion(case, ba, opts, vel, mu, dt_diff, rtol, atol, comp_apply=None):
     return np.stack(out), infos
 
 
-def two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f, dt=None, opts=None, limiter=1):
+def two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f, dt=None, opts=None, limiter=1, clock=None):
     """One advect + MAC-project step of two AMR levels (level 1 = fba at
     ratio 2, no subcycling), constant density case.rho0: the Godunov branch of
     ApplyPredictor with every operator looping over levels
@@ -491,7 +552,8 @@ def two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f, dt=None, opts=None, limi
     onto the coarse faces, :418-422 divergence, CompRHSOps.H:93-117 predictor
     per level), then average_down of the new velocity.  vel_c / vel_f: cell
     MultiFabs with 3 ghosts (filled here); gp_*: ngrow 0.  Returns (vel_c
-    new, vel_f new, diagnostics)."""
+    new, vel_f new, diagnostics).  clock: as for advect_project_step, with
+    the source terms of both levels."""
     ba = case.ba
     Gc, Gf = Geom(ba), Geom(fba)
     opts = opts or case_opts(case)
@@ -500,8 +562,14 @@ def two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f, dt=None, opts=None, limi
     fill_ghosts(Gc, vel_c, CELL, 3, 3, bcrec, bcval)
     fillpatch_two_levels(Gf, vel_f, 3, Gc, vel_c, 3, 3, 2, bcrec, bcval)
     if dt is None:
+        clk = {} if clock is None else clock
         c = max(cfl_max(Gc, vel_c, 3), cfl_max(Gf, vel_f, 3))
-        dt = 2.0 * case.cfl / (c + math.sqrt(c * c))
+        srcs = clk.get("src_nph")
+        rr = [np.full(b.mf_size(CELL, 1, 3), case.rho0) for b in (ba, fba)]
+        if srcs is None:
+            srcs = [src_nph_of(ba, gp_c, rr[0]), src_nph_of(fba, gp_f, rr[1])]
+        f = max(force_cfl_max(Gc, srcs[0], 0, rr[0], 3), force_cfl_max(Gf, srcs[1], 0, rr[1], 3))
+        dt = set_current_cfl(case.cfl, c, f, clk.get("dt_prev", 0.0))
     sbc = bcrec.copy()
     sbc[sbc != 0] = 2
     lev = []
@@ -555,14 +623,14 @@ def two_level_step(case, fba, vel_c, vel_f, gp_c, gp_f, dt=None, opts=None, limi
     for L in lev:
         b = L["ba"]
         conv = compute_divergence(L["G"], L["flux"], 3, -1.0, 0)
-        src_nph = np.zeros(b.mf_size(CELL, 3, 0))
-        for i in range(b.nbox):
-            g_ = b.fab_view(L["gp"], i, CELL, 3, 0)
-            r_ = b.fab_view(L["rho"], i, CELL, 1, 3)[:, 3:-3, 3:-3, 3:-3]
-            b.fab_view(src_nph, i, CELL, 3, 0)[...] = (-(1.0 * g_) * (1.0 / r_)) * r_
+        src_nph = src_nph_of(b, L["gp"], L["rho"])
+        L["src_nph"] = src_nph
         vn = np.zeros_like(L["vel"])
         predictor_update(L["G"], vn, 3, L["vel"], 3, L["rho"], 3, L["rho"], 3, conv, src_nph, None, 0.0, dt)
         out.append(vn)
+    if clock is not None:
+        clock["dt_prev"] = dt
+        clock["src_nph"] = [L["src_nph"] for L in lev]
     average_down(Gf, out[1], 3, Gc, out[0], 3, 3, 2)
     fill_ghosts(Gc, out[0], CELL, 3, 3, bcrec, bcval)
     fillpatch_two_levels(Gf, out[1], 3, Gc, out[0], 3, 3, 2, bcrec, bcval)
@@ -591,20 +659,23 @@ def _set_inflow(case, ba, mac):
 
 
 def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, scal=None, scal_bc=None,
-                        scal_bcval=None, diffusion=None):
+                        scal_bcval=None, diffusion=None, clock=None):
     """One step; returns (vel_new (ng 3, ghosts filled), diagnostics dict).
     With scal (cell, 1 comp, ng 3) the passive scalar is advected too; the
     new scalar is diag["scal"].  diffusion = dict(mu, type (1 Crank-Nicolson,
     2 Implicit), rtol, atol): the viscous term with constant mu and density
     rho_0 as ApplyPredictor carries it for Godunov (incflo_advance.cpp:246-291,
-    380-389)."""
+    380-389).  clock: a dict carried from step to step (dt_prev, src_nph)
+    for compute_dt's growth cap and forcing CFL; updated in place."""
     ba = case.ba
     opts = opts or case_opts(case)
+    if np.any(ba.gather(rho, CELL, 1, 3) != case.rho0):
+        # the step projects with beta = 1/rho_0 (MacProjOp::init_projector(Real),
+        # icns_advection.cpp:143-154); the reference's variable-density branch
+        # (face beta 1/rho, :234-291) is not restated
+        raise ValueError("advect_project_step: density differs from rho_0")
     if dt is None:
-        # SimTime::set_current_cfl (SimTime.cpp:168-193) with no diffusive or
-        # forcing CFL: dt = 2 cfl / (C + sqrt(C^2))
-        c = cfl_max(G, vel, 3)
-        dt = 2.0 * case.cfl / (c + math.sqrt(c * c))
+        dt = compute_dt(case, G, vel, rho, gp, clock)
     cn = diffusion is not None and diffusion["type"] == 1
     divtau = None
     if cn:
@@ -663,11 +734,10 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
     flux, face = compute_fluxes(G, 3, q, q_nph, mac, None, fq, dt, case.bcrec, [1, 1, 1], 1, 0, 1, 0, limiter, 0)
     conv = compute_divergence(G, flux, 3, -1.0, 0)
     # src at n+1/2 in momentum form: -gp (constant density)
-    src_nph = np.zeros(ba.mf_size(CELL, 3, 0))
-    for b in range(ba.nbox):
-        g_ = ba.fab_view(gp, b, CELL, 3, 0)
-        r_ = ba.fab_view(rho, b, CELL, 1, 3)[:, 3:-3, 3:-3, 3:-3]
-        ba.fab_view(src_nph, b, CELL, 3, 0)[...] = (-(1.0 * g_) * (1.0 / r_)) * r_
+    src_nph = src_nph_of(ba, gp, rho)
+    if clock is not None:
+        clock["dt_prev"] = dt
+        clock["src_nph"] = src_nph
     vel_new = np.zeros_like(vel)
     predictor_update(G, vel_new, 3, vel, 3, rho, 3, rho, 3, conv, src_nph, divtau, 0.5 if cn else 0.0, dt)
     dinfo = None

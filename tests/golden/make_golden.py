@@ -11,7 +11,9 @@ without the workload generators.
 Fixtures:
   tgv8_step.npz   8^3 periodic Taylor-Green + noise, 8 boxes of 4^3
   wall16_step.npz 16x12x16, periodic x/y, no-slip bottom / slip top, ragged
-                  boxes, varying density and a pressure gradient
+                  boxes, a pressure gradient (in the forcing and in the
+                  forcing CFL of compute_dt); density rho_0, the only
+                  density the step's projections take
 """
 import importlib
 import os
@@ -35,7 +37,7 @@ def cases():
     tgv = wl.taylor_green(8, 4, noise=1e-3, seed=1, cfl=0.5)
     rng = np.random.Generator(np.random.Philox(12))
     wall = wl.random_case((16, 12, 16), boxes=lay.chop_random((16, 12, 16), [4, 8], rng), seed=5, periodic=(1, 1, 0),
-                          gp_amp=0.3, rho_var=0.5, cfl=0.7)
+                          gp_amp=0.3, cfl=0.7)
     return {"tgv8_step": tgv, "wall16_step": wall}
 
 

@@ -31,3 +31,23 @@ def test_library_is_gfx950_code_object():
     path = os.path.join(ROOT, "kynema-sgf_amd", "libksgf.so")
     data = open(path, "rb").read()
     assert b"gfx950" in data
+
+
+def test_boxarray_rejects_overlapping_boxes(ksgf):
+    """Two boxes that overlap (with a gap of the same size elsewhere, so the
+    cell count still equals the domain's) are refused before any device
+    call; so are overlapping boxes on a level that does not cover its
+    domain."""
+    import ctypes as C
+
+    L = ksgf.load()
+    g = ksgf.KsgfGeometry()
+    for d in range(3):
+        g.domain_lo[d], g.domain_hi[d] = 0, 7
+        g.prob_lo[d], g.prob_hi[d] = 0.0, 1.0
+        g.is_periodic[d] = 1
+    boxes = (C.c_int * 12)(0, 0, 0, 7, 7, 3, 0, 0, 3, 7, 7, 6)  # z 0..3 and 3..6: overlap at z 3, gap at z 7
+    out = C.c_void_p()
+    assert L.ksgf_boxarray_create(C.byref(g), boxes, 2, C.byref(out)) == ksgf.KSGF_ERR_INVALID
+    assert b"overlap" in L.ksgf_last_error()
+    assert L.ksgf_boxarray_create_level(C.byref(g), boxes, 2, 0, C.byref(out)) == ksgf.KSGF_ERR_INVALID

@@ -81,8 +81,9 @@ def test_host_step_with_diffusion(ksgf, oracle, workloads, gpu, diff_type):
     mu, rtol, atol = 30.0, 1e-11, 1e-14
     S.enable_diffusion(mu, diff_type, rtol, atol)
     diff = {"mu": mu, "type": diff_type, "rtol": rtol, "atol": atol}
+    clock = {}
     for n in range(2):
-        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, diffusion=diff)
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, diffusion=diff, clock=clock)
         S.step()
         di = S.diffusion_info()
         _say(f"  step {n}: iters gpu={[d['iters'] for d in di]} oracle={[d['iters'] for d in diag['diffusion']]}")

@@ -211,8 +211,9 @@ def test_host_step_with_nodal_projection(ksgf, oracle, workloads, gpu, which):
     S.load()
     rtol, atol = EPS * 1e5, EPS * 1e2  # nodal_proj defaults (MLMGOptions.H:61-65)
     S.enable_nodal_projection(rtol, atol)
+    clock = {}
     for n in range(2):
-        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp)
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, clock=clock)
         oinfo, ophi = oracle.apply_projection(case, G, vel, rho, gp, diag["dt"], rtol, atol)
         S.step()
         ninfo = S.nodal_info()

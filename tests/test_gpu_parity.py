@@ -190,8 +190,10 @@ def test_scalar_step_parity(ksgf, oracle, workloads, gpu):
     H.load()
     H.enable_scalar(sbc)
     H.load_scalar(blob)
+    clock = {}
     for step in range(3):
-        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, scal=s, scal_bc=sbc, scal_bcval=np.zeros((6, 1)))
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, scal=s, scal_bc=sbc, scal_bcval=np.zeros((6, 1)),
+                                               clock=clock)
         s = diag["scal"]
         S.step()
         H.step()
@@ -220,8 +222,9 @@ def test_step_parity(ksgf, oracle, workloads, gpu, which):
     else:
         S = st_mod.AdvectProjectStep(case)
     S.load()
+    clock = {}
     for n in range(2):
-        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp)
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, clock=clock)
         S.step()
         assert S.last_info["iters"] == diag["mlmg"]["iters"], f"step {n}"
         # step 0 starts from identical bits; later steps inherit the MG's

@@ -24,8 +24,9 @@ def test_ragged_layout_tracks_oracle(ksgf, oracle, workloads, gpu):
     G, vel, rho, gp = oracle.make_state(case)
     S = st_mod.AdvectProjectStep(case)
     S.load()
+    clock = {}
     for n in range(2):
-        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp)
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, clock=clock)
         S.step()
         assert S.last_info["iters"] == diag["mlmg"]["iters"], f"step {n}"
         a = case.ba.gather(to_np(S.vel), CELL, 3, 3)

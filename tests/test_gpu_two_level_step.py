@@ -29,8 +29,9 @@ def test_two_level_steps_match_oracle(ksgf, oracle, gpu, walls):
     S.load(vel_c, vel_f, gp_c, gp_f)
     ref_c, ref_f = vel_c.copy(), vel_f.copy()
     tot0 = case.ba.gather(vel_c, CELL, 3, 3).sum(axis=(1, 2, 3))
+    clock = {}
     for s in range(2):
-        rc, rf, rinfo = oracle.two_level_step(case, fba, ref_c, ref_f, gp_c, gp_f)
+        rc, rf, rinfo = oracle.two_level_step(case, fba, ref_c, ref_f, gp_c, gp_f, clock=clock)
         ref_c, ref_f = rc, rf
         dt = S.step()
         print(f"  step {s}: dt {dt:.6e} (oracle {rinfo['dt']:.6e}), composite iterations {S.last_info['iters']} "

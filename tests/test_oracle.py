@@ -5,6 +5,7 @@ the single-phase analogue of its uniform-flow momentum test
 invariants / brute force for what those leave open (each chosen so that a
 dropped term, a wrong sign or index, or a transposed operand fails it)."""
 import importlib
+import math
 
 import numpy as np
 import pytest
@@ -412,3 +413,41 @@ def test_passive_scalar_conservation_and_constant(oracle, workloads):
         assert abs(got.sum() - blob.sum()) <= 1e-13 * blob.sum()
         c = case.ba.gather(diag_c["scal"], CELL, 1, 3)
         assert np.max(np.abs(c - 2.5)) <= 1e-8
+
+
+def test_compute_dt_forcing_cfl_and_growth_cap(oracle, workloads):
+    """SimTime::set_current_cfl (SimTime.cpp:162-205) with the forcing CFL of
+    incflo::compute_dt (incflo_compute_dt.cpp:168-198), worked by hand:
+    u = 1 everywhere on dx = 0.1 gives C = 10; gp = 2.2 in x on rho = 1 gives
+    F = |gp| / dx / rho = 22 - so dt = 2 cfl / (C + sqrt(C^2 + 4 F)) =
+    1 / (10 + sqrt(188)) at cfl 0.5.  A previous step of 0.03 caps it at
+    1.1 x 0.03 = 0.033 (max_dt_growth 0.1)."""
+    case = workloads.random_case((8, 8, 8), seed=3, extent=(0.8, 0.8, 0.8), amp=0.0)
+    case.vel[...] = 0.0
+    case.vel[0] = 1.0
+    case.gp[...] = 0.0
+    case.gp[0] = 2.2
+    G, vel, rho, gp = oracle.make_state(case)
+    assert oracle.cfl_max(G, vel, 3) == pytest.approx(10.0, rel=1e-15)
+    assert oracle.force_cfl_max(G, oracle.src_nph_of(case.ba, gp, rho), 0, rho, 3) == pytest.approx(22.0, rel=1e-15)
+    want = 1.0 / (10.0 + math.sqrt(188.0))
+    assert oracle.compute_dt(case, G, vel, rho, gp) == pytest.approx(want, rel=1e-15)
+    # without the forcing term the step would be cfl / C = 0.05: the forcing
+    # CFL shrinks it
+    assert want < 0.05
+    assert oracle.compute_dt(case, G, vel, rho, gp, {"dt_prev": 0.03}) == pytest.approx(0.033, rel=1e-15)
+    assert oracle.compute_dt(case, G, vel, rho, gp, {"dt_prev": 0.1}) == pytest.approx(want, rel=1e-15)
+    # a step carries its source term and dt to the next compute_dt
+    clock = {}
+    oracle.advect_project_step(case, G, vel, rho, gp, clock=clock)
+    assert clock["dt_prev"] == pytest.approx(want, rel=1e-15)
+    assert oracle.force_cfl_max(G, clock["src_nph"], 0, rho, 3) == pytest.approx(22.0, rel=1e-15)
+
+
+def test_step_rejects_variable_density(oracle, workloads):
+    """The step's projections take beta = 1/rho_0; a density field that is
+    not rho_0 is refused rather than projected with the wrong coefficients"""
+    case = workloads.random_case((8, 8, 8), seed=3, rho_var=0.5)
+    G, vel, rho, gp = oracle.make_state(case)
+    with pytest.raises(ValueError):
+        oracle.advect_project_step(case, G, vel, rho, gp)
