@@ -35,8 +35,9 @@ def ctv_errors(oracle, workloads, n, nz=4):
     dt = T_END / nsteps
     clock = {}
     for _ in range(nsteps):
-        vel, _ = oracle.advect_project_step(case, G, vel, rho, gp, dt=dt, clock=clock)
-        oracle.apply_projection(case, G, vel, rho, gp, dt, RTOL_NODAL, ATOL_NODAL)
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, dt=dt, clock=clock)
+        info, _ = oracle.apply_projection(case, G, vel, rho, gp, dt, RTOL_NODAL, ATOL_NODAL)
+        assert diag["mlmg"]["converged"] and info["converged"]
     want, _ = case.extra["exact"](T_END)
     got = case.ba.gather(vel, CELL, 3, 3)
     return [math.sqrt(np.mean((got[c] - want[c]) ** 2)) for c in range(2)]

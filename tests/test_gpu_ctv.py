@@ -1,8 +1,9 @@
 """Order of accuracy of the HIP step (the C++ host layer's GodunovPredictor:
 advect + MAC project + predictor + nodal projection) on the reference's
 convecting Taylor vortex verification case (ConvectingTaylorVortex.cpp:216-337,
-docs/sphinx/developer/verification.rst:89-107; [0, 2]^3 periodic, u0 = v0 = 1,
-pressure gradient starting exact, t = 0.2, CFL 0.4, inviscid, 4 cells in z).
+docs/sphinx/developer/verification.rst:89-107; the deck's [0, 2]^3 periodic
+cube at n^3, u0 = v0 = 1, pressure gradient starting exact, t = 0.2,
+CFL 0.4, inviscid).
 
 The GPU step tracks the oracle's step to 1e-10 (the multigrid solves differ
 by reduction order only) and converges at order > 1.8 for u and v over
@@ -24,7 +25,7 @@ RTOL_NODAL, ATOL_NODAL = 1e-11, 1e-14
 
 
 def _gpu_ctv(hs, workloads, n):
-    case = workloads.convecting_taylor_vortex(n, 4)
+    case = workloads.convecting_taylor_vortex(n, n)
     S = hs.HostStepper(case)
     S.load()
     S.enable_nodal_projection(RTOL_NODAL, ATOL_NODAL)
