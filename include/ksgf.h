@@ -404,6 +404,19 @@ int ksgf_set_bndry(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, in
 int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out,
                  ksgf_stream_t stream);
 
+/* No counterpart in the reference.  ksgf_cfl_max for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same result, with the reduction's
+ * partial maxima in a file-scope device workspace of 8 MiB per device (held
+ * until the process ends) instead of library-allocated memory.  It calls no
+ * hipMalloc or hipFree.  Limits: at most 268,435,456 valid cells, and the
+ * velocity's ngrow within 0..3 (the launch tables ksgf_boxarray_create
+ * builds); beyond either it returns KSGF_ERR_INVALID.  Every call on one
+ * device shares that device's workspace: calls that overlap on one device
+ * (other threads, other streams) race.  Synchronises. */
+int ksgf_cfl_max_noalloc(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out,
+                         ksgf_stream_t stream);
+
 /* incflo::compute_dt forcing part (incflo_compute_dt.cpp:168-198, taken when
  * time.use_force_cfl, the default): max over valid cells of
  * max_d |src_d| / dx_d / rho, with src the ICNS source term the step last

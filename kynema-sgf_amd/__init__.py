@@ -55,7 +55,7 @@ EXPORTS = [
     "ksgf_fill_boundary", "ksgf_fill_physbc", "ksgf_set_inflow_velocity", "ksgf_ppm_predict_state_on_face",
     "ksgf_extrap_vel_to_faces", "ksgf_compute_fluxes_from_state", "ksgf_compute_divergence",
     "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
-    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_force_cfl_max",
+    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_cfl_max_noalloc", "ksgf_force_cfl_max",
     "ksgf_predictor_update",
     "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
     "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
@@ -150,6 +150,7 @@ def load(path=None):
     L.ksgf_macproj_levels_get_phi.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_macproj_levels_destroy.argtypes = [C.c_void_p]
     L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_cfl_max_noalloc.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_force_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
                                         _dp, C.c_double, C.c_double, C.c_int, C.c_void_p]
@@ -387,6 +388,11 @@ class DeviceBoxArray:
     def cfl_max(self, vel, ngrow):
         out = C.c_double()
         check(load().ksgf_cfl_max(self.h, _ptr(vel), ngrow, C.byref(out), _stream()), "cfl_max")
+        return out.value
+
+    def cfl_max_noalloc(self, vel, ngrow):
+        out = C.c_double()
+        check(load().ksgf_cfl_max_noalloc(self.h, _ptr(vel), ngrow, C.byref(out), _stream()), "cfl_max_noalloc")
         return out.value
 
     def force_cfl_max(self, src, ng_src, rho, ng_rho):

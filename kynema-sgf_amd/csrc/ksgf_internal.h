@@ -104,6 +104,8 @@ namespace ksgf {
 // region helpers (host)
 const Region* get_region(const ksgf_boxarray* ba, const int glo[3], const int ghi[3], int* err);
 const Region* fab_region(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
+// a launch table that already exists, or null (never allocates)
+const Region* find_region(const ksgf_boxarray* ba, const int glo[3], const int ghi[3]);
 const HaloTable* get_halo(const ksgf_boxarray* ba, int itype, int ngrow, int* err);
 double* get_scratch(const ksgf_boxarray* ba, int slot, size_t n, int* err);
 int owner_map(const ksgf_boxarray* ba, hipStream_t st, const int** out);

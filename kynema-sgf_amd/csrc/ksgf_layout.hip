@@ -84,6 +84,18 @@ const Region* get_region(const ksgf_boxarray* ba, const int glo[3], const int gh
     return r;
 }
 
+const Region* find_region(const ksgf_boxarray* ba, const int glo[3], const int ghi[3])
+{
+    RegionKey key;
+    for (int d = 0; d < 3; ++d) {
+        key.glo[d] = glo[d];
+        key.ghi[d] = ghi[d];
+    }
+    std::lock_guard<std::mutex> lk(ba->mu);
+    auto it = ba->regions.find(key);
+    return it == ba->regions.end() ? nullptr : it->second;
+}
+
 const Region* fab_region(const ksgf_boxarray* ba, int itype, int ngrow, int* err)
 {
     int glo[3], ghi[3];
@@ -484,6 +496,15 @@ int ksgf_boxarray_create_level(const ksgf_geometry* geom, const int* boxes, int 
         delete ba;
         set_error("ksgf_boxarray_create: device box table");
         return KSGF_ERR_HIP;
+    }
+    // launch tables of the cell MultiFabs a step uses (ngrow 0..3), built
+    // here so that entry points which must not allocate find them
+    for (int g = 0; g <= 3; ++g) {
+        int err = KSGF_OK;
+        if (!fab_region(ba, KSGF_CELL, g, &err)) {
+            ksgf_boxarray_destroy(ba);
+            return err;
+        }
     }
     *out = ba;
     return KSGF_OK;

@@ -4,6 +4,8 @@
 //   ICNS source term           icns_ops.H:98-111
 //   momentum formation         icns_advection.H:281-314 (Copy + Multiply)
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "ksgf_internal.h"
 
@@ -31,6 +33,73 @@ __global__ void __launch_bounds__(256) k_cfl(CMFDev v, RegDev r, const int* __re
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) q = fmax(q, sh[w]);
         atomicMax(out, (unsigned long long)__double_as_longlong(q));
     }
+}
+
+// ---------------------------------------------------------------------------
+// ksgf_cfl_max_noalloc: the CFL reduction with its workspace in a file-scope
+// device array instead of library-allocated memory.  One partial maximum per
+// 256-cell chunk of the valid region, then one workgroup reduces the
+// partials into the last slot.  Every call on a device shares that device's
+// array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocPartials = 1 << 20;  // chunks of 256 cells: up to 268,435,456 valid cells per call
+static __device__ __align__(256) double scratch[kNoAllocPartials + 1];
+
+__global__ void __launch_bounds__(256) k_cfl_partial(CMFDev v, RegDev r, const int* __restrict__ box, double dxi,
+                                                     double dyi, double dzi, double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    int b, i, j, k;
+    double m = 0.0;
+    if (region_locate(r, box, b, i, j, k)) {
+        CFab f = fab_at(v, box, b);
+        double a = fabs(f(i, j, k, 0)) * dxi;
+        double bb = fabs(f(i, j, k, 1)) * dyi;
+        double e = fabs(f(i, j, k, 2)) * dzi;
+        m = fmax(a, fmax(bb, e));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// one workgroup: out = max(part[0 .. n))
+__global__ void __launch_bounds__(256) k_max_partials(const double* __restrict__ part, int n, double* __restrict__ out)
+{
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) m = fmax(m, part[t]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// device address of `scratch` on each device, found once per device with that
+// device current (each device holds its own copy of the array)
+static double* noalloc_scratch(int* err)
+{
+    static std::mutex mu;
+    static double* addr[64] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("ksgf_cfl_max_noalloc: no current device");
+        *err = KSGF_ERR_HIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("ksgf_cfl_max_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            *err = KSGF_ERR_HIP;
+            return nullptr;
+        }
+        addr[dev] = p;
+    }
+    return addr[dev];
 }
 
 __global__ void __launch_bounds__(256) k_force_cfl(CMFDev s, CMFDev rho, RegDev r, const int* __restrict__ box,
@@ -444,6 +513,49 @@ int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* 
     (void)h;
     // ParReduce floor of -1 (incflo_compute_dt.cpp:86-90): an empty or zero
     // field reports 0 here, which the caller treats as "no constraint"
+    *out = m;
+    return KSGF_OK;
+}
+
+int ksgf_cfl_max_noalloc(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out, ksgf_stream_t stream)
+{
+    if (!ba || !vel || !out || ngrow < 0) {
+        set_error("ksgf_cfl_max_noalloc: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    // the BoxArray's launch tables for the valid cells and for the velocity's
+    // storage are built when it is created (ngrow 0..3); never build one here
+    const int z[3] = {0, 0, 0}, g[3] = {ngrow, ngrow, ngrow};
+    const Region* r = find_region(ba, z, z);
+    const Region* rv = find_region(ba, g, g);
+    if (!r || !rv) {
+        set_error("ksgf_cfl_max_noalloc: velocity ngrow " + std::to_string(ngrow) +
+                  " has no launch table (0..3 are built with the BoxArray)");
+        return KSGF_ERR_INVALID;
+    }
+    if (r->nchunks > kNoAllocPartials) {
+        set_error("ksgf_cfl_max_noalloc: " + std::to_string(r->total) + " valid cells exceed the workspace (at most " +
+                  std::to_string((long long)kNoAllocPartials * kChunk) + ")");
+        return KSGF_ERR_INVALID;
+    }
+    int err = KSGF_OK;
+    double* part = noalloc_scratch(&err);
+    if (!part) return err;
+    CMFDev v;
+    v.p = vel;
+    v.pre = rv->pre;
+    v.itype = KSGF_CELL;
+    v.ngrow = ngrow;
+    v.ncomp = 3;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_cfl_partial, dim3(r->nchunks), dim3(256), 0, st, v, regdev(r, ba->nbox), ba->dbox,
+                       1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], part);
+    KSGF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, r->nchunks, part + kNoAllocPartials);
+    KSGF_LAUNCH_CHECK();
+    double m = 0.0;
+    KSGF_HIP_CHECK(hipMemcpyAsync(&m, part + kNoAllocPartials, sizeof(double), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
     *out = m;
     return KSGF_OK;
 }
