@@ -39,6 +39,7 @@ def parse():
     p.add_argument("--n", type=int, nargs=3, default=[256, 256, 256])
     p.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-theta", action="store_true", help="skip the run with the potential temperature advected")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
     p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal", "levels", "levels-step"], default="abl",
                    help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
@@ -50,6 +51,15 @@ def parse():
                    help="summarise four rocprofv3 --pmc output directories (FETCH_SIZE and WRITE_SIZE of a "
                         "1-step mixed run with KSGF_SWEEP_PLANES fixed, and of --copy-probe) into HBM bytes per "
                         "launch of the level-0 half sweep; prints text for profiles/")
+    p.add_argument("--trace-summary", nargs=2, metavar=("TRACE_DIR", "STEPS"),
+                   help="summarise a rocprofv3 --kernel-trace output directory of STEPS advect + project steps "
+                        "by (kernel, multigrid level): calls, average and total time per step; prints text for "
+                        "profiles/")
+    p.add_argument("--pmc-kernels", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "SQ_DIR", "COPY_FETCH_DIR",
+                                                      "COPY_WRITE_DIR"),
+                   help="summarise rocprofv3 --pmc runs (FETCH_SIZE; WRITE_SIZE; SQ_INSTS_VALU, SQ_INSTS_SALU, "
+                        "SQ_INSTS_VMEM_RD, SQ_INSTS_VMEM_WR, SQ_WAVES) per (kernel, multigrid level): HBM bytes per "
+                        "launch (scaled by the copy probe) and instructions per wave; prints text for profiles/")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB -> 1 GiB elementwise copy (x*1.0) 4 times: the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -69,7 +79,14 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 4}))
 
 
-def run_gpu(case, precision, steps, warmup, profile):
+THETA_BC = [[0, 0, 2, 0, 0, 2]]  # potential temperature: periodic x/y, zero gradient (foextrap) at both walls
+
+
+def run_gpu(case, precision, steps, warmup, profile, theta=False):
+    """W untimed steps, then K steps bracketed by synchronisation, with an
+    event between steps on the stream the host layer launches on (torch's
+    current stream): the mean (ms) and the per-step times (step_ms).  With
+    theta the potential temperature is advected as a passive scalar."""
     import torch
 
     ksgf = importlib.import_module("kynema-sgf_amd")
@@ -78,30 +95,45 @@ def run_gpu(case, precision, steps, warmup, profile):
     opts.precision = 1 if precision == "mixed" else 0
     S = hs.HostStepper(case, options=opts)
     S.load()
+    if theta:
+        S.enable_scalar(np.asarray(THETA_BC, dtype=np.int32))
+        S.load_scalar(case.extra["theta"][None])
     torch.cuda.synchronize()
     for _ in range(warmup):
         S.step()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     infos = []
     if profile:
         ksgf.prof_reset()
-    ev0.record(stream)
-    for _ in range(steps):
+    evs[0].record(stream)
+    for s in range(steps):
         S.step()
+        evs[s + 1].record(stream)
         infos.append(dict(S.last_info))
-    ev1.record(stream)
     torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / steps
+    step_ms = [evs[s].elapsed_time(evs[s + 1]) for s in range(steps)]
+    ms = evs[0].elapsed_time(evs[-1]) / steps
     prof = ksgf.prof_read() if profile else None
     # one more (untimed) step with the post-projection divergence check
     S.step(check_divergence=True)
     divmax = S.last_divmax
     vel = S.velocity()
     S.close()
-    return ms, infos, prof, divmax, vel
+    return ms, infos, prof, divmax, vel, step_ms
+
+
+def step_bytes_per_cell(info, precision):
+    """Algorithmic HBM bytes per cell of one advect + MAC-project step: 616 B
+    for the streaming calls (CFL 24, ExtrapVelToFaces 72, MAC rhs 32,
+    correction 56, fluxes 224, divergence 96, predictor 112) plus per V-cycle
+    166.9 B in FP64, or 83.4 B of FP32 V-cycle and 40 B of FP64 residual and
+    correction per defect-correction iteration in mixed precision."""
+    nv = info["iters"]
+    if precision == "mixed":
+        return 616.0 + nv * 83.4 + max(info.get("outer_iters", nv), nv) * 40.0
+    return 616.0 + nv * 166.9
 
 
 def accuracy(case, nsteps):
@@ -261,6 +293,149 @@ def pmc_summary(fdir, wdir, cfdir, cwdir, n=(256, 256, 256), kz=4):
         allb += [x[0] + x[1] for x in v]
         lines.append(f"  {name:40s} n={len(v):4d} fetch {fb / 1e6:8.2f} MB  write {wb / 1e6:8.2f} MB")
     lines.append(f"traffic_bytes_per_launch: {sum(allb) / len(allb):.0f}")
+    return "\n".join(lines)
+
+
+# Multigrid level of each launch, from the launch order of one solve: the
+# outer residual starts at level 0, a restriction moves one level down, the
+# first launch of a level's post-smoothing (prolongation) returns to the level
+# it came from.
+_MG_DOWN = ("k_resid_restrict_rt", "k_restrict")
+_MG_UP = ("k_pk_prolong_color", "k_pk_prolong_add", "k_mg_prolong_red", "k_mg_prolong_add")
+_MG_RESET = ("k_update_residual",)
+_MG_KERNELS = ("k_pk_half", "k_mg_smooth", "k_mg_ghosts", "k_fill", "k_coarse_vcycle") + _MG_DOWN + _MG_UP + \
+    _MG_RESET
+
+
+def _short(name):
+    import re
+
+    n = re.sub(r"\(.*", "", name).replace("void ", "").replace("ksgf::", "")
+    return n.strip()
+
+
+def _levels(rows):
+    """rows: (dispatch id, short kernel name, ...) sorted by dispatch id;
+    returns the multigrid level of each row ('-' for other kernels)"""
+    lev, stack, out = 0, [], []
+    for r in rows:
+        k = r[1]
+        base = k.split("<")[0]
+        if base in _MG_RESET:
+            lev, stack = 0, []
+            out.append("0")
+        elif base in _MG_UP:
+            lev = stack.pop() if stack else lev
+            out.append(str(lev))
+        elif base in _MG_DOWN:
+            out.append(str(lev))
+            stack.append(lev)
+            lev += 1
+        elif base == "k_coarse_vcycle":
+            out.append(f"{lev}+")
+        elif base in _MG_KERNELS:
+            out.append(str(lev))
+        else:
+            out.append("-")
+    return out
+
+
+def _trace_rows(d):
+    import csv
+    import glob
+
+    rows = []
+    for path in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            g = int(r["Grid_Size"]) if "Grid_Size" in r else \
+                int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+            rows.append((int(r["Dispatch_Id"]), _short(r["Kernel_Name"]), g,
+                         (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3))
+    rows.sort()
+    return rows
+
+
+def _step_window(rows):
+    """the launches of the steps: from the first CFL reduction to the last
+    predictor update (the state upload, the projector's autotuner and the
+    final download fall outside)"""
+    names = [r[1] for r in rows]
+    first = next((i for i, n in enumerate(names) if n == "k_cfl"), 0)
+    last = max((i for i, n in enumerate(names) if n == "k_predictor"), default=len(rows) - 1)
+    return first, last + 1
+
+
+def trace_summary(d, steps):
+    import collections
+
+    rows = _trace_rows(d)
+    a, b = _step_window(rows)
+    rows = rows[a:b]
+    lv = _levels(rows)
+    acc = collections.defaultdict(list)
+    for r, l in zip(rows, lv):
+        acc[(r[1], l)].append(r[3])
+    tot = sum(r[3] for r in rows)
+    lines = [f"{'kernel':54s} {'level':>5s} {'calls':>6s} {'avg us':>9s} {'ms/step':>8s} {'%':>6s}"]
+    for (k, l), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        t = sum(v)
+        if t < 0.002 * tot:
+            continue
+        lines.append(f"{k[:54]:54s} {l:>5s} {len(v):6d} {t / len(v):9.1f} {t * 1e-3 / steps:8.3f} {100 * t / tot:6.2f}")
+    lines.append(f"total kernel time {tot * 1e-3 / steps:.3f} ms/step over {steps} steps")
+    return "\n".join(lines)
+
+
+def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir):
+    """per (kernel, level): mean FETCH_SIZE / WRITE_SIZE bytes per launch
+    (scaled by the copy probe) and SQ instruction counts per launch"""
+    import collections
+    import csv
+    import glob
+
+    def rows(d):
+        per = collections.defaultdict(dict)
+        names = {}
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(path)):
+                i = int(r["Dispatch_Id"])
+                names[i] = _short(r["Kernel_Name"])
+                per[i][r["Counter_Name"]] = per[i].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        ids = sorted(per)
+        return [(i, names[i]) for i in ids], [per[i] for i in ids]
+
+    true_kib = (1 << 27) * 8 / 1024.0
+    scale = []
+    for d, c in ((cfdir, "FETCH_SIZE"), (cwdir, "WRITE_SIZE")):
+        k, v = rows(d)
+        vals = [x[c] for kk, x in zip(k, v) if "MulFunctor" in kk[1]]
+        scale.append(true_kib / (sum(vals) / len(vals)))
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d, sc in ((fdir, scale[0]), (wdir, scale[1]), (sqdir, None)):
+        k, v = rows(d)
+        a, b = _step_window(k)
+        k, v = k[a:b], v[a:b]
+        lv = _levels(k)
+        for kk, x, l in zip(k, v, lv):
+            for c, val in x.items():
+                out[(kk[1], l)][c].append(val * sc * 1024.0 if sc else val)
+    lines = [f"copy probe scale: FETCH_SIZE x{scale[0]:.4f}, WRITE_SIZE x{scale[1]:.4f} "
+             f"(1 GiB read + 1 GiB written by torch.mul(x, 1.0))",
+             f"{'kernel':48s} {'level':>5s} {'calls':>5s} {'read MB':>9s} {'write MB':>9s} {'VALU/wave':>10s} "
+             f"{'SALU/wave':>10s} {'VMEM rd/wave':>12s} {'VMEM wr/wave':>12s} {'waves':>9s}"]
+
+    def m(v):
+        return sum(v) / len(v) if v else float("nan")
+
+    for (k, l), c in sorted(out.items(), key=lambda kv: -m(kv[1].get("FETCH_SIZE", [0])) - m(kv[1].get("WRITE_SIZE",
+                                                                                                         [0]))):
+        if k.startswith("__amd") or "MulFunctor" in k or "elementwise" in k:
+            continue
+        w = m(c.get("SQ_WAVES", []))
+        lines.append(f"{k[:48]:48s} {l:>5s} {len(c.get('FETCH_SIZE', [])):5d} {m(c.get('FETCH_SIZE', [])) / 1e6:9.2f} "
+                     f"{m(c.get('WRITE_SIZE', [])) / 1e6:9.2f} {m(c.get('SQ_INSTS_VALU', [])) / w:10.1f} "
+                     f"{m(c.get('SQ_INSTS_SALU', [])) / w:10.1f} {m(c.get('SQ_INSTS_VMEM_RD', [])) / w:12.1f} "
+                     f"{m(c.get('SQ_INSTS_VMEM_WR', [])) / w:12.1f} {w:9.0f}")
     return "\n".join(lines)
 
 
@@ -604,6 +779,12 @@ def main():
     if a.copy_probe:
         copy_probe()
         return
+    if a.trace_summary:
+        print(trace_summary(a.trace_summary[0], int(a.trace_summary[1])))
+        return
+    if a.pmc_kernels:
+        print(pmc_kernels(*a.pmc_kernels))
+        return
     if a.pmc_summary:
         print(pmc_summary(*a.pmc_summary))
         return
@@ -628,8 +809,10 @@ def main():
     for p in precs:
         res[p] = run_gpu(case, p, a.steps, a.warmup, profile=(p == precs[-1]))
     head = precs[-1]
-    ms, infos, prof, divmax, vel = res[head]
+    ms, infos, prof, divmax, vel, step_ms = res[head]
     value = case.ncell / (ms * 1e-3)
+    med = float(np.median(step_ms))
+    bpc = float(np.mean([step_bytes_per_cell(i, head) for i in infos]))
     out = {
         "metric": "cell-updates/s per advect+project step at 256^3 (FP64 vs mixed), %HBM roofline",
         "value": value,
@@ -648,11 +831,17 @@ def main():
                    "precision": head, "mlmg_iters": [i["iters"] for i in infos],
                    "mlmg_r0": [i["r0"] for i in infos], "mlmg_rfinal": [i["rfinal"] for i in infos],
                    "post_projection_max_div": divmax},
+        "median_ms_per_step": med,
+        "value_median": case.ncell / (med * 1e-3),
+        "step_ms": step_ms,
     }
     if "fp64" in res and head != "fp64":
-        ms64, infos64, _, div64, vel64 = res["fp64"]
-        out["fp64"] = {"value": case.ncell / (ms64 * 1e-3), "ms_per_step": ms64,
-                       "mlmg_iters": [i["iters"] for i in infos64], "post_projection_max_div": div64}
+        ms64, infos64, _, div64, vel64, step64 = res["fp64"]
+        med64 = float(np.median(step64))
+        out["fp64"] = {"value": case.ncell / (ms64 * 1e-3), "ms_per_step": ms64, "median_ms_per_step": med64,
+                       "mlmg_iters": [i["iters"] for i in infos64], "post_projection_max_div": div64,
+                       "full_step_gbs": case.ncell * float(np.mean([step_bytes_per_cell(i, "fp64") for i in infos64]))
+                       / (med64 * 1e-3) / 1e9}
         out["speedup_mixed_vs_fp64"] = ms64 / ms
         out["mixed_vs_fp64"] = accuracy(case, 10)
     if prof and prof.get("launches"):
@@ -664,6 +853,18 @@ def main():
                            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                            "launches": prof["launches"], "avg_us": avg_s * 1e6,
                            "bytes_per_launch": bpl}
+    # the whole step against the HBM roofline: the algorithmic bytes of the
+    # logged V-cycle counts over the median step
+    ach = case.ncell * bpc / (med * 1e-3) / 1e9
+    out.setdefault("roofline", {})["full_step"] = {
+        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+        "bytes_per_cell": bpc, "bytes_per_step": case.ncell * bpc, "ms": med}
+    # the same step with the potential temperature advected (C2 and C3 carry it)
+    if "theta" in case.extra and not a.no_theta:
+        msT, infosT, _, _, _, stepT = run_gpu(case, head, a.steps, a.warmup, profile=False, theta=True)
+        medT = float(np.median(stepT))
+        out["with_theta"] = {"value": case.ncell / (msT * 1e-3), "ms_per_step": msT, "median_ms_per_step": medT,
+                             "mlmg_iters": [i["iters"] for i in infosT]}
     if not a.no_cpu:
         out["cpu_baseline"], (c2, reps, vref, pref) = cpu_baseline(a.cpu_n)
         out["vs_oracle"] = vs_oracle(c2, reps, vref, pref)
