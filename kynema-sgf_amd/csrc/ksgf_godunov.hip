@@ -533,6 +533,499 @@ __global__ void __launch_bounds__(256) k_final(CMFDev q, CMFDev qbc, CMFDev f, C
     }
 }
 
+// ---------------------------------------------------------------------------
+// stages B and C fused: corner-coupled states and final states of one
+// component in one kernel.  A workgroup owns a 16 x 16 tile of (x, y)
+// positions of one box -- a 14 x 14 interior of output positions and a ring
+// of one -- and marches up z.  At each plane every thread forms the six
+// corner states at its position (the expressions of k_corner, reading the
+// stage-A arrays) into LDS; interior threads then form the final states of
+// the plane below (x and y faces) and of the current z face (the expressions
+// of k_final) from their neighbours' corner states.  The corner arrays never
+// reach HBM.
+//   VEL: ExtrapVelToFaces (corner states of the third component, advected
+//        by the Burgers velocities; finals: the normal velocity -> u_mac);
+//   otherwise ComputeFluxesOnBoxFromState, one component per workgroup
+//        (blockIdx.y): face states and fluxes.
+// A position (i, j, k) owns the low x, y and z faces of cell (i, j, k); the
+// box's top faces are owned by the positions one past its high end.
+// ---------------------------------------------------------------------------
+constexpr int kCfT = 16;   // tile edge (threads per row and rows)
+constexpr int kCfIn = 14;  // interior (output) positions per tile edge
+
+struct CfTile {
+    const int* tiles;  // 3 ints per tile: box, x0, y0 (first interior position)
+};
+
+struct CfArgs {
+    CMFDev q, qbc, f, divu;
+    int has_divu, ncomp;
+    CMF3 lo, hi, ed;   // stage A: face states and upwinded states (faces grown by 1)
+    CMF3 vel;          // advecting velocities: MAC (fluxes) or Burgers (VEL), ngrow 1
+    MF3 state, flux;   // outputs (fluxes); state[d] = u_mac[d] (VEL)
+    double c3[3], c6[3], c2[3], c4[3], area[3];
+    double hdt;
+    int uft, is_velocity, area_weighted, allow_inflow, cons_mask;
+};
+
+// the corner arithmetic of k_corner: low / high side states of the D face
+// corrected along T (eq*/ep*: upwinded T states on the T faces of the low
+// and high cells, w*: their advecting velocities, ql/qh: the cells' values)
+template <bool VEL>
+__device__ __forceinline__ void cf_corner_states(double lo, double hi, double wql, double wqh, double wpl, double wph,
+                                                 double eql, double eqh, double epl, double eph, double ql, double qh,
+                                                 bool cons, double c3, double c6, double& lov, double& hiv)
+{
+    if (!VEL && cons) {
+        lov = lo - c3 * (wqh * eqh - wql * eql) + c3 * ql * (wqh - wql);
+        hiv = hi - c3 * (wph * eph - wpl * epl) + c3 * qh * (wph - wpl);
+    } else {
+        lov = lo - c6 * (wqh + wql) * (eqh - eql);
+        hiv = hi - c6 * (wph + wpl) * (eph - epl);
+    }
+}
+
+// BC codes of one component: lo/hi per direction
+struct Bc1 {
+    int lo[3], hi[3];
+};
+__device__ __forceinline__ Bc1 bc_of(const BCs& bc, int c)
+{
+    Bc1 r;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        r.lo[d] = bc.lo[0][d];
+        r.hi[d] = bc.hi[0][d];
+    }
+#pragma unroll
+    for (int m = 1; m < kMaxComp; ++m)
+        if (m == c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                r.lo[d] = bc.lo[m][d];
+                r.hi[d] = bc.hi[m][d];
+            }
+    return r;
+}
+
+// the value of component c of qbc in the ghost cell beyond the domain
+// boundary face of direction D at (i, j, k) (low side when lowside)
+template <int D>
+__device__ __noinline__ double cf_ghost(const double* qp, const long long* pre, int ncomp, const int* box, int b,
+                                        int i, int j, int k, int c, int xg)
+{
+    CMFDev qbc;
+    qbc.p = qp;
+    qbc.pre = pre;
+    qbc.itype = KSGF_CELL;
+    qbc.ngrow = 3;
+    qbc.ncomp = ncomp;
+    int gi = i, gj = j, gk = k;
+    const int x = xg;
+    if (D == 0) gi = x;
+    if (D == 1) gj = x;
+    if (D == 2) gk = x;
+    return fab_at(qbc, box, b)(gi, gj, gk, c);
+}
+
+// boundary treatment + upwinding of a corner state on the D face at (i, j, k)
+template <int D, bool VEL>
+__device__ __forceinline__ double cf_corner_finish(const CfArgs& a, const int* box, int b, int i, int j, int k, int c,
+                                                   const Bc1& bcc, bool both, double lov, double hiv, double u,
+                                                   const GeomDev& g)
+{
+    const int x = pick<D>(i, j, k);
+    if (x == g.dlo[D] || x == g.dhi[D] + 1) {
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = cf_ghost<D>(a.qbc.p, a.qbc.pre, a.qbc.ncomp, box, b, i, j, k, c, g.dlo[D] - 1);
+        if (x == g.dhi[D] + 1) ghi = cf_ghost<D>(a.qbc.p, a.qbc.pre, a.qbc.ncomp, box, b, i, j, k, c, g.dhi[D] + 1);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bcc.lo[D], bcc.hi[D], glo, ghi, both, lov, hiv);
+    }
+    return upwind(lov, hiv, u);
+}
+
+// final state of the D face at (i, j, k) (the arithmetic of k_final<D>):
+// lo/hi: stage-A states of the face; A_*: corner states normal to
+// T1 = (D+1)%3 corrected along T2 at q = p - e_D, q + e_T1, p, p + e_T1;
+// B_*: normal to T2 corrected along T1 at q, q + e_T2, p, p + e_T2; v1*, v2*:
+// the T1 / T2 advecting velocities at the same faces; fl/fh: the forcing of
+// the low / high cell.  Returns the upwinded state (Burgers for VEL).
+template <int D, bool VEL>
+__device__ __forceinline__ double cf_final(const CfArgs& a, const int* box, int b, int i, int j, int k, int c,
+                                           const Bc1& bcc, bool normal, bool cons, double lo, double hi, double a_ql,
+                                           double a_qh, double a_pl, double a_ph, double b_ql, double b_qh,
+                                           double b_pl, double b_ph, double v1ql, double v1qh, double v1pl,
+                                           double v1ph, double v2ql, double v2qh, double v2pl, double v2ph, double ql,
+                                           double qh, double dql, double dqh, double fl, double fh, double u,
+                                           const GeomDev& g)
+{
+    constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+    const int x = pick<D>(i, j, k);
+    double stl, sth;
+    if (!VEL && cons) {
+        stl = lo - a.c2[T1] * (v1qh * a_qh - v1ql * a_ql) - a.c2[T2] * (v2qh * b_qh - v2ql * b_ql) +
+              a.c2[T1] * ql * (v1qh - v1ql) + a.c2[T2] * ql * (v2qh - v2ql);
+        sth = hi - a.c2[T1] * (v1ph * a_ph - v1pl * a_pl) - a.c2[T2] * (v2ph * b_ph - v2pl * b_pl) +
+              a.c2[T1] * qh * (v1ph - v1pl) + a.c2[T2] * qh * (v2ph - v2pl);
+        if (a.has_divu) {
+            stl -= a.hdt * ql * dql;
+            sth -= a.hdt * qh * dqh;
+        }
+    } else {
+        stl = lo - a.c4[T1] * (v1qh + v1ql) * (a_qh - a_ql) - a.c4[T2] * (v2qh + v2ql) * (b_qh - b_ql);
+        sth = hi - a.c4[T1] * (v1ph + v1pl) * (a_ph - a_pl) - a.c4[T2] * (v2ph + v2pl) * (b_ph - b_pl);
+    }
+    if (!a.uft) {
+        stl += a.hdt * fl;
+        sth += a.hdt * fh;
+    }
+    if (x == g.dlo[D] || x == g.dhi[D] + 1) {
+        double glo = 0.0, ghi = 0.0;
+        if (x == g.dlo[D]) glo = cf_ghost<D>(a.qbc.p, a.qbc.pre, a.qbc.ncomp, box, b, i, j, k, c, g.dlo[D] - 1);
+        if (x == g.dhi[D] + 1) ghi = cf_ghost<D>(a.qbc.p, a.qbc.pre, a.qbc.ncomp, box, b, i, j, k, c, g.dhi[D] + 1);
+        edge_bcs(x, g.dlo[D], g.dhi[D], bcc.lo[D], bcc.hi[D], glo, ghi, normal, stl, sth);
+        if (normal && !a.allow_inflow) {
+            const int tl = bcc.lo[D], th = bcc.hi[D];
+            if (x == g.dlo[D] && (tl == KSGF_BC_FOEXTRAP || tl == KSGF_BC_HOEXTRAP)) {
+                sth = fmin(sth, 0.0);
+                stl = sth;
+            }
+            if (x == g.dhi[D] + 1 && (th == KSGF_BC_FOEXTRAP || th == KSGF_BC_HOEXTRAP)) {
+                stl = fmax(stl, 0.0);
+                sth = stl;
+            }
+        }
+    }
+    return VEL ? burgers(stl, sth) : upwind(stl, sth, u);
+}
+
+template <bool VEL>
+__global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const int* __restrict__ box, BCs bc,
+                                                      GeomDev g)
+{
+    constexpr int NS = VEL ? 2 : 1;  // components of each upwinded state the corners read
+    constexpr int NP = kCfT * kCfT;
+    // inputs of plane k & 1: upwinded T states, advecting velocities, cell values
+    __shared__ double s_ed[3][NS][2][NP];
+    __shared__ double s_vel[3][2][NP];
+    __shared__ double s_q[VEL ? 1 : 2][NP];
+    // corner states: xy, yx, zx, zy at planes k & 1; xz, yz at plane k - 1
+    __shared__ double s_c[4][2][NP];
+    __shared__ double s_cz[2][NP];
+    const int t = blockIdx.x;
+    const int b = tl.tiles[3 * t];
+    const int x0 = tl.tiles[3 * t + 1], y0 = tl.tiles[3 * t + 2];
+    const int tx = threadIdx.x % kCfT, ty = threadIdx.x / kCfT;
+    const int i = x0 - 1 + tx, j = y0 - 1 + ty;
+    const int n = VEL ? 0 : (int)blockIdx.y;
+    const int* bx = box + 6 * b;
+    const int l0 = bx[0], l1 = bx[1], l2 = bx[2], h0 = bx[3], h1 = bx[4], h2 = bx[5];
+    const int P = threadIdx.x;
+    const int PXm = P - 1, PXp = P + 1, PYm = P - kCfT, PYp = P + kCfT;
+    // storage of the face arrays with one ghost layer (stage A, advecting
+    // velocities; u_mac for VEL): per direction, plane stride and the
+    // thread's offset at plane l2 - 1; component stride = whole FAB
+    const int e0 = h0 - l0 + 3, e1 = h1 - l1 + 3, e2 = h2 - l2 + 3;
+    int ps1[3], cs1[3], o1[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int m0 = e0 + (d == 0), m1 = e1 + (d == 1), m2 = e2 + (d == 2);
+        ps1[d] = m0 * m1;
+        cs1[d] = ps1[d] * m2;
+        o1[d] = (i - l0 + 1) + m0 * (j - l1 + 1);  // + ps1 * (k - l2 + 1)
+    }
+    const double *lo_p[3], *hi_p[3], *ed_p[3], *vel_p[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo_p[d] = a.lo.m[d].p + (long long)a.lo.m[d].ncomp * a.lo.m[d].pre[b];
+        hi_p[d] = a.hi.m[d].p + (long long)a.hi.m[d].ncomp * a.hi.m[d].pre[b];
+        ed_p[d] = a.ed.m[d].p + (long long)a.ed.m[d].ncomp * a.ed.m[d].pre[b];
+        vel_p[d] = a.vel.m[d].p + a.vel.m[d].pre[b];
+    }
+    // cell arrays: q (ngrow 3), f and divu (ngrow 1)
+    const int q0 = h0 - l0 + 7, q1 = h1 - l1 + 7;
+    const int psq = q0 * q1, csq = psq * (h2 - l2 + 7);
+    const int oq = (i - l0 + 3) + q0 * (j - l1 + 3);  // + psq * (k - l2 + 3)
+    const double* q_p = a.q.p + (long long)a.q.ncomp * a.q.pre[b];
+    const int f0 = h0 - l0 + 3, f1 = h1 - l1 + 3;
+    const int psf = f0 * f1, csf = psf * (h2 - l2 + 3);
+    const int of = (i - l0 + 1) + f0 * (j - l1 + 1);  // + psf * (k - l2 + 1)
+    const double* f_p = a.f.p + (long long)a.f.ncomp * a.f.pre[b];
+    const double* dv_p = a.has_divu ? a.divu.p + a.divu.pre[b] : nullptr;
+    // outputs: faces with no ghost layer (fluxes) or u_mac (ngrow 1, VEL)
+    int ps0[3], cs0[3], o0[3];
+    double *st_p[3], *fl_p[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (VEL) {
+            ps0[d] = ps1[d];
+            cs0[d] = cs1[d];
+            o0[d] = o1[d];  // u_mac: the same storage as the stage arrays
+        } else {
+            const int m0 = e0 - 2 + (d == 0), m1 = e1 - 2 + (d == 1), m2 = e2 - 2 + (d == 2);
+            ps0[d] = m0 * m1;
+            cs0[d] = ps0[d] * m2;
+            o0[d] = (i - l0) + m0 * (j - l1);  // + ps0 * (k - l2)
+        }
+        st_p[d] = a.state.m[d].p + (long long)a.state.m[d].ncomp * a.state.m[d].pre[b];
+        fl_p[d] = VEL ? nullptr : a.flux.m[d].p + (long long)a.flux.m[d].ncomp * a.flux.m[d].pre[b];
+    }
+    // box ranges of the corner states (faces of D, E = 3-D-T grown by one)
+    const bool ri_x = (i >= l0 && i <= h0 + 1), ri_c = (i >= l0 && i <= h0), ri_g = (i >= l0 - 1 && i <= h0 + 1);
+    const bool rj_y = (j >= l1 && j <= h1 + 1), rj_c = (j >= l1 && j <= h1), rj_g = (j >= l1 - 1 && j <= h1 + 1);
+    // tile positions whose corner states an interior final reads (their
+    // inputs then lie inside the tile)
+    const bool tx_in = tx >= 1 && tx <= kCfIn, ty_in = ty >= 1 && ty <= kCfIn;
+    const bool m_xy = tx >= 1 && ty_in && ri_x && rj_c;
+    const bool m_xz = tx >= 1 && ty <= kCfIn && ri_x && rj_g;
+    const bool m_yx = tx_in && ty >= 1 && ri_c && rj_y;
+    const bool m_yz = tx <= kCfIn && ty >= 1 && ri_g && rj_y;
+    const bool m_zx = tx_in && ty <= kCfIn && ri_c && rj_g;
+    const bool m_zy = tx <= kCfIn && ty_in && ri_g && rj_c;
+    const bool inner = tx_in && ty_in;
+    // stage-loadable positions: face storage per direction (x, y extents)
+    bool okf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        okf[d] = i >= l0 - 1 && i <= h0 + 1 + (d == 0) && j >= l1 - 1 && j <= h1 + 1 + (d == 1);
+    const bool q_ok = i >= l0 - 3 && i <= h0 + 3 && j >= l1 - 3 && j <= h1 + 3;
+    const bool cons = !VEL && ((a.cons_mask >> n) & 1);
+    const Bc1 bcn = bc_of(bc, n);
+    Bc1 bcv[3];
+    if (VEL)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bcv[d] = bc_of(bc, d);
+    // own-position inputs of plane kk (prefetched one plane ahead)
+    double p_ed[3][NS], p_vel[3], p_q = 0.0;
+    auto load_plane = [&](int kk) {
+        const int kz = kk - l2 + 1;
+#pragma unroll
+        for (int T = 0; T < 3; ++T) {
+            const bool ok = okf[T] && (kk <= h2 + 1 + (T == 2));
+            const int o = o1[T] + ps1[T] * kz;
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                const int cT = VEL ? (T + 2 - sl) % 3 : n;
+                p_ed[T][sl] = ok ? ed_p[T][o + cT * cs1[T]] : 0.0;
+            }
+            p_vel[T] = ok ? vel_p[T][o] : 0.0;
+        }
+        if (!VEL) p_q = q_ok ? q_p[oq + psq * (kk - l2 + 3) + n * csq] : 0.0;
+    };
+    load_plane(l2 - 1);
+    for (int k = l2 - 1; k <= h2 + 1; ++k) {
+        const int s = k & 1, s1 = s ^ 1;
+        const int kz1 = k - l2 + 1;  // plane index in storage with one ghost layer
+        const bool rk_z = (k >= l2 && k <= h2 + 1), rk_c1 = (k - 1 >= l2 && k - 1 <= h2);
+        // stage plane k
+#pragma unroll
+        for (int T = 0; T < 3; ++T) {
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) s_ed[T][sl][s][P] = p_ed[T][sl];
+            s_vel[T][s][P] = p_vel[T];
+        }
+        if (!VEL) s_q[s][P] = p_q;
+        // the stage-A states and forcing this thread's corners and finals
+        // read, all issued before the first barrier of the plane
+        const bool d_xy = m_xy, d_yx = m_yx, d_zx = m_zx && rk_z, d_zy = m_zy && rk_z, d_xz = m_xz && rk_c1,
+                   d_yz = m_yz && rk_c1;
+        const bool f_x = inner && rk_c1 && ri_x && rj_c, f_y = inner && rk_c1 && ri_c && rj_y,
+                   f_z = inner && rk_z && ri_c && rj_c;
+        const int ox = o1[0] + ps1[0] * kz1, oy = o1[1] + ps1[1] * kz1, oz = o1[2] + ps1[2] * kz1;
+        const int oxm = ox - ps1[0], oym = oy - ps1[1];
+        double lxy = 0, hxy = 0, lyx = 0, hyx = 0, lzx = 0, hzx = 0, lzy = 0, hzy = 0, lxz = 0, hxz = 0, lyz = 0,
+               hyz = 0, lfx = 0, hfx = 0, lfy = 0, hfy = 0, lfz = 0, hfz = 0;
+        constexpr int cxy = VEL ? 2 : 0, cyx = VEL ? 2 : 0, czx = VEL ? 1 : 0, czy = 0, cxz = VEL ? 1 : 0,
+                      cyz = 0, cfx = 0, cfy = VEL ? 1 : 0, cfz = VEL ? 2 : 0;
+        const int nn = VEL ? 0 : n;
+        if (d_xy) {
+            lxy = lo_p[0][ox + (cxy + nn) * cs1[0]];
+            hxy = hi_p[0][ox + (cxy + nn) * cs1[0]];
+        }
+        if (d_yx) {
+            lyx = lo_p[1][oy + (cyx + nn) * cs1[1]];
+            hyx = hi_p[1][oy + (cyx + nn) * cs1[1]];
+        }
+        if (d_zx || d_zy || f_z) {
+            lzx = lo_p[2][oz + (czx + nn) * cs1[2]];
+            hzx = hi_p[2][oz + (czx + nn) * cs1[2]];
+            if (VEL) {
+                lzy = lo_p[2][oz + czy * cs1[2]];
+                hzy = hi_p[2][oz + czy * cs1[2]];
+                lfz = lo_p[2][oz + cfz * cs1[2]];
+                hfz = hi_p[2][oz + cfz * cs1[2]];
+            } else {
+                lzy = lfz = lzx;
+                hzy = hfz = hzx;
+            }
+        }
+        if (d_xz || f_x) {
+            lxz = lo_p[0][oxm + (cxz + nn) * cs1[0]];
+            hxz = hi_p[0][oxm + (cxz + nn) * cs1[0]];
+            if (VEL) {
+                lfx = lo_p[0][oxm + cfx * cs1[0]];
+                hfx = hi_p[0][oxm + cfx * cs1[0]];
+            } else {
+                lfx = lxz;
+                hfx = hxz;
+            }
+        }
+        if (d_yz || f_y) {
+            lyz = lo_p[1][oym + (cyz + nn) * cs1[1]];
+            hyz = hi_p[1][oym + (cyz + nn) * cs1[1]];
+            if (VEL) {
+                lfy = lo_p[1][oym + cfy * cs1[1]];
+                hfy = hi_p[1][oym + cfy * cs1[1]];
+            } else {
+                lfy = lyz;
+                hfy = hyz;
+            }
+        }
+        const int kf = k - l2 + 1;  // plane k in the ngrow-1 cell storage of f / divu
+        double fxl = 0, fxh = 0, fyl = 0, fyh = 0, fzl = 0, fzh = 0;
+        if (!a.uft) {
+            if (f_x) {
+                const int o = of + psf * (kf - 1) + (VEL ? 0 : n) * csf;
+                fxl = f_p[o - 1];
+                fxh = f_p[o];
+            }
+            if (f_y) {
+                const int o = of + psf * (kf - 1) + (VEL ? 1 : n) * csf;
+                fyl = f_p[o - f0];
+                fyh = f_p[o];
+            }
+            if (f_z) {
+                const int o = of + psf * kf + (VEL ? 2 : n) * csf;
+                fzl = f_p[o - psf];
+                fzh = f_p[o];
+            }
+        }
+        // prefetch the stage inputs of plane k + 1
+        if (k + 1 <= h2 + 1) load_plane(k + 1);
+        __syncthreads();
+        double lov, hiv;
+        // xy (x face, corrected along y, plane k); VEL: component z (slot 1)
+        if (d_xy) {
+            const int sl = VEL ? 1 : 0;
+            cf_corner_states<VEL>(lxy, hxy, s_vel[1][s][PXm], s_vel[1][s][PXm + kCfT], s_vel[1][s][P],
+                                  s_vel[1][s][PYp], s_ed[1][sl][s][PXm], s_ed[1][sl][s][PXm + kCfT],
+                                  s_ed[1][sl][s][P], s_ed[1][sl][s][PYp], VEL ? 0.0 : s_q[s][PXm],
+                                  VEL ? 0.0 : s_q[s][P], cons, a.c3[1], a.c6[1], lov, hiv);
+            s_c[0][s][P] = cf_corner_finish<0, VEL>(a, box, b, i, j, k, VEL ? cxy : n, VEL ? bcv[cxy] : bcn,
+                                                    VEL ? true : (a.is_velocity && n == 0), lov, hiv,
+                                                    s_vel[0][s][P], g);
+        }
+        // yx (y face, corrected along x, plane k); VEL: component z (slot 0)
+        if (d_yx) {
+            cf_corner_states<VEL>(lyx, hyx, s_vel[0][s][PYm], s_vel[0][s][PYm + 1], s_vel[0][s][P], s_vel[0][s][PXp],
+                                  s_ed[0][0][s][PYm], s_ed[0][0][s][PYm + 1], s_ed[0][0][s][P], s_ed[0][0][s][PXp],
+                                  VEL ? 0.0 : s_q[s][PYm], VEL ? 0.0 : s_q[s][P], cons, a.c3[0], a.c6[0], lov, hiv);
+            s_c[1][s][P] = cf_corner_finish<1, VEL>(a, box, b, i, j, k, VEL ? cyx : n, VEL ? bcv[cyx] : bcn,
+                                                    VEL ? true : (a.is_velocity && n == 1), lov, hiv,
+                                                    s_vel[1][s][P], g);
+        }
+        // zx (z face k, corrected along x); VEL: component y (slot 1)
+        if (d_zx) {
+            const int sl = VEL ? 1 : 0;
+            cf_corner_states<VEL>(lzx, hzx, s_vel[0][s1][P], s_vel[0][s1][PXp], s_vel[0][s][P], s_vel[0][s][PXp],
+                                  s_ed[0][sl][s1][P], s_ed[0][sl][s1][PXp], s_ed[0][sl][s][P], s_ed[0][sl][s][PXp],
+                                  VEL ? 0.0 : s_q[s1][P], VEL ? 0.0 : s_q[s][P], cons, a.c3[0], a.c6[0], lov, hiv);
+            s_c[2][s][P] = cf_corner_finish<2, VEL>(a, box, b, i, j, k, VEL ? czx : n, VEL ? bcv[czx] : bcn,
+                                                    VEL ? true : (a.is_velocity && n == 2), lov, hiv,
+                                                    s_vel[2][s][P], g);
+        }
+        // zy (z face k, corrected along y); VEL: component x (slot 0)
+        if (d_zy) {
+            cf_corner_states<VEL>(lzy, hzy, s_vel[1][s1][P], s_vel[1][s1][PYp], s_vel[1][s][P], s_vel[1][s][PYp],
+                                  s_ed[1][0][s1][P], s_ed[1][0][s1][PYp], s_ed[1][0][s][P], s_ed[1][0][s][PYp],
+                                  VEL ? 0.0 : s_q[s1][P], VEL ? 0.0 : s_q[s][P], cons, a.c3[1], a.c6[1], lov, hiv);
+            s_c[3][s][P] = cf_corner_finish<2, VEL>(a, box, b, i, j, k, VEL ? czy : n, VEL ? bcv[czy] : bcn,
+                                                    VEL ? true : (a.is_velocity && n == 2), lov, hiv,
+                                                    s_vel[2][s][P], g);
+        }
+        // xz (x face, corrected along z, plane k - 1); VEL: component y (slot 0)
+        if (d_xz) {
+            cf_corner_states<VEL>(lxz, hxz, s_vel[2][s1][PXm], s_vel[2][s][PXm], s_vel[2][s1][P], s_vel[2][s][P],
+                                  s_ed[2][0][s1][PXm], s_ed[2][0][s][PXm], s_ed[2][0][s1][P], s_ed[2][0][s][P],
+                                  VEL ? 0.0 : s_q[s1][PXm], VEL ? 0.0 : s_q[s1][P], cons, a.c3[2], a.c6[2], lov, hiv);
+            s_cz[0][P] = cf_corner_finish<0, VEL>(a, box, b, i, j, k - 1, VEL ? cxz : n, VEL ? bcv[cxz] : bcn,
+                                                  VEL ? true : (a.is_velocity && n == 0), lov, hiv, s_vel[0][s1][P],
+                                                  g);
+        }
+        // yz (y face, corrected along z, plane k - 1); VEL: component x (slot 1)
+        if (d_yz) {
+            const int sl = VEL ? 1 : 0;
+            cf_corner_states<VEL>(lyz, hyz, s_vel[2][s1][PYm], s_vel[2][s][PYm], s_vel[2][s1][P], s_vel[2][s][P],
+                                  s_ed[2][sl][s1][PYm], s_ed[2][sl][s][PYm], s_ed[2][sl][s1][P], s_ed[2][sl][s][P],
+                                  VEL ? 0.0 : s_q[s1][PYm], VEL ? 0.0 : s_q[s1][P], cons, a.c3[2], a.c6[2], lov,
+                                  hiv);
+            s_cz[1][P] = cf_corner_finish<1, VEL>(a, box, b, i, j, k - 1, VEL ? cyz : n, VEL ? bcv[cyz] : bcn,
+                                                  VEL ? true : (a.is_velocity && n == 1), lov, hiv, s_vel[1][s1][P],
+                                                  g);
+        }
+        __syncthreads();
+        // x face of cell (i, j, k-1): A = yz (plane k-1), B = zy (planes k-1, k)
+        if (f_x) {
+            const double dql = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1) - 1] : 0.0;
+            const double dqh = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1)] : 0.0;
+            const double e = cf_final<0, VEL>(
+                a, box, b, i, j, k - 1, VEL ? cfx : n, VEL ? bcv[cfx] : bcn, VEL ? true : (a.is_velocity && n == 0),
+                cons, lfx, hfx, s_cz[1][PXm], s_cz[1][PXm + kCfT], s_cz[1][P], s_cz[1][PYp], s_c[3][s1][PXm],
+                s_c[3][s][PXm], s_c[3][s1][P], s_c[3][s][P], s_vel[1][s1][PXm], s_vel[1][s1][PXm + kCfT],
+                s_vel[1][s1][P], s_vel[1][s1][PYp], s_vel[2][s1][PXm], s_vel[2][s][PXm], s_vel[2][s1][P],
+                s_vel[2][s][P], VEL ? 0.0 : s_q[s1][PXm], VEL ? 0.0 : s_q[s1][P], dql, dqh, fxl, fxh,
+                s_vel[0][s1][P], g);
+            const double u = s_vel[0][s1][P];
+            const int oo = o0[0] + ps0[0] * (VEL ? kz1 - 1 : k - 1 - l2);
+            if (VEL) {
+                st_p[0][oo] = e;
+            } else {
+                st_p[0][oo + n * cs0[0]] = e;
+                fl_p[0][oo + n * cs0[0]] = a.area_weighted ? (u * a.area[0]) * e : u * e;
+            }
+        }
+        // y face of cell (i, j, k-1): A = zx (planes k-1, k), B = xz (plane k-1)
+        if (f_y) {
+            const double dql = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1) - f0] : 0.0;
+            const double dqh = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1)] : 0.0;
+            const double e = cf_final<1, VEL>(
+                a, box, b, i, j, k - 1, VEL ? cfy : n, VEL ? bcv[cfy] : bcn, VEL ? true : (a.is_velocity && n == 1),
+                cons, lfy, hfy, s_c[2][s1][PYm], s_c[2][s][PYm], s_c[2][s1][P], s_c[2][s][P], s_cz[0][PYm],
+                s_cz[0][PYm + 1], s_cz[0][P], s_cz[0][PXp], s_vel[2][s1][PYm], s_vel[2][s][PYm], s_vel[2][s1][P],
+                s_vel[2][s][P], s_vel[0][s1][PYm], s_vel[0][s1][PYm + 1], s_vel[0][s1][P], s_vel[0][s1][PXp],
+                VEL ? 0.0 : s_q[s1][PYm], VEL ? 0.0 : s_q[s1][P], dql, dqh, fyl, fyh, s_vel[1][s1][P], g);
+            const double u = s_vel[1][s1][P];
+            const int oo = o0[1] + ps0[1] * (VEL ? kz1 - 1 : k - 1 - l2);
+            if (VEL) {
+                st_p[1][oo] = e;
+            } else {
+                st_p[1][oo + n * cs0[1]] = e;
+                fl_p[1][oo + n * cs0[1]] = a.area_weighted ? (u * a.area[1]) * e : u * e;
+            }
+        }
+        // z face k: A = xy (planes k-1, k), B = yx (planes k-1, k)
+        if (f_z) {
+            const double dql = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1)] : 0.0;
+            const double dqh = (!VEL && a.has_divu) ? dv_p[of + psf * kf] : 0.0;
+            const double e = cf_final<2, VEL>(
+                a, box, b, i, j, k, VEL ? cfz : n, VEL ? bcv[cfz] : bcn, VEL ? true : (a.is_velocity && n == 2),
+                cons, lfz, hfz, s_c[0][s1][P], s_c[0][s1][PXp], s_c[0][s][P], s_c[0][s][PXp], s_c[1][s1][P],
+                s_c[1][s1][PYp], s_c[1][s][P], s_c[1][s][PYp], s_vel[0][s1][P], s_vel[0][s1][PXp], s_vel[0][s][P],
+                s_vel[0][s][PXp], s_vel[1][s1][P], s_vel[1][s1][PYp], s_vel[1][s][P], s_vel[1][s][PYp],
+                VEL ? 0.0 : s_q[s1][P], VEL ? 0.0 : s_q[s][P], dql, dqh, fzl, fzh, s_vel[2][s][P], g);
+            const double u = s_vel[2][s][P];
+            const int oo = o0[2] + ps0[2] * (VEL ? kz1 : k - l2);
+            if (VEL) {
+                st_p[2][oo] = e;
+            } else {
+                st_p[2][oo + n * cs0[2]] = e;
+                fl_p[2][oo + n * cs0[2]] = a.area_weighted ? (u * a.area[2]) * e : u * e;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // known edge state: flux = u * face
 template <int D>
 __global__ void __launch_bounds__(256) k_known_flux(CMFDev face, CMFDev velD, MFDev flux, int ncomp, RegDev r,
@@ -741,6 +1234,57 @@ static const Region* get_seg_region(const ksgf_boxarray* ba, int D, int* err)
     return r;
 }
 
+// tile table of the fused corner + final kernel: per box, 14 x 14 tiles of
+// output positions (x in lo..hi+1, y in lo..hi+1), kept with the BoxArray's
+// launch tables under a key outside their range
+static const Region* get_cf_tiles(const ksgf_boxarray* ba, int* err)
+{
+    RegionKey key;
+    for (int d = 0; d < 3; ++d) key.glo[d] = key.ghi[d] = 0;
+    key.glo[0] = 300;
+    std::lock_guard<std::mutex> lk(ba->mu);
+    auto it = ba->regions.find(key);
+    if (it != ba->regions.end()) return it->second;
+    std::vector<int> t;
+    for (int b = 0; b < ba->nbox; ++b) {
+        const int* bx = &ba->hbox[6 * b];
+        for (int y0 = bx[1]; y0 <= bx[4] + 1; y0 += kCfIn)
+            for (int x0 = bx[0]; x0 <= bx[3] + 1; x0 += kCfIn) {
+                t.push_back(b);
+                t.push_back(x0);
+                t.push_back(y0);
+            }
+    }
+    Region* r = new Region();
+    r->key = key;
+    r->nchunks = (int)(t.size() / 3);
+    if (hipMalloc(&r->chunks, sizeof(int) * t.size()) != hipSuccess ||
+        hipMemcpy(r->chunks, t.data(), sizeof(int) * t.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (r->chunks) (void)hipFree(r->chunks);
+        delete r;
+        set_error("corner/final tile table upload");
+        *err = KSGF_ERR_NOMEM;
+        return nullptr;
+    }
+    ba->regions[key] = r;
+    return r;
+}
+
+template <bool VEL>
+static int launch_corner_final(const ksgf_boxarray* ba, const CfArgs& a, const BCs& bc, const GeomDev& g, int ncomp,
+                               hipStream_t st)
+{
+    int err = KSGF_OK;
+    const Region* r = get_cf_tiles(ba, &err);
+    if (!r) return err;
+    CfTile tl;
+    tl.tiles = r->chunks;
+    hipLaunchKernelGGL(k_corner_final<VEL>, dim3(r->nchunks, VEL ? 1 : ncomp), dim3(kCfT * kCfT), 0, st, a, tl,
+                       ba->dbox, bc, g);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
 template <int D>
 static int launch_faceA(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev macD,
                         const Stages& S, const BCs& bc, const GeomDev& g, double dtdx, double hdt, int uft,
@@ -937,6 +1481,36 @@ int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const d
     KSGF_TRY(launch_faceA<2>(ba, q, q, f, 3, q, S, bc, g, dtdx[2], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
     CMFDev uad[3];
     for (int d = 0; d < 3; ++d) uad[d] = ccast(S.uad[d]);
+    if (!keep) {
+        // corner and final states in one pass, the corner states kept in LDS
+        CfArgs a{};
+        a.q = q;
+        a.qbc = q;
+        a.f = f;
+        a.divu = q;
+        a.has_divu = 0;
+        a.ncomp = 3;
+        for (int d = 0; d < 3; ++d) {
+            a.lo.m[d] = ccast(S.lo[d]);
+            a.hi.m[d] = ccast(S.hi[d]);
+            a.ed.m[d] = ccast(S.ed[d]);
+            a.vel.m[d] = uad[d];
+            a.state.m[d] = mac[d];
+            a.flux.m[d] = mac[d];
+            a.c6[d] = c6[d];
+            a.c4[d] = c4[d];
+            a.c2[d] = c2[d];
+            a.c3[d] = 0.0;
+            a.area[d] = 0.0;
+        }
+        a.hdt = hdt;
+        a.uft = use_forces_in_trans;
+        a.is_velocity = 1;
+        a.area_weighted = 0;
+        a.allow_inflow = allow_inflow_on_outflow;
+        a.cons_mask = 0;
+        return launch_corner_final<true>(ba, a, bc, g, 3, st);
+    }
     KSGF_TRY((launch_corner<0, 1>(ba, q, q, 3, uad[1], uad[0], S, bc, g, 0.0, c6[1], 1, 1, 0, st)));
     KSGF_TRY((launch_corner<0, 2>(ba, q, q, 3, uad[2], uad[0], S, bc, g, 0.0, c6[2], 1, 1, 0, st)));
     KSGF_TRY((launch_corner<1, 0>(ba, q, q, 3, uad[0], uad[1], S, bc, g, 0.0, c6[0], 1, 1, 0, st)));
@@ -1037,6 +1611,37 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
     KSGF_TRY(launch_faceA<0>(ba, qv, qb, fv, ncomp, mac[0], S, bc, g, dtdx[0], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
     KSGF_TRY(launch_faceA<1>(ba, qv, qb, fv, ncomp, mac[1], S, bc, g, dtdx[1], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
     KSGF_TRY(launch_faceA<2>(ba, qv, qb, fv, ncomp, mac[2], S, bc, g, dtdx[2], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
+    if (!keep) {
+        // corner and final states in one pass per component, the corner
+        // states kept in LDS
+        CfArgs a{};
+        a.q = qv;
+        a.qbc = qb;
+        a.f = fv;
+        a.divu = dv;
+        a.has_divu = has_divu;
+        a.ncomp = ncomp;
+        for (int d = 0; d < 3; ++d) {
+            a.lo.m[d] = ccast(S.lo[d]);
+            a.hi.m[d] = ccast(S.hi[d]);
+            a.ed.m[d] = ccast(S.ed[d]);
+            a.vel.m[d] = mac[d];
+            a.state.m[d] = face[d];
+            a.flux.m[d] = flux[d];
+            a.c3[d] = c3[d];
+            a.c6[d] = c6[d];
+            a.c4[d] = c4[d];
+            a.c2[d] = c2[d];
+            a.area[d] = area[d];
+        }
+        a.hdt = hdt;
+        a.uft = use_forces_in_trans;
+        a.is_velocity = is_velocity;
+        a.area_weighted = fluxes_are_area_weighted;
+        a.allow_inflow = allow_inflow_on_outflow;
+        a.cons_mask = cons_mask;
+        return launch_corner_final<false>(ba, a, bc, g, ncomp, st);
+    }
     KSGF_TRY((launch_corner<0, 1>(ba, qv, qb, ncomp, mac[1], mac[0], S, bc, g, c3[1], c6[1], 0, is_velocity, cons_mask, st)));
     KSGF_TRY((launch_corner<0, 2>(ba, qv, qb, ncomp, mac[2], mac[0], S, bc, g, c3[2], c6[2], 0, is_velocity, cons_mask, st)));
     KSGF_TRY((launch_corner<1, 0>(ba, qv, qb, ncomp, mac[0], mac[1], S, bc, g, c3[0], c6[0], 0, is_velocity, cons_mask, st)));
