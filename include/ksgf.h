@@ -193,6 +193,24 @@ void ksgf_mlmg_default_options(ksgf_mlmg_options* opts);
 int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc,
                         double beta, const ksgf_mlmg_options* opts, ksgf_macproj** out);
 int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta);
+/* initProjector(LPInfo, Vector<Array<const MultiFab*,3>> beta) /
+ * updateBeta(faces) (icns_advection.cpp:143-154, 289): beta on the faces,
+ * beta_x/y/z: face MultiFabs, 1 comp, ngrow >= 0 (valid faces read); the
+ * reference passes factor / rho_face when density varies
+ * (icns_advection.cpp:234-291).  Until set_beta(Real) is called again,
+ * project / solve / get_phi use a variable-coefficient multigrid in FP64
+ * (whatever opts->precision).  Periodic directions need even extents. */
+int ksgf_macproj_set_beta_faces(ksgf_macproj* mp, const double* beta_x, const double* beta_y,
+                                const double* beta_z, int ngrow, ksgf_stream_t stream);
+/* amrex::average_cellcenter_to_face (icns_advection.cpp:259-260): face value =
+ * 0.5 (cell below + cell above), on the valid faces of each box; cc: cell,
+ * 1 comp, ng_cc >= 1 with its ghosts filled. */
+int ksgf_average_cellcenter_to_face(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx,
+                                    double* fy, double* fz, int ng_face, ksgf_stream_t stream);
+/* MultiFab::invert(numerator, nghost) (icns_advection.cpp:282): every point
+ * of the valid region grown by nghost becomes numerator / itself. */
+int ksgf_invert(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow, int nghost,
+                double numerator, ksgf_stream_t stream);
 int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac,
                          double rtol, double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
 /* copy the last solution phi (domain-wide, n0*n1*n2, Fortran order) */

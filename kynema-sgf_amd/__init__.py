@@ -55,7 +55,8 @@ EXPORTS = [
     "ksgf_fill_boundary", "ksgf_fill_physbc", "ksgf_set_inflow_velocity", "ksgf_ppm_predict_state_on_face",
     "ksgf_extrap_vel_to_faces", "ksgf_compute_fluxes_from_state", "ksgf_compute_divergence",
     "ksgf_mlmg_default_options", "ksgf_macproj_create", "ksgf_macproj_set_beta", "ksgf_macproj_project",
-    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_cfl_max_noalloc", "ksgf_force_cfl_max",
+    "ksgf_macproj_get_phi", "ksgf_macproj_destroy", "ksgf_macproj_solve", "ksgf_cfl_max", "ksgf_cfl_max_noalloc", "ksgf_force_cfl_max", "ksgf_macproj_set_beta_faces",
+    "ksgf_average_cellcenter_to_face", "ksgf_invert",
     "ksgf_predictor_update",
     "ksgf_icns_src_term", "ksgf_scale_by_rho", "ksgf_prof_reset", "ksgf_prof_read", "ksgf_mac_divergence_max",
     "ksgf_poisson_batch_create", "ksgf_poisson_batch_solve", "ksgf_poisson_batch_destroy",
@@ -150,6 +151,9 @@ def load(path=None):
     L.ksgf_macproj_levels_get_phi.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_macproj_levels_destroy.argtypes = [C.c_void_p]
     L.ksgf_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+    L.ksgf_macproj_set_beta_faces.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_void_p]
+    L.ksgf_invert.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
+    L.ksgf_average_cellcenter_to_face.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_cfl_max_noalloc.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_force_cfl_max.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.ksgf_predictor_update.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
@@ -395,6 +399,16 @@ class DeviceBoxArray:
         check(load().ksgf_cfl_max_noalloc(self.h, _ptr(vel), ngrow, C.byref(out), _stream()), "cfl_max_noalloc")
         return out.value
 
+    def average_cellcenter_to_face(self, cc, ng_cc, ng_face=0):
+        out = [self.alloc(XFACE + d, 1, ng_face) for d in range(3)]
+        check(load().ksgf_average_cellcenter_to_face(self.h, _ptr(cc), ng_cc, _ptr(out[0]), _ptr(out[1]),
+                                                     _ptr(out[2]), ng_face, _stream()), "average_cellcenter_to_face")
+        return out
+
+    def invert(self, mf, itype, ncomp, ngrow, nghost, numerator):
+        check(load().ksgf_invert(self.h, _ptr(mf), itype, ncomp, ngrow, nghost, numerator, _stream()), "invert")
+        return mf
+
     def force_cfl_max(self, src, ng_src, rho, ng_rho):
         out = C.c_double()
         check(load().ksgf_force_cfl_max(self.h, _ptr(src), ng_src, _ptr(rho), ng_rho, C.byref(out), _stream()),
@@ -495,6 +509,11 @@ class MacProjector:
 
     def update_beta(self, beta):
         check(load().ksgf_macproj_set_beta(self.h, beta), "macproj_set_beta")
+
+    def update_beta_faces(self, b3, ngrow=0):
+        """beta on the faces: three face MultiFabs (device tensors)"""
+        check(load().ksgf_macproj_set_beta_faces(self.h, _ptr(b3[0]), _ptr(b3[1]), _ptr(b3[2]), ngrow, _stream()),
+              "macproj_set_beta_faces")
 
     def project(self, mac, rtol, atol):
         info = KsgfMLMGInfo()

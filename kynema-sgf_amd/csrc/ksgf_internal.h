@@ -292,4 +292,15 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
                         int is_velocity, int fluxes_are_area_weighted, int limiter_type,
                         int allow_inflow_on_outflow, ksgf_stream_t stream, GodunovStages* keep);
 
+// MAC projection with face coefficients (ksgf_mlmg_vb.hip)
+struct VbSolver;
+VbSolver* vb_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, const ksgf_mlmg_options& opt, int* err);
+void vb_destroy(VbSolver* S);
+int vb_set_faces(VbSolver* S, const ksgf_boxarray* ba, const double* const* b3, int ng, hipStream_t st);
+int vb_project(VbSolver* S, const ksgf_boxarray* ba, double* const* mac, int ng, double rtol, double atol,
+               ksgf_mlmg_info* info, hipStream_t st);
+int vb_solve(VbSolver* S, const double* rhs, double* phi, double rtol, double atol, ksgf_mlmg_info* info,
+             hipStream_t st);
+int vb_get_phi(VbSolver* S, double* phi, hipStream_t st);
+
 }  // namespace ksgf

@@ -927,6 +927,10 @@ struct ksgf_macproj {
     unsigned long long* hnorm = nullptr;  // pinned
     BottomWork bw{};
     int* hstats = nullptr;  // pinned
+    // face coefficients (set_beta_faces): the variable-coefficient solver
+    // replaces the constant-beta hierarchy until set_beta is called again
+    ksgf::VbSolver* vb = nullptr;
+    int use_vb = 0;
 };
 
 namespace ksgf {
@@ -1554,6 +1558,7 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     if (mp->dnorm) (void)hipFree(mp->dnorm);
     if (mp->hnorm) (void)hipHostFree(mp->hnorm);
     if (mp->hstats) (void)hipHostFree(mp->hstats);
+    vb_destroy(mp->vb);
     delete mp;
     return KSGF_OK;
 }
@@ -1765,10 +1770,34 @@ int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta)
         return KSGF_ERR_INVALID;
     }
     mp->beta = beta;
+    mp->use_vb = 0;
     auto h = level_h(mp);
     for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta, mp->alpha);
     fill_coefs(mp->P, h[0].data(), beta, mp->alpha);
     return upload_level_tables(mp);
+}
+
+int ksgf_macproj_set_beta_faces(ksgf_macproj* mp, const double* beta_x, const double* beta_y, const double* beta_z,
+                                int ngrow, ksgf_stream_t stream)
+{
+    if (!mp || !beta_x || !beta_y || !beta_z || ngrow < 0) {
+        set_error("ksgf_macproj_set_beta_faces: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (mp->alpha != 0.0) {
+        set_error("ksgf_macproj_set_beta_faces: face coefficients are for the MAC projection (no diagonal term)");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    if (!mp->vb) {
+        int err = KSGF_OK;
+        mp->vb = vb_create(mp->ba, mp->lobc, mp->hibc, mp->opt, &err);
+        if (!mp->vb) return err;
+    }
+    const double* b3[3] = {beta_x, beta_y, beta_z};
+    const int e = vb_set_faces(mp->vb, mp->ba, b3, ngrow, (hipStream_t)stream);
+    if (e) return e;
+    mp->use_vb = 1;
+    return KSGF_OK;
 }
 
 int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac, double rtol, double atol,
@@ -1784,6 +1813,7 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
     MFDev mu[3];
     double* mac[3] = {u_mac, v_mac, w_mac};
     int e;
+    if (mp->use_vb) return vb_project(mp->vb, ba, mac, 1, rtol, atol, info, st);
     if ((e = cmfdev(ba, u_mac, KSGF_XFACE, 1, 1, &cu)) || (e = cmfdev(ba, v_mac, KSGF_YFACE, 1, 1, &cv)) ||
         (e = cmfdev(ba, w_mac, KSGF_ZFACE, 1, 1, &cw)))
         return e;
@@ -1829,6 +1859,7 @@ int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double 
         return KSGF_ERR_INVALID;
     }
     hipStream_t st = (hipStream_t)stream;
+    if (mp->use_vb) return vb_solve(mp->vb, rhs, phi, rtol, atol, info, st);
     const LevelGeom& L0 = mp->P;
     long long n = (long long)L0.n0 * L0.n1 * L0.n2;
     hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, rhs, mp->rhs, L0, 1);
@@ -1983,6 +2014,7 @@ int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream)
         set_error("ksgf_macproj_get_phi: null argument");
         return KSGF_ERR_INVALID;
     }
+    if (mp->use_vb) return vb_get_phi(mp->vb, phi, (hipStream_t)stream);
     const LevelGeom& L0 = mp->P;
     long long n = (long long)L0.n0 * L0.n1 * L0.n2;
     hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, (const double*)mp->phi, phi,

@@ -128,6 +128,30 @@ __global__ void __launch_bounds__(256) k_force_cfl(CMFDev s, CMFDev rho, RegDev 
     }
 }
 
+// MultiFab::invert(numerator, nghost): a = numerator / a on every point of a region
+__global__ void __launch_bounds__(256) k_invert(MFDev m, int ncomp, RegDev r, const int* __restrict__ box,
+                                                double numer)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    Fab f = fab_at(m, box, b);
+    for (int n = 0; n < ncomp; ++n) f(i, j, k, n) = numer / f(i, j, k, n);
+}
+
+// amrex::average_cellcenter_to_face: the mean of the two cells of each face
+template <int D>
+__global__ void __launch_bounds__(256) k_cc_to_face(CMFDev cc, MFDev fc, RegDev r, const int* __restrict__ box)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    CFab c = fab_at(cc, box, b);
+    int qi = i, qj = j, qk = k;
+    if (D == 0) qi -= 1;
+    if (D == 1) qj -= 1;
+    if (D == 2) qk -= 1;
+    fab_at(fc, box, b)(i, j, k) = 0.5 * (c(qi, qj, qk) + c(i, j, k));
+}
+
 __global__ void __launch_bounds__(256) k_predictor(MFDev vn, CMFDev vo, CMFDev ro, CMFDev rn, CMFDev conv,
                                                    CMFDev src, CMFDev diff, int has_diff, double kappa, double dt,
                                                    int ncomp, RegDev r, const int* __restrict__ box)
@@ -514,6 +538,63 @@ int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* 
     // ParReduce floor of -1 (incflo_compute_dt.cpp:86-90): an empty or zero
     // field reports 0 here, which the caller treats as "no constraint"
     *out = m;
+    return KSGF_OK;
+}
+
+int ksgf_invert(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int ngrow, int nghost, double numerator,
+                ksgf_stream_t stream)
+{
+    if (!ba || !mf || itype < KSGF_CELL || itype > KSGF_ZFACE || ncomp <= 0 || ngrow < 0 || nghost < 0 ||
+        nghost > ngrow) {
+        set_error("ksgf_invert: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    MFDev m;
+    int e = mfdev(ba, mf, itype, ncomp, ngrow, &m);
+    if (e) return e;
+    int err = KSGF_OK;
+    int glo[3], ghi[3];
+    for (int d = 0; d < 3; ++d) {
+        glo[d] = nghost;
+        ghi[d] = nghost + (itype == d + 1 ? 1 : 0);
+    }
+    const Region* r = get_region(ba, glo, ghi, &err);
+    if (!r) return err;
+    hipLaunchKernelGGL(k_invert, dim3(r->nchunks), dim3(256), 0, (hipStream_t)stream, m, ncomp, regdev(r, ba->nbox),
+                       ba->dbox, numerator);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+int ksgf_average_cellcenter_to_face(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx, double* fy,
+                                    double* fz, int ng_face, ksgf_stream_t stream)
+{
+    if (!ba || !cc || !fx || !fy || !fz || ng_cc < 1 || ng_face < 0) {
+        set_error("ksgf_average_cellcenter_to_face: bad argument (the cell MultiFab needs a filled ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    CMFDev c;
+    int e = cmfdev(ba, cc, KSGF_CELL, 1, ng_cc, &c);
+    if (e) return e;
+    double* out[3] = {fx, fy, fz};
+    for (int d = 0; d < 3; ++d) {
+        MFDev m;
+        if ((e = mfdev(ba, out[d], KSGF_XFACE + d, 1, ng_face, &m))) return e;
+        int err = KSGF_OK;
+        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        ghi[d] = 1;
+        const Region* r = get_region(ba, glo, ghi, &err);
+        if (!r) return err;
+        dim3 g(r->nchunks);
+        if (d == 0)
+            hipLaunchKernelGGL(k_cc_to_face<0>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox);
+        else if (d == 1)
+            hipLaunchKernelGGL(k_cc_to_face<1>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox);
+        else
+            hipLaunchKernelGGL(k_cc_to_face<2>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox);
+        KSGF_LAUNCH_CHECK();
+    }
     return KSGF_OK;
 }
 

@@ -261,3 +261,89 @@ int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
 }
 
 }  // extern "C"
+
+extern "C" int ksgf_host_mac_project_density(const ksgf_geometry* geom, const int* boxes, int nbox, const int* lobc,
+                                             const int* hibc, const double* rho, double factor, double* umac,
+                                             double* vmac, double* wmac, double rtol, double atol,
+                                             const ksgf_mlmg_options* opts, ksgf_mlmg_info* info, double* phi)
+{
+    using ksgf::amr::BoxArray;
+    using ksgf::amr::IndexType;
+    using ksgf::amr::MultiFab;
+    try {
+        if (!geom || !boxes || nbox <= 0 || !lobc || !hibc || !rho || !umac || !vmac || !wmac)
+            throw std::runtime_error("null argument");
+        Geometry g;
+        for (int d = 0; d < 3; ++d) {
+            g.domain.lo[d] = geom->domain_lo[d];
+            g.domain.hi[d] = geom->domain_hi[d];
+            g.prob_lo[d] = geom->prob_lo[d];
+            g.prob_hi[d] = geom->prob_hi[d];
+            g.is_periodic[d] = geom->is_periodic[d];
+        }
+        std::vector<Box> bx(nbox);
+        for (int b = 0; b < nbox; ++b)
+            for (int d = 0; d < 3; ++d) {
+                bx[b].lo[d] = boxes[6 * b + d];
+                bx[b].hi[d] = boxes[6 * b + 3 + d];
+            }
+        BoxArray ba(g, bx);
+        // density with one ghost layer: periodic images, first-order
+        // extrapolation at walls
+        MultiFab rmf(ba, 1, 1);
+        rmf.copyFromDomain(rho);
+        rmf.FillBoundary(nullptr);
+        std::vector<ksgf::amr::BCRec> rbc(1);
+        for (int d = 0; d < 3; ++d) {
+            rbc[0].lo[d] = g.isPeriodic(d) ? KSGF_BC_INT_DIR : KSGF_BC_FOEXTRAP;
+            rbc[0].hi[d] = g.isPeriodic(d) ? KSGF_BC_INT_DIR : KSGF_BC_FOEXTRAP;
+        }
+        rmf.FillPhysBC(rbc, std::vector<double>(6, 0.0), nullptr);
+        MultiFab bxf(ba, 1, 0, IndexType::XFace), byf(ba, 1, 0, IndexType::YFace), bzf(ba, 1, 0, IndexType::ZFace);
+        ksgf::amr::face_beta_from_density({&bxf, &byf, &bzf}, rmf, factor, nullptr);
+        MultiFab u(ba, 1, 1, IndexType::XFace), v(ba, 1, 1, IndexType::YFace), w(ba, 1, 1, IndexType::ZFace);
+        u.copyFromDomain(umac);
+        v.copyFromDomain(vmac);
+        w.copyFromDomain(wmac);
+        Hydro::MacProjector mp({g});
+        Hydro::LPInfo lp;
+        ksgf_mlmg_options o;
+        if (opts)
+            o = *opts;
+        else
+            ksgf_mlmg_default_options(&o);
+        lp.do_semicoarsening = o.do_semicoarsening != 0;
+        lp.max_coarsening_level = o.max_coarsening_level;
+        mp.getMLMG().opts = o;
+        mp.initProjector({&ba}, lp, {&bxf, &byf, &bzf});
+        std::array<Hydro::LinOpBCType, 3> lo, hi;
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = (Hydro::LinOpBCType)lobc[d];
+            hi[d] = (Hydro::LinOpBCType)hibc[d];
+        }
+        mp.setDomainBC(lo, hi);
+        mp.setUMAC({{&u, &v, &w}});
+        mp.project(rtol, atol, nullptr);
+        if (info) *info = mp.getMLMG().last;
+        if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("synchronize");
+        u.copyToDomain(umac);
+        v.copyToDomain(vmac);
+        w.copyToDomain(wmac);
+        if (phi) {
+            const size_t n = (size_t)g.domain.numPts();
+            double* d = nullptr;
+            if (hipMalloc((void**)&d, n * sizeof(double)) != hipSuccess) throw std::runtime_error("hipMalloc");
+            mp.getPhi(d, nullptr);
+            const hipError_t e = hipMemcpy(phi, d, n * sizeof(double), hipMemcpyDeviceToHost);
+            (void)hipFree(d);
+            if (e != hipSuccess) throw std::runtime_error("copy phi");
+        }
+        return mp.getMLMG().last.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+    } catch (const ksgf::amr::UnsupportedError& e) {
+        g_err = std::string("ksgf_host_mac_project_density: ") + e.what();
+        return KSGF_ERR_UNSUPPORTED;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_mac_project_density: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}

@@ -53,6 +53,18 @@ int ksgf_host_stepper_enable_diffusion(ksgf_host_stepper* s, double mu, int type
 int ksgf_host_stepper_diffusion_info(ksgf_host_stepper* s, ksgf_mlmg_info* info3);
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
 
+/* MacProjOp::operator() with variable density (icns_advection.cpp:234-299)
+ * on the Hydro layer, for one level: rho_face = average_cellcenter_to_face
+ * of rho (cell MultiFab, one ghost layer filled by the host layer's
+ * FillBoundary / foextrap walls), beta = factor / rho_face (invert), then
+ * initProjector(LPInfo, beta faces), setDomainBC, setUMAC, project.  rho:
+ * domain-wide host (N); umac/vmac/wmac: domain-wide host face arrays,
+ * corrected in place; phi (may be NULL): domain-wide host (N). */
+int ksgf_host_mac_project_density(const ksgf_geometry* geom, const int* boxes, int nbox, const int* lobc,
+                                  const int* hibc, const double* rho, double factor, double* umac, double* vmac,
+                                  double* wmac, double rtol, double atol, const ksgf_mlmg_options* opts,
+                                  ksgf_mlmg_info* info, double* phi);
+
 #ifdef __cplusplus
 }
 #endif

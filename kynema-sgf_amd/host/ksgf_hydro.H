@@ -307,6 +307,32 @@ inline void ComputeDivergence(MultiFab& div, const MultiFab& fx, const MultiFab&
 // ---------------------------------------------------------------------------
 // Hydro::MacProjector
 // ---------------------------------------------------------------------------
+namespace ksgf::amr {
+
+// amrex::average_cellcenter_to_face (one component)
+inline void average_cellcenter_to_face(const std::array<MultiFab*, 3>& faces, const MultiFab& cc,
+                                       hipStream_t st = nullptr)
+{
+    check(ksgf_average_cellcenter_to_face(cc.boxArray().handle(), cc.data(), cc.nGrow(), faces[0]->data(),
+                                          faces[1]->data(), faces[2]->data(), faces[0]->nGrow(), (ksgf_stream_t)st),
+          "average_cellcenter_to_face");
+}
+
+// MacProjOp's variable-density coefficients (icns_advection.cpp:255-283):
+// rho_face = average_cellcenter_to_face(rho), then invert(factor): beta on
+// the faces = factor / rho_face
+inline void face_beta_from_density(const std::array<MultiFab*, 3>& beta, const MultiFab& rho, double factor,
+                                   hipStream_t st = nullptr)
+{
+    average_cellcenter_to_face(beta, rho, st);
+    for (int d = 0; d < 3; ++d)
+        check(ksgf_invert(rho.boxArray().handle(), beta[d]->data(), KSGF_XFACE + d, 1, beta[d]->nGrow(), 0, factor,
+                          (ksgf_stream_t)st),
+              "invert");
+}
+
+}  // namespace ksgf::amr
+
 namespace Hydro {
 
 enum class LinOpBCType { Periodic = KSGF_LO_PERIODIC, Neumann = KSGF_LO_NEUMANN, Dirichlet = KSGF_LO_DIRICHLET };
@@ -358,6 +384,17 @@ public:
         mlmg_.opts.do_semicoarsening = info.do_semicoarsening ? 1 : 0;
         mlmg_.opts.max_coarsening_level = info.max_coarsening_level;
     }
+    // initProjector(LPInfo, Vector<Array<const MultiFab*,3>> beta) (one
+    // level): beta on the faces; the reference passes factor / rho_face
+    // when density varies (icns_advection.cpp:143-154, 234-291)
+    void initProjector(const std::vector<const ksgf::amr::BoxArray*>& ba, const LPInfo& info,
+                       const std::array<const ksgf::amr::MultiFab*, 3>& beta)
+    {
+        if (ba.size() != 1 || geom_.size() != 1)
+            throw ksgf::amr::UnsupportedError("MacProjector: face coefficients on one level only");
+        initProjector(ba, info, 1.0);
+        beta_faces_ = beta;
+    }
     void setDomainBC(const std::array<LinOpBCType, 3>& lo, const std::array<LinOpBCType, 3>& hi)
     {
         for (int d = 0; d < 3; ++d) {
@@ -369,8 +406,18 @@ public:
     void updateBeta(double beta)
     {
         beta_ = beta;
+        beta_faces_ = {};
         if (mp_) ksgf::amr::check(ksgf_macproj_set_beta(mp_, beta), "updateBeta");
         if (mpl_) build();
+    }
+    // updateBeta(Vector<Array<const MultiFab*,3>>) (icns_advection.cpp:289),
+    // enqueued on st
+    void updateBeta(const std::array<const ksgf::amr::MultiFab*, 3>& beta, hipStream_t st = nullptr)
+    {
+        if (mpl_ || fba_) throw ksgf::amr::UnsupportedError("MacProjector: face coefficients on one level only");
+        beta_faces_ = beta;
+        if (!mp_) build();
+        setFaces(st);
     }
     void setUMAC(const std::vector<std::array<ksgf::amr::MultiFab*, 3>>& umac)
     {
@@ -429,7 +476,19 @@ private:
                              "MacProjector");
         else
             ksgf::amr::check(ksgf_macproj_create(ba_->handle(), lo_, hi_, beta_, &mlmg_.opts, &mp_), "MacProjector");
+        if (mp_ && beta_faces_[0]) setFaces(nullptr);
     }
+    void setFaces(hipStream_t st)
+    {
+        const auto& b = beta_faces_;
+        if (!b[0] || !b[1] || !b[2]) throw std::runtime_error("MacProjector: three face coefficient MultiFabs");
+        if (b[0]->nGrow() != b[1]->nGrow() || b[0]->nGrow() != b[2]->nGrow())
+            throw std::runtime_error("MacProjector: face coefficients need one ngrow");
+        ksgf::amr::check(ksgf_macproj_set_beta_faces(mp_, b[0]->data(), b[1]->data(), b[2]->data(), b[0]->nGrow(),
+                                                     (ksgf_stream_t)st),
+                         "MacProjector::updateBeta(faces)");
+    }
+    std::array<const ksgf::amr::MultiFab*, 3> beta_faces_{};
     std::vector<ksgf::amr::Geometry> geom_;
     const ksgf::amr::BoxArray* ba_ = nullptr;
     const ksgf::amr::BoxArray* fba_ = nullptr;

@@ -47,6 +47,12 @@ def host_lib():
                                                          C.POINTER(KsgfMLMGOptions)]
         L.ksgf_host_stepper_diffusion_info.argtypes = [C.c_void_p, C.POINTER(KsgfMLMGInfo)]
         L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
+        L.ksgf_host_mac_project_density.argtypes = [C.POINTER(KsgfGeometry), C.POINTER(C.c_int), C.c_int,
+                                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                                    C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_double), C.c_double, C.c_double,
+                                                    C.POINTER(KsgfMLMGOptions), C.POINTER(KsgfMLMGInfo),
+                                                    C.POINTER(C.c_double)]
         _h = L
     return _h
 
@@ -186,3 +192,30 @@ class HostStepper:
             self.close()
         except Exception:
             pass
+
+
+def mac_project_density(ba, lobc, hibc, rho, mac, factor=1.0, rtol=1e-11, atol=0.0, options=None):
+    """MacProjOp's variable-density projection through the C++ Hydro layer
+    (face beta = factor / rho_face): rho (nz, ny, nx), mac = domain-wide face
+    arrays (corrected copies are returned with phi and the MLMG info)"""
+    L = host_lib()
+    g = KsgfGeometry()
+    for d in range(3):
+        g.domain_lo[d] = ba.geom.dlo[d]
+        g.domain_hi[d] = ba.geom.dhi[d]
+        g.prob_lo[d] = ba.geom.prob_lo[d]
+        g.prob_hi[d] = ba.geom.prob_hi[d]
+        g.is_periodic[d] = int(ba.geom.periodic[d])
+    boxes = np.ascontiguousarray(ba.boxes, dtype=np.int32)
+    lo = np.ascontiguousarray(lobc, dtype=np.int32)
+    hi = np.ascontiguousarray(hibc, dtype=np.int32)
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    out = [np.ascontiguousarray(m, dtype=np.float64).copy() for m in mac]
+    phi = np.zeros(r.size)
+    info = KsgfMLMGInfo()
+    o = options or default_options()
+    _chk(L.ksgf_host_mac_project_density(C.byref(g), boxes.ctypes.data_as(C.POINTER(C.c_int)), ba.nbox,
+                                         lo.ctypes.data_as(C.POINTER(C.c_int)), hi.ctypes.data_as(C.POINTER(C.c_int)),
+                                         _dp(r), factor, _dp(out[0]), _dp(out[1]), _dp(out[2]), rtol, atol,
+                                         C.byref(o), C.byref(info), _dp(phi)), "mac_project_density")
+    return out, phi, info.as_dict()

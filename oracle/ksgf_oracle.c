@@ -1431,6 +1431,303 @@ int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int*
 }
 
 /* ------------------------------------------------------------------ */
+/* MAC projection with face coefficients (Hydro::MacProjector::        */
+/* initProjector(LPInfo, Array<const MultiFab*,3> beta) / updateBeta,  */
+/* used with beta = factor / rho_face when density varies,             */
+/* icns_advection.cpp:234-291): div(beta grad phi) = rhs, beta on the  */
+/* faces.  Geometric multigrid restated from the published method:     */
+/* red-black Gauss-Seidel on the variable 7-point operator, residual   */
+/* restricted by the cell average, face coefficients coarsened by the  */
+/* average of the fine faces on each coarse face (as AMReX coarsens    */
+/* b-coefficients), piecewise-constant prolongation, and red-black     */
+/* Gauss-Seidel sweeps on the coarsest level as the bottom solve.      */
+/* Domain sides in the stencil: periodic wrap; Neumann: no flux;       */
+/* Dirichlet (phi = 0 on the face): 2 beta / h^2 on the diagonal.      */
+/* Parity unpinned: AMReX's MLABecLaplacian source (its bottom solver, */
+/* exact coarsening of coefficients) is not in the reference tree.     */
+/* Every neighbour term is summed in the order x-, x+, y-, y+, z-, z+. */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int n[3];
+    double h[3], ih2[3];
+    int r[3];
+    double *b[3];  /* face coefficients, (n_d + 1) along d */
+    double *e, *f; /* correction and right-hand side */
+} vlev_t;
+
+typedef struct {
+    int nlev;
+    vlev_t L[MAXLEV];
+    int lobc[3], hibc[3];
+    int nbot;
+} vmg_t;
+
+static inline long VC(const vlev_t* L, int i, int j, int k) { return i + (long)L->n[0] * (j + (long)L->n[1] * k); }
+static inline long VF(const vlev_t* L, int d, int i, int j, int k)
+{
+    const int m0 = L->n[0] + (d == 0), m1 = L->n[1] + (d == 1);
+    return i + (long)m0 * (j + (long)m1 * k);
+}
+
+/* sum of the neighbour terms (acc) and the diagonal (dg) of cell (i,j,k) */
+static void vb_cell(const vmg_t* M, const vlev_t* L, const double* x, int i, int j, int k, double* acc, double* dg)
+{
+    double a = 0.0, g = 0.0;
+    const int c[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) {
+        for (int side = 0; side < 2; ++side) {
+            int p[3] = {i, j, k};
+            int fc[3] = {i, j, k};
+            if (side) fc[d] += 1;
+            const double b = L->b[d][VF(L, d, fc[0], fc[1], fc[2])] * L->ih2[d];
+            const int bc = side ? M->hibc[d] : M->lobc[d];
+            const int edge = side ? (c[d] == L->n[d] - 1) : (c[d] == 0);
+            if (edge && bc == KO_LO_NEUMANN) continue;
+            if (edge && bc == KO_LO_DIRICHLET) {
+                g += 2.0 * b;
+                continue;
+            }
+            p[d] += side ? 1 : -1;
+            if (p[d] < 0) p[d] += L->n[d];
+            if (p[d] >= L->n[d]) p[d] -= L->n[d];
+            a += b * x[VC(L, p[0], p[1], p[2])];
+            g += b;
+        }
+    }
+    *acc = a;
+    *dg = g;
+}
+
+static void vb_sweep(const vmg_t* M, vlev_t* L, int color)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                if (((i + j + k) & 1) != color) continue;
+                double a, g;
+                vb_cell(M, L, L->e, i, j, k, &a, &g);
+                L->e[VC(L, i, j, k)] = (a - L->f[VC(L, i, j, k)]) / g;
+            }
+}
+
+/* r = f - A e of one fine cell */
+static double vb_resid(const vmg_t* M, const vlev_t* L, int i, int j, int k)
+{
+    double a, g;
+    vb_cell(M, L, L->e, i, j, k, &a, &g);
+    return L->f[VC(L, i, j, k)] - (a - g * L->e[VC(L, i, j, k)]);
+}
+
+static void vb_build(vmg_t* M, const int* n, const double* dx, const ko_mlmg_opts* o)
+{
+    int nl = 0;
+    int cn[3] = {n[0], n[1], n[2]};
+    double ch[3] = {dx[0], dx[1], dx[2]};
+    for (;;) {
+        vlev_t* L = &M->L[nl];
+        for (int d = 0; d < 3; ++d) {
+            L->n[d] = cn[d];
+            L->h[d] = ch[d];
+            L->ih2[d] = 1.0 / (ch[d] * ch[d]);
+            L->r[d] = 1;
+        }
+        ++nl;
+        if (nl >= MAXLEV || nl > o->max_coarsening_level) break;
+        /* a periodic extent stays even: red and black then alternate
+         * across the periodic wrap too */
+        int can[3], any = 0;
+        for (int d = 0; d < 3; ++d)
+            can[d] = (cn[d] % 2 == 0 && cn[d] / 2 >= 2 && (M->lobc[d] != KO_LO_PERIODIC || (cn[d] / 2) % 2 == 0));
+        if (!o->semicoarsening) {
+            if (!(can[0] && can[1] && can[2])) break;
+            for (int d = 0; d < 3; ++d) L->r[d] = 2;
+            any = 1;
+        } else {
+            double hmin = 1e300;
+            for (int d = 0; d < 3; ++d) if (ch[d] < hmin) hmin = ch[d];
+            for (int d = 0; d < 3; ++d)
+                if (can[d] && ch[d] <= 1.5 * hmin) { L->r[d] = 2; any = 1; }
+            if (!any) break;
+        }
+        for (int d = 0; d < 3; ++d) { cn[d] /= L->r[d]; ch[d] *= L->r[d]; }
+    }
+    M->nlev = nl;
+    for (int l = 0; l < nl; ++l) {
+        vlev_t* L = &M->L[l];
+        const long nc = (long)L->n[0] * L->n[1] * L->n[2];
+        L->e = (double*)calloc((size_t)nc, sizeof(double));
+        L->f = (double*)calloc((size_t)nc, sizeof(double));
+        for (int d = 0; d < 3; ++d)
+            L->b[d] = (double*)calloc((size_t)(nc / L->n[d] * (L->n[d] + 1)), sizeof(double));
+    }
+    const vlev_t* C = &M->L[nl - 1];
+    int m = C->n[0];
+    if (C->n[1] > m) m = C->n[1];
+    if (C->n[2] > m) m = C->n[2];
+    M->nbot = 2 * m * m + 8;
+    if (M->nbot > 400) M->nbot = 400;
+}
+
+/* coarse face coefficients: the average of the fine faces on each coarse face */
+static void vb_coarsen_coefs(vmg_t* M)
+{
+    for (int l = 0; l + 1 < M->nlev; ++l) {
+        const vlev_t* F = &M->L[l];
+        vlev_t* C = &M->L[l + 1];
+        for (int d = 0; d < 3; ++d) {
+            const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+            const int r1 = F->r[t1], r2 = F->r[t2];
+            const double w = 1.0 / (double)(r1 * r2);
+            int m[3] = {C->n[0], C->n[1], C->n[2]};
+            m[d] += 1;
+            for (int K = 0; K < m[2]; ++K)
+                for (int J = 0; J < m[1]; ++J)
+                    for (int I = 0; I < m[0]; ++I) {
+                        int cI[3] = {I, J, K};
+                        double s = 0.0;
+                        for (int a2 = 0; a2 < r2; ++a2)
+                            for (int a1 = 0; a1 < r1; ++a1) {
+                                int fI[3];
+                                fI[d] = cI[d] * F->r[d];
+                                fI[t1] = cI[t1] * r1 + a1;
+                                fI[t2] = cI[t2] * r2 + a2;
+                                s += F->b[d][VF(F, d, fI[0], fI[1], fI[2])];
+                            }
+                        C->b[d][VF(C, d, I, J, K)] = s * w;
+                    }
+        }
+    }
+}
+
+static void vb_vcycle(vmg_t* M, int l, const ko_mlmg_opts* o)
+{
+    vlev_t* L = &M->L[l];
+    const long nc = (long)L->n[0] * L->n[1] * L->n[2];
+    for (long c = 0; c < nc; ++c) L->e[c] = 0.0;
+    if (l == M->nlev - 1) {
+        for (int s = 0; s < M->nbot; ++s) {
+            vb_sweep(M, L, 0);
+            vb_sweep(M, L, 1);
+        }
+        return;
+    }
+    for (int s = 0; s < o->nu1; ++s) {
+        vb_sweep(M, L, 0);
+        vb_sweep(M, L, 1);
+    }
+    vlev_t* C = &M->L[l + 1];
+    const double w = 1.0 / (double)(L->r[0] * L->r[1] * L->r[2]);
+#pragma omp parallel for schedule(static)
+    for (int K = 0; K < C->n[2]; ++K)
+        for (int J = 0; J < C->n[1]; ++J)
+            for (int I = 0; I < C->n[0]; ++I) {
+                double s = 0.0;
+                for (int dk = 0; dk < L->r[2]; ++dk)
+                    for (int dj = 0; dj < L->r[1]; ++dj)
+                        for (int di = 0; di < L->r[0]; ++di)
+                            s += vb_resid(M, L, I * L->r[0] + di, J * L->r[1] + dj, K * L->r[2] + dk);
+                C->f[VC(C, I, J, K)] = s * w;
+            }
+    vb_vcycle(M, l + 1, o);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i)
+                L->e[VC(L, i, j, k)] += C->e[VC(C, i / L->r[0], j / L->r[1], k / L->r[2])];
+    for (int s = 0; s < o->nu2; ++s) {
+        vb_sweep(M, L, 0);
+        vb_sweep(M, L, 1);
+    }
+}
+
+/* div(beta grad phi) = rhs with beta on the faces: bx (n0+1, n1, n2), by,
+ * bz domain-wide (x fastest); rhs and phi (n0, n1, n2).  Singular problems
+ * (no Dirichlet side) have the rhs mean removed (summed in order).  MLMG's
+ * convergence test and divergence bound as ko_poisson_solve. */
+int ko_vb_solve(const int* n, const double* dx, const int* lobc, const int* hibc, const double* bx, const double* by,
+                const double* bz, const double* rhs_in, double* phi_out, double rtol, double atol,
+                const ko_mlmg_opts* opts, ko_mlmg_info* info)
+{
+    vmg_t* M = (vmg_t*)calloc(1, sizeof(vmg_t));
+    int singular = 1;
+    for (int d = 0; d < 3; ++d) {
+        M->lobc[d] = lobc[d];
+        M->hibc[d] = hibc[d];
+        if (lobc[d] == KO_LO_DIRICHLET || hibc[d] == KO_LO_DIRICHLET) singular = 0;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (lobc[d] == KO_LO_PERIODIC && n[d] % 2) {
+            free(M);
+            return -2; /* an odd periodic extent has no red-black ordering */
+        }
+    vb_build(M, n, dx, opts);
+    vlev_t* L0 = &M->L[0];
+    const double* bin[3] = {bx, by, bz};
+    const long nc = (long)n[0] * n[1] * n[2];
+    for (int d = 0; d < 3; ++d) memcpy(L0->b[d], bin[d], sizeof(double) * (size_t)(nc / n[d] * (n[d] + 1)));
+    vb_coarsen_coefs(M);
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* r = (double*)malloc(sizeof(double) * (size_t)nc);
+    memcpy(rhs, rhs_in, sizeof(double) * (size_t)nc);
+    if (singular) {
+        double s = 0.0;
+        for (long c = 0; c < nc; ++c) s += rhs[c];
+        const double mean = s / (double)nc;
+        for (long c = 0; c < nc; ++c) rhs[c] -= mean;
+    }
+    for (long c = 0; c < nc; ++c) phi_out[c] = 0.0;
+    double rhsnorm = 0.0;
+    for (long c = 0; c < nc; ++c) rhsnorm = fmax(rhsnorm, fabs(rhs[c]));
+    if (!(rhsnorm == rhsnorm)) rhsnorm = INFINITY;
+    const double target = fmax(atol, rtol * rhsnorm);
+    memset(info, 0, sizeof(*info));
+    info->r0 = rhsnorm;
+    info->nlevels = M->nlev;
+    memcpy(r, rhs, sizeof(double) * (size_t)nc);
+    double rn = rhsnorm;
+    info->converged = (rn <= target);
+    int diverged = 0;
+    for (int it = 1; !info->converged && it <= opts->maxiter; ++it) {
+        memcpy(L0->f, r, sizeof(double) * (size_t)nc);
+        vb_vcycle(M, 0, opts);
+        for (long c = 0; c < nc; ++c) phi_out[c] += L0->e[c];
+        /* r = rhs - A phi on level 0 */
+        rn = 0.0;
+        double* save_e = L0->e;
+        L0->e = phi_out;
+        double* save_f = L0->f;
+        L0->f = rhs;
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i) {
+                    const double v = vb_resid(M, L0, i, j, k);
+                    r[VC(L0, i, j, k)] = v;
+                    const double av = fabs(v);
+                    rn = (av == av) ? fmax(rn, av) : INFINITY;
+                }
+        L0->e = save_e;
+        L0->f = save_f;
+        info->iters = it;
+        if (rn <= target) info->converged = 1;
+        else if (!(rn <= KO_MG_DIVERGE * rhsnorm)) {
+            diverged = 1;
+            break;
+        }
+    }
+    info->rfinal = rn;
+    for (int l = 0; l < M->nlev; ++l) {
+        free(M->L[l].e);
+        free(M->L[l].f);
+        for (int d = 0; d < 3; ++d) free(M->L[l].b[d]);
+    }
+    free(M);
+    free(rhs);
+    free(r);
+    return diverged ? KO_MG_DIVERGED : info->converged ? 0 : 1;
+}
+
+/* ------------------------------------------------------------------ */
 /* MLABecLaplacian with constant coefficients (the implicit diffusion  */
 /* solve, DiffusionOps.cpp:154-160: setScalars(1, dt), a = rho,        */
 /* b = mu): alpha a phi - beta b lap(phi) = rhs.  The multigrid above  */

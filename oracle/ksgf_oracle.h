@@ -117,6 +117,11 @@ int ko_mac_project(const ko_geom* g, double* umac, double* vmac, double* wmac,
 #define KO_MG_DIVERGE 1.0e20
 #define KO_MG_DIVERGED (-5)
 
+/* div(beta grad phi) = rhs with face coefficients (MacProjector's
+ * Array<MultiFab,3> beta): bx (n0+1, n1, n2), by, bz domain-wide */
+int ko_vb_solve(const int* n, const double* dx, const int* lobc, const int* hibc, const double* bx,
+                const double* by, const double* bz, const double* rhs_in, double* phi_out, double rtol,
+                double atol, const ko_mlmg_opts* opts, ko_mlmg_info* info);
 int ko_poisson_solve(const int* n, const double* dx, const int* lobc, const int* hibc,
                      double beta, const double* rhs, double* phi, double rtol, double atol,
                      const ko_mlmg_opts* opts, ko_mlmg_info* info);

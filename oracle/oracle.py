@@ -85,6 +85,8 @@ def lib():
                                      C.POINTER(KoOpts), C.POINTER(KoInfo), _dp]
         L.ko_poisson_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, _dp, C.c_double, C.c_double,
                                        C.POINTER(KoOpts), C.POINTER(KoInfo)]
+        L.ko_vb_solve.argtypes = [_ip, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                  C.POINTER(KoOpts), C.POINTER(KoInfo)]
         L.ko_nodal_dims.argtypes = [_ip, _ip, _ip]
         L.ko_nodal_rhs.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, _ip, _ip, _dp]
         L.ko_nodal_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, _dp, C.c_double, C.c_double,
@@ -245,6 +247,66 @@ def poisson_solve(n, dx, lobc, hibc, beta, rhs, rtol, atol, opts=None):
     st = lib().ko_poisson_solve(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip),
                                 beta, _p(rhs), _p(phi), rtol, atol, C.byref(opts), C.byref(info))
     return phi, dict(info.as_dict(), status=st)
+
+
+def vb_solve(n, dx, lobc, hibc, b3, rhs, rtol, atol, opts=None):
+    """div(beta grad phi) = rhs with face coefficients b3 = (bx, by, bz),
+    domain-wide (x fastest; bx has n0 + 1 faces along x); returns (phi,
+    info)."""
+    opts = opts or default_opts()
+    info = KoInfo()
+    phi = np.zeros(int(np.prod(n)))
+    b3 = [np.ascontiguousarray(b, dtype=np.float64).ravel() for b in b3]
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64).ravel()
+    nn, lo, hi = _iarr(n), _iarr(lobc), _iarr(hibc)
+    dd = np.ascontiguousarray(dx, dtype=np.float64)
+    rc = lib().ko_vb_solve(nn[1], _p(dd), lo[1], hi[1], _p(b3[0]), _p(b3[1]), _p(b3[2]), _p(rhs), _p(phi), rtol,
+                           atol, C.byref(opts), C.byref(info))
+    if rc == -2:
+        raise ValueError("face-coefficient solve: odd periodic extent")
+    return phi, dict(info.as_dict(), status=rc)
+
+
+def mac_project_vb(ba, mac, lobc, hibc, b3, rtol, atol, opts=None):
+    """MacProjector with face coefficients on the domain-wide face velocities
+    mac = (u (nz, ny, nx+1), v, w) and b3 of the same shapes: rhs = div u
+    (k_mac_rhs's expression), the face-coefficient solve, then
+    u -= (beta_face * (1/dx)) * (phi_hi - phi_lo) on every face (periodic wrap;
+    Neumann sides unchanged; Dirichlet sides with the ghost -phi).  Returns
+    (corrected faces, phi, info)."""
+    g = ba.geom
+    n = tuple(int(v) for v in g.n_cell)
+    dx = g.dx
+    u, v, w = (np.asarray(a, dtype=np.float64) for a in mac)
+    rhs = (u[:, :, 1:] - u[:, :, :-1]) * (1.0 / dx[0]) + (v[:, 1:, :] - v[:, :-1, :]) * (1.0 / dx[1]) + \
+        (w[1:, :, :] - w[:-1, :, :]) * (1.0 / dx[2])
+    phi, info = vb_solve(n, dx, lobc, hibc, b3, rhs, rtol, atol, opts)
+    phi = phi.reshape(n[2], n[1], n[0])
+    out = []
+    for d, a in enumerate((u, v, w)):
+        ax = 2 - d
+        a = a.copy()
+        lo_ghost = {0: np.take(phi, [-1], axis=ax), 1: np.take(phi, [0], axis=ax),
+                    2: -np.take(phi, [0], axis=ax)}[lobc[d]]
+        hi_ghost = {0: np.take(phi, [0], axis=ax), 1: np.take(phi, [-1], axis=ax),
+                    2: -np.take(phi, [-1], axis=ax)}[hibc[d]]
+        ext = np.concatenate([lo_ghost, phi, hi_ghost], axis=ax)
+        sl_hi = [slice(None)] * 3
+        sl_lo = [slice(None)] * 3
+        sl_hi[ax] = slice(1, None)
+        sl_lo[ax] = slice(None, -1)
+        bd = np.asarray(b3[d], dtype=np.float64) * (1.0 / dx[d])
+        corr = bd * (ext[tuple(sl_hi)] - ext[tuple(sl_lo)])
+        if lobc[d] == 1:
+            s0 = [slice(None)] * 3
+            s0[ax] = 0
+            corr[tuple(s0)] = 0.0
+        if hibc[d] == 1:
+            s1 = [slice(None)] * 3
+            s1[ax] = -1
+            corr[tuple(s1)] = 0.0
+        out.append(a - corr)
+    return out, phi, info
 
 
 def _iarr(a):
