@@ -458,8 +458,9 @@ __global__ void __launch_bounds__(256) k_tl_absmax(const double* __restrict__ a,
     if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
 }
 
-// sums for the composite mean: coarse over uncovered cells, fine over the
-// valid cells of a MultiFab
+// block partial sums for the composite mean (coarse over uncovered cells,
+// fine over the valid cells of a MultiFab), one slot per block, summed in
+// block order by k_tl_mean: the mean is the same bits on every run
 __global__ void __launch_bounds__(256) k_tl_sum_c(const double* __restrict__ a, const unsigned char* __restrict__ cv,
                                                   long long n, double* __restrict__ out)
 {
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(256) k_tl_sum_c(const double* __restrict__ a, 
         if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(out, sm[0]);
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 __global__ void __launch_bounds__(256) k_tl_sum_f(CMFDev f, const int* __restrict__ box, RegDev r, int nchunks,
                                                   double* __restrict__ out)
@@ -492,7 +493,7 @@ __global__ void __launch_bounds__(256) k_tl_sum_f(CMFDev f, const int* __restric
         if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(out, sm[0]);
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 __global__ void __launch_bounds__(256) k_tl_shift_c(double* __restrict__ a, const unsigned char* __restrict__ cv,
                                                     long long n, const double* __restrict__ mu, int all)
@@ -507,10 +508,19 @@ __global__ void __launch_bounds__(256) k_tl_shift_f(MFDev f, const int* __restri
     if (!region_locate(r, box, b, i, j, k)) return;
     fab_at(f, box, b)(i, j, k) -= *mu;
 }
-__global__ void k_tl_mean(double* __restrict__ s, double wc, double wf, double den)
+constexpr int kTlSumC = 1024, kTlSumF = 2048;  // blocks of the composite-mean partial sums
+
+__global__ void k_tl_mean(double* __restrict__ s, const double* __restrict__ pc, int nbc,
+                          const double* __restrict__ pf, int nbf, double wc, double wf, double den)
 {
-    // s[0]: coarse sum, s[1]: fine sum -> s[2] = (8 sc + sf) / den
-    s[2] = (wc * s[0] + wf * s[1]) / den;
+    // s[0]: coarse sum, s[1]: fine sum (block partials in block order) ->
+    // s[2] = (8 sc + sf) / den
+    double sc = 0.0, sf = 0.0;
+    for (int b = 0; b < nbc; ++b) sc += pc[b];
+    for (int b = 0; b < nbf; ++b) sf += pf[b];
+    s[0] = sc;
+    s[1] = sf;
+    s[2] = (wc * sc + wf * sf) / den;
 }
 
 }  // namespace
@@ -535,7 +545,7 @@ struct ksgf_macproj_levels {
     long long nc = 0, nfg = 0, nf0 = 0;
     long long ncov = 0, nfine = 0;
     unsigned long long* dmax = nullptr;  // 2 slots
-    double* dsum = nullptr;              // 3 slots
+    double* dsum = nullptr;              // 3 slots + block partials (coarse, fine)
     unsigned long long* hmax = nullptr;  // pinned
 };
 
@@ -623,13 +633,15 @@ struct TLRun {
     // from the covered coarse cells
     int sub_mean(double* a, MFDev f, CMFDev cf, int all_coarse)
     {
-        KSGF_HIP_CHECK(hipMemsetAsync(m->dsum, 0, 3 * sizeof(double), st));
-        hipLaunchKernelGGL(k_tl_sum_c, dim3(std::min(1024, grid_for(m->nc, 256))), dim3(256), 0, st, (const double*)a,
-                           (const unsigned char*)m->cv, m->nc, m->dsum);
-        hipLaunchKernelGGL(k_tl_sum_f, dim3(std::min(nv_chunks, 2048)), dim3(256), 0, st, cf, m->fba->dbox, rv,
-                           nv_chunks, m->dsum + 1);
+        const int nbc = std::min(kTlSumC, grid_for(m->nc, 256)), nbf = std::min(nv_chunks, kTlSumF);
+        double* pc = m->dsum + 3;
+        double* pf = pc + kTlSumC;
+        hipLaunchKernelGGL(k_tl_sum_c, dim3(nbc), dim3(256), 0, st, (const double*)a, (const unsigned char*)m->cv,
+                           m->nc, pc);
+        hipLaunchKernelGGL(k_tl_sum_f, dim3(nbf), dim3(256), 0, st, cf, m->fba->dbox, rv, nv_chunks, pf);
         const double den = 8.0 * (double)(m->nc - m->ncov) + (double)m->nfine;
-        hipLaunchKernelGGL(k_tl_mean, dim3(1), dim3(1), 0, st, m->dsum, 8.0, 1.0, den);
+        hipLaunchKernelGGL(k_tl_mean, dim3(1), dim3(1), 0, st, m->dsum, (const double*)pc, nbc, (const double*)pf,
+                           nbf, 8.0, 1.0, den);
         hipLaunchKernelGGL(k_tl_shift_c, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, a, (const unsigned char*)m->cv,
                            m->nc, (const double*)(m->dsum + 2), all_coarse);
         hipLaunchKernelGGL(k_tl_shift_f, dim3(nv_chunks), dim3(256), 0, st, f, m->fba->dbox, rv,
@@ -734,7 +746,7 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
         hipMalloc(&m->bf, m->nf0 * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->rf, m->nf0 * sizeof(double)) != hipSuccess ||
         hipMalloc(&m->dmax, 2 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&m->dsum, 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&m->dsum, (3 + kTlSumC + kTlSumF) * sizeof(double)) != hipSuccess ||
         hipHostMalloc(&m->hmax, sizeof(unsigned long long)) != hipSuccess) {
         set_error("ksgf_macproj_levels_create: out of device memory");
         return fail(KSGF_ERR_HIP);

@@ -157,3 +157,34 @@ def test_macproj_levels_maxiter_reports_not_converged(ksgf, oracle, gpu):
                                                 0.0, o)
     assert rinfo["status"] == 1 and rinfo["iters"] == 2
     assert abs(info["rfinal"] - rinfo["rfinal"]) <= 1e-8 * rinfo["r0"]
+
+
+def test_macproj_levels_bitwise_reproducible(ksgf, gpu):
+    """the composite mean that makes the singular (no Dirichlet side) solve
+    unique is summed in a fixed block order: two projections of the same
+    input give the same bits (and the same phi)"""
+    import test_multilevel_oracle as T
+
+    _say("macproj_levels reproducible")
+    rng = np.random.default_rng(23)
+    n = (16, 12, 16)
+    cg = lay.Geometry(n, prob_hi=(1.0, 0.75, 1.0), periodic=(1, 0, 1))
+    cba = lay.BoxArray(cg, lay.chop_random(n, [4, 8], rng, granularity=4))
+    nf = tuple(2 * v for v in n)
+    fg = lay.Geometry(nf, prob_hi=cg.prob_hi, periodic=(1, 0, 1))
+    fba = T.FineBoxArray(fg, [[0, 4, 8, 7, 11, 23], [24, 8, 8, 31, 19, 15], [8, 12, 24, 23, 23, 31]])
+    ucd = _random_faces(rng, n, True)
+    ufd = _random_faces(rng, nf, True)
+    dc = ksgf.DeviceBoxArray(cba)
+    df = ksgf.DeviceBoxArray(fba, covers_domain=False)
+    mp = ksgf.MacProjectorLevels(dc, df, (P_, N_, P_), (P_, N_, P_), 1.3)
+    outs = []
+    for _ in range(2):
+        mc = [dc.upload(cba.scatter(ucd[d][None], 1 + d, 1)) for d in range(3)]
+        mf = [df.upload(fba.scatter(ufd[d][None], 1 + d, 1)) for d in range(3)]
+        info = mp.project(mc, mf, 1, 1e-11, 0.0)
+        assert info["converged"] == 1
+        pc, pf = mp.phi()
+        outs.append([to_np(a).copy() for a in mc + mf] + [to_np(pc).copy(), to_np(pf).copy()])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
