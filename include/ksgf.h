@@ -330,6 +330,44 @@ int ksgf_abeclap_apply(ksgf_abeclap* ab, double* out, int ng_out, const double* 
                        int ncomp, int comp, ksgf_stream_t stream);
 int ksgf_abeclap_destroy(ksgf_abeclap* ab);
 
+/* amrex::MLTensorOp solved by MLMG: the default ICNS viscous operator
+ * (ICNSDiffTensorOp, src/equation_systems/icns/icns_diffusion.H:18-50, chosen
+ * by use_tensor_operator = true at :492-506; set up by
+ * DiffSolverIface::setup_operator, DiffusionOps.cpp:57-110: setScalars,
+ * setLevelBC, setACoeffs(rho), setShearViscosity(eta faces) at
+ * DiffusionOps.H:54-71; no bulk viscosity):
+ *   L u = alpha a u - beta div(eta (grad u + grad u^T) - (2/3) eta (div u) I).
+ * create: lobc / hibc hold 9 KSGF_LO_* values, [component * 3 + direction]
+ * (get_diffuse_tensor_bc's Vector<Array<LinOpBCType,3>>, incflo_diffusion.cpp:
+ * 11-69); Dirichlet and Neumann sides are homogeneous (walls at rest);
+ * periodic exactly in the geometry's periodic directions, with even extents.
+ * Coefficients start as MLABecLaplacian's (a = 0, eta = 1); the scalars must
+ * be set before apply / solve.  set_acoeffs: cell MultiFab, 1 comp;
+ * set_shear_viscosity: face MultiFabs (ksgf_average_velocity_eta_to_faces of
+ * the cell viscosity).  apply = MLMG::apply (compute_diff_term: scalars
+ * (0, -1), out = div tau); solve = MLMG::solve (linsys_solve: scalars
+ * (1, dt_diff), rhs = rho u): 3-component cell MultiFabs, valid cells read /
+ * written, sol starts from zero; FP64 (alpha >= 0, beta > 0; a singular
+ * operator is KSGF_ERR_INVALID). */
+typedef struct ksgf_tensorop ksgf_tensorop;
+int ksgf_tensorop_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc,
+                         const ksgf_mlmg_options* opts, ksgf_tensorop** out);
+int ksgf_tensorop_set_scalars(ksgf_tensorop* op, double alpha, double beta);
+int ksgf_tensorop_set_acoeffs(ksgf_tensorop* op, const double* a, int ngrow, ksgf_stream_t stream);
+int ksgf_tensorop_set_shear_viscosity(ksgf_tensorop* op, const double* eta_x, const double* eta_y,
+                                      const double* eta_z, int ngrow, ksgf_stream_t stream);
+int ksgf_tensorop_apply(ksgf_tensorop* op, double* out, int ng_out, const double* in, int ng_in,
+                        ksgf_stream_t stream);
+int ksgf_tensorop_solve(ksgf_tensorop* op, double* sol, int ng_sol, const double* rhs, int ng_rhs,
+                        double rtol, double atol, ksgf_mlmg_info* info, ksgf_stream_t stream);
+int ksgf_tensorop_destroy(ksgf_tensorop* op);
+/* diffusion::average_velocity_eta_to_faces (src/diffusion/incflo_diffusion.cpp:
+ * 108-128): average_cellcenter_to_face of the cell viscosity, then
+ * fixup_eta_on_domain_faces (:130-200): a face on a non-periodic domain side
+ * takes the one cell beside it.  cc: cell, 1 comp, ng_cc >= 1 filled. */
+int ksgf_average_velocity_eta_to_faces(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx,
+                                       double* fy, double* fz, int ng_face, ksgf_stream_t stream);
+
 /* Two-level averaging, fine level at refinement ratio `ratio` (2 or 4; the fine
  * domain is the coarse domain refined, the fine boxes coarsen exactly):
  * amrex::average_down of cell data (gp after the nodal projection,

@@ -69,6 +69,9 @@ EXPORTS = [
     "ksgf_icns_src_term_divtau", "ksgf_boxarray_create_level", "ksgf_average_down", "ksgf_average_down_faces",
     "ksgf_fillpatch_two_levels", "ksgf_macproj_levels_create", "ksgf_macproj_levels_project",
     "ksgf_macproj_levels_get_phi", "ksgf_macproj_levels_destroy", "ksgf_fillpatch_faces_two_levels",
+    "ksgf_tensorop_create", "ksgf_tensorop_set_scalars", "ksgf_tensorop_set_acoeffs",
+    "ksgf_tensorop_set_shear_viscosity", "ksgf_tensorop_apply", "ksgf_tensorop_solve", "ksgf_tensorop_destroy",
+    "ksgf_average_velocity_eta_to_faces",
 ]
 
 _lib = None
@@ -190,6 +193,15 @@ def load(path=None):
                                      C.c_double, C.POINTER(KsgfMLMGInfo), C.c_void_p]
     L.ksgf_abeclap_apply.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_abeclap_destroy.argtypes = [C.c_void_p]
+    L.ksgf_tensorop_create.argtypes = [C.c_void_p, _ip, _ip, C.c_void_p, C.POINTER(C.c_void_p)]
+    L.ksgf_tensorop_set_scalars.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    L.ksgf_tensorop_set_acoeffs.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p]
+    L.ksgf_tensorop_set_shear_viscosity.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_void_p]
+    L.ksgf_tensorop_apply.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_void_p]
+    L.ksgf_tensorop_solve.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double, C.c_void_p,
+                                      C.c_void_p]
+    L.ksgf_tensorop_destroy.argtypes = [C.c_void_p]
+    L.ksgf_average_velocity_eta_to_faces.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p]
     L.ksgf_boxarray_create_level.argtypes = [C.POINTER(KsgfGeometry), _ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.ksgf_average_down.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.ksgf_average_down_faces.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_void_p, _dp, _dp, _dp, C.c_int,
@@ -405,6 +417,15 @@ class DeviceBoxArray:
                                                      _ptr(out[2]), ng_face, _stream()), "average_cellcenter_to_face")
         return out
 
+    def average_velocity_eta_to_faces(self, cc, ng_cc, ng_face=0):
+        """diffusion::average_velocity_eta_to_faces: face means of a cell
+        viscosity, the adjacent cell on non-periodic domain faces"""
+        out = [self.alloc(XFACE + d, 1, ng_face) for d in range(3)]
+        check(load().ksgf_average_velocity_eta_to_faces(self.h, _ptr(cc), ng_cc, _ptr(out[0]), _ptr(out[1]),
+                                                        _ptr(out[2]), ng_face, _stream()),
+              "average_velocity_eta_to_faces")
+        return out
+
     def invert(self, mf, itype, ncomp, ngrow, nghost, numerator):
         check(load().ksgf_invert(self.h, _ptr(mf), itype, ncomp, ngrow, nghost, numerator, _stream()), "invert")
         return mf
@@ -590,6 +611,60 @@ class MacProjectorLevels:
     def close(self):
         if self.h:
             load().ksgf_macproj_levels_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TensorOp:
+    """amrex::MLTensorOp + MLMG over a DeviceBoxArray (the default ICNS viscous
+    operator): alpha a u - beta div(eta (grad u + grad u^T) - (2/3) eta (div u)
+    I) on 3-component cell MultiFabs.  lobc / hibc: (3 components, 3
+    directions) of KSGF_LO_* values."""
+
+    def __init__(self, dba: DeviceBoxArray, lobc, hibc, options=None):
+        import numpy as np
+
+        L = load()
+        self.dba = dba
+        self._lo = np.ascontiguousarray(lobc, dtype=np.int32).ravel()
+        self._hi = np.ascontiguousarray(hibc, dtype=np.int32).ravel()
+        if self._lo.size != 9 or self._hi.size != 9:
+            raise ValueError("TensorOp: lobc / hibc need 3 components x 3 directions")
+        self.options = options or default_options()
+        h = C.c_void_p()
+        check(L.ksgf_tensorop_create(dba.h, self._lo.ctypes.data_as(_ip), self._hi.ctypes.data_as(_ip),
+                                     C.byref(self.options), C.byref(h)), "ksgf_tensorop_create")
+        self.h = h
+        self.info = None
+
+    def set_scalars(self, alpha, beta):
+        check(load().ksgf_tensorop_set_scalars(self.h, alpha, beta), "tensorop_set_scalars")
+
+    def set_acoeffs(self, a, ngrow):
+        check(load().ksgf_tensorop_set_acoeffs(self.h, _ptr(a), ngrow, _stream()), "tensorop_set_acoeffs")
+
+    def set_shear_viscosity(self, faces, ngrow):
+        check(load().ksgf_tensorop_set_shear_viscosity(self.h, _ptr(faces[0]), _ptr(faces[1]), _ptr(faces[2]), ngrow,
+                                                       _stream()), "tensorop_set_shear_viscosity")
+
+    def apply(self, out, ng_out, x, ng_x):
+        check(load().ksgf_tensorop_apply(self.h, _ptr(out), ng_out, _ptr(x), ng_x, _stream()), "tensorop_apply")
+
+    def solve(self, sol, ng_sol, rhs, ng_rhs, rtol, atol):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_tensorop_solve(self.h, _ptr(sol), ng_sol, _ptr(rhs), ng_rhs, rtol, atol, C.byref(info),
+                                         _stream()), "tensorop_solve", allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def close(self):
+        if self.h:
+            load().ksgf_tensorop_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -22,7 +22,7 @@ LEVCHK = os.path.join(HERE, "ksgf_levels_check")
 HOSTDIR = os.path.join(HERE, "host")
 
 SOURCES = ["ksgf_layout.hip", "ksgf_godunov.hip", "ksgf_mlmg.hip", "ksgf_ops.hip", "ksgf_adjoint.hip",
-           "ksgf_io.hip", "ksgf_nodal.hip", "ksgf_levels.hip", "ksgf_mlmg_vb.hip"]
+           "ksgf_io.hip", "ksgf_nodal.hip", "ksgf_levels.hip", "ksgf_mlmg_vb.hip", "ksgf_tensor.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -294,7 +294,10 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
 
 // MAC projection with face coefficients (ksgf_mlmg_vb.hip)
 struct VbSolver;
-VbSolver* vb_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, const ksgf_mlmg_options& opt, int* err);
+// inner_only: no level-0 phi / rhs / residual arrays (the caller drives
+// vb_vcycle on the level-0 f and e itself)
+VbSolver* vb_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, const ksgf_mlmg_options& opt, int* err,
+                    int inner_only = 0);
 void vb_destroy(VbSolver* S);
 int vb_set_faces(VbSolver* S, const ksgf_boxarray* ba, const double* const* b3, int ng, hipStream_t st);
 int vb_project(VbSolver* S, const ksgf_boxarray* ba, double* const* mac, int ng, double rtol, double atol,
@@ -302,5 +305,15 @@ int vb_project(VbSolver* S, const ksgf_boxarray* ba, double* const* mac, int ng,
 int vb_solve(VbSolver* S, const double* rhs, double* phi, double rtol, double atol, ksgf_mlmg_info* info,
              hipStream_t st);
 int vb_get_phi(VbSolver* S, double* phi, hipStream_t st);
+// level-0 coefficients from domain-wide arrays: faces scale[d] * b3[d] and the
+// zeroth-order term ac_scale * a0 (a0 == nullptr: constant), coarsened to
+// every level; the operator becomes div(b grad e) - ac e
+int vb_set_coefs(VbSolver* S, const double* const* b3, const double* scale, const double* a0, double ac_scale,
+                 hipStream_t st);
+// one V-cycle from zero: level-0 e from level-0 f (div(b grad e) - ac e = f)
+void vb_vcycle(VbSolver* S, hipStream_t st);
+double* vb_level0_f(VbSolver* S);
+const double* vb_level0_e(const VbSolver* S);
+int vb_nlevels(const VbSolver* S);
 
 }  // namespace ksgf

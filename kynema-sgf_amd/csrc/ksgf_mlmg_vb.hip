@@ -32,6 +32,7 @@ struct VbLev {
     int r[3];  // coarsening ratio to the next level
     double ih2[3];
     double *b[3], *e, *f;
+    double* ac = nullptr;  // zeroth-order coefficient (alpha a), when set
     long long nc;
 };
 
@@ -39,6 +40,7 @@ struct VbDevLev {  // kernel argument: one level
     int n0, n1, n2;
     double ih2x, ih2y, ih2z;
     const double *bx, *by, *bz;
+    const double* ac;  // nullptr: no zeroth-order term
 };
 
 struct VbSolver {
@@ -93,6 +95,7 @@ __device__ __forceinline__ void vb_cell(const VbDevLev& L, const VbBc& bc, const
             g += b;
         }
     }
+    if (L.ac) g += L.ac[i + (long long)L.n0 * (j + (long long)L.n1 * k)];
     acc = a;
     dg = g;
 }
@@ -387,6 +390,7 @@ static VbDevLev vb_dev(const VbLev& L)
     d.bx = L.b[0];
     d.by = L.b[1];
     d.bz = L.b[2];
+    d.ac = L.ac;
     return d;
 }
 
@@ -402,6 +406,8 @@ static VbBc vb_bc(const VbSolver* S)
 
 static int vb_grid(long long n) { return (int)((n + 255) / 256); }
 
+static int vb_coarsen_faces(VbSolver* S, hipStream_t st);
+
 void vb_destroy(VbSolver* S)
 {
     if (!S) return;
@@ -410,6 +416,7 @@ void vb_destroy(VbSolver* S)
             if (S->L[l].b[d]) (void)hipFree(S->L[l].b[d]);
         if (S->L[l].e) (void)hipFree(S->L[l].e);
         if (S->L[l].f) (void)hipFree(S->L[l].f);
+        if (S->L[l].ac) (void)hipFree(S->L[l].ac);
     }
     if (S->phi) (void)hipFree(S->phi);
     if (S->rhs) (void)hipFree(S->rhs);
@@ -419,7 +426,8 @@ void vb_destroy(VbSolver* S)
     delete S;
 }
 
-VbSolver* vb_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, const ksgf_mlmg_options& opt, int* err)
+VbSolver* vb_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, const ksgf_mlmg_options& opt, int* err,
+                    int inner_only)
 {
     VbSolver* S = new VbSolver();
     S->opt = opt;
@@ -492,7 +500,8 @@ VbSolver* vb_create(const ksgf_boxarray* ba, const int* lobc, const int* hibc, c
         for (int d = 0; d < 3 && ok; ++d) ok = alloc(&L.b[d], L.nc / L.n[d] * (L.n[d] + 1));
     }
     const long long n0 = S->L[0].nc;
-    ok = ok && alloc(&S->phi, n0) && alloc(&S->rhs, n0) && alloc(&S->r, n0) && alloc(&S->part, kVbBlocks + 1);
+    if (!inner_only) ok = ok && alloc(&S->phi, n0) && alloc(&S->rhs, n0) && alloc(&S->r, n0);
+    ok = ok && alloc(&S->part, kVbBlocks + 1);
     if (ok && hipHostMalloc((void**)&S->hbuf, sizeof(double)) != hipSuccess) {
         set_error("face-coefficient MAC projection: pinned buffer");
         ok = false;
@@ -530,6 +539,11 @@ int vb_set_faces(VbSolver* S, const ksgf_boxarray* ba, const double* const* b3, 
                                ba->dlo[0], ba->dlo[1], ba->dlo[2], S->n[0], S->n[1], S->L[0].b[2]);
         KSGF_LAUNCH_CHECK();
     }
+    return vb_coarsen_faces(S, st);
+}
+
+static int vb_coarsen_faces(VbSolver* S, hipStream_t st)
+{
     for (int l = 0; l + 1 < S->nlev; ++l) {
         const VbLev& F = S->L[l];
         const VbLev& C = S->L[l + 1];
@@ -562,7 +576,7 @@ static void vb_sweeps(VbSolver* S, int l, int nsweep, hipStream_t st)
                                color);
 }
 
-static void vb_vcycle(VbSolver* S, hipStream_t st)
+void vb_vcycle(VbSolver* S, hipStream_t st)
 {
     const VbBc bc = vb_bc(S);
     const int last = S->nlev - 1;
@@ -590,6 +604,66 @@ static void vb_vcycle(VbSolver* S, hipStream_t st)
         vb_sweeps(S, l, S->opt.num_post_smooth, st);
     }
 }
+
+// dst = s * src (src == nullptr: dst = s)
+__global__ void __launch_bounds__(256) k_vb_scale(double* __restrict__ dst, const double* __restrict__ src, long long n,
+                                                  double s)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src ? s * src[t] : s;
+}
+
+// coarse cell coefficient: the average of the fine cells it covers
+__global__ void __launch_bounds__(256) k_vb_coarsen_cells(const double* __restrict__ af, int f0, int f1,
+                                                          double* __restrict__ ac, int c0, int c1, int c2, int r0,
+                                                          int r1, int r2)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)c0 * c1 * c2) return;
+    int I, J, K;
+    vb_decode(t, c0, c1, I, J, K);
+    double s = 0.0;
+    for (int dk = 0; dk < r2; ++dk)
+        for (int dj = 0; dj < r1; ++dj)
+            for (int di = 0; di < r0; ++di)
+                s += af[(I * r0 + di) + (long long)f0 * ((J * r1 + dj) + (long long)f1 * (K * r2 + dk))];
+    ac[t] = s * (1.0 / (double)(r0 * r1 * r2));
+}
+
+// level-0 coefficients from domain-wide arrays: face d = scale[d] * b3[d],
+// zeroth-order term ac_scale * a0 (a0 == nullptr: the constant ac_scale);
+// then every coarser level by averaging
+int vb_set_coefs(VbSolver* S, const double* const* b3, const double* scale, const double* a0, double ac_scale,
+                 hipStream_t st)
+{
+    for (int l = 0; l < S->nlev; ++l)
+        if (!S->L[l].ac && hipMalloc(&S->L[l].ac, sizeof(double) * (size_t)S->L[l].nc) != hipSuccess) {
+            set_error("face-coefficient multigrid: hipMalloc");
+            return KSGF_ERR_NOMEM;
+        }
+    const VbLev& L0 = S->L[0];
+    for (int d = 0; d < 3; ++d) {
+        const long long m = L0.nc / L0.n[d] * (L0.n[d] + 1);
+        hipLaunchKernelGGL(k_vb_scale, dim3(vb_grid(m)), dim3(256), 0, st, L0.b[d], b3[d], m, scale[d]);
+    }
+    hipLaunchKernelGGL(k_vb_scale, dim3(vb_grid(L0.nc)), dim3(256), 0, st, L0.ac, a0, L0.nc, ac_scale);
+    KSGF_LAUNCH_CHECK();
+    int e = vb_coarsen_faces(S, st);
+    if (e) return e;
+    for (int l = 0; l + 1 < S->nlev; ++l) {
+        const VbLev& F = S->L[l];
+        const VbLev& C = S->L[l + 1];
+        hipLaunchKernelGGL(k_vb_coarsen_cells, dim3(vb_grid(C.nc)), dim3(256), 0, st, (const double*)F.ac, F.n[0],
+                           F.n[1], C.ac, C.n[0], C.n[1], C.n[2], F.r[0], F.r[1], F.r[2]);
+    }
+    KSGF_LAUNCH_CHECK();
+    S->singular = 0;
+    return KSGF_OK;
+}
+
+double* vb_level0_f(VbSolver* S) { return S->L[0].f; }
+const double* vb_level0_e(const VbSolver* S) { return S->L[0].e; }
+int vb_nlevels(const VbSolver* S) { return S->nlev; }
 
 static int vb_read(VbSolver* S, const double* dsrc, hipStream_t st, double* out)
 {

@@ -7,6 +7,8 @@
 #include <mutex>
 #include <string>
 
+#include <climits>
+
 #include "ksgf_internal.h"
 
 namespace ksgf {
@@ -140,7 +142,8 @@ __global__ void __launch_bounds__(256) k_invert(MFDev m, int ncomp, RegDev r, co
 
 // amrex::average_cellcenter_to_face: the mean of the two cells of each face
 template <int D>
-__global__ void __launch_bounds__(256) k_cc_to_face(CMFDev cc, MFDev fc, RegDev r, const int* __restrict__ box)
+__global__ void __launch_bounds__(256) k_cc_to_face(CMFDev cc, MFDev fc, RegDev r, const int* __restrict__ box,
+                                                    int fix_lo, int fix_hi)
 {
     int b, i, j, k;
     if (!region_locate(r, box, b, i, j, k)) return;
@@ -149,7 +152,15 @@ __global__ void __launch_bounds__(256) k_cc_to_face(CMFDev cc, MFDev fc, RegDev 
     if (D == 0) qi -= 1;
     if (D == 1) qj -= 1;
     if (D == 2) qk -= 1;
-    fab_at(fc, box, b)(i, j, k) = 0.5 * (c(qi, qj, qk) + c(i, j, k));
+    const int x = (D == 0) ? i : (D == 1) ? j : k;
+    double v;
+    if (x == fix_lo)
+        v = c(i, j, k);  // a non-periodic domain face: the one cell beside it
+    else if (x == fix_hi)
+        v = c(qi, qj, qk);
+    else
+        v = 0.5 * (c(qi, qj, qk) + c(i, j, k));
+    fab_at(fc, box, b)(i, j, k) = v;
 }
 
 __global__ void __launch_bounds__(256) k_predictor(MFDev vn, CMFDev vo, CMFDev ro, CMFDev rn, CMFDev conv,
@@ -566,14 +577,10 @@ int ksgf_invert(const ksgf_boxarray* ba, double* mf, int itype, int ncomp, int n
     return KSGF_OK;
 }
 
-int ksgf_average_cellcenter_to_face(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx, double* fy,
-                                    double* fz, int ng_face, ksgf_stream_t stream)
+// fixup: faces on a non-periodic domain side take the one cell beside them
+static int cc_to_face(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx, double* fy, double* fz,
+                      int ng_face, int fixup, hipStream_t st)
 {
-    if (!ba || !cc || !fx || !fy || !fz || ng_cc < 1 || ng_face < 0) {
-        set_error("ksgf_average_cellcenter_to_face: bad argument (the cell MultiFab needs a filled ghost layer)");
-        return KSGF_ERR_INVALID;
-    }
-    hipStream_t st = (hipStream_t)stream;
     CMFDev c;
     int e = cmfdev(ba, cc, KSGF_CELL, 1, ng_cc, &c);
     if (e) return e;
@@ -587,15 +594,37 @@ int ksgf_average_cellcenter_to_face(const ksgf_boxarray* ba, const double* cc, i
         const Region* r = get_region(ba, glo, ghi, &err);
         if (!r) return err;
         dim3 g(r->nchunks);
+        const bool fix = fixup && !ba->periodic[d];
+        const int flo = fix ? ba->dlo[d] : INT_MIN, fhi = fix ? ba->dhi[d] + 1 : INT_MIN;
         if (d == 0)
-            hipLaunchKernelGGL(k_cc_to_face<0>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox);
+            hipLaunchKernelGGL(k_cc_to_face<0>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox, flo, fhi);
         else if (d == 1)
-            hipLaunchKernelGGL(k_cc_to_face<1>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox);
+            hipLaunchKernelGGL(k_cc_to_face<1>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox, flo, fhi);
         else
-            hipLaunchKernelGGL(k_cc_to_face<2>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox);
+            hipLaunchKernelGGL(k_cc_to_face<2>, g, dim3(256), 0, st, c, m, regdev(r, ba->nbox), ba->dbox, flo, fhi);
         KSGF_LAUNCH_CHECK();
     }
     return KSGF_OK;
+}
+
+int ksgf_average_cellcenter_to_face(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx, double* fy,
+                                    double* fz, int ng_face, ksgf_stream_t stream)
+{
+    if (!ba || !cc || !fx || !fy || !fz || ng_cc < 1 || ng_face < 0) {
+        set_error("ksgf_average_cellcenter_to_face: bad argument (the cell MultiFab needs a filled ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    return cc_to_face(ba, cc, ng_cc, fx, fy, fz, ng_face, 0, (hipStream_t)stream);
+}
+
+int ksgf_average_velocity_eta_to_faces(const ksgf_boxarray* ba, const double* cc, int ng_cc, double* fx, double* fy,
+                                       double* fz, int ng_face, ksgf_stream_t stream)
+{
+    if (!ba || !cc || !fx || !fy || !fz || ng_cc < 1 || ng_face < 0) {
+        set_error("ksgf_average_velocity_eta_to_faces: bad argument (the cell MultiFab needs a filled ghost layer)");
+        return KSGF_ERR_INVALID;
+    }
+    return cc_to_face(ba, cc, ng_cc, fx, fy, fz, ng_face, 1, (hipStream_t)stream);
 }
 
 int ksgf_cfl_max_noalloc(const ksgf_boxarray* ba, const double* vel, int ngrow, double* out, ksgf_stream_t stream)

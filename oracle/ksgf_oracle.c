@@ -34,7 +34,8 @@
 /* Planted mistakes for the tests that show a check can fail (never set
  * outside tests): bit 0 drops the transverse terms of ExtrapVelToFaces,
  * bit 1 doubles them, bit 2 flips their sign, bit 3 drops the transverse
- * terms of ComputeFluxesOnBoxFromState. */
+ * terms of ComputeFluxesOnBoxFromState; bits 4-6 break the tensor viscous
+ * operator (ko_tensor_apply). */
 static int ko_plant = 0;
 void ko_set_plant(int bits) { ko_plant = bits; }
 
@@ -3379,5 +3380,238 @@ int ko_nodal_project(const ko_geom* g, double* vel, int ng, const int* lobc, con
     if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)T);
     free(rhs);
     free(phi);
+    return ret;
+}
+
+/* ------------------------------------------------------------------ */
+/* MLTensorOp: the default ICNS viscous operator (f2)                  */
+/* ------------------------------------------------------------------ */
+/* ICNSDiffTensorOp (icns_diffusion.H:18-50) drives amrex::MLTensorOp through
+ * DiffSolverIface (DiffusionOps.cpp:57-110, DiffusionOps.H:54-71):
+ * setScalars(alpha, beta), setACoeffs(rho), setShearViscosity(eta on faces);
+ * no bulk viscosity is set, so
+ *   L u = alpha a u - beta div(tau),
+ *   tau = eta (grad u + grad u^T) - (2/3) eta (div u) I.
+ * Face discretisation (the published MLTensorOp method; the AMReX source is
+ * absent, so agreement with its rounding is unpinned).  Face f of direction
+ * d lies between the cells f - e_d (below) and f (above); for component n
+ *   tau_dn(f) = eta_d(f) * g,
+ *   d == n: g = (4/3) N_n - (2/3) sum_{m != n, m ascending} T_mm
+ *   d != n: g = N_n + T_dn
+ * with N_n the difference of u_n across the face over dx_d, and T_qm the
+ * derivative of u_q along m (m != d) on the face: the mean of the central
+ * differences of the two cells beside it, (D(above) + D(below)) (dxi_m / 4).
+ *   (L u)_n = alpha a u_n - beta ((tau_xn(+) - tau_xn(-)) dxi_x
+ *             + (tau_yn(+) - tau_yn(-)) dxi_y + (tau_zn(+) - tau_zn(-)) dxi_z)
+ * Domain sides, per component and side (get_diffuse_tensor_bc,
+ * incflo_diffusion.cpp:11-69), homogeneous (walls at rest):
+ *   Dirichlet: u = 0 on the face: ghost -u (the max_order 2 linear rule);
+ *     on the face itself the derivatives along it (of the boundary value) are 0;
+ *   Neumann: ghost u, N = 0 on the face; derivatives along the face from the
+ *     interior cell alone, D (dxi_m / 2);
+ *   periodic: wrap.
+ * Planted mistakes (tests only): bit 4 drops T_dn (the transpose term), bit 5
+ * drops the -(2/3) divergence term, bit 6 takes T from the cell above alone. */
+typedef struct {
+    int n[3];
+    double dxi[3];
+    const int *lo, *hi; /* [component * 3 + direction] */
+    const double* u;    /* 3 components, component slowest */
+    long nc;
+} tn_t;
+
+static double tn_at(const tn_t* T, int q, const int* p)
+{
+    return T->u[q * T->nc + p[0] + (long)T->n[0] * (p[1] + (long)T->n[1] * p[2])];
+}
+
+/* u_q at p + s e_m (s = +-1), the ghost rule of component q beyond a side */
+static double tn_nb(const tn_t* T, int q, const int* p, int m, int s)
+{
+    int c[3] = {p[0], p[1], p[2]};
+    c[m] += s;
+    if (c[m] < 0 || c[m] >= T->n[m]) {
+        const int bc = (s < 0) ? T->lo[q * 3 + m] : T->hi[q * 3 + m];
+        if (bc != KO_LO_PERIODIC) {
+            const double v = tn_at(T, q, p);
+            return (bc == KO_LO_NEUMANN) ? v : -v;
+        }
+        c[m] = (c[m] < 0) ? c[m] + T->n[m] : c[m] - T->n[m];
+    }
+    return tn_at(T, q, c);
+}
+
+/* undivided central difference of u_q along m at cell p */
+static double tn_diff(const tn_t* T, int q, int m, const int* p)
+{
+    return tn_nb(T, q, p, m, 1) - tn_nb(T, q, p, m, -1);
+}
+
+/* T_qm on face f of direction d (m != d) */
+static double tn_tr(const tn_t* T, int q, int m, int d, const int* f)
+{
+    int a[3] = {f[0], f[1], f[2]}, b[3] = {f[0], f[1], f[2]};
+    b[d] -= 1;
+    const int at_lo = (f[d] == 0), at_hi = (f[d] == T->n[d]);
+    if (at_lo || at_hi) {
+        const int bc = at_lo ? T->lo[q * 3 + d] : T->hi[q * 3 + d];
+        if (bc == KO_LO_DIRICHLET) return 0.0;
+        if (bc == KO_LO_NEUMANN) return tn_diff(T, q, m, at_lo ? a : b) * (0.5 * T->dxi[m]);
+        if (at_lo)
+            b[d] = T->n[d] - 1;
+        else
+            a[d] = 0;
+    }
+    if (ko_plant & 64) return tn_diff(T, q, m, a) * (0.5 * T->dxi[m]);
+    return (tn_diff(T, q, m, a) + tn_diff(T, q, m, b)) * (0.25 * T->dxi[m]);
+}
+
+/* N_q: the difference of u_q across face f of direction d over dx_d */
+static double tn_nd(const tn_t* T, int q, int d, const int* f)
+{
+    int a[3] = {f[0], f[1], f[2]}, b[3] = {f[0], f[1], f[2]};
+    b[d] -= 1;
+    const int at_lo = (f[d] == 0), at_hi = (f[d] == T->n[d]);
+    if (at_lo || at_hi) {
+        const int bc = at_lo ? T->lo[q * 3 + d] : T->hi[q * 3 + d];
+        if (bc == KO_LO_NEUMANN) return 0.0;
+        if (bc == KO_LO_DIRICHLET) {
+            const double v = tn_at(T, q, at_lo ? a : b);
+            return at_lo ? (v - (-v)) * T->dxi[d] : ((-v) - v) * T->dxi[d];
+        }
+        if (at_lo)
+            b[d] = T->n[d] - 1;
+        else
+            a[d] = 0;
+    }
+    return (tn_at(T, q, a) - tn_at(T, q, b)) * T->dxi[d];
+}
+
+static long tn_fidx(const int* n, int d, const int* f)
+{
+    const int m0 = n[0] + (d == 0), m1 = n[1] + (d == 1);
+    return f[0] + (long)m0 * (f[1] + (long)m1 * f[2]);
+}
+
+static double tn_tau(const tn_t* T, const double* eta, int n, int d, const int* f)
+{
+    double g;
+    if (d == n) {
+        g = (4.0 / 3.0) * tn_nd(T, n, d, f);
+        double s = 0.0;
+        for (int m = 0; m < 3; ++m)
+            if (m != n) s += tn_tr(T, m, m, d, f);
+        if (!(ko_plant & 32)) g = g - (2.0 / 3.0) * s;
+    } else {
+        g = tn_nd(T, n, d, f);
+        if (!(ko_plant & 16)) g = g + tn_tr(T, d, n, d, f);
+    }
+    return eta[tn_fidx(T->n, d, f)] * g;
+}
+
+void ko_tensor_apply(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha,
+                     const double* a, double beta, const double* ex, const double* ey, const double* ez,
+                     const double* u, double* out)
+{
+    tn_t T;
+    for (int d = 0; d < 3; ++d) {
+        T.n[d] = n[d];
+        T.dxi[d] = 1.0 / dx[d];
+    }
+    T.lo = lobc;
+    T.hi = hibc;
+    T.u = u;
+    T.nc = (long)n[0] * n[1] * n[2];
+    const double* eta[3] = {ex, ey, ez};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int q = 0; q < 3; ++q)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    double s = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        int f1[3] = {i, j, k};
+                        f1[d] += 1;
+                        const double t = (tn_tau(&T, eta[d], q, d, f1) - tn_tau(&T, eta[d], q, d, c)) * T.dxi[d];
+                        s = (d == 0) ? t : s + t;
+                    }
+                    const long ic = i + (long)n[0] * (j + (long)n[1] * k);
+                    const double av = a ? a[ic] : 1.0;
+                    out[q * T.nc + ic] = alpha * av * tn_at(&T, q, c) - beta * s;
+                }
+}
+
+static double tn_dot(const double* x, const double* y, long n)
+{
+    double s = 0.0;
+    for (long t = 0; t < n; ++t) s += x[t] * y[t];
+    return s;
+}
+
+static double tn_inf(const double* x, long n)
+{
+    double m = 0.0;
+    for (long t = 0; t < n; ++t) {
+        const double v = fabs(x[t]);
+        if (!(v <= m)) m = (v == v) ? v : INFINITY;
+    }
+    return m;
+}
+
+/* L u = rhs by BiCGStab (van der Vorst 1992) on the full operator, u from
+ * zero, until |rhs - L u|_inf <= max(atol, rtol |rhs|_inf) (recomputed from
+ * u, not the recurrence).  Returns 0 converged, 1 maxiter, KO_MG_DIVERGED on
+ * a breakdown or a residual above KO_MG_DIVERGE |rhs|.  The checker's solve:
+ * its iterates are not MLMG's, only the converged answer is compared. */
+int ko_tensor_solve(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha,
+                    const double* a, double beta, const double* ex, const double* ey, const double* ez,
+                    const double* rhs, double* sol, double rtol, double atol, int maxiter, ko_mlmg_info* info)
+{
+    const long N = 3L * n[0] * n[1] * n[2];
+    double* w = (double*)calloc((size_t)(7 * N), sizeof(double));
+    double *r = w, *rh = w + N, *p = w + 2 * N, *v = w + 3 * N, *s = w + 4 * N, *t = w + 5 * N, *tr = w + 6 * N;
+    memset(sol, 0, sizeof(double) * (size_t)N);
+    memcpy(r, rhs, sizeof(double) * (size_t)N);
+    memcpy(rh, rhs, sizeof(double) * (size_t)N);
+    const double bnorm = tn_inf(rhs, N);
+    const double target = fmax(atol, rtol * bnorm);
+    ko_mlmg_info inf;
+    memset(&inf, 0, sizeof(inf));
+    inf.r0 = bnorm;
+    inf.nlevels = 1;
+    double rn = bnorm, rho = 1.0, al = 1.0, om = 1.0;
+    int ret = 1;
+    if (rn <= target) ret = 0;
+    for (int it = 1; ret == 1 && it <= maxiter; ++it) {
+        const double rho1 = tn_dot(rh, r, N);
+        if (rho1 == 0.0 || om == 0.0) {
+            ret = KO_MG_DIVERGED;
+            break;
+        }
+        const double bt = (rho1 / rho) * (al / om);
+        for (long q = 0; q < N; ++q) p[q] = r[q] + bt * (p[q] - om * v[q]);
+        ko_tensor_apply(n, dx, lobc, hibc, alpha, a, beta, ex, ey, ez, p, v);
+        al = rho1 / tn_dot(rh, v, N);
+        for (long q = 0; q < N; ++q) s[q] = r[q] - al * v[q];
+        ko_tensor_apply(n, dx, lobc, hibc, alpha, a, beta, ex, ey, ez, s, t);
+        const double tt = tn_dot(t, t, N);
+        om = (tt > 0.0) ? tn_dot(t, s, N) / tt : 0.0;
+        for (long q = 0; q < N; ++q) {
+            sol[q] += al * p[q] + om * s[q];
+            r[q] = s[q] - om * t[q];
+        }
+        rho = rho1;
+        inf.iters = it;
+        ko_tensor_apply(n, dx, lobc, hibc, alpha, a, beta, ex, ey, ez, sol, tr);
+        for (long q = 0; q < N; ++q) tr[q] = rhs[q] - tr[q];
+        rn = tn_inf(tr, N);
+        if (rn <= target) ret = 0;
+        else if (!(rn <= KO_MG_DIVERGE * bnorm)) ret = KO_MG_DIVERGED;
+    }
+    inf.rfinal = rn;
+    inf.converged = (ret == 0);
+    if (info) *info = inf;
+    free(w);
     return ret;
 }

@@ -151,6 +151,17 @@ int ko_abeclap_solve(const int* n, const double* dx, const int* lobc, const int*
 void ko_abeclap_apply(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha_a,
                       double beta_b, const double* x, double* y);
 
+/* MLTensorOp (the default ICNS viscous operator): out = alpha a u - beta
+ * div(eta (grad u + grad u^T) - (2/3) eta (div u) I), domain-wide, u / out
+ * 3 components (component slowest), a cell (NULL = 1), ex / ey / ez face
+ * viscosities; lobc / hibc [component * 3 + direction] (KO_LO_*) */
+void ko_tensor_apply(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha,
+                     const double* a, double beta, const double* ex, const double* ey, const double* ez,
+                     const double* u, double* out);
+int ko_tensor_solve(const int* n, const double* dx, const int* lobc, const int* hibc, double alpha,
+                    const double* a, double beta, const double* ex, const double* ey, const double* ez,
+                    const double* rhs, double* sol, double rtol, double atol, int maxiter, ko_mlmg_info* info);
+
 /* amrex::average_down / average_down_faces at refinement ratio `ratio` */
 void ko_average_down(const ko_geom* fg, const double* fine, int ng_f, const ko_geom* cg, double* crse,
                      int ng_c, int ncomp, int ratio);

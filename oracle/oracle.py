@@ -97,6 +97,9 @@ def lib():
         L.ko_abeclap_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_double,
                                        C.POINTER(KoOpts), C.POINTER(KoInfo)]
         L.ko_abeclap_apply.argtypes = [_ip, _dp, _ip, _ip, C.c_double, C.c_double, _dp, _dp]
+        L.ko_tensor_apply.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.ko_tensor_solve.argtypes = [_ip, _dp, _ip, _ip, C.c_double, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp,
+                                      C.c_double, C.c_double, C.c_int, C.POINTER(KoInfo)]
         L.ko_average_down.argtypes = [C.POINTER(KoGeom), _dp, C.c_int, C.POINTER(KoGeom), _dp, C.c_int, C.c_int,
                                       C.c_int]
         L.ko_average_down_faces.argtypes = [C.POINTER(KoGeom), C.POINTER(_dp), C.c_int, C.POINTER(KoGeom),
@@ -382,6 +385,69 @@ def abeclap_apply(n, dx, lobc, hibc, alpha_a, beta_b, x):
     y = np.zeros_like(x)
     lib().ko_abeclap_apply(pn, _p(dd), plo, phi_, alpha_a, beta_b, _p(x), _p(y))
     return y
+
+
+def _tensor_args(n, dx, lobc, hibc, a, eta3):
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    dd = np.ascontiguousarray(dx, dtype=np.float64)
+    lo = np.ascontiguousarray(lobc, dtype=np.int32).ravel()
+    hi = np.ascontiguousarray(hibc, dtype=np.int32).ravel()
+    assert lo.size == 9 and hi.size == 9
+    a = None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+    e = [np.ascontiguousarray(b, dtype=np.float64).ravel() for b in eta3]
+    return nn, dd, lo, hi, a, e
+
+
+def tensor_apply(n, dx, lobc, hibc, alpha, a, beta, eta3, u):
+    """MLTensorOp apply: alpha a u - beta div(eta (grad u + grad u^T) -
+    (2/3) eta (div u) I).  u: (3, ncell) domain-wide (x fastest); a: cell
+    array or None (1); eta3: face viscosities (x faces (nz, ny, nx+1), ...);
+    lobc / hibc: (3 components, 3 directions) of KO_LO_* values."""
+    nn, dd, lo, hi, a, e = _tensor_args(n, dx, lobc, hibc, a, eta3)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(3, -1)
+    out = np.empty_like(u)
+    lib().ko_tensor_apply(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip), alpha,
+                          _p(a), beta, _p(e[0]), _p(e[1]), _p(e[2]), _p(u), _p(out))
+    return out
+
+
+def tensor_solve(n, dx, lobc, hibc, alpha, a, beta, eta3, rhs, rtol, atol, maxiter=20000):
+    """L u = rhs for the MLTensorOp operator (BiCGStab from zero; returns
+    (u (3, ncell), info))"""
+    nn, dd, lo, hi, a, e = _tensor_args(n, dx, lobc, hibc, a, eta3)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(3, -1)
+    sol = np.zeros_like(rhs)
+    info = KoInfo()
+    st = lib().ko_tensor_solve(nn.ctypes.data_as(_ip), _p(dd), lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip),
+                               alpha, _p(a), beta, _p(e[0]), _p(e[1]), _p(e[2]), _p(rhs), _p(sol), rtol, atol,
+                               int(maxiter), C.byref(info))
+    return sol, dict(info.as_dict(), status=st)
+
+
+def velocity_eta_to_faces(cc, periodic):
+    """diffusion::average_velocity_eta_to_faces (incflo_diffusion.cpp:108-128)
+    on a domain-wide cell array cc (nz, ny, nx): each face the mean
+    0.5 (below + above) of its cells (periodic wrap on a periodic side), a
+    face on a non-periodic domain side the value of its one cell
+    (fixup_eta_on_domain_faces, :130-200)."""
+    cc = np.asarray(cc, dtype=np.float64)
+    out = []
+    for d in range(3):
+        ax = 2 - d
+        lo = np.take(cc, [-1], axis=ax) if periodic[d] else np.take(cc, [0], axis=ax)
+        hi = np.take(cc, [0], axis=ax) if periodic[d] else np.take(cc, [-1], axis=ax)
+        below = np.concatenate([lo, cc], axis=ax)
+        above = np.concatenate([cc, hi], axis=ax)
+        f = 0.5 * (below + above)
+        if not periodic[d]:
+            sl0 = [slice(None)] * 3
+            sl0[ax] = slice(0, 1)
+            sl1 = [slice(None)] * 3
+            sl1[ax] = slice(-1, None)
+            f[tuple(sl0)] = np.take(cc, [0], axis=ax)
+            f[tuple(sl1)] = np.take(cc, [-1], axis=ax)
+        out.append(np.ascontiguousarray(f))
+    return out
 
 
 def average_down(Gf, fine, ng_f, Gc, crse, ng_c, ncomp, ratio):
