@@ -24,9 +24,14 @@
 //     added to the forcing before the division by density, kappa = 1/2 in
 //     the predictor, then (rho - dt/2 mu lap) u = rho u*;
 //   Implicit: kappa = 0, then (rho - dt mu lap) u = rho u*;
-// one MLABecLaplacian solve per velocity component with that component's
-// BCs (get_diffuse_tensor_bc, incflo_diffusion.cpp:11-69: periodic; ext_dir
-// -> Dirichlet; foextrap / hoextrap -> Neumann).
+// by default through the tensor operator, as ICNS's DiffusionOp does
+// (use_tensor_operator = true, icns_diffusion.H:492-506): MLTensorOp with
+// a = rho, the viscosity averaged to the faces (average_velocity_eta_to_faces)
+// and divtau = div(mu (grad u + grad u^T) - (2/3) mu (div u) I); or, with the
+// scalar operator, one MLABecLaplacian solve per velocity component
+// (divtau = mu lap(u)).  Each component takes its BCs from
+// get_diffuse_tensor_bc (incflo_diffusion.cpp:11-69: periodic; ext_dir ->
+// Dirichlet; foextrap / hoextrap -> Neumann).
 // With enableNodalProjection the step ends with incflo::ApplyProjection
 // (incflo_advance.cpp:412-414; incflo_apply_nodal_projection.cpp:149-465,
 // constant density, not incremental, no small-dt variant):
@@ -188,13 +193,15 @@ public:
 
     // diffusion: type 1 Crank-Nicolson, 2 Implicit (incflo.diffusion_type);
     // Dirichlet velocity BCs must hold zero (walls at rest)
-    void enableDiffusion(double mu, int type, double rtol, double atol, const ksgf_mlmg_options& opts)
+    void enableDiffusion(double mu, int type, double rtol, double atol, const ksgf_mlmg_options& opts,
+                         bool tensor = true)
     {
         if (type != 1 && type != 2) throw std::runtime_error("enableDiffusion: type 1 (Crank-Nicolson) or 2 (Implicit)");
         if (!(mu > 0.0)) throw std::runtime_error("enableDiffusion: mu must be positive");
         Hydro::LPInfo info;
         info.do_semicoarsening = opts.do_semicoarsening != 0;
         info.max_coarsening_level = opts.max_coarsening_level;
+        std::array<std::array<Hydro::LinOpBCType, 3>, 3> tlo{}, thi{};
         for (int c = 0; c < 3; ++c) {
             std::array<Hydro::LinOpBCType, 3> lo{}, hi{};
             for (int d = 0; d < 3; ++d) {
@@ -210,10 +217,29 @@ public:
                     (side ? hi : lo)[d] = lt;
                 }
             }
+            tlo[c] = lo;
+            thi[c] = hi;
+            lap_[c].reset();
+            if (tensor) continue;
             lap_[c] = std::make_unique<Hydro::MLABecLaplacian>(ba_, info, opts);
             lap_[c]->setDomainBC(lo, hi);
             lap_[c]->setACoeffs(rho0_);
             lap_[c]->setBCoeffs(mu);
+        }
+        tensor_.reset();
+        eta_.reset();
+        if (tensor) {
+            tensor_ = std::make_unique<Hydro::MLTensorOp>(ba_, info, opts);
+            tensor_->setDomainBC(tlo, thi);
+            // the (constant) viscosity on the faces
+            MultiFab mu_cc(ba_, 1, 1);
+            std::vector<double> h((size_t)ba_.geom().domain.numPts(), mu);
+            mu_cc.copyFromDomain(h.data());
+            mu_cc.FillBoundary(st_);
+            mu_cc.FillPhysBC(rho_bc_, std::vector<double>(6, 0.0), st_);
+            eta_ = std::make_unique<std::array<MultiFab, 3>>(ksgf::amr::average_velocity_eta_to_faces(mu_cc, st_));
+            tensor_->setShearViscosity({&(*eta_)[0], &(*eta_)[1], &(*eta_)[2]}, st_);
+            (void)hipStreamSynchronize(st_);
         }
         divtau_ = std::make_unique<MultiFab>(ba_, 3, 0);
         drhs_ = std::make_unique<MultiFab>(ba_, 3, 0);
@@ -330,11 +356,15 @@ public:
         s.dt = dt;
         // explicit viscous term of the old state (Crank-Nicolson)
         const bool cn = (diff_type_ == 1);
-        if (cn)
+        if (cn && tensor_) {
+            tensor_->setScalars(0.0, -1.0);
+            tensor_->apply(*divtau_, *vel_, st_);
+        } else if (cn) {
             for (int c = 0; c < 3; ++c) {
                 lap_[c]->setScalars(0.0, -1.0);
                 lap_[c]->apply(*divtau_, *vel_, c, st_);
             }
+        }
         // forcing in velocity form with one ghost
         if (cn)
             check(ksgf_icns_src_term_divtau(h, src_.data(), 1, gp_.data(), divtau_->data(), rho_.data(), 3, st),
@@ -383,9 +413,16 @@ public:
             const double dt_diff = cn ? 0.5 * dt : dt;
             check(ksgf_scale_by_rho(h, drhs_->data(), 0, vel_new_->data(), 3, rho_.data(), 3, 3, 0, 0, st),
                   "diffusion rhs = rho u*");
-            for (int c = 0; c < 3; ++c) {
-                lap_[c]->setScalars(1.0, dt_diff);
-                s.diffusion[c] = lap_[c]->solve(*vel_new_, *drhs_, c, diff_rtol_, diff_atol_, st_);
+            if (tensor_) {
+                // one coupled solve of the three components (its statistics in diffusion[0])
+                tensor_->setScalars(1.0, dt_diff);
+                tensor_->setACoeffs(rho_, st_);
+                s.diffusion[0] = tensor_->solve(*vel_new_, *drhs_, diff_rtol_, diff_atol_, st_);
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    lap_[c]->setScalars(1.0, dt_diff);
+                    s.diffusion[c] = lap_[c]->solve(*vel_new_, *drhs_, c, diff_rtol_, diff_atol_, st_);
+                }
             }
         }
         vel_new_->FillBoundary(st_);
@@ -458,6 +495,8 @@ private:
     std::unique_ptr<Scalar> sc_;
     std::unique_ptr<Hydro::NodalProjector> nodal_;
     std::unique_ptr<Hydro::MLABecLaplacian> lap_[3];
+    std::unique_ptr<Hydro::MLTensorOp> tensor_;
+    std::unique_ptr<std::array<MultiFab, 3>> eta_;
     std::unique_ptr<MultiFab> divtau_, drhs_;
     int diff_type_ = 0;
     double diff_rtol_ = 0.0, diff_atol_ = 0.0;

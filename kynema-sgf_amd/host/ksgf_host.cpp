@@ -226,7 +226,7 @@ int ksgf_host_stepper_get_nodal_phi(ksgf_host_stepper* s, double* phi)
 }
 
 int ksgf_host_stepper_enable_diffusion(ksgf_host_stepper* s, double mu, int type, double rtol, double atol,
-                                       const ksgf_mlmg_options* opts)
+                                       const ksgf_mlmg_options* opts, int tensor)
 {
     try {
         if (!s) throw std::runtime_error("null stepper");
@@ -235,7 +235,7 @@ int ksgf_host_stepper_enable_diffusion(ksgf_host_stepper* s, double mu, int type
             o = *opts;
         else
             ksgf_mlmg_default_options(&o);
-        reinterpret_cast<GodunovPredictor*>(s)->enableDiffusion(mu, type, rtol, atol, o);
+        reinterpret_cast<GodunovPredictor*>(s)->enableDiffusion(mu, type, rtol, atol, o, tensor != 0);
         return KSGF_OK;
     } catch (const std::exception& e) {
         g_err = std::string("ksgf_host_stepper_enable_diffusion: ") + e.what();

@@ -46,10 +46,12 @@ int ksgf_host_stepper_nodal_info(ksgf_host_stepper* s, ksgf_mlmg_info* info);
 int ksgf_host_stepper_get_gradp(ksgf_host_stepper* s, double* gp);
 int ksgf_host_stepper_get_nodal_phi(ksgf_host_stepper* s, double* phi);
 /* viscous term with constant viscosity mu: type 1 Crank-Nicolson, 2 Implicit
- * (incflo.diffusion_type); diffusion_info: the last step's three
- * component solves */
+ * (incflo.diffusion_type); tensor 1: the tensor operator (MLTensorOp, the
+ * reference's default, use_tensor_operator = true), 0: the scalar operator
+ * (one MLABecLaplacian per component); diffusion_info: the last step's three
+ * component solves (the tensor operator's one coupled solve in [0]) */
 int ksgf_host_stepper_enable_diffusion(ksgf_host_stepper* s, double mu, int type, double rtol, double atol,
-                                       const ksgf_mlmg_options* opts);
+                                       const ksgf_mlmg_options* opts, int tensor);
 int ksgf_host_stepper_diffusion_info(ksgf_host_stepper* s, ksgf_mlmg_info* info3);
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s);
 

@@ -123,6 +123,11 @@ public:
     }
     MultiFab(const MultiFab&) = delete;
     MultiFab& operator=(const MultiFab&) = delete;
+    MultiFab(MultiFab&& o) noexcept : ba_(o.ba_), ncomp_(o.ncomp_), ngrow_(o.ngrow_), it_(o.it_), n_(o.n_), p_(o.p_)
+    {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
 
     double* data() { return p_; }
     const double* data() const { return p_; }
@@ -329,6 +334,20 @@ inline void face_beta_from_density(const std::array<MultiFab*, 3>& beta, const M
         check(ksgf_invert(rho.boxArray().handle(), beta[d]->data(), KSGF_XFACE + d, 1, beta[d]->nGrow(), 0, factor,
                           (ksgf_stream_t)st),
               "invert");
+}
+
+// diffusion::average_velocity_eta_to_faces (incflo_diffusion.cpp:108-128):
+// the cell viscosity (1 comp, a filled ghost layer) averaged to the faces,
+// the adjacent cell on a non-periodic domain face (fixup_eta_on_domain_faces)
+inline std::array<MultiFab, 3> average_velocity_eta_to_faces(const MultiFab& cc_eta, hipStream_t st = nullptr)
+{
+    const BoxArray& ba = cc_eta.boxArray();
+    std::array<MultiFab, 3> r{MultiFab(ba, 1, 0, IndexType::XFace), MultiFab(ba, 1, 0, IndexType::YFace),
+                              MultiFab(ba, 1, 0, IndexType::ZFace)};
+    check(ksgf_average_velocity_eta_to_faces(ba.handle(), cc_eta.data(), cc_eta.nGrow(), r[0].data(), r[1].data(),
+                                             r[2].data(), 0, (ksgf_stream_t)st),
+          "average_velocity_eta_to_faces");
+    return r;
 }
 
 }  // namespace ksgf::amr
@@ -572,6 +591,82 @@ private:
     int lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
     bool rebuild_ = true;
     ksgf_abeclap* ab_ = nullptr;
+};
+
+// amrex::MLTensorOp + MLMG as ICNSDiffTensorOp drives it (icns_diffusion.H:
+// 18-50; DiffSolverIface::setup_operator and set_bcoeffs, DiffusionOps.cpp:
+// 57-110, DiffusionOps.H:54-71): setDomainBC with one LinOpBCType triple per
+// velocity component (get_diffuse_tensor_bc), setScalars, setACoeffs(rho),
+// setShearViscosity(eta on faces), then MLMG::apply (compute_diff_term) or
+// MLMG::solve (linsys_solve) on the three velocity components.
+class MLTensorOp {
+public:
+    MLTensorOp(const ksgf::amr::BoxArray& ba, const LPInfo& info, const ksgf_mlmg_options& opts)
+        : ba_(&ba), opts_(opts)
+    {
+        opts_.do_semicoarsening = info.do_semicoarsening ? 1 : 0;
+        opts_.max_coarsening_level = info.max_coarsening_level;
+    }
+    ~MLTensorOp()
+    {
+        if (op_) ksgf_tensorop_destroy(op_);
+    }
+    MLTensorOp(const MLTensorOp&) = delete;
+    MLTensorOp& operator=(const MLTensorOp&) = delete;
+
+    void setDomainBC(const std::array<std::array<LinOpBCType, 3>, 3>& lo,
+                     const std::array<std::array<LinOpBCType, 3>, 3>& hi)
+    {
+        int l[9], h[9];
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d) {
+                l[c * 3 + d] = (int)lo[c][d];
+                h[c * 3 + d] = (int)hi[c][d];
+            }
+        if (op_) ksgf_tensorop_destroy(op_);
+        op_ = nullptr;
+        ksgf::amr::check(ksgf_tensorop_create(ba_->handle(), l, h, &opts_, &op_), "MLTensorOp::setDomainBC");
+    }
+    void setScalars(double alpha, double beta)
+    {
+        ksgf::amr::check(ksgf_tensorop_set_scalars(need(), alpha, beta), "MLTensorOp::setScalars");
+    }
+    void setACoeffs(const ksgf::amr::MultiFab& a, hipStream_t st = nullptr)
+    {
+        ksgf::amr::check(ksgf_tensorop_set_acoeffs(need(), a.data(), a.nGrow(), (ksgf_stream_t)st),
+                         "MLTensorOp::setACoeffs");
+    }
+    void setShearViscosity(const std::array<const ksgf::amr::MultiFab*, 3>& eta, hipStream_t st = nullptr)
+    {
+        ksgf::amr::check(ksgf_tensorop_set_shear_viscosity(need(), eta[0]->data(), eta[1]->data(), eta[2]->data(),
+                                                           eta[0]->nGrow(), (ksgf_stream_t)st),
+                         "MLTensorOp::setShearViscosity");
+    }
+    ksgf_mlmg_info solve(ksgf::amr::MultiFab& sol, const ksgf::amr::MultiFab& rhs, double rtol, double atol,
+                         hipStream_t st = nullptr)
+    {
+        ksgf_mlmg_info info{};
+        ksgf::amr::check(ksgf_tensorop_solve(need(), sol.data(), sol.nGrow(), rhs.data(), rhs.nGrow(), rtol, atol,
+                                             &info, (ksgf_stream_t)st),
+                         "MLTensorOp::solve");
+        return info;
+    }
+    void apply(ksgf::amr::MultiFab& out, const ksgf::amr::MultiFab& in, hipStream_t st = nullptr)
+    {
+        ksgf::amr::check(ksgf_tensorop_apply(need(), out.data(), out.nGrow(), in.data(), in.nGrow(),
+                                             (ksgf_stream_t)st),
+                         "MLTensorOp::apply");
+    }
+
+private:
+    ksgf_tensorop* need() const
+    {
+        if (!op_) throw std::runtime_error("MLTensorOp: setDomainBC first");
+        return op_;
+    }
+    const ksgf::amr::BoxArray* ba_;
+    ksgf_mlmg_options opts_;
+    ksgf_tensorop* op_ = nullptr;
 };
 
 // Hydro::NodalProjector with a constant sigma, as incflo::ApplyProjection

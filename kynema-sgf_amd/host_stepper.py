@@ -44,7 +44,7 @@ def host_lib():
         L.ksgf_host_stepper_get_gradp.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_nodal_phi.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_enable_diffusion.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double,
-                                                         C.POINTER(KsgfMLMGOptions)]
+                                                         C.POINTER(KsgfMLMGOptions), C.c_int]
         L.ksgf_host_stepper_diffusion_info.argtypes = [C.c_void_p, C.POINTER(KsgfMLMGInfo)]
         L.ksgf_host_stepper_destroy.argtypes = [C.c_void_p]
         L.ksgf_host_mac_project_density.argtypes = [C.POINTER(KsgfGeometry), C.POINTER(C.c_int), C.c_int,
@@ -148,10 +148,13 @@ class HostStepper:
              "enable_nodal_projection")
         self.nodal = True
 
-    def enable_diffusion(self, mu, diff_type, rtol, atol, options=None):
-        """viscous term, constant mu: diff_type 1 Crank-Nicolson, 2 Implicit"""
+    def enable_diffusion(self, mu, diff_type, rtol, atol, options=None, tensor=True):
+        """viscous term, constant mu: diff_type 1 Crank-Nicolson, 2 Implicit;
+        tensor: MLTensorOp (the reference's default) or one MLABecLaplacian per
+        component"""
         self._dopts = options or default_options(**self.case.mlmg)
-        _chk(host_lib().ksgf_host_stepper_enable_diffusion(self.h, mu, diff_type, rtol, atol, C.byref(self._dopts)),
+        _chk(host_lib().ksgf_host_stepper_enable_diffusion(self.h, mu, diff_type, rtol, atol, C.byref(self._dopts),
+                                                           int(bool(tensor))),
              "enable_diffusion")
 
     def diffusion_info(self):

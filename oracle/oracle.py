@@ -649,13 +649,26 @@ def diffusion_bcs(case, c):
     return tuple(lo), tuple(hi)
 
 
-def _diffusion(case, ba, opts, vel, mu, dt_diff, rtol, atol, comp_apply=None):
+def _diffusion(case, ba, opts, vel, mu, dt_diff, rtol, atol, comp_apply=None, tensor=False):
     """component-wise (rho_0 - dt_diff mu lap) u = rho_0 u (solve) on the
     valid cells of vel (3 comp, ng 3), or with comp_apply the apply
-    mu lap(u) (scalars (0, -1)) returned as a (3, ncell) array"""
+    mu lap(u) (scalars (0, -1)) returned as a (3, ncell) array.  With tensor
+    the operator is MLTensorOp's (ICNSDiffTensorOp): a = rho_0, eta = mu on
+    every face (average_velocity_eta_to_faces of a constant), one coupled
+    solve of the three components (BiCGStab: its iterates are not MLMG's)."""
     n = tuple(int(v) for v in case.geom.n_cell)
     dx = case.geom.dx
     u = ba.gather(vel, CELL, 3, 3)
+    if tensor:
+        lo = [diffusion_bcs(case, c)[0] for c in range(3)]
+        hi = [diffusion_bcs(case, c)[1] for c in range(3)]
+        per = tuple(int(case.geom.periodic[d]) for d in range(3))
+        eta3 = velocity_eta_to_faces(np.full((n[2], n[1], n[0]), float(mu)), per)
+        a = np.full(int(np.prod(n)), float(case.rho0))
+        if comp_apply:
+            return tensor_apply(n, dx, lo, hi, 0.0, a, -1.0, eta3, u.reshape(3, -1)), []
+        x, info = tensor_solve(n, dx, lo, hi, 1.0, a, dt_diff, eta3, (u * case.rho0).reshape(3, -1), rtol, atol)
+        return x, [info]
     out = []
     infos = []
     for c in range(3):
@@ -791,9 +804,10 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
     """One step; returns (vel_new (ng 3, ghosts filled), diagnostics dict).
     With scal (cell, 1 comp, ng 3) the passive scalar is advected too; the
     new scalar is diag["scal"].  diffusion = dict(mu, type (1 Crank-Nicolson,
-    2 Implicit), rtol, atol): the viscous term with constant mu and density
-    rho_0 as ApplyPredictor carries it for Godunov (incflo_advance.cpp:246-291,
-    380-389).  clock: a dict carried from step to step (dt_prev, src_nph)
+    2 Implicit), rtol, atol, tensor (default True: MLTensorOp, the reference's
+    use_tensor_operator default; False: one MLABecLaplacian per component)):
+    the viscous term with constant mu and density rho_0 as ApplyPredictor
+    carries it for Godunov (incflo_advance.cpp:246-291, 380-389).  clock: a dict carried from step to step (dt_prev, src_nph)
     for compute_dt's growth cap and forcing CFL; updated in place."""
     ba = case.ba
     opts = opts or case_opts(case)
@@ -807,7 +821,8 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
     cn = diffusion is not None and diffusion["type"] == 1
     divtau = None
     if cn:
-        tau, _ = _diffusion(case, ba, opts, vel, diffusion["mu"], 0.0, 0.0, 0.0, comp_apply=True)
+        tau, _ = _diffusion(case, ba, opts, vel, diffusion["mu"], 0.0, 0.0, 0.0, comp_apply=True,
+                            tensor=diffusion.get("tensor", True))
         n = case.geom.n_cell
         divtau = ba.scatter(tau.reshape(3, n[2], n[1], n[0]), CELL, 0)
     # sources: src = -gp/rho (velocity form), ghost 1 (foextrap-like fill)
@@ -871,7 +886,8 @@ def advect_project_step(case, G, vel, rho, gp, dt=None, opts=None, limiter=1, sc
     dinfo = None
     if diffusion is not None:
         dt_diff = 0.5 * dt if cn else dt
-        x, dinfo = _diffusion(case, ba, opts, vel_new, diffusion["mu"], dt_diff, diffusion["rtol"], diffusion["atol"])
+        x, dinfo = _diffusion(case, ba, opts, vel_new, diffusion["mu"], dt_diff, diffusion["rtol"], diffusion["atol"],
+                              tensor=diffusion.get("tensor", True))
         n = case.geom.n_cell
         ba.scatter(x.reshape(3, n[2], n[1], n[0]), CELL, 3, out=vel_new)
     fill_ghosts(G, vel_new, CELL, 3, 3, case.bcrec, case.bcval)

@@ -71,7 +71,8 @@ def test_abeclap_tracks_oracle(ksgf, oracle, gpu, name):
 @pytest.mark.parametrize("diff_type", [1, 2])
 def test_host_step_with_diffusion(ksgf, oracle, workloads, gpu, diff_type):
     """GodunovPredictor with the viscous term (Crank-Nicolson / Implicit,
-    constant mu) against the oracle's step with the same diffusion"""
+    constant mu) through the scalar operator (one MLABecLaplacian per
+    component) against the oracle's step with the same diffusion"""
     _say(f"host step + diffusion type={diff_type}")
     hs = importlib.import_module("kynema-sgf_amd.host_stepper")
     case = workloads.abl((64, 48, 32), extent=(1280.0, 960.0, 128.0), sizes=(16, 32), nsteps=2)
@@ -79,8 +80,8 @@ def test_host_step_with_diffusion(ksgf, oracle, workloads, gpu, diff_type):
     S = hs.HostStepper(case)
     S.load()
     mu, rtol, atol = 30.0, 1e-11, 1e-14
-    S.enable_diffusion(mu, diff_type, rtol, atol)
-    diff = {"mu": mu, "type": diff_type, "rtol": rtol, "atol": atol}
+    S.enable_diffusion(mu, diff_type, rtol, atol, tensor=False)
+    diff = {"mu": mu, "type": diff_type, "rtol": rtol, "atol": atol, "tensor": False}
     clock = {}
     for n in range(2):
         vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, diffusion=diff, clock=clock)
@@ -90,4 +91,32 @@ def test_host_step_with_diffusion(ksgf, oracle, workloads, gpu, diff_type):
         assert [d["iters"] for d in di] == [d["iters"] for d in diag["diffusion"]]
         assert S.last_info["iters"] == diag["mlmg"]["iters"]
         assert rel_l2(S.velocity(), case.ba.gather(vel, CELL, 3, 3)) <= 1e-11, f"step {n}"
+    S.close()
+
+
+@pytest.mark.parametrize("diff_type", [1, 2])
+def test_host_step_with_tensor_diffusion(ksgf, oracle, workloads, gpu, diff_type):
+    """GodunovPredictor with the reference's default viscous operator
+    (MLTensorOp: Crank-Nicolson's divtau and the implicit solve) against the
+    oracle's step with the tensor operator.  The oracle solves with BiCGStab,
+    so the velocities agree to the solve tolerance, not the iterates."""
+    _say(f"host step + tensor diffusion type={diff_type}")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    case = workloads.abl((64, 48, 32), extent=(1280.0, 960.0, 128.0), sizes=(16, 32), nsteps=2)
+    G, vel, rho, gp = oracle.make_state(case)
+    S = hs.HostStepper(case)
+    S.load()
+    mu, rtol, atol = 30.0, 1e-12, 1e-14
+    S.enable_diffusion(mu, diff_type, rtol, atol, tensor=True)
+    diff = {"mu": mu, "type": diff_type, "rtol": rtol, "atol": atol, "tensor": True}
+    clock = {}
+    for n in range(2):
+        vel, diag = oracle.advect_project_step(case, G, vel, rho, gp, diffusion=diff, clock=clock)
+        S.step()
+        di = S.diffusion_info()
+        _say(f"  step {n}: tensor solve iters gpu={di[0]['iters']} (r {di[0]['rfinal']:.2e} of {di[0]['r0']:.2e}); "
+             f"oracle BiCGStab {diag['diffusion'][0]['iters']}")
+        assert di[0]["converged"] == 1
+        assert abs(S.last_info["iters"] - diag["mlmg"]["iters"]) <= 1
+        assert rel_l2(S.velocity(), case.ba.gather(vel, CELL, 3, 3)) <= 1e-10, f"step {n}"
     S.close()
