@@ -217,6 +217,27 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
 int ksgf_macproj_get_phi(ksgf_macproj* mp, double* phi, ksgf_stream_t stream);
 int ksgf_macproj_destroy(ksgf_macproj* mp);
 
+/* MacProjector::project split for a step captured into a hipGraph.  No
+ * counterpart in the reference, whose MLMG tests its residual norm on the
+ * host after every iteration; for callers that replay a fixed-dt step as a
+ * graph and read the solve's state once per step.
+ *   begin: div u_mac, the first residual, then `ncycles` iterations (rounded
+ *     up to even), each followed by the FP64 residual and a device-side test
+ *     (the host loop's: converged, diverging, maxiter) after which the later
+ *     iterations' kernels return at once.  No host synchronisation and no
+ *     memset: a stream capture records it.
+ *   finish: reads the solve's state (one host synchronisation); a solve still
+ *     iterating after ncycles continues with host-side tests.  Not capturable.
+ *   correct: u_mac -= beta grad phi (capturable).
+ * begin + finish + correct give ksgf_macproj_project's faces and phi bit for
+ * bit.  One beta (no face coefficients).  Calls on one projector must not
+ * overlap. */
+int ksgf_macproj_project_begin(ksgf_macproj* mp, const double* u_mac, const double* v_mac,
+                               const double* w_mac, double rtol, double atol, int ncycles,
+                               ksgf_stream_t stream);
+int ksgf_macproj_project_finish(ksgf_macproj* mp, ksgf_mlmg_info* info, ksgf_stream_t stream);
+int ksgf_macproj_project_correct(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac,
+                                 ksgf_stream_t stream);
 /* Standalone Poisson solve div(beta grad phi) = rhs on the projector's
  * domain (the MLMG solve inside project); rhs/phi domain-wide (n0*n1*n2). */
 int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol,

@@ -71,7 +71,8 @@ EXPORTS = [
     "ksgf_macproj_levels_get_phi", "ksgf_macproj_levels_destroy", "ksgf_fillpatch_faces_two_levels",
     "ksgf_tensorop_create", "ksgf_tensorop_set_scalars", "ksgf_tensorop_set_acoeffs",
     "ksgf_tensorop_set_shear_viscosity", "ksgf_tensorop_apply", "ksgf_tensorop_solve", "ksgf_tensorop_destroy",
-    "ksgf_average_velocity_eta_to_faces",
+    "ksgf_average_velocity_eta_to_faces", "ksgf_macproj_project_begin", "ksgf_macproj_project_finish",
+    "ksgf_macproj_project_correct",
 ]
 
 _lib = None
@@ -132,6 +133,9 @@ def load(path=None):
     L.ksgf_macproj_set_beta.argtypes = [C.c_void_p, C.c_double]
     L.ksgf_macproj_project.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo),
                                        C.c_void_p]
+    L.ksgf_macproj_project_begin.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p]
+    L.ksgf_macproj_project_finish.argtypes = [C.c_void_p, C.POINTER(KsgfMLMGInfo), C.c_void_p]
+    L.ksgf_macproj_project_correct.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_void_p]
     L.ksgf_macproj_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.POINTER(KsgfMLMGInfo),
                                      C.c_void_p]
     L.ksgf_macproj_get_phi.argtypes = [C.c_void_p, _dp, C.c_void_p]
@@ -542,6 +546,23 @@ class MacProjector:
                                           _stream()), "macproj_project", allow=(0, 1))
         self.info = info.as_dict()
         return self.info
+
+    def project_begin(self, mac, rtol, atol, ncycles):
+        """div u_mac and ncycles device-tested iterations (no host
+        synchronisation: capturable)"""
+        check(load().ksgf_macproj_project_begin(self.h, _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]), rtol, atol,
+                                                int(ncycles), _stream()), "macproj_project_begin")
+
+    def project_finish(self):
+        info = KsgfMLMGInfo()
+        check(load().ksgf_macproj_project_finish(self.h, C.byref(info), _stream()), "macproj_project_finish",
+              allow=(0, 1))
+        self.info = info.as_dict()
+        return self.info
+
+    def project_correct(self, mac):
+        check(load().ksgf_macproj_project_correct(self.h, _ptr(mac[0]), _ptr(mac[1]), _ptr(mac[2]), _stream()),
+              "macproj_project_correct")
 
     def solve(self, rhs, phi, rtol, atol):
         info = KsgfMLMGInfo()

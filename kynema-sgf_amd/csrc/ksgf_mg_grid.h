@@ -35,6 +35,9 @@ struct LevelGeom {
     int ghost_mode;    // natural layout only: explicit ghost fills before every colour
     int packed;        // 1: red-black packed layout
     int prow, pplane, pcol;  // packed strides: row (n0/2), plane, colour block
+    // device-controlled solves: the kernels of a V-cycle return at once when
+    // *act reads 0 (the solve has converged); nullptr otherwise
+    const int* act = nullptr;
 };
 
 constexpr int kXOff = 16;

@@ -133,6 +133,7 @@ __host__ __device__ __forceinline__ long long ghost_count(const LevelGeom& L)
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_ghosts(T* __restrict__ a, LevelGeom L)
 {
+    if (L.act && *L.act == 0) return;
     const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (t < (int)ghost_count(L)) ghost_point<T>(a, L, t);
 }
@@ -148,6 +149,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ x, const T* __restrict__ b, LevelGeom L, int color,
                                                    const T* __restrict__ xc, LevelGeom C, int r0, int r1, int r2)
 {
+    if (L.act && *L.act == 0) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int h0 = (L.n0 + 1) / 2;
     const long long tot = (long long)h0 * L.n1 * L.n2;
@@ -168,6 +170,7 @@ __global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ x, const T* _
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_smooth_first(T* __restrict__ x, const T* __restrict__ b, LevelGeom L)
 {
+    if (L.act && *L.act == 0) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)L.n0 * L.n1 * L.n2;
     if (t >= tot) return;
@@ -216,6 +219,7 @@ __global__ void __launch_bounds__(256) k_mg_prolong_red(T* __restrict__ x, const
                                                         const T* __restrict__ b, LevelGeom F, LevelGeom C, int r0,
                                                         int r1, int r2)
 {
+    if (F.act && *F.act == 0) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)F.n0 * F.n1 * F.n2;
     if (t >= tot) return;
@@ -235,6 +239,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const T* __restrict__ c, LevelGeom F,
                                                         LevelGeom C, int r0, int r1, int r2)
 {
+    if (F.act && *F.act == 0) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)F.n0 * F.n1 * F.n2;
     if (t >= tot) return;
@@ -249,6 +254,7 @@ __global__ void __launch_bounds__(256) k_mg_prolong_add(T* __restrict__ f, const
 template <typename T, int MODE, bool CP>
 __global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTiles t)
 {
+    if (a.L.act && *a.L.act == 0) return;
     int h, j, k0, k1;
     row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, h, j, k0, k1, a.L.n2);
     if (h >= a.L.prow || j >= a.L.n1) return;
@@ -259,6 +265,7 @@ __global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTile
 template <typename T, bool CP>
 __global__ void __launch_bounds__(256) k_pk_prolong_color(PkArgs<T> a, int color, RowTiles t)
 {
+    if (a.L.act && *a.L.act == 0) return;
     int h, j, k0, k1;
     row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, h, j, k0, k1, a.L.n2);
     if (h >= a.L.prow || j >= a.L.n1) return;
@@ -271,6 +278,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_pk_prolong_add(T* __restrict__ x, const T* __restrict__ xc, LevelGeom F,
                                                         LevelGeom C, int s0, int s1, int s2, RowTiles t)
 {
+    if (F.act && *F.act == 0) return;
     int i, j, k0, k1;
     row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, i, j, k0, k1, F.n2);
     if (i >= F.n0 || j >= F.n1) return;
@@ -285,6 +293,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__
                                                            T* __restrict__ cres, LevelGeom F, LevelGeom C, RowTiles t,
                                                            int r0, int r1, int r2, T w)
 {
+    if (F.act && *F.act == 0) return;
     int I, J, K0, K1;
     row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, I, J, K0, K1, C.n2);
     if (I >= C.n0 || J >= C.n1) return;
@@ -434,6 +443,7 @@ __global__ void __launch_bounds__(256) k_update_residual(const double* __restric
                                                          R* __restrict__ r, LevelGeom P, LevelGeom L, RowTiles tl,
                                                          double* __restrict__ part)
 {
+    if (P.act && *P.act == 0) return;
     const double m = update_residual_tile<C, R, CP>(phi, cor, phi_out, rhs, r, P, L, tl,
                                                     xcd_remap((int)blockIdx.x, (int)gridDim.x));
     block_max_part(m, part);
@@ -444,8 +454,10 @@ __global__ void __launch_bounds__(256) k_update_residual(const double* __restric
 // partials are finite or +inf, so the maximum does not depend on the order)
 constexpr int kMaxPartsThreads = 1024;
 __global__ void __launch_bounds__(kMaxPartsThreads) k_max_parts(const double* __restrict__ part, int n,
-                                                                unsigned long long* __restrict__ norm)
+                                                                unsigned long long* __restrict__ norm,
+                                                                const int* __restrict__ act)
 {
+    if (act && *act == 0) return;
     constexpr int B = kMaxPartsThreads;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
     int t = (int)threadIdx.x;
@@ -812,8 +824,10 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
 template <typename T>
 __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtrs<T>* __restrict__ lp, int lc, int nlev,
                                                                   BottomWork w, int singular, double rtol, double atol,
-                                                                  int maxiter, int nub, int nuf, int nu1, int nu2)
+                                                                  int maxiter, int nub, int nuf, int nu1, int nu2,
+                                                                  const int* __restrict__ act)
 {
+    if (act && *act == 0) return;
     __shared__ double sh[kCoarseThreads];
     __shared__ __align__(16) char pool[kLdsPoolBytes];
     coarse_vcycle_body<T>(lp, lc, nlev, w, singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh, pool);
@@ -895,6 +909,59 @@ __global__ void __launch_bounds__(256) k_pack(const double* __restrict__ a, doub
         out[t] = a[c];
 }
 
+// state of a device-controlled solve: the convergence test of every
+// iteration runs on the device, and the kernels of later iterations return
+// at once (LevelGeom::act points at `active`)
+struct MgCtl {
+    int active;    // 1 while iterating
+    int iters;     // iterations done
+    int diverged;  // the residual exceeded kDivergeFactor |rhs| (or is not finite)
+    int pad;
+    double rhsnorm, target, r;
+};
+
+// the first residual's norm (|rhs|_inf) sets the target max(atol, rtol |rhs|)
+__global__ void k_ctl_init(MgCtl* __restrict__ c, const unsigned long long* __restrict__ norm, double rtol,
+                           double atol, int maxiter)
+{
+    if (threadIdx.x != 0) return;
+    const double rn = __longlong_as_double((long long)*norm);
+    c->rhsnorm = rn;
+    c->r = rn;
+    c->iters = 0;
+    c->diverged = 0;
+    c->target = fmax(atol, rtol * rn);
+    c->active = (rn <= c->target || maxiter < 1) ? 0 : 1;
+}
+
+// after an iteration: stop at convergence, divergence or maxiter (the host
+// loop's tests, in its order)
+__global__ void k_ctl_step(MgCtl* __restrict__ c, const unsigned long long* __restrict__ norm, int maxiter)
+{
+    if (threadIdx.x != 0 || !c->active) return;
+    const double r = __longlong_as_double((long long)*norm);
+    c->r = r;
+    c->iters += 1;
+    if (r <= c->target)
+        c->active = 0;
+    else if (!(r <= kDivergeFactor * c->rhsnorm)) {
+        c->diverged = 1;
+        c->active = 0;
+    } else if (c->iters >= maxiter)
+        c->active = 0;
+}
+
+// the outer update writes phi out of place: after m iterations of the ncap
+// enqueued ones the iterate sits in the buffer the host names phi when m and
+// ncap have the same parity, else in phi2; bring it into phi
+__global__ void __launch_bounds__(256) k_ctl_select(const MgCtl* __restrict__ c, int ncap, double* __restrict__ home,
+                                                    const double* __restrict__ other, long long n)
+{
+    if (((c->iters - ncap) & 1) == 0) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        home[t] = other[t];
+}
+
 }  // namespace ksgf
 
 using namespace ksgf;
@@ -931,6 +998,14 @@ struct ksgf_macproj {
     // replaces the constant-beta hierarchy until set_beta is called again
     ksgf::VbSolver* vb = nullptr;
     int use_vb = 0;
+    // coefficients last written into the level tables (set_beta skips a refill
+    // that would change nothing)
+    double coef_beta = -1.0, coef_alpha = -1.0;
+    // device-controlled solves (project_begin / finish / correct)
+    ksgf::MgCtl* ctl = nullptr;   // device
+    ksgf::MgCtl* hctl = nullptr;  // pinned
+    int ncap = 0;                 // V-cycles the last begin enqueued
+    double* phi_a = nullptr;      // the buffer a device-controlled solve leaves the iterate in
 };
 
 namespace ksgf {
@@ -1243,7 +1318,7 @@ static int tune_residual(ksgf_macproj* mp)
                 hipLaunchKernelGGL((k_update_residual<T, T, false>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
                                    cor, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
             hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, nullptr, (const double*)mp->rpart, nb,
-                               mp->dnorm);
+                               mp->dnorm, (const int*)nullptr);
         };
         launch();
         (void)hipEventRecord(e0, nullptr);
@@ -1323,7 +1398,7 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
         hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(kCoarseThreads), 0, st, dlp, l, mp->nlev, mp->bw,
                            mp->singular, mp->opt.bottom_rtol, mp->opt.bottom_atol, mp->opt.bottom_maxiter,
                            mp->opt.num_bottom_smooth, mp->opt.num_final_smooth, mp->opt.num_pre_smooth,
-                           mp->opt.num_post_smooth);
+                           mp->opt.num_post_smooth, mp->G[0].act);
         return;
     }
     const LevelGeom& L = mp->G[l];
@@ -1404,7 +1479,8 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
                                cor, mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
         if (cor) std::swap(mp->phi, mp->phi2);
     }
-    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, st, (const double*)mp->rpart, nb, mp->dnorm);
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, st, (const double*)mp->rpart, nb, mp->dnorm,
+                       mp->P.act);
 }
 
 // solve with the level-0 rhs in mp->rhs (ghost layout); phi in mp->phi
@@ -1464,6 +1540,116 @@ static int mg_solve(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* 
     if (info) *info = inf;
     if (diverged) {
         set_error("MLMG: the residual " + std::to_string(r) + " exceeds 1e20 |rhs| = " + std::to_string(rhsnorm) +
+                  " after " + std::to_string(inf.iters) + " iterations (diverging)");
+        return KSGF_ERR_DIVERGED;
+    }
+    return inf.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+}
+
+static void set_act(ksgf_macproj* mp, const int* act)
+{
+    for (auto& L : mp->G) L.act = act;
+    mp->P.act = act;
+}
+
+// mg_solve's kernels with the convergence test on the device: the first
+// residual, then ncyc iterations whose kernels return at once after the test
+// has stopped the solve, then the iterate brought into the buffer the host
+// names phi.  No host synchronisation and no memset: a stream capture can
+// record it.
+static int mg_solve_begin(ksgf_macproj* mp, double rtol, double atol, int ncyc, hipStream_t st)
+{
+    // an even number of enqueued iterations ends in the buffer it starts
+    // from: phi_a, whatever replays or host-side solves ran in between
+    if (mp->phi != mp->phi_a) std::swap(mp->phi, mp->phi2);
+    ncyc += ncyc & 1;
+    const LevelGeom& L0 = mp->P;
+    const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
+    set_act(mp, nullptr);
+    launch_fill<double>(mp->phi, L0.tot, 0.0, st);
+    if (mp->singular) launch_sub_mean(mp->rhs, L0, mp->part, st);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(256), 0, st, mp->bw.stats, 2LL, 0);
+    if (mixed)
+        launch_update_residual<float, float>(mp, nullptr, mp->res32[0], st);
+    else
+        launch_update_residual<double, double>(mp, nullptr, mp->res[0], st);
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, mp->ctl, (const unsigned long long*)mp->dnorm, rtol, atol,
+                       mp->opt.maxiter);
+    set_act(mp, &mp->ctl->active);
+    for (int it = 1; it <= ncyc; ++it) {
+        if (mixed) {
+            vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->dlp32, st);
+            launch_update_residual<float, float>(mp, mp->cor32[0], mp->res32[0], st);
+        } else {
+            vcycle<double>(mp, 0, mp->cor, mp->res, mp->dlp64, st);
+            launch_update_residual<double, double>(mp, mp->cor[0], mp->res[0], st);
+        }
+        hipLaunchKernelGGL(k_ctl_step, dim3(1), dim3(64), 0, st, mp->ctl, (const unsigned long long*)mp->dnorm,
+                           mp->opt.maxiter);
+    }
+    set_act(mp, nullptr);
+    if (ncyc > 0)
+        hipLaunchKernelGGL(k_ctl_select, dim3(std::min<long long>(1024, grid_for(L0.tot, 256))), dim3(256), 0, st,
+                           (const MgCtl*)mp->ctl, ncyc, mp->phi, (const double*)mp->phi2, L0.tot);
+    KSGF_LAUNCH_CHECK();
+    mp->ncap = ncyc;
+    return KSGF_OK;
+}
+
+// read the device-controlled solve's state (one host synchronisation); an
+// unfinished solve continues with host-side tests, as mg_solve iterates
+static int mg_solve_finish(ksgf_macproj* mp, ksgf_mlmg_info* info, hipStream_t st)
+{
+    const bool mixed = (mp->opt.precision == KSGF_MG_MIXED);
+    KSGF_HIP_CHECK(hipMemcpyAsync(mp->hctl, mp->ctl, sizeof(MgCtl), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    const MgCtl c = *mp->hctl;
+    if (mp->phi != mp->phi_a) std::swap(mp->phi, mp->phi2);  // begin left the iterate in phi_a
+    ksgf_mlmg_info inf{};
+    inf.r0 = c.rhsnorm;
+    inf.nlevels = mp->nlev;
+    inf.iters = c.iters;
+    double r = c.r;
+    bool diverged = c.diverged != 0;
+    inf.converged = (r <= c.target);
+    if (c.active) {
+        double* home = mp->phi;
+        for (int it = c.iters + 1; it <= mp->opt.maxiter; ++it) {
+            if (mixed) {
+                vcycle<float>(mp, 0, mp->cor32, mp->res32, mp->dlp32, st);
+                launch_update_residual<float, float>(mp, mp->cor32[0], mp->res32[0], st);
+            } else {
+                vcycle<double>(mp, 0, mp->cor, mp->res, mp->dlp64, st);
+                launch_update_residual<double, double>(mp, mp->cor[0], mp->res[0], st);
+            }
+            KSGF_LAUNCH_CHECK();
+            int e = read_norm(mp, st, &r);
+            if (e) return e;
+            inf.iters = it;
+            if (r <= c.target) {
+                inf.converged = 1;
+                break;
+            }
+            if (!(r <= kDivergeFactor * c.rhsnorm)) {
+                diverged = true;
+                break;
+            }
+        }
+        if (mp->phi != home) {  // keep the iterate where a captured correction reads it
+            KSGF_HIP_CHECK(hipMemcpyAsync(home, mp->phi, sizeof(double) * (size_t)mp->P.tot, hipMemcpyDeviceToDevice,
+                                          st));
+            std::swap(mp->phi, mp->phi2);
+        }
+    }
+    KSGF_HIP_CHECK(hipMemcpyAsync(mp->hstats, mp->bw.stats, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    KSGF_HIP_CHECK(hipStreamSynchronize(st));
+    inf.rfinal = r;
+    inf.outer_iters = mixed ? inf.iters : 0;
+    inf.bottom_iters = mp->hstats[0];
+    inf.bottom_fail = mp->hstats[1];
+    if (info) *info = inf;
+    if (diverged) {
+        set_error("MLMG: the residual " + std::to_string(r) + " exceeds 1e20 |rhs| = " + std::to_string(c.rhsnorm) +
                   " after " + std::to_string(inf.iters) + " iterations (diverging)");
         return KSGF_ERR_DIVERGED;
     }
@@ -1558,6 +1744,8 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     if (mp->dnorm) (void)hipFree(mp->dnorm);
     if (mp->hnorm) (void)hipHostFree(mp->hnorm);
     if (mp->hstats) (void)hipHostFree(mp->hstats);
+    if (mp->ctl) (void)hipFree(mp->ctl);
+    if (mp->hctl) (void)hipHostFree(mp->hctl);
     vb_destroy(mp->vb);
     delete mp;
     return KSGF_OK;
@@ -1701,6 +1889,7 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
     const size_t n0 = (size_t)mp->P.tot;
     if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
     mp->phi = (double*)p;
+    mp->phi_a = mp->phi;
     if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
     mp->phi2 = (double*)p;
     if ((e = mp_alloc(&p, n0 * sizeof(double)))) { ksgf_macproj_destroy(mp); return e; }
@@ -1732,8 +1921,11 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         if ((e = mp_alloc(&p, sizeof(LevelPtrs<double>) * mp->nlev))) { ksgf_macproj_destroy(mp); return e; }
         mp->dlp64 = (LevelPtrs<double>*)p;
     }
+    if ((e = mp_alloc(&p, sizeof(MgCtl)))) { ksgf_macproj_destroy(mp); return e; }
+    mp->ctl = (MgCtl*)p;
     if (hipHostMalloc(&mp->hnorm, 8, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&mp->hstats, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&mp->hstats, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&mp->hctl, sizeof(MgCtl), hipHostMallocDefault) != hipSuccess) {
         ksgf_macproj_destroy(mp);
         set_error("ksgf_macproj_create: pinned scalars");
         return KSGF_ERR_NOMEM;
@@ -1742,6 +1934,8 @@ int ksgf_macproj_create(const ksgf_boxarray* ba, const int* lobc, const int* hib
         ksgf_macproj_destroy(mp);
         return e;
     }
+    mp->coef_beta = beta;
+    mp->coef_alpha = mp->alpha;
     if (!mp->P.ghost_mode) mp->kz_res = mixed ? tune_residual<float>(mp) : tune_residual<double>(mp);
     mp->kz_sweep.assign(mp->nlev, 8);
     mp->kz_rst.assign(mp->nlev, 1);
@@ -1771,10 +1965,16 @@ int ksgf_macproj_set_beta(ksgf_macproj* mp, double beta)
     }
     mp->beta = beta;
     mp->use_vb = 0;
+    if (beta == mp->coef_beta && mp->alpha == mp->coef_alpha) return KSGF_OK;  // the tables hold these already
     auto h = level_h(mp);
     for (int l = 0; l < mp->nlev; ++l) fill_coefs(mp->G[l], h[l].data(), beta, mp->alpha);
     fill_coefs(mp->P, h[0].data(), beta, mp->alpha);
-    return upload_level_tables(mp);
+    const int e = upload_level_tables(mp);
+    if (e == KSGF_OK) {
+        mp->coef_beta = beta;
+        mp->coef_alpha = mp->alpha;
+    }
+    return e;
 }
 
 int ksgf_macproj_set_beta_faces(ksgf_macproj* mp, const double* beta_x, const double* beta_y, const double* beta_z,
@@ -1849,6 +2049,100 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
         KSGF_LAUNCH_CHECK();
     }
     return ret;
+}
+
+static int mac_rhs(ksgf_macproj* mp, const double* u_mac, const double* v_mac, const double* w_mac, hipStream_t st)
+{
+    const ksgf_boxarray* ba = mp->ba;
+    CMFDev cu, cv, cw;
+    int e;
+    if ((e = cmfdev(ba, u_mac, KSGF_XFACE, 1, 1, &cu)) || (e = cmfdev(ba, v_mac, KSGF_YFACE, 1, 1, &cv)) ||
+        (e = cmfdev(ba, w_mac, KSGF_ZFACE, 1, 1, &cw)))
+        return e;
+    int err = KSGF_OK;
+    int z[3] = {0, 0, 0};
+    const Region* rv = get_region(ba, z, z, &err);
+    if (!rv) return err;
+    hipLaunchKernelGGL(k_mac_rhs, dim3(rv->nchunks), dim3(256), 0, st, cu, cv, cw, mp->rhs, mp->P,
+                       regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], 1.0 / ba->dx[0],
+                       1.0 / ba->dx[1], 1.0 / ba->dx[2]);
+    KSGF_LAUNCH_CHECK();
+    return KSGF_OK;
+}
+
+static int mac_correct(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac, hipStream_t st)
+{
+    const ksgf_boxarray* ba = mp->ba;
+    double* mac[3] = {u_mac, v_mac, w_mac};
+    MFDev mu[3];
+    int e, err = KSGF_OK;
+    for (int d = 0; d < 3; ++d)
+        if ((e = mfdev(ba, mac[d], KSGF_XFACE + d, 1, 1, &mu[d]))) return e;
+    const LevelGeom& L0 = mp->P;
+    for (int d = 0; d < 3; ++d) {
+        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        ghi[d] = 1;
+        const Region* rf = get_region(ba, glo, ghi, &err);
+        if (!rf) return err;
+        double bd = mp->beta * (1.0 / ba->dx[d]);
+        dim3 gr(rf->nchunks);
+        if (d == 0)
+            hipLaunchKernelGGL(k_mac_correct<0>, gr, dim3(256), 0, st, mu[0], (const double*)mp->phi, L0,
+                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        else if (d == 1)
+            hipLaunchKernelGGL(k_mac_correct<1>, gr, dim3(256), 0, st, mu[1], (const double*)mp->phi, L0,
+                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        else
+            hipLaunchKernelGGL(k_mac_correct<2>, gr, dim3(256), 0, st, mu[2], (const double*)mp->phi, L0,
+                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        KSGF_LAUNCH_CHECK();
+    }
+    return KSGF_OK;
+}
+
+int ksgf_macproj_project_begin(ksgf_macproj* mp, const double* u_mac, const double* v_mac, const double* w_mac,
+                               double rtol, double atol, int ncycles, ksgf_stream_t stream)
+{
+    if (!mp || !u_mac || !v_mac || !w_mac || ncycles < 0 || !(rtol >= 0.0) || !(atol >= 0.0)) {
+        set_error("ksgf_macproj_project_begin: bad argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (mp->use_vb) {
+        set_error("ksgf_macproj_project_begin: face coefficients solve with host-side tests only");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int e = mac_rhs(mp, u_mac, v_mac, w_mac, st);
+    if (e) return e;
+    return mg_solve_begin(mp, rtol, atol, std::min(ncycles, mp->opt.maxiter), st);
+}
+
+int ksgf_macproj_project_finish(ksgf_macproj* mp, ksgf_mlmg_info* info, ksgf_stream_t stream)
+{
+    if (!mp) {
+        set_error("ksgf_macproj_project_finish: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (mp->use_vb) {
+        set_error("ksgf_macproj_project_finish: face coefficients solve with host-side tests only");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    return mg_solve_finish(mp, info, (hipStream_t)stream);
+}
+
+int ksgf_macproj_project_correct(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w_mac, ksgf_stream_t stream)
+{
+    if (!mp || !u_mac || !v_mac || !w_mac) {
+        set_error("ksgf_macproj_project_correct: null argument");
+        return KSGF_ERR_INVALID;
+    }
+    if (mp->use_vb) {
+        set_error("ksgf_macproj_project_correct: face coefficients solve with host-side tests only");
+        return KSGF_ERR_UNSUPPORTED;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    launch_ghosts<double>(mp->phi, mp->P, st);
+    return mac_correct(mp, u_mac, v_mac, w_mac, st);
 }
 
 int ksgf_macproj_solve(ksgf_macproj* mp, const double* rhs, double* phi, double rtol, double atol, ksgf_mlmg_info* info,
