@@ -41,7 +41,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-theta", action="store_true", help="skip the run with the potential temperature advected")
     p.add_argument("--cpu-n", type=int, nargs=3, default=[128, 128, 128])
-    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal", "levels", "levels-step"], default="abl",
+    p.add_argument("--workload", choices=["abl", "poisson-batch", "nodal", "levels", "levels-step", "fixed-dt"],
+                   default="abl",
                    help="poisson-batch: 64 independent periodic Poisson problems of 32^3..128^3 (zero-mean "
                         "N(0,1) right-hand sides, seed 5) solved as one batch (hipGraph replay and eager) and one "
                         "after another; nodal: the full incflo step (advect + MAC projection + ApplyProjection's "
@@ -583,6 +584,66 @@ def nodal_step(n, steps, warmup, no_cpu, cpu_n):
     return out
 
 
+def fixed_dt_step(n, steps, warmup):
+    """Fixed-dt steps (time.fixed_dt, the setting of most reference decks)
+    eager and replayed from hipGraphs with one host read per step
+    (HostStepper.enable_graph), on the Taylor-Green 64^3 case with 32^3 boxes
+    when n is 64^3, else the neutral ABL box of size n; mixed-precision MAC
+    solve.  dt = the first adaptive step's.  Both runs must end bit-identical."""
+    import torch
+
+    ksgf = importlib.import_module("kynema-sgf_amd")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    wl = importlib.import_module("kynema-sgf_amd.workloads")
+    if tuple(n) == (64, 64, 64):
+        case = wl.taylor_green(64, 32)
+        name = "Taylor-Green 64^3, 8 boxes of 32^3"
+    else:
+        case = wl.abl(tuple(n), extent=(20.0 * n[0], 20.0 * n[1], 4.0 * n[2]), nsteps=steps)
+        name = f"neutral ABL {n[0]}x{n[1]}x{n[2]}, uneven boxes {{32,48,64}}"
+    res = {}
+    vels = {}
+    for graph in (False, True):
+        opts = ksgf.default_options(**case.mlmg)
+        opts.precision = 1
+        S = hs.HostStepper(case, options=opts)
+        S.load()
+        if graph:
+            S.enable_graph(True)
+        S.step()
+        dt = S.last_dt
+        for _ in range(warmup + 2):  # the first fixed-dt steps capture the graphs
+            S.step(dt)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            S.step(dt)
+            its.append(S.last_info["iters"])
+        torch.cuda.synchronize()
+        res[graph] = ((time.perf_counter() - t) / steps * 1e3, its, dt)
+        vels[graph] = S.velocity()
+        S.close()
+    ms_g, ms_e = res[True][0], res[False][0]
+    return {
+        "metric": "cell-updates/s per fixed-dt advect + MAC-project step, replayed from hipGraphs",
+        "value": case.ncell / (ms_g * 1e-3),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms_g,
+        "higher_is_better": True,
+        "dtype": "f64+f32mg",
+        "data": "synthetic",
+        "config": {"workload": f"{name}, fixed dt {res[True][2]:.6g}, PPM, MAC mixed precision rtol 1e-11",
+                   "mlmg_iters": res[True][1]},
+        "ms_per_step_eager": ms_e,
+        "speedup_graph_vs_eager": ms_e / ms_g,
+        "bit_identical": bool(np.array_equal(vels[True], vels[False])),
+    }
+
+
 def _faces_on_device(ba, d, ng, fn):
     """face MultiFab (1 comp, ngrow ng) of direction d on the device, each FAB
     (ghost faces included) filled with fn(X, Y, Z) at its face centres"""
@@ -796,6 +857,9 @@ def main():
         return
     if a.workload == "levels":
         print(json.dumps(levels_project(a.steps, a.warmup, a.no_cpu)))
+        return
+    if a.workload == "fixed-dt":
+        print(json.dumps(fixed_dt_step(tuple(a.n), a.steps, a.warmup)))
         return
     if a.workload == "levels-step":
         print(json.dumps(levels_step(a.steps, a.warmup, a.no_cpu)))

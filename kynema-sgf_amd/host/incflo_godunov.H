@@ -123,7 +123,22 @@ public:
         hibc_ = hibc;
     }
 
-    void setStream(hipStream_t st) { st_ = st; }
+    ~GodunovPredictor()
+    {
+        clearGraphs();
+        if (gst_) {
+            (void)hipStreamSynchronize(gst_);
+            (void)hipStreamDestroy(gst_);
+        }
+    }
+    GodunovPredictor(const GodunovPredictor&) = delete;
+    GodunovPredictor& operator=(const GodunovPredictor&) = delete;
+
+    void setStream(hipStream_t st)
+    {
+        if (graph_) throw std::runtime_error("setStream: graph replay owns its stream");
+        st_ = st;
+    }
     const BoxArray& boxArray() const { return ba_; }
 
     // domain-wide host arrays: vel (3 x N), rho (N), gp (3 x N).  The density
@@ -189,13 +204,18 @@ public:
     MultiFab& vmac() { return vmac_; }
     MultiFab& wmac() { return wmac_; }
 
-    void setCheckDivergence(bool on) { check_div_ = on; }
+    void setCheckDivergence(bool on)
+    {
+        if (on && graph_) throw ksgf::amr::UnsupportedError("setCheckDivergence: not with graph replay");
+        check_div_ = on;
+    }
 
     // diffusion: type 1 Crank-Nicolson, 2 Implicit (incflo.diffusion_type);
     // Dirichlet velocity BCs must hold zero (walls at rest)
     void enableDiffusion(double mu, int type, double rtol, double atol, const ksgf_mlmg_options& opts,
                          bool tensor = true)
     {
+        if (graph_) throw ksgf::amr::UnsupportedError("enableDiffusion: not with graph replay");
         if (type != 1 && type != 2) throw std::runtime_error("enableDiffusion: type 1 (Crank-Nicolson) or 2 (Implicit)");
         if (!(mu > 0.0)) throw std::runtime_error("enableDiffusion: mu must be positive");
         Hydro::LPInfo info;
@@ -253,6 +273,7 @@ public:
     // pressure BCs the same way, incflo_apply_nodal_projection.cpp:28-58)
     void enableNodalProjection(double rtol, double atol, const ksgf_mlmg_options& opts)
     {
+        if (graph_) throw ksgf::amr::UnsupportedError("enableNodalProjection: not with graph replay");
         Hydro::LPInfo info;
         info.do_semicoarsening = opts.do_semicoarsening != 0;
         info.max_coarsening_level = opts.max_coarsening_level;
@@ -332,6 +353,7 @@ public:
 
     StepStats advance(double fixed_dt = -1.0)
     {
+        if (graph_ && fixed_dt > 0.0) return advanceGraph(fixed_dt);
         StepStats s;
         const ksgf_boxarray* h = ba_.handle();
         ksgf_stream_t st = (ksgf_stream_t)st_;
@@ -354,6 +376,133 @@ public:
         }
         dt_prev_ = dt;
         s.dt = dt;
+        preProjection(dt);
+        mac_proj_.project(rtol_, atol_, st_);
+        s.mlmg = mac_proj_.getMLMG().last;
+        if (check_div_)
+            check(ksgf_mac_divergence_max(h, umac_.data(), vmac_.data(), wmac_.data(), 1, &s.divmax, st),
+                  "divergence check");
+        postProjection(dt, &s);
+        swapStates();
+        if (nodal_) applyProjection(dt, s);
+        last_stats_ = s;
+        return s;
+    }
+
+    // A fixed-dt step (time.fixed_dt > 0) replayed from hipGraphs: the part
+    // before the MAC projection's solve and the solve's first `ncycles`
+    // iterations (tested on the device) in one graph, then one host read of
+    // the solve's state (an unfinished solve completes on the host path),
+    // then the rest of the step in a second graph.  The first fixed-dt step
+    // runs eagerly (it builds every launch table and autotunes) and sizes the
+    // captured solve at its iteration count (rounded up to even; iterations
+    // past convergence cost a dispatch per kernel, a later step that needs
+    // more continues on the host path).  Graphs are kept per parity
+    // of the state buffers and per dt.  Steps equal the eager ones bit for
+    // bit.  Not with diffusion, the nodal projection or the divergence check
+    // (they read values on the host inside the step).  Owns a stream of its
+    // own from here on.
+    void enableGraph(bool on)
+    {
+        if (on && (diff_type_ > 0 || nodal_ || check_div_))
+            throw ksgf::amr::UnsupportedError(
+                "enableGraph: not with diffusion, the nodal projection or the divergence check");
+        clearGraphs();
+        graph_ = on;
+        graph_ncyc_ = 0;
+        if (on && !gst_) {
+            if (hipStreamCreateWithFlags(&gst_, hipStreamNonBlocking) != hipSuccess)
+                throw std::runtime_error("enableGraph: hipStreamCreateWithFlags");
+            (void)hipStreamSynchronize(st_);
+            st_ = gst_;
+        }
+    }
+
+private:
+    struct StepGraph {
+        hipGraphExec_t pre = nullptr, post = nullptr;
+        double dt = 0.0;
+    };
+
+    void clearGraphs()
+    {
+        for (auto& g : graphs_) {
+            if (g.pre) (void)hipGraphExecDestroy(g.pre);
+            if (g.post) (void)hipGraphExecDestroy(g.post);
+            g = StepGraph{};
+        }
+    }
+
+    template <typename F>
+    hipGraphExec_t capture(F&& body)
+    {
+        check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : KSGF_ERR_HIP,
+              "hipStreamBeginCapture");
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) throw std::runtime_error("hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) throw std::runtime_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return exec;
+    }
+
+    StepStats advanceGraph(double dt)
+    {
+        if (graph_ncyc_ == 0) {
+            graph_ = false;
+            StepStats s;
+            try {
+                s = advance(dt);
+            } catch (...) {
+                graph_ = true;
+                throw;
+            }
+            graph_ = true;
+            graph_ncyc_ = std::max(1, s.mlmg.iters);
+            return s;
+        }
+        StepGraph& g = graphs_[parity_];
+        if (!g.pre || g.dt != dt) {
+            if (g.pre) (void)hipGraphExecDestroy(g.pre);
+            if (g.post) (void)hipGraphExecDestroy(g.post);
+            g = StepGraph{};
+            g.pre = capture([&] {
+                preProjection(dt);
+                mac_proj_.projectBegin(rtol_, atol_, graph_ncyc_, st_);
+            });
+            g.post = capture([&] {
+                mac_proj_.projectCorrect(st_);
+                postProjection(dt, nullptr);
+            });
+            g.dt = dt;
+        }
+        StepStats s;
+        s.dt = dt;
+        dt_prev_ = dt;
+        check(hipGraphLaunch(g.pre, st_) == hipSuccess ? 0 : KSGF_ERR_HIP, "hipGraphLaunch");
+        mac_proj_.projectFinish(st_);
+        s.mlmg = mac_proj_.getMLMG().last;
+        check(hipGraphLaunch(g.post, st_) == hipSuccess ? 0 : KSGF_ERR_HIP, "hipGraphLaunch");
+        swapStates();
+        last_stats_ = s;
+        return s;
+    }
+
+    // the step up to the MAC projection: explicit viscous term, forcing,
+    // ExtrapVelToFaces, inflow faces, beta
+    void preProjection(double dt)
+    {
+        const ksgf_boxarray* h = ba_.handle();
+        ksgf_stream_t st = (ksgf_stream_t)st_;
         // explicit viscous term of the old state (Crank-Nicolson)
         const bool cn = (diff_type_ == 1);
         if (cn && tensor_) {
@@ -380,11 +529,16 @@ public:
                                        vel_bcval_.data(), st),
               "set_inflow_velocity");
         mac_proj_.updateBeta(1.0 / rho0_);
-        mac_proj_.project(rtol_, atol_, st_);
-        s.mlmg = mac_proj_.getMLMG().last;
-        if (check_div_)
-            check(ksgf_mac_divergence_max(h, umac_.data(), vmac_.data(), wmac_.data(), 1, &s.divmax, st),
-                  "divergence check");
+    }
+
+    // the step after the MAC projection: face fills, scalar, momentum fluxes,
+    // divergence, predictor, implicit viscous solve, new-state fills (the
+    // states are swapped by the caller)
+    void postProjection(double dt, StepStats* s)
+    {
+        const ksgf_boxarray* h = ba_.handle();
+        ksgf_stream_t st = (ksgf_stream_t)st_;
+        const bool cn = (diff_type_ == 1);
         MultiFab* mac[3] = {&umac_, &vmac_, &wmac_};
         for (int d = 0; d < 3; ++d) {
             std::vector<BCRec> bc1{vel_bc_[d]};
@@ -417,23 +571,27 @@ public:
                 // one coupled solve of the three components (its statistics in diffusion[0])
                 tensor_->setScalars(1.0, dt_diff);
                 tensor_->setACoeffs(rho_, st_);
-                s.diffusion[0] = tensor_->solve(*vel_new_, *drhs_, diff_rtol_, diff_atol_, st_);
+                const ksgf_mlmg_info di = tensor_->solve(*vel_new_, *drhs_, diff_rtol_, diff_atol_, st_);
+                if (s) s->diffusion[0] = di;
             } else {
                 for (int c = 0; c < 3; ++c) {
                     lap_[c]->setScalars(1.0, dt_diff);
-                    s.diffusion[c] = lap_[c]->solve(*vel_new_, *drhs_, c, diff_rtol_, diff_atol_, st_);
+                    const ksgf_mlmg_info di = lap_[c]->solve(*vel_new_, *drhs_, c, diff_rtol_, diff_atol_, st_);
+                    if (s) s->diffusion[c] = di;
                 }
             }
         }
         vel_new_->FillBoundary(st_);
         vel_new_->FillPhysBC(vel_bc_, vel_bcval_, st_);
-        std::swap(vel_, vel_new_);
-        if (nodal_) applyProjection(dt, s);
-        last_stats_ = s;
-        return s;
     }
 
-private:
+    void swapStates()
+    {
+        std::swap(vel_, vel_new_);
+        if (sc_) std::swap(sc_->s, sc_->s_new);
+        parity_ ^= 1;
+    }
+
     void applyProjection(double dt, StepStats& s)
     {
         const ksgf_boxarray* h = ba_.handle();
@@ -489,7 +647,6 @@ private:
               "scalar predictor_rhs");
         S.s_new->FillBoundary(st_);
         S.s_new->FillPhysBC(S.bc, S.bcval, st_);
-        std::swap(S.s, S.s_new);
     }
 
     std::unique_ptr<Scalar> sc_;
@@ -512,6 +669,12 @@ private:
     bool use_force_cfl_ = true;
     hipStream_t st_ = nullptr;
     bool check_div_ = false;
+    // graph replay of fixed-dt steps
+    bool graph_ = false;
+    int graph_ncyc_ = 0;
+    int parity_ = 0;  // flips with every swap of the state buffers
+    StepGraph graphs_[2];
+    hipStream_t gst_ = nullptr;
     std::unique_ptr<MultiFab> vel_, vel_new_;
     MultiFab rho_, gp_, src_, src_nph_, q_, fq_;
     MultiFab umac_, vmac_, wmac_;

@@ -101,6 +101,21 @@ int ksgf_host_stepper_step(ksgf_host_stepper* s, double fixed_dt, double* dt_use
     }
 }
 
+int ksgf_host_stepper_enable_graph(ksgf_host_stepper* s, int on)
+{
+    try {
+        if (!s) throw std::runtime_error("null stepper");
+        reinterpret_cast<GodunovPredictor*>(s)->enableGraph(on != 0);
+        return KSGF_OK;
+    } catch (const ksgf::amr::UnsupportedError& e) {
+        g_err = std::string("ksgf_host_stepper_enable_graph: ") + e.what();
+        return KSGF_ERR_UNSUPPORTED;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_stepper_enable_graph: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
 int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel)
 {
     try {

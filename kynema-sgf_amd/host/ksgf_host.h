@@ -25,6 +25,11 @@ int ksgf_host_stepper_set_state(ksgf_host_stepper* s, const double* vel, const d
  * |div u_mac| after the MAC projection */
 int ksgf_host_stepper_step(ksgf_host_stepper* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info,
                            double* divmax);
+/* fixed-dt steps (fixed_dt > 0) replayed from hipGraphs with one host read
+ * per step (GodunovPredictor::enableGraph); equal to the eager steps bit for
+ * bit; KSGF_ERR_UNSUPPORTED with diffusion, the nodal projection or the
+ * divergence check */
+int ksgf_host_stepper_enable_graph(ksgf_host_stepper* s, int on);
 int ksgf_host_stepper_get_velocity(ksgf_host_stepper* s, double* vel);
 /* the MAC projection's phi of the last step, domain-wide host array (n0*n1*n2) */
 int ksgf_host_stepper_get_phi(ksgf_host_stepper* s, double* phi);

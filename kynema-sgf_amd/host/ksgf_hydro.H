@@ -461,6 +461,31 @@ public:
         }
         ksgf::amr::check(rc, "MacProjector::project");  // KSGF_ERR_NOT_CONVERGED (> 0) is not an abort
     }
+    // project split for a step captured into a hipGraph: begin (div u_mac and
+    // `ncycles` iterations tested on the device; capturable), finish (reads
+    // the solve's state, one host synchronisation, and completes an
+    // unfinished solve), correct (u_mac -= beta grad phi; capturable).  One
+    // level, one beta.
+    void projectBegin(double rtol, double atol, int ncycles, hipStream_t st = nullptr)
+    {
+        if (mpl_ || fba_ || beta_faces_[0]) throw ksgf::amr::UnsupportedError("MacProjector::projectBegin: one level, one beta");
+        if (!mp_) build();
+        ksgf::amr::check(ksgf_macproj_project_begin(mp_, umac_[0]->data(), umac_[1]->data(), umac_[2]->data(), rtol,
+                                                    atol, ncycles, (ksgf_stream_t)st),
+                         "MacProjector::projectBegin");
+    }
+    void projectFinish(hipStream_t st = nullptr)
+    {
+        if (!mp_) throw std::runtime_error("MacProjector::projectFinish: no projectBegin");
+        ksgf::amr::check(ksgf_macproj_project_finish(mp_, &mlmg_.last, (ksgf_stream_t)st), "MacProjector::projectFinish");
+    }
+    void projectCorrect(hipStream_t st = nullptr)
+    {
+        if (!mp_) throw std::runtime_error("MacProjector::projectCorrect: no projectBegin");
+        ksgf::amr::check(ksgf_macproj_project_correct(mp_, umac_[0]->data(), umac_[1]->data(), umac_[2]->data(),
+                                                      (ksgf_stream_t)st),
+                         "MacProjector::projectCorrect");
+    }
     MLMG& getMLMG() { return mlmg_; }
     void rebuild() { build(); }
     // the last solution phi, domain-wide (n0*n1*n2 doubles, device), as

@@ -33,6 +33,7 @@ def host_lib():
         L.ksgf_host_stepper_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(KsgfMLMGInfo),
                                              C.POINTER(C.c_double)]
         L.ksgf_host_stepper_get_velocity.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ksgf_host_stepper_enable_graph.argtypes = [C.c_void_p, C.c_int]
         L.ksgf_host_stepper_get_phi.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ksgf_host_stepper_enable_scalar.argtypes = [C.c_void_p, C.POINTER(KsgfBCRec), C.POINTER(C.c_double)]
         L.ksgf_host_stepper_set_scalar.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
@@ -100,6 +101,10 @@ class HostStepper:
         r = np.ascontiguousarray(c.rho if rho is None else rho, dtype=np.float64)
         g = np.ascontiguousarray(c.gp if gp is None else gp, dtype=np.float64)
         _chk(host_lib().ksgf_host_stepper_set_state(self.h, _dp(v), _dp(r), _dp(g)), "set_state")
+
+    def enable_graph(self, on=True):
+        """replay fixed-dt steps (step(dt) with dt > 0) from hipGraphs"""
+        _chk(host_lib().ksgf_host_stepper_enable_graph(self.h, int(bool(on))), "enable_graph")
 
     def step(self, dt=-1.0, check_divergence=False):
         info = KsgfMLMGInfo()

@@ -99,3 +99,58 @@ def test_graph_replay_equals_eager(ksgf, oracle, workloads, gpu):
         assert np.array_equal(to_np(mp.phi()), refs[rep][1]), rep
     del graph
     mp.close()
+
+
+@pytest.mark.parametrize("which", ["tgv64", "abl_scalar"])
+def test_host_step_graph_equals_eager(ksgf, oracle, workloads, gpu, which):
+    """GodunovPredictor's fixed-dt steps replayed from hipGraphs (one host
+    read per step) equal the eager steps bit for bit: velocity, the scalar,
+    the MAC solve's iteration count, over steps that capture both parities"""
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    if which == "tgv64":
+        case = workloads.taylor_green(64, 32)
+    else:
+        case = workloads.abl((64, 48, 32), extent=(1280.0, 960.0, 128.0), sizes=(16, 32), nsteps=1)
+    _say(f"host step graph {which}")
+    n = case.geom.n_cell
+    z, y, x = np.meshgrid(*[(np.arange(int(m)) + 0.5) / int(m) for m in n[::-1]], indexing="ij")
+    blob = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) / (2 * 0.1 ** 2))[None]
+    sbc = np.zeros((1, 6), dtype=np.int32)
+    if which == "abl_scalar":
+        sbc[0, 2] = sbc[0, 5] = 2  # foextrap at the walls
+    steppers = []
+    for graph in (False, True):
+        H = hs.HostStepper(case)
+        H.load()
+        if which == "abl_scalar":
+            H.enable_scalar(sbc)
+            H.load_scalar(blob)
+        if graph:
+            H.enable_graph(True)
+        steppers.append(H)
+    E, Gs = steppers
+    E.step()  # one adaptive step gives the fixed dt
+    dt = E.last_dt
+    Gs.step()
+    assert np.array_equal(E.velocity(), Gs.velocity())
+    for k in range(5):
+        E.step(dt)
+        Gs.step(dt)
+        _say(f"  step {k}: MAC iterations eager {E.last_info['iters']} graph {Gs.last_info['iters']}")
+        assert Gs.last_info["iters"] == E.last_info["iters"] and Gs.last_dt == dt
+        assert np.array_equal(E.velocity(), Gs.velocity()), k
+        if which == "abl_scalar":
+            assert np.array_equal(E.scalar(), Gs.scalar()), k
+    for H in steppers:
+        H.close()
+
+
+def test_host_graph_rejects_host_reads(ksgf, workloads, gpu):
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    case = workloads.taylor_green(16, 16)
+    H = hs.HostStepper(case)
+    H.load()
+    H.enable_diffusion(0.01, 2, 1e-10, 1e-14)
+    with pytest.raises(RuntimeError, match="enable_graph"):
+        H.enable_graph(True)
+    H.close()
