@@ -732,6 +732,207 @@ __device__ void bottom_solve(double* x, double* b, const LevelGeom& L, const Bot
     __syncthreads();
 }
 
+// The same BiCGStab on a bottom level of at most 64 K cells (K = 1, 2) run by one
+// wavefront with no workgroup barrier: lane l holds cells l, l + 64, ...
+// (slot s = cell / 64) of every vector in registers, neighbours come by lane
+// shuffles (the ghost rules of bc_ghost applied to the lane's own value), and
+// the sums and maxima are wave butterflies.  With K = 1 (a bottom of at most
+// 64 cells) the sums take blk_sum's order, so the iterates are those of
+// bottom_solve bit for bit.  Returns (through the pointers) bottom_solve's
+// status and iteration count; x is written back to memory.
+template <int K>
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+template <int K>
+__device__ __forceinline__ double wave_get(const double (&v)[K], int cell)
+{
+    // every lane takes part in every shuffle (uniform slot loop)
+    double r = 0.0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const double t = __shfl(v[s], cell & 63, 64);
+        if ((cell >> 6) == s) r = t;
+    }
+    return r;
+}
+
+template <int K>
+__device__ __forceinline__ void wave_apply(const double (&x)[K], double (&y)[K], const LevelGeom& L, int lane)
+{
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2, ncell = n0 * n1 * n2;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const int t = s * 64 + lane;
+        const int tt = (t < ncell) ? t : 0;
+        const int i = tt % n0, j = (tt / n0) % n1, k = tt / (n0 * n1);
+        const double self = x[s];
+        auto nb = [&](int ii, int jj, int kk, int d, int side) {
+            const int pos = (d == 0) ? ii : (d == 1) ? jj : kk;
+            const int n = (d == 0) ? n0 : (d == 1) ? n1 : n2;
+            int c[3] = {ii, jj, kk};
+            const bool out = pos < 0 || pos >= n;
+            if (out) c[d] = (pos < 0) ? n - 1 : 0;
+            const double v = wave_get<K>(x, c[0] + n0 * (c[1] + n1 * c[2]));
+            if (!out) return v;
+            return bc_ghost<double>(side ? L.hi[d] : L.lo[d], self, v);
+        };
+        const double w = nb(i - 1, j, k, 0, 0), e = nb(i + 1, j, k, 0, 1);
+        const double so = nb(i, j - 1, k, 1, 0), no = nb(i, j + 1, k, 1, 1);
+        const double bo = nb(i, j, k - 1, 2, 0), to = nb(i, j, k + 1, 2, 1);
+        y[s] = (L.cx * (w + e) + L.cy * (so + no) + L.cz * (bo + to)) - L.dg * self;
+    }
+}
+
+template <int K>
+__device__ void wave_bottom_solve(double* xm, double* bm, const LevelGeom& L, int singular, double rtol, double atol,
+                                  int maxiter, int* ret_out, int* it_out)
+{
+    const int lane = (int)threadIdx.x;
+    const int n0 = L.n0, n1 = L.n1, ncell = L.n0 * L.n1 * L.n2;
+    bool valid[K];
+    double b[K], x[K], r[K], rh[K], p[K], v[K], sv[K], t[K];
+    auto idx = [&](int s) {
+        const int cc = s * 64 + lane;
+        return LIdx(L, cc % n0, (cc / n0) % n1, cc / (n0 * n1));
+    };
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        valid[s] = s * 64 + lane < ncell;
+        b[s] = valid[s] ? bm[idx(s)] : 0.0;
+    }
+    if (singular) {
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) acc += b[s];
+        const double mean = wave_sum<K>(acc) / (double)ncell;
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+            if (valid[s]) {
+                b[s] -= mean;
+                bm[idx(s)] = b[s];
+            }
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        x[s] = 0.0;
+        r[s] = b[s];
+        rh[s] = b[s];
+        p[s] = v[s] = sv[s] = t[s] = 0.0;
+    }
+    auto vmax = [&](const double (&a)[K]) {
+        double m = 0.0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) m = fmax(m, fabs(a[s]));
+        return wave_max(m);
+    };
+    auto vdot = [&](const double (&a)[K], const double (&c)[K]) {
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) acc += a[s] * c[s];
+        return wave_sum<K>(acc);
+    };
+    const double rnorm0 = vmax(r);
+    const double tol = fmax(rtol * rnorm0, atol);
+    int ret = 1, it = 0;
+    if (rnorm0 == 0.0 || rnorm0 <= tol) {
+        ret = 0;
+    } else {
+        double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+        for (it = 1; it <= maxiter; ++it) {
+            const double rho = vdot(rh, r);
+            if (rho == 0.0) { ret = 2; break; }
+            if (it == 1) {
+#pragma unroll
+                for (int s = 0; s < K; ++s) p[s] = r[s];
+            } else {
+                const double beta = (rho / rho_prev) * (alpha / omega);
+#pragma unroll
+                for (int s = 0; s < K; ++s) p[s] = r[s] + beta * (p[s] - omega * v[s]);
+            }
+            wave_apply<K>(p, v, L, lane);
+#pragma unroll
+            for (int s = 0; s < K; ++s)
+                if (!valid[s]) v[s] = 0.0;
+            const double rv = vdot(rh, v);
+            if (rv == 0.0) { ret = 2; break; }
+            alpha = rho / rv;
+#pragma unroll
+            for (int s = 0; s < K; ++s) sv[s] = r[s] - alpha * v[s];
+            if (vmax(sv) <= tol) {
+#pragma unroll
+                for (int s = 0; s < K; ++s) x[s] += alpha * p[s];
+                ret = 0;
+                break;
+            }
+            wave_apply<K>(sv, t, L, lane);
+#pragma unroll
+            for (int s = 0; s < K; ++s)
+                if (!valid[s]) t[s] = 0.0;
+            const double tt = vdot(t, t);
+            if (tt == 0.0) { ret = 2; break; }
+            omega = vdot(t, sv) / tt;
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                x[s] += alpha * p[s] + omega * sv[s];
+                r[s] = sv[s] - omega * t[s];
+            }
+            if (vmax(r) <= tol) { ret = 0; break; }
+            if (omega == 0.0) { ret = 2; break; }
+            rho_prev = rho;
+        }
+        if (it > maxiter) it = maxiter;
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+        if (valid[s]) xm[idx(s)] = x[s];
+    *ret_out = ret;
+    *it_out = it;
+}
+
+// bottom_solve with the one-wavefront BiCGStab where the bottom level has at
+// most 128 cells: wave 0 solves while the other waves wait at the barrier
+__device__ void bottom_solve_any(double* x, double* b, const LevelGeom& L, const BottomWork& w, int singular,
+                                 double rtol, double atol, int maxiter, int nub, int nuf, double* sh)
+{
+    const int ncell = L.n0 * L.n1 * L.n2;
+    if (ncell > 128) {
+        bottom_solve(x, b, L, w, singular, rtol, atol, maxiter, nub, nuf, sh);
+        return;
+    }
+    // the ghost layer keeps its old values; the final smoothing refills it
+    for (long long c = threadIdx.x; c < L.tot; c += blockDim.x) x[c] = 0.0;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int ret = 0, it = 0;
+        if (ncell <= 64)
+            wave_bottom_solve<1>(x, b, L, singular, rtol, atol, maxiter, &ret, &it);
+        else
+            wave_bottom_solve<2>(x, b, L, singular, rtol, atol, maxiter, &ret, &it);
+        if (threadIdx.x == 0) {
+            sh[0] = (double)ret;
+            sh[1] = (double)it;
+        }
+    }
+    __syncthreads();
+    const int ret = (int)sh[0], it = (int)sh[1];
+    __syncthreads();
+    blk_smooth<double>(x, b, L, (ret == 0) ? nub : nuf);
+    if (threadIdx.x == 0) {
+        w.stats[0] += it;
+        if (ret != 0) w.stats[1] += 1;
+    }
+    __syncthreads();
+}
+
 // levels lc .. nlev-1 of a V-cycle: zero, pre-smooth, residual, restrict
 // down to the bottom, bottom solve (FP64), prolong, post-smooth back up
 template <typename T>
@@ -798,7 +999,7 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
             for (long long c = threadIdx.x; c < Lb.tot; c += blockDim.x) b64[c] = (double)bb[c];
             __syncthreads();
         }
-        bottom_solve(x64, b64, Lb, w, singular, rtol, atol, maxiter, nub, nuf, sh);
+        bottom_solve_any(x64, b64, Lb, w, singular, rtol, atol, maxiter, nub, nuf, sh);
         if (sizeof(T) != sizeof(double)) {
             for (long long c = threadIdx.x; c < Lb.tot; c += blockDim.x) xb[c] = (T)x64[c];
             __syncthreads();
