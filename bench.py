@@ -56,11 +56,13 @@ def parse():
                    help="summarise a rocprofv3 --kernel-trace output directory of STEPS advect + project steps "
                         "by (kernel, multigrid level): calls, average and total time per step; prints text for "
                         "profiles/")
-    p.add_argument("--pmc-kernels", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "SQ_DIR", "COPY_FETCH_DIR",
-                                                      "COPY_WRITE_DIR"),
-                   help="summarise rocprofv3 --pmc runs (FETCH_SIZE; WRITE_SIZE; SQ_INSTS_VALU, SQ_INSTS_SALU, "
-                        "SQ_INSTS_VMEM_RD, SQ_INSTS_VMEM_WR, SQ_WAVES) per (kernel, multigrid level): HBM bytes per "
-                        "launch (scaled by the copy probe) and instructions per wave; prints text for profiles/")
+    p.add_argument("--pmc-kernels", nargs="+", metavar="DIR",
+                   help="FETCH_DIR WRITE_DIR SQ_DIR COPY_FETCH_DIR COPY_WRITE_DIR [SQ2_DIR]: summarise rocprofv3 "
+                        "--pmc runs (FETCH_SIZE; WRITE_SIZE; SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_VMEM_RD, "
+                        "SQ_INSTS_VMEM_WR, SQ_INSTS_LDS, SQ_LDS_BANK_CONFLICT, SQ_WAVES; optionally SQ_WAVE_CYCLES, "
+                        "SQ_ACTIVE_INST_VALU, SQ_ACTIVE_INST_LDS, SQ_ACTIVE_INST_VMEM, SQ_WAIT_INST_ANY) per (kernel, "
+                        "multigrid level): HBM bytes per launch (scaled by the copy probe), instructions per wave and "
+                        "busy fractions; prints text for profiles/")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB -> 1 GiB elementwise copy (x*1.0) 4 times: the known-size "
                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -387,9 +389,12 @@ def trace_summary(d, steps):
     return "\n".join(lines)
 
 
-def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir):
+def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir, sq2dir=None):
     """per (kernel, level): mean FETCH_SIZE / WRITE_SIZE bytes per launch
-    (scaled by the copy probe) and SQ instruction counts per launch"""
+    (scaled by the copy probe) and SQ instruction counts per launch; with
+    sq2dir (SQ_WAVE_CYCLES, SQ_ACTIVE_INST_VALU / _LDS / _VMEM,
+    SQ_WAIT_INST_ANY) also the fractions of wave time issuing VALU, LDS and
+    memory instructions and waiting on a dependency"""
     import collections
     import csv
     import glob
@@ -412,7 +417,7 @@ def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir):
         vals = [x[c] for kk, x in zip(k, v) if "MulFunctor" in kk[1]]
         scale.append(true_kib / (sum(vals) / len(vals)))
     out = collections.defaultdict(lambda: collections.defaultdict(list))
-    for d, sc in ((fdir, scale[0]), (wdir, scale[1]), (sqdir, None)):
+    for d, sc in ((fdir, scale[0]), (wdir, scale[1]), (sqdir, None)) + (((sq2dir, None),) if sq2dir else ()):
         k, v = rows(d)
         a, b = _step_window(k)
         k, v = k[a:b], v[a:b]
@@ -423,7 +428,8 @@ def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir):
     lines = [f"copy probe scale: FETCH_SIZE x{scale[0]:.4f}, WRITE_SIZE x{scale[1]:.4f} "
              f"(1 GiB read + 1 GiB written by torch.mul(x, 1.0))",
              f"{'kernel':48s} {'level':>5s} {'calls':>5s} {'read MB':>9s} {'write MB':>9s} {'VALU/wave':>10s} "
-             f"{'SALU/wave':>10s} {'VMEM rd/wave':>12s} {'VMEM wr/wave':>12s} {'waves':>9s}"]
+             f"{'SALU/wave':>10s} {'VMEM rd/wave':>12s} {'VMEM wr/wave':>12s} {'LDS/wave':>9s} {'LDS bank cf':>11s} "
+             f"{'waves':>9s}" + (f" {'VALU busy':>9s} {'LDS busy':>8s} {'VMEM busy':>9s} {'dep wait':>8s}" if sq2dir else "")]
 
     def m(v):
         return sum(v) / len(v) if v else float("nan")
@@ -436,7 +442,13 @@ def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir):
         lines.append(f"{k[:48]:48s} {l:>5s} {len(c.get('FETCH_SIZE', [])):5d} {m(c.get('FETCH_SIZE', [])) / 1e6:9.2f} "
                      f"{m(c.get('WRITE_SIZE', [])) / 1e6:9.2f} {m(c.get('SQ_INSTS_VALU', [])) / w:10.1f} "
                      f"{m(c.get('SQ_INSTS_SALU', [])) / w:10.1f} {m(c.get('SQ_INSTS_VMEM_RD', [])) / w:12.1f} "
-                     f"{m(c.get('SQ_INSTS_VMEM_WR', [])) / w:12.1f} {w:9.0f}")
+                     f"{m(c.get('SQ_INSTS_VMEM_WR', [])) / w:12.1f} {m(c.get('SQ_INSTS_LDS', [])) / w:9.1f} "
+                     f"{m(c.get('SQ_LDS_BANK_CONFLICT', [])) / w:11.1f} {w:9.0f}" +
+                     ((lambda wc: f" {m(c.get('SQ_ACTIVE_INST_VALU', [])) / wc:9.3f} "
+                                  f"{m(c.get('SQ_ACTIVE_INST_LDS', [])) / wc:8.3f} "
+                                  f"{m(c.get('SQ_ACTIVE_INST_VMEM', [])) / wc:9.3f} "
+                                  f"{m(c.get('SQ_WAIT_INST_ANY', [])) / wc:8.3f}")(m(c.get('SQ_WAVE_CYCLES', [float('nan')])))
+                      if sq2dir else ""))
     return "\n".join(lines)
 
 
@@ -844,6 +856,8 @@ def main():
         print(trace_summary(a.trace_summary[0], int(a.trace_summary[1])))
         return
     if a.pmc_kernels:
+        if len(a.pmc_kernels) not in (5, 6):
+            sys.exit("--pmc-kernels: FETCH_DIR WRITE_DIR SQ_DIR COPY_FETCH_DIR COPY_WRITE_DIR [SQ2_DIR]")
         print(pmc_kernels(*a.pmc_kernels))
         return
     if a.pmc_summary:

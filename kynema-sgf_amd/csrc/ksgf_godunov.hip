@@ -591,6 +591,8 @@ struct Bc1 {
 };
 __device__ __forceinline__ Bc1 bc_of(const BCs& bc, int c)
 {
+    // a select per entry (constant indices): a runtime index into the
+    // by-value argument would copy it to private memory
     Bc1 r;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -598,13 +600,14 @@ __device__ __forceinline__ Bc1 bc_of(const BCs& bc, int c)
         r.hi[d] = bc.hi[0][d];
     }
 #pragma unroll
-    for (int m = 1; m < kMaxComp; ++m)
-        if (m == c)
+    for (int m = 1; m < kMaxComp; ++m) {
+        const bool sel = (m == c);
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                r.lo[d] = bc.lo[m][d];
-                r.hi[d] = bc.hi[m][d];
-            }
+        for (int d = 0; d < 3; ++d) {
+            r.lo[d] = sel ? bc.lo[m][d] : r.lo[d];
+            r.hi[d] = sel ? bc.hi[m][d] : r.hi[d];
+        }
+    }
     return r;
 }
 
@@ -712,7 +715,12 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
     // corner states: xy, yx, zx, zy at planes k & 1; xz, yz at plane k - 1
     __shared__ double s_c[4][2][NP];
     __shared__ double s_cz[2][NP];
-    const int t = blockIdx.x;
+    // forcing of planes k & 1 (components x, y, z for VEL); the finals read
+    // their cells and the low neighbours from here
+    __shared__ double s_f[VEL ? 3 : 1][2][NP];
+    // consecutive tiles (x fastest within a box row) on one XCD, so the rows
+    // tiles share (halos, cache lines across tile edges) meet in its L2
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int b = tl.tiles[3 * t];
     const int x0 = tl.tiles[3 * t + 1], y0 = tl.tiles[3 * t + 2];
     const int tx = threadIdx.x % kCfT, ty = threadIdx.x / kCfT;
@@ -796,7 +804,8 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
 #pragma unroll
         for (int d = 0; d < 3; ++d) bcv[d] = bc_of(bc, d);
     // own-position inputs of plane kk (prefetched one plane ahead)
-    double p_ed[3][NS], p_vel[3], p_q = 0.0;
+    double p_ed[3][NS], p_vel[3], p_q = 0.0, p_f[VEL ? 3 : 1];
+    const bool f_ok = i >= l0 - 1 && i <= h0 + 1 && j >= l1 - 1 && j <= h1 + 1;
     auto load_plane = [&](int kk) {
         const int kz = kk - l2 + 1;
 #pragma unroll
@@ -811,7 +820,15 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
             p_vel[T] = ok ? vel_p[T][o] : 0.0;
         }
         if (!VEL) p_q = q_ok ? q_p[oq + psq * (kk - l2 + 3) + n * csq] : 0.0;
+#pragma unroll
+        for (int c = 0; c < (VEL ? 3 : 1); ++c)
+            p_f[c] = (!a.uft && f_ok) ? f_p[of + psf * (kk - l2 + 1) + (VEL ? c : n) * csf] : 0.0;
     };
+    // momentum: the x- and y-face states of plane k - 1 are those this
+    // thread loaded at the plane before (carried in registers)
+    const bool lx_load = m_xy || m_xz || (inner && ri_x && rj_c);
+    const bool ly_load = m_yx || m_yz || (inner && ri_c && rj_y);
+    double cxl = 0.0, cxh = 0.0, cyl = 0.0, cyh = 0.0;
     load_plane(l2 - 1);
     for (int k = l2 - 1; k <= h2 + 1; ++k) {
         const int s = k & 1, s1 = s ^ 1;
@@ -825,6 +842,8 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
             s_vel[T][s][P] = p_vel[T];
         }
         if (!VEL) s_q[s][P] = p_q;
+#pragma unroll
+        for (int c = 0; c < (VEL ? 3 : 1); ++c) s_f[c][s][P] = p_f[c];
         // the stage-A states and forcing this thread's corners and finals
         // read, all issued before the first barrier of the plane
         const bool d_xy = m_xy, d_yx = m_yx, d_zx = m_zx && rk_z, d_zy = m_zy && rk_z, d_xz = m_xz && rk_c1,
@@ -838,11 +857,11 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
         constexpr int cxy = VEL ? 2 : 0, cyx = VEL ? 2 : 0, czx = VEL ? 1 : 0, czy = 0, cxz = VEL ? 1 : 0,
                       cyz = 0, cfx = 0, cfy = VEL ? 1 : 0, cfz = VEL ? 2 : 0;
         const int nn = VEL ? 0 : n;
-        if (d_xy) {
+        if (VEL ? d_xy : lx_load) {
             lxy = lo_p[0][ox + (cxy + nn) * cs1[0]];
             hxy = hi_p[0][ox + (cxy + nn) * cs1[0]];
         }
-        if (d_yx) {
+        if (VEL ? d_yx : ly_load) {
             lyx = lo_p[1][oy + (cyx + nn) * cs1[1]];
             hyx = hi_p[1][oy + (cyx + nn) * cs1[1]];
         }
@@ -860,46 +879,34 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
             }
         }
         if (d_xz || f_x) {
-            lxz = lo_p[0][oxm + (cxz + nn) * cs1[0]];
-            hxz = hi_p[0][oxm + (cxz + nn) * cs1[0]];
             if (VEL) {
+                lxz = lo_p[0][oxm + (cxz + nn) * cs1[0]];
+                hxz = hi_p[0][oxm + (cxz + nn) * cs1[0]];
                 lfx = lo_p[0][oxm + cfx * cs1[0]];
                 hfx = hi_p[0][oxm + cfx * cs1[0]];
             } else {
-                lfx = lxz;
-                hfx = hxz;
+                lxz = lfx = cxl;
+                hxz = hfx = cxh;
             }
         }
         if (d_yz || f_y) {
-            lyz = lo_p[1][oym + (cyz + nn) * cs1[1]];
-            hyz = hi_p[1][oym + (cyz + nn) * cs1[1]];
             if (VEL) {
+                lyz = lo_p[1][oym + (cyz + nn) * cs1[1]];
+                hyz = hi_p[1][oym + (cyz + nn) * cs1[1]];
                 lfy = lo_p[1][oym + cfy * cs1[1]];
                 hfy = hi_p[1][oym + cfy * cs1[1]];
             } else {
-                lfy = lyz;
-                hfy = hyz;
+                lyz = lfy = cyl;
+                hyz = hfy = cyh;
             }
+        }
+        if (!VEL) {
+            cxl = lxy;
+            cxh = hxy;
+            cyl = lyx;
+            cyh = hyx;
         }
         const int kf = k - l2 + 1;  // plane k in the ngrow-1 cell storage of f / divu
-        double fxl = 0, fxh = 0, fyl = 0, fyh = 0, fzl = 0, fzh = 0;
-        if (!a.uft) {
-            if (f_x) {
-                const int o = of + psf * (kf - 1) + (VEL ? 0 : n) * csf;
-                fxl = f_p[o - 1];
-                fxh = f_p[o];
-            }
-            if (f_y) {
-                const int o = of + psf * (kf - 1) + (VEL ? 1 : n) * csf;
-                fyl = f_p[o - f0];
-                fyh = f_p[o];
-            }
-            if (f_z) {
-                const int o = of + psf * kf + (VEL ? 2 : n) * csf;
-                fzl = f_p[o - psf];
-                fzh = f_p[o];
-            }
-        }
         // prefetch the stage inputs of plane k + 1
         if (k + 1 <= h2 + 1) load_plane(k + 1);
         __syncthreads();
@@ -964,6 +971,11 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
                                                   g);
         }
         __syncthreads();
+        // forcing of the finals' cells (0 with forces in the transverse terms)
+        constexpr int ffx = VEL ? 0 : 0, ffy = VEL ? 1 : 0, ffz = VEL ? 2 : 0;
+        const double fxl = f_x ? s_f[ffx][s1][PXm] : 0.0, fxh = f_x ? s_f[ffx][s1][P] : 0.0;
+        const double fyl = f_y ? s_f[ffy][s1][PYm] : 0.0, fyh = f_y ? s_f[ffy][s1][P] : 0.0;
+        const double fzl = f_z ? s_f[ffz][s1][P] : 0.0, fzh = f_z ? s_f[ffz][s][P] : 0.0;
         // x face of cell (i, j, k-1): A = yz (plane k-1), B = zy (planes k-1, k)
         if (f_x) {
             const double dql = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1) - 1] : 0.0;
