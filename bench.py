@@ -759,22 +759,24 @@ def _tgv_cells(geom):
 
 
 def levels_step(steps, warmup, no_cpu):
-    """Two-level advect + MAC-project steps (levels.TwoLevelStep: per-level
-    Godunov, composite MAC projection, fill-patches, flux averaging,
-    average_down) on the 1,500-box ragged 320x320x480 level 0 with 600
-    level-1 boxes over 40% of the domain; the oracle's two-level step on the
-    16^3 Taylor-Green copy with three level-1 boxes is the CPU baseline."""
+    """Two-level advect + MAC-project steps (the C++ TwoLevelPredictor:
+    per-level Godunov, composite MAC projection, fill-patches, flux
+    averaging, average_down) on the 1,500-box ragged 320x320x480 level 0 with
+    600 level-1 boxes over 40% of the domain; the oracle's two-level step on a
+    64x64x96 copy of the layout is the CPU baseline."""
     import torch
 
+    ksgf = importlib.import_module("kynema-sgf_amd")
     wl = importlib.import_module("kynema-sgf_amd.workloads")
-    lv = importlib.import_module("kynema-sgf_amd.levels")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
     case, fba = wl.ragged_two_level()
-    S = lv.TwoLevelStep(case, fba)
+    dc, df = ksgf.DeviceBoxArray(case.ba), ksgf.DeviceBoxArray(fba, covers_domain=False)
     vc = _cells_on_device(case.ba, 3, 3, _tgv_cells(case.geom))
     vf = _cells_on_device(fba, 3, 3, _tgv_cells(fba.geom))
-    S.df.average_down_to(vf, 3, S.dc, vc, 3, 3)
+    df.average_down_to(vf, 3, dc, vc, 3, 3)
+    S = hs.HostTwoLevel(case, fba)
     S.load(vc, vf)
-    del vc, vf
+    del vc, vf, dc, df
     st = torch.cuda.current_stream()
     ms, its = [], []
     for s in range(warmup + steps):

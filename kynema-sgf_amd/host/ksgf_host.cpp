@@ -5,6 +5,7 @@
 #include <string>
 
 #include "incflo_godunov.H"
+#include "incflo_two_level.H"
 #include "ksgf_host.h"
 
 namespace {
@@ -14,6 +15,7 @@ thread_local std::string g_err;
 using ksgf::amr::Box;
 using ksgf::amr::Geometry;
 using ksgf::amr::GodunovPredictor;
+using ksgf::amr::TwoLevelPredictor;
 
 extern "C" {
 
@@ -272,6 +274,100 @@ int ksgf_host_stepper_diffusion_info(ksgf_host_stepper* s, ksgf_mlmg_info* info3
 int ksgf_host_stepper_destroy(ksgf_host_stepper* s)
 {
     delete reinterpret_cast<GodunovPredictor*>(s);
+    return KSGF_OK;
+}
+
+// ---- two levels -----------------------------------------------------------
+int ksgf_host_two_level_create(const ksgf_geometry* geom0, const int* boxes0, int nbox0, const int* boxes1,
+                               int nbox1, const ksgf_bcrec* vel_bc, const double* vel_bcval, const int* lobc,
+                               const int* hibc, double rho0, double cfl, double mg_rtol, double mg_atol,
+                               const ksgf_mlmg_options* opts, int limiter, ksgf_host_two_level** out)
+{
+    try {
+        if (!geom0 || !boxes0 || nbox0 <= 0 || !boxes1 || nbox1 <= 0 || !vel_bc || !vel_bcval || !lobc || !hibc ||
+            !out)
+            throw std::runtime_error("null argument");
+        Geometry g;
+        for (int d = 0; d < 3; ++d) {
+            g.domain.lo[d] = geom0->domain_lo[d];
+            g.domain.hi[d] = geom0->domain_hi[d];
+            g.prob_lo[d] = geom0->prob_lo[d];
+            g.prob_hi[d] = geom0->prob_hi[d];
+            g.is_periodic[d] = geom0->is_periodic[d];
+        }
+        auto boxes = [](const int* flat, int n) {
+            std::vector<Box> bx(n);
+            for (int b = 0; b < n; ++b)
+                for (int d = 0; d < 3; ++d) {
+                    bx[b].lo[d] = flat[6 * b + d];
+                    bx[b].hi[d] = flat[6 * b + 3 + d];
+                }
+            return bx;
+        };
+        std::vector<ksgf::amr::BCRec> bc(vel_bc, vel_bc + 3);
+        std::vector<double> bv(vel_bcval, vel_bcval + 18);
+        std::array<Hydro::LinOpBCType, 3> lo, hi;
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = (Hydro::LinOpBCType)lobc[d];
+            hi[d] = (Hydro::LinOpBCType)hibc[d];
+        }
+        ksgf_mlmg_options o;
+        if (opts)
+            o = *opts;
+        else
+            ksgf_mlmg_default_options(&o);
+        auto* s = new TwoLevelPredictor(g, boxes(boxes0, nbox0), boxes(boxes1, nbox1), bc, bv, lo, hi, rho0, cfl,
+                                        mg_rtol, mg_atol, o, limiter);
+        *out = reinterpret_cast<ksgf_host_two_level*>(s);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_two_level_create: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_two_level_set_state(ksgf_host_two_level* s, const double* vel0, const double* vel1, const double* gp0,
+                                  const double* gp1, int on_device)
+{
+    try {
+        if (!s || !vel0 || !vel1) throw std::runtime_error("null argument");
+        reinterpret_cast<TwoLevelPredictor*>(s)->setState(vel0, vel1, gp0, gp1, on_device != 0);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_two_level_set_state: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_two_level_step(ksgf_host_two_level* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info)
+{
+    try {
+        if (!s) throw std::runtime_error("null stepper");
+        auto st = reinterpret_cast<TwoLevelPredictor*>(s)->advance(fixed_dt);
+        if (dt_used) *dt_used = st.dt;
+        if (info) *info = st.mlmg;
+        return st.mlmg.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_two_level_step: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_two_level_get_velocity(ksgf_host_two_level* s, double* vel0, double* vel1)
+{
+    try {
+        if (!s || !vel0 || !vel1) throw std::runtime_error("null argument");
+        reinterpret_cast<TwoLevelPredictor*>(s)->getVelocity(vel0, vel1);
+        return KSGF_OK;
+    } catch (const std::exception& e) {
+        g_err = std::string("ksgf_host_two_level_get_velocity: ") + e.what();
+        return KSGF_ERR_INVALID;
+    }
+}
+
+int ksgf_host_two_level_destroy(ksgf_host_two_level* s)
+{
+    delete reinterpret_cast<TwoLevelPredictor*>(s);
     return KSGF_OK;
 }
 

@@ -72,6 +72,23 @@ int ksgf_host_mac_project_density(const ksgf_geometry* geom, const int* boxes, i
                                   double* wmac, double rtol, double atol, const ksgf_mlmg_options* opts,
                                   ksgf_mlmg_info* info, double* phi);
 
+/* Two AMR levels (level 1 at ratio 2, boxes in geom0's index space refined
+ * by 2, aligned to 2): the Godunov advect + composite MAC-project step of
+ * ksgf::amr::TwoLevelPredictor (incflo_two_level.H).  set_state / get_velocity
+ * take arrays in the FAB layout of each level's MultiFab (velocity with 3
+ * ghost layers; gp with none, may be NULL), in host memory (set_state:
+ * device memory with on_device).  limiter: KSGF_PPM_*. */
+typedef struct ksgf_host_two_level ksgf_host_two_level;
+int ksgf_host_two_level_create(const ksgf_geometry* geom0, const int* boxes0, int nbox0, const int* boxes1,
+                               int nbox1, const ksgf_bcrec* vel_bc, const double* vel_bcval, const int* lobc,
+                               const int* hibc, double rho0, double cfl, double mg_rtol, double mg_atol,
+                               const ksgf_mlmg_options* opts, int limiter, ksgf_host_two_level** out);
+int ksgf_host_two_level_set_state(ksgf_host_two_level* s, const double* vel0, const double* vel1, const double* gp0,
+                                  const double* gp1, int on_device);
+int ksgf_host_two_level_step(ksgf_host_two_level* s, double fixed_dt, double* dt_used, ksgf_mlmg_info* info);
+int ksgf_host_two_level_get_velocity(ksgf_host_two_level* s, double* vel0, double* vel1);
+int ksgf_host_two_level_destroy(ksgf_host_two_level* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,17 @@ def host_lib():
                                                C.POINTER(C.c_int), C.c_double, C.c_double, C.c_double, C.c_double,
                                                C.POINTER(KsgfMLMGOptions), C.POINTER(C.c_void_p)]
         L.ksgf_host_stepper_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+        L.ksgf_host_two_level_create.argtypes = [C.POINTER(KsgfGeometry), C.POINTER(C.c_int), C.c_int,
+                                                 C.POINTER(C.c_int), C.c_int, C.POINTER(KsgfBCRec),
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                 C.c_double, C.c_double, C.c_double, C.c_double,
+                                                 C.POINTER(KsgfMLMGOptions), C.c_int, C.POINTER(C.c_void_p)]
+        L.ksgf_host_two_level_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                    C.c_int]
+        L.ksgf_host_two_level_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double),
+                                               C.POINTER(KsgfMLMGInfo)]
+        L.ksgf_host_two_level_get_velocity.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ksgf_host_two_level_destroy.argtypes = [C.c_void_p]
         L.ksgf_host_stepper_set_state.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                   C.POINTER(C.c_double)]
         L.ksgf_host_stepper_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(KsgfMLMGInfo),
@@ -227,3 +238,88 @@ def mac_project_density(ba, lobc, hibc, rho, mac, factor=1.0, rtol=1e-11, atol=0
                                          _dp(r), factor, _dp(out[0]), _dp(out[1]), _dp(out[2]), rtol, atol,
                                          C.byref(o), C.byref(info), _dp(phi)), "mac_project_density")
     return out, phi, info.as_dict()
+
+
+class HostTwoLevel:
+    """The C++ two-level step driver (ksgf::amr::TwoLevelPredictor) through
+    libksgf_host.so: case is the level-0 workload, fba the level-1 BoxArray
+    (boxes aligned to 2).  States are host arrays in each level's FAB layout
+    (velocity with 3 ghost layers, gp with none)."""
+
+    def __init__(self, case, fba, options=None, limiter=1):
+        L = host_lib()
+        self.case = case
+        self.fba = fba
+        g = KsgfGeometry()
+        for d in range(3):
+            g.domain_lo[d] = case.geom.dlo[d]
+            g.domain_hi[d] = case.geom.dhi[d]
+            g.prob_lo[d] = case.geom.prob_lo[d]
+            g.prob_hi[d] = case.geom.prob_hi[d]
+            g.is_periodic[d] = int(case.geom.periodic[d])
+        self._b0 = np.ascontiguousarray(case.ba.boxes, dtype=np.int32)
+        self._b1 = np.ascontiguousarray(fba.boxes, dtype=np.int32)
+        self._bcval = np.ascontiguousarray(case.bcval, dtype=np.float64)
+        self._lo = np.ascontiguousarray(case.lobc, dtype=np.int32)
+        self._hi = np.ascontiguousarray(case.hibc, dtype=np.int32)
+        self.options = options or default_options(**case.mlmg)
+        ip = C.POINTER(C.c_int)
+        h = C.c_void_p()
+        _chk(L.ksgf_host_two_level_create(C.byref(g), self._b0.ctypes.data_as(ip), case.ba.nbox,
+                                          self._b1.ctypes.data_as(ip), fba.nbox, bcrecs(case.bcrec), _dp(self._bcval),
+                                          self._lo.ctypes.data_as(ip), self._hi.ctypes.data_as(ip), case.rho0,
+                                          case.cfl, case.mg_rtol, case.mg_atol, C.byref(self.options), int(limiter),
+                                          C.byref(h)), "two_level_create")
+        self.h = h
+        self.last_info = None
+        self.last_dt = None
+        self._n = [case.ba.mf_size(0, 3, 3), fba.mf_size(0, 3, 3)]
+
+    def load(self, vel_c, vel_f, gp_c=None, gp_f=None):
+        """host arrays, or device tensors (all of one kind) in the FAB layout"""
+        import torch
+
+        dev = torch.is_tensor(vel_c)
+        keep = []
+
+        def ptr(x):
+            if x is None:
+                return None
+            if dev:
+                x = x.contiguous()
+                keep.append(x)
+                return C.c_void_p(x.data_ptr())
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            keep.append(x)
+            return C.c_void_p(x.ctypes.data)
+
+        if dev:
+            torch.cuda.synchronize()
+        _chk(host_lib().ksgf_host_two_level_set_state(self.h, ptr(vel_c), ptr(vel_f), ptr(gp_c), ptr(gp_f), int(dev)),
+             "two_level_set_state")
+
+    def step(self, dt=None):
+        info = KsgfMLMGInfo()
+        dtu = C.c_double()
+        _chk(host_lib().ksgf_host_two_level_step(self.h, -1.0 if dt is None else dt, C.byref(dtu), C.byref(info)),
+             "two_level_step", allow=(0, 1))
+        self.last_info = info.as_dict()
+        self.last_dt = dtu.value
+        return self.last_dt
+
+    def velocity(self):
+        """(level 0, level 1) velocity in the FAB layout, 3 ghost layers"""
+        v0, v1 = np.zeros(self._n[0]), np.zeros(self._n[1])
+        _chk(host_lib().ksgf_host_two_level_get_velocity(self.h, _dp(v0), _dp(v1)), "two_level_get_velocity")
+        return v0, v1
+
+    def close(self):
+        if self.h:
+            host_lib().ksgf_host_two_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -19,6 +19,9 @@ the levels (level 1 at ratio 2, one dt for both levels, constant density):
 
 Level-1 ghost cells come from ksgf_fillpatch_two_levels (conservative linear
 interpolation), its ghost faces from ksgf_fillpatch_faces_two_levels.
+
+The C++ host layer's ksgf::amr::TwoLevelPredictor (host/incflo_two_level.H)
+makes the same calls in the same order; the two give the same bits.
 """
 from __future__ import annotations
 

@@ -51,3 +51,31 @@ def test_two_level_steps_match_oracle(ksgf, oracle, gpu, walls):
     tot2 = vc.sum(axis=(1, 2, 3))
     assert np.abs(tot2 - tot0).max() <= 1e-13 * np.abs(vc).sum()
     S.close()
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_cpp_two_level_driver_equals_python(ksgf, oracle, gpu, walls):
+    """ksgf::amr::TwoLevelPredictor (the C++ host layer, through
+    libksgf_host.so) makes the same calls as levels.TwoLevelStep in the same
+    order: two steps give the same dt, iteration counts and velocities on
+    both levels (ghosts included) bit for bit"""
+    import test_multilevel_oracle as T
+
+    print(f"C++ two-level driver, y walls {walls}", flush=True)
+    lv = importlib.import_module("kynema-sgf_amd.levels")
+    hs = importlib.import_module("kynema-sgf_amd.host_stepper")
+    case, fba, vel_c, vel_f, gp_c, gp_f = T.tgv_two_level_walls() if walls else T.tgv_two_level()
+    P = lv.TwoLevelStep(case, fba)
+    P.load(vel_c, vel_f, gp_c, gp_f)
+    H = hs.HostTwoLevel(case, fba)
+    H.load(vel_c, vel_f, gp_c, gp_f)
+    for s in range(2):
+        dp = P.step()
+        dh = H.step()
+        print(f"  step {s}: dt {dh:.6e}, composite iterations {H.last_info['iters']}", flush=True)
+        assert dh == dp and H.last_info["iters"] == P.last_info["iters"]
+        pc, pf = P.velocity()
+        hc, hf = H.velocity()
+        assert np.array_equal(hc, to_np(pc)) and np.array_equal(hf, to_np(pf)), s
+    H.close()
+    P.close()
