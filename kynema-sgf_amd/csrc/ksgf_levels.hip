@@ -28,11 +28,15 @@
 // Data: coarse unknowns, right-hand side and residual are domain-wide arrays
 // (the coarse level covers the domain; neighbours resolve by index
 // arithmetic with periodic wrap); fine ones are MultiFabs of the fine
-// BoxArray (phi with one ghost layer, filled by k_tl_ghosts from a list of
-// the FABs' face ghosts before every fine stencil pass; the Jacobi sweeps
-// write the other of two phi buffers).  Every expression is evaluated in the operand order of
+// BoxArray (phi with one ghost layer; the fine stencil passes evaluate a
+// face ghost where they read it, from a per-ghost source code: another fine
+// box's cell, a wall, or the coarse-fine quadratic with the coarse
+// interpolation tabulated once per coarse correction; the Jacobi sweeps
+// write the other of two phi buffers; the ghost layer itself is written once,
+// for the final gradient).  Every expression is evaluated in the operand order of
 // the CPU oracle (oracle/ksgf_oracle.c, composite two-level section).
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 
 #include "ksgf_internal.h"
@@ -134,10 +138,64 @@ __global__ void __launch_bounds__(256) k_tl_cover(const int* __restrict__ fown, 
     cv[t] = fown[tl_fi(T, 2 * i, 2 * j, 2 * k)] >= 0 ? 1 : 0;
 }
 
-// the face-neighbour ghosts of every fine FAB (edges and corners are not
-// read by the 7-point stencil): another fine box's cell, a physical-boundary
-// ghost, or a coarse-fine ghost.  One thread per entry of the ghost list:
-// (fine box, linear index of the ghost in its FAB storage of ngrow 1).
+// A face ghost of a fine FAB (edges and corners are not read by the 7-point
+// stencil) is another fine box's cell, a physical-boundary ghost, or a
+// coarse-fine ghost.  The ghost list holds, for every fine box, the six
+// one-cell slabs outside its faces, x fastest within a slab, in the order
+// (x lo, x hi, y lo, y hi, z lo, z hi); an entry is (fine box, linear index
+// of the ghost in its FAB storage of ngrow 1).
+struct TLGhost {
+    int b, d, s;  // box, normal direction, side (-1 low, +1 high)
+    int q[3];     // ghost position
+};
+__device__ __forceinline__ TLGhost tl_ghost_at(const int* __restrict__ fbox, int2 ent)
+{
+    TLGhost g;
+    g.b = ent.x;
+    const int* bx = fbox + 6 * g.b;
+    const int n0 = bx[3] - bx[0] + 3, n1 = bx[4] - bx[1] + 3;
+    g.q[0] = bx[0] - 1 + ent.y % n0;
+    g.q[1] = bx[1] - 1 + (ent.y / n0) % n1;
+    g.q[2] = bx[2] - 1 + ent.y / (n0 * n1);
+    g.d = 0;
+    g.s = 0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        if (g.q[e] < bx[e]) {
+            g.d = e;
+            g.s = -1;
+        } else if (g.q[e] > bx[3 + e]) {
+            g.d = e;
+            g.s = 1;
+        }
+    }
+    return g;
+}
+
+// the coarse interpolation b of a coarse-fine ghost at fine position w
+// (inside the domain, wrapped), normal direction d
+__device__ __forceinline__ double tl_cf_bq(const TLGeom& T, const double* pc, const unsigned char* cv, const int* w,
+                                           int d)
+{
+    int Q[3];
+    double x[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        Q[e] = w[e] / 2;
+        x[e] = ((double)(w[e] - 2 * Q[e]) + 0.5) * 0.5 - 0.5;
+    }
+    x[d] = 0.0;
+    return tl_cinterp(T, pc, cv, Q, d, x);
+}
+
+// the value of a coarse-fine ghost from b and the two fine cells inside:
+// u0 next to the face, u1 one further in
+__device__ __forceinline__ double tl_cf_ghost(double bq, double u0, double u1)
+{
+    return ((8.0 / 15.0) * bq + (2.0 / 3.0) * u0) - 0.2 * u1;
+}
+
+// One thread per entry of the ghost list: fills the ghost in the FABs of pf.
 __global__ void __launch_bounds__(256) k_tl_ghosts(MFDev pf, const int* __restrict__ fbox,
                                                    const int2* __restrict__ gl, long long ngl,
                                                    const int* __restrict__ fown, const double* __restrict__ pc,
@@ -145,68 +203,141 @@ __global__ void __launch_bounds__(256) k_tl_ghosts(MFDev pf, const int* __restri
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ngl) return;
-    const int2 ent = gl[t];
-    const int b = ent.x;
-    const int* bx = fbox + 6 * b;
-    const int n0 = bx[3] - bx[0] + 3, n1 = bx[4] - bx[1] + 3;
-    const int i = bx[0] - 1 + ent.y % n0, j = bx[1] - 1 + (ent.y / n0) % n1, k = bx[2] - 1 + ent.y / (n0 * n1);
-    const int q[3] = {i, j, k};
-    int nout = 0, d = 0, s = 0;
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        if (q[e] < bx[e]) {
-            ++nout;
-            d = e;
-            s = -1;
-        } else if (q[e] > bx[3 + e]) {
-            ++nout;
-            d = e;
-            s = 1;
-        }
-    }
-    if (nout != 1) return;
-    Fab f = fab_at(pf, fbox, b);
-    int i0[3] = {i, j, k};
+    const TLGhost g = tl_ghost_at(fbox, gl[t]);
+    const int d = g.d, s = g.s;
+    Fab f = fab_at(pf, fbox, g.b);
+    int i0[3] = {g.q[0], g.q[1], g.q[2]};
     i0[d] -= s;
     const double u0 = f(i0[0], i0[1], i0[2]);
     double v;
-    int w[3] = {i, j, k};
+    int w[3] = {g.q[0], g.q[1], g.q[2]};
     if (w[d] < 0 || w[d] >= T.nf[d]) {
         if (!T.per[d]) {
             const int bc = (s < 0) ? T.lobc[d] : T.hibc[d];
-            f(i, j, k) = (bc == KSGF_LO_DIRICHLET) ? -u0 : u0;
+            f(g.q[0], g.q[1], g.q[2]) = (bc == KSGF_LO_DIRICHLET) ? -u0 : u0;
             return;
         }
         w[d] = wrapi(w[d], T.nf[d]);
     }
     const int o = fown[tl_fi(T, w[0], w[1], w[2])];
     if (o >= 0) {
-        Fab g = fab_at(pf, fbox, o);
-        v = g(w[0], w[1], w[2]);
+        Fab h = fab_at(pf, fbox, o);
+        v = h(w[0], w[1], w[2]);
     } else {
-        int Q[3];
-        double x[3];
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            Q[e] = w[e] / 2;
-            x[e] = ((double)(w[e] - 2 * Q[e]) + 0.5) * 0.5 - 0.5;
-        }
-        x[d] = 0.0;
-        const double bq = tl_cinterp(T, pc, cv, Q, d, x);
         int i1[3] = {i0[0], i0[1], i0[2]};
         i1[d] -= s;
-        const double u1 = f(i1[0], i1[1], i1[2]);
-        v = ((8.0 / 15.0) * bq + (2.0 / 3.0) * u0) - 0.2 * u1;
+        v = tl_cf_ghost(tl_cf_bq(T, pc, cv, w, d), u0, f(i1[0], i1[1], i1[2]));
     }
-    f(i, j, k) = v;
+    f(g.q[0], g.q[1], g.q[2]) = v;
 }
 
-// The fine stencil passes (ghosts filled): one thread per segment of kTlKz
-// cells of one (i, j) column of a fine box, walking up in z with the cell
-// and its z neighbours in registers, its FABs located once.  The residual
-// is R = b - div(beta grad phi) with the fluxes of tl_fflux (the oracle's
-// operand order).  SMOOTH: one damped-Jacobi sweep, phi + w R, into the
-// other phi buffer; otherwise R into rf.
+// Ghost source codes, one per entry of the ghost list, so the stencil passes
+// evaluate the ghosts where they read them: code >= 0 is the storage offset
+// of the source cell (another fine box's) in the fine phi MultiFab (the two
+// phi buffers share the layout); kGhNeumann and kGhDirichlet are physical
+// walls (ghost u0 or -u0); kGhCF is a coarse-fine ghost, whose coarse
+// interpolation the table bq holds at the same entry.
+constexpr long long kGhNeumann = -1, kGhDirichlet = -2, kGhCF = -3;
+
+__global__ void __launch_bounds__(256) k_tl_gsrc(CMFDev pf, const int* __restrict__ fbox,
+                                                 const int2* __restrict__ gl, long long ngl,
+                                                 const int* __restrict__ fown, TLGeom T, long long* __restrict__ code)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ngl) return;
+    const TLGhost g = tl_ghost_at(fbox, gl[t]);
+    const int d = g.d, s = g.s;
+    int w[3] = {g.q[0], g.q[1], g.q[2]};
+    if (w[d] < 0 || w[d] >= T.nf[d]) {
+        if (!T.per[d]) {
+            const int bc = (s < 0) ? T.lobc[d] : T.hibc[d];
+            code[t] = (bc == KSGF_LO_DIRICHLET) ? kGhDirichlet : kGhNeumann;
+            return;
+        }
+        w[d] = wrapi(w[d], T.nf[d]);
+    }
+    const int o = fown[tl_fi(T, w[0], w[1], w[2])];
+    if (o >= 0) {
+        const CFab h = fab_at(pf, fbox, o);
+        code[t] = pf.pre[o] + h.idx(w[0], w[1], w[2]);
+    } else {
+        code[t] = kGhCF;
+    }
+}
+
+// the coarse interpolations of the coarse-fine ghosts (entries cfl of the
+// ghost list) from the current coarse solution, stored at their entries
+__global__ void __launch_bounds__(256) k_tl_bq(const int* __restrict__ fbox, const int2* __restrict__ gl,
+                                               const long long* __restrict__ cfl, long long ncf,
+                                               const double* __restrict__ pc, const unsigned char* __restrict__ cv,
+                                               TLGeom T, double* __restrict__ bq)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncf) return;
+    const long long e = cfl[t];
+    const TLGhost g = tl_ghost_at(fbox, gl[e]);
+    int w[3] = {g.q[0], g.q[1], g.q[2]};
+    w[g.d] = wrapi(w[g.d], T.nf[g.d]);
+    bq[e] = tl_cf_bq(T, pc, cv, w, g.d);
+}
+
+// One ghost slab (a face of a fine box), classified on the host from the
+// codes of its entries.  kFaceAffine: every ghost is a cell of one other
+// fine box, at offset off + a sa + b sb for slab coordinates (a, b) (the two
+// tangential directions in x, y, z order); the other kinds hold for the
+// whole slab (walls, coarse-fine) or fall back to the codes (mixed).  Non-
+// affine slabs keep off = sa = sb = 0, so an unconditional affine load reads
+// the buffer's first value.
+enum { kFaceAffine = 0, kFaceNeumann = 1, kFaceDirichlet = 2, kFaceCF = 3, kFaceMixed = 4 };
+struct TLFace {
+    long long off;
+    int sa, sb, kind, pad;
+};
+
+struct GhDev {
+    const long long* code;  // ghost source code of every ghost-list entry
+    const long long* off;   // first ghost-list entry of each fine box
+    const TLFace* face;     // six slabs per fine box
+    const double* bq;       // coarse interpolations at the coarse-fine entries
+};
+
+// a ghost from its code (entry t): u0 is the fine cell next to the face, u1
+// the one further in
+__device__ __forceinline__ double tl_gval(long long code, long long t, const double* __restrict__ base, double u0,
+                                          double u1, const double* __restrict__ bq)
+{
+    if (code >= 0) return base[code];
+    if (code == kGhNeumann) return u0;
+    if (code == kGhDirichlet) return -u0;
+    return tl_cf_ghost(bq[t], u0, u1);
+}
+
+// a ghost of a non-affine slab F (entry t)
+__device__ __forceinline__ double tl_fghost(const TLFace& F, long long t, const GhDev& gh,
+                                            const double* __restrict__ base, double u0, double u1)
+{
+    if (F.kind == kFaceNeumann) return u0;
+    if (F.kind == kFaceDirichlet) return -u0;
+    if (F.kind == kFaceCF) return tl_cf_ghost(gh.bq[t], u0, u1);
+    return tl_gval(gh.code[t], t, base, u0, u1, gh.bq);
+}
+
+// a ghost of slab F at slab coordinates (a, b), entry t
+__device__ __forceinline__ double tl_face_ghost(const TLFace& F, int a, int b, long long t, const GhDev& gh,
+                                                const double* __restrict__ base, double u0, double u1)
+{
+    if (F.kind == kFaceAffine) return base[F.off + a * (long long)F.sa + b * (long long)F.sb];
+    return tl_fghost(F, t, gh, base, u0, u1);
+}
+
+// The fine stencil passes: one thread per segment of kTlKz cells of one
+// (i, j) column of a fine box, walking up in z with the cell and its z
+// neighbours in registers, its FABs located once.  A neighbour outside the
+// box is evaluated where it is read, from its slab's descriptor or its ghost
+// code (another box's cell of the input buffer, a wall, or the coarse-fine
+// quadratic through the input's cells), so no ghost fill precedes a pass.  The residual is R = b - div(beta grad
+// phi) (the oracle's operand order).  SMOOTH: one damped-Jacobi sweep,
+// phi + w R, into the other phi buffer; otherwise R into rf.
 constexpr int kTlKz = 16;
 
 struct ColDev {
@@ -215,7 +346,7 @@ struct ColDev {
 
 template <bool SMOOTH>
 __global__ void __launch_bounds__(256) k_tl_march(CMFDev pin, CMFDev bf, MFDev out, const int* __restrict__ fbox,
-                                                  TLGeom T, double w, ColDev cd)
+                                                  TLGeom T, double w, ColDev cd, GhDev gh)
 {
     const int b = cd.chunks[2 * blockIdx.x];
     const unsigned loc = (unsigned)cd.chunks[2 * blockIdx.x + 1] + threadIdx.x;
@@ -224,7 +355,8 @@ __global__ void __launch_bounds__(256) k_tl_march(CMFDev pin, CMFDev bf, MFDev o
     const int nz = bx[5] - bx[2] + 1, nseg = (nz + kTlKz - 1) / kTlKz;
     if (loc >= nx * ny * (unsigned)nseg) return;
     const unsigned q = loc / nx;
-    const int i = bx[0] + (int)(loc - q * nx), j = bx[1] + (int)(q % ny), sg = (int)(q / ny);
+    const int ii = (int)(loc - q * nx), jj = (int)(q % ny), sg = (int)(q / ny);
+    const int i = bx[0] + ii, j = bx[1] + jj;
     const int k0 = bx[2] + sg * kTlKz, k1 = min(k0 + kTlKz, bx[5] + 1);
     const CFab p = fab_at(pin, fbox, b);
     const CFab rb = fab_at(bf, fbox, b);
@@ -234,22 +366,61 @@ __global__ void __launch_bounds__(256) k_tl_march(CMFDev pin, CMFDev bf, MFDev o
     const double* __restrict__ pp = p.p;
     const double* __restrict__ bp = rb.p;
     double* __restrict__ op = o.p;
+    const double* __restrict__ base = pin.p;
     const int sy = p.n0, sz = p.n0 * p.n1, sbz = rb.n0 * rb.n1, soz = o.n0 * o.n1;
+    // this column's slabs and entries: x lo/hi (slab coordinates (j, k)),
+    // y lo/hi ((i, k)), z lo/hi ((i, j)); boxes have at least two cells per
+    // direction, so a lane is on at most one x and one y face
+    const int nyz = (int)ny * nz, nxz = (int)nx * nz;
+    const long long g0 = gh.off[b];
+    const bool bw = ii == 0, be = ii == (int)nx - 1, bs = jj == 0, bn = jj == (int)ny - 1;
+    const TLFace FX = gh.face[6 * b + (be ? 1 : 0)];
+    const double* __restrict__ xg = base + FX.off + jj * (long long)FX.sa + (k0 - bx[2]) * (long long)FX.sb;
+    const long long tx = g0 + (be ? nyz : 0) + jj;                // + ny (k - z0)
+    const long long ty = g0 + 2 * nyz + (bn ? nxz : 0) + ii;      // + nx (k - z0)
+    const long long tz = g0 + 2 * (nyz + nxz) + ii + (int)nx * jj;  // + nx ny on the top
+    // the stencil loads the FAB's ghost slots (in bounds) everywhere and
+    // replaces them by the ghost's value on the box faces
     int c = p.idx(i, j, k0), cb = rb.idx(i, j, k0), co = o.idx(i, j, k0);
-    double um = pp[c - sz], u0 = pp[c];
+    double u0 = pp[c];
+    double um = pp[c - sz];
+    if (k0 == bx[2]) um = tl_face_ghost(gh.face[6 * b + 4], ii, jj, tz, gh, base, u0, pp[c + sz]);
     const int len = k1 - k0;
-#pragma unroll 4
-    for (int t = 0; t < len; ++t) {
-        const double up = pp[c + sz];
+    int kr = k0 - bx[2];
+    // not unrolled: 60 VGPRs (8 waves per SIMD); unrolled by 2 or 4 the
+    // face loads take 88 / 111 VGPRs and it is no faster or slower
+#pragma unroll 1
+    for (int t = 0; t < len; ++t, ++kr) {
+        double up = pp[c + sz];
+        if (kr == nz - 1) up = tl_face_ghost(gh.face[6 * b + 5], ii, jj, tz + (int)nx * ny, gh, base, u0, um);
+        double e = pp[c + 1], wv = pp[c - 1];
+        // x faces are in every wave (a wave is a run of x): the affine
+        // ghost is loaded without a branch (lanes off the face read the
+        // same address as the face lane of their slab)
+        {
+            double gv = *xg;
+            xg += FX.sb;
+            if ((bw | be) && FX.kind != kFaceAffine)
+                gv = tl_fghost(FX, tx + (long long)ny * kr, gh, base, u0, bw ? e : wv);
+            if (bw) wv = gv;
+            if (be) e = gv;
+        }
+        double nv = pp[c + sy], sv = pp[c - sy];
+        if (bs | bn) {
+            const double gv = tl_face_ghost(gh.face[6 * b + (bn ? 3 : 2)], ii, kr, ty + (long long)nx * kr, gh, base,
+                                            u0, bs ? nv : sv);
+            if (bs) sv = gv;
+            if (bn) nv = gv;
+        }
         double acc = 0.0;
         {
-            const double Fp = T.beta * (pp[c + 1] - u0) * T.ihf[0];
-            const double Fm = T.beta * (u0 - pp[c - 1]) * T.ihf[0];
+            const double Fp = T.beta * (e - u0) * T.ihf[0];
+            const double Fm = T.beta * (u0 - wv) * T.ihf[0];
             acc += (Fp - Fm) * T.ihf[0];
         }
         {
-            const double Fp = T.beta * (pp[c + sy] - u0) * T.ihf[1];
-            const double Fm = T.beta * (u0 - pp[c - sy]) * T.ihf[1];
+            const double Fp = T.beta * (nv - u0) * T.ihf[1];
+            const double Fm = T.beta * (u0 - sv) * T.ihf[1];
             acc += (Fp - Fm) * T.ihf[1];
         }
         {
@@ -267,14 +438,15 @@ __global__ void __launch_bounds__(256) k_tl_march(CMFDev pin, CMFDev bf, MFDev o
     }
 }
 
-// fine flux beta dphi/dx_d through the face of fine cell f on side s (the
-// neighbour is in the FAB: valid or ghost)
-__device__ __forceinline__ double tl_fflux(const TLGeom& T, const CFab& p, const int* f, int d, int s)
+// position in the ghost list of box bx's ghost at (i, j, k), outside the box
+// in direction d on the high (hi != 0) or low side
+__device__ __forceinline__ long long tl_gidx(const int* bx, int d, int hi, int i, int j, int k)
 {
-    int g[3] = {f[0], f[1], f[2]};
-    g[d] += s;
-    const double u0 = p(f[0], f[1], f[2]), gv = p(g[0], g[1], g[2]);
-    return (s > 0) ? T.beta * (gv - u0) * T.ihf[d] : T.beta * (u0 - gv) * T.ihf[d];
+    const int nx = bx[3] - bx[0] + 1, ny = bx[4] - bx[1] + 1, nz = bx[5] - bx[2] + 1;
+    const long long nyz = (long long)ny * nz, nxz = (long long)nx * nz;
+    if (d == 0) return (hi ? nyz : 0) + (j - bx[1]) + (long long)ny * (k - bx[2]);
+    if (d == 1) return 2 * nyz + (hi ? nxz : 0) + (i - bx[0]) + (long long)nx * (k - bx[2]);
+    return 2 * (nyz + nxz) + (hi ? (long long)nx * ny : 0) + (i - bx[0]) + (long long)nx * (j - bx[1]);
 }
 
 // composite residual on the coarse cells: uncovered R = b - Lc (averaged fine
@@ -282,7 +454,7 @@ __device__ __forceinline__ double tl_fflux(const TLGeom& T, const CFab& p, const
 __global__ void __launch_bounds__(256) k_tl_cres(const double* __restrict__ pc, const double* __restrict__ bc,
                                                  double* __restrict__ rc, const unsigned char* __restrict__ cv,
                                                  CMFDev pf, const int* __restrict__ fbox, const int* __restrict__ fown,
-                                                 TLGeom T)
+                                                 TLGeom T, GhDev gh)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long nc = (long long)T.n[0] * T.n[1] * T.n[2];
@@ -319,8 +491,16 @@ __global__ void __launch_bounds__(256) k_tl_cres(const double* __restrict__ pc, 
                     fi[d] = (s > 0) ? 2 * N[d] : 2 * N[d] + 1;
                     fi[t1] = 2 * N[t1] + (m & 1);
                     fi[t2] = 2 * N[t2] + ((m >> 1) & 1);
+                    // the flux through fi's face towards C: its ghost there is
+                    // a coarse-fine ghost, evaluated from its code
                     const int o = fown[tl_fi(T, fi[0], fi[1], fi[2])];
-                    ff[m] = tl_fflux(T, fab_at(pf, fbox, o), fi, d, -s);
+                    const CFab pb = fab_at(pf, fbox, o);
+                    int fn[3] = {fi[0], fi[1], fi[2]};
+                    fn[d] += s;
+                    const double u0 = pb(fi[0], fi[1], fi[2]);
+                    const long long ge = gh.off[o] + tl_gidx(fbox + 6 * o, d, s < 0, fi[0], fi[1], fi[2]);
+                    const double gv = tl_gval(gh.code[ge], ge, pf.p, u0, pb(fn[0], fn[1], fn[2]), gh.bq);
+                    ff[m] = (s < 0) ? T.beta * (gv - u0) * T.ihf[d] : T.beta * (u0 - gv) * T.ihf[d];
                 }
                 F[a] = 0.25 * ((ff[0] + ff[1]) + (ff[2] + ff[3]));
             } else {
@@ -540,6 +720,12 @@ struct ksgf_macproj_levels {
     double *bf = nullptr, *rf = nullptr;
     int2* gl = nullptr;  // face-ghost list of the fine FABs
     long long ngl = 0;
+    long long* gcode = nullptr;  // ghost source code of every ghost-list entry
+    long long* goff = nullptr;   // first ghost-list entry of each fine box
+    TLFace* face = nullptr;      // six slab descriptors per fine box
+    long long* cfl = nullptr;    // ghost-list entries of the coarse-fine ghosts
+    double* bq = nullptr;        // their coarse interpolations (current coarse solution), by entry
+    long long ncf = 0;
     int* cols = nullptr;  // column-segment launch table of the fine boxes (k_tl_march)
     int ncols_chunks = 0;
     long long nc = 0, nfg = 0, nf0 = 0;
@@ -572,45 +758,52 @@ struct TLRun {
     MFDev rf;
     RegDev rv;  // fine valid cells
     int nv_chunks;
-    bool ghosts_ok = false;  // the current phi buffer's ghosts match phi and the coarse solution
 
     const MFDev& pf() const { return pfm[m->cur]; }
     const CMFDev& cpf() const { return cpfm[m->cur]; }
+    GhDev gh() const { return GhDev{m->gcode, m->goff, m->face, m->bq}; }
+    // the face ghosts of the current phi buffer, written into its FABs (the
+    // stencil passes evaluate them where they read them; this is for the
+    // final gradient and the returned solution)
     void ghosts()
     {
-        if (ghosts_ok) return;
-        ghosts_ok = true;
         if (m->ngl == 0) return;
         hipLaunchKernelGGL(k_tl_ghosts, dim3(grid_for(m->ngl, 256)), dim3(256), 0, st, pf(), m->fba->dbox,
                            (const int2*)m->gl, m->ngl, m->fown, (const double*)m->pc, (const unsigned char*)m->cv,
                            m->T);
     }
+    // the coarse interpolations of the coarse-fine ghosts, after every change
+    // of the coarse solution
+    void coarse_changed()
+    {
+        if (m->ncf == 0) return;
+        hipLaunchKernelGGL(k_tl_bq, dim3(grid_for(m->ncf, 256)), dim3(256), 0, st, m->fba->dbox, (const int2*)m->gl,
+                           (const long long*)m->cfl, m->ncf, (const double*)m->pc, (const unsigned char*)m->cv, m->T,
+                           m->bq);
+    }
     void fres()
     {
         hipLaunchKernelGGL(k_tl_march<false>, dim3(m->ncols_chunks), dim3(256), 0, st, cpf(), bf, rf, m->fba->dbox,
-                           m->T, 0.0, ColDev{m->cols});
+                           m->T, 0.0, ColDev{m->cols}, gh());
     }
     void cres()
     {
         hipLaunchKernelGGL(k_tl_cres, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, (const double*)m->pc,
                            (const double*)m->bc, m->rc, (const unsigned char*)m->cv, cpf(), m->fba->dbox, m->fown,
-                           m->T);
+                           m->T, gh());
     }
     // composite residual on both levels
     void residual()
     {
-        ghosts();
         fres();
         cres();
     }
     void smooth(int nsweep, double w)
     {
         for (int s = 0; s < nsweep; ++s) {
-            ghosts();
             hipLaunchKernelGGL(k_tl_march<true>, dim3(m->ncols_chunks), dim3(256), 0, st, cpf(), bf, pfm[1 - m->cur],
-                               m->fba->dbox, m->T, w, ColDev{m->cols});
+                               m->fba->dbox, m->T, w, ColDev{m->cols}, gh());
             m->cur ^= 1;
-            ghosts_ok = false;
         }
     }
     int norm(double* out)
@@ -756,7 +949,9 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
     // faces (edges and corners excluded), x fastest within a slab
     {
         std::vector<int2> h;
+        std::vector<long long> goff(fine_ba->nbox);
         for (int b = 0; b < fine_ba->nbox; ++b) {
+            goff[b] = (long long)h.size();
             const int* x = &fine_ba->hbox[6 * b];
             const int n0 = x[3] - x[0] + 3, n1 = x[4] - x[1] + 3;
             for (int d = 0; d < 3; ++d)
@@ -790,6 +985,72 @@ int ksgf_macproj_levels_create(const ksgf_boxarray* crse_ba, const ksgf_boxarray
         if (m->ngl && (hipMalloc(&m->gl, h.size() * sizeof(int2)) != hipSuccess ||
                        hipMemcpy(m->gl, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)) {
             set_error("ksgf_macproj_levels_create: ghost list");
+            return fail(KSGF_ERR_HIP);
+        }
+        // ghost source codes (on the device, from the owner map), then the
+        // list of coarse-fine entries and the slab descriptors on the host
+        CMFDev cpf;
+        if ((e = cmfdev(fine_ba, m->pf[0], KSGF_CELL, 1, 1, &cpf))) return fail(e);
+        std::vector<long long> code(m->ngl), cfl;
+        if (hipMalloc(&m->goff, goff.size() * sizeof(long long)) != hipSuccess ||
+            hipMemcpy(m->goff, goff.data(), goff.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMalloc(&m->gcode, std::max<long long>(1, m->ngl) * sizeof(long long)) != hipSuccess) {
+            set_error("ksgf_macproj_levels_create: ghost codes");
+            return fail(KSGF_ERR_HIP);
+        }
+        if (m->ngl) {
+            hipLaunchKernelGGL(k_tl_gsrc, dim3(grid_for(m->ngl, 256)), dim3(256), 0, nullptr, cpf, fine_ba->dbox,
+                               (const int2*)m->gl, m->ngl, m->fown, T, m->gcode);
+            if (hipMemcpy(code.data(), m->gcode, code.size() * sizeof(long long), hipMemcpyDeviceToHost) !=
+                hipSuccess) {
+                set_error("ksgf_macproj_levels_create: ghost codes");
+                return fail(KSGF_ERR_HIP);
+            }
+        }
+        for (long long t = 0; t < m->ngl; ++t)
+            if (code[t] == kGhCF) cfl.push_back(t);
+        // slab descriptors: affine where every entry is a cell of one box at
+        // an offset affine in the slab coordinates, else one kind for the
+        // whole slab or mixed
+        std::vector<TLFace> face(6 * (size_t)fine_ba->nbox);
+        for (int b = 0; b < fine_ba->nbox; ++b) {
+            const int* x = &fine_ba->hbox[6 * b];
+            const int n[3] = {x[3] - x[0] + 1, x[4] - x[1] + 1, x[5] - x[2] + 1};
+            long long t0 = goff[b];
+            for (int d = 0; d < 3; ++d) {
+                const int na = (d == 0) ? n[1] : n[0], nb = (d == 2) ? n[1] : n[2];
+                for (int side = 0; side < 2; ++side, t0 += (long long)na * nb) {
+                    TLFace& F = face[6 * b + 2 * d + side];
+                    F = TLFace{0, 0, 0, kFaceMixed, 0};
+                    const long long* cd = code.data() + t0;
+                    const long long c0 = cd[0];
+                    bool same = true;
+                    for (long long e = 1; e < (long long)na * nb && same; ++e) same = (cd[e] == c0);
+                    if (same && c0 == kGhNeumann) F.kind = kFaceNeumann;
+                    if (same && c0 == kGhDirichlet) F.kind = kFaceDirichlet;
+                    if (same && c0 == kGhCF) F.kind = kFaceCF;
+                    if (c0 < 0 || cd[1] < 0 || cd[na] < 0) continue;
+                    const long long sa = cd[1] - c0, sb = cd[na] - c0;
+                    if (sa <= 0 || sb <= 0 || sa > INT32_MAX || sb > INT32_MAX) continue;
+                    bool aff = true;
+                    for (int bb = 0; bb < nb && aff; ++bb)
+                        for (int aa = 0; aa < na && aff; ++aa)
+                            aff = (cd[aa + (long long)na * bb] == c0 + aa * sa + bb * sb);
+                    if (aff) F = TLFace{c0, (int)sa, (int)sb, kFaceAffine, 0};
+                }
+            }
+        }
+        if (hipMalloc(&m->face, face.size() * sizeof(TLFace)) != hipSuccess ||
+            hipMemcpy(m->face, face.data(), face.size() * sizeof(TLFace), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("ksgf_macproj_levels_create: ghost slab table");
+            return fail(KSGF_ERR_HIP);
+        }
+        m->ncf = (long long)cfl.size();
+        if (hipMalloc(&m->cfl, std::max<long long>(1, m->ncf) * sizeof(long long)) != hipSuccess ||
+            hipMalloc(&m->bq, std::max<long long>(1, m->ngl) * sizeof(double)) != hipSuccess ||
+            (m->ncf && hipMemcpy(m->cfl, cfl.data(), cfl.size() * sizeof(long long), hipMemcpyHostToDevice) !=
+                           hipSuccess)) {
+            set_error("ksgf_macproj_levels_create: coarse-fine ghost table");
             return fail(KSGF_ERR_HIP);
         }
     }
@@ -852,6 +1113,7 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     KSGF_HIP_CHECK(hipMemsetAsync(m->pc, 0, m->nc * sizeof(double), st));
     m->cur = 0;
     KSGF_HIP_CHECK(hipMemsetAsync(m->pf[0], 0, m->nfg * sizeof(double), st));
+    R.coarse_changed();
     const double diag = -2.0 * T.beta * ((1.0 / (T.hf[0] * T.hf[0]) + 1.0 / (T.hf[1] * T.hf[1])) +
                                          1.0 / (T.hf[2] * T.hf[2]));
     const double w = kOmega / diag;
@@ -878,7 +1140,7 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
         hipLaunchKernelGGL(k_tl_addc, dim3(grid_for(m->nc, 256)), dim3(256), 0, st, m->pc, (const double*)m->ec, m->nc);
         hipLaunchKernelGGL(k_tl_prolong, dim3(R.nv_chunks), dim3(256), 0, st, R.pf(), (const double*)m->ec, fba->dbox,
                            T, R.rv);
-        R.ghosts_ok = false;
+        R.coarse_changed();
         R.smooth(2 * m->opt.num_post_smooth, w);
         R.residual();
         KSGF_LAUNCH_CHECK();
@@ -905,7 +1167,6 @@ int ksgf_macproj_levels_project(ksgf_macproj_levels* m, double* const* umac_c, d
     if (m->singular && (e = R.sub_mean(m->pc, R.pf(), R.cpf(), 1))) return e;
     // u -= beta grad phi on both levels, then the fine average on the coarse
     // faces under and around the fine level
-    R.ghosts_ok = false;
     R.ghosts();
     for (int d = 0; d < 3; ++d) {
         int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
@@ -963,6 +1224,11 @@ int ksgf_macproj_levels_destroy(ksgf_macproj_levels* m)
     hipFree(m->pf[0]);
     hipFree(m->pf[1]);
     hipFree(m->gl);
+    hipFree(m->gcode);
+    hipFree(m->goff);
+    hipFree(m->face);
+    hipFree(m->cfl);
+    hipFree(m->bq);
     hipFree(m->cols);
     hipFree(m->bf);
     hipFree(m->rf);
