@@ -535,9 +535,9 @@ __global__ void __launch_bounds__(256) k_final(CMFDev q, CMFDev qbc, CMFDev f, C
 
 // ---------------------------------------------------------------------------
 // stages B and C fused: corner-coupled states and final states of one
-// component in one kernel.  A workgroup owns a 16 x 16 tile of (x, y)
-// positions of one box -- a 14 x 14 interior of output positions and a ring
-// of one -- and marches up z.  At each plane every thread forms the six
+// component in one kernel.  A workgroup owns a tile of (x, y) positions of
+// one box -- an interior of output positions and a ring of one (sizes
+// below) -- and marches up z.  At each plane every thread forms the six
 // corner states at its position (the expressions of k_corner, reading the
 // stage-A arrays) into LDS; interior threads then form the final states of
 // the plane below (x and y faces) and of the current z face (the expressions
@@ -550,8 +550,16 @@ __global__ void __launch_bounds__(256) k_final(CMFDev q, CMFDev qbc, CMFDev f, C
 // A position (i, j, k) owns the low x, y and z faces of cell (i, j, k); the
 // box's top faces are owned by the positions one past its high end.
 // ---------------------------------------------------------------------------
-constexpr int kCfT = 16;   // tile edge (threads per row and rows)
-constexpr int kCfIn = 14;  // interior (output) positions per tile edge
+// A tile is kCfTx x kCfTy positions, one thread each (247 of the block's 256
+// threads; the rest idle), of which the interior kCfInX x kCfInY write
+// outputs.  A box of n cells has n + 1 face positions per direction; for the
+// box edges 32 / 48 / 64 the 17 x 11 interior wastes fewer threads on partial
+// tiles than a square 14 x 14 (0.61 against 0.52 output positions per
+// thread-plane on the 256^3 ABL layout).
+constexpr int kCfTx = 19, kCfTy = 13;
+constexpr int kCfInX = kCfTx - 2, kCfInY = kCfTy - 2;
+constexpr int kCfThreads = 256;
+static_assert(kCfTx * kCfTy <= kCfThreads, "tile larger than the block");
 
 struct CfTile {
     const int* tiles;  // 3 ints per tile: box, x0, y0 (first interior position)
@@ -707,7 +715,7 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
                                                       GeomDev g)
 {
     constexpr int NS = VEL ? 2 : 1;  // components of each upwinded state the corners read
-    constexpr int NP = kCfT * kCfT;
+    constexpr int NP = kCfThreads;  // LDS slots: one per thread
     // inputs of plane k & 1: upwinded T states, advecting velocities, cell values
     __shared__ double s_ed[3][NS][2][NP];
     __shared__ double s_vel[3][2][NP];
@@ -723,13 +731,13 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int b = tl.tiles[3 * t];
     const int x0 = tl.tiles[3 * t + 1], y0 = tl.tiles[3 * t + 2];
-    const int tx = threadIdx.x % kCfT, ty = threadIdx.x / kCfT;
+    const int tx = threadIdx.x % kCfTx, ty = threadIdx.x / kCfTx;  // ty == kCfTy: idle thread
     const int i = x0 - 1 + tx, j = y0 - 1 + ty;
     const int n = VEL ? 0 : (int)blockIdx.y;
     const int* bx = box + 6 * b;
     const int l0 = bx[0], l1 = bx[1], l2 = bx[2], h0 = bx[3], h1 = bx[4], h2 = bx[5];
     const int P = threadIdx.x;
-    const int PXm = P - 1, PXp = P + 1, PYm = P - kCfT, PYp = P + kCfT;
+    const int PXm = P - 1, PXp = P + 1, PYm = P - kCfTx, PYp = P + kCfTx;
     // storage of the face arrays with one ghost layer (stage A, advecting
     // velocities; u_mac for VEL): per direction, plane stride and the
     // thread's offset at plane l2 - 1; component stride = whole FAB
@@ -783,13 +791,14 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
     const bool rj_y = (j >= l1 && j <= h1 + 1), rj_c = (j >= l1 && j <= h1), rj_g = (j >= l1 - 1 && j <= h1 + 1);
     // tile positions whose corner states an interior final reads (their
     // inputs then lie inside the tile)
-    const bool tx_in = tx >= 1 && tx <= kCfIn, ty_in = ty >= 1 && ty <= kCfIn;
+    const bool live = ty < kCfTy;
+    const bool tx_in = tx >= 1 && tx <= kCfInX, ty_in = ty >= 1 && ty <= kCfInY;
     const bool m_xy = tx >= 1 && ty_in && ri_x && rj_c;
-    const bool m_xz = tx >= 1 && ty <= kCfIn && ri_x && rj_g;
+    const bool m_xz = tx >= 1 && ty <= kCfInY && ri_x && rj_g;
     const bool m_yx = tx_in && ty >= 1 && ri_c && rj_y;
-    const bool m_yz = tx <= kCfIn && ty >= 1 && ri_g && rj_y;
-    const bool m_zx = tx_in && ty <= kCfIn && ri_c && rj_g;
-    const bool m_zy = tx <= kCfIn && ty_in && ri_g && rj_c;
+    const bool m_yz = live && tx <= kCfInX && ty >= 1 && ri_g && rj_y;
+    const bool m_zx = tx_in && ty <= kCfInY && ri_c && rj_g;
+    const bool m_zy = tx <= kCfInX && ty_in && ri_g && rj_c;
     const bool inner = tx_in && ty_in;
     // stage-loadable positions: face storage per direction (x, y extents)
     bool okf[3];
@@ -828,8 +837,58 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
     // thread loaded at the plane before (carried in registers)
     const bool lx_load = m_xy || m_xz || (inner && ri_x && rj_c);
     const bool ly_load = m_yx || m_yz || (inner && ri_c && rj_y);
+    // the stage-A face states this thread's corners and finals read at
+    // plane kk (the x and y ones corrected along z at plane kk - 1), loaded
+    // one plane ahead like the staged inputs
+    struct FaceIn {
+        double lxy, hxy, lyx, hyx, lzx, hzx, lzy, hzy, lfz, hfz, lxz, hxz, lfx, hfx, lyz, hyz, lfy, hfy;
+    };
+    constexpr int cxy = VEL ? 2 : 0, cyx = VEL ? 2 : 0, czx = VEL ? 1 : 0, czy = 0, cxz = VEL ? 1 : 0, cyz = 0,
+                  cfx = 0, cfy = VEL ? 1 : 0, cfz = VEL ? 2 : 0;
+    const int nn = VEL ? 0 : n;
+    auto load_faces = [&](int kk, FaceIn& F) {
+        const int kz = kk - l2 + 1;
+        const bool rz = (kk >= l2 && kk <= h2 + 1), rc1 = (kk - 1 >= l2 && kk - 1 <= h2);
+        const bool zl = (m_zx || m_zy || (inner && ri_c && rj_c)) && rz;
+        const bool xl = (m_xz || (inner && ri_x && rj_c)) && rc1, yl = (m_yz || (inner && ri_c && rj_y)) && rc1;
+        const int ox = o1[0] + ps1[0] * kz, oy = o1[1] + ps1[1] * kz, oz = o1[2] + ps1[2] * kz;
+        const int oxm = ox - ps1[0], oym = oy - ps1[1];
+        F = FaceIn{};
+        if (VEL ? m_xy : lx_load) {
+            F.lxy = lo_p[0][ox + (cxy + nn) * cs1[0]];
+            F.hxy = hi_p[0][ox + (cxy + nn) * cs1[0]];
+        }
+        if (VEL ? m_yx : ly_load) {
+            F.lyx = lo_p[1][oy + (cyx + nn) * cs1[1]];
+            F.hyx = hi_p[1][oy + (cyx + nn) * cs1[1]];
+        }
+        if (zl) {
+            F.lzx = lo_p[2][oz + (czx + nn) * cs1[2]];
+            F.hzx = hi_p[2][oz + (czx + nn) * cs1[2]];
+            if (VEL) {
+                F.lzy = lo_p[2][oz + czy * cs1[2]];
+                F.hzy = hi_p[2][oz + czy * cs1[2]];
+                F.lfz = lo_p[2][oz + cfz * cs1[2]];
+                F.hfz = hi_p[2][oz + cfz * cs1[2]];
+            }
+        }
+        if (VEL && xl) {
+            F.lxz = lo_p[0][oxm + (cxz + nn) * cs1[0]];
+            F.hxz = hi_p[0][oxm + (cxz + nn) * cs1[0]];
+            F.lfx = lo_p[0][oxm + cfx * cs1[0]];
+            F.hfx = hi_p[0][oxm + cfx * cs1[0]];
+        }
+        if (VEL && yl) {
+            F.lyz = lo_p[1][oym + (cyz + nn) * cs1[1]];
+            F.hyz = hi_p[1][oym + (cyz + nn) * cs1[1]];
+            F.lfy = lo_p[1][oym + cfy * cs1[1]];
+            F.hfy = hi_p[1][oym + cfy * cs1[1]];
+        }
+    };
     double cxl = 0.0, cxh = 0.0, cyl = 0.0, cyh = 0.0;
+    FaceIn Fn;
     load_plane(l2 - 1);
+    load_faces(l2 - 1, Fn);
     for (int k = l2 - 1; k <= h2 + 1; ++k) {
         const int s = k & 1, s1 = s ^ 1;
         const int kz1 = k - l2 + 1;  // plane index in storage with one ghost layer
@@ -844,78 +903,42 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
         if (!VEL) s_q[s][P] = p_q;
 #pragma unroll
         for (int c = 0; c < (VEL ? 3 : 1); ++c) s_f[c][s][P] = p_f[c];
-        // the stage-A states and forcing this thread's corners and finals
-        // read, all issued before the first barrier of the plane
         const bool d_xy = m_xy, d_yx = m_yx, d_zx = m_zx && rk_z, d_zy = m_zy && rk_z, d_xz = m_xz && rk_c1,
                    d_yz = m_yz && rk_c1;
         const bool f_x = inner && rk_c1 && ri_x && rj_c, f_y = inner && rk_c1 && ri_c && rj_y,
                    f_z = inner && rk_z && ri_c && rj_c;
-        const int ox = o1[0] + ps1[0] * kz1, oy = o1[1] + ps1[1] * kz1, oz = o1[2] + ps1[2] * kz1;
-        const int oxm = ox - ps1[0], oym = oy - ps1[1];
-        double lxy = 0, hxy = 0, lyx = 0, hyx = 0, lzx = 0, hzx = 0, lzy = 0, hzy = 0, lxz = 0, hxz = 0, lyz = 0,
-               hyz = 0, lfx = 0, hfx = 0, lfy = 0, hfy = 0, lfz = 0, hfz = 0;
-        constexpr int cxy = VEL ? 2 : 0, cyx = VEL ? 2 : 0, czx = VEL ? 1 : 0, czy = 0, cxz = VEL ? 1 : 0,
-                      cyz = 0, cfx = 0, cfy = VEL ? 1 : 0, cfz = VEL ? 2 : 0;
-        const int nn = VEL ? 0 : n;
-        if (VEL ? d_xy : lx_load) {
-            lxy = lo_p[0][ox + (cxy + nn) * cs1[0]];
-            hxy = hi_p[0][ox + (cxy + nn) * cs1[0]];
-        }
-        if (VEL ? d_yx : ly_load) {
-            lyx = lo_p[1][oy + (cyx + nn) * cs1[1]];
-            hyx = hi_p[1][oy + (cyx + nn) * cs1[1]];
-        }
-        if (d_zx || d_zy || f_z) {
-            lzx = lo_p[2][oz + (czx + nn) * cs1[2]];
-            hzx = hi_p[2][oz + (czx + nn) * cs1[2]];
-            if (VEL) {
-                lzy = lo_p[2][oz + czy * cs1[2]];
-                hzy = hi_p[2][oz + czy * cs1[2]];
-                lfz = lo_p[2][oz + cfz * cs1[2]];
-                hfz = hi_p[2][oz + cfz * cs1[2]];
-            } else {
-                lzy = lfz = lzx;
-                hzy = hfz = hzx;
-            }
-        }
-        if (d_xz || f_x) {
-            if (VEL) {
-                lxz = lo_p[0][oxm + (cxz + nn) * cs1[0]];
-                hxz = hi_p[0][oxm + (cxz + nn) * cs1[0]];
-                lfx = lo_p[0][oxm + cfx * cs1[0]];
-                hfx = hi_p[0][oxm + cfx * cs1[0]];
-            } else {
-                lxz = lfx = cxl;
-                hxz = hfx = cxh;
-            }
-        }
-        if (d_yz || f_y) {
-            if (VEL) {
-                lyz = lo_p[1][oym + (cyz + nn) * cs1[1]];
-                hyz = hi_p[1][oym + (cyz + nn) * cs1[1]];
-                lfy = lo_p[1][oym + cfy * cs1[1]];
-                hfy = hi_p[1][oym + cfy * cs1[1]];
-            } else {
-                lyz = lfy = cyl;
-                hyz = hfy = cyh;
-            }
-        }
-        if (!VEL) {
+        // the face states of plane k (loaded at the plane before)
+        const double lxy = Fn.lxy, hxy = Fn.hxy, lyx = Fn.lyx, hyx = Fn.hyx, lzx = Fn.lzx, hzx = Fn.hzx;
+        double lzy, hzy, lfz, hfz, lxz, hxz, lfx, hfx, lyz, hyz, lfy, hfy;
+        if (VEL) {
+            lzy = Fn.lzy, hzy = Fn.hzy, lfz = Fn.lfz, hfz = Fn.hfz;
+            lxz = Fn.lxz, hxz = Fn.hxz, lfx = Fn.lfx, hfx = Fn.hfx;
+            lyz = Fn.lyz, hyz = Fn.hyz, lfy = Fn.lfy, hfy = Fn.hfy;
+        } else {
+            lzy = lfz = lzx;
+            hzy = hfz = hzx;
+            lxz = lfx = cxl;
+            hxz = hfx = cxh;
+            lyz = lfy = cyl;
+            hyz = hfy = cyh;
             cxl = lxy;
             cxh = hxy;
             cyl = lyx;
             cyh = hyx;
         }
         const int kf = k - l2 + 1;  // plane k in the ngrow-1 cell storage of f / divu
-        // prefetch the stage inputs of plane k + 1
-        if (k + 1 <= h2 + 1) load_plane(k + 1);
+        // prefetch the stage inputs and face states of plane k + 1
+        if (k + 1 <= h2 + 1) {
+            load_plane(k + 1);
+            load_faces(k + 1, Fn);
+        }
         __syncthreads();
         double lov, hiv;
         // xy (x face, corrected along y, plane k); VEL: component z (slot 1)
         if (d_xy) {
             const int sl = VEL ? 1 : 0;
-            cf_corner_states<VEL>(lxy, hxy, s_vel[1][s][PXm], s_vel[1][s][PXm + kCfT], s_vel[1][s][P],
-                                  s_vel[1][s][PYp], s_ed[1][sl][s][PXm], s_ed[1][sl][s][PXm + kCfT],
+            cf_corner_states<VEL>(lxy, hxy, s_vel[1][s][PXm], s_vel[1][s][PXm + kCfTx], s_vel[1][s][P],
+                                  s_vel[1][s][PYp], s_ed[1][sl][s][PXm], s_ed[1][sl][s][PXm + kCfTx],
                                   s_ed[1][sl][s][P], s_ed[1][sl][s][PYp], VEL ? 0.0 : s_q[s][PXm],
                                   VEL ? 0.0 : s_q[s][P], cons, a.c3[1], a.c6[1], lov, hiv);
             s_c[0][s][P] = cf_corner_finish<0, VEL>(a, box, b, i, j, k, VEL ? cxy : n, VEL ? bcv[cxy] : bcn,
@@ -982,8 +1005,8 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
             const double dqh = (!VEL && a.has_divu) ? dv_p[of + psf * (kf - 1)] : 0.0;
             const double e = cf_final<0, VEL>(
                 a, box, b, i, j, k - 1, VEL ? cfx : n, VEL ? bcv[cfx] : bcn, VEL ? true : (a.is_velocity && n == 0),
-                cons, lfx, hfx, s_cz[1][PXm], s_cz[1][PXm + kCfT], s_cz[1][P], s_cz[1][PYp], s_c[3][s1][PXm],
-                s_c[3][s][PXm], s_c[3][s1][P], s_c[3][s][P], s_vel[1][s1][PXm], s_vel[1][s1][PXm + kCfT],
+                cons, lfx, hfx, s_cz[1][PXm], s_cz[1][PXm + kCfTx], s_cz[1][P], s_cz[1][PYp], s_c[3][s1][PXm],
+                s_c[3][s][PXm], s_c[3][s1][P], s_c[3][s][P], s_vel[1][s1][PXm], s_vel[1][s1][PXm + kCfTx],
                 s_vel[1][s1][P], s_vel[1][s1][PYp], s_vel[2][s1][PXm], s_vel[2][s][PXm], s_vel[2][s1][P],
                 s_vel[2][s][P], VEL ? 0.0 : s_q[s1][PXm], VEL ? 0.0 : s_q[s1][P], dql, dqh, fxl, fxh,
                 s_vel[0][s1][P], g);
@@ -1246,7 +1269,7 @@ static const Region* get_seg_region(const ksgf_boxarray* ba, int D, int* err)
     return r;
 }
 
-// tile table of the fused corner + final kernel: per box, 14 x 14 tiles of
+// tile table of the fused corner + final kernel: per box, kCfInX x kCfInY tiles of
 // output positions (x in lo..hi+1, y in lo..hi+1), kept with the BoxArray's
 // launch tables under a key outside their range
 static const Region* get_cf_tiles(const ksgf_boxarray* ba, int* err)
@@ -1260,8 +1283,8 @@ static const Region* get_cf_tiles(const ksgf_boxarray* ba, int* err)
     std::vector<int> t;
     for (int b = 0; b < ba->nbox; ++b) {
         const int* bx = &ba->hbox[6 * b];
-        for (int y0 = bx[1]; y0 <= bx[4] + 1; y0 += kCfIn)
-            for (int x0 = bx[0]; x0 <= bx[3] + 1; x0 += kCfIn) {
+        for (int y0 = bx[1]; y0 <= bx[4] + 1; y0 += kCfInY)
+            for (int x0 = bx[0]; x0 <= bx[3] + 1; x0 += kCfInX) {
                 t.push_back(b);
                 t.push_back(x0);
                 t.push_back(y0);
@@ -1291,7 +1314,7 @@ static int launch_corner_final(const ksgf_boxarray* ba, const CfArgs& a, const B
     if (!r) return err;
     CfTile tl;
     tl.tiles = r->chunks;
-    hipLaunchKernelGGL(k_corner_final<VEL>, dim3(r->nchunks, VEL ? 1 : ncomp), dim3(kCfT * kCfT), 0, st, a, tl,
+    hipLaunchKernelGGL(k_corner_final<VEL>, dim3(r->nchunks, VEL ? 1 : ncomp), dim3(kCfThreads), 0, st, a, tl,
                        ba->dbox, bc, g);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;

@@ -201,6 +201,80 @@ KSGF_HD void pk_relax(const PkArgs<T>& a, int color, int h, int j, int k)
     a.x[c] = ((cx * (w + e) + cy * (s + n) + cz * (bo + t)) - a.b[c]) * idg;
 }
 
+// four consecutive packed values, one 16-byte (FP32) or two (FP64) loads;
+// the arrays start 256-byte aligned and a quad starts at a multiple of 4
+template <typename T>
+struct alignas(4 * sizeof(T)) PkQuad {
+    T v[4];
+};
+
+// pk_relax for the four colour-`color` cells at packed positions 4 hq ..
+// 4 hq + 3 of row (j, k), plain or zero-initial-guess modes (row length
+// prow a multiple of 4): the same value per cell, in the same operand order.
+// The four cells share the row's parity, so their x neighbours are the
+// other colour's quad at the same positions plus one value beyond it (before
+// it on even rows, after it on odd ones); y and z neighbours are quads of
+// the neighbouring rows and planes.  Only the first and the last quad of a
+// row can touch an x side.
+template <typename T, int MODE>
+KSGF_HD void pk_relax4(const PkArgs<T>& a, int color, int hq, int j, int k)
+{
+    static_assert(MODE == kPkPlain || MODE == kPkZero, "plain and zero-guess sweeps only");
+    const LevelGeom& L = a.L;
+    const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, idg = (T)L.idg;
+    const int h0 = 4 * hq;
+    const int row = k * L.pplane + j * L.prow;
+    const int c = color * L.pcol + row + h0;
+    const PkQuad<T> bq = *(const PkQuad<T>*)(a.b + c);
+    PkQuad<T> out;
+    if (MODE == kPkZero && color == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) out.v[m] = ((T)0 - bq.v[m]) * idg;
+        *(PkQuad<T>*)(a.x + c) = out;
+        return;
+    }
+    const int par = (j + k + color) & 1;
+    const bool px = L.lo[0] == KSGF_LO_PERIODIC, py = L.lo[1] == KSGF_LO_PERIODIC, pz = L.lo[2] == KSGF_LO_PERIODIC;
+    // x sides: cell 0 of the first quad of an even row is i = 0, cell 3 of
+    // the last quad of an odd row is i = n0 - 1
+    const bool in_w = !(h0 == 0 && par == 0) || px, in_e = !(h0 + 4 == L.prow && par == 1) || px;
+    const bool in_s = j > 0 || py, in_n = j < L.n1 - 1 || py;
+    const bool in_b = k > 0 || pz, in_t = k < L.n2 - 1 || pz;
+    PkQuad<T> self{};
+    if (MODE != kPkZero && !(in_w && in_e && in_s && in_n && in_b && in_t)) self = *(const PkQuad<T>*)(a.x + c);
+    const T* ob = a.x + (1 - color) * L.pcol;
+    const PkQuad<T> o = *(const PkQuad<T>*)(ob + row + h0);
+    // the x neighbour beyond the quad (wrapped at the row ends)
+    const int hx = par == 0 ? (h0 > 0 ? h0 - 1 : L.prow - 1) : (h0 + 4 < L.prow ? h0 + 4 : 0);
+    const T xb = ob[row + hx];
+    PkQuad<T> w, e;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        w.v[m] = par == 0 ? (m == 0 ? xb : o.v[m - 1]) : o.v[m];
+        e.v[m] = par == 0 ? o.v[m] : (m == 3 ? xb : o.v[m + 1]);
+    }
+    if (!in_w) w.v[0] = pk_wall<T>(L.lo[0]) * self.v[0];
+    if (!in_e) e.v[3] = pk_wall<T>(L.hi[0]) * self.v[3];
+    auto quad = [&](bool in, int jj, int kk, int bc) {
+        PkQuad<T> q;
+        if (in) {
+            q = *(const PkQuad<T>*)(ob + kk * L.pplane + jj * L.prow + h0);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) q.v[m] = pk_wall<T>(bc) * self.v[m];
+        }
+        return q;
+    };
+    const PkQuad<T> s = quad(in_s, j > 0 ? j - 1 : L.n1 - 1, k, L.lo[1]);
+    const PkQuad<T> n = quad(in_n, j < L.n1 - 1 ? j + 1 : 0, k, L.hi[1]);
+    const PkQuad<T> bo = quad(in_b, j, k > 0 ? k - 1 : L.n2 - 1, L.lo[2]);
+    const PkQuad<T> t = quad(in_t, j, k < L.n2 - 1 ? k + 1 : 0, L.hi[2]);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        out.v[m] = ((cx * (w.v[m] + e.v[m]) + cy * (s.v[m] + n.v[m]) + cz * (bo.v[m] + t.v[m])) - bq.v[m]) * idg;
+    *(PkQuad<T>*)(a.x + c) = out;
+}
+
 // x += coarse correction of the parent, for the colour-`color` cell at
 // packed position (h, j, k) (the same sum the on-the-fly prolongation forms)
 template <typename T, bool CP>

@@ -261,6 +261,33 @@ __global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTile
     for (int k = k0; k < k1; ++k) pk_relax<T, MODE, CP>(a, color, h, j, k);
 }
 
+// the same half sweep (plain or zero-guess mode) four cells per thread: one
+// thread per quad of a packed row (rows of a multiple of 4 cells), 256
+// consecutive (quad, row) pairs of one plane chunk per workgroup, kz planes
+struct QuadTiles {
+    int nq, nb, kz;  // quads per row, workgroups per plane chunk, planes per workgroup
+};
+inline QuadTiles quad_tiles(const LevelGeom& L, int kz)
+{
+    QuadTiles t;
+    t.nq = L.prow / 4;
+    t.nb = (int)(((long long)t.nq * L.n1 + 255) / 256);
+    t.kz = kz;
+    return t;
+}
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_pk_half4(PkArgs<T> a, int color, QuadTiles t)
+{
+    if (a.L.act && *a.L.act == 0) return;
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int kc = lid / t.nb, wb = lid - kc * t.nb;
+    const unsigned idx = (unsigned)wb * 256u + threadIdx.x;
+    const unsigned j = idx / (unsigned)t.nq, hq = idx - j * (unsigned)t.nq;
+    if ((int)j >= a.L.n1) return;
+    const int k0 = kc * t.kz, k1 = min(k0 + t.kz, a.L.n2);
+    for (int k = k0; k < k1; ++k) pk_relax4<T, MODE>(a, color, (int)hq, (int)j, k);
+}
+
 // x += coarse correction on the colour-`color` cells of a packed level
 template <typename T, bool CP>
 __global__ void __launch_bounds__(256) k_pk_prolong_color(PkArgs<T> a, int color, RowTiles t)
@@ -1378,8 +1405,18 @@ template <typename T>
 static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hipStream_t st,
                              hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
+    const dim3 blk(256);
+    if ((mode == kPkPlain || mode == kPkZero) && a.L.prow % 4 == 0) {
+        const QuadTiles q = quad_tiles(a.L, kz);
+        const dim3 gq(q.nb * ((a.L.n2 + kz - 1) / kz));
+        if (mode == kPkZero)
+            hipExtLaunchKernelGGL((k_pk_half4<T, kPkZero>), gq, blk, 0, st, e0, e1, 0, a, color, q);
+        else
+            hipExtLaunchKernelGGL((k_pk_half4<T, kPkPlain>), gq, blk, 0, st, e0, e1, 0, a, color, q);
+        return;
+    }
     const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, kz);
-    const dim3 g(t.ntx * t.nty * t.nkc), blk(256);
+    const dim3 g(t.ntx * t.nty * t.nkc);
     if (mode == kPkZero)
         hipExtLaunchKernelGGL((k_pk_half<T, kPkZero, false>), g, blk, 0, st, e0, e1, 0, a, color, t);
     else if (mode == kPkProlongWalls && a.C.packed)

@@ -6,8 +6,9 @@
 // half sweeps after explicit ghost fills, and residual + average restriction
 // from a ghost-filled array.  Every sweep mode (plain, zero initial guess,
 // fused prolongation), both colours, periodic / Neumann / Dirichlet sides,
-// semicoarsened ratios, FP32 and FP64.  Built with AddressSanitizer, so
-// every array access is bounds checked.
+// semicoarsened ratios, FP32 and FP64; the four-cell form of the plain and
+// zero-guess half sweeps against the one-cell form.  Built with
+// AddressSanitizer, so every array access is bounds checked.
 //
 // Prints "ok <n>" and exits 0 when every case matches.
 #include <cmath>
@@ -145,6 +146,23 @@ static void run_half(int n0, int n1, int n2, const int lo[3], const int hi[3], c
     a.s0 = ratio[0] == 2;
     a.s1 = ratio[1] == 2;
     a.s2 = ratio[2] == 2;
+    // the four-cell form (plain and zero-guess modes, rows of a multiple of
+    // 4 packed cells) on 64-byte aligned copies of the same arrays
+    const bool quads = (MODE == kPkPlain || MODE == kPkZero) && n0 % 8 == 0;
+    T *xq = nullptr, *bq = nullptr;
+    if (quads) {
+        const size_t bytes = ((size_t)P.tot * sizeof(T) + 63) / 64 * 64;
+        xq = (T*)std::aligned_alloc(64, bytes);
+        bq = (T*)std::aligned_alloc(64, bytes);
+        std::memcpy(xq, xp.data(), (size_t)P.tot * sizeof(T));
+        std::memcpy(bq, bp.data(), (size_t)P.tot * sizeof(T));
+        PkArgs<T> q = a;
+        q.x = xq;
+        q.b = bq;
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int hq = 0; hq < n0 / 8; ++hq) pk_relax4<T, MODE == kPkZero ? kPkZero : kPkPlain>(q, color, hq, j, k);
+    }
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
             for (int h = 0; h < n0 / 2; ++h) {
@@ -167,6 +185,12 @@ static void run_half(int n0, int n1, int n2, const int lo[3], const int hi[3], c
             }
     if (bad) fail<T>("half sweep", n0, n1, n2, MODE, color, bad);
     ++g_cases;
+    if (quads) {
+        if (std::memcmp(xq, xp.data(), (size_t)P.tot * sizeof(T)) != 0) fail<T>("four-cell half sweep", n0, n1, n2, MODE, color, 1);
+        ++g_cases;
+        std::free(xq);
+        std::free(bq);
+    }
 }
 
 // the first post-smoothing sweep in two forms: on-the-fly prolongation
@@ -296,6 +320,12 @@ int main()
     run_all<float>(6, 10, 6, mixlo, mixhi, r222, 71u);
     run_all<double>(2, 2, 2, allp, allp, r222, 81u);
     run_all<double>(4, 18, 8, abl, abl, r221, 91u);
+    // rows of 4 and 8 packed cells with x walls and x periodic (four-cell form)
+    run_all<double>(16, 6, 5, wall_lo, wall_hi, r221, 101u);
+    run_all<float>(16, 6, 5, wall_lo, wall_hi, r221, 111u);
+    run_all<double>(8, 10, 6, mixlo, mixhi, r222, 121u);
+    run_all<float>(16, 10, 6, mixhi, mixlo, r222, 131u);
+    run_all<float>(24, 8, 4, allp, allp, r112, 141u);
     std::printf("ok %d\n", g_cases);
     return 0;
 }
