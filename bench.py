@@ -303,11 +303,11 @@ def pmc_summary(fdir, wdir, cfdir, cwdir, n=(256, 256, 256), kz=4):
 # outer residual starts at level 0, a restriction moves one level down, the
 # first launch of a level's post-smoothing (prolongation) returns to the level
 # it came from.
-_MG_DOWN = ("k_resid_restrict_rt", "k_restrict")
-_MG_UP = ("k_pk_prolong_color", "k_pk_prolong_add", "k_mg_prolong_red", "k_mg_prolong_add")
-_MG_RESET = ("k_update_residual",)
-_MG_KERNELS = ("k_pk_half", "k_mg_smooth", "k_mg_ghosts", "k_fill", "k_coarse_vcycle") + _MG_DOWN + _MG_UP + \
-    _MG_RESET
+_MG_DOWN = ("k_resid_restrict_rt", "k_resid_restrict4", "k_restrict")
+_MG_UP = ("k_pk_prolong_color", "k_pk_prolong_color4", "k_pk_prolong_add", "k_mg_prolong_red", "k_mg_prolong_add")
+_MG_RESET = ("k_update_residual", "k_update_residual4")
+_MG_KERNELS = ("k_pk_half", "k_pk_half4", "k_mg_smooth", "k_mg_ghosts", "k_fill", "k_coarse_vcycle") + _MG_DOWN + \
+    _MG_UP + _MG_RESET
 
 
 def _short(name):

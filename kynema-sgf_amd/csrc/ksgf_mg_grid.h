@@ -207,19 +207,26 @@ template <typename T>
 struct alignas(4 * sizeof(T)) PkQuad {
     T v[4];
 };
+// two consecutive packed values of one colour (an aligned pair of a quad's
+// worth of cells: four consecutive cells of a row hold two of each colour)
+template <typename T>
+struct alignas(2 * sizeof(T)) PkPair {
+    T v[2];
+};
 
 // pk_relax for the four colour-`color` cells at packed positions 4 hq ..
-// 4 hq + 3 of row (j, k), plain or zero-initial-guess modes (row length
-// prow a multiple of 4): the same value per cell, in the same operand order.
+// 4 hq + 3 of row (j, k) (row length prow a multiple of 4), plain, zero
+// initial guess or wall prolongation modes: the same value per cell, in the
+// same operand order.
 // The four cells share the row's parity, so their x neighbours are the
 // other colour's quad at the same positions plus one value beyond it (before
 // it on even rows, after it on odd ones); y and z neighbours are quads of
 // the neighbouring rows and planes.  Only the first and the last quad of a
 // row can touch an x side.
-template <typename T, int MODE>
+template <typename T, int MODE, bool CP>
 KSGF_HD void pk_relax4(const PkArgs<T>& a, int color, int hq, int j, int k)
 {
-    static_assert(MODE == kPkPlain || MODE == kPkZero, "plain and zero-guess sweeps only");
+    static_assert(MODE != kPkProlong, "the on-the-fly prolongation has no four-cell form");
     const LevelGeom& L = a.L;
     const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz, idg = (T)L.idg;
     const int h0 = 4 * hq;
@@ -241,7 +248,13 @@ KSGF_HD void pk_relax4(const PkArgs<T>& a, int color, int hq, int j, int k)
     const bool in_s = j > 0 || py, in_n = j < L.n1 - 1 || py;
     const bool in_b = k > 0 || pz, in_t = k < L.n2 - 1 || pz;
     PkQuad<T> self{};
-    if (MODE != kPkZero && !(in_w && in_e && in_s && in_n && in_b && in_t)) self = *(const PkQuad<T>*)(a.x + c);
+    if (MODE != kPkZero && !(in_w && in_e && in_s && in_n && in_b && in_t)) {
+        self = *(const PkQuad<T>*)(a.x + c);
+        if (MODE == kPkProlongWalls)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                self.v[m] = self.v[m] + a.xc[LvIdx<CP>(a.C, (2 * (h0 + m) + par) >> a.s0, j >> a.s1, k >> a.s2)];
+    }
     const T* ob = a.x + (1 - color) * L.pcol;
     const PkQuad<T> o = *(const PkQuad<T>*)(ob + row + h0);
     // the x neighbour beyond the quad (wrapped at the row ends)
@@ -286,6 +299,20 @@ KSGF_HD void pk_prolong_color(const PkArgs<T>& a, int color, int h, int j, int k
     a.x[c] = a.x[c] + a.xc[LvIdx<CP>(a.C, i >> a.s0, j >> a.s1, k >> a.s2)];
 }
 
+// pk_prolong_color for the four cells at packed positions 4 hq .. 4 hq + 3
+template <typename T, bool CP>
+KSGF_HD void pk_prolong_color4(const PkArgs<T>& a, int color, int hq, int j, int k)
+{
+    const LevelGeom& L = a.L;
+    const int h0 = 4 * hq, par = (j + k + color) & 1;
+    const int c = color * L.pcol + k * L.pplane + j * L.prow + h0;
+    PkQuad<T> x = *(const PkQuad<T>*)(a.x + c);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        x.v[m] = x.v[m] + a.xc[LvIdx<CP>(a.C, (2 * (h0 + m) + par) >> a.s0, j >> a.s1, k >> a.s2)];
+    *(PkQuad<T>*)(a.x + c) = x;
+}
+
 // residual of the fine children of coarse cell (I, J, K) of a packed fine
 // level, averaged: sum of (b - (stencil(x) - diag*x)) times w, the oracle's
 // summation order (i fastest); walls mirror the cell, periodic sides wrap
@@ -319,6 +346,61 @@ KSGF_HD T pk_restrict_cell(const T* x, const T* b, const LevelGeom& F, int I, in
                 s += v;
             }
     return s * w;
+}
+
+// pk_restrict_cell for the four coarse cells I0 .. I0 + 3 of row (J, K)
+// when the x ratio is 1 (semicoarsening in y / z; n0 a multiple of 4): the
+// children of the four are the same four-cell runs of the fine rows, read as
+// colour pairs, with single loads for the x neighbours beyond a run; the
+// same per-child values, summed in the same order
+template <typename T>
+KSGF_HD void pk_restrict_quad(const T* x, const T* b, const LevelGeom& F, int I0, int J, int K, int r1, int r2, T w,
+                              T* out)
+{
+    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
+    const bool px = F.lo[0] == KSGF_LO_PERIODIC, py = F.lo[1] == KSGF_LO_PERIODIC, pz = F.lo[2] == KSGF_LO_PERIODIC;
+    const int h0 = I0 >> 1;
+    // the four cells i = I0 .. I0 + 3 of row (j, k) of array a
+    auto run = [&](const T* a, int j, int k, T* v) {
+        const int rp = (I0 + j + k) & 1, row = k * F.pplane + j * F.prow + h0;
+        const PkPair<T> p = *(const PkPair<T>*)(a + rp * F.pcol + row);
+        const PkPair<T> q = *(const PkPair<T>*)(a + (1 - rp) * F.pcol + row);
+        v[0] = p.v[0];
+        v[1] = q.v[0];
+        v[2] = p.v[1];
+        v[3] = q.v[1];
+    };
+    const bool in_w = I0 > 0 || px, in_e = I0 + 4 < F.n0 || px;
+    const int iw = I0 > 0 ? I0 - 1 : F.n0 - 1, ie = I0 + 4 < F.n0 ? I0 + 4 : 0;
+    T s[4] = {(T)0, (T)0, (T)0, (T)0};
+    for (int dk = 0; dk < r2; ++dk)
+        for (int dj = 0; dj < r1; ++dj) {
+            const int j = J * r1 + dj, k = K * r2 + dk;
+            T self[4], bb[4], vs[4], vn[4], vb[4], vt[4];
+            run(x, j, k, self);
+            run(b, j, k, bb);
+            const T W = in_w ? x[PIdx(F, iw, j, k)] : pk_wall<T>(F.lo[0]) * self[0];
+            const T E = in_e ? x[PIdx(F, ie, j, k)] : pk_wall<T>(F.hi[0]) * self[3];
+            auto side = [&](bool in, int jj, int kk, int bc, T* v) {
+                if (in) {
+                    run(x, jj, kk, v);
+                } else {
+                    for (int m = 0; m < 4; ++m) v[m] = pk_wall<T>(bc) * self[m];
+                }
+            };
+            side(j > 0 || py, j > 0 ? j - 1 : F.n1 - 1, k, F.lo[1], vs);
+            side(j < F.n1 - 1 || py, j < F.n1 - 1 ? j + 1 : 0, k, F.hi[1], vn);
+            side(k > 0 || pz, j, k > 0 ? k - 1 : F.n2 - 1, F.lo[2], vb);
+            side(k < F.n2 - 1 || pz, j, k < F.n2 - 1 ? k + 1 : 0, F.hi[2], vt);
+            for (int m = 0; m < 4; ++m) {
+                const T vw = m == 0 ? W : self[m - 1];
+                const T ve = m == 3 ? E : self[m + 1];
+                const T st = cx * (vw + ve) + cy * (vs[m] + vn[m]) + cz * (vb[m] + vt[m]);
+                const T v = bb[m] - (st - dg * self[m]);
+                s[m] += v;
+            }
+        }
+    for (int m = 0; m < 4; ++m) out[m] = s[m] * w;
 }
 
 // residual of the fine children of coarse cell (I, J, K) of a natural fine

@@ -261,9 +261,10 @@ __global__ void __launch_bounds__(256) k_pk_half(PkArgs<T> a, int color, RowTile
     for (int k = k0; k < k1; ++k) pk_relax<T, MODE, CP>(a, color, h, j, k);
 }
 
-// the same half sweep (plain or zero-guess mode) four cells per thread: one
-// thread per quad of a packed row (rows of a multiple of 4 cells), 256
-// consecutive (quad, row) pairs of one plane chunk per workgroup, kz planes
+// the same half sweep (plain, zero-guess or wall-prolongation mode) and the
+// colour prolongation four cells per thread: one thread per quad of a packed
+// row (rows of a multiple of 4 cells), 256 consecutive (quad, row) pairs of
+// one plane chunk per workgroup, kz planes
 struct QuadTiles {
     int nq, nb, kz;  // quads per row, workgroups per plane chunk, planes per workgroup
 };
@@ -275,17 +276,33 @@ inline QuadTiles quad_tiles(const LevelGeom& L, int kz)
     t.kz = kz;
     return t;
 }
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_pk_half4(PkArgs<T> a, int color, QuadTiles t)
+__device__ __forceinline__ bool quad_tile(const QuadTiles& t, int n1, int n2, int& hq, int& j, int& k0, int& k1)
 {
-    if (a.L.act && *a.L.act == 0) return;
     const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int kc = lid / t.nb, wb = lid - kc * t.nb;
     const unsigned idx = (unsigned)wb * 256u + threadIdx.x;
-    const unsigned j = idx / (unsigned)t.nq, hq = idx - j * (unsigned)t.nq;
-    if ((int)j >= a.L.n1) return;
-    const int k0 = kc * t.kz, k1 = min(k0 + t.kz, a.L.n2);
-    for (int k = k0; k < k1; ++k) pk_relax4<T, MODE>(a, color, (int)hq, (int)j, k);
+    const unsigned jj = idx / (unsigned)t.nq;
+    hq = (int)(idx - jj * (unsigned)t.nq);
+    j = (int)jj;
+    k0 = kc * t.kz;
+    k1 = min(k0 + t.kz, n2);
+    return j < n1;
+}
+template <typename T, int MODE, bool CP>
+__global__ void __launch_bounds__(256) k_pk_half4(PkArgs<T> a, int color, QuadTiles t)
+{
+    if (a.L.act && *a.L.act == 0) return;
+    int hq, j, k0, k1;
+    if (!quad_tile(t, a.L.n1, a.L.n2, hq, j, k0, k1)) return;
+    for (int k = k0; k < k1; ++k) pk_relax4<T, MODE, CP>(a, color, hq, j, k);
+}
+template <typename T, bool CP>
+__global__ void __launch_bounds__(256) k_pk_prolong_color4(PkArgs<T> a, int color, QuadTiles t)
+{
+    if (a.L.act && *a.L.act == 0) return;
+    int hq, j, k0, k1;
+    if (!quad_tile(t, a.L.n1, a.L.n2, hq, j, k0, k1)) return;
+    for (int k = k0; k < k1; ++k) pk_prolong_color4<T, CP>(a, color, hq, j, k);
 }
 
 // x += coarse correction on the colour-`color` cells of a packed level
@@ -333,6 +350,31 @@ __global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__
                 return stencil<T>(x, c, i, j, k, F, cx, cy, cz, self);
             };
             cres[LIdx(C, I, J, K)] = restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w, st);
+        }
+    }
+}
+
+// the same over quads of coarse cells when the x ratio is 1 (n0 a multiple
+// of 4, packed fine level): four coarse cells I0 .. I0 + 3 per thread
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_restrict4(const T* __restrict__ x, const T* __restrict__ b,
+                                                         T* __restrict__ cres, LevelGeom F, LevelGeom C, RowTiles t,
+                                                         int r1, int r2, T w)
+{
+    if (F.act && *F.act == 0) return;
+    int IQ, J, K0, K1;
+    row_tile(t, xcd_remap((int)blockIdx.x, (int)gridDim.x), (int)threadIdx.x, IQ, J, K0, K1, C.n2);
+    const int I0 = 4 * IQ;
+    if (I0 >= C.n0 || J >= C.n1) return;
+    for (int K = K0; K < K1; ++K) {
+        T q[4];
+        pk_restrict_quad<T>(x, b, F, I0, J, K, r1, r2, w, q);
+        if (C.packed) {
+            const int rp = (I0 + J + K) & 1, row = K * C.pplane + J * C.prow + (I0 >> 1);
+            *(PkPair<T>*)(cres + rp * C.pcol + row) = PkPair<T>{{q[0], q[2]}};
+            *(PkPair<T>*)(cres + (1 - rp) * C.pcol + row) = PkPair<T>{{q[1], q[3]}};
+        } else {
+            *(PkQuad<T>*)(cres + LIdx(C, I0, J, K)) = PkQuad<T>{{q[0], q[1], q[2], q[3]}};
         }
     }
 }
@@ -462,6 +504,108 @@ __device__ __forceinline__ double update_residual_tile(const double* __restrict_
         }
     }
     return m;
+}
+
+// The same residual four cells per thread (x extent a multiple of 4, no
+// explicit ghosts): a thread marches the column of cells 4 iq .. 4 iq + 3 of
+// row j.  phi, rhs and phi_out move as quads; a packed correction and
+// residual as two pairs per quad (cells 0 and 2 have the row's colour, 1 and
+// 3 the other, at the same two packed positions); the x neighbours beyond
+// the quad are single loads.  Every cell's value, operand order and the
+// block maxima are those of update_residual_tile.
+template <typename C, typename R, bool CP>
+__device__ __forceinline__ double update_residual_quads(const double* __restrict__ phi, const C* __restrict__ cor,
+                                                        double* __restrict__ phi_out, const double* __restrict__ rhs,
+                                                        R* __restrict__ r, const LevelGeom& P, const LevelGeom& L,
+                                                        const RowTiles& tl, int lid)
+{
+    int iq, j, k0, k1;
+    row_tile(tl, lid, (int)threadIdx.x, iq, j, k0, k1, P.n2);
+    const int i0 = 4 * iq;
+    double m = 0.0;
+    if (i0 >= P.n0 || j >= P.n1) return m;
+    const bool px = P.lo[0] == KSGF_LO_PERIODIC, py = P.lo[1] == KSGF_LO_PERIODIC, pz = P.lo[2] == KSGF_LO_PERIODIC;
+    const bool in_w = i0 > 0 || px, in_e = i0 + 4 < P.n0 || px, in_s = j > 0 || py, in_n = j < P.n1 - 1 || py;
+    const int js = j > 0 ? j - 1 : P.n1 - 1, jn = j < P.n1 - 1 ? j + 1 : 0;
+    const int iw = i0 > 0 ? i0 - 1 : P.n0 - 1, ie = i0 + 4 < P.n0 ? i0 + 4 : 0;
+    // phi + cor of cells i0 .. i0 + 3 of row (jj, kk)
+    auto quad_at = [&](int jj, int kk) {
+        const int c = (int)LIdx(P, i0, jj, kk);
+        PkQuad<double> x = *(const PkQuad<double>*)(phi + c);
+        if (cor) {
+            if (CP) {
+                const int rp = (i0 + jj + kk) & 1, row = kk * L.pplane + jj * L.prow + (i0 >> 1);
+                const PkPair<C> a = *(const PkPair<C>*)(cor + rp * L.pcol + row);
+                const PkPair<C> b = *(const PkPair<C>*)(cor + (1 - rp) * L.pcol + row);
+                x.v[0] = x.v[0] + (double)a.v[0];
+                x.v[1] = x.v[1] + (double)b.v[0];
+                x.v[2] = x.v[2] + (double)a.v[1];
+                x.v[3] = x.v[3] + (double)b.v[1];
+            } else {
+                const PkQuad<C> q = *(const PkQuad<C>*)(cor + c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x.v[e] = x.v[e] + (double)q.v[e];
+            }
+        }
+        return x;
+    };
+    // phi + cor of the single cell (ii, jj, kk)
+    auto one_at = [&](int ii, int jj, int kk) {
+        const int c = (int)LIdx(P, ii, jj, kk);
+        double x = phi[c];
+        if (cor) x = x + (double)cor[CP ? (int)PIdx(L, ii, jj, kk) : c];
+        return x;
+    };
+    PkQuad<double> vm{}, v0 = quad_at(j, k0), vp{};
+    if (k0 > 0 || pz) vm = quad_at(j, k0 > 0 ? k0 - 1 : P.n2 - 1);
+    for (int k = k0; k < k1; ++k) {
+        if (k < P.n2 - 1 || pz) vp = quad_at(j, k < P.n2 - 1 ? k + 1 : 0);
+        const int c = (int)LIdx(P, i0, j, k);
+        const PkQuad<double> self = v0;
+        auto wall = [&](int bc, int e) { return bc == KSGF_LO_NEUMANN ? self.v[e] : -self.v[e]; };
+        const double wb = in_w ? one_at(iw, j, k) : wall(P.lo[0], 0);
+        const double eb = in_e ? one_at(ie, j, k) : wall(P.hi[0], 3);
+        PkQuad<double> sq, nq;
+        if (in_s) sq = quad_at(js, k);
+        if (in_n) nq = quad_at(jn, k);
+        const PkQuad<double> rq = *(const PkQuad<double>*)(rhs + c);
+        PkQuad<double> v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double w = e == 0 ? wb : self.v[e - 1];
+            const double ea = e == 3 ? eb : self.v[e + 1];
+            const double sv = in_s ? sq.v[e] : wall(P.lo[1], e);
+            const double nv = in_n ? nq.v[e] : wall(P.hi[1], e);
+            const double bo = (k > 0 || pz) ? vm.v[e] : wall(P.lo[2], e);
+            const double t = (k < P.n2 - 1 || pz) ? vp.v[e] : wall(P.hi[2], e);
+            const double st = P.cx * (w + ea) + P.cy * (sv + nv) + P.cz * (bo + t);
+            v.v[e] = rq.v[e] - (st - P.dg * self.v[e]);
+            m = fmax(m, abs_or_inf(v.v[e]));
+        }
+        if (cor) *(PkQuad<double>*)(phi_out + c) = self;
+        if (CP) {
+            const int rp = (i0 + j + k) & 1, row = k * L.pplane + j * L.prow + (i0 >> 1);
+            *(PkPair<R>*)(r + rp * L.pcol + row) = PkPair<R>{{(R)v.v[0], (R)v.v[2]}};
+            *(PkPair<R>*)(r + (1 - rp) * L.pcol + row) = PkPair<R>{{(R)v.v[1], (R)v.v[3]}};
+        } else {
+            *(PkQuad<R>*)(r + c) = PkQuad<R>{{(R)v.v[0], (R)v.v[1], (R)v.v[2], (R)v.v[3]}};
+        }
+        vm = v0;
+        v0 = vp;
+    }
+    return m;
+}
+
+template <typename C, typename R, bool CP>
+__global__ void __launch_bounds__(256) k_update_residual4(const double* __restrict__ phi, const C* __restrict__ cor,
+                                                          double* __restrict__ phi_out, const double* __restrict__ rhs,
+                                                          R* __restrict__ r, LevelGeom P, LevelGeom L, RowTiles tl,
+                                                          double* __restrict__ part)
+{
+    if (P.act && *P.act == 0) return;
+    const double m = update_residual_quads<C, R, CP>(phi, cor, phi_out, rhs, r, P, L, tl,
+                                                     xcd_remap((int)blockIdx.x, (int)gridDim.x));
+    block_max_part(m, part);
 }
 
 template <typename C, typename R, bool CP>
@@ -1406,13 +1550,17 @@ static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hi
                              hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const dim3 blk(256);
-    if ((mode == kPkPlain || mode == kPkZero) && a.L.prow % 4 == 0) {
+    if (mode != kPkProlong && a.L.prow % 4 == 0) {
         const QuadTiles q = quad_tiles(a.L, kz);
         const dim3 gq(q.nb * ((a.L.n2 + kz - 1) / kz));
         if (mode == kPkZero)
-            hipExtLaunchKernelGGL((k_pk_half4<T, kPkZero>), gq, blk, 0, st, e0, e1, 0, a, color, q);
+            hipExtLaunchKernelGGL((k_pk_half4<T, kPkZero, false>), gq, blk, 0, st, e0, e1, 0, a, color, q);
+        else if (mode == kPkProlongWalls && a.C.packed)
+            hipExtLaunchKernelGGL((k_pk_half4<T, kPkProlongWalls, true>), gq, blk, 0, st, e0, e1, 0, a, color, q);
+        else if (mode == kPkProlongWalls)
+            hipExtLaunchKernelGGL((k_pk_half4<T, kPkProlongWalls, false>), gq, blk, 0, st, e0, e1, 0, a, color, q);
         else
-            hipExtLaunchKernelGGL((k_pk_half4<T, kPkPlain>), gq, blk, 0, st, e0, e1, 0, a, color, q);
+            hipExtLaunchKernelGGL((k_pk_half4<T, kPkPlain, false>), gq, blk, 0, st, e0, e1, 0, a, color, q);
         return;
     }
     const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, kz);
@@ -1441,11 +1589,23 @@ static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b,
         // its own; the red half then relaxes against plain neighbours (a
         // prolongation inside the red half's six neighbour reads was latency
         // bound at twice a plain half's time), the black half is plain
-        const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, mp->kz_sweep[l]);
-        if (a.C.packed)
-            hipLaunchKernelGGL((k_pk_prolong_color<T, true>), dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, a, 1, t);
-        else
-            hipLaunchKernelGGL((k_pk_prolong_color<T, false>), dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, a, 1, t);
+        const int kz = mp->kz_sweep[l];
+        if (a.L.prow % 4 == 0) {
+            const QuadTiles q = quad_tiles(a.L, kz);
+            const dim3 gq(q.nb * ((a.L.n2 + kz - 1) / kz));
+            if (a.C.packed)
+                hipLaunchKernelGGL((k_pk_prolong_color4<T, true>), gq, dim3(256), 0, st, a, 1, q);
+            else
+                hipLaunchKernelGGL((k_pk_prolong_color4<T, false>), gq, dim3(256), 0, st, a, 1, q);
+        } else {
+            const RowTiles t = row_tiles(a.L.prow, a.L.n1, a.L.n2, kz);
+            if (a.C.packed)
+                hipLaunchKernelGGL((k_pk_prolong_color<T, true>), dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, a, 1,
+                                   t);
+            else
+                hipLaunchKernelGGL((k_pk_prolong_color<T, false>), dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, a,
+                                   1, t);
+        }
     }
     for (int color = 0; color < 2; ++color) {
         const int m = (mode == kPkProlong) ? (color == 0 ? kPkProlongWalls : kPkPlain) : mode;
@@ -1518,6 +1678,35 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
     return best;
 }
 
+// the FP64 outer residual without explicit ghosts at kz planes per thread,
+// four cells per thread where the x extent allows; returns the number of
+// workgroups (block maxima in mp->rpart)
+template <typename C, typename R>
+static int launch_ures(ksgf_macproj* mp, const C* cor, R* r, int kz, hipStream_t st)
+{
+    const LevelGeom& P = mp->P;
+    const LevelGeom& L0 = mp->G[0];
+    const bool quads = P.n0 % 4 == 0;
+    const RowTiles t = row_tiles(quads ? P.n0 / 4 : P.n0, P.n1, P.n2, kz);
+    const int nb = t.ntx * t.nty * t.nkc;
+    const dim3 g(nb), blk(256);
+    const double* phi = mp->phi;
+    const double* rhs = mp->rhs;
+    if (quads && L0.packed)
+        hipLaunchKernelGGL((k_update_residual4<C, R, true>), g, blk, 0, st, phi, cor, mp->phi2, rhs, r, P, L0, t,
+                           mp->rpart);
+    else if (quads)
+        hipLaunchKernelGGL((k_update_residual4<C, R, false>), g, blk, 0, st, phi, cor, mp->phi2, rhs, r, P, L0, t,
+                           mp->rpart);
+    else if (L0.packed)
+        hipLaunchKernelGGL((k_update_residual<C, R, true>), g, blk, 0, st, phi, cor, mp->phi2, rhs, r, P, L0, t,
+                           mp->rpart);
+    else
+        hipLaunchKernelGGL((k_update_residual<C, R, false>), g, blk, 0, st, phi, cor, mp->phi2, rhs, r, P, L0, t,
+                           mp->rpart);
+    return nb;
+}
+
 // Autotuner of the planes per thread of the FP64 outer residual (the
 // z-march keeps a column's planes in registers: more planes fetch less halo,
 // fewer planes give more threads in flight).  Each candidate is timed as the
@@ -1545,16 +1734,8 @@ static int tune_residual(ksgf_macproj* mp)
     float best_ms = 1e30f;
     for (int kz : {1, 2, 4, 8, 16, 32, 64}) {
         if (kz < kResidualKzMin || kz > std::max(P.n2, kResidualKzMin)) continue;
-        const RowTiles t = row_tiles(P.n0, P.n1, P.n2, kz);
-        const int nb = t.ntx * t.nty * t.nkc;
-        const dim3 g(nb);
         auto launch = [&]() {
-            if (mp->G[0].packed)
-                hipLaunchKernelGGL((k_update_residual<T, T, true>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
-                                   cor, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
-            else
-                hipLaunchKernelGGL((k_update_residual<T, T, false>), g, dim3(256), 0, nullptr, (const double*)mp->phi,
-                                   cor, mp->phi2, (const double*)mp->rhs, r, P, mp->G[0], t, mp->rpart);
+            const int nb = launch_ures<T, T>(mp, cor, r, kz, nullptr);
             hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, nullptr, (const double*)mp->rpart, nb,
                                mp->dnorm, (const int*)nullptr);
         };
@@ -1586,6 +1767,12 @@ static void launch_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* 
     const LevelGeom& C = mp->G[l + 1];
     const int* r = &mp->ratio[3 * l];
     const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+    if (r[0] == 1 && L.packed && L.n0 % 4 == 0) {
+        const RowTiles t = row_tiles(C.n0 / 4, C.n1, C.n2, kz);
+        hipLaunchKernelGGL(k_resid_restrict4<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, L, C, t,
+                           r[1], r[2], w);
+        return;
+    }
     const RowTiles t = row_tiles(C.n0, C.n1, C.n2, kz);
     hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, L, C, t, r[0],
                        r[1], r[2], w);
@@ -1697,7 +1884,7 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
     const LevelGeom& P = mp->P;
     const LevelGeom& L0 = mp->G[0];
     const RowTiles t = row_tiles(P.n0, P.n1, P.n2, mp->kz_res);
-    const int nb = t.ntx * t.nty * t.nkc;
+    int nb = t.ntx * t.nty * t.nkc;
     if (P.ghost_mode) {
         // explicit ghosts: phi += cor first, then the residual from ghosts
         if (cor) {
@@ -1709,12 +1896,7 @@ static void launch_update_residual(ksgf_macproj* mp, const C* cor, R* r, hipStre
         hipLaunchKernelGGL((k_update_residual<C, R, false>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
                            (const C*)nullptr, (double*)nullptr, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
     } else {
-        if (L0.packed)
-            hipLaunchKernelGGL((k_update_residual<C, R, true>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi, cor,
-                               mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
-        else
-            hipLaunchKernelGGL((k_update_residual<C, R, false>), dim3(nb), dim3(256), 0, st, (const double*)mp->phi,
-                               cor, mp->phi2, (const double*)mp->rhs, r, P, L0, t, mp->rpart);
+        nb = launch_ures<C, R>(mp, cor, r, mp->kz_res, st);
         if (cor) std::swap(mp->phi, mp->phi2);
     }
     hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(kMaxPartsThreads), 0, st, (const double*)mp->rpart, nb, mp->dnorm,

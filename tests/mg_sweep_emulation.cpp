@@ -161,7 +161,8 @@ static void run_half(int n0, int n1, int n2, const int lo[3], const int hi[3], c
         q.b = bq;
         for (int k = 0; k < n2; ++k)
             for (int j = 0; j < n1; ++j)
-                for (int hq = 0; hq < n0 / 8; ++hq) pk_relax4<T, MODE == kPkZero ? kPkZero : kPkPlain>(q, color, hq, j, k);
+                for (int hq = 0; hq < n0 / 8; ++hq)
+                    pk_relax4<T, MODE == kPkZero ? kPkZero : kPkPlain, false>(q, color, hq, j, k);
     }
     for (int k = 0; k < n2; ++k)
         for (int j = 0; j < n1; ++j)
@@ -218,6 +219,7 @@ static void run_split_prolong(int n0, int n1, int n2, const int lo[3], const int
         for (int j = 0; j < C.n1; ++j)
             for (int i = 0; i < C.n0; ++i) xc[AnyIdx(C, i, j, k)] = (T)U(rng);
     std::vector<T> x2 = x1;
+    const std::vector<T> x0 = x1;
     auto args = [&](std::vector<T>& x) {
         PkArgs<T> a{};
         a.x = x.data();
@@ -249,6 +251,28 @@ static void run_split_prolong(int n0, int n1, int n2, const int lo[3], const int
                 const T g = x2[PIdx(P, i, j, k)], r = x1[PIdx(P, i, j, k)];
                 if (std::memcmp(&g, &r, sizeof(T)) != 0) ++bad;
             }
+    // the split sweep in the four-cell form, on 64-byte aligned copies
+    if (n0 % 8 == 0) {
+        const size_t bytes = ((size_t)P.tot * sizeof(T) + 63) / 64 * 64;
+        T* xq = (T*)std::aligned_alloc(64, bytes);
+        T* bq = (T*)std::aligned_alloc(64, bytes);
+        std::memcpy(xq, x0.data(), (size_t)P.tot * sizeof(T));
+        std::memcpy(bq, bp.data(), (size_t)P.tot * sizeof(T));
+        PkArgs<T> q = a2;
+        q.x = xq;
+        q.b = bq;
+        auto quads = [&](auto&& f) {
+            for (int k = 0; k < n2; ++k)
+                for (int j = 0; j < n1; ++j)
+                    for (int hq = 0; hq < n0 / 8; ++hq) f(hq, j, k);
+        };
+        quads([&](int h, int j, int k) { cp ? pk_prolong_color4<T, true>(q, 1, h, j, k) : pk_prolong_color4<T, false>(q, 1, h, j, k); });
+        quads([&](int h, int j, int k) { cp ? pk_relax4<T, kPkProlongWalls, true>(q, 0, h, j, k) : pk_relax4<T, kPkProlongWalls, false>(q, 0, h, j, k); });
+        quads([&](int h, int j, int k) { pk_relax4<T, kPkPlain, false>(q, 1, h, j, k); });
+        if (std::memcmp(xq, x2.data(), (size_t)P.tot * sizeof(T)) != 0) ++bad;
+        std::free(xq);
+        std::free(bq);
+    }
     if (bad) fail<T>("split prolongation sweep", n0, n1, n2, kPkProlongWalls, -1, bad);
     ++g_cases;
 }
@@ -273,6 +297,12 @@ static void run_restrict(int n0, int n1, int n2, const int lo[3], const int hi[3
                 bp[PIdx(P, i, j, k)] = b;
             }
     fill_ghosts(xn, N);
+    // 64-byte aligned copies of the packed arrays for the four-cell form
+    const size_t pbytes = ((size_t)P.tot * sizeof(T) + 63) / 64 * 64;
+    T* xa = (T*)std::aligned_alloc(64, pbytes);
+    T* ba = (T*)std::aligned_alloc(64, pbytes);
+    std::memcpy(xa, xp.data(), (size_t)P.tot * sizeof(T));
+    std::memcpy(ba, bp.data(), (size_t)P.tot * sizeof(T));
     const T cx = (T)N.cx, cy = (T)N.cy, cz = (T)N.cz;
     auto st = [&](long long c, int, int, int, T) {
         return cx * (xn[c - 1] + xn[c + 1]) + cy * (xn[c - N.sy] + xn[c + N.sy]) + cz * (xn[c - N.sz] + xn[c + N.sz]);
@@ -285,7 +315,18 @@ static void run_restrict(int n0, int n1, int n2, const int lo[3], const int hi[3
                 const T g = pk_restrict_cell<T>(xp.data(), bp.data(), P, I, J, K, ratio[0], ratio[1], ratio[2], w);
                 const T r = restrict_cell<T>(xn.data(), bn.data(), N, I, J, K, ratio[0], ratio[1], ratio[2], w, st);
                 if (std::memcmp(&g, &r, sizeof(T)) != 0) ++bad;
+                // the four-cell form (x ratio 1) at the first cell of each quad
+                if (ratio[0] == 1 && n0 % 4 == 0 && I % 4 == 0) {
+                    T q[4];
+                    pk_restrict_quad<T>(xa, ba, P, I, J, K, ratio[1], ratio[2], w, q);
+                    for (int m = 0; m < 4; ++m) {
+                        const T gm = pk_restrict_cell<T>(xp.data(), bp.data(), P, I + m, J, K, 1, ratio[1], ratio[2], w);
+                        if (std::memcmp(&q[m], &gm, sizeof(T)) != 0) ++bad;
+                    }
+                }
             }
+    std::free(xa);
+    std::free(ba);
     if (bad) fail<T>("restriction", n0, n1, n2, -1, -1, bad);
     ++g_cases;
 }
@@ -326,6 +367,11 @@ int main()
     run_all<double>(8, 10, 6, mixlo, mixhi, r222, 121u);
     run_all<float>(16, 10, 6, mixhi, mixlo, r222, 131u);
     run_all<float>(24, 8, 4, allp, allp, r112, 141u);
+    // semicoarsened restriction of rows of 4 and 8 cells next to x walls
+    const int r121[3] = {1, 2, 1};
+    run_all<double>(16, 6, 6, wall_lo, wall_hi, r112, 151u);
+    run_all<float>(8, 10, 6, mixlo, mixhi, r112, 161u);
+    run_all<double>(8, 10, 6, mixhi, mixlo, r121, 171u);
     std::printf("ok %d\n", g_cases);
     return 0;
 }
