@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) k_faceA(CMFDev q, CMFDev qbc, CMFDev f, i
             hiv = hivD;
             flo_(i, j, k, n) = lov;
             fhi_(i, j, k, n) = hiv;
-            fed_(i, j, k, n) = upwind(lov, hiv, u);
+            if (fed.p) fed_(i, j, k, n) = upwind(lov, hiv, u);
             continue;
         }
         face_states(n, lov, hiv);
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_faceA(CMFDev q, CMFDev qbc, CMFDev f, i
         edge_bcs(x, g.dlo[D], g.dhi[D], bc.lo[n][D], bc.hi[n][D], glo, ghi, both, lov, hiv);
         flo_(i, j, k, n) = lov;
         fhi_(i, j, k, n) = hiv;
-        fed_(i, j, k, n) = upwind(lov, hiv, u);
+        if (fed.p) fed_(i, j, k, n) = upwind(lov, hiv, u);
     }
 }
 
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(256) k_faceA_march(CMFDev q, CMFDev qbc, CMFDe
             const int o = op + t * ost;
             flo_.p[o + n * flo_.cs] = lov;
             fhi_.p[o + n * fhi_.cs] = hiv;
-            fed_.p[o + n * fed_.cs] = upwind(lov, hiv, u);
+            if (fed.p) fed_.p[o + n * fed_.cs] = upwind(lov, hiv, u);
             // shift: p becomes the next face's q
             smq = smp;
             spq = spp;
@@ -568,12 +568,14 @@ struct CfTile {
 struct CfArgs {
     CMFDev q, qbc, f, divu;
     int has_divu, ncomp;
-    CMF3 lo, hi, ed;   // stage A: face states and upwinded states (faces grown by 1)
+    CMF3 lo, hi;       // stage A: face states of x and y faces (faces grown by 1); z unused
     CMF3 vel;          // advecting velocities: MAC (fluxes) or Burgers (VEL), ngrow 1
     MF3 state, flux;   // outputs (fluxes); state[d] = u_mac[d] (VEL)
     double c3[3], c6[3], c2[3], c4[3], area[3];
     double hdt;
+    double dtdx2;  // stage A of the z faces: dt / dz
     int uft, is_velocity, area_weighted, allow_inflow, cons_mask;
+    int lim;  // stage A of the z faces: PPM limiter
 };
 
 // the corner arithmetic of k_corner: low / high side states of the D face
@@ -715,6 +717,7 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
                                                       GeomDev g)
 {
     constexpr int NS = VEL ? 2 : 1;  // components of each upwinded state the corners read
+    constexpr int NC = VEL ? 3 : 1;  // components the workgroup forms states of
     constexpr int NP = kCfThreads;  // LDS slots: one per thread
     // inputs of plane k & 1: upwinded T states, advecting velocities, cell values
     __shared__ double s_ed[3][NS][2][NP];
@@ -725,7 +728,7 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
     __shared__ double s_cz[2][NP];
     // forcing of planes k & 1 (components x, y, z for VEL); the finals read
     // their cells and the low neighbours from here
-    __shared__ double s_f[VEL ? 3 : 1][2][NP];
+    __shared__ double s_f[NC][2][NP];
     // consecutive tiles (x fastest within a box row) on one XCD, so the rows
     // tiles share (halos, cache lines across tile edges) meet in its L2
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -750,12 +753,12 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
         cs1[d] = ps1[d] * m2;
         o1[d] = (i - l0 + 1) + m0 * (j - l1 + 1);  // + ps1 * (k - l2 + 1)
     }
-    const double *lo_p[3], *hi_p[3], *ed_p[3], *vel_p[3];
+    // stage-A face states and advecting velocities
+    const double *lo_p[3], *hi_p[3], *vel_p[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         lo_p[d] = a.lo.m[d].p + (long long)a.lo.m[d].ncomp * a.lo.m[d].pre[b];
         hi_p[d] = a.hi.m[d].p + (long long)a.hi.m[d].ncomp * a.hi.m[d].pre[b];
-        ed_p[d] = a.ed.m[d].p + (long long)a.ed.m[d].ncomp * a.ed.m[d].pre[b];
         vel_p[d] = a.vel.m[d].p + a.vel.m[d].pre[b];
     }
     // cell arrays: q (ngrow 3), f and divu (ngrow 1)
@@ -806,132 +809,108 @@ __global__ void __launch_bounds__(256) k_corner_final(CfArgs a, CfTile tl, const
     for (int d = 0; d < 3; ++d)
         okf[d] = i >= l0 - 1 && i <= h0 + 1 + (d == 0) && j >= l1 - 1 && j <= h1 + 1 + (d == 1);
     const bool q_ok = i >= l0 - 3 && i <= h0 + 3 && j >= l1 - 3 && j <= h1 + 3;
+    const bool f_ok = i >= l0 - 1 && i <= h0 + 1 && j >= l1 - 1 && j <= h1 + 1;
     const bool cons = !VEL && ((a.cons_mask >> n) & 1);
     const Bc1 bcn = bc_of(bc, n);
     Bc1 bcv[3];
     if (VEL)
 #pragma unroll
         for (int d = 0; d < 3; ++d) bcv[d] = bc_of(bc, d);
-    // own-position inputs of plane kk (prefetched one plane ahead)
-    double p_ed[3][NS], p_vel[3], p_q = 0.0, p_f[VEL ? 3 : 1];
-    const bool f_ok = i >= l0 - 1 && i <= h0 + 1 && j >= l1 - 1 && j <= h1 + 1;
+    // own-position inputs of plane kk, prefetched one plane ahead: the
+    // stage-A states of the x, y and z faces of every component the
+    // workgroup forms (the upwinded states are formed from them here),
+    // advecting velocities, forcing and (momentum) the cell value
+    double p_lx[NC], p_hx[NC], p_ly[NC], p_hy[NC], p_lz[NC], p_hz[NC], p_vel[3], p_f[NC], p_q = 0.0;
     auto load_plane = [&](int kk) {
         const int kz = kk - l2 + 1;
+        const bool okx = okf[0] && kk <= h2 + 1, oky = okf[1] && kk <= h2 + 1;
+        const bool okz = okf[2] && kk >= l2 && kk <= h2 + 1;
 #pragma unroll
-        for (int T = 0; T < 3; ++T) {
-            const bool ok = okf[T] && (kk <= h2 + 1 + (T == 2));
-            const int o = o1[T] + ps1[T] * kz;
-#pragma unroll
-            for (int sl = 0; sl < NS; ++sl) {
-                const int cT = VEL ? (T + 2 - sl) % 3 : n;
-                p_ed[T][sl] = ok ? ed_p[T][o + cT * cs1[T]] : 0.0;
-            }
-            p_vel[T] = ok ? vel_p[T][o] : 0.0;
+        for (int c = 0; c < NC; ++c) {
+            const int cc = VEL ? c : n;
+            const int ox = o1[0] + ps1[0] * kz + cc * cs1[0], oy = o1[1] + ps1[1] * kz + cc * cs1[1];
+            const int oz = o1[2] + ps1[2] * kz + cc * cs1[2];
+            p_lx[c] = okx ? lo_p[0][ox] : 0.0;
+            p_hx[c] = okx ? hi_p[0][ox] : 0.0;
+            p_ly[c] = oky ? lo_p[1][oy] : 0.0;
+            p_hy[c] = oky ? hi_p[1][oy] : 0.0;
+            p_lz[c] = okz ? lo_p[2][oz] : 0.0;
+            p_hz[c] = okz ? hi_p[2][oz] : 0.0;
+            p_f[c] = (!a.uft && f_ok) ? f_p[of + psf * kz + cc * csf] : 0.0;
         }
         if (!VEL) p_q = q_ok ? q_p[oq + psq * (kk - l2 + 3) + n * csq] : 0.0;
 #pragma unroll
-        for (int c = 0; c < (VEL ? 3 : 1); ++c)
-            p_f[c] = (!a.uft && f_ok) ? f_p[of + psf * (kk - l2 + 1) + (VEL ? c : n) * csf] : 0.0;
-    };
-    // momentum: the x- and y-face states of plane k - 1 are those this
-    // thread loaded at the plane before (carried in registers)
-    const bool lx_load = m_xy || m_xz || (inner && ri_x && rj_c);
-    const bool ly_load = m_yx || m_yz || (inner && ri_c && rj_y);
-    // the stage-A face states this thread's corners and finals read at
-    // plane kk (the x and y ones corrected along z at plane kk - 1), loaded
-    // one plane ahead like the staged inputs
-    struct FaceIn {
-        double lxy, hxy, lyx, hyx, lzx, hzx, lzy, hzy, lfz, hfz, lxz, hxz, lfx, hfx, lyz, hyz, lfy, hfy;
-    };
-    constexpr int cxy = VEL ? 2 : 0, cyx = VEL ? 2 : 0, czx = VEL ? 1 : 0, czy = 0, cxz = VEL ? 1 : 0, cyz = 0,
-                  cfx = 0, cfy = VEL ? 1 : 0, cfz = VEL ? 2 : 0;
-    const int nn = VEL ? 0 : n;
-    auto load_faces = [&](int kk, FaceIn& F) {
-        const int kz = kk - l2 + 1;
-        const bool rz = (kk >= l2 && kk <= h2 + 1), rc1 = (kk - 1 >= l2 && kk - 1 <= h2);
-        const bool zl = (m_zx || m_zy || (inner && ri_c && rj_c)) && rz;
-        const bool xl = (m_xz || (inner && ri_x && rj_c)) && rc1, yl = (m_yz || (inner && ri_c && rj_y)) && rc1;
-        const int ox = o1[0] + ps1[0] * kz, oy = o1[1] + ps1[1] * kz, oz = o1[2] + ps1[2] * kz;
-        const int oxm = ox - ps1[0], oym = oy - ps1[1];
-        F = FaceIn{};
-        if (VEL ? m_xy : lx_load) {
-            F.lxy = lo_p[0][ox + (cxy + nn) * cs1[0]];
-            F.hxy = hi_p[0][ox + (cxy + nn) * cs1[0]];
-        }
-        if (VEL ? m_yx : ly_load) {
-            F.lyx = lo_p[1][oy + (cyx + nn) * cs1[1]];
-            F.hyx = hi_p[1][oy + (cyx + nn) * cs1[1]];
-        }
-        if (zl) {
-            F.lzx = lo_p[2][oz + (czx + nn) * cs1[2]];
-            F.hzx = hi_p[2][oz + (czx + nn) * cs1[2]];
-            if (VEL) {
-                F.lzy = lo_p[2][oz + czy * cs1[2]];
-                F.hzy = hi_p[2][oz + czy * cs1[2]];
-                F.lfz = lo_p[2][oz + cfz * cs1[2]];
-                F.hfz = hi_p[2][oz + cfz * cs1[2]];
-            }
-        }
-        if (VEL && xl) {
-            F.lxz = lo_p[0][oxm + (cxz + nn) * cs1[0]];
-            F.hxz = hi_p[0][oxm + (cxz + nn) * cs1[0]];
-            F.lfx = lo_p[0][oxm + cfx * cs1[0]];
-            F.hfx = hi_p[0][oxm + cfx * cs1[0]];
-        }
-        if (VEL && yl) {
-            F.lyz = lo_p[1][oym + (cyz + nn) * cs1[1]];
-            F.hyz = hi_p[1][oym + (cyz + nn) * cs1[1]];
-            F.lfy = lo_p[1][oym + cfy * cs1[1]];
-            F.hfy = hi_p[1][oym + cfy * cs1[1]];
+        for (int T = 0; T < 3; ++T) {
+            const bool ok = okf[T] && (kk <= h2 + 1 + (T == 2));
+            p_vel[T] = ok ? vel_p[T][o1[T] + ps1[T] * kz] : 0.0;
         }
     };
-    double cxl = 0.0, cxh = 0.0, cyl = 0.0, cyh = 0.0;
-    FaceIn Fn;
+    // x and y face states of plane k - 1: the finals and the corners along z
+    // read them one plane after they are loaded
+    double cxl[NC], cxh[NC], cyl[NC], cyh[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cxl[c] = cxh[c] = cyl[c] = cyh[c] = 0.0;
     load_plane(l2 - 1);
-    load_faces(l2 - 1, Fn);
     for (int k = l2 - 1; k <= h2 + 1; ++k) {
         const int s = k & 1, s1 = s ^ 1;
         const int kz1 = k - l2 + 1;  // plane index in storage with one ghost layer
         const bool rk_z = (k >= l2 && k <= h2 + 1), rk_c1 = (k - 1 >= l2 && k - 1 <= h2);
-        // stage plane k
+        // the own-position inputs of plane k
+        double lx[NC], hx[NC], ly[NC], hy[NC], lz[NC], hz[NC], fz[NC], vel[3];
 #pragma unroll
-        for (int T = 0; T < 3; ++T) {
-#pragma unroll
-            for (int sl = 0; sl < NS; ++sl) s_ed[T][sl][s][P] = p_ed[T][sl];
-            s_vel[T][s][P] = p_vel[T];
+        for (int c = 0; c < NC; ++c) {
+            lx[c] = p_lx[c];
+            hx[c] = p_hx[c];
+            ly[c] = p_ly[c];
+            hy[c] = p_hy[c];
+            lz[c] = p_lz[c];
+            hz[c] = p_hz[c];
+            fz[c] = p_f[c];
         }
-        if (!VEL) s_q[s][P] = p_q;
+        const double qk = p_q;
 #pragma unroll
-        for (int c = 0; c < (VEL ? 3 : 1); ++c) s_f[c][s][P] = p_f[c];
+        for (int T = 0; T < 3; ++T) vel[T] = p_vel[T];
+        // prefetch plane k + 1
+        if (k + 1 <= h2 + 1) load_plane(k + 1);
+        // stage plane k: upwinded states (VEL: the transverse components,
+        // x faces z, y; y faces x, z; z faces y, x), velocities, cells, forcing
+        if (VEL) {
+            s_ed[0][0][s][P] = upwind(lx[NC - 1], hx[NC - 1], vel[0]);
+            s_ed[0][NS - 1][s][P] = upwind(lx[NC > 1 ? 1 : 0], hx[NC > 1 ? 1 : 0], vel[0]);
+            s_ed[1][0][s][P] = upwind(ly[0], hy[0], vel[1]);
+            s_ed[1][NS - 1][s][P] = upwind(ly[NC - 1], hy[NC - 1], vel[1]);
+            s_ed[2][0][s][P] = upwind(lz[NC > 1 ? 1 : 0], hz[NC > 1 ? 1 : 0], vel[2]);
+            s_ed[2][NS - 1][s][P] = upwind(lz[0], hz[0], vel[2]);
+        } else {
+            s_ed[0][0][s][P] = upwind(lx[0], hx[0], vel[0]);
+            s_ed[1][0][s][P] = upwind(ly[0], hy[0], vel[1]);
+            s_ed[2][0][s][P] = upwind(lz[0], hz[0], vel[2]);
+            s_q[s][P] = qk;
+        }
+#pragma unroll
+        for (int T = 0; T < 3; ++T) s_vel[T][s][P] = vel[T];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s_f[c][s][P] = fz[c];
         const bool d_xy = m_xy, d_yx = m_yx, d_zx = m_zx && rk_z, d_zy = m_zy && rk_z, d_xz = m_xz && rk_c1,
                    d_yz = m_yz && rk_c1;
         const bool f_x = inner && rk_c1 && ri_x && rj_c, f_y = inner && rk_c1 && ri_c && rj_y,
                    f_z = inner && rk_z && ri_c && rj_c;
-        // the face states of plane k (loaded at the plane before)
-        const double lxy = Fn.lxy, hxy = Fn.hxy, lyx = Fn.lyx, hyx = Fn.hyx, lzx = Fn.lzx, hzx = Fn.hzx;
-        double lzy, hzy, lfz, hfz, lxz, hxz, lfx, hfx, lyz, hyz, lfy, hfy;
-        if (VEL) {
-            lzy = Fn.lzy, hzy = Fn.hzy, lfz = Fn.lfz, hfz = Fn.hfz;
-            lxz = Fn.lxz, hxz = Fn.hxz, lfx = Fn.lfx, hfx = Fn.hfx;
-            lyz = Fn.lyz, hyz = Fn.hyz, lfy = Fn.lfy, hfy = Fn.hfy;
-        } else {
-            lzy = lfz = lzx;
-            hzy = hfz = hzx;
-            lxz = lfx = cxl;
-            hxz = hfx = cxh;
-            lyz = lfy = cyl;
-            hyz = hfy = cyh;
-            cxl = lxy;
-            cxh = hxy;
-            cyl = lyx;
-            cyh = hyx;
+        // the face states the corners and finals read: own faces of plane k
+        // (z face k), the x and y faces of plane k - 1 carried over
+        constexpr int cxy = VEL ? 2 : 0, cyx = VEL ? 2 : 0, czx = VEL ? 1 : 0, czy = 0, cxz = VEL ? 1 : 0, cyz = 0,
+                      cfx = 0, cfy = VEL ? 1 : 0, cfz = VEL ? 2 : 0;
+        const double lxy = lx[cxy], hxy = hx[cxy], lyx = ly[cyx], hyx = hy[cyx], lzx = lz[czx], hzx = hz[czx];
+        const double lzy = lz[czy], hzy = hz[czy], lfz = lz[cfz], hfz = hz[cfz];
+        const double lxz = cxl[cxz], hxz = cxh[cxz], lfx = cxl[cfx], hfx = cxh[cfx];
+        const double lyz = cyl[cyz], hyz = cyh[cyz], lfy = cyl[cfy], hfy = cyh[cfy];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cxl[c] = lx[c];
+            cxh[c] = hx[c];
+            cyl[c] = ly[c];
+            cyh[c] = hy[c];
         }
         const int kf = k - l2 + 1;  // plane k in the ngrow-1 cell storage of f / divu
-        // prefetch the stage inputs and face states of plane k + 1
-        if (k + 1 <= h2 + 1) {
-            load_plane(k + 1);
-            load_faces(k + 1, Fn);
-        }
         __syncthreads();
         double lov, hiv;
         // xy (x face, corrected along y, plane k); VEL: component z (slot 1)
@@ -1323,9 +1302,11 @@ static int launch_corner_final(const ksgf_boxarray* ba, const CfArgs& a, const B
 template <int D>
 static int launch_faceA(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f, int ncomp, CMFDev macD,
                         const Stages& S, const BCs& bc, const GeomDev& g, double dtdx, double hdt, int uft,
-                        int vel_mode, int is_velocity, int lim, hipStream_t st)
+                        int vel_mode, int is_velocity, int lim, bool write_ed, hipStream_t st)
 {
     int err = KSGF_OK;
+    MFDev ed = S.ed[D];
+    if (!write_ed) ed.p = nullptr;
     // marching along y and z: consecutive lanes are consecutive x columns, so
     // loads and stores stay coalesced.  Along x the lanes of a march would sit
     // kMarch faces apart (measured 3.5x slower), so x keeps one face per thread.
@@ -1339,7 +1320,7 @@ static int launch_faceA(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f,
         sd.glo2 = (D == 2) ? 0 : 1;
         sd.ghi0 = sd.ghi1 = sd.ghi2 = 1;
         hipLaunchKernelGGL(k_faceA_march<D>, dim3(r->nchunks), dim3(256), 0, st, q, qbc, f, ncomp, macD, S.lo[D],
-                           S.hi[D], S.ed[D], S.uad[D], sd, ba->dbox, bc, g, dtdx, hdt, uft, vel_mode, is_velocity,
+                           S.hi[D], ed, S.uad[D], sd, ba->dbox, bc, g, dtdx, hdt, uft, vel_mode, is_velocity,
                            lim);
         KSGF_LAUNCH_CHECK();
         return KSGF_OK;
@@ -1352,7 +1333,7 @@ static int launch_faceA(const ksgf_boxarray* ba, CMFDev q, CMFDev qbc, CMFDev f,
     const Region* r = get_region(ba, glo, ghi, &err);
     if (!r) return err;
     hipLaunchKernelGGL(k_faceA<D>, dim3(r->nchunks), dim3(256), 0, st, q, qbc, f, ncomp, macD, S.lo[D],
-                       S.hi[D], S.ed[D], S.uad[D], regdev(r, ba->nbox), ba->dbox, bc, g, dtdx, hdt, uft, vel_mode,
+                       S.hi[D], ed, S.uad[D], regdev(r, ba->nbox), ba->dbox, bc, g, dtdx, hdt, uft, vel_mode,
                        is_velocity, lim);
     KSGF_LAUNCH_CHECK();
     return KSGF_OK;
@@ -1511,9 +1492,15 @@ int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const d
         c2[d] = dt / (2.0 * ba->dx[d]);
     }
     const double hdt = 0.5 * dt;
-    KSGF_TRY(launch_faceA<0>(ba, q, q, f, 3, q, S, bc, g, dtdx[0], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
-    KSGF_TRY(launch_faceA<1>(ba, q, q, f, 3, q, S, bc, g, dtdx[1], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
-    KSGF_TRY(launch_faceA<2>(ba, q, q, f, 3, q, S, bc, g, dtdx[2], hdt, use_forces_in_trans, 1, 1, limiter_type, st));
+    // stage A of the x and y faces (without the upwinded states when the
+    // fused corner + final kernel forms them, and the z faces, from the
+    // states); every stage of every direction for the staged kernels
+    KSGF_TRY(launch_faceA<0>(ba, q, q, f, 3, q, S, bc, g, dtdx[0], hdt, use_forces_in_trans, 1, 1, limiter_type,
+                             keep != nullptr, st));
+    KSGF_TRY(launch_faceA<1>(ba, q, q, f, 3, q, S, bc, g, dtdx[1], hdt, use_forces_in_trans, 1, 1, limiter_type,
+                             keep != nullptr, st));
+    KSGF_TRY(launch_faceA<2>(ba, q, q, f, 3, q, S, bc, g, dtdx[2], hdt, use_forces_in_trans, 1, 1, limiter_type,
+                             keep != nullptr, st));
     CMFDev uad[3];
     for (int d = 0; d < 3; ++d) uad[d] = ccast(S.uad[d]);
     if (!keep) {
@@ -1528,7 +1515,6 @@ int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const d
         for (int d = 0; d < 3; ++d) {
             a.lo.m[d] = ccast(S.lo[d]);
             a.hi.m[d] = ccast(S.hi[d]);
-            a.ed.m[d] = ccast(S.ed[d]);
             a.vel.m[d] = uad[d];
             a.state.m[d] = mac[d];
             a.flux.m[d] = mac[d];
@@ -1544,6 +1530,8 @@ int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const d
         a.area_weighted = 0;
         a.allow_inflow = allow_inflow_on_outflow;
         a.cons_mask = 0;
+        a.dtdx2 = dtdx[2];
+        a.lim = limiter_type;
         return launch_corner_final<true>(ba, a, bc, g, 3, st);
     }
     KSGF_TRY((launch_corner<0, 1>(ba, q, q, 3, uad[1], uad[0], S, bc, g, 0.0, c6[1], 1, 1, 0, st)));
@@ -1643,9 +1631,12 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
         c4[d] = dt / (4.0 * ba->dx[d]);
     }
     const double hdt = 0.5 * dt;
-    KSGF_TRY(launch_faceA<0>(ba, qv, qb, fv, ncomp, mac[0], S, bc, g, dtdx[0], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
-    KSGF_TRY(launch_faceA<1>(ba, qv, qb, fv, ncomp, mac[1], S, bc, g, dtdx[1], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
-    KSGF_TRY(launch_faceA<2>(ba, qv, qb, fv, ncomp, mac[2], S, bc, g, dtdx[2], hdt, use_forces_in_trans, 0, is_velocity, limiter_type, st));
+    KSGF_TRY(launch_faceA<0>(ba, qv, qb, fv, ncomp, mac[0], S, bc, g, dtdx[0], hdt, use_forces_in_trans, 0,
+                             is_velocity, limiter_type, keep != nullptr, st));
+    KSGF_TRY(launch_faceA<1>(ba, qv, qb, fv, ncomp, mac[1], S, bc, g, dtdx[1], hdt, use_forces_in_trans, 0,
+                             is_velocity, limiter_type, keep != nullptr, st));
+    KSGF_TRY(launch_faceA<2>(ba, qv, qb, fv, ncomp, mac[2], S, bc, g, dtdx[2], hdt, use_forces_in_trans, 0,
+                             is_velocity, limiter_type, keep != nullptr, st));
     if (!keep) {
         // corner and final states in one pass per component, the corner
         // states kept in LDS
@@ -1659,7 +1650,6 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
         for (int d = 0; d < 3; ++d) {
             a.lo.m[d] = ccast(S.lo[d]);
             a.hi.m[d] = ccast(S.hi[d]);
-            a.ed.m[d] = ccast(S.ed[d]);
             a.vel.m[d] = mac[d];
             a.state.m[d] = face[d];
             a.flux.m[d] = flux[d];
@@ -1675,6 +1665,8 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
         a.area_weighted = fluxes_are_area_weighted;
         a.allow_inflow = allow_inflow_on_outflow;
         a.cons_mask = cons_mask;
+        a.dtdx2 = dtdx[2];
+        a.lim = limiter_type;
         return launch_corner_final<false>(ba, a, bc, g, ncomp, st);
     }
     KSGF_TRY((launch_corner<0, 1>(ba, qv, qb, ncomp, mac[1], mac[0], S, bc, g, c3[1], c6[1], 0, is_velocity, cons_mask, st)));
