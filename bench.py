@@ -363,7 +363,7 @@ def _step_window(rows):
     predictor update (the state upload, the projector's autotuner and the
     final download fall outside)"""
     names = [r[1] for r in rows]
-    first = next((i for i, n in enumerate(names) if n == "k_cfl"), 0)
+    first = next((i for i, n in enumerate(names) if n.startswith("k_cfl")), 0)
     last = max((i for i, n in enumerate(names) if n == "k_predictor"), default=len(rows) - 1)
     return first, last + 1
 
@@ -374,6 +374,8 @@ def trace_summary(d, steps):
     rows = _trace_rows(d)
     a, b = _step_window(rows)
     rows = rows[a:b]
+    # the steps in the window: one predictor update each
+    steps = sum(1 for r in rows if r[1] == "k_predictor") or steps
     lv = _levels(rows)
     acc = collections.defaultdict(list)
     for r, l in zip(rows, lv):

@@ -1378,6 +1378,14 @@ struct ksgf_macproj {
     ksgf::MgCtl* hctl = nullptr;  // pinned
     int ncap = 0;                 // V-cycles the last begin enqueued
     double* phi_a = nullptr;      // the buffer a device-controlled solve leaves the iterate in
+    // the projection's solve replayed from a hipGraph (mg_solve_graphed): the
+    // device-controlled iterations captured once per key, the iteration
+    // count of the last solve sizing the next capture
+    hipGraphExec_t sgraph = nullptr;
+    hipStream_t cap_st = nullptr;  // capture stream (nothing runs on it)
+    double g_rtol = -1.0, g_atol = -1.0, g_beta = -1.0, g_alpha = -1.0;
+    int g_ncyc = 0, g_singular = -1;
+    int iters_hint = 0;
 };
 
 namespace ksgf {
@@ -2076,6 +2084,67 @@ static int mg_solve_finish(ksgf_macproj* mp, ksgf_mlmg_info* info, hipStream_t s
     return inf.converged ? KSGF_OK : KSGF_ERR_NOT_CONVERGED;
 }
 
+// mg_solve with its iterations replayed from a hipGraph: the first solve of
+// a projector runs mg_solve (host-tested) and records its iteration count m;
+// later solves replay mg_solve_begin's device-controlled iterations, m
+// rounded down to even of them (captured once per count, tolerances and
+// coefficients), and mg_solve_finish reads the state once and finishes an
+// unconverged solve with host-side tests.  Iterates, residuals and statistics
+// equal mg_solve's bit for bit; the host waits once per solve instead of once
+// per V-cycle, and the V-cycles' launches leave no gaps.
+static int mg_solve_graphed(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* info, hipStream_t st)
+{
+    ksgf_mlmg_info inf{};
+    const int ncyc = std::min(mp->iters_hint & ~1, mp->opt.maxiter & ~1);
+    if (ncyc < 2) {
+        const int r = mg_solve(mp, rtol, atol, &inf, st);
+        if (info) *info = inf;
+        if (r >= 0) mp->iters_hint = inf.iters;
+        return r;
+    }
+    if (!mp->sgraph || mp->g_ncyc != ncyc || mp->g_rtol != rtol || mp->g_atol != atol ||
+        mp->g_beta != mp->coef_beta || mp->g_alpha != mp->coef_alpha || mp->g_singular != mp->singular) {
+        if (mp->sgraph) {
+            (void)hipGraphExecDestroy(mp->sgraph);
+            mp->sgraph = nullptr;
+        }
+        if (!mp->cap_st && hipStreamCreateWithFlags(&mp->cap_st, hipStreamNonBlocking) != hipSuccess) {
+            mp->cap_st = nullptr;
+            set_error("MLMG: capture stream");
+            return KSGF_ERR_HIP;
+        }
+        KSGF_HIP_CHECK(hipStreamBeginCapture(mp->cap_st, hipStreamCaptureModeThreadLocal));
+        const int e = mg_solve_begin(mp, rtol, atol, ncyc, mp->cap_st);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(mp->cap_st, &g);
+        if (e != KSGF_OK || ee != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            if (e == KSGF_OK) set_error("MLMG: solve capture");
+            return e != KSGF_OK ? e : KSGF_ERR_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&mp->sgraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            mp->sgraph = nullptr;
+            set_error("MLMG: solve graph instantiate");
+            return KSGF_ERR_HIP;
+        }
+        mp->g_ncyc = ncyc;
+        mp->g_rtol = rtol;
+        mp->g_atol = atol;
+        mp->g_beta = mp->coef_beta;
+        mp->g_alpha = mp->coef_alpha;
+        mp->g_singular = mp->singular;
+    }
+    KSGF_HIP_CHECK(hipGraphLaunch(mp->sgraph, st));
+    mp->ncap = ncyc;
+    const int r = mg_solve_finish(mp, &inf, st);
+    launch_ghosts<double>(mp->phi, mp->P, st);  // as mg_solve leaves phi
+    if (info) *info = inf;
+    if (r >= 0) mp->iters_hint = inf.iters;
+    return r;
+}
+
 }  // namespace ksgf
 
 namespace ksgf {
@@ -2166,6 +2235,8 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     if (mp->hstats) (void)hipHostFree(mp->hstats);
     if (mp->ctl) (void)hipFree(mp->ctl);
     if (mp->hctl) (void)hipHostFree(mp->hctl);
+    if (mp->sgraph) (void)hipGraphExecDestroy(mp->sgraph);
+    if (mp->cap_st) (void)hipStreamDestroy(mp->cap_st);
     vb_destroy(mp->vb);
     delete mp;
     return KSGF_OK;
@@ -2448,7 +2519,7 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
                        regdev(rv, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], 1.0 / ba->dx[0],
                        1.0 / ba->dx[1], 1.0 / ba->dx[2]);
     KSGF_LAUNCH_CHECK();
-    int ret = mg_solve(mp, rtol, atol, info, st);
+    int ret = mg_solve_graphed(mp, rtol, atol, info, st);
     if (ret < 0) return ret;
     for (int d = 0; d < 3; ++d) {
         int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};

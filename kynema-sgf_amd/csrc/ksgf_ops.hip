@@ -13,30 +13,6 @@
 
 namespace ksgf {
 
-__global__ void __launch_bounds__(256) k_cfl(CMFDev v, RegDev r, const int* __restrict__ box, double dxi,
-                                             double dyi, double dzi, unsigned long long* __restrict__ out)
-{
-    __shared__ double sh[4];
-    double m = 0.0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < r.total; t += (long long)gridDim.x * blockDim.x) {
-        int b, i, j, k;
-        region_locate_flat(r, box, t, b, i, j, k);
-        CFab f = fab_at(v, box, b);
-        double a = fabs(f(i, j, k, 0)) * dxi;
-        double bb = fabs(f(i, j, k, 1)) * dyi;
-        double e = fabs(f(i, j, k, 2)) * dzi;
-        m = fmax(m, fmax(a, fmax(bb, e)));
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double q = sh[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) q = fmax(q, sh[w]);
-        atomicMax(out, (unsigned long long)__double_as_longlong(q));
-    }
-}
-
 // ---------------------------------------------------------------------------
 // ksgf_cfl_max_noalloc: the CFL reduction with its workspace in a file-scope
 // device array instead of library-allocated memory.  One partial maximum per
@@ -66,16 +42,31 @@ __global__ void __launch_bounds__(256) k_cfl_partial(CMFDev v, RegDev r, const i
     if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-// one workgroup: out = max(part[0 .. n))
-__global__ void __launch_bounds__(256) k_max_partials(const double* __restrict__ part, int n, double* __restrict__ out)
+// one workgroup of kMaxPartialsThreads: out = max(part[0 .. n)), eight
+// independent loads in flight per thread
+constexpr int kMaxPartialsThreads = 1024;
+__global__ void __launch_bounds__(kMaxPartialsThreads) k_max_partials(const double* __restrict__ part, int n,
+                                                                      double* __restrict__ out)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[kMaxPartialsThreads / 64];
     double m = 0.0;
-    for (int t = threadIdx.x; t < n; t += 256) m = fmax(m, part[t]);
+    int t = threadIdx.x;
+    for (; t + 7 * kMaxPartialsThreads < n; t += 8 * kMaxPartialsThreads) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[t + u * kMaxPartialsThreads];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
+    }
+    for (; t < n; t += kMaxPartialsThreads) m = fmax(m, part[t]);
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) *out = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    if (threadIdx.x == 0) {
+        double q = sh[0];
+        for (int w = 1; w < kMaxPartialsThreads / 64; ++w) q = fmax(q, sh[w]);
+        *out = q;
+    }
 }
 
 // device address of `scratch` on each device, found once per device with that
@@ -104,30 +95,27 @@ static double* noalloc_scratch(int* err)
     return addr[dev];
 }
 
-__global__ void __launch_bounds__(256) k_force_cfl(CMFDev s, CMFDev rho, RegDev r, const int* __restrict__ box,
-                                                   double dxi, double dyi, double dzi,
-                                                   unsigned long long* __restrict__ out)
+// the forcing CFL per 256-cell chunk (one workgroup per chunk of the valid
+// region: no box search), reduced by k_max_partials
+__global__ void __launch_bounds__(256) k_force_cfl_partial(CMFDev s, CMFDev rho, RegDev r, const int* __restrict__ box,
+                                                           double dxi, double dyi, double dzi,
+                                                           double* __restrict__ part)
 {
     __shared__ double sh[4];
+    int b, i, j, k;
     double m = 0.0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < r.total; t += (long long)gridDim.x * blockDim.x) {
-        int b, i, j, k;
-        region_locate_flat(r, box, t, b, i, j, k);
+    if (region_locate(r, box, b, i, j, k)) {
         CFab f = fab_at(s, box, b);
         const double rr = fab_at(rho, box, b)(i, j, k);
         double a = fabs(f(i, j, k, 0)) * dxi / rr;
         double bb = fabs(f(i, j, k, 1)) * dyi / rr;
         double e = fabs(f(i, j, k, 2)) * dzi / rr;
-        m = fmax(m, fmax(a, fmax(bb, e)));
+        m = fmax(a, fmax(bb, e));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double q = sh[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) q = fmax(q, sh[w]);
-        atomicMax(out, (unsigned long long)__double_as_longlong(q));
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
 // MultiFab::invert(numerator, nghost): a = numerator / a on every point of a region
@@ -529,23 +517,19 @@ int ksgf_cfl_max(const ksgf_boxarray* ba, const double* vel, int ngrow, double* 
     int z[3] = {0, 0, 0};
     const Region* r = get_region(ba, z, z, &err);
     if (!r) return err;
-    unsigned long long* d = nullptr;
-    double* h = nullptr;
-    int* scratch_err = &err;
-    d = (unsigned long long*)get_scratch(ba, 40, 1, scratch_err);
-    if (!d) return err;
-    KSGF_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
-    int nb = grid_for(r->total, 256);
-    if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, v, regdev(r, ba->nbox), ba->dbox,
-                       1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], d);
+    // one partial maximum per 256-cell chunk, then one workgroup reduces them
+    // (the maximum does not depend on the order)
+    double* part = get_scratch(ba, 43, (size_t)r->nchunks + 1, &err);
+    if (!part) return err;
+    if (r->nchunks > 0)
+        hipLaunchKernelGGL(k_cfl_partial, dim3(r->nchunks), dim3(256), 0, st, v, regdev(r, ba->nbox), ba->dbox,
+                           1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], part);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(kMaxPartialsThreads), 0, st, (const double*)part, r->nchunks,
+                       part + r->nchunks);
     KSGF_LAUNCH_CHECK();
-    unsigned long long bits = 0;
-    KSGF_HIP_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st));
+    double m = 0.0;
+    KSGF_HIP_CHECK(hipMemcpyAsync(&m, part + r->nchunks, sizeof(m), hipMemcpyDeviceToHost, st));
     KSGF_HIP_CHECK(hipStreamSynchronize(st));
-    double m;
-    std::memcpy(&m, &bits, sizeof(double));
-    (void)h;
     // ParReduce floor of -1 (incflo_compute_dt.cpp:86-90): an empty or zero
     // field reports 0 here, which the caller treats as "no constraint"
     *out = m;
@@ -661,7 +645,8 @@ int ksgf_cfl_max_noalloc(const ksgf_boxarray* ba, const double* vel, int ngrow, 
     hipLaunchKernelGGL(k_cfl_partial, dim3(r->nchunks), dim3(256), 0, st, v, regdev(r, ba->nbox), ba->dbox,
                        1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], part);
     KSGF_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, r->nchunks, part + kNoAllocPartials);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(kMaxPartialsThreads), 0, st, part, r->nchunks,
+                       part + kNoAllocPartials);
     KSGF_LAUNCH_CHECK();
     double m = 0.0;
     KSGF_HIP_CHECK(hipMemcpyAsync(&m, part + kNoAllocPartials, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -686,19 +671,17 @@ int ksgf_force_cfl_max(const ksgf_boxarray* ba, const double* src, int ng_src, c
     int z[3] = {0, 0, 0};
     const Region* r = get_region(ba, z, z, &err);
     if (!r) return err;
-    unsigned long long* d = (unsigned long long*)get_scratch(ba, 42, 1, &err);
-    if (!d) return err;
-    KSGF_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
-    int nb = grid_for(r->total, 256);
-    if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_force_cfl, dim3(nb), dim3(256), 0, st, s, rh, regdev(r, ba->nbox), ba->dbox,
-                       1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], d);
+    double* part = get_scratch(ba, 44, (size_t)r->nchunks + 1, &err);
+    if (!part) return err;
+    if (r->nchunks > 0)
+        hipLaunchKernelGGL(k_force_cfl_partial, dim3(r->nchunks), dim3(256), 0, st, s, rh, regdev(r, ba->nbox),
+                           ba->dbox, 1.0 / ba->dx[0], 1.0 / ba->dx[1], 1.0 / ba->dx[2], part);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(kMaxPartialsThreads), 0, st, (const double*)part, r->nchunks,
+                       part + r->nchunks);
     KSGF_LAUNCH_CHECK();
-    unsigned long long bits = 0;
-    KSGF_HIP_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st));
+    double m = 0.0;
+    KSGF_HIP_CHECK(hipMemcpyAsync(&m, part + r->nchunks, sizeof(m), hipMemcpyDeviceToHost, st));
     KSGF_HIP_CHECK(hipStreamSynchronize(st));
-    double m;
-    std::memcpy(&m, &bits, sizeof(double));
     *out = m;
     return KSGF_OK;
 }
