@@ -331,11 +331,14 @@ __global__ void __launch_bounds__(256) k_pk_prolong_add(T* __restrict__ x, const
 
 // residual + restriction over row tiles of the coarse level; a packed fine
 // level may restrict into a packed or a natural coarse level, a natural one
-// only into a natural one
+// only into a natural one.  cred (packed coarse level, optional): the coarse
+// correction, whose red cells get the red half of a zero-initial-guess
+// sweep, -res / diag, as pk_relax's zero mode forms it (so the coarse
+// level's first sweep starts with its black half)
 template <typename T>
 __global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__ x, const T* __restrict__ b,
-                                                           T* __restrict__ cres, LevelGeom F, LevelGeom C, RowTiles t,
-                                                           int r0, int r1, int r2, T w)
+                                                           T* __restrict__ cres, T* __restrict__ cred, LevelGeom F,
+                                                           LevelGeom C, RowTiles t, int r0, int r1, int r2, T w)
 {
     if (F.act && *F.act == 0) return;
     int I, J, K0, K1;
@@ -344,7 +347,10 @@ __global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__
     const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz;
     for (int K = K0; K < K1; ++K) {
         if (F.packed) {
-            cres[AnyIdx(C, I, J, K)] = pk_restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w);
+            const long long o = AnyIdx(C, I, J, K);
+            const T v = pk_restrict_cell<T>(x, b, F, I, J, K, r0, r1, r2, w);
+            cres[o] = v;
+            if (cred && ((I + J + K) & 1) == 0) cred[o] = ((T)0 - v) * (T)C.idg;
         } else {
             auto st = [&](long long c, int i, int j, int k, T self) {
                 return stencil<T>(x, c, i, j, k, F, cx, cy, cz, self);
@@ -358,8 +364,8 @@ __global__ void __launch_bounds__(256) k_resid_restrict_rt(const T* __restrict__
 // of 4, packed fine level): four coarse cells I0 .. I0 + 3 per thread
 template <typename T>
 __global__ void __launch_bounds__(256) k_resid_restrict4(const T* __restrict__ x, const T* __restrict__ b,
-                                                         T* __restrict__ cres, LevelGeom F, LevelGeom C, RowTiles t,
-                                                         int r1, int r2, T w)
+                                                         T* __restrict__ cres, T* __restrict__ cred, LevelGeom F,
+                                                         LevelGeom C, RowTiles t, int r1, int r2, T w)
 {
     if (F.act && *F.act == 0) return;
     int IQ, J, K0, K1;
@@ -373,6 +379,11 @@ __global__ void __launch_bounds__(256) k_resid_restrict4(const T* __restrict__ x
             const int rp = (I0 + J + K) & 1, row = K * C.pplane + J * C.prow + (I0 >> 1);
             *(PkPair<T>*)(cres + rp * C.pcol + row) = PkPair<T>{{q[0], q[2]}};
             *(PkPair<T>*)(cres + (1 - rp) * C.pcol + row) = PkPair<T>{{q[1], q[3]}};
+            if (cred) {
+                const T ra = rp == 0 ? q[0] : q[1], rb = rp == 0 ? q[2] : q[3];
+                const T idg = (T)C.idg;
+                *(PkPair<T>*)(cred + row) = PkPair<T>{{((T)0 - ra) * idg, ((T)0 - rb) * idg}};
+            }
         } else {
             *(PkQuad<T>*)(cres + LIdx(C, I0, J, K)) = PkQuad<T>{{q[0], q[1], q[2], q[3]}};
         }
@@ -1589,7 +1600,8 @@ static void launch_pk_kernel(const PkArgs<T>& a, int color, int mode, int kz, hi
 
 // one red-black sweep (red half, black half) of packed level l
 template <typename T>
-static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b, const T* xc, hipStream_t st)
+static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b, const T* xc, hipStream_t st,
+                            int first_color = 0)
 {
     const PkArgs<T> a = pk_args<T>(mp, l, x, b, xc);
     if (mode == kPkProlong) {
@@ -1615,7 +1627,7 @@ static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b,
                                    1, t);
         }
     }
-    for (int color = 0; color < 2; ++color) {
+    for (int color = first_color; color < 2; ++color) {
         const int m = (mode == kPkProlong) ? (color == 0 ? kPkProlongWalls : kPkPlain) : mode;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (l == 0 && g_prof.on) {
@@ -1769,21 +1781,33 @@ static int tune_residual(ksgf_macproj* mp)
 // and residual arrays into the next level's residual (all scratch at set-up),
 // cached per (fine shape, coarse shape, precision)
 template <typename T>
-static void launch_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* out, int kz, hipStream_t st)
+// cred: the coarse correction, when the restriction also writes the red
+// half of the coarse level's zero-initial-guess sweep (packed coarse level)
+static void launch_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* out, int kz, hipStream_t st,
+                            T* cred = nullptr)
 {
     const LevelGeom& L = mp->G[l];
     const LevelGeom& C = mp->G[l + 1];
     const int* r = &mp->ratio[3 * l];
     const T w = (T)(1.0 / (double)(r[0] * r[1] * r[2]));
+    if (!C.packed || !L.packed) cred = nullptr;
     if (r[0] == 1 && L.packed && L.n0 % 4 == 0) {
         const RowTiles t = row_tiles(C.n0 / 4, C.n1, C.n2, kz);
-        hipLaunchKernelGGL(k_resid_restrict4<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, L, C, t,
-                           r[1], r[2], w);
+        hipLaunchKernelGGL(k_resid_restrict4<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, cred, L, C,
+                           t, r[1], r[2], w);
         return;
     }
     const RowTiles t = row_tiles(C.n0, C.n1, C.n2, kz);
-    hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, L, C, t, r[0],
-                       r[1], r[2], w);
+    hipLaunchKernelGGL(k_resid_restrict_rt<T>, dim3(t.ntx * t.nty * t.nkc), dim3(256), 0, st, x, b, out, cred, L, C,
+                       t, r[0], r[1], r[2], w);
+}
+
+// whether the restriction into level l writes the red half of l's first
+// (zero-initial-guess) sweep: packed grid levels above the coarse kernel's
+template <typename T>
+static bool restrict_writes_red(const ksgf_macproj* mp, int l)
+{
+    return l >= 1 && l < mp->lc && mp->G[l].packed && mp->G[l - 1].packed && mp->opt.num_pre_smooth > 0;
 }
 
 template <typename T>
@@ -1842,8 +1866,10 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
     // down
     if (L.packed) {
         if (nu1 == 0) launch_fill<T>(cor[l], L.tot, (T)0, st);
+        // the restriction into this level has formed the red half of the first sweep
+        const int red_done = restrict_writes_red<T>(mp, l) ? 1 : 0;
         for (int s = 0; s < nu1; ++s)
-            launch_pk_sweep<T>(mp, l, s == 0 ? kPkZero : kPkPlain, cor[l], res[l], nullptr, st);
+            launch_pk_sweep<T>(mp, l, s == 0 ? kPkZero : kPkPlain, cor[l], res[l], nullptr, st, s == 0 ? red_done : 0);
     } else if (nu1 > 0 && !L.ghost_mode) {
         hipLaunchKernelGGL(k_mg_smooth_first<T>, dim3(grid_for(nf, 256)), dim3(256), 0, st, cor[l], (const T*)res[l], L);
         launch_half_sweeps<T>(cor[l], res[l], L, 1, nu1, st);
@@ -1852,7 +1878,8 @@ static void vcycle(ksgf_macproj* mp, int l, std::vector<T*>& cor, std::vector<T*
         launch_half_sweeps<T>(cor[l], res[l], L, 0, nu1, st);
     }
     if (L.ghost_mode) launch_ghosts<T>(cor[l], L, st);
-    launch_restrict<T>(mp, l, (const T*)cor[l], (const T*)res[l], res[l + 1], mp->kz_rst[l], st);
+    launch_restrict<T>(mp, l, (const T*)cor[l], (const T*)res[l], res[l + 1], mp->kz_rst[l], st,
+                       restrict_writes_red<T>(mp, l + 1) ? cor[l + 1] : nullptr);
     vcycle<T>(mp, l + 1, cor, res, dlp, st);
     // up
     if (L.packed) {
