@@ -85,11 +85,15 @@ def copy_probe():
 THETA_BC = [[0, 0, 2, 0, 0, 2]]  # potential temperature: periodic x/y, zero gradient (foextrap) at both walls
 
 
-def run_gpu(case, precision, steps, warmup, profile, theta=False):
+def run_gpu(case, precision, steps, warmup, profile, theta=False, prof_steps=2):
     """W untimed steps, then K steps bracketed by synchronisation, with an
     event between steps on the stream the host layer launches on (torch's
     current stream): the mean (ms) and the per-step times (step_ms).  With
-    theta the potential temperature is advected as a passive scalar."""
+    theta the potential temperature is advected as a passive scalar.  With
+    profile, prof_steps more steps run with kernel timing on (each level-0
+    half sweep between its own HIP events; the MAC solve then runs its
+    host-tested path instead of the graph replay the timed steps use): the
+    roofline's kernel time."""
     import torch
 
     ksgf = importlib.import_module("kynema-sgf_amd")
@@ -108,8 +112,6 @@ def run_gpu(case, precision, steps, warmup, profile, theta=False):
     stream = torch.cuda.current_stream()
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     infos = []
-    if profile:
-        ksgf.prof_reset()
     evs[0].record(stream)
     for s in range(steps):
         S.step()
@@ -118,7 +120,13 @@ def run_gpu(case, precision, steps, warmup, profile, theta=False):
     torch.cuda.synchronize()
     step_ms = [evs[s].elapsed_time(evs[s + 1]) for s in range(steps)]
     ms = evs[0].elapsed_time(evs[-1]) / steps
-    prof = ksgf.prof_read() if profile else None
+    prof = None
+    if profile:
+        ksgf.prof_reset()
+        for _ in range(prof_steps):
+            S.step()
+        torch.cuda.synchronize()
+        prof = ksgf.prof_read()
     # one more (untimed) step with the post-projection divergence check
     S.step(check_divergence=True)
     divmax = S.last_divmax
@@ -246,7 +254,7 @@ def vs_oracle(case, reps, vel_ref, phi_ref):
     return out
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_pk_half_abl256_mixed.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_kernels_abl256_mixed.txt")
 
 
 def _pmc_rows(d):
@@ -455,14 +463,22 @@ def pmc_kernels(fdir, wdir, sqdir, cfdir, cwdir, sq2dir=None):
 
 
 def pmc_traffic():
-    """HBM bytes per level-0 half-sweep launch from the committed summary, or None."""
+    """HBM bytes per level-0 half-sweep launch from the committed counter
+    summary (read + write MB of the level-0 k_pk_half4 rows, weighted by
+    their launches), or None."""
+    tot = n = 0.0
     try:
         for line in open(PMC_SUMMARY):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+            if line.startswith("k_pk_half4<"):
+                name_end = line.index(">") + 1
+                rest = line[name_end:].split()
+                if rest[0] == "0":
+                    calls, rd, wr = int(rest[1]), float(rest[2]), float(rest[3])
+                    tot += calls * (rd + wr) * 1e6
+                    n += calls
     except OSError:
-        pass
-    return None
+        return None
+    return tot / n if n else None
 
 
 def poisson_batch(steps, warmup):

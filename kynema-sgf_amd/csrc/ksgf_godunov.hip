@@ -568,14 +568,12 @@ struct CfTile {
 struct CfArgs {
     CMFDev q, qbc, f, divu;
     int has_divu, ncomp;
-    CMF3 lo, hi;       // stage A: face states of x and y faces (faces grown by 1); z unused
+    CMF3 lo, hi;       // stage A: face states (faces grown by 1)
     CMF3 vel;          // advecting velocities: MAC (fluxes) or Burgers (VEL), ngrow 1
     MF3 state, flux;   // outputs (fluxes); state[d] = u_mac[d] (VEL)
     double c3[3], c6[3], c2[3], c4[3], area[3];
     double hdt;
-    double dtdx2;  // stage A of the z faces: dt / dz
     int uft, is_velocity, area_weighted, allow_inflow, cons_mask;
-    int lim;  // stage A of the z faces: PPM limiter
 };
 
 // the corner arithmetic of k_corner: low / high side states of the D face
@@ -1530,8 +1528,6 @@ int extrap_vel_to_faces_keep(const ksgf_boxarray* ba, const double* vel, const d
         a.area_weighted = 0;
         a.allow_inflow = allow_inflow_on_outflow;
         a.cons_mask = 0;
-        a.dtdx2 = dtdx[2];
-        a.lim = limiter_type;
         return launch_corner_final<true>(ba, a, bc, g, 3, st);
     }
     KSGF_TRY((launch_corner<0, 1>(ba, q, q, 3, uad[1], uad[0], S, bc, g, 0.0, c6[1], 1, 1, 0, st)));
@@ -1665,8 +1661,6 @@ int compute_fluxes_keep(const ksgf_boxarray* ba, int ncomp, const double* q, con
         a.area_weighted = fluxes_are_area_weighted;
         a.allow_inflow = allow_inflow_on_outflow;
         a.cons_mask = cons_mask;
-        a.dtdx2 = dtdx[2];
-        a.lim = limiter_type;
         return launch_corner_final<false>(ba, a, bc, g, ncomp, st);
     }
     KSGF_TRY((launch_corner<0, 1>(ba, qv, qb, ncomp, mac[1], mac[0], S, bc, g, c3[1], c6[1], 0, is_velocity, cons_mask, st)));

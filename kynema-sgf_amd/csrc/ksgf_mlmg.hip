@@ -1641,7 +1641,9 @@ static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b,
             const double words = (m == kPkZero && color == 0) ? 2.0 * half : 3.0 * half;
             g_prof.launches += 1;
             g_prof.bytes += sizeof(T) * words;
-            g_prof.name = sizeof(T) == 8 ? "k_pk_half<double>(level 0)" : "k_pk_half<float>(level 0)";
+            const bool quad = (m != kPkProlong && a.L.prow % 4 == 0);
+            g_prof.name = std::string(quad ? "k_pk_half4" : "k_pk_half") + (sizeof(T) == 8 ? "<double>" : "<float>") +
+                          "(level 0)";
         }
     }
 }
@@ -2119,11 +2121,13 @@ static int mg_solve_finish(ksgf_macproj* mp, ksgf_mlmg_info* info, hipStream_t s
 // unconverged solve with host-side tests.  Iterates, residuals and statistics
 // equal mg_solve's bit for bit; the host waits once per solve instead of once
 // per V-cycle, and the V-cycles' launches leave no gaps.
+// With kernel timing on (ksgf_prof_reset) every solve runs mg_solve, whose
+// level-0 half sweeps each carry their own start / end events.
 static int mg_solve_graphed(ksgf_macproj* mp, double rtol, double atol, ksgf_mlmg_info* info, hipStream_t st)
 {
     ksgf_mlmg_info inf{};
     const int ncyc = std::min(mp->iters_hint & ~1, mp->opt.maxiter & ~1);
-    if (ncyc < 2) {
+    if (ncyc < 2 || g_prof.on) {
         const int r = mg_solve(mp, rtol, atol, &inf, st);
         if (info) *info = inf;
         if (r >= 0) mp->iters_hint = inf.iters;
