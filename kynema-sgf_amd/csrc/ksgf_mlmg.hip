@@ -31,6 +31,7 @@
 // residual of the FP64 solution (fused with adding the previous correction),
 // runs one V-cycle on it with FP32 smoothing and grid transfers (the bottom
 // BiCGStab stays FP64), and adds the FP32 correction to the FP64 solution.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -1390,12 +1391,18 @@ struct ksgf_macproj {
     int ncap = 0;                 // V-cycles the last begin enqueued
     double* phi_a = nullptr;      // the buffer a device-controlled solve leaves the iterate in
     // the projection's solve replayed from a hipGraph (mg_solve_graphed): the
-    // device-controlled iterations captured once per key, the iteration
-    // count of the last solve sizing the next capture
-    hipGraphExec_t sgraph = nullptr;
+    // device-controlled iterations captured once per key (a few kept: a
+    // caller may alternate tolerances), the iteration count of the last
+    // solve sizing the next capture
+    struct SolveGraph {
+        hipGraphExec_t exec = nullptr;
+        double rtol, atol, beta, alpha;
+        int ncyc, singular;
+        long long used;  // replay stamp (the least recently used is replaced)
+    };
+    std::vector<SolveGraph> sgraphs;
+    long long sg_clock = 0;
     hipStream_t cap_st = nullptr;  // capture stream (nothing runs on it)
-    double g_rtol = -1.0, g_atol = -1.0, g_beta = -1.0, g_alpha = -1.0;
-    int g_ncyc = 0, g_singular = -1;
     int iters_hint = 0;
 };
 
@@ -2133,12 +2140,13 @@ static int mg_solve_graphed(ksgf_macproj* mp, double rtol, double atol, ksgf_mlm
         if (r >= 0) mp->iters_hint = inf.iters;
         return r;
     }
-    if (!mp->sgraph || mp->g_ncyc != ncyc || mp->g_rtol != rtol || mp->g_atol != atol ||
-        mp->g_beta != mp->coef_beta || mp->g_alpha != mp->coef_alpha || mp->g_singular != mp->singular) {
-        if (mp->sgraph) {
-            (void)hipGraphExecDestroy(mp->sgraph);
-            mp->sgraph = nullptr;
-        }
+    constexpr size_t kMaxGraphs = 4;
+    ksgf_macproj::SolveGraph* sg = nullptr;
+    for (auto& g : mp->sgraphs)
+        if (g.ncyc == ncyc && g.rtol == rtol && g.atol == atol && g.beta == mp->coef_beta &&
+            g.alpha == mp->coef_alpha && g.singular == mp->singular)
+            sg = &g;
+    if (!sg) {
         if (!mp->cap_st && hipStreamCreateWithFlags(&mp->cap_st, hipStreamNonBlocking) != hipSuccess) {
             mp->cap_st = nullptr;
             set_error("MLMG: capture stream");
@@ -2153,21 +2161,24 @@ static int mg_solve_graphed(ksgf_macproj* mp, double rtol, double atol, ksgf_mlm
             if (e == KSGF_OK) set_error("MLMG: solve capture");
             return e != KSGF_OK ? e : KSGF_ERR_HIP;
         }
-        const hipError_t ei = hipGraphInstantiate(&mp->sgraph, g, nullptr, nullptr, 0);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
-            mp->sgraph = nullptr;
             set_error("MLMG: solve graph instantiate");
             return KSGF_ERR_HIP;
         }
-        mp->g_ncyc = ncyc;
-        mp->g_rtol = rtol;
-        mp->g_atol = atol;
-        mp->g_beta = mp->coef_beta;
-        mp->g_alpha = mp->coef_alpha;
-        mp->g_singular = mp->singular;
+        if (mp->sgraphs.size() >= kMaxGraphs) {
+            auto old = std::min_element(mp->sgraphs.begin(), mp->sgraphs.end(),
+                                        [](const auto& x, const auto& y) { return x.used < y.used; });
+            (void)hipGraphExecDestroy(old->exec);
+            mp->sgraphs.erase(old);
+        }
+        mp->sgraphs.push_back({exec, rtol, atol, mp->coef_beta, mp->coef_alpha, ncyc, mp->singular, 0});
+        sg = &mp->sgraphs.back();
     }
-    KSGF_HIP_CHECK(hipGraphLaunch(mp->sgraph, st));
+    sg->used = ++mp->sg_clock;
+    KSGF_HIP_CHECK(hipGraphLaunch(sg->exec, st));
     mp->ncap = ncyc;
     const int r = mg_solve_finish(mp, &inf, st);
     launch_ghosts<double>(mp->phi, mp->P, st);  // as mg_solve leaves phi
@@ -2266,7 +2277,8 @@ int ksgf_macproj_destroy(ksgf_macproj* mp)
     if (mp->hstats) (void)hipHostFree(mp->hstats);
     if (mp->ctl) (void)hipFree(mp->ctl);
     if (mp->hctl) (void)hipHostFree(mp->hctl);
-    if (mp->sgraph) (void)hipGraphExecDestroy(mp->sgraph);
+    for (auto& g : mp->sgraphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (mp->cap_st) (void)hipStreamDestroy(mp->cap_st);
     vb_destroy(mp->vb);
     delete mp;
