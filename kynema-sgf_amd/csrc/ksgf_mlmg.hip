@@ -746,12 +746,28 @@ struct LevelPtrs {
 // first level live only inside the kernel, so from the bottom up as many of
 // them as fit (and the bottom solver's vectors) are kept in LDS instead of
 // HBM: every pass of the coarse cycle ends in a workgroup barrier, and LDS
-// latency instead of L2 latency is what makes those passes cheap.
-constexpr int kLdsPoolBytes = 96 * 1024;
+// latency instead of L2 latency is what makes those passes cheap.  The
+// first level joins them when it fits too: its residual (written by the
+// grid-level restriction) is copied in at the start and its correction
+// (read by the grid-level prolongation) copied out at the end.
+constexpr int kLdsPoolBytes = 152 * 1024;
 
 __host__ __device__ inline int lds_align(long long bytes)
 {
     return (int)((bytes + 15) / 16 * 16);
+}
+
+// Geometry of a level held in the pool: LIdx's index function with rows of
+// n0 + 2 elements instead of the HBM arrays' 64-byte aligned pitch (LIdx
+// puts cell i = -1 at element kXOff - 1 of a row, so the slice starts
+// kXOff - 1 elements early and holds (n0+2)(n1+2)(n2+2) cells after that)
+__host__ __device__ inline LevelGeom lds_geom(const LevelGeom& G)
+{
+    LevelGeom g = G;
+    g.sy = G.n0 + 2;
+    g.sz = g.sy * (G.n1 + 2);
+    g.tot = (kXOff - 1) + g.sz * (G.n2 + 2);
+    return g;
 }
 
 // block-wide sum / maximum: a wavefront butterfly, then the per-wave
@@ -1126,7 +1142,25 @@ __device__ __forceinline__ T* lv_cor(const LevelPtrs<T>* lp, int l, char* pool)
 template <typename T>
 __device__ __forceinline__ T* lv_res(const LevelPtrs<T>* lp, int l, char* pool)
 {
-    return lp[l].lds >= 0 ? (T*)(pool + lp[l].lds + lds_align((long long)lp[l].G.tot * sizeof(T))) : lp[l].res;
+    return lp[l].lds >= 0 ? (T*)(pool + lp[l].lds + lds_align((long long)lds_geom(lp[l].G).tot * sizeof(T)))
+                          : lp[l].res;
+}
+template <typename T>
+__device__ __forceinline__ LevelGeom lv_geom(const LevelPtrs<T>* lp, int l)
+{
+    return lp[l].lds >= 0 ? lds_geom(lp[l].G) : lp[l].G;
+}
+// copy the valid cells of level l between its HBM array (geometry G) and its
+// pool slice (geometry lds_geom(G)); ends in a workgroup barrier
+template <typename T>
+__device__ void lv_stage(T* dst, const LevelGeom& Gd, const T* src, const LevelGeom& Gs)
+{
+    BLK_FOR(Gd, t_, c)
+    {
+        const int q = t_ / Gd.n0;
+        dst[c] = src[LIdx(Gs, t_ % Gd.n0, q % Gd.n1, q / Gd.n1)];
+    }
+    __syncthreads();
 }
 
 template <typename T>
@@ -1136,13 +1170,14 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
 {
     BottomWork w = wg;
     if (w.lds >= 0) {
-        const int vb = lds_align((long long)lp[nlev - 1].G.tot * sizeof(double));
+        const int vb = lds_align((long long)lv_geom(lp, nlev - 1).tot * sizeof(double));
         double** v[8] = {&w.r, &w.rh, &w.p, &w.v, &w.s, &w.t, &w.x, &w.b};
         for (int q = 0; q < 8; ++q) *v[q] = (double*)(pool + w.lds + q * vb);
     }
+    if (lp[lc].lds >= 0) lv_stage<T>(lv_res(lp, lc, pool), lv_geom(lp, lc), lp[lc].res, lp[lc].G);
     for (int l = lc; l < nlev - 1; ++l) {
-        const LevelGeom F = lp[l].G;
-        const LevelGeom C = lp[l + 1].G;
+        const LevelGeom F = lv_geom(lp, l);
+        const LevelGeom C = lv_geom(lp, l + 1);
         T* x = lv_cor(lp, l, pool);
         const T* b = lv_res(lp, l, pool);
         for (long long c = threadIdx.x; c < F.tot; c += blockDim.x) x[c] = (T)0;
@@ -1173,7 +1208,7 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
     }
     // bottom
     {
-        const LevelGeom Lb = lp[nlev - 1].G;
+        const LevelGeom Lb = lv_geom(lp, nlev - 1);
         T* xb = lv_cor(lp, nlev - 1, pool);
         T* bb = lv_res(lp, nlev - 1, pool);
         double* x64 = (sizeof(T) == sizeof(double)) ? (double*)xb : w.x;
@@ -1189,8 +1224,8 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
         }
     }
     for (int l = nlev - 2; l >= lc; --l) {
-        const LevelGeom F = lp[l].G;
-        const LevelGeom C = lp[l + 1].G;
+        const LevelGeom F = lv_geom(lp, l);
+        const LevelGeom C = lv_geom(lp, l + 1);
         T* x = lv_cor(lp, l, pool);
         const T* xc = lv_cor(lp, l + 1, pool);
         const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
@@ -1203,6 +1238,7 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
         __syncthreads();
         blk_smooth<T>(x, lv_res(lp, l, pool), F, nu2);
     }
+    if (lp[lc].lds >= 0) lv_stage<T>(lp[lc].cor, lp[lc].G, lv_cor(lp, lc, pool), lv_geom(lp, lc));
 }
 
 template <typename T>
@@ -1212,7 +1248,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const LevelPtr
                                                                   const int* __restrict__ act)
 {
     if (act && *act == 0) return;
-    __shared__ double sh[kCoarseThreads];
+    __shared__ double sh[kCoarseThreads / 64];
     __shared__ __align__(16) char pool[kLdsPoolBytes];
     coarse_vcycle_body<T>(lp, lc, nlev, w, singular, rtol, atol, maxiter, nub, nuf, nu1, nu2, sh, pool);
 }
@@ -1447,19 +1483,23 @@ static std::vector<std::array<double, 3>> level_h(const ksgf_macproj* mp)
 static std::vector<int> lds_layout(ksgf_macproj* mp, size_t word, int* bw_lds)
 {
     std::vector<int> off(mp->nlev, -1);
-    long long used = 0;
     *bw_lds = -1;
-    const long long bw = 8LL * lds_align(mp->G[mp->nlev - 1].tot * (long long)sizeof(double));
-    if (bw <= kLdsPoolBytes) {
+    const int nb = mp->nlev - 1;
+    auto need = [&](int l) { return 2LL * lds_align(lds_geom(mp->G[l]).tot * (long long)word); };
+    // the bottom solver's eight FP64 vectors take the bottom level's
+    // geometry: the pool's when the bottom level is in the pool as well
+    const long long bw_pool = 8LL * lds_align(lds_geom(mp->G[nb]).tot * (long long)sizeof(double));
+    if (bw_pool + need(nb) <= kLdsPoolBytes) {
         *bw_lds = 0;
-        used = bw;
+        long long used = bw_pool;
+        for (int l = nb; l >= mp->lc; --l) {
+            if (used + need(l) > kLdsPoolBytes) break;
+            off[l] = (int)used;
+            used += need(l);
+        }
+        return off;
     }
-    for (int l = mp->nlev - 1; l > mp->lc; --l) {
-        const long long need = 2LL * lds_align(mp->G[l].tot * (long long)word);
-        if (used + need > kLdsPoolBytes) break;
-        off[l] = (int)used;
-        used += need;
-    }
+    if (8LL * lds_align(mp->G[nb].tot * (long long)sizeof(double)) <= kLdsPoolBytes) *bw_lds = 0;
     return off;
 }
 
@@ -2950,7 +2990,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_batch(const BCoarse<T
                                                                  double rtol, double atol, int maxiter, int nub,
                                                                  int nuf, int nu1, int nu2)
 {
-    __shared__ double sh[kCoarseThreads];
+    __shared__ double sh[kCoarseThreads / 64];
     __shared__ __align__(16) char pool[kLdsPoolBytes];
     const BCoarse<T>& c = cb[blockIdx.x];
     if (!S.active[c.prob]) return;
