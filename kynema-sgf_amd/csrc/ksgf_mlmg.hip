@@ -1150,17 +1150,292 @@ __device__ __forceinline__ LevelGeom lv_geom(const LevelPtrs<T>* lp, int l)
 {
     return lp[l].lds >= 0 ? lds_geom(lp[l].G) : lp[l].G;
 }
-// copy the valid cells of level l between its HBM array (geometry G) and its
-// pool slice (geometry lds_geom(G)); ends in a workgroup barrier
+// copy the valid cells of level l between its HBM array (geometry Gs or Gd)
+// and its pool slice (the other geometry), one (i, j) column per thread and
+// all of a column's loads issued before its stores; ends in a workgroup
+// barrier
+constexpr int kStageBatch = 8;
 template <typename T>
 __device__ void lv_stage(T* dst, const LevelGeom& Gd, const T* src, const LevelGeom& Gs)
 {
-    BLK_FOR(Gd, t_, c)
-    {
-        const int q = t_ / Gd.n0;
-        dst[c] = src[LIdx(Gs, t_ % Gd.n0, q % Gd.n1, q / Gd.n1)];
+    const int ncol = Gd.n0 * Gd.n1, szd = (int)Gd.sz, szs = (int)Gs.sz;
+    for (int t = (int)threadIdx.x; t < ncol; t += (int)blockDim.x) {
+        const int j = t / Gd.n0, i = t - j * Gd.n0;
+        const int cd = (int)LIdx(Gd, i, j, 0), cs = (int)LIdx(Gs, i, j, 0);
+        for (int k0 = 0; k0 < Gd.n2; k0 += kStageBatch) {
+            T v[kStageBatch];
+#pragma unroll
+            for (int m = 0; m < kStageBatch; ++m) v[m] = src[cs + min(k0 + m, Gd.n2 - 1) * szs];
+#pragma unroll
+            for (int m = 0; m < kStageBatch; ++m)
+                if (k0 + m < Gd.n2) dst[cd + (k0 + m) * szd] = v[m];
+        }
     }
     __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Passes of a level held in the pool (lds_geom layout) with the domain sides
+// inside the stencil instead of a ghost pass before every half: a periodic
+// side wraps, a Neumann side reads the cell itself, a Dirichlet side the
+// cell itself negated -- the values bc_ghost puts in the ghost layer, and
+// exactly those: wherever a half sweep reads a wrapped neighbour, that
+// neighbour has the other colour (no periodic direction of odd extent), so
+// it is not written during the half.  Each thread takes (i, j) columns and
+// marches along k, so the x and y neighbour offsets are formed once per
+// column; the loads of kPoolBatch cells of a column (half as many in FP64)
+// are issued before any of their stores (a half sweep reads the other colour, its own cells and b;
+// the restriction and prolongation write no array they read).  The pointers
+// are formed from the pool itself, so the accesses are LDS instructions.
+// ---------------------------------------------------------------------------
+constexpr int kPoolBatch = 4;
+
+__device__ __forceinline__ bool pool_sweepable(const LevelGeom& G)
+{
+    const int n[3] = {G.n0, G.n1, G.n2};
+    for (int d = 0; d < 3; ++d)
+        if ((G.lo[d] == KSGF_LO_PERIODIC || G.hi[d] == KSGF_LO_PERIODIC) && (n[d] & 1)) return false;
+    return true;
+}
+
+// the domain sides of a pool level: the offset a boundary cell's low / high
+// neighbour is at (the periodic image, or the cell itself), and whether that
+// value is negated (Dirichlet)
+struct PoolSides {
+    int n[3], s[3], wlo[3], whi[3];
+    unsigned nlo, nhi;  // bit d: negate at the low / high side of direction d
+};
+__device__ __forceinline__ PoolSides pool_sides(const LevelGeom& F)
+{
+    PoolSides p;
+    p.n[0] = F.n0;
+    p.n[1] = F.n1;
+    p.n[2] = F.n2;
+    p.s[0] = 1;
+    p.s[1] = (int)F.sy;
+    p.s[2] = (int)F.sz;
+    p.nlo = p.nhi = 0u;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        p.wlo[d] = F.lo[d] == KSGF_LO_PERIODIC ? (p.n[d] - 1) * p.s[d] : 0;
+        p.whi[d] = F.hi[d] == KSGF_LO_PERIODIC ? -(p.n[d] - 1) * p.s[d] : 0;
+        if (F.lo[d] != KSGF_LO_PERIODIC && F.lo[d] != KSGF_LO_NEUMANN) p.nlo |= 1u << d;
+        if (F.hi[d] != KSGF_LO_PERIODIC && F.hi[d] != KSGF_LO_NEUMANN) p.nhi |= 1u << d;
+    }
+    return p;
+}
+
+// one (i, j) column: offset of cell k = 0, the relative offsets of the W, E,
+// S, N neighbours, and which of those are negated (bits 0..3)
+struct PoolCol {
+    int c0, o[4];
+    unsigned neg;
+};
+__device__ __forceinline__ PoolCol pool_col(const LevelGeom& F, const PoolSides& p, int i, int j)
+{
+    PoolCol q;
+    q.c0 = (int)LIdx(F, i, j, 0);
+    const bool wl = i == 0, wh = i == p.n[0] - 1, sl = j == 0, sh = j == p.n[1] - 1;
+    q.o[0] = wl ? p.wlo[0] : -1;
+    q.o[1] = wh ? p.whi[0] : 1;
+    q.o[2] = sl ? p.wlo[1] : -p.s[1];
+    q.o[3] = sh ? p.whi[1] : p.s[1];
+    q.neg = ((wl && (p.nlo & 1u)) ? 1u : 0u) | ((wh && (p.nhi & 1u)) ? 2u : 0u) |
+            ((sl && (p.nlo & 2u)) ? 4u : 0u) | ((sh && (p.nhi & 2u)) ? 8u : 0u);
+    return q;
+}
+
+// the six neighbours of cell k of a column, sides applied, summed as
+// cx*(W+E) + cy*(S+N) + cz*(B+T) (the oracle's operand order)
+template <typename T>
+struct PoolNb6 {
+    T v[6];
+    unsigned neg;
+};
+template <typename T>
+__device__ __forceinline__ PoolNb6<T> pool_load_nb(const T* x, const PoolCol& q, const PoolSides& p, int c, int k)
+{
+    PoolNb6<T> r;
+    const bool bl = k == 0, bh = k == p.n[2] - 1;
+    r.v[0] = x[c + q.o[0]];
+    r.v[1] = x[c + q.o[1]];
+    r.v[2] = x[c + q.o[2]];
+    r.v[3] = x[c + q.o[3]];
+    r.v[4] = x[c + (bl ? p.wlo[2] : -p.s[2])];
+    r.v[5] = x[c + (bh ? p.whi[2] : p.s[2])];
+    r.neg = q.neg | ((bl && (p.nlo & 4u)) ? 16u : 0u) | ((bh && (p.nhi & 4u)) ? 32u : 0u);
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ T pool_sum6(const PoolNb6<T>& r, T cx, T cy, T cz)
+{
+    T u[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) u[s] = ((r.neg >> s) & 1u) ? -r.v[s] : r.v[s];
+    return cx * (u[0] + u[1]) + cy * (u[2] + u[3]) + cz * (u[4] + u[5]);
+}
+
+// one half sweep of colour col
+template <typename T>
+__device__ __forceinline__ void pool_half(T* x, const T* b, const LevelGeom& F, int col)
+{
+    constexpr int NB = sizeof(T) == sizeof(float) ? kPoolBatch : kPoolBatch / 2;
+    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, idg = (T)F.idg;
+    const PoolSides ps = pool_sides(F);
+    const int ncol = F.n0 * F.n1, sz = ps.s[2];
+    for (int t = (int)threadIdx.x; t < ncol; t += (int)blockDim.x) {
+        const int j = t / F.n0, i = t - j * F.n0;
+        const PoolCol pc = pool_col(F, ps, i, j);
+        for (int kb = (i + j + col) & 1; kb < F.n2; kb += 2 * NB) {
+            PoolNb6<T> nb[NB];
+            T bv[NB];
+#pragma unroll
+            for (int m = 0; m < NB; ++m) {
+                // past the column's end: load the batch's first cell again
+                const int k = (kb + 2 * m < F.n2) ? kb + 2 * m : kb, c = pc.c0 + k * sz;
+                nb[m] = pool_load_nb<T>(x, pc, ps, c, k);
+                bv[m] = b[c];
+            }
+#pragma unroll
+            for (int m = 0; m < NB; ++m) {
+                const int k = kb + 2 * m;
+                if (k < F.n2) x[pc.c0 + k * sz] = (pool_sum6<T>(nb[m], cx, cy, cz) - bv[m]) * idg;
+            }
+        }
+    }
+}
+
+// the first half of a sweep from a zero initial guess: colour-0 cells
+// (0 - b)/diag (their neighbours are all zero), colour-1 cells 0
+template <typename T>
+__device__ __forceinline__ void pool_half_zero(T* x, const T* b, const LevelGeom& F)
+{
+    const T idg = (T)F.idg;
+    const int ncol = F.n0 * F.n1, sz = (int)F.sz;
+    for (int t = (int)threadIdx.x; t < ncol; t += (int)blockDim.x) {
+        const int j = t / F.n0, i = t - j * F.n0;
+        const int c0 = (int)LIdx(F, i, j, 0);
+        for (int k = 0; k < F.n2; ++k) {
+            const int c = c0 + k * sz;
+            x[c] = ((i + j + k) & 1) == 0 ? ((T)0 - b[c]) * idg : (T)0;
+        }
+    }
+}
+
+// nu red-black sweeps; zero_first: from a zero initial guess
+template <typename T>
+__device__ __forceinline__ void pool_smooth(T* x, const T* b, const LevelGeom& F, int nu, bool zero_first)
+{
+    for (int s = 0; s < nu; ++s) {
+        if (zero_first && s == 0)
+            pool_half_zero<T>(x, b, F);
+        else
+            pool_half<T>(x, b, F, 0);
+        __syncthreads();
+        pool_half<T>(x, b, F, 1);
+        __syncthreads();
+    }
+}
+
+// x = 0 on the valid cells
+template <typename T>
+__device__ __forceinline__ void pool_zero(T* x, const LevelGeom& F)
+{
+    const int ncol = F.n0 * F.n1, sz = (int)F.sz;
+    for (int t = (int)threadIdx.x; t < ncol; t += (int)blockDim.x) {
+        const int j = t / F.n0, i = t - j * F.n0;
+        const int c0 = (int)LIdx(F, i, j, 0);
+        for (int k = 0; k < F.n2; ++k) x[c0 + k * sz] = (T)0;
+    }
+    __syncthreads();
+}
+
+// cres = the average over each coarse cell's children of b - A x (children
+// in (dk, dj, di) order, the oracle's); ratios of 1 or 2 per direction.
+// Each thread takes a coarse column and its up to four fine columns.
+template <typename T>
+__device__ __forceinline__ void pool_restrict(const T* x, const T* b, const LevelGeom& F, T* cres,
+                                              const LevelGeom& C, int r0, int r1, int r2)
+{
+    constexpr int CB = sizeof(T) == sizeof(float) ? 4 : 2;  // children whose loads are in flight together
+    const T wgt = (T)(1.0 / (double)(r0 * r1 * r2));
+    const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
+    const PoolSides ps = pool_sides(F);
+    const int ncol = C.n0 * C.n1, fsz = ps.s[2], csz = (int)C.sz;
+    for (int t = (int)threadIdx.x; t < ncol; t += (int)blockDim.x) {
+        const int J = t / C.n0, I = t - J * C.n0;
+        PoolCol pc[4];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+            const int di = ch & 1, dj = ch >> 1;
+            pc[ch] = pool_col(F, ps, I * r0 + (di < r0 ? di : 0), J * r1 + (dj < r1 ? dj : 0));
+        }
+        const int cc0 = (int)LIdx(C, I, J, 0);
+        for (int K = 0; K < C.n2; ++K) {
+            T s = (T)0;
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                if (dk >= r2) break;
+                const int k = K * r2 + dk;
+#pragma unroll
+                for (int g = 0; g < 4; g += CB) {
+                    PoolNb6<T> nb[CB];
+                    T bv[CB], xv[CB];
+#pragma unroll
+                    for (int m = 0; m < CB; ++m) {
+                        const int c = pc[g + m].c0 + k * fsz;
+                        nb[m] = pool_load_nb<T>(x, pc[g + m], ps, c, k);
+                        bv[m] = b[c];
+                        xv[m] = x[c];
+                    }
+#pragma unroll
+                    for (int m = 0; m < CB; ++m) {
+                        const int di = (g + m) & 1, dj = (g + m) >> 1;
+                        if (di < r0 && dj < r1) s += bv[m] - (pool_sum6<T>(nb[m], cx, cy, cz) - dg * xv[m]);
+                    }
+                }
+            }
+            cres[cc0 + K * csz] = s * wgt;
+        }
+    }
+    __syncthreads();
+}
+
+// x += the coarse correction of each cell's parent (ratios of 1 or 2)
+template <typename T>
+__device__ __forceinline__ void pool_prolong(T* x, const LevelGeom& F, const T* xc, const LevelGeom& C, int r0, int r1,
+                                             int r2)
+{
+    const int ncol = F.n0 * F.n1, sz = (int)F.sz, csz = (int)C.sz, s2 = r2 >> 1;
+    for (int t = (int)threadIdx.x; t < ncol; t += (int)blockDim.x) {
+        const int j = t / F.n0, i = t - j * F.n0;
+        const int c0 = (int)LIdx(F, i, j, 0), p0 = (int)LIdx(C, i >> (r0 >> 1), j >> (r1 >> 1), 0);
+        for (int kb = 0; kb < F.n2; kb += kStageBatch) {
+            T a[kStageBatch], p[kStageBatch];
+#pragma unroll
+            for (int m = 0; m < kStageBatch; ++m) {
+                const int k = min(kb + m, F.n2 - 1);
+                a[m] = x[c0 + k * sz];
+                p[m] = xc[p0 + (k >> s2) * csz];
+            }
+#pragma unroll
+            for (int m = 0; m < kStageBatch; ++m)
+                if (kb + m < F.n2) x[c0 + (kb + m) * sz] = a[m] + p[m];
+        }
+    }
+    __syncthreads();
+}
+
+// the correction / residual slice of a level known to be in the pool
+template <typename T>
+__device__ __forceinline__ T* pool_cor(const LevelPtrs<T>* lp, int l, char* pool)
+{
+    return (T*)(pool + lp[l].lds);
+}
+template <typename T>
+__device__ __forceinline__ T* pool_res(const LevelPtrs<T>* lp, int l, char* pool)
+{
+    return (T*)(pool + lp[l].lds + lds_align((long long)lds_geom(lp[l].G).tot * sizeof(T)));
 }
 
 template <typename T>
@@ -1174,17 +1449,28 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
         double** v[8] = {&w.r, &w.rh, &w.p, &w.v, &w.s, &w.t, &w.x, &w.b};
         for (int q = 0; q < 8; ++q) *v[q] = (double*)(pool + w.lds + q * vb);
     }
-    if (lp[lc].lds >= 0) lv_stage<T>(lv_res(lp, lc, pool), lv_geom(lp, lc), lp[lc].res, lp[lc].G);
+    if (lp[lc].lds >= 0) lv_stage<T>(pool_res(lp, lc, pool), lds_geom(lp[lc].G), lp[lc].res, lp[lc].G);
     for (int l = lc; l < nlev - 1; ++l) {
         const LevelGeom F = lv_geom(lp, l);
         const LevelGeom C = lv_geom(lp, l + 1);
+        const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
+        if (lp[l].lds >= 0 && pool_sweepable(F) && r0 <= 2 && r1 <= 2 && r2 <= 2) {
+            // pool levels are the coarsest ones, so level l + 1 is in the pool too
+            T* x = pool_cor(lp, l, pool);
+            const T* b = pool_res(lp, l, pool);
+            if (nu1 > 0)
+                pool_smooth<T>(x, b, F, nu1, true);
+            else
+                pool_zero<T>(x, F);
+            pool_restrict<T>(x, b, F, pool_res(lp, l + 1, pool), C, r0, r1, r2);
+            continue;
+        }
         T* x = lv_cor(lp, l, pool);
         const T* b = lv_res(lp, l, pool);
         for (long long c = threadIdx.x; c < F.tot; c += blockDim.x) x[c] = (T)0;
         __syncthreads();
         blk_smooth<T>(x, b, F, nu1);
         blk_ghosts<T>(x, F);
-        const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
         const T wgt = (T)(1.0 / (double)(r0 * r1 * r2));
         const T cx = (T)F.cx, cy = (T)F.cy, cz = (T)F.cz, dg = (T)F.dg;
         T* cres = lv_res(lp, l + 1, pool);
@@ -1226,9 +1512,15 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
     for (int l = nlev - 2; l >= lc; --l) {
         const LevelGeom F = lv_geom(lp, l);
         const LevelGeom C = lv_geom(lp, l + 1);
+        const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
+        if (lp[l].lds >= 0 && pool_sweepable(F) && r0 <= 2 && r1 <= 2 && r2 <= 2) {
+            T* x = pool_cor(lp, l, pool);
+            pool_prolong<T>(x, F, pool_cor(lp, l + 1, pool), C, r0, r1, r2);
+            pool_smooth<T>(x, pool_res(lp, l, pool), F, nu2, false);
+            continue;
+        }
         T* x = lv_cor(lp, l, pool);
         const T* xc = lv_cor(lp, l + 1, pool);
-        const int r0 = lp[l].r0, r1 = lp[l].r1, r2 = lp[l].r2;
         BLK_FOR(F, t_, c)
         {
             const int q = t_ / F.n0;
@@ -1238,7 +1530,7 @@ __device__ void coarse_vcycle_body(const LevelPtrs<T>* __restrict__ lp, int lc, 
         __syncthreads();
         blk_smooth<T>(x, lv_res(lp, l, pool), F, nu2);
     }
-    if (lp[lc].lds >= 0) lv_stage<T>(lp[lc].cor, lp[lc].G, lv_cor(lp, lc, pool), lv_geom(lp, lc));
+    if (lp[lc].lds >= 0) lv_stage<T>(lp[lc].cor, lp[lc].G, pool_cor(lp, lc, pool), lds_geom(lp[lc].G));
 }
 
 template <typename T>
