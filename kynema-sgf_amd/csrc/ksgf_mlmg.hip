@@ -1987,6 +1987,30 @@ static void launch_pk_sweep(ksgf_macproj* mp, int l, int mode, T* x, const T* b,
     }
 }
 
+// Duration of reps launches issued back to back between two stream events.
+// Levels below kTuneSmallCells cells are not timed: their kernels are bound
+// by latency, not bandwidth, so they take the most parallel shape (one plane
+// per thread); timing them back to back would measure the launch rate.
+constexpr long long kTuneSmallCells = 1LL << 20;
+template <typename L>
+static float time_launches(L&& launch, int reps)
+{
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1e30f;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 1e30f;
+    }
+    (void)hipEventRecord(e0, nullptr);
+    for (int rep = 0; rep < reps; ++rep) launch(rep);
+    (void)hipEventRecord(e1, nullptr);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
 // Autotuner for the planes per workgroup of the packed half sweep: every
 // candidate is timed on the level's own arrays (red halves on the
 // correction array, which every solve re-initialises before reading), the
@@ -2007,34 +2031,23 @@ static int tune_sweep(ksgf_macproj* mp, int l, T* x, const T* b)
         const int v = std::atoi(env);
         if (v == 1 || v == 2 || v == 4 || v == 8) return std::min(v, std::max(1, L.n2));
     }
+    if ((long long)L.n0 * L.n1 * L.n2 < kTuneSmallCells) return 1;
     const std::array<long long, 4> key{L.n0, L.n1, L.n2, (long long)sizeof(T)};
     auto it = kz_cache().find(key);
     if (it != kz_cache().end()) return it->second;
     const int cand[5] = {1, 2, 4, 8, 16};
     int best = 1;
     float best_ms = 1e30f;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return best;
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return best;
-    }
     const PkArgs<T> a = pk_args<T>(mp, l, x, b, nullptr);
     for (int c : cand) {
         if (c > L.n2) continue;
         launch_pk_kernel<T>(a, 0, kPkPlain, c, nullptr);
-        (void)hipEventRecord(e0, nullptr);
-        for (int rep = 0; rep < 4; ++rep) launch_pk_kernel<T>(a, rep & 1, kPkPlain, c, nullptr);
-        (void)hipEventRecord(e1, nullptr);
-        float ms = 0.f;
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        const float ms = time_launches([&](int rep) { launch_pk_kernel<T>(a, rep & 1, kPkPlain, c, nullptr); }, 4);
         if (ms < best_ms) {
             best_ms = ms;
             best = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     kz_cache()[key] = best;
     return best;
 }
@@ -2157,32 +2170,21 @@ static int tune_restrict(ksgf_macproj* mp, int l, const T* x, const T* b, T* out
     const LevelGeom& L = mp->G[l];
     const LevelGeom& C = mp->G[l + 1];
     static std::map<std::array<long long, 7>, int> cache;
+    if ((long long)L.n0 * L.n1 * L.n2 < kTuneSmallCells) return 1;
     const std::array<long long, 7> key{L.n0, L.n1, L.n2, C.n0, C.n1, C.n2, (long long)sizeof(T)};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int best = 1;
     float best_ms = 1e30f;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return best;
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return best;
-    }
     for (int kz : {1, 2, 4, 8, 16}) {
         if (kz > C.n2) continue;
         launch_restrict<T>(mp, l, x, b, out, kz, nullptr);
-        (void)hipEventRecord(e0, nullptr);
-        for (int rep = 0; rep < 3; ++rep) launch_restrict<T>(mp, l, x, b, out, kz, nullptr);
-        (void)hipEventRecord(e1, nullptr);
-        float ms = 0.f;
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        const float ms = time_launches([&](int) { launch_restrict<T>(mp, l, x, b, out, kz, nullptr); }, 3);
         if (ms < best_ms) {
             best_ms = ms;
             best = kz;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     cache[key] = best;
     return best;
 }
