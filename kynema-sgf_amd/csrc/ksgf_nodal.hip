@@ -248,6 +248,123 @@ __global__ void __launch_bounds__(256) k_nd_march(double* __restrict__ out, cons
     if (MODE == 2) nd_block_max(m, norm);
 }
 
+// The same sweeps for levels at least 64 nodes wide, with each plane of a
+// workgroup's 64 x 4 column tile staged through LDS: the workgroup loads the
+// tile and its one-node halo (66 x 6 nodes, periodic images and mirrors
+// resolved as nd_nb does) once, one or two loads per thread, and every
+// thread forms its plane sums from LDS.  The register form issues nine
+// global loads per node and plane, which kept the texture-address units
+// about three quarters busy and the sweep at a third of HBM bandwidth.
+// Plane p of the march lives in buffer p & 1; plane p + 2 is loaded into
+// registers one iteration ahead, so one barrier per plane separates its LDS
+// store from the reads.  Plane sums, node update and operand order are
+// k_nd_march's, so the results are its bits.
+constexpr int kNdLx = 66, kNdLy = 6, kNdLn = kNdLx * kNdLy;
+
+// storage index of a halo position g (one beyond either end at most) of a
+// direction with N nodes; positions further out are read by no node and
+// map to 0
+__device__ __forceinline__ int nd_halo(int g, int N, int per)
+{
+    if (g >= 0 && g < N) return g;
+    if (g == -1) return nd_nb(0, -1, N, per);
+    if (g == N) return nd_nb(N - 1, 1, N, per);
+    return 0;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_nd_march_lds(double* __restrict__ out, const double* __restrict__ x,
+                                                      const double* __restrict__ x2, const double* __restrict__ b,
+                                                      double* __restrict__ r, NLev L, int kz,
+                                                      unsigned long long* __restrict__ norm)
+{
+    constexpr bool SUM2 = (MODE == 2);
+    __shared__ double sp[2][kNdLn];
+    const int ntx = (L.N[0] + 63) >> 6, nty = (L.N[1] + 3) >> 2;
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = t % ntx, q = t / ntx;
+    const int x0 = tx * 64, y0 = (q % nty) * 4;
+    const int k0 = (q / nty) * kz;
+    const int k1 = min(k0 + kz, L.N[2]);
+    const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y;
+    const int i = x0 + (int)threadIdx.x, j = y0 + (int)threadIdx.y;
+    const bool act = i < L.N[0] && j < L.N[1];
+    const int sy = L.N[0], sz = L.N[0] * L.N[1];
+    // the (up to) two LDS slots this thread fills: in-plane storage offsets
+    int so[2];
+    bool sv[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int s = tid + 256 * m;
+        sv[m] = s < kNdLn;
+        const int hx = s % kNdLx, hy = s / kNdLx;
+        so[m] = sv[m] ? nd_halo(x0 - 1 + hx, L.N[0], L.per[0]) + sy * nd_halo(y0 - 1 + hy, L.N[1], L.per[1]) : 0;
+    }
+    // plane p of the march: p = 0 is k0's low neighbour, p >= 1 is
+    // nd_nb(k0 + p - 2, +1) (p = 1: k0 itself)
+    auto plane_k = [&](int p) { return p == 0 ? nd_nb(k0, -1, L.N[2], L.per[2]) : (p == 1 ? k0 : nd_nb(k0 + p - 2, 1, L.N[2], L.per[2])); };
+    auto gload = [&](int p, double (&v)[2]) {
+        const int z = plane_k(p) * sz;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) v[m] = sv[m] ? nd_val<SUM2>(x, x2, z + so[m]) : 0.0;
+    };
+    auto lstore = [&](int p, const double (&v)[2]) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+            if (sv[m]) sp[p & 1][tid + 256 * m] = v[m];
+    };
+    const int c0 = ((int)threadIdx.y + 1) * kNdLx + (int)threadIdx.x + 1;
+    auto psum = [&](int p) {
+        const double* a = sp[p & 1];
+        NdPlane s;
+        s.c = a[c0];
+        s.x = a[c0 - 1] + a[c0 + 1];
+        s.y = a[c0 - kNdLx] + a[c0 + kNdLx];
+        s.xy = (a[c0 - kNdLx - 1] + a[c0 - kNdLx + 1]) + (a[c0 + kNdLx - 1] + a[c0 + kNdLx + 1]);
+        return s;
+    };
+    const bool dxy = (L.dlo[0] && i == 0) || (L.dhi[0] && i == L.N[0] - 1) || (L.dlo[1] && j == 0) ||
+                     (L.dhi[1] && j == L.N[1] - 1);
+    const int cxy = act ? j * sy + i : 0;
+    double m = 0.0;
+    const int np = k1 - k0 + 2;  // planes of the march
+    double v[2];
+    gload(0, v);
+    lstore(0, v);
+    gload(1, v);
+    lstore(1, v);
+    __syncthreads();
+    NdPlane pm = psum(0), po = psum(1);
+    gload(2, v);
+    __syncthreads();
+    for (int n = 0; n < k1 - k0; ++n) {
+        const int k = k0 + n;
+        lstore(n + 2, v);
+        double bc = 0.0;
+        if (act) bc = b[k * sz + cxy];
+        if (n + 3 < np) gload(n + 3, v);
+        __syncthreads();
+        const NdPlane pp = psum(n + 2);
+        if (act) {
+            const int c = k * sz + cxy;
+            const bool dir = dxy || (L.dlo[2] && k == 0) || (L.dhi[2] && k == L.N[2] - 1);
+            if (MODE == 0) {
+                out[c] = dir ? 0.0 : po.c + (bc - nd_apply_planes(L, pm, po, pp)) * L.wd;
+            } else if (MODE == 1) {
+                out[c] = dir ? 0.0 : bc - nd_apply_planes(L, pm, po, pp);
+            } else {
+                out[c] = po.c;
+                const double rv = dir ? 0.0 : bc - nd_apply_planes(L, pm, po, pp);
+                r[c] = rv;
+                m = fmax(m, nd_abs_or_inf(rv));
+            }
+        }
+        pm = po;
+        po = pp;
+    }
+    if (MODE == 2) nd_block_max(m, norm);
+}
+
 // |a|_inf over a level (Dirichlet nodes hold zero)
 __global__ void __launch_bounds__(256) k_nd_norm(const double* __restrict__ a, NLev L,
                                                  unsigned long long* __restrict__ norm)
@@ -775,7 +892,10 @@ static void nd_launch_march(double* out, const double* x, const double* x2, cons
 {
     const int kz = nd_kz(L);
     const long long nb = (long long)((L.N[0] + 63) / 64) * ((L.N[1] + 3) / 4) * ((L.N[2] + kz - 1) / kz);
-    hipLaunchKernelGGL(k_nd_march<MODE>, dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz, norm);
+    if (L.N[0] >= 64 && L.N[1] >= 4)
+        hipLaunchKernelGGL(k_nd_march_lds<MODE>, dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz, norm);
+    else
+        hipLaunchKernelGGL(k_nd_march<MODE>, dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz, norm);
 }
 
 static int nd_alloc(ksgf_nodalproj* np, double** p, size_t n)
