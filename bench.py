@@ -56,6 +56,9 @@ def parse():
                    help="summarise a rocprofv3 --kernel-trace output directory of STEPS advect + project steps "
                         "by (kernel, multigrid level): calls, average and total time per step; prints text for "
                         "profiles/")
+    p.add_argument("--trace-grid-summary", nargs=2, metavar=("TRACE_DIR", "STEPS"),
+                   help="summarise every launch of a rocprofv3 --kernel-trace output directory by (kernel, grid "
+                        "size): calls, average time and total time divided by STEPS; prints text for profiles/")
     p.add_argument("--pmc-kernels", nargs="+", metavar="DIR",
                    help="FETCH_DIR WRITE_DIR SQ_DIR COPY_FETCH_DIR COPY_WRITE_DIR [SQ2_DIR]: summarise rocprofv3 "
                         "--pmc runs (FETCH_SIZE; WRITE_SIZE; SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_VMEM_RD, "
@@ -396,6 +399,25 @@ def trace_summary(d, steps):
             continue
         lines.append(f"{k[:54]:54s} {l:>5s} {len(v):6d} {t / len(v):9.1f} {t * 1e-3 / steps:8.3f} {100 * t / tot:6.2f}")
     lines.append(f"total kernel time {tot * 1e-3 / steps:.3f} ms/step over {steps} steps")
+    return "\n".join(lines)
+
+
+def trace_grid_summary(d, steps):
+    """every launch of a kernel trace grouped by (kernel, grid threads)"""
+    import collections
+
+    rows = _trace_rows(d)
+    acc = collections.defaultdict(list)
+    for r in rows:
+        acc[(r[1], r[2])].append(r[3])
+    tot = sum(r[3] for r in rows)
+    lines = [f"{'kernel':48s} {'grid':>10s} {'calls':>6s} {'avg us':>9s} {'ms/step':>8s} {'%':>6s}"]
+    for (k, g), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        t = sum(v)
+        if t < 0.002 * tot:
+            continue
+        lines.append(f"{k[:48]:48s} {g:10d} {len(v):6d} {t / len(v):9.1f} {t * 1e-3 / steps:8.3f} {100 * t / tot:6.2f}")
+    lines.append(f"total kernel time {tot * 1e-3:.2f} ms = {tot * 1e-3 / steps:.2f} ms/step over {steps} steps")
     return "\n".join(lines)
 
 
@@ -874,6 +896,9 @@ def main():
         return
     if a.trace_summary:
         print(trace_summary(a.trace_summary[0], int(a.trace_summary[1])))
+        return
+    if a.trace_grid_summary:
+        print(trace_grid_summary(a.trace_grid_summary[0], int(a.trace_grid_summary[1])))
         return
     if a.pmc_kernels:
         if len(a.pmc_kernels) not in (5, 6):

@@ -365,13 +365,16 @@ __global__ void __launch_bounds__(256) k_nd_march_lds(double* __restrict__ out, 
     if (MODE == 2) nd_block_max(m, norm);
 }
 
-// |a|_inf over a level (Dirichlet nodes hold zero)
+// |a|_inf over a level (Dirichlet nodes hold zero): a grid-stride pass of a
+// bounded grid, so one atomic per workgroup merges kNdNormBlocks maxima (one
+// workgroup per 256 nodes serialised 66 K atomics on one word at 256^3)
+constexpr int kNdNormBlocks = 1024;
 __global__ void __launch_bounds__(256) k_nd_norm(const double* __restrict__ a, NLev L,
                                                  unsigned long long* __restrict__ norm)
 {
-    int i, j, k;
     double m = 0.0;
-    if (nd_node(L, i, j, k)) m = nd_abs_or_inf(a[nd_idx(L, i, j, k)]);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.tot; c += (long long)gridDim.x * blockDim.x)
+        m = fmax(m, nd_abs_or_inf(a[c]));
     nd_block_max(m, norm);
 }
 
@@ -983,7 +986,8 @@ static int nd_solve(ksgf_nodalproj* np, double rtol, double atol, ksgf_mlmg_info
     hipLaunchKernelGGL(k_nd_sub, gr, kNdBlock, 0, st, np->rhs, L0, (const double*)nullptr, 1);
     if (np->singular) nd_weighted_mean_sub(np, 0, np->rhs, st);
     hipLaunchKernelGGL(k_nd_reset, dim3(1), dim3(1), 0, st, np->norm);
-    hipLaunchKernelGGL(k_nd_norm, gr, kNdBlock, 0, st, np->rhs, L0, np->norm);
+    hipLaunchKernelGGL(k_nd_norm, dim3(std::min(kNdNormBlocks, (L0.tot + 255) / 256)), dim3(256), 0, st, np->rhs, L0,
+                       np->norm);
     KSGF_LAUNCH_CHECK();
     KSGF_HIP_CHECK(hipMemsetAsync(np->stats, 0, 2 * sizeof(int), st));
     np->cur = 0;
