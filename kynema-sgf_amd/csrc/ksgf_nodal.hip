@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_nd_march(double* __restrict__ out, cons
 // registers one iteration ahead, so one barrier per plane separates its LDS
 // store from the reads.  Plane sums, node update and operand order are
 // k_nd_march's, so the results are its bits.
-constexpr int kNdLx = 66, kNdLy = 6, kNdLn = kNdLx * kNdLy;
+constexpr int kNdLx = 66;  // tile row of 64 nodes and its halo
 
 // storage index of a halo position g (one beyond either end at most) of a
 // direction with N nodes; positions further out are read by no node and
@@ -272,49 +272,51 @@ __device__ __forceinline__ int nd_halo(int g, int N, int per)
     return 0;
 }
 
-template <int MODE>
+template <int MODE, int RY>
 __global__ void __launch_bounds__(256) k_nd_march_lds(double* __restrict__ out, const double* __restrict__ x,
                                                       const double* __restrict__ x2, const double* __restrict__ b,
                                                       double* __restrict__ r, NLev L, int kz,
                                                       unsigned long long* __restrict__ norm)
 {
+    // RY rows per thread: a 64 x 4RY tile, threads (x, y) own rows y, y + 4, ...
     constexpr bool SUM2 = (MODE == 2);
-    __shared__ double sp[2][kNdLn];
-    const int ntx = (L.N[0] + 63) >> 6, nty = (L.N[1] + 3) >> 2;
+    constexpr int TY = 4 * RY, LY = TY + 2, LN = kNdLx * LY, NS = (LN + 255) / 256;
+    __shared__ double sp[2][LN];
+    const int ntx = (L.N[0] + 63) >> 6, nty = (L.N[1] + TY - 1) / TY;
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int tx = t % ntx, q = t / ntx;
-    const int x0 = tx * 64, y0 = (q % nty) * 4;
+    const int x0 = tx * 64, y0 = (q % nty) * TY;
     const int k0 = (q / nty) * kz;
     const int k1 = min(k0 + kz, L.N[2]);
     const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y;
-    const int i = x0 + (int)threadIdx.x, j = y0 + (int)threadIdx.y;
-    const bool act = i < L.N[0] && j < L.N[1];
+    const int i = x0 + (int)threadIdx.x;
     const int sy = L.N[0], sz = L.N[0] * L.N[1];
-    // the (up to) two LDS slots this thread fills: in-plane storage offsets
-    int so[2];
-    bool sv[2];
+    // the LDS slots this thread fills: in-plane storage offsets
+    int so[NS];
+    bool sv[NS];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < NS; ++m) {
         const int s = tid + 256 * m;
-        sv[m] = s < kNdLn;
+        sv[m] = s < LN;
         const int hx = s % kNdLx, hy = s / kNdLx;
         so[m] = sv[m] ? nd_halo(x0 - 1 + hx, L.N[0], L.per[0]) + sy * nd_halo(y0 - 1 + hy, L.N[1], L.per[1]) : 0;
     }
     // plane p of the march: p = 0 is k0's low neighbour, p >= 1 is
     // nd_nb(k0 + p - 2, +1) (p = 1: k0 itself)
-    auto plane_k = [&](int p) { return p == 0 ? nd_nb(k0, -1, L.N[2], L.per[2]) : (p == 1 ? k0 : nd_nb(k0 + p - 2, 1, L.N[2], L.per[2])); };
-    auto gload = [&](int p, double (&v)[2]) {
+    auto plane_k = [&](int p) {
+        return p == 0 ? nd_nb(k0, -1, L.N[2], L.per[2]) : (p == 1 ? k0 : nd_nb(k0 + p - 2, 1, L.N[2], L.per[2]));
+    };
+    auto gload = [&](int p, double (&v)[NS]) {
         const int z = plane_k(p) * sz;
 #pragma unroll
-        for (int m = 0; m < 2; ++m) v[m] = sv[m] ? nd_val<SUM2>(x, x2, z + so[m]) : 0.0;
+        for (int m = 0; m < NS; ++m) v[m] = sv[m] ? nd_val<SUM2>(x, x2, z + so[m]) : 0.0;
     };
-    auto lstore = [&](int p, const double (&v)[2]) {
+    auto lstore = [&](int p, const double (&v)[NS]) {
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < NS; ++m)
             if (sv[m]) sp[p & 1][tid + 256 * m] = v[m];
     };
-    const int c0 = ((int)threadIdx.y + 1) * kNdLx + (int)threadIdx.x + 1;
-    auto psum = [&](int p) {
+    auto psum = [&](int p, int c0) {
         const double* a = sp[p & 1];
         NdPlane s;
         s.c = a[c0];
@@ -323,44 +325,70 @@ __global__ void __launch_bounds__(256) k_nd_march_lds(double* __restrict__ out, 
         s.xy = (a[c0 - kNdLx - 1] + a[c0 - kNdLx + 1]) + (a[c0 + kNdLx - 1] + a[c0 + kNdLx + 1]);
         return s;
     };
-    const bool dxy = (L.dlo[0] && i == 0) || (L.dhi[0] && i == L.N[0] - 1) || (L.dlo[1] && j == 0) ||
-                     (L.dhi[1] && j == L.N[1] - 1);
-    const int cxy = act ? j * sy + i : 0;
+    int c0[RY], cxy[RY];
+    bool act[RY], dxy[RY];
+#pragma unroll
+    for (int w = 0; w < RY; ++w) {
+        const int ty = (int)threadIdx.y + 4 * w, j = y0 + ty;
+        c0[w] = (ty + 1) * kNdLx + (int)threadIdx.x + 1;
+        act[w] = i < L.N[0] && j < L.N[1];
+        cxy[w] = act[w] ? j * sy + i : 0;
+        dxy[w] = (L.dlo[0] && i == 0) || (L.dhi[0] && i == L.N[0] - 1) || (L.dlo[1] && j == 0) ||
+                 (L.dhi[1] && j == L.N[1] - 1);
+    }
     double m = 0.0;
-    const int np = k1 - k0 + 2;  // planes of the march
-    double v[2];
-    gload(0, v);
-    lstore(0, v);
-    gload(1, v);
-    lstore(1, v);
+    const int nk = k1 - k0, np = nk + 2;  // nodes per column, planes of the march
+    // planes n + 3 and n + 4 are in flight in two register sets while plane
+    // n + 2 goes to LDS (the loop takes two planes per trip, so each set has
+    // a fixed name)
+    double va[NS], vb[NS];
+    gload(0, va);
+    lstore(0, va);
+    gload(1, va);
+    lstore(1, va);
     __syncthreads();
-    NdPlane pm = psum(0), po = psum(1);
-    gload(2, v);
+    NdPlane pm[RY], po[RY];
+#pragma unroll
+    for (int w = 0; w < RY; ++w) {
+        pm[w] = psum(0, c0[w]);
+        po[w] = psum(1, c0[w]);
+    }
+    gload(2, va);
+    if (3 < np) gload(3, vb);
     __syncthreads();
-    for (int n = 0; n < k1 - k0; ++n) {
+    auto step = [&](int n, double (&v)[NS]) {
         const int k = k0 + n;
         lstore(n + 2, v);
-        double bc = 0.0;
-        if (act) bc = b[k * sz + cxy];
-        if (n + 3 < np) gload(n + 3, v);
+        double bc[RY];
+#pragma unroll
+        for (int w = 0; w < RY; ++w) bc[w] = act[w] ? b[k * sz + cxy[w]] : 0.0;
+        if (n + 4 < np) gload(n + 4, v);
         __syncthreads();
-        const NdPlane pp = psum(n + 2);
-        if (act) {
-            const int c = k * sz + cxy;
-            const bool dir = dxy || (L.dlo[2] && k == 0) || (L.dhi[2] && k == L.N[2] - 1);
-            if (MODE == 0) {
-                out[c] = dir ? 0.0 : po.c + (bc - nd_apply_planes(L, pm, po, pp)) * L.wd;
-            } else if (MODE == 1) {
-                out[c] = dir ? 0.0 : bc - nd_apply_planes(L, pm, po, pp);
-            } else {
-                out[c] = po.c;
-                const double rv = dir ? 0.0 : bc - nd_apply_planes(L, pm, po, pp);
-                r[c] = rv;
-                m = fmax(m, nd_abs_or_inf(rv));
+        const bool dz = (L.dlo[2] && k == 0) || (L.dhi[2] && k == L.N[2] - 1);
+#pragma unroll
+        for (int w = 0; w < RY; ++w) {
+            const NdPlane pp = psum(n + 2, c0[w]);
+            if (act[w]) {
+                const int c = k * sz + cxy[w];
+                const bool dir = dxy[w] || dz;
+                if (MODE == 0) {
+                    out[c] = dir ? 0.0 : po[w].c + (bc[w] - nd_apply_planes(L, pm[w], po[w], pp)) * L.wd;
+                } else if (MODE == 1) {
+                    out[c] = dir ? 0.0 : bc[w] - nd_apply_planes(L, pm[w], po[w], pp);
+                } else {
+                    out[c] = po[w].c;
+                    const double rv = dir ? 0.0 : bc[w] - nd_apply_planes(L, pm[w], po[w], pp);
+                    r[c] = rv;
+                    m = fmax(m, nd_abs_or_inf(rv));
+                }
             }
+            pm[w] = po[w];
+            po[w] = pp;
         }
-        pm = po;
-        po = pp;
+    };
+    for (int n = 0; n < nk; n += 2) {
+        step(n, va);
+        if (n + 1 < nk) step(n + 1, vb);
     }
     if (MODE == 2) nd_block_max(m, norm);
 }
@@ -882,7 +910,7 @@ static int nd_kz(const NLev& L)
     static const int kz_max = [] {
         const char* e = std::getenv("KSGF_ND_KZ");
         const int v = e ? std::atoi(e) : 0;
-        return (v >= 1 && v <= 64) ? v : 16;
+        return (v >= 1 && v <= 64) ? v : 32;
     }();
     const long long bxy = (long long)((L.N[0] + 63) / 64) * ((L.N[1] + 3) / 4);
     long long kz = (long long)L.N[2] * bxy / 2048;
@@ -895,10 +923,16 @@ static void nd_launch_march(double* out, const double* x, const double* x2, cons
 {
     const int kz = nd_kz(L);
     const long long nb = (long long)((L.N[0] + 63) / 64) * ((L.N[1] + 3) / 4) * ((L.N[2] + kz - 1) / kz);
-    if (L.N[0] >= 64 && L.N[1] >= 4)
-        hipLaunchKernelGGL(k_nd_march_lds<MODE>, dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz, norm);
-    else
+    if (L.N[0] >= 64 && L.N[1] >= 8) {
+        const long long nb2 = (long long)((L.N[0] + 63) / 64) * ((L.N[1] + 7) / 8) * ((L.N[2] + kz - 1) / kz);
+        hipLaunchKernelGGL((k_nd_march_lds<MODE, 2>), dim3((unsigned)nb2), kNdBlock, 0, st, out, x, x2, b, r, L, kz,
+                           norm);
+    } else if (L.N[0] >= 64 && L.N[1] >= 4) {
+        hipLaunchKernelGGL((k_nd_march_lds<MODE, 1>), dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz,
+                           norm);
+    } else {
         hipLaunchKernelGGL(k_nd_march<MODE>, dim3((unsigned)nb), kNdBlock, 0, st, out, x, x2, b, r, L, kz, norm);
+    }
 }
 
 static int nd_alloc(ksgf_nodalproj* np, double** p, size_t n)
