@@ -59,6 +59,11 @@ def parse():
     p.add_argument("--trace-grid-summary", nargs=2, metavar=("TRACE_DIR", "STEPS"),
                    help="summarise every launch of a rocprofv3 --kernel-trace output directory by (kernel, grid "
                         "size): calls, average time and total time divided by STEPS; prints text for profiles/")
+    p.add_argument("--pmc-grid-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
+                                                            "COPY_WRITE_DIR"),
+                   help="summarise rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (and the two of --copy-probe) "
+                        "by (kernel, grid size): HBM MB read and written per launch, scaled by the copy probe; "
+                        "prints text for profiles/")
     p.add_argument("--pmc-kernels", nargs="+", metavar="DIR",
                    help="FETCH_DIR WRITE_DIR SQ_DIR COPY_FETCH_DIR COPY_WRITE_DIR [SQ2_DIR]: summarise rocprofv3 "
                         "--pmc runs (FETCH_SIZE; WRITE_SIZE; SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_VMEM_RD, "
@@ -418,6 +423,51 @@ def trace_grid_summary(d, steps):
             continue
         lines.append(f"{k[:48]:48s} {g:10d} {len(v):6d} {t / len(v):9.1f} {t * 1e-3 / steps:8.3f} {100 * t / tot:6.2f}")
     lines.append(f"total kernel time {tot * 1e-3:.2f} ms = {tot * 1e-3 / steps:.2f} ms/step over {steps} steps")
+    return "\n".join(lines)
+
+
+def _pmc_dispatches(d):
+    """per dispatch: (kernel, grid threads) and the summed counter values"""
+    import collections
+    import csv
+    import glob
+
+    per = collections.defaultdict(dict)
+    key = {}
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            i = int(r["Dispatch_Id"])
+            key[i] = (_short(r["Kernel_Name"]), int(r["Grid_Size"]))
+            per[i][r["Counter_Name"]] = per[i].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return [(key[i], per[i]) for i in sorted(per)]
+
+
+def pmc_grid_summary(fdir, wdir, cfdir, cwdir):
+    """mean HBM MB read / written per launch by (kernel, grid), each counter
+    scaled so that the copy probe's 1 GiB reads and writes come out exact"""
+    import collections
+
+    true_kib = (1 << 27) * 8 / 1024.0
+    scale = []
+    for d, c in ((cfdir, "FETCH_SIZE"), (cwdir, "WRITE_SIZE")):
+        vals = [x[c] for k, x in _pmc_dispatches(d) if "MulFunctor" in k[0]]
+        scale.append(true_kib / (sum(vals) / len(vals)))
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d, c, sc in ((fdir, "FETCH_SIZE", scale[0]), (wdir, "WRITE_SIZE", scale[1])):
+        for k, x in _pmc_dispatches(d):
+            if c in x:
+                acc[k][c].append(x[c] * sc * 1024.0)
+    lines = [f"copy probe scale: FETCH_SIZE x{scale[0]:.4f}, WRITE_SIZE x{scale[1]:.4f}",
+             f"{'kernel':48s} {'grid':>10s} {'calls':>6s} {'read MB':>9s} {'write MB':>9s}"]
+
+    def m(v):
+        return sum(v) / len(v) if v else float("nan")
+
+    for (k, g), c in sorted(acc.items(), key=lambda kv: -m(kv[1].get("FETCH_SIZE", [0]))):
+        if k.startswith("__amd") or "MulFunctor" in k:
+            continue
+        lines.append(f"{k[:48]:48s} {g:10d} {len(c.get('FETCH_SIZE', [])):6d} "
+                     f"{m(c.get('FETCH_SIZE', [])) / 1e6:9.2f} {m(c.get('WRITE_SIZE', [])) / 1e6:9.2f}")
     return "\n".join(lines)
 
 
@@ -896,6 +946,9 @@ def main():
         return
     if a.trace_summary:
         print(trace_summary(a.trace_summary[0], int(a.trace_summary[1])))
+        return
+    if a.pmc_grid_summary:
+        print(pmc_grid_summary(*a.pmc_grid_summary))
         return
     if a.trace_grid_summary:
         print(trace_grid_summary(a.trace_grid_summary[0], int(a.trace_grid_summary[1])))
