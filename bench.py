@@ -329,7 +329,8 @@ _MG_KERNELS = ("k_pk_half", "k_pk_half4", "k_mg_smooth", "k_mg_ghosts", "k_fill"
 def _short(name):
     import re
 
-    n = re.sub(r"\(.*", "", name).replace("void ", "").replace("ksgf::", "")
+    n = name.replace("(anonymous namespace)::", "")
+    n = re.sub(r"\(.*", "", n).replace("void ", "").replace("ksgf::", "")
     return n.strip()
 
 
