@@ -1574,6 +1574,34 @@ __global__ void __launch_bounds__(256) k_mac_correct(MFDev u, const double* __re
     fu(i, j, k) = fu(i, j, k) - bd * (pr - pl);
 }
 
+// the three corrections in one pass over the cells grown by one on the high
+// sides: position (i, j, k) updates each face array whose face it is (an x
+// face for j, k inside the box, ...), so phi is read once for all three;
+// the per-face arithmetic is k_mac_correct's
+__global__ void __launch_bounds__(256) k_mac_correct3(MFDev u, MFDev v, MFDev w, const double* __restrict__ phi,
+                                                      LevelGeom L, RegDev r, const int* __restrict__ box, int dlo0,
+                                                      int dlo1, int dlo2, double bdx, double bdy, double bdz)
+{
+    int b, i, j, k;
+    if (!region_locate(r, box, b, i, j, k)) return;
+    const int* bx = box + 6 * b;
+    const bool ix = i <= bx[3], jy = j <= bx[4], kz = k <= bx[5];
+    const long long c = LIdx(L, i - dlo0, j - dlo1, k - dlo2);
+    const double pr = phi[c];
+    if (jy && kz) {
+        Fab fu = fab_at(u, box, b);
+        fu(i, j, k) = fu(i, j, k) - bdx * (pr - phi[c - 1]);
+    }
+    if (ix && kz) {
+        Fab fv = fab_at(v, box, b);
+        fv(i, j, k) = fv(i, j, k) - bdy * (pr - phi[c - L.sy]);
+    }
+    if (ix && jy) {
+        Fab fw = fab_at(w, box, b);
+        fw(i, j, k) = fw(i, j, k) - bdz * (pr - phi[c - L.sz]);
+    }
+}
+
 // one component of a cell MultiFab into the level-0 array (times scale), and back
 __global__ void __launch_bounds__(256) k_mf_gather(CMFDev m, int comp, double scale, double* __restrict__ out,
                                                    LevelGeom L, RegDev r, const int* __restrict__ box, int dlo0,
@@ -2898,22 +2926,14 @@ int ksgf_macproj_project(ksgf_macproj* mp, double* u_mac, double* v_mac, double*
     KSGF_LAUNCH_CHECK();
     int ret = mg_solve_graphed(mp, rtol, atol, info, st);
     if (ret < 0) return ret;
-    for (int d = 0; d < 3; ++d) {
-        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
-        ghi[d] = 1;
+    {
+        const int glo[3] = {0, 0, 0}, ghi[3] = {1, 1, 1};
         const Region* rf = get_region(ba, glo, ghi, &err);
         if (!rf) return err;
-        double bd = mp->beta * (1.0 / ba->dx[d]);
-        dim3 gr(rf->nchunks);
-        if (d == 0)
-            hipLaunchKernelGGL(k_mac_correct<0>, gr, dim3(256), 0, st, mu[0], (const double*)mp->phi, L0,
-                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
-        else if (d == 1)
-            hipLaunchKernelGGL(k_mac_correct<1>, gr, dim3(256), 0, st, mu[1], (const double*)mp->phi, L0,
-                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
-        else
-            hipLaunchKernelGGL(k_mac_correct<2>, gr, dim3(256), 0, st, mu[2], (const double*)mp->phi, L0,
-                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        hipLaunchKernelGGL(k_mac_correct3, dim3(rf->nchunks), dim3(256), 0, st, mu[0], mu[1], mu[2],
+                           (const double*)mp->phi, L0, regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1],
+                           ba->dlo[2], mp->beta * (1.0 / ba->dx[0]), mp->beta * (1.0 / ba->dx[1]),
+                           mp->beta * (1.0 / ba->dx[2]));
         KSGF_LAUNCH_CHECK();
     }
     return ret;
@@ -2947,22 +2967,14 @@ static int mac_correct(ksgf_macproj* mp, double* u_mac, double* v_mac, double* w
     for (int d = 0; d < 3; ++d)
         if ((e = mfdev(ba, mac[d], KSGF_XFACE + d, 1, 1, &mu[d]))) return e;
     const LevelGeom& L0 = mp->P;
-    for (int d = 0; d < 3; ++d) {
-        int glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
-        ghi[d] = 1;
+    {
+        const int glo[3] = {0, 0, 0}, ghi[3] = {1, 1, 1};
         const Region* rf = get_region(ba, glo, ghi, &err);
         if (!rf) return err;
-        double bd = mp->beta * (1.0 / ba->dx[d]);
-        dim3 gr(rf->nchunks);
-        if (d == 0)
-            hipLaunchKernelGGL(k_mac_correct<0>, gr, dim3(256), 0, st, mu[0], (const double*)mp->phi, L0,
-                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
-        else if (d == 1)
-            hipLaunchKernelGGL(k_mac_correct<1>, gr, dim3(256), 0, st, mu[1], (const double*)mp->phi, L0,
-                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
-        else
-            hipLaunchKernelGGL(k_mac_correct<2>, gr, dim3(256), 0, st, mu[2], (const double*)mp->phi, L0,
-                               regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1], ba->dlo[2], bd);
+        hipLaunchKernelGGL(k_mac_correct3, dim3(rf->nchunks), dim3(256), 0, st, mu[0], mu[1], mu[2],
+                           (const double*)mp->phi, L0, regdev(rf, ba->nbox), ba->dbox, ba->dlo[0], ba->dlo[1],
+                           ba->dlo[2], mp->beta * (1.0 / ba->dx[0]), mp->beta * (1.0 / ba->dx[1]),
+                           mp->beta * (1.0 / ba->dx[2]));
         KSGF_LAUNCH_CHECK();
     }
     return KSGF_OK;
