@@ -613,12 +613,10 @@ __device__ void nd_bapply(const NLev& L, const double* x, double* y)
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kNdBottomThreads) k_nd_bottom(double* __restrict__ x, double* __restrict__ b, NLev L,
-                                                                NdBottom W, int singular, double wsum, double rtol,
-                                                                double atol, int maxiter, int nsmooth_ok,
-                                                                int nsmooth_fail, int* __restrict__ stats)
+__device__ void nd_bottom_body(double* __restrict__ x, double* __restrict__ b, const NLev& L, const NdBottom& W,
+                               int singular, double wsum, double rtol, double atol, int maxiter, int nsmooth_ok,
+                               int nsmooth_fail, int* __restrict__ stats, double* sh)
 {
-    __shared__ double sh[kNdBottomThreads];
     const int T = L.tot;
     if (singular) {
         double s = 0.0;
@@ -711,6 +709,188 @@ __global__ void __launch_bounds__(kNdBottomThreads) k_nd_bottom(double* __restri
     if (threadIdx.x == 0) {
         stats[0] += it;
         stats[1] += (ret != 0) ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kNdBottomThreads) k_nd_bottom(double* __restrict__ x, double* __restrict__ b, NLev L,
+                                                                NdBottom W, int singular, double wsum, double rtol,
+                                                                double atol, int maxiter, int nsmooth_ok,
+                                                                int nsmooth_fail, int* __restrict__ stats)
+{
+    __shared__ double sh[kNdBottomThreads];
+    nd_bottom_body(x, b, L, W, singular, wsum, rtol, atol, maxiter, nsmooth_ok, nsmooth_fail, stats, sh);
+}
+
+// ---------------------------------------------------------------------------
+// The coarse levels (at most kNdCoarseNodes nodes each) of a V-cycle in one
+// workgroup: the grid kernels' sweeps, residual, transfers and bottom solve,
+// node loops over the workgroup with a barrier after every pass.  A grid
+// kernel per pass spent about 4 us on launch and drain for a few hundred
+// nodes.  Same arithmetic as the grid kernels (nd_apply_at is the plane-sum
+// stencil bit for bit), so the iterates are theirs.
+// ---------------------------------------------------------------------------
+constexpr int kNdCoarseNodes = 4096;
+
+struct NdCoarseLev {
+    double *cor, *res, *tmp;
+};
+
+// one Jacobi sweep xo = x + (b - L x) wd (zero: xo = b wd), Dirichlet nodes 0
+__device__ void ndc_sweep(const NLev& L, const double* x, const double* __restrict__ b, double* xo, bool zero)
+{
+    for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
+        int i, j, k;
+        nd_decode(L, c, i, j, k);
+        if (nd_dirichlet(L, i, j, k))
+            xo[c] = 0.0;
+        else
+            xo[c] = zero ? b[c] * L.wd : x[c] + (b[c] - nd_apply_at(L, x, i, j, k)) * L.wd;
+    }
+    __syncthreads();
+}
+
+// nsweep sweeps ending in x (t: scratch), as nd_smooth
+__device__ void ndc_smooth(const NLev& L, double* x, const double* __restrict__ b, double* t, int nsweep, bool zero)
+{
+    if (nsweep == 0) {
+        if (zero) {
+            for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) x[c] = 0.0;
+            __syncthreads();
+        }
+        return;
+    }
+    double* src = x;
+    double* dst = t;
+    for (int s = 0; s < nsweep; ++s) {
+        ndc_sweep(L, src, b, dst, zero && s == 0);
+        double* q = src;
+        src = dst;
+        dst = q;
+    }
+    if (src != x) {
+        for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) x[c] = src[c];
+        __syncthreads();
+    }
+}
+
+// out = b - L x, Dirichlet nodes 0
+__device__ void ndc_residual(const NLev& L, const double* x, const double* __restrict__ b, double* out)
+{
+    for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
+        int i, j, k;
+        nd_decode(L, c, i, j, k);
+        out[c] = nd_dirichlet(L, i, j, k) ? 0.0 : b[c] - nd_apply_at(L, x, i, j, k);
+    }
+    __syncthreads();
+}
+
+// k_nd_restrict's full weighting at coarse node o
+__device__ __forceinline__ double nd_restrict_at(const double* f, const NLev& F, int I, int J, int K)
+{
+    const int ci[3] = {I, J, K};
+    int idx[3][3];
+    double wt[3][3];
+    int cnt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (F.r[d] == 2) {
+            const int fi = 2 * ci[d];
+            idx[d][0] = nd_nb(fi, -1, F.N[d], F.per[d]);
+            idx[d][1] = fi;
+            idx[d][2] = nd_nb(fi, 1, F.N[d], F.per[d]);
+            wt[d][0] = 0.25;
+            wt[d][1] = 0.5;
+            wt[d][2] = 0.25;
+            cnt[d] = 3;
+        } else {
+            idx[d][0] = idx[d][1] = idx[d][2] = ci[d];
+            wt[d][0] = 1.0;
+            wt[d][1] = wt[d][2] = 0.0;
+            cnt[d] = 1;
+        }
+    }
+    double s = 0.0;
+    for (int a = 0; a < cnt[2]; ++a) {
+        double sy = 0.0;
+        for (int bb = 0; bb < cnt[1]; ++bb) {
+            double sx = 0.0;
+            for (int e = 0; e < cnt[0]; ++e) sx += wt[0][e] * f[nd_idx(F, idx[0][e], idx[1][bb], idx[2][a])];
+            sy += wt[1][bb] * sx;
+        }
+        s += wt[2][a] * sy;
+    }
+    return s;
+}
+
+// k_nd_prolong_add's trilinear value at fine node (i, j, k)
+__device__ __forceinline__ double nd_prolong_at(const double* c, const NLev& F, const NLev& C, int i, int j, int k)
+{
+    const int fi[3] = {i, j, k};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (F.r[d] == 2) {
+            lo[d] = fi[d] >> 1;
+            hi[d] = (fi[d] & 1) ? ((lo[d] + 1 == C.N[d]) ? 0 : lo[d] + 1) : lo[d];
+        } else {
+            lo[d] = hi[d] = fi[d];
+        }
+    }
+    const bool ox = lo[0] != hi[0], oy = lo[1] != hi[1], oz = lo[2] != hi[2];
+    double z0, z1;
+    {
+        double a = c[nd_idx(C, lo[0], lo[1], lo[2])], b = c[nd_idx(C, hi[0], lo[1], lo[2])];
+        double e = c[nd_idx(C, lo[0], hi[1], lo[2])], g = c[nd_idx(C, hi[0], hi[1], lo[2])];
+        double y0 = ox ? 0.5 * (a + b) : a, y1 = ox ? 0.5 * (e + g) : e;
+        z0 = oy ? 0.5 * (y0 + y1) : y0;
+    }
+    {
+        double a = c[nd_idx(C, lo[0], lo[1], hi[2])], b = c[nd_idx(C, hi[0], lo[1], hi[2])];
+        double e = c[nd_idx(C, lo[0], hi[1], hi[2])], g = c[nd_idx(C, hi[0], hi[1], hi[2])];
+        double y0 = ox ? 0.5 * (a + b) : a, y1 = ox ? 0.5 * (e + g) : e;
+        z1 = oy ? 0.5 * (y0 + y1) : y0;
+    }
+    return oz ? 0.5 * (z0 + z1) : z0;
+}
+
+__global__ void __launch_bounds__(kNdBottomThreads) k_nd_coarse(const NLev* __restrict__ Ls,
+                                                                const NdCoarseLev* __restrict__ lv, int lc, int nlev,
+                                                                NdBottom W, int singular, double wsum, double rtol,
+                                                                double atol, int maxiter, int nsmooth_ok,
+                                                                int nsmooth_fail, int nu1, int nu2,
+                                                                int* __restrict__ stats)
+{
+    __shared__ double sh[kNdBottomThreads];
+    for (int l = lc; l < nlev - 1; ++l) {
+        const NLev L = Ls[l], C = Ls[l + 1];
+        const NdCoarseLev v = lv[l];
+        ndc_smooth(L, v.cor, v.res, v.tmp, kNdSweeps * nu1, true);
+        ndc_residual(L, v.cor, v.res, v.tmp);
+        double* cres = lv[l + 1].res;
+        for (int o = threadIdx.x; o < C.tot; o += kNdBottomThreads) {
+            int I, J, K;
+            nd_decode(C, o, I, J, K);
+            cres[o] = nd_dirichlet(C, I, J, K) ? 0.0 : nd_restrict_at(v.tmp, L, I, J, K);
+        }
+        __syncthreads();
+    }
+    {
+        const NLev Lb = Ls[nlev - 1];
+        nd_bottom_body(lv[nlev - 1].cor, lv[nlev - 1].res, Lb, W, singular, wsum, rtol, atol, maxiter, nsmooth_ok,
+                       nsmooth_fail, stats, sh);
+        __syncthreads();
+    }
+    for (int l = nlev - 2; l >= lc; --l) {
+        const NLev L = Ls[l], C = Ls[l + 1];
+        const NdCoarseLev v = lv[l];
+        const double* xc = lv[l + 1].cor;
+        for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
+            int i, j, k;
+            nd_decode(L, c, i, j, k);
+            if (!nd_dirichlet(L, i, j, k)) v.cor[c] += nd_prolong_at(xc, L, C, i, j, k);
+        }
+        __syncthreads();
+        ndc_smooth(L, v.cor, v.res, v.tmp, kNdSweeps * nu2, false);
     }
 }
 
@@ -850,6 +1030,9 @@ struct ksgf_nodalproj {
     unsigned long long* norm = nullptr;
     int* stats = nullptr;
     ksgf::NdBottom bw{};
+    int lc = 0;                            // first level of the one-workgroup coarse cycle (k_nd_coarse)
+    ksgf::NLev* dL = nullptr;              // device copy of L
+    ksgf::NdCoarseLev* dlv = nullptr;      // device table of the level arrays
     std::vector<double*> allocs;
 };
 
@@ -945,6 +1128,15 @@ static int nd_alloc(ksgf_nodalproj* np, double** p, size_t n)
     return KSGF_OK;
 }
 
+// the level table of the one-workgroup coarse cycle (levels and their
+// coefficients: again after every coefficient change)
+static int nd_upload_levels(ksgf_nodalproj* np)
+{
+    if (!np->dL) return KSGF_OK;
+    KSGF_HIP_CHECK(hipMemcpy(np->dL, np->L, sizeof(NLev) * np->nlev, hipMemcpyHostToDevice));
+    return KSGF_OK;
+}
+
 static void nd_weighted_mean_sub(ksgf_nodalproj* np, int l, double* a, hipStream_t st)
 {
     const NLev& L = np->L[l];
@@ -984,6 +1176,13 @@ static void nd_smooth(ksgf_nodalproj* np, int l, double* x, const double* b, int
 static void nd_vcycle(ksgf_nodalproj* np, int l, hipStream_t st)
 {
     const ksgf_mlmg_options& o = np->opts;
+    if (l >= np->lc && np->dlv) {
+        hipLaunchKernelGGL(k_nd_coarse, dim3(1), dim3(kNdBottomThreads), 0, st, (const NLev*)np->dL,
+                           (const NdCoarseLev*)np->dlv, l, np->nlev, np->bw, np->singular, np->wsum[np->nlev - 1],
+                           o.bottom_rtol, o.bottom_atol, o.bottom_maxiter, o.num_bottom_smooth, o.num_final_smooth,
+                           o.num_pre_smooth, o.num_post_smooth, np->stats);
+        return;
+    }
     if (l == np->nlev - 1) {
         hipLaunchKernelGGL(k_nd_bottom, dim3(1), dim3(kNdBottomThreads), 0, st, np->cor[l], np->res[l], np->L[l],
                            np->bw, np->singular, np->wsum[l], o.bottom_rtol, o.bottom_atol, o.bottom_maxiter,
@@ -1247,6 +1446,25 @@ int ksgf_nodalproj_create(const ksgf_boxarray* ba, const int* lobc, const int* h
         set_error("hipMemset (nodal projector)");
         e = KSGF_ERR_HIP;
     }
+    if (!e) {
+        // levels of at most kNdCoarseNodes nodes, from the bottom up, run in
+        // the one-workgroup coarse cycle
+        np->lc = np->nlev - 1;
+        while (np->lc > 0 && np->L[np->lc - 1].tot <= kNdCoarseNodes) np->lc--;
+        double *a = nullptr, *t = nullptr;
+        if (!(e = nd_alloc(np, &a, (sizeof(NLev) * np->nlev + 7) / 8)) &&
+            !(e = nd_alloc(np, &t, (sizeof(NdCoarseLev) * np->nlev + 7) / 8))) {
+            std::vector<NdCoarseLev> lv(np->nlev);
+            for (int l = 0; l < np->nlev; ++l) lv[l] = {np->cor[l], np->res[l], np->tmp[l]};
+            np->dL = (NLev*)a;
+            np->dlv = (NdCoarseLev*)t;
+            if (hipMemcpy(np->dlv, lv.data(), sizeof(NdCoarseLev) * np->nlev, hipMemcpyHostToDevice) != hipSuccess) {
+                set_error("hipMemcpy (nodal projector)");
+                e = KSGF_ERR_HIP;
+            }
+            if (!e) e = nd_upload_levels(np);
+        }
+    }
     if (e) {
         ksgf_nodalproj_destroy(np);
         return e;
@@ -1263,7 +1481,7 @@ int ksgf_nodalproj_set_sigma(ksgf_nodalproj* np, double sigma)
     }
     np->sigma = sigma;
     nd_set_coefs(np);
-    return KSGF_OK;
+    return nd_upload_levels(np);
 }
 
 int ksgf_nodalproj_node_dims(const ksgf_nodalproj* np, int* N)
