@@ -736,7 +736,7 @@ struct NdCoarseLev {
 };
 
 // one Jacobi sweep xo = x + (b - L x) wd (zero: xo = b wd), Dirichlet nodes 0
-__device__ void ndc_sweep(const NLev& L, const double* x, const double* __restrict__ b, double* xo, bool zero)
+__device__ __forceinline__ void ndc_sweep(const NLev& L, const double* x, const double* __restrict__ b, double* xo, bool zero)
 {
     for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
         int i, j, k;
@@ -750,7 +750,7 @@ __device__ void ndc_sweep(const NLev& L, const double* x, const double* __restri
 }
 
 // nsweep sweeps ending in x (t: scratch), as nd_smooth
-__device__ void ndc_smooth(const NLev& L, double* x, const double* __restrict__ b, double* t, int nsweep, bool zero)
+__device__ __forceinline__ void ndc_smooth(const NLev& L, double* x, const double* __restrict__ b, double* t, int nsweep, bool zero)
 {
     if (nsweep == 0) {
         if (zero) {
@@ -774,7 +774,7 @@ __device__ void ndc_smooth(const NLev& L, double* x, const double* __restrict__ 
 }
 
 // out = b - L x, Dirichlet nodes 0
-__device__ void ndc_residual(const NLev& L, const double* x, const double* __restrict__ b, double* out)
+__device__ __forceinline__ void ndc_residual(const NLev& L, const double* x, const double* __restrict__ b, double* out)
 {
     for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
         int i, j, k;
@@ -853,6 +853,12 @@ __device__ __forceinline__ double nd_prolong_at(const double* c, const NLev& F, 
     return oz ? 0.5 * (z0 + z1) : z0;
 }
 
+// LDS pool of k_nd_coarse: when every coarse level (correction, residual,
+// scratch) and the bottom vectors fit, they live here for the whole launch;
+// the first coarse level's residual is copied in and its correction out
+constexpr int kNdPool = 5632;  // doubles (44 KB)
+
+template <bool LDS>
 __global__ void __launch_bounds__(kNdBottomThreads) k_nd_coarse(const NLev* __restrict__ Ls,
                                                                 const NdCoarseLev* __restrict__ lv, int lc, int nlev,
                                                                 NdBottom W, int singular, double wsum, double rtol,
@@ -861,12 +867,43 @@ __global__ void __launch_bounds__(kNdBottomThreads) k_nd_coarse(const NLev* __re
                                                                 int* __restrict__ stats)
 {
     __shared__ double sh[kNdBottomThreads];
+    __shared__ double pool[LDS ? kNdPool : 1];
+    // level l's arrays: in the pool (3 tot each, levels lc.. in order, then
+    // the bottom vectors) or the global ones
+    auto lev = [&](int l) {
+        NdCoarseLev v = lv[l];
+        if (LDS) {
+            int off = 0;
+            for (int m = lc; m < l; ++m) off += 3 * Ls[m].tot;
+            const int t = Ls[l].tot;
+            v.cor = pool + off;
+            v.res = pool + off + t;
+            v.tmp = pool + off + 2 * t;
+        }
+        return v;
+    };
+    if (LDS) {
+        const NdCoarseLev g = lv[lc], v = lev(lc);
+        for (int c = threadIdx.x; c < Ls[lc].tot; c += kNdBottomThreads) v.res[c] = g.res[c];
+        int off = 0;
+        for (int m = lc; m < nlev; ++m) off += 3 * Ls[m].tot;
+        const int tb = Ls[nlev - 1].tot;
+        double* q = pool + off;
+        W.r = q;
+        W.rh = q + tb;
+        W.p = q + 2 * tb;
+        W.v = q + 3 * tb;
+        W.s = q + 4 * tb;
+        W.t = q + 5 * tb;
+        W.tmp = q + 6 * tb;
+        __syncthreads();
+    }
     for (int l = lc; l < nlev - 1; ++l) {
         const NLev L = Ls[l], C = Ls[l + 1];
-        const NdCoarseLev v = lv[l];
+        const NdCoarseLev v = lev(l);
         ndc_smooth(L, v.cor, v.res, v.tmp, kNdSweeps * nu1, true);
         ndc_residual(L, v.cor, v.res, v.tmp);
-        double* cres = lv[l + 1].res;
+        double* cres = lev(l + 1).res;
         for (int o = threadIdx.x; o < C.tot; o += kNdBottomThreads) {
             int I, J, K;
             nd_decode(C, o, I, J, K);
@@ -876,14 +913,14 @@ __global__ void __launch_bounds__(kNdBottomThreads) k_nd_coarse(const NLev* __re
     }
     {
         const NLev Lb = Ls[nlev - 1];
-        nd_bottom_body(lv[nlev - 1].cor, lv[nlev - 1].res, Lb, W, singular, wsum, rtol, atol, maxiter, nsmooth_ok,
-                       nsmooth_fail, stats, sh);
+        const NdCoarseLev v = lev(nlev - 1);
+        nd_bottom_body(v.cor, v.res, Lb, W, singular, wsum, rtol, atol, maxiter, nsmooth_ok, nsmooth_fail, stats, sh);
         __syncthreads();
     }
     for (int l = nlev - 2; l >= lc; --l) {
         const NLev L = Ls[l], C = Ls[l + 1];
-        const NdCoarseLev v = lv[l];
-        const double* xc = lv[l + 1].cor;
+        const NdCoarseLev v = lev(l);
+        const double* xc = lev(l + 1).cor;
         for (int c = threadIdx.x; c < L.tot; c += kNdBottomThreads) {
             int i, j, k;
             nd_decode(L, c, i, j, k);
@@ -891,6 +928,10 @@ __global__ void __launch_bounds__(kNdBottomThreads) k_nd_coarse(const NLev* __re
         }
         __syncthreads();
         ndc_smooth(L, v.cor, v.res, v.tmp, kNdSweeps * nu2, false);
+    }
+    if (LDS) {
+        const NdCoarseLev g = lv[lc], v = lev(lc);
+        for (int c = threadIdx.x; c < Ls[lc].tot; c += kNdBottomThreads) g.cor[c] = v.cor[c];
     }
 }
 
@@ -1177,10 +1218,20 @@ static void nd_vcycle(ksgf_nodalproj* np, int l, hipStream_t st)
 {
     const ksgf_mlmg_options& o = np->opts;
     if (l >= np->lc && np->dlv) {
-        hipLaunchKernelGGL(k_nd_coarse, dim3(1), dim3(kNdBottomThreads), 0, st, (const NLev*)np->dL,
-                           (const NdCoarseLev*)np->dlv, l, np->nlev, np->bw, np->singular, np->wsum[np->nlev - 1],
-                           o.bottom_rtol, o.bottom_atol, o.bottom_maxiter, o.num_bottom_smooth, o.num_final_smooth,
-                           o.num_pre_smooth, o.num_post_smooth, np->stats);
+        long long need = 7LL * np->L[np->nlev - 1].tot;
+        for (int m = l; m < np->nlev; ++m) need += 3LL * np->L[m].tot;
+        if (need <= kNdPool)
+            hipLaunchKernelGGL(k_nd_coarse<true>, dim3(1), dim3(kNdBottomThreads), 0, st, (const NLev*)np->dL,
+                               (const NdCoarseLev*)np->dlv, l, np->nlev, np->bw, np->singular,
+                               np->wsum[np->nlev - 1], o.bottom_rtol, o.bottom_atol, o.bottom_maxiter,
+                               o.num_bottom_smooth, o.num_final_smooth, o.num_pre_smooth, o.num_post_smooth,
+                               np->stats);
+        else
+            hipLaunchKernelGGL(k_nd_coarse<false>, dim3(1), dim3(kNdBottomThreads), 0, st, (const NLev*)np->dL,
+                               (const NdCoarseLev*)np->dlv, l, np->nlev, np->bw, np->singular,
+                               np->wsum[np->nlev - 1], o.bottom_rtol, o.bottom_atol, o.bottom_maxiter,
+                               o.num_bottom_smooth, o.num_final_smooth, o.num_pre_smooth, o.num_post_smooth,
+                               np->stats);
         return;
     }
     if (l == np->nlev - 1) {
